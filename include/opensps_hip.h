@@ -1,0 +1,118 @@
+/*
+ * opensps_hip.h -- the C ABI of libopensps_hip.so: the MI355X (gfx950) kernels of OpenSPS's batched
+ * composite-population path, behind plain pointers and sizes.
+ *
+ * This is the layer the reference's Fortran batch driver reaches through `interface ... bind(C,
+ * name = "opensps_hip_*")` blocks with `type(c_ptr), value` device pointers
+ * (src/physics/opensps_csp_batch.f90:260-295, 2341-2349, 3742-3757;
+ * src/spectra/opensps_batch_photometry.f90:579-592, 916-930, 1182-1217). The reference's launchers
+ * are one per stage and run on the default stream with file-static scratch
+ * (src/accel/hip/opensps_hip_kernels.hip:2884-2885, 2988-2991, 3522-3525); here one context owns
+ * its resident tables, its stream and its scratch, and one call runs the whole batch on the device.
+ *
+ * Conventions: arrays are column-major with the quantity fastest (as the reference's C API,
+ * doc/C_API.md:484-496). Pointers documented "device" are HIP device addresses; "host" ones are
+ * ordinary host memory. Every function returns a status: 0 success, 1 bad context or not set up,
+ * 2 unsupported option, 3 bad argument or shape, 10 a HIP runtime error (opensps_hip_last_error
+ * gives the message).
+ */
+#ifndef OPENSPS_HIP_H
+#define OPENSPS_HIP_H
+
+#include <stddef.h>
+
+#include "opensps_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OPENSPS_HIP_FP64 0   /* FP64 age sums (f64 matrix cores) and FP64 chain */
+#define OPENSPS_HIP_MIXED 1  /* split-BF16 age sums (3 bf16 products, FP32 accumulation), FP32 chain */
+
+typedef struct OpsCtx opensps_hip_ctx_t;
+
+/* One batch on device-resident data (the arguments of compute_csp_batch, opensps_csp_batch.f90:431-457,
+ * for the outputs this path computes). Outputs may be NULL except mags. spec, young and old hold
+ * float values with OPENSPS_HIP_MIXED and double values with OPENSPS_HIP_FP64. */
+typedef struct {
+    int n;                       /* models */
+    int precision;               /* OPENSPS_HIP_FP64 or OPENSPS_HIP_MIXED */
+    int split_parts;             /* 1: young and old age sums apart (needed by a birth-cloud or old-star
+                                    screen, or for young / old); 0: one sum per model */
+    const double *prm;           /* device (OPS_NPAR, n): prm[slot * n + m] */
+    const double *ssp_weights;   /* device (nt, n) or NULL: SSP weights instead of the SFH parameters */
+    double *mags;                /* device (nbands, n) AB magnitudes, NaN for masked bands */
+    void *spec;                  /* device (nspec, n) L_sun/Hz per solar mass formed, or NULL */
+    double *mass, *lbol;         /* device (n): surviving mass, log10 L_bol; or NULL */
+    double *fband;               /* device (nbands, n): band fluxes before the distance term, or NULL */
+    void *young, *old;           /* device (nspec, n): the age sums before dust (split_parts = 1), or NULL */
+    int *idx;                    /* device (3, n): it_tesc, zlo (1-based), per-model status; or NULL */
+    double *weights;             /* device (nt, n): the FP64 SFH weights; or NULL */
+} opensps_hip_batch_t;
+
+/* ---- context ---- */
+/* No reference counterpart (the reference keeps device data in the Fortran context through OpenMP
+ * mappings, src/core/opensps_context.f90:1089): creates a context on HIP device `device`. */
+int opensps_hip_create(int device, opensps_hip_ctx_t **ctx);
+int opensps_hip_destroy(opensps_hip_ctx_t *ctx);
+/* Runs the context's work on `stream` (a hipStream_t; NULL: the context's own stream). */
+int opensps_hip_set_stream(opensps_hip_ctx_t *ctx, void *stream);
+
+/* The resident SSP bases and the grids, host arrays: lam (nspec, Angstrom ascending), log_age (nt,
+ * log10 yr ascending), logz (nz, log10 Z/Z_sun ascending), spec (nspec, nt, nz) L_sun/Hz, mass (nt,
+ * nz) surviving mass, lbol (nt, nz) log10 L_bol. Replaces opensps_context_update_ssp_basis
+ * (src/core/opensps_context.f90:757) plus the device residency of the basis (:1089). */
+int opensps_hip_set_grid(opensps_hip_ctx_t *ctx, int nspec, const double *lam, int nt, const double *log_age, int nz,
+                         const double *logz, const double *spec, const double *mass, const double *lbol);
+
+/* The filters, host (nspec, nbands), normalised as filters_build leaves them (src/spectra/opensps_filters.f90:395-503);
+ * the supports and trapezoid coefficients of prepare_batch_grid (src/spectra/opensps_batch_photometry.f90:237-394). */
+int opensps_hip_set_bands(opensps_hip_ctx_t *ctx, int nbands, const double *bands);
+/* The computed bands (opensps_context_set_mag_compute, include/opensps.h:233): mask[b] != 0; NULL: all. */
+int opensps_hip_set_mag_mask(opensps_hip_ctx_t *ctx, int nbands, const int *mask);
+/* The cosmology table (redshift, luminosity distance in pc), n rows, host (cosmospl,
+ * src/physics/opensps_cosmology.f90:288-300). */
+int opensps_hip_set_cosmology(opensps_hip_ctx_t *ctx, int n, const double *z, const double *dl);
+/* add_igm_absorption; smooth_fast = (smooth_velocity == 1 and smoothspec_fast == 1). */
+int opensps_hip_set_options(opensps_hip_ctx_t *ctx, int add_igm, int smooth_fast);
+
+/* The batch: compute_csp_batch's stages 1-8 (src/physics/opensps_csp_batch.f90:431-1214) on the device,
+ * asynchronous on the context's stream; the status of the models is checked by opensps_hip_sync. */
+int opensps_hip_csp_batch(opensps_hip_ctx_t *ctx, const opensps_hip_batch_t *args);
+/* Waits for the context's stream; status 2 if a model of the last batch was unsupported (idx row 3). */
+int opensps_hip_sync(opensps_hip_ctx_t *ctx);
+
+/* ---- kernel-level entry points (the reference's per-stage launchers) ---- */
+/* band_fluxes (opensps_hip_band_fluxes_sp/_dp, src/accel/hip/opensps_hip_kernels.hip:3566, 3623): device
+ * bands_t (nbands, nspec), j_lo, j_hi (1-based), do_band, wt (nspec), fobs_t (n, nspec) -> fband (nbands, n).
+ * precision selects FP64 or FP32 (float bands_t, wt, fobs_t) inputs; the sums are FP64. */
+int opensps_hip_band_fluxes(int precision, int n, int nspec, int nbands, const void *bands_t, const int *j_lo,
+                            const int *j_hi, const int *do_band, const void *wt, const void *fobs_t, double *fband,
+                            void *stream);
+/* observed_frame_z (opensps_hip_observed_frame, opensps_hip_kernels.hip:3503): device zc (n), lambda
+ * (nspec), total (nspec, n) of the precision's type -> fobs_t (n, nspec). */
+int opensps_hip_observed_frame(int precision, int n, int nspec, const double *zc, const double *lambda,
+                               const void *total, void *fobs_t, void *stream);
+
+/* Device self-test of the matrix-instruction lane layouts the age-sum products assume (as the reference's
+ * opensps_hip_selftest_eft checks its error-free transforms, opensps_hip_kernels.hip:3267-3294): which = 0
+ * runs one f64 16x16x4 instruction on a[64], b[64] (doubles, one per lane), which = 1 one bf16 16x16x32
+ * instruction on a[64][8], b[64][8] (bf16 bit patterns); d receives each lane's four accumulators
+ * (d[64][4], double or float). Host pointers. */
+int opensps_hip_selftest_mfma(int which, const void *a, const void *b, void *d);
+
+/* ---- memory (the Fortran host's device buffers) ---- */
+int opensps_hip_malloc(void **ptr, size_t bytes);
+int opensps_hip_free(void *ptr);
+int opensps_hip_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int opensps_hip_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int opensps_hip_device_count(int *n);
+
+/* The message of the last failure of ctx (or of the process-wide calls when ctx is NULL). */
+int opensps_hip_last_error(opensps_hip_ctx_t *ctx, char *msg, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

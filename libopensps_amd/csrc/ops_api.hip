@@ -1,0 +1,581 @@
+// The C ABI of libopensps_hip.so (include/opensps_hip.h): context set-up on the host (the tables the
+// kernels read, formed in FP64 once per grid) and the launch sequence of a batch.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/opensps_hip.h"
+#include "ops_common.h"
+#include "ops_launch.h"
+
+static std::mutex g_err_mu;
+static std::string g_err;
+
+int ops_fail(OpsCtx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) {
+        c->last_error = buf;
+    } else {
+        std::lock_guard<std::mutex> g(g_err_mu);
+        g_err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                           \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return ops_fail((c), 10, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+int ops_reserve(OpsCtx* c, OpsBuffer& b, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (b.bytes >= bytes) return 0;
+    if (b.p) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    HIPCHK(c, hipMalloc(&b.p, want));
+    b.bytes = want;
+    return 0;
+}
+
+template <typename T>
+static int upload(OpsCtx* c, T** dst, const T* src, size_t count) {
+    if (*dst) {
+        HIPCHK(c, hipFree(*dst));
+        *dst = nullptr;
+    }
+    HIPCHK(c, hipMalloc((void**)dst, sizeof(T) * (count ? count : 1)));
+    if (count) HIPCHK(c, hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice));
+    return 0;
+}
+
+#define UPLOAD(c, dst, src, count)                 \
+    do {                                           \
+        int st_ = upload((c), &(dst), (src), (count)); \
+        if (st_) return st_;                       \
+    } while (0)
+
+// ABS_MAG_ZEROPOINT_LOG = log10(L_sun / (4 pi (10 pc)^2)) (src/core/opensps_constants.f90:87-88)
+static double abs_mag_zeropoint_log() {
+    const double pi = acos(-1.0);
+    const double pc = (648000.0 / pi) * 1.495978707e13;
+    return log10(3.828e33 / (4.0 * pi * (10.0 * pc) * (10.0 * pc)));
+}
+
+// Madau (1995) wavelength factors of the IGM optical depth (igm_tau_pixel_terms,
+// src/physics/opensps_cosmology.f90:458-492)
+static void igm_pixel_terms(double w, double* a) {
+    static const double LYW[17] = {1215.67, 1025.72, 972.537, 949.743, 937.803, 930.748, 926.226, 923.150, 920.963,
+                                   919.352, 918.129, 917.181, 916.429, 915.824, 915.329, 914.919, 914.576};
+    static const double LYC[17] = {0.0036,    0.0017,    0.0011846, 0.0009410, 0.0007960, 0.0006967,
+                                   0.0006236, 0.0005665, 0.0005200, 0.0004817, 0.0004487, 0.0004200,
+                                   0.0003947, 0.000372,  0.000352,  0.0003334, 0.00031644};
+    const double LL = 911.75;
+    for (int k = 0; k < 6; k++) a[k] = 0.0;
+    if (w < LYW[0]) {
+        double ser = 0.0;
+        for (int i = 0; i < 17; i++)
+            if (w < LYW[i]) ser = ser + LYC[i] * pow(LYW[i], -3.46);
+        a[0] = ser * pow(w, 3.46);
+        a[1] = 0.0017 * pow(w / LYW[0], 1.68);
+    }
+    if (w < LL) {
+        double s = w / LL;
+        a[2] = 0.25 * pow(s, 3.0) * (1.0 - pow(s, 0.46));
+        a[3] = 9.4 * pow(s, 1.5) * (1.0 - pow(s, 0.18));
+        a[4] = -0.7 * pow(s, 3.0) * (pow(s, -1.32) - 1.0);
+        a[5] = -0.023 * (1.0 - pow(s, 1.68));
+    }
+}
+
+extern "C" {
+
+int opensps_hip_create(int device, opensps_hip_ctx_t** out) {
+    if (!out) return ops_fail(nullptr, 3, "opensps_hip_create: NULL output");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return ops_fail(nullptr, 10, "opensps_hip_create: no HIP device");
+    if (device < 0 || device >= ndev) return ops_fail(nullptr, 3, "opensps_hip_create: device %d of %d", device, ndev);
+    if (hipSetDevice(device) != hipSuccess) return ops_fail(nullptr, 10, "opensps_hip_create: hipSetDevice failed");
+    OpsCtx* c = new OpsCtx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return ops_fail(nullptr, 10, "opensps_hip_create: stream");
+    }
+    c->own_stream = true;
+    *out = c;
+    return 0;
+}
+
+static void free_ptr(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int opensps_hip_destroy(opensps_hip_ctx_t* c) {
+    if (!c) return 1;
+    (void)hipStreamSynchronize(c->stream);
+    void* ps[] = {c->d_lam, c->d_loglam, c->d_ltab, c->d_wl, c->d_dlam, c->d_time, c->d_logz, c->d_bmass, c->d_blbol,
+                  c->d_igm_a, c->d_basis64, c->d_A64, c->d_Ahi, c->d_Alo, c->d_bptr, c->d_bjlo, c->d_bnj, c->d_bc64,
+                  c->d_bc32, c->d_mask, c->d_cz, c->d_cdl, c->yo.p, c->wcol.p, c->w64.p, c->order.p, c->tiles.p,
+                  c->misc.p, c->spec.p, c->flag.p};
+    for (void* p : ps) free_ptr(p);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int opensps_hip_set_stream(opensps_hip_ctx_t* c, void* stream) {
+    if (!c) return 1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (stream) {
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
+        c->stream = (hipStream_t)stream;
+        c->own_stream = false;
+    } else if (!c->own_stream) {
+        HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+    }
+    return 0;
+}
+
+int opensps_hip_set_grid(opensps_hip_ctx_t* c, int nspec, const double* lam, int nt, const double* log_age, int nz,
+                         const double* logz, const double* spec, const double* mass, const double* lbol) {
+    if (!c) return 1;
+    if (nspec < 2 || nt < 2 || nz < 1 || !lam || !log_age || !logz || !spec || !mass || !lbol)
+        return ops_fail(c, 3, "set_grid: bad sizes or NULL arrays");
+    for (int i = 1; i < nspec; i++)
+        if (!(lam[i] > lam[i - 1])) return ops_fail(c, 3, "set_grid: wavelengths not strictly ascending at %d", i);
+    for (int i = 1; i < nt; i++)
+        if (!(log_age[i] > log_age[i - 1])) return ops_fail(c, 3, "set_grid: ages not strictly ascending at %d", i);
+    for (int i = 1; i < nz; i++)
+        if (!(logz[i] > logz[i - 1])) return ops_fail(c, 3, "set_grid: metallicities not strictly ascending at %d", i);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->nspec = nspec;
+    c->nt = nt;
+    c->nz = nz;
+    c->nsp = ((nspec + OPS_TILE_PX - 1) / OPS_TILE_PX) * OPS_TILE_PX;
+    c->npair = nz > 1 ? nz - 1 : 1;
+    c->kpair = nz > 1 ? 2 * nt : nt;
+    c->k64 = ((c->kpair + OPS_KSTEP_F64 - 1) / OPS_KSTEP_F64) * OPS_KSTEP_F64;
+    c->kbf = ((c->kpair + OPS_KSTEP_BF16 - 1) / OPS_KSTEP_BF16) * OPS_KSTEP_BF16;
+    c->h_lam.assign(lam, lam + nspec);
+    c->h_time.assign(log_age, log_age + nt);
+    c->h_logz.assign(logz, logz + nz);
+    // Grid-only tables in FP64 (the reference forms them once per context: prepare_batch_grid,
+    // attenuation_tables_cached; src/spectra/opensps_batch_photometry.f90:237-394, opensps_csp_batch.f90:3494-3566)
+    std::vector<double> loglam(nspec), ltab(nspec), wl(nspec), dlam(nspec);
+    for (int j = 0; j < nspec; j++) {
+        loglam[j] = log(lam[j]);
+        ltab[j] = log(lam[j] / 5500.0);
+        wl[j] = lam[j] > 2.2250738585072014e-308 ? 1.0 / lam[j] : 0.0;
+        dlam[j] = j + 1 < nspec ? lam[j + 1] - lam[j] : 0.0;
+    }
+    int jtop = 0;
+    for (int j = 1; j <= nspec; j++)
+        if (lam[j - 1] < 1215.67) jtop = j;
+    std::vector<double> igm((size_t)6 * (jtop ? jtop : 1));
+    for (int j = 0; j < jtop; j++) igm_pixel_terms(lam[j], &igm[(size_t)6 * j]);
+    c->jtop = jtop;
+    UPLOAD(c, c->d_lam, lam, (size_t)nspec);
+    UPLOAD(c, c->d_loglam, loglam.data(), (size_t)nspec);
+    UPLOAD(c, c->d_ltab, ltab.data(), (size_t)nspec);
+    UPLOAD(c, c->d_wl, wl.data(), (size_t)nspec);
+    UPLOAD(c, c->d_dlam, dlam.data(), (size_t)nspec);
+    UPLOAD(c, c->d_time, log_age, (size_t)nt);
+    UPLOAD(c, c->d_logz, logz, (size_t)nz);
+    UPLOAD(c, c->d_bmass, mass, (size_t)nt * nz);
+    UPLOAD(c, c->d_blbol, lbol, (size_t)nt * nz);
+    UPLOAD(c, c->d_igm_a, igm.data(), igm.size());
+    UPLOAD(c, c->d_basis64, spec, (size_t)nspec * nt * nz);
+    // Both packed operands now (the first batch of either precision then starts at once)
+    const int npxblk = c->nsp / 16;
+    PackArgs pa;
+    pa.nspec = nspec;
+    pa.nt = nt;
+    pa.nz = nz;
+    pa.npair = c->npair;
+    pa.kpair = c->kpair;
+    pa.npxblk = npxblk;
+    pa.basis = c->d_basis64;
+    {
+        size_t n64 = (size_t)c->npair * npxblk * (c->k64 / OPS_KSTEP_F64) * 64;
+        if (c->d_A64) HIPCHK(c, hipFree(c->d_A64));
+        HIPCHK(c, hipMalloc(&c->d_A64, n64 * sizeof(double)));
+        pa.kpad = c->k64;
+        pa.A64 = c->d_A64;
+        k_pack_f64<<<(unsigned)((n64 + 255) / 256), 256, 0, c->stream>>>(pa);
+        HIPCHK(c, hipGetLastError());
+    }
+    {
+        size_t nbf = (size_t)c->npair * npxblk * (c->kbf / OPS_KSTEP_BF16) * 64;
+        if (c->d_Ahi) HIPCHK(c, hipFree(c->d_Ahi));
+        if (c->d_Alo) HIPCHK(c, hipFree(c->d_Alo));
+        HIPCHK(c, hipMalloc(&c->d_Ahi, nbf * 8 * sizeof(uint16_t)));
+        HIPCHK(c, hipMalloc(&c->d_Alo, nbf * 8 * sizeof(uint16_t)));
+        pa.kpad = c->kbf;
+        pa.Ahi = c->d_Ahi;
+        pa.Alo = c->d_Alo;
+        k_pack_bf16<<<(unsigned)((nbf + 255) / 256), 256, 0, c->stream>>>(pa);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int opensps_hip_set_bands(opensps_hip_ctx_t* c, int nbands, const double* bands) {
+    if (!c) return 1;
+    if (c->nspec < 2) return ops_fail(c, 1, "set_bands: no grid");
+    if (nbands < 0 || (nbands > 0 && !bands)) return ops_fail(c, 3, "set_bands: bad arguments");
+    const int nspec = c->nspec;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> wtl(nspec, 0.0);
+    for (int j = 0; j + 1 < nspec; j++) wtl[j] = 0.5 * fabs(c->h_lam[j + 1] - c->h_lam[j]);
+    std::vector<int> bptr(nbands + 1), bjlo(nbands ? nbands : 1), bnj(nbands ? nbands : 1);
+    std::vector<double> c64;
+    int j0 = nspec, j1 = -1;
+    for (int b = 0; b < nbands; b++) {
+        // supports: prepare_batch_grid (src/spectra/opensps_batch_photometry.f90:293-307), 1-based
+        const double* B = bands + (size_t)b * nspec;
+        int ilo = nspec, ihi = 1;
+        for (int j = 1; j <= nspec; j++)
+            if (B[j - 1] != 0.0) {
+                ilo = j < ilo ? j : ilo;
+                ihi = j > ihi ? j : ihi;
+            }
+        int jlo = ilo - 1 > 1 ? ilo - 1 : 1, jhi = ihi < nspec - 1 ? ihi : nspec - 1;
+        if (ilo > ihi) { jlo = 1; jhi = 0; }
+        bptr[b] = (int)c64.size();
+        bjlo[b] = jlo - 1;
+        int nj = jhi >= jlo ? jhi - jlo + 2 : 0;   // pixels jlo .. jhi + 1 (the trapezoid reads j + 1)
+        bnj[b] = nj;
+        // per pixel the trapezoid weights of the two intervals it closes, times the band (the reference's
+        // sparse band matrix bk, :323-344), formed in FP64
+        for (int j = jlo; j < jlo + nj; j++) {
+            double cw = 0.0;
+            if (j >= jlo && j <= jhi) cw += wtl[j - 1];
+            if (j - 1 >= jlo && j - 1 <= jhi) cw += wtl[j - 2];
+            c64.push_back(B[j - 1] * cw);
+        }
+        if (nj > 0) {
+            j0 = jlo - 1 < j0 ? jlo - 1 : j0;
+            j1 = jlo - 1 + nj - 1 > j1 ? jlo - 1 + nj - 1 : j1;
+        }
+    }
+    bptr[nbands] = (int)c64.size();
+    std::vector<float> c32(c64.size());
+    for (size_t i = 0; i < c64.size(); i++) c32[i] = (float)c64[i];
+    c->nbands = nbands;
+    c->band_j0 = j1 >= 0 ? j0 : 0;
+    c->band_j1 = j1;
+    UPLOAD(c, c->d_bptr, bptr.data(), bptr.size());
+    UPLOAD(c, c->d_bjlo, bjlo.data(), bjlo.size());
+    UPLOAD(c, c->d_bnj, bnj.data(), bnj.size());
+    UPLOAD(c, c->d_bc64, c64.data(), c64.size());
+    UPLOAD(c, c->d_bc32, c32.data(), c32.size());
+    c->h_mask.assign(nbands ? nbands : 1, 1);
+    UPLOAD(c, c->d_mask, c->h_mask.data(), c->h_mask.size());
+    return 0;
+}
+
+int opensps_hip_set_mag_mask(opensps_hip_ctx_t* c, int nbands, const int* mask) {
+    if (!c) return 1;
+    if (nbands != c->nbands) return ops_fail(c, 3, "set_mag_mask: %d bands, context has %d", nbands, c->nbands);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int b = 0; b < nbands; b++) c->h_mask[b] = mask ? (mask[b] != 0) : 1;
+    if (nbands) HIPCHK(c, hipMemcpy(c->d_mask, c->h_mask.data(), sizeof(int) * nbands, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int opensps_hip_set_cosmology(opensps_hip_ctx_t* c, int n, const double* z, const double* dl) {
+    if (!c) return 1;
+    if (n < 2 || !z || !dl) return ops_fail(c, 3, "set_cosmology: bad table");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->ncosmo = n;
+    UPLOAD(c, c->d_cz, z, (size_t)n);
+    UPLOAD(c, c->d_cdl, dl, (size_t)n);
+    return 0;
+}
+
+int opensps_hip_set_options(opensps_hip_ctx_t* c, int add_igm, int smooth_fast) {
+    if (!c) return 1;
+    c->add_igm = add_igm != 0;
+    c->smooth_fast = smooth_fast != 0;
+    return 0;
+}
+
+int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
+    if (!c) return 1;
+    if (!a) return ops_fail(c, 3, "batch: NULL arguments");
+    if (c->nspec < 2) return ops_fail(c, 1, "batch: no SSP grid installed");
+    if (c->ncosmo < 2) return ops_fail(c, 1, "batch: no cosmology table installed");
+    const int n = a->n;
+    if (n < 0) return ops_fail(c, 3, "batch: n < 0");
+    if (n == 0) return 0;
+    if (!a->prm || (!a->mags && c->nbands > 0)) return ops_fail(c, 3, "batch: prm and mags are required");
+    if (a->precision != OPENSPS_HIP_FP64 && a->precision != OPENSPS_HIP_MIXED)
+        return ops_fail(c, 3, "batch: unknown precision %d", a->precision);
+    const int parts = a->split_parts ? 2 : 1;
+    if ((a->young || a->old) && parts != 2) return ops_fail(c, 3, "batch: young / old need split_parts = 1");
+    const bool mixed = a->precision == OPENSPS_HIP_MIXED;
+    const int kpad = mixed ? c->kbf : c->k64;
+    const int npxblk = c->nsp / 16, npxt = c->nsp / OPS_TILE_PX;
+    const size_t tsz = mixed ? sizeof(float) : sizeof(double);
+    const int maxtiles = (n * parts + OPS_TILE_COL - 1) / OPS_TILE_COL + c->npair;
+    HIPCHK(c, hipSetDevice(c->device));
+
+    // scratch: misc = zlo, it, status, order (n ints each) | ntiles | flag
+    int st;
+    if ((st = ops_reserve(c, c->misc, sizeof(int) * ((size_t)4 * n + 64)))) return st;
+    if ((st = ops_reserve(c, c->tiles, sizeof(int4) * (size_t)maxtiles))) return st;
+    if ((st = ops_reserve(c, c->yo, tsz * (size_t)parts * n * c->nsp))) return st;
+    if (mixed) {
+        if ((st = ops_reserve(c, c->wcol, sizeof(uint16_t) * 2 * (size_t)n * parts * kpad))) return st;
+    } else {
+        if ((st = ops_reserve(c, c->w64, sizeof(double) * (size_t)n * parts * kpad))) return st;
+    }
+    int* misc = (int*)c->misc.p;
+    int *d_zlo = misc, *d_it = misc + n, *d_status = misc + 2 * n, *d_order = misc + 3 * n;
+    int* d_ntiles = misc + 4 * n;
+    int* d_flag = misc + 4 * n + 16;
+    HIPCHK(c, hipMemsetAsync(d_ntiles, 0, 64 * sizeof(int), c->stream));
+    c->d_last_flag = d_flag;
+
+    PrepArgs pa;
+    pa.n = n;
+    pa.nt = c->nt;
+    pa.nz = c->nz;
+    pa.parts = parts;
+    pa.kpair = c->kpair;
+    pa.kpad = kpad;
+    pa.mixed = mixed;
+    pa.smooth_fast = c->smooth_fast;
+    pa.tiny_logt = c->tiny_logt;
+    pa.prm = a->prm;
+    pa.weights_in = a->ssp_weights;
+    pa.time = c->d_time;
+    pa.logz = c->d_logz;
+    pa.bmass = c->d_bmass;
+    pa.blbol = c->d_blbol;
+    pa.w_out = a->weights;
+    pa.mass = a->mass;
+    pa.lbol = a->lbol;
+    pa.w64 = mixed ? nullptr : (double*)c->w64.p;
+    pa.whi = mixed ? (uint16_t*)c->wcol.p : nullptr;
+    pa.wlo = mixed ? (uint16_t*)c->wcol.p + (size_t)n * parts * kpad : nullptr;
+    pa.zlo = d_zlo;
+    pa.it = d_it;
+    pa.idx = a->idx;
+    pa.status = d_status;
+    pa.flag = d_flag;
+    size_t lds_prep = sizeof(double) * 4 * 5 * (size_t)(c->nt + 1);
+    k_prep<<<(n + 3) / 4, 256, lds_prep, c->stream>>>(pa);
+    HIPCHK(c, hipGetLastError());
+
+    GroupArgs ga;
+    ga.n = n;
+    ga.npair = c->npair;
+    ga.parts = parts;
+    ga.zlo = d_zlo;
+    ga.order = d_order;
+    ga.ntiles = d_ntiles;
+    ga.tiles = (int4*)c->tiles.p;
+    k_group<<<1, 256, sizeof(int) * 256 * (size_t)c->npair, c->stream>>>(ga);
+    HIPCHK(c, hipGetLastError());
+
+    GemmArgs gm;
+    gm.n = n;
+    gm.parts = parts;
+    gm.nsp = c->nsp;
+    gm.npxt = npxt;
+    gm.npxblk = npxblk;
+    gm.kpad = kpad;
+    gm.order = d_order;
+    gm.ntiles = d_ntiles;
+    gm.tiles = (const int4*)c->tiles.p;
+    gm.A64 = c->d_A64;
+    gm.w64 = (const double*)c->w64.p;
+    gm.Ahi = c->d_Ahi;
+    gm.Alo = c->d_Alo;
+    gm.whi = pa.whi;
+    gm.wlo = pa.wlo;
+    gm.yo = c->yo.p;
+    const unsigned ptx = (unsigned)((npxt + 7) / 8);
+    const unsigned gblocks = 8u * ptx * (unsigned)maxtiles;
+    if (mixed)
+        k_gemm_bf16x3<<<gblocks, 256, 0, c->stream>>>(gm);
+    else
+        k_gemm_f64<<<gblocks, 256, 0, c->stream>>>(gm);
+    HIPCHK(c, hipGetLastError());
+
+    ChainArgs ca;
+    ca.n = n;
+    ca.nspec = c->nspec;
+    ca.nsp = c->nsp;
+    ca.parts = parts;
+    ca.nbands = c->nbands;
+    ca.jtop = c->jtop;
+    ca.add_igm = c->add_igm;
+    ca.smooth_fast = c->smooth_fast;
+    ca.band_j0 = c->band_j0;
+    ca.band_j1 = c->nbands > 0 ? c->band_j1 : -1;
+    ca.ncosmo = c->ncosmo;
+    ca.amzl = abs_mag_zeropoint_log();
+    ca.prm = a->prm;
+    ca.yo = c->yo.p;
+    ca.lam = c->d_lam;
+    ca.loglam = c->d_loglam;
+    ca.ltab = c->d_ltab;
+    ca.wl = c->d_wl;
+    ca.dlam = c->d_dlam;
+    ca.igm_a = c->d_igm_a;
+    ca.cz = c->d_cz;
+    ca.cdl = c->d_cdl;
+    ca.bptr = c->d_bptr;
+    ca.bjlo = c->d_bjlo;
+    ca.bnj = c->d_bnj;
+    ca.mask = c->d_mask;
+    ca.bc64 = c->d_bc64;
+    ca.bc32 = c->d_bc32;
+    ca.spec = a->spec;
+    ca.young = a->young;
+    ca.old = a->old;
+    ca.mags = a->mags;
+    ca.fband = a->fband;
+    size_t lds_chain = 2 * tsz * (size_t)c->nsp;
+    if (mixed) {
+        HIPCHK(c, hipFuncSetAttribute((const void*)k_chain<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_chain));
+        k_chain<float><<<n, 512, lds_chain, c->stream>>>(ca);
+    } else {
+        HIPCHK(c, hipFuncSetAttribute((const void*)k_chain<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_chain));
+        k_chain<double><<<n, 512, lds_chain, c->stream>>>(ca);
+    }
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int opensps_hip_sync(opensps_hip_ctx_t* c) {
+    if (!c) return 1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!c->d_last_flag) return 0;
+    int flag = 0;
+    HIPCHK(c, hipMemcpy(&flag, c->d_last_flag, sizeof(int), hipMemcpyDeviceToHost));
+    c->d_last_flag = nullptr;
+    if (flag == 2) return ops_fail(c, 2, "batch: a model needs an option this build does not compute (SFH type, "
+                                         "tage <= 0 without weights, or smoothing other than the fast velocity operator)");
+    if (flag == 4) return ops_fail(c, 3, "batch: split_parts = 0 with a birth-cloud or old-star screen");
+    return 0;
+}
+
+int opensps_hip_band_fluxes(int precision, int n, int nspec, int nbands, const void* bands_t, const int* j_lo,
+                            const int* j_hi, const int* do_band, const void* wt, const void* fobs_t, double* fband,
+                            void* stream) {
+    if (n <= 0 || nbands <= 0) return 0;
+    if (nspec < 2 || !bands_t || !j_lo || !j_hi || !do_band || !wt || !fobs_t || !fband)
+        return ops_fail(nullptr, 3, "band_fluxes: bad arguments");
+    dim3 grid((n + 3) / 4, nbands);
+    if (precision == OPENSPS_HIP_FP64)
+        k_band_fluxes<double><<<grid, 256, 0, (hipStream_t)stream>>>(n, nspec, nbands, (const double*)bands_t, j_lo, j_hi,
+                                                                     do_band, (const double*)wt, (const double*)fobs_t, fband);
+    else
+        k_band_fluxes<float><<<grid, 256, 0, (hipStream_t)stream>>>(n, nspec, nbands, (const float*)bands_t, j_lo, j_hi,
+                                                                    do_band, (const float*)wt, (const float*)fobs_t, fband);
+    HIPCHK(nullptr, hipGetLastError());
+    HIPCHK(nullptr, hipStreamSynchronize((hipStream_t)stream));
+    return 0;
+}
+
+int opensps_hip_observed_frame(int precision, int n, int nspec, const double* zc, const double* lambda, const void* total,
+                               void* fobs_t, void* stream) {
+    if (n <= 0) return 0;
+    if (nspec < 2 || !zc || !lambda || !total || !fobs_t) return ops_fail(nullptr, 3, "observed_frame: bad arguments");
+    dim3 grid((nspec + 255) / 256, n);
+    if (precision == OPENSPS_HIP_FP64)
+        k_observed_frame<double><<<grid, 256, 0, (hipStream_t)stream>>>(n, nspec, zc, lambda, (const double*)total,
+                                                                        (double*)fobs_t);
+    else
+        k_observed_frame<float><<<grid, 256, 0, (hipStream_t)stream>>>(n, nspec, zc, lambda, (const float*)total,
+                                                                       (float*)fobs_t);
+    HIPCHK(nullptr, hipGetLastError());
+    HIPCHK(nullptr, hipStreamSynchronize((hipStream_t)stream));
+    return 0;
+}
+
+int opensps_hip_selftest_mfma(int which, const void* a, const void* b, void* d) {
+    if (!a || !b || !d || (which != 0 && which != 1)) return ops_fail(nullptr, 3, "selftest_mfma: bad arguments");
+    size_t ab = which == 0 ? 64 * sizeof(double) : 64 * 8 * sizeof(uint16_t);
+    size_t db = which == 0 ? 64 * 4 * sizeof(double) : 64 * 4 * sizeof(float);
+    void *da = nullptr, *dbp = nullptr, *dd = nullptr;
+    HIPCHK(nullptr, hipMalloc(&da, ab));
+    HIPCHK(nullptr, hipMalloc(&dbp, ab));
+    HIPCHK(nullptr, hipMalloc(&dd, db));
+    HIPCHK(nullptr, hipMemcpy(da, a, ab, hipMemcpyHostToDevice));
+    HIPCHK(nullptr, hipMemcpy(dbp, b, ab, hipMemcpyHostToDevice));
+    k_selftest_mfma<<<1, 64>>>(which, da, dbp, dd);
+    HIPCHK(nullptr, hipGetLastError());
+    HIPCHK(nullptr, hipMemcpy(d, dd, db, hipMemcpyDeviceToHost));
+    (void)hipFree(da);
+    (void)hipFree(dbp);
+    (void)hipFree(dd);
+    return 0;
+}
+
+int opensps_hip_malloc(void** ptr, size_t bytes) {
+    if (!ptr) return 3;
+    *ptr = nullptr;
+    HIPCHK(nullptr, hipMalloc(ptr, bytes ? bytes : 1));
+    return 0;
+}
+
+int opensps_hip_free(void* ptr) {
+    if (ptr) HIPCHK(nullptr, hipFree(ptr));
+    return 0;
+}
+
+int opensps_hip_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    if (bytes) HIPCHK(nullptr, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int opensps_hip_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    if (bytes) HIPCHK(nullptr, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int opensps_hip_device_count(int* n) {
+    if (!n) return 3;
+    *n = 0;
+    if (hipGetDeviceCount(n) != hipSuccess) *n = 0;
+    return 0;
+}
+
+int opensps_hip_last_error(opensps_hip_ctx_t* c, char* msg, int len) {
+    if (!msg || len <= 0) return 3;
+    std::string s;
+    if (c) {
+        s = c->last_error;
+    } else {
+        std::lock_guard<std::mutex> g(g_err_mu);
+        s = g_err;
+    }
+    snprintf(msg, (size_t)len, "%s", s.c_str());
+    return 0;
+}
+
+}   // extern "C"

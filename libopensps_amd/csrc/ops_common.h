@@ -1,0 +1,77 @@
+// Device helpers shared by the kernels of the batched composite-population path (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/opensps_params.h"
+
+#define OPS_SAFE_FLOOR 2.2250738585072014e-308   // tiny(1.0_wp)
+#define OPS_C_LIGHT 2.99792458e18                // Angstrom / s
+#define OPS_LY_ALPHA 1215.67
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
+typedef __attribute__((__vector_size__(4 * sizeof(double)))) double f64x4_t;
+
+// Number of entries of the ascending array a[0..n) that are <= v (binary search). For a grid of
+// distinct ascending values, max(1, min(n - 1, count_le)) is the reference's clamped find_interval
+// (src/math/opensps_interpolation.f90:149-198 with the callers' clamp to [1, n - 1]), 1-based.
+__device__ __forceinline__ int count_le(const double* __restrict__ a, int n, double v) {
+    int lo = 0, hi = n;   // invariant: a[0..lo) <= v, a[hi..n) > v
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// The reference's find_interval for either order with its end rules, 1-based (0 below, n above).
+__device__ __forceinline__ int find_interval_ref(const double* __restrict__ a, int n, double v) {
+    if (n < 2) return 1;
+    bool asc = a[n - 1] >= a[0];
+    int lower = 0, upper = n + 1;
+    while (upper - lower > 1) {
+        int mid = (upper + lower) >> 1;
+        if (asc == (v >= a[mid - 1])) lower = mid; else upper = mid;
+    }
+    if (v == a[0]) return 1;
+    if (v == a[n - 1]) return n - 1;
+    return lower;
+}
+
+// exp(x) in FP32 for an FP64 argument: x = k ln 2 + r reduced in FP64, exp(r) in FP32, scaled by
+// 2^k. Keeps ~1 ulp of FP32 where rounding x itself to FP32 would cost |x| eps (the reference's
+// exp_kp, src/physics/opensps_csp_batch.f90:3886-3919). Flushes to 0 below the FP32 range.
+__device__ __forceinline__ float exp_kp(double x) {
+    const double LOG2E = 1.4426950408889634, LN2 = 0.6931471805599453;
+    x = fmin(1.0e4, fmax(-1.0e4, x));
+    double k = rint(x * LOG2E);
+    if (k < -149.0) return 0.0f;
+    if (k > 127.0) return 3.402823466e38f;
+    float r = (float)(x - k * LN2);
+    return ldexpf(__expf(r), (int)k);
+}
+
+__device__ __forceinline__ uint16_t bf16_rn(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+// Split an FP64 value into bf16 hi + lo (x ~ hi + lo to ~2^-16 relative).
+__device__ __forceinline__ void split_bf16(double x, uint16_t& hi, uint16_t& lo) {
+    hi = bf16_rn((float)x);
+    lo = bf16_rn((float)(x - (double)bf16_to_f32(hi)));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Maximum with the first index among equal maxima (the reference's strict '>' scan).
+__device__ __forceinline__ void argmax_merge(double& v, int& j, double v2, int j2) {
+    if (v2 > v || (v2 == v && j2 < j)) { v = v2; j = j2; }
+}

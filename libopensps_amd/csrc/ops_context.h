@@ -1,0 +1,68 @@
+// The device context of the batched composite-population path: the resident SSP bases and their
+// packed matrix-core operands, the wavelength and filter tables, the cosmology table, the options
+// of the batch, and the scratch of the last batch. One per public context handle; never shared.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#define OPS_TILE_PX 64     // pixels per workgroup tile of the age-sum products
+#define OPS_TILE_COL 64    // model columns per workgroup tile
+#define OPS_KSTEP_BF16 32  // K per bf16 matrix instruction (16x16x32)
+#define OPS_KSTEP_F64 4    // K per f64 matrix instruction (16x16x4)
+
+struct OpsBuffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct OpsCtx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+
+    // ---- the grid (set_grid) ----
+    int nspec = 0, nt = 0, nz = 0;
+    int nsp = 0;          // nspec padded to whole pixel tiles
+    int npair = 0;        // metallicity pairs (nz - 1), or 1 with one basis
+    int kpair = 0;        // K of one pair's product: 2 nt (blend) or nt
+    int k64 = 0, kbf = 0; // kpair padded to the f64 / bf16 instruction K
+    std::vector<double> h_lam, h_logz, h_time, h_basis, h_mass, h_lbol;
+    double *d_lam = nullptr, *d_loglam = nullptr, *d_ltab = nullptr, *d_wl = nullptr, *d_dlam = nullptr;
+    double *d_time = nullptr, *d_logz = nullptr, *d_bmass = nullptr, *d_blbol = nullptr;
+    int jtop = 0;                 // pixels below Lyman-alpha
+    double* d_igm_a = nullptr;    // (jtop, 6) IGM wavelength factors
+    double* d_basis64 = nullptr;  // nspec x nt x nz FP64 (for the packing)
+    double* d_A64 = nullptr;      // packed f64 operands [pair][pxblk][kstep][lane]
+    uint16_t *d_Ahi = nullptr, *d_Alo = nullptr;   // packed bf16 hi / lo [pair][pxblk][kstep][lane][8]
+
+    // ---- filters (set_bands) ----
+    int nbands = 0;
+    int band_j0 = 0, band_j1 = -1;                 // 0-based pixel range the band integrals read
+    int *d_bptr = nullptr, *d_bjlo = nullptr;      // per band: offset into the coefficients, first pixel
+    int* d_bnj = nullptr;                          // per band: number of pixels
+    double* d_bc64 = nullptr;                      // per (band, pixel of its support) coefficient, FP64
+    float* d_bc32 = nullptr;                       // the same in FP32
+    int* d_mask = nullptr;                         // per band: computed (1) or NaN (0)
+    std::vector<int> h_mask;
+
+    // ---- cosmology ----
+    int ncosmo = 0;
+    double *d_cz = nullptr, *d_cdl = nullptr;
+
+    // ---- options ----
+    int add_igm = 0;       // add_igm_absorption
+    int smooth_fast = 1;   // smooth_velocity == 1 and smoothspec_fast == 1
+    double tiny_logt = 0.0;
+
+    // ---- batch scratch (grown on demand) ----
+    OpsBuffer yo, wcol, w64, order, tiles, misc, spec, flag;
+    int* d_last_flag = nullptr;   // the per-batch status flag of the last batch (read by sync)
+
+    std::string last_error;
+};
+
+int ops_fail(OpsCtx* c, int code, const char* fmt, ...);
+int ops_reserve(OpsCtx* c, OpsBuffer& b, size_t bytes);

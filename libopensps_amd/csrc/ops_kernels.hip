@@ -1,0 +1,808 @@
+// Kernels of the batched composite-population path for gfx950 (MI355X).
+//
+//   k_prep      one wavefront per model: index selection (it_tesc, zlo), the FP64 SFH weights by the
+//               moment method, mass and L_bol, and the model's columns of the age-sum product
+//   k_group     one workgroup: the models sorted by metallicity pair, cut into column tiles
+//   k_gemm_*    the age sums yo = [basis(zlo) basis(zlo+1)] . W per pair on the matrix cores
+//               (FP64 16x16x4, or split-BF16 16x16x32 with three products and FP32 accumulation)
+//   k_chain     one workgroup per model: dust, IGM, velocity smoothing, the spectrum out, the
+//               observed frame, the band integrals and the magnitudes, the spectrum staged in LDS
+#include "ops_common.h"
+#include "ops_launch.h"
+
+// ------------------------------------------------------------------------------------------------
+// SFH weights (src/physics/opensps_sfh.f90). FP64 with contraction off, so that each operation
+// rounds as the reference's CPU build rounds it.
+// ------------------------------------------------------------------------------------------------
+// LDS written by some lanes of a wavefront and read by others: a compiler barrier plus wavefront-scope fences
+#define WAVE_SYNC()                                           \
+    do {                                                      \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); \
+        __builtin_amdgcn_wave_barrier();                      \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
+    } while (0)
+
+#pragma clang fp contract(off)
+
+struct SfhP {
+    double tau, tage, tburst, sf_trunc, sf_slope, tq, t0, tb;
+    int type;
+};
+
+__device__ double d_expi(double x) {   // opensps_special_functions.f90:50-122
+    const double gam = 0.57721566490153286060651209008240243104215933593992;
+    if (x == 0.0) return -INFINITY;
+    if (x < 0.0) return NAN;
+    double res = 1.0, r = 1.0;
+    if (fabs(x) <= 40.0) {
+        for (int k = 1; k <= 1000; k++) {
+            double td = (double)k + 1.0;
+            r = (r * (double)k * x) / (td * td);
+            res = res + r;
+            if (fabs(r) <= 1.0e-20 * fabs(res)) break;
+            if (k == 1000) return NAN;
+        }
+        return gam + log(x) + x * res;
+    }
+    double inv_x = 1.0 / x;
+    for (int k = 1; k <= 20; k++) { r = r * (double)k * inv_x; res = res + r; }
+    return exp(x) * inv_x * res;
+}
+
+__device__ void d_convert_sfh(const double* p, SfhP& s) {   // convert_sfhparams_pc (opensps_sfh.f90:1305-1355)
+    int sfh = (int)p[OPS_P_SFH];
+    double start = (sfh == 1 || sfh == 4 || sfh == 5) ? p[OPS_P_SF_START] * 1.0e9 : 0.0;
+    s.tage = p[OPS_P_TAGE] * 1.0e9 - start;
+    s.tburst = p[OPS_P_TBURST] * 1.0e9 - start;
+    s.sf_trunc = p[OPS_P_SF_TRUNC] * 1.0e9 - start;
+    s.tau = p[OPS_P_TAU] * 1.0e9;
+    s.sf_slope = -p[OPS_P_SF_SLOPE] / 1.0e9;
+    s.tb = s.tage - s.tburst;
+    s.tq = (s.sf_trunc <= 0.0 || s.sf_trunc > s.tage) ? 0.0 : s.tage - s.sf_trunc;
+    s.t0 = (s.sf_slope > OPS_SAFE_FLOOR) ? s.tq - 1.0 / s.sf_slope : 0.0;
+    if (s.t0 > s.tq || s.t0 <= 0.0) s.t0 = 0.0;
+    s.type = 0;
+}
+
+// eval_indefinite_moments_log (opensps_sfh.f90:852-910), types 0, 1, 4
+__device__ void d_moments(double log_t, const SfhP& s, double& m0, double& m1) {
+    const double LOG_E = 0.4342944819032518;   // log10(e)
+    double t_lin = pow(10.0, log_t);
+    if (s.type == 0) {
+        m0 = t_lin;
+        m1 = t_lin * (log_t - LOG_E);
+    } else if (s.type == 1 || s.type == 4) {
+        if (s.tau <= OPS_SAFE_FLOOR) { m0 = 0.0; m1 = 0.0; return; }
+        double tt = fmax(t_lin / s.tau, OPS_SAFE_FLOOR);
+        double ei = d_expi(tt);
+        if (!isfinite(ei)) ei = 0.0;
+        double et = exp(tt);
+        if (s.type == 1) {
+            m0 = et;
+            m1 = log_t * m0 - LOG_E * ei;
+        } else {
+            m0 = -(t_lin - s.tage - s.tau) * et;
+            double tc = s.tau * LOG_E;
+            m1 = -((t_lin - s.tage - s.tau) * log_t - tc) * et - (s.tage + s.tau) * LOG_E * ei;
+        }
+    } else {
+        m0 = 0.0; m1 = 0.0;
+    }
+}
+
+// compute_ssp_weights (opensps_sfh.f90:97-257) by one wavefront: the moments at the clamped nodes are
+// independent, so each lane takes nodes; each node then adds the right part of the segment below and
+// the left part of the segment above, in the reference's order. w, m0s, m1s: LDS (nt + 1).
+__device__ void wave_ssp_weights(const double* __restrict__ tf, int nt, double tiny_logt, const SfhP& s, int idx_min,
+                                 int idx_max, double* w, double* m0s, double* m1s, int lane) {
+    for (int i = lane; i < nt; i += 64) w[i] = 0.0;
+    WAVE_SYNC();
+    if (s.type == -1) {
+        if (lane == 0 && s.tb >= 0.0) {
+            double log_tb = log10(fmax(s.tb, pow(10.0, tiny_logt)));
+            if (log_tb <= tf[0]) {
+                w[0] = 1.0;
+            } else {
+                int j = find_interval_ref(tf, nt, log_tb);
+                j = max(1, min(j, nt - 1));
+                double bdt = tf[j] - tf[j - 1];
+                w[j - 1] = (tf[j] - log_tb) / bdt;
+                w[j] = (log_tb - tf[j - 1]) / bdt;
+            }
+        }
+        WAVE_SYNC();
+        return;
+    }
+    double lmin = pow(10.0, tiny_logt);
+    double lts = log10(fmax(s.tq, lmin)), lte = log10(fmax(s.tage, lmin));
+    int js = find_interval_ref(tf, nt, lts);
+    int je = find_interval_ref(tf, nt, lte) + 1;
+    js = max(js, max(idx_min, 1));
+    je = min(je, min(idx_max, nt - 1));
+    // node moments M(L(j)), j = js .. je + 1 (1-based), L(j) = clamp(tf(j))
+    for (int j = js + lane; j <= je + 1; j += 64) {
+        double L = fmin(fmax(tf[j - 1], lts), lte);
+        d_moments(L, s, m0s[j], m1s[j]);
+    }
+    WAVE_SYNC();
+    for (int j = js + lane; j <= je + 1; j += 64) {
+        double Lj = fmin(fmax(tf[j - 1], lts), lte);
+        double acc = 0.0;
+        if (j - 1 >= js) {   // segment j - 1: node j is its upper node
+            double Lp = fmin(fmax(tf[j - 2], lts), lte);
+            if (Lj > Lp) {
+                double m0 = m0s[j] - m0s[j - 1], m1 = m1s[j] - m1s[j - 1];
+                double nlo = tf[j - 2], nhi = tf[j - 1];
+                double inv = 1.0 / (nhi - nlo);
+                acc = acc + (m1 - nlo * m0) * inv;
+            }
+        }
+        if (j <= je) {       // segment j: node j is its lower node
+            double Ln = fmin(fmax(tf[j], lts), lte);
+            if (Ln > Lj) {
+                double m0 = m0s[j + 1] - m0s[j], m1 = m1s[j + 1] - m1s[j];
+                double nlo = tf[j - 1], nhi = tf[j];
+                double inv = 1.0 / (nhi - nlo);
+                acc = acc + (nhi * m0 - m1) * inv;
+            }
+        }
+        w[j - 1] = acc;
+    }
+    WAVE_SYNC();
+    if (idx_min == 0 && lane == 0) {
+        double l1 = fmin(fmax(tiny_logt, lts), lte), l2 = fmin(fmax(tf[0], lts), lte);
+        if (l1 < l2) {
+            double a0, a1, b0, b1;
+            d_moments(l2, s, a0, a1);
+            d_moments(l1, s, b0, b1);
+            w[0] = w[0] + (a0 - b0);
+        }
+    }
+    WAVE_SYNC();
+}
+
+// sfh_weights_core_pc (opensps_sfh.f90:1077-1294) for sfh 0, 1, 4. Returns 0, or 2 (unsupported).
+__device__ int wave_sfh_weights(const double* __restrict__ tf, int nt, double tiny_logt, const double* p, double* w,
+                                double* t1, double* t2, double* m0s, double* m1s, int lane) {
+    SfhP s;
+    d_convert_sfh(p, s);
+    int sfh = (int)p[OPS_P_SFH];
+    int imax = find_interval_ref(tf, nt, log10(s.tage)) + 2;
+    imax = min(max(imax, 1), nt);
+    if (sfh == 0) {
+        s.type = -1;
+        s.tb = s.tage;
+        wave_ssp_weights(tf, nt, tiny_logt, s, max(imax - 2, 1), imax, w, m0s, m1s, lane);
+        return 0;
+    }
+    if (sfh != 1 && sfh != 4) return 2;
+    s.type = sfh;
+    wave_ssp_weights(tf, nt, tiny_logt, s, 0, imax, w, m0s, m1s, lane);
+    double mass1 = 0.0;
+    for (int i = 0; i < imax; i++) mass1 = mass1 + w[i];   // (every lane: the reference's sequential sum)
+    if (mass1 < OPS_SAFE_FLOOR) mass1 = 1.0;
+    WAVE_SYNC();
+    for (int i = lane; i < nt; i += 64) w[i] = w[i] / mass1;
+    WAVE_SYNC();
+    double cst = p[OPS_P_CONST], fb = p[OPS_P_FBURST];
+    if (cst > 0.0 || fb > OPS_SAFE_FLOOR) {
+        s.type = 0;
+        wave_ssp_weights(tf, nt, tiny_logt, s, 0, imax, t1, m0s, m1s, lane);
+        double mc = 0.0;
+        for (int i = 0; i < imax; i++) mc = mc + t1[i];
+        if (mc < OPS_SAFE_FLOOR) mc = 1.0;
+        double fbv = 0.0;
+        if (s.tb >= 0.0) {
+            s.type = -1;
+            wave_ssp_weights(tf, nt, tiny_logt, s, 0, imax, t2, m0s, m1s, lane);
+            fbv = fb;
+        } else {
+            for (int i = lane; i < nt; i += 64) t2[i] = 0.0;
+            WAVE_SYNC();
+        }
+        for (int i = lane; i < nt; i += 64) w[i] = (1.0 - cst - fbv) * w[i] + cst * (t1[i] / mc) + fbv * t2[i];
+        WAVE_SYNC();
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_prep: one wavefront per model (4 per workgroup)
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int m = blockIdx.x * 4 + wv;
+    if (m >= a.n) return;
+    const int nt = a.nt, n = a.n;
+    double* w = lds + (size_t)wv * 5 * (nt + 1);
+    double* t1 = w + (nt + 1);
+    double* t2 = t1 + (nt + 1);
+    double* m0s = t2 + (nt + 1);
+    double* m1s = m0s + (nt + 1);
+    double p[OPS_NPAR];
+#pragma unroll
+    for (int q = 0; q < OPS_NPAR; q++) p[q] = a.prm[(size_t)q * n + m];
+
+    int st = 0;
+    if (!a.weights_in && p[OPS_P_TAGE] <= 0.0) st = 2;                          // batch_check_models :1312
+    if (!a.smooth_fast && p[OPS_P_SIGMA_SMOOTH] > OPS_SAFE_FLOOR) st = 2;     // host smoothing modes: not built
+    // split_parts = 0 folds young and old into one sum: only without birth-cloud and old-star screens
+    bool needs_parts = (p[OPS_P_DUST1] > OPS_SAFE_FLOOR && 1.0 - p[OPS_P_FRAC_OBRUN] > OPS_SAFE_FLOOR) ||
+                       fabs(p[OPS_P_DUST3]) > OPS_SAFE_FLOOR;
+    if (needs_parts && a.parts == 1) st = 4;
+    if (st == 0) {
+        if (a.weights_in) {
+            for (int k = lane; k < nt; k += 64) w[k] = a.weights_in[(size_t)m * nt + k];
+            WAVE_SYNC();
+        } else {
+            st = wave_sfh_weights(a.time, nt, a.tiny_logt, p, w, t1, t2, m0s, m1s, lane);
+        }
+    }
+    if (st != 0) {
+        if (lane == 0) {
+            atomicMax(a.flag, st);
+            if (a.status) a.status[m] = st;
+        }
+        for (int k = lane; k < nt; k += 64) w[k] = 0.0;
+        WAVE_SYNC();
+    } else if (lane == 0 && a.status) {
+        a.status[m] = 0;
+    }
+
+    // stage 1 indices: batch_model_indices (opensps_csp_batch.f90:1541-1576), metallicity_interval (:2073-2084)
+    double dal = p[OPS_P_DUST_TESC] > OPS_SAFE_FLOOR ? p[OPS_P_DUST_TESC] : 7.0;
+    int it = max(1, min(find_interval_ref(a.time, nt, dal), nt));
+    int zlo = 1;
+    double dz = 0.0;
+    if (a.nz > 1) {
+        zlo = max(1, min(find_interval_ref(a.logz, a.nz, p[OPS_P_LOGZSOL]), a.nz - 1));
+        dz = (p[OPS_P_LOGZSOL] - a.logz[zlo - 1]) / (a.logz[zlo] - a.logz[zlo - 1]);
+    }
+
+    // model_sums (:2129-2176): mass and L_bol over the ages (terms per lane, then a wave sum)
+    double ms = 0.0, ls = 0.0;
+    for (int k = lane; k < nt; k += 64) {
+        double wk = w[k];
+        if (wk > OPS_SAFE_FLOOR) {
+            double mk, lk;
+            if (a.nz > 1) {
+                mk = (1.0 - dz) * a.bmass[(size_t)(zlo - 1) * nt + k] + dz * a.bmass[(size_t)zlo * nt + k];
+                lk = (1.0 - dz) * a.blbol[(size_t)(zlo - 1) * nt + k] + dz * a.blbol[(size_t)zlo * nt + k];
+                mk = wk * mk;
+                lk = wk * pow(10.0, lk);
+            } else {
+                mk = wk * a.bmass[k];
+                lk = wk * pow(10.0, a.blbol[k]);
+            }
+            ms += mk;
+            ls += lk;
+        }
+    }
+    ms = wave_sum(ms);
+    ls = wave_sum(ls);
+    if (lane == 0) {
+        if (a.mass) a.mass[m] = ms;
+        if (a.lbol) a.lbol[m] = ls > OPS_SAFE_FLOOR ? log10(ls) : 0.0;
+        a.zlo[m] = zlo;
+        a.it[m] = it;
+        if (a.idx) { a.idx[3 * m] = it; a.idx[3 * m + 1] = zlo; a.idx[3 * m + 2] = st; }
+    }
+    if (a.w_out)
+        for (int k = lane; k < nt; k += 64) a.w_out[(size_t)m * nt + k] = w[k];
+
+    // The model's columns of the age-sum product: K = [slab zlo ages | slab zlo+1 ages] (blend) or the
+    // ages (one basis); the weight below SAFE_FLOOR is 0 (model_sums); young columns keep the ages
+    // <= it_tesc, old ones the others; with one part both.
+    const double c2 = dz, c1 = 1.0 - dz;
+    for (int part = 0; part < a.parts; part++) {
+        size_t col = (size_t)m * a.parts + part;
+        for (int k = lane; k < a.kpad; k += 64) {
+            double v = 0.0;
+            if (k < a.kpair) {
+                int age = (a.nz > 1) ? (k % nt) : k;   // 0-based age
+                double wk = w[age];
+                if (!(wk > OPS_SAFE_FLOOR)) wk = 0.0;
+                bool young = age + 1 <= it;
+                bool keep = (a.parts == 1) || (part == 0 ? young : !young);
+                if (keep) v = (a.nz > 1) ? ((k < nt) ? c1 * wk : c2 * wk) : wk;
+            }
+            if (a.mixed) {
+                uint16_t hi, lo;
+                split_bf16(v, hi, lo);
+                a.whi[col * a.kpad + k] = hi;
+                a.wlo[col * a.kpad + k] = lo;
+            } else {
+                a.w64[col * a.kpad + k] = v;
+            }
+        }
+    }
+}
+
+#pragma clang fp contract(on)
+
+// ------------------------------------------------------------------------------------------------
+// k_group: the models sorted by metallicity pair (stable), the column tiles of each pair
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_group(GroupArgs a) {
+    extern __shared__ int cnt[];   // [256][npair]
+    __shared__ int gstart[256], gtot[256], tbase[257];
+    const int t = threadIdx.x, G = a.npair, n = a.n;
+    const int chunk = (n + 255) / 256;
+    const int m0 = t * chunk, m1 = min(n, m0 + chunk);
+    for (int g = 0; g < G; g++) cnt[t * G + g] = 0;
+    for (int m = m0; m < m1; m++) cnt[t * G + (a.zlo[m] - 1)]++;
+    __syncthreads();
+    for (int g = t; g < G; g += 256) {
+        int run = 0;
+        for (int u = 0; u < 256; u++) { int v = cnt[u * G + g]; cnt[u * G + g] = run; run += v; }
+        gtot[g] = run;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int s = 0, tb = 0;
+        for (int g = 0; g < G; g++) {
+            gstart[g] = s;
+            tbase[g] = tb;
+            s += gtot[g];
+            tb += (gtot[g] * a.parts + OPS_TILE_COL - 1) / OPS_TILE_COL;
+        }
+        tbase[G] = tb;
+        *a.ntiles = tb;
+    }
+    __syncthreads();
+    for (int m = m0; m < m1; m++) {
+        int g = a.zlo[m] - 1;
+        int pos = gstart[g] + cnt[t * G + g]++;
+        a.order[pos] = m;
+    }
+    for (int g = t; g < G; g += 256) {
+        int ncol = gtot[g] * a.parts;
+        for (int i = 0; tbase[g] + i < tbase[g + 1]; i++) {
+            int4 d;
+            d.x = g;
+            d.y = gstart[g] * a.parts + i * OPS_TILE_COL;
+            d.z = min(OPS_TILE_COL, ncol - i * OPS_TILE_COL);
+            d.w = 0;
+            a.tiles[tbase[g] + i] = d;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Age-sum products on the matrix cores. Workgroup tile: 64 pixels x 64 columns, four wavefronts of
+// 32 x 32 (2 x 2 blocks of 16 x 16). Operands are packed in instruction order: A at set-up (the pair's
+// two slabs side by side in K), B per model by k_prep. The blockIdx -> tile map keeps each XCD on
+// one eighth of the pixel tiles, so a pair's A rows stay in that XCD's L2 across column tiles.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool gemm_tile(const GemmArgs& a, int& pt, int& ct) {
+    const int b = blockIdx.x;
+    const int xcd = b & 7, r = b >> 3;
+    const int ptx = (a.npxt + 7) >> 3;   // pixel tiles per XCD
+    pt = (r % ptx) * 8 + xcd;
+    ct = r / ptx;
+    return pt < a.npxt && ct < *a.ntiles;
+}
+
+__global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs a) {
+    int pt, ct;
+    if (!gemm_tile(a, pt, ct)) return;
+    const int4 td = a.tiles[ct];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    const int blk0 = pt * 4 + wr * 2;   // 16-pixel blocks of this wave
+    const int ksteps = a.kpad / OPS_KSTEP_F64;
+    const double* Wc[2];
+    int model[2], part[2];
+    bool valid[2];
+    for (int cb = 0; cb < 2; cb++) {
+        int c = wc * 32 + cb * 16 + (lane & 15);
+        valid[cb] = c < td.z;
+        int cc = td.y + (valid[cb] ? c : 0);
+        int pos = cc / a.parts;
+        part[cb] = cc - pos * a.parts;
+        model[cb] = a.order[pos];
+        Wc[cb] = a.w64 + ((size_t)model[cb] * a.parts + part[cb]) * a.kpad;
+    }
+    f64x4_t acc[2][2];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = (f64x4_t){0.0, 0.0, 0.0, 0.0};
+    const double* Ab = a.A64 + ((size_t)td.x * a.npxblk + blk0) * ksteps * 64 + lane;
+    const int kl = lane >> 4;
+    for (int ks = 0; ks < ksteps; ks++) {
+        double av[2], bv[2];
+        av[0] = Ab[(size_t)ks * 64];
+        av[1] = Ab[(size_t)(ksteps + ks) * 64];
+        bv[0] = valid[0] ? Wc[0][ks * 4 + kl] : 0.0;
+        bv[1] = valid[1] ? Wc[1][ks * 4 + kl] : 0.0;
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+    double* yo = (double*)a.yo;
+    for (int j = 0; j < 2; j++) {
+        if (!valid[j]) continue;
+        double* col = yo + ((size_t)part[j] * a.n + model[j]) * a.nsp;
+        for (int i = 0; i < 2; i++) {
+            int px = (blk0 + i) * 16 + 4 * (lane >> 4);
+            for (int r = 0; r < 4; r++) col[px + r] = acc[i][j][r];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs a) {
+    int pt, ct;
+    if (!gemm_tile(a, pt, ct)) return;
+    const int4 td = a.tiles[ct];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    const int blk0 = pt * 4 + wr * 2;
+    const int ksteps = a.kpad / OPS_KSTEP_BF16;
+    const uint16_t *Wh[2], *Wl[2];
+    int model[2], part[2];
+    bool valid[2];
+    for (int cb = 0; cb < 2; cb++) {
+        int c = wc * 32 + cb * 16 + (lane & 15);
+        valid[cb] = c < td.z;
+        int cc = td.y + (valid[cb] ? c : 0);
+        int pos = cc / a.parts;
+        part[cb] = cc - pos * a.parts;
+        model[cb] = a.order[pos];
+        size_t off = ((size_t)model[cb] * a.parts + part[cb]) * a.kpad + 8 * (lane >> 4);
+        Wh[cb] = a.whi + off;
+        Wl[cb] = a.wlo + off;
+    }
+    f32x4_t acc[2][2];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    const size_t abase = ((size_t)td.x * a.npxblk + blk0) * ksteps * 64 + lane;
+    const bf16x8_t* Ah = (const bf16x8_t*)a.Ahi + abase;
+    const bf16x8_t* Al = (const bf16x8_t*)a.Alo + abase;
+    const bf16x8_t zero8 = (bf16x8_t){};
+    for (int ks = 0; ks < ksteps; ks++) {
+        bf16x8_t ah[2], al[2], bh[2], bl[2];
+        ah[0] = Ah[(size_t)ks * 64];
+        ah[1] = Ah[(size_t)(ksteps + ks) * 64];
+        al[0] = Al[(size_t)ks * 64];
+        al[1] = Al[(size_t)(ksteps + ks) * 64];
+        for (int j = 0; j < 2; j++) {
+            bh[j] = valid[j] ? *(const bf16x8_t*)(Wh[j] + ks * 32) : zero8;
+            bl[j] = valid[j] ? *(const bf16x8_t*)(Wl[j] + ks * 32) : zero8;
+        }
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            }
+    }
+    float* yo = (float*)a.yo;
+    for (int j = 0; j < 2; j++) {
+        if (!valid[j]) continue;
+        float* col = yo + ((size_t)part[j] * a.n + model[j]) * a.nsp;
+        for (int i = 0; i < 2; i++) {
+            int px = (blk0 + i) * 16 + 4 * (lane >> 4);
+            *(f32x4_t*)(col + px) = acc[i][j];
+        }
+    }
+}
+
+// Packing of the resident bases into the instruction order of the products (once per grid)
+__global__ void k_pack_f64(PackArgs a) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ksteps = a.kpad / OPS_KSTEP_F64;
+    size_t total = (size_t)a.npair * a.npxblk * ksteps * 64;
+    if (e >= total) return;
+    int lane = e % 64;
+    size_t r = e / 64;
+    int ks = r % ksteps;
+    r /= ksteps;
+    int blk = r % a.npxblk;
+    int pair = r / a.npxblk;
+    int row = blk * 16 + (lane & 15);
+    int k = ks * 4 + (lane >> 4);
+    double v = 0.0;
+    if (row < a.nspec && k < a.kpair) {
+        int slab = (a.nz > 1) ? pair + (k >= a.nt ? 1 : 0) : 0;
+        int age = (a.nz > 1) ? k % a.nt : k;
+        v = a.basis[((size_t)slab * a.nt + age) * a.nspec + row];
+    }
+    a.A64[e] = v;
+}
+
+__global__ void k_pack_bf16(PackArgs a) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ksteps = a.kpad / OPS_KSTEP_BF16;
+    size_t total = (size_t)a.npair * a.npxblk * ksteps * 64;
+    if (e >= total) return;
+    int lane = e % 64;
+    size_t r = e / 64;
+    int ks = r % ksteps;
+    r /= ksteps;
+    int blk = r % a.npxblk;
+    int pair = r / a.npxblk;
+    int row = blk * 16 + (lane & 15);
+    for (int q = 0; q < 8; q++) {
+        int k = ks * 32 + 8 * (lane >> 4) + q;
+        double v = 0.0;
+        if (row < a.nspec && k < a.kpair) {
+            int slab = (a.nz > 1) ? pair + (k >= a.nt ? 1 : 0) : 0;
+            int age = (a.nz > 1) ? k % a.nt : k;
+            v = a.basis[((size_t)slab * a.nt + age) * a.nspec + row];
+        }
+        uint16_t hi, lo;
+        split_bf16(v, hi, lo);
+        a.Ahi[e * 8 + q] = hi;
+        a.Alo[e * 8 + q] = lo;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_chain: one workgroup per model. Stages 5-8 of compute_csp_batch in the reference's order:
+// attenuation (attenuate_batch, pixel_transmissions: opensps_csp_batch.f90:3576-3818), IGM
+// (apply_igm_z :4205-4319), velocity smoothing (build_velocity_kernel, src/spectra/opensps_smoothing.f90:553-633,
+// formed per pixel and model on the fly), the observed frame (observed_frame_z,
+// src/spectra/opensps_batch_photometry.f90:819-903), the band integrals (band_fluxes :1049-1171) and the
+// magnitudes (magnitudes_from_fluxes :185-232). T: the kernel precision (double, or float with FP64
+// formed arguments).
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T dust_exp(double x);
+template <>
+__device__ __forceinline__ double dust_exp<double>(double x) { return exp(x); }
+template <>
+__device__ __forceinline__ float dust_exp<float>(double x) { return exp_kp(x); }
+
+// exp(-arg) of a smoothing weight (arg <= 8 inside the +-4 sigma window): FP64, or FP32 of the FP64-formed argument
+template <typename T>
+__device__ __forceinline__ double smooth_exp(double arg);
+template <>
+__device__ __forceinline__ double smooth_exp<double>(double arg) { return exp(-arg); }
+template <>
+__device__ __forceinline__ double smooth_exp<float>(double arg) { return (double)__expf(-(float)arg); }
+
+template <typename T>
+__device__ __forceinline__ T tr_exp(T x);
+template <>
+__device__ __forceinline__ double tr_exp<double>(double x) { return exp(x); }
+template <>
+__device__ __forceinline__ float tr_exp<float>(float x) { return expf(x); }
+
+template <typename T>
+__global__ __launch_bounds__(512) void k_chain(ChainArgs a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* s0 = (T*)lds_raw;
+    T* s1 = s0 + a.nsp;
+    __shared__ double red_v[16];
+    __shared__ int red_j[16];
+    __shared__ double sh_dist;
+    const int m = blockIdx.x, n = a.n, nspec = a.nspec;
+    const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wv = tid >> 6, nwv = nth >> 6;
+    const double z = a.prm[(size_t)OPS_P_ZRED * n + m];
+    const double dust1 = a.prm[(size_t)OPS_P_DUST1 * n + m], dust2 = a.prm[(size_t)OPS_P_DUST2 * n + m];
+    const double dust3 = a.prm[(size_t)OPS_P_DUST3 * n + m];
+    const double ed = a.prm[(size_t)OPS_P_DUST_INDEX * n + m], ed1 = a.prm[(size_t)OPS_P_DUST1_INDEX * n + m];
+    const double fob = a.prm[(size_t)OPS_P_FRAC_OBRUN * n + m], fnd = a.prm[(size_t)OPS_P_FRAC_NODUST * n + m];
+    const double igmf = a.prm[(size_t)OPS_P_IGM_FACTOR * n + m];
+    const double sig = a.prm[(size_t)OPS_P_SIGMA_SMOOTH * n + m];
+    const double minl = a.prm[(size_t)OPS_P_MIN_WAVE_SMOOTH * n + m], maxl = a.prm[(size_t)OPS_P_MAX_WAVE_SMOOTH * n + m];
+    const bool no_birth = dust1 <= OPS_SAFE_FLOOR || 1.0 - fob <= OPS_SAFE_FLOOR;
+    const bool no_old = fabs(dust3) <= OPS_SAFE_FLOOR;
+    const bool no_diffuse_mix = 1.0 - fnd <= OPS_SAFE_FLOOR;
+    const bool igm_on = a.add_igm && z > OPS_SAFE_FLOOR && a.jtop > 0;
+
+    // IGM: the optical depth's cap needs its maximum over the pixels below Lyman-alpha first
+    double b6[6];
+    double tmax = 0.0;
+    int jmax = 1;
+    if (igm_on) {
+        double opz = 1.0 + z;
+        b6[0] = pow(opz, 3.46);
+        b6[1] = pow(opz, 1.68);
+        b6[2] = pow(opz, 3.0 + 0.46);
+        b6[3] = pow(opz, 1.5 + 0.18);
+        b6[4] = pow(opz, 3.0 - 1.32);
+        b6[5] = pow(opz, 1.68);
+        double v = 0.0;
+        int jv = 1;
+        for (int j = tid; j < a.jtop; j += nth) {
+            const double* aj = a.igm_a + (size_t)j * 6;
+            double tau = aj[0] * b6[0] + aj[1] * b6[1] + aj[2] * b6[2] + aj[3] * b6[3] + aj[4] * b6[4] + aj[5] * b6[5];
+            argmax_merge(v, jv, tau, j + 1);
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            double v2 = __shfl_xor(v, o, 64);
+            int j2 = __shfl_xor(jv, o, 64);
+            argmax_merge(v, jv, v2, j2);
+        }
+        if (lane == 0) { red_v[wv] = v; red_j[wv] = jv; }
+        __syncthreads();
+        tmax = 0.0; jmax = 1;
+        for (int q = 0; q < nwv; q++) argmax_merge(tmax, jmax, red_v[q], red_j[q]);
+    }
+
+    // Attenuation and IGM -> s0
+    const T* y0 = (const T*)a.yo + (size_t)m * a.nsp;
+    const T* o0 = a.parts == 2 ? (const T*)a.yo + ((size_t)n + m) * a.nsp : nullptr;
+    const T cd2 = (T)dust2, cd1 = (T)dust1, cd3 = (T)dust3, cfob = (T)fob, cfnd = (T)fnd;
+    for (int i = tid; i < nspec; i += nth) {
+        T y = y0[i], o = o0 ? o0[i] : (T)0;
+        if (a.young) ((T*)a.young)[(size_t)m * nspec + i] = y;
+        if (a.old) ((T*)a.old)[(size_t)m * nspec + i] = o;
+        double L = a.ltab[i];
+        T curve = (ed != 0.0) ? dust_exp<T>(ed * L) : (T)1;
+        T tr_d = tr_exp<T>(-(cd2 * curve));
+        T tr_b = (T)1, tr_o = (T)1;
+        if (!no_birth) tr_b = tr_exp<T>(-(cd1 * dust_exp<T>(ed1 * L)));
+        if (!no_old) tr_o = tr_exp<T>(-(cd3 * curve));
+        T ssum = (y * tr_b * ((T)1 - cfob) + y * cfob) + o * tr_o;
+        T tot = no_diffuse_mix ? ssum : ssum * (tr_d * ((T)1 - cfnd) + cfnd);
+        if (igm_on && i < a.jtop) {
+            double tau;
+            if (jmax > 1 && i + 1 <= jmax) {
+                tau = tmax;
+            } else {
+                const double* aj = a.igm_a + (size_t)i * 6;
+                tau = aj[0] * b6[0] + aj[1] * b6[1] + aj[2] * b6[2] + aj[3] * b6[3] + aj[4] * b6[4] + aj[5] * b6[5];
+            }
+            tot = tot * dust_exp<T>(-tau * igmf);
+        }
+        s0[i] = tot;
+    }
+    __syncthreads();
+
+    // Velocity smoothing -> s1, and the spectrum out
+    const bool smooth = a.smooth_fast && sig > OPS_SAFE_FLOOR;
+    double sb = 0.0, inv2 = 0.0, fblue = 0.0, fred = 0.0;
+    if (smooth) {
+        sb = sig / (OPS_C_LIGHT * 1.0e-13);
+        inv2 = 0.5 * (1.0 / sb) * (1.0 / sb);
+        fblue = exp(-4.0 * sb);
+        fred = exp(4.0 * sb);
+    }
+    T* spec_out = a.spec ? (T*)a.spec + (size_t)m * nspec : nullptr;
+    for (int i = tid; i < nspec; i += nth) {
+        T out = s0[i];
+        double li = a.lam[i];
+        if (smooth && li >= minl && li <= maxl) {
+            double vl = li * fblue, vh = li * fred;
+            int il = i, ih = i;
+            while (il > 0 && a.lam[il] > vl) il--;
+            while (ih < nspec - 1 && a.lam[ih] < vh) ih++;
+            if (ih - il + 1 >= 2) {
+                double lli = a.loglam[i];
+                double tw = 0.0, acc = 0.0;
+                for (int k = il; k <= ih; k++) {
+                    double d = lli - a.loglam[k];
+                    double wk = smooth_exp<T>(inv2 * (d * d)) / a.lam[k];
+                    double dl = k > il ? a.dlam[k - 1] : 0.0;
+                    double dr = k < ih ? a.dlam[k] : 0.0;
+                    wk = 0.5 * wk * (dl + dr);
+                    tw += wk;
+                    acc += wk * (double)s0[k];
+                }
+                if (tw > OPS_SAFE_FLOOR) out = (T)(acc / tw);
+            }
+        }
+        s1[i] = out;
+        if (spec_out) spec_out[i] = out;
+    }
+    __syncthreads();
+
+    // Observed frame over the pixels the bands read -> s0
+    const int J0 = a.band_j0, J1 = a.band_j1;
+    if (J1 >= J0) {
+        const int npx = J1 - J0 + 1;
+        const int per = (npx + nth - 1) / nth;
+        const int j0 = J0 + tid * per, j1 = min(J1, j0 + per - 1);
+        const bool rest = fabs(z) <= OPS_SAFE_FLOOR;
+        int il = -1;
+        for (int j = j0; j <= j1; j++) {
+            double v;
+            if (rest) {
+                v = (double)s1[j];
+            } else {
+                double tg = a.lam[j] / (1.0 + z);
+                if (il < 0) {
+                    il = max(1, min(nspec - 1, count_le(a.lam, nspec, tg))) - 1;
+                } else {
+                    while (il < nspec - 2 && a.lam[il + 1] <= tg) il++;
+                }
+                double t = (tg - a.lam[il]) / (a.lam[il + 1] - a.lam[il]);
+                v = (double)s1[il] + t * ((double)s1[il + 1] - (double)s1[il]);
+            }
+            if (v != v) v = 0.0;
+            s0[j] = (T)fmax(v, 0.0) * (T)a.wl[j];
+        }
+    }
+    if (tid == 0) {
+        double dist = 0.0;
+        if (fabs(z) > OPS_SAFE_FLOOR) {
+            int i = max(1, min(find_interval_ref(a.cz, a.ncosmo, z), a.ncosmo - 1));
+            double slope = (a.cdl[i] - a.cdl[i - 1]) / (a.cz[i] - a.cz[i - 1]);
+            double dm = a.cdl[i - 1] + slope * (z - a.cz[i - 1]);
+            if (dm == dm && dm > OPS_SAFE_FLOOR) dist = 5.0 * log10(dm / 10.0) + (-2.5 * log10(1.0 + z));
+        }
+        sh_dist = dist;
+    }
+    __syncthreads();
+
+    // Band integrals (FP64 sums) and magnitudes
+    const double dist = sh_dist;
+    for (int b = wv; b < a.nbands; b += nwv) {
+        const int nj = a.bnj[b], j0 = a.bjlo[b], off = a.bptr[b];
+        double s = 0.0;
+        if (a.mask[b])
+            for (int q = lane; q < nj; q += 64) s += (double)coef_at<T>(a, off + q) * (double)s0[j0 + q];
+        s = wave_sum(s);
+        if (lane == 0) {
+            double mag = NAN;
+            if (a.mask[b]) mag = (s > 0.0 ? -2.5 * log10(s) : NAN) - 48.60 - 2.5 * a.amzl + dist;
+            a.mags[(size_t)m * a.nbands + b] = mag;
+            if (a.fband) a.fband[(size_t)m * a.nbands + b] = s;
+        }
+    }
+}
+
+template __global__ void k_chain<double>(ChainArgs);
+template __global__ void k_chain<float>(ChainArgs);
+
+// ------------------------------------------------------------------------------------------------
+// Kernel-level entry points: band_fluxes and observed_frame_z in the reference's own layouts
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void k_band_fluxes(int n, int nspec, int nbands, const T* bt, const int* jlo, const int* jhi, const int* do_band,
+                              const T* wt, const T* fobs_t, double* fband) {
+    const int b = blockIdx.y, m = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (m >= n) return;
+    double s = 0.0;
+    if (do_band[b])
+        for (int j = jlo[b] + lane; j <= jhi[b]; j += 64)
+            s += (double)wt[j - 1] * ((double)fobs_t[(size_t)(j - 1) * n + m] * (double)bt[(size_t)(j - 1) * nbands + b] +
+                                      (double)fobs_t[(size_t)j * n + m] * (double)bt[(size_t)j * nbands + b]);
+    s = wave_sum(s);
+    if (lane == 0) fband[(size_t)m * nbands + b] = s;
+}
+template __global__ void k_band_fluxes<double>(int, int, int, const double*, const int*, const int*, const int*,
+                                               const double*, const double*, double*);
+template __global__ void k_band_fluxes<float>(int, int, int, const float*, const int*, const int*, const int*,
+                                              const float*, const float*, double*);
+
+template <typename T>
+__global__ void k_observed_frame(int n, int nspec, const double* zc, const double* lam, const T* total, T* fobs_t) {
+    const int m = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nspec) return;
+    const double z = zc[m];
+    const T* tc = total + (size_t)m * nspec;
+    double v;
+    if (fabs(z) > OPS_SAFE_FLOOR) {
+        double tg = lam[j] / (1.0 + z);
+        int il = max(1, min(nspec - 1, count_le(lam, nspec, tg))) - 1;
+        double t = (tg - lam[il]) / (lam[il + 1] - lam[il]);
+        v = (double)tc[il] + t * ((double)tc[il + 1] - (double)tc[il]);
+    } else {
+        v = (double)tc[j];
+    }
+    if (v != v) v = 0.0;
+    double wl = lam[j] > 2.2250738585072014e-308 ? 1.0 / lam[j] : 0.0;
+    fobs_t[(size_t)j * n + m] = (T)fmax(v, 0.0) * (T)wl;
+}
+template __global__ void k_observed_frame<double>(int, int, const double*, const double*, const double*, double*);
+template __global__ void k_observed_frame<float>(int, int, const double*, const double*, const float*, float*);
+
+// Self-test of the matrix-instruction lane layouts the products assume (one wavefront): which = 0 runs
+// f64 16x16x4 on a[64], b[64] (doubles); which = 1 runs bf16 16x16x32 on a[64][8], b[64][8] (uint16).
+// d[64][4] receives each lane's accumulator.
+__global__ void k_selftest_mfma(int which, const void* a, const void* b, void* d) {
+    const int lane = threadIdx.x;
+    if (which == 0) {
+        f64x4_t acc = (f64x4_t){0.0, 0.0, 0.0, 0.0};
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(((const double*)a)[lane], ((const double*)b)[lane], acc, 0, 0, 0);
+        for (int r = 0; r < 4; r++) ((double*)d)[lane * 4 + r] = acc[r];
+    } else {
+        f32x4_t acc = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+        bf16x8_t av = ((const bf16x8_t*)a)[lane], bv = ((const bf16x8_t*)b)[lane];
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+        for (int r = 0; r < 4; r++) ((float*)d)[lane * 4 + r] = acc[r];
+    }
+}

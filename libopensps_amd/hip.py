@@ -1,0 +1,206 @@
+"""ctypes binding of libopensps_hip.so (include/opensps_hip.h) with device buffers from PyTorch.
+
+PyTorch provides device memory and streams only; every computation is a kernel of libopensps_hip.so.
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import build as _build
+from .params import NPAR
+
+FP64 = 0
+MIXED = 1
+
+_lib = None
+
+
+class BatchArgs(C.Structure):
+    _fields_ = [("n", C.c_int), ("precision", C.c_int), ("split_parts", C.c_int), ("prm", C.c_void_p),
+                ("ssp_weights", C.c_void_p), ("mags", C.c_void_p), ("spec", C.c_void_p), ("mass", C.c_void_p),
+                ("lbol", C.c_void_p), ("fband", C.c_void_p), ("young", C.c_void_p), ("old", C.c_void_p),
+                ("idx", C.c_void_p), ("weights", C.c_void_p)]
+
+
+class OpenSPSHipError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_build.HIP_SO):
+            raise OpenSPSHipError(1, f"{_build.HIP_SO} is not built (run __graft_entry__.build())")
+        L = C.CDLL(_build.HIP_SO)
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        L.opensps_hip_create.argtypes = [i, C.POINTER(vp)]
+        L.opensps_hip_destroy.argtypes = [vp]
+        L.opensps_hip_set_stream.argtypes = [vp, vp]
+        L.opensps_hip_set_grid.argtypes = [vp, i, vp, i, vp, i, vp, vp, vp, vp]
+        L.opensps_hip_set_bands.argtypes = [vp, i, vp]
+        L.opensps_hip_set_mag_mask.argtypes = [vp, i, vp]
+        L.opensps_hip_set_cosmology.argtypes = [vp, i, vp, vp]
+        L.opensps_hip_set_options.argtypes = [vp, i, i]
+        L.opensps_hip_csp_batch.argtypes = [vp, C.POINTER(BatchArgs)]
+        L.opensps_hip_sync.argtypes = [vp]
+        L.opensps_hip_band_fluxes.argtypes = [i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.opensps_hip_observed_frame.argtypes = [i, i, i, vp, vp, vp, vp, vp]
+        L.opensps_hip_selftest_mfma.argtypes = [i, vp, vp, vp]
+        L.opensps_hip_last_error.argtypes = [vp, C.c_char_p, i]
+        L.opensps_hip_device_count.argtypes = [C.POINTER(i)]
+        _lib = L
+    return _lib
+
+
+def last_error(ctx=None) -> str:
+    buf = C.create_string_buffer(512)
+    lib().opensps_hip_last_error(ctx, buf, 512)
+    return buf.value.decode()
+
+
+def check(st, ctx=None):
+    if st != 0:
+        raise OpenSPSHipError(st, last_error(ctx))
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _host(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class DeviceContext:
+    """One libopensps_hip context: resident bases, filters and cosmology on the GPU."""
+
+    def __init__(self, device: int = 0):
+        import torch  # noqa: F401  (device memory for the batch buffers)
+        self._ctx = C.c_void_p()
+        check(lib().opensps_hip_create(device, C.byref(self._ctx)))
+        self.grid = None
+        self.nbands = 0
+
+    def close(self):
+        if self._ctx:
+            lib().opensps_hip_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        check(st, self._ctx)
+
+    def set_stream(self, stream_handle):
+        self._check(lib().opensps_hip_set_stream(self._ctx, C.c_void_p(stream_handle)))
+
+    def set_grid(self, grid):
+        lam, age, logz = _host(grid.lam), _host(grid.log_age), _host(grid.logz)
+        spec, mass, lbol = _host(grid.spec), _host(grid.mass), _host(grid.lbol)
+        self._check(lib().opensps_hip_set_grid(self._ctx, grid.nspec, lam.ctypes.data, grid.nt, age.ctypes.data,
+                                               grid.nz, logz.ctypes.data, spec.ctypes.data, mass.ctypes.data,
+                                               lbol.ctypes.data))
+        self.grid = grid
+
+    def set_bands(self, bands):
+        b = _host(bands)
+        self._check(lib().opensps_hip_set_bands(self._ctx, b.shape[0], b.ctypes.data))
+        self.nbands = b.shape[0]
+
+    def set_mag_mask(self, mask):
+        m = None if mask is None else _host(mask, np.int32)
+        self._check(lib().opensps_hip_set_mag_mask(self._ctx, self.nbands, None if m is None else m.ctypes.data))
+
+    def set_cosmology(self, z, dl):
+        z, dl = _host(z), _host(dl)
+        self._check(lib().opensps_hip_set_cosmology(self._ctx, z.size, z.ctypes.data, dl.ctypes.data))
+
+    def set_options(self, add_igm=1, smooth_fast=1):
+        self._check(lib().opensps_hip_set_options(self._ctx, int(add_igm), int(smooth_fast)))
+
+    def alloc_outputs(self, n, precision, want=("mags", "spec", "mass", "lbol")):
+        import torch
+        dev = "cuda"
+        sdt = torch.float32 if precision == MIXED else torch.float64
+        g = self.grid
+        out = {"mags": torch.empty((n, self.nbands), dtype=torch.float64, device=dev)}
+        shapes = {"spec": ((n, g.nspec), sdt), "mass": ((n,), torch.float64), "lbol": ((n,), torch.float64),
+                  "fband": ((n, self.nbands), torch.float64), "young": ((n, g.nspec), sdt),
+                  "old": ((n, g.nspec), sdt), "idx": ((n, 3), torch.int32), "weights": ((n, g.nt), torch.float64)}
+        for k in want:
+            if k in shapes:
+                out[k] = torch.empty(shapes[k][0], dtype=shapes[k][1], device=dev)
+        return out
+
+    def launch(self, prm, out, precision=MIXED, split_parts=0, ssp_weights=None):
+        """Enqueues one batch on device-resident prm (NPAR, n) float64 and the buffers of out."""
+        n = prm.shape[1]
+        assert prm.shape[0] == NPAR and prm.is_contiguous() and prm.dtype.itemsize == 8
+        a = BatchArgs()
+        a.n, a.precision, a.split_parts = n, precision, int(split_parts)
+        a.prm = prm.data_ptr()
+        a.ssp_weights = ssp_weights.data_ptr() if ssp_weights is not None else None
+        for k in ("mags", "spec", "mass", "lbol", "fband", "young", "old", "idx", "weights"):
+            setattr(a, k, out[k].data_ptr() if k in out else None)
+        self._check(lib().opensps_hip_csp_batch(self._ctx, C.byref(a)))
+
+    def sync(self):
+        self._check(lib().opensps_hip_sync(self._ctx))
+
+    def batch(self, prm, precision=MIXED, split_parts=0, ssp_weights=None, want=("mags", "spec", "mass", "lbol")):
+        import torch
+        n = prm.shape[1]
+        out = self.alloc_outputs(n, precision, want)
+        torch.cuda.synchronize()
+        self.launch(prm, out, precision, split_parts, ssp_weights)
+        self.sync()
+        return out
+
+
+def needs_split(prm_host) -> bool:
+    """True if some model has a birth-cloud or an old-star screen (the young and old sums are then apart)."""
+    from .params import SLOT
+    tiny = np.finfo(np.float64).tiny
+    d1, fob, d3 = prm_host[SLOT["dust1"]], prm_host[SLOT["frac_obrun"]], prm_host[SLOT["dust3"]]
+    return bool(np.any(((d1 > tiny) & (1.0 - fob > tiny)) | (np.abs(d3) > tiny)))
+
+
+def selftest_mfma(which, a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    d = np.zeros((64, 4), dtype=np.float64 if which == 0 else np.float32)
+    check(lib().opensps_hip_selftest_mfma(which, a.ctypes.data, b.ctypes.data, d.ctypes.data))
+    return d
+
+
+def band_fluxes(precision, bands_t, jlo, jhi, do_band, wt, fobs_t):
+    """Device tensors: bands_t (nspec, nbands) C order [= Fortran (nbands, nspec)], fobs_t (nspec, n) C order
+    [= Fortran (n, nspec)]. Returns fband (n, nbands)."""
+    import torch
+    nspec, nbands = bands_t.shape
+    n = fobs_t.shape[1]
+    out = torch.zeros((n, nbands), dtype=torch.float64, device=bands_t.device)
+    torch.cuda.synchronize()
+    check(lib().opensps_hip_band_fluxes(precision, n, nspec, nbands, _ptr(bands_t), _ptr(jlo), _ptr(jhi),
+                                        _ptr(do_band), _ptr(wt), _ptr(fobs_t), _ptr(out), None))
+    return out
+
+
+def observed_frame(precision, zc, lam, total):
+    """Device tensors: zc (n), lam (nspec), total (n, nspec) C order. Returns fobs_t (nspec, n) C order."""
+    import torch
+    n, nspec = total.shape
+    out = torch.zeros((nspec, n), dtype=total.dtype, device=total.device)
+    torch.cuda.synchronize()
+    check(lib().opensps_hip_observed_frame(precision, n, nspec, _ptr(zc), _ptr(lam), _ptr(total), _ptr(out), None))
+    return out

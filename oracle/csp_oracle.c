@@ -1,0 +1,789 @@
+/*
+ * csp_oracle.c -- FP64 CPU restatement of OpenSPS's batched composite-population path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the correctness oracle of the GPU path: only the tests,
+ * the smoke check and the CPU-baseline leg of the benchmark load it, and only as the checker. The
+ * product never links or calls it.
+ *
+ * Each function restates one routine of the reference's default FP64 build (kernel_precision =
+ * double, CPU) and cites it. The reference itself cannot be built here under the project's rules
+ * (46 Fortran modules behind Meson plus three configure templates that have to be generated), so
+ * this restatement is the oracle, pinned by the checks in tests/test_oracle_*.py: brute force on
+ * tiny grids, closed forms, independent library routines (scipy) and the reference's own unit-test
+ * fixtures.
+ *
+ * Conventions: arrays are column-major as in the reference (quantity fastest); "index" values that
+ * the reference computes with find_interval stay 1-based and are converted where arrays are read.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/opensps_params.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* LOG_E = log10(exp(1)) (opensps_sfh.f90:61), formed as the reference forms it */
+#define LOG_E (log10(exp(1.0)))
+
+#define SAFE_FLOOR DBL_MIN                /* tiny(1.0_wp): opensps_constants.f90 */
+#define C_LIGHT 2.99792458e18             /* A/s: opensps_constants.f90 */
+#define N_SIGMA_RANGE 4.0                 /* opensps_smoothing.f90:28 */
+#define AB_ZEROPOINT 48.60                /* opensps_batch_photometry.f90:37 */
+#define N_COSMO_GRID 10000                /* opensps_cosmology.f90:43 */
+
+/* ABS_MAG_ZEROPOINT_LOG = log10(L_sun / (4 pi (10 pc)^2)) (opensps_constants.f90:87-88) */
+static double abs_mag_zeropoint_log(void) {
+    const double pi = acos(-1.0);
+    const double pc = (648000.0 / pi) * 1.495978707e13;
+    return log10(3.828e33 / (4.0 * pi * (10.0 * pc) * (10.0 * pc)));
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Interval search: find_interval_no_hint (src/math/opensps_interpolation.f90:149-198).
+ * Returns the 1-based lower index i with a(i) <= v < a(i+1) for ascending a (the mirrored
+ * condition for descending a), 0 below / n above the ends, 1 at v == a(1) and n-1 at v == a(n).
+ * ------------------------------------------------------------------------------------------- */
+int orc_find_interval(const double *a, int n, double v) {
+    if (n < 2) return 1;
+    int asc = a[n - 1] >= a[0];
+    int lower = 0, upper = n + 1;
+    while (upper - lower > 1) {
+        int mid = (upper + lower) / 2;
+        if (asc == (v >= a[mid - 1])) lower = mid; else upper = mid;
+    }
+    if (v == a[0]) return 1;
+    if (v == a[n - 1]) return n - 1;
+    return lower;
+}
+
+/* interpolate_linear_scalar (opensps_interpolation.f90:45-72) */
+static double interp_linear(const double *x, const double *y, int n, double xo) {
+    if (n < 2) return NAN;
+    int i = orc_find_interval(x, n, xo);
+    if (i < 1) i = 1;
+    if (i > n - 1) i = n - 1;
+    double slope = (y[i] - y[i - 1]) / (x[i] - x[i - 1]);
+    return y[i - 1] + slope * (xo - x[i - 1]);
+}
+
+/* mag_from_flux (src/math/opensps_special_functions.f90:34-48) */
+static double mag_from_flux(double f) { return f > 0.0 ? -2.5 * log10(f) : NAN; }
+
+/* expi: Ei(x), series for x <= 40, asymptotic above (opensps_special_functions.f90:50-122) */
+double orc_expi(double x) {
+    const double gam = 0.57721566490153286060651209008240243104215933593992;
+    if (x == 0.0) return -INFINITY;
+    if (x < 0.0) return NAN;
+    double res, r;
+    if (fabs(x) <= 40.0) {
+        res = 1.0; r = 1.0;
+        for (int k = 1; k <= 1000; k++) {
+            double td = (double)k + 1.0;
+            r = (r * (double)k * x) / (td * td);
+            res = res + r;
+            if (fabs(r) <= 1.0e-20 * fabs(res)) break;
+            if (k == 1000) return NAN;
+        }
+        return gam + log(x) + x * res;
+    }
+    res = 1.0; r = 1.0;
+    double inv_x = 1.0 / x;
+    for (int k = 1; k <= 20; k++) { r = r * (double)k * inv_x; res = res + r; }
+    return exp(x) * inv_x * res;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * SFH weights (src/physics/opensps_sfh.f90)
+ * ------------------------------------------------------------------------------------------- */
+typedef struct {
+    double tau, tage, tburst, sf_trunc, sf_slope, tq, t0, tb;
+    int type, use_simha_limits;
+} sfhp_t;
+
+/* convert_sfhparams_pc (opensps_sfh.f90:1305-1355) */
+static void convert_sfhparams(const double *p, double tage, sfhp_t *s) {
+    int sfh = (int)p[OPS_P_SFH];
+    double start = (sfh == 1 || sfh == 4 || sfh == 5) ? p[OPS_P_SF_START] * 1.0e9 : 0.0;
+    s->tage = tage * 1.0e9 - start;
+    s->tburst = p[OPS_P_TBURST] * 1.0e9 - start;
+    s->sf_trunc = p[OPS_P_SF_TRUNC] * 1.0e9 - start;
+    s->tau = p[OPS_P_TAU] * 1.0e9;
+    s->sf_slope = -p[OPS_P_SF_SLOPE] / 1.0e9;
+    s->tb = s->tage - s->tburst;
+    if (s->sf_trunc <= 0.0 || s->sf_trunc > s->tage) s->tq = 0.0; else s->tq = s->tage - s->sf_trunc;
+    if (s->sf_slope > SAFE_FLOOR) s->t0 = s->tq - 1.0 / s->sf_slope; else s->t0 = 0.0;
+    if (s->t0 > s->tq || s->t0 <= 0.0) s->t0 = 0.0;
+    s->use_simha_limits = 0;
+    s->type = 0;
+}
+
+/* eval_indefinite_moments_log (opensps_sfh.f90:852-910), types 0, 1, 4 */
+static void moments_log(double log_t, const sfhp_t *s, double *m0, double *m1) {
+    double t_lin = pow(10.0, log_t);
+    switch (s->type) {
+    case 0:
+        *m0 = t_lin;
+        *m1 = t_lin * (log_t - LOG_E);
+        return;
+    case 1: {
+        if (s->tau <= SAFE_FLOOR) { *m0 = 0.0; *m1 = 0.0; return; }
+        double tt = fmax(t_lin / s->tau, SAFE_FLOOR);
+        double ei = orc_expi(tt);
+        if (!isfinite(ei)) ei = 0.0;
+        *m0 = exp(tt);
+        *m1 = log_t * *m0 - LOG_E * ei;
+        return;
+    }
+    case 4: {
+        if (s->tau <= SAFE_FLOOR) { *m0 = 0.0; *m1 = 0.0; return; }
+        double tt = fmax(t_lin / s->tau, SAFE_FLOOR);
+        double ei = orc_expi(tt);
+        if (!isfinite(ei)) ei = 0.0;
+        double et = exp(tt);
+        *m0 = -(t_lin - s->tage - s->tau) * et;
+        double term_const = s->tau * LOG_E;
+        *m1 = -((t_lin - s->tage - s->tau) * log_t - term_const) * et - (s->tage + s->tau) * LOG_E * ei;
+        return;
+    }
+    default:
+        *m0 = 0.0; *m1 = 0.0;
+    }
+}
+
+/* clamp_integration_limits (opensps_sfh.f90:967-1000) */
+static double clamp_limit(double tiny_logt, double log_t, const sfhp_t *s) {
+    double lmin = pow(10.0, tiny_logt);
+    double ts = s->use_simha_limits ? s->t0 : s->tq;
+    double te = s->use_simha_limits ? s->tq : s->tage;
+    double ls = log10(fmax(ts, lmin)), le = log10(fmax(te, lmin));
+    return fmin(fmax(log_t, ls), le);
+}
+
+/* compute_ssp_weights (opensps_sfh.f90:97-257), logarithmic interpolation (interpolation_type 0) */
+static void compute_ssp_weights(const double *tf, int nt, double tiny_logt, const sfhp_t *s, int idx_min,
+                                int idx_max, double *w) {
+    for (int i = 0; i < nt; i++) w[i] = 0.0;
+    if (s->type == -1) {
+        if (s->tb < 0.0) return;
+        double log_tb = log10(fmax(s->tb, pow(10.0, tiny_logt)));
+        if (log_tb <= tf[0]) { w[0] = 1.0; return; }
+        int j = orc_find_interval(tf, nt, log_tb);
+        if (j < 1) j = 1;
+        if (j > nt - 1) j = nt - 1;
+        double bdt = tf[j] - tf[j - 1];
+        w[j - 1] = (tf[j] - log_tb) / bdt;
+        w[j] = (log_tb - tf[j - 1]) / bdt;
+        return;
+    }
+    double lmin = pow(10.0, tiny_logt);
+    double ts = s->use_simha_limits ? s->t0 : s->tq;
+    double te = s->use_simha_limits ? s->tq : s->tage;
+    double lts = log10(fmax(ts, lmin)), lte = log10(fmax(te, lmin));
+    int j_start = orc_find_interval(tf, nt, lts);
+    int j_end = orc_find_interval(tf, nt, lte) + 1;
+    int lo = idx_min > 1 ? idx_min : 1;
+    if (j_start < lo) j_start = lo;
+    int hi = idx_max < nt - 1 ? idx_max : nt - 1;
+    if (j_end > hi) j_end = hi;
+    double lim1 = fmin(fmax(tf[j_start - 1], lts), lte);
+    double lim_prev = lim1, m0p, m1p;
+    moments_log(lim1, s, &m0p, &m1p);
+    for (int j = j_start; j <= j_end; j++) {
+        double lim2 = fmin(fmax(tf[j], lts), lte);
+        if (lim2 <= lim_prev) { lim_prev = lim2; continue; }
+        double m0c, m1c;
+        moments_log(lim2, s, &m0c, &m1c);
+        double m0 = m0c - m0p, m1 = m1c - m1p;
+        m0p = m0c; m1p = m1c;
+        double nlo = tf[j - 1], nhi = tf[j];
+        double inv_dt = 1.0 / (nhi - nlo);
+        w[j - 1] = w[j - 1] + (nhi * m0 - m1) * inv_dt;
+        w[j] = w[j] + (m1 - nlo * m0) * inv_dt;
+        lim_prev = lim2;
+    }
+    if (idx_min == 0) {
+        double l1 = clamp_limit(tiny_logt, tiny_logt, s);
+        double l2 = clamp_limit(tiny_logt, tf[0], s);
+        if (l1 < l2) {
+            double a0, a1, b0, b1;
+            moments_log(l2, s, &a0, &a1);
+            moments_log(l1, s, &b0, &b1);
+            w[0] = w[0] + (a0 - b0);
+        }
+    }
+}
+
+/* sfh_weights_core_pc (opensps_sfh.f90:1077-1294) for sfh = 0, 1, 4 (with const and burst).
+ * Returns 0, or 2 for an SFH type this build does not compute. */
+int orc_sfh_weights(const double *tf, int nt, double tiny_logt, const double *p, double *w, double *tmp1,
+                    double *tmp2) {
+    sfhp_t s;
+    int sfh = (int)p[OPS_P_SFH];
+    double tage = p[OPS_P_TAGE];
+    for (int i = 0; i < nt; i++) { w[i] = 0.0; tmp1[i] = 0.0; tmp2[i] = 0.0; }
+    convert_sfhparams(p, tage, &s);
+    int imax = orc_find_interval(tf, nt, log10(s.tage)) + 2;
+    if (imax < 1) imax = 1;
+    if (imax > nt) imax = nt;
+    if (sfh == 0) {
+        s.type = -1;
+        s.tb = s.tage;
+        int imin = imax - 2 > 1 ? imax - 2 : 1;
+        compute_ssp_weights(tf, nt, tiny_logt, &s, imin, imax, w);
+        return 0;
+    }
+    if (sfh != 1 && sfh != 4) return 2;
+    s.type = sfh;
+    compute_ssp_weights(tf, nt, tiny_logt, &s, 0, imax, w);
+    double mass1 = 0.0;
+    for (int i = 0; i < imax; i++) mass1 += w[i];
+    if (mass1 < SAFE_FLOOR) mass1 = 1.0;
+    for (int i = 0; i < nt; i++) w[i] = w[i] / mass1;
+    double cst = p[OPS_P_CONST], fb = p[OPS_P_FBURST];
+    if (cst > 0.0 || fb > SAFE_FLOOR) {
+        s.type = 0;
+        compute_ssp_weights(tf, nt, tiny_logt, &s, 0, imax, tmp1);
+        mass1 = 0.0;
+        for (int i = 0; i < imax; i++) mass1 += tmp1[i];
+        if (mass1 < SAFE_FLOOR) mass1 = 1.0;
+        for (int i = 0; i < nt; i++) tmp2[i] = 0.0;
+        double fbv = 0.0;
+        if (s.tb >= 0.0) {
+            s.type = -1;
+            compute_ssp_weights(tf, nt, tiny_logt, &s, 0, imax, tmp2);
+            fbv = fb;
+        }
+        for (int i = 0; i < nt; i++)
+            w[i] = (1.0 - cst - fbv) * w[i] + cst * (tmp1[i] / mass1) + fbv * tmp2[i];
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * IGM (src/physics/opensps_cosmology.f90:65-95, 333-507)
+ * ------------------------------------------------------------------------------------------- */
+static const double LY_WAVE[17] = {1215.67, 1025.72, 972.537, 949.743, 937.803, 930.748, 926.226, 923.150,
+                                   920.963, 919.352, 918.129, 917.181, 916.429, 915.824, 915.329, 914.919,
+                                   914.576};
+static const double LY_COEFF[17] = {0.0036,    0.0017,    0.0011846, 0.0009410, 0.0007960, 0.0006967,
+                                    0.0006236, 0.0005665, 0.0005200, 0.0004817, 0.0004487, 0.0004200,
+                                    0.0003947, 0.000372,  0.000352,  0.0003334, 0.00031644};
+#define LYMAN_LIMIT 911.75
+#define A_METAL 0.0017
+#define MC1 0.25
+#define MC2 9.4
+#define MC3 0.7
+#define MC4 0.023
+#define MP1 0.46
+#define MP2 0.18
+#define MP3 1.32
+#define MP4 1.68
+
+/* igm_tau: Madau (1995) optical depth at rest wavelength w for 1 + z = opz (cosmology.f90:406-447) */
+double orc_igm_tau(double w, double opz) {
+    double obs = w * opz, lr = obs / LYMAN_LIMIT, tau = 0.0;
+    for (int i = 0; i < 17; i++) {
+        if (w < LY_WAVE[i]) {
+            tau = tau + LY_COEFF[i] * pow(obs / LY_WAVE[i], 3.46);
+            if (i == 0) tau = tau + A_METAL * pow(obs / LY_WAVE[i], 1.68);
+        }
+    }
+    if (w < LYMAN_LIMIT)
+        tau = tau + (MC1 * pow(lr, 3.0) * (pow(opz, MP1) - pow(lr, MP1))) +
+              (MC2 * pow(lr, 1.5) * (pow(opz, MP2) - pow(lr, MP2))) -
+              (MC3 * pow(lr, 3.0) * (pow(lr, -MP3) - pow(opz, -MP3))) - (MC4 * (pow(opz, MP4) - pow(lr, MP4)));
+    return tau;
+}
+
+/* igm_tau_pixel_terms (cosmology.f90:458-492) */
+void orc_igm_pixel_terms(double w, double *a) {
+    for (int k = 0; k < 6; k++) a[k] = 0.0;
+    if (w < LY_WAVE[0]) {
+        double ser = 0.0;
+        for (int i = 0; i < 17; i++)
+            if (w < LY_WAVE[i]) ser = ser + LY_COEFF[i] * pow(LY_WAVE[i], -3.46);
+        a[0] = ser * pow(w, 3.46);
+        a[1] = A_METAL * pow(w / LY_WAVE[0], 1.68);
+    }
+    if (w < LYMAN_LIMIT) {
+        double s = w / LYMAN_LIMIT;
+        a[2] = MC1 * pow(s, 3.0) * (1.0 - pow(s, MP1));
+        a[3] = MC2 * pow(s, 1.5) * (1.0 - pow(s, MP2));
+        a[4] = -MC3 * pow(s, 3.0) * (pow(s, -MP3) - 1.0);
+        a[5] = -MC4 * (1.0 - pow(s, MP4));
+    }
+}
+
+/* igm_tau_z_terms (cosmology.f90:495-507) */
+void orc_igm_z_terms(double opz, double *b) {
+    b[0] = pow(opz, 3.46);
+    b[1] = pow(opz, 1.68);
+    b[2] = pow(opz, 3.0 + MP1);
+    b[3] = pow(opz, 1.5 + MP2);
+    b[4] = pow(opz, 3.0 - MP3);
+    b[5] = pow(opz, MP4);
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Cosmology table and distance term (cosmology.f90:244-305; batch_photometry.f90:784-794)
+ * ------------------------------------------------------------------------------------------- */
+/* get_luminosity_distance (cosmology.f90:249-279), parsecs */
+double orc_lum_distance(double h0, double om0, double ol0, double z) {
+    if (z <= 0.0) return 0.0;
+    double hd = ((C_LIGHT * 1.0e-13) / h0) * 1.0e6;
+    double dz = z / (double)(N_COSMO_GRID - 1);
+    double prev = 1.0 / sqrt(om0 + ol0), sum = 0.0;
+    for (int i = 2; i <= N_COSMO_GRID; i++) {
+        double zc = (double)(i - 1) * dz;
+        double opz = 1.0 + zc;
+        double now = 1.0 / sqrt(om0 * (opz * opz * opz) + ol0);
+        sum = sum + 0.5 * dz * (prev + now);
+        prev = now;
+    }
+    return (1.0 + z) * hd * sum;
+}
+
+/* update_cosmology_table (cosmology.f90:288-300): 500 redshifts (descending) and distances */
+void orc_cosmology_table(double h0, double om0, double ol0, double *cz, double *cdl) {
+    for (int i = 1; i <= 500; i++) {
+        double a = (double)(i - 1) / 499.0 * (1.0 - 1.0 / 1001.0) + 1.0 / 1001.0;
+        cz[i - 1] = (1.0 / a) - 1.0;
+        cdl[i - 1] = orc_lum_distance(h0, om0, ol0, cz[i - 1]);
+    }
+}
+
+/* distance_term (batch_photometry.f90:784-794) */
+double orc_distance_term(const double *cz, const double *cdl, int nc, double z) {
+    if (fabs(z) <= SAFE_FLOOR) return 0.0;
+    double dm = interp_linear(cz, cdl, nc, z);
+    if (dm != dm || dm <= SAFE_FLOOR) return 0.0;
+    return 5.0 * log10(dm / 10.0) + mag_from_flux(1.0 + z);
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Velocity smoothing operator (src/spectra/opensps_smoothing.f90:553-633)
+ * lo/hi are 1-based (lo == 0: pixel passes through); coef is sized nspec * (2 * wmax + 1) by the
+ * caller through orc_velocity_kernel_size. Returns the number of coefficients.
+ * ------------------------------------------------------------------------------------------- */
+int orc_velocity_kernel(const double *lam, int n, double sigma, double minl, double maxl, int *lo, int *hi,
+                        int *off, double *coef, int cap) {
+    for (int i = 0; i < n; i++) { lo[i] = 0; hi[i] = -1; off[i] = 0; }
+    if (n < 2 || sigma <= SAFE_FLOOR) return 0;
+    double sb = sigma / (C_LIGHT * 1.0e-13);
+    double inv2 = 0.5 * (1.0 / sb) * (1.0 / sb);
+    double fblue = exp(-N_SIGMA_RANGE * sb), fred = exp(N_SIGMA_RANGE * sb);
+    int *wlo = (int *)calloc(n, sizeof(int)), *whi = (int *)calloc(n, sizeof(int));
+    double *w = (double *)malloc(sizeof(double) * n);
+    int il = 1, ih = 1;
+    for (int i = 1; i <= n; i++) {
+        wlo[i - 1] = 0; whi[i - 1] = -1;
+        double li = lam[i - 1];
+        if (li < minl || li > maxl) continue;
+        double vl = li * fblue, vh = li * fred;
+        while (il > 1 && lam[il - 1] > vl) il--;
+        while (il < n && lam[il - 1] < vl) il++;
+        if (il > 1 && lam[il - 1] > vl) il--;
+        if (ih < il) ih = il;
+        while (ih > il && lam[ih - 1] > vh) ih--;
+        while (ih < n && lam[ih - 1] < vh) ih++;
+        if (ih - il + 1 < 2) continue;
+        wlo[i - 1] = il; whi[i - 1] = ih;
+    }
+    int nt = 0;
+    for (int i = 1; i <= n; i++) {
+        if (wlo[i - 1] == 0) continue;
+        int a = wlo[i - 1], b = whi[i - 1], nw = b - a + 1;
+        double lli = log(lam[i - 1]), tw = 0.0;
+        for (int m = 0; m < nw; m++) {
+            double d = lli - log(lam[a - 1 + m]);
+            w[m] = exp(-inv2 * (d * d)) / lam[a - 1 + m];
+        }
+        for (int m = 1; m <= nw; m++) {
+            double dl = 0.0, dr = 0.0;
+            if (m > 1) dl = lam[a + m - 2] - lam[a + m - 3];
+            if (m < nw) dr = lam[a + m - 1] - lam[a + m - 2];
+            w[m - 1] = 0.5 * w[m - 1] * (dl + dr);
+            tw = tw + w[m - 1];
+        }
+        if (tw <= SAFE_FLOOR) continue;
+        if (nt + nw > cap) { nt = -1; break; }
+        lo[i - 1] = a; hi[i - 1] = b; off[i - 1] = nt + 1;
+        for (int m = 0; m < nw; m++) coef[nt + m] = w[m] / tw;
+        nt += nw;
+    }
+    free(wlo); free(whi); free(w);
+    return nt;
+}
+
+/* smooth_kernel, FP64 sums (batch_photometry.f90:740-780): one column in place */
+static void apply_velocity_kernel(int n, const int *lo, const int *hi, const int *off, const double *coef,
+                                  double *spec, double *work) {
+    for (int i = 0; i < n; i++) {
+        int a = lo[i];
+        if (a == 0) { work[i] = spec[i]; continue; }
+        int o = off[i] - a;   /* 1-based: coef(off + k) for k = lo .. hi */
+        double s = 0.0;
+        for (int k = a; k <= hi[i]; k++) s = s + coef[o + k - 1] * spec[k - 1];
+        work[i] = s;
+    }
+    memcpy(spec, work, sizeof(double) * n);
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Photometry (src/spectra/opensps_batch_photometry.f90)
+ * ------------------------------------------------------------------------------------------- */
+/* observed_frame_z for one column (batch_photometry.f90:819-903): fobs(j) = max(interp, 0) / lambda */
+void orc_observed_frame(const double *lam, int n, double z, const double *total, double *fobs) {
+    for (int j = 0; j < n; j++) {
+        double v;
+        if (fabs(z) > SAFE_FLOOR) {
+            double tg = lam[j] / (1.0 + z);
+            int il = orc_find_interval(lam, n, tg);
+            if (il < 1) il = 1;
+            if (il > n - 1) il = n - 1;
+            double t = (tg - lam[il - 1]) / (lam[il] - lam[il - 1]);
+            v = total[il - 1] + t * (total[il] - total[il - 1]);
+        } else {
+            v = total[j];
+        }
+        if (v != v) v = 0.0;
+        double wl = lam[j] > DBL_MIN ? 1.0 / lam[j] : 0.0;
+        fobs[j] = fmax(v, 0.0) * wl;
+    }
+}
+
+/* Band supports (prepare_batch_grid, batch_photometry.f90:293-307): 1-based j_lo, j_hi */
+void orc_band_supports(const double *bands, int nspec, int nbands, int *jlo, int *jhi) {
+    for (int b = 0; b < nbands; b++) {
+        int ilo = nspec, ihi = 1;
+        for (int j = 1; j <= nspec; j++)
+            if (bands[(size_t)b * nspec + j - 1] != 0.0) {
+                if (j < ilo) ilo = j;
+                if (j > ihi) ihi = j;
+            }
+        jlo[b] = ilo - 1 > 1 ? ilo - 1 : 1;
+        jhi[b] = ihi < nspec - 1 ? ihi : nspec - 1;
+        if (ilo > ihi) { jlo[b] = 1; jhi[b] = 0; }
+    }
+}
+
+/* band_fluxes, FP64 sums (batch_photometry.f90:1049-1171), one column, transposed bands
+ * bt(nbands, nspec); wt the trapezoid half-widths */
+void orc_band_fluxes_col(int nspec, int nbands, const double *bt, const int *jlo, const int *jhi,
+                         const int *do_band, const double *wt, const double *fobs, double *fband) {
+    for (int b = 0; b < nbands; b++) {
+        double s = 0.0;
+        if (do_band[b] != 0)
+            for (int j = jlo[b]; j <= jhi[b]; j++)
+                s = s + wt[j - 1] * (fobs[j - 1] * bt[(size_t)(j - 1) * nbands + b] +
+                                     fobs[j] * bt[(size_t)j * nbands + b]);
+        fband[b] = s;
+    }
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Filters (src/spectra/opensps_filters.f90:319-503): resample a piecewise-linear curve onto the
+ * grid, clip negatives, and normalise integral T / lambda d lambda = 1 by the trapezoid rule.
+ * mode 0: point interpolation (resample_point), 1: hat projection (resample_hat). detector 1 is
+ * energy counting (T / lambda at the grid points). Returns 0, or 1 for a dead filter.
+ * ------------------------------------------------------------------------------------------- */
+static int cell_of(const double *g, int ng, double x) {
+    if (x <= g[0]) return 1;
+    if (x >= g[ng - 1]) return ng - 1;
+    int lo = 1, hi = ng;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) / 2;
+        if (g[mid - 1] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+int orc_filter_band(const double *lc, const double *rc, int nc, const double *g, int ng, int mode,
+                    int detector, double *band) {
+    for (int j = 0; j < ng; j++) band[j] = 0.0;
+    if (nc < 2) return 1;
+    if (mode == 1) {
+        double *acc = (double *)calloc(ng, sizeof(double));
+        for (int k = 1; k <= nc - 1; k++) {
+            double x0 = fmax(lc[k - 1], g[0]), x1 = fmin(lc[k], g[ng - 1]);
+            if (x1 <= x0) continue;
+            int j = cell_of(g, ng, x0);
+            while (j <= ng - 1) {
+                if (g[j - 1] >= x1) break;
+                double a = fmax(x0, g[j - 1]), b = fmin(x1, g[j]);
+                if (b > a) {
+                    double dx = g[j] - g[j - 1];
+                    double t = (a - lc[k - 1]) / (lc[k] - lc[k - 1]);
+                    double ya = rc[k - 1] + t * (rc[k] - rc[k - 1]);
+                    t = (b - lc[k - 1]) / (lc[k] - lc[k - 1]);
+                    double yb = rc[k - 1] + t * (rc[k] - rc[k - 1]);
+                    double m = 0.5 * (a + b), ym = 0.5 * (ya + yb);
+                    double ha = (g[j] - a) / dx, hb = (g[j] - b) / dx, hm = (g[j] - m) / dx;
+                    acc[j - 1] = acc[j - 1] + (b - a) / 6.0 * (ya * ha + 4.0 * ym * hm + yb * hb);
+                    ha = (a - g[j - 1]) / dx; hb = (b - g[j - 1]) / dx; hm = (m - g[j - 1]) / dx;
+                    acc[j] = acc[j] + (b - a) / 6.0 * (ya * ha + 4.0 * ym * hm + yb * hb);
+                }
+                j = j + 1;
+            }
+        }
+        band[0] = acc[0] / (0.5 * (g[1] - g[0]));
+        for (int j = 2; j <= ng - 1; j++) band[j - 1] = acc[j - 1] / (0.5 * (g[j] - g[j - 2]));
+        band[ng - 1] = acc[ng - 1] / (0.5 * (g[ng - 1] - g[ng - 2]));
+        free(acc);
+        for (int j = 0; j < ng; j++) if (band[j] < 0.0) band[j] = 0.0;
+    } else {
+        int si, ei;
+        if (lc[0] <= g[0]) si = 1; else { si = orc_find_interval(g, ng, lc[0]); if (si < 1) si = 1; }
+        if (lc[nc - 1] >= g[ng - 1]) ei = ng; else { ei = orc_find_interval(g, ng, lc[nc - 1]); if (ei > ng) ei = ng; }
+        if (ei > si)
+            for (int i = si; i <= ei; i++) band[i - 1] = interp_linear(lc, rc, nc, g[i - 1]);
+    }
+    if (detector == 1) for (int j = 0; j < ng; j++) band[j] = band[j] / g[j];
+    for (int j = 0; j < ng; j++) if (band[j] < 0.0) band[j] = 0.0;
+    /* integrate_trapezoid_array(grid, band / (grid + SAFE_FLOOR)) */
+    double norm = 0.0;
+    for (int j = 0; j < ng - 1; j++)
+        norm = norm + 0.5 * (g[j + 1] - g[j]) * (band[j] / (g[j] + SAFE_FLOOR) + band[j + 1] / (g[j + 1] + SAFE_FLOOR));
+    if (norm > SAFE_FLOOR) {
+        for (int j = 0; j < ng; j++) band[j] = band[j] / norm;
+        return 0;
+    }
+    for (int j = 0; j < ng; j++) band[j] = 0.0;
+    return 1;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * The batch (src/physics/opensps_csp_batch.f90:431-1214, the stages that the configurations of
+ * this build exercise: parametric or given SFH weights, the resident bases blended in logzsol,
+ * power-law dust with birth cloud and old-star screens, per-model IGM, velocity smoothing,
+ * redshift and band magnitudes).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct {
+    int nspec, nt, nz, nbands;
+    const double *lambda;     /* nspec */
+    const double *time_full;  /* nt, log10 yr */
+    const double *logz;       /* nz, log10(Z / Z_sun) */
+    const double *basis;      /* nspec x nt x nz, L_sun/Hz per solar mass */
+    const double *bmass;      /* nt x nz, surviving mass */
+    const double *blbol;      /* nt x nz, log10 L_bol */
+    const double *bands;      /* nspec x nbands, normalised */
+    const int *mag_mask;      /* nbands or NULL */
+    int ncosmo;
+    const double *cz, *cdl;   /* the cosmology table */
+    int add_igm;              /* add_igm_absorption */
+    int smooth_fast;          /* smooth_velocity == 1 and smoothspec_fast == 1 */
+    double tiny_logt;
+} orc_ctx_t;
+
+/* Per-model birth-cloud age index: batch_model_indices (opensps_csp_batch.f90:1541-1576) */
+static int model_it_tesc(const orc_ctx_t *c, double dust_tesc) {
+    double dal = dust_tesc > SAFE_FLOOR ? dust_tesc : 7.0;
+    int it = orc_find_interval(c->time_full, c->nt, dal);
+    if (it > c->nt) it = c->nt;
+    if (it < 1) it = 1;
+    return it;
+}
+
+/* metallicity_interval (opensps_csp_batch.f90:2073-2084): weight not clamped */
+static void model_zlo(const orc_ctx_t *c, double logzsol, int *zlo, double *dz) {
+    int nz = c->nz;
+    int z = orc_find_interval(c->logz, nz, logzsol);
+    if (z > nz - 1) z = nz - 1;
+    if (z < 1) z = 1;
+    *zlo = z;
+    *dz = (logzsol - c->logz[z - 1]) / (c->logz[z] - c->logz[z - 1]);
+}
+
+/*
+ * orc_csp_batch: n models with parameters prm[slot * n + m]. ssp_weights (nt x n) replaces the SFH
+ * parameters when non-NULL. Outputs (any NULL but mags): mags (nbands x n), spec (nspec x n),
+ * mass, lbol (n), young / old (nspec x n, the age sums before dust), idx (3 x n: it_tesc, zlo,
+ * 1-based; zlo = 1 with one basis), w_out (nt x n, the FP64 SFH weights), fband (nbands x n).
+ * Returns 0, 2 (unsupported SFH type), 3 (bad arguments).
+ */
+int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ssp_weights, double *mags,
+                  double *spec, double *mass, double *lbol, double *young, double *old, int *idx, double *w_out,
+                  double *fband_out) {
+    const int nspec = c->nspec, nt = c->nt, nz = c->nz, nb = c->nbands;
+    if (n < 0 || nspec < 2 || nt < 2 || nz < 1 || !mags) return 3;
+    if (n == 0) return 0;
+    const double amzl = abs_mag_zeropoint_log();
+    int status = 0;
+    int *jlo = (int *)malloc(sizeof(int) * (nb > 0 ? nb : 1)), *jhi = (int *)malloc(sizeof(int) * (nb > 0 ? nb : 1));
+    int *do_band = (int *)malloc(sizeof(int) * (nb > 0 ? nb : 1));
+    double *bt = (double *)malloc(sizeof(double) * (size_t)(nb > 0 ? nb : 1) * nspec);
+    double *wtl = (double *)malloc(sizeof(double) * nspec), *ltab = (double *)malloc(sizeof(double) * nspec);
+    orc_band_supports(c->bands, nspec, nb, jlo, jhi);
+    for (int b = 0; b < nb; b++) {
+        do_band[b] = c->mag_mask ? c->mag_mask[b] : 1;
+        for (int j = 0; j < nspec; j++) bt[(size_t)j * nb + b] = c->bands[(size_t)b * nspec + j];
+    }
+    for (int j = 0; j < nspec - 1; j++) wtl[j] = 0.5 * fabs(c->lambda[j + 1] - c->lambda[j]);
+    wtl[nspec - 1] = 0.0;
+    for (int j = 0; j < nspec; j++) ltab[j] = log(c->lambda[j] / 5500.0);
+    /* jtop: the last pixel below Lyman-alpha (apply_igm_z, opensps_csp_batch.f90:4221-4224) */
+    int jtop = 0;
+    for (int j = 1; j <= nspec; j++) if (c->lambda[j - 1] < 1215.67) jtop = j;
+    /* z_varies: the reference takes compute_igm_transmission's direct form when every model shares
+     * zred and igm_factor (opensps_csp_batch.f90:1011-1015, prepare_batch_grid) */
+    int z_varies = 0;
+    for (int m = 1; m < n; m++)
+        if (prm[OPS_P_ZRED * n + m] != prm[OPS_P_ZRED * n] ||
+            prm[OPS_P_IGM_FACTOR * n + m] != prm[OPS_P_IGM_FACTOR * n]) z_varies = 1;
+    int cap = nspec * 64;
+
+    #pragma omp parallel
+    {
+        double *p = (double *)malloc(sizeof(double) * OPS_NPAR);
+        double *w = (double *)malloc(sizeof(double) * nt), *t1 = (double *)malloc(sizeof(double) * nt);
+        double *t2 = (double *)malloc(sizeof(double) * nt);
+        double *y = (double *)malloc(sizeof(double) * nspec), *o = (double *)malloc(sizeof(double) * nspec);
+        double *tot = (double *)malloc(sizeof(double) * nspec), *work = (double *)malloc(sizeof(double) * nspec);
+        double *fobs = (double *)malloc(sizeof(double) * nspec), *fb = (double *)malloc(sizeof(double) * (nb + 1));
+        int *slo = (int *)malloc(sizeof(int) * nspec), *shi = (int *)malloc(sizeof(int) * nspec);
+        int *soff = (int *)malloc(sizeof(int) * nspec);
+        double *scoef = (double *)malloc(sizeof(double) * cap);
+        double *tau = (double *)malloc(sizeof(double) * (jtop + 1));
+        double sm_key[3] = {NAN, NAN, NAN};
+        int sm_n = 0;
+
+        #pragma omp for schedule(dynamic, 4)
+        for (int m = 0; m < n; m++) {
+            for (int q = 0; q < OPS_NPAR; q++) p[q] = prm[(size_t)q * n + m];
+            /* stage 1: indices */
+            int it = model_it_tesc(c, p[OPS_P_DUST_TESC]);
+            int zlo = 1;
+            double dz = 0.0;
+            int blend = nz > 1;
+            if (blend) model_zlo(c, p[OPS_P_LOGZSOL], &zlo, &dz);
+            if (idx) { idx[3 * m] = it; idx[3 * m + 1] = zlo; idx[3 * m + 2] = 0; }
+            /* stage 0 checks (batch_check_models :1312-1320; plan_smoothing host modes are not built) */
+            if ((!ssp_weights && p[OPS_P_TAGE] <= 0.0) || (!c->smooth_fast && p[OPS_P_SIGMA_SMOOTH] > SAFE_FLOOR)) {
+                #pragma omp atomic write
+                status = 2;
+                continue;
+            }
+            /* stage 2: SFH weights (batch_sfh_weights, :1865-1917) */
+            if (ssp_weights) {
+                for (int k = 0; k < nt; k++) w[k] = ssp_weights[(size_t)m * nt + k];
+            } else {
+                int st = orc_sfh_weights(c->time_full, nt, c->tiny_logt, p, w, t1, t2);
+                if (st != 0) {
+                    #pragma omp atomic write
+                    status = st;
+                    continue;
+                }
+            }
+            if (w_out) for (int k = 0; k < nt; k++) w_out[(size_t)m * nt + k] = w[k];
+            /* stage 3: model_sums (:2129-2176) and the age sums (age_sums_host, :2447-2511) */
+            double ms = 0.0, ls = 0.0;
+            double c2 = dz, c1 = 1.0 - c2;
+            const double *blo = c->basis + (size_t)(zlo - 1) * nspec * nt;
+            const double *bhi = blend ? c->basis + (size_t)zlo * nspec * nt : blo;
+            for (int i = 0; i < nspec; i++) { y[i] = 0.0; o[i] = 0.0; }
+            for (int k = 1; k <= nt; k++) {
+                double wk = w[k - 1];
+                if (wk > SAFE_FLOOR) {
+                    double mk, lk;
+                    if (blend) {
+                        mk = (1.0 - dz) * c->bmass[(size_t)(zlo - 1) * nt + k - 1] + dz * c->bmass[(size_t)zlo * nt + k - 1];
+                        lk = (1.0 - dz) * c->blbol[(size_t)(zlo - 1) * nt + k - 1] + dz * c->blbol[(size_t)zlo * nt + k - 1];
+                        mk = wk * mk;
+                        lk = wk * pow(10.0, lk);
+                    } else {
+                        mk = wk * c->bmass[k - 1];
+                        lk = wk * pow(10.0, c->blbol[k - 1]);
+                    }
+                    ms = ms + mk;
+                    ls = ls + lk;
+                } else {
+                    wk = 0.0;
+                }
+                double *acc = k <= it ? y : o;
+                const double *bl = blo + (size_t)(k - 1) * nspec, *bh = bhi + (size_t)(k - 1) * nspec;
+                if (blend)
+                    for (int i = 0; i < nspec; i++) acc[i] = acc[i] + wk * (c1 * bl[i] + c2 * bh[i]);
+                else
+                    for (int i = 0; i < nspec; i++) acc[i] = acc[i] + wk * (c1 * bl[i]);
+            }
+            if (mass) mass[m] = ms;
+            if (lbol) lbol[m] = ls > SAFE_FLOOR ? log10(ls) : 0.0;
+            if (young) memcpy(young + (size_t)m * nspec, y, sizeof(double) * nspec);
+            if (old) memcpy(old + (size_t)m * nspec, o, sizeof(double) * nspec);
+            /* stage 5: attenuate_batch + pixel_transmissions, dust_type 0 (:3576-3818) */
+            double fob = p[OPS_P_FRAC_OBRUN], fnd = p[OPS_P_FRAC_NODUST];
+            double ed = p[OPS_P_DUST_INDEX];
+            int no_birth = p[OPS_P_DUST1] <= SAFE_FLOOR || 1.0 - fob <= SAFE_FLOOR;
+            int no_old = fabs(p[OPS_P_DUST3]) <= SAFE_FLOOR;
+            for (int i = 0; i < nspec; i++) {
+                double curve = 1.0;
+                if (ed != 0.0) curve = curve * exp(ed * ltab[i]);
+                double tau_d = p[OPS_P_DUST2] * curve, tr_d = exp(-tau_d);
+                double tr_b = 1.0, tr_o = 1.0;
+                if (!no_birth) tr_b = exp(-(p[OPS_P_DUST1] * exp(p[OPS_P_DUST1_INDEX] * ltab[i])));
+                if (!no_old) tr_o = exp(-(p[OPS_P_DUST3] * curve));
+                double ssum = (y[i] * tr_b * (1.0 - fob) + y[i] * fob) + o[i] * tr_o;
+                if (1.0 - fnd <= SAFE_FLOOR) tot[i] = ssum;
+                else tot[i] = ssum * (tr_d * (1.0 - fnd) + fnd);
+            }
+            /* stage 6: IGM (apply_igm_z :4205-4319; uniform z: compute_igm_transmission) */
+            double z = p[OPS_P_ZRED];
+            if (c->add_igm && z > SAFE_FLOOR && jtop > 0) {
+                double opz = 1.0 + z;
+                if (z_varies) {
+                    double a6[6], b6[6];
+                    orc_igm_z_terms(opz, b6);
+                    for (int j = 0; j < jtop; j++) {
+                        orc_igm_pixel_terms(c->lambda[j], a6);
+                        tau[j] = a6[0] * b6[0] + a6[1] * b6[1] + a6[2] * b6[2] + a6[3] * b6[3] + a6[4] * b6[4] + a6[5] * b6[5];
+                    }
+                } else {
+                    for (int j = 0; j < jtop; j++) tau[j] = orc_igm_tau(c->lambda[j], opz);
+                }
+                double tmax = 0.0;
+                int jmax = 1;
+                for (int j = 1; j <= jtop; j++) if (tau[j - 1] > tmax) { tmax = tau[j - 1]; jmax = j; }
+                if (jmax > 1) for (int j = 1; j <= jmax; j++) tau[j - 1] = tmax;
+                for (int j = 0; j < jtop; j++) tot[j] = tot[j] * exp(-tau[j] * p[OPS_P_IGM_FACTOR]);
+            }
+            /* stage 7: velocity smoothing (smooth_columns / smooth_kernel FP64, build_velocity_kernel) */
+            double sig = p[OPS_P_SIGMA_SMOOTH];
+            if (c->smooth_fast && sig > SAFE_FLOOR) {
+                if (!(sm_key[0] == sig && sm_key[1] == p[OPS_P_MIN_WAVE_SMOOTH] && sm_key[2] == p[OPS_P_MAX_WAVE_SMOOTH])) {
+                    sm_n = orc_velocity_kernel(c->lambda, nspec, sig, p[OPS_P_MIN_WAVE_SMOOTH], p[OPS_P_MAX_WAVE_SMOOTH],
+                                               slo, shi, soff, scoef, cap);
+                    sm_key[0] = sig; sm_key[1] = p[OPS_P_MIN_WAVE_SMOOTH]; sm_key[2] = p[OPS_P_MAX_WAVE_SMOOTH];
+                }
+                if (sm_n > 0) apply_velocity_kernel(nspec, slo, shi, soff, scoef, tot, work);
+            }
+            if (spec) memcpy(spec + (size_t)m * nspec, tot, sizeof(double) * nspec);
+            /* stage 8: magnitudes (compute_magnitudes_columns, observed_frame_z, band_fluxes) */
+            orc_observed_frame(c->lambda, nspec, z, tot, fobs);
+            orc_band_fluxes_col(nspec, nb, bt, jlo, jhi, do_band, wtl, fobs, fb);
+            double dist = orc_distance_term(c->cz, c->cdl, c->ncosmo, z);
+            for (int b = 0; b < nb; b++) {
+                double v = NAN;
+                if (do_band[b] != 0) v = mag_from_flux(fb[b]) - AB_ZEROPOINT - 2.5 * amzl + dist;
+                mags[(size_t)m * nb + b] = v;
+                if (fband_out) fband_out[(size_t)m * nb + b] = fb[b];
+            }
+        }
+        free(p); free(w); free(t1); free(t2); free(y); free(o); free(tot); free(work); free(fobs); free(fb);
+        free(slo); free(shi); free(soff); free(scoef); free(tau);
+    }
+    free(jlo); free(jhi); free(do_band); free(bt); free(wtl); free(ltab);
+    return status;
+}
+
+/* Thread count the OpenMP runtime uses (for the CPU-baseline report) */
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

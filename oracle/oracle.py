@@ -1,0 +1,235 @@
+"""ctypes front end of the FP64 CPU oracle (csp_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the tests, the smoke check and the CPU-baseline leg of the benchmark use
+it as the checker of the GPU path; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_find_interval.argtypes = [_dp, C.c_int, C.c_double]
+        L.orc_find_interval.restype = C.c_int
+        L.orc_expi.argtypes = [C.c_double]
+        L.orc_expi.restype = C.c_double
+        L.orc_sfh_weights.argtypes = [_dp, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+        L.orc_sfh_weights.restype = C.c_int
+        L.orc_igm_tau.argtypes = [C.c_double, C.c_double]
+        L.orc_igm_tau.restype = C.c_double
+        L.orc_igm_pixel_terms.argtypes = [C.c_double, _dp]
+        L.orc_igm_z_terms.argtypes = [C.c_double, _dp]
+        L.orc_lum_distance.argtypes = [C.c_double] * 4
+        L.orc_lum_distance.restype = C.c_double
+        L.orc_cosmology_table.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp]
+        L.orc_distance_term.argtypes = [_dp, _dp, C.c_int, C.c_double]
+        L.orc_distance_term.restype = C.c_double
+        L.orc_velocity_kernel.argtypes = [_dp, C.c_int, C.c_double, C.c_double, C.c_double, _ip, _ip, _ip, _dp,
+                                          C.c_int]
+        L.orc_velocity_kernel.restype = C.c_int
+        L.orc_observed_frame.argtypes = [_dp, C.c_int, C.c_double, _dp, _dp]
+        L.orc_band_supports.argtypes = [_dp, C.c_int, C.c_int, _ip, _ip]
+        L.orc_band_fluxes_col.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _ip, _dp, _dp, _dp]
+        L.orc_filter_band.argtypes = [_dp, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, _dp]
+        L.orc_filter_band.restype = C.c_int
+        L.orc_csp_batch.restype = C.c_int
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _d(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def _i(x):
+    return np.ascontiguousarray(x, dtype=np.int32)
+
+
+def find_interval(a, v) -> int:
+    a = _d(a)
+    return lib().orc_find_interval(a, a.size, float(v))
+
+
+def expi(x) -> float:
+    return lib().orc_expi(float(x))
+
+
+def sfh_weights(time_full, p, tiny_logt=0.0):
+    tf = _d(time_full)
+    w = np.zeros(tf.size)
+    t1 = np.zeros(tf.size)
+    t2 = np.zeros(tf.size)
+    st = lib().orc_sfh_weights(tf, tf.size, float(tiny_logt), _d(p), w, t1, t2)
+    return w, st
+
+
+def igm_tau(w, opz) -> float:
+    return lib().orc_igm_tau(float(w), float(opz))
+
+
+def igm_pixel_terms(w):
+    a = np.zeros(6)
+    lib().orc_igm_pixel_terms(float(w), a)
+    return a
+
+
+def igm_z_terms(opz):
+    b = np.zeros(6)
+    lib().orc_igm_z_terms(float(opz), b)
+    return b
+
+
+def lum_distance(h0, om0, ol0, z) -> float:
+    return lib().orc_lum_distance(float(h0), float(om0), float(ol0), float(z))
+
+
+def cosmology_table(h0=72.0, om0=0.27, ol0=0.73):
+    cz = np.zeros(500)
+    cdl = np.zeros(500)
+    lib().orc_cosmology_table(float(h0), float(om0), float(ol0), cz, cdl)
+    return cz, cdl
+
+
+def distance_term(cz, cdl, z) -> float:
+    cz = _d(cz)
+    return lib().orc_distance_term(cz, _d(cdl), cz.size, float(z))
+
+
+def velocity_kernel(lam, sigma, minl=1.0e3, maxl=1.0e4):
+    lam = _d(lam)
+    n = lam.size
+    lo = np.zeros(n, np.int32)
+    hi = np.zeros(n, np.int32)
+    off = np.zeros(n, np.int32)
+    cap = n * 64
+    coef = np.zeros(cap)
+    nc = lib().orc_velocity_kernel(lam, n, float(sigma), float(minl), float(maxl), lo, hi, off, coef, cap)
+    if nc < 0:
+        raise RuntimeError("velocity kernel larger than its buffer")
+    return lo, hi, off, coef[:nc]
+
+
+def observed_frame(lam, z, total):
+    lam = _d(lam)
+    f = np.zeros(lam.size)
+    lib().orc_observed_frame(lam, lam.size, float(z), _d(total), f)
+    return f
+
+
+def band_supports(bands):
+    """bands: (nbands, nspec) C order (= Fortran (nspec, nbands))."""
+    b = _d(bands)
+    nb, ns = b.shape
+    jlo = np.zeros(nb, np.int32)
+    jhi = np.zeros(nb, np.int32)
+    lib().orc_band_supports(b, ns, nb, jlo, jhi)
+    return jlo, jhi
+
+
+def band_fluxes_col(bands_t, jlo, jhi, do_band, wt, fobs):
+    """bands_t: (nspec, nbands) C order (= Fortran bands_t(nbands, nspec))."""
+    bt = _d(bands_t)
+    ns, nb = bt.shape
+    out = np.zeros(nb)
+    lib().orc_band_fluxes_col(ns, nb, bt, _i(jlo), _i(jhi), _i(do_band), _d(wt), _d(fobs), out)
+    return out
+
+
+def filter_band(lc, rc, grid, mode=0, detector=0):
+    g = _d(grid)
+    band = np.zeros(g.size)
+    lc = _d(lc)
+    st = lib().orc_filter_band(lc, _d(rc), lc.size, g, g.size, int(mode), int(detector), band)
+    return band, st
+
+
+def build_bands(fset, grid, mode=0):
+    """(nbands, nspec) normalised bands of a synth.FilterSet on grid."""
+    out = np.zeros((len(fset.lam), len(grid)))
+    for b, (lc, rc) in enumerate(zip(fset.lam, fset.resp)):
+        out[b], _ = filter_band(lc, rc, grid, mode, fset.detector)
+    return out
+
+
+class _Ctx(C.Structure):
+    _fields_ = [("nspec", C.c_int), ("nt", C.c_int), ("nz", C.c_int), ("nbands", C.c_int),
+                ("lambda_", C.c_void_p), ("time_full", C.c_void_p), ("logz", C.c_void_p),
+                ("basis", C.c_void_p), ("bmass", C.c_void_p), ("blbol", C.c_void_p), ("bands", C.c_void_p),
+                ("mag_mask", C.c_void_p), ("ncosmo", C.c_int), ("cz", C.c_void_p), ("cdl", C.c_void_p),
+                ("add_igm", C.c_int), ("smooth_fast", C.c_int), ("tiny_logt", C.c_double)]
+
+
+def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_mask=None, cosmo=None,
+              want=("mags", "spec", "mass", "lbol")):
+    """Runs the oracle's batch. grid: synth.SSPGrid; bands (nbands, nspec); prm (NPAR, n).
+    Returns a dict of outputs (C-order arrays: mags (n, nbands), spec (n, nspec), ...) and the status."""
+    keep = []
+
+    def ptr(a):
+        keep.append(a)
+        return a.ctypes.data
+
+    n = prm.shape[1]
+    bands = _d(bands)
+    nb = bands.shape[0]
+    cz, cdl = cosmo if cosmo is not None else cosmology_table()
+    ctx = _Ctx()
+    ctx.nspec, ctx.nt, ctx.nz, ctx.nbands = grid.nspec, grid.nt, grid.nz, nb
+    ctx.lambda_ = ptr(_d(grid.lam))
+    ctx.time_full = ptr(_d(grid.log_age))
+    ctx.logz = ptr(_d(grid.logz))
+    ctx.basis = ptr(_d(grid.spec))
+    ctx.bmass = ptr(_d(grid.mass))
+    ctx.blbol = ptr(_d(grid.lbol))
+    ctx.bands = ptr(bands)
+    ctx.mag_mask = ptr(_i(mag_mask)) if mag_mask is not None else None
+    ctx.ncosmo = len(cz)
+    ctx.cz = ptr(_d(cz))
+    ctx.cdl = ptr(_d(cdl))
+    ctx.add_igm = int(add_igm)
+    ctx.smooth_fast = int(smooth_fast)
+    ctx.tiny_logt = 0.0
+    out = {"mags": np.zeros((n, nb))}
+    for k, shape in (("spec", (n, grid.nspec)), ("mass", (n,)), ("lbol", (n,)), ("young", (n, grid.nspec)),
+                     ("old", (n, grid.nspec)), ("w", (n, grid.nt)), ("fband", (n, nb))):
+        if k in want:
+            out[k] = np.zeros(shape)
+    idx = np.zeros((n, 3), np.int32) if "idx" in want else None
+    if idx is not None:
+        out["idx"] = idx
+    wts = _d(ssp_weights) if ssp_weights is not None else None
+    p = _d(prm)
+    f = lib().orc_csp_batch
+    vp = C.c_void_p
+    st = f(C.byref(ctx), C.c_int(n), vp(p.ctypes.data), vp(wts.ctypes.data if wts is not None else None),
+           vp(out["mags"].ctypes.data), *[vp(out[k].ctypes.data) if k in out else vp(None)
+                                          for k in ("spec", "mass", "lbol", "young", "old")],
+           vp(idx.ctypes.data if idx is not None else None), vp(out["w"].ctypes.data if "w" in out else None),
+           vp(out["fband"].ctypes.data if "fband" in out else None))
+    out["status"] = st
+    return out
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
