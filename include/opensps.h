@@ -1,0 +1,125 @@
+/*
+ * opensps.h -- the public C ABI of libopensps.so for OpenSPS's batched composite-population path on MI355X.
+ *
+ * A drop-in for the reference's context API on this path (include/opensps.h of the reference; bindings in
+ * src/bindings/c/opensps_c_api.f90). The names, argument meaning, array layout (column-major, quantity fastest)
+ * and status codes are the reference's; each declaration cites the one it replaces. Entry points without a
+ * reference counterpart are marked "No reference counterpart". The batch runs on the GPU: without a HIP device
+ * a batch returns status 5, as the reference's offload builds do when their target regions would run on the host.
+ */
+#ifndef OPENSPS_H
+#define OPENSPS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OPENSPS_NBGRAD 44   /* rows of params_bar (reference include/opensps.h:15) */
+
+#define OPENSPS_STATUS_OK 0
+#define OPENSPS_ERR_UNKNOWN_INT_PARAM 101
+#define OPENSPS_ERR_UNKNOWN_FLOAT_PARAM 102
+#define OPENSPS_ERR_UNKNOWN_STRING_PARAM 103
+#define OPENSPS_ERR_COMPUTE_UNINIT 301
+#define OPENSPS_ERR_CSP_UNINIT 302
+
+#define OPENSPS_DETECTOR_PHOTON 0
+#define OPENSPS_DETECTOR_ENERGY 1
+#define OPENSPS_RESAMPLE_LINEAR 0
+#define OPENSPS_RESAMPLE_CONSERVING 1
+#define OPENSPS_GRID_POLICY_ERROR 0
+#define OPENSPS_GRID_POLICY_WARN 1
+#define OPENSPS_GRID_POLICY_ALLOW 2
+
+/* ---- driver utilities (reference include/opensps.h:109-126) ---- */
+void opensps_set_debug(int flag);
+void opensps_clear_error(void);
+void opensps_get_last_error(int *status, char *message, int message_len);
+void opensps_get_driver_version(int *major, int *minor, int *patch);
+const char *opensps_get_version_string(void);
+const char *opensps_get_build_id(void);
+void opensps_get_nbgrad(int *nbgrad);
+void opensps_get_grad_param_name(int grad_idx, char *name, int max_len, int *status);
+
+/* ---- contexts (reference include/opensps.h:132-151) ---- */
+void opensps_context_create(int *handle, int *status);
+void opensps_context_destroy(int handle, int *status);
+/* Set-up from a data release is not part of this build: status 301 (reference include/opensps.h:134-135). */
+void opensps_context_setup(int zin, const char *isoc_type, const char *spec_type, const char *dust_type, int handle,
+                           int *status);
+/* No reference counterpart (the reference builds its basis at set-up and installs it with
+ * opensps_context_update_ssp_basis, src/core/opensps_context.f90:757): a resident SSP basis from the caller.
+ * lambda (nspec) Angstrom increasing; log_age (ntfull) log10 yr increasing; zlegend (nz) metallicity mass
+ * fractions increasing; spec (nspec x ntfull x nz) L_sun/Hz per solar mass; mass, lbol (ntfull x nz) surviving
+ * mass and log10 L_bol. status 1 bad handle, 3 bad sizes or grids. Resets the filters. */
+void opensps_context_set_ssp_basis(int handle, int nspec, const double *lambda, int ntfull, const double *log_age,
+                                   int nz, const double *zlegend, double zsol, const double *spec,
+                                   const double *mass, const double *lbol, int *status);
+/* Parameter setters (reference include/opensps.h:145-147). Besides the reference's keys, set_str
+ * "kernel_precision" = "double" (default: FP64 age sums and chain) or "mixed" (split-BF16 age sums on the matrix
+ * cores, FP32 chain), and set_int "mixed_precision" = 0 / 1 select the kernels at run time (the reference selects
+ * them at build time, meson option kernel_precision). */
+void opensps_context_set_int(int handle, const char *key, int value, int *status);
+void opensps_context_set_float(int handle, const char *key, double value, int *status);
+void opensps_context_set_str(int handle, const char *key, const char *value, int *status);
+void opensps_context_set_fast_mode(int handle, int fast_mode, int *status);   /* include/opensps.h:150 */
+
+/* ---- dimensions and grids (reference include/opensps.h:153-175) ---- */
+void opensps_context_get_dims(int handle, int *n_spec, int *n_time, int *status);
+void opensps_context_get_nspec(int handle, int *n_spec, int *status);
+void opensps_context_get_ntfull(int handle, int *n_time, int *status);
+void opensps_context_get_nbands(int handle, int *n_bands, int *status);
+void opensps_context_get_nindx(int handle, int *n_indices, int *status);
+void opensps_context_get_nz(int handle, int *n_z, int *status);
+void opensps_context_get_nemline(int handle, int *n_line, int *status);
+void opensps_context_get_zsol(int handle, double *z_sol, int *status);
+void opensps_context_get_lambda(int handle, double *lambda, int *status);
+void opensps_context_get_ages(int handle, double *log_age, int *status);
+void opensps_context_get_zlegend(int handle, double *z, int *status);
+/* wave_eff: pivot wavelengths; mag_vega, mag_sun are 99 (a synthetic basis has no Vega or solar spectrum). */
+void opensps_context_get_filter_data(int handle, double *wave_eff, double *mag_vega, double *mag_sun, int *status);
+
+/* ---- filters (reference include/opensps.h:192-233) ---- */
+void opensps_filterset_create(int *fset, int *status);
+void opensps_filterset_add(int fset, const char *name, int n, const double *lambda_aa, const double *response,
+                           int detector, double cal_lambda_ref, double cal_exponent, int *status);
+void opensps_filterset_count(int fset, int *n, int *status);
+void opensps_filterset_free(int fset, int *status);
+void opensps_context_set_filters(int handle, int fset, int resample, int grid_policy, int vega_ref_band, int *status);
+void opensps_context_set_mag_compute(int handle, int n_bands, const int *mask, int *status);
+/* No reference counterpart: the context's bands on the model grid (nspec x nbands), normalised. */
+void opensps_context_get_bands(int handle, double *bands, int *status);
+
+/* ---- the batch (reference include/opensps.h:264-316) ---- */
+/* status 0, 1 bad handle / no basis, 2 unsupported (a parameter that changes the SSPs, spectral indices, dust
+ * emission, nebular emission, Vega magnitudes, an SFH type other than 0, 1, 4), 3 unknown parameter or bad
+ * shape, 5 no HIP device, 10 device error. */
+void opensps_context_compute_csp_batch(int handle, int n, int n_params, const char *const *names,
+                                       const double *const *values, const double *ssp_weights, double *mags,
+                                       double *spec, double *mass, double *lbol, double *mdust, double *sfr,
+                                       double *emlines, double *indx, int *status);
+void opensps_context_compute_csp_batch_parts(int handle, int n, int n_params, const char *const *names,
+                                             const double *const *values, const double *ssp_weights, double *mags,
+                                             double *spec, double *mass, double *lbol, double *mdust, double *sfr,
+                                             double *emlines, double *indx, double *young, double *old,
+                                             double *mass_young, double *mass_old, double *lbol_young,
+                                             double *lbol_old, int *status);
+void opensps_context_get_batch_maggies(int handle, double *maggies, int *n_bands, int *n, int *status);
+/* *bytes: 4 with kernel_precision "mixed", 8 with "double". */
+void opensps_context_batch_spec_device(int handle, void **ptr, int *nspec, int *n, int *bytes, int *status);
+
+/* No reference counterpart: the batch on device-resident arrays for callers that keep their data on the GPU
+ * (the reference's batch_spec_device serves them on the output side). d_prm: device OPS_NPAR x n doubles in the
+ * slots of opensps_params.h; d_weights: device ntfull x n or NULL; precision -1 (the context's), 0 FP64, 1
+ * mixed; split 1: young and old sums apart (birth-cloud or old-star screens). Device outputs: d_mags (nbands x n,
+ * required), d_spec (nspec x n, float with mixed precision, double with FP64), d_mass, d_lbol (n). */
+void opensps_context_compute_csp_batch_device(int handle, int n, const double *d_prm, const double *d_weights,
+                                              int precision, int split, double *d_mags, void *d_spec, double *d_mass,
+                                              double *d_lbol, int *status);
+/* No reference counterpart: the context's parameter values in the slot order of opensps_params.h. */
+void opensps_context_get_params(int handle, double *prm, int *status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,274 @@
+"""ctypes binding of the public C API (libopensps.so, include/opensps.h): the Python side of the drop-in.
+
+The calls mirror the reference's C API for the batched path (include/opensps.h:132-455), with the same names,
+argument meaning and status codes; the Fortran host behind them runs the batch on the GPU through
+libopensps_hip.so. Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import build as _build
+from .params import NPAR, PARAM_NAMES
+
+_lib = None
+
+STATUS_MESSAGES = {0: "ok", 1: "bad handle or not set up", 2: "unsupported", 3: "unknown parameter or bad shape",
+                   5: "no device", 10: "device error", 301: "not set up"}
+
+
+class OpenSPSError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_build.API_SO):
+            raise OpenSPSError(1, f"{_build.API_SO} is not built (run __graft_entry__.build())")
+        L = C.CDLL(_build.API_SO)
+        i, d, vp, ip, cp = C.c_int, C.c_double, C.c_void_p, C.POINTER(C.c_int), C.c_char_p
+        L.opensps_context_create.argtypes = [ip, ip]
+        L.opensps_context_destroy.argtypes = [i, ip]
+        L.opensps_context_setup.argtypes = [i, cp, cp, cp, i, ip]
+        L.opensps_context_set_ssp_basis.argtypes = [i, i, vp, i, vp, i, vp, d, vp, vp, vp, ip]
+        L.opensps_context_set_int.argtypes = [i, cp, i, ip]
+        L.opensps_context_set_float.argtypes = [i, cp, d, ip]
+        L.opensps_context_set_str.argtypes = [i, cp, cp, ip]
+        L.opensps_context_get_dims.argtypes = [i, ip, ip, ip]
+        L.opensps_context_get_nbands.argtypes = [i, ip, ip]
+        L.opensps_context_get_nz.argtypes = [i, ip, ip]
+        L.opensps_context_get_lambda.argtypes = [i, vp, ip]
+        L.opensps_context_get_bands.argtypes = [i, vp, ip]
+        L.opensps_context_get_params.argtypes = [i, vp, ip]
+        L.opensps_context_get_filter_data.argtypes = [i, vp, vp, vp, ip]
+        L.opensps_filterset_create.argtypes = [ip, ip]
+        L.opensps_filterset_add.argtypes = [i, cp, i, vp, vp, i, d, d, ip]
+        L.opensps_filterset_count.argtypes = [i, ip, ip]
+        L.opensps_filterset_free.argtypes = [i, ip]
+        L.opensps_context_set_filters.argtypes = [i, i, i, i, i, ip]
+        L.opensps_context_set_mag_compute.argtypes = [i, i, vp, ip]
+        L.opensps_context_compute_csp_batch.argtypes = [i, i, i, vp, vp, vp] + [vp] * 8 + [ip]
+        L.opensps_context_compute_csp_batch_parts.argtypes = [i, i, i, vp, vp, vp] + [vp] * 14 + [ip]
+        L.opensps_context_compute_csp_batch_device.argtypes = [i, i, vp, vp, i, i, vp, vp, vp, vp, ip]
+        L.opensps_context_batch_spec_device.argtypes = [i, C.POINTER(vp), ip, ip, ip, ip]
+        L.opensps_context_get_batch_maggies.argtypes = [i, vp, ip, ip, ip]
+        L.opensps_get_last_error.argtypes = [ip, C.c_char_p, i]
+        L.opensps_get_nbgrad.argtypes = [ip]
+        L.opensps_get_grad_param_name.argtypes = [i, C.c_char_p, i, ip]
+        L.opensps_get_version_string.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    st = C.c_int()
+    buf = C.create_string_buffer(512)
+    lib().opensps_get_last_error(C.byref(st), buf, 512)
+    return buf.value.decode()
+
+
+def _check(st):
+    v = st.value if isinstance(st, C.c_int) else int(st)
+    if v != 0:
+        raise OpenSPSError(v, last_error())
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One OpenSPS context (an int handle of the C API)."""
+
+    def __init__(self):
+        h, st = C.c_int(), C.c_int()
+        lib().opensps_context_create(C.byref(h), C.byref(st))
+        _check(st)
+        self.handle = h.value
+        self.nspec = self.nt = self.nz = self.nbands = 0
+
+    def close(self):
+        if self.handle:
+            st = C.c_int()
+            lib().opensps_context_destroy(self.handle, C.byref(st))
+            self.handle = 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- settings ----
+    def set_int(self, key, value):
+        st = C.c_int()
+        lib().opensps_context_set_int(self.handle, key.encode(), int(value), C.byref(st))
+        _check(st)
+
+    def set_float(self, key, value):
+        st = C.c_int()
+        lib().opensps_context_set_float(self.handle, key.encode(), float(value), C.byref(st))
+        _check(st)
+
+    def set_str(self, key, value):
+        st = C.c_int()
+        lib().opensps_context_set_str(self.handle, key.encode(), value.encode(), C.byref(st))
+        _check(st)
+
+    def set_ssp_basis(self, grid, zsol=0.0142):
+        st = C.c_int()
+        lam, age, zl = _d(grid.lam), _d(grid.log_age), _d(zsol * 10.0 ** np.asarray(grid.logz))
+        spec, mass, lbol = _d(grid.spec), _d(grid.mass), _d(grid.lbol)
+        lib().opensps_context_set_ssp_basis(self.handle, grid.nspec, lam.ctypes.data, grid.nt, age.ctypes.data,
+                                            grid.nz, zl.ctypes.data, float(zsol), spec.ctypes.data, mass.ctypes.data,
+                                            lbol.ctypes.data, C.byref(st))
+        _check(st)
+        self.nspec, self.nt, self.nz = grid.nspec, grid.nt, grid.nz
+
+    def set_filters(self, fset, resample=0, grid_policy=2):
+        fs, st = C.c_int(), C.c_int()
+        lib().opensps_filterset_create(C.byref(fs), C.byref(st))
+        _check(st)
+        try:
+            for name, lam, resp in zip(fset.names, fset.lam, fset.resp):
+                l, r = _d(lam), _d(resp)
+                lib().opensps_filterset_add(fs.value, name.encode(), l.size, l.ctypes.data, r.ctypes.data,
+                                            int(fset.detector), 0.0, 0.0, C.byref(st))
+                _check(st)
+            lib().opensps_context_set_filters(self.handle, fs.value, int(resample), int(grid_policy), 0, C.byref(st))
+            _check(st)
+        finally:
+            lib().opensps_filterset_free(fs.value, C.byref(C.c_int()))
+        nb = C.c_int()
+        lib().opensps_context_get_nbands(self.handle, C.byref(nb), C.byref(st))
+        self.nbands = nb.value
+
+    def set_mag_compute(self, mask):
+        st = C.c_int()
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.int32)
+        lib().opensps_context_set_mag_compute(self.handle, self.nbands if m is None else m.size,
+                                              None if m is None else m.ctypes.data, C.byref(st))
+        _check(st)
+
+    def bands(self):
+        """(nbands, nspec) bands on the grid."""
+        out = np.zeros((self.nbands, self.nspec))
+        st = C.c_int()
+        lib().opensps_context_get_bands(self.handle, out.ctypes.data, C.byref(st))
+        _check(st)
+        return out
+
+    def params(self):
+        out = np.zeros(NPAR)
+        st = C.c_int()
+        lib().opensps_context_get_params(self.handle, out.ctypes.data, C.byref(st))
+        _check(st)
+        return out
+
+    def wave_eff(self):
+        out = np.zeros(self.nbands)
+        st = C.c_int()
+        lib().opensps_context_get_filter_data(self.handle, out.ctypes.data, None, None, C.byref(st))
+        _check(st)
+        return out
+
+    # ---- batches ----
+    def compute_csp_batch(self, values: dict, n=None, ssp_weights=None, want=("mags", "spec", "mass", "lbol"),
+                          parts=False, raw_status=False):
+        """values: {name: array of n}. Returns a dict of (n, ...) C-order arrays (quantity fastest in memory)."""
+        names = list(values.keys())
+        if n is None:
+            n = len(next(iter(values.values()))) if values else (ssp_weights.shape[0] if ssp_weights is not None else 0)
+        keep = [_d(values[k]) for k in names]
+        cnames = (C.c_char_p * max(len(names), 1))(*[k.encode() for k in names])
+        cvals = (C.c_void_p * max(len(names), 1))(*[a.ctypes.data for a in keep])
+        out = {"mags": np.zeros((n, self.nbands))}
+        shapes = {"spec": (n, self.nspec), "mass": (n,), "lbol": (n,), "mdust": (n,), "sfr": (n,),
+                  "emlines": (n, 166), "young": (n, self.nspec), "old": (n, self.nspec), "mass_young": (n,),
+                  "mass_old": (n,), "lbol_young": (n,), "lbol_old": (n,)}
+        for k in want:
+            if k in shapes:
+                out[k] = np.zeros(shapes[k])
+        if parts:
+            for k in ("young", "old"):
+                out.setdefault(k, np.zeros(shapes[k]))
+        w = _d(ssp_weights) if ssp_weights is not None else None
+        P = lambda k: out[k].ctypes.data if k in out else None
+        st = C.c_int()
+        if parts:
+            lib().opensps_context_compute_csp_batch_parts(
+                self.handle, n, len(names), C.cast(cnames, C.c_void_p), C.cast(cvals, C.c_void_p),
+                None if w is None else w.ctypes.data, P("mags"), P("spec"), P("mass"), P("lbol"), P("mdust"),
+                P("sfr"), P("emlines"), None, P("young"), P("old"), P("mass_young"), P("mass_old"),
+                P("lbol_young"), P("lbol_old"), C.byref(st))
+        else:
+            lib().opensps_context_compute_csp_batch(
+                self.handle, n, len(names), C.cast(cnames, C.c_void_p), C.cast(cvals, C.c_void_p),
+                None if w is None else w.ctypes.data, P("mags"), P("spec"), P("mass"), P("lbol"), P("mdust"),
+                P("sfr"), P("emlines"), None, C.byref(st))
+        if raw_status:
+            out["status"] = st.value
+            return out
+        _check(st)
+        return out
+
+    def compute_csp_batch_params(self, prm, names, **kw):
+        """The batch with the named rows of a (NPAR, n) parameter array as names[] / values[]."""
+        from .params import SLOT
+        return self.compute_csp_batch({k: prm[SLOT[k]] for k in names}, n=prm.shape[1], **kw)
+
+    def compute_csp_batch_device(self, d_prm, d_mags, d_spec=None, d_mass=None, d_lbol=None, d_weights=None,
+                                 precision=-1, split=0):
+        """The batch on device-resident torch tensors (no host transfer)."""
+        st = C.c_int()
+        ptr = lambda t: None if t is None else t.data_ptr()
+        lib().opensps_context_compute_csp_batch_device(self.handle, d_prm.shape[1], ptr(d_prm), ptr(d_weights),
+                                                       int(precision), int(split), ptr(d_mags), ptr(d_spec),
+                                                       ptr(d_mass), ptr(d_lbol), C.byref(st))
+        _check(st)
+
+    def batch_spec_device(self):
+        p, ns, n, b, st = C.c_void_p(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        lib().opensps_context_batch_spec_device(self.handle, C.byref(p), C.byref(ns), C.byref(n), C.byref(b),
+                                                C.byref(st))
+        _check(st)
+        return p.value, ns.value, n.value, b.value
+
+    def batch_maggies(self):
+        nb, n, st = C.c_int(), C.c_int(), C.c_int()
+        lib().opensps_context_get_batch_maggies(self.handle, None, C.byref(nb), C.byref(n), C.byref(st))
+        _check(st)
+        out = np.zeros((n.value, nb.value))
+        lib().opensps_context_get_batch_maggies(self.handle, out.ctypes.data, C.byref(nb), C.byref(n), C.byref(st))
+        _check(st)
+        return out
+
+
+def nbgrad() -> int:
+    v = C.c_int()
+    lib().opensps_get_nbgrad(C.byref(v))
+    return v.value
+
+
+def grad_param_name(i) -> str:
+    buf = C.create_string_buffer(32)
+    st = C.c_int()
+    lib().opensps_get_grad_param_name(int(i), buf, 32, C.byref(st))
+    _check(st)
+    return buf.value.decode()
+
+
+def version() -> str:
+    return lib().opensps_get_version_string().decode()

@@ -421,7 +421,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     if (mixed)
         k_gemm_bf16x3<<<gblocks, 256, 0, c->stream>>>(gm);
     else
-        k_gemm_f64<<<gblocks, 256, 0, c->stream>>>(gm);
+        k_gemm_f64_vec<<<gblocks, 256, 0, c->stream>>>(gm);
     HIPCHK(c, hipGetLastError());
 
     ChainArgs ca;

@@ -429,6 +429,71 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs a) {
     }
 }
 
+// FP64 age sums with vector FMAs: each thread 4 pixels x 4 columns of the 64 x 64 tile, K staged through LDS
+// in chunks of 16 (the A operand read from its packed layout).
+__global__ __launch_bounds__(256) void k_gemm_f64_vec(GemmArgs a) {
+    int pt, ct;
+    if (!gemm_tile(a, pt, ct)) return;
+    const int4 td = a.tiles[ct];
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int ksteps = a.kpad / OPS_KSTEP_F64;
+    __shared__ double As[16][64 + 1];
+    __shared__ double Bs[16][64 + 1];
+    __shared__ const double* wcol[64];
+    __shared__ int ocol[64];
+    if (tid < 64) {
+        int c = tid;
+        bool v = c < td.z;
+        int cc = td.y + (v ? c : 0);
+        int pos = cc / a.parts;
+        int part = cc - pos * a.parts;
+        int model = a.order[pos];
+        wcol[c] = v ? a.w64 + ((size_t)model * a.parts + part) * a.kpad : nullptr;
+        ocol[c] = v ? part * a.n + model : -1;
+    }
+    __syncthreads();
+    double acc[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    const size_t pair_base = (size_t)td.x * a.npxblk;
+    for (int k0 = 0; k0 < a.kpad; k0 += 16) {
+        // A tile: 64 rows x 16 k (1024 values, 4 per thread)
+        for (int e = tid; e < 1024; e += 256) {
+            int r = e & 63, kk = e >> 6;
+            int row = pt * 64 + r, k = k0 + kk;
+            double v = 0.0;
+            if (k < a.kpad) {
+                size_t blk = pair_base + row / 16;
+                v = a.A64[(blk * ksteps + k / 4) * 64 + (row & 15) + 16 * (k & 3)];
+            }
+            As[kk][r] = v;
+        }
+        for (int e = tid; e < 1024; e += 256) {
+            int c = e & 63, kk = e >> 6;
+            int k = k0 + kk;
+            Bs[kk][c] = (wcol[c] && k < a.kpad) ? wcol[c][k] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            double av[4], bv[4];
+            for (int i = 0; i < 4; i++) av[i] = As[kk][ty * 4 + i];
+            for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx * 4 + j];
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+    double* yo = (double*)a.yo;
+    for (int j = 0; j < 4; j++) {
+        int oc = ocol[tx * 4 + j];
+        if (oc < 0) continue;
+        double* col = yo + (size_t)oc * a.nsp + pt * 64 + ty * 4;
+        for (int i = 0; i < 4; i++) col[i] = acc[i][j];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs a) {
     int pt, ct;
     if (!gemm_tile(a, pt, ct)) return;

@@ -61,6 +61,7 @@ __device__ __forceinline__ float coef_at<float>(const ChainArgs& a, int i) { ret
 __global__ void k_prep(PrepArgs a);
 __global__ void k_group(GroupArgs a);
 __global__ void k_gemm_f64(GemmArgs a);
+__global__ void k_gemm_f64_vec(GemmArgs a);
 __global__ void k_gemm_bf16x3(GemmArgs a);
 __global__ void k_pack_f64(PackArgs a);
 __global__ void k_pack_bf16(PackArgs a);

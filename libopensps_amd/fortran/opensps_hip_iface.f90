@@ -1,0 +1,117 @@
+!> iso_c_binding interfaces of libopensps_hip.so (include/opensps_hip.h): the device layer the Fortran host
+!> calls with device pointers, as the reference's batch driver calls its HIP launchers through bind(C)
+!> interface blocks (src/physics/opensps_csp_batch.f90:260-295, 2341-2349, 3742-3757).
+module opensps_hip_iface
+    use, intrinsic :: iso_c_binding, only: c_int, c_double, c_ptr, c_size_t, c_char, c_null_ptr
+    implicit none
+    private
+
+    integer(c_int), parameter, public :: HIP_FP64 = 0, HIP_MIXED = 1
+
+    !> opensps_hip_batch_t
+    type, bind(C), public :: hip_batch_t
+        integer(c_int) :: n = 0, precision = 0, split_parts = 0
+        type(c_ptr) :: prm = c_null_ptr, ssp_weights = c_null_ptr, mags = c_null_ptr, spec = c_null_ptr
+        type(c_ptr) :: mass = c_null_ptr, lbol = c_null_ptr, fband = c_null_ptr, young = c_null_ptr, old = c_null_ptr
+        type(c_ptr) :: idx = c_null_ptr, weights = c_null_ptr
+    end type hip_batch_t
+
+    public :: opensps_hip_create, opensps_hip_destroy, opensps_hip_set_grid, opensps_hip_set_bands, &
+              opensps_hip_set_mag_mask, opensps_hip_set_cosmology, opensps_hip_set_options, opensps_hip_csp_batch, &
+              opensps_hip_sync, opensps_hip_malloc, opensps_hip_free, opensps_hip_memcpy_h2d, opensps_hip_memcpy_d2h, &
+              opensps_hip_last_error, opensps_hip_device_count
+
+    interface
+        integer(c_int) function opensps_hip_create(device, ctx) bind(C, name="opensps_hip_create")
+            import :: c_int, c_ptr
+            integer(c_int), value :: device
+            type(c_ptr), intent(out) :: ctx
+        end function
+
+        integer(c_int) function opensps_hip_destroy(ctx) bind(C, name="opensps_hip_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function opensps_hip_set_grid(ctx, nspec, lam, nt, log_age, nz, logz, spec, mass, lbol) &
+            bind(C, name="opensps_hip_set_grid")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nspec, nt, nz
+            real(c_double), intent(in) :: lam(*), log_age(*), logz(*), spec(*), mass(*), lbol(*)
+        end function
+
+        integer(c_int) function opensps_hip_set_bands(ctx, nbands, bands) bind(C, name="opensps_hip_set_bands")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nbands
+            real(c_double), intent(in) :: bands(*)
+        end function
+
+        integer(c_int) function opensps_hip_set_mag_mask(ctx, nbands, mask) bind(C, name="opensps_hip_set_mag_mask")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nbands
+            integer(c_int), intent(in) :: mask(*)
+        end function
+
+        integer(c_int) function opensps_hip_set_cosmology(ctx, n, z, dl) bind(C, name="opensps_hip_set_cosmology")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: n
+            real(c_double), intent(in) :: z(*), dl(*)
+        end function
+
+        integer(c_int) function opensps_hip_set_options(ctx, add_igm, smooth_fast) bind(C, name="opensps_hip_set_options")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: add_igm, smooth_fast
+        end function
+
+        integer(c_int) function opensps_hip_csp_batch(ctx, args) bind(C, name="opensps_hip_csp_batch")
+            import :: c_int, c_ptr, hip_batch_t
+            type(c_ptr), value :: ctx
+            type(hip_batch_t), intent(in) :: args
+        end function
+
+        integer(c_int) function opensps_hip_sync(ctx) bind(C, name="opensps_hip_sync")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function opensps_hip_malloc(ptr, bytes) bind(C, name="opensps_hip_malloc")
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), intent(out) :: ptr
+            integer(c_size_t), value :: bytes
+        end function
+
+        integer(c_int) function opensps_hip_free(ptr) bind(C, name="opensps_hip_free")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ptr
+        end function
+
+        integer(c_int) function opensps_hip_memcpy_h2d(dst, src, bytes) bind(C, name="opensps_hip_memcpy_h2d")
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: dst, src
+            integer(c_size_t), value :: bytes
+        end function
+
+        integer(c_int) function opensps_hip_memcpy_d2h(dst, src, bytes) bind(C, name="opensps_hip_memcpy_d2h")
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: dst, src
+            integer(c_size_t), value :: bytes
+        end function
+
+        integer(c_int) function opensps_hip_last_error(ctx, msg, len) bind(C, name="opensps_hip_last_error")
+            import :: c_int, c_ptr, c_char
+            type(c_ptr), value :: ctx
+            character(kind=c_char), intent(out) :: msg(*)
+            integer(c_int), value :: len
+        end function
+
+        integer(c_int) function opensps_hip_device_count(n) bind(C, name="opensps_hip_device_count")
+            import :: c_int
+            integer(c_int), intent(out) :: n
+        end function
+    end interface
+end module opensps_hip_iface

@@ -1,0 +1,470 @@
+!> The batch driver of the Fortran host: the C API's names / values become one FP64 array of per-model
+!> parameters (a slot per parameter, as the reference's batch_handle resolves each name to one word of
+!> PARAMS_CORE, src/bindings/c/opensps_c_api.f90:663-961), the checks of compute_csp_batch's stage 0 run on the
+!> host (src/physics/opensps_csp_batch.f90:608-787), and stages 1-8 run on the GPU through libopensps_hip.
+module opensps_host_batch
+    use, intrinsic :: iso_c_binding
+    use opensps_host_types
+    use opensps_host_context
+    use opensps_hip_iface
+    implicit none
+    private
+    public :: batch_run, batch_run_device, cstring
+
+contains
+
+    !> A NUL-terminated C string as a Fortran string
+    function cstring(p) result(s)
+        type(c_ptr), intent(in) :: p
+        character(len=:), allocatable :: s
+        character(kind=c_char), pointer :: a(:)
+        integer :: n
+        s = ''
+        if (.not. c_associated(p)) return
+        call c_f_pointer(p, a, [4096])
+        n = 0
+        do while (n < 4096)
+            if (a(n + 1) == c_null_char) exit
+            n = n + 1
+        end do
+        allocate(character(len=n) :: s)
+        s = transfer(a(1:n), s)
+    end function cstring
+
+    pure real(WP) function expm1_ref(x) result(v)   ! opensps_expm1 (src/math/opensps_special_functions.f90:219-233)
+        real(WP), intent(in) :: x
+        if (abs(x) < 1.0e-5_wp) then
+            v = x * (1.0_wp + x * (0.5_wp + x * (1.0_wp / 6.0_wp)))
+        else
+            v = exp(x) - 1.0_wp
+        end if
+    end function expm1_ref
+
+    pure real(WP) function gammainc12(p, x) result(r)   ! gammainc for p = 1, 2 (opensps_special_functions.f90:142-211)
+        integer, intent(in) :: p
+        real(WP), intent(in) :: x
+        if (x < 0.0_wp) then
+            r = 0.0_wp
+        else if (p == 1) then
+            r = -expm1_ref(-x)
+        else
+            r = -expm1_ref(-x) - x * exp(-x)
+        end if
+    end function gammainc12
+
+    !> The SFR of a model at its age per unit mass formed: get_sfh_properties_at_age (src/physics/opensps_sfh.f90:282-410)
+    !> for sfh 0, 1, 4, divided by the mass fraction formed as batch_model_indices does (opensps_csp_batch.f90:1556-1560)
+    real(WP) function sfr_at_age(p, tmax_log) result(sfr)
+        real(WP), intent(in) :: p(NPAR), tmax_log
+        real(WP) :: t_max, t_prime, t_trunc, tau_g, tr, total_tau, mass_tau, sfr_tau, mfrac_b, sfr_c, mass_c, norm, mfrac
+        integer :: pw, sfh
+        sfh = nint(p(P_SFH))
+        sfr = 0.0_wp
+        if (sfh /= 1 .and. sfh /= 4) return
+        t_max = 10.0_wp**(tmax_log - 9.0_wp) - p(P_SF_START)
+        t_prime = p(P_TAGE) - p(P_SF_START)
+        if (p(P_SF_TRUNC) < SAFE_FLOOR .or. p(P_SF_TRUNC) < p(P_SF_START)) then
+            t_trunc = t_max
+        else
+            t_trunc = p(P_SF_TRUNC) - p(P_SF_START)
+        end if
+        tau_g = max(p(P_TAU), SAFE_FLOOR)
+        pw = merge(1, 2, sfh == 1)
+        total_tau = tau_g * gammainc12(pw, min(t_max, t_trunc) / tau_g)
+        if (t_prime < 0.0_wp) then
+            mass_tau = 0.0_wp
+            sfr_tau = 0.0_wp
+        else
+            tr = min(t_prime, t_trunc) / tau_g
+            mass_tau = tau_g * gammainc12(pw, tr)
+            sfr_tau = (tr**(pw - 1)) * exp(-tr)
+        end if
+        mfrac_b = merge(1.0_wp, 0.0_wp, t_prime > (p(P_TBURST) - p(P_SF_START)))
+        if (t_prime > 0.0_wp) then
+            sfr_c = 1.0_wp / min(t_max, t_trunc)
+            mass_c = max(min(t_prime, t_trunc), 0.0_wp) * sfr_c
+        else
+            sfr_c = 0.0_wp
+            mass_c = 0.0_wp
+        end if
+        norm = 1.0_wp - p(P_CONST) - p(P_FBURST)
+        mfrac = (norm * mass_tau / total_tau) + (p(P_CONST) * mass_c) + (p(P_FBURST) * mfrac_b)
+        if (t_prime > t_trunc) then
+            sfr = 0.0_wp
+        else
+            sfr = (norm * sfr_tau / total_tau) + (p(P_CONST) * sfr_c)
+        end if
+        sfr = sfr / 1.0e9_wp
+        if (mfrac > SAFE_FLOOR) sfr = sfr / mfrac
+    end function sfr_at_age
+
+    !> Per-model checks of stage 0 that this build adds to the reference's: options whose stages it does
+    !> not compute. status 2 with a message, else 0.
+    subroutine model_checks(c, prm, n, st)
+        type(host_ctx_t), intent(in) :: c
+        real(WP), intent(in) :: prm(n, NPAR)
+        integer, intent(in) :: n
+        integer, intent(out) :: st
+        st = 0
+        if (c%add_dust_emission == 1 .and. any(prm(:, P_DUST1) > SAFE_FLOOR .or. prm(:, P_DUST2) > SAFE_FLOOR)) then
+            st = 2
+            call set_error(2, 'batch: dust emission (add_dust_emission = 1 with dust) is not part of this build')
+        else if (c%add_neb_emission == 1) then
+            st = 2
+            call set_error(2, 'batch: nebular emission is not part of this build')
+        else if (c%compute_vega_mags == 1) then
+            st = 2
+            call set_error(2, 'batch: Vega magnitudes need a Vega spectrum, which a synthetic basis does not have')
+        else if (c%redshift_colors /= 0) then
+            st = 2
+            call set_error(2, 'batch: redshifted colours need zred_mags')
+        else if (c%dust_type /= 0) then
+            st = 2
+            call set_error(2, 'batch: only the power-law attenuation curve (dust_type 0) is part of this build')
+        end if
+    end subroutine model_checks
+
+    !> compute_csp_batch through the C API (batch_handle without the gradients)
+    subroutine batch_run(h, n, n_params, c_names, c_values, c_weights, c_mags, c_spec, c_mass, c_lbol, c_mdust, c_sfr, &
+                         c_emlines, c_indx, c_young, c_old, c_mass_young, c_mass_old, c_lbol_young, c_lbol_old, status)
+        integer, intent(in) :: h, n, n_params
+        type(c_ptr), intent(in) :: c_names, c_values, c_weights, c_mags, c_spec, c_mass, c_lbol, c_mdust, c_sfr, &
+                                   c_emlines, c_indx, c_young, c_old, c_mass_young, c_mass_old, c_lbol_young, c_lbol_old
+        integer, intent(out) :: status
+        type(c_ptr), pointer :: np(:), vp(:)
+        real(c_double), pointer :: v(:), outv(:), outm(:, :), w_in(:, :)
+        real(WP), allocatable, target :: prm(:, :), wts(:, :), hbuf(:, :), mblend(:)
+        real(c_float), allocatable, target :: fbuf(:, :)
+        integer(c_int), allocatable, target :: idx(:, :)
+        integer :: i, k, m, st, nb, ns, nt, bytes, it
+        logical :: parts, split, want_w
+        character(len=:), allocatable :: nm
+        type(hip_batch_t) :: a
+        integer(c_int) :: rc
+        real(WP) :: mk, lk, dz, wk
+        real(WP), allocatable :: msy(:), mso(:), lsy(:), lso(:)
+
+        status = 0
+        if (.not. ctx_valid(h)) then
+            status = 1
+            call set_error(1, 'batch: bad handle')
+            return
+        end if
+        associate(c => ctxs(h))
+        if (.not. c%has_basis) then
+            status = 1
+            call set_error(1, 'batch: no SSP basis installed')
+            return
+        end if
+        if (n < 0 .or. n_params < 0) then
+            status = 3
+            call set_error(3, 'batch: n or n_params < 0')
+            return
+        end if
+        if (n == 0) return
+        if (.not. c_associated(c_mags)) then
+            status = 3
+            call set_error(3, 'batch: mags is required')
+            return
+        end if
+        if (c_associated(c_indx)) then
+            status = 2
+            call set_error(2, 'batch: spectral indices are not part of this build')
+            return
+        end if
+        nb = c%nbands
+        ns = c%nspec
+        nt = c%nt
+        ! names -> slots; the values of the named parameters per model over the context's
+        allocate(prm(n, NPAR))
+        do k = 1, NPAR
+            prm(:, k) = c%pset(k)
+        end do
+        if (n_params > 0) then
+            if (.not. c_associated(c_names) .or. .not. c_associated(c_values)) then
+                status = 3
+                call set_error(3, 'batch: names / values are NULL')
+                return
+            end if
+            call c_f_pointer(c_names, np, [n_params])
+            call c_f_pointer(c_values, vp, [n_params])
+            do i = 1, n_params
+                nm = lower(cstring(np(i)))
+                if (any(SSP_NAMES == nm)) then
+                    status = 2
+                    call set_error(2, 'batch: ' // nm // ' changes the SSPs and cannot vary in a batch')
+                    return
+                end if
+                k = param_slot(nm)
+                if (k == 0) then
+                    status = 3
+                    call set_error(3, 'batch: unknown parameter ' // nm)
+                    return
+                end if
+                if (.not. c_associated(vp(i))) then
+                    status = 3
+                    call set_error(3, 'batch: NULL values for ' // nm)
+                    return
+                end if
+                call c_f_pointer(vp(i), v, [n])
+                prm(:, k) = v(1:n)
+            end do
+        end if
+        call model_checks(c, prm, n, st)
+        if (st /= 0) then
+            status = st
+            return
+        end if
+        parts = c_associated(c_young) .and. c_associated(c_old)
+        split = parts .or. any((prm(:, P_DUST1) > SAFE_FLOOR .and. 1.0_wp - prm(:, P_FRAC_OBRUN) > SAFE_FLOOR) .or. &
+                               abs(prm(:, P_DUST3)) > SAFE_FLOOR)
+        want_w = c_associated(c_mass_young) .or. c_associated(c_mass_old) .or. c_associated(c_lbol_young) .or. &
+                 c_associated(c_lbol_old)
+        st = ctx_device(h)
+        if (st /= 0) then
+            status = st
+            return
+        end if
+        bytes = merge(4, 8, c%mixed == 1)
+        st = 0
+        st = st + dbuf_reserve(c%d_prm, int(8, c_size_t) * NPAR * n)
+        st = st + dbuf_reserve(c%d_mags, int(8, c_size_t) * max(nb, 1) * n)
+        st = st + dbuf_reserve(c%d_spec, int(bytes, c_size_t) * ns * n)
+        st = st + dbuf_reserve(c%d_mass, int(8, c_size_t) * n)
+        st = st + dbuf_reserve(c%d_lbol, int(8, c_size_t) * n)
+        st = st + dbuf_reserve(c%d_idx, int(4, c_size_t) * 3 * n)
+        if (c%keep_maggies == 1) st = st + dbuf_reserve(c%d_fband, int(8, c_size_t) * max(nb, 1) * n)
+        if (c_associated(c_weights) .or. want_w) st = st + dbuf_reserve(c%d_w, int(8, c_size_t) * nt * n)
+        if (parts) then
+            st = st + dbuf_reserve(c%d_young, int(bytes, c_size_t) * ns * n)
+            st = st + dbuf_reserve(c%d_old, int(bytes, c_size_t) * ns * n)
+        end if
+        if (st /= 0) then
+            status = 10
+            call set_error(10, 'batch: device allocation failed')
+            return
+        end if
+        rc = opensps_hip_memcpy_h2d(c%d_prm%p, c_loc(prm), int(8, c_size_t) * NPAR * n)
+        if (c_associated(c_weights)) then
+            call c_f_pointer(c_weights, w_in, [nt, n])
+            allocate(wts(nt, n))
+            wts = w_in
+            rc = opensps_hip_memcpy_h2d(c%d_w%p, c_loc(wts), int(8, c_size_t) * nt * n)
+        end if
+        a%n = n
+        a%precision = merge(HIP_MIXED, HIP_FP64, c%mixed == 1)
+        a%split_parts = merge(1, 0, split)
+        a%prm = c%d_prm%p
+        a%ssp_weights = c_null_ptr
+        if (c_associated(c_weights)) a%ssp_weights = c%d_w%p
+        a%mags = c%d_mags%p
+        a%spec = c%d_spec%p
+        a%mass = c%d_mass%p
+        a%lbol = c%d_lbol%p
+        a%fband = c_null_ptr
+        if (c%keep_maggies == 1) a%fband = c%d_fband%p
+        a%young = c_null_ptr
+        a%old = c_null_ptr
+        if (parts) then
+            a%young = c%d_young%p
+            a%old = c%d_old%p
+        end if
+        a%idx = c%d_idx%p
+        a%weights = c_null_ptr
+        if (want_w .and. .not. c_associated(c_weights)) a%weights = c%d_w%p
+        rc = opensps_hip_csp_batch(c%dctx, a)
+        if (rc == 0) rc = opensps_hip_sync(c%dctx)
+        if (rc /= 0) then
+            status = merge(int(rc), 10, rc == 2 .or. rc == 3)
+            call set_error(status, 'batch: ' // trim(hip_message(c%dctx)))
+            return
+        end if
+        c%last_n = n
+        c%last_bytes = bytes
+        ! outputs to the caller
+        if (nb > 0) rc = opensps_hip_memcpy_d2h(c_mags, c%d_mags%p, int(8, c_size_t) * nb * n)
+        if (c_associated(c_mass)) rc = opensps_hip_memcpy_d2h(c_mass, c%d_mass%p, int(8, c_size_t) * n)
+        if (c_associated(c_lbol)) rc = opensps_hip_memcpy_d2h(c_lbol, c%d_lbol%p, int(8, c_size_t) * n)
+        call copy_spectra(c_spec, c%d_spec%p)
+        if (parts) then
+            call copy_spectra(c_young, c%d_young%p)
+            call copy_spectra(c_old, c%d_old%p)
+        end if
+        if (c_associated(c_mdust)) then
+            call c_f_pointer(c_mdust, outv, [n])
+            outv = SAFE_FLOOR
+        end if
+        if (c_associated(c_emlines)) then
+            call c_f_pointer(c_emlines, outm, [166, n])
+            outm = 0.0_wp
+        end if
+        if (c_associated(c_sfr)) then
+            call c_f_pointer(c_sfr, outv, [n])
+            do m = 1, n
+                if (c_associated(c_weights)) then
+                    outv(m) = ieee_nan()
+                else
+                    outv(m) = sfr_at_age(prm(m, :), c%log_age(nt))
+                end if
+            end do
+        end if
+        if (c%keep_maggies == 1 .and. nb > 0) then
+            if (allocated(c%last_maggies)) deallocate(c%last_maggies)
+            allocate(c%last_maggies(nb, n))
+            call maggies_from_device(c, n, prm)
+        end if
+        if (want_w) then
+            ! the masses and L_bol of the young and old parts from the weights (model_sums, :2129-2176)
+            allocate(hbuf(nt, n), idx(3, n), msy(n), mso(n), lsy(n), lso(n))
+            rc = opensps_hip_memcpy_d2h(c_loc(hbuf), c%d_w%p, int(8, c_size_t) * nt * n)
+            rc = opensps_hip_memcpy_d2h(c_loc(idx), c%d_idx%p, int(4, c_size_t) * 3 * n)
+            msy = 0.0_wp; mso = 0.0_wp; lsy = 0.0_wp; lso = 0.0_wp
+            do m = 1, n
+                it = idx(1, m)
+                dz = 0.0_wp
+                if (c%nz > 1) dz = (prm(m, P_LOGZSOL) - c%logz(idx(2, m))) / (c%logz(idx(2, m) + 1) - c%logz(idx(2, m)))
+                do k = 1, nt
+                    wk = hbuf(k, m)
+                    if (.not. (wk > SAFE_FLOOR)) cycle
+                    if (c%nz > 1) then
+                        mk = wk * ((1.0_wp - dz) * c%mass(k, idx(2, m)) + dz * c%mass(k, idx(2, m) + 1))
+                        lk = wk * 10.0_wp**((1.0_wp - dz) * c%lbol(k, idx(2, m)) + dz * c%lbol(k, idx(2, m) + 1))
+                    else
+                        mk = wk * c%mass(k, 1)
+                        lk = wk * 10.0_wp**c%lbol(k, 1)
+                    end if
+                    if (k <= it) then
+                        msy(m) = msy(m) + mk; lsy(m) = lsy(m) + lk
+                    else
+                        mso(m) = mso(m) + mk; lso(m) = lso(m) + lk
+                    end if
+                end do
+            end do
+            call put(c_mass_young, msy)
+            call put(c_mass_old, mso)
+            call put(c_lbol_young, log10(max(lsy, tiny(1.0_wp))))
+            call put(c_lbol_old, log10(max(lso, tiny(1.0_wp))))
+        end if
+        end associate
+
+    contains
+
+        subroutine put(p, x)
+            type(c_ptr), intent(in) :: p
+            real(WP), intent(in) :: x(:)
+            real(c_double), pointer :: o(:)
+            if (.not. c_associated(p)) return
+            call c_f_pointer(p, o, [size(x)])
+            o = x
+        end subroutine put
+
+        !> spectra (nspec, n) from the device to the caller's doubles (converted from FP32 in mixed precision)
+        subroutine copy_spectra(dst, src)
+            type(c_ptr), intent(in) :: dst, src
+            real(c_double), pointer :: o(:, :)
+            integer(c_int) :: r
+            if (.not. c_associated(dst)) return
+            if (bytes == 8) then
+                r = opensps_hip_memcpy_d2h(dst, src, int(8, c_size_t) * ns * n)
+            else
+                if (.not. allocated(fbuf)) allocate(fbuf(ns, n))
+                r = opensps_hip_memcpy_d2h(c_loc(fbuf), src, int(4, c_size_t) * ns * n)
+                call c_f_pointer(dst, o, [ns, n])
+                o = real(fbuf, WP)
+            end if
+        end subroutine copy_spectra
+
+    end subroutine batch_run
+
+    !> Linear AB fluxes of the last batch (compute_magnitudes_columns' maggies_out,
+    !> src/spectra/opensps_batch_photometry.f90:166-179): band flux x 10^(0.4 (48.60 + 2.5 ABS_MAG_ZEROPOINT_LOG -
+    !> distance term)), NaN for bands not computed.
+    subroutine maggies_from_device(c, n, prm)
+        type(host_ctx_t), intent(inout) :: c
+        integer, intent(in) :: n
+        real(WP), intent(in) :: prm(n, NPAR)
+        real(WP), allocatable, target :: fb(:, :)
+        integer(c_int) :: rc
+        integer :: m, b, i
+        real(WP) :: z, dm, dist, scale, amzl, pi, pc, slope
+        allocate(fb(c%nbands, n))
+        rc = opensps_hip_memcpy_d2h(c_loc(fb), c%d_fband%p, int(8, c_size_t) * c%nbands * n)
+        pi = acos(-1.0_wp)
+        pc = (648000.0_wp / pi) * 1.495978707e13_wp
+        amzl = log10(3.828e33_wp / (4.0_wp * pi * (10.0_wp * pc) * (10.0_wp * pc)))
+        do m = 1, n
+            z = prm(m, P_ZRED)
+            dist = 0.0_wp
+            if (abs(z) > SAFE_FLOOR) then
+                i = 1
+                do while (i < 499 .and. c%cz(i + 1) > z)
+                    i = i + 1
+                end do
+                slope = (c%cdl(i + 1) - c%cdl(i)) / (c%cz(i + 1) - c%cz(i))
+                dm = c%cdl(i) + slope * (z - c%cz(i))
+                if (dm == dm .and. dm > SAFE_FLOOR) dist = 5.0_wp * log10(dm / 10.0_wp) - 2.5_wp * log10(1.0_wp + z)
+            end if
+            scale = 10.0_wp**(0.4_wp * (48.60_wp + 2.5_wp * amzl - dist))
+            do b = 1, c%nbands
+                if (allocated(c%mag_mask)) then
+                    if (c%mag_mask(b) == 0) then
+                        c%last_maggies(b, m) = ieee_nan()
+                        cycle
+                    end if
+                end if
+                c%last_maggies(b, m) = fb(b, m) * scale
+            end do
+        end do
+    end subroutine maggies_from_device
+
+    real(WP) function ieee_nan()
+        use, intrinsic :: ieee_arithmetic, only: ieee_value, ieee_quiet_nan
+        ieee_nan = ieee_value(1.0_wp, ieee_quiet_nan)
+    end function ieee_nan
+
+    !> A batch on device-resident data: d_prm (NPAR, n) per-model parameters in the slots of
+    !> include/opensps_params.h, d_weights (ntfull, n) or NULL, device outputs (mags required). precision
+    !> -1: the context's; split 1: young and old apart. No transfer to or from the host.
+    subroutine batch_run_device(h, n, d_prm, d_weights, precision, split, d_mags, d_spec, d_mass, d_lbol, status)
+        integer, intent(in) :: h, n, precision, split
+        type(c_ptr), intent(in) :: d_prm, d_weights, d_mags, d_spec, d_mass, d_lbol
+        integer, intent(out) :: status
+        type(hip_batch_t) :: a
+        integer(c_int) :: rc
+        status = 0
+        if (.not. ctx_valid(h)) then
+            status = 1
+            call set_error(1, 'batch_device: bad handle')
+            return
+        end if
+        if (.not. ctxs(h)%has_basis) then
+            status = 1
+            call set_error(1, 'batch_device: no SSP basis installed')
+            return
+        end if
+        if (n < 0 .or. .not. c_associated(d_prm) .or. (.not. c_associated(d_mags) .and. ctxs(h)%nbands > 0)) then
+            status = 3
+            call set_error(3, 'batch_device: bad arguments')
+            return
+        end if
+        if (n == 0) return
+        status = ctx_device(h)
+        if (status /= 0) return
+        a%n = n
+        a%precision = merge(HIP_MIXED, HIP_FP64, (precision < 0 .and. ctxs(h)%mixed == 1) .or. precision == 1)
+        a%split_parts = merge(1, 0, split /= 0)
+        a%prm = d_prm
+        a%ssp_weights = d_weights
+        a%mags = d_mags
+        a%spec = d_spec
+        a%mass = d_mass
+        a%lbol = d_lbol
+        rc = opensps_hip_csp_batch(ctxs(h)%dctx, a)
+        if (rc == 0) rc = opensps_hip_sync(ctxs(h)%dctx)
+        if (rc /= 0) then
+            status = merge(int(rc), 10, rc == 2 .or. rc == 3)
+            call set_error(status, 'batch_device: ' // trim(hip_message(ctxs(h)%dctx)))
+        end if
+    end subroutine batch_run_device
+
+end module opensps_host_batch

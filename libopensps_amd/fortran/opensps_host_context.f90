@@ -1,0 +1,339 @@
+!> Contexts of the Fortran host: settings and parameters (the reference's opensps_context_t, its key/value
+!> setters src/core/opensps_context.f90:294-666 and defaults :60-100), the resident SSP basis and grids, the
+!> filters, the cosmology table, and the device side (a libopensps_hip context plus the device buffers of the
+!> batch, kept between calls as the reference keeps batch_out, src/core/opensps_context_types.f90:62-85).
+!> The device is touched on the first batch, so that a context can be set up and checked without a GPU.
+module opensps_host_context
+    use, intrinsic :: iso_c_binding, only: c_ptr, c_null_ptr, c_associated, c_int, c_size_t, c_double, c_char, c_loc
+    use opensps_host_types
+    use opensps_hip_iface
+    use opensps_host_cosmology, only: cosmology_table
+    implicit none
+    private
+
+    type, public :: dbuf_t
+        type(c_ptr) :: p = c_null_ptr
+        integer(c_size_t) :: bytes = 0
+    end type dbuf_t
+
+    type, public :: host_ctx_t
+        logical :: used = .false.
+        ! settings (integer keys)
+        integer :: add_igm_absorption = 0, add_dust_emission = 1, add_neb_emission = 0, add_agn_dust = 1
+        integer :: smooth_velocity = 1, smoothspec_fast = 1, dust_type = 0, compute_vega_mags = 0
+        integer :: redshift_colors = 0, keep_maggies = 0, imf_type = 2, mixed = 0
+        real(WP) :: h0 = 72.0_wp, om0 = 0.27_wp, ol0 = 0.73_wp
+        real(WP) :: pset(NPAR) = PARAM_DEFAULTS
+        ! the resident basis
+        logical :: has_basis = .false.
+        integer :: nspec = 0, nt = 0, nz = 0
+        real(WP) :: zsol = 0.0142_wp
+        real(WP), allocatable :: lam(:), log_age(:), zlegend(:), logz(:), spec(:, :, :), mass(:, :), lbol(:, :)
+        ! filters
+        integer :: nbands = 0
+        real(WP), allocatable :: bands(:, :), wave_eff(:)
+        integer, allocatable :: mag_mask(:), bflags(:)
+        integer :: resample = 0, grid_policy = 2
+        ! cosmology
+        real(WP) :: cz(500) = 0.0_wp, cdl(500) = 0.0_wp
+        logical :: cosmo_dirty = .true.
+        ! device side
+        type(c_ptr) :: dctx = c_null_ptr
+        logical :: dev_grid = .false., dev_bands = .false., dev_cosmo = .false., dev_mask = .false.
+        type(dbuf_t) :: d_prm, d_w, d_mags, d_spec, d_mass, d_lbol, d_fband, d_young, d_old, d_idx
+        integer :: last_n = 0, last_bytes = 8
+        real(WP), allocatable :: last_maggies(:, :)
+    end type host_ctx_t
+
+    type(host_ctx_t), allocatable, target, public, save :: ctxs(:)
+    type(host_ctx_t), save :: blank_ctx   ! (default initialised: the state of a new context)
+
+    public :: ctx_create, ctx_destroy, ctx_valid, ctx_set_int, ctx_set_float, ctx_set_str, ctx_set_basis, &
+              ctx_device, dbuf_reserve, ctx_refresh_cosmology, hip_message
+
+contains
+
+    logical function ctx_valid(h)
+        integer, intent(in) :: h
+        ctx_valid = .false.
+        if (.not. allocated(ctxs)) return
+        if (h < 1 .or. h > size(ctxs)) return
+        ctx_valid = ctxs(h)%used
+    end function ctx_valid
+
+    integer function ctx_create() result(h)
+        type(host_ctx_t), allocatable :: tmp(:)
+        integer :: i, n
+        if (.not. allocated(ctxs)) allocate(ctxs(8))
+        do i = 1, size(ctxs)
+            if (.not. ctxs(i)%used) then
+                ctxs(i) = blank_ctx
+                ctxs(i)%used = .true.
+                h = i
+                return
+            end if
+        end do
+        n = size(ctxs)
+        allocate(tmp(2 * n))
+        tmp(1:n) = ctxs
+        call move_alloc(tmp, ctxs)
+        ctxs(n + 1) = blank_ctx
+        ctxs(n + 1)%used = .true.
+        h = n + 1
+    end function ctx_create
+
+    subroutine dbuf_free(b)
+        type(dbuf_t), intent(inout) :: b
+        integer(c_int) :: st
+        if (c_associated(b%p)) st = opensps_hip_free(b%p)
+        b%p = c_null_ptr
+        b%bytes = 0
+    end subroutine dbuf_free
+
+    subroutine ctx_destroy(h)
+        integer, intent(in) :: h
+        integer(c_int) :: st
+        if (.not. ctx_valid(h)) return
+        associate(c => ctxs(h))
+            call dbuf_free(c%d_prm); call dbuf_free(c%d_w); call dbuf_free(c%d_mags); call dbuf_free(c%d_spec)
+            call dbuf_free(c%d_mass); call dbuf_free(c%d_lbol); call dbuf_free(c%d_fband); call dbuf_free(c%d_young)
+            call dbuf_free(c%d_old); call dbuf_free(c%d_idx)
+            if (c_associated(c%dctx)) st = opensps_hip_destroy(c%dctx)
+        end associate
+        ctxs(h) = blank_ctx
+    end subroutine ctx_destroy
+
+    !> A device buffer of at least `bytes` (grown, never shrunk); status 10 on a device error
+    integer function dbuf_reserve(b, bytes) result(st)
+        type(dbuf_t), intent(inout) :: b
+        integer(c_size_t), intent(in) :: bytes
+        st = 0
+        if (b%bytes >= bytes .and. c_associated(b%p)) return
+        call dbuf_free(b)
+        st = opensps_hip_malloc(b%p, max(bytes, 256_c_size_t))
+        if (st == 0) b%bytes = max(bytes, 256_c_size_t)
+    end function dbuf_reserve
+
+    !> Integer settings (opensps_context_set_int). status 0, or 101 for an unknown key
+    subroutine ctx_set_int(h, key, v, status)
+        integer, intent(in) :: h, v
+        character(len=*), intent(in) :: key
+        integer, intent(out) :: status
+        integer :: k
+        status = 0
+        associate(c => ctxs(h))
+        select case (lower(trim(key)))
+        case ('add_igm_absorption'); c%add_igm_absorption = v; c%dev_grid = .false.
+        case ('add_dust_emission'); c%add_dust_emission = v
+        case ('add_neb_emission'); c%add_neb_emission = v
+        case ('add_agn_dust'); c%add_agn_dust = v
+        case ('smooth_velocity'); c%smooth_velocity = v; c%dev_grid = .false.
+        case ('smoothspec_fast'); c%smoothspec_fast = v; c%dev_grid = .false.
+        case ('dust_type'); c%dust_type = v
+        case ('compute_vega_mags'); c%compute_vega_mags = v
+        case ('redshift_colors'); c%redshift_colors = v
+        case ('keep_maggies'); c%keep_maggies = v
+        case ('imf_type'); c%imf_type = v
+        case ('mixed_precision'); c%mixed = merge(1, 0, v /= 0)
+        case ('sfh'); c%pset(P_SFH) = real(v, WP)
+        case default
+            k = param_slot(key)
+            if (k > 0) then
+                c%pset(k) = real(v, WP)
+            else
+                status = 101
+            end if
+        end select
+        end associate
+    end subroutine ctx_set_int
+
+    !> Float settings and parameters (opensps_context_set_float). status 0, or 102 for an unknown key
+    subroutine ctx_set_float(h, key, v, status)
+        integer, intent(in) :: h
+        character(len=*), intent(in) :: key
+        real(WP), intent(in) :: v
+        integer, intent(out) :: status
+        integer :: k
+        status = 0
+        associate(c => ctxs(h))
+        select case (lower(trim(key)))
+        case ('h0')
+            if (v > 0.0_wp) then
+                if (v /= c%h0) c%cosmo_dirty = .true.
+                c%h0 = v
+            end if
+        case ('om0')
+            if (v >= 0.0_wp .and. v + c%ol0 > 0.0_wp) then
+                if (v /= c%om0) c%cosmo_dirty = .true.
+                c%om0 = v
+            end if
+        case ('ol0')
+            if (c%om0 + v > 0.0_wp) then
+                if (v /= c%ol0) c%cosmo_dirty = .true.
+                c%ol0 = v
+            end if
+        case default
+            k = param_slot(key)
+            if (k > 0) then
+                c%pset(k) = v
+            else
+                status = 102
+            end if
+        end select
+        end associate
+    end subroutine ctx_set_float
+
+    !> String settings. kernel_precision: "double" (FP64 age sums and chain, the reference's default build) or
+    !> "mixed" / "single" (split-BF16 age sums, FP32 chain). status 0, or 103 for an unknown key or value
+    subroutine ctx_set_str(h, key, v, status)
+        integer, intent(in) :: h
+        character(len=*), intent(in) :: key, v
+        integer, intent(out) :: status
+        status = 0
+        select case (lower(trim(key)))
+        case ('kernel_precision')
+            select case (lower(trim(v)))
+            case ('double', 'fp64'); ctxs(h)%mixed = 0
+            case ('mixed', 'single', 'fp32'); ctxs(h)%mixed = 1
+            case default; status = 103
+            end select
+        case default
+            status = 103
+        end select
+    end subroutine ctx_set_str
+
+    !> Installs a resident SSP basis (the reference's set-up plus opensps_context_update_ssp_basis,
+    !> src/core/opensps_context.f90:757): lam (nspec), log_age (nt), zlegend (nz, mass fractions), spec (nspec,
+    !> nt, nz), mass (nt, nz), lbol (nt, nz). status 3 for bad sizes or non-increasing grids.
+    subroutine ctx_set_basis(h, nspec, lam, nt, log_age, nz, zlegend, zsol, spec, mass, lbol, status)
+        integer, intent(in) :: h, nspec, nt, nz
+        real(WP), intent(in) :: lam(nspec), log_age(nt), zlegend(nz), zsol, spec(nspec, nt, nz), mass(nt, nz), &
+                                lbol(nt, nz)
+        integer, intent(out) :: status
+        integer :: i
+        status = 0
+        if (nspec < 2 .or. nt < 2 .or. nz < 1 .or. zsol <= 0.0_wp) then
+            status = 3; return
+        end if
+        do i = 2, nspec
+            if (.not. (lam(i) > lam(i - 1))) then
+                status = 3; return
+            end if
+        end do
+        do i = 2, nt
+            if (.not. (log_age(i) > log_age(i - 1))) then
+                status = 3; return
+            end if
+        end do
+        if (any(zlegend <= 0.0_wp)) then
+            status = 3; return
+        end if
+        do i = 2, nz
+            if (.not. (zlegend(i) > zlegend(i - 1))) then
+                status = 3; return
+            end if
+        end do
+        associate(c => ctxs(h))
+            c%nspec = nspec; c%nt = nt; c%nz = nz; c%zsol = zsol
+            c%lam = lam; c%log_age = log_age; c%zlegend = zlegend
+            c%logz = log10(zlegend / zsol)
+            c%spec = spec; c%mass = mass; c%lbol = lbol
+            c%has_basis = .true.
+            c%dev_grid = .false.
+            ! a new grid invalidates the filters (they are built on the grid)
+            c%nbands = 0
+            if (allocated(c%bands)) deallocate(c%bands)
+            c%dev_bands = .false.
+        end associate
+    end subroutine ctx_set_basis
+
+    subroutine ctx_refresh_cosmology(h)
+        integer, intent(in) :: h
+        associate(c => ctxs(h))
+            if (c%cosmo_dirty) then
+                call cosmology_table(c%h0, c%om0, c%ol0, c%cz, c%cdl)
+                c%cosmo_dirty = .false.
+                c%dev_cosmo = .false.
+            end if
+        end associate
+    end subroutine ctx_refresh_cosmology
+
+    function hip_message(dctx) result(msg)
+        type(c_ptr), intent(in) :: dctx
+        character(len=512) :: msg
+        character(kind=c_char) :: buf(512)
+        integer(c_int) :: st
+        integer :: i
+        st = opensps_hip_last_error(dctx, buf, 512_c_int)
+        msg = ''
+        do i = 1, 512
+            if (buf(i) == char(0)) exit
+            msg(i:i) = buf(i)
+        end do
+    end function hip_message
+
+    !> The device side of a context, created and brought up to date: the grid, the bands and their mask,
+    !> the cosmology table and the options. status 0, 5 without a HIP device, 10 on a device error.
+    integer function ctx_device(h) result(st)
+        integer, intent(in) :: h
+        integer(c_int) :: ndev, rc
+        integer, allocatable, target :: mask(:)
+        st = 0
+        associate(c => ctxs(h))
+        if (.not. c_associated(c%dctx)) then
+            rc = opensps_hip_device_count(ndev)
+            if (ndev < 1) then
+                call set_error(5, 'batch: no HIP device (this build runs the batch on the GPU only)')
+                st = 5
+                return
+            end if
+            rc = opensps_hip_create(0_c_int, c%dctx)
+            if (rc /= 0) then
+                call set_error(10, 'batch: ' // trim(hip_message(c_null_ptr)))
+                st = 10
+                return
+            end if
+            c%dev_grid = .false.; c%dev_bands = .false.; c%dev_cosmo = .false.; c%dev_mask = .false.
+        end if
+        if (.not. c%dev_grid) then
+            rc = opensps_hip_set_grid(c%dctx, int(c%nspec, c_int), c%lam, int(c%nt, c_int), c%log_age, &
+                                      int(c%nz, c_int), c%logz, c%spec, c%mass, c%lbol)
+            if (rc == 0) rc = opensps_hip_set_options(c%dctx, int(c%add_igm_absorption, c_int), &
+                                                      int(merge(1, 0, c%smooth_velocity == 1 .and. &
+                                                                c%smoothspec_fast == 1), c_int))
+            if (rc /= 0) goto 900
+            c%dev_grid = .true.
+            c%dev_bands = .false.
+        end if
+        if (.not. c%dev_bands) then
+            if (c%nbands > 0) then
+                rc = opensps_hip_set_bands(c%dctx, int(c%nbands, c_int), c%bands)
+            else
+                rc = opensps_hip_set_bands(c%dctx, 0_c_int, [0.0_wp])
+            end if
+            if (rc /= 0) goto 900
+            c%dev_bands = .true.
+            c%dev_mask = .false.
+        end if
+        if (.not. c%dev_mask .and. c%nbands > 0) then
+            allocate(mask(c%nbands))
+            mask = 1
+            if (allocated(c%mag_mask)) mask = c%mag_mask(1:c%nbands)
+            rc = opensps_hip_set_mag_mask(c%dctx, int(c%nbands, c_int), mask)
+            if (rc /= 0) goto 900
+            c%dev_mask = .true.
+        end if
+        call ctx_refresh_cosmology(h)
+        if (.not. c%dev_cosmo) then
+            rc = opensps_hip_set_cosmology(c%dctx, 500_c_int, c%cz, c%cdl)
+            if (rc /= 0) goto 900
+            c%dev_cosmo = .true.
+        end if
+        return
+900     continue
+        st = merge(10, int(rc), rc == 10)
+        call set_error(st, 'device: ' // trim(hip_message(c%dctx)))
+        end associate
+    end function ctx_device
+
+end module opensps_host_context

@@ -1,0 +1,83 @@
+!> Constants, parameter slots and the error state of the Fortran host.
+!>
+!> The per-model parameter slots are those of include/opensps_params.h (the members of the reference's
+!> PARAMS_CORE that the batch reads, src/core/opensps_types.f90:33-45), 1-based here: slot(name) = the
+!> header's OPS_P_<NAME> + 1.
+module opensps_host_types
+    use, intrinsic :: iso_c_binding, only: c_double, c_int
+    implicit none
+    private
+
+    integer, parameter, public :: WP = c_double
+    real(WP), parameter, public :: SAFE_FLOOR = tiny(1.0_wp)
+
+    integer, parameter, public :: NPAR = 28
+    character(len=16), parameter, public :: PARAM_NAMES(NPAR) = [character(len=16) :: &
+        'tage', 'tau', 'const', 'fburst', 'tburst', 'sf_start', 'sf_trunc', 'sf_slope', 'sfh', 'logzsol', &
+        'dust1', 'dust2', 'dust3', 'dust_index', 'dust1_index', 'dust_tesc', 'frac_obrun', 'frac_nodust', &
+        'zred', 'igm_factor', 'sigma_smooth', 'min_wave_smooth', 'max_wave_smooth', 'imf1', 'imf2', 'imf3', &
+        'imf_upper_limit', 'imf_lower_limit']
+    real(WP), parameter, public :: PARAM_DEFAULTS(NPAR) = [ &
+        0.0_wp, 1.0_wp, 0.0_wp, 0.0_wp, 11.0_wp, 0.0_wp, 0.0_wp, 0.0_wp, 0.0_wp, 0.0_wp, &
+        0.0_wp, 0.0_wp, 0.0_wp, -0.7_wp, -1.0_wp, 7.0_wp, 0.0_wp, 0.0_wp, &
+        0.0_wp, 1.0_wp, 0.0_wp, 1.0e3_wp, 1.0e4_wp, 1.3_wp, 2.3_wp, 2.3_wp, &
+        120.0_wp, 0.08_wp]
+    integer, parameter, public :: P_TAGE = 1, P_SFH = 9, P_LOGZSOL = 10, P_DUST1 = 11, P_DUST2 = 12, P_DUST3 = 13, &
+                                  P_FRAC_OBRUN = 17, P_ZRED = 19, P_IGM_FACTOR = 20, P_SIGMA_SMOOTH = 21, &
+                                  P_TAU = 2, P_CONST = 3, P_FBURST = 4, P_TBURST = 5, P_SF_START = 6, P_SF_TRUNC = 7, &
+                                  P_IMF1 = 24
+
+    !> Parameters the batch refuses to vary per model (SSP-sensitive: they change the resident basis;
+    !> the reference's batch_handle returns status 2 for them, src/bindings/c/opensps_c_api.f90:761-788)
+    character(len=16), parameter, public :: SSP_NAMES(14) = [character(len=16) :: &
+        'imf_type', 'zmet', 'pagb', 'dell', 'delt', 'fbhb', 'sbss', 'redgb', 'agb', 'masscut', 'vdmc', 'mdave', &
+        'evtype', 'fcstar']
+
+    !> Rows of params_bar (include/opensps.h:12-59), as opensps_get_nbgrad reports them
+    integer, parameter, public :: NBGRAD = 44
+    character(len=16), parameter, public :: GRAD_NAMES(NBGRAD) = [character(len=16) :: &
+        'fagn', 'dust1', 'dust2', 'dust3', 'dust_index', 'dust1_index', 'frac_obrun', 'frac_nodust', 'mwr', 'uvb', &
+        'duste_qpah', 'duste_umin', 'duste_gamma', 'gas_logz', 'gas_logu', 'logzsol', 'tage', 'tau', 'const', &
+        'fburst', 'tburst', 'sf_start', 'zred', 'sigma_smooth', 'sf_trunc', 'sf_slope', 'imf1', 'imf2', 'imf3', &
+        'vdmc', 'mdave', 'agn_tau', 'igm_factor', 'gas_logn', 'gas_logno', 'gas_logco', 'afe', 'fbhb', 'sbss', &
+        'agb', 'pagb', 'redgb', 'delt', 'dell']
+
+    character(len=512), public :: last_error = ''
+    integer, public :: last_status = 0
+
+    public :: param_slot, set_error, lower
+
+contains
+
+    pure function lower(s) result(r)
+        character(len=*), intent(in) :: s
+        character(len=len(s)) :: r
+        integer :: i, c
+        r = s
+        do i = 1, len(s)
+            c = iachar(s(i:i))
+            if (c >= 65 .and. c <= 90) r(i:i) = achar(c + 32)
+        end do
+    end function lower
+
+    !> The slot of a parameter name (1-based), 0 if it is not a per-model parameter of the batch
+    pure integer function param_slot(name) result(k)
+        character(len=*), intent(in) :: name
+        integer :: i
+        k = 0
+        do i = 1, NPAR
+            if (trim(PARAM_NAMES(i)) == lower(trim(name))) then
+                k = i
+                return
+            end if
+        end do
+    end function param_slot
+
+    subroutine set_error(status, msg)
+        integer, intent(in) :: status
+        character(len=*), intent(in) :: msg
+        last_status = status
+        last_error = msg
+    end subroutine set_error
+
+end module opensps_host_types

@@ -1,0 +1,158 @@
+"""The C-ABI libraries without a GPU: every declared symbol is exported, and the host logic of the public API
+(handles, setters, the basis, filters and the statuses a batch returns before it reaches the device) behaves as
+the reference's C API does (tests/test_c_batch.c:137-146, tests/test_c_ssp_operator.c:45-56)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import grid
+from libopensps_amd import api, build, synth
+from libopensps_amd.params import NPAR, PARAM_DEFAULTS, PARAM_NAMES
+
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    txt = re.sub(r"//.*", "", txt)
+    return sorted(set(re.findall(r"\b(opensps_\w+)\s*\(", txt)))
+
+
+def _exported(so):
+    out = subprocess.run(["nm", "-D", "--defined-only", so], stdout=subprocess.PIPE, text=True, check=True).stdout
+    return {l.split()[-1] for l in out.splitlines() if " T " in l}
+
+
+@pytest.mark.parametrize("header,so", [("opensps.h", build.API_SO), ("opensps_hip.h", build.HIP_SO)])
+def test_every_declared_symbol_is_exported(header, so):
+    assert os.path.exists(so), f"{so} not built"
+    C.CDLL(so)   # loads without a GPU
+    missing = [s for s in _declared(header) if s not in _exported(so)]
+    assert not missing, missing
+
+
+def test_param_slots_match_header():
+    """The Fortran host's slot table equals include/opensps_params.h (names and defaults, via get_params)."""
+    with api.Context() as ctx:
+        np.testing.assert_array_equal(ctx.params(), np.asarray(PARAM_DEFAULTS))
+    src = open(os.path.join(ROOT, "libopensps_amd", "fortran", "opensps_host_types.f90")).read()
+    names = re.search(r"PARAM_NAMES\(NPAR\) = \[character\(len=16\) :: &(.*?)\]", src, re.S).group(1)
+    assert re.findall(r"'(\w+)'", names) == PARAM_NAMES
+    assert NPAR == 28
+
+
+def test_handles_and_setters():
+    with api.Context() as ctx:
+        ctx.set_int("add_igm_absorption", 1)
+        ctx.set_float("dust2", 0.3)
+        ctx.set_str("kernel_precision", "mixed")
+        with pytest.raises(api.OpenSPSError) as e:
+            ctx.set_int("no_such_key", 1)
+        assert e.value.status == 101
+        with pytest.raises(api.OpenSPSError) as e:
+            ctx.set_float("no_such_key", 1.0)
+        assert e.value.status == 102
+        with pytest.raises(api.OpenSPSError) as e:
+            ctx.set_str("kernel_precision", "quad")
+        assert e.value.status == 103
+        assert ctx.params()[PARAM_NAMES.index("dust2")] == 0.3
+    st = C.c_int()
+    api.lib().opensps_context_destroy(98765, C.byref(st))
+    assert st.value == 1
+    assert api.nbgrad() == 44
+    assert api.grad_param_name(0) == "fagn" and api.grad_param_name(22) == "zred"
+
+
+def test_setup_from_data_release_is_refused():
+    with api.Context() as ctx:
+        st = C.c_int()
+        api.lib().opensps_context_setup(0, b"mist", b"miles", b"dl07", ctx.handle, C.byref(st))
+        assert st.value == 301
+
+
+def test_basis_and_filters_match_oracle():
+    """The Fortran host's filters_build equals the oracle's (point and hat resampling), and dims / grids are kept."""
+    g = grid(22)
+    fs = synth.gaussian_filters(30, seed=7)
+    for resample in (0, 1):
+        with api.Context() as ctx:
+            ctx.set_ssp_basis(g)
+            ctx.set_filters(fs, resample=resample)
+            b = ctx.bands()
+            ref = O.build_bands(fs, g.lam, resample)
+            np.testing.assert_allclose(b, ref, rtol=1e-13, atol=0)
+            assert ctx.nbands == 30
+            we = ctx.wave_eff()
+            cen = np.sort(np.array([0.5 * (l[0] + l[-1]) for l in fs.lam]))
+            assert np.all(np.abs(we / cen - 1.0) < 0.01)
+
+
+def test_bad_basis_and_filters():
+    g = grid(1)
+    with api.Context() as ctx:
+        bad = synth.SSPGrid(lam=g.lam[::-1].copy(), log_age=g.log_age, logz=g.logz, spec=g.spec, mass=g.mass,
+                            lbol=g.lbol)
+        with pytest.raises(api.OpenSPSError) as e:
+            ctx.set_ssp_basis(bad)
+        assert e.value.status == 3
+        fs = synth.gaussian_filters(2)
+        with pytest.raises(api.OpenSPSError) as e:
+            ctx.set_filters(fs)
+        assert e.value.status == 301
+        ctx.set_ssp_basis(g)
+        far = synth.FilterSet(lam=[np.linspace(2e8, 3e8, 50)], resp=[np.ones(50)], names=["far"])
+        with pytest.raises(api.OpenSPSError) as e:
+            ctx.set_filters(far, grid_policy=0)
+        assert e.value.status in (321, 322)
+        nanf = synth.FilterSet(lam=[np.array([1000.0, np.nan, 2000.0])], resp=[np.ones(3)], names=["nan"])
+        with pytest.raises(api.OpenSPSError) as e:
+            ctx.set_filters(nanf)
+        assert e.value.status == 320
+
+
+def test_batch_statuses_before_the_device():
+    """Statuses the host returns before any device work: unknown name 3, an SSP parameter 2, no basis 1,
+    spectral indices 2, dust emission with dust 2, n = 0 is a no-op."""
+    g = grid(1)
+    fs = synth.gaussian_filters(4)
+    with api.Context() as ctx:
+        r = ctx.compute_csp_batch({"tage": np.ones(3)}, raw_status=True)
+        assert r["status"] == 1
+        ctx.set_ssp_basis(g)
+        ctx.set_filters(fs)
+        r = ctx.compute_csp_batch({"no_such_param": np.ones(3)}, raw_status=True)
+        assert r["status"] == 3
+        r = ctx.compute_csp_batch({"imf_type": np.ones(3)}, raw_status=True)
+        assert r["status"] == 2
+        r = ctx.compute_csp_batch({"tage": np.ones(3), "dust2": np.ones(3)}, raw_status=True)
+        assert r["status"] == 2      # add_dust_emission = 1 by default (reference src/core/opensps_context.f90:78)
+        assert "dust emission" in api.last_error()
+        ctx.set_int("add_dust_emission", 0)
+        r = ctx.compute_csp_batch({}, n=0, raw_status=True)
+        assert r["status"] == 0
+        st = C.c_int()
+        mags = np.zeros((2, 4))
+        indx = np.zeros(10)
+        api.lib().opensps_context_compute_csp_batch(ctx.handle, 2, 0, None, None, None, mags.ctypes.data, None, None,
+                                                    None, None, None, None, indx.ctypes.data, C.byref(st))
+        assert st.value == 2
+
+
+def test_batch_without_device_is_status_5():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    g = grid(1)
+    with api.Context() as ctx:
+        ctx.set_ssp_basis(g)
+        ctx.set_filters(synth.gaussian_filters(4))
+        ctx.set_int("add_dust_emission", 0)
+        r = ctx.compute_csp_batch({"tage": np.full(2, 5.0), "sfh": np.full(2, 4.0)}, raw_status=True)
+        assert r["status"] == 5
