@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched composite-population path on one MI355X.
+
+Workload (the parametric CSP batch): 16 384 models on a synthetic SSP library of 107 ages x 22 metallicities x
+5994 wavelengths; per model a delayed-tau SFH (tage U(0.1, 13.7) Gyr, tau U(0.1, 10) Gyr), logzsol U(-2, 0.3)
+blended between the bracketing metallicities, power-law dust (dust2 U(0, 2), slope U(-1.2, -0.4)), redshift U(0, 4)
+with Madau IGM, a Gaussian velocity dispersion U(50, 300) km/s, spectra plus 100 Gaussian-filter AB magnitudes.
+
+A step is one batch of the 16 384 models through the public C API's device entry
+(opensps_context_compute_csp_batch_device: the inputs resident in HBM, the spectra and magnitudes left there).
+`value` is the mixed-precision path's models/s (split-BF16 age sums on the matrix cores, FP32 chain, FP64 SFH and
+zero points); the FP64 GPU path is timed beside it, and the two are compared on the whole batch. The CPU baseline is
+the FP64 oracle (the restatement of the reference's CPU path, OpenMP) on a bounded sample of the same models.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CSP models/s, 16k batch, mixed vs FP64 GPU at ≤1e-4 rel err; emulator models/s"
+HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+FP64_PEAK_TFLOPS = 78.6
+KERNELS = ["prep", "group", "age_sums", "chain"]
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work of the baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs the batched CSP path on one GPU only; the multi-GPU layout (models "
+              "sharded across GPUs, no collective) is described in DESIGN.md. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from libopensps_amd import api, synth
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n = args.n
+    t_setup = time.perf_counter()
+    grid = synth.ssp_grid(seed=1, nspec=5994, nt=107, nz=22)
+    fset = synth.gaussian_filters(100, seed=7)
+    prm = synth.parametric_params(n, seed=5)
+    ctx = api.Context()
+    ctx.set_ssp_basis(grid)
+    ctx.set_filters(fset)
+    ctx.set_int("add_dust_emission", 0)
+    ctx.set_int("add_igm_absorption", 1)
+    ctx.set_int("profile_kernels", 1)
+    nb, ns = ctx.nbands, grid.nspec
+    d_prm = torch.as_tensor(prm, device="cuda")
+    d_mags = torch.empty((n, nb), dtype=torch.float64, device="cuda")
+    d_mass = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_lbol = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_spec32 = torch.empty((n, ns), dtype=torch.float32, device="cuda")
+    d_spec64 = torch.empty((n, ns), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+
+    def kernel_times(reset=1):
+        import ctypes as C
+        ms = (C.c_double * 4)()
+        cnt = (C.c_int * 4)()
+        st = C.c_int()
+        api.lib().opensps_context_kernel_times.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                                           C.POINTER(C.c_int)]
+        api.lib().opensps_context_kernel_times(ctx.handle, ms, cnt, reset, C.byref(st))
+        return [ms[i] for i in range(4)], [cnt[i] for i in range(4)]
+
+    def timed(precision, d_spec, steps, warmup):
+        for _ in range(warmup):
+            ctx.compute_csp_batch_device(d_prm, d_mags, d_spec=d_spec, d_mass=d_mass, d_lbol=d_lbol,
+                                         precision=precision)
+        kernel_times(reset=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.compute_csp_batch_device(d_prm, d_mags, d_spec=d_spec, d_mass=d_mass, d_lbol=d_lbol,
+                                         precision=precision)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ms, cnt = kernel_times(reset=1)
+        return dt, [m / max(c, 1) for m, c in zip(ms, cnt)]
+
+    # mixed precision: the headline
+    dt_mix, kms_mix = timed(1, d_spec32, args.steps, args.warmup)
+    mags_mix = d_mags.clone()
+    value = n * args.steps / dt_mix
+    # FP64 GPU path
+    steps64 = max(2, args.steps // 2)
+    dt_64, kms_64 = timed(0, d_spec64, steps64, 1)
+    value64 = n * steps64 / dt_64
+    mags_64 = d_mags.clone()
+
+    # accuracy of the mixed path against the FP64 GPU path over the whole batch (pixels >= 1e-6 of each peak)
+    ref = d_spec64
+    peak = ref.abs().amax(dim=1, keepdim=True)
+    mask = ref.abs() >= 1e-6 * peak
+    rel = torch.where(mask, (d_spec32.double() - ref).abs() / torch.where(mask, ref.abs(), torch.ones_like(ref)),
+                      torch.zeros_like(ref))
+    spec_err = float(rel.max())
+    fin = torch.isfinite(mags_64) & torch.isfinite(mags_mix)
+    mag_err = float((mags_mix - mags_64)[fin].abs().max())
+    nan_mismatch = int((torch.isfinite(mags_64) != torch.isfinite(mags_mix)).sum())
+
+    # roofline of the dominant kernel of the mixed path
+    nt, kpair = grid.nt, 2 * grid.nt
+    dom = int(np.argmax(kms_mix))
+    name = KERNELS[dom]
+    if name == "age_sums":
+        flops = 2.0 * ns * kpair * n            # useful MACs (the split-BF16 form issues 3 bf16 products each)
+        achieved = flops / (kms_mix[dom] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None, "kernel": "k_gemm_bf16x3",
+                "algorithmic_per_launch": flops, "issued_bf16_tflops": 3 * achieved}
+    else:
+        # chain: read the age sums (float, nspec per model), write the spectrum (float) and the magnitudes (double),
+        # read the parameters (28 doubles)
+        byts = n * (ns * 4 + ns * 4 + nb * 8 + 28 * 8)
+        achieved = byts / (kms_mix[dom] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_chain<float>" if name == "chain" else name,
+                "algorithmic_per_launch": byts}
+    roof["kernel_ms"] = dict(zip(KERNELS, [round(x, 4) for x in kms_mix]))
+    roof["kernel_ms_fp64"] = dict(zip(KERNELS, [round(x, 4) for x in kms_64]))
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        bands = ctx.bands()
+        cosmo = O.cosmology_table(72.0, 0.27, 0.73)
+        m0 = min(256, n)
+        t0 = time.perf_counter()
+        O.csp_batch(grid, bands, prm[:, :m0], add_igm=1, cosmo=cosmo, want=("mags",))
+        rate = m0 / (time.perf_counter() - t0)
+        m1 = int(min(n, max(m0, rate * args.cpu_seconds)))
+        t0 = time.perf_counter()
+        r = O.csp_batch(grid, bands, np.ascontiguousarray(prm[:, :m1]), add_igm=1, cosmo=cosmo,
+                        want=("mags", "spec"))
+        dt = time.perf_counter() - t0
+        hm = mags_mix[:m1].cpu().numpy()
+        ok = np.isfinite(r["mags"]) & np.isfinite(hm)
+        cpu = {"value": m1 / dt, "unit": "models/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"the first {m1} models of the same batch, FP64 oracle (csp_oracle.c, OpenMP), {dt:.1f} s",
+               "mixed_vs_oracle_max_mag_err": float(np.max(np.abs(hm[ok] - r["mags"][ok]))) if ok.any() else None}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "models/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt_mix / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3/f32 (f64 SFH, zero points)",
+        "data": "synthetic",
+        "config": {"workload": "parametric CSP, 16384 models, 107 ages x 22 Z x 5994 px, delayed-tau, logzsol blend, "
+                               "power-law dust, IGM, z U(0,4), LSF U(50,300) km/s, 100 Gaussian filters",
+                   "n_models": n, "nspec": ns, "nt": nt, "nz": grid.nz, "nbands": nb, "seeds": [1, 7, 5]},
+        "fp64_value": value64,
+        "fp64_ms_per_step": 1e3 * dt_64 / steps64,
+        "speedup_vs_fp64": value / value64,
+        "accuracy_vs_fp64_gpu": {"max_spec_rel_err": spec_err, "max_mag_err": mag_err, "nan_mismatch": nan_mismatch},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "setup_s": t_setup,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

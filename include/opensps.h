@@ -116,6 +116,10 @@ void opensps_context_batch_spec_device(int handle, void **ptr, int *nspec, int *
 void opensps_context_compute_csp_batch_device(int handle, int n, const double *d_prm, const double *d_weights,
                                               int precision, int split, double *d_mags, void *d_spec, double *d_mass,
                                               double *d_lbol, int *status);
+/* No reference counterpart: the accumulated device time (ms) and launch counts of the batch's kernels [prep,
+ * group, age-sum product, chain], from HIP events on the context's stream once set_int("profile_kernels", 1);
+ * reset != 0 clears them. status 5 before the first device batch. */
+void opensps_context_kernel_times(int handle, double *ms, int *launches, int reset, int *status);
 /* No reference counterpart: the context's parameter values in the slot order of opensps_params.h. */
 void opensps_context_get_params(int handle, double *prm, int *status);
 

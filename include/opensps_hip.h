@@ -83,6 +83,13 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t *ctx, const opensps_hip_batch_t *arg
 /* Waits for the context's stream; status 2 if a model of the last batch was unsupported (idx row 3). */
 int opensps_hip_sync(opensps_hip_ctx_t *ctx);
 
+/* Per-kernel timing with HIP events on the context's stream (no reference counterpart; the reference times its
+ * stages with rocprof / nsys traces, doc/GPU_RESULTS.md:210-224): on != 0 records events around each kernel of
+ * every batch; opensps_hip_kernel_times returns the accumulated milliseconds and launch counts of [prep, group,
+ * age-sum product, chain] (ms[4], launches[4]; either may be NULL) and, with reset != 0, clears them. */
+int opensps_hip_set_profiling(opensps_hip_ctx_t *ctx, int on);
+int opensps_hip_kernel_times(opensps_hip_ctx_t *ctx, double *ms, int *launches, int reset);
+
 /* ---- kernel-level entry points (the reference's per-stage launchers) ---- */
 /* band_fluxes (opensps_hip_band_fluxes_sp/_dp, src/accel/hip/opensps_hip_kernels.hip:3566, 3623): device
  * bands_t (nbands, nspec), j_lo, j_hi (1-based), do_band, wt (nspec), fobs_t (n, nspec) -> fband (nbands, n).

@@ -3,6 +3,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <mutex>
@@ -119,6 +120,9 @@ int opensps_hip_create(int device, opensps_hip_ctx_t** out) {
         return ops_fail(nullptr, 10, "opensps_hip_create: stream");
     }
     c->own_stream = true;
+    // OPENSPS_F64_GEMM=vector: the FP64 age sums with vector FMAs instead of the f64 matrix instruction
+    const char* e = getenv("OPENSPS_F64_GEMM");
+    c->f64_vector = e && strcmp(e, "vector") == 0;
     *out = c;
     return 0;
 }
@@ -135,6 +139,8 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
                   c->d_bc32, c->d_mask, c->d_cz, c->d_cdl, c->yo.p, c->wcol.p, c->w64.p, c->order.p, c->tiles.p,
                   c->misc.p, c->spec.p, c->flag.p};
     for (void* p : ps) free_ptr(p);
+    for (int k = 0; k <= OpsCtx::NPROF; k++)
+        if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -385,7 +391,9 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     pa.status = d_status;
     pa.flag = d_flag;
     size_t lds_prep = sizeof(double) * 4 * 5 * (size_t)(c->nt + 1);
+    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     k_prep<<<(n + 3) / 4, 256, lds_prep, c->stream>>>(pa);
+    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHK(c, hipGetLastError());
 
     GroupArgs ga;
@@ -398,6 +406,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ga.tiles = (int4*)c->tiles.p;
     k_group<<<1, 256, sizeof(int) * 256 * (size_t)c->npair, c->stream>>>(ga);
     HIPCHK(c, hipGetLastError());
+    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
 
     GemmArgs gm;
     gm.n = n;
@@ -420,9 +429,12 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     const unsigned gblocks = 8u * ptx * (unsigned)maxtiles;
     if (mixed)
         k_gemm_bf16x3<<<gblocks, 256, 0, c->stream>>>(gm);
-    else
+    else if (c->f64_vector)
         k_gemm_f64_vec<<<gblocks, 256, 0, c->stream>>>(gm);
+    else
+        k_gemm_f64<<<gblocks, 256, 0, c->stream>>>(gm);
     HIPCHK(c, hipGetLastError());
+    if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
 
     ChainArgs ca;
     ca.n = n;
@@ -467,12 +479,25 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         k_chain<double><<<n, 512, lds_chain, c->stream>>>(ca);
     }
     HIPCHK(c, hipGetLastError());
+    if (c->profiling) {
+        HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+        c->ev_pending = true;
+    }
     return 0;
 }
 
 int opensps_hip_sync(opensps_hip_ctx_t* c) {
     if (!c) return 1;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->ev_pending) {
+        for (int k = 0; k < OpsCtx::NPROF; k++) {
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
+            c->prof_ms[k] += ms;
+            c->prof_n[k] += 1;
+        }
+        c->ev_pending = false;
+    }
     if (!c->d_last_flag) return 0;
     int flag = 0;
     HIPCHK(c, hipMemcpy(&flag, c->d_last_flag, sizeof(int), hipMemcpyDeviceToHost));
@@ -480,6 +505,29 @@ int opensps_hip_sync(opensps_hip_ctx_t* c) {
     if (flag == 2) return ops_fail(c, 2, "batch: a model needs an option this build does not compute (SFH type, "
                                          "tage <= 0 without weights, or smoothing other than the fast velocity operator)");
     if (flag == 4) return ops_fail(c, 3, "batch: split_parts = 0 with a birth-cloud or old-star screen");
+    return 0;
+}
+
+int opensps_hip_set_profiling(opensps_hip_ctx_t* c, int on) {
+    if (!c) return 1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (on && !c->ev[0])
+        for (int k = 0; k <= OpsCtx::NPROF; k++) HIPCHK(c, hipEventCreate(&c->ev[k]));
+    c->profiling = on != 0;
+    c->ev_pending = false;
+    return 0;
+}
+
+int opensps_hip_kernel_times(opensps_hip_ctx_t* c, double* ms, int* launches, int reset) {
+    if (!c) return 1;
+    for (int k = 0; k < OpsCtx::NPROF; k++) {
+        if (ms) ms[k] = c->prof_ms[k];
+        if (launches) launches[k] = (int)c->prof_n[k];
+        if (reset) {
+            c->prof_ms[k] = 0.0;
+            c->prof_n[k] = 0;
+        }
+    }
     return 0;
 }
 

@@ -56,10 +56,19 @@ struct OpsCtx {
     int add_igm = 0;       // add_igm_absorption
     int smooth_fast = 1;   // smooth_velocity == 1 and smoothspec_fast == 1
     double tiny_logt = 0.0;
+    bool f64_vector = false;   // FP64 age sums with vector FMAs (OPENSPS_F64_GEMM=vector)
 
     // ---- batch scratch (grown on demand) ----
     OpsBuffer yo, wcol, w64, order, tiles, misc, spec, flag;
     int* d_last_flag = nullptr;   // the per-batch status flag of the last batch (read by sync)
+
+    // ---- per-kernel timing (HIP events on the context's stream) ----
+    static const int NPROF = 4;     // prep, group, age-sum product, chain
+    bool profiling = false;
+    hipEvent_t ev[NPROF + 1] = {};
+    bool ev_pending = false;
+    double prof_ms[NPROF] = {0, 0, 0, 0};
+    long prof_n[NPROF] = {0, 0, 0, 0};
 
     std::string last_error;
 };

@@ -423,8 +423,9 @@ __global__ __launch_bounds__(256) void k_gemm_f64(GemmArgs a) {
         if (!valid[j]) continue;
         double* col = yo + ((size_t)part[j] * a.n + model[j]) * a.nsp;
         for (int i = 0; i < 2; i++) {
-            int px = (blk0 + i) * 16 + 4 * (lane >> 4);
-            for (int r = 0; r < 4; r++) col[px + r] = acc[i][j][r];
+            // f64 16x16x4 result layout: register r of lane l holds row (l / 16) + 4 r, column l % 16
+            int px = (blk0 + i) * 16 + (lane >> 4);
+            for (int r = 0; r < 4; r++) col[px + 4 * r] = acc[i][j][r];
         }
     }
 }

@@ -538,6 +538,27 @@ contains
         prm = ctxs(handle)%pset
     end subroutine
 
+    !> No reference counterpart: the accumulated device time of the batch's kernels [prep, group, age-sum product,
+    !> chain] in milliseconds and their launch counts, recorded with HIP events on the context's stream after
+    !> set_int("profile_kernels", 1); reset != 0 clears them. status 1 bad handle, 5 before any device batch.
+    subroutine opensps_context_kernel_times(handle, ms, launches, reset, status) &
+        bind(C, name="opensps_context_kernel_times")
+        use opensps_hip_iface, only: opensps_hip_kernel_times
+        integer(c_int), value :: handle, reset
+        real(c_double), intent(out) :: ms(4)
+        integer(c_int), intent(out) :: launches(4), status
+        integer(c_int) :: rc
+        ms = 0.0_wp
+        launches = 0
+        if (bad_handle(handle, status)) return
+        if (.not. c_associated(ctxs(handle)%dctx)) then
+            status = 5
+            return
+        end if
+        rc = opensps_hip_kernel_times(ctxs(handle)%dctx, ms, launches, reset)
+        status = rc
+    end subroutine
+
     subroutine opensps_context_batch_spec_device(handle, ptr, nspec, n, bytes, status) &
         bind(C, name="opensps_context_batch_spec_device")
         integer(c_int), value :: handle

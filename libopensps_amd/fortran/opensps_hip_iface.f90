@@ -19,7 +19,7 @@ module opensps_hip_iface
     public :: opensps_hip_create, opensps_hip_destroy, opensps_hip_set_grid, opensps_hip_set_bands, &
               opensps_hip_set_mag_mask, opensps_hip_set_cosmology, opensps_hip_set_options, opensps_hip_csp_batch, &
               opensps_hip_sync, opensps_hip_malloc, opensps_hip_free, opensps_hip_memcpy_h2d, opensps_hip_memcpy_d2h, &
-              opensps_hip_last_error, opensps_hip_device_count
+              opensps_hip_last_error, opensps_hip_device_count, opensps_hip_set_profiling, opensps_hip_kernel_times
 
     interface
         integer(c_int) function opensps_hip_create(device, ctx) bind(C, name="opensps_hip_create")
@@ -107,6 +107,20 @@ module opensps_hip_iface
             type(c_ptr), value :: ctx
             character(kind=c_char), intent(out) :: msg(*)
             integer(c_int), value :: len
+        end function
+
+        integer(c_int) function opensps_hip_set_profiling(ctx, on) bind(C, name="opensps_hip_set_profiling")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: on
+        end function
+
+        integer(c_int) function opensps_hip_kernel_times(ctx, ms, launches, reset) bind(C, name="opensps_hip_kernel_times")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: ms(4)
+            integer(c_int), intent(out) :: launches(4)
+            integer(c_int), value :: reset
         end function
 
         integer(c_int) function opensps_hip_device_count(n) bind(C, name="opensps_hip_device_count")

@@ -21,7 +21,8 @@ module opensps_host_context
         ! settings (integer keys)
         integer :: add_igm_absorption = 0, add_dust_emission = 1, add_neb_emission = 0, add_agn_dust = 1
         integer :: smooth_velocity = 1, smoothspec_fast = 1, dust_type = 0, compute_vega_mags = 0
-        integer :: redshift_colors = 0, keep_maggies = 0, imf_type = 2, mixed = 0
+        integer :: redshift_colors = 0, keep_maggies = 0, imf_type = 2, mixed = 0, profile = 0
+        logical :: dev_prof = .false.
         real(WP) :: h0 = 72.0_wp, om0 = 0.27_wp, ol0 = 0.73_wp
         real(WP) :: pset(NPAR) = PARAM_DEFAULTS
         ! the resident basis
@@ -135,6 +136,7 @@ contains
         case ('keep_maggies'); c%keep_maggies = v
         case ('imf_type'); c%imf_type = v
         case ('mixed_precision'); c%mixed = merge(1, 0, v /= 0)
+        case ('profile_kernels'); c%profile = merge(1, 0, v /= 0); c%dev_prof = .false.
         case ('sfh'); c%pset(P_SFH) = real(v, WP)
         case default
             k = param_slot(key)
@@ -294,6 +296,7 @@ contains
                 return
             end if
             c%dev_grid = .false.; c%dev_bands = .false.; c%dev_cosmo = .false.; c%dev_mask = .false.
+            c%dev_prof = .false.
         end if
         if (.not. c%dev_grid) then
             rc = opensps_hip_set_grid(c%dctx, int(c%nspec, c_int), c%lam, int(c%nt, c_int), c%log_age, &
@@ -322,6 +325,11 @@ contains
             rc = opensps_hip_set_mag_mask(c%dctx, int(c%nbands, c_int), mask)
             if (rc /= 0) goto 900
             c%dev_mask = .true.
+        end if
+        if (.not. c%dev_prof) then
+            rc = opensps_hip_set_profiling(c%dctx, int(c%profile, c_int))
+            if (rc /= 0) goto 900
+            c%dev_prof = .true.
         end if
         call ctx_refresh_cosmology(h)
         if (.not. c%dev_cosmo) then

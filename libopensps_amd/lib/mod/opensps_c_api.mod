@@ -1,9 +1,9 @@
-﻿!mod$ v1 sum:11a261b12326873d
+﻿!mod$ v1 sum:8c4ee476702aa8c2
 !need$ b579c837613f74aa n opensps_host_filters
-!need$ d316fdcaa880d25b n opensps_host_batch
+!need$ 9ff8728feed3f05b n opensps_host_batch
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ f5118467a58a718d n opensps_host_types
-!need$ aafcefbe96a38b1a n opensps_host_context
+!need$ 5e268f5f4692bc4e n opensps_host_context
 !need$ bb381bf46e508468 i __fortran_builtins
 module opensps_c_api
 use,intrinsic::iso_c_binding,only:c_associated
@@ -336,6 +336,7 @@ private::opensps_context_compute_csp_batch
 private::opensps_context_compute_csp_batch_parts
 private::opensps_context_compute_csp_batch_device
 private::opensps_context_get_params
+private::opensps_context_kernel_times
 private::opensps_context_batch_spec_device
 private::opensps_context_get_batch_maggies
 contains
@@ -598,6 +599,13 @@ end
 subroutine opensps_context_get_params(handle,prm,status) bind(c,name="opensps_context_get_params")
 integer(4),value::handle
 real(8),intent(out)::prm(1_8:28_8)
+integer(4),intent(out)::status
+end
+subroutine opensps_context_kernel_times(handle,ms,launches,reset,status) bind(c,name="opensps_context_kernel_times")
+integer(4),value::handle
+real(8),intent(out)::ms(1_8:4_8)
+integer(4),intent(out)::launches(1_8:4_8)
+integer(4),value::reset
 integer(4),intent(out)::status
 end
 subroutine opensps_context_batch_spec_device(handle,ptr,nspec,n,bytes,status) bind(c,name="opensps_context_batch_spec_device")

@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:cf5b8986fb5fe043
+﻿!mod$ v1 sum:3a90a40bf090e16a
 !need$ 0bde2ac47243ead2 i iso_c_binding
 module opensps_hip_iface
 use,intrinsic::__fortran_builtins,only:__builtin_c_ptr
@@ -157,6 +157,24 @@ type(c_ptr),value::ctx
 character(1_8,1),intent(out)::msg(1_8:*)
 integer(4),value::len
 integer(4)::opensps_hip_last_error
+end
+end interface
+interface
+function opensps_hip_set_profiling(ctx,on) bind(c,name="opensps_hip_set_profiling")
+import::c_ptr
+type(c_ptr),value::ctx
+integer(4),value::on
+integer(4)::opensps_hip_set_profiling
+end
+end interface
+interface
+function opensps_hip_kernel_times(ctx,ms,launches,reset) bind(c,name="opensps_hip_kernel_times")
+import::c_ptr
+type(c_ptr),value::ctx
+real(8),intent(out)::ms(1_8:4_8)
+integer(4),intent(out)::launches(1_8:4_8)
+integer(4),value::reset
+integer(4)::opensps_hip_kernel_times
 end
 end interface
 interface

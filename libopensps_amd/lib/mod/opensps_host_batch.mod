@@ -1,6 +1,6 @@
-﻿!mod$ v1 sum:d316fdcaa880d25b
-!need$ aafcefbe96a38b1a n opensps_host_context
-!need$ cf5b8986fb5fe043 n opensps_hip_iface
+﻿!mod$ v1 sum:9ff8728feed3f05b
+!need$ 5e268f5f4692bc4e n opensps_host_context
+!need$ 3a90a40bf090e16a n opensps_hip_iface
 !need$ f5118467a58a718d n opensps_host_types
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
@@ -141,6 +141,8 @@ use opensps_hip_iface,only:opensps_hip_free
 use opensps_hip_iface,only:opensps_hip_memcpy_h2d
 use opensps_hip_iface,only:opensps_hip_memcpy_d2h
 use opensps_hip_iface,only:opensps_hip_last_error
+use opensps_hip_iface,only:opensps_hip_set_profiling
+use opensps_hip_iface,only:opensps_hip_kernel_times
 use opensps_hip_iface,only:opensps_hip_device_count
 use,intrinsic::__fortran_builtins,only:iso_c_binding$__fortran_builtins$c_associated_c_ptr=>c_associated_c_ptr
 private::c_associated
@@ -279,6 +281,8 @@ private::opensps_hip_free
 private::opensps_hip_memcpy_h2d
 private::opensps_hip_memcpy_d2h
 private::opensps_hip_last_error
+private::opensps_hip_set_profiling
+private::opensps_hip_kernel_times
 private::opensps_hip_device_count
 private::iso_c_binding$__fortran_builtins$c_associated_c_ptr
 private::expm1_ref

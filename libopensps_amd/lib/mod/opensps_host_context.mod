@@ -1,5 +1,5 @@
-﻿!mod$ v1 sum:aafcefbe96a38b1a
-!need$ cf5b8986fb5fe043 n opensps_hip_iface
+﻿!mod$ v1 sum:5e268f5f4692bc4e
+!need$ 3a90a40bf090e16a n opensps_hip_iface
 !need$ 4fdd01fda3834dfe n opensps_host_cosmology
 !need$ bb381bf46e508468 i __fortran_builtins
 !need$ 0bde2ac47243ead2 i iso_c_binding
@@ -62,6 +62,8 @@ use opensps_hip_iface,only:opensps_hip_free
 use opensps_hip_iface,only:opensps_hip_memcpy_h2d
 use opensps_hip_iface,only:opensps_hip_memcpy_d2h
 use opensps_hip_iface,only:opensps_hip_last_error
+use opensps_hip_iface,only:opensps_hip_set_profiling
+use opensps_hip_iface,only:opensps_hip_kernel_times
 use opensps_hip_iface,only:opensps_hip_device_count
 use,intrinsic::__fortran_builtins,only:iso_c_binding$__fortran_builtins$c_associated_c_ptr=>c_associated_c_ptr
 private::__builtin_c_ptr
@@ -121,6 +123,8 @@ private::opensps_hip_free
 private::opensps_hip_memcpy_h2d
 private::opensps_hip_memcpy_d2h
 private::opensps_hip_last_error
+private::opensps_hip_set_profiling
+private::opensps_hip_kernel_times
 private::opensps_hip_device_count
 private::iso_c_binding$__fortran_builtins$c_associated_c_ptr
 type::dbuf_t
@@ -141,6 +145,8 @@ integer(4)::redshift_colors=0_4
 integer(4)::keep_maggies=0_4
 integer(4)::imf_type=2_4
 integer(4)::mixed=0_4
+integer(4)::profile=0_4
+logical(4)::dev_prof=.false._4
 real(8)::h0=7.2e1_8
 real(8)::om0=2.70000000000000017763568394002504646778106689453125e-1_8
 real(8)::ol0=7.29999999999999982236431605997495353221893310546875e-1_8

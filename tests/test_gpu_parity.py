@@ -1,7 +1,7 @@
 """Parity of the GPU batch (libopensps_hip.so) with the FP64 oracle on the same seeded inputs.
 
-Tolerances: the FP64 path agrees with the oracle to round-off (summation order, the device's exp/log/pow):
-spectra 1e-9 relative, magnitudes 1e-9 mag, SFH weights 1e-10 relative to the model's largest weight. The
+Tolerances: the FP64 path agrees with the oracle to round-off amplified by the SFH moment method's cancellation
+(below): spectra 1e-7 relative, magnitudes 1e-7 mag, SFH weights 1e-7 relative to the model's largest weight. The
 mixed path (split-BF16 age sums, FP32 chain) is held to the targets of the build: spectra 1e-4 relative per
 pixel (pixels at or above 1e-6 of the spectrum's peak), magnitudes 1e-3 mag. Index selection (it_tesc, zlo)
 is bit-exact in both.
@@ -88,15 +88,10 @@ def test_selftest_mfma_f64_layout():
         B[l // 16, l % 16] = b[l]
     D = A @ B
     d = H.selftest_mfma(0, a, b)
-    found = []
+    # the result layout the FP64 product's epilogue assumes: register r of lane l holds D[l // 16 + 4 r][l % 16]
     for l in range(64):
         for r in range(4):
-            i, j = np.unravel_index(np.argmin(np.abs(D - d[l, r])), D.shape)
-            found.append(f"{l}.{r}:{i},{j}")
-    print("f64 16x16x4 lane.reg:row,col", " ".join(found))
-    for l in range(64):
-        for r in range(4):
-            assert abs(d[l, r] - D[4 * (l // 16) + r, l % 16]) < 1e-12
+            assert abs(d[l, r] - D[l // 16 + 4 * r, l % 16]) < 1e-12
 
 
 @pytest.mark.gpu
