@@ -26,6 +26,7 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 FP64_PEAK_TFLOPS = 78.6
 KERNELS = ["prep", "group", "age_sums", "chain"]
+MAG_RANGE = 40.0
 
 
 def parse():
@@ -116,7 +117,14 @@ def main():
                       torch.zeros_like(ref))
     spec_err = float(rel.max())
     fin = torch.isfinite(mags_64) & torch.isfinite(mags_mix)
-    mag_err = float((mags_mix - mags_64)[fin].abs().max())
+    dmag = torch.where(fin, (mags_mix - mags_64).abs(), torch.zeros_like(mags_64))
+    mag_err_all = float(dmag.max())
+    # bands within MAG_RANGE of the model's brightest band (flux ratio 1e-16): fainter bands sit in the FP32
+    # chain's subnormal range (IGM- and dust-extinguished ultraviolet at z ~ 4) and are reported apart
+    bright = torch.where(fin, mags_64, torch.full_like(mags_64, float("inf"))).amin(dim=1, keepdim=True)
+    inrange = fin & (mags_64 - bright <= MAG_RANGE)
+    mag_err = float(torch.where(inrange, dmag, torch.zeros_like(dmag)).max())
+    n_faint = int((fin & ~inrange).sum())
     nan_mismatch = int((torch.isfinite(mags_64) != torch.isfinite(mags_mix)).sum())
 
     # roofline of the dominant kernel of the mixed path
@@ -146,20 +154,28 @@ def main():
         import oracle as O
         bands = ctx.bands()
         cosmo = O.cosmology_table(72.0, 0.27, 0.73)
+        # a bounded sample: the first m1 models of the batch, repeated until about cpu_seconds of CPU work
         m0 = min(256, n)
         t0 = time.perf_counter()
         O.csp_batch(grid, bands, prm[:, :m0], add_igm=1, cosmo=cosmo, want=("mags",))
         rate = m0 / (time.perf_counter() - t0)
         m1 = int(min(n, max(m0, rate * args.cpu_seconds)))
+        reps = int(max(1, min(50, round(rate * args.cpu_seconds / m1))))
+        sub = np.ascontiguousarray(prm[:, :m1])
         t0 = time.perf_counter()
-        r = O.csp_batch(grid, bands, np.ascontiguousarray(prm[:, :m1]), add_igm=1, cosmo=cosmo,
-                        want=("mags", "spec"))
+        for _ in range(reps):
+            r = O.csp_batch(grid, bands, sub, add_igm=1, cosmo=cosmo, want=("mags", "spec"))
         dt = time.perf_counter() - t0
         hm = mags_mix[:m1].cpu().numpy()
+        h64 = mags_64[:m1].cpu().numpy()
         ok = np.isfinite(r["mags"]) & np.isfinite(hm)
-        cpu = {"value": m1 / dt, "unit": "models/s", "cores": O.num_threads(), "kind": "port",
-               "sample": f"the first {m1} models of the same batch, FP64 oracle (csp_oracle.c, OpenMP), {dt:.1f} s",
-               "mixed_vs_oracle_max_mag_err": float(np.max(np.abs(hm[ok] - r["mags"][ok]))) if ok.any() else None}
+        br = np.min(np.where(ok, r["mags"], np.inf), axis=1, keepdims=True)
+        ok_r = ok & (r["mags"] - br <= MAG_RANGE)
+        cpu = {"value": m1 * reps / dt, "unit": "models/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"the first {m1} models of the same batch x {reps} passes, FP64 oracle (csp_oracle.c, "
+                         f"OpenMP, {O.num_threads()} threads), {dt:.1f} s",
+               "mixed_vs_oracle_max_mag_err": float(np.max(np.abs(hm[ok_r] - r["mags"][ok_r]))) if ok_r.any() else None,
+               "fp64_vs_oracle_max_mag_err": float(np.max(np.abs(h64[ok_r] - r["mags"][ok_r]))) if ok_r.any() else None}
 
     out = {
         "metric": METRIC,
@@ -180,7 +196,11 @@ def main():
         "fp64_value": value64,
         "fp64_ms_per_step": 1e3 * dt_64 / steps64,
         "speedup_vs_fp64": value / value64,
-        "accuracy_vs_fp64_gpu": {"max_spec_rel_err": spec_err, "max_mag_err": mag_err, "nan_mismatch": nan_mismatch},
+        "accuracy_vs_fp64_gpu": {"max_spec_rel_err": spec_err, "spec_pixels": "flux >= 1e-6 of the model's peak",
+                                 "max_mag_err": mag_err, "mag_bands": f"within {MAG_RANGE:g} mag of the model's "
+                                                                      f"brightest band",
+                                 "max_mag_err_all_bands": mag_err_all, "bands_fainter": n_faint,
+                                 "nan_mismatch": nan_mismatch},
         "roofline": roof,
         "cpu_baseline": cpu,
         "setup_s": t_setup,

@@ -62,7 +62,8 @@ int opensps_hip_set_stream(opensps_hip_ctx_t *ctx, void *stream);
 /* The resident SSP bases and the grids, host arrays: lam (nspec, Angstrom ascending), log_age (nt,
  * log10 yr ascending), logz (nz, log10 Z/Z_sun ascending), spec (nspec, nt, nz) L_sun/Hz, mass (nt,
  * nz) surviving mass, lbol (nt, nz) log10 L_bol. Replaces opensps_context_update_ssp_basis
- * (src/core/opensps_context.f90:757) plus the device residency of the basis (:1089). */
+ * (src/core/opensps_context.f90:757) plus the device residency of the basis (:1089). Drops the bands, which
+ * index the grid: call opensps_hip_set_bands after it. */
 int opensps_hip_set_grid(opensps_hip_ctx_t *ctx, int nspec, const double *lam, int nt, const double *log_age, int nz,
                          const double *logz, const double *spec, const double *mass, const double *lbol);
 

@@ -83,6 +83,21 @@ def _d(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def _need(t, name, numel, itemsize, exact_shape=None):
+    """Rejects a device tensor that the batch would overrun: wrong element size, too few elements, not contiguous,
+    not on a GPU."""
+    if t is None:
+        return
+    if not t.is_cuda or not t.is_contiguous():
+        raise OpenSPSError(3, f"{name}: a contiguous GPU tensor is required")
+    if t.element_size() != itemsize:
+        raise OpenSPSError(3, f"{name}: {itemsize}-byte elements required for this precision, got {t.dtype}")
+    if exact_shape is not None and tuple(t.shape) != tuple(exact_shape):
+        raise OpenSPSError(3, f"{name}: shape {tuple(exact_shape)} required, got {tuple(t.shape)}")
+    if t.numel() < numel:
+        raise OpenSPSError(3, f"{name}: {numel} elements required, got {t.numel()}")
+
+
 class Context:
     """One OpenSPS context (an int handle of the C API)."""
 
@@ -92,6 +107,7 @@ class Context:
         _check(st)
         self.handle = h.value
         self.nspec = self.nt = self.nz = self.nbands = 0
+        self.mixed = False      # the context's kernel_precision ("mixed" or "double")
 
     def close(self):
         if self.handle:
@@ -116,6 +132,8 @@ class Context:
         st = C.c_int()
         lib().opensps_context_set_int(self.handle, key.encode(), int(value), C.byref(st))
         _check(st)
+        if key.strip().lower() == "mixed_precision":
+            self.mixed = int(value) != 0
 
     def set_float(self, key, value):
         st = C.c_int()
@@ -126,6 +144,8 @@ class Context:
         st = C.c_int()
         lib().opensps_context_set_str(self.handle, key.encode(), value.encode(), C.byref(st))
         _check(st)
+        if key.strip().lower() == "kernel_precision":
+            self.mixed = value.strip().lower() == "mixed"
 
     def set_ssp_basis(self, grid, zsol=0.0142):
         st = C.c_int()
@@ -231,7 +251,16 @@ class Context:
 
     def compute_csp_batch_device(self, d_prm, d_mags, d_spec=None, d_mass=None, d_lbol=None, d_weights=None,
                                  precision=-1, split=0):
-        """The batch on device-resident torch tensors (no host transfer)."""
+        """The batch on device-resident torch tensors (no host transfer). The tensors are checked against the
+        batch before the launch: the kernels write n x nspec spectra of 4 bytes (mixed) or 8 bytes (FP64)."""
+        n = d_prm.shape[1] if d_prm.dim() == 2 else -1
+        mixed = self.mixed if precision < 0 else precision == 1
+        _need(d_prm, "d_prm", NPAR * n, 8, exact_shape=(NPAR, n))
+        _need(d_mags, "d_mags", n * self.nbands, 8)
+        _need(d_spec, "d_spec", n * self.nspec, 4 if mixed else 8)
+        _need(d_mass, "d_mass", n, 8)
+        _need(d_lbol, "d_lbol", n, 8)
+        _need(d_weights, "d_weights", n * self.nt, 8)
         st = C.c_int()
         ptr = lambda t: None if t is None else t.data_ptr()
         lib().opensps_context_compute_csp_batch_device(self.handle, d_prm.shape[1], ptr(d_prm), ptr(d_weights),

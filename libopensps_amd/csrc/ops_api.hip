@@ -135,9 +135,9 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
     if (!c) return 1;
     (void)hipStreamSynchronize(c->stream);
     void* ps[] = {c->d_lam, c->d_loglam, c->d_ltab, c->d_wl, c->d_dlam, c->d_time, c->d_logz, c->d_bmass, c->d_blbol,
-                  c->d_igm_a, c->d_basis64, c->d_A64, c->d_Ahi, c->d_Alo, c->d_bptr, c->d_bjlo, c->d_bnj, c->d_bc64,
+                  c->d_igm_a, c->d_basis64, c->d_A64, c->d_Ahi, c->d_Alo, c->d_bmeta, c->d_bc64,
                   c->d_bc32, c->d_mask, c->d_cz, c->d_cdl, c->yo.p, c->wcol.p, c->w64.p, c->order.p, c->tiles.p,
-                  c->misc.p, c->spec.p, c->flag.p};
+                  c->misc.p, c->spec.p, c->flag.p, c->d_rdlam, c->d_ltab32, c->d_wl32, c->d_sm32, c->d_bkt, c->hdr.p};
     for (void* p : ps) free_ptr(p);
     for (int k = 0; k <= OpsCtx::NPROF; k++)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
@@ -177,6 +177,9 @@ int opensps_hip_set_grid(opensps_hip_ctx_t* c, int nspec, const double* lam, int
     c->nt = nt;
     c->nz = nz;
     c->nsp = ((nspec + OPS_TILE_PX - 1) / OPS_TILE_PX) * OPS_TILE_PX;
+    c->nbands = 0;   // the band tables index the old grid: set_bands again
+    c->band_j0 = 0;
+    c->band_j1 = -1;
     c->npair = nz > 1 ? nz - 1 : 1;
     c->kpair = nz > 1 ? 2 * nt : nt;
     c->k64 = ((c->kpair + OPS_KSTEP_F64 - 1) / OPS_KSTEP_F64) * OPS_KSTEP_F64;
@@ -186,13 +189,46 @@ int opensps_hip_set_grid(opensps_hip_ctx_t* c, int nspec, const double* lam, int
     c->h_logz.assign(logz, logz + nz);
     // Grid-only tables in FP64 (the reference forms them once per context: prepare_batch_grid,
     // attenuation_tables_cached; src/spectra/opensps_batch_photometry.f90:237-394, opensps_csp_batch.f90:3494-3566)
-    std::vector<double> loglam(nspec), ltab(nspec), wl(nspec), dlam(nspec);
+    // ltab and ltab32 are padded to nsp (the chain reads them by whole quads)
+    std::vector<double> loglam(nspec), ltab(c->nsp, 0.0), wl(nspec), dlam(nspec);
     for (int j = 0; j < nspec; j++) {
         loglam[j] = log(lam[j]);
         ltab[j] = log(lam[j] / 5500.0);
         wl[j] = lam[j] > 2.2250738585072014e-308 ? 1.0 / lam[j] : 0.0;
         dlam[j] = j + 1 < nspec ? lam[j + 1] - lam[j] : 0.0;
     }
+    // FP32 copies and the per-pixel smoothing factors of the FP32 chain
+    std::vector<double> rdlam(nspec);
+    std::vector<float> ltab32(c->nsp, 0.0f), wl32(nspec);
+    std::vector<float4> sm32(nspec);
+    for (int j = 0; j < nspec; j++) {
+        rdlam[j] = j + 1 < nspec ? 1.0 / dlam[j] : 0.0;
+        ltab32[j] = (float)ltab[j];
+        wl32[j] = (float)wl[j];
+        sm32[j] = make_float4((float)wl[j], j > 0 ? (float)(0.5 * dlam[j - 1]) : 0.0f, (float)(0.5 * dlam[j]), 0.0f);
+    }
+    // Buckets over ln(lambda) no wider than the finest pixel: a lookup lands within a pixel or two of
+    // the interval, and an exact walk on lambda finishes it
+    double minsp = 1e300;
+    for (int j = 0; j + 1 < nspec; j++) minsp = fmin(minsp, loglam[j + 1] - loglam[j]);
+    const double range = loglam[nspec - 1] - loglam[0];
+    double nb = range / minsp + 2.0;
+    int nbk = nb > (double)(1 << 20) ? (1 << 20) : (int)nb + 1;
+    double bw = nb > (double)(1 << 20) ? range / (double)(nbk - 2) : minsp;
+    std::vector<int> bkt(nbk);
+    for (int b = 0, j = 0; b < nbk; b++) {
+        const double x = loglam[0] + (double)b * bw;
+        while (j < nspec && loglam[j] < x) j++;
+        bkt[b] = j;
+    }
+    c->nbk = nbk;
+    c->lnl0 = loglam[0];
+    c->bkinv = 1.0 / bw;
+    UPLOAD(c, c->d_rdlam, rdlam.data(), (size_t)nspec);
+    UPLOAD(c, c->d_ltab32, ltab32.data(), ltab32.size());
+    UPLOAD(c, c->d_wl32, wl32.data(), (size_t)nspec);
+    UPLOAD(c, c->d_sm32, sm32.data(), (size_t)nspec);
+    UPLOAD(c, c->d_bkt, bkt.data(), (size_t)nbk);
     int jtop = 0;
     for (int j = 1; j <= nspec; j++)
         if (lam[j - 1] < 1215.67) jtop = j;
@@ -201,7 +237,7 @@ int opensps_hip_set_grid(opensps_hip_ctx_t* c, int nspec, const double* lam, int
     c->jtop = jtop;
     UPLOAD(c, c->d_lam, lam, (size_t)nspec);
     UPLOAD(c, c->d_loglam, loglam.data(), (size_t)nspec);
-    UPLOAD(c, c->d_ltab, ltab.data(), (size_t)nspec);
+    UPLOAD(c, c->d_ltab, ltab.data(), ltab.size());
     UPLOAD(c, c->d_wl, wl.data(), (size_t)nspec);
     UPLOAD(c, c->d_dlam, dlam.data(), (size_t)nspec);
     UPLOAD(c, c->d_time, log_age, (size_t)nt);
@@ -253,7 +289,9 @@ int opensps_hip_set_bands(opensps_hip_ctx_t* c, int nbands, const double* bands)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<double> wtl(nspec, 0.0);
     for (int j = 0; j + 1 < nspec; j++) wtl[j] = 0.5 * fabs(c->h_lam[j + 1] - c->h_lam[j]);
-    std::vector<int> bptr(nbands + 1), bjlo(nbands ? nbands : 1), bnj(nbands ? nbands : 1);
+    // Per band a quad-aligned run of pixels [q0, q0 + nq) holding the support (zeros around it), and its
+    // coefficients at a quad-aligned offset: the chain reads both 4 pixels at a time
+    std::vector<int4> meta(nbands ? nbands : 1, make_int4(0, 0, 0, 0));
     std::vector<double> c64;
     int j0 = nspec, j1 = -1;
     for (int b = 0; b < nbands; b++) {
@@ -267,32 +305,33 @@ int opensps_hip_set_bands(opensps_hip_ctx_t* c, int nbands, const double* bands)
             }
         int jlo = ilo - 1 > 1 ? ilo - 1 : 1, jhi = ihi < nspec - 1 ? ihi : nspec - 1;
         if (ilo > ihi) { jlo = 1; jhi = 0; }
-        bptr[b] = (int)c64.size();
-        bjlo[b] = jlo - 1;
-        int nj = jhi >= jlo ? jhi - jlo + 2 : 0;   // pixels jlo .. jhi + 1 (the trapezoid reads j + 1)
-        bnj[b] = nj;
+        const int nj = jhi >= jlo ? jhi - jlo + 2 : 0;   // pixels jlo .. jhi + 1 (the trapezoid reads j + 1)
+        if (nj == 0) continue;
+        const int p0 = jlo - 1;                           // 0-based first pixel
+        const int q0 = p0 & ~3, nq = ((p0 + nj - q0) + 3) & ~3;
+        meta[b] = make_int4((int)c64.size(), q0, nq, 0);
         // per pixel the trapezoid weights of the two intervals it closes, times the band (the reference's
         // sparse band matrix bk, :323-344), formed in FP64
-        for (int j = jlo; j < jlo + nj; j++) {
+        for (int p = q0; p < q0 + nq; p++) {
+            const int j = p + 1;   // 1-based
             double cw = 0.0;
-            if (j >= jlo && j <= jhi) cw += wtl[j - 1];
-            if (j - 1 >= jlo && j - 1 <= jhi) cw += wtl[j - 2];
-            c64.push_back(B[j - 1] * cw);
+            if (j >= jlo && j < jlo + nj) {
+                if (j >= jlo && j <= jhi) cw += wtl[j - 1];
+                if (j - 1 >= jlo && j - 1 <= jhi) cw += wtl[j - 2];
+                cw *= B[j - 1];
+            }
+            c64.push_back(cw);
         }
-        if (nj > 0) {
-            j0 = jlo - 1 < j0 ? jlo - 1 : j0;
-            j1 = jlo - 1 + nj - 1 > j1 ? jlo - 1 + nj - 1 : j1;
-        }
+        j0 = q0 < j0 ? q0 : j0;
+        j1 = q0 + nq - 1 > j1 ? q0 + nq - 1 : j1;
     }
-    bptr[nbands] = (int)c64.size();
+    if (c64.empty()) c64.assign(4, 0.0);
     std::vector<float> c32(c64.size());
     for (size_t i = 0; i < c64.size(); i++) c32[i] = (float)c64[i];
     c->nbands = nbands;
     c->band_j0 = j1 >= 0 ? j0 : 0;
-    c->band_j1 = j1;
-    UPLOAD(c, c->d_bptr, bptr.data(), bptr.size());
-    UPLOAD(c, c->d_bjlo, bjlo.data(), bjlo.size());
-    UPLOAD(c, c->d_bnj, bnj.data(), bnj.size());
+    c->band_j1 = j1;   // < nsp: the quads end at or before the padded grid's end
+    UPLOAD(c, c->d_bmeta, meta.data(), meta.size());
     UPLOAD(c, c->d_bc64, c64.data(), c64.size());
     UPLOAD(c, c->d_bc32, c32.data(), c32.size());
     c->h_mask.assign(nbands ? nbands : 1, 1);
@@ -350,6 +389,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     int st;
     if ((st = ops_reserve(c, c->misc, sizeof(int) * ((size_t)4 * n + 64)))) return st;
     if ((st = ops_reserve(c, c->tiles, sizeof(int4) * (size_t)maxtiles))) return st;
+    if ((st = ops_reserve(c, c->hdr, sizeof(double) * OPS_HDR_N * (size_t)n))) return st;
     if ((st = ops_reserve(c, c->yo, tsz * (size_t)parts * n * c->nsp))) return st;
     if (mixed) {
         if ((st = ops_reserve(c, c->wcol, sizeof(uint16_t) * 2 * (size_t)n * parts * kpad))) return st;
@@ -390,6 +430,21 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     pa.idx = a->idx;
     pa.status = d_status;
     pa.flag = d_flag;
+    LamLookup lk;
+    lk.nspec = c->nspec;
+    lk.nbk = c->nbk;
+    lk.lnl0 = c->lnl0;
+    lk.bkinv = c->bkinv;
+    lk.lam = c->d_lam;
+    lk.bkt = c->d_bkt;
+    pa.add_igm = c->add_igm;
+    pa.jtop = c->jtop;
+    pa.ncosmo = c->ncosmo;
+    pa.lk = lk;
+    pa.igm_a = c->d_igm_a;
+    pa.cz = c->d_cz;
+    pa.cdl = c->d_cdl;
+    pa.hdr = (double*)c->hdr.p;
     size_t lds_prep = sizeof(double) * 4 * 5 * (size_t)(c->nt + 1);
     if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     k_prep<<<(n + 3) / 4, 256, lds_prep, c->stream>>>(pa);
@@ -447,8 +502,13 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ca.smooth_fast = c->smooth_fast;
     ca.band_j0 = c->band_j0;
     ca.band_j1 = c->nbands > 0 ? c->band_j1 : -1;
-    ca.ncosmo = c->ncosmo;
     ca.amzl = abs_mag_zeropoint_log();
+    ca.lk = lk;
+    ca.hdr = (const double*)c->hdr.p;
+    ca.rdlam = c->d_rdlam;
+    ca.ltab32 = c->d_ltab32;
+    ca.wl32 = c->d_wl32;
+    ca.sm32 = c->d_sm32;
     ca.prm = a->prm;
     ca.yo = c->yo.p;
     ca.lam = c->d_lam;
@@ -457,11 +517,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ca.wl = c->d_wl;
     ca.dlam = c->d_dlam;
     ca.igm_a = c->d_igm_a;
-    ca.cz = c->d_cz;
-    ca.cdl = c->d_cdl;
-    ca.bptr = c->d_bptr;
-    ca.bjlo = c->d_bjlo;
-    ca.bnj = c->d_bnj;
+    ca.bmeta = c->d_bmeta;
     ca.mask = c->d_mask;
     ca.bc64 = c->d_bc64;
     ca.bc32 = c->d_bc32;

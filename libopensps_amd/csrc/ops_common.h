@@ -39,6 +39,26 @@ __device__ __forceinline__ int find_interval_ref(const double* __restrict__ a, i
     return lower;
 }
 
+// #{j : lam[j] <= x} (STRICT: lam[j] < x) on the ascending grid of the lookup: the bucket of lnx (ln x to
+// within a few ulp) starts a pixel or two away, and a walk with exact FP64 compares on lam finishes it.
+// Always within [0, nspec]; a NaN x counts 0.
+template <bool STRICT>
+__device__ __forceinline__ int lam_count(int nspec, int nbk, double lnl0, double bkinv, const double* __restrict__ lam,
+                                         const int* __restrict__ bkt, double x, double lnx) {
+    const double u = (lnx - lnl0) * bkinv;
+    int b = 0;
+    if (u > 0.0) b = u >= (double)(nbk - 1) ? nbk - 1 : (int)u;
+    int c = bkt[b];
+    if (STRICT) {
+        while (c < nspec && lam[c] < x) c++;
+        while (c > 0 && !(lam[c - 1] < x)) c--;
+    } else {
+        while (c < nspec && lam[c] <= x) c++;
+        while (c > 0 && lam[c - 1] > x) c--;
+    }
+    return c;
+}
+
 // exp(x) in FP32 for an FP64 argument: x = k ln 2 + r reduced in FP64, exp(r) in FP32, scaled by
 // 2^k. Keeps ~1 ulp of FP32 where rounding x itself to FP32 would cost |x| eps (the reference's
 // exp_kp, src/physics/opensps_csp_batch.f90:3886-3919). Flushes to 0 below the FP32 range.

@@ -32,6 +32,12 @@ struct OpsCtx {
     std::vector<double> h_lam, h_logz, h_time, h_basis, h_mass, h_lbol;
     double *d_lam = nullptr, *d_loglam = nullptr, *d_ltab = nullptr, *d_wl = nullptr, *d_dlam = nullptr;
     double *d_time = nullptr, *d_logz = nullptr, *d_bmass = nullptr, *d_blbol = nullptr;
+    double* d_rdlam = nullptr;    // 1 / (lam[j + 1] - lam[j]) (0 at the last pixel)
+    float *d_ltab32 = nullptr, *d_wl32 = nullptr;   // FP32 copies for the FP32 chain
+    float4* d_sm32 = nullptr;     // per pixel {1 / lam, dlam[j - 1] / 2, dlam[j] / 2, 0}: smoothing weights, FP32
+    int nbk = 0;                  // buckets of the wavelength lookup
+    double lnl0 = 0.0, bkinv = 0.0;   // bucket b starts at ln(lam) = lnl0 + b / bkinv
+    int* d_bkt = nullptr;         // per bucket: the number of pixels with ln(lam) below the bucket's start
     int jtop = 0;                 // pixels below Lyman-alpha
     double* d_igm_a = nullptr;    // (jtop, 6) IGM wavelength factors
     double* d_basis64 = nullptr;  // nspec x nt x nz FP64 (for the packing)
@@ -40,9 +46,8 @@ struct OpsCtx {
 
     // ---- filters (set_bands) ----
     int nbands = 0;
-    int band_j0 = 0, band_j1 = -1;                 // 0-based pixel range the band integrals read
-    int *d_bptr = nullptr, *d_bjlo = nullptr;      // per band: offset into the coefficients, first pixel
-    int* d_bnj = nullptr;                          // per band: number of pixels
+    int band_j0 = 0, band_j1 = -1;                 // 0-based pixel range the band integrals read (whole quads)
+    int4* d_bmeta = nullptr;                       // per band: {coefficient offset, first pixel, pixels, 0}
     double* d_bc64 = nullptr;                      // per (band, pixel of its support) coefficient, FP64
     float* d_bc32 = nullptr;                       // the same in FP32
     int* d_mask = nullptr;                         // per band: computed (1) or NaN (0)
@@ -60,6 +65,7 @@ struct OpsCtx {
 
     // ---- batch scratch (grown on demand) ----
     OpsBuffer yo, wcol, w64, order, tiles, misc, spec, flag;
+    OpsBuffer hdr;   // per-model constants of the chain (OPS_HDR_N doubles per model)
     int* d_last_flag = nullptr;   // the per-batch status flag of the last batch (read by sync)
 
     // ---- per-kernel timing (HIP events on the context's stream) ----

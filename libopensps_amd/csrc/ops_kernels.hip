@@ -7,6 +7,8 @@
 //               (FP64 16x16x4, or split-BF16 16x16x32 with three products and FP32 accumulation)
 //   k_chain     one workgroup per model: dust, IGM, velocity smoothing, the spectrum out, the
 //               observed frame, the band integrals and the magnitudes, the spectrum staged in LDS
+#include <type_traits>
+
 #include "ops_common.h"
 #include "ops_launch.h"
 
@@ -209,6 +211,8 @@ __device__ int wave_sfh_weights(const double* __restrict__ tf, int nt, double ti
 // ------------------------------------------------------------------------------------------------
 // k_prep: one wavefront per model (4 per workgroup)
 // ------------------------------------------------------------------------------------------------
+__device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, int lane);
+
 __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -317,6 +321,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
             }
         }
     }
+    wave_chain_header(a, m, p, lane);
 }
 
 #pragma clang fp contract(on)
@@ -603,209 +608,350 @@ __global__ void k_pack_bf16(PackArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// The chain's per-model constants, one wave per model at the end of k_prep: the uniform FP64 math of
+// the chain formed once per model rather than once per thread of the chain. The six (1 + z) powers of
+// the IGM optical depth and its cap, the maximum over the pixels below Lyman-alpha (apply_igm_z,
+// opensps_csp_batch.f90:4205-4319); the velocity kernel's constants and smoothed pixel range
+// (build_velocity_kernel, src/spectra/opensps_smoothing.f90:553-633); the distance term (distance_term,
+// src/spectra/opensps_batch_photometry.f90:784-794, with the clamped find_interval of z in the cosmology
+// table, descending in z as the reference builds it, or ascending).
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ double igm_tau6(const double* __restrict__ aj, const double* b6) {
+    return aj[0] * b6[0] + aj[1] * b6[1] + aj[2] * b6[2] + aj[3] * b6[3] + aj[4] * b6[4] + aj[5] * b6[5];
+}
+
+__device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, int lane) {
+    const double z = p[OPS_P_ZRED], opz = 1.0 + z;
+    double pw = 0.0;
+    if (lane < 6) {
+        const double e = lane == 0 ? 3.46 : lane == 1 ? 1.68 : lane == 2 ? 3.0 + 0.46
+                       : lane == 3 ? 1.5 + 0.18 : lane == 4 ? 3.0 - 1.32 : 1.68;
+        pw = pow(opz, e);
+    }
+    double b6[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) b6[k] = __shfl(pw, k, 64);
+    double tmax = 0.0;
+    int jmax = 1;
+    if (a.add_igm && z > OPS_SAFE_FLOOR && a.jtop > 0) {
+        for (int j = lane; j < a.jtop; j += 64) argmax_merge(tmax, jmax, igm_tau6(a.igm_a + (size_t)j * 6, b6), j + 1);
+        for (int o = 32; o > 0; o >>= 1) {
+            double v2 = __shfl_xor(tmax, o, 64);
+            int j2 = __shfl_xor(jmax, o, 64);
+            argmax_merge(tmax, jmax, v2, j2);
+        }
+    }
+    const bool asc = a.cz[a.ncosmo - 1] >= a.cz[0];
+    int cnt = 0;
+    for (int q = lane; q < a.ncosmo; q += 64) cnt += (asc ? a.cz[q] <= z : z < a.cz[q]) ? 1 : 0;
+    cnt = wave_sum(cnt);
+    const double sig = p[OPS_P_SIGMA_SMOOTH];
+    const double minl = p[OPS_P_MIN_WAVE_SMOOTH], maxl = p[OPS_P_MAX_WAVE_SMOOTH];
+    const bool smooth = a.smooth_fast && sig > OPS_SAFE_FLOOR;
+    const double sb = sig / (OPS_C_LIGHT * 1.0e-13);
+    const LamLookup& L = a.lk;
+    double v = 0.0;
+    switch (lane) {
+    case OPS_HDR_RZ: v = 1.0 / opz; break;
+    case OPS_HDR_LNOPZ: v = log1p(z); break;
+    case OPS_HDR_INV2: v = smooth ? 0.5 * (1.0 / sb) * (1.0 / sb) : 0.0; break;
+    case OPS_HDR_FBLUE: v = smooth ? exp(-4.0 * sb) : 1.0; break;
+    case OPS_HDR_FRED: v = smooth ? exp(4.0 * sb) : 1.0; break;
+    case OPS_HDR_DIST:
+        if (fabs(z) > OPS_SAFE_FLOOR) {
+            const int i = max(1, min(cnt, a.ncosmo - 1));
+            const double slope = (a.cdl[i] - a.cdl[i - 1]) / (a.cz[i] - a.cz[i - 1]);
+            const double dm = a.cdl[i - 1] + slope * (z - a.cz[i - 1]);
+            if (dm == dm && dm > OPS_SAFE_FLOOR) v = 5.0 * log10(dm / 10.0) + (-2.5 * log10(1.0 + z));
+        }
+        break;
+    case OPS_HDR_TMAX: v = tmax; break;
+    case OPS_HDR_JMAX: v = (double)jmax; break;
+    case OPS_HDR_W0:   // #{lam < minl}
+        if (smooth && minl <= maxl) v = (double)lam_count<true>(L.nspec, L.nbk, L.lnl0, L.bkinv, L.lam, L.bkt, minl, log(minl));
+        break;
+    case OPS_HDR_W1:   // #{lam <= maxl}
+        if (smooth && minl <= maxl) v = (double)lam_count<false>(L.nspec, L.nbk, L.lnl0, L.bkinv, L.lam, L.bkt, maxl, log(maxl));
+        break;
+    default: break;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+        if (lane == OPS_HDR_B6 + k) v = b6[k];
+    if (lane < OPS_HDR_N) a.hdr[(size_t)m * OPS_HDR_N + lane] = v;
+}
+
+// ------------------------------------------------------------------------------------------------
 // k_chain: one workgroup per model. Stages 5-8 of compute_csp_batch in the reference's order:
 // attenuation (attenuate_batch, pixel_transmissions: opensps_csp_batch.f90:3576-3818), IGM
 // (apply_igm_z :4205-4319), velocity smoothing (build_velocity_kernel, src/spectra/opensps_smoothing.f90:553-633,
 // formed per pixel and model on the fly), the observed frame (observed_frame_z,
 // src/spectra/opensps_batch_photometry.f90:819-903), the band integrals (band_fluxes :1049-1171) and the
-// magnitudes (magnitudes_from_fluxes :185-232). T: the kernel precision (double, or float with FP64
-// formed arguments).
+// magnitudes (magnitudes_from_fluxes :185-232).
+// T = double: FP64 throughout. T = float: the FP32 chain. Its transmissions, smoothing weights, interpolation
+// and band products are FP32 (from FP32 tables); the IGM optical depths, the interval selection, the window
+// edges, the band sums across lanes and the magnitudes stay FP64, so every index the chain selects is the FP64
+// path's. The per-model constants come from k_prep (hdr).
+// Layout: a thread takes quads of 4 consecutive pixels (16- or 32-byte loads and LDS accesses); the band
+// coefficients are quad-aligned (zeros pad each band to whole quads), and the observed frame covers the padded
+// band pixels, with 0 past the grid.
+// Phases (one barrier each): attenuation x IGM -> s0 | smoothing s0 -> s1 and the spectrum out | observed
+// frame s1 -> s0 over the band quads | band sums from s0, 16 lanes per band -> LDS | magnitudes, a thread per band.
 // ------------------------------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T dust_exp(double x);
-template <>
-__device__ __forceinline__ double dust_exp<double>(double x) { return exp(x); }
-template <>
-__device__ __forceinline__ float dust_exp<float>(double x) { return exp_kp(x); }
+template <typename T> struct Quad;
+template <> struct Quad<float> { typedef f32x4_t type; };
+template <> struct Quad<double> { typedef f64x4_t type; };
 
-// exp(-arg) of a smoothing weight (arg <= 8 inside the +-4 sigma window): FP64, or FP32 of the FP64-formed argument
+// The velocity-smoothed value of pixel i (the window and weights of build_velocity_kernel)
 template <typename T>
-__device__ __forceinline__ double smooth_exp(double arg);
-template <>
-__device__ __forceinline__ double smooth_exp<double>(double arg) { return exp(-arg); }
-template <>
-__device__ __forceinline__ double smooth_exp<float>(double arg) { return (double)__expf(-(float)arg); }
-
-template <typename T>
-__device__ __forceinline__ T tr_exp(T x);
-template <>
-__device__ __forceinline__ double tr_exp<double>(double x) { return exp(x); }
-template <>
-__device__ __forceinline__ float tr_exp<float>(float x) { return expf(x); }
+__device__ __forceinline__ T smooth_px(const ChainArgs& a, const T* __restrict__ s0, int i, double inv2, double fblue,
+                                       double fred, T out) {
+    const int nspec = a.nspec;
+    const double li = a.lam[i];
+    const double vl = li * fblue, vh = li * fred;
+    int il = i, ih = i;
+    while (il > 0 && a.lam[il] > vl) il--;
+    while (ih < nspec - 1 && a.lam[ih] < vh) ih++;
+    if (ih - il + 1 < 2) return out;
+    const double lli = a.loglam[i];
+    if constexpr (std::is_same<T, float>::value) {
+        const float c2 = (float)inv2;
+        float tw = 0.0f, acc = 0.0f;
+        for (int k = il; k <= ih; k++) {
+            const float d = (float)(lli - a.loglam[k]);
+            const float4 f = a.sm32[k];
+            float wk = __expf(-(c2 * (d * d))) * f.x;
+            wk *= (k > il ? f.y : 0.0f) + (k < ih ? f.z : 0.0f);
+            tw += wk;
+            acc += wk * s0[k];
+        }
+        if (tw > 0.0f) out = acc / tw;
+    } else {
+        double tw = 0.0, acc = 0.0;
+        for (int k = il; k <= ih; k++) {
+            const double d = lli - a.loglam[k];
+            double wk = exp(-(inv2 * (d * d))) * a.wl[k];
+            const double dl = k > il ? a.dlam[k - 1] : 0.0;
+            const double dr = k < ih ? a.dlam[k] : 0.0;
+            wk = 0.5 * wk * (dl + dr);
+            tw += wk;
+            acc += wk * s0[k];
+        }
+        if (tw > OPS_SAFE_FLOOR) out = acc / tw;
+    }
+    return out;
+}
 
 template <typename T>
 __global__ __launch_bounds__(512) void k_chain(ChainArgs a) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
+    constexpr bool F32 = std::is_same<T, float>::value;
+    typedef typename Quad<T>::type Q;
+    extern __shared__ __align__(32) unsigned char lds_raw[];
     T* s0 = (T*)lds_raw;
     T* s1 = s0 + a.nsp;
-    __shared__ double red_v[16];
-    __shared__ int red_j[16];
-    __shared__ double sh_dist;
-    const int m = blockIdx.x, n = a.n, nspec = a.nspec;
-    const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wv = tid >> 6, nwv = nth >> 6;
+    Q* s0q = (Q*)s0;
+    Q* s1q = (Q*)s1;
+    const int m = blockIdx.x, n = a.n, nspec = a.nspec, nq = a.nsp >> 2;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const double* h = a.hdr + (size_t)m * OPS_HDR_N;
     const double z = a.prm[(size_t)OPS_P_ZRED * n + m];
     const double dust1 = a.prm[(size_t)OPS_P_DUST1 * n + m], dust2 = a.prm[(size_t)OPS_P_DUST2 * n + m];
     const double dust3 = a.prm[(size_t)OPS_P_DUST3 * n + m];
     const double ed = a.prm[(size_t)OPS_P_DUST_INDEX * n + m], ed1 = a.prm[(size_t)OPS_P_DUST1_INDEX * n + m];
     const double fob = a.prm[(size_t)OPS_P_FRAC_OBRUN * n + m], fnd = a.prm[(size_t)OPS_P_FRAC_NODUST * n + m];
     const double igmf = a.prm[(size_t)OPS_P_IGM_FACTOR * n + m];
-    const double sig = a.prm[(size_t)OPS_P_SIGMA_SMOOTH * n + m];
-    const double minl = a.prm[(size_t)OPS_P_MIN_WAVE_SMOOTH * n + m], maxl = a.prm[(size_t)OPS_P_MAX_WAVE_SMOOTH * n + m];
     const bool no_birth = dust1 <= OPS_SAFE_FLOOR || 1.0 - fob <= OPS_SAFE_FLOOR;
     const bool no_old = fabs(dust3) <= OPS_SAFE_FLOOR;
     const bool no_diffuse_mix = 1.0 - fnd <= OPS_SAFE_FLOOR;
     const bool igm_on = a.add_igm && z > OPS_SAFE_FLOOR && a.jtop > 0;
 
-    // IGM: the optical depth's cap needs its maximum over the pixels below Lyman-alpha first
+    // ---- phase 1: attenuation and IGM -> s0 (padded pixels: the zero age sums) ----
+    const Q* y4 = (const Q*)((const T*)a.yo + (size_t)m * a.nsp);
+    const Q* o4 = a.parts == 2 ? (const Q*)((const T*)a.yo + ((size_t)n + m) * a.nsp) : nullptr;
+    const int jt = igm_on ? a.jtop : 0;
     double b6[6];
-    double tmax = 0.0;
-    int jmax = 1;
-    if (igm_on) {
-        double opz = 1.0 + z;
-        b6[0] = pow(opz, 3.46);
-        b6[1] = pow(opz, 1.68);
-        b6[2] = pow(opz, 3.0 + 0.46);
-        b6[3] = pow(opz, 1.5 + 0.18);
-        b6[4] = pow(opz, 3.0 - 1.32);
-        b6[5] = pow(opz, 1.68);
-        double v = 0.0;
-        int jv = 1;
-        for (int j = tid; j < a.jtop; j += nth) {
-            const double* aj = a.igm_a + (size_t)j * 6;
-            double tau = aj[0] * b6[0] + aj[1] * b6[1] + aj[2] * b6[2] + aj[3] * b6[3] + aj[4] * b6[4] + aj[5] * b6[5];
-            argmax_merge(v, jv, tau, j + 1);
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            double v2 = __shfl_xor(v, o, 64);
-            int j2 = __shfl_xor(jv, o, 64);
-            argmax_merge(v, jv, v2, j2);
-        }
-        if (lane == 0) { red_v[wv] = v; red_j[wv] = jv; }
-        __syncthreads();
-        tmax = 0.0; jmax = 1;
-        for (int q = 0; q < nwv; q++) argmax_merge(tmax, jmax, red_v[q], red_j[q]);
-    }
-
-    // Attenuation and IGM -> s0
-    const T* y0 = (const T*)a.yo + (size_t)m * a.nsp;
-    const T* o0 = a.parts == 2 ? (const T*)a.yo + ((size_t)n + m) * a.nsp : nullptr;
-    const T cd2 = (T)dust2, cd1 = (T)dust1, cd3 = (T)dust3, cfob = (T)fob, cfnd = (T)fnd;
-    for (int i = tid; i < nspec; i += nth) {
-        T y = y0[i], o = o0 ? o0[i] : (T)0;
-        if (a.young) ((T*)a.young)[(size_t)m * nspec + i] = y;
-        if (a.old) ((T*)a.old)[(size_t)m * nspec + i] = o;
-        double L = a.ltab[i];
-        T curve = (ed != 0.0) ? dust_exp<T>(ed * L) : (T)1;
-        T tr_d = tr_exp<T>(-(cd2 * curve));
-        T tr_b = (T)1, tr_o = (T)1;
-        if (!no_birth) tr_b = tr_exp<T>(-(cd1 * dust_exp<T>(ed1 * L)));
-        if (!no_old) tr_o = tr_exp<T>(-(cd3 * curve));
-        T ssum = (y * tr_b * ((T)1 - cfob) + y * cfob) + o * tr_o;
-        T tot = no_diffuse_mix ? ssum : ssum * (tr_d * ((T)1 - cfnd) + cfnd);
-        if (igm_on && i < a.jtop) {
-            double tau;
-            if (jmax > 1 && i + 1 <= jmax) {
-                tau = tmax;
-            } else {
-                const double* aj = a.igm_a + (size_t)i * 6;
-                tau = aj[0] * b6[0] + aj[1] * b6[1] + aj[2] * b6[2] + aj[3] * b6[3] + aj[4] * b6[4] + aj[5] * b6[5];
+#pragma unroll
+    for (int k = 0; k < 6; k++) b6[k] = h[OPS_HDR_B6 + k];
+    const double tmax = h[OPS_HDR_TMAX];
+    const int jmax = (int)h[OPS_HDR_JMAX];
+    const int jcap = jmax > 1 ? jmax : 0;   // pixels i < jcap take the capped depth
+    for (int q = tid; q < nq; q += nth) {
+        const Q y = y4[q];
+        Q o = {};
+        if (o4) o = o4[q];
+        Q tot;
+        if constexpr (F32) {
+            const f32x4_t L = ((const f32x4_t*)a.ltab32)[q];
+            const float cf = (float)fob, cn = (float)fnd;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const float curve = (ed != 0.0) ? __expf((float)ed * L[e]) : 1.0f;
+                const float tr_d = __expf(-((float)dust2 * curve));
+                float tr_b = 1.0f, tr_o = 1.0f;
+                if (!no_birth) tr_b = __expf(-((float)dust1 * __expf((float)ed1 * L[e])));
+                if (!no_old) tr_o = __expf(-((float)dust3 * curve));
+                const float ssum = (y[e] * tr_b * (1.0f - cf) + y[e] * cf) + o[e] * tr_o;
+                tot[e] = no_diffuse_mix ? ssum : ssum * (tr_d * (1.0f - cn) + cn);
             }
-            tot = tot * dust_exp<T>(-tau * igmf);
+        } else {
+            const f64x4_t L = ((const f64x4_t*)a.ltab)[q];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const double curve = (ed != 0.0) ? exp(ed * L[e]) : 1.0;
+                const double tr_d = exp(-(dust2 * curve));
+                double tr_b = 1.0, tr_o = 1.0;
+                if (!no_birth) tr_b = exp(-(dust1 * exp(ed1 * L[e])));
+                if (!no_old) tr_o = exp(-(dust3 * curve));
+                const double ssum = (y[e] * tr_b * (1.0 - fob) + y[e] * fob) + o[e] * tr_o;
+                tot[e] = no_diffuse_mix ? ssum : ssum * (tr_d * (1.0 - fnd) + fnd);
+            }
         }
-        s0[i] = tot;
+        if (4 * q < jt) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int i = 4 * q + e;
+                if (i < jt) {
+                    const double tau = i < jcap ? tmax : igm_tau6(a.igm_a + (size_t)i * 6, b6);
+                    if constexpr (F32)
+                        tot[e] = tot[e] * exp_kp(-tau * igmf);
+                    else
+                        tot[e] = tot[e] * exp(-tau * igmf);
+                }
+            }
+        }
+        s0q[q] = tot;
+        if (a.young || a.old) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int i = 4 * q + e;
+                if (i < nspec) {
+                    if (a.young) ((T*)a.young)[(size_t)m * nspec + i] = y[e];
+                    if (a.old) ((T*)a.old)[(size_t)m * nspec + i] = o[e];
+                }
+            }
+        }
     }
     __syncthreads();
 
-    // Velocity smoothing -> s1, and the spectrum out
-    const bool smooth = a.smooth_fast && sig > OPS_SAFE_FLOOR;
-    double sb = 0.0, inv2 = 0.0, fblue = 0.0, fred = 0.0;
-    if (smooth) {
-        sb = sig / (OPS_C_LIGHT * 1.0e-13);
-        inv2 = 0.5 * (1.0 / sb) * (1.0 / sb);
-        fblue = exp(-4.0 * sb);
-        fred = exp(4.0 * sb);
-    }
+    // ---- phase 2: velocity smoothing -> s1, and the spectrum out. Quads outside the smoothed range [w0, w1)
+    // copy; the range's pixels are spread over the threads one by one ----
+    const double inv2 = h[OPS_HDR_INV2], fblue = h[OPS_HDR_FBLUE], fred = h[OPS_HDR_FRED];
+    const int w0 = (int)h[OPS_HDR_W0], w1 = (int)h[OPS_HDR_W1];
     T* spec_out = a.spec ? (T*)a.spec + (size_t)m * nspec : nullptr;
-    for (int i = tid; i < nspec; i += nth) {
-        T out = s0[i];
-        double li = a.lam[i];
-        if (smooth && li >= minl && li <= maxl) {
-            double vl = li * fblue, vh = li * fred;
-            int il = i, ih = i;
-            while (il > 0 && a.lam[il] > vl) il--;
-            while (ih < nspec - 1 && a.lam[ih] < vh) ih++;
-            if (ih - il + 1 >= 2) {
-                double lli = a.loglam[i];
-                double tw = 0.0, acc = 0.0;
-                for (int k = il; k <= ih; k++) {
-                    double d = lli - a.loglam[k];
-                    double wk = smooth_exp<T>(inv2 * (d * d)) / a.lam[k];
-                    double dl = k > il ? a.dlam[k - 1] : 0.0;
-                    double dr = k < ih ? a.dlam[k] : 0.0;
-                    wk = 0.5 * wk * (dl + dr);
-                    tw += wk;
-                    acc += wk * (double)s0[k];
-                }
-                if (tw > OPS_SAFE_FLOOR) out = (T)(acc / tw);
-            }
+    for (int q = tid; q < nq; q += nth) {
+        const int i0 = 4 * q;
+        const Q v = s0q[q];
+        if (i0 + 3 < w0 || i0 >= w1) {
+            s1q[q] = v;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (i0 + e < w0 || i0 + e >= w1) s1[i0 + e] = v[e];
         }
+        if (spec_out) {
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (i0 + e < nspec && (i0 + e < w0 || i0 + e >= w1)) spec_out[i0 + e] = v[e];
+        }
+    }
+    for (int i = w0 + tid; i < w1; i += nth) {
+        const T out = smooth_px<T>(a, s0, i, inv2, fblue, fred, s0[i]);
         s1[i] = out;
         if (spec_out) spec_out[i] = out;
     }
     __syncthreads();
 
-    // Observed frame over the pixels the bands read -> s0
-    const int J0 = a.band_j0, J1 = a.band_j1;
-    if (J1 >= J0) {
-        const int npx = J1 - J0 + 1;
-        const int per = (npx + nth - 1) / nth;
-        const int j0 = J0 + tid * per, j1 = min(J1, j0 + per - 1);
-        const bool rest = fabs(z) <= OPS_SAFE_FLOOR;
-        int il = -1;
-        for (int j = j0; j <= j1; j++) {
-            double v;
-            if (rest) {
-                v = (double)s1[j];
-            } else {
-                double tg = a.lam[j] / (1.0 + z);
-                if (il < 0) {
-                    il = max(1, min(nspec - 1, count_le(a.lam, nspec, tg))) - 1;
+    // ---- phase 3: the observed frame over the band quads -> s0 (times 1 / lambda; 0 past the grid). The first
+    // pixel of a quad finds its interval by the bucket lookup, the next three walk on from it ----
+    const bool rest = fabs(z) <= OPS_SAFE_FLOOR;
+    const double rz = h[OPS_HDR_RZ], lnopz = h[OPS_HDR_LNOPZ];
+    const LamLookup& L = a.lk;
+    for (int q = (a.band_j0 >> 2) + tid; 4 * q <= a.band_j1; q += nth) {
+        Q out;
+        int c = -1;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int j = 4 * q + e;
+            T v = (T)0;
+            if (j < nspec) {
+                if (rest) {
+                    v = s1[j];
                 } else {
-                    while (il < nspec - 2 && a.lam[il + 1] <= tg) il++;
+                    const double tg = a.lam[j] * rz;
+                    if (c < 0) {
+                        c = lam_count<false>(L.nspec, L.nbk, L.lnl0, L.bkinv, L.lam, L.bkt, tg, a.loglam[j] - lnopz);
+                    } else {
+                        while (c < nspec && a.lam[c] <= tg) c++;
+                    }
+                    const int il = max(1, min(nspec - 1, c)) - 1;
+                    const double t = (tg - a.lam[il]) * a.rdlam[il];
+                    const T lo = s1[il], hi = s1[il + 1];
+                    v = lo + (T)t * (hi - lo);
                 }
-                double t = (tg - a.lam[il]) / (a.lam[il + 1] - a.lam[il]);
-                v = (double)s1[il] + t * ((double)s1[il + 1] - (double)s1[il]);
+                if (v != v) v = (T)0;
+                if constexpr (F32)
+                    v = fmaxf(v, 0.0f) * a.wl32[j];
+                else
+                    v = fmax(v, 0.0) * a.wl[j];
             }
-            if (v != v) v = 0.0;
-            s0[j] = (T)fmax(v, 0.0) * (T)a.wl[j];
+            out[e] = v;
         }
-    }
-    if (tid == 0) {
-        double dist = 0.0;
-        if (fabs(z) > OPS_SAFE_FLOOR) {
-            int i = max(1, min(find_interval_ref(a.cz, a.ncosmo, z), a.ncosmo - 1));
-            double slope = (a.cdl[i] - a.cdl[i - 1]) / (a.cz[i] - a.cz[i - 1]);
-            double dm = a.cdl[i - 1] + slope * (z - a.cz[i - 1]);
-            if (dm == dm && dm > OPS_SAFE_FLOOR) dist = 5.0 * log10(dm / 10.0) + (-2.5 * log10(1.0 + z));
-        }
-        sh_dist = dist;
+        s0q[q] = out;
     }
     __syncthreads();
 
-    // Band integrals (FP64 sums) and magnitudes
-    const double dist = sh_dist;
-    for (int b = wv; b < a.nbands; b += nwv) {
-        const int nj = a.bnj[b], j0 = a.bjlo[b], off = a.bptr[b];
+    // ---- phase 4: band sums (16 lanes per band, 4 pixels a lane, FP64 across the lanes) -> LDS; magnitudes, a
+    // thread per band ----
+    double* sband = (double*)s1;   // s1 is free now; nbands doubles fit when 8 nbands <= nsp sizeof(T)
+    const bool staged = (size_t)a.nbands * sizeof(double) <= (size_t)a.nsp * sizeof(T);
+    const double dist = h[OPS_HDR_DIST];
+    const int g = tid >> 4, gl = tid & 15, ng = nth >> 4;
+    for (int b = g; b < a.nbands; b += ng) {
+        const int4 bm = a.bmeta[b];   // {coefficient offset, first pixel, pixels}: multiples of 4
+        const bool on = a.mask[b] != 0;
         double s = 0.0;
-        if (a.mask[b])
-            for (int q = lane; q < nj; q += 64) s += (double)coef_at<T>(a, off + q) * (double)s0[j0 + q];
-        s = wave_sum(s);
-        if (lane == 0) {
-            double mag = NAN;
-            if (a.mask[b]) mag = (s > 0.0 ? -2.5 * log10(s) : NAN) - 48.60 - 2.5 * a.amzl + dist;
-            a.mags[(size_t)m * a.nbands + b] = mag;
-            if (a.fband) a.fband[(size_t)m * a.nbands + b] = s;
+        if (on) {
+            if constexpr (F32) {
+                const f32x4_t* cq = (const f32x4_t*)(a.bc32 + bm.x);
+                const f32x4_t* fq = (const f32x4_t*)(s0 + bm.y);
+                float acc = 0.0f;
+                for (int q = gl; 4 * q < bm.z; q += 16) {
+                    const f32x4_t cc = cq[q], ff = fq[q];
+                    acc = fmaf(cc[0], ff[0], acc);
+                    acc = fmaf(cc[1], ff[1], acc);
+                    acc = fmaf(cc[2], ff[2], acc);
+                    acc = fmaf(cc[3], ff[3], acc);
+                }
+                s = (double)acc;
+            } else {
+                const f64x4_t* cq = (const f64x4_t*)(a.bc64 + bm.x);
+                const f64x4_t* fq = (const f64x4_t*)(s0 + bm.y);
+                for (int q = gl; 4 * q < bm.z; q += 16) {
+                    const f64x4_t cc = cq[q], ff = fq[q];
+                    s += cc[0] * ff[0] + cc[1] * ff[1] + cc[2] * ff[2] + cc[3] * ff[3];
+                }
+            }
         }
+        for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+        if (gl == 0) {
+            if (staged) {
+                sband[b] = s;
+            } else {
+                double mag = NAN;
+                if (on) mag = (s > 0.0 ? -2.5 * log10(s) : NAN) - 48.60 - 2.5 * a.amzl + dist;
+                a.mags[(size_t)m * a.nbands + b] = mag;
+                if (a.fband) a.fband[(size_t)m * a.nbands + b] = s;
+            }
+        }
+    }
+    if (!staged) return;
+    __syncthreads();
+    for (int b = tid; b < a.nbands; b += nth) {
+        const double s = sband[b];
+        double mag = NAN;
+        if (a.mask[b]) mag = (s > 0.0 ? -2.5 * log10(s) : NAN) - 48.60 - 2.5 * a.amzl + dist;
+        a.mags[(size_t)m * a.nbands + b] = mag;
+        if (a.fband) a.fband[(size_t)m * a.nbands + b] = s;
     }
 }
 

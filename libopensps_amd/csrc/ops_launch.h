@@ -5,6 +5,28 @@
 
 #include "ops_context.h"
 
+// Per-model constants of the chain, formed once per model by k_prep (FP64): hdr[m * OPS_HDR_N + slot]
+#define OPS_HDR_RZ 0       // 1 / (1 + z)
+#define OPS_HDR_LNOPZ 1    // ln(1 + z)
+#define OPS_HDR_INV2 2     // 1 / (2 sigma_ln^2) of the velocity kernel
+#define OPS_HDR_FBLUE 3    // exp(-4 sigma_ln), exp(+4 sigma_ln): the window edges
+#define OPS_HDR_FRED 4
+#define OPS_HDR_DIST 5     // distance modulus term of the magnitudes
+#define OPS_HDR_TMAX 6     // the IGM optical depth's maximum below Lyman-alpha and its 1-based pixel
+#define OPS_HDR_JMAX 7
+#define OPS_HDR_W0 8       // smoothed pixels [w0, w1): minl <= lambda <= maxl
+#define OPS_HDR_W1 9
+#define OPS_HDR_B6 10      // the six (1 + z) powers of the IGM optical depth
+#define OPS_HDR_N 16
+
+// The wavelength lookup: the ascending grid, and buckets over ln(lambda) no wider than its finest pixel
+struct LamLookup {
+    int nspec, nbk;
+    double lnl0, bkinv;
+    const double* lam;
+    const int* bkt;
+};
+
 struct PrepArgs {
     int n, nt, nz, parts, kpair, kpad, mixed, smooth_fast;
     double tiny_logt;
@@ -12,6 +34,11 @@ struct PrepArgs {
     double *w_out, *mass, *lbol, *w64;
     uint16_t *whi, *wlo;
     int *zlo, *it, *idx, *status, *flag;
+    // the chain's per-model constants
+    int add_igm, jtop, ncosmo;
+    LamLookup lk;
+    const double *igm_a, *cz, *cdl;
+    double* hdr;
 };
 
 struct GroupArgs {
@@ -39,24 +66,22 @@ struct PackArgs {
 };
 
 struct ChainArgs {
-    int n, nspec, nsp, parts, nbands, jtop, add_igm, smooth_fast, band_j0, band_j1, ncosmo;
+    int n, nspec, nsp, parts, nbands, jtop, add_igm, smooth_fast, band_j0, band_j1;
     double amzl;
+    LamLookup lk;
     const double* prm;
+    const double* hdr;
     const void* yo;
-    const double *lam, *loglam, *ltab, *wl, *dlam, *igm_a, *cz, *cdl;
-    const int *bptr, *bjlo, *bnj, *mask;
+    const double *lam, *loglam, *ltab, *wl, *dlam, *rdlam, *igm_a;
+    const float *ltab32, *wl32;
+    const float4* sm32;
+    const int4* bmeta;   // per band {coefficient offset, first pixel, pixels}, multiples of 4
+    const int* mask;
     const double* bc64;
     const float* bc32;
     void *spec, *young, *old;
     double *mags, *fband;
 };
-
-template <typename T>
-__device__ __forceinline__ T coef_at(const ChainArgs& a, int i);
-template <>
-__device__ __forceinline__ double coef_at<double>(const ChainArgs& a, int i) { return a.bc64[i]; }
-template <>
-__device__ __forceinline__ float coef_at<float>(const ChainArgs& a, int i) { return a.bc32[i]; }
 
 __global__ void k_prep(PrepArgs a);
 __global__ void k_group(GroupArgs a);

@@ -146,6 +146,17 @@ class DeviceContext:
         """Enqueues one batch on device-resident prm (NPAR, n) float64 and the buffers of out."""
         n = prm.shape[1]
         assert prm.shape[0] == NPAR and prm.is_contiguous() and prm.dtype.itemsize == 8
+        g = self.grid
+        isz = 4 if precision == MIXED else 8
+        sizes = {"mags": (n * self.nbands, 8), "spec": (n * g.nspec, isz), "young": (n * g.nspec, isz),
+                 "old": (n * g.nspec, isz), "mass": (n, 8), "lbol": (n, 8), "fband": (n * self.nbands, 8),
+                 "idx": (3 * n, 4), "weights": (n * g.nt, 8)}
+        for k, (numel, itemsize) in sizes.items():
+            t = out.get(k)
+            if t is not None and (not t.is_contiguous() or t.element_size() != itemsize or t.numel() < numel):
+                raise OpenSPSHipError(3, f"{k}: {numel} contiguous {itemsize}-byte elements required")
+        if ssp_weights is not None and (ssp_weights.element_size() != 8 or ssp_weights.numel() < n * g.nt):
+            raise OpenSPSHipError(3, "ssp_weights: nt x n float64 required")
         a = BatchArgs()
         a.n, a.precision, a.split_parts = n, precision, int(split_parts)
         a.prm = prm.data_ptr()

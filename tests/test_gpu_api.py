@@ -107,3 +107,33 @@ def test_api_device_resident_batch():
         ctx.compute_csp_batch_device(d_prm, d_mags, d_spec=d_spec)
         np.testing.assert_array_equal(d_mags.cpu().numpy(), r["mags"])
         np.testing.assert_array_equal(d_spec.cpu().numpy().astype(np.float64), r["spec"])
+
+
+@pytest.mark.gpu
+def test_api_device_buffers_checked():
+    """A device buffer the batch would overrun is refused on the host (status 3) before anything is launched: a float
+    spectrum buffer for an FP64 batch, too few magnitudes, a parameter array of the wrong shape."""
+    import torch
+    from libopensps_amd import api
+    g = grid(3)
+    fs = synth.gaussian_filters(5)
+    n = 8
+    prm = synth.parametric_params(n, seed=29)
+    with _ctx(g, fs, "mixed") as ctx:
+        d_prm = torch.as_tensor(prm, device="cuda")
+        d_mags = torch.empty((n, ctx.nbands), dtype=torch.float64, device="cuda")
+        f32 = torch.empty((n, g.nspec), dtype=torch.float32, device="cuda")
+        cases = [dict(d_spec=f32, precision=0),
+                 dict(d_spec=f32.double(), precision=1),
+                 dict(d_spec=f32, d_mass=torch.empty(n - 1, dtype=torch.float64, device="cuda"))]
+        for kw in cases:
+            with pytest.raises(api.OpenSPSError) as e:
+                ctx.compute_csp_batch_device(d_prm, d_mags, **kw)
+            assert e.value.status == 3
+        with pytest.raises(api.OpenSPSError):
+            ctx.compute_csp_batch_device(d_prm, d_mags[:-1])
+        with pytest.raises(api.OpenSPSError):
+            ctx.compute_csp_batch_device(d_prm[:-1].contiguous(), d_mags, d_spec=f32)
+        ctx.compute_csp_batch_device(d_prm, d_mags, d_spec=f32)   # the matching buffers still run
+        torch.cuda.synchronize()
+        assert np.isfinite(d_mags.cpu().numpy()).any()
