@@ -123,6 +123,10 @@ int opensps_hip_create(int device, opensps_hip_ctx_t** out) {
     // OPENSPS_F64_GEMM=vector: the FP64 age sums with vector FMAs instead of the f64 matrix instruction
     const char* e = getenv("OPENSPS_F64_GEMM");
     c->f64_vector = e && strcmp(e, "vector") == 0;
+    // OPENSPS_CHAIN=two: the two-buffer chain for every batch (it otherwise runs only where the band pixels
+    // outgrow the single-buffer chain's registers)
+    const char* ch = getenv("OPENSPS_CHAIN");
+    c->chain_two = ch && strcmp(ch, "two") == 0;
     *out = c;
     return 0;
 }
@@ -526,13 +530,31 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ca.old = a->old;
     ca.mags = a->mags;
     ca.fband = a->fband;
-    size_t lds_chain = 2 * tsz * (size_t)c->nsp;
+    // One LDS spectrum buffer when the band quads fit the registers of the single-buffer chain, else two;
+    // then the band sums (up to 2048 bands) for the magnitudes a thread per band
+    const int chain_threads = 512;
+    const int qbands = c->band_j1 >= c->band_j0 ? (c->band_j1 + 1 - (c->band_j0 & ~3)) / 4 : 0;
+    const bool one = qbands <= OPS_CHAIN_KQ * chain_threads && !c->chain_two;
+    ca.sband_staged = (size_t)c->nbands * sizeof(double) <= 16384;
+    ca.stage = nullptr;
+    if (one && !a->spec) {   // staging rows for smoothing windows wider than the registers hold
+        if ((st = ops_reserve(c, c->spec, tsz * (size_t)n * c->nspec))) return st;
+        ca.stage = c->spec.p;
+    }
+    const size_t lds_chain = (one ? 1 : 2) * tsz * (size_t)c->nsp + (ca.sband_staged ? sizeof(double) * c->nbands : 0);
+    const void* kfn = mixed ? (one ? (const void*)k_chain<float, true> : (const void*)k_chain<float, false>)
+                            : (one ? (const void*)k_chain<double, true> : (const void*)k_chain<double, false>);
+    HIPCHK(c, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_chain));
     if (mixed) {
-        HIPCHK(c, hipFuncSetAttribute((const void*)k_chain<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_chain));
-        k_chain<float><<<n, 512, lds_chain, c->stream>>>(ca);
+        if (one)
+            k_chain<float, true><<<n, chain_threads, lds_chain, c->stream>>>(ca);
+        else
+            k_chain<float, false><<<n, chain_threads, lds_chain, c->stream>>>(ca);
     } else {
-        HIPCHK(c, hipFuncSetAttribute((const void*)k_chain<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_chain));
-        k_chain<double><<<n, 512, lds_chain, c->stream>>>(ca);
+        if (one)
+            k_chain<double, true><<<n, chain_threads, lds_chain, c->stream>>>(ca);
+        else
+            k_chain<double, false><<<n, chain_threads, lds_chain, c->stream>>>(ca);
     }
     HIPCHK(c, hipGetLastError());
     if (c->profiling) {

@@ -62,6 +62,7 @@ struct OpsCtx {
     int smooth_fast = 1;   // smooth_velocity == 1 and smoothspec_fast == 1
     double tiny_logt = 0.0;
     bool f64_vector = false;   // FP64 age sums with vector FMAs (OPENSPS_F64_GEMM=vector)
+    bool chain_two = false;    // the two-buffer chain always (OPENSPS_CHAIN=two)
 
     // ---- batch scratch (grown on demand) ----
     OpsBuffer yo, wcol, w64, order, tiles, misc, spec, flag;

@@ -742,13 +742,57 @@ __device__ __forceinline__ T smooth_px(const ChainArgs& a, const T* __restrict__
     return out;
 }
 
+// The observed-frame quad q (pixels 4q .. 4q + 3, times 1 / lambda; 0 past the grid) from the rest-frame
+// spectrum src: the first pixel finds its interval by the bucket lookup, the next three walk on from it
 template <typename T>
-__global__ __launch_bounds__(512) void k_chain(ChainArgs a) {
+__device__ __forceinline__ typename Quad<T>::type obs_quad(const ChainArgs& a, const T* __restrict__ src, int q,
+                                                           bool rest, double rz, double lnopz) {
+    const int nspec = a.nspec;
+    const LamLookup& L = a.lk;
+    typename Quad<T>::type out;
+    int c = -1;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int j = 4 * q + e;
+        T v = (T)0;
+        if (j < nspec) {
+            if (rest) {
+                v = src[j];
+            } else {
+                const double tg = a.lam[j] * rz;
+                if (c < 0) {
+                    c = lam_count<false>(L.nspec, L.nbk, L.lnl0, L.bkinv, L.lam, L.bkt, tg, a.loglam[j] - lnopz);
+                } else {
+                    while (c < nspec && a.lam[c] <= tg) c++;
+                }
+                const int il = max(1, min(nspec - 1, c)) - 1;
+                const double t = (tg - a.lam[il]) * a.rdlam[il];
+                const T lo = src[il], hi = src[il + 1];
+                v = lo + (T)t * (hi - lo);
+            }
+            if (v != v) v = (T)0;
+            if constexpr (std::is_same<T, float>::value)
+                v = fmaxf(v, 0.0f) * a.wl32[j];
+            else
+                v = fmax(v, 0.0) * a.wl[j];
+        }
+        out[e] = v;
+    }
+    return out;
+}
+
+// ONE: one spectrum buffer in LDS (nsp values): the smoothed values and the observed-frame quads wait in
+// registers for a barrier and then overwrite it (windows wider than OPS_CHAIN_KS pixels a thread go through
+// the spectrum out, or a staging row, in global memory). Otherwise two buffers. The host takes ONE when the
+// band quads fit OPS_CHAIN_KQ a thread.
+template <typename T, bool ONE>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value ? 8 : 6)))
+void k_chain(ChainArgs a) {
     constexpr bool F32 = std::is_same<T, float>::value;
     typedef typename Quad<T>::type Q;
     extern __shared__ __align__(32) unsigned char lds_raw[];
     T* s0 = (T*)lds_raw;
-    T* s1 = s0 + a.nsp;
+    T* s1 = ONE ? s0 : s0 + a.nsp;
     Q* s0q = (Q*)s0;
     Q* s1q = (Q*)s1;
     const int m = blockIdx.x, n = a.n, nspec = a.nspec, nq = a.nsp >> 2;
@@ -833,77 +877,89 @@ __global__ __launch_bounds__(512) void k_chain(ChainArgs a) {
     }
     __syncthreads();
 
-    // ---- phase 2: velocity smoothing -> s1, and the spectrum out. Quads outside the smoothed range [w0, w1)
-    // copy; the range's pixels are spread over the threads one by one ----
+    // ---- phase 2: velocity smoothing (-> s1, or in place with ONE), and the spectrum out. Pixels outside the
+    // smoothed range [w0, w1) keep their values; the range's pixels are spread over the threads one by one ----
     const double inv2 = h[OPS_HDR_INV2], fblue = h[OPS_HDR_FBLUE], fred = h[OPS_HDR_FRED];
     const int w0 = (int)h[OPS_HDR_W0], w1 = (int)h[OPS_HDR_W1];
     T* spec_out = a.spec ? (T*)a.spec + (size_t)m * nspec : nullptr;
     for (int q = tid; q < nq; q += nth) {
         const int i0 = 4 * q;
-        const Q v = s0q[q];
-        if (i0 + 3 < w0 || i0 >= w1) {
-            s1q[q] = v;
-        } else {
+        const bool outside = i0 + 3 < w0 || i0 >= w1;
+        if (!ONE || spec_out) {
+            const Q v = s0q[q];
+            if constexpr (!ONE) {
+                if (outside) {
+                    s1q[q] = v;
+                } else {
 #pragma unroll
-            for (int e = 0; e < 4; e++)
-                if (i0 + e < w0 || i0 + e >= w1) s1[i0 + e] = v[e];
-        }
-        if (spec_out) {
+                    for (int e = 0; e < 4; e++)
+                        if (i0 + e < w0 || i0 + e >= w1) s1[i0 + e] = v[e];
+                }
+            }
+            if (spec_out) {
 #pragma unroll
-            for (int e = 0; e < 4; e++)
-                if (i0 + e < nspec && (i0 + e < w0 || i0 + e >= w1)) spec_out[i0 + e] = v[e];
+                for (int e = 0; e < 4; e++)
+                    if (i0 + e < nspec && (i0 + e < w0 || i0 + e >= w1)) spec_out[i0 + e] = v[e];
+            }
         }
     }
-    for (int i = w0 + tid; i < w1; i += nth) {
-        const T out = smooth_px<T>(a, s0, i, inv2, fblue, fred, s0[i]);
-        s1[i] = out;
-        if (spec_out) spec_out[i] = out;
+    if constexpr (ONE) {
+        if (w1 - w0 <= OPS_CHAIN_KS * nth) {
+            T sv[OPS_CHAIN_KS];
+#pragma unroll
+            for (int k = 0; k < OPS_CHAIN_KS; k++) {
+                const int i = w0 + tid + k * nth;
+                sv[k] = i < w1 ? smooth_px<T>(a, s0, i, inv2, fblue, fred, s0[i]) : (T)0;
+                if (i < w1 && spec_out) spec_out[i] = sv[k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < OPS_CHAIN_KS; k++) {
+                const int i = w0 + tid + k * nth;
+                if (i < w1) s0[i] = sv[k];
+            }
+        } else {
+            T* G = spec_out ? spec_out : (T*)a.stage + (size_t)m * nspec;
+            for (int i = w0 + tid; i < w1; i += nth) G[i] = smooth_px<T>(a, s0, i, inv2, fblue, fred, s0[i]);
+            __syncthreads();
+            for (int i = w0 + tid; i < w1; i += nth) s0[i] = G[i];
+        }
+    } else {
+        for (int i = w0 + tid; i < w1; i += nth) {
+            const T out = smooth_px<T>(a, s0, i, inv2, fblue, fred, s0[i]);
+            s1[i] = out;
+            if (spec_out) spec_out[i] = out;
+        }
     }
     __syncthreads();
 
-    // ---- phase 3: the observed frame over the band quads -> s0 (times 1 / lambda; 0 past the grid). The first
-    // pixel of a quad finds its interval by the bucket lookup, the next three walk on from it ----
+    // ---- phase 3: the observed frame over the band quads (from s1) -> s0 ----
     const bool rest = fabs(z) <= OPS_SAFE_FLOOR;
     const double rz = h[OPS_HDR_RZ], lnopz = h[OPS_HDR_LNOPZ];
-    const LamLookup& L = a.lk;
-    for (int q = (a.band_j0 >> 2) + tid; 4 * q <= a.band_j1; q += nth) {
-        Q out;
-        int c = -1;
+    const int qb0 = a.band_j0 >> 2, qb1 = (a.band_j1 + 1) >> 2;   // band quads [qb0, qb1)
+    if constexpr (ONE) {
+        Q fo[OPS_CHAIN_KQ];
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int j = 4 * q + e;
-            T v = (T)0;
-            if (j < nspec) {
-                if (rest) {
-                    v = s1[j];
-                } else {
-                    const double tg = a.lam[j] * rz;
-                    if (c < 0) {
-                        c = lam_count<false>(L.nspec, L.nbk, L.lnl0, L.bkinv, L.lam, L.bkt, tg, a.loglam[j] - lnopz);
-                    } else {
-                        while (c < nspec && a.lam[c] <= tg) c++;
-                    }
-                    const int il = max(1, min(nspec - 1, c)) - 1;
-                    const double t = (tg - a.lam[il]) * a.rdlam[il];
-                    const T lo = s1[il], hi = s1[il + 1];
-                    v = lo + (T)t * (hi - lo);
-                }
-                if (v != v) v = (T)0;
-                if constexpr (F32)
-                    v = fmaxf(v, 0.0f) * a.wl32[j];
-                else
-                    v = fmax(v, 0.0) * a.wl[j];
-            }
-            out[e] = v;
+        for (int k = 0; k < OPS_CHAIN_KQ; k++) {
+            const int q = qb0 + tid + k * nth;
+            if (q < qb1) fo[k] = obs_quad<T>(a, s1, q, rest, rz, lnopz);
         }
-        s0q[q] = out;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < OPS_CHAIN_KQ; k++) {
+            const int q = qb0 + tid + k * nth;
+            if (q < qb1) s0q[q] = fo[k];
+        }
+    } else {
+        for (int q = qb0 + tid; q < qb1; q += nth) s0q[q] = obs_quad<T>(a, s1, q, rest, rz, lnopz);
     }
     __syncthreads();
 
     // ---- phase 4: band sums (16 lanes per band, 4 pixels a lane, FP64 across the lanes) -> LDS; magnitudes, a
     // thread per band ----
-    double* sband = (double*)s1;   // s1 is free now; nbands doubles fit when 8 nbands <= nsp sizeof(T)
-    const bool staged = (size_t)a.nbands * sizeof(double) <= (size_t)a.nsp * sizeof(T);
+    // the sums wait in LDS after the spectrum (sband_staged: the host sized the LDS for them)
+    double* sband = (double*)(lds_raw + (ONE ? 1 : 2) * (size_t)a.nsp * sizeof(T));
+    const bool staged = a.sband_staged != 0;
     const double dist = h[OPS_HDR_DIST];
     const int g = tid >> 4, gl = tid & 15, ng = nth >> 4;
     for (int b = g; b < a.nbands; b += ng) {
@@ -955,8 +1011,10 @@ __global__ __launch_bounds__(512) void k_chain(ChainArgs a) {
     }
 }
 
-template __global__ void k_chain<double>(ChainArgs);
-template __global__ void k_chain<float>(ChainArgs);
+template __global__ void k_chain<double, false>(ChainArgs);
+template __global__ void k_chain<float, false>(ChainArgs);
+template __global__ void k_chain<double, true>(ChainArgs);
+template __global__ void k_chain<float, true>(ChainArgs);
 
 // ------------------------------------------------------------------------------------------------
 // Kernel-level entry points: band_fluxes and observed_frame_z in the reference's own layouts

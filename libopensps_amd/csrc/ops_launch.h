@@ -66,7 +66,7 @@ struct PackArgs {
 };
 
 struct ChainArgs {
-    int n, nspec, nsp, parts, nbands, jtop, add_igm, smooth_fast, band_j0, band_j1;
+    int n, nspec, nsp, parts, nbands, jtop, add_igm, smooth_fast, band_j0, band_j1, sband_staged;
     double amzl;
     LamLookup lk;
     const double* prm;
@@ -80,6 +80,7 @@ struct ChainArgs {
     const double* bc64;
     const float* bc32;
     void *spec, *young, *old;
+    void* stage;   // n x nspec staging rows of wide smoothing windows without a spectrum out (single-buffer chain)
     double *mags, *fband;
 };
 
@@ -90,7 +91,9 @@ __global__ void k_gemm_f64_vec(GemmArgs a);
 __global__ void k_gemm_bf16x3(GemmArgs a);
 __global__ void k_pack_f64(PackArgs a);
 __global__ void k_pack_bf16(PackArgs a);
-template <typename T>
+#define OPS_CHAIN_KS 4   // smoothed pixels a thread holds in registers (single-buffer chain)
+#define OPS_CHAIN_KQ 3   // observed-frame quads a thread holds in registers (single-buffer chain)
+template <typename T, bool ONE>
 __global__ void k_chain(ChainArgs a);
 template <typename T>
 __global__ void k_band_fluxes(int n, int nspec, int nbands, const T* bt, const int* jlo, const int* jhi, const int* do_band,

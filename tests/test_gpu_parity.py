@@ -273,3 +273,36 @@ def test_observed_frame_reference_fixture(precision):
         ref = O.observed_frame(lam, zc[m], tot[m].astype(np.float64))
         tol = 1e-15 if precision == 0 else 4.0 * np.finfo(np.float32).eps
         np.testing.assert_allclose(f[:, m], ref, rtol=tol, atol=0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("with_spec", [True, False])
+def test_wide_smoothing_window(precision, with_spec):
+    """Smoothing over the whole grid (min_wave_smooth 50 A, max_wave_smooth 2e8 A): more smoothed pixels than the
+    single-buffer chain holds in registers, so they pass through the spectrum out, or through the staging rows
+    when no spectrum is asked for."""
+    g, b, prm = _parametric_case(48, seed=31)
+    prm[SLOT["min_wave_smooth"]] = 50.0
+    prm[SLOT["max_wave_smooth"]] = 2.0e8
+    want = ("mags", "spec", "mass", "lbol", "idx") if with_spec else ("mags", "mass", "lbol", "idx")
+    ref, got = _run_both(g, b, prm, precision, want=want)
+    fp64 = precision == 0
+    em = mag_err(got["mags"], ref["mags"])
+    print(f"wide window: max mag err {em:.3e}")
+    assert em < (FP64_MAG if fp64 else MIX_MAG)
+    if with_spec:
+        es = spec_rel_err(got["spec"], ref["spec"])
+        print(f"wide window: max spectral rel err {es:.3e}")
+        assert es < (FP64_SPEC if fp64 else MIX_SPEC)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_two_buffer_chain(precision, monkeypatch):
+    """The two-buffer chain (taken where the band pixels outgrow the single-buffer chain's registers), forced on the
+    parametric batch."""
+    monkeypatch.setenv("OPENSPS_CHAIN", "two")
+    g, b, prm = _parametric_case(64, seed=37)
+    ref, got = _run_both(g, b, prm, precision)
+    _check(ref, got, precision)
