@@ -27,6 +27,8 @@ BF16_PEAK_TFLOPS = 2500.0
 FP64_PEAK_TFLOPS = 78.6
 KERNELS = ["prep", "group", "age_sums", "chain"]
 MAG_RANGE = 40.0
+CHAIN_KERNEL = "k_chain<float, true>"
+TRAFFIC_REPORT = "k_chain_f32_parametric16k_traffic.md"
 
 
 def parse():
@@ -37,6 +39,8 @@ def parse():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="end with a 256 MiB device-to-device copy (the calibration of the HBM counter runs)")
     return ap.parse_args()
 
 
@@ -142,9 +146,14 @@ def main():
         # read the parameters (28 doubles)
         byts = n * (ns * 4 + ns * 4 + nb * 8 + 28 * 8)
         achieved = byts / (kms_mix[dom] * 1e-3) / 1e9
+        kname = CHAIN_KERNEL if name == "chain" else name
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_chain<float>" if name == "chain" else name,
-                "algorithmic_per_launch": byts}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kname, "algorithmic_per_launch": byts}
+        from libopensps_amd.profiling import read_traffic
+        tr = read_traffic(os.path.join(ROOT, "profiles", TRAFFIC_REPORT))
+        if tr and kname in tr[0]:
+            roof["traffic"] = tr[1]
+            roof["traffic_source"] = "profiles/" + TRAFFIC_REPORT
     roof["kernel_ms"] = dict(zip(KERNELS, [round(x, 4) for x in kms_mix]))
     roof["kernel_ms_fp64"] = dict(zip(KERNELS, [round(x, 4) for x in kms_64]))
 
@@ -206,6 +215,12 @@ def main():
         "setup_s": t_setup,
     }
     print(json.dumps(out))
+    if args.copy_probe:
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(2):
+            y.copy_(x)
+        torch.cuda.synchronize()
     ctx.close()
 
 

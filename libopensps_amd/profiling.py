@@ -1,0 +1,137 @@
+"""Summaries of rocprofv3 output for the batched composite-population kernels.
+
+Two reports, written as short Markdown files:
+
+* ``stats``: the per-kernel table of a ``rocprofv3 --kernel-trace --stats`` run (its SQLite results file):
+  calls, total and average duration.
+* ``traffic``: the HBM bytes per launch of one kernel from two counter runs, ``--pmc FETCH_SIZE`` and
+  ``--pmc WRITE_SIZE`` (CSV output, KiB per dispatch). Each counter is scaled by what it reads for a plain
+  device-to-device copy of known size in the same run (the last two ``__amd_rocclr_copyBuffer`` dispatches,
+  which ``bench.py --copy-probe`` issues after everything else): factor = bytes the copy moves / bytes the counter reports.
+
+The traffic report ends in a line ``traffic_bytes_per_launch: <int>`` that ``bench.py`` reads for the roofline's
+``traffic`` field when the kernel matches.
+
+    python -m libopensps_amd.profiling stats --db RUN_results.db --out profiles/X.md --command "..."
+    python -m libopensps_amd.profiling traffic --fetch F.csv --write W.csv --kernel 'k_chain<float, true>' \\
+        --copy-bytes 268435456 --out profiles/Y.md --command "..."
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import sqlite3
+import statistics
+
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def kernel_stats(db_path):
+    """[(name, calls, total_us, avg_us, percent)] from the results database of a --kernel-trace --stats run."""
+    con = sqlite3.connect(db_path)
+    try:
+        rows = con.execute("select name, total_calls, total_duration, average, percentage from top_kernels "
+                           "order by total_duration desc").fetchall()
+    finally:
+        con.close()
+    return rows
+
+
+def pmc_values(csv_path, counter):
+    """[(dispatch id, kernel name, grid size, value)] of one counter from a --pmc CSV."""
+    out = []
+    with open(csv_path, newline="") as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                out.append((int(r["Dispatch_Id"]), r["Kernel_Name"], int(r["Grid_Size"]), float(r["Counter_Value"])))
+    return sorted(out)
+
+
+def _copy_factor(vals, copy_bytes, nprobe=2):
+    """bytes moved / counter bytes for the probe: the last nprobe copy dispatches of the run."""
+    copies = [v for v in vals if v[1].startswith(COPY_KERNEL)]
+    if len(copies) < nprobe:
+        raise ValueError("no copy probe in the counter run (run bench.py with --copy-probe)")
+    kib = [v[3] for v in copies[-nprobe:]]
+    return copy_bytes / (1024.0 * statistics.median(kib)), len(kib)
+
+
+def traffic(fetch_csv, write_csv, kernel, copy_bytes):
+    """HBM bytes per launch of kernel (exact name prefix) from the FETCH_SIZE and WRITE_SIZE runs."""
+    out = {"kernel": kernel, "copy_bytes": copy_bytes}
+    for key, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"), ("write", write_csv, "WRITE_SIZE")):
+        vals = pmc_values(path, counter)
+        fac, ncopy = _copy_factor(vals, copy_bytes)
+        kib = [v[3] for v in vals if v[1].startswith(kernel)]
+        if not kib:
+            raise ValueError(f"no dispatch of {kernel} in {path}")
+        out[key + "_factor"] = fac
+        out[key + "_copy_dispatches"] = ncopy
+        out[key + "_kib_median"] = statistics.median(kib)
+        out[key + "_dispatches"] = len(kib)
+        out[key + "_bytes"] = fac * 1024.0 * statistics.median(kib)
+    out["bytes"] = out["fetch_bytes"] + out["write_bytes"]
+    return out
+
+
+def _write_stats(args):
+    rows = kernel_stats(args.db)
+    lines = [f"# {args.title}", "", "Command:", "", "```", args.command, "```", "",
+             "| kernel | calls | total (us) | average (us) | % |", "|---|---|---|---|---|"]
+    for name, calls, tot, avg, pct in rows[:args.top]:
+        lines.append(f"| `{name[:90]}` | {calls} | {tot:.1f} | {avg:.1f} | {pct:.1f} |")
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def _write_traffic(args):
+    t = traffic(args.fetch, args.write, args.kernel, args.copy_bytes)
+    lines = [f"# {args.title}", "", "Commands (one counter per run):", "", "```", args.command, "```", "",
+             f"Kernel `{t['kernel']}`: median over {t['fetch_dispatches']} dispatches.", "",
+             "| counter | KiB per dispatch | copy-probe factor | bytes per launch |", "|---|---|---|---|",
+             f"| FETCH_SIZE | {t['fetch_kib_median']:.1f} | {t['fetch_factor']:.4f} | {t['fetch_bytes']:.4e} |",
+             f"| WRITE_SIZE | {t['write_kib_median']:.1f} | {t['write_factor']:.4f} | {t['write_bytes']:.4e} |", "",
+             f"Copy probe: a {args.copy_bytes} B device-to-device copy (`{COPY_KERNEL}`); factor = bytes moved / "
+             "counter bytes.", "",
+             f"traffic_kernel: {t['kernel']}",
+             f"traffic_bytes_per_launch: {int(round(t['bytes']))}"]
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def read_traffic(path):
+    """(kernel, bytes per launch) from a traffic report, or None."""
+    try:
+        with open(path) as f:
+            kv = dict(line.strip().split(": ", 1) for line in f if line.startswith("traffic_"))
+        return kv["traffic_kernel"], int(kv["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("--db", required=True)
+    s.add_argument("--out", required=True)
+    s.add_argument("--title", default="rocprofv3 kernel statistics")
+    s.add_argument("--command", required=True)
+    s.add_argument("--top", type=int, default=12)
+    t = sub.add_parser("traffic")
+    t.add_argument("--fetch", required=True)
+    t.add_argument("--write", required=True)
+    t.add_argument("--kernel", required=True)
+    t.add_argument("--copy-bytes", type=int, default=256 << 20)
+    t.add_argument("--out", required=True)
+    t.add_argument("--title", default="HBM traffic per launch")
+    t.add_argument("--command", required=True)
+    args = ap.parse_args(argv)
+    if args.cmd == "stats":
+        _write_stats(args)
+    else:
+        _write_traffic(args)
+
+
+if __name__ == "__main__":
+    main()
