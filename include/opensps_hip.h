@@ -40,6 +40,8 @@ typedef struct {
     int precision;               /* OPENSPS_HIP_FP64 or OPENSPS_HIP_MIXED */
     int split_parts;             /* 1: young and old age sums apart (needed by a birth-cloud or old-star
                                     screen, or for young / old); 0: one sum per model */
+    int use_ops;                 /* 1: the age sums from the SSP operator with each model's IMF
+                                    (opensps_hip_set_ssp_ops; compute_csp_batch's ssp_ops argument) */
     const double *prm;           /* device (OPS_NPAR, n): prm[slot * n + m] */
     const double *ssp_weights;   /* device (nt, n) or NULL: SSP weights instead of the SFH parameters */
     double *mags;                /* device (nbands, n) AB magnitudes, NaN for masked bands */
@@ -49,6 +51,8 @@ typedef struct {
     void *young, *old;           /* device (nspec, n): the age sums before dust (split_parts = 1), or NULL */
     int *idx;                    /* device (3, n): it_tesc, zlo (1-based), per-model status; or NULL */
     double *weights;             /* device (nt, n): the FP64 SFH weights; or NULL */
+    int *op_count;               /* device (2, n): with the operator, the points of each metallicity pass with a
+                                    nonzero IMF weight (the isochrone points the IMF limits select); or NULL */
 } opensps_hip_batch_t;
 
 /* ---- context ---- */
@@ -84,10 +88,40 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t *ctx, const opensps_hip_batch_t *arg
 /* Waits for the context's stream; status 2 if a model of the last batch was unsupported (idx row 3). */
 int opensps_hip_sync(opensps_hip_ctx_t *ctx);
 
+/* ---- the SSP operator with the IMF per model ---- */
+/* A caller-built operator set: the isochrone points of every (metallicity, age), host arrays. Replaces the
+ * tables of ssp_opset_t (src/physics/opensps_ssp_batch.f90:99-147) that the reference builds from isochrones
+ * (opensps_context_build_ssp_ops, include/opensps.h:430): here one template library serves every metallicity.
+ * Points of (iz, t) are [age_ptr[iz nt + t], age_ptr[iz nt + t + 1]), sorted by initial mass within an age;
+ * m1, m2: the mass bin (first = 1: the first bin of an age, whose lower edge is the IMF's lower bound); cm, cl,
+ * q: surviving mass, L_bol and ionizing photons per star; age_mmax: the largest living initial mass of each
+ * isochrone (the remnants); each point's nent entries (template, value; template -1: none); etpl: the
+ * library (ntpl, nspec) on the context's grid. nz and nt must be the grid's. */
+typedef struct {
+    int nz, nt, npts, ntpl, nent;
+    const int *age_ptr;          /* nz nt + 1 */
+    const double *mini, *m1, *m2;
+    const int *first;
+    const double *cm, *cl, *q;
+    const double *age_mmax;      /* nz nt */
+    const int *ent_tpl;          /* npts nent */
+    const double *ent_val;       /* npts nent */
+    const double *etpl;          /* ntpl nspec: template e's spectrum at etpl[e nspec + j] */
+} opensps_hip_ssp_ops_t;
+int opensps_hip_set_ssp_ops(opensps_hip_ctx_t *ctx, const opensps_hip_ssp_ops_t *ops);
+/* opensps_context_free_ssp_ops (include/opensps.h:445) */
+int opensps_hip_free_ssp_ops(opensps_hip_ctx_t *ctx);
+/* The context's IMF settings for the operator (imf_type: 0 Salpeter, 2 Kroupa, 4 Dave; add_stellar_remnants;
+ * salp_ind, mdave; the black-hole and neutron-star progenitor limits mlim_bh, mlim_ns). Per-model slopes and
+ * limits come from the slots imf1..imf3, imf_upper_limit, imf_lower_limit of the batch. */
+int opensps_hip_set_imf(opensps_hip_ctx_t *ctx, int imf_type, int remnants, double salp_ind, double mdave,
+                        double mlim_bh, double mlim_ns);
+
 /* Per-kernel timing with HIP events on the context's stream (no reference counterpart; the reference times its
  * stages with rocprof / nsys traces, doc/GPU_RESULTS.md:210-224): on != 0 records events around each kernel of
  * every batch; opensps_hip_kernel_times returns the accumulated milliseconds and launch counts of [prep, group,
- * age-sum product, chain] (ms[4], launches[4]; either may be NULL) and, with reset != 0, clears them. */
+ * age-sum product, chain] (ms[4], launches[4]; either may be NULL) and, with reset != 0, clears them. With the
+ * operator, "group" also holds the IMF weights, the fold and the mass sums, and the product is the template one. */
 int opensps_hip_set_profiling(opensps_hip_ctx_t *ctx, int on);
 int opensps_hip_kernel_times(opensps_hip_ctx_t *ctx, double *ms, int *launches, int reset);
 

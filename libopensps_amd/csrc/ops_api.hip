@@ -6,6 +6,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -131,6 +132,8 @@ int opensps_hip_create(int device, opensps_hip_ctx_t** out) {
     return 0;
 }
 
+static void free_operator(OpsCtx* c);
+
 static void free_ptr(void* p) {
     if (p) (void)hipFree(p);
 }
@@ -141,8 +144,10 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
     void* ps[] = {c->d_lam, c->d_loglam, c->d_ltab, c->d_wl, c->d_dlam, c->d_time, c->d_logz, c->d_bmass, c->d_blbol,
                   c->d_igm_a, c->d_basis64, c->d_A64, c->d_Ahi, c->d_Alo, c->d_bmeta, c->d_bc64,
                   c->d_bc32, c->d_mask, c->d_cz, c->d_cdl, c->yo.p, c->wcol.p, c->w64.p, c->order.p, c->tiles.p,
-                  c->misc.p, c->spec.p, c->flag.p, c->d_rdlam, c->d_ltab32, c->d_wl32, c->d_sm32, c->d_bkt, c->hdr.p};
+                  c->misc.p, c->spec.p, c->flag.p, c->d_rdlam, c->d_ltab32, c->d_wl32, c->d_sm32, c->d_bkt, c->hdr.p, c->wsfh.p,
+                  c->opu.p, c->opwc.p, c->opsum.p};
     for (void* p : ps) free_ptr(p);
+    free_operator(c);
     for (int k = 0; k <= OpsCtx::NPROF; k++)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -352,6 +357,164 @@ int opensps_hip_set_mag_mask(opensps_hip_ctx_t* c, int nbands, const int* mask) 
     return 0;
 }
 
+static void free_operator(OpsCtx* c) {
+    void* ps[] = {c->op.d_age_ptr, c->op.d_zoff, c->op.d_first, c->op.d_tz_ptr, c->op.d_pt, c->op.d_age_mmax,
+                  c->op.d_te, c->op.d_te_val64, c->op.d_te_val32, c->op.d_A64, c->op.d_Ahi, c->op.d_Alo};
+    for (void* p : ps) free_ptr(p);
+    c->op = OpsCtx::OpsOperator();
+}
+
+// imf_bin_logs (src/physics/opensps_imf.f90:564-579): ln m1 and ln(m2 / m1) in Kahan's log1p form
+static void bin_logs(double m1, double m2, double* llo, double* dl) {
+    *llo = 0.0;
+    *dl = 0.0;
+    if (m1 <= 0.0 || m2 <= m1) return;
+    *llo = log(m1);
+    const double x = (m2 - m1) / m1, u = 1.0 + x;
+    *dl = (u == 1.0) ? x : log(u) * x / (u - 1.0);
+}
+
+int opensps_hip_set_ssp_ops(opensps_hip_ctx_t* c, const opensps_hip_ssp_ops_t* o) {
+    if (!c) return 1;
+    if (c->nspec < 2) return ops_fail(c, 1, "set_ssp_ops: no grid installed");
+    if (!o || !o->age_ptr || !o->mini || !o->m1 || !o->m2 || !o->first || !o->cm || !o->cl || !o->q || !o->age_mmax ||
+        !o->ent_tpl || !o->ent_val || !o->etpl)
+        return ops_fail(c, 3, "set_ssp_ops: NULL arrays");
+    if (o->nz != c->nz || o->nt != c->nt)
+        return ops_fail(c, 3, "set_ssp_ops: %d metallicities x %d ages, the grid has %d x %d", o->nz, o->nt, c->nz, c->nt);
+    if (o->npts < 1 || o->ntpl < 1 || o->nent < 1) return ops_fail(c, 3, "set_ssp_ops: empty operator");
+    const int nz = o->nz, nt = o->nt, np = o->npts, ntpl = o->ntpl, ne = o->nent, nspec = c->nspec;
+    if (o->age_ptr[0] != 0 || o->age_ptr[nz * nt] != np) return ops_fail(c, 3, "set_ssp_ops: age_ptr must span 0 .. npts");
+    for (int a = 0; a < nz * nt; a++)
+        if (o->age_ptr[a + 1] < o->age_ptr[a]) return ops_fail(c, 3, "set_ssp_ops: age_ptr decreases at %d", a);
+    for (size_t k = 0; k < (size_t)np * ne; k++)
+        if (o->ent_tpl[k] < -1 || o->ent_tpl[k] >= ntpl) return ops_fail(c, 3, "set_ssp_ops: template index out of range");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_operator(c);
+    // points: the bins' logarithms formed here once (model-independent), OPS_PT_N doubles each
+    std::vector<double> pt((size_t)np * OPS_PT_N);
+    for (int k = 0; k < np; k++) {
+        double* q = &pt[(size_t)k * OPS_PT_N];
+        q[0] = o->mini[k];
+        q[1] = o->m1[k];
+        q[2] = o->m2[k];
+        bin_logs(o->m1[k], o->m2[k], &q[3], &q[4]);
+        q[5] = o->cm[k];
+        q[6] = o->cl[k];
+        q[7] = o->q[k];
+    }
+    std::vector<int> zoff(nz + 1);
+    int npz = 1;
+    for (int iz = 0; iz <= nz; iz++) zoff[iz] = o->age_ptr[iz * nt];
+    for (int iz = 0; iz < nz; iz++) npz = std::max(npz, zoff[iz + 1] - zoff[iz]);
+    // entries by template per metallicity (CSR), in point order within a template, so sorted by age
+    std::vector<int> tz_ptr((size_t)nz * (ntpl + 1), 0);
+    for (int iz = 0; iz < nz; iz++)
+        for (int k = zoff[iz]; k < zoff[iz + 1]; k++)
+            for (int j = 0; j < ne; j++) {
+                const int e = o->ent_tpl[(size_t)k * ne + j];
+                if (e >= 0) tz_ptr[(size_t)iz * (ntpl + 1) + e + 1]++;
+            }
+    int tot = 0;
+    for (int iz = 0; iz < nz; iz++) {
+        int* tp = &tz_ptr[(size_t)iz * (ntpl + 1)];
+        tp[0] = tot;
+        for (int e = 0; e < ntpl; e++) tp[e + 1] += tp[e];
+        tot = tp[ntpl];
+    }
+    const int nentries = tot;
+    std::vector<int2> te(nentries > 0 ? nentries : 1);
+    std::vector<double> tv64(nentries > 0 ? nentries : 1);
+    std::vector<float> tv32(nentries > 0 ? nentries : 1);
+    std::vector<int> fill(tz_ptr);
+    for (int iz = 0; iz < nz; iz++)
+        for (int t = 0; t < nt; t++)
+            for (int k = o->age_ptr[iz * nt + t]; k < o->age_ptr[iz * nt + t + 1]; k++)
+                for (int j = 0; j < ne; j++) {
+                    const int e = o->ent_tpl[(size_t)k * ne + j];
+                    if (e < 0) continue;
+                    const int at = fill[(size_t)iz * (ntpl + 1) + e]++;
+                    te[at] = make_int2(k - zoff[iz], t);
+                    tv64[at] = o->ent_val[(size_t)k * ne + j];
+                    tv32[at] = (float)tv64[at];
+                }
+    c->op.nz = nz;
+    c->op.nt = nt;
+    c->op.npts = np;
+    c->op.ntpl = ntpl;
+    c->op.npz = npz;
+    // K padded to whole blocks of the fold (8 templates; a multiple of the f64 instruction's 4 and of 32 for bf16)
+    c->op.k64 = ((ntpl + OPS_TPL_BLOCK - 1) / OPS_TPL_BLOCK) * OPS_TPL_BLOCK;
+    c->op.kbf = ((ntpl + OPS_KSTEP_BF16 - 1) / OPS_KSTEP_BF16) * OPS_KSTEP_BF16;
+    UPLOAD(c, c->op.d_age_ptr, o->age_ptr, (size_t)nz * nt + 1);
+    UPLOAD(c, c->op.d_zoff, zoff.data(), zoff.size());
+    UPLOAD(c, c->op.d_first, o->first, (size_t)np);
+    UPLOAD(c, c->op.d_pt, pt.data(), pt.size());
+    UPLOAD(c, c->op.d_age_mmax, o->age_mmax, (size_t)nz * nt);
+    UPLOAD(c, c->op.d_tz_ptr, tz_ptr.data(), tz_ptr.size());
+    UPLOAD(c, c->op.d_te, te.data(), te.size());
+    UPLOAD(c, c->op.d_te_val64, tv64.data(), tv64.size());
+    UPLOAD(c, c->op.d_te_val32, tv32.data(), tv32.size());
+    // the library as the A operand of the template product, packed like a basis of one slab and ntpl "ages"
+    double* lib = nullptr;
+    HIPCHK(c, hipMalloc((void**)&lib, sizeof(double) * (size_t)ntpl * nspec));
+    HIPCHK(c, hipMemcpy(lib, o->etpl, sizeof(double) * (size_t)ntpl * nspec, hipMemcpyHostToDevice));
+    const int npxblk = c->nsp / 16;
+    PackArgs pa;
+    pa.nspec = nspec;
+    pa.nt = ntpl;
+    pa.nz = 1;
+    pa.npair = 1;
+    pa.kpair = ntpl;
+    pa.npxblk = npxblk;
+    pa.basis = lib;
+    {
+        size_t n64 = (size_t)npxblk * (c->op.k64 / OPS_KSTEP_F64) * 64;
+        HIPCHK(c, hipMalloc((void**)&c->op.d_A64, n64 * sizeof(double)));
+        pa.kpad = c->op.k64;
+        pa.A64 = c->op.d_A64;
+        k_pack_f64<<<(unsigned)((n64 + 255) / 256), 256, 0, c->stream>>>(pa);
+        HIPCHK(c, hipGetLastError());
+    }
+    {
+        size_t nbf = (size_t)npxblk * (c->op.kbf / OPS_KSTEP_BF16) * 64;
+        HIPCHK(c, hipMalloc((void**)&c->op.d_Ahi, nbf * 8 * sizeof(uint16_t)));
+        HIPCHK(c, hipMalloc((void**)&c->op.d_Alo, nbf * 8 * sizeof(uint16_t)));
+        pa.kpad = c->op.kbf;
+        pa.Ahi = c->op.d_Ahi;
+        pa.Alo = c->op.d_Alo;
+        k_pack_bf16<<<(unsigned)((nbf + 255) / 256), 256, 0, c->stream>>>(pa);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(lib));
+    c->op.on = true;
+    return 0;
+}
+
+int opensps_hip_free_ssp_ops(opensps_hip_ctx_t* c) {
+    if (!c) return 1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_operator(c);
+    return 0;
+}
+
+int opensps_hip_set_imf(opensps_hip_ctx_t* c, int imf_type, int remnants, double salp_ind, double mdave,
+                        double mlim_bh, double mlim_ns) {
+    if (!c) return 1;
+    const int t = imf_type % 10;
+    if (imf_type < 0 || (t != 0 && t != 2 && t != 4))
+        return ops_fail(c, 2, "set_imf: IMF type %d is not a power law this build integrates (0, 2, 4)", imf_type);
+    c->imf_type = imf_type;
+    c->remnants = remnants != 0;
+    c->salp_ind = salp_ind;
+    c->mdave = mdave;
+    c->mlim_bh = mlim_bh;
+    c->mlim_ns = mlim_ns;
+    return 0;
+}
+
 int opensps_hip_set_cosmology(opensps_hip_ctx_t* c, int n, const double* z, const double* dl) {
     if (!c) return 1;
     if (n < 2 || !z || !dl) return ops_fail(c, 3, "set_cosmology: bad table");
@@ -383,16 +546,22 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     const int parts = a->split_parts ? 2 : 1;
     if ((a->young || a->old) && parts != 2) return ops_fail(c, 3, "batch: young / old need split_parts = 1");
     const bool mixed = a->precision == OPENSPS_HIP_MIXED;
-    const int kpad = mixed ? c->kbf : c->k64;
+    const bool ops = a->use_ops != 0;
+    if (ops && !c->op.on) return ops_fail(c, 1, "batch: no SSP operator installed (opensps_hip_set_ssp_ops)");
+    const int kpad = ops ? (mixed ? c->op.kbf : c->op.k64) : (mixed ? c->kbf : c->k64);
     const int npxblk = c->nsp / 16, npxt = c->nsp / OPS_TILE_PX;
     const size_t tsz = mixed ? sizeof(float) : sizeof(double);
-    const int maxtiles = (n * parts + OPS_TILE_COL - 1) / OPS_TILE_COL + c->npair;
+    // column tiles: by metallicity pair over model columns (the operator groups models, parts = 1), then with the
+    // operator the template product's uniform tiles after them
+    const int gparts = ops ? 1 : parts;
+    const int maxtiles = (n * gparts + OPS_TILE_COL - 1) / OPS_TILE_COL + c->npair;
+    const int optiles = (n * parts + OPS_TILE_COL - 1) / OPS_TILE_COL;
     HIPCHK(c, hipSetDevice(c->device));
 
     // scratch: misc = zlo, it, status, order (n ints each) | ntiles | flag
     int st;
     if ((st = ops_reserve(c, c->misc, sizeof(int) * ((size_t)4 * n + 64)))) return st;
-    if ((st = ops_reserve(c, c->tiles, sizeof(int4) * (size_t)maxtiles))) return st;
+    if ((st = ops_reserve(c, c->tiles, sizeof(int4) * (size_t)(maxtiles + (ops ? optiles : 0))))) return st;
     if ((st = ops_reserve(c, c->hdr, sizeof(double) * OPS_HDR_N * (size_t)n))) return st;
     if ((st = ops_reserve(c, c->yo, tsz * (size_t)parts * n * c->nsp))) return st;
     if (mixed) {
@@ -404,6 +573,14 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     int *d_zlo = misc, *d_it = misc + n, *d_status = misc + 2 * n, *d_order = misc + 3 * n;
     int* d_ntiles = misc + 4 * n;
     int* d_flag = misc + 4 * n + 16;
+    int* d_optiles = misc + 4 * n + 32;
+    const int npass = (ops && c->nz > 1) ? 2 : 1;
+    if (ops) {
+        if ((st = ops_reserve(c, c->wsfh, sizeof(double) * (size_t)n * c->nt))) return st;
+        if ((st = ops_reserve(c, c->opu, tsz * (size_t)npass * maxtiles * c->op.npz * 64))) return st;
+        if ((st = ops_reserve(c, c->opwc, tsz * (size_t)npass * maxtiles * c->nt * 64))) return st;
+        if ((st = ops_reserve(c, c->opsum, sizeof(double) * 2 * (size_t)npass * n * c->nt))) return st;
+    }
     HIPCHK(c, hipMemsetAsync(d_ntiles, 0, 64 * sizeof(int), c->stream));
     c->d_last_flag = d_flag;
 
@@ -416,6 +593,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     pa.kpad = kpad;
     pa.mixed = mixed;
     pa.smooth_fast = c->smooth_fast;
+    pa.ops = ops;
     pa.tiny_logt = c->tiny_logt;
     pa.prm = a->prm;
     pa.weights_in = a->ssp_weights;
@@ -423,7 +601,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     pa.logz = c->d_logz;
     pa.bmass = c->d_bmass;
     pa.blbol = c->d_blbol;
-    pa.w_out = a->weights;
+    pa.w_out = ops ? (double*)c->wsfh.p : a->weights;
     pa.mass = a->mass;
     pa.lbol = a->lbol;
     pa.w64 = mixed ? nullptr : (double*)c->w64.p;
@@ -458,13 +636,97 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     GroupArgs ga;
     ga.n = n;
     ga.npair = c->npair;
-    ga.parts = parts;
+    ga.parts = gparts;
     ga.zlo = d_zlo;
     ga.order = d_order;
     ga.ntiles = d_ntiles;
     ga.tiles = (int4*)c->tiles.p;
     k_group<<<1, 256, sizeof(int) * 256 * (size_t)c->npair, c->stream>>>(ga);
     HIPCHK(c, hipGetLastError());
+    if (ops) {
+        if (a->weights)
+            HIPCHK(c, hipMemcpyAsync(a->weights, c->wsfh.p, sizeof(double) * (size_t)n * c->nt, hipMemcpyDeviceToDevice,
+                                     c->stream));
+        double* smass_p = (double*)c->opsum.p;
+        double* slum_p = smass_p + (size_t)npass * n * c->nt;
+        OpWArgs w;
+        w.n = n;
+        w.nt = c->nt;
+        w.npass = npass;
+        w.npz = c->op.npz;
+        w.maxtiles = maxtiles;
+        w.imf_type = c->imf_type;
+        w.remnants = c->remnants;
+        w.salp_ind = c->salp_ind;
+        w.mdave = c->mdave;
+        w.mlim_bh = c->mlim_bh;
+        w.mlim_ns = c->mlim_ns;
+        w.ntiles = d_ntiles;
+        w.tiles = (const int4*)c->tiles.p;
+        w.order = d_order;
+        w.prm = a->prm;
+        w.hdr = (const double*)c->hdr.p;
+        w.wsfh = (const double*)c->wsfh.p;
+        w.age_ptr = c->op.d_age_ptr;
+        w.zoff = c->op.d_zoff;
+        w.first = c->op.d_first;
+        w.pt = c->op.d_pt;
+        w.age_mmax = c->op.d_age_mmax;
+        w.u = c->opu.p;
+        w.wc = c->opwc.p;
+        w.smass_p = smass_p;
+        w.slum_p = slum_p;
+        w.count = a->op_count;
+        if (a->op_count) HIPCHK(c, hipMemsetAsync(a->op_count, 0, sizeof(int) * 2 * (size_t)n, c->stream));
+        dim3 gw((c->nt + 3) / 4, maxtiles, npass);
+        if (mixed)
+            k_op_weights<float><<<gw, 256, 0, c->stream>>>(w);
+        else
+            k_op_weights<double><<<gw, 256, 0, c->stream>>>(w);
+        HIPCHK(c, hipGetLastError());
+        OpFArgs f;
+        f.n = n;
+        f.nt = c->nt;
+        f.npass = npass;
+        f.npz = c->op.npz;
+        f.maxtiles = maxtiles;
+        f.ntpl = c->op.ntpl;
+        f.kpad = kpad;
+        f.parts = parts;
+        f.mixed = mixed;
+        f.ntiles = d_ntiles;
+        f.tiles = (const int4*)c->tiles.p;
+        f.order = d_order;
+        f.it = d_it;
+        f.tz_ptr = c->op.d_tz_ptr;
+        f.te = c->op.d_te;
+        f.te_val64 = c->op.d_te_val64;
+        f.te_val32 = c->op.d_te_val32;
+        f.u = c->opu.p;
+        f.wc = c->opwc.p;
+        f.whi = mixed ? (uint16_t*)c->wcol.p : nullptr;
+        f.wlo = mixed ? (uint16_t*)c->wcol.p + (size_t)n * parts * kpad : nullptr;
+        f.w64 = mixed ? nullptr : (double*)c->w64.p;
+        dim3 gf((kpad + 4 * OPS_TPL_BLOCK - 1) / (4 * OPS_TPL_BLOCK), maxtiles);
+        if (mixed)
+            k_op_fold<float><<<gf, 256, 0, c->stream>>>(f);
+        else
+            k_op_fold<double><<<gf, 256, 0, c->stream>>>(f);
+        HIPCHK(c, hipGetLastError());
+        OpSArgs sa;
+        sa.n = n;
+        sa.nt = c->nt;
+        sa.npass = npass;
+        sa.wsfh = (const double*)c->wsfh.p;
+        sa.smass_p = smass_p;
+        sa.slum_p = slum_p;
+        sa.mass = a->mass;
+        sa.lbol = a->lbol;
+        k_op_sums<<<(n + 255) / 256, 256, 0, c->stream>>>(sa);
+        HIPCHK(c, hipGetLastError());
+        k_op_tiles<<<1, 256, 0, c->stream>>>(n * parts, d_optiles, (int4*)c->tiles.p + maxtiles);
+        HIPCHK(c, hipGetLastError());
+    }
     if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
 
     GemmArgs gm;
@@ -475,17 +737,17 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     gm.npxblk = npxblk;
     gm.kpad = kpad;
     gm.order = d_order;
-    gm.ntiles = d_ntiles;
-    gm.tiles = (const int4*)c->tiles.p;
-    gm.A64 = c->d_A64;
+    gm.ntiles = ops ? d_optiles : d_ntiles;
+    gm.tiles = (const int4*)c->tiles.p + (ops ? maxtiles : 0);
+    gm.A64 = ops ? c->op.d_A64 : c->d_A64;
     gm.w64 = (const double*)c->w64.p;
-    gm.Ahi = c->d_Ahi;
-    gm.Alo = c->d_Alo;
+    gm.Ahi = ops ? c->op.d_Ahi : c->d_Ahi;
+    gm.Alo = ops ? c->op.d_Alo : c->d_Alo;
     gm.whi = pa.whi;
     gm.wlo = pa.wlo;
     gm.yo = c->yo.p;
     const unsigned ptx = (unsigned)((npxt + 7) / 8);
-    const unsigned gblocks = 8u * ptx * (unsigned)maxtiles;
+    const unsigned gblocks = 8u * ptx * (unsigned)(ops ? optiles : maxtiles);
     if (mixed)
         k_gemm_bf16x3<<<gblocks, 256, 0, c->stream>>>(gm);
     else if (c->f64_vector)

@@ -44,6 +44,22 @@ struct OpsCtx {
     double* d_A64 = nullptr;      // packed f64 operands [pair][pxblk][kstep][lane]
     uint16_t *d_Ahi = nullptr, *d_Alo = nullptr;   // packed bf16 hi / lo [pair][pxblk][kstep][lane][8]
 
+    // ---- the SSP operator with the IMF per model (set_ssp_ops) ----
+    struct OpsOperator {
+        bool on = false;
+        int nz = 0, nt = 0, npts = 0, ntpl = 0, npz = 0;   // npz: the most points of one metallicity
+        int k64 = 0, kbf = 0;                              // ntpl padded to the f64 / bf16 instruction K
+        int *d_age_ptr = nullptr, *d_zoff = nullptr, *d_first = nullptr, *d_tz_ptr = nullptr;
+        double *d_pt = nullptr, *d_age_mmax = nullptr;     // points: OPS_PT_N doubles each
+        int2* d_te = nullptr;                              // entries by template: {point in its metallicity, age}
+        double* d_te_val64 = nullptr;
+        float* d_te_val32 = nullptr;
+        double* d_A64 = nullptr;                           // the library as the product's A operand
+        uint16_t *d_Ahi = nullptr, *d_Alo = nullptr;
+    } op;
+    int imf_type = 2, remnants = 1;
+    double salp_ind = 2.35, mdave = 0.5, mlim_bh = 40.0, mlim_ns = 8.5;
+
     // ---- filters (set_bands) ----
     int nbands = 0;
     int band_j0 = 0, band_j1 = -1;                 // 0-based pixel range the band integrals read (whole quads)
@@ -67,6 +83,7 @@ struct OpsCtx {
     // ---- batch scratch (grown on demand) ----
     OpsBuffer yo, wcol, w64, order, tiles, misc, spec, flag;
     OpsBuffer hdr;   // per-model constants of the chain (OPS_HDR_N doubles per model)
+    OpsBuffer wsfh, opu, opwc, opsum;   // the operator path: SFH weights, point weights, age factors, per-age sums
     int* d_last_flag = nullptr;   // the per-batch status flag of the last batch (read by sync)
 
     // ---- per-kernel timing (HIP events on the context's stream) ----

@@ -211,7 +211,7 @@ __device__ int wave_sfh_weights(const double* __restrict__ tf, int nt, double ti
 // ------------------------------------------------------------------------------------------------
 // k_prep: one wavefront per model (4 per workgroup)
 // ------------------------------------------------------------------------------------------------
-__device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, int lane);
+__device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, double dz, int lane);
 
 __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     extern __shared__ double lds[];
@@ -264,9 +264,10 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
         dz = (p[OPS_P_LOGZSOL] - a.logz[zlo - 1]) / (a.logz[zlo] - a.logz[zlo - 1]);
     }
 
-    // model_sums (:2129-2176): mass and L_bol over the ages (terms per lane, then a wave sum)
+    // model_sums (:2129-2176): mass and L_bol over the ages (terms per lane, then a wave sum); with the operator
+    // they come from the per-age sums of the IMF weights (k_op_sums)
     double ms = 0.0, ls = 0.0;
-    for (int k = lane; k < nt; k += 64) {
+    for (int k = a.ops ? nt : lane; k < nt; k += 64) {
         double wk = w[k];
         if (wk > OPS_SAFE_FLOOR) {
             double mk, lk;
@@ -286,8 +287,8 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     ms = wave_sum(ms);
     ls = wave_sum(ls);
     if (lane == 0) {
-        if (a.mass) a.mass[m] = ms;
-        if (a.lbol) a.lbol[m] = ls > OPS_SAFE_FLOOR ? log10(ls) : 0.0;
+        if (a.mass && !a.ops) a.mass[m] = ms;
+        if (a.lbol && !a.ops) a.lbol[m] = ls > OPS_SAFE_FLOOR ? log10(ls) : 0.0;
         a.zlo[m] = zlo;
         a.it[m] = it;
         if (a.idx) { a.idx[3 * m] = it; a.idx[3 * m + 1] = zlo; a.idx[3 * m + 2] = st; }
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     // ages (one basis); the weight below SAFE_FLOOR is 0 (model_sums); young columns keep the ages
     // <= it_tesc, old ones the others; with one part both.
     const double c2 = dz, c1 = 1.0 - dz;
-    for (int part = 0; part < a.parts; part++) {
+    for (int part = 0; part < (a.ops ? 0 : a.parts); part++) {
         size_t col = (size_t)m * a.parts + part;
         for (int k = lane; k < a.kpad; k += 64) {
             double v = 0.0;
@@ -321,7 +322,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
             }
         }
     }
-    wave_chain_header(a, m, p, lane);
+    wave_chain_header(a, m, p, dz, lane);
 }
 
 #pragma clang fp contract(on)
@@ -620,7 +621,7 @@ __device__ __forceinline__ double igm_tau6(const double* __restrict__ aj, const 
     return aj[0] * b6[0] + aj[1] * b6[1] + aj[2] * b6[2] + aj[3] * b6[3] + aj[4] * b6[4] + aj[5] * b6[5];
 }
 
-__device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, int lane) {
+__device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, double dz, int lane) {
     const double z = p[OPS_P_ZRED], opz = 1.0 + z;
     double pw = 0.0;
     if (lane < 6) {
@@ -670,6 +671,7 @@ __device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, int
     case OPS_HDR_W0:   // #{lam < minl}
         if (smooth && minl <= maxl) v = (double)lam_count<true>(L.nspec, L.nbk, L.lnl0, L.bkinv, L.lam, L.bkt, minl, log(minl));
         break;
+    case OPS_HDR_DZ: v = dz; break;
     case OPS_HDR_W1:   // #{lam <= maxl}
         if (smooth && minl <= maxl) v = (double)lam_count<false>(L.nspec, L.nbk, L.lnl0, L.bkinv, L.lam, L.bkt, maxl, log(maxl));
         break;

@@ -17,7 +17,8 @@
 #define OPS_HDR_W0 8       // smoothed pixels [w0, w1): minl <= lambda <= maxl
 #define OPS_HDR_W1 9
 #define OPS_HDR_B6 10      // the six (1 + z) powers of the IGM optical depth
-#define OPS_HDR_N 16
+#define OPS_HDR_DZ 16      // the metallicity blend weight dz (not clamped)
+#define OPS_HDR_N 20
 
 // The wavelength lookup: the ascending grid, and buckets over ln(lambda) no wider than its finest pixel
 struct LamLookup {
@@ -28,7 +29,7 @@ struct LamLookup {
 };
 
 struct PrepArgs {
-    int n, nt, nz, parts, kpair, kpad, mixed, smooth_fast;
+    int n, nt, nz, parts, kpair, kpad, mixed, smooth_fast, ops;   // ops: the operator path (no W columns, no sums)
     double tiny_logt;
     const double *prm, *weights_in, *time, *logz, *bmass, *blbol;
     double *w_out, *mass, *lbol, *w64;
@@ -63,6 +64,45 @@ struct PackArgs {
     const double* basis;
     double* A64;
     uint16_t *Ahi, *Alo;
+};
+
+// ---- the SSP operator with the IMF per model ----
+#define OPS_PT_N 8         // per point: mini, m1, m2, ln m1, ln(m2/m1), surviving mass, L_bol, ionizing photons
+#define OPS_PIECE 32       // points per piece of an age (the IMF weights restart their chain of shared edges)
+#define OPS_TPL_BLOCK 8    // templates per wavefront of the fold
+
+struct OpWArgs {           // k_op_weights
+    int n, nt, npass, npz, maxtiles, imf_type, remnants;
+    double salp_ind, mdave, mlim_bh, mlim_ns;
+    const int* ntiles;
+    const int4* tiles;
+    const int* order;
+    const double *prm, *hdr, *wsfh;
+    const int *age_ptr, *zoff, *first;
+    const double *pt, *age_mmax;
+    void *u, *wc;          // per (pass, tile): the points' weights (npz x 64), the ages' SFH x blend factors (nt x 64)
+    double *smass_p, *slum_p;   // (pass, model, age)
+    int* count;                 // (2, n) points with a nonzero weight per pass, or NULL
+};
+
+struct OpFArgs {           // k_op_fold
+    int n, nt, npass, npz, maxtiles, ntpl, kpad, parts, mixed;
+    const int* ntiles;
+    const int4* tiles;
+    const int *order, *it;
+    const int* tz_ptr;     // nz x (ntpl + 1): entries of each template per metallicity
+    const int2* te;        // per entry {point (local to its metallicity), age}
+    const double* te_val64;
+    const float* te_val32;
+    const void *u, *wc;
+    uint16_t *whi, *wlo;   // the B operand of the template product (split BF16), [column][kpad]
+    double* w64;           // or FP64
+};
+
+struct OpSArgs {           // k_op_sums
+    int n, nt, npass;
+    const double *wsfh, *smass_p, *slum_p;
+    double *mass, *lbol;
 };
 
 struct ChainArgs {
@@ -101,3 +141,9 @@ __global__ void k_band_fluxes(int n, int nspec, int nbands, const T* bt, const i
 template <typename T>
 __global__ void k_observed_frame(int n, int nspec, const double* zc, const double* lam, const T* total, T* fobs_t);
 __global__ void k_selftest_mfma(int which, const void* a, const void* b, void* d);
+template <typename T>
+__global__ void k_op_weights(OpWArgs a);
+template <typename T>
+__global__ void k_op_fold(OpFArgs a);
+__global__ void k_op_sums(OpSArgs a);
+__global__ void k_op_tiles(int ncols, int* ntiles, int4* tiles);

@@ -20,10 +20,18 @@ _lib = None
 
 
 class BatchArgs(C.Structure):
-    _fields_ = [("n", C.c_int), ("precision", C.c_int), ("split_parts", C.c_int), ("prm", C.c_void_p),
+    _fields_ = [("n", C.c_int), ("precision", C.c_int), ("split_parts", C.c_int), ("use_ops", C.c_int),
+                ("prm", C.c_void_p),
                 ("ssp_weights", C.c_void_p), ("mags", C.c_void_p), ("spec", C.c_void_p), ("mass", C.c_void_p),
                 ("lbol", C.c_void_p), ("fband", C.c_void_p), ("young", C.c_void_p), ("old", C.c_void_p),
-                ("idx", C.c_void_p), ("weights", C.c_void_p)]
+                ("idx", C.c_void_p), ("weights", C.c_void_p), ("op_count", C.c_void_p)]
+
+
+class SspOpsArgs(C.Structure):
+    _fields_ = [("nz", C.c_int), ("nt", C.c_int), ("npts", C.c_int), ("ntpl", C.c_int), ("nent", C.c_int),
+                ("age_ptr", C.c_void_p), ("mini", C.c_void_p), ("m1", C.c_void_p), ("m2", C.c_void_p),
+                ("first", C.c_void_p), ("cm", C.c_void_p), ("cl", C.c_void_p), ("q", C.c_void_p),
+                ("age_mmax", C.c_void_p), ("ent_tpl", C.c_void_p), ("ent_val", C.c_void_p), ("etpl", C.c_void_p)]
 
 
 class OpenSPSHipError(RuntimeError):
@@ -54,6 +62,9 @@ def lib():
         L.opensps_hip_selftest_mfma.argtypes = [i, vp, vp, vp]
         L.opensps_hip_last_error.argtypes = [vp, C.c_char_p, i]
         L.opensps_hip_device_count.argtypes = [C.POINTER(i)]
+        L.opensps_hip_set_ssp_ops.argtypes = [vp, C.POINTER(SspOpsArgs)]
+        L.opensps_hip_free_ssp_ops.argtypes = [vp]
+        L.opensps_hip_set_imf.argtypes = [vp, i, i, d, d, d, d]
         _lib = L
     return _lib
 
@@ -128,6 +139,28 @@ class DeviceContext:
     def set_options(self, add_igm=1, smooth_fast=1):
         self._check(lib().opensps_hip_set_options(self._ctx, int(add_igm), int(smooth_fast)))
 
+    def set_ssp_ops(self, op):
+        """Installs a synth.SSPOperator (the IMF-per-model operator set) on the context."""
+        a = SspOpsArgs()
+        a.nz, a.nt, a.npts, a.ntpl, a.nent = op.nz, op.nt, op.npts, op.ntpl, op.nent
+        keep = []
+        for k in ("age_ptr", "first", "ent_tpl"):
+            v = _host(getattr(op, k), np.int32)
+            keep.append(v)
+            setattr(a, k, v.ctypes.data)
+        for k in ("mini", "m1", "m2", "cm", "cl", "q", "age_mmax", "ent_val", "etpl"):
+            v = _host(getattr(op, k))
+            keep.append(v)
+            setattr(a, k, v.ctypes.data)
+        self._check(lib().opensps_hip_set_ssp_ops(self._ctx, C.byref(a)))
+
+    def free_ssp_ops(self):
+        self._check(lib().opensps_hip_free_ssp_ops(self._ctx))
+
+    def set_imf(self, imf_type=2, remnants=1, salp_ind=2.35, mdave=0.5, mlim_bh=40.0, mlim_ns=8.5):
+        self._check(lib().opensps_hip_set_imf(self._ctx, int(imf_type), int(remnants), float(salp_ind), float(mdave),
+                                              float(mlim_bh), float(mlim_ns)))
+
     def alloc_outputs(self, n, precision, want=("mags", "spec", "mass", "lbol")):
         import torch
         dev = "cuda"
@@ -136,13 +169,14 @@ class DeviceContext:
         out = {"mags": torch.empty((n, self.nbands), dtype=torch.float64, device=dev)}
         shapes = {"spec": ((n, g.nspec), sdt), "mass": ((n,), torch.float64), "lbol": ((n,), torch.float64),
                   "fband": ((n, self.nbands), torch.float64), "young": ((n, g.nspec), sdt),
-                  "old": ((n, g.nspec), sdt), "idx": ((n, 3), torch.int32), "weights": ((n, g.nt), torch.float64)}
+                  "old": ((n, g.nspec), sdt), "idx": ((n, 3), torch.int32), "weights": ((n, g.nt), torch.float64),
+                  "op_count": ((n, 2), torch.int32)}
         for k in want:
             if k in shapes:
                 out[k] = torch.empty(shapes[k][0], dtype=shapes[k][1], device=dev)
         return out
 
-    def launch(self, prm, out, precision=MIXED, split_parts=0, ssp_weights=None):
+    def launch(self, prm, out, precision=MIXED, split_parts=0, ssp_weights=None, use_ops=0):
         """Enqueues one batch on device-resident prm (NPAR, n) float64 and the buffers of out."""
         n = prm.shape[1]
         assert prm.shape[0] == NPAR and prm.is_contiguous() and prm.dtype.itemsize == 8
@@ -150,7 +184,7 @@ class DeviceContext:
         isz = 4 if precision == MIXED else 8
         sizes = {"mags": (n * self.nbands, 8), "spec": (n * g.nspec, isz), "young": (n * g.nspec, isz),
                  "old": (n * g.nspec, isz), "mass": (n, 8), "lbol": (n, 8), "fband": (n * self.nbands, 8),
-                 "idx": (3 * n, 4), "weights": (n * g.nt, 8)}
+                 "idx": (3 * n, 4), "weights": (n * g.nt, 8), "op_count": (2 * n, 4)}
         for k, (numel, itemsize) in sizes.items():
             t = out.get(k)
             if t is not None and (not t.is_contiguous() or t.element_size() != itemsize or t.numel() < numel):
@@ -158,22 +192,23 @@ class DeviceContext:
         if ssp_weights is not None and (ssp_weights.element_size() != 8 or ssp_weights.numel() < n * g.nt):
             raise OpenSPSHipError(3, "ssp_weights: nt x n float64 required")
         a = BatchArgs()
-        a.n, a.precision, a.split_parts = n, precision, int(split_parts)
+        a.n, a.precision, a.split_parts, a.use_ops = n, precision, int(split_parts), int(use_ops)
         a.prm = prm.data_ptr()
         a.ssp_weights = ssp_weights.data_ptr() if ssp_weights is not None else None
-        for k in ("mags", "spec", "mass", "lbol", "fband", "young", "old", "idx", "weights"):
+        for k in ("mags", "spec", "mass", "lbol", "fband", "young", "old", "idx", "weights", "op_count"):
             setattr(a, k, out[k].data_ptr() if k in out else None)
         self._check(lib().opensps_hip_csp_batch(self._ctx, C.byref(a)))
 
     def sync(self):
         self._check(lib().opensps_hip_sync(self._ctx))
 
-    def batch(self, prm, precision=MIXED, split_parts=0, ssp_weights=None, want=("mags", "spec", "mass", "lbol")):
+    def batch(self, prm, precision=MIXED, split_parts=0, ssp_weights=None, want=("mags", "spec", "mass", "lbol"),
+              use_ops=0):
         import torch
         n = prm.shape[1]
         out = self.alloc_outputs(n, precision, want)
         torch.cuda.synchronize()
-        self.launch(prm, out, precision, split_parts, ssp_weights)
+        self.launch(prm, out, precision, split_parts, ssp_weights, use_ops)
         self.sync()
         return out
 

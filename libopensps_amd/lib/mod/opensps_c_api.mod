@@ -1,9 +1,9 @@
-﻿!mod$ v1 sum:8c4ee476702aa8c2
+﻿!mod$ v1 sum:48fa3b6c809f4a58
 !need$ b579c837613f74aa n opensps_host_filters
-!need$ 9ff8728feed3f05b n opensps_host_batch
+!need$ 05e2a849ea85cd1b n opensps_host_batch
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ f5118467a58a718d n opensps_host_types
-!need$ 5e268f5f4692bc4e n opensps_host_context
+!need$ 8abfb921d159308f n opensps_host_context
 !need$ bb381bf46e508468 i __fortran_builtins
 module opensps_c_api
 use,intrinsic::iso_c_binding,only:c_associated

@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:3a90a40bf090e16a
+﻿!mod$ v1 sum:5106ef5fd61c8351
 !need$ 0bde2ac47243ead2 i iso_c_binding
 module opensps_hip_iface
 use,intrinsic::__fortran_builtins,only:__builtin_c_ptr
@@ -21,6 +21,7 @@ type,bind(c)::hip_batch_t
 integer(4)::n=0_4
 integer(4)::precision=0_4
 integer(4)::split_parts=0_4
+integer(4)::use_ops=0_4
 type(c_ptr)::prm=__builtin_c_ptr(__address=0_8)
 type(c_ptr)::ssp_weights=__builtin_c_ptr(__address=0_8)
 type(c_ptr)::mags=__builtin_c_ptr(__address=0_8)
@@ -32,6 +33,7 @@ type(c_ptr)::young=__builtin_c_ptr(__address=0_8)
 type(c_ptr)::old=__builtin_c_ptr(__address=0_8)
 type(c_ptr)::idx=__builtin_c_ptr(__address=0_8)
 type(c_ptr)::weights=__builtin_c_ptr(__address=0_8)
+type(c_ptr)::op_count=__builtin_c_ptr(__address=0_8)
 end type
 interface
 function opensps_hip_create(device,ctx) bind(c,name="opensps_hip_create")

@@ -146,3 +146,139 @@ def varied_names() -> list:
 
 
 assert NPAR == len(PARAM_DEFAULTS)
+
+
+# ------------------------------------------------------------------------------------------------
+# The SSP operator of the IMF-per-model workload
+# ------------------------------------------------------------------------------------------------
+@dataclass
+class SSPOperator:
+    """Isochrone points of every (metallicity, age), their template entries and the template library.
+
+    Points of (iz, t) are [age_ptr[iz * nt + t], age_ptr[iz * nt + t + 1]), sorted by initial mass; their mass bins
+    m1 / m2 are the midpoints to the neighbours (the first point of an age takes the IMF's lower bound, flag
+    first = 1; the last bin ends at its own mass). cm: surviving mass, cl: L_bol (L_sun), q: ionizing photons (1/s)
+    per star. Each point has nent entries (template, value): its L_bol shared between the two library templates
+    that bracket its Teff in log Teff. The library (ntpl, nspec) holds blackbody spectra per unit L_bol (1/Hz),
+    ordered by the first age that uses them, so k_age[t] (templates first used at ages <= t) is increasing.
+    """
+    nz: int
+    nt: int
+    age_ptr: np.ndarray     # (nz * nt + 1,) int32
+    mini: np.ndarray        # (npts,)
+    m1: np.ndarray
+    m2: np.ndarray
+    first: np.ndarray       # (npts,) int32
+    cm: np.ndarray
+    cl: np.ndarray
+    q: np.ndarray
+    age_mmax: np.ndarray    # (nz * nt,) largest living initial mass of each isochrone
+    ent_tpl: np.ndarray     # (npts, nent) int32, -1: no entry
+    ent_val: np.ndarray     # (npts, nent)
+    etpl: np.ndarray        # (ntpl, nspec) per unit L_bol
+    teff: np.ndarray        # (ntpl,) the templates' temperatures
+    k_age: np.ndarray       # (nt,) int32
+    seed: int = 0
+
+    @property
+    def npts(self) -> int:
+        return self.mini.size
+
+    @property
+    def ntpl(self) -> int:
+        return self.teff.size
+
+    @property
+    def nent(self) -> int:
+        return self.ent_tpl.shape[1]
+
+
+def ssp_operator(lam: np.ndarray, log_age: np.ndarray, nz: int, npts: int = 1500, ntpl: int = 10000,
+                 seed: int = 11, tmin: float = 2500.0, tmax: float = 5.0e4) -> SSPOperator:
+    """The workload's synthetic operator: per (metallicity, age) npts initial masses log-uniform between 0.08 M_sun
+    and the isochrone's largest living mass (a main-sequence lifetime of 1e10 yr m^-2.5, capped at 120 M_sun),
+    sorted; log Teff linear in log m from tmin at 0.08 M_sun to tmax at 120 M_sun (0.01 dex scatter, -0.003 dex per
+    metallicity step), L ~ m^3.5 (0.05 dex scatter); ntpl blackbody templates log-spaced in [tmin, tmax]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nt = log_age.size
+    ages = 10.0 ** log_age
+    mmax_t = np.clip((1.0e10 / ages) ** (1.0 / 2.5), 0.9, 120.0)
+    lteff_tpl = np.linspace(np.log10(tmin), np.log10(tmax), ntpl)
+    teff_tpl = 10.0 ** lteff_tpl
+    # blackbody per unit bolometric luminosity: B_nu / integral(B_nu dnu) in 1/Hz
+    nu = 2.99792458e18 / lam
+    x = HC_OVER_K / (lam[None, :] * teff_tpl[:, None])
+    with np.errstate(over="ignore"):
+        bnu = nu[None, :] ** 3 / np.expm1(x)
+    # integral of nu^3 / (exp(h nu / kT) - 1) dnu = (pi^4 / 15) (k T / h)^4
+    kt_h = teff_tpl * 2.99792458e18 / HC_OVER_K
+    etpl = bnu / ((np.pi ** 4 / 15.0) * kt_h[:, None] ** 4)
+    per = nz * nt
+    mini = np.empty(per * npts)
+    teff = np.empty(per * npts)
+    lum = np.empty(per * npts)
+    age_mmax = np.empty(per)
+    for iz in range(nz):
+        for t in range(nt):
+            a = iz * nt + t
+            mm = mmax_t[t]
+            m = np.sort(10.0 ** rng.uniform(np.log10(0.08), np.log10(mm), npts))
+            sl = slice(a * npts, (a + 1) * npts)
+            mini[sl] = m
+            frac = (np.log10(m) - np.log10(0.08)) / (np.log10(120.0) - np.log10(0.08))
+            lt = (np.log10(tmin) + frac * np.log10(tmax / tmin) - 0.003 * iz
+                  + 0.01 * rng.standard_normal(npts))
+            teff[sl] = 10.0 ** np.clip(lt, np.log10(tmin), np.log10(tmax))
+            lum[sl] = m ** 3.5 * 10.0 ** (0.05 * rng.standard_normal(npts))
+            age_mmax[a] = m[-1]
+    age_ptr = (np.arange(per + 1) * npts).astype(np.int32)
+    m1 = np.empty_like(mini)
+    m2 = np.empty_like(mini)
+    first = np.zeros(mini.size, np.int32)
+    for a in range(per):
+        m = mini[a * npts:(a + 1) * npts]
+        lo = np.empty(npts)
+        hi = np.empty(npts)
+        lo[1:] = m[1:] - 0.5 * (m[1:] - m[:-1])
+        lo[0] = m[0]
+        hi[:-1] = m[:-1] + 0.5 * (m[1:] - m[:-1])
+        hi[-1] = m[-1]
+        m1[a * npts:(a + 1) * npts] = lo
+        m2[a * npts:(a + 1) * npts] = hi
+        first[a * npts] = 1
+    cm = mini * rng.uniform(0.9, 1.0, mini.size)
+    q = lum * 1.0e44 * (teff / tmax) ** 4
+    # entries: the two bracketing templates in log Teff, linear weights, times L_bol
+    lt = np.log10(teff)
+    j = np.clip(np.searchsorted(lteff_tpl, lt), 1, ntpl - 1)
+    f = (lt - lteff_tpl[j - 1]) / (lteff_tpl[j] - lteff_tpl[j - 1])
+    ent_tpl = np.stack([j - 1, j], axis=1).astype(np.int64)
+    ent_val = np.stack([(1.0 - f) * lum, f * lum], axis=1)
+    # order the library by the first age that uses a template (any metallicity)
+    age_of_pt = np.repeat(np.tile(np.arange(nt), nz), npts)
+    first_age = np.full(ntpl, nt, np.int64)
+    np.minimum.at(first_age, ent_tpl[:, 0], age_of_pt)
+    np.minimum.at(first_age, ent_tpl[:, 1], age_of_pt)
+    order = np.lexsort((np.arange(ntpl), first_age))
+    relabel = np.empty(ntpl, np.int64)
+    relabel[order] = np.arange(ntpl)
+    ent_tpl = relabel[ent_tpl].astype(np.int32)
+    etpl = etpl[order]
+    teff_tpl = teff_tpl[order]
+    fa = first_age[order]
+    k_age = np.array([np.searchsorted(fa, t, side="right") for t in range(nt)], np.int32)
+    return SSPOperator(nz=nz, nt=nt, age_ptr=age_ptr, mini=mini, m1=m1, m2=m2, first=first, cm=cm, cl=lum, q=q,
+                       age_mmax=age_mmax, ent_tpl=ent_tpl, ent_val=ent_val, etpl=np.ascontiguousarray(etpl),
+                       teff=teff_tpl, k_age=k_age, seed=seed)
+
+
+def imf_params(n: int, seed: int = 13, slopes=(0.3, 3.3), upper=(60.0, 120.0), p=None) -> np.ndarray:
+    """(NPAR, n) parameters with a per-model Kroupa IMF: slopes imf1, imf2, imf3 U(slopes), imf_upper_limit
+    U(upper); the rest from p (or the defaults)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = default_params(n) if p is None else p.copy()
+    out[SLOT["imf1"]] = rng.uniform(slopes[0], slopes[1], n)
+    out[SLOT["imf2"]] = rng.uniform(slopes[0], slopes[1], n)
+    out[SLOT["imf3"]] = rng.uniform(slopes[0], slopes[1], n)
+    out[SLOT["imf_upper_limit"]] = rng.uniform(upper[0], upper[1], n)
+    return out

@@ -557,6 +557,270 @@ int orc_filter_band(const double *lc, const double *rc, int nc, const double *g,
 }
 
 /* ---------------------------------------------------------------------------------------------
+ * IMF integrals and bin weights (src/physics/opensps_imf.f90). The packed IMF parameters p[] of
+ * this restatement: the reference's IMF_P_* slots that the power-law types read.
+ * ------------------------------------------------------------------------------------------- */
+#define ORC_IMF_SALP 0
+#define ORC_IMF_ALPHA 1     /* 1..3: imf1, imf2, imf3 */
+#define ORC_IMF_VDMC 4
+#define ORC_IMF_MDAVE 5
+#define ORC_IMF_LOWER_LIMIT 6
+#define ORC_IMF_UPPER_LIMIT 7
+#define ORC_IMF_LOWER_BOUND 8
+#define ORC_IMF_MLIM_BH 9
+#define ORC_IMF_MLIM_NS 10
+#define ORC_IMF_NPAR 11
+
+/* powerlaw_bin (opensps_imf.f90:754-792): int_lo^hi m^(e-1) dm given plo = lo^e, as
+ * plo expm1(e dl) / e with dl = ln(hi/lo) in Kahan's log1p form; series below |e dl| = 1/4 */
+double orc_powerlaw_bin(double plo, double lo, double hi, double e) {
+    const double I2 = 1.0 / 2, I3 = 1.0 / 3, I4 = 1.0 / 4, I5 = 1.0 / 5, I6 = 1.0 / 6, I7 = 1.0 / 7, I8 = 1.0 / 8,
+                 I9 = 1.0 / 9, I10 = 1.0 / 10, I11 = 1.0 / 11, I12 = 1.0 / 12, I13 = 1.0 / 13;
+    double x = (hi - lo) / lo, u = 1.0 + x, dl;
+    if (u == 1.0) dl = x;
+    else dl = log(u) * x / (u - 1.0);
+    if (e == 0.0) return plo * dl;
+    double y = e * dl;
+    if (fabs(y) < 0.25) {
+        double r = 1.0 + y * I13;
+        r = 1.0 + y * I12 * r; r = 1.0 + y * I11 * r; r = 1.0 + y * I10 * r; r = 1.0 + y * I9 * r;
+        r = 1.0 + y * I8 * r; r = 1.0 + y * I7 * r; r = 1.0 + y * I6 * r; r = 1.0 + y * I5 * r;
+        r = 1.0 + y * I4 * r; r = 1.0 + y * I3 * r; r = 1.0 + y * I2 * r;
+        return plo * dl * r;
+    }
+    return plo * (exp(y) - 1.0) / e;
+}
+
+/* expm1_series (opensps_imf.f90:641-667) */
+static double expm1_series(double y) {
+    const double I2 = 1.0 / 2, I3 = 1.0 / 3, I4 = 1.0 / 4, I5 = 1.0 / 5, I6 = 1.0 / 6, I7 = 1.0 / 7, I8 = 1.0 / 8,
+                 I9 = 1.0 / 9, I10 = 1.0 / 10, I11 = 1.0 / 11, I12 = 1.0 / 12, I13 = 1.0 / 13;
+    if (fabs(y) < 0.25) {
+        double r = 1.0 + y * I13;
+        r = 1.0 + y * I12 * r; r = 1.0 + y * I11 * r; r = 1.0 + y * I10 * r; r = 1.0 + y * I9 * r;
+        r = 1.0 + y * I8 * r; r = 1.0 + y * I7 * r; r = 1.0 + y * I6 * r; r = 1.0 + y * I5 * r;
+        r = 1.0 + y * I4 * r; r = 1.0 + y * I3 * r; r = 1.0 + y * I2 * r;
+        return y * r;
+    }
+    return exp(y) - 1.0;
+}
+
+/* integrate_powerlaw (opensps_imf.f90:725-748) */
+static double integrate_powerlaw(double lo, double hi, double alpha, int mass_weighted, double coeff) {
+    if (hi <= lo) return 0.0;
+    double denom = mass_weighted ? 2.0 - alpha : 1.0 - alpha;
+    return coeff * orc_powerlaw_bin(pow(lo, denom), lo, hi, denom);
+}
+
+/* imf_integral / imf_integral_p (opensps_imf.f90:86-96, 206-315) for the power-law types: 0 Salpeter,
+ * 2 Kroupa (breaks 0.08, 0.5, 1), 4 Dave (break mdave). Other types integrate to 0 here (the batch
+ * refuses them). */
+double orc_imf_integral(int imf_type, const double *p, double m1, double m2, int mass_weighted) {
+    double lo = fmin(m1, m2), hi = fmax(m1, m2), v = 0.0;
+    if (hi <= lo) return 0.0;
+    int t = imf_type % 10;
+    if (t == 0) {
+        v = integrate_powerlaw(lo, hi, p[ORC_IMF_SALP], mass_weighted, 1.0);
+    } else if (t == 2) {
+        double a1 = p[ORC_IMF_ALPHA], a2 = p[ORC_IMF_ALPHA + 1], a3 = p[ORC_IMF_ALPHA + 2];
+        double c2 = pow(0.5, -a1 + a2), sl, sh;
+        sl = fmax(lo, 0.08); sh = fmin(hi, 0.5);
+        if (sh > sl) v = v + integrate_powerlaw(sl, sh, a1, mass_weighted, 1.0);
+        sl = fmax(lo, 0.5); sh = fmin(hi, 1.0);
+        if (sh > sl) v = v + integrate_powerlaw(sl, sh, a2, mass_weighted, c2);
+        sl = fmax(lo, 1.0); sh = hi;
+        if (sh > sl) v = v + integrate_powerlaw(sl, sh, a3, mass_weighted, c2);
+    } else if (t == 4) {
+        double a1 = p[ORC_IMF_ALPHA], a2 = p[ORC_IMF_ALPHA + 1], md = p[ORC_IMF_MDAVE], sl, sh;
+        sl = fmax(lo, 0.08); sh = fmin(hi, md);
+        if (sh > sl) v = v + integrate_powerlaw(sl, sh, a1, mass_weighted, 1.0);
+        sl = fmax(lo, md); sh = hi;
+        if (sh > sl) v = v + integrate_powerlaw(sl, sh, a2, mass_weighted, pow(md, -a1 + a2));
+    }
+    return v;
+}
+
+/* imf_bin_logs (opensps_imf.f90:564-579): llo = ln m1, dl = ln(m2 / m1) in Kahan's log1p form */
+void orc_imf_bin_logs(double m1, double m2, double *llo, double *dl) {
+    *llo = 0.0; *dl = 0.0;
+    if (m1 <= 0.0 || m2 <= m1) return;
+    *llo = log(m1);
+    double x = (m2 - m1) / m1, u = 1.0 + x;
+    *dl = (u == 1.0) ? x : log(u) * x / (u - 1.0);
+}
+
+/* imf_bin_weights_tab (opensps_imf.f90:439-559), FP64 build: the weights of a run of consecutive bins
+ * (one piece of an age), stars per bin per unit initial mass; a bin inside one power-law segment is
+ * c lo^e expm1(e dl) / e, lo^e carried from the previous bin when they share an edge. */
+void orc_imf_bin_weights_tab(int imf_type, const double *p, double total_mass, int n, const double *mini,
+                             const double *m1, const double *m2, const double *llo, const double *dl, const int *first,
+                             double *w) {
+    int t = imf_type % 10, cur = 0;
+    double lim_lo = p[ORC_IMF_LOWER_LIMIT], lim_hi = p[ORC_IMF_UPPER_LIMIT];
+    double e = 0.0, c = 0.0, d = 1.0, ce = 0.0, prev_hi = -1.0, prev_phi = 0.0;
+    double inv_total = total_mass > 0.0 ? 1.0 / total_mass : 1.0;
+    for (int k = 0; k < n; k++) {
+        w[k] = 0.0;
+        if (mini[k] < lim_lo || mini[k] > lim_hi) continue;
+        double lo = first[k] == 1 ? p[ORC_IMF_LOWER_BOUND] : m1[k], hi = m2[k], val;
+        if (hi <= lo) continue;
+        int seg = 0;
+        if (t == 0) {
+            seg = 1;
+        } else if (t == 2) {
+            if (lo >= 0.08 && hi <= 0.5) seg = 1;
+            else if (lo >= 0.5 && hi <= 1.0) seg = 2;
+            else if (lo >= 1.0) seg = 3;
+        } else if (t == 4) {
+            if (lo >= 0.08 && hi <= p[ORC_IMF_MDAVE]) seg = 1;
+            else if (lo >= p[ORC_IMF_MDAVE]) seg = 2;
+        }
+        if (seg > 0 && seg != cur) {
+            double alpha, coeff;
+            if (t == 0) {
+                alpha = p[ORC_IMF_SALP]; coeff = 1.0;
+            } else if (t == 2) {
+                alpha = p[ORC_IMF_ALPHA + seg - 1]; coeff = 1.0;
+                if (seg > 1) coeff = pow(0.5, -p[ORC_IMF_ALPHA] + p[ORC_IMF_ALPHA + 1]);
+            } else {
+                alpha = p[ORC_IMF_ALPHA + seg - 1]; coeff = 1.0;
+                if (seg > 1) coeff = pow(p[ORC_IMF_MDAVE], -p[ORC_IMF_ALPHA] + p[ORC_IMF_ALPHA + 1]);
+            }
+            e = 1.0 - alpha; c = coeff; d = 1.0 - alpha;
+            ce = 0.0;
+            if (e != 0.0) ce = c / e;
+            cur = seg;
+            prev_hi = -1.0;
+        }
+        if (seg > 0 && cur == seg) {
+            if (first[k] == 1) {
+                if (fabs(d) <= 1.0e-12) val = c * log(hi / lo);
+                else val = c * (pow(hi, e) - pow(lo, e)) / d;
+            } else if (e == 0.0) {
+                val = c * dl[k];
+            } else {
+                double plo = (lo == prev_hi) ? prev_phi : exp(e * llo[k]);
+                double em1 = expm1_series(e * dl[k]);
+                val = ce * plo * em1;
+                prev_hi = hi;
+                prev_phi = plo + plo * em1;
+            }
+        } else {
+            val = orc_imf_integral(imf_type, p, lo, hi, 0);
+            prev_hi = -1.0;
+            if (seg == 0) cur = 0;
+        }
+        w[k] = val * inv_total;
+    }
+}
+
+/* add_remnant_mass_p (opensps_stellar_modifications.f90:729-816): black holes (half the initial
+ * mass above max(mlim_bh, the largest living mass)), neutron stars (1.4 M_sun), white dwarfs
+ * (0.48 + 0.077 m) from the stars that have died, per unit initial mass */
+double orc_remnant_mass(int imf_type, const double *p, double max_living_mass) {
+    double lim_hi = p[ORC_IMF_UPPER_LIMIT], lbh = p[ORC_IMF_MLIM_BH], lns = p[ORC_IMF_MLIM_NS], add = 0.0;
+    double norm = orc_imf_integral(imf_type, p, p[ORC_IMF_LOWER_LIMIT], lim_hi, 1);
+    if (norm <= 0.0) return 0.0;
+    double a = fmin(fmax(lbh, max_living_mass), lim_hi);
+    if (a < lim_hi) add = add + 0.5 * orc_imf_integral(imf_type, p, a, lim_hi, 1) / norm;
+    if (max_living_mass <= lbh) {
+        a = fmin(fmax(lns, max_living_mass), lim_hi);
+        double b = fmin(lbh, lim_hi);
+        if (a < b) add = add + 1.4 * orc_imf_integral(imf_type, p, a, b, 0) / norm;
+    }
+    if (max_living_mass <= lns) {
+        a = fmin(max_living_mass, lim_hi);
+        double b = fmin(lns, lim_hi);
+        if (a < b)
+            add = add + (0.48 * orc_imf_integral(imf_type, p, a, b, 0)) / norm +
+                  (0.077 * orc_imf_integral(imf_type, p, a, b, 1)) / norm;
+    }
+    return add;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * The SSP operator set with the IMF per model (src/physics/opensps_ssp_batch.f90): the isochrone
+ * points of every (metallicity, age) with their mass bins and per-star mass, L_bol and ionizing
+ * photons; each point's template entries; the template library (ntpl x nspec).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct {
+    int nz, nt, npts, ntpl, nent;
+    const int *age_ptr;       /* nz * nt + 1: points of (iz, t) are [age_ptr[iz nt + t], age_ptr[iz nt + t + 1]) */
+    const double *mini, *m1, *m2;
+    const int *first;
+    const double *cm, *cl, *q;
+    const double *age_mmax;   /* nz * nt: the largest living initial mass of each isochrone */
+    const int *ent_tpl;       /* npts * nent, template of each entry (-1: none) */
+    const double *ent_val;    /* npts * nent */
+    const double *etpl;       /* ntpl x nspec */
+} orc_ops_t;
+
+#define ORC_PIECE 32   /* points per piece (the reference's pieces of an age, pc_ptr, <= 32 points) */
+
+/* One model's operator age sums, ssp_batch_age_sums_z (opensps_ssp_batch.f90:527-785) with
+ * imf_pieces_multi (:2913-3091) and fold_multi (:3095-3188): two passes, the operators of
+ * metallicities zlo (weight 1 - dz) and zlo + 1 (weight dz; one pass with one metallicity); per age
+ * the IMF weights of its points piece by piece, their mass / L_bol / photon sums (remnants added),
+ * and the fold of the weighted points into template weights split at it_tesc; then the template
+ * products. Outputs: young, old (nspec), smass, slum (nt, slum linear), vy, vo (ntpl scratch),
+ * w (scratch, the largest age's points). */
+void orc_ops_model(const orc_ops_t *op, int nspec, int imf_type, int remnants, const double *pimf, int zlo, double dz,
+                   const double *wsfh, int it_tesc, double *young, double *old, double *smass, double *slum,
+                   double *vy, double *vo, double *w, double *llo, double *dl, int *count) {
+    const int nt = op->nt, nent = op->nent;
+    int npass = op->nz > 1 ? 2 : 1;
+    double tot = orc_imf_integral(imf_type, pimf, pimf[ORC_IMF_LOWER_LIMIT], pimf[ORC_IMF_UPPER_LIMIT], 1);
+    for (int t = 0; t < nt; t++) { smass[t] = 0.0; slum[t] = 0.0; }
+    for (int e = 0; e < op->ntpl; e++) { vy[e] = 0.0; vo[e] = 0.0; }
+    if (count) { count[0] = 0; count[1] = 0; }
+    for (int pass = 0; pass < npass; pass++) {
+        int iz = (npass == 1) ? 0 : zlo - 1 + pass;
+        double cz = (npass == 1) ? 1.0 : (pass == 0 ? 1.0 - dz : dz);
+        if (cz == 0.0) continue;   /* group_models: models without weight in a pass are left out */
+        for (int t = 0; t < nt; t++) {
+            int k0 = op->age_ptr[iz * nt + t], k1 = op->age_ptr[iz * nt + t + 1];
+            double sm = 0.0, sl = 0.0, sq = 0.0;
+            for (int k = k0; k < k1; k++) orc_imf_bin_logs(op->m1[k], op->m2[k], &llo[k - k0], &dl[k - k0]);
+            for (int q0 = k0; q0 < k1; q0 += ORC_PIECE) {
+                int np = k1 - q0 < ORC_PIECE ? k1 - q0 : ORC_PIECE;
+                orc_imf_bin_weights_tab(imf_type, pimf, tot, np, op->mini + q0, op->m1 + q0, op->m2 + q0,
+                                        llo + (q0 - k0), dl + (q0 - k0), op->first + q0, w + (q0 - k0));
+                double pm = 0.0, pl = 0.0, pq = 0.0;
+                for (int k = q0; k < q0 + np; k++) {
+                    if (count && w[k - k0] > 0.0) count[pass]++;
+                    pm = pm + op->cm[k] * w[k - k0];
+                    pl = pl + op->cl[k] * w[k - k0];
+                    pq = pq + op->q[k] * w[k - k0];
+                }
+                sm = sm + pm; sl = sl + pl; sq = sq + pq;
+            }
+            if (remnants && k1 > k0) sm = sm + orc_remnant_mass(imf_type, pimf, op->age_mmax[iz * nt + t]);
+            smass[t] = smass[t] + cz * sm;
+            slum[t] = slum[t] + cz * log10(fmax(sl, SAFE_FLOOR));
+            /* the fold: weighted points into the template weights, young (t <= it_tesc) or old */
+            double x0 = wsfh[t] * cz;
+            double *v = (t + 1 <= it_tesc) ? vy : vo;
+            for (int k = k0; k < k1; k++) {
+                double x = w[k - k0] * x0;
+                for (int j = 0; j < nent; j++) {
+                    int e = op->ent_tpl[(size_t)k * nent + j];
+                    if (e >= 0) v[e] = v[e] + op->ent_val[(size_t)k * nent + j] * x;
+                }
+            }
+        }
+    }
+    for (int t = 0; t < nt; t++) slum[t] = pow(10.0, slum[t]);
+    /* template products (template_columns, :3247-3392) */
+    for (int i = 0; i < nspec; i++) { young[i] = 0.0; old[i] = 0.0; }
+    for (int e = 0; e < op->ntpl; e++) {
+        const double *s = op->etpl + (size_t)e * nspec;
+        double a = vy[e], b = vo[e];
+        if (a != 0.0) for (int i = 0; i < nspec; i++) young[i] = young[i] + a * s[i];
+        if (b != 0.0) for (int i = 0; i < nspec; i++) old[i] = old[i] + b * s[i];
+    }
+}
+
+/* ---------------------------------------------------------------------------------------------
  * The batch (src/physics/opensps_csp_batch.f90:431-1214, the stages that the configurations of
  * this build exercise: parametric or given SFH weights, the resident bases blended in logzsol,
  * power-law dust with birth cloud and old-star screens, per-model IGM, velocity smoothing,
@@ -577,6 +841,11 @@ typedef struct {
     int add_igm;              /* add_igm_absorption */
     int smooth_fast;          /* smooth_velocity == 1 and smoothspec_fast == 1 */
     double tiny_logt;
+    /* the IMF per model (the operator set, or NULL: the resident bases) */
+    const orc_ops_t *ops;
+    int imf_type, remnants;   /* imf_type, add_stellar_remnants */
+    double salp_ind, mdave, mlim_bh, mlim_ns;
+    int *op_count;            /* 2 x n or NULL: points with a nonzero IMF weight per metallicity pass */
 } orc_ctx_t;
 
 /* Per-model birth-cloud age index: batch_model_indices (opensps_csp_batch.f90:1541-1576) */
@@ -650,6 +919,16 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
         double *tau = (double *)malloc(sizeof(double) * (jtop + 1));
         double sm_key[3] = {NAN, NAN, NAN};
         int sm_n = 0;
+        double *smass = NULL, *slum = NULL, *vy = NULL, *vo = NULL, *wpt = NULL, *llo = NULL, *dl = NULL;
+        if (c->ops) {
+            int maxpts = 1;
+            for (int a = 0; a < c->ops->nz * nt; a++)
+                if (c->ops->age_ptr[a + 1] - c->ops->age_ptr[a] > maxpts) maxpts = c->ops->age_ptr[a + 1] - c->ops->age_ptr[a];
+            smass = (double *)malloc(sizeof(double) * nt); slum = (double *)malloc(sizeof(double) * nt);
+            vy = (double *)malloc(sizeof(double) * c->ops->ntpl); vo = (double *)malloc(sizeof(double) * c->ops->ntpl);
+            wpt = (double *)malloc(sizeof(double) * maxpts); llo = (double *)malloc(sizeof(double) * maxpts);
+            dl = (double *)malloc(sizeof(double) * maxpts);
+        }
 
         #pragma omp for schedule(dynamic, 4)
         for (int m = 0; m < n; m++) {
@@ -679,8 +958,29 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
                 }
             }
             if (w_out) for (int k = 0; k < nt; k++) w_out[(size_t)m * nt + k] = w[k];
-            /* stage 3: model_sums (:2129-2176) and the age sums (age_sums_host, :2447-2511) */
+            /* stage 3: model_sums (:2129-2176) and the age sums (age_sums_host, :2447-2511); with the IMF per
+             * model, the operator's age sums and per-age sums (batch_imf_vectors :1823-1853,
+             * ssp_batch_age_sums_z, model_sums MSUMS_OWN) */
             double ms = 0.0, ls = 0.0;
+            if (c->ops) {
+                double pimf[ORC_IMF_NPAR] = {0};
+                pimf[ORC_IMF_SALP] = c->salp_ind;
+                pimf[ORC_IMF_ALPHA] = p[OPS_P_IMF1];
+                pimf[ORC_IMF_ALPHA + 1] = p[OPS_P_IMF2];
+                pimf[ORC_IMF_ALPHA + 2] = p[OPS_P_IMF3];
+                pimf[ORC_IMF_MDAVE] = c->mdave;
+                pimf[ORC_IMF_LOWER_LIMIT] = p[OPS_P_IMF_LOWER];
+                pimf[ORC_IMF_UPPER_LIMIT] = p[OPS_P_IMF_UPPER];
+                pimf[ORC_IMF_LOWER_BOUND] = p[OPS_P_IMF_LOWER];
+                pimf[ORC_IMF_MLIM_BH] = c->mlim_bh;
+                pimf[ORC_IMF_MLIM_NS] = c->mlim_ns;
+                for (int k = 0; k < nt; k++) t1[k] = w[k] > SAFE_FLOOR ? w[k] : 0.0;
+                orc_ops_model(c->ops, nspec, c->imf_type, c->remnants, pimf, zlo, dz, t1, it, y, o, smass, slum, vy, vo,
+                              wpt, llo, dl, c->op_count ? c->op_count + 2 * m : NULL);
+                for (int k = 0; k < nt; k++)
+                    if (w[k] > SAFE_FLOOR) { ms = ms + w[k] * smass[k]; ls = ls + w[k] * slum[k]; }
+                goto sums_done;
+            }
             double c2 = dz, c1 = 1.0 - c2;
             const double *blo = c->basis + (size_t)(zlo - 1) * nspec * nt;
             const double *bhi = blend ? c->basis + (size_t)zlo * nspec * nt : blo;
@@ -710,6 +1010,7 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
                 else
                     for (int i = 0; i < nspec; i++) acc[i] = acc[i] + wk * (c1 * bl[i]);
             }
+        sums_done:
             if (mass) mass[m] = ms;
             if (lbol) lbol[m] = ls > SAFE_FLOOR ? log10(ls) : 0.0;
             if (young) memcpy(young + (size_t)m * nspec, y, sizeof(double) * nspec);
@@ -774,6 +1075,7 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
         }
         free(p); free(w); free(t1); free(t2); free(y); free(o); free(tot); free(work); free(fobs); free(fb);
         free(slo); free(shi); free(soff); free(scoef); free(tau);
+        free(smass); free(slum); free(vy); free(vo); free(wpt); free(llo); free(dl);
     }
     free(jlo); free(jhi); free(do_band); free(bt); free(wtl); free(ltab);
     return status;

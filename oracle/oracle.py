@@ -54,6 +54,14 @@ def lib():
         L.orc_filter_band.restype = C.c_int
         L.orc_csp_batch.restype = C.c_int
         L.orc_num_threads.restype = C.c_int
+        L.orc_powerlaw_bin.argtypes = [C.c_double] * 4
+        L.orc_powerlaw_bin.restype = C.c_double
+        L.orc_imf_integral.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_int]
+        L.orc_imf_integral.restype = C.c_double
+        L.orc_imf_bin_logs.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_imf_bin_weights_tab.argtypes = [C.c_int, _dp, C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, _dp]
+        L.orc_remnant_mass.argtypes = [C.c_int, _dp, C.c_double]
+        L.orc_remnant_mass.restype = C.c_double
         _lib = L
     return _lib
 
@@ -172,17 +180,76 @@ def build_bands(fset, grid, mode=0):
     return out
 
 
+# The packed IMF parameters of the oracle (csp_oracle.c ORC_IMF_*)
+IMF_SALP, IMF_ALPHA, IMF_VDMC, IMF_MDAVE, IMF_LOWER_LIMIT, IMF_UPPER_LIMIT, IMF_LOWER_BOUND, IMF_MLIM_BH, \
+    IMF_MLIM_NS, IMF_NPAR = 0, 1, 4, 5, 6, 7, 8, 9, 10, 11
+
+
+def imf_vector(slopes=(1.3, 2.3, 2.3), lower=0.08, upper=120.0, salp=2.35, mdave=0.5, mlim_bh=40.0, mlim_ns=8.5):
+    p = np.zeros(IMF_NPAR)
+    p[IMF_SALP] = salp
+    p[IMF_ALPHA:IMF_ALPHA + 3] = slopes
+    p[IMF_MDAVE] = mdave
+    p[IMF_LOWER_LIMIT] = lower
+    p[IMF_UPPER_LIMIT] = upper
+    p[IMF_LOWER_BOUND] = lower
+    p[IMF_MLIM_BH] = mlim_bh
+    p[IMF_MLIM_NS] = mlim_ns
+    return p
+
+
+def powerlaw_bin(plo, lo, hi, e) -> float:
+    return lib().orc_powerlaw_bin(float(plo), float(lo), float(hi), float(e))
+
+
+def imf_integral(imf_type, p, m1, m2, mass_weighted) -> float:
+    return lib().orc_imf_integral(int(imf_type), _d(p), float(m1), float(m2), int(mass_weighted))
+
+
+def imf_bin_logs(m1, m2):
+    a, b = C.c_double(), C.c_double()
+    lib().orc_imf_bin_logs(float(m1), float(m2), C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def imf_bin_weights_tab(imf_type, p, total, mini, m1, m2, first):
+    """Weights of one run of bins (logs formed by imf_bin_logs)."""
+    mini, m1, m2 = _d(mini), _d(m1), _d(m2)
+    llo = np.zeros(mini.size)
+    dl = np.zeros(mini.size)
+    for k in range(mini.size):
+        llo[k], dl[k] = imf_bin_logs(m1[k], m2[k])
+    w = np.zeros(mini.size)
+    lib().orc_imf_bin_weights_tab(int(imf_type), _d(p), float(total), mini.size, mini, m1, m2, llo, dl, _i(first), w)
+    return w
+
+
+def remnant_mass(imf_type, p, max_living_mass) -> float:
+    return lib().orc_remnant_mass(int(imf_type), _d(p), float(max_living_mass))
+
+
+class _Ops(C.Structure):
+    _fields_ = [("nz", C.c_int), ("nt", C.c_int), ("npts", C.c_int), ("ntpl", C.c_int), ("nent", C.c_int),
+                ("age_ptr", C.c_void_p), ("mini", C.c_void_p), ("m1", C.c_void_p), ("m2", C.c_void_p),
+                ("first", C.c_void_p), ("cm", C.c_void_p), ("cl", C.c_void_p), ("q", C.c_void_p),
+                ("age_mmax", C.c_void_p), ("ent_tpl", C.c_void_p), ("ent_val", C.c_void_p), ("etpl", C.c_void_p)]
+
+
 class _Ctx(C.Structure):
     _fields_ = [("nspec", C.c_int), ("nt", C.c_int), ("nz", C.c_int), ("nbands", C.c_int),
                 ("lambda_", C.c_void_p), ("time_full", C.c_void_p), ("logz", C.c_void_p),
                 ("basis", C.c_void_p), ("bmass", C.c_void_p), ("blbol", C.c_void_p), ("bands", C.c_void_p),
                 ("mag_mask", C.c_void_p), ("ncosmo", C.c_int), ("cz", C.c_void_p), ("cdl", C.c_void_p),
-                ("add_igm", C.c_int), ("smooth_fast", C.c_int), ("tiny_logt", C.c_double)]
+                ("add_igm", C.c_int), ("smooth_fast", C.c_int), ("tiny_logt", C.c_double),
+                ("ops", C.c_void_p), ("imf_type", C.c_int), ("remnants", C.c_int), ("salp_ind", C.c_double),
+                ("mdave", C.c_double), ("mlim_bh", C.c_double), ("mlim_ns", C.c_double), ("op_count", C.c_void_p)]
 
 
 def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_mask=None, cosmo=None,
-              want=("mags", "spec", "mass", "lbol")):
-    """Runs the oracle's batch. grid: synth.SSPGrid; bands (nbands, nspec); prm (NPAR, n).
+              want=("mags", "spec", "mass", "lbol"), ops=None, imf_type=2, remnants=1, salp_ind=2.35, mdave=0.5,
+              mlim_bh=40.0, mlim_ns=8.5):
+    """Runs the oracle's batch. grid: synth.SSPGrid; bands (nbands, nspec); prm (NPAR, n); ops: a
+    synth.SSPOperator (the IMF per model; the grid's bases are then unused).
     Returns a dict of outputs (C-order arrays: mags (n, nbands), spec (n, nspec), ...) and the status."""
     keep = []
 
@@ -210,6 +277,21 @@ def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_
     ctx.add_igm = int(add_igm)
     ctx.smooth_fast = int(smooth_fast)
     ctx.tiny_logt = 0.0
+    if ops is not None:
+        o = _Ops()
+        o.nz, o.nt, o.npts, o.ntpl, o.nent = ops.nz, ops.nt, ops.npts, ops.ntpl, ops.nent
+        for k in ("age_ptr", "first", "ent_tpl"):
+            setattr(o, k, ptr(_i(getattr(ops, k))))
+        for k in ("mini", "m1", "m2", "cm", "cl", "q", "age_mmax", "ent_val", "etpl"):
+            setattr(o, k, ptr(_d(getattr(ops, k))))
+        keep.append(o)
+        ctx.ops = C.cast(C.pointer(o), C.c_void_p)
+        ctx.imf_type, ctx.remnants = int(imf_type), int(remnants)
+        ctx.salp_ind, ctx.mdave, ctx.mlim_bh, ctx.mlim_ns = salp_ind, mdave, mlim_bh, mlim_ns
+        if "op_count" in want:
+            cnt = np.zeros((n, 2), np.int32)
+            keep.append(cnt)
+            ctx.op_count = cnt.ctypes.data
     out = {"mags": np.zeros((n, nb))}
     for k, shape in (("spec", (n, grid.nspec)), ("mass", (n,)), ("lbol", (n,)), ("young", (n, grid.nspec)),
                      ("old", (n, grid.nspec)), ("w", (n, grid.nt)), ("fband", (n, nb))):
@@ -228,6 +310,8 @@ def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_
            vp(idx.ctypes.data if idx is not None else None), vp(out["w"].ctypes.data if "w" in out else None),
            vp(out["fband"].ctypes.data if "fband" in out else None))
     out["status"] = st
+    if ops is not None and "op_count" in want:
+        out["op_count"] = cnt
     return out
 
 

@@ -41,3 +41,9 @@ def bands(nbands=20, nspec=5994, seed=7, mode=0):
 @functools.lru_cache(maxsize=None)
 def cosmo():
     return O.cosmology_table(72.0, 0.27, 0.73)
+
+
+@functools.lru_cache(maxsize=None)
+def operator(nz=3, npts=40, ntpl=300, seed=11):
+    g = grid(nz)
+    return synth.ssp_operator(g.lam, g.log_age, nz, npts=npts, ntpl=ntpl, seed=seed)
