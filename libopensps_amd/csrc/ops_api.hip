@@ -145,7 +145,7 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
                   c->d_igm_a, c->d_basis64, c->d_A64, c->d_Ahi, c->d_Alo, c->d_bmeta, c->d_bc64,
                   c->d_bc32, c->d_mask, c->d_cz, c->d_cdl, c->yo.p, c->wcol.p, c->w64.p, c->order.p, c->tiles.p,
                   c->misc.p, c->spec.p, c->flag.p, c->d_rdlam, c->d_ltab32, c->d_wl32, c->d_sm32, c->d_bkt, c->hdr.p, c->wsfh.p,
-                  c->opu.p, c->opwc.p, c->opsum.p};
+                  c->opu.p, c->opwc.p, c->opsum.p, c->opseg.p, c->opimf.p, c->opspart.p};
     for (void* p : ps) free_ptr(p);
     free_operator(c);
     for (int k = 0; k <= OpsCtx::NPROF; k++)
@@ -359,7 +359,8 @@ int opensps_hip_set_mag_mask(opensps_hip_ctx_t* c, int nbands, const int* mask) 
 
 static void free_operator(OpsCtx* c) {
     void* ps[] = {c->op.d_age_ptr, c->op.d_zoff, c->op.d_first, c->op.d_tz_ptr, c->op.d_pt, c->op.d_age_mmax,
-                  c->op.d_te, c->op.d_te_val64, c->op.d_te_val32, c->op.d_A64, c->op.d_Ahi, c->op.d_Alo};
+                  c->op.d_te, c->op.d_te_val64, c->op.d_te_val32, c->op.d_A64, c->op.d_Ahi, c->op.d_Alo,
+                  c->op.d_seg, c->op.d_segoff, c->op.d_tseg, c->op.d_slow_ptr, c->op.d_slow_pt};
     for (void* p : ps) free_ptr(p);
     c->op = OpsCtx::OpsOperator();
 }
@@ -372,6 +373,25 @@ static void bin_logs(double m1, double m2, double* llo, double* dl) {
     *llo = log(m1);
     const double x = (m2 - m1) / m1, u = 1.0 + x;
     *dl = (u == 1.0) ? x : log(u) * x / (u - 1.0);
+}
+
+// The bins k_op_slow takes for the context's IMF type: per (metallicity, age) the first bin, the bins across a
+// power-law break and the empty bins (bin_class 0 or m2 <= m1), in point order
+static int build_slow_list(OpsCtx* c) {
+    const int nzt = c->op.nz * c->op.nt, ty = c->imf_type % 10;
+    std::vector<int> ptr(nzt + 1, 0), pts;
+    for (int a = 0; a < nzt; a++) {
+        for (int k = c->op.h_age_ptr[a]; k < c->op.h_age_ptr[a + 1]; k++) {
+            const double m1 = c->op.h_m1[k], m2 = c->op.h_m2[k];
+            if (c->op.h_first[k] == 1 || !(m2 > m1) || bin_class(ty, m1, m2, c->mdave) == 0) pts.push_back(k);
+        }
+        ptr[a + 1] = (int)pts.size();
+    }
+    c->op.nslow = (int)pts.size();
+    if (pts.empty()) pts.push_back(0);
+    UPLOAD(c, c->op.d_slow_ptr, ptr.data(), ptr.size());
+    UPLOAD(c, c->op.d_slow_pt, pts.data(), pts.size());
+    return 0;
 }
 
 int opensps_hip_set_ssp_ops(opensps_hip_ctx_t* c, const opensps_hip_ssp_ops_t* o) {
@@ -439,6 +459,23 @@ int opensps_hip_set_ssp_ops(opensps_hip_ctx_t* c, const opensps_hip_ssp_ops_t* o
                     tv64[at] = o->ent_val[(size_t)k * ne + j];
                     tv32[at] = (float)tv64[at];
                 }
+    // segments: each template's entries in a metallicity cut into runs of at most OPS_SEG
+    std::vector<int2> seg;
+    std::vector<int> segoff(nz + 1), tseg((size_t)nz * (ntpl + 1));
+    int nseg_max = 1;
+    for (int iz = 0; iz < nz; iz++) {
+        segoff[iz] = (int)seg.size();
+        const int* tp = &tz_ptr[(size_t)iz * (ntpl + 1)];
+        for (int e = 0; e < ntpl; e++) {
+            tseg[(size_t)iz * (ntpl + 1) + e] = (int)seg.size() - segoff[iz];
+            for (int j = tp[e]; j < tp[e + 1]; j += OPS_SEG) seg.push_back(make_int2(j, std::min(tp[e + 1], j + OPS_SEG)));
+        }
+        tseg[(size_t)iz * (ntpl + 1) + ntpl] = (int)seg.size() - segoff[iz];
+        nseg_max = std::max(nseg_max, (int)seg.size() - segoff[iz]);
+    }
+    segoff[nz] = (int)seg.size();
+    if (seg.empty()) seg.push_back(make_int2(0, 0));
+    c->op.nseg_max = nseg_max;
     c->op.nz = nz;
     c->op.nt = nt;
     c->op.npts = np;
@@ -452,7 +489,9 @@ int opensps_hip_set_ssp_ops(opensps_hip_ctx_t* c, const opensps_hip_ssp_ops_t* o
     UPLOAD(c, c->op.d_first, o->first, (size_t)np);
     UPLOAD(c, c->op.d_pt, pt.data(), pt.size());
     UPLOAD(c, c->op.d_age_mmax, o->age_mmax, (size_t)nz * nt);
-    UPLOAD(c, c->op.d_tz_ptr, tz_ptr.data(), tz_ptr.size());
+    UPLOAD(c, c->op.d_seg, seg.data(), seg.size());
+    UPLOAD(c, c->op.d_segoff, segoff.data(), segoff.size());
+    UPLOAD(c, c->op.d_tseg, tseg.data(), tseg.size());
     UPLOAD(c, c->op.d_te, te.data(), te.size());
     UPLOAD(c, c->op.d_te_val64, tv64.data(), tv64.size());
     UPLOAD(c, c->op.d_te_val32, tv32.data(), tv32.size());
@@ -489,6 +528,12 @@ int opensps_hip_set_ssp_ops(opensps_hip_ctx_t* c, const opensps_hip_ssp_ops_t* o
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipFree(lib));
+    c->op.h_age_ptr.assign(o->age_ptr, o->age_ptr + (size_t)nz * nt + 1);
+    c->op.h_first.assign(o->first, o->first + np);
+    c->op.h_m1.assign(o->m1, o->m1 + np);
+    c->op.h_m2.assign(o->m2, o->m2 + np);
+    int st = build_slow_list(c);
+    if (st) return st;
     c->op.on = true;
     return 0;
 }
@@ -506,12 +551,15 @@ int opensps_hip_set_imf(opensps_hip_ctx_t* c, int imf_type, int remnants, double
     const int t = imf_type % 10;
     if (imf_type < 0 || (t != 0 && t != 2 && t != 4))
         return ops_fail(c, 2, "set_imf: IMF type %d is not a power law this build integrates (0, 2, 4)", imf_type);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const bool relist = imf_type % 10 != c->imf_type % 10 || mdave != c->mdave;
     c->imf_type = imf_type;
     c->remnants = remnants != 0;
     c->salp_ind = salp_ind;
     c->mdave = mdave;
     c->mlim_bh = mlim_bh;
     c->mlim_ns = mlim_ns;
+    if (c->op.on && relist) return build_slow_list(c);
     return 0;
 }
 
@@ -580,6 +628,10 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         if ((st = ops_reserve(c, c->opu, tsz * (size_t)npass * maxtiles * c->op.npz * 64))) return st;
         if ((st = ops_reserve(c, c->opwc, tsz * (size_t)npass * maxtiles * c->nt * 64))) return st;
         if ((st = ops_reserve(c, c->opsum, sizeof(double) * 2 * (size_t)npass * n * c->nt))) return st;
+        if ((st = ops_reserve(c, c->opseg, 2 * tsz * (size_t)npass * maxtiles * c->op.nseg_max * 64))) return st;
+        if ((st = ops_reserve(c, c->opimf, sizeof(double) * OPS_IMFC_N * (size_t)n))) return st;
+        if ((st = ops_reserve(c, c->opspart, (3 * sizeof(double) + sizeof(int)) * (size_t)npass * maxtiles * c->nt * 64)))
+            return st;
     }
     HIPCHK(c, hipMemsetAsync(d_ntiles, 0, 64 * sizeof(int), c->stream));
     c->d_last_flag = d_flag;
@@ -677,33 +729,59 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         w.smass_p = smass_p;
         w.slum_p = slum_p;
         w.count = a->op_count;
+        w.impc = (double*)c->opimf.p;
+        w.slow_ptr = c->op.d_slow_ptr;
+        w.slow_pt = c->op.d_slow_pt;
+        w.spart = (double*)c->opspart.p;
+        w.scnt = (int*)(w.spart + 3 * (size_t)npass * maxtiles * c->nt * 64);
         if (a->op_count) HIPCHK(c, hipMemsetAsync(a->op_count, 0, sizeof(int) * 2 * (size_t)n, c->stream));
+        k_op_imf<<<(n + 255) / 256, 256, 0, c->stream>>>(w);
+        HIPCHK(c, hipGetLastError());
         dim3 gw((c->nt + 3) / 4, maxtiles, npass);
-        if (mixed)
+        if (mixed) {
+            k_op_slow<float><<<gw, 256, 0, c->stream>>>(w);
             k_op_weights<float><<<gw, 256, 0, c->stream>>>(w);
-        else
+        } else {
+            k_op_slow<double><<<gw, 256, 0, c->stream>>>(w);
             k_op_weights<double><<<gw, 256, 0, c->stream>>>(w);
+        }
+        HIPCHK(c, hipGetLastError());
+        OpGArgs g;
+        g.nt = c->nt;
+        g.npass = npass;
+        g.npz = c->op.npz;
+        g.maxtiles = maxtiles;
+        g.nseg_max = c->op.nseg_max;
+        g.ntiles = d_ntiles;
+        g.tiles = (const int4*)c->tiles.p;
+        g.order = d_order;
+        g.it = d_it;
+        g.segoff = c->op.d_segoff;
+        g.seg = c->op.d_seg;
+        g.te = c->op.d_te;
+        g.te_val64 = c->op.d_te_val64;
+        g.te_val32 = c->op.d_te_val32;
+        g.u = c->opu.p;
+        g.wc = c->opwc.p;
+        g.part = c->opseg.p;
+        dim3 gg((c->op.nseg_max + 3) / 4, maxtiles, npass);
+        if (mixed)
+            k_op_segments<float><<<gg, 256, 0, c->stream>>>(g);
+        else
+            k_op_segments<double><<<gg, 256, 0, c->stream>>>(g);
         HIPCHK(c, hipGetLastError());
         OpFArgs f;
-        f.n = n;
-        f.nt = c->nt;
-        f.npass = npass;
-        f.npz = c->op.npz;
-        f.maxtiles = maxtiles;
         f.ntpl = c->op.ntpl;
+        f.npass = npass;
+        f.maxtiles = maxtiles;
+        f.nseg_max = c->op.nseg_max;
         f.kpad = kpad;
         f.parts = parts;
-        f.mixed = mixed;
         f.ntiles = d_ntiles;
         f.tiles = (const int4*)c->tiles.p;
         f.order = d_order;
-        f.it = d_it;
-        f.tz_ptr = c->op.d_tz_ptr;
-        f.te = c->op.d_te;
-        f.te_val64 = c->op.d_te_val64;
-        f.te_val32 = c->op.d_te_val32;
-        f.u = c->opu.p;
-        f.wc = c->opwc.p;
+        f.tseg = c->op.d_tseg;
+        f.part = c->opseg.p;
         f.whi = mixed ? (uint16_t*)c->wcol.p : nullptr;
         f.wlo = mixed ? (uint16_t*)c->wcol.p + (size_t)n * parts * kpad : nullptr;
         f.w64 = mixed ? nullptr : (double*)c->w64.p;

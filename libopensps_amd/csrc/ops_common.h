@@ -5,6 +5,14 @@
 
 #include "../../include/opensps_params.h"
 
+// LDS written by some lanes of a wavefront and read by others: a compiler barrier plus wavefront-scope fences
+#define WAVE_SYNC()                                           \
+    do {                                                      \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); \
+        __builtin_amdgcn_wave_barrier();                      \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
+    } while (0)
+
 #define OPS_SAFE_FLOOR 2.2250738585072014e-308   // tiny(1.0_wp)
 #define OPS_C_LIGHT 2.99792458e18                // Angstrom / s
 #define OPS_LY_ALPHA 1215.67

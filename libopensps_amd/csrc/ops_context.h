@@ -52,10 +52,19 @@ struct OpsCtx {
         int *d_age_ptr = nullptr, *d_zoff = nullptr, *d_first = nullptr, *d_tz_ptr = nullptr;
         double *d_pt = nullptr, *d_age_mmax = nullptr;     // points: OPS_PT_N doubles each
         int2* d_te = nullptr;                              // entries by template: {point in its metallicity, age}
+        int2* d_seg = nullptr;                             // segments (<= OPS_SEG entries of one template): entry range
+        int *d_segoff = nullptr, *d_tseg = nullptr;        // first segment per metallicity; segments per template
+        int nseg_max = 0;                                  // the most segments of one metallicity
         double* d_te_val64 = nullptr;
         float* d_te_val32 = nullptr;
         double* d_A64 = nullptr;                           // the library as the product's A operand
         uint16_t *d_Ahi = nullptr, *d_Alo = nullptr;
+        // the bins off the common path for the context's IMF type (first bins, bins across a break, empty bins):
+        // per (metallicity, age) CSR of point indices, rebuilt when the IMF type or mdave changes
+        std::vector<int> h_age_ptr, h_first;
+        std::vector<double> h_m1, h_m2;
+        int *d_slow_ptr = nullptr, *d_slow_pt = nullptr;
+        int nslow = 0;
     } op;
     int imf_type = 2, remnants = 1;
     double salp_ind = 2.35, mdave = 0.5, mlim_bh = 40.0, mlim_ns = 8.5;
@@ -83,7 +92,9 @@ struct OpsCtx {
     // ---- batch scratch (grown on demand) ----
     OpsBuffer yo, wcol, w64, order, tiles, misc, spec, flag;
     OpsBuffer hdr;   // per-model constants of the chain (OPS_HDR_N doubles per model)
-    OpsBuffer wsfh, opu, opwc, opsum;   // the operator path: SFH weights, point weights, age factors, per-age sums
+    OpsBuffer wsfh, opu, opwc, opsum, opseg;   // the operator path: SFH weights, point weights, age factors,
+                                               // per-age sums, segment sums
+    OpsBuffer opimf, opspart;                  // per-model IMF constants; the off-path bins' per-age sums and counts
     int* d_last_flag = nullptr;   // the per-batch status flag of the last batch (read by sync)
 
     // ---- per-kernel timing (HIP events on the context's stream) ----

@@ -16,13 +16,6 @@
 // SFH weights (src/physics/opensps_sfh.f90). FP64 with contraction off, so that each operation
 // rounds as the reference's CPU build rounds it.
 // ------------------------------------------------------------------------------------------------
-// LDS written by some lanes of a wavefront and read by others: a compiler barrier plus wavefront-scope fences
-#define WAVE_SYNC()                                           \
-    do {                                                      \
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); \
-        __builtin_amdgcn_wave_barrier();                      \
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); \
-    } while (0)
 
 #pragma clang fp contract(off)
 

@@ -69,7 +69,8 @@ struct PackArgs {
 // ---- the SSP operator with the IMF per model ----
 #define OPS_PT_N 8         // per point: mini, m1, m2, ln m1, ln(m2/m1), surviving mass, L_bol, ionizing photons
 #define OPS_PIECE 32       // points per piece of an age (the IMF weights restart their chain of shared edges)
-#define OPS_TPL_BLOCK 8    // templates per wavefront of the fold
+#define OPS_TPL_BLOCK 8    // templates per wavefront of the fold's sums
+#define OPS_SEG 64         // entries per segment of a template's list (one wavefront of the fold each)
 
 struct OpWArgs {           // k_op_weights
     int n, nt, npass, npz, maxtiles, imf_type, remnants;
@@ -83,18 +84,53 @@ struct OpWArgs {           // k_op_weights
     void *u, *wc;          // per (pass, tile): the points' weights (npz x 64), the ages' SFH x blend factors (nt x 64)
     double *smass_p, *slum_p;   // (pass, model, age)
     int* count;                 // (2, n) points with a nonzero weight per pass, or NULL
+    double* impc;               // per model OPS_IMFC_N: 1 / total mass integral, the segments' e, c, c / e
+    const int *slow_ptr, *slow_pt;   // per (metallicity, age) the bins that leave the common path
+    double* spart;              // (pass, tile, age): their sums and the remnants, 3 x 64 per wavefront
+    int* scnt;                  // (pass, tile, age): their nonzero weights, 64 per wavefront
 };
+// The power-law segment of a bin [lo, hi] (imf_bin_weights_tab's classification, :483-498): 0 when the bin
+// crosses a break or lies below 0.08 M_sun (the general integral then)
+__host__ __device__ inline int bin_class(int t, double lo, double hi, double mdave) {
+    if (t == 0) return 1;
+    if (t == 2) {
+        if (lo >= 0.08 && hi <= 0.5) return 1;
+        if (lo >= 0.5 && hi <= 1.0) return 2;
+        if (lo >= 1.0) return 3;
+        return 0;
+    }
+    if (lo >= 0.08 && hi <= mdave) return 1;
+    if (lo >= mdave) return 2;
+    return 0;
+}
 
-struct OpFArgs {           // k_op_fold
-    int n, nt, npass, npz, maxtiles, ntpl, kpad, parts, mixed;
+#define OPS_IMFC_INV_TOTAL 0
+#define OPS_IMFC_E 1
+#define OPS_IMFC_C 4
+#define OPS_IMFC_CE 7
+#define OPS_IMFC_N 10
+
+struct OpGArgs {           // k_op_segments: one segment's entries per wavefront
+    int nt, npass, npz, maxtiles, nseg_max;
     const int* ntiles;
     const int4* tiles;
     const int *order, *it;
-    const int* tz_ptr;     // nz x (ntpl + 1): entries of each template per metallicity
+    const int* segoff;     // nz + 1: first segment of each metallicity
+    const int2* seg;       // per segment its entry range
     const int2* te;        // per entry {point (local to its metallicity), age}
     const double* te_val64;
     const float* te_val32;
     const void *u, *wc;
+    void* part;            // (pass, tile, segment): per lane {young, old}
+};
+
+struct OpFArgs {           // k_op_fold: per template the sums of its segments, as the product's B operand
+    int ntpl, npass, maxtiles, nseg_max, kpad, parts;
+    const int* ntiles;
+    const int4* tiles;
+    const int* order;
+    const int* tseg;       // nz x (ntpl + 1): segments of each template per metallicity
+    const void* part;
     uint16_t *whi, *wlo;   // the B operand of the template product (split BF16), [column][kpad]
     double* w64;           // or FP64
 };
@@ -143,6 +179,11 @@ __global__ void k_observed_frame(int n, int nspec, const double* zc, const doubl
 __global__ void k_selftest_mfma(int which, const void* a, const void* b, void* d);
 template <typename T>
 __global__ void k_op_weights(OpWArgs a);
+template <typename T>
+__global__ void k_op_slow(OpWArgs a);
+__global__ void k_op_imf(OpWArgs a);
+template <typename T>
+__global__ void k_op_segments(OpGArgs a);
 template <typename T>
 __global__ void k_op_fold(OpFArgs a);
 __global__ void k_op_sums(OpSArgs a);

@@ -8,10 +8,12 @@
 //                 imf_bin_weights_tab, src/physics/opensps_imf.f90:439-559); per-age mass, L_bol and
 //                 photon sums with the remnants (add_remnant_mass_p,
 //                 src/physics/opensps_stellar_modifications.f90:729-816)
-//   k_op_fold     one wavefront per (tile, block of 8 templates): the entries of each template (CSR by
-//                 template, sorted by age) times the points' weights and the SFH weights, split young / old
-//                 at it_tesc (fold_multi, opensps_ssp_batch.f90:3095-3188), both metallicity passes summed;
-//                 written as the B operand of the template product (split BF16 [column][K] or FP64)
+//   k_op_segments one wavefront per (segment of <= 64 entries of one template's list, tile, pass): the
+//                 entries (CSR by template, sorted by age) times the points' weights and the SFH weights,
+//                 split young / old at it_tesc (fold_multi's segments, opensps_ssp_batch.f90:3095-3163)
+//   k_op_fold     one wavefront per (tile, block of 8 templates): the segments' sums per template over both
+//                 metallicity passes (:3164-3186), written as the B operand of the template product (split
+//                 BF16 [column][K] or FP64)
 //   k_op_sums     per model: the surviving mass and L_bol from the per-age sums (model_sums MSUMS_OWN,
 //                 opensps_csp_batch.f90:2129-2176)
 //   k_op_tiles    the uniform column tiles of the template product (one shared template library)
@@ -70,8 +72,9 @@ __device__ double d_integrate_powerlaw(double lo, double hi, double alpha, bool 
     return coeff * d_powerlaw_bin(pow(lo, denom), lo, hi, denom);
 }
 
-// imf_integral (opensps_imf.f90:206-315) for the power-law types 0 (Salpeter), 2 (Kroupa), 4 (Dave)
-__device__ double d_imf_integral(int type, const ImfP& p, double m1, double m2, bool mw) {
+// imf_integral (opensps_imf.f90:206-315) for the power-law types 0 (Salpeter), 2 (Kroupa), 4 (Dave). Not inlined:
+// the weights' hot loop calls it only for bins across a break, and keeps its registers for the common path.
+__device__ __attribute__((noinline)) double d_imf_integral(int type, const ImfP& p, double m1, double m2, bool mw) {
     const double lo = fmin(m1, m2), hi = fmax(m1, m2);
     double v = 0.0;
     if (hi <= lo) return 0.0;
@@ -96,7 +99,7 @@ __device__ double d_imf_integral(int type, const ImfP& p, double m1, double m2, 
 }
 
 // add_remnant_mass_p (opensps_stellar_modifications.f90:729-816): black holes, neutron stars, white dwarfs
-__device__ double d_remnant_mass(int type, const ImfP& p, double mmax) {
+__device__ __attribute__((noinline)) double d_remnant_mass(int type, const ImfP& p, double mmax) {
     const double hi = p.upper, lbh = p.mlim_bh, lns = p.mlim_ns;
     const double norm = d_imf_integral(type, p, p.lower, hi, true);
     double add = 0.0;
@@ -128,9 +131,35 @@ __device__ __forceinline__ void bin_reset(BinState& s) {
     s.cur = 0; s.e = 0.0; s.c = 0.0; s.d = 1.0; s.ce = 0.0; s.prev_hi = -1.0; s.prev_phi = 0.0;
 }
 
+// A new power-law segment's constants (imf_bin_weights_tab, opensps_imf.f90:500-525), once per segment change
+__device__ __attribute__((noinline)) void bin_segment(int t, const ImfP& p, int seg, BinState& s) {
+    double alpha, coeff = 1.0;
+    if (t == 0) {
+        alpha = p.salp;
+    } else if (t == 2) {
+        alpha = seg == 1 ? p.a1 : (seg == 2 ? p.a2 : p.a3);
+        if (seg > 1) coeff = pow(0.5, -p.a1 + p.a2);
+    } else {
+        alpha = seg == 1 ? p.a1 : p.a2;
+        if (seg > 1) coeff = pow(p.mdave, -p.a1 + p.a2);
+    }
+    s.e = 1.0 - alpha; s.c = coeff; s.d = 1.0 - alpha;
+    s.ce = 0.0;
+    if (s.e != 0.0) s.ce = s.c / s.e;
+    s.cur = seg;
+    s.prev_hi = -1.0;
+}
+
+// The first bin of an age (lower edge at the IMF's lower bound): the difference of powers (:531-535)
+__device__ __attribute__((noinline)) double bin_first(const BinState& s, double lo, double hi) {
+    if (fabs(s.d) <= 1.0e-12) return s.c * log(hi / lo);
+    return s.c * (pow(hi, s.e) - pow(lo, s.e)) / s.d;
+}
+
 // One bin of imf_bin_weights_tab (opensps_imf.f90:466-557), FP64 build: stars per unit initial mass. The
 // segment classification and the limits decide which bins weigh nothing, exactly as the reference.
-__device__ double d_bin_weight(int type, const ImfP& p, double inv_total, double mini, double m1, double m2,
+__device__ __attribute__((noinline)) double d_bin_weight(int type, const ImfP& p, double inv_total, double mini, double m1,
+                                                      double m2,
                                double llo, double dl, int first, BinState& s) {
     if (mini < p.lower || mini > p.upper) return 0.0;
     const double lo = first == 1 ? p.lower_bound : m1, hi = m2;
@@ -147,28 +176,11 @@ __device__ double d_bin_weight(int type, const ImfP& p, double inv_total, double
         if (lo >= 0.08 && hi <= p.mdave) seg = 1;
         else if (lo >= p.mdave) seg = 2;
     }
-    if (seg > 0 && seg != s.cur) {
-        double alpha, coeff = 1.0;
-        if (t == 0) {
-            alpha = p.salp;
-        } else if (t == 2) {
-            alpha = seg == 1 ? p.a1 : (seg == 2 ? p.a2 : p.a3);
-            if (seg > 1) coeff = pow(0.5, -p.a1 + p.a2);
-        } else {
-            alpha = seg == 1 ? p.a1 : p.a2;
-            if (seg > 1) coeff = pow(p.mdave, -p.a1 + p.a2);
-        }
-        s.e = 1.0 - alpha; s.c = coeff; s.d = 1.0 - alpha;
-        s.ce = 0.0;
-        if (s.e != 0.0) s.ce = s.c / s.e;
-        s.cur = seg;
-        s.prev_hi = -1.0;
-    }
+    if (seg > 0 && seg != s.cur) bin_segment(t, p, seg, s);
     double val;
     if (seg > 0 && s.cur == seg) {
         if (first == 1) {
-            if (fabs(s.d) <= 1.0e-12) val = s.c * log(hi / lo);
-            else val = s.c * (pow(hi, s.e) - pow(lo, s.e)) / s.d;
+            val = bin_first(s, lo, hi);
         } else if (s.e == 0.0) {
             val = s.c * dl;
         } else {
@@ -186,20 +198,9 @@ __device__ double d_bin_weight(int type, const ImfP& p, double inv_total, double
     return val * inv_total;
 }
 
-// ------------------------------------------------------------------------------------------------
-// k_op_weights: grid (ceil(nt / 4), max tiles, passes), 256 threads = 4 ages. Pass 0 takes the operator of
-// metallicity zlo with weight 1 - dz, pass 1 that of zlo + 1 with weight dz (one pass with one metallicity).
-// ------------------------------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int t = blockIdx.x * 4 + wv, tile = blockIdx.y, pass = blockIdx.z;
-    if (t >= a.nt || tile >= *a.ntiles) return;
-    const int4 td = a.tiles[tile];   // {metallicity pair (zlo - 1), first column, columns}
-    const int iz = a.npass == 1 ? 0 : td.x + pass;
-    const bool valid = lane < td.z;
-    const int m = a.order[td.y + (valid ? lane : 0)];
-    const int n = a.n, nt = a.nt;
+// The model's IMF from its parameter slots and the context's settings
+__device__ __forceinline__ ImfP model_imf(const OpWArgs& a, int m) {
+    const int n = a.n;
     ImfP p;
     p.salp = a.salp_ind;
     p.a1 = a.prm[(size_t)OPS_P_IMF1 * n + m];
@@ -211,22 +212,184 @@ __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
     p.lower_bound = p.lower;
     p.mlim_bh = a.mlim_bh;
     p.mlim_ns = a.mlim_ns;
+    return p;
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_op_imf: one thread per model: the total mass integral over [lower, upper] (the weights' normalisation) and
+// each power-law segment's constants e = 1 - alpha, c, c / e (bin_segment), formed once per model.
+// ------------------------------------------------------------------------------------------------
+__global__ void k_op_imf(OpWArgs a) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.n) return;
+    const ImfP p = model_imf(a, m);
+    const int t = a.imf_type % 10;
+    const double tot = d_imf_integral(a.imf_type, p, p.lower, p.upper, true);
+    double* o = a.impc + (size_t)m * OPS_IMFC_N;
+    o[OPS_IMFC_INV_TOTAL] = tot > 0.0 ? 1.0 / tot : 1.0;
+    for (int seg = 1; seg <= 3; seg++) {
+        BinState s;
+        bin_reset(s);
+        if (t == 0 && seg > 1) {
+            s.e = 0.0; s.c = 0.0; s.ce = 0.0;
+        } else if (t == 4 && seg > 2) {
+            s.e = 0.0; s.c = 0.0; s.ce = 0.0;
+        } else {
+            bin_segment(t, p, seg, s);
+        }
+        o[OPS_IMFC_E + seg - 1] = s.e;
+        o[OPS_IMFC_C + seg - 1] = s.c;
+        o[OPS_IMFC_CE + seg - 1] = s.ce;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_op_slow: grid (ceil(nt / 4), max tiles, passes), one wavefront per (tile, age, pass) as k_op_weights. The
+// age's bins that leave the common path (its first bin, whose lower edge is each model's lower limit; bins
+// across a break; empty bins; listed per age by the host) by the full imf_bin_weights_tab per lane, and the
+// age's remnants: their weights go to the points' rows, their sums to the age's partial sums, which k_op_weights
+// adds.
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_op_slow(OpWArgs a) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int t = blockIdx.x * 4 + wv, tile = blockIdx.y, pass = blockIdx.z;
+    if (t >= a.nt || tile >= *a.ntiles) return;
+    const int4 td = a.tiles[tile];
+    const int iz = a.npass == 1 ? 0 : td.x + pass;
+    const bool valid = lane < td.z;
+    const int m = a.order[td.y + (valid ? lane : 0)];
+    const int nt = a.nt;
+    const ImfP p = model_imf(a, m);
+    const double inv_total = a.impc[(size_t)m * OPS_IMFC_N + OPS_IMFC_INV_TOTAL];
+    const int zo = a.zoff[iz];
+    T* u = (T*)a.u + ((size_t)pass * a.maxtiles + tile) * a.npz * 64 + lane;
+    double sm = 0.0, sl = 0.0, sq = 0.0;
+    int cnt = 0;
+    const int j0 = a.slow_ptr[iz * nt + t], j1 = a.slow_ptr[iz * nt + t + 1];
+    for (int j = j0; j < j1; j++) {
+        const int k = a.slow_pt[j];
+        const double* pt = a.pt + (size_t)k * OPS_PT_N;
+        BinState s;
+        bin_reset(s);
+        const double w = d_bin_weight(a.imf_type, p, inv_total, pt[0], pt[1], pt[2], pt[3], pt[4], a.first[k], s);
+        u[(size_t)(k - zo) * 64] = (T)w;
+        cnt += w > 0.0 ? 1 : 0;
+        sm = sm + pt[5] * w;
+        sl = sl + pt[6] * w;
+        sq = sq + pt[7] * w;
+    }
+    const int k0 = a.age_ptr[iz * nt + t], k1 = a.age_ptr[iz * nt + t + 1];
+    if (a.remnants && k1 > k0) sm = sm + d_remnant_mass(a.imf_type, p, a.age_mmax[iz * nt + t]);
+    double* sp = a.spart + (((size_t)pass * a.maxtiles + tile) * nt + t) * 3 * 64 + lane;
+    sp[0] = sm;
+    sp[64] = sl;
+    sp[128] = sq;
+    a.scnt[(((size_t)pass * a.maxtiles + tile) * nt + t) * 64 + lane] = cnt;
+}
+
+#pragma clang fp contract(on)
+// ------------------------------------------------------------------------------------------------
+// k_op_weights: grid (ceil(nt / 4), max tiles, passes), 256 threads = 4 ages. Pass 0 takes the operator of
+// metallicity zlo with weight 1 - dz, pass 1 that of zlo + 1 with weight dz (one pass with one metallicity).
+// The common bins of an age: not its first, inside one power-law segment. Their segment is then the same for
+// every lane (the bins are the points', the breaks the IMF type's), so the wavefront stays together; per lane
+// only the limits, a segment change and a broken chain of shared edges differ, each taken by the lanes that
+// need it (imf_bin_weights_tab's chain: hi^e = lo^e (1 + expm1(e dl)), exp only where the chain breaks; expm1 by
+// the series with fused multiply-adds). The bins k_op_slow takes reset the chain as the reference's do.
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int t = blockIdx.x * 4 + wv, tile = blockIdx.y, pass = blockIdx.z;
+    if (t >= a.nt || tile >= *a.ntiles) return;
+    const int4 td = a.tiles[tile];   // {metallicity pair (zlo - 1), first column, columns}
+    const int iz = a.npass == 1 ? 0 : td.x + pass;
+    const bool valid = lane < td.z;
+    const int m = a.order[td.y + (valid ? lane : 0)];
+    const int n = a.n, nt = a.nt, ty = a.imf_type % 10;
+    const double lower = a.prm[(size_t)OPS_P_IMF_LOWER * n + m], upper = a.prm[(size_t)OPS_P_IMF_UPPER * n + m];
+    const double* ic = a.impc + (size_t)m * OPS_IMFC_N;
+    const double inv_total = ic[OPS_IMFC_INV_TOTAL];
+    double se[3], sc[3], sce[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        se[q] = ic[OPS_IMFC_E + q];
+        sc[q] = ic[OPS_IMFC_C + q];
+        sce[q] = ic[OPS_IMFC_CE + q];
+    }
     const double dz = a.hdr[(size_t)m * OPS_HDR_N + OPS_HDR_DZ];
     const double cz = a.npass == 1 ? 1.0 : (pass == 0 ? 1.0 - dz : dz);
-    const double tot = d_imf_integral(a.imf_type, p, p.lower, p.upper, true);
-    const double inv_total = tot > 0.0 ? 1.0 / tot : 1.0;
     const int k0 = a.age_ptr[iz * nt + t], k1 = a.age_ptr[iz * nt + t + 1], zo = a.zoff[iz];
     T* u = (T*)a.u + ((size_t)pass * a.maxtiles + tile) * a.npz * 64 + lane;
     double sm = 0.0, sl = 0.0, sq = 0.0;
     int cnt = 0;
-    BinState s;
-    for (int q0 = k0; q0 < k1; q0 += OPS_PIECE) {
+    // The pieces' point data pass through this wavefront's two LDS buffers: the lanes load the next piece
+    // (contiguous, 4 doubles and one flag a lane) while the current one is computed from the other buffer
+    __shared__ double spt[4][2][OPS_PIECE * OPS_PT_N];
+    __shared__ int sfirst[4][2][OPS_PIECE];
+    double r[4];
+    int rf = 0;
+    auto fetch = [&](int q0) {
+        const int nv = min(k1 - q0, OPS_PIECE) * OPS_PT_N;
+#pragma unroll
+        for (int i = 0; i < 4; i++) r[i] = lane + 64 * i < nv ? a.pt[(size_t)q0 * OPS_PT_N + lane + 64 * i] : 0.0;
+        rf = lane < min(k1 - q0, OPS_PIECE) ? a.first[q0 + lane] : 0;
+    };
+    auto stash = [&](int b) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) spt[wv][b][lane + 64 * i] = r[i];
+        if (lane < OPS_PIECE) sfirst[wv][b][lane] = rf;
+    };
+    if (k0 < k1) {
+        fetch(k0);
+        stash(0);
+    }
+    WAVE_SYNC();
+    int b = 0;
+    for (int q0 = k0; q0 < k1; q0 += OPS_PIECE, b ^= 1) {
         const int q1 = min(k1, q0 + OPS_PIECE);
-        bin_reset(s);
+        if (q1 < k1) fetch(q1);
+        int cur = 0;                   // the chain restarts with each piece (bin_reset)
+        double prev_hi = -1.0, prev_phi = 0.0;
         double pm = 0.0, pl = 0.0, pq = 0.0;
+        const double* pb = spt[wv][b];
         for (int k = q0; k < q1; k++) {
-            const double* pt = a.pt + (size_t)k * OPS_PT_N;
-            const double w = d_bin_weight(a.imf_type, p, inv_total, pt[0], pt[1], pt[2], pt[3], pt[4], a.first[k], s);
+            const double* pt = pb + (k - q0) * OPS_PT_N;
+            const double mini = pt[0], m1 = pt[1], m2 = pt[2];
+            const int seg = (sfirst[wv][b][k - q0] == 1 || !(m2 > m1)) ? 0 : bin_class(ty, m1, m2, a.mdave);
+            if (seg == 0) {
+                // k_op_slow's bin: the chain restarts after it (an empty bin changes nothing)
+                if (m2 > m1 || sfirst[wv][b][k - q0] == 1) {
+                    prev_hi = -1.0;
+                    cur = -1;
+                }
+                continue;
+            }
+            const bool inside = !(mini < lower || mini > upper);
+            if (inside && seg != cur) {
+                cur = seg;
+                prev_hi = -1.0;
+            }
+            const double e = seg == 1 ? se[0] : (seg == 2 ? se[1] : se[2]);
+            const double c = seg == 1 ? sc[0] : (seg == 2 ? sc[1] : sc[2]);
+            const double ce = seg == 1 ? sce[0] : (seg == 2 ? sce[1] : sce[2]);
+            const double dl = pt[4];
+            double val = c * dl;       // e == 0: c ln(hi / lo)
+            if (e != 0.0) {
+                const bool hit = m1 == prev_hi;
+                double plo = prev_phi;
+                if (__any(inside && !hit)) plo = hit ? plo : exp(e * pt[3]);
+                const double y = e * dl;
+                double em1 = d_expm1_series(y);
+                if (__any(inside && !(fabs(y) < 0.25))) em1 = fabs(y) < 0.25 ? em1 : exp(y) - 1.0;
+                val = ce * plo * em1;
+                if (inside) {
+                    prev_hi = m2;
+                    prev_phi = plo + plo * em1;
+                }
+            }
+            const double w = inside ? val * inv_total : 0.0;
             u[(size_t)(k - zo) * 64] = (T)w;
             cnt += w > 0.0 ? 1 : 0;
             pm = pm + pt[5] * w;
@@ -234,8 +397,16 @@ __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
             pq = pq + pt[7] * w;
         }
         sm = sm + pm; sl = sl + pl; sq = sq + pq;
+        if (q1 < k1) stash(b ^ 1);
+        WAVE_SYNC();
     }
-    if (a.remnants && k1 > k0) sm = sm + d_remnant_mass(a.imf_type, p, a.age_mmax[iz * nt + t]);
+    // the slow bins' and the remnants' sums (k_op_slow)
+    const size_t ra = ((size_t)pass * a.maxtiles + tile) * nt + t;
+    const double* sp = a.spart + ra * 3 * 64 + lane;
+    sm = sm + sp[0];
+    sl = sl + sp[64];
+    sq = sq + sp[128];
+    cnt += a.scnt[ra * 64 + lane];
     if (valid) {
         const size_t r = ((size_t)pass * n + m) * nt + t;
         a.smass_p[r] = cz * sm;
@@ -243,10 +414,11 @@ __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
         if (a.count && cnt && cz != 0.0) atomicAdd(a.count + 2 * m + pass, cnt);
     }
     const double ws = a.wsfh[(size_t)m * nt + t];
-    T* wc = (T*)a.wc + (((size_t)pass * a.maxtiles + tile) * nt + t) * 64 + lane;
+    T* wc = (T*)a.wc + (ra * 64) + lane;
     *wc = (T)((ws > OPS_SAFE_FLOOR ? ws : 0.0) * cz);
     (void)sq;   // the ionizing photons serve nebular emission, which this build does not add
 }
+#pragma clang fp contract(off)
 
 // k_op_sums: one thread per model; the passes' per-age sums, then model_sums (MSUMS_OWN)
 __global__ void k_op_sums(OpSArgs a) {
@@ -273,47 +445,103 @@ __global__ void k_op_sums(OpSArgs a) {
 
 #pragma clang fp contract(on)
 
+template <typename T> struct Pair;
+template <> struct Pair<float> { typedef float2 type; };
+template <> struct Pair<double> { typedef double2 type; };
+
 // ------------------------------------------------------------------------------------------------
-// k_op_fold: grid (ceil(kpad / 32), max tiles), 256 threads = 4 blocks of 8 templates. Per template and pass
-// the entries (point, age, value) are uniform across the wavefront; each lane reads its model's point weight
-// and SFH x blend factor (256-byte rows) and sums young (age <= it_tesc) and old in the kernel precision.
+// k_op_segments: grid (ceil(nseg_max / 4), max tiles, passes), 256 threads = 4 segments. A segment is at most
+// OPS_SEG consecutive entries of one template's list (sorted by age) in one metallicity, so every wavefront has
+// about the same work however unevenly the templates are used. The entries (point, age, value) are uniform across
+// the wavefront; they are taken eight at a time, then each lane gathers its model's point weights and SFH x blend
+// factors for the eight (256-byte rows) before summing young (age <= it_tesc) and old in the kernel precision.
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_op_segments(OpGArgs a) {
+    typedef typename Pair<T>::type T2;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int sg = blockIdx.x * 4 + wv, tile = blockIdx.y, pass = blockIdx.z;
+    if (tile >= *a.ntiles) return;
+    const int4 td = a.tiles[tile];
+    const int iz = a.npass == 1 ? 0 : td.x + pass;
+    if (sg >= a.segoff[iz + 1] - a.segoff[iz]) return;
+    const bool valid = lane < td.z;
+    const int m = a.order[td.y + (valid ? lane : 0)];
+    const int it = a.it[m];
+    const int2 rng = a.seg[a.segoff[iz] + sg];
+    const T* u = (const T*)a.u + ((size_t)pass * a.maxtiles + tile) * a.npz * 64 + lane;
+    const T* wc = (const T*)a.wc + ((size_t)pass * a.maxtiles + tile) * a.nt * 64 + lane;
+    // lane l holds entry rng.x + l (a segment has at most 64), then the entries go round by broadcast
+    int2 me = make_int2(0, 0);
+    T mv = (T)0;
+    if (rng.x + lane < rng.y) {
+        me = a.te[rng.x + lane];
+        if constexpr (std::is_same<T, float>::value)
+            mv = a.te_val32[rng.x + lane];
+        else
+            mv = a.te_val64[rng.x + lane];
+    }
+    const int ne = rng.y - rng.x;
+    T y = (T)0, o = (T)0;
+    for (int j = 0; j < ne; j += 8) {
+        int px[8], ag[8];
+        T val[8], uu[8], ww[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int src = min(j + q, 63);
+            px[q] = __shfl(me.x, src, 64);
+            ag[q] = __shfl(me.y, src, 64);
+            val[q] = j + q < ne ? __shfl(mv, src, 64) : (T)0;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            uu[q] = u[(size_t)px[q] * 64];
+            ww[q] = wc[(size_t)ag[q] * 64];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const T x = val[q] * (uu[q] * ww[q]);
+            if (ag[q] + 1 <= it) y += x;
+            else o += x;
+        }
+    }
+    T2 r;
+    r.x = y;
+    r.y = o;
+    ((T2*)a.part)[(((size_t)pass * a.maxtiles + tile) * a.nseg_max + sg) * 64 + lane] = r;
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_op_fold: grid (ceil(kpad / 32), max tiles), 256 threads = 4 blocks of 8 templates: per template the sums of
+// its segments over both metallicity passes (fold_multi's per-template sum, opensps_ssp_batch.f90:3164-3186),
+// written as the B operand of the template product (split BF16 [column][K], or FP64), young and old apart or
+// summed.
 // ------------------------------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_op_fold(OpFArgs a) {
+    typedef typename Pair<T>::type T2;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int e0 = (blockIdx.x * 4 + wv) * OPS_TPL_BLOCK, tile = blockIdx.y;
     if (e0 >= a.kpad || tile >= *a.ntiles) return;
     const int4 td = a.tiles[tile];
     const bool valid = lane < td.z;
     const int m = a.order[td.y + (valid ? lane : 0)];
-    const int it = a.it[m], nt = a.nt;
     T vy[OPS_TPL_BLOCK], vo[OPS_TPL_BLOCK];
 #pragma unroll
     for (int q = 0; q < OPS_TPL_BLOCK; q++) { vy[q] = (T)0; vo[q] = (T)0; }
     for (int pass = 0; pass < a.npass; pass++) {
         const int iz = a.npass == 1 ? 0 : td.x + pass;
-        const T* u = (const T*)a.u + ((size_t)pass * a.maxtiles + tile) * a.npz * 64 + lane;
-        const T* wc = (const T*)a.wc + ((size_t)pass * a.maxtiles + tile) * nt * 64 + lane;
-        const int* ptr = a.tz_ptr + (size_t)iz * (a.ntpl + 1);
+        const int* ts = a.tseg + (size_t)iz * (a.ntpl + 1);
+        const T2* P = (const T2*)a.part + ((size_t)pass * a.maxtiles + tile) * a.nseg_max * 64 + lane;
 #pragma unroll
         for (int q = 0; q < OPS_TPL_BLOCK; q++) {
             const int e = e0 + q;
             if (e >= a.ntpl) break;
-            const int j0 = ptr[e], j1 = ptr[e + 1];
-            T y = (T)0, o = (T)0;
-            for (int j = j0; j < j1; j++) {
-                const int2 pe = a.te[j];
-                T val;
-                if constexpr (std::is_same<T, float>::value)
-                    val = a.te_val32[j];
-                else
-                    val = a.te_val64[j];
-                const T x = val * (u[(size_t)pe.x * 64] * wc[(size_t)pe.y * 64]);
-                if (pe.y + 1 <= it) y += x;
-                else o += x;
+            for (int sg = ts[e]; sg < ts[e + 1]; sg++) {
+                const T2 r = P[(size_t)sg * 64];
+                vy[q] += r.x;
+                vo[q] += r.y;
             }
-            vy[q] += y;
-            vo[q] += o;
         }
     }
     if (!valid) return;
@@ -351,5 +579,9 @@ __global__ void k_op_tiles(int ncols, int* ntiles, int4* tiles) {
 
 template __global__ void k_op_weights<float>(OpWArgs);
 template __global__ void k_op_weights<double>(OpWArgs);
+template __global__ void k_op_slow<float>(OpWArgs);
+template __global__ void k_op_slow<double>(OpWArgs);
+template __global__ void k_op_segments<float>(OpGArgs);
+template __global__ void k_op_segments<double>(OpGArgs);
 template __global__ void k_op_fold<float>(OpFArgs);
 template __global__ void k_op_fold<double>(OpFArgs);

@@ -156,8 +156,8 @@ class SSPOperator:
     """Isochrone points of every (metallicity, age), their template entries and the template library.
 
     Points of (iz, t) are [age_ptr[iz * nt + t], age_ptr[iz * nt + t + 1]), sorted by initial mass; their mass bins
-    m1 / m2 are the midpoints to the neighbours (the first point of an age takes the IMF's lower bound, flag
-    first = 1; the last bin ends at its own mass). cm: surviving mass, cl: L_bol (L_sun), q: ionizing photons (1/s)
+    m1 / m2 are the midpoints to the neighbours, consecutive bins sharing their edge (the first point of an age takes
+    the IMF's lower bound, flag first = 1, m1 = 0; the last bin ends at its own mass). cm: surviving mass, cl: L_bol (L_sun), q: ionizing photons (1/s)
     per star. Each point has nent entries (template, value): its L_bol shared between the two library templates
     that bracket its Teff in log Teff. The library (ntpl, nspec) holds blackbody spectra per unit L_bol (1/Hz),
     ordered by the first age that uses them, so k_age[t] (templates first used at ages <= t) is increasing.
@@ -237,12 +237,14 @@ def ssp_operator(lam: np.ndarray, log_age: np.ndarray, nz: int, npts: int = 1500
     first = np.zeros(mini.size, np.int32)
     for a in range(per):
         m = mini[a * npts:(a + 1) * npts]
-        lo = np.empty(npts)
+        # bins share their edges exactly, as the reference's operator build (g_m1(k) = g_m2(k - 1),
+        # src/physics/opensps_ssp_operator.f90:414-423); the first bin's lower edge is 0 (the IMF's lower bound)
         hi = np.empty(npts)
-        lo[1:] = m[1:] - 0.5 * (m[1:] - m[:-1])
-        lo[0] = m[0]
         hi[:-1] = m[:-1] + 0.5 * (m[1:] - m[:-1])
         hi[-1] = m[-1]
+        lo = np.empty(npts)
+        lo[1:] = hi[:-1]
+        lo[0] = 0.0
         m1[a * npts:(a + 1) * npts] = lo
         m2[a * npts:(a + 1) * npts] = hi
         first[a * npts] = 1
