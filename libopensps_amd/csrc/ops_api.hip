@@ -360,7 +360,8 @@ int opensps_hip_set_mag_mask(opensps_hip_ctx_t* c, int nbands, const int* mask) 
 static void free_operator(OpsCtx* c) {
     void* ps[] = {c->op.d_age_ptr, c->op.d_zoff, c->op.d_first, c->op.d_tz_ptr, c->op.d_pt, c->op.d_age_mmax,
                   c->op.d_te, c->op.d_te_val64, c->op.d_te_val32, c->op.d_A64, c->op.d_Ahi, c->op.d_Alo,
-                  c->op.d_seg, c->op.d_segoff, c->op.d_tseg, c->op.d_slow_ptr, c->op.d_slow_pt};
+                  c->op.d_seg, c->op.d_segoff, c->op.d_tseg, c->op.d_slow_ptr, c->op.d_slow_pt, c->op.d_wp,
+                  c->op.d_poly};
     for (void* p : ps) free_ptr(p);
     c->op = OpsCtx::OpsOperator();
 }
@@ -375,15 +376,43 @@ static void bin_logs(double m1, double m2, double* llo, double* dl) {
     *dl = (u == 1.0) ? x : log(u) * x / (u - 1.0);
 }
 
-// The bins k_op_slow takes for the context's IMF type: per (metallicity, age) the first bin, the bins across a
-// power-law break and the empty bins (bin_class 0 or m2 <= m1), in point order
-static int build_slow_list(OpsCtx* c) {
-    const int nzt = c->op.nz * c->op.nt, ty = c->imf_type % 10;
+// The tables that depend on the context's IMF type. The bins k_op_slow takes: per (metallicity, age) the first
+// bin, the bins across a power-law break and the empty bins (bin_class 0 or m2 <= m1), in point order. The records
+// of k_op_weights, with each common bin's segment and whether lo^e comes down the chain: the previous common bin
+// of the same piece of OPS_PIECE (imf_bin_weights_tab's pieces) is in the same segment and ends where this one
+// begins, and no non-empty bin k_op_slow takes lies between them (empty bins leave the chain as they are).
+static int build_imf_tables(OpsCtx* c) {
+    const int nzt = c->op.nz * c->op.nt, ty = c->imf_type % 10, np = c->op.npts;
     std::vector<int> ptr(nzt + 1, 0), pts;
+    std::vector<OpsWPoint> wp(np);
     for (int a = 0; a < nzt; a++) {
-        for (int k = c->op.h_age_ptr[a]; k < c->op.h_age_ptr[a + 1]; k++) {
-            const double m1 = c->op.h_m1[k], m2 = c->op.h_m2[k];
-            if (c->op.h_first[k] == 1 || !(m2 > m1) || bin_class(ty, m1, m2, c->mdave) == 0) pts.push_back(k);
+        const int k0 = c->op.h_age_ptr[a], k1 = c->op.h_age_ptr[a + 1];
+        bool chain = false;
+        int pseg = 0;
+        double pm2 = 0.0;
+        for (int k = k0; k < k1; k++) {
+            const double* q = &c->op.h_pt[(size_t)k * OPS_PT_N];
+            const double m1 = q[1], m2 = q[2];
+            OpsWPoint& r = wp[k];
+            r = OpsWPoint();
+            r.mini = q[0];
+            r.llo = q[3];
+            r.dl = q[4];
+            r.cm = q[5];
+            r.cl = q[6];
+            if ((k - k0) % OPS_PIECE == 0) chain = false;
+            const int first = c->op.h_first[k] == 1;
+            const int seg = (first || !(m2 > m1)) ? 0 : bin_class(ty, m1, m2, c->mdave);
+            if (seg == 0) {
+                pts.push_back(k);
+                r.code = 0;
+                if (first || m2 > m1) chain = false;
+                continue;
+            }
+            r.code = seg | ((chain && seg == pseg && m1 == pm2) ? 4 : 0);
+            chain = true;
+            pseg = seg;
+            pm2 = m2;
         }
         ptr[a + 1] = (int)pts.size();
     }
@@ -391,6 +420,13 @@ static int build_slow_list(OpsCtx* c) {
     if (pts.empty()) pts.push_back(0);
     UPLOAD(c, c->op.d_slow_ptr, ptr.data(), ptr.size());
     UPLOAD(c, c->op.d_slow_pt, pts.data(), pts.size());
+    UPLOAD(c, c->op.d_wp, wp.data(), wp.size());
+    double poly[13], f = 1.0;
+    for (int j = 0; j < 13; j++) {
+        f *= (double)(j + 1);
+        poly[j] = 1.0 / f;
+    }
+    UPLOAD(c, c->op.d_poly, poly, 13);
     return 0;
 }
 
@@ -530,9 +566,8 @@ int opensps_hip_set_ssp_ops(opensps_hip_ctx_t* c, const opensps_hip_ssp_ops_t* o
     HIPCHK(c, hipFree(lib));
     c->op.h_age_ptr.assign(o->age_ptr, o->age_ptr + (size_t)nz * nt + 1);
     c->op.h_first.assign(o->first, o->first + np);
-    c->op.h_m1.assign(o->m1, o->m1 + np);
-    c->op.h_m2.assign(o->m2, o->m2 + np);
-    int st = build_slow_list(c);
+    c->op.h_pt.swap(pt);
+    int st = build_imf_tables(c);
     if (st) return st;
     c->op.on = true;
     return 0;
@@ -559,7 +594,7 @@ int opensps_hip_set_imf(opensps_hip_ctx_t* c, int imf_type, int remnants, double
     c->mdave = mdave;
     c->mlim_bh = mlim_bh;
     c->mlim_ns = mlim_ns;
-    if (c->op.on && relist) return build_slow_list(c);
+    if (c->op.on && relist) return build_imf_tables(c);
     return 0;
 }
 
@@ -732,6 +767,8 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         w.impc = (double*)c->opimf.p;
         w.slow_ptr = c->op.d_slow_ptr;
         w.slow_pt = c->op.d_slow_pt;
+        w.wp = c->op.d_wp;
+        w.poly = c->op.d_poly;
         w.spart = (double*)c->opspart.p;
         w.scnt = (int*)(w.spart + 3 * (size_t)npass * maxtiles * c->nt * 64);
         if (a->op_count) HIPCHK(c, hipMemsetAsync(a->op_count, 0, sizeof(int) * 2 * (size_t)n, c->stream));

@@ -18,6 +18,8 @@ struct OpsBuffer {
     size_t bytes = 0;
 };
 
+struct OpsWPoint;   // ops_launch.h
+
 struct OpsCtx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -62,8 +64,10 @@ struct OpsCtx {
         // the bins off the common path for the context's IMF type (first bins, bins across a break, empty bins):
         // per (metallicity, age) CSR of point indices, rebuilt when the IMF type or mdave changes
         std::vector<int> h_age_ptr, h_first;
-        std::vector<double> h_m1, h_m2;
+        std::vector<double> h_pt;                          // the points as uploaded (OPS_PT_N doubles each)
         int *d_slow_ptr = nullptr, *d_slow_pt = nullptr;
+        OpsWPoint* d_wp = nullptr;                         // k_op_weights' records
+        double* d_poly = nullptr;                          // k_op_weights' expm1 coefficients
         int nslow = 0;
     } op;
     int imf_type = 2, remnants = 1;

@@ -72,6 +72,15 @@ struct PackArgs {
 #define OPS_TPL_BLOCK 8    // templates per wavefront of the fold's sums
 #define OPS_SEG 64         // entries per segment of a template's list (one wavefront of the fold each)
 
+// One isochrone point as k_op_weights reads it (64 bytes, one scalar load): its initial mass, ln m1, ln(m2 / m1),
+// surviving mass and L_bol per star, and code = the power-law segment (1..3; 0: a bin k_op_slow takes) | 4 when
+// lo^e comes down the chain from the previous common bin (same piece of 32, same segment, shared edge)
+struct alignas(64) OpsWPoint {
+    double mini, llo, dl, cm, cl;
+    int code, pad0;
+    double pad1, pad2;
+};
+
 struct OpWArgs {           // k_op_weights
     int n, nt, npass, npz, maxtiles, imf_type, remnants;
     double salp_ind, mdave, mlim_bh, mlim_ns;
@@ -84,7 +93,9 @@ struct OpWArgs {           // k_op_weights
     void *u, *wc;          // per (pass, tile): the points' weights (npz x 64), the ages' SFH x blend factors (nt x 64)
     double *smass_p, *slum_p;   // (pass, model, age)
     int* count;                 // (2, n) points with a nonzero weight per pass, or NULL
-    double* impc;               // per model OPS_IMFC_N: 1 / total mass integral, the segments' e, c, c / e
+    const OpsWPoint* wp;        // per point: k_op_weights' record for the context's IMF type
+    const double* poly;         // 1 / (j + 1)!, j = 0 .. 12
+    double* impc;               // per model OPS_IMFC_N: 1 / total mass integral, the segments' e, c, c / total
     const int *slow_ptr, *slow_pt;   // per (metallicity, age) the bins that leave the common path
     double* spart;              // (pass, tile, age): their sums and the remnants, 3 x 64 per wavefront
     int* scnt;                  // (pass, tile, age): their nonzero weights, 64 per wavefront
@@ -107,7 +118,7 @@ __host__ __device__ inline int bin_class(int t, double lo, double hi, double mda
 #define OPS_IMFC_INV_TOTAL 0
 #define OPS_IMFC_E 1
 #define OPS_IMFC_C 4
-#define OPS_IMFC_CE 7
+#define OPS_IMFC_CT 7
 #define OPS_IMFC_N 10
 
 struct OpGArgs {           // k_op_segments: one segment's entries per wavefront

@@ -217,7 +217,7 @@ __device__ __forceinline__ ImfP model_imf(const OpWArgs& a, int m) {
 
 // ------------------------------------------------------------------------------------------------
 // k_op_imf: one thread per model: the total mass integral over [lower, upper] (the weights' normalisation) and
-// each power-law segment's constants e = 1 - alpha, c, c / e (bin_segment), formed once per model.
+// each power-law segment's constants e = 1 - alpha, c and c / total (bin_segment), formed once per model.
 // ------------------------------------------------------------------------------------------------
 __global__ void k_op_imf(OpWArgs a) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -239,7 +239,7 @@ __global__ void k_op_imf(OpWArgs a) {
         }
         o[OPS_IMFC_E + seg - 1] = s.e;
         o[OPS_IMFC_C + seg - 1] = s.c;
-        o[OPS_IMFC_CE + seg - 1] = s.ce;
+        o[OPS_IMFC_CT + seg - 1] = s.c * o[OPS_IMFC_INV_TOTAL];
     }
 }
 
@@ -289,123 +289,89 @@ __global__ __launch_bounds__(256) void k_op_slow(OpWArgs a) {
 }
 
 #pragma clang fp contract(on)
+// expm1(y) / y = sum_j y^j / (j + 1)!, j = 0 .. 12, by Horner with fused multiply-adds: expm1_series's
+// (opensps_imf.f90:641-667) polynomial, the same terms, in 12 FMAs; |y| < 1/4 as there. The coefficients
+// 1 / (j + 1)! come from a device table (scalar registers, the addends of the FMAs).
+__device__ __forceinline__ double d_expm1_over_y(double y, const double* pc) {
+    double s = pc[12];
+#pragma unroll
+    for (int j = 11; j >= 0; j--) s = fma(s, y, pc[j]);
+    return s;
+}
+
+typedef __attribute__((address_space(4))) const double* OpsDoubleC;
+typedef __attribute__((address_space(4))) const OpsWPoint* OpsWPointC;
+
 // ------------------------------------------------------------------------------------------------
 // k_op_weights: grid (ceil(nt / 4), max tiles, passes), 256 threads = 4 ages. Pass 0 takes the operator of
 // metallicity zlo with weight 1 - dz, pass 1 that of zlo + 1 with weight dz (one pass with one metallicity).
-// The common bins of an age: not its first, inside one power-law segment. Their segment is then the same for
-// every lane (the bins are the points', the breaks the IMF type's), so the wavefront stays together; per lane
-// only the limits, a segment change and a broken chain of shared edges differ, each taken by the lanes that
-// need it (imf_bin_weights_tab's chain: hi^e = lo^e (1 + expm1(e dl)), exp only where the chain breaks; expm1 by
-// the series with fused multiply-adds). The bins k_op_slow takes reset the chain as the reference's do.
+// The common bins of an age (not its first, inside one power-law segment, not empty): a point's record is the
+// same for every lane (the lanes are models), so the records come through scalar loads and the segment and the
+// chain of shared edges are uniform, as the host coded them for the context's IMF type (OpsWPoint::code). Every
+// lane runs the chain hi^e = lo^e (1 + expm1(e dl)) over every common point (exp only where the host broke the
+// chain: a new piece of 32 as imf_bin_weights_tab's, a new segment, a bin k_op_slow took, or edges not shared),
+// and the limits select the weights. Per bin: w = c lo^e dl (expm1(e dl) / (e dl)) / total, which is c dl / total
+// for e = 0 (imf_bin_weights_tab :537-551 with :502-525's constants, c and 1 / total from k_op_imf).
 // ------------------------------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int t = blockIdx.x * 4 + wv, tile = blockIdx.y, pass = blockIdx.z;
     if (t >= a.nt || tile >= *a.ntiles) return;
     const int4 td = a.tiles[tile];   // {metallicity pair (zlo - 1), first column, columns}
-    const int iz = a.npass == 1 ? 0 : td.x + pass;
+    const int iz = a.npass == 1 ? 0 : __builtin_amdgcn_readfirstlane(td.x) + pass;
     const bool valid = lane < td.z;
     const int m = a.order[td.y + (valid ? lane : 0)];
-    const int n = a.n, nt = a.nt, ty = a.imf_type % 10;
+    const int n = a.n, nt = a.nt;
     const double lower = a.prm[(size_t)OPS_P_IMF_LOWER * n + m], upper = a.prm[(size_t)OPS_P_IMF_UPPER * n + m];
     const double* ic = a.impc + (size_t)m * OPS_IMFC_N;
-    const double inv_total = ic[OPS_IMFC_INV_TOTAL];
-    double se[3], sc[3], sce[3];
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        se[q] = ic[OPS_IMFC_E + q];
-        sc[q] = ic[OPS_IMFC_C + q];
-        sce[q] = ic[OPS_IMFC_CE + q];
-    }
+    const double e1 = ic[OPS_IMFC_E], e2 = ic[OPS_IMFC_E + 1], e3 = ic[OPS_IMFC_E + 2];
+    const double c1 = ic[OPS_IMFC_CT], c2 = ic[OPS_IMFC_CT + 1], c3 = ic[OPS_IMFC_CT + 2];
     const double dz = a.hdr[(size_t)m * OPS_HDR_N + OPS_HDR_DZ];
     const double cz = a.npass == 1 ? 1.0 : (pass == 0 ? 1.0 - dz : dz);
-    const int k0 = a.age_ptr[iz * nt + t], k1 = a.age_ptr[iz * nt + t + 1], zo = a.zoff[iz];
+    const int k0 = __builtin_amdgcn_readfirstlane(a.age_ptr[iz * nt + t]);
+    const int k1 = __builtin_amdgcn_readfirstlane(a.age_ptr[iz * nt + t + 1]);
+    const int zo = __builtin_amdgcn_readfirstlane(a.zoff[iz]);
     T* u = (T*)a.u + ((size_t)pass * a.maxtiles + tile) * a.npz * 64 + lane;
-    double sm = 0.0, sl = 0.0, sq = 0.0;
+    const OpsWPointC wp = (OpsWPointC)a.wp;
+    const OpsDoubleC pcs = (OpsDoubleC)a.poly;
+    double pc[13];
+#pragma unroll
+    for (int j = 0; j < 13; j++) pc[j] = pcs[j];
+    double sm = 0.0, sl = 0.0, phi = 1.0;
     int cnt = 0;
-    // The pieces' point data pass through this wavefront's two LDS buffers: the lanes load the next piece
-    // (contiguous, 4 doubles and one flag a lane) while the current one is computed from the other buffer
-    __shared__ double spt[4][2][OPS_PIECE * OPS_PT_N];
-    __shared__ int sfirst[4][2][OPS_PIECE];
-    double r[4];
-    int rf = 0;
-    auto fetch = [&](int q0) {
-        const int nv = min(k1 - q0, OPS_PIECE) * OPS_PT_N;
-#pragma unroll
-        for (int i = 0; i < 4; i++) r[i] = lane + 64 * i < nv ? a.pt[(size_t)q0 * OPS_PT_N + lane + 64 * i] : 0.0;
-        rf = lane < min(k1 - q0, OPS_PIECE) ? a.first[q0 + lane] : 0;
-    };
-    auto stash = [&](int b) {
-#pragma unroll
-        for (int i = 0; i < 4; i++) spt[wv][b][lane + 64 * i] = r[i];
-        if (lane < OPS_PIECE) sfirst[wv][b][lane] = rf;
-    };
-    if (k0 < k1) {
-        fetch(k0);
-        stash(0);
-    }
-    WAVE_SYNC();
-    int b = 0;
-    for (int q0 = k0; q0 < k1; q0 += OPS_PIECE, b ^= 1) {
-        const int q1 = min(k1, q0 + OPS_PIECE);
-        if (q1 < k1) fetch(q1);
-        int cur = 0;                   // the chain restarts with each piece (bin_reset)
-        double prev_hi = -1.0, prev_phi = 0.0;
-        double pm = 0.0, pl = 0.0, pq = 0.0;
-        const double* pb = spt[wv][b];
-        for (int k = q0; k < q1; k++) {
-            const double* pt = pb + (k - q0) * OPS_PT_N;
-            const double mini = pt[0], m1 = pt[1], m2 = pt[2];
-            const int seg = (sfirst[wv][b][k - q0] == 1 || !(m2 > m1)) ? 0 : bin_class(ty, m1, m2, a.mdave);
-            if (seg == 0) {
-                // k_op_slow's bin: the chain restarts after it (an empty bin changes nothing)
-                if (m2 > m1 || sfirst[wv][b][k - q0] == 1) {
-                    prev_hi = -1.0;
-                    cur = -1;
-                }
-                continue;
-            }
-            const bool inside = !(mini < lower || mini > upper);
-            if (inside && seg != cur) {
-                cur = seg;
-                prev_hi = -1.0;
-            }
-            const double e = seg == 1 ? se[0] : (seg == 2 ? se[1] : se[2]);
-            const double c = seg == 1 ? sc[0] : (seg == 2 ? sc[1] : sc[2]);
-            const double ce = seg == 1 ? sce[0] : (seg == 2 ? sce[1] : sce[2]);
-            const double dl = pt[4];
-            double val = c * dl;       // e == 0: c ln(hi / lo)
-            if (e != 0.0) {
-                const bool hit = m1 == prev_hi;
-                double plo = prev_phi;
-                if (__any(inside && !hit)) plo = hit ? plo : exp(e * pt[3]);
-                const double y = e * dl;
-                double em1 = d_expm1_series(y);
-                if (__any(inside && !(fabs(y) < 0.25))) em1 = fabs(y) < 0.25 ? em1 : exp(y) - 1.0;
-                val = ce * plo * em1;
-                if (inside) {
-                    prev_hi = m2;
-                    prev_phi = plo + plo * em1;
-                }
-            }
-            const double w = inside ? val * inv_total : 0.0;
+    int k = k0;
+    while (k < k1) {   // runs of one power-law segment
+        const int seg = wp[k].code & 3;
+        if (seg == 0) {   // k_op_slow's bin
+            k++;
+            continue;
+        }
+        const double e = seg == 1 ? e1 : (seg == 2 ? e2 : e3);
+        const double c = seg == 1 ? c1 : (seg == 2 ? c2 : c3);
+        for (; k < k1; k++) {
+            const int code = wp[k].code;
+            if ((code & 3) == 0) continue;
+            if ((code & 3) != seg) break;
+            const double dl = wp[k].dl;
+            const double plo = (code & 4) ? phi : exp(e * wp[k].llo);
+            const double y = e * dl;
+            double s = d_expm1_over_y(y, pc);
+            if (!(fabs(y) < 0.25)) s = (exp(y) - 1.0) / y;
+            phi = fma(plo, y * s, plo);
+            const double mini = wp[k].mini;
+            const double w = (mini < lower || mini > upper) ? 0.0 : c * dl * plo * s;
             u[(size_t)(k - zo) * 64] = (T)w;
             cnt += w > 0.0 ? 1 : 0;
-            pm = pm + pt[5] * w;
-            pl = pl + pt[6] * w;
-            pq = pq + pt[7] * w;
+            sm = fma(wp[k].cm, w, sm);
+            sl = fma(wp[k].cl, w, sl);
         }
-        sm = sm + pm; sl = sl + pl; sq = sq + pq;
-        if (q1 < k1) stash(b ^ 1);
-        WAVE_SYNC();
     }
     // the slow bins' and the remnants' sums (k_op_slow)
     const size_t ra = ((size_t)pass * a.maxtiles + tile) * nt + t;
     const double* sp = a.spart + ra * 3 * 64 + lane;
     sm = sm + sp[0];
     sl = sl + sp[64];
-    sq = sq + sp[128];
     cnt += a.scnt[ra * 64 + lane];
     if (valid) {
         const size_t r = ((size_t)pass * n + m) * nt + t;
@@ -416,7 +382,6 @@ __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
     const double ws = a.wsfh[(size_t)m * nt + t];
     T* wc = (T*)a.wc + (ra * 64) + lane;
     *wc = (T)((ws > OPS_SAFE_FLOOR ? ws : 0.0) * cz);
-    (void)sq;   // the ionizing photons serve nebular emission, which this build does not add
 }
 #pragma clang fp contract(off)
 
