@@ -140,6 +140,57 @@ def parametric_params(n: int, seed: int = 5, zred=(0.0, 4.0), sigma=(50.0, 300.0
     return p
 
 
+def lognormal_weights_z(n: int, log_age: np.ndarray, logz: np.ndarray, seed: int = 17, sigma_dex: float = 1.0,
+                        zwidth: float = 0.3) -> np.ndarray:
+    """(n, nz, nt) weights over ages and metallicities of the adjoint workload: per model a log-normal SFH over the
+    ages (sigma = 1 dex in age, peak log-uniform over the grid) and, per age, an enrichment history over the
+    metallicities (a Gaussian in log Z of width zwidth dex, centred on a metallicity that rises from the model's
+    initial value at the oldest age to its final value at the youngest), scaled to 1 solar mass formed."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    la = np.asarray(log_age, dtype=np.float64)
+    lz = np.asarray(logz, dtype=np.float64)
+    peak = rng.uniform(la[0], la[-1], n)
+    w_t = np.exp(-0.5 * ((la[None, :] - peak[:, None]) / sigma_dex) ** 2)
+    w_t /= w_t.sum(axis=1, keepdims=True)
+    z0 = rng.uniform(lz[0], lz[0] + 0.5 * (lz[-1] - lz[0]), n)
+    z1 = rng.uniform(lz[0] + 0.5 * (lz[-1] - lz[0]), lz[-1], n)
+    frac = (la[-1] - la) / (la[-1] - la[0])                        # 1 at the youngest age, 0 at the oldest
+    zc = z0[:, None] + (z1 - z0)[:, None] * frac[None, :]           # (n, nt)
+    e = np.exp(-0.5 * ((lz[None, :, None] - zc[:, None, :]) / zwidth) ** 2)   # (n, nz, nt)
+    e /= e.sum(axis=1, keepdims=True)
+    return e * w_t[:, None, :]
+
+
+def adjoint_params(n: int, seed: int = 19, birth_cloud: bool = False) -> np.ndarray:
+    """(NPAR, n) physical parameters of the adjoint workload, uniform in the parametric workload's ranges: dust2
+    U(0, 2), dust_index U(-1.2, -0.4), redshift U(0, 4), sigma_smooth U(50, 300) km/s; with birth_cloud also
+    dust1 U(0, 1), dust1_index U(-1.5, -0.5), dust3 U(0, 0.5), frac_obrun U(0, 0.5) and frac_nodust U(0, 0.5)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = default_params(n)
+    p[SLOT["dust2"]] = rng.uniform(0.0, 2.0, n)
+    p[SLOT["dust_index"]] = rng.uniform(-1.2, -0.4, n)
+    p[SLOT["zred"]] = rng.uniform(0.0, 4.0, n)
+    p[SLOT["sigma_smooth"]] = rng.uniform(50.0, 300.0, n)
+    if birth_cloud:
+        p[SLOT["dust1"]] = rng.uniform(0.0, 1.0, n)
+        p[SLOT["dust1_index"]] = rng.uniform(-1.5, -0.5, n)
+        p[SLOT["dust3"]] = rng.uniform(0.0, 0.5, n)
+        p[SLOT["frac_obrun"]] = rng.uniform(0.0, 0.5, n)
+        p[SLOT["frac_nodust"]] = rng.uniform(0.0, 0.5, n)
+    return p
+
+
+def chi2_cotangent(mags: np.ndarray, seed: int = 23, frac_err: float = 0.05):
+    """The adjoint workload's loss: chi^2 of the magnitudes against synthetic photometry (mags perturbed by 5 %
+    Gaussian flux errors drawn from a held-out seed). Returns (loss, d loss / d mags); NaN bands are left out."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sig = 2.5 / np.log(10.0) * frac_err
+    obs = mags + sig * rng.standard_normal(mags.shape) + 0.1
+    ok = np.isfinite(mags)
+    r = np.where(ok, (mags - np.where(ok, obs, 0.0)) / sig, 0.0)
+    return float(np.sum(r * r)), np.where(ok, 2.0 * r / sig, 0.0)
+
+
 def varied_names() -> list:
     """The parameters the parametric workload varies per model (the names[] of its batch call)."""
     return ["sfh", "tage", "tau", "logzsol", "dust2", "dust_index", "zred", "sigma_smooth"]

@@ -846,6 +846,9 @@ typedef struct {
     int imf_type, remnants;   /* imf_type, add_stellar_remnants */
     double salp_ind, mdave, mlim_bh, mlim_ns;
     int *op_count;            /* 2 x n or NULL: points with a nonzero IMF weight per metallicity pass */
+    /* weights over ages and metallicities (ssp_weights_z: nt x nz x n) or NULL; they replace the SFH and the
+     * metallicity blend (age_sums_w / model_sums_zw, opensps_csp_batch.f90:2086-2127, 2269-2324) */
+    const double *ssp_weights_z;
 } orc_ctx_t;
 
 /* Per-model birth-cloud age index: batch_model_indices (opensps_csp_batch.f90:1541-1576) */
@@ -941,10 +944,36 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
             if (blend) model_zlo(c, p[OPS_P_LOGZSOL], &zlo, &dz);
             if (idx) { idx[3 * m] = it; idx[3 * m + 1] = zlo; idx[3 * m + 2] = 0; }
             /* stage 0 checks (batch_check_models :1312-1320; plan_smoothing host modes are not built) */
-            if ((!ssp_weights && p[OPS_P_TAGE] <= 0.0) || (!c->smooth_fast && p[OPS_P_SIGMA_SMOOTH] > SAFE_FLOOR)) {
+            if ((!ssp_weights && !c->ssp_weights_z && p[OPS_P_TAGE] <= 0.0) || (!c->smooth_fast && p[OPS_P_SIGMA_SMOOTH] > SAFE_FLOOR)) {
                 #pragma omp atomic write
                 status = 2;
                 continue;
+            }
+            /* weights over ages and metallicities: model_sums_zw and age_sums_w (metallicities outer; young
+             * ages k <= it_tesc) */
+            if (c->ssp_weights_z) {
+                const double *wz = c->ssp_weights_z + (size_t)m * nt * nz;
+                double ms = 0.0, ls = 0.0;
+                for (int i = 0; i < nspec; i++) { y[i] = 0.0; o[i] = 0.0; }
+                for (int zz = 0; zz < nz; zz++)
+                    for (int k = 1; k <= nt; k++) {
+                        double wk = wz[(size_t)zz * nt + k - 1];
+                        if (wk > SAFE_FLOOR) {
+                            ms = ms + wk * c->bmass[(size_t)zz * nt + k - 1];
+                            ls = ls + wk * pow(10.0, c->blbol[(size_t)zz * nt + k - 1]);
+                        } else {
+                            wk = 0.0;
+                        }
+                        double *acc = k <= it ? y : o;
+                        const double *bs = c->basis + ((size_t)zz * nt + k - 1) * nspec;
+                        for (int i = 0; i < nspec; i++) acc[i] = acc[i] + wk * bs[i];
+                    }
+                if (w_out) for (int k = 0; k < nt; k++) w_out[(size_t)m * nt + k] = 0.0;
+                if (mass) mass[m] = ms;
+                if (lbol) lbol[m] = ls > SAFE_FLOOR ? log10(ls) : 0.0;
+                if (young) memcpy(young + (size_t)m * nspec, y, sizeof(double) * nspec);
+                if (old) memcpy(old + (size_t)m * nspec, o, sizeof(double) * nspec);
+                goto dust;
             }
             /* stage 2: SFH weights (batch_sfh_weights, :1865-1917) */
             if (ssp_weights) {
@@ -1015,6 +1044,7 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
             if (lbol) lbol[m] = ls > SAFE_FLOOR ? log10(ls) : 0.0;
             if (young) memcpy(young + (size_t)m * nspec, y, sizeof(double) * nspec);
             if (old) memcpy(old + (size_t)m * nspec, o, sizeof(double) * nspec);
+        dust:;
             /* stage 5: attenuate_batch + pixel_transmissions, dust_type 0 (:3576-3818) */
             double fob = p[OPS_P_FRAC_OBRUN], fnd = p[OPS_P_FRAC_NODUST];
             double ed = p[OPS_P_DUST_INDEX];
@@ -1078,6 +1108,376 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
         free(smass); free(slum); free(vy); free(vo); free(wpt); free(llo); free(dl);
     }
     free(jlo); free(jhi); free(do_band); free(bt); free(wtl); free(ltab);
+    return status;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * The reverse pass: batch_backward (opensps_csp_batch.f90:4713-5136) for the stages above, in FP64, as
+ * the transposes of the forward maps with the forward's transmissions recomputed:
+ *   magnitudes -> band fluxes (mags = -2.5 log10 F + const + distance term; distance_term_dz,
+ *     src/spectra/opensps_batch_photometry.f90:798-813)
+ *   band fluxes -> observed frame -> the smoothed rest-frame spectrum (backward_observed_frame :6958-7136:
+ *     the clamp max(v, 0) passes no gradient; dz through the interpolation fraction)
+ *   velocity smoothing: d sigma (smooth_sigma_adjoint, opensps_batch_photometry.f90:627-700:
+ *     dC/dsigma = C (x^2 - <x^2>) / (sigma_beta^3 c 1e-13)) and the transpose (smooth_columns_adjoint :604-625)
+ *   IGM and attenuation (atten_adjoint_chunk :6476-6665: the dust parameters, frac_obrun, frac_nodust, the
+ *     redshift through the IGM optical depth, capped pixels taking the cap pixel's d tau / dz, igm_factor)
+ *   the age sums (backward_age_sums_w :5288-5330, backward_age_sums + age_sums_adj_slabs.inc, and the mass
+ *     cotangent of batch_mass_cotangent :5142): w_bar = basis^T y_bar + L_bar L_age + M_bar M_age, the
+ *     metallicity weight's cotangent into logzsol.
+ * params_bar rows (0-based, the reference's BGRAD_* minus 1, :166-213): 1 dust1, 2 dust2, 3 dust3,
+ * 4 dust_index, 5 dust1_index, 6 frac_obrun, 7 frac_nodust, 15 logzsol, 22 zred, 23 sigma_smooth,
+ * 32 igm_factor; the others (the SFH parameters among them) are zero. The IGM optical depth is the
+ * six-term form (igm_tau_pixel_terms x igm_tau_z_terms), which the reference uses whenever the
+ * redshifts differ across the batch.
+ * ------------------------------------------------------------------------------------------- */
+#define ORC_NBGRAD 44
+
+static const double IGM_EXP[6] = {3.46, 1.68, 3.0 + MP1, 1.5 + MP2, 3.0 - MP3, MP4};
+
+int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const double *ssp_weights,
+                       const double *mags_bar, const double *spec_bar, const double *mass_bar,
+                       const double *lbol_bar, double *mags_out, double *params_bar, double *weights_bar,
+                       double *weights_z_bar) {
+    const int nspec = c->nspec, nt = c->nt, nz = c->nz, nb = c->nbands;
+    if (n < 0 || nspec < 2 || nt < 2 || nz < 1 || c->ops) return 3;
+    if (n == 0) return 0;
+    const double amzl = abs_mag_zeropoint_log(), ln10 = log(10.0);
+    int status = 0;
+    int *jlo = (int *)malloc(sizeof(int) * (nb > 0 ? nb : 1)), *jhi = (int *)malloc(sizeof(int) * (nb > 0 ? nb : 1));
+    int *do_band = (int *)malloc(sizeof(int) * (nb > 0 ? nb : 1));
+    double *bt = (double *)malloc(sizeof(double) * (size_t)(nb > 0 ? nb : 1) * nspec);
+    double *wtl = (double *)malloc(sizeof(double) * nspec), *ltab = (double *)malloc(sizeof(double) * nspec);
+    orc_band_supports(c->bands, nspec, nb, jlo, jhi);
+    for (int b = 0; b < nb; b++) {
+        do_band[b] = c->mag_mask ? c->mag_mask[b] : 1;
+        for (int j = 0; j < nspec; j++) bt[(size_t)j * nb + b] = c->bands[(size_t)b * nspec + j];
+    }
+    for (int j = 0; j < nspec - 1; j++) wtl[j] = 0.5 * fabs(c->lambda[j + 1] - c->lambda[j]);
+    wtl[nspec - 1] = 0.0;
+    for (int j = 0; j < nspec; j++) ltab[j] = log(c->lambda[j] / 5500.0);
+    int jtop = 0;
+    for (int j = 1; j <= nspec; j++) if (c->lambda[j - 1] < 1215.67) jtop = j;
+    double *pa = (double *)malloc(sizeof(double) * 6 * (jtop > 0 ? jtop : 1));
+    for (int j = 0; j < jtop; j++) orc_igm_pixel_terms(c->lambda[j], pa + 6 * j);
+    const int cap = nspec * 64;
+    const int wz_on = c->ssp_weights_z != NULL;
+
+    #pragma omp parallel
+    {
+        double *p = (double *)malloc(sizeof(double) * OPS_NPAR);
+        double *w = (double *)malloc(sizeof(double) * nt), *t1 = (double *)malloc(sizeof(double) * nt);
+        double *t2 = (double *)malloc(sizeof(double) * nt);
+        double *y = (double *)malloc(sizeof(double) * nspec), *o = (double *)malloc(sizeof(double) * nspec);
+        double *A = (double *)malloc(sizeof(double) * nspec), *S0 = (double *)malloc(sizeof(double) * nspec);
+        double *S1 = (double *)malloc(sizeof(double) * nspec), *tig = (double *)malloc(sizeof(double) * nspec);
+        double *tauv = (double *)malloc(sizeof(double) * nspec), *work = (double *)malloc(sizeof(double) * nspec);
+        double *fobs = (double *)malloc(sizeof(double) * nspec), *fb = (double *)malloc(sizeof(double) * (nb + 1));
+        double *fbb = (double *)malloc(sizeof(double) * (nb + 1)), *vb = (double *)malloc(sizeof(double) * nspec);
+        double *g1 = (double *)malloc(sizeof(double) * nspec), *g0 = (double *)malloc(sizeof(double) * nspec);
+        double *yb = (double *)malloc(sizeof(double) * nspec), *ob = (double *)malloc(sizeof(double) * nspec);
+        int *slo = (int *)malloc(sizeof(int) * nspec), *shi = (int *)malloc(sizeof(int) * nspec);
+        int *soff = (int *)malloc(sizeof(int) * nspec);
+        double *scoef = (double *)malloc(sizeof(double) * cap);
+
+        #pragma omp for schedule(dynamic, 4)
+        for (int m = 0; m < n; m++) {
+            for (int q = 0; q < OPS_NPAR; q++) p[q] = prm[(size_t)q * n + m];
+            const int it = model_it_tesc(c, p[OPS_P_DUST_TESC]);
+            int zlo = 1;
+            double dz = 0.0;
+            const int blend = nz > 1 && !wz_on;
+            if (blend) model_zlo(c, p[OPS_P_LOGZSOL], &zlo, &dz);
+            if ((!ssp_weights && !wz_on && p[OPS_P_TAGE] <= 0.0) || (!c->smooth_fast && p[OPS_P_SIGMA_SMOOTH] > SAFE_FLOOR)) {
+                #pragma omp atomic write
+                status = 2;
+                continue;
+            }
+            /* ---- forward ---- */
+            double ms = 0.0, ls = 0.0;
+            for (int i = 0; i < nspec; i++) { y[i] = 0.0; o[i] = 0.0; }
+            const double *wz = wz_on ? c->ssp_weights_z + (size_t)m * nt * nz : NULL;
+            if (wz_on) {
+                for (int zz = 0; zz < nz; zz++)
+                    for (int k = 1; k <= nt; k++) {
+                        double wk = wz[(size_t)zz * nt + k - 1];
+                        if (wk > SAFE_FLOOR) {
+                            ms = ms + wk * c->bmass[(size_t)zz * nt + k - 1];
+                            ls = ls + wk * pow(10.0, c->blbol[(size_t)zz * nt + k - 1]);
+                        } else {
+                            wk = 0.0;
+                        }
+                        double *acc = k <= it ? y : o;
+                        const double *bs = c->basis + ((size_t)zz * nt + k - 1) * nspec;
+                        for (int i = 0; i < nspec; i++) acc[i] = acc[i] + wk * bs[i];
+                    }
+            } else {
+                if (ssp_weights) {
+                    for (int k = 0; k < nt; k++) w[k] = ssp_weights[(size_t)m * nt + k];
+                } else {
+                    int st = orc_sfh_weights(c->time_full, nt, c->tiny_logt, p, w, t1, t2);
+                    if (st != 0) {
+                        #pragma omp atomic write
+                        status = st;
+                        continue;
+                    }
+                }
+                const double *blo = c->basis + (size_t)(zlo - 1) * nspec * nt;
+                const double *bhi = blend ? c->basis + (size_t)zlo * nspec * nt : blo;
+                for (int k = 1; k <= nt; k++) {
+                    double wk = w[k - 1];
+                    if (wk > SAFE_FLOOR) {
+                        double mk, lk;
+                        if (blend) {
+                            mk = (1.0 - dz) * c->bmass[(size_t)(zlo - 1) * nt + k - 1] + dz * c->bmass[(size_t)zlo * nt + k - 1];
+                            lk = (1.0 - dz) * c->blbol[(size_t)(zlo - 1) * nt + k - 1] + dz * c->blbol[(size_t)zlo * nt + k - 1];
+                            ms = ms + wk * mk;
+                            ls = ls + wk * pow(10.0, lk);
+                        } else {
+                            ms = ms + wk * c->bmass[k - 1];
+                            ls = ls + wk * pow(10.0, c->blbol[k - 1]);
+                        }
+                    } else {
+                        wk = 0.0;
+                    }
+                    double *acc = k <= it ? y : o;
+                    const double *bl = blo + (size_t)(k - 1) * nspec, *bh = bhi + (size_t)(k - 1) * nspec;
+                    if (blend)
+                        for (int i = 0; i < nspec; i++) acc[i] = acc[i] + wk * ((1.0 - dz) * bl[i] + dz * bh[i]);
+                    else
+                        for (int i = 0; i < nspec; i++) acc[i] = acc[i] + wk * ((1.0 - dz) * bl[i]);
+                }
+            }
+            const double fob = p[OPS_P_FRAC_OBRUN], fnd = p[OPS_P_FRAC_NODUST], ed = p[OPS_P_DUST_INDEX];
+            const double dust1 = p[OPS_P_DUST1], dust2 = p[OPS_P_DUST2], dust3 = p[OPS_P_DUST3];
+            const double ed1 = p[OPS_P_DUST1_INDEX];
+            const int no_birth = dust1 <= SAFE_FLOOR || 1.0 - fob <= SAFE_FLOOR;
+            const int no_old = fabs(dust3) <= SAFE_FLOOR;
+            const int nodust = 1.0 - fnd > SAFE_FLOOR;
+            for (int i = 0; i < nspec; i++) {
+                double curve = ed != 0.0 ? exp(ed * ltab[i]) : 1.0;
+                double tr_d = exp(-(dust2 * curve)), tr_b = 1.0, tr_o = 1.0;
+                if (!no_birth) tr_b = exp(-(dust1 * exp(ed1 * ltab[i])));
+                if (!no_old) tr_o = exp(-(dust3 * curve));
+                double ssum = (y[i] * tr_b * (1.0 - fob) + y[i] * fob) + o[i] * tr_o;
+                A[i] = nodust ? ssum * (tr_d * (1.0 - fnd) + fnd) : ssum;
+                S0[i] = A[i];
+                tig[i] = 1.0;
+                tauv[i] = 0.0;
+            }
+            const double z = p[OPS_P_ZRED], opz = 1.0 + z, igmf = p[OPS_P_IGM_FACTOR];
+            const int igm_on = c->add_igm && z > SAFE_FLOOR && jtop > 0;
+            double b6[6], bd6[6];
+            int jmax = 1;
+            if (igm_on) {
+                orc_igm_z_terms(opz, b6);
+                for (int k = 0; k < 6; k++) bd6[k] = IGM_EXP[k] * b6[k] / opz;
+                double tmax = 0.0;
+                for (int j = 0; j < jtop; j++) {
+                    const double *a6 = pa + 6 * j;
+                    tauv[j] = a6[0] * b6[0] + a6[1] * b6[1] + a6[2] * b6[2] + a6[3] * b6[3] + a6[4] * b6[4] + a6[5] * b6[5];
+                }
+                for (int j = 1; j <= jtop; j++) if (tauv[j - 1] > tmax) { tmax = tauv[j - 1]; jmax = j; }
+                if (jmax > 1) for (int j = 1; j <= jmax; j++) tauv[j - 1] = tmax;
+                for (int j = 0; j < jtop; j++) {
+                    tig[j] = exp(-tauv[j] * igmf);
+                    S0[j] = A[j] * tig[j];
+                }
+            }
+            const double sig = p[OPS_P_SIGMA_SMOOTH];
+            int sm_n = 0;
+            memcpy(S1, S0, sizeof(double) * nspec);
+            if (c->smooth_fast && sig > SAFE_FLOOR) {
+                sm_n = orc_velocity_kernel(c->lambda, nspec, sig, p[OPS_P_MIN_WAVE_SMOOTH], p[OPS_P_MAX_WAVE_SMOOTH], slo, shi,
+                                           soff, scoef, cap);
+                if (sm_n > 0) apply_velocity_kernel(nspec, slo, shi, soff, scoef, S1, work);
+            }
+            orc_observed_frame(c->lambda, nspec, z, S1, fobs);
+            orc_band_fluxes_col(nspec, nb, bt, jlo, jhi, do_band, wtl, fobs, fb);
+            const double dist = orc_distance_term(c->cz, c->cdl, c->ncosmo, z);
+            double *mg = mags_out ? mags_out + (size_t)m * nb : NULL;
+            for (int b = 0; b < nb; b++) {
+                double v = NAN;
+                if (do_band[b] != 0) v = mag_from_flux(fb[b]) - AB_ZEROPOINT - 2.5 * amzl + dist;
+                if (mg) mg[b] = v;
+                fbb[b] = 0.0;
+                if (mags_bar && v == v && fb[b] > 0.0) fbb[b] = mags_bar[(size_t)m * nb + b] * (-2.5 / ln10) / fb[b];
+            }
+            /* ---- reverse ---- */
+            /* the distance term (distance_term_dz) */
+            double dzd = 0.0;
+            if (mags_bar && fabs(z) > SAFE_FLOOR) {
+                double msum = 0.0;
+                for (int b = 0; b < nb; b++) if (fbb[b] != 0.0) msum = msum + mags_bar[(size_t)m * nb + b];
+                const double dm = interp_linear(c->cz, c->cdl, c->ncosmo, z);
+                if (msum != 0.0 && dm == dm && dm > SAFE_FLOOR) {
+                    int i = orc_find_interval(c->cz, c->ncosmo, z);
+                    if (i > c->ncosmo - 1) i = c->ncosmo - 1;
+                    if (i < 1) i = 1;
+                    const double slope = (c->cdl[i] - c->cdl[i - 1]) / (c->cz[i] - c->cz[i - 1]);
+                    dzd = msum * (5.0 / ln10 * slope / dm - 2.5 / (ln10 * (1.0 + z)));
+                }
+            }
+            /* band fluxes -> fobs -> v (the clamp passes nothing) */
+            for (int j = 0; j < nspec; j++) vb[j] = 0.0;
+            for (int b = 0; b < nb; b++) {
+                if (fbb[b] == 0.0) continue;
+                for (int j = jlo[b]; j <= jhi[b]; j++) {
+                    vb[j - 1] += fbb[b] * wtl[j - 1] * bt[(size_t)(j - 1) * nb + b];
+                    vb[j] += fbb[b] * wtl[j - 1] * bt[(size_t)j * nb + b];
+                }
+            }
+            double dzo = 0.0;
+            for (int j = 0; j < nspec; j++) g1[j] = spec_bar ? spec_bar[(size_t)m * nspec + j] : 0.0;
+            for (int j = 0; j < nspec; j++) {
+                double v, t = 0.0;
+                int il = j + 1;
+                const int moved = fabs(z) > SAFE_FLOOR;
+                if (moved) {
+                    double tg = c->lambda[j] / (1.0 + z);
+                    il = orc_find_interval(c->lambda, nspec, tg);
+                    if (il < 1) il = 1;
+                    if (il > nspec - 1) il = nspec - 1;
+                    t = (tg - c->lambda[il - 1]) / (c->lambda[il] - c->lambda[il - 1]);
+                    v = S1[il - 1] + t * (S1[il] - S1[il - 1]);
+                } else {
+                    v = S1[j];
+                }
+                const double wl = c->lambda[j] > DBL_MIN ? 1.0 / c->lambda[j] : 0.0;
+                double gv = (v == v && v > 0.0) ? vb[j] * wl : 0.0;
+                if (gv == 0.0) continue;
+                if (moved) {
+                    dzo += gv * (S1[il] - S1[il - 1]) * (-c->lambda[j] / ((1.0 + z) * (1.0 + z))) /
+                           (c->lambda[il] - c->lambda[il - 1]);
+                    g1[il - 1] += (1.0 - t) * gv;
+                    g1[il] += t * gv;
+                } else {
+                    g1[j] += gv;
+                }
+            }
+            /* smoothing: d sigma, then the transpose */
+            double dsg = 0.0;
+            memcpy(g0, g1, sizeof(double) * nspec);
+            if (sm_n > 0) {
+                const double sb = sig / (C_LIGHT * 1.0e-13);
+                const double kfac = 1.0 / (sb * sb * sb * C_LIGHT * 1.0e-13);
+                for (int i = 0; i < nspec; i++) if (slo[i] != 0) g0[i] = 0.0;
+                for (int i = 0; i < nspec; i++) {
+                    const int a = slo[i];
+                    if (a == 0) continue;
+                    const int off = soff[i] - a;   /* coef(off + k), 1-based k */
+                    const double lli = log(c->lambda[i]);
+                    double x2m = 0.0;
+                    for (int k = a; k <= shi[i]; k++) {
+                        const double x = lli - log(c->lambda[k - 1]);
+                        x2m += scoef[off + k - 1] * x * x;
+                    }
+                    double sx = 0.0;
+                    for (int k = a; k <= shi[i]; k++) {
+                        const double x = lli - log(c->lambda[k - 1]), cik = scoef[off + k - 1];
+                        sx += cik * (x * x - x2m) * S0[k - 1];
+                        g0[k - 1] += cik * g1[i];
+                    }
+                    dsg += g1[i] * sx;
+                }
+                dsg *= kfac;
+            }
+            /* IGM and attenuation */
+            double d[8] = {0}, d13 = 0.0, d14 = 0.0;
+            for (int i = 0; i < nspec; i++) {
+                const double ga = g0[i] * tig[i];
+                const double curve = ed != 0.0 ? exp(ed * ltab[i]) : 1.0;
+                const double td = exp(-(dust2 * curve));
+                const double pb = exp(ed1 * ltab[i]);
+                const double tb = no_birth ? 1.0 : exp(-(dust1 * pb));
+                const double to = no_old ? 1.0 : exp(-(dust3 * curve));
+                const double dfac = nodust ? td * (1.0 - fnd) + fnd : 1.0;
+                const double ay = tb * (1.0 - fob) + fob;
+                yb[i] = ga * ay * dfac;
+                ob[i] = ga * to * dfac;
+                const double gd = ga * (y[i] * ay + o[i] * to);
+                const double gtd = nodust ? gd * (1.0 - fnd) : 0.0;
+                const double gtb = ga * y[i] * (1.0 - fob) * dfac;
+                const double gto = ga * o[i] * dfac;
+                if (igm_on && i < jtop) {
+                    const int pp = (i + 1 > jmax ? i + 1 : jmax) - 1;
+                    const double *a6 = pa + 6 * pp;
+                    const double dtz = a6[0] * bd6[0] + a6[1] * bd6[1] + a6[2] * bd6[2] + a6[3] * bd6[3] + a6[4] * bd6[4] +
+                                       a6[5] * bd6[5];
+                    d13 += -ga * A[i] * igmf * dtz;
+                    d14 += -ga * A[i] * tauv[i];
+                }
+                if (no_birth) {
+                    d[1] += -gtb * pb;
+                } else {
+                    d[1] += -gtb * pb * tb;
+                    d[5] += -gtb * dust1 * pb * ltab[i] * tb;
+                }
+                d[2] += -gtd * td * curve;
+                d[3] += -gto * to * curve;
+                d[4] += -(gtd * td * dust2 + gto * to * dust3) * curve * ltab[i];
+                d[6] += ga * y[i] * (1.0 - tb) * dfac;
+                if (nodust) d[7] += gd * (1.0 - td);
+            }
+            /* the age sums */
+            const double lb = lbol_bar ? lbol_bar[m] : 0.0, mb = mass_bar ? mass_bar[m] : 0.0;
+            const double lsb = ls > SAFE_FLOOR ? lb / (ls * ln10) : 0.0;
+            double dzb = 0.0;
+            if (wz_on) {
+                for (int zz = 0; zz < nz; zz++)
+                    for (int k = 1; k <= nt; k++) {
+                        const double *bs = c->basis + ((size_t)zz * nt + k - 1) * nspec;
+                        const double *gb = k <= it ? yb : ob;
+                        double s = 0.0;
+                        for (int i = 0; i < nspec; i++) s += bs[i] * gb[i];
+                        if (weights_z_bar)
+                            weights_z_bar[((size_t)m * nz + zz) * nt + k - 1] =
+                                s + lsb * pow(10.0, c->blbol[(size_t)zz * nt + k - 1]) + mb * c->bmass[(size_t)zz * nt + k - 1];
+                    }
+                if (weights_bar) for (int k = 0; k < nt; k++) weights_bar[(size_t)m * nt + k] = 0.0;
+            } else {
+                const int zl = zlo - 1, zh = blend ? zlo : zlo - 1;
+                for (int k = 1; k <= nt; k++) {
+                    const double *gb = k <= it ? yb : ob;
+                    const double *bl = c->basis + ((size_t)zl * nt + k - 1) * nspec, *bh = c->basis + ((size_t)zh * nt + k - 1) * nspec;
+                    double sl = 0.0, sh = 0.0;
+                    for (int i = 0; i < nspec; i++) { sl += bl[i] * gb[i]; sh += bh[i] * gb[i]; }
+                    double mk, lk, wb;
+                    if (blend) {
+                        mk = (1.0 - dz) * c->bmass[(size_t)zl * nt + k - 1] + dz * c->bmass[(size_t)zh * nt + k - 1];
+                        lk = pow(10.0, (1.0 - dz) * c->blbol[(size_t)zl * nt + k - 1] + dz * c->blbol[(size_t)zh * nt + k - 1]);
+                        wb = (1.0 - dz) * sl + dz * sh;
+                    } else {
+                        mk = c->bmass[k - 1];
+                        lk = pow(10.0, c->blbol[k - 1]);
+                        wb = (1.0 - dz) * sl;
+                    }
+                    if (weights_bar) weights_bar[(size_t)m * nt + k - 1] = wb + lsb * lk + mb * mk;
+                    if (blend) {
+                        const double wk = w[k - 1] > SAFE_FLOOR ? w[k - 1] : 0.0;
+                        dzb += wk * (sh - sl);
+                        if (wk > 0.0) {
+                            dzb += lsb * wk * lk * ln10 * (c->blbol[(size_t)zh * nt + k - 1] - c->blbol[(size_t)zl * nt + k - 1]);
+                            dzb += mb * wk * (c->bmass[(size_t)zh * nt + k - 1] - c->bmass[(size_t)zl * nt + k - 1]);
+                        }
+                    }
+                }
+            }
+            if (params_bar) {
+                double *pb = params_bar + (size_t)m * ORC_NBGRAD;
+                for (int r = 0; r < ORC_NBGRAD; r++) pb[r] = 0.0;
+                for (int r = 1; r <= 7; r++) pb[r] = d[r];
+                if (blend) pb[15] = dzb / (c->logz[zlo] - c->logz[zlo - 1]);
+                pb[22] = dzo + d13 + dzd;
+                pb[23] = dsg;
+                pb[32] = d14;
+            }
+        }
+        free(p); free(w); free(t1); free(t2); free(y); free(o); free(A); free(S0); free(S1); free(tig); free(tauv);
+        free(work); free(fobs); free(fb); free(fbb); free(vb); free(g1); free(g0); free(yb); free(ob);
+        free(slo); free(shi); free(soff); free(scoef);
+    }
+    free(jlo); free(jhi); free(do_band); free(bt); free(wtl); free(ltab); free(pa);
     return status;
 }
 

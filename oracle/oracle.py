@@ -53,6 +53,7 @@ def lib():
         L.orc_filter_band.argtypes = [_dp, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, _dp]
         L.orc_filter_band.restype = C.c_int
         L.orc_csp_batch.restype = C.c_int
+        L.orc_csp_batch_grad.restype = C.c_int
         L.orc_num_threads.restype = C.c_int
         L.orc_powerlaw_bin.argtypes = [C.c_double] * 4
         L.orc_powerlaw_bin.restype = C.c_double
@@ -242,14 +243,74 @@ class _Ctx(C.Structure):
                 ("mag_mask", C.c_void_p), ("ncosmo", C.c_int), ("cz", C.c_void_p), ("cdl", C.c_void_p),
                 ("add_igm", C.c_int), ("smooth_fast", C.c_int), ("tiny_logt", C.c_double),
                 ("ops", C.c_void_p), ("imf_type", C.c_int), ("remnants", C.c_int), ("salp_ind", C.c_double),
-                ("mdave", C.c_double), ("mlim_bh", C.c_double), ("mlim_ns", C.c_double), ("op_count", C.c_void_p)]
+                ("mdave", C.c_double), ("mlim_bh", C.c_double), ("mlim_ns", C.c_double), ("op_count", C.c_void_p),
+                ("ssp_weights_z", C.c_void_p)]
+
+
+def _context(grid, bands, add_igm, smooth_fast, mag_mask, cosmo, ptr):
+    cz, cdl = cosmo if cosmo is not None else cosmology_table()
+    ctx = _Ctx()
+    ctx.nspec, ctx.nt, ctx.nz, ctx.nbands = grid.nspec, grid.nt, grid.nz, bands.shape[0]
+    ctx.lambda_ = ptr(_d(grid.lam))
+    ctx.time_full = ptr(_d(grid.log_age))
+    ctx.logz = ptr(_d(grid.logz))
+    ctx.basis = ptr(_d(grid.spec))
+    ctx.bmass = ptr(_d(grid.mass))
+    ctx.blbol = ptr(_d(grid.lbol))
+    ctx.bands = ptr(bands)
+    ctx.mag_mask = ptr(_i(mag_mask)) if mag_mask is not None else None
+    ctx.ncosmo = len(cz)
+    ctx.cz = ptr(_d(cz))
+    ctx.cdl = ptr(_d(cdl))
+    ctx.add_igm = int(add_igm)
+    ctx.smooth_fast = int(smooth_fast)
+    ctx.tiny_logt = 0.0
+    return ctx
+
+
+NBGRAD = 44
+# params_bar rows (0-based) this build computes: the reference's BGRAD_* - 1 (opensps_csp_batch.f90:166-213)
+GRAD_ROWS = {"dust1": 1, "dust2": 2, "dust3": 3, "dust_index": 4, "dust1_index": 5, "frac_obrun": 6,
+             "frac_nodust": 7, "logzsol": 15, "zred": 22, "sigma_smooth": 23, "igm_factor": 32}
+
+
+def csp_batch_grad(grid, bands, prm, ssp_weights=None, ssp_weights_z=None, mags_bar=None, spec_bar=None,
+                   mass_bar=None, lbol_bar=None, add_igm=1, smooth_fast=1, mag_mask=None, cosmo=None):
+    """The oracle's reverse pass (FP64). prm (NPAR, n); ssp_weights (n, nt); ssp_weights_z (n, nz, nt);
+    cotangents mags_bar (n, nbands), spec_bar (n, nspec), mass_bar, lbol_bar (n). Returns mags (n, nbands),
+    params_bar (n, NBGRAD), weights_bar (n, nt), weights_z_bar (n, nz, nt) and the status."""
+    keep = []
+
+    def ptr(a):
+        keep.append(a)
+        return a.ctypes.data
+
+    n = prm.shape[1]
+    bands = _d(bands)
+    ctx = _context(grid, bands, add_igm, smooth_fast, mag_mask, cosmo, ptr)
+    if ssp_weights_z is not None:
+        ctx.ssp_weights_z = ptr(_d(ssp_weights_z))
+    out = {"mags": np.zeros((n, bands.shape[0])), "params_bar": np.zeros((n, NBGRAD)),
+           "weights_bar": np.zeros((n, grid.nt)), "weights_z_bar": np.zeros((n, grid.nz, grid.nt))}
+    vp = C.c_void_p
+
+    def opt(a):
+        return vp(ptr(_d(a)) if a is not None else None)
+
+    st = lib().orc_csp_batch_grad(C.byref(ctx), C.c_int(n), vp(ptr(_d(prm))), opt(ssp_weights), opt(mags_bar),
+                                  opt(spec_bar), opt(mass_bar), opt(lbol_bar), vp(out["mags"].ctypes.data),
+                                  vp(out["params_bar"].ctypes.data), vp(out["weights_bar"].ctypes.data),
+                                  vp(out["weights_z_bar"].ctypes.data))
+    out["status"] = st
+    return out
 
 
 def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_mask=None, cosmo=None,
               want=("mags", "spec", "mass", "lbol"), ops=None, imf_type=2, remnants=1, salp_ind=2.35, mdave=0.5,
-              mlim_bh=40.0, mlim_ns=8.5):
+              mlim_bh=40.0, mlim_ns=8.5, ssp_weights_z=None):
     """Runs the oracle's batch. grid: synth.SSPGrid; bands (nbands, nspec); prm (NPAR, n); ops: a
-    synth.SSPOperator (the IMF per model; the grid's bases are then unused).
+    synth.SSPOperator (the IMF per model; the grid's bases are then unused); ssp_weights_z (n, nz, nt): weights
+    over ages and metallicities.
     Returns a dict of outputs (C-order arrays: mags (n, nbands), spec (n, nspec), ...) and the status."""
     keep = []
 
@@ -262,6 +323,8 @@ def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_
     nb = bands.shape[0]
     cz, cdl = cosmo if cosmo is not None else cosmology_table()
     ctx = _Ctx()
+    if ssp_weights_z is not None:
+        ctx.ssp_weights_z = ptr(_d(ssp_weights_z))
     ctx.nspec, ctx.nt, ctx.nz, ctx.nbands = grid.nspec, grid.nt, grid.nz, nb
     ctx.lambda_ = ptr(_d(grid.lam))
     ctx.time_full = ptr(_d(grid.log_age))

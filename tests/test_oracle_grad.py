@@ -1,0 +1,154 @@
+"""Pins of the oracle's reverse pass (orc_csp_batch_grad, the restatement of batch_backward,
+src/physics/opensps_csp_batch.f90:4713-5136) by central finite differences of the oracle's forward batch
+(orc_csp_batch), h = 1e-6 relative, on a small grid: every params_bar row this build computes, entries of
+weights_bar and weights_z_bar, and the mass / L_bol cotangents. A dropped term (the birth cloud's or the old
+stars' screen, the IGM's dz, the distance term, the smoothing's sigma), a wrong sign or a transposed operand
+(basis^T y_bar) fails one of them."""
+import numpy as np
+import pytest
+
+from libopensps_amd import synth
+from libopensps_amd.params import SLOT
+
+import oracle as O
+
+NSPEC, NT, NZ, NB = 900, 24, 4, 12
+
+
+def _params(n, seed, birth):
+    """The adjoint workload's parameters with a velocity dispersion the coarse test grid resolves (its pixels are
+    ~4600 km/s wide; the workload's 50-300 km/s would smooth nothing here)."""
+    prm = synth.adjoint_params(n, seed=seed, birth_cloud=birth)
+    prm[SLOT["sigma_smooth"]] = np.linspace(3000.0, 9000.0, n)
+    return prm
+
+
+def _setup(nz=NZ):
+    g = synth.ssp_grid(seed=3, nspec=NSPEC, nt=NT, nz=nz, )
+    fs = synth.gaussian_filters(NB, seed=7)
+    return g, O.build_bands(fs, g.lam), O.cosmology_table(72.0, 0.27, 0.73)
+
+
+def _loss(fwd, cot):
+    L = 0.0
+    for k in ("mags", "spec", "mass", "lbol"):
+        if k in cot:
+            v = np.where(np.isfinite(fwd[k]), fwd[k], 0.0)
+            L = L + np.sum(cot[k] * v, axis=tuple(range(1, v.ndim)))
+    return L   # per model
+
+
+def _cot(n, g, seed=5):
+    rng = np.random.default_rng(seed)
+    return {"mags": rng.standard_normal((n, NB)), "spec": rng.standard_normal((n, NSPEC)) * 1e3,
+            "mass": rng.standard_normal(n), "lbol": rng.standard_normal(n)}
+
+
+def _fwd(g, b, cz, prm, **kw):
+    r = O.csp_batch(g, b, prm, cosmo=cz, want=("mags", "spec", "mass", "lbol"), **kw)
+    assert r["status"] == 0
+    return r
+
+
+def _grad(g, b, cz, prm, cot, **kw):
+    r = O.csp_batch_grad(g, b, prm, cosmo=cz, mags_bar=cot["mags"], spec_bar=cot["spec"], mass_bar=cot["mass"],
+                         lbol_bar=cot["lbol"], **kw)
+    assert r["status"] == 0
+    return r
+
+
+def _fd_param(g, b, cz, prm, cot, slot, rel=1e-6, **kw):
+    x = prm[slot].copy()
+    h = np.where(x != 0.0, np.abs(x) * rel, rel)
+    pp, pm = prm.copy(), prm.copy()
+    pp[slot] = x + h
+    pm[slot] = x - h
+    return (_loss(_fwd(g, b, cz, pp, **kw), cot) - _loss(_fwd(g, b, cz, pm, **kw), cot)) / (2 * h)
+
+
+def _check(fd, an, tol, what):
+    scale = np.maximum(np.abs(fd), 1e-4 * np.max(np.abs(fd)) + 1e-300)   # (entries below FD resolution)
+    err = np.abs(an - fd) / scale
+    assert np.all(err < tol), f"{what}: fd {fd} analytic {an} err {err}"
+
+
+@pytest.mark.parametrize("birth", [False, True])
+def test_params_bar_weights_z(birth):
+    g, b, cz = _setup()
+    n = 6
+    prm = _params(n, 29, birth)
+    prm[SLOT["zred"]] = np.array([0.0, 0.01, 0.8, 1.7, 2.9, 3.9])   # z = 0: no IGM, no distance term
+    wz = synth.lognormal_weights_z(n, g.log_age, g.logz, seed=31)
+    cot = _cot(n, g)
+    an = _grad(g, b, cz, prm, cot, ssp_weights_z=wz)["params_bar"]
+    rows = ["dust2", "dust_index", "zred", "sigma_smooth", "igm_factor"]
+    if birth:
+        rows += ["dust1", "dust1_index", "dust3", "frac_obrun", "frac_nodust"]
+    for name in rows:
+        fd = _fd_param(g, b, cz, prm, cot, SLOT[name], ssp_weights_z=wz)
+        a = an[:, O.GRAD_ROWS[name]]
+        if name == "zred":
+            # z = 0 is the rest frame (absolute magnitudes, no IGM): the reference's derivative there is zero
+            # (distance_term_dz), and the forward is not continuous across it
+            assert a[0] == 0.0
+            fd, a = fd[1:], a[1:]
+        _check(fd, a, 2e-4, name)
+    # rows without a dependence in this batch (the SFH parameters, logzsol with weights over metallicities)
+    assert np.all(an[:, O.GRAD_ROWS["logzsol"]] == 0.0)
+
+
+def test_weights_z_bar_entries():
+    g, b, cz = _setup()
+    n = 4
+    prm = _params(n, 37, True)
+    wz = synth.lognormal_weights_z(n, g.log_age, g.logz, seed=41)
+    cot = _cot(n, g, seed=9)
+    an = _grad(g, b, cz, prm, cot, ssp_weights_z=wz)["weights_z_bar"]
+    # the entries where the models hold the most mass (an entry far from every model's enrichment history
+    # holds weights too small to difference)
+    order = np.argsort(-np.median(wz, axis=0).ravel())
+    checked = 0
+    for e in order[:12:2]:
+        zz, t = divmod(int(e), NT)
+        h = 1e-6 * wz[:].sum(axis=(1, 2))   # 1e-6 of the mass formed (the weights enter the sums linearly)
+        wp, wm = wz.copy(), wz.copy()
+        wp[:, zz, t] += h
+        wm[:, zz, t] -= h
+        fd = (_loss(_fwd(g, b, cz, prm, ssp_weights_z=wp), cot) - _loss(_fwd(g, b, cz, prm, ssp_weights_z=wm), cot)) / (2 * h)
+        # (a weight within h of zero meets the forward's clamp of weights <= tiny: one-sided there, as the
+        # reference's adjoint, which is the derivative from above)
+        ok = wz[:, zz, t] > 2 * h
+        if ok.any():
+            _check(fd[ok], an[ok, zz, t], 1e-5, f"weights_z_bar[{zz},{t}]")
+            checked += int(ok.sum())
+    assert checked >= 10
+
+
+def test_blended_weights_and_logzsol():
+    g, b, cz = _setup()
+    n = 5
+    prm = _params(n, 43, True)
+    prm[SLOT["logzsol"]] = np.linspace(-1.9, 0.2, n)
+    w = synth.dirichlet_weights(n, NT, seed=47, alpha=2.0)
+    cot = _cot(n, g, seed=13)
+    r = _grad(g, b, cz, prm, cot, ssp_weights=w)
+    fd = _fd_param(g, b, cz, prm, cot, SLOT["logzsol"], ssp_weights=w)
+    _check(fd, r["params_bar"][:, O.GRAD_ROWS["logzsol"]], 1e-5, "logzsol")
+    for t in (0, 5, NT - 1):
+        h = 1e-6 * w.sum(axis=1)
+        wp, wm = w.copy(), w.copy()
+        wp[:, t] += h
+        wm[:, t] -= h
+        fd = (_loss(_fwd(g, b, cz, prm, ssp_weights=wp), cot) - _loss(_fwd(g, b, cz, prm, ssp_weights=wm), cot)) / (2 * h)
+        _check(fd, r["weights_bar"][:, t], 1e-5, f"weights_bar[{t}]")
+
+
+def test_grad_forward_matches_batch():
+    """The reverse pass's own forward is the batch's (its magnitudes)."""
+    g, b, cz = _setup()
+    n = 5
+    prm = _params(n, 53, True)
+    wz = synth.lognormal_weights_z(n, g.log_age, g.logz, seed=59)
+    r = _grad(g, b, cz, prm, _cot(n, g), ssp_weights_z=wz)
+    f = _fwd(g, b, cz, prm, ssp_weights_z=wz)
+    np.testing.assert_allclose(r["mags"], f["mags"], rtol=0, atol=1e-9)
