@@ -53,7 +53,26 @@ typedef struct {
     double *weights;             /* device (nt, n): the FP64 SFH weights; or NULL */
     int *op_count;               /* device (2, n): with the operator, the points of each metallicity pass with a
                                     nonzero IMF weight (the isochrone points the IMF limits select); or NULL */
+    const double *ssp_weights_z; /* device (nt, nz, n) or NULL: weights over ages and metallicities (compute_csp_batch's
+                                    ssp_weights_z, opensps_csp_batch.f90:616-618: age_sums_w / model_sums_zw); they
+                                    replace the SFH and the per-model metallicity blend */
 } opensps_hip_batch_t;
+
+/* The reverse pass of a batch (compute_csp_batch with params_bar / weights_bar / weights_z_bar, include/opensps.h:
+ * 324-348; batch_backward, opensps_csp_batch.f90:4713-5136): the forward batch fwd (its outputs are written as
+ * opensps_hip_csp_batch writes them; fwd.use_ops must be 0), then the derivatives of the loss whose cotangents are
+ * given (device arrays, any NULL): mags_bar (nbands, n), spec_bar (nspec, n), mass_bar, lbol_bar (n). Outputs
+ * (device, any NULL): params_bar (OPENSPS_HIP_NBGRAD, n) with the rows of the reference's BGRAD_* numbering
+ * (opensps_csp_batch.f90:166-213) that this path computes (dust1, dust2, dust3, dust_index, dust1_index,
+ * frac_obrun, frac_nodust, logzsol, zred, sigma_smooth, igm_factor; the others are 0), weights_bar (nt, n) (with
+ * fwd.ssp_weights or SFH parameters), weights_z_bar (nt, nz, n) (with fwd.ssp_weights_z). params_bar with SFH
+ * parameters (no given weights) is status 2: the SFH tangents (opensps_sfh_tangent.f90) are not built. */
+#define OPENSPS_HIP_NBGRAD 44
+typedef struct {
+    opensps_hip_batch_t fwd;
+    const double *mags_bar, *spec_bar, *mass_bar, *lbol_bar;
+    double *params_bar, *weights_bar, *weights_z_bar;
+} opensps_hip_grad_t;
 
 /* ---- context ---- */
 /* No reference counterpart (the reference keeps device data in the Fortran context through OpenMP
@@ -85,6 +104,11 @@ int opensps_hip_set_options(opensps_hip_ctx_t *ctx, int add_igm, int smooth_fast
 /* The batch: compute_csp_batch's stages 1-8 (src/physics/opensps_csp_batch.f90:431-1214) on the device,
  * asynchronous on the context's stream; the status of the models is checked by opensps_hip_sync. */
 int opensps_hip_csp_batch(opensps_hip_ctx_t *ctx, const opensps_hip_batch_t *args);
+/* The forward batch and its reverse pass, asynchronous on the context's stream (opensps_hip_sync waits). Replaces
+ * batch_backward's stages for the outputs above (opensps_csp_batch.f90:4713-5136; backward_observed_frame :6958,
+ * smooth_columns_adjoint / smooth_sigma_adjoint, opensps_batch_photometry.f90:604-700, atten_adjoint_chunk
+ * :6476-6665, backward_age_sums_w :5288-5330, backward_age_sums :5226-5286, batch_mass_cotangent :5142). */
+int opensps_hip_csp_batch_grad(opensps_hip_ctx_t *ctx, const opensps_hip_grad_t *args);
 /* Waits for the context's stream; status 2 if a model of the last batch was unsupported (idx row 3). */
 int opensps_hip_sync(opensps_hip_ctx_t *ctx);
 

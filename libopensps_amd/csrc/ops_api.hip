@@ -133,6 +133,7 @@ int opensps_hip_create(int device, opensps_hip_ctx_t** out) {
 }
 
 static void free_operator(OpsCtx* c);
+static void free_full(OpsCtx* c);
 
 static void free_ptr(void* p) {
     if (p) (void)hipFree(p);
@@ -145,9 +146,11 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
                   c->d_igm_a, c->d_basis64, c->d_A64, c->d_Ahi, c->d_Alo, c->d_bmeta, c->d_bc64,
                   c->d_bc32, c->d_mask, c->d_cz, c->d_cdl, c->yo.p, c->wcol.p, c->w64.p, c->order.p, c->tiles.p,
                   c->misc.p, c->spec.p, c->flag.p, c->d_rdlam, c->d_ltab32, c->d_wl32, c->d_sm32, c->d_bkt, c->hdr.p, c->wsfh.p,
-                  c->opu.p, c->opwc.p, c->opsum.p, c->opseg.p, c->opimf.p, c->opspart.p};
+                  c->opu.p, c->opwc.p, c->opsum.p, c->opseg.p, c->opimf.p, c->opspart.p, c->d_bp_ptr, c->d_bp_band,
+                  c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p};
     for (void* p : ps) free_ptr(p);
     free_operator(c);
+    free_full(c);
     for (int k = 0; k <= OpsCtx::NPROF; k++)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -255,6 +258,7 @@ int opensps_hip_set_grid(opensps_hip_ctx_t* c, int nspec, const double* lam, int
     UPLOAD(c, c->d_blbol, lbol, (size_t)nt * nz);
     UPLOAD(c, c->d_igm_a, igm.data(), igm.size());
     UPLOAD(c, c->d_basis64, spec, (size_t)nspec * nt * nz);
+    free_full(c);   // (the operands over every (age, Z) are formed again at their first use)
     // Both packed operands now (the first batch of either precision then starts at once)
     const int npxblk = c->nsp / 16;
     PackArgs pa;
@@ -334,6 +338,34 @@ int opensps_hip_set_bands(opensps_hip_ctx_t* c, int nbands, const double* bands)
         j0 = q0 < j0 ? q0 : j0;
         j1 = q0 + nq - 1 > j1 ? q0 + nq - 1 : j1;
     }
+    // the same coefficients by pixel over [j0, j1] for the reverse pass (each pixel's bands, in band order)
+    {
+        const int npix = j1 >= j0 ? j1 - j0 + 1 : 0;
+        std::vector<int> bptr(npix + 1, 0), bband;
+        std::vector<double> bc;
+        std::vector<std::vector<std::pair<int, double>>> per(npix);
+        for (int b = 0; b < nbands; b++) {
+            const int4 mt = meta[b];
+            for (int q = 0; q < mt.z; q++) {
+                const double cw = c64[(size_t)mt.x + q];
+                if (cw != 0.0) per[mt.y + q - j0].push_back(std::make_pair(b, cw));
+            }
+        }
+        for (int p = 0; p < npix; p++) {
+            for (auto& e : per[p]) {
+                bband.push_back(e.first);
+                bc.push_back(e.second);
+            }
+            bptr[p + 1] = (int)bband.size();
+        }
+        if (bband.empty()) {
+            bband.push_back(0);
+            bc.push_back(0.0);
+        }
+        UPLOAD(c, c->d_bp_ptr, bptr.data(), bptr.size());
+        UPLOAD(c, c->d_bp_band, bband.data(), bband.size());
+        UPLOAD(c, c->d_bp_c64, bc.data(), bc.size());
+    }
     if (c64.empty()) c64.assign(4, 0.0);
     std::vector<float> c32(c64.size());
     for (size_t i = 0; i < c64.size(); i++) c32[i] = (float)c64[i];
@@ -364,6 +396,73 @@ static void free_operator(OpsCtx* c) {
                   c->op.d_poly};
     for (void* p : ps) free_ptr(p);
     c->op = OpsCtx::OpsOperator();
+}
+
+static void free_full(OpsCtx* c) {
+    void* ps[] = {c->full.A64, c->full.Ahi, c->full.Alo, c->full.At64, c->full.Ath, c->full.Atl};
+    for (void* p : ps) free_ptr(p);
+    c->full = OpsCtx::Full();
+}
+
+// The basis over every (age, Z) as the operands of the weights over metallicities: the forward product's packed A
+// (K = nt nz, metallicities outer, as ssp_weights_z) and, with transposed, the reverse product's rows; formed at
+// their first use after set_grid, for the precision asked
+static int ensure_full(OpsCtx* c, bool mixed, bool transposed) {
+    OpsCtx::Full& f = c->full;
+    const int kfull = c->nt * c->nz, npxblk = c->nsp / 16;
+    if (f.kfull == 0) {
+        f.kfull = kfull;
+        f.k64 = ((kfull + OPS_KSTEP_F64 - 1) / OPS_KSTEP_F64) * OPS_KSTEP_F64;
+        f.kbf = ((kfull + OPS_KSTEP_BF16 - 1) / OPS_KSTEP_BF16) * OPS_KSTEP_BF16;
+        f.nrt = (kfull + 63) / 64;
+    }
+    PackArgs pa;
+    pa.nspec = c->nspec;
+    pa.nt = kfull;
+    pa.nz = 1;
+    pa.npair = 1;
+    pa.kpair = kfull;
+    pa.npxblk = npxblk;
+    pa.basis = c->d_basis64;
+    if (!mixed && !f.A64) {
+        const size_t n64 = (size_t)npxblk * (f.k64 / OPS_KSTEP_F64) * 64;
+        HIPCHK(c, hipMalloc((void**)&f.A64, n64 * sizeof(double)));
+        pa.kpad = f.k64;
+        pa.A64 = f.A64;
+        k_pack_f64<<<(unsigned)((n64 + 255) / 256), 256, 0, c->stream>>>(pa);
+        HIPCHK(c, hipGetLastError());
+    }
+    if (mixed && !f.Ahi) {
+        const size_t nbf = (size_t)npxblk * (f.kbf / OPS_KSTEP_BF16) * 64;
+        HIPCHK(c, hipMalloc((void**)&f.Ahi, nbf * 8 * sizeof(uint16_t)));
+        HIPCHK(c, hipMalloc((void**)&f.Alo, nbf * 8 * sizeof(uint16_t)));
+        pa.kpad = f.kbf;
+        pa.Ahi = f.Ahi;
+        pa.Alo = f.Alo;
+        k_pack_bf16<<<(unsigned)((nbf + 255) / 256), 256, 0, c->stream>>>(pa);
+        HIPCHK(c, hipGetLastError());
+    }
+    if (transposed) {
+        const size_t nel = (size_t)f.nrt * 64 * c->nsp;
+        double* a64 = nullptr;
+        uint16_t *ah = nullptr, *al = nullptr;
+        if (!mixed && !f.At64) {
+            HIPCHK(c, hipMalloc((void**)&f.At64, nel * sizeof(double)));
+            a64 = f.At64;
+        }
+        if (mixed && !f.Ath) {
+            HIPCHK(c, hipMalloc((void**)&f.Ath, nel * sizeof(uint16_t)));
+            HIPCHK(c, hipMalloc((void**)&f.Atl, nel * sizeof(uint16_t)));
+            ah = f.Ath;
+            al = f.Atl;
+        }
+        if (a64 || ah) {
+            k_pack_t<<<(unsigned)((nel + 255) / 256), 256, 0, c->stream>>>(c->nspec, c->nsp, kfull, f.nrt * 64,
+                                                                          c->d_basis64, a64, ah, al);
+            HIPCHK(c, hipGetLastError());
+        }
+    }
+    return 0;
 }
 
 // imf_bin_logs (src/physics/opensps_imf.f90:564-579): ln m1 and ln(m2 / m1) in Kahan's log1p form
@@ -631,7 +730,14 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     const bool mixed = a->precision == OPENSPS_HIP_MIXED;
     const bool ops = a->use_ops != 0;
     if (ops && !c->op.on) return ops_fail(c, 1, "batch: no SSP operator installed (opensps_hip_set_ssp_ops)");
-    const int kpad = ops ? (mixed ? c->op.kbf : c->op.k64) : (mixed ? c->kbf : c->k64);
+    const bool wz = a->ssp_weights_z != nullptr;
+    if (wz && ops) return ops_fail(c, 2, "batch: ssp_weights_z with the SSP operator (the reference refuses it too)");
+    if (wz && a->ssp_weights) return ops_fail(c, 3, "batch: ssp_weights and ssp_weights_z both given");
+    int st;
+    if (wz && (st = ensure_full(c, mixed, false))) return st;
+    const bool uni = ops || wz;   // one A operand for every column (uniform column tiles)
+    const int kpad = ops ? (mixed ? c->op.kbf : c->op.k64)
+                         : wz ? (mixed ? c->full.kbf : c->full.k64) : (mixed ? c->kbf : c->k64);
     const int npxblk = c->nsp / 16, npxt = c->nsp / OPS_TILE_PX;
     const size_t tsz = mixed ? sizeof(float) : sizeof(double);
     // column tiles: by metallicity pair over model columns (the operator groups models, parts = 1), then with the
@@ -642,9 +748,8 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     HIPCHK(c, hipSetDevice(c->device));
 
     // scratch: misc = zlo, it, status, order (n ints each) | ntiles | flag
-    int st;
     if ((st = ops_reserve(c, c->misc, sizeof(int) * ((size_t)4 * n + 64)))) return st;
-    if ((st = ops_reserve(c, c->tiles, sizeof(int4) * (size_t)(maxtiles + (ops ? optiles : 0))))) return st;
+    if ((st = ops_reserve(c, c->tiles, sizeof(int4) * (size_t)(maxtiles + (uni ? optiles : 0))))) return st;
     if ((st = ops_reserve(c, c->hdr, sizeof(double) * OPS_HDR_N * (size_t)n))) return st;
     if ((st = ops_reserve(c, c->yo, tsz * (size_t)parts * n * c->nsp))) return st;
     if (mixed) {
@@ -676,11 +781,13 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     pa.nt = c->nt;
     pa.nz = c->nz;
     pa.parts = parts;
-    pa.kpair = c->kpair;
+    pa.kpair = wz ? c->full.kfull : c->kpair;
     pa.kpad = kpad;
     pa.mixed = mixed;
     pa.smooth_fast = c->smooth_fast;
     pa.ops = ops;
+    pa.wz = a->ssp_weights_z;
+    pa.lsum = c->lsum_out;
     pa.tiny_logt = c->tiny_logt;
     pa.prm = a->prm;
     pa.weights_in = a->ssp_weights;
@@ -730,6 +837,10 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ga.tiles = (int4*)c->tiles.p;
     k_group<<<1, 256, sizeof(int) * 256 * (size_t)c->npair, c->stream>>>(ga);
     HIPCHK(c, hipGetLastError());
+    if (wz) {
+        k_op_tiles<<<1, 256, 0, c->stream>>>(n * parts, d_optiles, (int4*)c->tiles.p + maxtiles);
+        HIPCHK(c, hipGetLastError());
+    }
     if (ops) {
         if (a->weights)
             HIPCHK(c, hipMemcpyAsync(a->weights, c->wsfh.p, sizeof(double) * (size_t)n * c->nt, hipMemcpyDeviceToDevice,
@@ -852,17 +963,17 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     gm.npxblk = npxblk;
     gm.kpad = kpad;
     gm.order = d_order;
-    gm.ntiles = ops ? d_optiles : d_ntiles;
-    gm.tiles = (const int4*)c->tiles.p + (ops ? maxtiles : 0);
-    gm.A64 = ops ? c->op.d_A64 : c->d_A64;
+    gm.ntiles = uni ? d_optiles : d_ntiles;
+    gm.tiles = (const int4*)c->tiles.p + (uni ? maxtiles : 0);
+    gm.A64 = ops ? c->op.d_A64 : wz ? c->full.A64 : c->d_A64;
     gm.w64 = (const double*)c->w64.p;
-    gm.Ahi = ops ? c->op.d_Ahi : c->d_Ahi;
-    gm.Alo = ops ? c->op.d_Alo : c->d_Alo;
+    gm.Ahi = ops ? c->op.d_Ahi : wz ? c->full.Ahi : c->d_Ahi;
+    gm.Alo = ops ? c->op.d_Alo : wz ? c->full.Alo : c->d_Alo;
     gm.whi = pa.whi;
     gm.wlo = pa.wlo;
     gm.yo = c->yo.p;
     const unsigned ptx = (unsigned)((npxt + 7) / 8);
-    const unsigned gblocks = 8u * ptx * (unsigned)(ops ? optiles : maxtiles);
+    const unsigned gblocks = 8u * ptx * (unsigned)(uni ? optiles : maxtiles);
     if (mixed)
         k_gemm_bf16x3<<<gblocks, 256, 0, c->stream>>>(gm);
     else if (c->f64_vector)
@@ -938,6 +1049,152 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
         c->ev_pending = true;
     }
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// The reverse pass: the forward batch (keeping its age sums, band fluxes, SFH weights and L_bol sums), then
+// k_grad_chain -> the transposed product -> k_grad_out, on the context's stream
+// ------------------------------------------------------------------------------------------------
+int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g) {
+    if (!c) return 1;
+    if (!g) return ops_fail(c, 3, "batch_grad: NULL arguments");
+    const opensps_hip_batch_t& f = g->fwd;
+    const int n = f.n;
+    if (f.use_ops) return ops_fail(c, 2, "batch_grad: the SSP operator's reverse pass (the IMF rows) is not built");
+    if (g->params_bar && !f.ssp_weights && !f.ssp_weights_z)
+        return ops_fail(c, 2, "batch_grad: params_bar with SFH parameters (the SFH tangents) is not built; give "
+                              "ssp_weights or ssp_weights_z");
+    if (g->weights_z_bar && !f.ssp_weights_z) return ops_fail(c, 3, "batch_grad: weights_z_bar needs ssp_weights_z");
+    if (n < 0) return ops_fail(c, 3, "batch_grad: n < 0");
+    if (n == 0) return 0;
+    if (c->nsp % 32 != 0) return ops_fail(c, 3, "batch_grad: padded grid not a multiple of 32");
+    const bool mixed = f.precision == OPENSPS_HIP_MIXED;
+    // young and old sums always apart: the derivatives of the screens' parameters (dust1, dust3, frac_obrun) need
+    // the split where the forward does not (at dust1 = 0 or dust3 = 0, one-sided, as atten_adjoint_chunk)
+    const int parts = 2, nsp = c->nsp, nt = c->nt, nz = c->nz;
+    const bool wz = f.ssp_weights_z != nullptr;
+    int st;
+    if ((st = ensure_full(c, mixed, true))) return st;
+    const int kfull = c->full.kfull, nrt = c->full.nrt, ldc = nrt * 64, ncols = n * parts;
+    if ((st = ops_reserve(c, c->glsum, sizeof(double) * (size_t)n))) return st;
+    if ((st = ops_reserve(c, c->gfband, sizeof(double) * (size_t)n * (c->nbands > 0 ? c->nbands : 1)))) return st;
+    if ((st = ops_reserve(c, c->gw, sizeof(double) * (size_t)n * nt))) return st;
+    if ((st = ops_reserve(c, c->gout, sizeof(double) * OPS_GOUT_N * (size_t)n))) return st;
+    if ((st = ops_reserve(c, c->gyb, (mixed ? 2 * sizeof(uint16_t) : sizeof(double)) * (size_t)ncols * nsp))) return st;
+    if ((st = ops_reserve(c, c->gC, (mixed ? sizeof(float) : sizeof(double)) * (size_t)ncols * ldc))) return st;
+    // the forward batch
+    opensps_hip_batch_t fw = f;
+    fw.split_parts = 1;
+    if (!fw.fband) fw.fband = (double*)c->gfband.p;
+    if (!fw.weights) fw.weights = (double*)c->gw.p;
+    c->lsum_out = (double*)c->glsum.p;
+    st = opensps_hip_csp_batch(c, &fw);
+    c->lsum_out = nullptr;
+    if (st) return st;
+    int* misc = (int*)c->misc.p;
+    const int *d_zlo = misc, *d_it = misc + n;
+
+    GradArgs ga;
+    ga.n = n;
+    ga.nspec = c->nspec;
+    ga.nsp = nsp;
+    ga.parts = parts;
+    ga.nbands = c->nbands;
+    ga.jtop = c->jtop;
+    ga.add_igm = c->add_igm;
+    ga.band_j0 = c->band_j0;
+    ga.band_j1 = c->nbands > 0 ? c->band_j1 : -1;
+    ga.ncosmo = c->ncosmo;
+    LamLookup lk;
+    lk.nspec = c->nspec;
+    lk.nbk = c->nbk;
+    lk.lnl0 = c->lnl0;
+    lk.bkinv = c->bkinv;
+    lk.lam = c->d_lam;
+    lk.bkt = c->d_bkt;
+    ga.lk = lk;
+    ga.prm = f.prm;
+    ga.hdr = (const double*)c->hdr.p;
+    ga.yo = c->yo.p;
+    ga.lam = c->d_lam;
+    ga.loglam = c->d_loglam;
+    ga.ltab = c->d_ltab;
+    ga.wl = c->d_wl;
+    ga.dlam = c->d_dlam;
+    ga.rdlam = c->d_rdlam;
+    ga.igm_a = c->d_igm_a;
+    ga.cz = c->d_cz;
+    ga.cdl = c->d_cdl;
+    ga.sm32 = c->d_sm32;
+    ga.mask = c->d_mask;
+    ga.bp_ptr = c->d_bp_ptr;
+    ga.bp_band = c->d_bp_band;
+    ga.bp_c64 = c->d_bp_c64;
+    ga.mags = fw.mags;
+    ga.fband = fw.fband;
+    ga.lsum = (const double*)c->glsum.p;
+    ga.mags_bar = g->mags_bar;
+    ga.spec_bar = g->spec_bar;
+    ga.lbol_bar = g->lbol_bar;
+    ga.ybh = c->gyb.p;
+    ga.ybl = mixed ? (void*)((uint16_t*)c->gyb.p + (size_t)ncols * nsp) : nullptr;
+    ga.gout = (double*)c->gout.p;
+    const size_t tsz = mixed ? sizeof(float) : sizeof(double);
+    const size_t lds = 3 * tsz * (size_t)nsp + sizeof(double) * ((c->nbands > 0 ? c->nbands : 1) + 8 * 16);
+    const void* kfn = mixed ? (const void*)k_grad_chain<float> : (const void*)k_grad_chain<double>;
+    HIPCHK(c, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (mixed)
+        k_grad_chain<float><<<n, 512, lds, c->stream>>>(ga);
+    else
+        k_grad_chain<double><<<n, 512, lds, c->stream>>>(ga);
+    HIPCHK(c, hipGetLastError());
+
+    GemmTArgs gt;
+    gt.krows = kfull;
+    gt.nrt = nrt;
+    gt.ncols = ncols;
+    gt.nct = (ncols + 63) / 64;
+    gt.nsp = nsp;
+    gt.ldc = ldc;
+    gt.Ah = c->full.Ath;
+    gt.Al = c->full.Atl;
+    gt.Bh = (const uint16_t*)ga.ybh;
+    gt.Bl = (const uint16_t*)ga.ybl;
+    gt.A64 = c->full.At64;
+    gt.B64 = (const double*)c->gyb.p;
+    gt.C = c->gC.p;
+    const unsigned gblocks = 8u * (unsigned)((nrt + 7) / 8) * (unsigned)gt.nct;
+    if (mixed)
+        k_gemmt_bf16x3<<<gblocks, 256, 0, c->stream>>>(gt);
+    else
+        k_gemmt_f64<<<gblocks, 256, 0, c->stream>>>(gt);
+    HIPCHK(c, hipGetLastError());
+
+    GradOutArgs go;
+    go.n = n;
+    go.nt = nt;
+    go.nz = nz;
+    go.parts = parts;
+    go.kfull = kfull;
+    go.ldc = ldc;
+    go.wz = wz;
+    go.zlo = d_zlo;
+    go.it = d_it;
+    go.hdr = (const double*)c->hdr.p;
+    go.logz = c->d_logz;
+    go.bmass = c->d_bmass;
+    go.blbol = c->d_blbol;
+    go.w = fw.weights;
+    go.mass_bar = g->mass_bar;
+    go.gout = (const double*)c->gout.p;
+    go.C = c->gC.p;
+    go.mixed = mixed;
+    go.params_bar = g->params_bar;
+    go.weights_bar = g->weights_bar;
+    go.weights_z_bar = g->weights_z_bar;
+    k_grad_out<<<n, 256, 0, c->stream>>>(go);
+    HIPCHK(c, hipGetLastError());
     return 0;
 }
 

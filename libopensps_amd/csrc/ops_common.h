@@ -103,3 +103,9 @@ __device__ __forceinline__ T wave_sum(T v) {
 __device__ __forceinline__ void argmax_merge(double& v, int& j, double v2, int j2) {
     if (v2 > v || (v2 == v && j2 < j)) { v = v2; j = j2; }
 }
+
+// The six-term IGM optical depth of a pixel: its terms a (igm_tau_pixel_terms) times the (1 + z) powers b6
+// (igm_tau_z_terms), src/physics/opensps_cosmology.f90:458-507
+__device__ __forceinline__ double igm_tau6(const double* __restrict__ aj, const double* b6) {
+    return aj[0] * b6[0] + aj[1] * b6[1] + aj[2] * b6[2] + aj[3] * b6[3] + aj[4] * b6[4] + aj[5] * b6[5];
+}

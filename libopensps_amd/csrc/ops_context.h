@@ -80,6 +80,8 @@ struct OpsCtx {
     double* d_bc64 = nullptr;                      // per (band, pixel of its support) coefficient, FP64
     float* d_bc32 = nullptr;                       // the same in FP32
     int* d_mask = nullptr;                         // per band: computed (1) or NaN (0)
+    int *d_bp_ptr = nullptr, *d_bp_band = nullptr; // the band coefficients by pixel over [band_j0, band_j1] (CSR)
+    double* d_bp_c64 = nullptr;
     std::vector<int> h_mask;
 
     // ---- cosmology ----
@@ -99,6 +101,17 @@ struct OpsCtx {
     OpsBuffer wsfh, opu, opwc, opsum, opseg;   // the operator path: SFH weights, point weights, age factors,
                                                // per-age sums, segment sums
     OpsBuffer opimf, opspart;                  // per-model IMF constants; the off-path bins' per-age sums and counts
+    // ---- weights over ages and metallicities, and the reverse pass ----
+    struct Full {
+        int kfull = 0, k64 = 0, kbf = 0, nrt = 0;  // K = nt nz; padded to the products' K; row tiles of 64
+        double* A64 = nullptr;                      // the basis as the forward product's packed A (K = every (age, Z))
+        uint16_t *Ahi = nullptr, *Alo = nullptr;
+        double* At64 = nullptr;                     // the basis as the transposed product's rows (k, pixel)
+        uint16_t *Ath = nullptr, *Atl = nullptr;
+    } full;
+    OpsBuffer glsum, gfband, gw, gout, gyb, gC;   // reverse pass: L_bol sums, band fluxes, SFH weights, per-model
+                                                  // scalars, y_bar rows, the transposed product
+    double* lsum_out = nullptr;                   // set by the reverse pass around its forward batch
     int* d_last_flag = nullptr;   // the per-batch status flag of the last batch (read by sync)
 
     // ---- per-kernel timing (HIP events on the context's stream) ----

@@ -222,13 +222,16 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     for (int q = 0; q < OPS_NPAR; q++) p[q] = a.prm[(size_t)q * n + m];
 
     int st = 0;
-    if (!a.weights_in && p[OPS_P_TAGE] <= 0.0) st = 2;                          // batch_check_models :1312
+    if (!a.weights_in && !a.wz && p[OPS_P_TAGE] <= 0.0) st = 2;                // batch_check_models :1312
     if (!a.smooth_fast && p[OPS_P_SIGMA_SMOOTH] > OPS_SAFE_FLOOR) st = 2;     // host smoothing modes: not built
     // split_parts = 0 folds young and old into one sum: only without birth-cloud and old-star screens
     bool needs_parts = (p[OPS_P_DUST1] > OPS_SAFE_FLOOR && 1.0 - p[OPS_P_FRAC_OBRUN] > OPS_SAFE_FLOOR) ||
                        fabs(p[OPS_P_DUST3]) > OPS_SAFE_FLOOR;
     if (needs_parts && a.parts == 1) st = 4;
-    if (st == 0) {
+    if (st == 0 && a.wz) {
+        for (int k = lane; k < nt; k += 64) w[k] = 0.0;   // (no SFH weights: the weights run over metallicities)
+        WAVE_SYNC();
+    } else if (st == 0) {
         if (a.weights_in) {
             for (int k = lane; k < nt; k += 64) w[k] = a.weights_in[(size_t)m * nt + k];
             WAVE_SYNC();
@@ -252,7 +255,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     int it = max(1, min(find_interval_ref(a.time, nt, dal), nt));
     int zlo = 1;
     double dz = 0.0;
-    if (a.nz > 1) {
+    if (a.nz > 1 && !a.wz) {
         zlo = max(1, min(find_interval_ref(a.logz, a.nz, p[OPS_P_LOGZSOL]), a.nz - 1));
         dz = (p[OPS_P_LOGZSOL] - a.logz[zlo - 1]) / (a.logz[zlo] - a.logz[zlo - 1]);
     }
@@ -277,8 +280,19 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
             ls += lk;
         }
     }
+    // weights over ages and metallicities: model_sums_zw (opensps_csp_batch.f90:2086-2127), every (age, Z)
+    const double* wzm = a.wz ? a.wz + (size_t)m * nt * a.nz : nullptr;
+    if (wzm && st == 0)
+        for (int k = lane; k < nt * a.nz; k += 64) {
+            const double wk = wzm[k];
+            if (wk > OPS_SAFE_FLOOR) {
+                ms += wk * a.bmass[k];
+                ls += wk * pow(10.0, a.blbol[k]);
+            }
+        }
     ms = wave_sum(ms);
     ls = wave_sum(ls);
+    if (lane == 0 && a.lsum) a.lsum[m] = ls;
     if (lane == 0) {
         if (a.mass && !a.ops) a.mass[m] = ms;
         if (a.lbol && !a.ops) a.lbol[m] = ls > OPS_SAFE_FLOOR ? log10(ls) : 0.0;
@@ -297,7 +311,16 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
         size_t col = (size_t)m * a.parts + part;
         for (int k = lane; k < a.kpad; k += 64) {
             double v = 0.0;
-            if (k < a.kpair) {
+            if (wzm) {
+                // K = every (age, Z), metallicities outer: age_sums_w (:2269-2324)
+                if (k < a.kpair && st == 0) {
+                    const int age = k % nt;
+                    double wk = wzm[k];
+                    if (!(wk > OPS_SAFE_FLOOR)) wk = 0.0;
+                    const bool young = age + 1 <= it;
+                    if ((a.parts == 1) || (part == 0 ? young : !young)) v = wk;
+                }
+            } else if (k < a.kpair) {
                 int age = (a.nz > 1) ? (k % nt) : k;   // 0-based age
                 double wk = w[age];
                 if (!(wk > OPS_SAFE_FLOOR)) wk = 0.0;
@@ -610,9 +633,6 @@ __global__ void k_pack_bf16(PackArgs a) {
 // src/spectra/opensps_batch_photometry.f90:784-794, with the clamped find_interval of z in the cosmology
 // table, descending in z as the reference builds it, or ascending).
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ double igm_tau6(const double* __restrict__ aj, const double* b6) {
-    return aj[0] * b6[0] + aj[1] * b6[1] + aj[2] * b6[2] + aj[3] * b6[3] + aj[4] * b6[4] + aj[5] * b6[5];
-}
 
 __device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, double dz, int lane) {
     const double z = p[OPS_P_ZRED], opz = 1.0 + z;

@@ -32,6 +32,8 @@ struct PrepArgs {
     int n, nt, nz, parts, kpair, kpad, mixed, smooth_fast, ops;   // ops: the operator path (no W columns, no sums)
     double tiny_logt;
     const double *prm, *weights_in, *time, *logz, *bmass, *blbol;
+    const double* wz;           // weights over ages and metallicities (nt nz per model) or NULL
+    double* lsum;               // per model the L_bol sum before the log (the reverse pass), or NULL
     double *w_out, *mass, *lbol, *w64;
     uint16_t *whi, *wlo;
     int *zlo, *it, *idx, *status, *flag;
@@ -170,6 +172,58 @@ struct ChainArgs {
     void* stage;   // n x nspec staging rows of wide smoothing windows without a spectrum out (single-buffer chain)
     double *mags, *fband;
 };
+
+// ---- the reverse pass (ops_grad.hip) ----
+#define OPENSPS_HIP_NBGRAD_ROWS 44   // params_bar rows (the reference's NBGRAD, opensps_csp_batch.f90:213)
+#define OPS_GOUT_N 16   // per model: the reverse chain's scalars (k_grad_chain -> k_grad_out)
+#define OPS_GOUT_C 0        // cscale: the model's cotangents are divided by it (FP32 range), results times it
+#define OPS_GOUT_LSB 1      // dL / dL_bol-sum (scaled)
+#define OPS_GOUT_D 2        // 2..8: dust1, dust2, dust3, dust_index, dust1_index, frac_obrun, frac_nodust (scaled)
+#define OPS_GOUT_DZO 9      // the observed frame's dz (scaled)
+#define OPS_GOUT_DZI 10     // the IGM's dz (scaled)
+#define OPS_GOUT_DIGM 11    // igm_factor (scaled)
+#define OPS_GOUT_DSG 12     // sigma_smooth (scaled)
+#define OPS_GOUT_DZD 13     // the distance term's dz (not scaled)
+
+struct GradArgs {
+    int n, nspec, nsp, parts, nbands, jtop, add_igm, band_j0, band_j1, ncosmo;
+    LamLookup lk;
+    const double *prm, *hdr;
+    const void* yo;
+    const double *lam, *loglam, *ltab, *wl, *dlam, *rdlam, *igm_a, *cz, *cdl;
+    const float4* sm32;
+    const int* mask;
+    const int *bp_ptr, *bp_band;   // band coefficients by pixel over [band_j0, band_j1] (CSR)
+    const double* bp_c64;
+    const double *mags, *fband, *lsum;
+    const double *mags_bar, *spec_bar, *lbol_bar;
+    void *ybh, *ybl;               // y_bar rows (model part) x nsp: bf16 hi / lo (mixed) or FP64 (ybh)
+    double* gout;                  // per model OPS_GOUT_N
+};
+
+struct GemmTArgs {                 // C(col, k) = sum_j At(k, j) Yb(col, j)
+    int krows, nrt, ncols, nct, nsp, ldc;
+    const uint16_t *Ah, *Al, *Bh, *Bl;   // bf16: At (nrt 64 x nsp), Yb (ncols x nsp)
+    const double *A64, *B64;
+    void* C;                       // (ncols, ldc) FP32 or FP64
+};
+
+struct GradOutArgs {
+    int n, nt, nz, parts, kfull, ldc, wz;
+    const int *zlo, *it;
+    const double *hdr, *logz, *bmass, *blbol, *w, *mass_bar, *gout;
+    const void* C;
+    int mixed;
+    double *params_bar, *weights_bar, *weights_z_bar;
+};
+
+template <typename T>
+__global__ void k_grad_chain(GradArgs a);
+__global__ void k_gemmt_bf16x3(GemmTArgs a);
+__global__ void k_gemmt_f64(GemmTArgs a);
+__global__ void k_grad_out(GradOutArgs a);
+__global__ void k_pack_t(int nspec, int nsp, int kfull, int nrows, const double* basis, double* A64, uint16_t* Ah,
+                         uint16_t* Al);
 
 __global__ void k_prep(PrepArgs a);
 __global__ void k_group(GroupArgs a);

@@ -13,7 +13,7 @@ module opensps_hip_iface
         integer(c_int) :: n = 0, precision = 0, split_parts = 0, use_ops = 0
         type(c_ptr) :: prm = c_null_ptr, ssp_weights = c_null_ptr, mags = c_null_ptr, spec = c_null_ptr
         type(c_ptr) :: mass = c_null_ptr, lbol = c_null_ptr, fband = c_null_ptr, young = c_null_ptr, old = c_null_ptr
-        type(c_ptr) :: idx = c_null_ptr, weights = c_null_ptr, op_count = c_null_ptr
+        type(c_ptr) :: idx = c_null_ptr, weights = c_null_ptr, op_count = c_null_ptr, ssp_weights_z = c_null_ptr
     end type hip_batch_t
 
     public :: opensps_hip_create, opensps_hip_destroy, opensps_hip_set_grid, opensps_hip_set_bands, &

@@ -24,7 +24,17 @@ class BatchArgs(C.Structure):
                 ("prm", C.c_void_p),
                 ("ssp_weights", C.c_void_p), ("mags", C.c_void_p), ("spec", C.c_void_p), ("mass", C.c_void_p),
                 ("lbol", C.c_void_p), ("fband", C.c_void_p), ("young", C.c_void_p), ("old", C.c_void_p),
-                ("idx", C.c_void_p), ("weights", C.c_void_p), ("op_count", C.c_void_p)]
+                ("idx", C.c_void_p), ("weights", C.c_void_p), ("op_count", C.c_void_p),
+                ("ssp_weights_z", C.c_void_p)]
+
+
+class GradArgs(C.Structure):
+    _fields_ = [("fwd", BatchArgs), ("mags_bar", C.c_void_p), ("spec_bar", C.c_void_p), ("mass_bar", C.c_void_p),
+                ("lbol_bar", C.c_void_p), ("params_bar", C.c_void_p), ("weights_bar", C.c_void_p),
+                ("weights_z_bar", C.c_void_p)]
+
+
+NBGRAD = 44
 
 
 class SspOpsArgs(C.Structure):
@@ -65,6 +75,7 @@ def lib():
         L.opensps_hip_set_ssp_ops.argtypes = [vp, C.POINTER(SspOpsArgs)]
         L.opensps_hip_free_ssp_ops.argtypes = [vp]
         L.opensps_hip_set_imf.argtypes = [vp, i, i, d, d, d, d]
+        L.opensps_hip_csp_batch_grad.argtypes = [vp, C.POINTER(GradArgs)]
         _lib = L
     return _lib
 
@@ -176,8 +187,7 @@ class DeviceContext:
                 out[k] = torch.empty(shapes[k][0], dtype=shapes[k][1], device=dev)
         return out
 
-    def launch(self, prm, out, precision=MIXED, split_parts=0, ssp_weights=None, use_ops=0):
-        """Enqueues one batch on device-resident prm (NPAR, n) float64 and the buffers of out."""
+    def _batch_args(self, prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z=None):
         n = prm.shape[1]
         assert prm.shape[0] == NPAR and prm.is_contiguous() and prm.dtype.itemsize == 8
         g = self.grid
@@ -189,27 +199,76 @@ class DeviceContext:
             t = out.get(k)
             if t is not None and (not t.is_contiguous() or t.element_size() != itemsize or t.numel() < numel):
                 raise OpenSPSHipError(3, f"{k}: {numel} contiguous {itemsize}-byte elements required")
-        if ssp_weights is not None and (ssp_weights.element_size() != 8 or ssp_weights.numel() < n * g.nt):
+        if ssp_weights is not None and (ssp_weights.element_size() != 8 or ssp_weights.numel() < n * g.nt
+                                        or not ssp_weights.is_contiguous()):
             raise OpenSPSHipError(3, "ssp_weights: nt x n float64 required")
+        if ssp_weights_z is not None and (ssp_weights_z.element_size() != 8 or not ssp_weights_z.is_contiguous()
+                                          or ssp_weights_z.numel() < n * g.nt * g.nz):
+            raise OpenSPSHipError(3, "ssp_weights_z: nt x nz x n float64 required")
         a = BatchArgs()
         a.n, a.precision, a.split_parts, a.use_ops = n, precision, int(split_parts), int(use_ops)
         a.prm = prm.data_ptr()
         a.ssp_weights = ssp_weights.data_ptr() if ssp_weights is not None else None
+        a.ssp_weights_z = ssp_weights_z.data_ptr() if ssp_weights_z is not None else None
         for k in ("mags", "spec", "mass", "lbol", "fband", "young", "old", "idx", "weights", "op_count"):
             setattr(a, k, out[k].data_ptr() if k in out else None)
+        return a
+
+    def launch_grad(self, prm, out, cot, grads, precision=MIXED, split_parts=0, ssp_weights=None, ssp_weights_z=None):
+        """Enqueues a batch and its reverse pass. cot: device cotangents {"mags", "spec", "mass", "lbol"} (any
+        absent); grads: device outputs {"params_bar" (n, NBGRAD), "weights_bar" (n, nt), "weights_z_bar"
+        (n, nz, nt)} (any absent), float64."""
+        n = prm.shape[1]
+        g = self.grid
+        want = {"mags": n * self.nbands, "spec": n * g.nspec, "mass": n, "lbol": n}
+        for k, numel in want.items():
+            t = cot.get(k)
+            if t is not None and (t.element_size() != 8 or not t.is_contiguous() or t.numel() < numel):
+                raise OpenSPSHipError(3, f"{k}_bar: {numel} contiguous float64 required")
+        gsz = {"params_bar": n * NBGRAD, "weights_bar": n * g.nt, "weights_z_bar": n * g.nt * g.nz}
+        for k, numel in gsz.items():
+            t = grads.get(k)
+            if t is not None and (t.element_size() != 8 or not t.is_contiguous() or t.numel() < numel):
+                raise OpenSPSHipError(3, f"{k}: {numel} contiguous float64 required")
+        a = GradArgs()
+        a.fwd = self._batch_args(prm, out, precision, split_parts, ssp_weights, 0, ssp_weights_z)
+        for k in ("mags", "spec", "mass", "lbol"):
+            setattr(a, k + "_bar", cot[k].data_ptr() if cot.get(k) is not None else None)
+        for k in ("params_bar", "weights_bar", "weights_z_bar"):
+            setattr(a, k, grads[k].data_ptr() if grads.get(k) is not None else None)
+        self._check(lib().opensps_hip_csp_batch_grad(self._ctx, C.byref(a)))
+
+    def launch(self, prm, out, precision=MIXED, split_parts=0, ssp_weights=None, use_ops=0, ssp_weights_z=None):
+        """Enqueues one batch on device-resident prm (NPAR, n) float64 and the buffers of out."""
+        a = self._batch_args(prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z)
         self._check(lib().opensps_hip_csp_batch(self._ctx, C.byref(a)))
 
     def sync(self):
         self._check(lib().opensps_hip_sync(self._ctx))
 
     def batch(self, prm, precision=MIXED, split_parts=0, ssp_weights=None, want=("mags", "spec", "mass", "lbol"),
-              use_ops=0):
+              use_ops=0, ssp_weights_z=None):
         import torch
         n = prm.shape[1]
         out = self.alloc_outputs(n, precision, want)
         torch.cuda.synchronize()
-        self.launch(prm, out, precision, split_parts, ssp_weights, use_ops)
+        self.launch(prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z)
         self.sync()
+        return out
+
+    def batch_grad(self, prm, cot, precision=MIXED, split_parts=0, ssp_weights=None, ssp_weights_z=None,
+                   want=("mags", "spec", "mass", "lbol"), grads=("params_bar", "weights_bar", "weights_z_bar")):
+        """The forward outputs and the reverse pass's outputs (device tensors)."""
+        import torch
+        n = prm.shape[1]
+        g = self.grid
+        out = self.alloc_outputs(n, precision, want)
+        shapes = {"params_bar": (n, NBGRAD), "weights_bar": (n, g.nt), "weights_z_bar": (n, g.nz, g.nt)}
+        gr = {k: torch.full(shapes[k], float("nan"), dtype=torch.float64, device="cuda") for k in grads}
+        torch.cuda.synchronize()
+        self.launch_grad(prm, out, cot, gr, precision, split_parts, ssp_weights, ssp_weights_z)
+        self.sync()
+        out.update(gr)
         return out
 
 

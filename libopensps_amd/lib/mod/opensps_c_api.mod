@@ -1,9 +1,9 @@
-﻿!mod$ v1 sum:48fa3b6c809f4a58
+﻿!mod$ v1 sum:c9bbbe67e345fe08
 !need$ b579c837613f74aa n opensps_host_filters
-!need$ 05e2a849ea85cd1b n opensps_host_batch
+!need$ 453d3a3323c64737 n opensps_host_batch
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ f5118467a58a718d n opensps_host_types
-!need$ 8abfb921d159308f n opensps_host_context
+!need$ 5788f3e20047167b n opensps_host_context
 !need$ bb381bf46e508468 i __fortran_builtins
 module opensps_c_api
 use,intrinsic::iso_c_binding,only:c_associated

@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:5106ef5fd61c8351
+﻿!mod$ v1 sum:3ae39e8c3541c9ad
 !need$ 0bde2ac47243ead2 i iso_c_binding
 module opensps_hip_iface
 use,intrinsic::__fortran_builtins,only:__builtin_c_ptr
@@ -34,6 +34,7 @@ type(c_ptr)::old=__builtin_c_ptr(__address=0_8)
 type(c_ptr)::idx=__builtin_c_ptr(__address=0_8)
 type(c_ptr)::weights=__builtin_c_ptr(__address=0_8)
 type(c_ptr)::op_count=__builtin_c_ptr(__address=0_8)
+type(c_ptr)::ssp_weights_z=__builtin_c_ptr(__address=0_8)
 end type
 interface
 function opensps_hip_create(device,ctx) bind(c,name="opensps_hip_create")

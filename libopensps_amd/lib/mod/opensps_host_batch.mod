@@ -1,6 +1,6 @@
-﻿!mod$ v1 sum:05e2a849ea85cd1b
-!need$ 8abfb921d159308f n opensps_host_context
-!need$ 5106ef5fd61c8351 n opensps_hip_iface
+﻿!mod$ v1 sum:453d3a3323c64737
+!need$ 5788f3e20047167b n opensps_host_context
+!need$ 3ae39e8c3541c9ad n opensps_hip_iface
 !need$ f5118467a58a718d n opensps_host_types
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins

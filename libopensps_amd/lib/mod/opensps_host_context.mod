@@ -1,5 +1,5 @@
-﻿!mod$ v1 sum:8abfb921d159308f
-!need$ 5106ef5fd61c8351 n opensps_hip_iface
+﻿!mod$ v1 sum:5788f3e20047167b
+!need$ 3ae39e8c3541c9ad n opensps_hip_iface
 !need$ 4fdd01fda3834dfe n opensps_host_cosmology
 !need$ bb381bf46e508468 i __fortran_builtins
 !need$ 0bde2ac47243ead2 i iso_c_binding

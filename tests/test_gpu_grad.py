@@ -1,0 +1,153 @@
+"""The reverse pass on the GPU (opensps_hip_csp_batch_grad: k_grad_chain, the transposed product, k_grad_out)
+against the oracle's reverse pass (orc_csp_batch_grad, itself pinned by central differences of the oracle's
+forward): params_bar rows, weights_z_bar (weights over ages and metallicities) and weights_bar with the metallicity
+blend, the mass and L_bol cotangents, with and without the birth-cloud and old-star screens. FP64 path: relative
+to each output's largest magnitude per model; mixed path (FP32 chain, split-BF16 transposed product): looser."""
+import numpy as np
+import pytest
+
+from helpers import cosmo, mag_err
+from libopensps_amd import synth
+from libopensps_amd.params import SLOT
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NSPEC, NT, NZ, NB = 5994, 40, 5, 30
+# FP64: summation order and the six-term IGM against the oracle; mixed: FP32 spectra and transmissions in the chain,
+# bf16 hi/lo rows of y_bar in the transposed product
+TOL = {0: {"params": 1.0e-10, "weights": 1.0e-11, "mags": 1.0e-10},
+       1: {"params": 2.0e-4, "weights": 1.0e-4, "mags": 1.0e-4}}
+
+
+def _dev(a):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+
+
+def _setup():
+    g = synth.ssp_grid(seed=1, nspec=NSPEC, nt=NT, nz=NZ)
+    fs = synth.gaussian_filters(NB, seed=7)
+    return g, O.build_bands(fs, g.lam)
+
+
+def _ctx(g, b):
+    from libopensps_amd import hip as H
+    ctx = H.DeviceContext(0)
+    ctx.set_grid(g)
+    ctx.set_bands(b)
+    cz, cdl = cosmo()
+    ctx.set_cosmology(cz, cdl)
+    ctx.set_options(1, 1)
+    return ctx
+
+
+def _cot(n, seed=5):
+    rng = np.random.default_rng(seed)
+    return {"mags": rng.standard_normal((n, NB)), "spec": rng.standard_normal((n, NSPEC)) * 1e2,
+            "mass": rng.standard_normal(n), "lbol": rng.standard_normal(n)}
+
+
+def _rel(got, ref):
+    """max |got - ref| / max |ref| per model (rows), over models."""
+    got = np.asarray(got).reshape(got.shape[0], -1)
+    ref = np.asarray(ref).reshape(ref.shape[0], -1)
+    scale = np.max(np.abs(ref), axis=1, keepdims=True) + 1e-300
+    return float(np.max(np.abs(got - ref) / scale))
+
+
+ROWS = ["dust1", "dust2", "dust3", "dust_index", "dust1_index", "frac_obrun", "frac_nodust", "logzsol", "zred",
+        "sigma_smooth", "igm_factor"]
+
+
+def _compare(ref, got, precision, rows):
+    tol = TOL[precision]
+    em = mag_err(got["mags"], ref["mags"])
+    print(f"  mags {em:.2e}")
+    assert em < tol["mags"]
+    pr, pg = ref["params_bar"], got["params_bar"]
+    for name in rows:
+        r = O.GRAD_ROWS[name]
+        scale = np.max(np.abs(pr[:, r])) + 1e-300
+        e = float(np.max(np.abs(pg[:, r] - pr[:, r]))) / scale
+        print(f"  {name:12s} {e:.2e}")
+        assert e < tol["params"], name
+    other = [r for r in range(O.NBGRAD) if r not in [O.GRAD_ROWS[k] for k in rows]]
+    assert np.all(pg[:, other] == 0.0)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("birth", [False, True])
+def test_grad_weights_z(precision, birth):
+    g, b = _setup()
+    n = 24
+    prm = synth.adjoint_params(n, seed=61, birth_cloud=birth)
+    prm[SLOT["zred"], :3] = [0.0, 0.005, 6.5]   # the rest frame, a small redshift, beyond the IGM table's usual range
+    wz = synth.lognormal_weights_z(n, g.log_age, g.logz, seed=67)
+    cot = _cot(n)
+    ref = O.csp_batch_grad(g, b, prm, ssp_weights_z=wz, mags_bar=cot["mags"], spec_bar=cot["spec"],
+                           mass_bar=cot["mass"], lbol_bar=cot["lbol"], cosmo=cosmo())
+    assert ref["status"] == 0
+    ctx = _ctx(g, b)
+    print(f"case: grad weights_z n={n} birth={birth} precision={precision}", flush=True)
+    out = ctx.batch_grad(_dev(prm), {k: _dev(v) for k, v in cot.items()}, precision=precision,
+                         split_parts=1 if birth else 0, ssp_weights_z=_dev(wz),
+                         grads=("params_bar", "weights_bar", "weights_z_bar"))
+    got = {k: v.detach().cpu().numpy() for k, v in out.items()}
+    ctx.close()
+    rows = ["dust2", "dust_index", "zred", "sigma_smooth", "igm_factor", "dust1"]
+    if birth:
+        rows += ["dust3", "dust1_index", "frac_obrun", "frac_nodust"]
+    else:
+        rows += ["dust3", "frac_obrun", "frac_nodust"]
+    _compare(ref, got, precision, rows)
+    ew = _rel(got["weights_z_bar"], ref["weights_z_bar"])
+    print(f"  weights_z_bar {ew:.2e}")
+    assert ew < TOL[precision]["weights"]
+    assert np.all(got["weights_bar"] == 0.0)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_grad_blend_weights(precision):
+    g, b = _setup()
+    n = 20
+    prm = synth.adjoint_params(n, seed=71, birth_cloud=True)
+    prm[SLOT["logzsol"]] = np.linspace(-2.0, 0.3, n)
+    w = synth.dirichlet_weights(n, NT, seed=73, alpha=0.7)
+    cot = _cot(n, seed=11)
+    ref = O.csp_batch_grad(g, b, prm, ssp_weights=w, mags_bar=cot["mags"], spec_bar=cot["spec"],
+                           mass_bar=cot["mass"], lbol_bar=cot["lbol"], cosmo=cosmo())
+    assert ref["status"] == 0
+    ctx = _ctx(g, b)
+    print(f"case: grad blend n={n} precision={precision}", flush=True)
+    out = ctx.batch_grad(_dev(prm), {k: _dev(v) for k, v in cot.items()}, precision=precision, split_parts=1,
+                         ssp_weights=_dev(w), grads=("params_bar", "weights_bar"))
+    got = {k: v.detach().cpu().numpy() for k, v in out.items()}
+    ctx.close()
+    _compare(ref, got, precision, ROWS)
+    ew = _rel(got["weights_bar"], ref["weights_bar"])
+    print(f"  weights_bar {ew:.2e}")
+    assert ew < TOL[precision]["weights"]
+
+
+def test_grad_statuses():
+    """params_bar with SFH parameters (the SFH tangents are not built) and the operator's reverse pass: status 2."""
+    from libopensps_amd import hip as H
+    import torch
+    g, b = _setup()
+    n = 4
+    prm = synth.parametric_params(n, seed=5)
+    ctx = _ctx(g, b)
+    print("case: grad statuses", flush=True)
+    out = ctx.alloc_outputs(n, 0)
+    grads = {"params_bar": torch.zeros((n, H.NBGRAD), dtype=torch.float64, device="cuda")}
+    with pytest.raises(H.OpenSPSHipError) as e:
+        ctx.launch_grad(_dev(prm), out, {}, grads, precision=0)
+    assert e.value.status == 2
+    # weights_bar alone is computed with SFH parameters
+    wb = {"weights_bar": torch.zeros((n, NT), dtype=torch.float64, device="cuda")}
+    ctx.launch_grad(_dev(prm), out, {"mags": _dev(np.ones((n, NB)))}, wb, precision=0)
+    ctx.sync()
+    assert torch.isfinite(wb["weights_bar"]).all()
+    ctx.close()
