@@ -104,6 +104,31 @@ void opensps_context_compute_csp_batch_parts(int handle, int n, int n_params, co
                                              double *emlines, double *indx, double *young, double *old,
                                              double *mass_young, double *mass_old, double *lbol_young,
                                              double *lbol_old, int *status);
+/* The batch with gradients (include/opensps.h:324-364 of the reference): the forward outputs and the derivatives
+ * of the loss whose cotangents are given (mags_bar nbands x n, spec_bar nspec x n, mass_bar, lbol_bar n; any
+ * NULL) with respect to params_bar (NBGRAD x n), weights_bar (ntfull x n) and weights_z_bar (ntfull x nz x n).
+ * ssp_weights_z (ntfull x nz x n) replaces ssp_weights and the metallicity blend. This build computes the rows of
+ * dust1, dust2, dust3, dust_index, dust1_index, frac_obrun, frac_nodust, logzsol, zred, sigma_smooth and
+ * igm_factor (the others are 0); params_bar with SFH parameters (no weights given), zred_mags and the SSP
+ * operator's rows are status 2. _rows: params_bar's leading dimension (fewer than NBGRAD: status 3, nothing
+ * written; rows beyond NBGRAD are left alone). */
+void opensps_context_compute_csp_batch_grad(int handle, int n, int n_params, const char *const *names,
+                                            const double *const *values, const double *ssp_weights,
+                                            const double *ssp_weights_z, const double *zred_mags,
+                                            const double *mags_bar, const double *spec_bar, const double *mass_bar,
+                                            const double *lbol_bar, double *mags, double *spec, double *mass,
+                                            double *lbol, double *mdust, double *sfr, double *emlines, double *indx,
+                                            double *params_bar, double *weights_bar, double *weights_z_bar,
+                                            int *status);
+void opensps_context_compute_csp_batch_grad_rows(int handle, int n, int n_params, const char *const *names,
+                                                 const double *const *values, const double *ssp_weights,
+                                                 const double *ssp_weights_z, const double *zred_mags,
+                                                 const double *mags_bar, const double *spec_bar,
+                                                 const double *mass_bar, const double *lbol_bar, double *mags,
+                                                 double *spec, double *mass, double *lbol, double *mdust,
+                                                 double *sfr, double *emlines, double *indx, double *params_bar,
+                                                 int params_bar_rows, double *weights_bar, double *weights_z_bar,
+                                                 int *status);
 void opensps_context_get_batch_maggies(int handle, double *maggies, int *n_bands, int *n, int *status);
 /* *bytes: 4 with kernel_precision "mixed", 8 with "double". */
 void opensps_context_batch_spec_device(int handle, void **ptr, int *nspec, int *n, int *bytes, int *status);

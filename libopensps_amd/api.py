@@ -14,6 +14,8 @@ import numpy as np
 from . import build as _build
 from .params import NPAR, PARAM_NAMES
 
+NBGRAD = 44   # params_bar rows (opensps_get_nbgrad)
+
 _lib = None
 
 STATUS_MESSAGES = {0: "ok", 1: "bad handle or not set up", 2: "unsupported", 3: "unknown parameter or bad shape",
@@ -56,6 +58,8 @@ def lib():
         L.opensps_context_compute_csp_batch.argtypes = [i, i, i, vp, vp, vp] + [vp] * 8 + [ip]
         L.opensps_context_compute_csp_batch_parts.argtypes = [i, i, i, vp, vp, vp] + [vp] * 14 + [ip]
         L.opensps_context_compute_csp_batch_device.argtypes = [i, i, vp, vp, i, i, vp, vp, vp, vp, ip]
+        L.opensps_context_compute_csp_batch_grad.argtypes = [i, i, i, vp, vp] + [vp] * 18 + [ip]
+        L.opensps_context_compute_csp_batch_grad_rows.argtypes = [i, i, i, vp, vp] + [vp] * 16 + [i, vp, vp, ip]
         L.opensps_context_batch_spec_device.argtypes = [i, C.POINTER(vp), ip, ip, ip, ip]
         L.opensps_context_get_batch_maggies.argtypes = [i, vp, ip, ip, ip]
         L.opensps_get_last_error.argtypes = [ip, C.c_char_p, i]
@@ -238,6 +242,48 @@ class Context:
                 self.handle, n, len(names), C.cast(cnames, C.c_void_p), C.cast(cvals, C.c_void_p),
                 None if w is None else w.ctypes.data, P("mags"), P("spec"), P("mass"), P("lbol"), P("mdust"),
                 P("sfr"), P("emlines"), None, C.byref(st))
+        if raw_status:
+            out["status"] = st.value
+            return out
+        _check(st)
+        return out
+
+    def compute_csp_batch_grad(self, values: dict, n=None, ssp_weights=None, ssp_weights_z=None, mags_bar=None,
+                               spec_bar=None, mass_bar=None, lbol_bar=None, want=("mags",),
+                               grads=("params_bar", "weights_bar", "weights_z_bar"), params_bar_rows=None,
+                               raw_status=False):
+        """The batch with gradients (opensps_context_compute_csp_batch_grad / _rows). values: {name: array of n};
+        ssp_weights (n, ntfull); ssp_weights_z (n, nz, ntfull); cotangents mags_bar (n, nbands), spec_bar
+        (n, nspec), mass_bar, lbol_bar (n). Returns the forward outputs and params_bar (n, rows), weights_bar
+        (n, ntfull), weights_z_bar (n, nz, ntfull)."""
+        names = list(values.keys())
+        if n is None:
+            n = len(next(iter(values.values())))
+        keep = [_d(values[k]) for k in names]
+        cnames = (C.c_char_p * max(len(names), 1))(*[k.encode() for k in names])
+        cvals = (C.c_void_p * max(len(names), 1))(*[a.ctypes.data for a in keep])
+        rows = NBGRAD if params_bar_rows is None else int(params_bar_rows)
+        out = {"mags": np.zeros((n, self.nbands))}
+        shapes = {"spec": (n, self.nspec), "mass": (n,), "lbol": (n,), "mdust": (n,), "sfr": (n,),
+                  "params_bar": (n, max(rows, 1)), "weights_bar": (n, self.nt),
+                  "weights_z_bar": (n, self.nz, self.nt)}
+        for k in tuple(want) + tuple(grads):
+            if k in shapes:
+                out[k] = np.zeros(shapes[k])
+        opt = lambda a: None if a is None else _d(a)
+        ins = {k: opt(v) for k, v in (("w", ssp_weights), ("wz", ssp_weights_z), ("mb", mags_bar), ("sb", spec_bar),
+                                      ("mab", mass_bar), ("lb", lbol_bar))}
+        I = lambda k: None if ins[k] is None else ins[k].ctypes.data
+        P = lambda k: out[k].ctypes.data if k in out else None
+        st = C.c_int()
+        args = [self.handle, n, len(names), C.cast(cnames, C.c_void_p), C.cast(cvals, C.c_void_p), I("w"), I("wz"), None,
+                I("mb"), I("sb"), I("mab"), I("lb"), P("mags"), P("spec"), P("mass"), P("lbol"), P("mdust"), P("sfr"),
+                None, None, P("params_bar")]
+        if params_bar_rows is None:
+            lib().opensps_context_compute_csp_batch_grad(*args, P("weights_bar"), P("weights_z_bar"), C.byref(st))
+        else:
+            lib().opensps_context_compute_csp_batch_grad_rows(*args, rows, P("weights_bar"), P("weights_z_bar"),
+                                                              C.byref(st))
         if raw_status:
             out["status"] = st.value
             return out

@@ -51,7 +51,7 @@ def build_hip() -> str:
 
 FORTRAN_ORDER = ["opensps_hip_iface.f90", "opensps_host_types.f90", "opensps_host_filters.f90",
                  "opensps_host_cosmology.f90", "opensps_host_context.f90", "opensps_host_batch.f90",
-                 "opensps_c_api.f90"]
+                 "opensps_host_grad.f90", "opensps_c_api.f90"]
 
 
 def build_fortran() -> str:

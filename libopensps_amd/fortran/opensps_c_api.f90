@@ -509,6 +509,46 @@ contains
         status = st
     end subroutine
 
+    subroutine opensps_context_compute_csp_batch_grad(handle, n, n_params, c_names, c_values, c_weights, &
+                                                      c_weights_z, c_zred_mags, c_mags_bar, c_spec_bar, c_mass_bar, &
+                                                      c_lbol_bar, c_mags, c_spec, c_mass, c_lbol, c_mdust, c_sfr, &
+                                                      c_emlines, c_indx, c_params_bar, c_weights_bar, &
+                                                      c_weights_z_bar, status) &
+        bind(C, name="opensps_context_compute_csp_batch_grad")
+        use opensps_host_grad, only: batch_run_grad
+        integer(c_int), value :: handle, n, n_params
+        type(c_ptr), value :: c_names, c_values, c_weights, c_weights_z, c_zred_mags, c_mags_bar, c_spec_bar, &
+                              c_mass_bar, c_lbol_bar, c_mags, c_spec, c_mass, c_lbol, c_mdust, c_sfr, c_emlines, &
+                              c_indx, c_params_bar, c_weights_bar, c_weights_z_bar
+        integer(c_int), intent(out) :: status
+        integer :: st
+        call batch_run_grad(int(handle), int(n), int(n_params), c_names, c_values, c_weights, c_weights_z, &
+                            c_zred_mags, c_mags_bar, c_spec_bar, c_mass_bar, c_lbol_bar, c_mags, c_spec, c_mass, &
+                            c_lbol, c_mdust, c_sfr, c_emlines, c_indx, c_params_bar, NBGRAD, c_weights_bar, &
+                            c_weights_z_bar, st)
+        status = st
+    end subroutine
+
+    subroutine opensps_context_compute_csp_batch_grad_rows(handle, n, n_params, c_names, c_values, c_weights, &
+                                                           c_weights_z, c_zred_mags, c_mags_bar, c_spec_bar, &
+                                                           c_mass_bar, c_lbol_bar, c_mags, c_spec, c_mass, c_lbol, &
+                                                           c_mdust, c_sfr, c_emlines, c_indx, c_params_bar, &
+                                                           params_bar_rows, c_weights_bar, c_weights_z_bar, status) &
+        bind(C, name="opensps_context_compute_csp_batch_grad_rows")
+        use opensps_host_grad, only: batch_run_grad
+        integer(c_int), value :: handle, n, n_params, params_bar_rows
+        type(c_ptr), value :: c_names, c_values, c_weights, c_weights_z, c_zred_mags, c_mags_bar, c_spec_bar, &
+                              c_mass_bar, c_lbol_bar, c_mags, c_spec, c_mass, c_lbol, c_mdust, c_sfr, c_emlines, &
+                              c_indx, c_params_bar, c_weights_bar, c_weights_z_bar
+        integer(c_int), intent(out) :: status
+        integer :: st
+        call batch_run_grad(int(handle), int(n), int(n_params), c_names, c_values, c_weights, c_weights_z, &
+                            c_zred_mags, c_mags_bar, c_spec_bar, c_mass_bar, c_lbol_bar, c_mags, c_spec, c_mass, &
+                            c_lbol, c_mdust, c_sfr, c_emlines, c_indx, c_params_bar, int(params_bar_rows), &
+                            c_weights_bar, c_weights_z_bar, st)
+        status = st
+    end subroutine
+
     !> No reference counterpart: the batch on device-resident arrays, for callers that keep their data on the
     !> GPU (the reference's opensps_context_batch_spec_device serves the same callers on the output side,
     !> include/opensps.h:308-316). d_prm: device (OPS_NPAR x n) doubles, the per-model parameters in the slots of

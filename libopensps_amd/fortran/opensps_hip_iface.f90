@@ -16,10 +16,18 @@ module opensps_hip_iface
         type(c_ptr) :: idx = c_null_ptr, weights = c_null_ptr, op_count = c_null_ptr, ssp_weights_z = c_null_ptr
     end type hip_batch_t
 
+    !> opensps_hip_grad_t
+    type, bind(C), public :: hip_grad_t
+        type(hip_batch_t) :: fwd
+        type(c_ptr) :: mags_bar = c_null_ptr, spec_bar = c_null_ptr, mass_bar = c_null_ptr, lbol_bar = c_null_ptr
+        type(c_ptr) :: params_bar = c_null_ptr, weights_bar = c_null_ptr, weights_z_bar = c_null_ptr
+    end type hip_grad_t
+
     public :: opensps_hip_create, opensps_hip_destroy, opensps_hip_set_grid, opensps_hip_set_bands, &
               opensps_hip_set_mag_mask, opensps_hip_set_cosmology, opensps_hip_set_options, opensps_hip_csp_batch, &
               opensps_hip_sync, opensps_hip_malloc, opensps_hip_free, opensps_hip_memcpy_h2d, opensps_hip_memcpy_d2h, &
-              opensps_hip_last_error, opensps_hip_device_count, opensps_hip_set_profiling, opensps_hip_kernel_times
+              opensps_hip_last_error, opensps_hip_device_count, opensps_hip_set_profiling, opensps_hip_kernel_times, &
+              opensps_hip_csp_batch_grad
 
     interface
         integer(c_int) function opensps_hip_create(device, ctx) bind(C, name="opensps_hip_create")
@@ -72,6 +80,12 @@ module opensps_hip_iface
             import :: c_int, c_ptr, hip_batch_t
             type(c_ptr), value :: ctx
             type(hip_batch_t), intent(in) :: args
+        end function
+
+        integer(c_int) function opensps_hip_csp_batch_grad(ctx, args) bind(C, name="opensps_hip_csp_batch_grad")
+            import :: c_int, c_ptr, hip_grad_t
+            type(c_ptr), value :: ctx
+            type(hip_grad_t), intent(in) :: args
         end function
 
         integer(c_int) function opensps_hip_sync(ctx) bind(C, name="opensps_hip_sync")

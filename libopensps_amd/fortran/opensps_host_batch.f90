@@ -9,7 +9,7 @@ module opensps_host_batch
     use opensps_hip_iface
     implicit none
     private
-    public :: batch_run, batch_run_device, cstring
+    public :: batch_run, batch_run_device, cstring, resolve_params, model_checks
 
 contains
 
@@ -124,6 +124,55 @@ contains
         end if
     end subroutine model_checks
 
+    !> names[] / values[] -> prm(n, NPAR): the context's values, overwritten per model by the named parameters
+    !> (batch_handle's name resolution, src/bindings/c/opensps_c_api.f90:797-853); status 2 for a parameter that
+    !> changes the SSPs, 3 for an unknown name or NULL values.
+    subroutine resolve_params(c, n, n_params, c_names, c_values, prm, status)
+        type(host_ctx_t), intent(in) :: c
+        integer, intent(in) :: n, n_params
+        type(c_ptr), intent(in) :: c_names, c_values
+        real(WP), allocatable, intent(out) :: prm(:, :)
+        integer, intent(out) :: status
+        type(c_ptr), pointer :: np(:), vp(:)
+        real(c_double), pointer :: v(:)
+        character(len=:), allocatable :: nm
+        integer :: i, k
+        status = 0
+        allocate(prm(n, NPAR))
+        do k = 1, NPAR
+            prm(:, k) = c%pset(k)
+        end do
+        if (n_params <= 0) return
+        if (.not. c_associated(c_names) .or. .not. c_associated(c_values)) then
+            status = 3
+            call set_error(3, 'batch: names / values are NULL')
+            return
+        end if
+        call c_f_pointer(c_names, np, [n_params])
+        call c_f_pointer(c_values, vp, [n_params])
+        do i = 1, n_params
+            nm = lower(cstring(np(i)))
+            if (any(SSP_NAMES == nm)) then
+                status = 2
+                call set_error(2, 'batch: ' // nm // ' changes the SSPs and cannot vary in a batch')
+                return
+            end if
+            k = param_slot(nm)
+            if (k == 0) then
+                status = 3
+                call set_error(3, 'batch: unknown parameter ' // nm)
+                return
+            end if
+            if (.not. c_associated(vp(i))) then
+                status = 3
+                call set_error(3, 'batch: NULL values for ' // nm)
+                return
+            end if
+            call c_f_pointer(vp(i), v, [n])
+            prm(:, k) = v(1:n)
+        end do
+    end subroutine resolve_params
+
     !> compute_csp_batch through the C API (batch_handle without the gradients)
     subroutine batch_run(h, n, n_params, c_names, c_values, c_weights, c_mags, c_spec, c_mass, c_lbol, c_mdust, c_sfr, &
                          c_emlines, c_indx, c_young, c_old, c_mass_young, c_mass_old, c_lbol_young, c_lbol_old, status)
@@ -176,40 +225,8 @@ contains
         ns = c%nspec
         nt = c%nt
         ! names -> slots; the values of the named parameters per model over the context's
-        allocate(prm(n, NPAR))
-        do k = 1, NPAR
-            prm(:, k) = c%pset(k)
-        end do
-        if (n_params > 0) then
-            if (.not. c_associated(c_names) .or. .not. c_associated(c_values)) then
-                status = 3
-                call set_error(3, 'batch: names / values are NULL')
-                return
-            end if
-            call c_f_pointer(c_names, np, [n_params])
-            call c_f_pointer(c_values, vp, [n_params])
-            do i = 1, n_params
-                nm = lower(cstring(np(i)))
-                if (any(SSP_NAMES == nm)) then
-                    status = 2
-                    call set_error(2, 'batch: ' // nm // ' changes the SSPs and cannot vary in a batch')
-                    return
-                end if
-                k = param_slot(nm)
-                if (k == 0) then
-                    status = 3
-                    call set_error(3, 'batch: unknown parameter ' // nm)
-                    return
-                end if
-                if (.not. c_associated(vp(i))) then
-                    status = 3
-                    call set_error(3, 'batch: NULL values for ' // nm)
-                    return
-                end if
-                call c_f_pointer(vp(i), v, [n])
-                prm(:, k) = v(1:n)
-            end do
-        end if
+        call resolve_params(c, n, n_params, c_names, c_values, prm, status)
+        if (status /= 0) return
         call model_checks(c, prm, n, st)
         if (st /= 0) then
             status = st

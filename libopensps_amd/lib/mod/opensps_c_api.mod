@@ -1,9 +1,9 @@
-﻿!mod$ v1 sum:c9bbbe67e345fe08
+﻿!mod$ v1 sum:7b340f3eda27cdf3
 !need$ b579c837613f74aa n opensps_host_filters
-!need$ 453d3a3323c64737 n opensps_host_batch
+!need$ 555e300ca16dfe42 n opensps_host_batch
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ f5118467a58a718d n opensps_host_types
-!need$ 5788f3e20047167b n opensps_host_context
+!need$ cd8a862e941f3165 n opensps_host_context
 !need$ bb381bf46e508468 i __fortran_builtins
 module opensps_c_api
 use,intrinsic::iso_c_binding,only:c_associated
@@ -146,6 +146,8 @@ use opensps_host_filters,only:trapezoid
 use opensps_host_filters,only:filters_build
 use opensps_host_filters,only:band_pivot
 use opensps_host_batch,only:cstring
+use opensps_host_batch,only:model_checks
+use opensps_host_batch,only:resolve_params
 use opensps_host_batch,only:batch_run
 use opensps_host_batch,only:batch_run_device
 use,intrinsic::__fortran_builtins,only:iso_c_binding$__fortran_builtins$c_associated_c_ptr=>c_associated_c_ptr
@@ -289,6 +291,8 @@ private::trapezoid
 private::filters_build
 private::band_pivot
 private::cstring
+private::model_checks
+private::resolve_params
 private::batch_run
 private::batch_run_device
 private::iso_c_binding$__fortran_builtins$c_associated_c_ptr
@@ -334,6 +338,8 @@ private::opensps_context_get_bands
 private::opensps_context_set_mag_compute
 private::opensps_context_compute_csp_batch
 private::opensps_context_compute_csp_batch_parts
+private::opensps_context_compute_csp_batch_grad
+private::opensps_context_compute_csp_batch_grad_rows
 private::opensps_context_compute_csp_batch_device
 private::opensps_context_get_params
 private::opensps_context_kernel_times
@@ -581,6 +587,59 @@ type(c_ptr),value::c_mass_young
 type(c_ptr),value::c_mass_old
 type(c_ptr),value::c_lbol_young
 type(c_ptr),value::c_lbol_old
+integer(4),intent(out)::status
+end
+subroutine opensps_context_compute_csp_batch_grad(handle,n,n_params,c_names,c_values,c_weights,c_weights_z,c_zred_mags,c_mags_bar,c_spec_bar,c_mass_bar,c_lbol_bar,c_mags,c_spec,c_mass,c_lbol,c_mdust,c_sfr,c_emlines,c_indx,c_params_bar,c_weights_bar,c_weights_z_bar,status) bind(c,name="opensps_context_compute_csp_batch_grad")
+integer(4),value::handle
+integer(4),value::n
+integer(4),value::n_params
+type(c_ptr),value::c_names
+type(c_ptr),value::c_values
+type(c_ptr),value::c_weights
+type(c_ptr),value::c_weights_z
+type(c_ptr),value::c_zred_mags
+type(c_ptr),value::c_mags_bar
+type(c_ptr),value::c_spec_bar
+type(c_ptr),value::c_mass_bar
+type(c_ptr),value::c_lbol_bar
+type(c_ptr),value::c_mags
+type(c_ptr),value::c_spec
+type(c_ptr),value::c_mass
+type(c_ptr),value::c_lbol
+type(c_ptr),value::c_mdust
+type(c_ptr),value::c_sfr
+type(c_ptr),value::c_emlines
+type(c_ptr),value::c_indx
+type(c_ptr),value::c_params_bar
+type(c_ptr),value::c_weights_bar
+type(c_ptr),value::c_weights_z_bar
+integer(4),intent(out)::status
+end
+subroutine opensps_context_compute_csp_batch_grad_rows(handle,n,n_params,c_names,c_values,c_weights,c_weights_z,c_zred_mags,c_mags_bar,c_spec_bar,c_mass_bar,c_lbol_bar,c_mags,c_spec,c_mass,c_lbol,c_mdust,c_sfr,c_emlines,c_indx,c_params_bar,params_bar_rows,c_weights_bar,c_weights_z_bar,status) bind(c,name="opensps_context_compute_csp_batch_grad_rows")
+integer(4),value::handle
+integer(4),value::n
+integer(4),value::n_params
+type(c_ptr),value::c_names
+type(c_ptr),value::c_values
+type(c_ptr),value::c_weights
+type(c_ptr),value::c_weights_z
+type(c_ptr),value::c_zred_mags
+type(c_ptr),value::c_mags_bar
+type(c_ptr),value::c_spec_bar
+type(c_ptr),value::c_mass_bar
+type(c_ptr),value::c_lbol_bar
+type(c_ptr),value::c_mags
+type(c_ptr),value::c_spec
+type(c_ptr),value::c_mass
+type(c_ptr),value::c_lbol
+type(c_ptr),value::c_mdust
+type(c_ptr),value::c_sfr
+type(c_ptr),value::c_emlines
+type(c_ptr),value::c_indx
+type(c_ptr),value::c_params_bar
+integer(4),value::params_bar_rows
+type(c_ptr),value::c_weights_bar
+type(c_ptr),value::c_weights_z_bar
 integer(4),intent(out)::status
 end
 subroutine opensps_context_compute_csp_batch_device(handle,n,d_prm,d_weights,precision,split,d_mags,d_spec,d_mass,d_lbol,status) bind(c,name="opensps_context_compute_csp_batch_device")

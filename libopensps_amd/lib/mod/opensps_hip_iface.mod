@@ -1,4 +1,4 @@
-﻿!mod$ v1 sum:3ae39e8c3541c9ad
+﻿!mod$ v1 sum:9ef4185c7839806c
 !need$ 0bde2ac47243ead2 i iso_c_binding
 module opensps_hip_iface
 use,intrinsic::__fortran_builtins,only:__builtin_c_ptr
@@ -35,6 +35,16 @@ type(c_ptr)::idx=__builtin_c_ptr(__address=0_8)
 type(c_ptr)::weights=__builtin_c_ptr(__address=0_8)
 type(c_ptr)::op_count=__builtin_c_ptr(__address=0_8)
 type(c_ptr)::ssp_weights_z=__builtin_c_ptr(__address=0_8)
+end type
+type,bind(c)::hip_grad_t
+type(hip_batch_t)::fwd
+type(c_ptr)::mags_bar=__builtin_c_ptr(__address=0_8)
+type(c_ptr)::spec_bar=__builtin_c_ptr(__address=0_8)
+type(c_ptr)::mass_bar=__builtin_c_ptr(__address=0_8)
+type(c_ptr)::lbol_bar=__builtin_c_ptr(__address=0_8)
+type(c_ptr)::params_bar=__builtin_c_ptr(__address=0_8)
+type(c_ptr)::weights_bar=__builtin_c_ptr(__address=0_8)
+type(c_ptr)::weights_z_bar=__builtin_c_ptr(__address=0_8)
 end type
 interface
 function opensps_hip_create(device,ctx) bind(c,name="opensps_hip_create")
@@ -111,6 +121,15 @@ import::hip_batch_t
 type(c_ptr),value::ctx
 type(hip_batch_t),intent(in)::args
 integer(4)::opensps_hip_csp_batch
+end
+end interface
+interface
+function opensps_hip_csp_batch_grad(ctx,args) bind(c,name="opensps_hip_csp_batch_grad")
+import::c_ptr
+import::hip_grad_t
+type(c_ptr),value::ctx
+type(hip_grad_t),intent(in)::args
+integer(4)::opensps_hip_csp_batch_grad
 end
 end interface
 interface

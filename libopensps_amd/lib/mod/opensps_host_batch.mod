@@ -1,6 +1,6 @@
-﻿!mod$ v1 sum:453d3a3323c64737
-!need$ 5788f3e20047167b n opensps_host_context
-!need$ 3ae39e8c3541c9ad n opensps_hip_iface
+﻿!mod$ v1 sum:555e300ca16dfe42
+!need$ cd8a862e941f3165 n opensps_host_context
+!need$ 9ef4185c7839806c n opensps_hip_iface
 !need$ f5118467a58a718d n opensps_host_types
 !need$ 0bde2ac47243ead2 i iso_c_binding
 !need$ bb381bf46e508468 i __fortran_builtins
@@ -127,6 +127,7 @@ use opensps_host_context,only:ctx_device
 use opensps_hip_iface,only:hip_fp64
 use opensps_hip_iface,only:hip_mixed
 use opensps_hip_iface,only:hip_batch_t
+use opensps_hip_iface,only:hip_grad_t
 use opensps_hip_iface,only:opensps_hip_create
 use opensps_hip_iface,only:opensps_hip_destroy
 use opensps_hip_iface,only:opensps_hip_set_grid
@@ -135,6 +136,7 @@ use opensps_hip_iface,only:opensps_hip_set_mag_mask
 use opensps_hip_iface,only:opensps_hip_set_cosmology
 use opensps_hip_iface,only:opensps_hip_set_options
 use opensps_hip_iface,only:opensps_hip_csp_batch
+use opensps_hip_iface,only:opensps_hip_csp_batch_grad
 use opensps_hip_iface,only:opensps_hip_sync
 use opensps_hip_iface,only:opensps_hip_malloc
 use opensps_hip_iface,only:opensps_hip_free
@@ -267,6 +269,7 @@ private::ctx_device
 private::hip_fp64
 private::hip_mixed
 private::hip_batch_t
+private::hip_grad_t
 private::opensps_hip_create
 private::opensps_hip_destroy
 private::opensps_hip_set_grid
@@ -275,6 +278,7 @@ private::opensps_hip_set_mag_mask
 private::opensps_hip_set_cosmology
 private::opensps_hip_set_options
 private::opensps_hip_csp_batch
+private::opensps_hip_csp_batch_grad
 private::opensps_hip_sync
 private::opensps_hip_malloc
 private::opensps_hip_free
@@ -288,7 +292,6 @@ private::iso_c_binding$__fortran_builtins$c_associated_c_ptr
 private::expm1_ref
 private::gammainc12
 private::sfr_at_age
-private::model_checks
 private::maggies_from_device
 private::ieee_nan
 contains
@@ -315,6 +318,15 @@ type(host_ctx_t),intent(in)::c
 integer(4),intent(in)::n
 real(8),intent(in)::prm(1_8:int(n,kind=8),1_8:28_8)
 integer(4),intent(out)::st
+end
+subroutine resolve_params(c,n,n_params,c_names,c_values,prm,status)
+type(host_ctx_t),intent(in)::c
+integer(4),intent(in)::n
+integer(4),intent(in)::n_params
+type(c_ptr),intent(in)::c_names
+type(c_ptr),intent(in)::c_values
+real(8),allocatable,intent(out)::prm(:,:)
+integer(4),intent(out)::status
 end
 subroutine batch_run(h,n,n_params,c_names,c_values,c_weights,c_mags,c_spec,c_mass,c_lbol,c_mdust,c_sfr,c_emlines,c_indx,c_young,c_old,c_mass_young,c_mass_old,c_lbol_young,c_lbol_old,status)
 integer(4),intent(in)::h

@@ -1,5 +1,5 @@
-﻿!mod$ v1 sum:5788f3e20047167b
-!need$ 3ae39e8c3541c9ad n opensps_hip_iface
+﻿!mod$ v1 sum:cd8a862e941f3165
+!need$ 9ef4185c7839806c n opensps_hip_iface
 !need$ 4fdd01fda3834dfe n opensps_host_cosmology
 !need$ bb381bf46e508468 i __fortran_builtins
 !need$ 0bde2ac47243ead2 i iso_c_binding
@@ -48,6 +48,7 @@ use opensps_host_types,only:set_error
 use opensps_hip_iface,only:hip_fp64
 use opensps_hip_iface,only:hip_mixed
 use opensps_hip_iface,only:hip_batch_t
+use opensps_hip_iface,only:hip_grad_t
 use opensps_hip_iface,only:opensps_hip_create
 use opensps_hip_iface,only:opensps_hip_destroy
 use opensps_hip_iface,only:opensps_hip_set_grid
@@ -56,6 +57,7 @@ use opensps_hip_iface,only:opensps_hip_set_mag_mask
 use opensps_hip_iface,only:opensps_hip_set_cosmology
 use opensps_hip_iface,only:opensps_hip_set_options
 use opensps_hip_iface,only:opensps_hip_csp_batch
+use opensps_hip_iface,only:opensps_hip_csp_batch_grad
 use opensps_hip_iface,only:opensps_hip_sync
 use opensps_hip_iface,only:opensps_hip_malloc
 use opensps_hip_iface,only:opensps_hip_free
@@ -109,6 +111,7 @@ private::set_error
 private::hip_fp64
 private::hip_mixed
 private::hip_batch_t
+private::hip_grad_t
 private::opensps_hip_create
 private::opensps_hip_destroy
 private::opensps_hip_set_grid
@@ -117,6 +120,7 @@ private::opensps_hip_set_mag_mask
 private::opensps_hip_set_cosmology
 private::opensps_hip_set_options
 private::opensps_hip_csp_batch
+private::opensps_hip_csp_batch_grad
 private::opensps_hip_sync
 private::opensps_hip_malloc
 private::opensps_hip_free

@@ -156,3 +156,32 @@ def test_batch_without_device_is_status_5():
         ctx.set_int("add_dust_emission", 0)
         r = ctx.compute_csp_batch({"tage": np.full(2, 5.0), "sfh": np.full(2, 4.0)}, raw_status=True)
         assert r["status"] == 5
+
+
+def test_batch_grad_statuses_before_the_device():
+    """compute_csp_batch_grad's host checks (opensps_c_api.f90:729-735 for the row count): fewer params_bar rows
+    than NBGRAD is status 3 with nothing written; zred_mags and spectral indices are status 2; both weight forms
+    at once is status 3."""
+    g = grid(1, 600, 20)
+    fs = synth.gaussian_filters(4, seed=7)
+    with api.Context() as ctx:
+        ctx.set_ssp_basis(g)
+        ctx.set_filters(fs)
+        n = 3
+        vals = {"dust2": np.full(n, 0.3)}
+        w = np.full((n, g.nt), 1.0 / g.nt)
+        r = ctx.compute_csp_batch_grad(vals, ssp_weights=w, grads=("params_bar",), params_bar_rows=10,
+                                       raw_status=True)
+        assert r["status"] == 3 and np.all(r["params_bar"] == 0.0)
+        r = ctx.compute_csp_batch_grad(vals, ssp_weights=w, ssp_weights_z=np.ones((n, 1, g.nt)), raw_status=True)
+        assert r["status"] == 3
+        zm = np.zeros(n)
+        st = C.c_int()
+        keep = [np.ascontiguousarray(vals["dust2"])]
+        names = (C.c_char_p * 1)(b"dust2")
+        vp = (C.c_void_p * 1)(keep[0].ctypes.data)
+        mags = np.zeros((n, 4))
+        api.lib().opensps_context_compute_csp_batch_grad(
+            ctx.handle, n, 1, C.cast(names, C.c_void_p), C.cast(vp, C.c_void_p), w.ctypes.data, None, zm.ctypes.data,
+            None, None, None, None, mags.ctypes.data, *([None] * 10), C.byref(st))
+        assert st.value == 2

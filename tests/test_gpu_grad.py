@@ -151,3 +151,74 @@ def test_grad_statuses():
     ctx.sync()
     assert torch.isfinite(wb["weights_bar"]).all()
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_public_api_grad(precision):
+    """opensps_context_compute_csp_batch_grad through the Fortran host against the oracle."""
+    from libopensps_amd import api
+    g, _ = _setup()
+    fs = synth.gaussian_filters(NB, seed=7)
+    n = 12
+    prm = synth.adjoint_params(n, seed=83, birth_cloud=True)
+    wz = synth.lognormal_weights_z(n, g.log_age, g.logz, seed=89)
+    cot = _cot(n, seed=17)
+    names = ["dust1", "dust2", "dust3", "dust_index", "dust1_index", "frac_obrun", "frac_nodust", "zred",
+             "sigma_smooth"]
+    print(f"case: public grad n={n} precision={precision}", flush=True)
+    with api.Context() as ctx:
+        ctx.set_ssp_basis(g)
+        ctx.set_filters(fs)
+        ctx.set_int("add_igm_absorption", 1)
+        ctx.set_int("add_dust_emission", 0)
+        ctx.set_str("kernel_precision", precision)
+        r = ctx.compute_csp_batch_grad({k: prm[SLOT[k]] for k in names}, ssp_weights_z=wz, mags_bar=cot["mags"],
+                                       spec_bar=cot["spec"], mass_bar=cot["mass"], lbol_bar=cot["lbol"],
+                                       want=("mags", "spec", "mass", "lbol"), params_bar_rows=50)
+        bands = ctx.bands()
+    ref = O.csp_batch_grad(g, bands, prm, ssp_weights_z=wz, mags_bar=cot["mags"], spec_bar=cot["spec"],
+                           mass_bar=cot["mass"], lbol_bar=cot["lbol"], cosmo=cosmo())
+    assert ref["status"] == 0
+    p = 0 if precision == "double" else 1
+    assert np.all(r["params_bar"][:, O.NBGRAD:] == 0.0)   # rows past NBGRAD left as they were
+    got = {"mags": r["mags"], "params_bar": r["params_bar"][:, :O.NBGRAD]}
+    _compare(ref, got, p, ["dust1", "dust2", "dust3", "dust_index", "dust1_index", "frac_obrun", "frac_nodust",
+                           "zred", "sigma_smooth", "igm_factor"])
+    ew = _rel(r["weights_z_bar"], ref["weights_z_bar"])
+    print(f"  weights_z_bar {ew:.2e}")
+    assert ew < TOL[p]["weights"]
+
+
+def test_autograd_chi2():
+    """The torch.autograd.Function: a chi^2 loss on the magnitudes, loss.backward() -> prm.grad and the weights'
+    grad against the oracle's reverse pass with the same cotangent."""
+    import torch
+    from libopensps_amd import autograd as AG
+    g, b = _setup()
+    n = 16
+    prm = synth.adjoint_params(n, seed=97, birth_cloud=False)
+    wz = synth.lognormal_weights_z(n, g.log_age, g.logz, seed=101)
+    ctx = _ctx(g, b)
+    print(f"case: autograd chi2 n={n}", flush=True)
+    p_t = torch.tensor(prm, device="cuda", requires_grad=True)
+    w_t = torch.tensor(wz, device="cuda", requires_grad=True)
+    mags = AG.CSPMags.apply(ctx, p_t, w_t, 0)
+    m_np = mags.detach().cpu().numpy()
+    _, dmag = synth.chi2_cotangent(m_np)
+    ok = torch.isfinite(mags)
+    sig = 2.5 / np.log(10.0) * 0.05
+    obs = torch.tensor(m_np - 0.5 * sig * sig * dmag, device="cuda")   # so that d loss / d mags = dmag
+    r = torch.where(ok, (mags - obs) / sig, torch.zeros_like(mags))
+    loss = (r * r).sum()
+    loss.backward()
+    ctx.close()
+    ref = O.csp_batch_grad(g, b, prm, ssp_weights_z=wz, mags_bar=dmag, cosmo=cosmo())
+    for name in ("dust2", "dust_index", "zred", "sigma_smooth", "igm_factor"):
+        a = p_t.grad[SLOT[name]].cpu().numpy()
+        e = ref["params_bar"][:, O.GRAD_ROWS[name]]
+        err = float(np.max(np.abs(a - e)) / (np.max(np.abs(e)) + 1e-300))
+        print(f"  {name:12s} {err:.2e}")
+        assert err < 1e-9, name
+    ew = _rel(w_t.grad.cpu().numpy(), ref["weights_z_bar"])
+    print(f"  weights_z {ew:.2e}")
+    assert ew < 1e-10
