@@ -148,6 +148,9 @@ int opensps_hip_set_imf(opensps_hip_ctx_t *ctx, int imf_type, int remnants, doub
  * operator, "group" also holds the IMF weights, the fold and the mass sums, and the product is the template one. */
 int opensps_hip_set_profiling(opensps_hip_ctx_t *ctx, int on);
 int opensps_hip_kernel_times(opensps_hip_ctx_t *ctx, double *ms, int *launches, int reset);
+/* The same for the reverse pass's kernels: [reverse chain, transposed product, weights and params_bar] (ms[3],
+ * launches[3]); the forward half of a batch with gradients counts in opensps_hip_kernel_times. */
+int opensps_hip_grad_kernel_times(opensps_hip_ctx_t *ctx, double *ms, int *launches, int reset);
 
 /* ---- kernel-level entry points (the reference's per-stage launchers) ---- */
 /* band_fluxes (opensps_hip_band_fluxes_sp/_dp, src/accel/hip/opensps_hip_kernels.hip:3566, 3623): device

@@ -153,6 +153,8 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
     free_full(c);
     for (int k = 0; k <= OpsCtx::NPROF; k++)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    for (int k = 0; k <= OpsCtx::NGPROF; k++)
+        if (c->gev[k]) (void)hipEventDestroy(c->gev[k]);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -1144,11 +1146,13 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     const size_t lds = 3 * tsz * (size_t)nsp + sizeof(double) * ((c->nbands > 0 ? c->nbands : 1) + 8 * 16);
     const void* kfn = mixed ? (const void*)k_grad_chain<float> : (const void*)k_grad_chain<double>;
     HIPCHK(c, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (c->profiling) HIPCHK(c, hipEventRecord(c->gev[0], c->stream));
     if (mixed)
         k_grad_chain<float><<<n, 512, lds, c->stream>>>(ga);
     else
         k_grad_chain<double><<<n, 512, lds, c->stream>>>(ga);
     HIPCHK(c, hipGetLastError());
+    if (c->profiling) HIPCHK(c, hipEventRecord(c->gev[1], c->stream));
 
     GemmTArgs gt;
     gt.krows = kfull;
@@ -1170,6 +1174,7 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     else
         k_gemmt_f64<<<gblocks, 256, 0, c->stream>>>(gt);
     HIPCHK(c, hipGetLastError());
+    if (c->profiling) HIPCHK(c, hipEventRecord(c->gev[2], c->stream));
 
     GradOutArgs go;
     go.n = n;
@@ -1195,6 +1200,23 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     go.weights_z_bar = g->weights_z_bar;
     k_grad_out<<<n, 256, 0, c->stream>>>(go);
     HIPCHK(c, hipGetLastError());
+    if (c->profiling) {
+        HIPCHK(c, hipEventRecord(c->gev[3], c->stream));
+        c->gev_pending = true;
+    }
+    return 0;
+}
+
+int opensps_hip_grad_kernel_times(opensps_hip_ctx_t* c, double* ms, int* launches, int reset) {
+    if (!c) return 1;
+    for (int k = 0; k < OpsCtx::NGPROF; k++) {
+        if (ms) ms[k] = c->gprof_ms[k];
+        if (launches) launches[k] = (int)c->gprof_n[k];
+        if (reset) {
+            c->gprof_ms[k] = 0.0;
+            c->gprof_n[k] = 0;
+        }
+    }
     return 0;
 }
 
@@ -1209,6 +1231,15 @@ int opensps_hip_sync(opensps_hip_ctx_t* c) {
             c->prof_n[k] += 1;
         }
         c->ev_pending = false;
+    }
+    if (c->gev_pending) {
+        for (int k = 0; k < OpsCtx::NGPROF; k++) {
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->gev[k], c->gev[k + 1]));
+            c->gprof_ms[k] += ms;
+            c->gprof_n[k] += 1;
+        }
+        c->gev_pending = false;
     }
     if (!c->d_last_flag) return 0;
     int flag = 0;
@@ -1225,8 +1256,11 @@ int opensps_hip_set_profiling(opensps_hip_ctx_t* c, int on) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (on && !c->ev[0])
         for (int k = 0; k <= OpsCtx::NPROF; k++) HIPCHK(c, hipEventCreate(&c->ev[k]));
+    if (on && !c->gev[0])
+        for (int k = 0; k <= OpsCtx::NGPROF; k++) HIPCHK(c, hipEventCreate(&c->gev[k]));
     c->profiling = on != 0;
     c->ev_pending = false;
+    c->gev_pending = false;
     return 0;
 }
 

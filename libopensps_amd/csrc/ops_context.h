@@ -121,6 +121,12 @@ struct OpsCtx {
     bool ev_pending = false;
     double prof_ms[NPROF] = {0, 0, 0, 0};
     long prof_n[NPROF] = {0, 0, 0, 0};
+    // the reverse pass: [reverse chain, transposed product, weights / params_bar]
+    static const int NGPROF = 3;
+    hipEvent_t gev[NGPROF + 1] = {};
+    bool gev_pending = false;
+    double gprof_ms[NGPROF] = {0, 0, 0};
+    long gprof_n[NGPROF] = {0, 0, 0};
 
     std::string last_error;
 };

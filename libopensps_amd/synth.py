@@ -180,13 +180,17 @@ def adjoint_params(n: int, seed: int = 19, birth_cloud: bool = False) -> np.ndar
     return p
 
 
-def chi2_cotangent(mags: np.ndarray, seed: int = 23, frac_err: float = 0.05):
+def chi2_cotangent(mags: np.ndarray, seed: int = 23, frac_err: float = 0.05, detect_range: float = 40.0):
     """The adjoint workload's loss: chi^2 of the magnitudes against synthetic photometry (mags perturbed by 5 %
-    Gaussian flux errors drawn from a held-out seed). Returns (loss, d loss / d mags); NaN bands are left out."""
+    Gaussian flux errors drawn from a held-out seed). Returns (loss, d loss / d mags). NaN bands, and bands more
+    than detect_range magnitudes fainter than the model's brightest (non-detections: below the Lyman limit at high
+    redshift the IGM leaves ~1e-30 of the flux), are left out."""
     rng = np.random.Generator(np.random.PCG64(seed))
     sig = 2.5 / np.log(10.0) * frac_err
     obs = mags + sig * rng.standard_normal(mags.shape) + 0.1
-    ok = np.isfinite(mags)
+    fin = np.isfinite(mags)
+    bright = np.min(np.where(fin, mags, np.inf), axis=1, keepdims=True)
+    ok = fin & (mags - bright <= detect_range)
     r = np.where(ok, (mags - np.where(ok, obs, 0.0)) / sig, 0.0)
     return float(np.sum(r * r)), np.where(ok, 2.0 * r / sig, 0.0)
 
