@@ -30,7 +30,8 @@ def _exported(so):
     return {l.split()[-1] for l in out.splitlines() if " T " in l}
 
 
-@pytest.mark.parametrize("header,so", [("opensps.h", build.API_SO), ("opensps_hip.h", build.HIP_SO)])
+@pytest.mark.parametrize("header,so", [("opensps.h", build.API_SO), ("opensps_hip.h", build.HIP_SO),
+                                       ("opensps_emu.h", build.HIP_SO)])
 def test_every_declared_symbol_is_exported(header, so):
     assert os.path.exists(so), f"{so} not built"
     C.CDLL(so)   # loads without a GPU
