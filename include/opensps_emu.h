@@ -43,7 +43,7 @@ int opensps_emu_set_loss_weights(opensps_emu_t *emu, const float *w);
 int opensps_emu_set_adam(opensps_emu_t *emu, float beta1, float beta2, float eps);
 
 /* Inference: device x (n x n_in) -> device y (n x n_out), asynchronous on stream (a hipStream_t; NULL: the
- * emulator's own). */
+ * emulator's stream). */
 int opensps_emu_predict(opensps_emu_t *emu, int n, const float *x, float *y, void *stream);
 /* The training loss L = sum over samples and outputs of w_o (y'_o - t'_o)^2 / n (t' the scaled targets) and its
  * gradient with respect to the flat vector, without an update: device x (n x n_in), t (n x n_out); host grads
@@ -52,6 +52,9 @@ int opensps_emu_gradients(opensps_emu_t *emu, int n, const float *x, const float
 /* One Adam step on a minibatch (device x, t as above) with learning rate lr; *loss (host, or NULL) receives the
  * minibatch loss before the step. Asynchronous unless loss is given. */
 int opensps_emu_train_step(opensps_emu_t *emu, int n, const float *x, const float *t, float lr, float *loss);
+/* Runs the emulator's work on stream (a hipStream_t; NULL: the null stream) instead of its own, so that it is
+ * ordered with the caller's work on its inputs and outputs. */
+int opensps_emu_set_stream(opensps_emu_t *emu, void *stream);
 /* Waits for the emulator's stream. */
 int opensps_emu_sync(opensps_emu_t *emu);
 

@@ -48,6 +48,7 @@ struct EmuGemmArgs {
     int ldct;
     const float *omu, *osd; // inference's last layer: Cf = C osd + omu, rows < mvalid, columns < nvalid
     int mvalid, nvalid;
+    int ksplit;             // > 1: blockIdx.z takes K / ksplit of the sum and adds it to Cf atomically (no epilogue)
 };
 
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
@@ -60,13 +61,14 @@ __global__ __launch_bounds__(256) void k_emu_gemm(EmuGemmArgs a) {
     __shared__ __align__(16) uint16_t Bs[2][EM_T * EM_LDR];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, wr = wv >> 1, wc = wv & 1;
     const int row0 = blockIdx.y * EM_T, col0 = blockIdx.x * EM_T;
+    const int kspan = a.ksplit > 1 ? a.K / a.ksplit : a.K, kbase = a.ksplit > 1 ? blockIdx.z * kspan : 0;
     // the loader: 2 x 16 bytes of A and of B a thread per K tile (rows tid / 4 and + 64, k chunk of 8)
     const int lr = tid >> 2, lk = (tid & 3) * 8;
-    const uint16_t* pa0 = a.A + (size_t)(row0 + lr) * a.lda + lk;
+    const uint16_t* pa0 = a.A + (size_t)(row0 + lr) * a.lda + kbase + lk;
     const uint16_t* pa1 = pa0 + (size_t)64 * a.lda;
     const bool vb0 = col0 + lr < a.N, vb1 = col0 + lr + 64 < a.N;
-    const uint16_t* pb0 = a.B + (size_t)(vb0 ? col0 + lr : 0) * a.ldb + lk;
-    const uint16_t* pb1 = a.B + (size_t)(vb1 ? col0 + lr + 64 : 0) * a.ldb + lk;
+    const uint16_t* pb0 = a.B + (size_t)(vb0 ? col0 + lr : 0) * a.ldb + kbase + lk;
+    const uint16_t* pb1 = a.B + (size_t)(vb1 ? col0 + lr + 64 : 0) * a.ldb + kbase + lk;
     uint4 ra0, ra1, rb0, rb1;
     auto fetch = [&](int k0) {
         ra0 = *(const uint4*)(pa0 + k0);
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(256) void k_emu_gemm(EmuGemmArgs a) {
     fetch(0);
     stash(0);
     __syncthreads();
-    const int nk = a.K / EM_K;
+    const int nk = kspan / EM_K;
     for (int kt = 0; kt < nk; kt++) {
         const int buf = kt & 1;
         if (kt + 1 < nk) fetch((kt + 1) * EM_K);
@@ -111,6 +113,19 @@ __global__ __launch_bounds__(256) void k_emu_gemm(EmuGemmArgs a) {
         __syncthreads();
     }
     if (!wave_on) return;
+    if (a.ksplit > 1) {   // a partial sum of K: added to the FP32 output
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int c = col0 + wc * 64 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r0 = row0 + wr * 64 + i * 16 + 4 * (lane >> 4);
+#pragma unroll
+                for (int e = 0; e < 4; e++) atomicAdd(a.Cf + (size_t)(r0 + e) * a.ldcf + c, acc[i][j][e]);
+            }
+        }
+        return;
+    }
     // epilogue: lane l holds rows 4 (l / 16) + e of each block, column l % 16
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -255,7 +270,8 @@ struct EmuLayer {
 
 struct OpsEmu {
     int device = 0, n_in = 0, n_out = 0, hidden = 0, layers = 0, kin = 64, noutp = 256;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // the work's stream: the emulator's own, or the caller's (set_stream)
+    hipStream_t own = nullptr;
     std::vector<EmuLayer> L;            // layers + 1
     float *imu = nullptr, *isd = nullptr, *omu = nullptr, *osd = nullptr, *lw = nullptr, *dloss = nullptr;
     float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
@@ -288,7 +304,7 @@ static void emu_free_train(OpsEmu* e) {
 }
 
 static int emu_gemm(OpsEmu* e, const EmuGemmArgs& g, hipStream_t s) {
-    dim3 grid((g.N + EM_T - 1) / EM_T, g.M / EM_T);
+    dim3 grid((g.N + EM_T - 1) / EM_T, g.M / EM_T, g.ksplit > 1 ? g.ksplit : 1);
     k_emu_gemm<<<grid, 256, 0, s>>>(g);
     EMU_CHK(hipGetLastError());
     return 0;
@@ -321,10 +337,11 @@ int opensps_emu_create(int device, int n_in, int n_out, int hidden, int layers, 
     e->n_out = n_out;
     e->hidden = hidden;
     e->layers = layers;
-    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&e->own, hipStreamNonBlocking) != hipSuccess) {
         delete e;
         return ops_fail(nullptr, 10, "emu_create: stream");
     }
+    e->stream = e->own;
     std::mt19937 rng(seed);
     for (int l = 0; l <= layers; l++) {
         EmuLayer Ly;
@@ -380,8 +397,15 @@ int opensps_emu_destroy(opensps_emu_t* e) {
     void* ps[] = {e->imu, e->isd, e->omu, e->osd, e->lw, e->dloss, e->X0, e->X1};
     for (void* p : ps) emu_free(p);
     emu_free_train(e);
-    (void)hipStreamDestroy(e->stream);
+    (void)hipStreamDestroy(e->own);
     delete e;
+    return 0;
+}
+
+int opensps_emu_set_stream(opensps_emu_t* e, void* stream) {
+    if (!e) return 1;
+    EMU_CHK(hipStreamSynchronize(e->stream));
+    e->stream = (hipStream_t)stream;   // NULL: the null (default) stream
     return 0;
 }
 
@@ -572,10 +596,15 @@ static int emu_forward_backward(OpsEmu* e, int n, const float* x, const float* t
     uint16_t *dcur = e->dA, *dcurT = e->dAT, *dnext = e->dB, *dnextT = e->dBT;
     for (int l = NL - 1; l >= 0; l--) {
         const EmuLayer& Ly = e->L[l];
-        // dW = dPre^T X : (N x Bp) . (K x Bp)^T
+        // dW = dPre^T X : (N x Bp) . (K x Bp)^T, the batch split over enough workgroups to fill the chip
         EmuGemmArgs gw = gemm_args(Ly.N, Ly.K, Bp, dcurT, Bp, e->XTt[l], Bp);
         gw.Cf = Ly.gW;
         gw.ldcf = Ly.K;
+        const int tiles = (Ly.N / EM_T) * ((Ly.K + EM_T - 1) / EM_T);
+        int ks = 1;
+        while (ks < 64 && tiles * ks < 512 && (Bp / EM_K) % (2 * ks) == 0) ks *= 2;
+        gw.ksplit = ks;
+        if (ks > 1) EMU_CHK(hipMemsetAsync(Ly.gW, 0, (size_t)Ly.N * Ly.K * sizeof(float), s));
         if ((st = emu_gemm(e, gw, s))) return st;
         k_emu_rowsum<<<Ly.N, 256, 0, s>>>(Ly.N, Bp, dcurT, Ly.gb);
         EMU_CHK(hipGetLastError());

@@ -36,6 +36,7 @@ def lib():
         L.opensps_emu_gradients.argtypes = [vp, i, vp, vp, vp, vp]
         L.opensps_emu_train_step.argtypes = [vp, i, vp, vp, f, vp]
         L.opensps_emu_sync.argtypes = [vp]
+        L.opensps_emu_set_stream.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -48,9 +49,14 @@ class MLP:
     """One emulator network on the GPU (n_in -> layers x hidden GELU -> n_out)."""
 
     def __init__(self, n_in, n_out, hidden=512, layers=4, seed=1, device=0):
+        import torch
         self.dims = (n_in, n_out, hidden, layers)
         self._e = C.c_void_p()
         H.check(lib().opensps_emu_create(device, n_in, n_out, hidden, layers, seed, C.byref(self._e)))
+        # the kernels run on torch's current stream: ordered with the torch work that writes their inputs, and the
+        # caching allocator's reuse of those tensors is safe (it is stream-ordered)
+        self._stream = torch.cuda.current_stream(device).cuda_stream
+        H.check(lib().opensps_emu_set_stream(self._e, C.c_void_p(self._stream)))
 
     def close(self):
         if self._e:
@@ -91,22 +97,19 @@ class MLP:
         H.check(lib().opensps_emu_set_adam(self._e, beta1, beta2, eps))
 
     def predict(self, x, out=None, stream=None, sync=True):
-        """x: device float32 (n, n_in) tensor -> device (n, n_out). The kernels run on the emulator's own stream
-        (or stream); with sync the call waits for them (torch's stream does not see that stream's work)."""
+        """x: device float32 (n, n_in) tensor -> device (n, n_out), on torch's current stream (or stream); with
+        sync the call waits for the result."""
         import torch
         n = x.shape[0]
         assert x.dtype == torch.float32 and x.is_contiguous() and x.shape[1] == self.dims[0]
         if out is None:
             out = torch.empty((n, self.dims[1]), dtype=torch.float32, device=x.device)
-        torch.cuda.current_stream().synchronize()   # x written by torch's stream
         H.check(lib().opensps_emu_predict(self._e, n, x.data_ptr(), out.data_ptr(), stream))
         if sync:
             self.sync()
         return out
 
     def gradients(self, x, t):
-        import torch
-        torch.cuda.current_stream().synchronize()
         n = x.shape[0]
         g = np.zeros(self.nparams(), np.float32)
         loss = C.c_float()
@@ -114,7 +117,7 @@ class MLP:
         return loss.value, g
 
     def train_step(self, x, t, lr, want_loss=False):
-        """One Adam step on device x, t (written before the call; the emulator's stream runs after the host call)."""
+        """One Adam step on device x, t (stream-ordered after the torch work that wrote them)."""
         loss = C.c_float()
         H.check(lib().opensps_emu_train_step(self._e, x.shape[0], x.data_ptr(), t.data_ptr(), float(lr),
                                              C.byref(loss) if want_loss else None))
