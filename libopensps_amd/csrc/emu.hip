@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <stdlib.h>
+
 #include <cmath>
 #include <random>
 #include <vector>
@@ -161,6 +163,109 @@ __global__ __launch_bounds__(256) void k_emu_gemm(EmuGemmArgs a) {
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// k_emu_fused: inference through every layer in one kernel. A workgroup (1024 threads, 16 wavefronts) takes 128
+// samples; their activations (128 x hidden bf16) stay in LDS from layer to layer. Wavefront w computes the
+// 16-column blocks w and w + 16 of the layer's outputs for all 128 rows (8 x 2 blocks of the 16x16x32 instruction)
+// from the activations in LDS (the A operand) and the layer's weights from L2 (the B operand: each fragment serves
+// the 8 row blocks, which halves the weights' L2 traffic against 64-row tiles; the next K step's fragments are
+// loaded during this step's products). The slices go back in place (bias, GELU, bf16) after a barrier; the last layer
+// writes the scaled outputs. hidden <= 512.
+// ------------------------------------------------------------------------------------------------
+#define EF_ROWS 128
+struct EmuFusedArgs {
+    int n, n_in, kin, hidden, layers, noutp, n_out, ldrow;
+    const float *x, *imu, *isd, *omu, *osd;
+    const uint16_t* const* W;   // device arrays of the layers' weight and bias pointers
+    const float* const* b;
+    float* y;
+};
+
+template <int LDR>   // the LDS row (bf16)
+__device__ __forceinline__ void fused_layer(const EmuFusedArgs& a, uint16_t* act, int K, int N, const uint16_t* W,
+                                            const float* bias, bool last) {
+    // wavefront w owns the 16-column blocks w and w + 16 (those below N / 16) for all 128 rows: each weight
+    // fragment from L2 serves 8 row blocks
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nblk = N / 16;
+    const bool on0 = wv < nblk, on1 = wv + 16 < nblk;
+    if (!on0) {
+        __syncthreads();
+        __syncthreads();
+        return;
+    }
+    f32x4_t acc[8][2];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        acc[i][0] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+        acc[i][1] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    }
+    const int kq = 8 * (lane >> 4);
+    const uint16_t* wp0 = W + (size_t)(wv * 16 + (lane & 15)) * K + kq;
+    const uint16_t* wp1 = W + (size_t)((on1 ? wv + 16 : wv) * 16 + (lane & 15)) * K + kq;
+    const uint16_t* ap = act + (lane & 15) * LDR + kq;   // row block i: + 16 i LDR
+    bf16x8_t n0 = *(const bf16x8_t*)(wp0), n1 = *(const bf16x8_t*)(wp1);
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        const bf16x8_t b0 = n0, b1 = n1;
+        if (k0 + 32 < K) {   // the next K step's weights, in flight during this step's products
+            n0 = *(const bf16x8_t*)(wp0 + k0 + 32);
+            n1 = *(const bf16x8_t*)(wp1 + k0 + 32);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const bf16x8_t av = *(const bf16x8_t*)(ap + i * 16 * LDR + k0);
+            acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b0, acc[i][0], 0, 0, 0);
+            if (on1) acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b1, acc[i][1], 0, 0, 0);
+        }
+    }
+    __syncthreads();   // every wavefront has read the layer's input
+    // lane l holds rows 4 (l / 16) + e of each block, column l % 16; the last layer leaves FP32 rows of 256 in LDS
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        if (j == 1 && !on1) break;
+        const int c = (wv + 16 * j) * 16 + (lane & 15);
+        const float bc = bias[c];
+        uint16_t* ao = act + 4 * (lane >> 4) * LDR + c;
+        float* fo = (float*)act + 4 * (lane >> 4) * 256 + c;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const float v = acc[i][j][e] + bc;
+                if (last)
+                    fo[(i * 16 + e) * 256] = v;
+                else
+                    ao[(i * 16 + e) * LDR] = bf16_rn(gelu(v));
+            }
+    }
+    __syncthreads();   // the layer's output is in place
+}
+
+template <int HID>
+__global__ __launch_bounds__(1024) void k_emu_fused(EmuFusedArgs a) {
+    constexpr int LDR = HID + 8;
+    extern __shared__ __align__(16) uint16_t act[];
+    const int row0 = blockIdx.x * EF_ROWS;
+    for (int e = threadIdx.x; e < EF_ROWS * a.kin; e += blockDim.x) {
+        const int r = e / a.kin, k = e % a.kin;
+        float v = 0.0f;
+        if (row0 + r < a.n && k < a.n_in) v = (a.x[(size_t)(row0 + r) * a.n_in + k] - a.imu[k]) / a.isd[k];
+        act[r * LDR + k] = bf16_rn(v);
+    }
+    __syncthreads();
+    for (int l = 0; l <= a.layers; l++) {
+        const int K = l == 0 ? a.kin : HID;
+        const bool last = l == a.layers;
+        fused_layer<LDR>(a, act, K, last ? 256 : HID, a.W[l], a.b[l], last);
+    }
+    // the outputs, scaled, from the FP32 rows in LDS
+    const float* fo = (const float*)act;
+    for (int e = threadIdx.x; e < EF_ROWS * a.n_out; e += blockDim.x) {
+        const int r = e / a.n_out, c = e % a.n_out;
+        if (row0 + r < a.n) a.y[(size_t)(row0 + r) * a.n_out + c] = fo[r * 256 + c] * a.osd[c] + a.omu[c];
+    }
+}
+
 // raw inputs (n x n_in, FP32) -> scaled bf16 rows X (Bp x kin) and, for training, XT (kin x Bp); zero padding
 __global__ void k_emu_input(int n, int n_in, int kin, int Bp, const float* x, const float* mu, const float* sd,
                             uint16_t* X, uint16_t* XT) {
@@ -276,6 +381,8 @@ struct OpsEmu {
     float *imu = nullptr, *isd = nullptr, *omu = nullptr, *osd = nullptr, *lw = nullptr, *dloss = nullptr;
     float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
     long step = 0;
+    bool per_layer = false;             // inference layer by layer (OPENSPS_EMU_PER_LAYER=1) instead of fused
+    void** d_ptrs = nullptr;            // device: the layers' bf16 weight pointers [17], then their bias pointers
     // activations (grown on demand): inference ping-pong; training per-layer
     int cap_inf = 0, cap_tr = 0;
     uint16_t *X0 = nullptr, *X1 = nullptr;
@@ -342,6 +449,8 @@ int opensps_emu_create(int device, int n_in, int n_out, int hidden, int layers, 
         return ops_fail(nullptr, 10, "emu_create: stream");
     }
     e->stream = e->own;
+    const char* pl = getenv("OPENSPS_EMU_PER_LAYER");
+    e->per_layer = pl && pl[0] == '1';
     std::mt19937 rng(seed);
     for (int l = 0; l <= layers; l++) {
         EmuLayer Ly;
@@ -373,6 +482,15 @@ int opensps_emu_create(int device, int n_in, int n_out, int hidden, int layers, 
         EMU_CHK(hipGetLastError());
         e->L.push_back(Ly);
     }
+    {
+        std::vector<void*> ptrs(34, nullptr);
+        for (int l = 0; l <= layers; l++) {
+            ptrs[l] = e->L[l].Wb;
+            ptrs[17 + l] = e->L[l].b;
+        }
+        EMU_CHK(hipMalloc((void**)&e->d_ptrs, 34 * sizeof(void*)));
+        EMU_CHK(hipMemcpy(e->d_ptrs, ptrs.data(), 34 * sizeof(void*), hipMemcpyHostToDevice));
+    }
     std::vector<float> zeros(256, 0.0f), ones(256, 1.0f);
     float** vecs[] = {&e->imu, &e->isd, &e->omu, &e->osd, &e->lw};
     for (float** p : vecs) EMU_CHK(hipMalloc((void**)p, 256 * sizeof(float)));
@@ -394,7 +512,7 @@ int opensps_emu_destroy(opensps_emu_t* e) {
         void* ps[] = {Ly.W, Ly.b, Ly.mW, Ly.vW, Ly.mb, Ly.vb, Ly.gW, Ly.gb, Ly.Wb, Ly.WTb};
         for (void* p : ps) emu_free(p);
     }
-    void* ps[] = {e->imu, e->isd, e->omu, e->osd, e->lw, e->dloss, e->X0, e->X1};
+    void* ps[] = {e->imu, e->isd, e->omu, e->osd, e->lw, e->dloss, e->X0, e->X1, e->d_ptrs};
     for (void* p : ps) emu_free(p);
     emu_free_train(e);
     (void)hipStreamDestroy(e->own);
@@ -515,6 +633,41 @@ int opensps_emu_predict(opensps_emu_t* e, int n, const float* x, float* y, void*
         EMU_CHK(hipMalloc((void**)&e->X0, (size_t)Bp * w * sizeof(uint16_t)));
         EMU_CHK(hipMalloc((void**)&e->X1, (size_t)Bp * w * sizeof(uint16_t)));
         e->cap_inf = Bp;
+    }
+    if (e->hidden <= 512 && e->layers <= 16 && !e->per_layer) {
+        // one kernel through every layer, the activations in LDS
+        EmuFusedArgs f;
+        f.n = n;
+        f.n_in = e->n_in;
+        f.kin = e->kin;
+        f.hidden = e->hidden;
+        f.layers = e->layers;
+        f.noutp = e->noutp;
+        f.n_out = e->n_out;
+        f.ldrow = std::max(e->hidden, e->kin) + 8;
+        f.x = x;
+        f.imu = e->imu;
+        f.isd = e->isd;
+        f.omu = e->omu;
+        f.osd = e->osd;
+        f.W = (const uint16_t* const*)e->d_ptrs;
+        f.b = (const float* const*)(e->d_ptrs + 17);
+        f.y = y;
+        const int hid = e->hidden;
+        f.ldrow = hid + 8;
+        // LDS: the activations, or the last layer's FP32 rows of 256 (whichever is larger)
+        const size_t lds = std::max((size_t)EF_ROWS * f.ldrow * sizeof(uint16_t), (size_t)EF_ROWS * 256 * sizeof(float));
+        const void* kfn = hid == 512 ? (const void*)k_emu_fused<512>
+                        : hid == 384 ? (const void*)k_emu_fused<384>
+                        : hid == 256 ? (const void*)k_emu_fused<256> : (const void*)k_emu_fused<128>;
+        EMU_CHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const unsigned grid = (unsigned)((n + EF_ROWS - 1) / EF_ROWS);
+        if (hid == 512) k_emu_fused<512><<<grid, 1024, lds, s>>>(f);
+        else if (hid == 384) k_emu_fused<384><<<grid, 1024, lds, s>>>(f);
+        else if (hid == 256) k_emu_fused<256><<<grid, 1024, lds, s>>>(f);
+        else k_emu_fused<128><<<grid, 1024, lds, s>>>(f);
+        EMU_CHK(hipGetLastError());
+        return 0;
     }
     const size_t nin = (size_t)Bp * e->kin;
     k_emu_input<<<(unsigned)((nin + 255) / 256), 256, 0, s>>>(n, e->n_in, e->kin, Bp, x, e->imu, e->isd, e->X0, nullptr);

@@ -202,7 +202,7 @@ class Emulator:
                         log(f"epoch {ep + 1}/{epochs}: loss {l:.5f}")
         mlp.sync()
         t_train = time.perf_counter() - t1
-        box = (np.asarray(x_tr.min(axis=0)), np.asarray(x_tr.max(axis=0)))
+        box = (np.array([BOX[k][0] for k in INPUTS]), np.array([BOX[k][1] for k in INPUTS]))   # the sampled box
         emu = Emulator(dctx, mlp, pca, n_pca, nb, box, cosmo, precision)
         emu.train_info = {"n_train": n_train, "n_pca_fit": n_pca_fit, "pca_explained": pca.explained, "epochs": epochs,
                           "batch": batch, "lr": lr, "steps": total, "loss_history": hist, "pca_s": t_pca,

@@ -40,10 +40,14 @@ def _setup(dims, seed=0, n=300):
 DIMS = [(7, 228, 512, 4), (13, 100, 256, 2)]
 
 
-@pytest.mark.parametrize("dims", DIMS)
-def test_predict_matches_oracle(dims):
+@pytest.mark.parametrize("per_layer", [False, True])
+@pytest.mark.parametrize("dims", DIMS + [(8, 64, 384, 3), (5, 200, 128, 1)])
+def test_predict_matches_oracle(dims, per_layer, monkeypatch):
+    """The fused inference kernel (activations in LDS) and the layer-by-layer one."""
+    if per_layer:
+        monkeypatch.setenv("OPENSPS_EMU_PER_LAYER", "1")
     net, p, scale, x, _ = _setup(dims)
-    print(f"case: emu predict dims={dims}", flush=True)
+    print(f"case: emu predict dims={dims} per_layer={per_layer}", flush=True)
     y = net.predict(_dev(x)).cpu().numpy()
     net.sync()
     ref, _ = E.forward(p, x, dims, scale, round_bf16=True)
