@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-operator", action="store_true", help="skip the IMF-per-model workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the batch-adjoint workload")
+    ap.add_argument("--no-emulator", action="store_true", help="skip the emulator workload")
+    ap.add_argument("--emu-train", type=int, default=4_000_000, help="emulator training spectra")
+    ap.add_argument("--emu-epochs", type=int, default=30)
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a 256 MiB device-to-device copy (the calibration of the HBM counter runs)")
     return ap.parse_args()
@@ -239,6 +242,65 @@ def adjoint_workload(args, n=8192):
     return result
 
 
+def emulator_workload(args, csp_models_per_s):
+    """The learned emulator: a 7 -> 4 x 512 GELU -> 128 PCA + 100 magnitudes MLP trained on 4M spectra that the
+    batch generates (the parametric workload's grid, filters and ranges), Adam with cosine decay, on the project's
+    own forward / backward / Adam kernels. Reports the training time, the validation error against the FP64 batch
+    (16k models), the inference rate on 1M models (the fused kernel) against the mixed batch's, and the serving
+    path's fallback rate with a 1 % spot-check at 0.01 mag."""
+    import numpy as np
+    import torch
+    from libopensps_amd import emu_train as T, hip as H, synth
+
+    t0 = time.perf_counter()
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    grid = synth.ssp_grid(seed=1, nspec=5994, nt=107, nz=22)
+    fset = synth.gaussian_filters(100, seed=7)
+    cz, cdl = O.cosmology_table()
+    dctx = H.DeviceContext(0)
+    dctx.set_grid(grid)
+    dctx.set_bands(O.build_bands(fset, grid.lam))
+    dctx.set_cosmology(cz, cdl)
+    dctx.set_options(1, 1)
+    emu = T.Emulator.train(dctx, (cz, cdl), n_train=args.emu_train, n_pca_fit=200_000, n_pca=128,
+                           epochs=args.emu_epochs, batch=8192, lr=1e-3, hidden=512, layers=4)
+    t_total = time.perf_counter() - t0
+    val = T.validate(emu, dctx, n=16384)
+    # inference on 1M models
+    n = 1 << 20
+    x = T.sample_inputs(n, 777)
+    fd, dd = emu.inputs(x)
+    emu.predict_raw(fd)
+    torch.cuda.synchronize()
+    reps = 5
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        emu.mlp.predict(fd, sync=False)
+    emu.mlp.sync()
+    dt = (time.perf_counter() - t1) / reps
+    rate = n / dt
+    flops = 2.0 * (T.N_FEAT * 512 + 3 * 512 * 512 + 512 * (128 + 100)) * n
+    # the serving path: 16 batches of 65536 with a 1 % spot-check at 0.01 mag
+    emu.stats = {k: 0 for k in emu.stats}
+    for b in range(16):
+        emu.predict(T.sample_inputs(65536, 2000 + b), spot=0.01, tol=0.01, seed=b)
+    info = emu.train_info
+    out = {"workload": "emulator: 7 parameters -> 4 x 512 GELU -> 128 PCA coefficients of the log-spectrum + 100 "
+                       f"magnitudes; trained on {info['n_train']} spectra of the batch (PCA from {info['n_pca_fit']})",
+           "value": rate, "unit": "models/s (inference, 1M batch)", "vs_csp_mixed": rate / csp_models_per_s,
+           "inference_ms_per_1M": 1e3 * dt * (1 << 20) / n,
+           "roofline": {"bound": "mfma", "kernel": "k_emu_fused<512>", "achieved": flops / dt / 1e12,
+                        "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": flops / dt / 1e12 / BF16_PEAK_TFLOPS},
+           "validation_vs_fp64_batch": val,
+           "fallback": {"rate": emu.fallback_rate(), "spot": 0.01, "tol_mag": 0.01, **emu.stats},
+           "training": {k: v for k, v in info.items() if k != "loss_history"},
+           "loss_first_last": [info["loss_history"][0], info["loss_history"][-1]], "total_s": t_total}
+    emu.mlp.close()
+    dctx.close()
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -414,6 +476,8 @@ def main():
         out["operator"] = operator_workload(args)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_workload(args)
+    if not args.no_emulator:
+        out["emulator"] = emulator_workload(args, value)
     print(json.dumps(out))
     if args.copy_probe:
         x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
