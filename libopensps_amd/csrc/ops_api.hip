@@ -400,6 +400,17 @@ static void free_operator(OpsCtx* c) {
     c->op = OpsCtx::OpsOperator();
 }
 
+// k_mm_bf16x3 over every row tile and at most maxct column tiles (grouped: the kernel reads the count on the device)
+static int launch_mm(OpsCtx* c, const MMArgs& mm, int maxct) {
+    const size_t lds = 8 * 512 * sizeof(bf16x8_t) + 2 * OPS_MM_COLS * sizeof(int);
+    const unsigned blocks = 8u * (unsigned)((mm.nrt + 7) / 8) * (unsigned)maxct;
+    if (blocks == 0) return 0;
+    HIPCHK(c, hipFuncSetAttribute((const void*)k_mm_bf16x3, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    k_mm_bf16x3<<<blocks, 256, lds, c->stream>>>(mm);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
 static void free_full(OpsCtx* c) {
     void* ps[] = {c->full.A64, c->full.Ahi, c->full.Alo, c->full.At64, c->full.Ath, c->full.Atl};
     for (void* p : ps) free_ptr(p);
@@ -747,11 +758,14 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     const int gparts = ops ? 1 : parts;
     const int maxtiles = (n * gparts + OPS_TILE_COL - 1) / OPS_TILE_COL + c->npair;
     const int optiles = (n * parts + OPS_TILE_COL - 1) / OPS_TILE_COL;
+    // the split-BF16 products' wider tiles (k_mm_bf16x3)
+    const int maxm = mixed ? (n * gparts + OPS_MM_COLS - 1) / OPS_MM_COLS + c->npair : 0;
+    const int optm = mixed && uni ? (n * parts + OPS_MM_COLS - 1) / OPS_MM_COLS : 0;
     HIPCHK(c, hipSetDevice(c->device));
 
     // scratch: misc = zlo, it, status, order (n ints each) | ntiles | flag
     if ((st = ops_reserve(c, c->misc, sizeof(int) * ((size_t)4 * n + 64)))) return st;
-    if ((st = ops_reserve(c, c->tiles, sizeof(int4) * (size_t)(maxtiles + (uni ? optiles : 0))))) return st;
+    if ((st = ops_reserve(c, c->tiles, sizeof(int4) * (size_t)(maxtiles + (uni ? optiles : 0) + maxm + optm)))) return st;
     if ((st = ops_reserve(c, c->hdr, sizeof(double) * OPS_HDR_N * (size_t)n))) return st;
     if ((st = ops_reserve(c, c->yo, tsz * (size_t)parts * n * c->nsp))) return st;
     if (mixed) {
@@ -764,6 +778,9 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     int* d_ntiles = misc + 4 * n;
     int* d_flag = misc + 4 * n + 16;
     int* d_optiles = misc + 4 * n + 32;
+    int* d_nmtiles = misc + 4 * n + 40;
+    int* d_opmtiles = misc + 4 * n + 48;
+    int4* mtiles = (int4*)c->tiles.p + maxtiles + (uni ? optiles : 0);   // then the uniform ones
     const int npass = (ops && c->nz > 1) ? 2 : 1;
     if (ops) {
         if ((st = ops_reserve(c, c->wsfh, sizeof(double) * (size_t)n * c->nt))) return st;
@@ -837,10 +854,16 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ga.order = d_order;
     ga.ntiles = d_ntiles;
     ga.tiles = (int4*)c->tiles.p;
+    ga.nmtiles = mixed && !uni ? d_nmtiles : nullptr;
+    ga.mtiles = mixed && !uni ? mtiles : nullptr;
     k_group<<<1, 256, sizeof(int) * 256 * (size_t)c->npair, c->stream>>>(ga);
     HIPCHK(c, hipGetLastError());
+    if (uni && mixed) {
+        k_op_tiles<<<1, 256, 0, c->stream>>>(n * parts, OPS_MM_COLS, d_opmtiles, mtiles + maxm);
+        HIPCHK(c, hipGetLastError());
+    }
     if (wz) {
-        k_op_tiles<<<1, 256, 0, c->stream>>>(n * parts, d_optiles, (int4*)c->tiles.p + maxtiles);
+        k_op_tiles<<<1, 256, 0, c->stream>>>(n * parts, OPS_TILE_COL, d_optiles, (int4*)c->tiles.p + maxtiles);
         HIPCHK(c, hipGetLastError());
     }
     if (ops) {
@@ -952,7 +975,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         sa.lbol = a->lbol;
         k_op_sums<<<(n + 255) / 256, 256, 0, c->stream>>>(sa);
         HIPCHK(c, hipGetLastError());
-        k_op_tiles<<<1, 256, 0, c->stream>>>(n * parts, d_optiles, (int4*)c->tiles.p + maxtiles);
+        k_op_tiles<<<1, 256, 0, c->stream>>>(n * parts, OPS_TILE_COL, d_optiles, (int4*)c->tiles.p + maxtiles);
         HIPCHK(c, hipGetLastError());
     }
     if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
@@ -969,16 +992,32 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     gm.tiles = (const int4*)c->tiles.p + (uni ? maxtiles : 0);
     gm.A64 = ops ? c->op.d_A64 : wz ? c->full.A64 : c->d_A64;
     gm.w64 = (const double*)c->w64.p;
-    gm.Ahi = ops ? c->op.d_Ahi : wz ? c->full.Ahi : c->d_Ahi;
-    gm.Alo = ops ? c->op.d_Alo : wz ? c->full.Alo : c->d_Alo;
-    gm.whi = pa.whi;
-    gm.wlo = pa.wlo;
     gm.yo = c->yo.p;
     const unsigned ptx = (unsigned)((npxt + 7) / 8);
     const unsigned gblocks = 8u * ptx * (unsigned)(uni ? optiles : maxtiles);
-    if (mixed)
-        k_gemm_bf16x3<<<gblocks, 256, 0, c->stream>>>(gm);
-    else if (c->f64_vector)
+    if (mixed) {
+        MMArgs mm;
+        mm.mblk = npxblk;
+        mm.nrt = (npxblk + 7) / 8;
+        mm.ksteps = kpad / OPS_KSTEP_BF16;
+        mm.mode = OPS_MM_GROUPED;
+        mm.n = n;
+        mm.parts = parts;
+        mm.order = d_order;
+        mm.ntiles = uni ? d_opmtiles : d_nmtiles;
+        mm.tiles = uni ? mtiles + maxm : mtiles;
+        mm.ncols = 0;
+        mm.nct = 0;
+        mm.Ah = ops ? c->op.d_Ahi : wz ? c->full.Ahi : c->d_Ahi;
+        mm.Al = ops ? c->op.d_Alo : wz ? c->full.Alo : c->d_Alo;
+        mm.slab = (size_t)npxblk * mm.ksteps * 64 * 8;
+        mm.Bh = pa.whi;
+        mm.Bl = pa.wlo;
+        mm.ldb = kpad;
+        mm.C = (float*)c->yo.p;
+        mm.ldc = c->nsp;
+        if ((st = launch_mm(c, mm, uni ? optm : maxm))) return st;
+    } else if (c->f64_vector)
         k_gemm_f64_vec<<<gblocks, 256, 0, c->stream>>>(gm);
     else
         k_gemm_f64<<<gblocks, 256, 0, c->stream>>>(gm);
@@ -1161,19 +1200,36 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     gt.nct = (ncols + 63) / 64;
     gt.nsp = nsp;
     gt.ldc = ldc;
-    gt.Ah = c->full.Ath;
-    gt.Al = c->full.Atl;
-    gt.Bh = (const uint16_t*)ga.ybh;
-    gt.Bl = (const uint16_t*)ga.ybl;
     gt.A64 = c->full.At64;
     gt.B64 = (const double*)c->gyb.p;
     gt.C = c->gC.p;
-    const unsigned gblocks = 8u * (unsigned)((nrt + 7) / 8) * (unsigned)gt.nct;
-    if (mixed)
-        k_gemmt_bf16x3<<<gblocks, 256, 0, c->stream>>>(gt);
-    else
+    if (mixed) {
+        MMArgs mm;
+        mm.mblk = nrt * 4;
+        mm.nrt = (mm.mblk + 7) / 8;
+        mm.ksteps = nsp / OPS_KSTEP_BF16;
+        mm.mode = OPS_MM_DIRECT;
+        mm.n = n;
+        mm.parts = parts;
+        mm.order = nullptr;
+        mm.ntiles = nullptr;
+        mm.tiles = nullptr;
+        mm.ncols = ncols;
+        mm.nct = (ncols + OPS_MM_COLS - 1) / OPS_MM_COLS;
+        mm.Ah = c->full.Ath;
+        mm.Al = c->full.Atl;
+        mm.slab = 0;
+        mm.Bh = (const uint16_t*)ga.ybh;
+        mm.Bl = (const uint16_t*)ga.ybl;
+        mm.ldb = nsp;
+        mm.C = (float*)c->gC.p;
+        mm.ldc = ldc;
+        if ((st = launch_mm(c, mm, mm.nct))) return st;
+    } else {
+        const unsigned gblocks = 8u * (unsigned)((nrt + 7) / 8) * (unsigned)gt.nct;
         k_gemmt_f64<<<gblocks, 256, 0, c->stream>>>(gt);
-    HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipGetLastError());
+    }
     if (c->profiling) HIPCHK(c, hipEventRecord(c->gev[2], c->stream));
 
     GradOutArgs go;

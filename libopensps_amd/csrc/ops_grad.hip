@@ -3,7 +3,8 @@
 //                 band integrals, the observed frame, the velocity smoothing, the IGM and the attenuation, with
 //                 the forward's spectra and transmissions recomputed (not stored), to the cotangents of the age
 //                 sums (the rows of the transposed product) and the per-model scalars of the parameters
-//   k_gemmt_*     w_bar = basis^T y_bar on the matrix cores (split-BF16, three products, FP32 accumulation; or f64)
+//   k_gemmt_f64   w_bar = basis^T y_bar on the matrix cores in FP64 (the split-BF16 product is k_mm_bf16x3,
+//                 ops_mm.hip, with the basis packed by k_pack_t)
 //   k_grad_out    params_bar, weights_bar / weights_z_bar with the L_bol and mass terms and the metallicity blend
 #include "ops_common.h"
 #include "ops_launch.h"
@@ -420,52 +421,6 @@ __device__ __forceinline__ bool gemmt_tile(const GemmTArgs& a, int& rt, int& ct)
     return rt < a.nrt && ct < a.nct;
 }
 
-__global__ __launch_bounds__(256) void k_gemmt_bf16x3(GemmTArgs a) {
-    int rt, ct;
-    if (!gemmt_tile(a, rt, ct)) return;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, wr = wv >> 1, wc = wv & 1;
-    const int row0 = rt * 64 + wr * 32, col0 = ct * 64 + wc * 32;
-    const int kq = 8 * (lane >> 4);
-    const bf16x8_t *Ah[2], *Al[2], *Bh[2], *Bl[2];
-    bool valid[2];
-    for (int i = 0; i < 2; i++) {
-        const size_t ra = (size_t)(row0 + i * 16 + (lane & 15)) * a.nsp + kq;
-        Ah[i] = (const bf16x8_t*)(a.Ah + ra);
-        Al[i] = (const bf16x8_t*)(a.Al + ra);
-        const int col = col0 + i * 16 + (lane & 15);
-        valid[i] = col < a.ncols;
-        const size_t rb = (size_t)(valid[i] ? col : 0) * a.nsp + kq;
-        Bh[i] = (const bf16x8_t*)(a.Bh + rb);
-        Bl[i] = (const bf16x8_t*)(a.Bl + rb);
-    }
-    f32x4_t acc[2][2];
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) acc[i][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
-    const bf16x8_t zero8 = (bf16x8_t){};
-    const int ksteps = a.nsp / 32;
-    for (int ks = 0; ks < ksteps; ks++) {
-        bf16x8_t ah[2], al[2], bh[2], bl[2];
-        for (int i = 0; i < 2; i++) {
-            ah[i] = Ah[i][ks * 4];
-            al[i] = Al[i][ks * 4];
-            bh[i] = valid[i] ? Bh[i][ks * 4] : zero8;
-            bl[i] = valid[i] ? Bl[i][ks * 4] : zero8;
-        }
-        for (int i = 0; i < 2; i++)
-            for (int j = 0; j < 2; j++) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-            }
-    }
-    float* C = (float*)a.C;
-    for (int j = 0; j < 2; j++) {
-        if (!valid[j]) continue;
-        float* col = C + (size_t)(col0 + j * 16 + (lane & 15)) * a.ldc;
-        for (int i = 0; i < 2; i++) *(f32x4_t*)(col + row0 + i * 16 + 4 * (lane >> 4)) = acc[i][j];
-    }
-}
-
 __global__ __launch_bounds__(256) void k_gemmt_f64(GemmTArgs a) {
     int rt, ct;
     if (!gemmt_tile(a, rt, ct)) return;
@@ -508,7 +463,9 @@ __global__ __launch_bounds__(256) void k_gemmt_f64(GemmTArgs a) {
     }
 }
 
-// The basis as the transposed product's A: rows k = z nt + t (nrows, zero past kfull), nsp pixels (zero past nspec)
+// The basis as the transposed product's A: rows k = z nt + t (nrows, zero past kfull), nsp pixels (zero past nspec).
+// FP64 row-major; split BF16 in k_mm_bf16x3's instruction order: 16-row block k / 16, K step j / 32, lane
+// k % 16 + 16 ((j % 32) / 8), element j % 8.
 __global__ void k_pack_t(int nspec, int nsp, int kfull, int nrows, const double* basis, double* A64, uint16_t* Ah,
                          uint16_t* Al) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -519,8 +476,10 @@ __global__ void k_pack_t(int nspec, int nsp, int kfull, int nrows, const double*
     if (Ah) {
         uint16_t hi, lo;
         split_bf16(v, hi, lo);
-        Ah[e] = hi;
-        Al[e] = lo;
+        const int ksteps = nsp / 32;
+        const size_t at = (((size_t)(k >> 4) * ksteps + (j >> 5)) * 64 + (k & 15) + 16 * ((j & 31) >> 3)) * 8 + (j & 7);
+        Ah[at] = hi;
+        Al[at] = lo;
     }
 }
 

@@ -3,8 +3,8 @@
 //   k_prep      one wavefront per model: index selection (it_tesc, zlo), the FP64 SFH weights by the
 //               moment method, mass and L_bol, and the model's columns of the age-sum product
 //   k_group     one workgroup: the models sorted by metallicity pair, cut into column tiles
-//   k_gemm_*    the age sums yo = [basis(zlo) basis(zlo+1)] . W per pair on the matrix cores
-//               (FP64 16x16x4, or split-BF16 16x16x32 with three products and FP32 accumulation)
+//   k_gemm_f64  the FP64 age sums yo = [basis(zlo) basis(zlo+1)] . W per pair on the matrix cores (16x16x4);
+//               the split-BF16 ones are k_mm_bf16x3 (ops_mm.hip)
 //   k_chain     one workgroup per model: dust, IGM, velocity smoothing, the spectrum out, the
 //               observed frame, the band integrals and the magnitudes, the spectrum staged in LDS
 #include <type_traits>
@@ -344,11 +344,12 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
 #pragma clang fp contract(on)
 
 // ------------------------------------------------------------------------------------------------
-// k_group: the models sorted by metallicity pair (stable), the column tiles of each pair
+// k_group: the models sorted by metallicity pair (stable), the column tiles of each pair (OPS_TILE_COL columns,
+// and with mtiles the OPS_MM_COLS columns of the split-BF16 products)
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_group(GroupArgs a) {
     extern __shared__ int cnt[];   // [256][npair]
-    __shared__ int gstart[256], gtot[256], tbase[257];
+    __shared__ int gstart[256], gtot[256], tbase[257], mbase[257];
     const int t = threadIdx.x, G = a.npair, n = a.n;
     const int chunk = (n + 255) / 256;
     const int m0 = t * chunk, m1 = min(n, m0 + chunk);
@@ -362,15 +363,19 @@ __global__ __launch_bounds__(256) void k_group(GroupArgs a) {
     }
     __syncthreads();
     if (t == 0) {
-        int s = 0, tb = 0;
+        int s = 0, tb = 0, mb = 0;
         for (int g = 0; g < G; g++) {
             gstart[g] = s;
             tbase[g] = tb;
+            mbase[g] = mb;
             s += gtot[g];
             tb += (gtot[g] * a.parts + OPS_TILE_COL - 1) / OPS_TILE_COL;
+            mb += (gtot[g] * a.parts + OPS_MM_COLS - 1) / OPS_MM_COLS;
         }
         tbase[G] = tb;
+        mbase[G] = mb;
         *a.ntiles = tb;
+        if (a.mtiles) *a.nmtiles = mb;
     }
     __syncthreads();
     for (int m = m0; m < m1; m++) {
@@ -388,14 +393,17 @@ __global__ __launch_bounds__(256) void k_group(GroupArgs a) {
             d.w = 0;
             a.tiles[tbase[g] + i] = d;
         }
+        for (int i = 0; a.mtiles && mbase[g] + i < mbase[g + 1]; i++)
+            a.mtiles[mbase[g] + i] = make_int4(g, gstart[g] * a.parts + i * OPS_MM_COLS,
+                                               min(OPS_MM_COLS, ncol - i * OPS_MM_COLS), 0);
     }
 }
 
 // ------------------------------------------------------------------------------------------------
-// Age-sum products on the matrix cores. Workgroup tile: 64 pixels x 64 columns, four wavefronts of
-// 32 x 32 (2 x 2 blocks of 16 x 16). Operands are packed in instruction order: A at set-up (the pair's
-// two slabs side by side in K), B per model by k_prep. The blockIdx -> tile map keeps each XCD on
-// one eighth of the pixel tiles, so a pair's A rows stay in that XCD's L2 across column tiles.
+// FP64 age-sum products on the matrix cores. Workgroup tile: 64 pixels x 64 columns, four wavefronts of
+// 32 x 32 (2 x 2 blocks of 16 x 16). A is packed in instruction order at set-up (the pair's two slabs side by
+// side in K), B per model by k_prep. The blockIdx -> tile map keeps each XCD on one eighth of the pixel
+// tiles, so a pair's A rows stay in that XCD's L2 across column tiles.
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ bool gemm_tile(const GemmArgs& a, int& pt, int& ct) {
     const int b = blockIdx.x;
@@ -514,63 +522,6 @@ __global__ __launch_bounds__(256) void k_gemm_f64_vec(GemmArgs a) {
         if (oc < 0) continue;
         double* col = yo + (size_t)oc * a.nsp + pt * 64 + ty * 4;
         for (int i = 0; i < 4; i++) col[i] = acc[i][j];
-    }
-}
-
-__global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs a) {
-    int pt, ct;
-    if (!gemm_tile(a, pt, ct)) return;
-    const int4 td = a.tiles[ct];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int wr = wv >> 1, wc = wv & 1;
-    const int blk0 = pt * 4 + wr * 2;
-    const int ksteps = a.kpad / OPS_KSTEP_BF16;
-    const uint16_t *Wh[2], *Wl[2];
-    int model[2], part[2];
-    bool valid[2];
-    for (int cb = 0; cb < 2; cb++) {
-        int c = wc * 32 + cb * 16 + (lane & 15);
-        valid[cb] = c < td.z;
-        int cc = td.y + (valid[cb] ? c : 0);
-        int pos = cc / a.parts;
-        part[cb] = cc - pos * a.parts;
-        model[cb] = a.order[pos];
-        size_t off = ((size_t)model[cb] * a.parts + part[cb]) * a.kpad + 8 * (lane >> 4);
-        Wh[cb] = a.whi + off;
-        Wl[cb] = a.wlo + off;
-    }
-    f32x4_t acc[2][2];
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) acc[i][j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
-    const size_t abase = ((size_t)td.x * a.npxblk + blk0) * ksteps * 64 + lane;
-    const bf16x8_t* Ah = (const bf16x8_t*)a.Ahi + abase;
-    const bf16x8_t* Al = (const bf16x8_t*)a.Alo + abase;
-    const bf16x8_t zero8 = (bf16x8_t){};
-    for (int ks = 0; ks < ksteps; ks++) {
-        bf16x8_t ah[2], al[2], bh[2], bl[2];
-        ah[0] = Ah[(size_t)ks * 64];
-        ah[1] = Ah[(size_t)(ksteps + ks) * 64];
-        al[0] = Al[(size_t)ks * 64];
-        al[1] = Al[(size_t)(ksteps + ks) * 64];
-        for (int j = 0; j < 2; j++) {
-            bh[j] = valid[j] ? *(const bf16x8_t*)(Wh[j] + ks * 32) : zero8;
-            bl[j] = valid[j] ? *(const bf16x8_t*)(Wl[j] + ks * 32) : zero8;
-        }
-        for (int i = 0; i < 2; i++)
-            for (int j = 0; j < 2; j++) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-            }
-    }
-    float* yo = (float*)a.yo;
-    for (int j = 0; j < 2; j++) {
-        if (!valid[j]) continue;
-        float* col = yo + ((size_t)part[j] * a.n + model[j]) * a.nsp;
-        for (int i = 0; i < 2; i++) {
-            int px = (blk0 + i) * 16 + 4 * (lane >> 4);
-            *(f32x4_t*)(col + px) = acc[i][j];
-        }
     }
 }
 

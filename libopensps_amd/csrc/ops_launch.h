@@ -49,6 +49,8 @@ struct GroupArgs {
     const int* zlo;
     int *order, *ntiles;
     int4* tiles;
+    int* nmtiles;    // the tiles of k_mm_bf16x3 (null: none)
+    int4* mtiles;
 };
 
 struct GemmArgs {
@@ -57,9 +59,36 @@ struct GemmArgs {
     const int* ntiles;
     const int4* tiles;
     const double *A64, *w64;
-    const uint16_t *Ahi, *Alo, *whi, *wlo;
     void* yo;
 };
+
+// The split-BF16 products with LDS tiles (k_mm_bf16x3): C(row, col) = sum_k A(row, k) B(col, k), 128 rows x 128
+// columns a workgroup. A in instruction order [slab][16-row block][K / 32][lane][8] (hi and lo apart); B by
+// columns, K contiguous; C by columns, rows contiguous (FP32).
+#define OPS_MM_ROWS 128
+#define OPS_MM_COLS 128
+#define OPS_MM_GROUPED 0   // columns cc of a tile {slab, first, count}: model order[cc / parts], part cc % parts
+#define OPS_MM_DIRECT 1    // columns 0 .. ncols - 1 as they are
+
+struct MMArgs {
+    int mblk;                   // 16-row blocks of A (and the rows of C written: 16 mblk)
+    int nrt;                    // row tiles (ceil(mblk / 8))
+    int ksteps;                 // K / 32
+    int mode;
+    int n, parts;               // grouped: B column model * parts + part, C column part * n + model
+    const int* order;
+    const int* ntiles;          // grouped: the number of column tiles (device)
+    const int4* tiles;          // grouped: {slab, first column, columns, 0}
+    int ncols, nct;             // direct
+    const uint16_t *Ah, *Al;
+    size_t slab;                // A elements per slab
+    const uint16_t *Bh, *Bl;
+    size_t ldb;                 // B elements per column
+    float* C;
+    size_t ldc;                 // C elements per column
+};
+
+__global__ void k_mm_bf16x3(MMArgs a);
 
 struct PackArgs {
     int nspec, nt, nz, npair, kpair, kpad, npxblk;
@@ -203,7 +232,6 @@ struct GradArgs {
 
 struct GemmTArgs {                 // C(col, k) = sum_j At(k, j) Yb(col, j)
     int krows, nrt, ncols, nct, nsp, ldc;
-    const uint16_t *Ah, *Al, *Bh, *Bl;   // bf16: At (nrt 64 x nsp), Yb (ncols x nsp)
     const double *A64, *B64;
     void* C;                       // (ncols, ldc) FP32 or FP64
 };
@@ -219,7 +247,6 @@ struct GradOutArgs {
 
 template <typename T>
 __global__ void k_grad_chain(GradArgs a);
-__global__ void k_gemmt_bf16x3(GemmTArgs a);
 __global__ void k_gemmt_f64(GemmTArgs a);
 __global__ void k_grad_out(GradOutArgs a);
 __global__ void k_pack_t(int nspec, int nsp, int kfull, int nrows, const double* basis, double* A64, uint16_t* Ah,
@@ -229,7 +256,6 @@ __global__ void k_prep(PrepArgs a);
 __global__ void k_group(GroupArgs a);
 __global__ void k_gemm_f64(GemmArgs a);
 __global__ void k_gemm_f64_vec(GemmArgs a);
-__global__ void k_gemm_bf16x3(GemmArgs a);
 __global__ void k_pack_f64(PackArgs a);
 __global__ void k_pack_bf16(PackArgs a);
 #define OPS_CHAIN_KS 4   // smoothed pixels a thread holds in registers (single-buffer chain)
@@ -252,4 +278,4 @@ __global__ void k_op_segments(OpGArgs a);
 template <typename T>
 __global__ void k_op_fold(OpFArgs a);
 __global__ void k_op_sums(OpSArgs a);
-__global__ void k_op_tiles(int ncols, int* ntiles, int4* tiles);
+__global__ void k_op_tiles(int ncols, int width, int* ntiles, int4* tiles);

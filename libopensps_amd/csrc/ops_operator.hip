@@ -534,11 +534,11 @@ __global__ __launch_bounds__(256) void k_op_fold(OpFArgs a) {
     }
 }
 
-// The template product's column tiles: 64 consecutive columns each (position order), one library for all
-__global__ void k_op_tiles(int ncols, int* ntiles, int4* tiles) {
-    const int nt = (ncols + OPS_TILE_COL - 1) / OPS_TILE_COL;
-    for (int i = threadIdx.x; i < nt; i += blockDim.x)
-        tiles[i] = make_int4(0, i * OPS_TILE_COL, min(OPS_TILE_COL, ncols - i * OPS_TILE_COL), 0);
+// The uniform column tiles of one A for every column (the template library, the full basis): width consecutive
+// columns each (position order)
+__global__ void k_op_tiles(int ncols, int width, int* ntiles, int4* tiles) {
+    const int nt = (ncols + width - 1) / width;
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) tiles[i] = make_int4(0, i * width, min(width, ncols - i * width), 0);
     if (threadIdx.x == 0) *ntiles = nt;
 }
 
