@@ -115,7 +115,7 @@ def operator_workload(args, n=4096):
     # template product: useful flops 2 nspec ntpl per model column
     kmix = res["mixed"]["kernel_ms"]["template_product"]
     flops = 2.0 * grid.nspec * op.ntpl * n
-    roof = {"bound": "mfma", "kernel": "k_gemm_bf16x3 (template product)", "achieved": flops / (kmix * 1e-3) / 1e12,
+    roof = {"bound": "mfma", "kernel": "k_mm_bf16x3 (template product)", "achieved": flops / (kmix * 1e-3) / 1e12,
             "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": flops / (kmix * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
             "algorithmic_per_launch": flops}
     result = {"workload": f"IMF per model: {n} models, 107 ages x 22 Z x 1500 points, 10000 templates, 5994 px, "
@@ -217,7 +217,7 @@ def adjoint_workload(args, n=8192):
     kfull, nsp = grid.nt * grid.nz, ((grid.nspec + 63) // 64) * 64
     flops = 2.0 * kfull * grid.nspec * 2 * n
     tk = res["mixed"]["kernel_ms"]["rev_product"]
-    roof = {"bound": "mfma", "kernel": "k_gemmt_bf16x3 (w_bar = basis^T y_bar)", "achieved": flops / (tk * 1e-3) / 1e12,
+    roof = {"bound": "mfma", "kernel": "k_mm_bf16x3 (w_bar = basis^T y_bar)", "achieved": flops / (tk * 1e-3) / 1e12,
             "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": flops / (tk * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
             "algorithmic_per_launch": flops}
     result = {"workload": f"batch adjoint: {n} models, weights over 107 ages x 22 Z (log-normal SFH x enrichment "
@@ -396,7 +396,7 @@ def main():
         flops = 2.0 * ns * kpair * n            # useful MACs (the split-BF16 form issues 3 bf16 products each)
         achieved = flops / (kms_mix[dom] * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None, "kernel": "k_gemm_bf16x3",
+                "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None, "kernel": "k_mm_bf16x3",
                 "algorithmic_per_launch": flops, "issued_bf16_tflops": 3 * achieved}
     else:
         # chain: read the age sums (float, nspec per model), write the spectrum (float) and the magnitudes (double),

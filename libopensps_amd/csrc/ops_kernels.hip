@@ -24,18 +24,31 @@ struct SfhP {
     int type;
 };
 
+// The series' term ratios k / (k + 1)^2, k = 0 .. 1000 (correctly rounded at compile time): the series below
+// multiplies by them where the reference divides by (k + 1)^2, the same terms to an ulp and no FP64 division
+// in the loop (its index is uniform, so the ratio is a scalar operand)
+#define OPS_EXPI_KMAX 1000
+struct ExpiRatio {
+    double v[OPS_EXPI_KMAX + 1];
+};
+constexpr ExpiRatio make_expi_ratio() {
+    ExpiRatio t{};
+    for (int k = 0; k <= OPS_EXPI_KMAX; k++) t.v[k] = (double)k / (((double)k + 1.0) * ((double)k + 1.0));
+    return t;
+}
+__constant__ ExpiRatio c_expi = make_expi_ratio();
+
 __device__ double d_expi(double x) {   // opensps_special_functions.f90:50-122
     const double gam = 0.57721566490153286060651209008240243104215933593992;
     if (x == 0.0) return -INFINITY;
     if (x < 0.0) return NAN;
     double res = 1.0, r = 1.0;
     if (fabs(x) <= 40.0) {
-        for (int k = 1; k <= 1000; k++) {
-            double td = (double)k + 1.0;
-            r = (r * (double)k * x) / (td * td);
+        for (int k = 1; k <= OPS_EXPI_KMAX; k++) {
+            r = (r * x) * c_expi.v[k];
             res = res + r;
             if (fabs(r) <= 1.0e-20 * fabs(res)) break;
-            if (k == 1000) return NAN;
+            if (k == OPS_EXPI_KMAX) return NAN;
         }
         return gam + log(x) + x * res;
     }

@@ -3,7 +3,8 @@
 Two reports, written as short Markdown files:
 
 * ``stats``: the per-kernel table of a ``rocprofv3 --kernel-trace --stats`` run (its SQLite results file):
-  calls, total and average duration.
+  calls, total and average duration; with ``--by-launch`` one row per kernel and launch shape (a kernel that
+  serves several workloads of the run, such as the products, then shows each apart).
 * ``traffic``: the HBM bytes per launch of one kernel from two counter runs, ``--pmc FETCH_SIZE`` and
   ``--pmc WRITE_SIZE`` (CSV output, KiB per dispatch). Each counter is scaled by what it reads for a plain
   device-to-device copy of known size in the same run (the last two ``__amd_rocclr_copyBuffer`` dispatches,
@@ -35,6 +36,19 @@ def kernel_stats(db_path):
     finally:
         con.close()
     return rows
+
+
+def kernel_stats_by_launch(db_path):
+    """[(name, grid (x, y, z), workgroup, calls, total_us, avg_us, vgprs, lds bytes)] per kernel and launch shape
+    (one workload a shape) from the dispatch records of a --kernel-trace run; durations in ns in the database."""
+    con = sqlite3.connect(db_path)
+    try:
+        rows = con.execute("select name, grid_x, grid_y, grid_z, workgroup_x, count(*), sum(duration), avg(duration), "
+                           "max(vgpr_count + accum_vgpr_count), max(lds_size) from kernels "
+                           "group by name, grid_x, grid_y, grid_z, workgroup_x order by sum(duration) desc").fetchall()
+    finally:
+        con.close()
+    return [(r[0], (r[1], r[2], r[3]), r[4], r[5], r[6] / 1e3, r[7] / 1e3, r[8], r[9]) for r in rows]
 
 
 def pmc_values(csv_path, counter):
@@ -75,11 +89,21 @@ def traffic(fetch_csv, write_csv, kernel, copy_bytes):
 
 
 def _write_stats(args):
-    rows = kernel_stats(args.db)
-    lines = [f"# {args.title}", "", "Command:", "", "```", args.command, "```", "",
-             "| kernel | calls | total (us) | average (us) | % |", "|---|---|---|---|---|"]
-    for name, calls, tot, avg, pct in rows[:args.top]:
-        lines.append(f"| `{name[:90]}` | {calls} | {tot:.1f} | {avg:.1f} | {pct:.1f} |")
+    lines = [f"# {args.title}", "", "Command:", "", "```", args.command, "```", ""]
+    if args.by_launch:
+        rows = kernel_stats_by_launch(args.db)
+        grand = sum(r[4] for r in rows) or 1.0
+        lines += ["Per kernel and launch shape (grid in work-items; each shape is one workload of the run).", "",
+                  "| kernel | grid | workgroup | calls | total (us) | average (us) | % | VGPRs | LDS (B) |",
+                  "|---|---|---|---|---|---|---|---|---|"]
+        for name, grid, wg, calls, tot, avg, vg, lds in rows[:args.top]:
+            g = "x".join(str(v) for v in grid if v != 1) or "1"
+            lines.append(f"| `{name[:80]}` | {g} | {wg} | {calls} | {tot:.1f} | {avg:.1f} | {100 * tot / grand:.1f} | "
+                         f"{vg} | {lds} |")
+    else:
+        lines += ["| kernel | calls | total (us) | average (us) | % |", "|---|---|---|---|---|"]
+        for name, calls, tot, avg, pct in kernel_stats(args.db)[:args.top]:
+            lines.append(f"| `{name[:90]}` | {calls} | {tot:.1f} | {avg:.1f} | {pct:.1f} |")
     with open(args.out, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -118,6 +142,7 @@ def main(argv=None):
     s.add_argument("--title", default="rocprofv3 kernel statistics")
     s.add_argument("--command", required=True)
     s.add_argument("--top", type=int, default=12)
+    s.add_argument("--by-launch", action="store_true", help="one row per kernel and launch shape")
     t = sub.add_parser("traffic")
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)
