@@ -53,10 +53,20 @@ struct EmuGemmArgs {
     int ksplit;             // > 1: blockIdx.z takes K / ksplit of the sum and adds it to Cf atomically (no epilogue)
 };
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-__device__ __forceinline__ float gelu_grad(float x) {
-    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * __expf(-0.5f * x * x);
+// GELU = x Phi(x), Phi(x) = (1 + erf(x / sqrt 2)) / 2 with erf by Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7,
+// far below the bf16 activations' resolution): branch-free, one reciprocal and one exponential. Training and
+// inference share it; its derivative Phi(x) + x phi(x).
+__device__ __forceinline__ float emu_phi(float x) {
+    const float t = __builtin_amdgcn_rcpf(fmaf(fabsf(x), 0.3275911f * 0.70710678118654752f, 1.0f));
+    float p = fmaf(t, 0.5f * 1.061405429f, 0.5f * -1.453152027f);   // (a1 t + ... + a5 t^5) / 2
+    p = fmaf(t, p, 0.5f * 1.421413741f);
+    p = fmaf(t, p, 0.5f * -0.284496736f);
+    p = fmaf(t, p, 0.5f * 0.254829592f);
+    const float q = p * t * __expf(-0.5f * x * x);
+    return x >= 0.0f ? 1.0f - q : q;
 }
+__device__ __forceinline__ float gelu(float x) { return x * emu_phi(x); }
+__device__ __forceinline__ float gelu_grad(float x) { return emu_phi(x) + x * 0.39894228040143268f * __expf(-0.5f * x * x); }
 
 __global__ __launch_bounds__(256) void k_emu_gemm(EmuGemmArgs a) {
     __shared__ __align__(16) uint16_t As[2][EM_T * EM_LDR];
@@ -164,15 +174,114 @@ __global__ __launch_bounds__(256) void k_emu_gemm(EmuGemmArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// k_emu_fused: inference through every layer in one kernel. A workgroup (1024 threads, 16 wavefronts) takes 128
-// samples; their activations (128 x hidden bf16) stay in LDS from layer to layer. Wavefront w computes the
-// 16-column blocks w and w + 16 of the layer's outputs for all 128 rows (8 x 2 blocks of the 16x16x32 instruction)
-// from the activations in LDS (the A operand) and the layer's weights from L2 (the B operand: each fragment serves
-// the 8 row blocks, which halves the weights' L2 traffic against 64-row tiles; the next K step's fragments are
-// loaded during this step's products). The slices go back in place (bias, GELU, bf16) after a barrier; the last layer
-// writes the scaled outputs. hidden <= 512.
+// k_emu_fused: inference through every layer in one kernel. A workgroup of W wavefronts (16 for hidden 256 and 512,
+// else 8) takes 128 samples; their activations (128 x hidden bf16) stay in LDS from layer to layer. Wavefront w owns
+// the columns [w hidden / W, (w + 1) hidden / W) of a hidden layer's output ([w 256 / W, (w + 1) 256 / W) in the last
+// layer of 256) for all 128 rows: 8 row blocks x (columns / 16) blocks of the 16x16x32 instruction, the A operand from
+// LDS, the weights (B) from L2 two K steps ahead (global-space loads, so that waits for the A fragments do not wait
+// for them), each weight fragment serving 8 row blocks; the accumulators start from the bias. A layer's first two
+// weight steps are in flight during the previous layer's epilogue (GELU, bf16 back in place; the last layer: FP32
+// rows of 256 in place). hidden <= 512.
 // ------------------------------------------------------------------------------------------------
 #define EF_ROWS 128
+
+// The layers' weights and biases come from a device table of pointers, which the compiler takes for flat
+// (generic) addresses: flat loads also count against the LDS wait counter, so every wait for an A fragment would
+// wait for the weights in flight too. Global-space views of them keep the two counters apart.
+typedef const __attribute__((address_space(1))) uint16_t* gw_ptr;
+typedef const __attribute__((address_space(1))) float* gf_ptr;
+
+template <int NC>
+struct EfPre {   // the weight fragments of two K steps
+    bf16x8_t w[2][NC];
+};
+
+template <int NC, int K>
+__device__ __forceinline__ void ef_prefetch(EfPre<4>& p, gw_ptr W, int col0) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int j = 0; j < NC; j++)
+            p.w[s][j] = *(const __attribute__((address_space(1))) bf16x8_t*)(
+                W + (unsigned)((col0 + j * 16 + (lane & 15)) * K + 8 * (lane >> 4) + 32 * s));
+}
+
+// the products of a layer: acc[i][j] = rows 16 i.., columns col0 + 16 j.. (NC blocks), NKS K steps, from the bias
+template <int LDR, int NC, int NKS>
+__device__ __forceinline__ void ef_mm(const uint16_t* act, gw_ptr W, gf_ptr bias, int col0, EfPre<4>& p,
+                                      f32x4_t (&acc)[8][4]) {
+    constexpr int K = 32 * NKS;
+    const int lane = threadIdx.x & 63;
+    unsigned wo[NC];
+#pragma unroll
+    for (int j = 0; j < NC; j++) wo[j] = (unsigned)((col0 + j * 16 + (lane & 15)) * K + 8 * (lane >> 4));
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        const float bc = bias[col0 + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc[i][j] = (f32x4_t){bc, bc, bc, bc};
+    }
+    const uint16_t* ap = act + (lane & 15) * LDR + 8 * (lane >> 4);
+    static_assert(NKS % 2 == 0, "K steps in pairs");
+#pragma unroll 1
+    for (int k2 = 0; k2 < NKS; k2 += 2) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {   // step ks = k2 + s uses the fragments of slot s, then refills it for ks + 2
+            const int ks = k2 + s;
+            bf16x8_t b[NC];
+#pragma unroll
+            for (int j = 0; j < NC; j++) b[j] = p.w[s][j];
+            if (ks + 2 < NKS) {
+#pragma unroll
+                for (int j = 0; j < NC; j++)
+                    p.w[s][j] = *(const __attribute__((address_space(1))) bf16x8_t*)(W + (wo[j] + (ks + 2) * 32));
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++) {   // the row blocks four at a time (their A fragments)
+                bf16x8_t av[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) av[i] = *(const bf16x8_t*)(ap + (h * 4 + i) * 16 * LDR + ks * 32);
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < NC; j++)
+                        acc[h * 4 + i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], b[j], acc[h * 4 + i][j], 0, 0, 0);
+            }
+        }
+    }
+}
+
+// a hidden layer's epilogue: GELU, bf16 into the activations (lane l holds rows 4 (l / 16) + e of each block at
+// column l % 16)
+template <int LDR, int NC>
+__device__ __forceinline__ void ef_epi_hidden(uint16_t* act, int col0, f32x4_t (&acc)[8][4]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        uint16_t* o = act + 4 * (lane >> 4) * LDR + col0 + j * 16 + (lane & 15);
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) o[(i * 16 + e) * LDR] = bf16_rn(gelu(acc[i][j][e]));
+    }
+}
+
+// the last layer's epilogue: FP32 rows of 256 in place
+template <int NC>
+__device__ __forceinline__ void ef_epi_last(uint16_t* act, int col0, f32x4_t (&acc)[8][4]) {
+    const int lane = threadIdx.x & 63;
+    float* fo = (float*)act;
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        const int c = col0 + j * 16 + (lane & 15);
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) fo[(i * 16 + 4 * (lane >> 4) + e) * 256 + c] = acc[i][j][e];
+    }
+}
+
 struct EmuFusedArgs {
     int n, n_in, kin, hidden, layers, noutp, n_out, ldrow;
     const float *x, *imu, *isd, *omu, *osd;
@@ -181,88 +290,55 @@ struct EmuFusedArgs {
     float* y;
 };
 
-template <int LDR>   // the LDS row (bf16)
-__device__ __forceinline__ void fused_layer(const EmuFusedArgs& a, uint16_t* act, int K, int N, const uint16_t* W,
-                                            const float* bias, bool last) {
-    // wavefront w owns the 16-column blocks w and w + 16 (those below N / 16) for all 128 rows: each weight
-    // fragment from L2 serves 8 row blocks
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int nblk = N / 16;
-    const bool on0 = wv < nblk, on1 = wv + 16 < nblk;
-    if (!on0) {
-        __syncthreads();
-        __syncthreads();
-        return;
-    }
-    f32x4_t acc[8][2];
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        acc[i][0] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
-        acc[i][1] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
-    }
-    const int kq = 8 * (lane >> 4);
-    const uint16_t* wp0 = W + (size_t)(wv * 16 + (lane & 15)) * K + kq;
-    const uint16_t* wp1 = W + (size_t)((on1 ? wv + 16 : wv) * 16 + (lane & 15)) * K + kq;
-    const uint16_t* ap = act + (lane & 15) * LDR + kq;   // row block i: + 16 i LDR
-    bf16x8_t n0 = *(const bf16x8_t*)(wp0), n1 = *(const bf16x8_t*)(wp1);
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        const bf16x8_t b0 = n0, b1 = n1;
-        if (k0 + 32 < K) {   // the next K step's weights, in flight during this step's products
-            n0 = *(const bf16x8_t*)(wp0 + k0 + 32);
-            n1 = *(const bf16x8_t*)(wp1 + k0 + 32);
-        }
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const bf16x8_t av = *(const bf16x8_t*)(ap + i * 16 * LDR + k0);
-            acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b0, acc[i][0], 0, 0, 0);
-            if (on1) acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b1, acc[i][1], 0, 0, 0);
-        }
-    }
-    __syncthreads();   // every wavefront has read the layer's input
-    // lane l holds rows 4 (l / 16) + e of each block, column l % 16; the last layer leaves FP32 rows of 256 in LDS
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-        if (j == 1 && !on1) break;
-        const int c = (wv + 16 * j) * 16 + (lane & 15);
-        const float bc = bias[c];
-        uint16_t* ao = act + 4 * (lane >> 4) * LDR + c;
-        float* fo = (float*)act + 4 * (lane >> 4) * 256 + c;
-#pragma unroll
-        for (int i = 0; i < 8; i++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const float v = acc[i][j][e] + bc;
-                if (last)
-                    fo[(i * 16 + e) * 256] = v;
-                else
-                    ao[(i * 16 + e) * LDR] = bf16_rn(gelu(v));
-            }
-    }
-    __syncthreads();   // the layer's output is in place
-}
+#define EF_KIN 64   // the first layer's K (the emulator's padded inputs)
+#ifndef EF_W16
+#define EF_W16 16
+#endif
 
-template <int HID>
-__global__ __launch_bounds__(1024) void k_emu_fused(EmuFusedArgs a) {
+template <int HID, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_emu_fused(EmuFusedArgs a) {
     constexpr int LDR = HID + 8;
+    constexpr int NCH = HID / (16 * WAVES);   // column blocks of a wavefront in a hidden layer
+    constexpr int NCL = 256 / (16 * WAVES);   // and in the last layer
+    constexpr int NKH = HID / 32;    // K steps of a hidden layer
+    constexpr int NK0 = EF_KIN / 32;
     extern __shared__ __align__(16) uint16_t act[];
-    const int row0 = blockIdx.x * EF_ROWS;
-    for (int e = threadIdx.x; e < EF_ROWS * a.kin; e += blockDim.x) {
-        const int r = e / a.kin, k = e % a.kin;
+    const int row0 = blockIdx.x * EF_ROWS, wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int e = threadIdx.x; e < EF_ROWS * EF_KIN; e += blockDim.x) {
+        const int r = e / EF_KIN, k = e % EF_KIN;
         float v = 0.0f;
         if (row0 + r < a.n && k < a.n_in) v = (a.x[(size_t)(row0 + r) * a.n_in + k] - a.imu[k]) / a.isd[k];
         act[r * LDR + k] = bf16_rn(v);
     }
+    const int ch = wv * (HID / WAVES), cl = wv * (256 / WAVES);   // the wavefront's first column: hidden, last layer
+    const int L = a.layers;
+    f32x4_t acc[8][4];
+    EfPre<4> ph;   // (the first NCH, or 2 in the last layer, of 4)
+    ef_prefetch<NCH, EF_KIN>(ph, (gw_ptr)a.W[0], ch);
     __syncthreads();
-    for (int l = 0; l <= a.layers; l++) {
-        const int K = l == 0 ? a.kin : HID;
-        const bool last = l == a.layers;
-        fused_layer<LDR>(a, act, K, last ? 256 : HID, a.W[l], a.b[l], last);
+    ef_mm<LDR, NCH, NK0>(act, (gw_ptr)a.W[0], (gf_ptr)a.b[0], ch, ph, acc);
+    for (int l = 1; l < L; l++) {
+        // the layer's first weight steps, then the epilogue of layer l - 1, then the layer's products
+        ef_prefetch<NCH, HID>(ph, (gw_ptr)a.W[l], ch);
+        __syncthreads();   // every wavefront has read the layer's input
+        ef_epi_hidden<LDR, NCH>(act, ch, acc);
+        __syncthreads();   // the layer's output is in place
+        ef_mm<LDR, NCH, NKH>(act, (gw_ptr)a.W[l], (gf_ptr)a.b[l], ch, ph, acc);
     }
-    // the outputs, scaled, from the FP32 rows in LDS
+    // the last layer
+    ef_prefetch<NCL, HID>(ph, (gw_ptr)a.W[L], cl);
+    __syncthreads();
+    ef_epi_hidden<LDR, NCH>(act, ch, acc);
+    __syncthreads();
+    ef_mm<LDR, NCL, NKH>(act, (gw_ptr)a.W[L], (gf_ptr)a.b[L], cl, ph, acc);
+    __syncthreads();
+    ef_epi_last<NCL>(act, cl, acc);
+    __syncthreads();
+    // the outputs, scaled, from the FP32 rows in LDS: a wavefront per row, lanes over the columns
     const float* fo = (const float*)act;
-    for (int e = threadIdx.x; e < EF_ROWS * a.n_out; e += blockDim.x) {
-        const int r = e / a.n_out, c = e % a.n_out;
-        if (row0 + r < a.n) a.y[(size_t)(row0 + r) * a.n_out + c] = fo[r * 256 + c] * a.osd[c] + a.omu[c];
+    for (int r = wv; r < EF_ROWS; r += WAVES) {
+        if (row0 + r >= a.n) break;
+        for (int c = lane; c < a.n_out; c += 64) a.y[(size_t)(row0 + r) * a.n_out + c] = fo[r * 256 + c] * a.osd[c] + a.omu[c];
     }
 }
 
@@ -657,15 +733,17 @@ int opensps_emu_predict(opensps_emu_t* e, int n, const float* x, float* y, void*
         f.ldrow = hid + 8;
         // LDS: the activations, or the last layer's FP32 rows of 256 (whichever is larger)
         const size_t lds = std::max((size_t)EF_ROWS * f.ldrow * sizeof(uint16_t), (size_t)EF_ROWS * 256 * sizeof(float));
-        const void* kfn = hid == 512 ? (const void*)k_emu_fused<512>
-                        : hid == 384 ? (const void*)k_emu_fused<384>
-                        : hid == 256 ? (const void*)k_emu_fused<256> : (const void*)k_emu_fused<128>;
+        // wavefronts a workgroup: 16 where every wavefront keeps a whole column block in each layer, else 8
+        constexpr int W16 = EF_W16;
+        const void* kfn = hid == 512 ? (const void*)k_emu_fused<512, W16>
+                        : hid == 384 ? (const void*)k_emu_fused<384, 8>
+                        : hid == 256 ? (const void*)k_emu_fused<256, W16> : (const void*)k_emu_fused<128, 8>;
         EMU_CHK(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         const unsigned grid = (unsigned)((n + EF_ROWS - 1) / EF_ROWS);
-        if (hid == 512) k_emu_fused<512><<<grid, 1024, lds, s>>>(f);
-        else if (hid == 384) k_emu_fused<384><<<grid, 1024, lds, s>>>(f);
-        else if (hid == 256) k_emu_fused<256><<<grid, 1024, lds, s>>>(f);
-        else k_emu_fused<128><<<grid, 1024, lds, s>>>(f);
+        if (hid == 512) k_emu_fused<512, W16><<<grid, 64 * W16, lds, s>>>(f);
+        else if (hid == 384) k_emu_fused<384, 8><<<grid, 512, lds, s>>>(f);
+        else if (hid == 256) k_emu_fused<256, W16><<<grid, 64 * W16, lds, s>>>(f);
+        else k_emu_fused<128, 8><<<grid, 512, lds, s>>>(f);
         EMU_CHK(hipGetLastError());
         return 0;
     }

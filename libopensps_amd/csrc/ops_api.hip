@@ -143,7 +143,7 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
     if (!c) return 1;
     (void)hipStreamSynchronize(c->stream);
     void* ps[] = {c->d_lam, c->d_loglam, c->d_ltab, c->d_wl, c->d_dlam, c->d_time, c->d_logz, c->d_bmass, c->d_blbol,
-                  c->d_igm_a, c->d_basis64, c->d_A64, c->d_Ahi, c->d_Alo, c->d_bmeta, c->d_bc64,
+                  c->d_igm_a, c->d_igm64s, c->d_igm32s, c->d_basis64, c->d_A64, c->d_Ahi, c->d_Alo, c->d_bmeta, c->d_bc64,
                   c->d_bc32, c->d_mask, c->d_cz, c->d_cdl, c->yo.p, c->wcol.p, c->w64.p, c->order.p, c->tiles.p,
                   c->misc.p, c->spec.p, c->flag.p, c->d_rdlam, c->d_ltab32, c->d_wl32, c->d_sm32, c->d_bkt, c->hdr.p, c->wsfh.p,
                   c->opu.p, c->opwc.p, c->opsum.p, c->opseg.p, c->opimf.p, c->opspart.p, c->d_bp_ptr, c->d_bp_band,
@@ -259,6 +259,18 @@ int opensps_hip_set_grid(opensps_hip_ctx_t* c, int nspec, const double* lam, int
     UPLOAD(c, c->d_bmass, mass, (size_t)nt * nz);
     UPLOAD(c, c->d_blbol, lbol, (size_t)nt * nz);
     UPLOAD(c, c->d_igm_a, igm.data(), igm.size());
+    {
+        const size_t jn = jtop ? jtop : 1;
+        std::vector<double> i64(6 * jn, 0.0);
+        std::vector<float> i32(6 * jn, 0.0f);
+        for (int j = 0; j < jtop; j++)
+            for (int k = 0; k < 6; k++) {
+                i64[k * jn + j] = igm[(size_t)6 * j + k];
+                i32[k * jn + j] = (float)igm[(size_t)6 * j + k];
+            }
+        UPLOAD(c, c->d_igm64s, i64.data(), i64.size());
+        UPLOAD(c, c->d_igm32s, i32.data(), i32.size());
+    }
     UPLOAD(c, c->d_basis64, spec, (size_t)nspec * nt * nz);
     free_full(c);   // (the operands over every (age, Z) are formed again at their first use)
     // Both packed operands now (the first batch of either precision then starts at once)
@@ -1049,7 +1061,8 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ca.ltab = c->d_ltab;
     ca.wl = c->d_wl;
     ca.dlam = c->d_dlam;
-    ca.igm_a = c->d_igm_a;
+    ca.igm64s = c->d_igm64s;
+    ca.igm32s = c->d_igm32s;
     ca.bmeta = c->d_bmeta;
     ca.mask = c->d_mask;
     ca.bc64 = c->d_bc64;

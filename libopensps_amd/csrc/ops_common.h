@@ -80,11 +80,8 @@ __device__ __forceinline__ float exp_kp(double x) {
     return ldexpf(__expf(r), (int)k);
 }
 
-__device__ __forceinline__ uint16_t bf16_rn(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// FP32 -> bf16, round to nearest even (gfx950's v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint16_t bf16_rn(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
 // Split an FP64 value into bf16 hi + lo (x ~ hi + lo to ~2^-16 relative).

@@ -42,6 +42,8 @@ struct OpsCtx {
     int* d_bkt = nullptr;         // per bucket: the number of pixels with ln(lam) below the bucket's start
     int jtop = 0;                 // pixels below Lyman-alpha
     double* d_igm_a = nullptr;    // (jtop, 6) IGM wavelength factors
+    double* d_igm64s = nullptr;   // the same term-major (6, jtop): the chain's pixel-parallel reads
+    float* d_igm32s = nullptr;    // and in FP32 (the FP32 chain)
     double* d_basis64 = nullptr;  // nspec x nt x nz FP64 (for the packing)
     double* d_A64 = nullptr;      // packed f64 operands [pair][pxblk][kstep][lane]
     uint16_t *d_Ahi = nullptr, *d_Alo = nullptr;   // packed bf16 hi / lo [pair][pxblk][kstep][lane][8]

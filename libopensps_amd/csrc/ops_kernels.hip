@@ -681,16 +681,21 @@ template <typename T> struct Quad;
 template <> struct Quad<float> { typedef f32x4_t type; };
 template <> struct Quad<double> { typedef f64x4_t type; };
 
-// The velocity-smoothed value of pixel i (the window and weights of build_velocity_kernel)
-template <typename T>
-__device__ __forceinline__ T smooth_px(const ChainArgs& a, const T* __restrict__ s0, int i, double inv2, double fblue,
-                                       double fred, T out) {
-    const int nspec = a.nspec;
+// The velocity kernel's window [il, ih] of pixel i (build_velocity_kernel's edges): from i down to the last pixel
+// at or below lambda_i exp(-4 sigma), up to the first at or above lambda_i exp(4 sigma)
+__device__ __forceinline__ int2 chain_window(const ChainArgs& a, int i, double fblue, double fred) {
     const double li = a.lam[i];
     const double vl = li * fblue, vh = li * fred;
     int il = i, ih = i;
     while (il > 0 && a.lam[il] > vl) il--;
-    while (ih < nspec - 1 && a.lam[ih] < vh) ih++;
+    while (ih < a.nspec - 1 && a.lam[ih] < vh) ih++;
+    return make_int2(il, ih);
+}
+
+// The velocity-smoothed value of pixel i over its window w (the weights of build_velocity_kernel)
+template <typename T>
+__device__ __forceinline__ T smooth_px(const ChainArgs& a, const T* __restrict__ s0, int i, int2 w, double inv2, T out) {
+    const int il = w.x, ih = w.y;
     if (ih - il + 1 < 2) return out;
     const double lli = a.loglam[i];
     if constexpr (std::is_same<T, float>::value) {
@@ -721,33 +726,57 @@ __device__ __forceinline__ T smooth_px(const ChainArgs& a, const T* __restrict__
     return out;
 }
 
-// The observed-frame quad q (pixels 4q .. 4q + 3, times 1 / lambda; 0 past the grid) from the rest-frame
-// spectrum src: the first pixel finds its interval by the bucket lookup, the next three walk on from it
+// The observed frame's source interval of the band quad q (pixels 4q .. 4q + 3 at lambda / (1 + z)): per pixel the
+// interval il (-1 past the grid) and the fraction t. The first pixel finds its interval by the bucket lookup, the
+// next three walk on from it. Grid and redshift only: formed before the spectrum is.
 template <typename T>
-__device__ __forceinline__ typename Quad<T>::type obs_quad(const ChainArgs& a, const T* __restrict__ src, int q,
-                                                           bool rest, double rz, double lnopz) {
+struct ObsIdx {
+    int il[4];
+    T t[4];
+};
+
+template <typename T>
+__device__ __forceinline__ ObsIdx<T> obs_index(const ChainArgs& a, int q, double rz, double lnopz) {
     const int nspec = a.nspec;
     const LamLookup& L = a.lk;
-    typename Quad<T>::type out;
+    ObsIdx<T> r;
     int c = -1;
 #pragma unroll
     for (int e = 0; e < 4; e++) {
         const int j = 4 * q + e;
-        T v = (T)0;
+        r.il[e] = -1;
+        r.t[e] = (T)0;
         if (j < nspec) {
+            const double tg = a.lam[j] * rz;
+            if (c < 0) {
+                c = lam_count<false>(L.nspec, L.nbk, L.lnl0, L.bkinv, L.lam, L.bkt, tg, a.loglam[j] - lnopz);
+            } else {
+                while (c < nspec && a.lam[c] <= tg) c++;
+            }
+            const int il = max(1, min(nspec - 1, c)) - 1;
+            r.il[e] = il;
+            r.t[e] = (T)((tg - a.lam[il]) * a.rdlam[il]);
+        }
+    }
+    return r;
+}
+
+// The observed-frame quad q (times 1 / lambda; 0 past the grid) from the rest-frame spectrum src and its intervals
+template <typename T>
+__device__ __forceinline__ typename Quad<T>::type obs_quad(const ChainArgs& a, const T* __restrict__ src, int q,
+                                                           bool rest, const ObsIdx<T>& x) {
+    typename Quad<T>::type out;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int j = 4 * q + e;
+        T v = (T)0;
+        if (j < a.nspec) {
             if (rest) {
                 v = src[j];
             } else {
-                const double tg = a.lam[j] * rz;
-                if (c < 0) {
-                    c = lam_count<false>(L.nspec, L.nbk, L.lnl0, L.bkinv, L.lam, L.bkt, tg, a.loglam[j] - lnopz);
-                } else {
-                    while (c < nspec && a.lam[c] <= tg) c++;
-                }
-                const int il = max(1, min(nspec - 1, c)) - 1;
-                const double t = (tg - a.lam[il]) * a.rdlam[il];
+                const int il = x.il[e];
                 const T lo = src[il], hi = src[il + 1];
-                v = lo + (T)t * (hi - lo);
+                v = lo + x.t[e] * (hi - lo);
             }
             if (v != v) v = (T)0;
             if constexpr (std::is_same<T, float>::value)
@@ -764,10 +793,16 @@ __device__ __forceinline__ typename Quad<T>::type obs_quad(const ChainArgs& a, c
 // registers for a barrier and then overwrite it (windows wider than OPS_CHAIN_KS pixels a thread go through
 // the spectrum out, or a staging row, in global memory). Otherwise two buffers. The host takes ONE when the
 // band quads fit OPS_CHAIN_KQ a thread.
+// Latency: the chain's table lookups do not depend on the spectrum, so each thread starts them early: the windows
+// of its first OPS_CHAIN_KW smoothed pixels and the IGM transmissions of its pixels below Lyman-alpha
+// (pixel-parallel, applied after the attenuation's barrier) before the attenuation's loads, the source intervals of
+// its first observed-frame quad before the smoothing; the attenuation fetches KA quads a thread before it computes,
+// and the band sums fetch four bands a group at a time.
 template <typename T, bool ONE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value ? 8 : 6)))
 void k_chain(ChainArgs a) {
     constexpr bool F32 = std::is_same<T, float>::value;
+    constexpr int KA = F32 ? 2 : 1;
     typedef typename Quad<T>::type Q;
     extern __shared__ __align__(32) unsigned char lds_raw[];
     T* s0 = (T*)lds_raw;
@@ -787,79 +822,151 @@ void k_chain(ChainArgs a) {
     const bool no_old = fabs(dust3) <= OPS_SAFE_FLOOR;
     const bool no_diffuse_mix = 1.0 - fnd <= OPS_SAFE_FLOOR;
     const bool igm_on = a.add_igm && z > OPS_SAFE_FLOOR && a.jtop > 0;
+    const double inv2 = h[OPS_HDR_INV2], fblue = h[OPS_HDR_FBLUE], fred = h[OPS_HDR_FRED];
+    const int w0 = (int)h[OPS_HDR_W0], w1 = (int)h[OPS_HDR_W1];
+    const bool rest = fabs(z) <= OPS_SAFE_FLOOR;
+    const double rz = h[OPS_HDR_RZ], lnopz = h[OPS_HDR_LNOPZ];
+    const int qb0 = a.band_j0 >> 2, qb1 = (a.band_j1 + 1) >> 2;   // band quads [qb0, qb1)
 
-    // ---- phase 1: attenuation and IGM -> s0 (padded pixels: the zero age sums) ----
-    const Q* y4 = (const Q*)((const T*)a.yo + (size_t)m * a.nsp);
-    const Q* o4 = a.parts == 2 ? (const Q*)((const T*)a.yo + ((size_t)n + m) * a.nsp) : nullptr;
-    const int jt = igm_on ? a.jtop : 0;
-    double b6[6];
+    // ---- the lookups that need no spectrum (the FP64 chain, short of registers, does its lookups where it uses
+    // their results) ----
+    constexpr bool EARLY = F32;
+    int2 win[OPS_CHAIN_KW];
 #pragma unroll
-    for (int k = 0; k < 6; k++) b6[k] = h[OPS_HDR_B6 + k];
+    for (int k = 0; k < OPS_CHAIN_KW; k++) {
+        const int i = w0 + tid + k * nth;
+        win[k] = EARLY && i < w1 ? chain_window(a, i, fblue, fred) : make_int2(0, 0);
+    }
+    // IGM transmissions of pixels tid + k nth below jt (pixels i < jcap take the capped depth)
+    const int jt = igm_on ? a.jtop : 0;
     const double tmax = h[OPS_HDR_TMAX];
     const int jmax = (int)h[OPS_HDR_JMAX];
-    const int jcap = jmax > 1 ? jmax : 0;   // pixels i < jcap take the capped depth
-    for (int q = tid; q < nq; q += nth) {
-        const Q y = y4[q];
-        Q o = {};
-        if (o4) o = o4[q];
-        Q tot;
-        if constexpr (F32) {
-            const f32x4_t L = ((const f32x4_t*)a.ltab32)[q];
-            const float cf = (float)fob, cn = (float)fnd;
+    const int jcap = jmax > 1 ? jmax : 0;
+    T tig[OPS_CHAIN_KI];
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const float curve = (ed != 0.0) ? __expf((float)ed * L[e]) : 1.0f;
-                const float tr_d = __expf(-((float)dust2 * curve));
-                float tr_b = 1.0f, tr_o = 1.0f;
-                if (!no_birth) tr_b = __expf(-((float)dust1 * __expf((float)ed1 * L[e])));
-                if (!no_old) tr_o = __expf(-((float)dust3 * curve));
-                const float ssum = (y[e] * tr_b * (1.0f - cf) + y[e] * cf) + o[e] * tr_o;
-                tot[e] = no_diffuse_mix ? ssum : ssum * (tr_d * (1.0f - cn) + cn);
-            }
-        } else {
-            const f64x4_t L = ((const f64x4_t*)a.ltab)[q];
+    for (int k = 0; k < OPS_CHAIN_KI; k++) tig[k] = (T)1;
+    if (EARLY && jt > 0) {
+        T b6[6];
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const double curve = (ed != 0.0) ? exp(ed * L[e]) : 1.0;
-                const double tr_d = exp(-(dust2 * curve));
-                double tr_b = 1.0, tr_o = 1.0;
-                if (!no_birth) tr_b = exp(-(dust1 * exp(ed1 * L[e])));
-                if (!no_old) tr_o = exp(-(dust3 * curve));
-                const double ssum = (y[e] * tr_b * (1.0 - fob) + y[e] * fob) + o[e] * tr_o;
-                tot[e] = no_diffuse_mix ? ssum : ssum * (tr_d * (1.0 - fnd) + fnd);
-            }
-        }
-        if (4 * q < jt) {
+        for (int k = 0; k < 6; k++) b6[k] = (T)h[OPS_HDR_B6 + k];
+        const size_t jn = (size_t)a.jtop;
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int i = 4 * q + e;
-                if (i < jt) {
-                    const double tau = i < jcap ? tmax : igm_tau6(a.igm_a + (size_t)i * 6, b6);
-                    if constexpr (F32)
-                        tot[e] = tot[e] * exp_kp(-tau * igmf);
-                    else
-                        tot[e] = tot[e] * exp(-tau * igmf);
+        for (int k = 0; k < OPS_CHAIN_KI; k++) {
+            const int i = tid + k * nth;
+            if (i < jt) {
+                if constexpr (F32) {
+                    float tau = (float)tmax;
+                    if (i >= jcap) {
+                        const float* g = a.igm32s + i;
+                        tau = g[0] * b6[0] + g[jn] * b6[1] + g[2 * jn] * b6[2] + g[3 * jn] * b6[3] + g[4 * jn] * b6[4] +
+                              g[5 * jn] * b6[5];
+                    }
+                    tig[k] = __expf(-(tau * (float)igmf));
+                } else {
+                    double tau = tmax;
+                    if (i >= jcap) {
+                        const double* g = a.igm64s + i;
+                        tau = g[0] * b6[0] + g[jn] * b6[1] + g[2 * jn] * b6[2] + g[3 * jn] * b6[3] + g[4 * jn] * b6[4] +
+                              g[5 * jn] * b6[5];
+                    }
+                    tig[k] = exp(-tau * igmf);
                 }
             }
         }
-        s0q[q] = tot;
-        if (a.young || a.old) {
+    }
+
+    // ---- phase 1: attenuation -> s0 (padded pixels: the zero age sums), KA quads a thread fetched first ----
+    const Q* y4 = (const Q*)((const T*)a.yo + (size_t)m * a.nsp);
+    const Q* o4 = a.parts == 2 ? (const Q*)((const T*)a.yo + ((size_t)n + m) * a.nsp) : nullptr;
+    for (int q0 = tid; q0 < nq; q0 += KA * nth) {
+        Q y[KA], o[KA], L[KA];
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int i = 4 * q + e;
-                if (i < nspec) {
-                    if (a.young) ((T*)a.young)[(size_t)m * nspec + i] = y[e];
-                    if (a.old) ((T*)a.old)[(size_t)m * nspec + i] = o[e];
+        for (int u = 0; u < KA; u++) {
+            const int q = q0 + u * nth;
+            y[u] = (Q){};
+            o[u] = (Q){};
+            L[u] = (Q){};
+            if (q < nq) {
+                y[u] = y4[q];
+                if (o4) o[u] = o4[q];
+                if constexpr (F32)
+                    L[u] = ((const f32x4_t*)a.ltab32)[q];
+                else
+                    L[u] = ((const f64x4_t*)a.ltab)[q];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < KA; u++) {
+            const int q = q0 + u * nth;
+            if (q >= nq) break;
+            Q tot;
+            if constexpr (F32) {
+                const float cf = (float)fob, cn = (float)fnd;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const float curve = (ed != 0.0) ? __expf((float)ed * L[u][e]) : 1.0f;
+                    const float tr_d = __expf(-((float)dust2 * curve));
+                    float tr_b = 1.0f, tr_o = 1.0f;
+                    if (!no_birth) tr_b = __expf(-((float)dust1 * __expf((float)ed1 * L[u][e])));
+                    if (!no_old) tr_o = __expf(-((float)dust3 * curve));
+                    const float ssum = (y[u][e] * tr_b * (1.0f - cf) + y[u][e] * cf) + o[u][e] * tr_o;
+                    tot[e] = no_diffuse_mix ? ssum : ssum * (tr_d * (1.0f - cn) + cn);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const double curve = (ed != 0.0) ? exp(ed * L[u][e]) : 1.0;
+                    const double tr_d = exp(-(dust2 * curve));
+                    double tr_b = 1.0, tr_o = 1.0;
+                    if (!no_birth) tr_b = exp(-(dust1 * exp(ed1 * L[u][e])));
+                    if (!no_old) tr_o = exp(-(dust3 * curve));
+                    const double ssum = (y[u][e] * tr_b * (1.0 - fob) + y[u][e] * fob) + o[u][e] * tr_o;
+                    tot[e] = no_diffuse_mix ? ssum : ssum * (tr_d * (1.0 - fnd) + fnd);
+                }
+            }
+            s0q[q] = tot;
+            if (a.young || a.old) {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int i = 4 * q + e;
+                    if (i < nspec) {
+                        if (a.young) ((T*)a.young)[(size_t)m * nspec + i] = y[u][e];
+                        if (a.old) ((T*)a.old)[(size_t)m * nspec + i] = o[u][e];
+                    }
                 }
             }
         }
     }
     __syncthreads();
+    // the IGM (apply_igm_z): pixels below jt times their transmissions; pixels past KI a thread (grids with more
+    // than KI x 512 pixels below Lyman-alpha), and every pixel in the FP64 chain, form theirs here
+    if (jt > 0) {
+        if constexpr (EARLY) {
+#pragma unroll
+            for (int k = 0; k < OPS_CHAIN_KI; k++) {
+                const int i = tid + k * nth;
+                if (i < jt) s0[i] *= tig[k];
+            }
+        }
+        for (int i = tid + (EARLY ? OPS_CHAIN_KI * nth : 0); i < jt; i += nth) {
+            double tau = tmax;
+            if (i >= jcap) {
+                const double* g = a.igm64s + i;
+                const size_t jn = (size_t)a.jtop;
+                tau = g[0] * h[OPS_HDR_B6] + g[jn] * h[OPS_HDR_B6 + 1] + g[2 * jn] * h[OPS_HDR_B6 + 2] +
+                      g[3 * jn] * h[OPS_HDR_B6 + 3] + g[4 * jn] * h[OPS_HDR_B6 + 4] + g[5 * jn] * h[OPS_HDR_B6 + 5];
+            }
+            s0[i] *= (T)exp(-tau * igmf);
+        }
+        __syncthreads();
+    }
+
+    // the source intervals of the first observed-frame quad (their lookups overlap the smoothing's)
+    ObsIdx<T> ox0;
+    if (EARLY && !rest && qb0 + tid < qb1) ox0 = obs_index<T>(a, qb0 + tid, rz, lnopz);
 
     // ---- phase 2: velocity smoothing (-> s1, or in place with ONE), and the spectrum out. Pixels outside the
     // smoothed range [w0, w1) keep their values; the range's pixels are spread over the threads one by one ----
-    const double inv2 = h[OPS_HDR_INV2], fblue = h[OPS_HDR_FBLUE], fred = h[OPS_HDR_FRED];
-    const int w0 = (int)h[OPS_HDR_W0], w1 = (int)h[OPS_HDR_W1];
     T* spec_out = a.spec ? (T*)a.spec + (size_t)m * nspec : nullptr;
     for (int q = tid; q < nq; q += nth) {
         const int i0 = 4 * q;
@@ -882,13 +989,19 @@ void k_chain(ChainArgs a) {
             }
         }
     }
+    static_assert(OPS_CHAIN_KW == 2, "the windows looked up early: win[0], win[1]");
+    auto window = [&](int k, int i) -> int2 {
+        if (EARLY && k == 0) return win[0];
+        if (EARLY && k == 1) return win[1];
+        return chain_window(a, i, fblue, fred);
+    };
     if constexpr (ONE) {
         if (w1 - w0 <= OPS_CHAIN_KS * nth) {
             T sv[OPS_CHAIN_KS];
 #pragma unroll
             for (int k = 0; k < OPS_CHAIN_KS; k++) {
                 const int i = w0 + tid + k * nth;
-                sv[k] = i < w1 ? smooth_px<T>(a, s0, i, inv2, fblue, fred, s0[i]) : (T)0;
+                sv[k] = i < w1 ? smooth_px<T>(a, s0, i, window(k, i), inv2, s0[i]) : (T)0;
                 if (i < w1 && spec_out) spec_out[i] = sv[k];
             }
             __syncthreads();
@@ -899,13 +1012,15 @@ void k_chain(ChainArgs a) {
             }
         } else {
             T* G = spec_out ? spec_out : (T*)a.stage + (size_t)m * nspec;
-            for (int i = w0 + tid; i < w1; i += nth) G[i] = smooth_px<T>(a, s0, i, inv2, fblue, fred, s0[i]);
+            int k = 0;
+            for (int i = w0 + tid; i < w1; i += nth, k++) G[i] = smooth_px<T>(a, s0, i, window(k, i), inv2, s0[i]);
             __syncthreads();
             for (int i = w0 + tid; i < w1; i += nth) s0[i] = G[i];
         }
     } else {
-        for (int i = w0 + tid; i < w1; i += nth) {
-            const T out = smooth_px<T>(a, s0, i, inv2, fblue, fred, s0[i]);
+        int k = 0;
+        for (int i = w0 + tid; i < w1; i += nth, k++) {
+            const T out = smooth_px<T>(a, s0, i, window(k, i), inv2, s0[i]);
             s1[i] = out;
             if (spec_out) spec_out[i] = out;
         }
@@ -913,15 +1028,16 @@ void k_chain(ChainArgs a) {
     __syncthreads();
 
     // ---- phase 3: the observed frame over the band quads (from s1) -> s0 ----
-    const bool rest = fabs(z) <= OPS_SAFE_FLOOR;
-    const double rz = h[OPS_HDR_RZ], lnopz = h[OPS_HDR_LNOPZ];
-    const int qb0 = a.band_j0 >> 2, qb1 = (a.band_j1 + 1) >> 2;   // band quads [qb0, qb1)
+    auto oidx = [&](int k, int q) -> ObsIdx<T> {
+        if (rest) return ObsIdx<T>{};
+        return EARLY && k == 0 ? ox0 : obs_index<T>(a, q, rz, lnopz);
+    };
     if constexpr (ONE) {
         Q fo[OPS_CHAIN_KQ];
 #pragma unroll
         for (int k = 0; k < OPS_CHAIN_KQ; k++) {
             const int q = qb0 + tid + k * nth;
-            if (q < qb1) fo[k] = obs_quad<T>(a, s1, q, rest, rz, lnopz);
+            if (q < qb1) fo[k] = obs_quad<T>(a, s1, q, rest, oidx(k, q));
         }
         __syncthreads();
 #pragma unroll
@@ -930,52 +1046,83 @@ void k_chain(ChainArgs a) {
             if (q < qb1) s0q[q] = fo[k];
         }
     } else {
-        for (int q = qb0 + tid; q < qb1; q += nth) s0q[q] = obs_quad<T>(a, s1, q, rest, rz, lnopz);
+        int k = 0;
+        for (int q = qb0 + tid; q < qb1; q += nth, k++) s0q[q] = obs_quad<T>(a, s1, q, rest, oidx(k, q));
     }
     __syncthreads();
 
-    // ---- phase 4: band sums (16 lanes per band, 4 pixels a lane, FP64 across the lanes) -> LDS; magnitudes, a
-    // thread per band ----
+    // ---- phase 4: band sums (16 lanes per band, 4 pixels a lane, FP64 across the lanes, four bands a group at a
+    // time) -> LDS; magnitudes, a thread per band ----
     // the sums wait in LDS after the spectrum (sband_staged: the host sized the LDS for them)
     double* sband = (double*)(lds_raw + (ONE ? 1 : 2) * (size_t)a.nsp * sizeof(T));
     const bool staged = a.sband_staged != 0;
     const double dist = h[OPS_HDR_DIST];
     const int g = tid >> 4, gl = tid & 15, ng = nth >> 4;
-    for (int b = g; b < a.nbands; b += ng) {
-        const int4 bm = a.bmeta[b];   // {coefficient offset, first pixel, pixels}: multiples of 4
-        const bool on = a.mask[b] != 0;
-        double s = 0.0;
-        if (on) {
-            if constexpr (F32) {
-                const f32x4_t* cq = (const f32x4_t*)(a.bc32 + bm.x);
-                const f32x4_t* fq = (const f32x4_t*)(s0 + bm.y);
-                float acc = 0.0f;
-                for (int q = gl; 4 * q < bm.z; q += 16) {
-                    const f32x4_t cc = cq[q], ff = fq[q];
-                    acc = fmaf(cc[0], ff[0], acc);
-                    acc = fmaf(cc[1], ff[1], acc);
-                    acc = fmaf(cc[2], ff[2], acc);
-                    acc = fmaf(cc[3], ff[3], acc);
+    constexpr int KB = F32 ? 4 : 2;
+    for (int b0 = g; b0 < a.nbands; b0 += KB * ng) {
+        int4 bm[KB];
+        bool on[KB];
+        int nit = 0;
+#pragma unroll
+        for (int u = 0; u < KB; u++) {
+            const int b = b0 + u * ng;
+            bm[u] = make_int4(0, 0, 0, 0);
+            on[u] = false;
+            if (b < a.nbands) {
+                bm[u] = a.bmeta[b];   // {coefficient offset, first pixel, pixels}: multiples of 4
+                on[u] = a.mask[b] != 0;
+            }
+            if (on[u]) nit = max(nit, (bm[u].z / 4 + 15) / 16);
+        }
+        double s[KB];
+        if constexpr (F32) {
+            float acc[KB];
+#pragma unroll
+            for (int u = 0; u < KB; u++) acc[u] = 0.0f;
+            for (int it = 0; it < nit; it++) {
+                const int q = gl + 16 * it;
+#pragma unroll
+                for (int u = 0; u < KB; u++) {
+                    if (on[u] && 4 * q < bm[u].z) {
+                        const f32x4_t cc = *(const f32x4_t*)(a.bc32 + (unsigned)(bm[u].x + 4 * q));
+                        const f32x4_t ff = *(const f32x4_t*)(s0 + (unsigned)(bm[u].y + 4 * q));
+                        acc[u] = fmaf(cc[0], ff[0], acc[u]);
+                        acc[u] = fmaf(cc[1], ff[1], acc[u]);
+                        acc[u] = fmaf(cc[2], ff[2], acc[u]);
+                        acc[u] = fmaf(cc[3], ff[3], acc[u]);
+                    }
                 }
-                s = (double)acc;
-            } else {
-                const f64x4_t* cq = (const f64x4_t*)(a.bc64 + bm.x);
-                const f64x4_t* fq = (const f64x4_t*)(s0 + bm.y);
-                for (int q = gl; 4 * q < bm.z; q += 16) {
-                    const f64x4_t cc = cq[q], ff = fq[q];
-                    s += cc[0] * ff[0] + cc[1] * ff[1] + cc[2] * ff[2] + cc[3] * ff[3];
+            }
+#pragma unroll
+            for (int u = 0; u < KB; u++) s[u] = (double)acc[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < KB; u++) s[u] = 0.0;
+            for (int it = 0; it < nit; it++) {
+                const int q = gl + 16 * it;
+#pragma unroll
+                for (int u = 0; u < KB; u++) {
+                    if (on[u] && 4 * q < bm[u].z) {
+                        const f64x4_t cc = *(const f64x4_t*)(a.bc64 + (unsigned)(bm[u].x + 4 * q));
+                        const f64x4_t ff = *(const f64x4_t*)(s0 + (unsigned)(bm[u].y + 4 * q));
+                        s[u] += cc[0] * ff[0] + cc[1] * ff[1] + cc[2] * ff[2] + cc[3] * ff[3];
+                    }
                 }
             }
         }
-        for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
-        if (gl == 0) {
-            if (staged) {
-                sband[b] = s;
-            } else {
-                double mag = NAN;
-                if (on) mag = (s > 0.0 ? -2.5 * log10(s) : NAN) - 48.60 - 2.5 * a.amzl + dist;
-                a.mags[(size_t)m * a.nbands + b] = mag;
-                if (a.fband) a.fband[(size_t)m * a.nbands + b] = s;
+#pragma unroll
+        for (int u = 0; u < KB; u++) {
+            for (int o = 8; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o, 16);
+            const int b = b0 + u * ng;
+            if (gl == 0 && b < a.nbands) {
+                if (staged) {
+                    sband[b] = s[u];
+                } else {
+                    double mag = NAN;
+                    if (on[u]) mag = (s[u] > 0.0 ? -2.5 * log10(s[u]) : NAN) - 48.60 - 2.5 * a.amzl + dist;
+                    a.mags[(size_t)m * a.nbands + b] = mag;
+                    if (a.fband) a.fband[(size_t)m * a.nbands + b] = s[u];
+                }
             }
         }
     }

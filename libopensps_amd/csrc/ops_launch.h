@@ -190,7 +190,9 @@ struct ChainArgs {
     const double* prm;
     const double* hdr;
     const void* yo;
-    const double *lam, *loglam, *ltab, *wl, *dlam, *rdlam, *igm_a;
+    const double *lam, *loglam, *ltab, *wl, *dlam, *rdlam;
+    const double* igm64s;   // the IGM wavelength factors term-major (6, jtop), FP64 and FP32
+    const float* igm32s;
     const float *ltab32, *wl32;
     const float4* sm32;
     const int4* bmeta;   // per band {coefficient offset, first pixel, pixels}, multiples of 4
@@ -260,6 +262,8 @@ __global__ void k_pack_f64(PackArgs a);
 __global__ void k_pack_bf16(PackArgs a);
 #define OPS_CHAIN_KS 4   // smoothed pixels a thread holds in registers (single-buffer chain)
 #define OPS_CHAIN_KQ 3   // observed-frame quads a thread holds in registers (single-buffer chain)
+#define OPS_CHAIN_KW 2   // smoothing windows a thread looks up before the spectrum is formed
+#define OPS_CHAIN_KI 3   // IGM transmissions a thread forms before the spectrum is (pixels tid + k 512)
 template <typename T, bool ONE>
 __global__ void k_chain(ChainArgs a);
 template <typename T>
