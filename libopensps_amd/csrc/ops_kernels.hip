@@ -44,17 +44,37 @@ __device__ double d_expi(double x) {   // opensps_special_functions.f90:50-122
     if (x < 0.0) return NAN;
     double res = 1.0, r = 1.0;
     if (fabs(x) <= 40.0) {
-        for (int k = 1; k <= OPS_EXPI_KMAX; k++) {
-            r = (r * x) * c_expi.v[k];
-            res = res + r;
-            if (fabs(r) <= 1.0e-20 * fabs(res)) break;
-            if (k == OPS_EXPI_KMAX) return NAN;
+        // four terms a block: their ratios come in one scalar load, so the loop waits once per four terms
+        bool done = false;
+        for (int k0 = 1; k0 <= OPS_EXPI_KMAX && !done; k0 += 4) {
+            double c[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) c[q] = c_expi.v[k0 + q];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                r = (r * x) * c[q];
+                res = res + r;
+                if (fabs(r) <= 1.0e-20 * fabs(res)) { done = true; break; }
+                if (k0 + q == OPS_EXPI_KMAX) return NAN;
+            }
         }
         return gam + log(x) + x * res;
     }
     double inv_x = 1.0 / x;
     for (int k = 1; k <= 20; k++) { r = r * (double)k * inv_x; res = res + r; }
     return exp(x) * inv_x * res;
+}
+
+// find_interval_ref on a strictly ascending array by the whole wavefront (uniform control flow, every lane active):
+// the number of entries <= v, counted with ballots, is the binary search's result for an ascending array without
+// repeats (0 below and for NaN, n above), with the reference's end rules
+__device__ __forceinline__ int wave_find_interval(const double* __restrict__ a, int n, double v, int lane) {
+    if (n < 2) return 1;
+    int cnt = 0;
+    for (int i0 = 0; i0 < n; i0 += 64) cnt += __popcll(__ballot(i0 + lane < n && a[i0 + lane] <= v));
+    if (v == a[0]) return 1;
+    if (v == a[n - 1]) return n - 1;
+    return cnt;
 }
 
 __device__ void d_convert_sfh(const double* p, SfhP& s) {   // convert_sfhparams_pc (opensps_sfh.f90:1305-1355)
@@ -123,8 +143,8 @@ __device__ void wave_ssp_weights(const double* __restrict__ tf, int nt, double t
     }
     double lmin = pow(10.0, tiny_logt);
     double lts = log10(fmax(s.tq, lmin)), lte = log10(fmax(s.tage, lmin));
-    int js = find_interval_ref(tf, nt, lts);
-    int je = find_interval_ref(tf, nt, lte) + 1;
+    int js = wave_find_interval(tf, nt, lts, lane);
+    int je = wave_find_interval(tf, nt, lte, lane) + 1;
     js = max(js, max(idx_min, 1));
     je = min(je, min(idx_max, nt - 1));
     // node moments M(L(j)), j = js .. je + 1 (1-based), L(j) = clamp(tf(j))
@@ -175,7 +195,7 @@ __device__ int wave_sfh_weights(const double* __restrict__ tf, int nt, double ti
     SfhP s;
     d_convert_sfh(p, s);
     int sfh = (int)p[OPS_P_SFH];
-    int imax = find_interval_ref(tf, nt, log10(s.tage)) + 2;
+    int imax = wave_find_interval(tf, nt, log10(s.tage), lane) + 2;
     imax = min(max(imax, 1), nt);
     if (sfh == 0) {
         s.type = -1;
@@ -265,11 +285,11 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
 
     // stage 1 indices: batch_model_indices (opensps_csp_batch.f90:1541-1576), metallicity_interval (:2073-2084)
     double dal = p[OPS_P_DUST_TESC] > OPS_SAFE_FLOOR ? p[OPS_P_DUST_TESC] : 7.0;
-    int it = max(1, min(find_interval_ref(a.time, nt, dal), nt));
+    int it = max(1, min(wave_find_interval(a.time, nt, dal, lane), nt));
     int zlo = 1;
     double dz = 0.0;
     if (a.nz > 1 && !a.wz) {
-        zlo = max(1, min(find_interval_ref(a.logz, a.nz, p[OPS_P_LOGZSOL]), a.nz - 1));
+        zlo = max(1, min(wave_find_interval(a.logz, a.nz, p[OPS_P_LOGZSOL], lane), a.nz - 1));
         dz = (p[OPS_P_LOGZSOL] - a.logz[zlo - 1]) / (a.logz[zlo] - a.logz[zlo - 1]);
     }
 
@@ -609,50 +629,80 @@ __device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, dou
     double b6[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) b6[k] = __shfl(pw, k, 64);
+    // the IGM depth's maximum below Lyman-alpha: two pixels a lane at a time (their loads in flight together)
     double tmax = 0.0;
     int jmax = 1;
     if (a.add_igm && z > OPS_SAFE_FLOOR && a.jtop > 0) {
-        for (int j = lane; j < a.jtop; j += 64) argmax_merge(tmax, jmax, igm_tau6(a.igm_a + (size_t)j * 6, b6), j + 1);
+        const int jt = a.jtop;
+        for (int j0 = lane; j0 < jt; j0 += 128) {
+            double t4[2];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int j = j0 + 64 * u;
+                t4[u] = j < jt ? igm_tau6(a.igm_a + (size_t)j * 6, b6) : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+                if (j0 + 64 * u < jt) argmax_merge(tmax, jmax, t4[u], j0 + 64 * u + 1);
+        }
         for (int o = 32; o > 0; o >>= 1) {
             double v2 = __shfl_xor(tmax, o, 64);
             int j2 = __shfl_xor(jmax, o, 64);
             argmax_merge(tmax, jmax, v2, j2);
         }
     }
+    // the cosmology table's interval of z: eight entries a lane at a time
     const bool asc = a.cz[a.ncosmo - 1] >= a.cz[0];
     int cnt = 0;
-    for (int q = lane; q < a.ncosmo; q += 64) cnt += (asc ? a.cz[q] <= z : z < a.cz[q]) ? 1 : 0;
+    for (int q0 = lane; q0 < a.ncosmo; q0 += 512) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int q = q0 + 64 * u;
+            if (q < a.ncosmo) cnt += (asc ? a.cz[q] <= z : z < a.cz[q]) ? 1 : 0;
+        }
+    }
     cnt = wave_sum(cnt);
     const double sig = p[OPS_P_SIGMA_SMOOTH];
     const double minl = p[OPS_P_MIN_WAVE_SMOOTH], maxl = p[OPS_P_MAX_WAVE_SMOOTH];
     const bool smooth = a.smooth_fast && sig > OPS_SAFE_FLOOR;
+    const bool window = smooth && minl <= maxl;
     const double sb = sig / (OPS_C_LIGHT * 1.0e-13);
+    double dm = 0.0;
+    const bool dist_on = fabs(z) > OPS_SAFE_FLOOR;
+    if (dist_on) {
+        const int i = max(1, min(cnt, a.ncosmo - 1));
+        const double slope = (a.cdl[i] - a.cdl[i - 1]) / (a.cz[i] - a.cz[i - 1]);
+        dm = a.cdl[i - 1] + slope * (z - a.cz[i - 1]);
+    }
+    // the logarithms and exponentials of the header in one call each over the lanes (the lanes' cases would
+    // otherwise run one after another): ln(1 + z), ln(dm / 10), ln(minl), ln(maxl) as log1p; exp(-+4 sigma)
+    const double larg = lane == OPS_HDR_LNOPZ ? z : lane == OPS_HDR_DIST ? dm / 10.0 - 1.0
+                      : lane == OPS_HDR_W0 ? minl - 1.0 : lane == OPS_HDR_W1 ? maxl - 1.0 : 0.0;
+    const double lg = log1p(larg);
+    const double ln_opz = __shfl(lg, OPS_HDR_LNOPZ, 64);
+    const double ex = exp(lane == OPS_HDR_FBLUE ? -4.0 * sb : lane == OPS_HDR_FRED ? 4.0 * sb : 0.0);
     const LamLookup& L = a.lk;
     double v = 0.0;
     switch (lane) {
     case OPS_HDR_RZ: v = 1.0 / opz; break;
-    case OPS_HDR_LNOPZ: v = log1p(z); break;
+    case OPS_HDR_LNOPZ: v = lg; break;
     case OPS_HDR_INV2: v = smooth ? 0.5 * (1.0 / sb) * (1.0 / sb) : 0.0; break;
-    case OPS_HDR_FBLUE: v = smooth ? exp(-4.0 * sb) : 1.0; break;
-    case OPS_HDR_FRED: v = smooth ? exp(4.0 * sb) : 1.0; break;
-    case OPS_HDR_DIST:
-        if (fabs(z) > OPS_SAFE_FLOOR) {
-            const int i = max(1, min(cnt, a.ncosmo - 1));
-            const double slope = (a.cdl[i] - a.cdl[i - 1]) / (a.cz[i] - a.cz[i - 1]);
-            const double dm = a.cdl[i - 1] + slope * (z - a.cz[i - 1]);
-            if (dm == dm && dm > OPS_SAFE_FLOOR) v = 5.0 * log10(dm / 10.0) + (-2.5 * log10(1.0 + z));
-        }
+    case OPS_HDR_FBLUE: v = smooth ? ex : 1.0; break;
+    case OPS_HDR_FRED: v = smooth ? ex : 1.0; break;
+    case OPS_HDR_DIST:   // 5 log10(dm / 10) - 2.5 log10(1 + z)
+        if (dist_on && dm == dm && dm > OPS_SAFE_FLOOR)
+            v = 5.0 * (lg * 0.43429448190325176) + (-2.5 * (ln_opz * 0.43429448190325176));
         break;
     case OPS_HDR_TMAX: v = tmax; break;
     case OPS_HDR_JMAX: v = (double)jmax; break;
-    case OPS_HDR_W0:   // #{lam < minl}
-        if (smooth && minl <= maxl) v = (double)lam_count<true>(L.nspec, L.nbk, L.lnl0, L.bkinv, L.lam, L.bkt, minl, log(minl));
-        break;
     case OPS_HDR_DZ: v = dz; break;
-    case OPS_HDR_W1:   // #{lam <= maxl}
-        if (smooth && minl <= maxl) v = (double)lam_count<false>(L.nspec, L.nbk, L.lnl0, L.bkinv, L.lam, L.bkt, maxl, log(maxl));
-        break;
     default: break;
+    }
+    // #{lam < minl} and #{lam <= maxl}: the bucket lookup from the logarithm, then the exact walk
+    if (window && (lane == OPS_HDR_W0 || lane == OPS_HDR_W1)) {
+        const double x = lane == OPS_HDR_W0 ? minl : maxl;
+        v = lane == OPS_HDR_W0 ? (double)lam_count<true>(L.nspec, L.nbk, L.lnl0, L.bkinv, L.lam, L.bkt, x, lg)
+                               : (double)lam_count<false>(L.nspec, L.nbk, L.lnl0, L.bkinv, L.lam, L.bkt, x, lg);
     }
 #pragma unroll
     for (int k = 0; k < 6; k++)
