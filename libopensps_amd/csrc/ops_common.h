@@ -57,6 +57,13 @@ __device__ __forceinline__ int lam_count(int nspec, int nbk, double lnl0, double
     int b = 0;
     if (u > 0.0) b = u >= (double)(nbk - 1) ? nbk - 1 : (int)u;
     int c = bkt[b];
+    // the count is the length of the prefix of entries below x (<= x); entries before 0 count, entries past the grid
+    // do not. One batch of four loads around the bucket's start settles the usual case (the count within a pixel of
+    // it); otherwise the walks.
+    auto in = [&](double v) { return STRICT ? v < x : v <= x; };
+    const bool i0 = c - 2 < 0 || in(lam[c - 2]), i1 = c - 1 < 0 || in(lam[c - 1]);
+    const bool i2 = c < nspec && in(lam[c]), i3 = c + 1 < nspec && in(lam[c + 1]);
+    if (i0 && !i3) return c - 1 + (int)i1 + (int)i2;
     if (STRICT) {
         while (c < nspec && lam[c] < x) c++;
         while (c > 0 && !(lam[c - 1] < x)) c--;

@@ -749,17 +749,31 @@ __device__ __forceinline__ T smooth_px(const ChainArgs& a, const T* __restrict__
     if (ih - il + 1 < 2) return out;
     const double lli = a.loglam[i];
     if constexpr (std::is_same<T, float>::value) {
+        // four window pixels a step, their table loads issued together; the same sums in the same order
         const float c2 = (float)inv2;
         float tw = 0.0f, acc = 0.0f;
-        for (int k = il; k <= ih; k++) {
-            const float d = (float)(lli - a.loglam[k]);
-            const float4 f = a.sm32[k];
-            float wk = __expf(-(c2 * (d * d))) * f.x;
-            wk *= (k > il ? f.y : 0.0f) + (k < ih ? f.z : 0.0f);
-            tw += wk;
-            acc += wk * s0[k];
+        for (int k0 = il; k0 <= ih; k0 += 4) {
+            double ll[4];
+            float4 f[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int k = min(k0 + q, ih);
+                ll[q] = a.loglam[k];
+                f[q] = a.sm32[k];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int k = k0 + q;
+                if (k <= ih) {
+                    const float d = (float)(lli - ll[q]);
+                    float wk = __expf(-(c2 * (d * d))) * f[q].x;
+                    wk *= (k > il ? f[q].y : 0.0f) + (k < ih ? f[q].z : 0.0f);
+                    tw += wk;
+                    acc += wk * s0[k];
+                }
+            }
         }
-        if (tw > 0.0f) out = acc / tw;
+        if (tw > 0.0f) out = acc * __builtin_amdgcn_rcpf(tw);
     } else {
         double tw = 0.0, acc = 0.0;
         for (int k = il; k <= ih; k++) {
