@@ -290,7 +290,7 @@ def emulator_workload(args, csp_models_per_s):
                        f"magnitudes; trained on {info['n_train']} spectra of the batch (PCA from {info['n_pca_fit']})",
            "value": rate, "unit": "models/s (inference, 1M batch)", "vs_csp_mixed": rate / csp_models_per_s,
            "inference_ms_per_1M": 1e3 * dt * (1 << 20) / n,
-           "roofline": {"bound": "mfma", "kernel": "k_emu_fused<512>", "achieved": flops / dt / 1e12,
+           "roofline": {"bound": "mfma", "kernel": "k_emu_fused<512, 16>", "achieved": flops / dt / 1e12,
                         "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": flops / dt / 1e12 / BF16_PEAK_TFLOPS},
            "validation_vs_fp64_batch": val,
            "fallback": {"rate": emu.fallback_rate(), "spot": 0.01, "tol_mag": 0.01, **emu.stats},
