@@ -775,8 +775,11 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     const int optm = mixed && uni ? (n * parts + OPS_MM_COLS - 1) / OPS_MM_COLS : 0;
     HIPCHK(c, hipSetDevice(c->device));
 
-    // scratch: misc = zlo, it, status, order (n ints each) | ntiles | flag
-    if ((st = ops_reserve(c, c->misc, sizeof(int) * ((size_t)4 * n + 64)))) return st;
+    // scratch: misc = zlo, it, status, order (n ints each) | ntiles, flag, tile counts (64) | pair slot counts
+    // (256 x OPS_GCNT_SLOTS) | rank (n)
+    if (c->npair > 256) return ops_fail(c, 3, "batch: more than 257 metallicities");
+    const size_t ncnt = 64 + (size_t)256 * OPS_GCNT_SLOTS;
+    if ((st = ops_reserve(c, c->misc, sizeof(int) * ((size_t)5 * n + ncnt)))) return st;
     if ((st = ops_reserve(c, c->tiles, sizeof(int4) * (size_t)(maxtiles + (uni ? optiles : 0) + maxm + optm)))) return st;
     if ((st = ops_reserve(c, c->hdr, sizeof(double) * OPS_HDR_N * (size_t)n))) return st;
     if ((st = ops_reserve(c, c->yo, tsz * (size_t)parts * n * c->nsp))) return st;
@@ -792,6 +795,8 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     int* d_optiles = misc + 4 * n + 32;
     int* d_nmtiles = misc + 4 * n + 40;
     int* d_opmtiles = misc + 4 * n + 48;
+    int* d_gcnt = misc + 4 * n + 64;
+    int* d_rank = misc + 4 * n + ncnt;
     int4* mtiles = (int4*)c->tiles.p + maxtiles + (uni ? optiles : 0);   // then the uniform ones
     const int npass = (ops && c->nz > 1) ? 2 : 1;
     if (ops) {
@@ -804,7 +809,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         if ((st = ops_reserve(c, c->opspart, (3 * sizeof(double) + sizeof(int)) * (size_t)npass * maxtiles * c->nt * 64)))
             return st;
     }
-    HIPCHK(c, hipMemsetAsync(d_ntiles, 0, 64 * sizeof(int), c->stream));
+    HIPCHK(c, hipMemsetAsync(d_ntiles, 0, ncnt * sizeof(int), c->stream));   // the counters and the pair slot counts
     c->d_last_flag = d_flag;
 
     PrepArgs pa;
@@ -833,6 +838,8 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     pa.whi = mixed ? (uint16_t*)c->wcol.p : nullptr;
     pa.wlo = mixed ? (uint16_t*)c->wcol.p + (size_t)n * parts * kpad : nullptr;
     pa.zlo = d_zlo;
+    pa.gcnt = d_gcnt;
+    pa.rank = d_rank;
     pa.it = d_it;
     pa.idx = a->idx;
     pa.status = d_status;
@@ -863,12 +870,14 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ga.npair = c->npair;
     ga.parts = gparts;
     ga.zlo = d_zlo;
+    ga.gcnt = d_gcnt;
+    ga.rank = d_rank;
     ga.order = d_order;
     ga.ntiles = d_ntiles;
     ga.tiles = (int4*)c->tiles.p;
     ga.nmtiles = mixed && !uni ? d_nmtiles : nullptr;
     ga.mtiles = mixed && !uni ? mtiles : nullptr;
-    k_group<<<1, 256, sizeof(int) * 256 * (size_t)c->npair, c->stream>>>(ga);
+    k_group<<<1, 1024, 0, c->stream>>>(ga);
     HIPCHK(c, hipGetLastError());
     if (uni && mixed) {
         k_op_tiles<<<1, 256, 0, c->stream>>>(n * parts, OPS_MM_COLS, d_opmtiles, mtiles + maxm);

@@ -372,26 +372,31 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
         }
     }
     wave_chain_header(a, m, p, dz, lane);
+    // the model's rank in its pair and slot (k_group): last, after every load of the wavefront (the vector memory
+    // counter completes in order, so an atomic issued earlier would hold up every later wait); OPS_GCNT_SLOTS
+    // counters a pair keep the atomics on one address few
+    if (lane == 0) a.rank[m] = atomicAdd(a.gcnt + (zlo - 1) * OPS_GCNT_SLOTS + (m % OPS_GCNT_SLOTS), 1);
 }
 
 #pragma clang fp contract(on)
 
 // ------------------------------------------------------------------------------------------------
-// k_group: the models sorted by metallicity pair (stable), the column tiles of each pair (OPS_TILE_COL columns,
-// and with mtiles the OPS_MM_COLS columns of the split-BF16 products)
+// k_group: the models grouped by metallicity pair (k_prep counted each pair's models in OPS_GCNT_SLOTS slots, model m
+// in slot m % OPS_GCNT_SLOTS, and ranked each model in its slot with an atomic, so the order within a pair follows
+// the slots and the order of arrival: every column of the products is formed alone, so the results do not depend
+// on it), and the column tiles of each pair (OPS_TILE_COL columns, and with mtiles the OPS_MM_COLS columns of the
+// split-BF16 products). One workgroup: the slots' and pairs' prefixes, then the scatter.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_group(GroupArgs a) {
-    extern __shared__ int cnt[];   // [256][npair]
+__global__ __launch_bounds__(1024) void k_group(GroupArgs a) {
     __shared__ int gstart[256], gtot[256], tbase[257], mbase[257];
+    __shared__ int sstart[256 * OPS_GCNT_SLOTS];   // per (pair, slot) its first position within the pair
     const int t = threadIdx.x, G = a.npair, n = a.n;
-    const int chunk = (n + 255) / 256;
-    const int m0 = t * chunk, m1 = min(n, m0 + chunk);
-    for (int g = 0; g < G; g++) cnt[t * G + g] = 0;
-    for (int m = m0; m < m1; m++) cnt[t * G + (a.zlo[m] - 1)]++;
-    __syncthreads();
-    for (int g = t; g < G; g += 256) {
+    for (int g = t; g < G; g += blockDim.x) {
         int run = 0;
-        for (int u = 0; u < 256; u++) { int v = cnt[u * G + g]; cnt[u * G + g] = run; run += v; }
+        for (int q = 0; q < OPS_GCNT_SLOTS; q++) {
+            sstart[g * OPS_GCNT_SLOTS + q] = run;
+            run += a.gcnt[g * OPS_GCNT_SLOTS + q];
+        }
         gtot[g] = run;
     }
     __syncthreads();
@@ -411,13 +416,12 @@ __global__ __launch_bounds__(256) void k_group(GroupArgs a) {
         if (a.mtiles) *a.nmtiles = mb;
     }
     __syncthreads();
-    for (int m = m0; m < m1; m++) {
-        int g = a.zlo[m] - 1;
-        int pos = gstart[g] + cnt[t * G + g]++;
-        a.order[pos] = m;
+    for (int m = t; m < n; m += blockDim.x) {
+        const int g = a.zlo[m] - 1;
+        a.order[gstart[g] + sstart[g * OPS_GCNT_SLOTS + m % OPS_GCNT_SLOTS] + a.rank[m]] = m;
     }
-    for (int g = t; g < G; g += 256) {
-        int ncol = gtot[g] * a.parts;
+    for (int g = t; g < G; g += blockDim.x) {
+        const int ncol = gtot[g] * a.parts;
         for (int i = 0; tbase[g] + i < tbase[g + 1]; i++) {
             int4 d;
             d.x = g;

@@ -37,6 +37,7 @@ struct PrepArgs {
     double *w_out, *mass, *lbol, *w64;
     uint16_t *whi, *wlo;
     int *zlo, *it, *idx, *status, *flag;
+    int *gcnt, *rank;           // per pair the models counted so far; per model its rank in its pair (k_group)
     // the chain's per-model constants
     int add_igm, jtop, ncosmo;
     LamLookup lk;
@@ -44,9 +45,11 @@ struct PrepArgs {
     double* hdr;
 };
 
+#define OPS_GCNT_SLOTS 32   // counters a metallicity pair (k_prep's ranks, k_group's prefix)
+
 struct GroupArgs {
     int n, npair, parts;
-    const int* zlo;
+    const int *zlo, *gcnt, *rank;   // from k_prep: per pair its models, per model its rank among them
     int *order, *ntiles;
     int4* tiles;
     int* nmtiles;    // the tiles of k_mm_bf16x3 (null: none)
