@@ -64,9 +64,10 @@ typedef struct {
  * given (device arrays, any NULL): mags_bar (nbands, n), spec_bar (nspec, n), mass_bar, lbol_bar (n). Outputs
  * (device, any NULL): params_bar (OPENSPS_HIP_NBGRAD, n) with the rows of the reference's BGRAD_* numbering
  * (opensps_csp_batch.f90:166-213) that this path computes (dust1, dust2, dust3, dust_index, dust1_index,
- * frac_obrun, frac_nodust, logzsol, zred, sigma_smooth, igm_factor; the others are 0), weights_bar (nt, n) (with
- * fwd.ssp_weights or SFH parameters), weights_z_bar (nt, nz, n) (with fwd.ssp_weights_z). params_bar with SFH
- * parameters (no given weights) is status 2: the SFH tangents (opensps_sfh_tangent.f90) are not built. */
+ * frac_obrun, frac_nodust, logzsol, zred, sigma_smooth, igm_factor and, with SFH parameters (no given weights),
+ * the SFH rows tage, tau, const, fburst, tburst, sf_start, sf_trunc, sf_slope of batch_sfh_gradients
+ * (opensps_csp_batch.f90:5198-5217, SFHT_ROWS :215; sfh 0, 1, 4); the others are 0), weights_bar (nt, n) (with
+ * fwd.ssp_weights or SFH parameters), weights_z_bar (nt, nz, n) (with fwd.ssp_weights_z). */
 #define OPENSPS_HIP_NBGRAD 44
 typedef struct {
     opensps_hip_batch_t fwd;
@@ -148,7 +149,8 @@ int opensps_hip_set_imf(opensps_hip_ctx_t *ctx, int imf_type, int remnants, doub
  * operator, "group" also holds the IMF weights, the fold and the mass sums, and the product is the template one. */
 int opensps_hip_set_profiling(opensps_hip_ctx_t *ctx, int on);
 int opensps_hip_kernel_times(opensps_hip_ctx_t *ctx, double *ms, int *launches, int reset);
-/* The same for the reverse pass's kernels: [reverse chain, transposed product, weights and params_bar] (ms[3],
+/* The same for the reverse pass's kernels: [reverse chain, transposed product, weights and params_bar with the
+ * SFH tangents] (ms[3],
  * launches[3]); the forward half of a batch with gradients counts in opensps_hip_kernel_times. */
 int opensps_hip_grad_kernel_times(opensps_hip_ctx_t *ctx, double *ms, int *launches, int reset);
 

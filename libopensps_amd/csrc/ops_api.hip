@@ -147,7 +147,7 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
                   c->d_bc32, c->d_mask, c->d_cz, c->d_cdl, c->yo.p, c->wcol.p, c->w64.p, c->order.p, c->tiles.p,
                   c->misc.p, c->spec.p, c->flag.p, c->d_rdlam, c->d_ltab32, c->d_wl32, c->d_sm32, c->d_bkt, c->hdr.p, c->wsfh.p,
                   c->opu.p, c->opwc.p, c->opsum.p, c->opseg.p, c->opimf.p, c->opspart.p, c->d_bp_ptr, c->d_bp_band,
-                  c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p};
+                  c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p, c->gwb.p};
     for (void* p : ps) free_ptr(p);
     free_operator(c);
     free_full(c);
@@ -1125,9 +1125,6 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     const opensps_hip_batch_t& f = g->fwd;
     const int n = f.n;
     if (f.use_ops) return ops_fail(c, 2, "batch_grad: the SSP operator's reverse pass (the IMF rows) is not built");
-    if (g->params_bar && !f.ssp_weights && !f.ssp_weights_z)
-        return ops_fail(c, 2, "batch_grad: params_bar with SFH parameters (the SFH tangents) is not built; give "
-                              "ssp_weights or ssp_weights_z");
     if (g->weights_z_bar && !f.ssp_weights_z) return ops_fail(c, 3, "batch_grad: weights_z_bar needs ssp_weights_z");
     if (n < 0) return ops_fail(c, 3, "batch_grad: n < 0");
     if (n == 0) return 0;
@@ -1146,6 +1143,10 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     if ((st = ops_reserve(c, c->gout, sizeof(double) * OPS_GOUT_N * (size_t)n))) return st;
     if ((st = ops_reserve(c, c->gyb, (mixed ? 2 * sizeof(uint16_t) : sizeof(double)) * (size_t)ncols * nsp))) return st;
     if ((st = ops_reserve(c, c->gC, (mixed ? sizeof(float) : sizeof(double)) * (size_t)ncols * ldc))) return st;
+    // the SFH rows (batch_sfh_gradients, opensps_csp_batch.f90:1163-1166: only where the weights are not given) need
+    // d L / d w per age
+    const bool sfh_rows = g->params_bar && !f.ssp_weights && !wz;
+    if (sfh_rows && !g->weights_bar && (st = ops_reserve(c, c->gwb, sizeof(double) * (size_t)n * nt))) return st;
     // the forward batch
     opensps_hip_batch_t fw = f;
     fw.split_parts = 1;
@@ -1274,10 +1275,25 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     go.C = c->gC.p;
     go.mixed = mixed;
     go.params_bar = g->params_bar;
-    go.weights_bar = g->weights_bar;
+    go.weights_bar = g->weights_bar ? g->weights_bar : (sfh_rows ? (double*)c->gwb.p : nullptr);
     go.weights_z_bar = g->weights_z_bar;
     k_grad_out<<<n, 256, 0, c->stream>>>(go);
     HIPCHK(c, hipGetLastError());
+    if (sfh_rows) {
+        SfhGradArgs sa;
+        sa.n = n;
+        sa.nt = nt;
+        sa.tiny_logt = c->tiny_logt;
+        sa.time = c->d_time;
+        sa.prm = f.prm;
+        sa.wbar = go.weights_bar;
+        sa.params_bar = g->params_bar;
+        const size_t lds_sfh = sizeof(double) * 4 * 8 * (size_t)(nt + 2);
+        if (lds_sfh > 65536)
+            HIPCHK(c, hipFuncSetAttribute((const void*)k_sfh_grad, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_sfh));
+        k_sfh_grad<<<(n + 3) / 4, 256, lds_sfh, c->stream>>>(sa);
+        HIPCHK(c, hipGetLastError());
+    }
     if (c->profiling) {
         HIPCHK(c, hipEventRecord(c->gev[3], c->stream));
         c->gev_pending = true;

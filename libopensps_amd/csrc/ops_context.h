@@ -113,6 +113,7 @@ struct OpsCtx {
     } full;
     OpsBuffer glsum, gfband, gw, gout, gyb, gC;   // reverse pass: L_bol sums, band fluxes, SFH weights, per-model
                                                   // scalars, y_bar rows, the transposed product
+    OpsBuffer gwb;                                // reverse pass: d L / d w for the SFH rows (no weights_bar given)
     double* lsum_out = nullptr;                   // set by the reverse pass around its forward batch
     int* d_last_flag = nullptr;   // the per-batch status flag of the last batch (read by sync)
 

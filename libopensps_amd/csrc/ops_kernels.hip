@@ -235,6 +235,274 @@ __device__ int wave_sfh_weights(const double* __restrict__ tf, int nt, double ti
 }
 
 // ------------------------------------------------------------------------------------------------
+// SFH tangents (src/physics/opensps_sfh_tangent.f90) reduced to params_bar's SFH rows (batch_sfh_gradients,
+// src/physics/opensps_csp_batch.f90:5198-5217: row = sum over ages of d L / d w_k  d w_k / d theta).
+// The weights' tangent is linear in the tangents of the internal times the parameters move (tage, tau,
+// tq = tage - sf_trunc and tb = tage - tburst, in years), so one pass takes the derivative along each of them
+// (per year) and the eight rows are combinations of their sums S_e = sum_k wbar_k d w_k / d e (G = 1e9 yr / Gyr):
+//   tage = G (S_tage + S_tb + [tq > 0] S_tq)   tau = G S_tau       tburst = -G S_tb      sf_start = -G S_tage
+//   sf_trunc = -G [tq > 0] S_tq               const, fburst: the mixture's own terms    sf_slope = 0
+// (the slope acts on Simha's linear part only, sfh = 5, which the forward does not compute). sfh = 0 is a burst
+// at tb = tage: tage's row only.
+// ------------------------------------------------------------------------------------------------
+
+// moments_t (opensps_sfh_tangent.f90:426-467), logarithmic branch, types 0, 1, 4, along ND directions: the
+// moments at log time L and their derivatives for the time's tangents dL[e] and the parameters' dtg[e], dtau[e]
+template <int ND>
+__device__ void d_moments_t(double L, const double* dL, const double* dtg, const double* dtau, const SfhP& s,
+                            double& m0, double& m1, double* dm0, double* dm1) {
+    const double LOG_E = 0.4342944819032518, LN10 = 2.302585092994046;
+    const double t = pow(10.0, L);
+    m0 = 0.0;
+    m1 = 0.0;
+#pragma unroll
+    for (int e = 0; e < ND; e++) { dm0[e] = 0.0; dm1[e] = 0.0; }
+    if (s.type == 0) {
+        m0 = t;
+        m1 = t * (L - LOG_E);
+#pragma unroll
+        for (int e = 0; e < ND; e++) {
+            const double dt = t * LN10 * dL[e];
+            dm0[e] = dt;
+            dm1[e] = dt * (L - LOG_E) + t * dL[e];
+        }
+        return;
+    }
+    if ((s.type != 1 && s.type != 4) || s.tau <= OPS_SAFE_FLOOR) return;
+    const double tau = s.tau, tg = s.tage;
+    double x = t / tau;
+    const bool floored = x < OPS_SAFE_FLOOR;
+    if (floored) x = OPS_SAFE_FLOOR;
+    const double ex = exp(x);
+    double ei = d_expi(x);
+    const bool fin = isfinite(ei);
+    if (!fin) ei = 0.0;
+    const double a = t - tg - tau, b = a * L - tau * LOG_E;
+    if (s.type == 1) {
+        m0 = ex;
+        m1 = L * ex - LOG_E * ei;
+    } else {
+        m0 = -a * ex;
+        m1 = -b * ex - (tg + tau) * LOG_E * ei;
+    }
+#pragma unroll
+    for (int e = 0; e < ND; e++) {
+        const double dt = t * LN10 * dL[e];
+        const double dx = floored ? 0.0 : dt / tau - t * dtau[e] / (tau * tau);
+        const double dei = fin ? ex / x * dx : 0.0;
+        if (s.type == 1) {
+            dm0[e] = ex * dx;
+            dm1[e] = dL[e] * ex + L * dm0[e] - LOG_E * dei;
+        } else {
+            const double da = dt - dtg[e] - dtau[e];
+            dm0[e] = -(da * ex + a * ex * dx);
+            dm1[e] = -((da * L + a * dL[e] - dtau[e] * LOG_E) * ex + b * ex * dx) - (dtg[e] + dtau[e]) * LOG_E * ei -
+                     (tg + tau) * LOG_E * dei;
+        }
+    }
+}
+
+// One SFH component's weights and tangents over the window [tq, tage] reduced against wbar.
+struct SfhTanSums {
+    double mass, B;           // sum_{k < imax} w_k; sum_k wbar_k w_k
+    double dmass[3], A[3];    // per direction: sum_{k < imax} dw_k; sum_k wbar_k dw_k
+};
+
+// ssp_weights_t (opensps_sfh_tangent.f90:255-358) with idx_min = 0, by one wavefront. Directions: e = 0 tage,
+// then tau (ND = 3 only), then tq (last). The moments at the clamped nodes are independent (one node a lane);
+// node j then takes the upper end of segment j - 1 and the lower end of segment j. Zero-width segments stay in
+// (no weight; the tangent is the one-sided one where a window end sits on a grid age).
+// mm: LDS, (2 + 2 ND) (nt + 2) doubles: node j (1-based) at slot j, the young edge's limits at slots 0, nt + 1.
+template <int ND>
+__device__ void wave_weights_t(const double* __restrict__ tf, int nt, double tiny_logt, const SfhP& s, int imax,
+                               const double* __restrict__ wbar, double* mm, int lane, SfhTanSums& r) {
+    const double LN10 = 2.302585092994046;
+    const int S = nt + 2;
+    double *M0 = mm, *M1 = mm + S, *D0 = mm + 2 * S, *D1 = mm + (2 + ND) * S;
+    const double lmin = pow(10.0, tiny_logt);
+    const double lts = log10(fmax(s.tq, lmin)), lte = log10(fmax(s.tage, lmin));
+    const double uts = s.tq > lmin ? 1.0 / (s.tq * LN10) : 0.0;       // d lts / d tq
+    const double ute = s.tage > lmin ? 1.0 / (s.tage * LN10) : 0.0;   // d lte / d tage
+    const int js = max(wave_find_interval(tf, nt, lts, lane), 1);
+    const int je = min(wave_find_interval(tf, nt, lte, lane) + 1, min(imax, nt - 1));
+    const int nn = max(je + 2 - js, 0);   // nodes js .. je + 1
+    double dtg[ND], dtau[ND];
+#pragma unroll
+    for (int e = 0; e < ND; e++) {
+        dtg[e] = e == 0 ? 1.0 : 0.0;
+        dtau[e] = (ND == 3 && e == 1) ? 1.0 : 0.0;
+    }
+    auto clampL = [&](double node, int& fl) {   // clamp_t (opensps_sfh_tangent.f90:407-422)
+        double v = node;
+        fl = 0;
+        if (lts > node) { v = lts; fl = 1; }
+        if (lte < v) { v = lte; fl = 2; }
+        return v;
+    };
+    WAVE_SYNC();
+    for (int i = lane; i < nn + 2; i += 64) {
+        int slot;
+        double node;
+        if (i < nn) { slot = js + i; node = tf[slot - 1]; }
+        else if (i == nn) { slot = 0; node = tiny_logt; }
+        else { slot = nt + 1; node = tf[0]; }
+        int fl;
+        const double L = clampL(node, fl);
+        double dL[ND];
+#pragma unroll
+        for (int e = 0; e < ND; e++) dL[e] = 0.0;
+        if (fl == 2) dL[0] = ute;
+        if (fl == 1) dL[ND - 1] = uts;
+        double m0, m1, dm0[ND], dm1[ND];
+        d_moments_t<ND>(L, dL, dtg, dtau, s, m0, m1, dm0, dm1);
+        M0[slot] = m0;
+        M1[slot] = m1;
+#pragma unroll
+        for (int e = 0; e < ND; e++) {
+            D0[e * S + slot] = dm0[e];
+            D1[e * S + slot] = dm1[e];
+        }
+    }
+    WAVE_SYNC();
+    double mass = 0.0, B = 0.0, dmass[ND], A[ND];
+#pragma unroll
+    for (int e = 0; e < ND; e++) { dmass[e] = 0.0; A[e] = 0.0; }
+    for (int j = js + lane; j <= je + 1; j += 64) {
+        double w = 0.0, dw[ND];
+#pragma unroll
+        for (int e = 0; e < ND; e++) dw[e] = 0.0;
+        if (j - 1 >= js) {   // segment j - 1 (ages j - 1, j): node j is its upper end
+            const double nlo = tf[j - 2], inv = 1.0 / (tf[j - 1] - nlo);
+            w += ((M1[j] - M1[j - 1]) - nlo * (M0[j] - M0[j - 1])) * inv;
+#pragma unroll
+            for (int e = 0; e < ND; e++)
+                dw[e] += ((D1[e * S + j] - D1[e * S + j - 1]) - nlo * (D0[e * S + j] - D0[e * S + j - 1])) * inv;
+        }
+        if (j <= je) {       // segment j (ages j, j + 1): node j is its lower end
+            const double nhi = tf[j], inv = 1.0 / (nhi - tf[j - 1]);
+            w += (nhi * (M0[j + 1] - M0[j]) - (M1[j + 1] - M1[j])) * inv;
+#pragma unroll
+            for (int e = 0; e < ND; e++)
+                dw[e] += (nhi * (D0[e * S + j + 1] - D0[e * S + j]) - (D1[e * S + j + 1] - D1[e * S + j])) * inv;
+        }
+        const double wb = wbar[j - 1];
+        if (j - 1 < imax) {
+            mass += w;
+#pragma unroll
+            for (int e = 0; e < ND; e++) dmass[e] += dw[e];
+        }
+        B += wb * w;
+#pragma unroll
+        for (int e = 0; e < ND; e++) A[e] += wb * dw[e];
+    }
+    if (lane == 0) {   // the youngest edge: the mass formed before the first grid age goes to it
+        int f1, f2;
+        if (clampL(tiny_logt, f1) < clampL(tf[0], f2)) {
+            const double w = M0[nt + 1] - M0[0], wb = wbar[0];
+            mass += w;
+            B += wb * w;
+#pragma unroll
+            for (int e = 0; e < ND; e++) {
+                const double dw = D0[e * S + nt + 1] - D0[e * S];
+                dmass[e] += dw;
+                A[e] += wb * dw;
+            }
+        }
+    }
+    r.mass = wave_sum(mass);
+    r.B = wave_sum(B);
+#pragma unroll
+    for (int e = 0; e < 3; e++) { r.dmass[e] = 0.0; r.A[e] = 0.0; }
+#pragma unroll
+    for (int e = 0; e < ND; e++) {
+        r.dmass[e] = wave_sum(dmass[e]);
+        r.A[e] = wave_sum(A[e]);
+    }
+    WAVE_SYNC();
+}
+
+// The burst (ssp_weights_t, type -1): two weights linear in log tb between the bracketing ages. B = sum wbar w,
+// A = sum wbar dw / d tb (per year).
+__device__ void d_burst_t(const double* __restrict__ tf, int nt, double tiny_logt, double tb,
+                          const double* __restrict__ wbar, double& B, double& A) {
+    B = 0.0;
+    A = 0.0;
+    if (tb < 0.0) return;
+    const double lmin = pow(10.0, tiny_logt), ltb = log10(fmax(tb, lmin));
+    const double u = tb > lmin ? 1.0 / (tb * 2.302585092994046) : 0.0;
+    if (ltb <= tf[0]) { B = wbar[0]; return; }
+    const int j = max(1, min(find_interval_ref(tf, nt, ltb), nt - 1));
+    const double bdt = tf[j] - tf[j - 1];
+    B = wbar[j - 1] * ((tf[j] - ltb) / bdt) + wbar[j] * ((ltb - tf[j - 1]) / bdt);
+    A = (wbar[j] - wbar[j - 1]) * (u / bdt);
+}
+
+// k_sfh_grad: one wavefront per model (4 per workgroup); sfh_weights_tangent (opensps_sfh_tangent.f90:66-251)
+// along the four internal times at once, written to params_bar's rows 16 .. 21, 24, 25 (0-based). Other SFH types
+// (the forward has flagged them) get zeros.
+__global__ __launch_bounds__(256) void k_sfh_grad(SfhGradArgs a) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int m = blockIdx.x * 4 + wv;
+    if (m >= a.n) return;
+    const int nt = a.nt;
+    double* mm = lds + (size_t)wv * 8 * (nt + 2);
+    double p[OPS_NPAR];
+#pragma unroll
+    for (int q = 0; q < OPS_NPAR; q++) p[q] = a.prm[(size_t)q * a.n + m];
+    const double* wb = a.wbar + (size_t)m * nt;
+    const double G = 1.0e9;
+    double rows[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // tage tau const fburst tburst sf_start sf_trunc slope
+    const int sfh = (int)p[OPS_P_SFH];
+    if (sfh == 0 || sfh == 1 || sfh == 4) {
+        SfhP s;
+        d_convert_sfh(p, s);
+        const int imax = min(max(wave_find_interval(a.time, nt, log10(s.tage), lane) + 2, 1), nt);
+        if (sfh == 0) {
+            double B, A;
+            d_burst_t(a.time, nt, a.tiny_logt, s.tage, wb, B, A);
+            rows[0] = G * A;
+        } else {
+            SfhTanSums T, Cn;
+            s.type = sfh;
+            wave_weights_t<3>(a.time, nt, a.tiny_logt, s, imax, wb, mm, lane, T);
+            s.type = 0;
+            wave_weights_t<2>(a.time, nt, a.tiny_logt, s, imax, wb, mm, lane, Cn);
+            double mass1 = T.mass, dm1[3] = {T.dmass[0], T.dmass[1], T.dmass[2]};
+            if (mass1 < OPS_SAFE_FLOOR) { mass1 = 1.0; dm1[0] = dm1[1] = dm1[2] = 0.0; }
+            double massc = Cn.mass, dmc[2] = {Cn.dmass[0], Cn.dmass[1]};
+            if (massc < OPS_SAFE_FLOOR) { massc = 1.0; dmc[0] = dmc[1] = 0.0; }
+            const bool burst = s.tb >= 0.0;
+            double BB = 0.0, AB = 0.0;
+            if (burst) d_burst_t(a.time, nt, a.tiny_logt, s.tb, wb, BB, AB);
+            double c = p[OPS_P_CONST], fb = burst ? p[OPS_P_FBURST] : 0.0;
+            if (!(p[OPS_P_CONST] > 0.0 || p[OPS_P_FBURST] > OPS_SAFE_FLOOR)) { c = 0.0; fb = 0.0; }
+            const double bt = T.B / mass1, bc = Cn.B / massc;   // sum wbar wt, sum wbar cn (normalized parts)
+            // sum wbar d(w / mass) = (A - (B / mass) d mass) / mass
+            const double k = 1.0 - c - fb;
+            const double s_tg = k * ((T.A[0] - bt * dm1[0]) / mass1) + c * ((Cn.A[0] - bc * dmc[0]) / massc);
+            const double s_tau = k * ((T.A[1] - bt * dm1[1]) / mass1);
+            const double s_tq = k * ((T.A[2] - bt * dm1[2]) / mass1) + c * ((Cn.A[1] - bc * dmc[1]) / massc);
+            const double s_tb = fb * AB;
+            const bool tq_on = s.tq > 0.0;
+            rows[0] = G * (s_tg + s_tb + (tq_on ? s_tq : 0.0));
+            rows[1] = G * s_tau;
+            rows[2] = bc - bt;
+            rows[3] = burst ? BB - bt : 0.0;
+            rows[4] = -G * s_tb;
+            rows[5] = -G * s_tg;
+            rows[6] = tq_on ? -G * s_tq : 0.0;
+        }
+    }
+    if (lane == 0) {
+        double* pb = a.params_bar + (size_t)m * OPENSPS_HIP_NBGRAD_ROWS;
+        const int rr[8] = {16, 17, 18, 19, 20, 21, 24, 25};
+#pragma unroll
+        for (int q = 0; q < 8; q++) pb[rr[q]] = rows[q];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
 // k_prep: one wavefront per model (4 per workgroup)
 // ------------------------------------------------------------------------------------------------
 __device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, double dz, int lane);

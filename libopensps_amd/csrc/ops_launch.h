@@ -254,6 +254,15 @@ template <typename T>
 __global__ void k_grad_chain(GradArgs a);
 __global__ void k_gemmt_f64(GemmTArgs a);
 __global__ void k_grad_out(GradOutArgs a);
+
+// params_bar's SFH rows from d L / d w (k_sfh_grad, one wavefront per model)
+struct SfhGradArgs {
+    int n, nt;
+    double tiny_logt;
+    const double *time, *prm, *wbar;
+    double* params_bar;
+};
+__global__ void k_sfh_grad(SfhGradArgs a);
 __global__ void k_pack_t(int nspec, int nsp, int kfull, int nrows, const double* basis, double* A64, uint16_t* Ah,
                          uint16_t* Al);
 

@@ -264,6 +264,196 @@ int orc_sfh_weights(const double *tf, int nt, double tiny_logt, const double *p,
 }
 
 /* ---------------------------------------------------------------------------------------------
+ * SFH tangents (src/physics/opensps_sfh_tangent.f90): the weights' derivative with respect to one
+ * SFH parameter, forward mode, sfh = 0, 1, 4 with const, fburst and truncation, logarithmic
+ * interpolation. Pinned by central differences of orc_sfh_weights (tests/test_oracle_sfh_tangent.py).
+ * ------------------------------------------------------------------------------------------- */
+#define ORC_GYR 1.0e9
+enum { SFHT_TAGE = 1, SFHT_TAU, SFHT_CONST, SFHT_FBURST, SFHT_TBURST, SFHT_SF_START, SFHT_SF_TRUNC, SFHT_SF_SLOPE };
+
+/* clamp_t (opensps_sfh_tangent.f90:407-422): min(max(node, lts), lte) and the derivative of the kept bound */
+static void clamp_t(double node, double lts, double dlts, double lte, double dlte, double *v, double *dv) {
+    *v = node;
+    *dv = 0.0;
+    if (lts > node) { *v = lts; *dv = dlts; }
+    if (lte < *v) { *v = lte; *dv = dlte; }
+}
+
+/* moments_t (opensps_sfh_tangent.f90:426-467), logarithmic branch, types 0, 1, 4: the indefinite moments at
+ * log time lt and their total derivatives, with the time's (dlt) and the parameters' (dtg, dtau) tangents */
+static void moments_t(double lt, double dlt, const sfhp_t *s, double dtg, double dtau, double *m0, double *dm0,
+                      double *m1, double *dm1) {
+    const double ln10 = log(10.0);
+    const double t = pow(10.0, lt), dt = t * ln10 * dlt;
+    *m0 = 0.0; *dm0 = 0.0; *m1 = 0.0; *dm1 = 0.0;
+    if ((s->type == 1 || s->type == 4) && s->tau <= SAFE_FLOOR) return;
+    if (s->type == 0) {
+        *m0 = t;
+        *dm0 = dt;
+        *m1 = t * (lt - LOG_E);
+        *dm1 = dt * (lt - LOG_E) + t * dlt;
+        return;
+    }
+    if (s->type != 1 && s->type != 4) return;
+    const double tau = s->tau, tg = s->tage;
+    double x = t / tau, dx = dt / tau - t * dtau / (tau * tau);
+    if (x < SAFE_FLOOR) { x = SAFE_FLOOR; dx = 0.0; }
+    const double ex = exp(x);
+    double ei = orc_expi(x), dei = ex / x * dx;
+    if (!isfinite(ei)) { ei = 0.0; dei = 0.0; }
+    if (s->type == 1) {
+        *m0 = ex;
+        *dm0 = ex * dx;
+        *m1 = lt * ex - LOG_E * ei;
+        *dm1 = dlt * ex + lt * *dm0 - LOG_E * dei;
+    } else {
+        const double a = t - tg - tau, da = dt - dtg - dtau, b = a * lt - tau * LOG_E;
+        *m0 = -a * ex;
+        *dm0 = -(da * ex + a * ex * dx);
+        *m1 = -b * ex - (tg + tau) * LOG_E * ei;
+        *dm1 = -((da * lt + a * dlt - dtau * LOG_E) * ex + b * ex * dx) - (dtg + dtau) * LOG_E * ei -
+               (tg + tau) * LOG_E * dei;
+    }
+}
+
+/* ssp_weights_t (opensps_sfh_tangent.f90:255-358), logarithmic interpolation: compute_ssp_weights with the
+ * tangents dtg, dtau, dtb, dtq of tage, tau, tb and tq (years). Segments of zero width stay (their tangent is the
+ * one-sided one where a window end sits on a grid age). */
+static void ssp_weights_t(const double *tf, int nt, double tiny_logt, const sfhp_t *s, double dtg, double dtau,
+                          double dtb, double dtq, int idx_min, int idx_max, double *w, double *dw) {
+    const double ln10 = log(10.0), lmin = pow(10.0, tiny_logt);
+    for (int i = 0; i < nt; i++) { w[i] = 0.0; dw[i] = 0.0; }
+    if (s->type == -1) {
+        if (s->tb < 0.0) return;
+        const double ltb = log10(fmax(s->tb, lmin));
+        const double dltb = s->tb > lmin ? dtb / (s->tb * ln10) : 0.0;
+        if (ltb <= tf[0]) { w[0] = 1.0; return; }
+        int j = orc_find_interval(tf, nt, ltb);
+        if (j < 1) j = 1;
+        if (j > nt - 1) j = nt - 1;
+        const double bdt = tf[j] - tf[j - 1];
+        w[j - 1] = (tf[j] - ltb) / bdt;
+        w[j] = (ltb - tf[j - 1]) / bdt;
+        dw[j - 1] = -dltb / bdt;
+        dw[j] = dltb / bdt;
+        return;
+    }
+    const double lts = log10(fmax(s->tq, lmin)), dlts = s->tq > lmin ? dtq / (s->tq * ln10) : 0.0;
+    const double lte = log10(fmax(s->tage, lmin)), dlte = s->tage > lmin ? dtg / (s->tage * ln10) : 0.0;
+    int j_start = orc_find_interval(tf, nt, lts);
+    int j_end = orc_find_interval(tf, nt, lte) + 1;
+    if (j_start < (idx_min > 1 ? idx_min : 1)) j_start = idx_min > 1 ? idx_min : 1;
+    if (j_end > (idx_max < nt - 1 ? idx_max : nt - 1)) j_end = idx_max < nt - 1 ? idx_max : nt - 1;
+    double l1, dl1;
+    clamp_t(tf[j_start - 1], lts, dlts, lte, dlte, &l1, &dl1);
+    double lp = l1, m0p, dm0p, m1p, dm1p;
+    moments_t(l1, dl1, s, dtg, dtau, &m0p, &dm0p, &m1p, &dm1p);
+    for (int j = j_start; j <= j_end; j++) {
+        double l2, dl2;
+        clamp_t(tf[j], lts, dlts, lte, dlte, &l2, &dl2);
+        if (l2 < lp) { lp = l2; continue; }
+        double m0c, dm0c, m1c, dm1c;
+        moments_t(l2, dl2, s, dtg, dtau, &m0c, &dm0c, &m1c, &dm1c);
+        const double m0 = m0c - m0p, dm0 = dm0c - dm0p, m1 = m1c - m1p, dm1 = dm1c - dm1p;
+        m0p = m0c; dm0p = dm0c; m1p = m1c; dm1p = dm1c;
+        const double nlo = tf[j - 1], nhi = tf[j], inv = 1.0 / (nhi - nlo);
+        w[j - 1] += (nhi * m0 - m1) * inv;
+        w[j] += (m1 - nlo * m0) * inv;
+        dw[j - 1] += (nhi * dm0 - dm1) * inv;
+        dw[j] += (dm1 - nlo * dm0) * inv;
+        lp = l2;
+    }
+    if (idx_min == 0) {
+        double la, dla, lb, dlb;
+        clamp_t(tiny_logt, lts, dlts, lte, dlte, &la, &dla);
+        clamp_t(tf[0], lts, dlts, lte, dlte, &lb, &dlb);
+        if (la < lb) {
+            double ma, dma, mb, dmb, x1, x2;
+            moments_t(lb, dlb, s, dtg, dtau, &mb, &dmb, &x1, &x2);
+            moments_t(la, dla, s, dtg, dtau, &ma, &dma, &x1, &x2);
+            w[0] += mb - ma;
+            dw[0] += dmb - dma;
+        }
+    }
+}
+
+/* sfh_weights_tangent (opensps_sfh_tangent.f90:66-251) for sfh = 0, 1, 4: the weights w and their derivative dw
+ * with respect to parameter q (SFHT_*, in the units of the parameters: Gyr). s: scratch (4 nt). Returns 0, or 2 for
+ * an SFH type this build does not compute. */
+int orc_sfh_weights_tangent(const double *tf, int nt, double tiny_logt, const double *p, int q, double *w,
+                            double *dw, double *s) {
+    sfhp_t h;
+    const int sfh = (int)p[OPS_P_SFH];
+    for (int i = 0; i < nt; i++) { w[i] = 0.0; dw[i] = 0.0; }
+    if (sfh != 0 && sfh != 1 && sfh != 4) return 2;
+    convert_sfhparams(p, p[OPS_P_TAGE], &h);
+    double dtg = 0.0, dtau = 0.0, dtb = 0.0, dtq = 0.0;
+    switch (q) {
+    case SFHT_TAGE: dtg = ORC_GYR; dtb = ORC_GYR; if (h.tq > 0.0) dtq = ORC_GYR; break;
+    case SFHT_SF_TRUNC: if (h.tq > 0.0) dtq = -ORC_GYR; break;
+    case SFHT_TAU: dtau = ORC_GYR; break;
+    case SFHT_TBURST: dtb = -ORC_GYR; break;
+    case SFHT_SF_START: if (sfh != 0) dtg = -ORC_GYR; break;
+    default: break;   /* const, fburst: the mixture below; sf_slope: only Simha's linear part has it */
+    }
+    int imax = orc_find_interval(tf, nt, log10(h.tage)) + 2;
+    if (imax < 1) imax = 1;
+    if (imax > nt) imax = nt;
+    if (sfh == 0) {   /* a burst at tb = tage */
+        h.type = -1;
+        h.tb = h.tage;
+        ssp_weights_t(tf, nt, tiny_logt, &h, 0.0, 0.0, dtg, 0.0, imax - 2 > 1 ? imax - 2 : 1, imax, w, dw);
+        return 0;
+    }
+    double *c0 = s, *c1 = s + nt, *b0 = s + 2 * nt, *b1 = s + 3 * nt;
+    h.type = sfh;
+    ssp_weights_t(tf, nt, tiny_logt, &h, dtg, dtau, 0.0, dtq, 0, imax, w, dw);
+    double mass1 = 0.0, dmass1 = 0.0;
+    for (int i = 0; i < imax; i++) { mass1 += w[i]; dmass1 += dw[i]; }
+    if (mass1 < SAFE_FLOOR) { mass1 = 1.0; dmass1 = 0.0; }
+    /* the constant and burst components, whatever const and fburst (one-sided derivatives at 0) */
+    h.type = 0;
+    ssp_weights_t(tf, nt, tiny_logt, &h, dtg, 0.0, 0.0, dtq, 0, imax, c0, c1);
+    double massc = 0.0, dmassc = 0.0;
+    for (int i = 0; i < imax; i++) { massc += c0[i]; dmassc += c1[i]; }
+    if (massc < SAFE_FLOOR) { massc = 1.0; dmassc = 0.0; }
+    for (int i = 0; i < nt; i++) { b0[i] = 0.0; b1[i] = 0.0; }
+    double fb = 0.0;
+    if (h.tb >= 0.0) {
+        h.type = -1;
+        ssp_weights_t(tf, nt, tiny_logt, &h, 0.0, 0.0, dtb, 0.0, 0, imax, b0, b1);
+        fb = p[OPS_P_FBURST];
+    }
+    double cst = p[OPS_P_CONST];
+    if (!(p[OPS_P_CONST] > 0.0 || p[OPS_P_FBURST] > SAFE_FLOOR)) { cst = 0.0; fb = 0.0; }
+    for (int i = 0; i < nt; i++) {
+        const double wt = w[i] / mass1, dwt = (dw[i] - wt * dmass1) / mass1;
+        const double cn = c0[i] / massc, dcn = (c1[i] - cn * dmassc) / massc;
+        w[i] = (1.0 - cst - fb) * wt + cst * cn + fb * b0[i];
+        dw[i] = (1.0 - cst - fb) * dwt + cst * dcn + fb * b1[i];
+        if (q == SFHT_CONST) dw[i] += cn - wt;
+        if (q == SFHT_FBURST && h.tb >= 0.0) dw[i] += b0[i] - wt;
+    }
+    return 0;
+}
+
+/* batch_sfh_gradients (src/physics/opensps_csp_batch.f90:5198-5217): params_bar's SFH rows of one model (0-based:
+ * the reference's SFHT_ROWS = 17 18 19 20 21 22 25 26 of :215, minus 1) from d L / d w (wbar, nt). s: scratch
+ * (6 nt). Returns 0, or 2 for an SFH type this build does not compute. */
+static const int ORC_SFHT_ROWS[8] = {16, 17, 18, 19, 20, 21, 24, 25};
+int orc_sfh_gradients(const double *tf, int nt, double tiny_logt, const double *p, const double *wbar,
+                      double *pb, double *s) {
+    for (int q = 1; q <= 8; q++) {
+        const int st = orc_sfh_weights_tangent(tf, nt, tiny_logt, p, q, s, s + nt, s + 2 * nt);
+        if (st) return st;
+        double acc = 0.0;
+        for (int i = 0; i < nt; i++) acc += wbar[i] * s[nt + i];
+        pb[ORC_SFHT_ROWS[q - 1]] = acc;
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------------------
  * IGM (src/physics/opensps_cosmology.f90:65-95, 333-507)
  * ------------------------------------------------------------------------------------------- */
 static const double LY_WAVE[17] = {1215.67, 1025.72, 972.537, 949.743, 937.803, 930.748, 926.226, 923.150,
@@ -1127,7 +1317,9 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
  *     metallicity weight's cotangent into logzsol.
  * params_bar rows (0-based, the reference's BGRAD_* minus 1, :166-213): 1 dust1, 2 dust2, 3 dust3,
  * 4 dust_index, 5 dust1_index, 6 frac_obrun, 7 frac_nodust, 15 logzsol, 22 zred, 23 sigma_smooth,
- * 32 igm_factor; the others (the SFH parameters among them) are zero. The IGM optical depth is the
+ * 32 igm_factor and, where the SFH weights are computed here (no ssp_weights or ssp_weights_z), the SFH rows
+ * 16 tage, 17 tau, 18 const, 19 fburst, 20 tburst, 21 sf_start, 24 sf_trunc, 25 sf_slope (orc_sfh_gradients); the
+ * others are zero. The IGM optical depth is the
  * six-term form (igm_tau_pixel_terms x igm_tau_z_terms), which the reference uses whenever the
  * redshifts differ across the batch.
  * ------------------------------------------------------------------------------------------- */
@@ -1179,6 +1371,7 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
         int *slo = (int *)malloc(sizeof(int) * nspec), *shi = (int *)malloc(sizeof(int) * nspec);
         int *soff = (int *)malloc(sizeof(int) * nspec);
         double *scoef = (double *)malloc(sizeof(double) * cap);
+        double *wbl = (double *)malloc(sizeof(double) * nt), *tsc = (double *)malloc(sizeof(double) * 6 * nt);
 
         #pragma omp for schedule(dynamic, 4)
         for (int m = 0; m < n; m++) {
@@ -1452,7 +1645,8 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
                         lk = pow(10.0, c->blbol[k - 1]);
                         wb = (1.0 - dz) * sl;
                     }
-                    if (weights_bar) weights_bar[(size_t)m * nt + k - 1] = wb + lsb * lk + mb * mk;
+                    wbl[k - 1] = wb + lsb * lk + mb * mk;
+                    if (weights_bar) weights_bar[(size_t)m * nt + k - 1] = wbl[k - 1];
                     if (blend) {
                         const double wk = w[k - 1] > SAFE_FLOOR ? w[k - 1] : 0.0;
                         dzb += wk * (sh - sl);
@@ -1471,11 +1665,19 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
                 pb[22] = dzo + d13 + dzd;
                 pb[23] = dsg;
                 pb[32] = d14;
+                /* the SFH parameters (batch_sfh_gradients, :1163-1166: only where the weights are not given) */
+                if (!ssp_weights && !wz_on) {
+                    int st = orc_sfh_gradients(c->time_full, nt, c->tiny_logt, p, wbl, pb, tsc);
+                    if (st) {
+                        #pragma omp atomic write
+                        status = st;
+                    }
+                }
             }
         }
         free(p); free(w); free(t1); free(t2); free(y); free(o); free(A); free(S0); free(S1); free(tig); free(tauv);
         free(work); free(fobs); free(fb); free(fbb); free(vb); free(g1); free(g0); free(yb); free(ob);
-        free(slo); free(shi); free(soff); free(scoef);
+        free(slo); free(shi); free(soff); free(scoef); free(wbl); free(tsc);
     }
     free(jlo); free(jhi); free(do_band); free(bt); free(wtl); free(ltab); free(pa);
     return status;

@@ -35,6 +35,10 @@ def lib():
         L.orc_expi.restype = C.c_double
         L.orc_sfh_weights.argtypes = [_dp, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
         L.orc_sfh_weights.restype = C.c_int
+        L.orc_sfh_weights_tangent.argtypes = [_dp, C.c_int, C.c_double, _dp, C.c_int, _dp, _dp, _dp]
+        L.orc_sfh_weights_tangent.restype = C.c_int
+        L.orc_sfh_gradients.argtypes = [_dp, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+        L.orc_sfh_gradients.restype = C.c_int
         L.orc_igm_tau.argtypes = [C.c_double, C.c_double]
         L.orc_igm_tau.restype = C.c_double
         L.orc_igm_pixel_terms.argtypes = [C.c_double, _dp]
@@ -91,6 +95,29 @@ def sfh_weights(time_full, p, tiny_logt=0.0):
     t2 = np.zeros(tf.size)
     st = lib().orc_sfh_weights(tf, tf.size, float(tiny_logt), _d(p), w, t1, t2)
     return w, st
+
+
+# the SFH tangents' parameters in the order of sfh_weights_tangent's q = 1 .. 8
+SFHT_PARAMS = ["tage", "tau", "const", "fburst", "tburst", "sf_start", "sf_trunc", "sf_slope"]
+
+
+def sfh_weights_tangent(time_full, p, q, tiny_logt=0.0):
+    """The SFH weights and their derivative with respect to SFHT_PARAMS[q - 1] (per Gyr). Returns w, dw, status."""
+    tf = _d(time_full)
+    w = np.zeros(tf.size)
+    dw = np.zeros(tf.size)
+    s = np.zeros(4 * tf.size)
+    st = lib().orc_sfh_weights_tangent(tf, tf.size, float(tiny_logt), _d(p), int(q), w, dw, s)
+    return w, dw, st
+
+
+def sfh_gradients(time_full, p, wbar, tiny_logt=0.0):
+    """params_bar's SFH rows (NBGRAD, zero elsewhere) of one model from d L / d w. Returns (pb, status)."""
+    tf = _d(time_full)
+    pb = np.zeros(NBGRAD)
+    s = np.zeros(6 * tf.size)
+    st = lib().orc_sfh_gradients(tf, tf.size, float(tiny_logt), _d(p), _d(wbar), pb, s)
+    return pb, st
 
 
 def igm_tau(w, opz) -> float:
@@ -271,7 +298,9 @@ def _context(grid, bands, add_igm, smooth_fast, mag_mask, cosmo, ptr):
 NBGRAD = 44
 # params_bar rows (0-based) this build computes: the reference's BGRAD_* - 1 (opensps_csp_batch.f90:166-213)
 GRAD_ROWS = {"dust1": 1, "dust2": 2, "dust3": 3, "dust_index": 4, "dust1_index": 5, "frac_obrun": 6,
-             "frac_nodust": 7, "logzsol": 15, "zred": 22, "sigma_smooth": 23, "igm_factor": 32}
+             "frac_nodust": 7, "logzsol": 15, "zred": 22, "sigma_smooth": 23, "igm_factor": 32,
+             "tage": 16, "tau": 17, "const": 18, "fburst": 19, "tburst": 20, "sf_start": 21, "sf_trunc": 24,
+             "sf_slope": 25}
 
 
 def csp_batch_grad(grid, bands, prm, ssp_weights=None, ssp_weights_z=None, mags_bar=None, spec_bar=None,

@@ -1,7 +1,8 @@
-"""The reverse pass on the GPU (opensps_hip_csp_batch_grad: k_grad_chain, the transposed product, k_grad_out)
-against the oracle's reverse pass (orc_csp_batch_grad, itself pinned by central differences of the oracle's
-forward): params_bar rows, weights_z_bar (weights over ages and metallicities) and weights_bar with the metallicity
-blend, the mass and L_bol cotangents, with and without the birth-cloud and old-star screens. FP64 path: relative
+"""The reverse pass on the GPU (opensps_hip_csp_batch_grad: k_grad_chain, the transposed product, k_grad_out,
+k_sfh_grad) against the oracle's reverse pass (orc_csp_batch_grad, itself pinned by central differences of the
+oracle's forward): params_bar rows (the SFH parameters' rows with SFH parameters), weights_z_bar (weights over ages
+and metallicities) and weights_bar with the metallicity blend, the mass and L_bol cotangents, with and without the
+birth-cloud and old-star screens. FP64 path: relative
 to each output's largest magnitude per model; mixed path (FP32 chain, split-BF16 transposed product): looser."""
 import numpy as np
 import pytest
@@ -132,7 +133,7 @@ def test_grad_blend_weights(precision):
 
 
 def test_grad_statuses():
-    """params_bar with SFH parameters (the SFH tangents are not built) and the operator's reverse pass: status 2."""
+    """The operator's reverse pass (the IMF rows) is status 2; weights_bar alone is computed with SFH parameters."""
     from libopensps_amd import hip as H
     import torch
     g, b = _setup()
@@ -141,16 +142,57 @@ def test_grad_statuses():
     ctx = _ctx(g, b)
     print("case: grad statuses", flush=True)
     out = ctx.alloc_outputs(n, 0)
-    grads = {"params_bar": torch.zeros((n, H.NBGRAD), dtype=torch.float64, device="cuda")}
-    with pytest.raises(H.OpenSPSHipError) as e:
-        ctx.launch_grad(_dev(prm), out, {}, grads, precision=0)
-    assert e.value.status == 2
-    # weights_bar alone is computed with SFH parameters
     wb = {"weights_bar": torch.zeros((n, NT), dtype=torch.float64, device="cuda")}
     ctx.launch_grad(_dev(prm), out, {"mags": _dev(np.ones((n, NB)))}, wb, precision=0)
     ctx.sync()
     assert torch.isfinite(wb["weights_bar"]).all()
     ctx.close()
+
+
+SFH_ROWS = ["tage", "tau", "const", "fburst", "tburst", "sf_start", "sf_trunc"]
+
+
+def _sfh_prm(n, seed):
+    """SFH types 0, 1, 4 mixed, with const, bursts (before and after tage), start times and truncation."""
+    rng = np.random.default_rng(seed)
+    prm = synth.adjoint_params(n, seed=seed, birth_cloud=True)
+    prm[SLOT["sfh"]] = rng.choice([0, 1, 4], n)
+    tage = rng.uniform(0.2, 13.0, n)
+    prm[SLOT["tage"]] = tage
+    prm[SLOT["tau"]] = rng.uniform(0.2, 9.0, n)
+    prm[SLOT["const"]] = np.where(rng.random(n) < 0.5, rng.uniform(0.05, 0.4, n), 0.0)
+    prm[SLOT["fburst"]] = np.where(rng.random(n) < 0.5, rng.uniform(0.05, 0.4, n), 0.0)
+    prm[SLOT["tburst"]] = tage * rng.uniform(0.1, 1.3, n)
+    start = np.where(rng.random(n) < 0.3, tage * rng.uniform(0.05, 0.3, n), 0.0)
+    prm[SLOT["sf_start"]] = start
+    prm[SLOT["sf_trunc"]] = np.where(rng.random(n) < 0.4, start + (tage - start) * rng.uniform(0.2, 0.9, n), 0.0)
+    return prm
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("with_wbar", [False, True])
+def test_grad_sfh_rows(precision, with_wbar):
+    """params_bar with SFH parameters: the SFH tangents' rows (k_sfh_grad) against the oracle's batch_sfh_gradients,
+    with weights_bar requested or not (then the tangents read an internal d L / d w)."""
+    g, b = _setup()
+    n = 45
+    prm = _sfh_prm(n, 101)
+    cot = _cot(n, seed=13)
+    ref = O.csp_batch_grad(g, b, prm, mags_bar=cot["mags"], spec_bar=cot["spec"], mass_bar=cot["mass"],
+                           lbol_bar=cot["lbol"], cosmo=cosmo())
+    assert ref["status"] == 0
+    ctx = _ctx(g, b)
+    print(f"case: grad sfh rows n={n} precision={precision} with_wbar={with_wbar}", flush=True)
+    grads = ("params_bar", "weights_bar") if with_wbar else ("params_bar",)
+    out = ctx.batch_grad(_dev(prm), {k: _dev(v) for k, v in cot.items()}, precision=precision, split_parts=1,
+                         grads=grads)
+    got = {k: v.detach().cpu().numpy() for k, v in out.items()}
+    ctx.close()
+    _compare(ref, got, precision, ROWS + SFH_ROWS)
+    if with_wbar:
+        ew = _rel(got["weights_bar"], ref["weights_bar"])
+        print(f"  weights_bar {ew:.2e}")
+        assert ew < TOL[precision]["weights"]
 
 
 @pytest.mark.parametrize("precision", ["double", "mixed"])
@@ -222,3 +264,27 @@ def test_autograd_chi2():
     ew = _rel(w_t.grad.cpu().numpy(), ref["weights_z_bar"])
     print(f"  weights_z {ew:.2e}")
     assert ew < 1e-10
+
+
+def test_autograd_sfh_parameters():
+    """The torch.autograd.Function without weights: prm.grad in the SFH parameters' slots (the SFH tangents) against
+    the oracle's reverse pass."""
+    import torch
+    from libopensps_amd import autograd as AG
+    g, b = _setup()
+    n = 20
+    prm = _sfh_prm(n, 103)
+    ctx = _ctx(g, b)
+    print(f"case: autograd sfh n={n}", flush=True)
+    p_t = torch.tensor(prm, device="cuda", requires_grad=True)
+    mags = AG.CSPMags.apply(ctx, p_t, None, 0)
+    dmag = np.random.default_rng(3).standard_normal((n, NB))
+    (mags * torch.tensor(dmag, device="cuda")).sum().backward()
+    ctx.close()
+    ref = O.csp_batch_grad(g, b, prm, mags_bar=dmag, cosmo=cosmo())
+    for name in SFH_ROWS + ["sf_slope", "dust2", "logzsol"]:
+        a = p_t.grad[SLOT[name]].cpu().numpy()
+        e = ref["params_bar"][:, O.GRAD_ROWS[name]]
+        err = float(np.max(np.abs(a - e)) / (np.max(np.abs(e)) + 1e-300))
+        print(f"  {name:12s} {err:.2e}")
+        assert err < 1e-9 or np.all(a == e), name

@@ -152,3 +152,70 @@ def test_grad_forward_matches_batch():
     r = _grad(g, b, cz, prm, _cot(n, g), ssp_weights_z=wz)
     f = _fwd(g, b, cz, prm, ssp_weights_z=wz)
     np.testing.assert_allclose(r["mags"], f["mags"], rtol=0, atol=1e-9)
+
+
+def _sfh_models(sfh, g):
+    """Six models of SFH type sfh with const, a burst, a start time and truncation, their window ends and burst
+    ages away from the coarse test grid's ages (where the weights have kinks)."""
+    rng = np.random.default_rng(97 + sfh)
+    prm = _params(6, 61 + sfh, True)
+    prm[SLOT["sfh"]] = sfh
+    for m in range(6):
+        while True:
+            tage = rng.uniform(0.3, 12.0)
+            start = tage * rng.uniform(0.05, 0.3) if sfh != 0 and m % 2 else 0.0
+            trunc = start + (tage - start) * rng.uniform(0.2, 0.8) if sfh != 0 and m % 3 else 0.0
+            tburst = tage * rng.uniform(0.1, 0.9)
+            ends = [(tage - start) * 1e9, tage * 1e9, (tage - tburst) * 1e9]
+            if trunc > 0.0:
+                ends.append((tage - trunc) * 1e9)
+            if all(np.min(np.abs(g.log_age - np.log10(e))) > 2e-3 for e in ends):
+                break
+        prm[SLOT["tage"], m] = tage
+        prm[SLOT["tau"], m] = rng.uniform(0.3, 8.0)
+        prm[SLOT["const"], m] = rng.uniform(0.05, 0.3) if m % 2 == 0 else 0.0
+        prm[SLOT["fburst"], m] = rng.uniform(0.05, 0.3) if m < 4 else 0.0
+        prm[SLOT["tburst"], m] = tburst
+        prm[SLOT["sf_start"], m] = start
+        prm[SLOT["sf_trunc"], m] = trunc
+    return prm
+
+
+@pytest.mark.parametrize("sfh", [0, 1, 4])
+def test_sfh_rows(sfh):
+    """params_bar's SFH rows (batch_sfh_gradients, opensps_csp_batch.f90:5198-5217) against central differences of
+    the oracle's forward batch (h = 1e-4 relative: the SFH weights round at ~1e-10); const and fburst at zero against
+    the upward difference; sf_slope's row is zero (Simha's linear part only)."""
+    g, b, cz = _setup()
+    prm = _sfh_models(sfh, g)
+    n = prm.shape[1]
+    cot = _cot(n, g, seed=19)
+    an = _grad(g, b, cz, prm, cot)["params_bar"]
+    names = ["tage"] if sfh == 0 else ["tage", "tau", "const", "fburst", "tburst", "sf_start", "sf_trunc"]
+    for name in names:
+        s = SLOT[name]
+        x = prm[s].copy()
+        h = np.where(x != 0.0, np.abs(x) * 1e-4, 1e-5)
+        pp, pm = prm.copy(), prm.copy()
+        pp[s] = x + h
+        # upward at zero: const and fburst switch the mixture on there, a negative sf_start turns truncation on
+        pm[s] = np.where(x != 0.0, x - h, x)
+        fd = (_loss(_fwd(g, b, cz, pp), cot) - _loss(_fwd(g, b, cz, pm), cot)) / np.where(x != 0.0, 2 * h, h)
+        a = an[:, O.GRAD_ROWS[name]]
+        ok = (x != 0.0) | (name != "sf_trunc")
+        if name == "sf_trunc":   # zero at zero: no truncation (a small positive sf_trunc truncates almost all)
+            assert np.all(a[x == 0.0] == 0.0), name
+        _check(fd[ok], a[ok], 2e-3 if np.any(x[ok] == 0.0) else 1e-4, name)
+    for name in ("sf_slope",) + (("tau", "const", "fburst", "tburst", "sf_start", "sf_trunc") if sfh == 0 else ()):
+        assert np.all(an[:, O.GRAD_ROWS[name]] == 0.0), name
+
+
+def test_sfh_rows_not_with_given_weights():
+    """With weights given (ssp_weights or ssp_weights_z) the SFH rows stay zero (opensps_csp_batch.f90:1163-1166)."""
+    g, b, cz = _setup()
+    prm = _sfh_models(4, g)
+    n = prm.shape[1]
+    w = synth.dirichlet_weights(n, NT, seed=3, alpha=2.0)
+    an = _grad(g, b, cz, prm, _cot(n, g, seed=19), ssp_weights=w)["params_bar"]
+    for name in O.SFHT_PARAMS:
+        assert np.all(an[:, O.GRAD_ROWS[name]] == 0.0), name
