@@ -76,8 +76,19 @@ void opensps_context_get_zsol(int handle, double *z_sol, int *status);
 void opensps_context_get_lambda(int handle, double *lambda, int *status);
 void opensps_context_get_ages(int handle, double *log_age, int *status);
 void opensps_context_get_zlegend(int handle, double *z, int *status);
-/* wave_eff: pivot wavelengths; mag_vega, mag_sun are 99 (a synthetic basis has no Vega or solar spectrum). */
+/* wave_eff: pivot wavelengths; mag_vega, mag_sun: the AB magnitudes of Vega and the Sun per band (the Sun's in the
+ * Vega system when compute_vega_mags was 1 at set_filters), 99 without reference spectra
+ * (opensps_context_set_reference_spectra) or without flux. */
 void opensps_context_get_filter_data(int handle, double *wave_eff, double *mag_vega, double *mag_sun, int *status);
+/* No reference counterpart (the reference reads Vega and the Sun from its data release at set-up, load_standard_sed,
+ * src/io/opensps_io.f90:1516-1582): the reference spectra as tables, wavelength (Angstrom, increasing) and flux,
+ * F_lambda (flambda != 0; Vega's table) or f_nu (the Sun's). They are placed on the model grid as load_standard_sed
+ * places them (log-log interpolation, F_lambda times lambda^2, tiny redward of the table) whenever filters are
+ * installed, and give the Vega and solar zero points and the Vega system (set_int "compute_vega_mags" = 1). status 1
+ * bad handle, 3 fewer than 2 points, a non-increasing or non-positive wavelength, a negative or non-finite flux. */
+void opensps_context_set_reference_spectra(int handle, int n_vega, const double *lam_vega, const double *flux_vega,
+                                           int vega_flambda, int n_sun, const double *lam_sun, const double *flux_sun,
+                                           int sun_flambda, int *status);
 
 /* ---- filters (reference include/opensps.h:192-233) ---- */
 void opensps_filterset_create(int *fset, int *status);
@@ -92,8 +103,8 @@ void opensps_context_get_bands(int handle, double *bands, int *status);
 
 /* ---- the batch (reference include/opensps.h:264-316) ---- */
 /* status 0, 1 bad handle / no basis, 2 unsupported (a parameter that changes the SSPs, spectral indices, dust
- * emission, nebular emission, Vega magnitudes, an SFH type other than 0, 1, 4), 3 unknown parameter or bad
- * shape, 5 no HIP device, 10 device error. */
+ * emission, nebular emission, Vega magnitudes without reference spectra, an SFH type other than 0, 1, 4), 3 unknown
+ * parameter or bad shape, 5 no HIP device, 10 device error. */
 void opensps_context_compute_csp_batch(int handle, int n, int n_params, const char *const *names,
                                        const double *const *values, const double *ssp_weights, double *mags,
                                        double *spec, double *mass, double *lbol, double *mdust, double *sfr,
@@ -108,9 +119,9 @@ void opensps_context_compute_csp_batch_parts(int handle, int n, int n_params, co
  * of the loss whose cotangents are given (mags_bar nbands x n, spec_bar nspec x n, mass_bar, lbol_bar n; any
  * NULL) with respect to params_bar (NBGRAD x n), weights_bar (ntfull x n) and weights_z_bar (ntfull x nz x n).
  * ssp_weights_z (ntfull x nz x n) replaces ssp_weights and the metallicity blend. This build computes the rows of
- * dust1, dust2, dust3, dust_index, dust1_index, frac_obrun, frac_nodust, logzsol, zred, sigma_smooth and
- * igm_factor (the others are 0); params_bar with SFH parameters (no weights given), zred_mags and the SSP
- * operator's rows are status 2. _rows: params_bar's leading dimension (fewer than NBGRAD: status 3, nothing
+ * dust1, dust2, dust3, dust_index, dust1_index, frac_obrun, frac_nodust, logzsol, zred, sigma_smooth, igm_factor
+ * and, with SFH parameters (no weights given; sfh 0, 1, 4), tage, tau, const, fburst, tburst, sf_start, sf_trunc
+ * and sf_slope (the others are 0); zred_mags and the SSP operator's rows are status 2. _rows: params_bar's leading dimension (fewer than NBGRAD: status 3, nothing
  * written; rows beyond NBGRAD are left alone). */
 void opensps_context_compute_csp_batch_grad(int handle, int n, int n_params, const char *const *names,
                                             const double *const *values, const double *ssp_weights,

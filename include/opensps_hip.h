@@ -96,6 +96,12 @@ int opensps_hip_set_grid(opensps_hip_ctx_t *ctx, int nspec, const double *lam, i
 int opensps_hip_set_bands(opensps_hip_ctx_t *ctx, int nbands, const double *bands);
 /* The computed bands (opensps_context_set_mag_compute, include/opensps.h:233): mask[b] != 0; NULL: all. */
 int opensps_hip_set_mag_mask(opensps_hip_ctx_t *ctx, int nbands, const int *mask);
+/* The Vega system (magnitudes_from_fluxes, src/spectra/opensps_batch_photometry.f90:185-232, with
+ * compute_vega_mags = 1): magvega (host, nbands) Vega's AB magnitude per band (filters_zero_points,
+ * src/spectra/opensps_filters.f90:536-556), vref the reference band (1-based; 0: Vega is magnitude 0 in every band;
+ * k > 0: band k stays AB, is always computed, and the others are AB - magvega(b) + magvega(k)). magvega NULL or
+ * nbands 0: AB magnitudes. Call after opensps_hip_set_bands (which returns to AB). */
+int opensps_hip_set_vega(opensps_hip_ctx_t *ctx, int nbands, const double *magvega, int vref);
 /* The cosmology table (redshift, luminosity distance in pc), n rows, host (cosmospl,
  * src/physics/opensps_cosmology.f90:288-300). */
 int opensps_hip_set_cosmology(opensps_hip_ctx_t *ctx, int n, const double *z, const double *dl);

@@ -49,6 +49,7 @@ def lib():
         L.opensps_context_get_bands.argtypes = [i, vp, ip]
         L.opensps_context_get_params.argtypes = [i, vp, ip]
         L.opensps_context_get_filter_data.argtypes = [i, vp, vp, vp, ip]
+        L.opensps_context_set_reference_spectra.argtypes = [i, i, vp, vp, i, i, vp, vp, i, ip]
         L.opensps_filterset_create.argtypes = [ip, ip]
         L.opensps_filterset_add.argtypes = [i, cp, i, vp, vp, i, d, d, ip]
         L.opensps_filterset_count.argtypes = [i, ip, ip]
@@ -161,17 +162,32 @@ class Context:
         _check(st)
         self.nspec, self.nt, self.nz = grid.nspec, grid.nt, grid.nz
 
-    def set_filters(self, fset, resample=0, grid_policy=2):
+    def set_reference_spectra(self, vega, sun):
+        """vega, sun: (wavelength in Angstrom, flux, is_f_lambda) tables, e.g. datalib.reference_spectrum(root,
+        "vega"); they give the filters' Vega and solar zero points and the Vega system (compute_vega_mags)."""
+        (lv, fv, flv), (ls, fs, fls) = vega, sun
+        lv, fv, ls, fs = _d(lv), _d(fv), _d(ls), _d(fs)
+        st = C.c_int()
+        lib().opensps_context_set_reference_spectra(self.handle, lv.size, lv.ctypes.data, fv.ctypes.data, int(bool(flv)),
+                                                    ls.size, ls.ctypes.data, fs.ctypes.data, int(bool(fls)),
+                                                    C.byref(st))
+        _check(st)
+
+    def set_filters(self, fset, resample=0, grid_policy=2, vega_ref_band=0):
         fs, st = C.c_int(), C.c_int()
         lib().opensps_filterset_create(C.byref(fs), C.byref(st))
         _check(st)
+        nf = len(fset.names)
+        det = fset.detector if isinstance(fset.detector, (list, tuple, np.ndarray)) else [fset.detector] * nf
+        cal = getattr(fset, "cal", None) or [(0.0, 0.0)] * nf   # (lambda_ref, exponent) per filter; 0: none
         try:
-            for name, lam, resp in zip(fset.names, fset.lam, fset.resp):
+            for k, (name, lam, resp) in enumerate(zip(fset.names, fset.lam, fset.resp)):
                 l, r = _d(lam), _d(resp)
                 lib().opensps_filterset_add(fs.value, name.encode(), l.size, l.ctypes.data, r.ctypes.data,
-                                            int(fset.detector), 0.0, 0.0, C.byref(st))
+                                            int(det[k]), float(cal[k][0]), float(cal[k][1]), C.byref(st))
                 _check(st)
-            lib().opensps_context_set_filters(self.handle, fs.value, int(resample), int(grid_policy), 0, C.byref(st))
+            lib().opensps_context_set_filters(self.handle, fs.value, int(resample), int(grid_policy),
+                                              int(vega_ref_band), C.byref(st))
             _check(st)
         finally:
             lib().opensps_filterset_free(fs.value, C.byref(C.c_int()))
@@ -207,6 +223,15 @@ class Context:
         lib().opensps_context_get_filter_data(self.handle, out.ctypes.data, None, None, C.byref(st))
         _check(st)
         return out
+
+    def filter_data(self):
+        """(wave_eff, mag_vega, mag_sun) per band (opensps_context_get_filter_data)."""
+        out = np.zeros((3, self.nbands))
+        st = C.c_int()
+        lib().opensps_context_get_filter_data(self.handle, out[0].ctypes.data, out[1].ctypes.data, out[2].ctypes.data,
+                                              C.byref(st))
+        _check(st)
+        return out[0], out[1], out[2]
 
     # ---- batches ----
     def compute_csp_batch(self, values: dict, n=None, ssp_weights=None, want=("mags", "spec", "mass", "lbol"),

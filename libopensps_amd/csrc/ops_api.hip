@@ -147,7 +147,7 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
                   c->d_bc32, c->d_mask, c->d_cz, c->d_cdl, c->yo.p, c->wcol.p, c->w64.p, c->order.p, c->tiles.p,
                   c->misc.p, c->spec.p, c->flag.p, c->d_rdlam, c->d_ltab32, c->d_wl32, c->d_sm32, c->d_bkt, c->hdr.p, c->wsfh.p,
                   c->opu.p, c->opwc.p, c->opsum.p, c->opseg.p, c->opimf.p, c->opspart.p, c->d_bp_ptr, c->d_bp_band,
-                  c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p, c->gwb.p};
+                  c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p, c->gwb.p, c->d_magvega};
     for (void* p : ps) free_ptr(p);
     free_operator(c);
     free_full(c);
@@ -391,15 +391,46 @@ int opensps_hip_set_bands(opensps_hip_ctx_t* c, int nbands, const double* bands)
     UPLOAD(c, c->d_bc32, c32.data(), c32.size());
     c->h_mask.assign(nbands ? nbands : 1, 1);
     UPLOAD(c, c->d_mask, c->h_mask.data(), c->h_mask.size());
+    c->vega_on = 0;   // (the Vega system's zero points are per band: set again after new bands)
+    c->vega_ref = 0;
+    return 0;
+}
+
+// The device mask: the caller's, with the Vega system's reference band always computed (compute_magnitudes_columns,
+// src/spectra/opensps_batch_photometry.f90:105-110)
+static int upload_mask(OpsCtx* c) {
+    if (!c->nbands) return 0;
+    std::vector<int> m(c->h_mask.begin(), c->h_mask.begin() + c->nbands);
+    if (c->vega_on && c->vega_ref > 0) m[c->vega_ref - 1] = 1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_mask, m.data(), sizeof(int) * c->nbands, hipMemcpyHostToDevice));
     return 0;
 }
 
 int opensps_hip_set_mag_mask(opensps_hip_ctx_t* c, int nbands, const int* mask) {
     if (!c) return 1;
     if (nbands != c->nbands) return ops_fail(c, 3, "set_mag_mask: %d bands, context has %d", nbands, c->nbands);
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int b = 0; b < nbands; b++) c->h_mask[b] = mask ? (mask[b] != 0) : 1;
-    if (nbands) HIPCHK(c, hipMemcpy(c->d_mask, c->h_mask.data(), sizeof(int) * nbands, hipMemcpyHostToDevice));
+    return upload_mask(c);
+}
+
+int opensps_hip_set_vega(opensps_hip_ctx_t* c, int nbands, const double* magvega, int vref) {
+    if (!c) return 1;
+    if (!magvega || nbands == 0) {
+        const bool was = c->vega_on && c->vega_ref > 0;
+        c->vega_on = 0;
+        c->vega_ref = 0;
+        return was ? upload_mask(c) : 0;
+    }
+    if (nbands != c->nbands) return ops_fail(c, 3, "set_vega: %d bands, context has %d", nbands, c->nbands);
+    if (vref < 0 || vref > nbands) return ops_fail(c, 3, "set_vega: reference band %d of %d", vref, nbands);
+    int st;
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // (a batch in flight may still read the old zero points)
+    UPLOAD(c, c->d_magvega, magvega, (size_t)nbands);
+    c->vega_on = 1;
+    c->vega_ref = vref;
+    c->vega_anchor = vref > 0 ? magvega[vref - 1] : 0.0;
+    if ((st = upload_mask(c))) return st;
     return 0;
 }
 
@@ -1111,6 +1142,12 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     if (c->profiling) {
         HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
         c->ev_pending = true;
+    }
+    if (c->vega_on && c->nbands > 0) {   // the Vega system on the magnitudes (after the chain, on its stream)
+        const size_t cnt = (size_t)n * c->nbands;
+        k_vega_mags<<<(unsigned)((cnt + 255) / 256), 256, 0, c->stream>>>(n, c->nbands, a->mags, c->d_magvega,
+                                                                          c->vega_ref, c->vega_anchor);
+        HIPCHK(c, hipGetLastError());
     }
     return 0;
 }

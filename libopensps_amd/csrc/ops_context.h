@@ -84,7 +84,11 @@ struct OpsCtx {
     int* d_mask = nullptr;                         // per band: computed (1) or NaN (0)
     int *d_bp_ptr = nullptr, *d_bp_band = nullptr; // the band coefficients by pixel over [band_j0, band_j1] (CSR)
     double* d_bp_c64 = nullptr;
-    std::vector<int> h_mask;
+    std::vector<int> h_mask;                       // the caller's mask (the device's adds the Vega reference band)
+    // the Vega system (set_vega): per band Vega's AB magnitude, the 1-based reference band (0: none) and its value
+    double* d_magvega = nullptr;
+    int vega_on = 0, vega_ref = 0;
+    double vega_anchor = 0.0;
 
     // ---- cosmology ----
     int ncosmo = 0;

@@ -502,6 +502,25 @@ __global__ __launch_bounds__(256) void k_sfh_grad(SfhGradArgs a) {
     }
 }
 
+// The Vega system (magnitudes_from_fluxes, src/spectra/opensps_batch_photometry.f90:185-232), a thread per
+// (model, band) of mags (nbands, n): vref = 0: AB less Vega's AB magnitude in every band; vref > 0: band vref stays
+// AB and the others are AB - magvega(b) + magvega(vref), NaN where the model's vref magnitude is NaN. Band vref is
+// only read, so no thread reads a value another writes.
+__global__ void k_vega_mags(int n, int nb, double* mags, const double* magvega, int vref, double anchor) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * nb) return;
+    const int m = (int)(i / nb), b = (int)(i - (size_t)m * nb);
+    if (b == vref - 1) return;
+    double v = mags[i];
+    if (vref > 0) {
+        const double vv = mags[(size_t)m * nb + vref - 1];
+        v = vv == vv ? v - magvega[b] + anchor : (double)NAN;
+    } else {
+        v = v - magvega[b];
+    }
+    mags[i] = v;
+}
+
 // ------------------------------------------------------------------------------------------------
 // k_prep: one wavefront per model (4 per workgroup)
 // ------------------------------------------------------------------------------------------------

@@ -263,6 +263,7 @@ struct SfhGradArgs {
     double* params_bar;
 };
 __global__ void k_sfh_grad(SfhGradArgs a);
+__global__ void k_vega_mags(int n, int nb, double* mags, const double* magvega, int vref, double anchor);
 __global__ void k_pack_t(int nspec, int nsp, int kfull, int nrows, const double* basis, double* A64, uint16_t* Ah,
                          uint16_t* Al);
 

@@ -291,7 +291,8 @@ contains
         z(1:ctxs(handle)%nz) = ctxs(handle)%zlegend
     end subroutine
 
-    !> Effective (pivot) wavelengths; the Vega and solar magnitudes need spectra a synthetic basis does not have (99)
+    !> Effective (pivot) wavelengths and the AB magnitudes of Vega and of the Sun in every band (the Sun's in the Vega
+    !> system when compute_vega_mags was 1 at set_filters); 99 without reference spectra or without flux
     subroutine opensps_context_get_filter_data(handle, wave_eff, mag_vega, mag_sun, status) &
         bind(C, name="opensps_context_get_filter_data")
         integer(c_int), value :: handle
@@ -309,12 +310,75 @@ contains
         if (c_associated(mag_vega)) then
             call c_f_pointer(mag_vega, o, [nb])
             o = 99.0_wp
+            if (allocated(ctxs(handle)%magvega)) o = ctxs(handle)%magvega
         end if
         if (c_associated(mag_sun)) then
             call c_f_pointer(mag_sun, o, [nb])
             o = 99.0_wp
+            if (allocated(ctxs(handle)%magsun)) o = ctxs(handle)%magsun
         end if
     end subroutine
+
+    !> No reference counterpart (the reference reads Vega and the Sun from its data release at set-up,
+    !> load_standard_sed, src/io/opensps_io.f90:1516-1582): the reference spectra as tables, wavelength (Angstrom,
+    !> increasing) and flux, F_lambda (flambda != 0, Vega's table) or f_nu (the Sun's, at 10 pc). They are placed on
+    !> the model grid as the reference places them when filters are installed, and give the Vega and solar zero
+    !> points (opensps_context_get_filter_data) and the Vega system (compute_vega_mags = 1). status 1 bad handle, 3
+    !> fewer than 2 points, a non-increasing or non-positive wavelength, a negative or non-finite flux.
+    subroutine opensps_context_set_reference_spectra(handle, n_vega, lam_vega, flux_vega, vega_flambda, n_sun, &
+                                                     lam_sun, flux_sun, sun_flambda, status) &
+        bind(C, name="opensps_context_set_reference_spectra")
+        integer(c_int), value :: handle, n_vega, vega_flambda, n_sun, sun_flambda
+        real(c_double), intent(in) :: lam_vega(*), flux_vega(*), lam_sun(*), flux_sun(*)
+        integer(c_int), intent(out) :: status
+        if (bad_handle(handle, status)) return
+        if (.not. (table_ok(lam_vega(1:max(n_vega, 0)), flux_vega(1:max(n_vega, 0))) .and. &
+                   table_ok(lam_sun(1:max(n_sun, 0)), flux_sun(1:max(n_sun, 0))))) then
+            status = 3
+            call set_error(3, 'set_reference_spectra: a table has fewer than 2 points, a non-increasing or '// &
+                           'non-positive wavelength or a negative or non-finite flux')
+            return
+        end if
+        associate(c => ctxs(handle))
+            c%vega_lam = lam_vega(1:n_vega)
+            c%vega_flux = flux_vega(1:n_vega)
+            c%sun_lam = lam_sun(1:n_sun)
+            c%sun_flux = flux_sun(1:n_sun)
+            c%vega_flambda = vega_flambda /= 0
+            c%sun_flambda = sun_flambda /= 0
+            c%has_ref = .true.
+        end associate
+        if (ctxs(handle)%nbands > 0) call install_zero_points(handle)
+    contains
+        logical function table_ok(l, f)
+            real(c_double), intent(in) :: l(:), f(:)
+            table_ok = size(l) >= 2
+            if (.not. table_ok) return
+            table_ok = all(l == l) .and. all(f == f) .and. all(abs(l) <= huge(1.0_wp)) .and. &
+                       all(abs(f) <= huge(1.0_wp)) .and. all(l > 0.0_wp) .and. all(f >= 0.0_wp)
+            if (table_ok) table_ok = all(l(2:) > l(:size(l) - 1))
+        end function table_ok
+    end subroutine
+
+    !> The filters' zero points from the reference spectra (opensps_context_set_filters's filters_zero_points after
+    !> the calibration, src/spectra/opensps_context_filters.f90:113-124, with sun_to_vega_system when
+    !> compute_vega_mags is 1); without reference spectra none (99)
+    subroutine install_zero_points(handle)
+        integer(c_int), intent(in) :: handle
+        real(WP), allocatable :: spec(:)
+        associate(c => ctxs(handle))
+            if (allocated(c%magvega)) deallocate(c%magvega)
+            if (allocated(c%magsun)) deallocate(c%magsun)
+            c%dev_vega = .false.
+            if (.not. c%has_ref .or. c%nbands < 1) return
+            allocate(c%magvega(c%nbands), c%magsun(c%nbands), spec(c%nspec))
+            call ref_spectrum_on_grid(c%vega_lam, c%vega_flux, c%vega_flambda, c%lam, spec)
+            call zero_points(c%lam, spec, c%bands, c%magvega)
+            call ref_spectrum_on_grid(c%sun_lam, c%sun_flux, c%sun_flambda, c%lam, spec)
+            call zero_points(c%lam, spec, c%bands, c%magsun)
+            if (c%compute_vega_mags == 1) call sun_to_vega(c%magsun, c%magvega, c%vega_ref)
+        end associate
+    end subroutine install_zero_points
 
     ! ---- filters ------------------------------------------------------------------------------------------------
     subroutine opensps_filterset_create(fset, status) bind(C, name="opensps_filterset_create")
@@ -373,7 +437,8 @@ contains
 
     !> opensps_context_set_filters (include/opensps.h:209-219): status 301 no basis, 320 unusable curve, 321 a
     !> filter beyond the grid (grid_policy 0), 322 no throughput (grid_policy 0); the context keeps its filters on
-    !> an error. vega_ref_band is accepted for the interface (Vega magnitudes are not computed by this build).
+    !> an error. vega_ref_band: the Vega system's reference band (0: Vega is magnitude 0 in every band; k > 0: band k
+    !> stays AB and the others are shifted relative to it).
     subroutine opensps_context_set_filters(handle, fset, resample, grid_policy, vega_ref_band, status) &
         bind(C, name="opensps_context_set_filters")
         integer(c_int), value :: handle, fset, resample, grid_policy, vega_ref_band
@@ -428,8 +493,10 @@ contains
             end do
             c%resample = resample
             c%grid_policy = grid_policy
+            c%vega_ref = vega_ref_band
             c%dev_bands = .false.
         end associate
+        call install_zero_points(handle)
     end subroutine
 
     !> No reference counterpart: the context's bands on the model grid (nspec x nbands, column-major, normalised as

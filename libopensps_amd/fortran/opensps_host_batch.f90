@@ -112,9 +112,9 @@ contains
         else if (c%add_neb_emission == 1) then
             st = 2
             call set_error(2, 'batch: nebular emission is not part of this build')
-        else if (c%compute_vega_mags == 1) then
+        else if (c%compute_vega_mags == 1 .and. .not. allocated(c%magvega)) then
             st = 2
-            call set_error(2, 'batch: Vega magnitudes need a Vega spectrum, which a synthetic basis does not have')
+            call set_error(2, 'batch: Vega magnitudes need the Vega spectrum (opensps_context_set_reference_spectra)')
         else if (c%redshift_colors /= 0) then
             st = 2
             call set_error(2, 'batch: redshifted colours need zred_mags')
@@ -423,7 +423,7 @@ contains
             end if
             scale = 10.0_wp**(0.4_wp * (48.60_wp + 2.5_wp * amzl - dist))
             do b = 1, c%nbands
-                if (allocated(c%mag_mask)) then
+                if (allocated(c%mag_mask) .and. .not. (c%compute_vega_mags == 1 .and. b == c%vega_ref)) then
                     if (c%mag_mask(b) == 0) then
                         c%last_maggies(b, m) = ieee_nan()
                         cycle

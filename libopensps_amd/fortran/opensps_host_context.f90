@@ -35,12 +35,18 @@ module opensps_host_context
         real(WP), allocatable :: bands(:, :), wave_eff(:)
         integer, allocatable :: mag_mask(:), bflags(:)
         integer :: resample = 0, grid_policy = 2
+        ! the reference spectra as given (Vega, the Sun: wavelength, flux, F_lambda or f_nu) and the filters' zero
+        ! points from them (magvega, magsun: 99 without them), the Vega system's reference band
+        logical :: has_ref = .false., vega_flambda = .true., sun_flambda = .false.
+        real(WP), allocatable :: vega_lam(:), vega_flux(:), sun_lam(:), sun_flux(:), magvega(:), magsun(:)
+        integer :: vega_ref = 0
         ! cosmology
         real(WP) :: cz(500) = 0.0_wp, cdl(500) = 0.0_wp
         logical :: cosmo_dirty = .true.
         ! device side
         type(c_ptr) :: dctx = c_null_ptr
         logical :: dev_grid = .false., dev_bands = .false., dev_cosmo = .false., dev_mask = .false.
+        logical :: dev_vega = .false.
         type(dbuf_t) :: d_prm, d_w, d_mags, d_spec, d_mass, d_lbol, d_fband, d_young, d_old, d_idx
         integer :: last_n = 0, last_bytes = 8
         real(WP), allocatable :: last_maggies(:, :)
@@ -131,7 +137,7 @@ contains
         case ('smooth_velocity'); c%smooth_velocity = v; c%dev_grid = .false.
         case ('smoothspec_fast'); c%smoothspec_fast = v; c%dev_grid = .false.
         case ('dust_type'); c%dust_type = v
-        case ('compute_vega_mags'); c%compute_vega_mags = v
+        case ('compute_vega_mags'); c%compute_vega_mags = v; c%dev_vega = .false.
         case ('redshift_colors'); c%redshift_colors = v
         case ('keep_maggies'); c%keep_maggies = v
         case ('imf_type'); c%imf_type = v
@@ -245,7 +251,10 @@ contains
             ! a new grid invalidates the filters (they are built on the grid)
             c%nbands = 0
             if (allocated(c%bands)) deallocate(c%bands)
+            if (allocated(c%magvega)) deallocate(c%magvega)
+            if (allocated(c%magsun)) deallocate(c%magsun)
             c%dev_bands = .false.
+            c%dev_vega = .false.
         end associate
     end subroutine ctx_set_basis
 
@@ -296,7 +305,7 @@ contains
                 return
             end if
             c%dev_grid = .false.; c%dev_bands = .false.; c%dev_cosmo = .false.; c%dev_mask = .false.
-            c%dev_prof = .false.
+            c%dev_prof = .false.; c%dev_vega = .false.
         end if
         if (.not. c%dev_grid) then
             rc = opensps_hip_set_grid(c%dctx, int(c%nspec, c_int), c%lam, int(c%nt, c_int), c%log_age, &
@@ -325,6 +334,17 @@ contains
             rc = opensps_hip_set_mag_mask(c%dctx, int(c%nbands, c_int), mask)
             if (rc /= 0) goto 900
             c%dev_mask = .true.
+            c%dev_vega = .false.
+        end if
+        if (.not. c%dev_vega) then
+            ! the Vega system (magnitudes_from_fluxes, src/spectra/opensps_batch_photometry.f90:185-232)
+            if (c%compute_vega_mags == 1 .and. c%nbands > 0 .and. allocated(c%magvega)) then
+                rc = opensps_hip_set_vega(c%dctx, int(c%nbands, c_int), c%magvega, int(c%vega_ref, c_int))
+            else
+                rc = opensps_hip_set_vega(c%dctx, 0_c_int, [0.0_wp], 0_c_int)
+            end if
+            if (rc /= 0) goto 900
+            c%dev_vega = .true.
         end if
         if (.not. c%dev_prof) then
             rc = opensps_hip_set_profiling(c%dctx, int(c%profile, c_int))

@@ -31,7 +31,8 @@ module opensps_host_filters
 
     type(filter_set_t), allocatable, public, save :: fsets(:)
 
-    public :: fset_create, fset_add, fset_free, fset_valid, filters_build, trapezoid, band_pivot
+    public :: fset_create, fset_add, fset_free, fset_valid, filters_build, trapezoid, band_pivot, ref_spectrum_on_grid, &
+              zero_points, sun_to_vega
 
 contains
 
@@ -388,5 +389,61 @@ contains
             leff = sqrt(leff / d)
         end if
     end function band_pivot
+
+    !> A reference spectrum (Vega, the Sun) on the model grid as load_standard_sed forms it (src/io/opensps_io.f90:
+    !> 1516-1582): log10(flux + tiny) interpolated linearly in log10(lambda), the end intervals' slopes carried beyond
+    !> the table, then exponentiated; an F_lambda table is multiplied by lambda^2 (f_nu up to a constant); tiny
+    !> redward of the table's last wavelength.
+    subroutine ref_spectrum_on_grid(lt, ft, flambda, grid, spec)
+        real(WP), intent(in) :: lt(:), ft(:), grid(:)
+        logical, intent(in) :: flambda
+        real(WP), intent(out) :: spec(:)
+        real(WP), allocatable :: x(:), y(:)
+        integer :: j, n
+        n = size(lt)
+        x = log10(lt)
+        y = log10(ft + SAFE_FLOOR)
+        do j = 1, size(grid)
+            spec(j) = 10.0_wp**interp_lin(x, y, n, log10(grid(j)))
+        end do
+        if (flambda) spec = spec * grid**2
+        where (grid > maxval(lt)) spec = SAFE_FLOOR
+    end subroutine ref_spectrum_on_grid
+
+    !> filters_zero_points (opensps_filters.f90:536-556) for one spectrum: the AB magnitude -2.5 log10(integral
+    !> f_nu T / lambda d lambda) - 48.60 of f_nu on the grid in every band, 99 where it has no flux
+    subroutine zero_points(grid, spec, bands, mag)
+        real(WP), intent(in) :: grid(:), spec(:), bands(:, :)
+        real(WP), intent(out) :: mag(:)
+        integer :: i
+        do i = 1, size(bands, 2)
+            mag(i) = trapezoid(grid, spec * bands(:, i) / grid)
+            if (mag(i) > SAFE_FLOOR) then
+                mag(i) = -2.5_wp * log10(mag(i)) - 48.60_wp
+            else
+                mag(i) = 99.0_wp
+            end if
+        end do
+    end subroutine zero_points
+
+    !> sun_to_vega_system (src/spectra/opensps_context_filters.f90:185-198): the Sun's magnitudes in the Vega system,
+    !> anchored on band vref (0: Vega is magnitude 0 in every band); bands without flux (99) stay 99
+    subroutine sun_to_vega(magsun, magvega, vref)
+        real(WP), intent(inout) :: magsun(:)
+        real(WP), intent(in) :: magvega(:)
+        integer, intent(in) :: vref
+        real(WP) :: s0
+        integer :: i
+        s0 = 0.0_wp
+        if (vref > 0) s0 = magsun(vref)
+        do i = 1, size(magsun)
+            if (magsun(i) == 99.0_wp .or. magvega(i) == 99.0_wp) cycle
+            if (vref > 0) then
+                magsun(i) = (magsun(i) - s0) - (magvega(i) - magvega(vref)) + s0
+            else
+                magsun(i) = magsun(i) - magvega(i)
+            end if
+        end do
+    end subroutine sun_to_vega
 
 end module opensps_host_filters

@@ -208,6 +208,63 @@ def build_bands(fset, grid, mode=0):
     return out
 
 
+# ---- the reference spectra and the Vega system (numpy restatement) ----
+TINY = np.finfo(np.float64).tiny
+
+
+def ref_spectrum_on_grid(lt, ft, flambda, grid):
+    """load_standard_sed (src/io/opensps_io.f90:1516-1582): log10(flux + tiny) linear in log10(lambda) with the
+    end intervals' slopes beyond the table (interpolate_linear, src/math/opensps_interpolation.f90:87-130), then
+    10^; an F_lambda table times lambda^2; tiny redward of the table."""
+    x, y = np.log10(np.asarray(lt, float)), np.log10(np.asarray(ft, float) + TINY)
+    g = np.asarray(grid, float)
+    xo = np.log10(g)
+    i = np.clip(np.searchsorted(x, xo, side="right") - 1, 0, x.size - 2)
+    i = np.where(xo == x[-1], x.size - 2, i)
+    slope = (y[i + 1] - y[i]) / (x[i + 1] - x[i])
+    spec = 10.0 ** (y[i] + slope * (xo - x[i]))
+    if flambda:
+        spec = spec * g ** 2
+    return np.where(g > np.max(lt), TINY, spec)
+
+
+def zero_points(grid, spec, bands):
+    """filters_zero_points (src/spectra/opensps_filters.f90:536-556): -2.5 log10(trapezoid(f_nu T / lambda)) - 48.60
+    per band (bands (nbands, nspec)), 99 without flux."""
+    g = np.asarray(grid, float)
+    out = np.empty(len(bands))
+    for b, t in enumerate(bands):
+        y = spec * t / g
+        v = float(np.sum(0.5 * (g[1:] - g[:-1]) * (y[1:] + y[:-1])))
+        out[b] = -2.5 * np.log10(v) - 48.60 if v > TINY else 99.0
+    return out
+
+
+def sun_to_vega(magsun, magvega, vref):
+    """sun_to_vega_system (src/spectra/opensps_context_filters.f90:185-198)."""
+    ms = np.array(magsun, float)
+    s0 = ms[vref - 1] if vref > 0 else 0.0
+    for i in range(ms.size):
+        if ms[i] == 99.0 or magvega[i] == 99.0:
+            continue
+        ms[i] = (ms[i] - s0) - (magvega[i] - magvega[vref - 1]) + s0 if vref > 0 else ms[i] - magvega[i]
+    return ms
+
+
+def vega_mags(mags_ab, magvega, vref):
+    """The Vega system on AB magnitudes (n, nbands) (magnitudes_from_fluxes, src/spectra/opensps_batch_photometry.f90:
+    185-232): vref = 0: AB - magvega; vref = k: band k stays AB, the others AB - magvega + magvega(k), NaN where the
+    model's band k is NaN."""
+    m = np.array(mags_ab, float)
+    mv = np.asarray(magvega, float)
+    if vref == 0:
+        return m - mv[None, :]
+    out = m - mv[None, :] + mv[vref - 1]
+    out[~np.isfinite(m[:, vref - 1])] = np.nan
+    out[:, vref - 1] = m[:, vref - 1]
+    return out
+
+
 # The packed IMF parameters of the oracle (csp_oracle.c ORC_IMF_*)
 IMF_SALP, IMF_ALPHA, IMF_VDMC, IMF_MDAVE, IMF_LOWER_LIMIT, IMF_UPPER_LIMIT, IMF_LOWER_BOUND, IMF_MLIM_BH, \
     IMF_MLIM_NS, IMF_NPAR = 0, 1, 4, 5, 6, 7, 8, 9, 10, 11

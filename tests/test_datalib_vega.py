@@ -1,0 +1,156 @@
+"""The standard data-library reader (libopensps_amd.datalib) on the reference's own test data release (the subset
+under tests/golden/opensps-test-data), the reference spectra on the model grid, the filters' Vega and solar zero
+points and the Vega system's host side. The oracle's restatement (oracle.ref_spectrum_on_grid, zero_points,
+sun_to_vega, vega_mags) is pinned here by what the mathematics fixes: an interpolation that returns the table at its
+nodes and is exact on a power law, F_lambda ~ lambda^-2 giving a flat f_nu, a flat 3631 Jy source at AB magnitude 0,
+and the reference's own checks of the Vega system (tests/test_filters.f90:564-593 of the reference)."""
+import hashlib
+import json
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from helpers import grid
+from libopensps_amd import api, datalib, synth
+
+import oracle as O
+
+ROOT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "opensps-test-data")
+
+
+def test_reads_the_test_release():
+    lam, flux, fl = datalib.reference_spectrum(ROOT, "vega")
+    assert fl and lam.size == flux.size == 1221 and np.all(np.diff(lam) > 0) and np.all(flux >= 0)
+    lam_s, flux_s, fl_s = datalib.reference_spectrum(ROOT, "sun")
+    assert not fl_s and lam_s.size == flux_s.size and np.all(np.diff(lam_s) > 0)
+    fl = datalib.filter_library(ROOT, "allfilters")
+    assert len(fl.names) == 159 and fl.names[0] == "V"
+    assert all(np.all(np.diff(l) != 0) for l in fl.lam)
+    cal = {n: c for n, c in zip(fl.names, fl.cal) if c[0] > 0}
+    assert cal["IRAC_1"] == (35500.0, -1.0)
+    assert set(fl.detector) <= {0, 1}
+
+
+def test_rejects_damaged_libraries(tmp_path):
+    root = tmp_path / "rel"
+    shutil.copytree(ROOT, root)
+    d = root / "libraries" / "reference_spectra" / "vega"
+    raw = (d / "flux.bin").read_bytes()
+    (d / "flux.bin").write_bytes(raw[:-4])   # one value short
+    with pytest.raises(datalib.DataLibraryError, match="bytes"):
+        datalib.reference_spectrum(str(root), "vega")
+    (d / "flux.bin").write_bytes(raw[:-4] + b"\0\0\0\0")   # right size, other bytes
+    with pytest.raises(datalib.DataLibraryError, match="SHA-256"):
+        datalib.reference_spectrum(str(root), "vega")
+    datalib.reference_spectrum(str(root), "vega", verify=False)
+    with pytest.raises(datalib.DataLibraryError, match="no library"):
+        datalib.reference_spectrum(str(root), "altair")
+    idx = json.loads((root / "opensps-data.json").read_text())
+    idx["format_version"] = 2
+    (root / "opensps-data.json").write_text(json.dumps(idx))
+    with pytest.raises(datalib.DataLibraryError, match="v1"):
+        datalib.open_root(str(root))
+    assert hashlib.sha256(raw).hexdigest() == json.loads((d / "manifest.json").read_text())["arrays"]["flux"]["sha256"]
+
+
+def test_oracle_reference_spectrum_pins():
+    g = np.geomspace(500.0, 5.0e4, 400)
+    lt = np.geomspace(1000.0, 2.0e4, 37)
+    # a power law in the table is a straight line in log-log: exact inside and (extrapolated) blueward
+    ft = 3.0e-9 * (lt / 5500.0) ** -2.7
+    s = O.ref_spectrum_on_grid(lt, ft, False, g)
+    inside = g <= lt[-1]
+    np.testing.assert_allclose(s[inside], 3.0e-9 * (g[inside] / 5500.0) ** -2.7, rtol=1e-12)
+    assert np.all(s[~inside] == O.TINY)
+    # the table's own nodes come back, F_lambda ~ lambda^-2 is a flat f_nu
+    s = O.ref_spectrum_on_grid(lt, ft, False, lt)
+    np.testing.assert_allclose(s, ft, rtol=1e-13)
+    s = O.ref_spectrum_on_grid(lt, 7.0 * lt ** -2.0, True, g[g <= lt[-1]])
+    np.testing.assert_allclose(s, 7.0, rtol=1e-12)
+
+
+def test_oracle_zero_point_of_a_flat_source():
+    """A flat f_nu of 3631 Jy is AB magnitude 0 in every normalised band (photon or energy counting)."""
+    g = grid(1)
+    fs = synth.gaussian_filters(12, seed=3)
+    bands = O.build_bands(fs, g.lam)
+    m = O.zero_points(g.lam, np.full(g.lam.size, 3631.0e-23), bands)
+    np.testing.assert_allclose(m, -2.5 * np.log10(3631.0e-23) - 48.60, atol=1e-12)
+    assert np.all(np.abs(m) < 1e-3)
+    assert np.all(O.zero_points(g.lam, np.zeros(g.lam.size), bands) == 99.0)
+
+
+def test_oracle_vega_system_pins():
+    """The reference's checks (tests/test_filters.f90:575-593): vref = 0 is AB - magvega in every band; with a
+    reference band that band stays AB and the others are shifted relative to it."""
+    rng = np.random.default_rng(2)
+    ab = rng.uniform(15, 25, (5, 6))
+    ab[3, 1] = np.nan
+    mv = rng.uniform(-1, 1, 6)
+    np.testing.assert_array_equal(O.vega_mags(ab, mv, 0), ab - mv)
+    v = O.vega_mags(ab, mv, 2)
+    np.testing.assert_array_equal(v[:, 1], ab[:, 1])
+    ok = np.isfinite(ab[:, 1])
+    np.testing.assert_allclose(v[ok][:, [0, 2, 3, 4, 5]], (ab - mv + mv[1])[ok][:, [0, 2, 3, 4, 5]], rtol=0, atol=1e-12)
+    assert np.all(np.isnan(v[3, [0, 2, 3, 4, 5]]))
+
+
+def _refs():
+    return datalib.reference_spectrum(ROOT, "vega"), datalib.reference_spectrum(ROOT, "sun")
+
+
+@pytest.mark.parametrize("resample", [0, 1])
+def test_host_zero_points(resample):
+    """The Fortran host's Vega and solar zero points (opensps_context_get_filter_data) equal the oracle's from the
+    same bands; without reference spectra they are 99; the Sun in the Vega system follows sun_to_vega_system."""
+    g = grid(1)
+    fs = synth.gaussian_filters(10, seed=5)
+    vega, sun = _refs()
+    with api.Context() as ctx:
+        ctx.set_ssp_basis(g)
+        ctx.set_filters(fs, resample=resample)
+        _, mv, ms = ctx.filter_data()
+        assert np.all(mv == 99.0) and np.all(ms == 99.0)
+        ctx.set_reference_spectra(vega, sun)          # after the filters: the zero points follow
+        _, mv, ms = ctx.filter_data()
+        bands = ctx.bands()
+        rv = O.zero_points(g.lam, O.ref_spectrum_on_grid(*vega, g.lam), bands)
+        rs = O.zero_points(g.lam, O.ref_spectrum_on_grid(*sun, g.lam), bands)
+        np.testing.assert_allclose(mv, rv, rtol=0, atol=1e-11)
+        np.testing.assert_allclose(ms, rs, rtol=0, atol=1e-11)
+        ctx.set_int("compute_vega_mags", 1)
+        for vref in (0, 3):
+            ctx.set_filters(fs, resample=resample, vega_ref_band=vref)
+            _, mv2, ms2 = ctx.filter_data()
+            np.testing.assert_array_equal(mv2, mv)
+            np.testing.assert_allclose(ms2, O.sun_to_vega(rs, rv, vref), rtol=0, atol=1e-11)
+
+
+def test_host_reference_spectra_errors():
+    g = grid(1)
+    vega, sun = _refs()
+    with api.Context() as ctx:
+        ctx.set_ssp_basis(g)
+        for bad in ((vega[0][::-1].copy(), vega[1], True), (vega[0][:1], vega[1][:1], True),
+                    (vega[0], -vega[1], True), (np.where(vega[0] > 5000, np.nan, vega[0]), vega[1], True)):
+            with pytest.raises(api.OpenSPSError) as e:
+                ctx.set_reference_spectra(bad, sun)
+            assert e.value.status == 3
+        ctx.set_filters(synth.gaussian_filters(4))
+        _, mv, _ = ctx.filter_data()
+        assert np.all(mv == 99.0)
+
+
+def test_vega_batch_needs_reference_spectra():
+    """compute_vega_mags = 1 without reference spectra is status 2 before any device work."""
+    g = grid(1)
+    with api.Context() as ctx:
+        ctx.set_ssp_basis(g)
+        ctx.set_filters(synth.gaussian_filters(4))
+        ctx.set_int("add_dust_emission", 0)
+        ctx.set_int("compute_vega_mags", 1)
+        r = ctx.compute_csp_batch({"tage": np.full(2, 5.0), "sfh": np.full(2, 4.0)}, raw_status=True)
+        assert r["status"] == 2
+        assert "Vega" in api.last_error()

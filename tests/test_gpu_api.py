@@ -137,3 +137,51 @@ def test_api_device_buffers_checked():
         ctx.compute_csp_batch_device(d_prm, d_mags, d_spec=f32)   # the matching buffers still run
         torch.cuda.synchronize()
         assert np.isfinite(d_mags.cpu().numpy()).any()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_api_vega_system(precision):
+    """compute_vega_mags = 1 with the reference spectra of the test data release (tests/golden/opensps-test-data):
+    vref = 0 gives AB - magvega in every band, vref = 3 keeps band 3 in AB (computed even where the mask leaves it
+    out) and shifts the others relative to it (the reference's tests/test_filters.f90:575-593 and
+    test_c_filters.c:120-128); the linear fluxes stay AB; the gradients' forward carries the Vega magnitudes."""
+    import os
+    from libopensps_amd import datalib
+    root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "opensps-test-data")
+    vega, sun = datalib.reference_spectrum(root, "vega"), datalib.reference_spectrum(root, "sun")
+    g = grid(1)
+    fs = synth.gaussian_filters(12, seed=9)
+    n = 40
+    prm = synth.parametric_params(n, seed=29)
+    names = synth.varied_names()
+    print(f"case: api vega n={n} {precision}", flush=True)
+    with _ctx(g, fs, precision) as ctx:
+        ctx.set_reference_spectra(vega, sun)
+        ctx.set_int("keep_maggies", 1)
+        ab = ctx.compute_csp_batch_params(prm, names, want=("mags",))["mags"]
+        mg_ab = ctx.batch_maggies()
+        _, mv, _ = ctx.filter_data()
+        assert np.all(np.abs(mv) < 99.0)
+        ctx.set_int("compute_vega_mags", 1)
+        v0 = ctx.compute_csp_batch_params(prm, names, want=("mags",))["mags"]
+        mg_v = ctx.batch_maggies()
+        np.testing.assert_allclose(v0, O.vega_mags(ab, mv, 0), rtol=0, atol=1e-12)
+        np.testing.assert_allclose(mg_v, mg_ab, rtol=1e-14, atol=0)
+        ctx.set_filters(fs, vega_ref_band=3)
+        mask = np.ones(12, dtype=np.int32)
+        mask[2] = 0
+        mask[5] = 0
+        ctx.set_mag_compute(mask)
+        v3 = ctx.compute_csp_batch_params(prm, names, want=("mags",))["mags"]
+        ref3 = O.vega_mags(ab, mv, 3)
+        ref3[:, 5] = np.nan
+        np.testing.assert_array_equal(np.isnan(v3), np.isnan(ref3))
+        ok = ~np.isnan(ref3)
+        np.testing.assert_allclose(v3[ok], ref3[ok], rtol=0, atol=1e-12)
+        # the reverse pass's forward half: the same Vega magnitudes (its young and old sums are always apart: with
+        # FP32 spectra they round differently from the single sum), finite gradients
+        r = ctx.compute_csp_batch_grad({k: prm[SLOT[k]] for k in names}, n=n, mags_bar=np.ones((n, 12)),
+                                       want=("mags",), grads=("params_bar",))
+        np.testing.assert_allclose(r["mags"][ok], ref3[ok], rtol=0, atol=1e-12 if precision == "double" else 1e-6)
+        assert np.all(np.isfinite(r["params_bar"]))
