@@ -103,7 +103,7 @@ void opensps_context_get_bands(int handle, double *bands, int *status);
 
 /* ---- the batch (reference include/opensps.h:264-316) ---- */
 /* status 0, 1 bad handle / no basis, 2 unsupported (a parameter that changes the SSPs, spectral indices, dust
- * emission, nebular emission, Vega magnitudes without reference spectra, an SFH type other than 0, 1, 4), 3 unknown
+ * emission, nebular emission, Vega magnitudes without reference spectra, an SFH type other than 0, 1, 4, 5), 3 unknown
  * parameter or bad shape, 5 no HIP device, 10 device error. */
 void opensps_context_compute_csp_batch(int handle, int n, int n_params, const char *const *names,
                                        const double *const *values, const double *ssp_weights, double *mags,
@@ -120,7 +120,7 @@ void opensps_context_compute_csp_batch_parts(int handle, int n, int n_params, co
  * NULL) with respect to params_bar (NBGRAD x n), weights_bar (ntfull x n) and weights_z_bar (ntfull x nz x n).
  * ssp_weights_z (ntfull x nz x n) replaces ssp_weights and the metallicity blend. This build computes the rows of
  * dust1, dust2, dust3, dust_index, dust1_index, frac_obrun, frac_nodust, logzsol, zred, sigma_smooth, igm_factor
- * and, with SFH parameters (no weights given; sfh 0, 1, 4), tage, tau, const, fburst, tburst, sf_start, sf_trunc
+ * and, with SFH parameters (no weights given; sfh 0, 1, 4, 5), tage, tau, const, fburst, tburst, sf_start, sf_trunc
  * and sf_slope (the others are 0); zred_mags and the SSP operator's rows are status 2. _rows: params_bar's leading dimension (fewer than NBGRAD: status 3, nothing
  * written; rows beyond NBGRAD are left alone). */
 void opensps_context_compute_csp_batch_grad(int handle, int n, int n_params, const char *const *names,

@@ -66,7 +66,7 @@ typedef struct {
  * (opensps_csp_batch.f90:166-213) that this path computes (dust1, dust2, dust3, dust_index, dust1_index,
  * frac_obrun, frac_nodust, logzsol, zred, sigma_smooth, igm_factor and, with SFH parameters (no given weights),
  * the SFH rows tage, tau, const, fburst, tburst, sf_start, sf_trunc, sf_slope of batch_sfh_gradients
- * (opensps_csp_batch.f90:5198-5217, SFHT_ROWS :215; sfh 0, 1, 4); the others are 0), weights_bar (nt, n) (with
+ * (opensps_csp_batch.f90:5198-5217, SFHT_ROWS :215; sfh 0, 1, 4, 5); the others are 0), weights_bar (nt, n) (with
  * fwd.ssp_weights or SFH parameters), weights_z_bar (nt, nz, n) (with fwd.ssp_weights_z). */
 #define OPENSPS_HIP_NBGRAD 44
 typedef struct {

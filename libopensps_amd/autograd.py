@@ -6,7 +6,7 @@ recomputes the transmissions rather than storing them).
     mags = CSPMags.apply(dctx, prm, weights_z, precision)
     loss = ((mags - obs) ** 2).sum(); loss.backward()   # prm.grad (NPAR, n) in the parameter slots, weights_z.grad
 
-With weights None the batch forms the SFH weights from the SFH parameters (sfh 0, 1, 4) and prm.grad also holds
+With weights None the batch forms the SFH weights from the SFH parameters (sfh 0, 1, 4, 5) and prm.grad also holds
 their rows (tage, tau, const, fburst, tburst, sf_start, sf_trunc, sf_slope: the SFH tangents).
 
 The device buffers stay torch tensors on the GPU; the kernels are libopensps_hip's (no torch arithmetic on the

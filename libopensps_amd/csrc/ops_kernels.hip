@@ -22,6 +22,7 @@
 struct SfhP {
     double tau, tage, tburst, sf_trunc, sf_slope, tq, t0, tb;
     int type;
+    int simha;   // use_simha_limits: the window is [t0, tq] (Simha's linear part), else [tq, tage]
 };
 
 // The series' term ratios k / (k + 1)^2, k = 0 .. 1000 (correctly rounded at compile time): the series below
@@ -90,15 +91,21 @@ __device__ void d_convert_sfh(const double* p, SfhP& s) {   // convert_sfhparams
     s.t0 = (s.sf_slope > OPS_SAFE_FLOOR) ? s.tq - 1.0 / s.sf_slope : 0.0;
     if (s.t0 > s.tq || s.t0 <= 0.0) s.t0 = 0.0;
     s.type = 0;
+    s.simha = 0;
 }
 
-// eval_indefinite_moments_log (opensps_sfh.f90:852-910), types 0, 1, 4
+// eval_indefinite_moments_log (opensps_sfh.f90:852-910), types 0, 1, 4, 5
 __device__ void d_moments(double log_t, const SfhP& s, double& m0, double& m1) {
     const double LOG_E = 0.4342944819032518;   // log10(e)
     double t_lin = pow(10.0, log_t);
     if (s.type == 0) {
         m0 = t_lin;
         m1 = t_lin * (log_t - LOG_E);
+    } else if (s.type == 5) {   // Simha's linear part: SFR ~ 1 - slope (tq - t) in lookback time
+        const double tp = fmax(0.0, s.tage - s.sf_trunc), sf = 1.0 - s.sf_slope * tp;
+        const double q = s.sf_slope * (t_lin * t_lin) / 2.0;
+        m0 = sf * t_lin + q;
+        m1 = sf * t_lin * (log_t - LOG_E) + q * (log_t - 0.5 * LOG_E);
     } else if (s.type == 1 || s.type == 4) {
         if (s.tau <= OPS_SAFE_FLOOR) { m0 = 0.0; m1 = 0.0; return; }
         double tt = fmax(t_lin / s.tau, OPS_SAFE_FLOOR);
@@ -142,7 +149,7 @@ __device__ void wave_ssp_weights(const double* __restrict__ tf, int nt, double t
         return;
     }
     double lmin = pow(10.0, tiny_logt);
-    double lts = log10(fmax(s.tq, lmin)), lte = log10(fmax(s.tage, lmin));
+    double lts = log10(fmax(s.simha ? s.t0 : s.tq, lmin)), lte = log10(fmax(s.simha ? s.tq : s.tage, lmin));
     int js = wave_find_interval(tf, nt, lts, lane);
     int je = wave_find_interval(tf, nt, lte, lane) + 1;
     js = max(js, max(idx_min, 1));
@@ -189,7 +196,34 @@ __device__ void wave_ssp_weights(const double* __restrict__ tf, int nt, double t
     WAVE_SYNC();
 }
 
-// sfh_weights_core_pc (opensps_sfh.f90:1077-1294) for sfh 0, 1, 4. Returns 0, or 2 (unsupported).
+// expm1 as the reference forms it (opensps_special_functions.f90:219-233) and gammainc(2, x) (:142-167)
+__device__ double d_expm1_ref(double x) {
+    if (fabs(x) < 1.0e-5) return x * (1.0 + x * (0.5 + x * (1.0 / 6.0)));
+    return exp(x) - 1.0;
+}
+__device__ double d_gammainc2(double x) { return x < 0.0 ? 0.0 : -d_expm1_ref(-x) - x * exp(-x); }
+
+// The Simha SFH's linear mass fraction (get_sfh_properties_at_age_pc's frac_linear for sfh = 5,
+// opensps_sfh.f90:327-455), Gyr; log_tmax: the oldest grid age (log10 yr)
+__device__ double d_simha_frac(const double* p, double log_tmax) {
+    const double sf_start = p[OPS_P_SF_START], sf_trunc = p[OPS_P_SF_TRUNC];
+    const double t_max = pow(10.0, log_tmax - 9.0) - sf_start, t_prime = p[OPS_P_TAGE] - sf_start;
+    const double t_trunc = (sf_trunc < OPS_SAFE_FLOOR || sf_trunc < sf_start) ? t_max : sf_trunc - sf_start;
+    const double tau = fmax(p[OPS_P_TAU], OPS_SAFE_FLOOR);
+    const double mass_tau = t_prime >= 0.0 ? tau * d_gammainc2(fmin(t_prime, t_trunc) / tau) : 0.0;
+    const double slope_m = -p[OPS_P_SF_SLOPE];
+    const double t_zero = slope_m > 0.0 ? t_trunc + 1.0 / slope_m : t_max;
+    double mass_linear = 0.0;
+    if (!(t_trunc <= 0.0 || t_trunc > t_max) && t_prime > t_trunc) {
+        const double sfr_trunc = (t_trunc / p[OPS_P_TAU]) * exp(-t_trunc / p[OPS_P_TAU]);
+        const double t_hi = fmin(t_zero, t_prime);
+        mass_linear = sfr_trunc * ((t_hi - t_trunc) - (slope_m / 2.0) * (t_hi * t_hi + t_trunc * t_trunc) +
+                                   slope_m * t_hi * t_trunc);
+    }
+    return (t_prime >= 0.0 && mass_linear + mass_tau > OPS_SAFE_FLOOR) ? mass_linear / (mass_linear + mass_tau) : 0.0;
+}
+
+// sfh_weights_core_pc (opensps_sfh.f90:1077-1294) for sfh 0, 1, 4, 5. Returns 0, or 2 (unsupported).
 __device__ int wave_sfh_weights(const double* __restrict__ tf, int nt, double tiny_logt, const double* p, double* w,
                                 double* t1, double* t2, double* m0s, double* m1s, int lane) {
     SfhP s;
@@ -201,6 +235,22 @@ __device__ int wave_sfh_weights(const double* __restrict__ tf, int nt, double ti
         s.type = -1;
         s.tb = s.tage;
         wave_ssp_weights(tf, nt, tiny_logt, s, max(imax - 2, 1), imax, w, m0s, m1s, lane);
+        return 0;
+    }
+    if (sfh == 5) {   // Simha: the delayed-tau part over [tq, tage], the linear part over [t0, tq], mixed
+        s.type = 4;
+        wave_ssp_weights(tf, nt, tiny_logt, s, 0, imax, t1, m0s, m1s, lane);
+        double m1 = 0.0, m2 = 0.0;
+        for (int i = 0; i < imax; i++) m1 = m1 + t1[i];
+        s.type = 5;
+        s.simha = 1;
+        wave_ssp_weights(tf, nt, tiny_logt, s, 0, imax, t2, m0s, m1s, lane);
+        for (int i = 0; i < imax; i++) m2 = m2 + t2[i];
+        if (m1 < OPS_SAFE_FLOOR) m1 = 1.0;
+        if (m2 < OPS_SAFE_FLOOR) m2 = 1.0;
+        const double fl = d_simha_frac(p, tf[nt - 1]);
+        for (int i = lane; i < nt; i += 64) w[i] = (t1[i] / m1) * (1.0 - fl) + (t2[i] / m2) * fl;
+        WAVE_SYNC();
         return 0;
     }
     if (sfh != 1 && sfh != 4) return 2;
@@ -242,15 +292,17 @@ __device__ int wave_sfh_weights(const double* __restrict__ tf, int nt, double ti
 // (per year) and the eight rows are combinations of their sums S_e = sum_k wbar_k d w_k / d e (G = 1e9 yr / Gyr):
 //   tage = G (S_tage + S_tb + [tq > 0] S_tq)   tau = G S_tau       tburst = -G S_tb      sf_start = -G S_tage
 //   sf_trunc = -G [tq > 0] S_tq               const, fburst: the mixture's own terms    sf_slope = 0
-// (the slope acts on Simha's linear part only, sfh = 5, which the forward does not compute). sfh = 0 is a burst
-// at tb = tage: tage's row only.
+// (the slope acts on Simha's linear part only). sfh = 0 is a burst at tb = tage: tage's row only. Simha (sfh = 5)
+// mixes a delayed-tau part (along tage, tau, tq) and a linear part over [t0, tq] (along tq and its slope) by the
+// linear part's mass fraction f, whose derivatives (simha_frac_t) add (S_lin - S_tau) df / d theta.
 // ------------------------------------------------------------------------------------------------
 
-// moments_t (opensps_sfh_tangent.f90:426-467), logarithmic branch, types 0, 1, 4, along ND directions: the
+// moments_t (opensps_sfh_tangent.f90:426-467), logarithmic branch, types 0, 1, 4, 5, along ND directions: the
 // moments at log time L and their derivatives for the time's tangents dL[e] and the parameters' dtg[e], dtau[e]
+// (tage, tau), dtp[e] (Simha's t' = tage - sf_trunc) and dsl[e] (its slope)
 template <int ND>
-__device__ void d_moments_t(double L, const double* dL, const double* dtg, const double* dtau, const SfhP& s,
-                            double& m0, double& m1, double* dm0, double* dm1) {
+__device__ void d_moments_t(double L, const double* dL, const double* dtg, const double* dtau, const double* dtp,
+                            const double* dsl, const SfhP& s, double& m0, double& m1, double* dm0, double* dm1) {
     const double LOG_E = 0.4342944819032518, LN10 = 2.302585092994046;
     const double t = pow(10.0, L);
     m0 = 0.0;
@@ -265,6 +317,20 @@ __device__ void d_moments_t(double L, const double* dL, const double* dtg, const
             const double dt = t * LN10 * dL[e];
             dm0[e] = dt;
             dm1[e] = dt * (L - LOG_E) + t * dL[e];
+        }
+        return;
+    }
+    if (s.type == 5) {   // Simha's linear part: slope factor 1 - s t', t' = max(0, tage - sf_trunc)
+        const double sl = s.sf_slope, tp = fmax(0.0, s.tage - s.sf_trunc), sf = 1.0 - sl * tp;
+        const double h = sl * (t * t) / 2.0;
+        m0 = sf * t + h;
+        m1 = sf * t * (L - LOG_E) + h * (L - 0.5 * LOG_E);
+#pragma unroll
+        for (int e = 0; e < ND; e++) {
+            const double dt = t * LN10 * dL[e];
+            const double dsf = -dsl[e] * tp - sl * dtp[e], dh = dsl[e] * (t * t) / 2.0 + sl * t * dt;
+            dm0[e] = dsf * t + sf * dt + dh;
+            dm1[e] = dsf * t * (L - LOG_E) + sf * (dt * (L - LOG_E) + t * dL[e]) + dh * (L - 0.5 * LOG_E) + h * dL[e];
         }
         return;
     }
@@ -302,17 +368,19 @@ __device__ void d_moments_t(double L, const double* dL, const double* dtg, const
     }
 }
 
-// One SFH component's weights and tangents over the window [tq, tage] reduced against wbar.
+// One SFH component's weights and tangents over its window reduced against wbar.
 struct SfhTanSums {
     double mass, B;           // sum_{k < imax} w_k; sum_k wbar_k w_k
     double dmass[3], A[3];    // per direction: sum_{k < imax} dw_k; sum_k wbar_k dw_k
 };
 
-// ssp_weights_t (opensps_sfh_tangent.f90:255-358) with idx_min = 0, by one wavefront. Directions: e = 0 tage,
-// then tau (ND = 3 only), then tq (last). The moments at the clamped nodes are independent (one node a lane);
-// node j then takes the upper end of segment j - 1 and the lower end of segment j. Zero-width segments stay in
-// (no weight; the tangent is the one-sided one where a window end sits on a grid age).
-// mm: LDS, (2 + 2 ND) (nt + 2) doubles: node j (1-based) at slot j, the young edge's limits at slots 0, nt + 1.
+// ssp_weights_t (opensps_sfh_tangent.f90:255-358) with idx_min = 0, by one wavefront. The window [tq, tage] with the
+// directions tage, then tau (ND = 3 only), then tq (last); Simha's linear part (s.simha, ND = 2): the window [t0, tq]
+// with the directions tq (t' = tage - sf_trunc and t0 = tq - 1 / slope move with it) and the internal slope. The
+// moments at the clamped nodes are independent (one node a lane); node j then takes the upper end of segment j - 1
+// and the lower end of segment j. Zero-width segments stay in (no weight; the tangent is the one-sided one where a
+// window end sits on a grid age). mm: LDS, (2 + 2 ND) (nt + 2) doubles: node j (1-based) at slot j, the young
+// edge's limits at slots 0, nt + 1.
 template <int ND>
 __device__ void wave_weights_t(const double* __restrict__ tf, int nt, double tiny_logt, const SfhP& s, int imax,
                                const double* __restrict__ wbar, double* mm, int lane, SfhTanSums& r) {
@@ -320,17 +388,30 @@ __device__ void wave_weights_t(const double* __restrict__ tf, int nt, double tin
     const int S = nt + 2;
     double *M0 = mm, *M1 = mm + S, *D0 = mm + 2 * S, *D1 = mm + (2 + ND) * S;
     const double lmin = pow(10.0, tiny_logt);
-    const double lts = log10(fmax(s.tq, lmin)), lte = log10(fmax(s.tage, lmin));
-    const double uts = s.tq > lmin ? 1.0 / (s.tq * LN10) : 0.0;       // d lts / d tq
-    const double ute = s.tage > lmin ? 1.0 / (s.tage * LN10) : 0.0;   // d lte / d tage
+    const double ts = s.simha ? s.t0 : s.tq, te = s.simha ? s.tq : s.tage;
+    const double lts = log10(fmax(ts, lmin)), lte = log10(fmax(te, lmin));
+    const double uts = ts > lmin ? 1.0 / (ts * LN10) : 0.0;   // d lts / d ts
+    const double ute = te > lmin ? 1.0 / (te * LN10) : 0.0;   // d lte / d te
     const int js = max(wave_find_interval(tf, nt, lts, lane), 1);
     const int je = min(wave_find_interval(tf, nt, lte, lane) + 1, min(imax, nt - 1));
     const int nn = max(je + 2 - js, 0);   // nodes js .. je + 1
-    double dtg[ND], dtau[ND];
+    // per direction: the window ends' tangents and the parameters' (tage, tau, t', slope)
+    double dts[ND], dte[ND], dtg[ND], dtau[ND], dtp[ND], dsl[ND];
 #pragma unroll
-    for (int e = 0; e < ND; e++) {
-        dtg[e] = e == 0 ? 1.0 : 0.0;
-        dtau[e] = (ND == 3 && e == 1) ? 1.0 : 0.0;
+    for (int e = 0; e < ND; e++) { dts[e] = 0.0; dte[e] = 0.0; dtg[e] = 0.0; dtau[e] = 0.0; dtp[e] = 0.0; dsl[e] = 0.0; }
+    if (s.simha) {
+        dte[0] = 1.0;                                    // tq: the window's young end, t' and t0 with it
+        dts[0] = s.t0 > 0.0 ? 1.0 : 0.0;
+        dtp[0] = s.tage - s.sf_trunc > 0.0 ? 1.0 : 0.0;
+        if (ND > 1) {                                    // the slope: t0 = tq - 1 / slope
+            dts[ND - 1] = s.t0 > 0.0 ? 1.0 / (s.sf_slope * s.sf_slope) : 0.0;
+            dsl[ND - 1] = 1.0;
+        }
+    } else {
+        dte[0] = 1.0;   // tage
+        dtg[0] = 1.0;
+        if (ND == 3) dtau[1] = 1.0;
+        dts[ND - 1] = 1.0;   // tq
     }
     auto clampL = [&](double node, int& fl) {   // clamp_t (opensps_sfh_tangent.f90:407-422)
         double v = node;
@@ -350,11 +431,9 @@ __device__ void wave_weights_t(const double* __restrict__ tf, int nt, double tin
         const double L = clampL(node, fl);
         double dL[ND];
 #pragma unroll
-        for (int e = 0; e < ND; e++) dL[e] = 0.0;
-        if (fl == 2) dL[0] = ute;
-        if (fl == 1) dL[ND - 1] = uts;
+        for (int e = 0; e < ND; e++) dL[e] = fl == 1 ? dts[e] * uts : (fl == 2 ? dte[e] * ute : 0.0);
         double m0, m1, dm0[ND], dm1[ND];
-        d_moments_t<ND>(L, dL, dtg, dtau, s, m0, m1, dm0, dm1);
+        d_moments_t<ND>(L, dL, dtg, dtau, dtp, dsl, s, m0, m1, dm0, dm1);
         M0[slot] = m0;
         M1[slot] = m1;
 #pragma unroll
@@ -421,6 +500,47 @@ __device__ void wave_weights_t(const double* __restrict__ tf, int nt, double tin
     WAVE_SYNC();
 }
 
+// simha_frac_t (opensps_sfh_tangent.f90:362-403): Simha's linear mass fraction and its derivatives (Gyr; per unit of
+// sf_slope) with respect to tage, tau, sf_start, sf_trunc, sf_slope (df[0..4])
+__device__ double d_simha_frac_t(const double* p, double log_tmax, double* df) {
+    const double sf_start = p[OPS_P_SF_START], sf_trunc = p[OPS_P_SF_TRUNC], tau_p = p[OPS_P_TAU];
+    const double tmax = pow(10.0, log_tmax - 9.0) - sf_start, tpr = p[OPS_P_TAGE] - sf_start;
+    const bool trunc_on = !(sf_trunc < OPS_SAFE_FLOOR || sf_trunc < sf_start);
+    const double ttr = trunc_on ? sf_trunc - sf_start : tmax, tau = fmax(tau_p, OPS_SAFE_FLOOR);
+    const double sm = -p[OPS_P_SF_SLOPE];
+#pragma unroll
+    for (int q = 0; q < 5; q++) df[q] = 0.0;
+    if (tpr < 0.0 || ttr <= 0.0 || ttr > tmax || tpr <= ttr) return 0.0;
+    const double x = ttr / tau, gx = d_gammainc2(x), mt = tau * gx;
+    const double tz = sm > 0.0 ? ttr + 1.0 / sm : tmax;
+    const bool zhi = tz <= tpr;
+    const double th = zhi ? tz : tpr;
+    const double r = ttr / tau_p, er = exp(-r), sfr = r * er;
+    const double bb = (th - ttr) - (sm / 2.0) * (th * th + ttr * ttr) + sm * th * ttr;
+    const double ml = sfr * bb;
+    if (!(ml + mt > OPS_SAFE_FLOOR)) return 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; q++) {   // seeds: tage, tau, sf_start, sf_trunc, sf_slope
+        double dtmax = 0.0, dtpr = 0.0, dttr = 0.0, dtau = 0.0, dsm = 0.0;
+        if (q == 0) dtpr = 1.0;
+        if (q == 1 && tau_p > OPS_SAFE_FLOOR) dtau = 1.0;
+        if (q == 2) { dtmax = -1.0; dtpr = -1.0; dttr = -1.0; }
+        if (q == 3 && trunc_on) dttr = 1.0;
+        if (q == 4) dsm = -1.0;
+        if (!trunc_on) dttr = dtmax;
+        const double dx = (dttr - x * dtau) / tau;   // (tpr > ttr: the tau part's mass ends at the truncation)
+        const double dmt = dtau * gx + tau * x * exp(-x) * dx;
+        const double dtz = sm > 0.0 ? dttr - dsm / (sm * sm) : dtmax;
+        const double dth = zhi ? dtz : dtpr;
+        const double dr = (dttr - r * dtau) / tau_p, dsfr = er * (1.0 - r) * dr;
+        const double dbb = (dth - dttr) - (dsm / 2.0) * (th * th + ttr * ttr) - sm * (th * dth + ttr * dttr) +
+                           dsm * th * ttr + sm * (dth * ttr + th * dttr);
+        const double dml = dsfr * bb + sfr * dbb;
+        df[q] = (dml * mt - ml * dmt) / ((ml + mt) * (ml + mt));
+    }
+    return ml / (ml + mt);
+}
+
 // The burst (ssp_weights_t, type -1): two weights linear in log tb between the bracketing ages. B = sum wbar w,
 // A = sum wbar dw / d tb (per year).
 __device__ void d_burst_t(const double* __restrict__ tf, int nt, double tiny_logt, double tb,
@@ -437,9 +557,9 @@ __device__ void d_burst_t(const double* __restrict__ tf, int nt, double tiny_log
     A = (wbar[j] - wbar[j - 1]) * (u / bdt);
 }
 
-// k_sfh_grad: one wavefront per model (4 per workgroup); sfh_weights_tangent (opensps_sfh_tangent.f90:66-251)
-// along the four internal times at once, written to params_bar's rows 16 .. 21, 24, 25 (0-based). Other SFH types
-// (the forward has flagged them) get zeros.
+// k_sfh_grad: one wavefront per model (4 per workgroup); sfh_weights_tangent (opensps_sfh_tangent.f90:66-251) for
+// sfh 0, 1, 4, 5 along the internal times at once, written to params_bar's rows 16 .. 21, 24, 25 (0-based). Other
+// SFH types (the forward has flagged them) get zeros.
 __global__ __launch_bounds__(256) void k_sfh_grad(SfhGradArgs a) {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -454,11 +574,37 @@ __global__ __launch_bounds__(256) void k_sfh_grad(SfhGradArgs a) {
     const double G = 1.0e9;
     double rows[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // tage tau const fburst tburst sf_start sf_trunc slope
     const int sfh = (int)p[OPS_P_SFH];
-    if (sfh == 0 || sfh == 1 || sfh == 4) {
+    if (sfh == 0 || sfh == 1 || sfh == 4 || sfh == 5) {
         SfhP s;
         d_convert_sfh(p, s);
         const int imax = min(max(wave_find_interval(a.time, nt, log10(s.tage), lane) + 2, 1), nt);
-        if (sfh == 0) {
+        if (sfh == 5) {
+            // Simha: (1 - f) wt + f cn, the delayed-tau part wt along tage, tau, tq, the linear part cn along tq and
+            // the slope, f the linear mass fraction (its derivatives from simha_frac_t)
+            SfhTanSums T, Ln;
+            s.type = 4;
+            wave_weights_t<3>(a.time, nt, a.tiny_logt, s, imax, wb, mm, lane, T);
+            s.type = 5;
+            s.simha = 1;
+            wave_weights_t<2>(a.time, nt, a.tiny_logt, s, imax, wb, mm, lane, Ln);
+            double m1 = T.mass, dm1[3] = {T.dmass[0], T.dmass[1], T.dmass[2]};
+            if (m1 < OPS_SAFE_FLOOR) { m1 = 1.0; dm1[0] = dm1[1] = dm1[2] = 0.0; }
+            double m2 = Ln.mass, dm2[2] = {Ln.dmass[0], Ln.dmass[1]};
+            if (m2 < OPS_SAFE_FLOOR) { m2 = 1.0; dm2[0] = dm2[1] = 0.0; }
+            double dfl[5];
+            const double fl = d_simha_frac_t(p, a.time[nt - 1], dfl);
+            const double bt = T.B / m1, bl = Ln.B / m2, mix = bl - bt;   // sum wbar (cn - wt)
+            const double t_tg = (T.A[0] - bt * dm1[0]) / m1, t_tau = (T.A[1] - bt * dm1[1]) / m1;
+            const double t_tq = (T.A[2] - bt * dm1[2]) / m1;
+            const double l_tq = (Ln.A[0] - bl * dm2[0]) / m2, l_sl = (Ln.A[1] - bl * dm2[1]) / m2;
+            const bool tq_on = s.tq > 0.0;
+            const double s_tq = (1.0 - fl) * t_tq + fl * l_tq;
+            rows[0] = G * ((1.0 - fl) * t_tg + (tq_on ? s_tq : 0.0)) + mix * dfl[0];
+            rows[1] = G * (1.0 - fl) * t_tau + mix * dfl[1];
+            rows[5] = -G * (1.0 - fl) * t_tg + mix * dfl[2];
+            rows[6] = (tq_on ? -G * s_tq : 0.0) + mix * dfl[3];
+            rows[7] = -(1.0 / G) * fl * l_sl + mix * dfl[4];
+        } else if (sfh == 0) {
             double B, A;
             d_burst_t(a.time, nt, a.tiny_logt, s.tage, wb, B, A);
             rows[0] = G * A;

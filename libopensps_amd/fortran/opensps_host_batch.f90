@@ -52,14 +52,68 @@ contains
         end if
     end function gammainc12
 
-    !> The SFR of a model at its age per unit mass formed: get_sfh_properties_at_age (src/physics/opensps_sfh.f90:282-410)
-    !> for sfh 0, 1, 4, divided by the mass fraction formed as batch_model_indices does (opensps_csp_batch.f90:1556-1560)
+    !> The SFR of a model at its age per unit mass formed: get_sfh_properties_at_age (src/physics/opensps_sfh.f90:282-486)
+    !> for sfh 0, 1, 4, 5, divided by the mass fraction formed as batch_model_indices does (opensps_csp_batch.f90:1556-1560)
     real(WP) function sfr_at_age(p, tmax_log) result(sfr)
         real(WP), intent(in) :: p(NPAR), tmax_log
         real(WP) :: t_max, t_prime, t_trunc, tau_g, tr, total_tau, mass_tau, sfr_tau, mfrac_b, sfr_c, mass_c, norm, mfrac
+        real(WP) :: slope_m, t_zero, t_hi, sfr_trunc, total_lin, mass_lin
         integer :: pw, sfh
         sfh = nint(p(P_SFH))
         sfr = 0.0_wp
+        if (sfh == 5) then
+            ! Simha: delayed tau up to the truncation, then linear (opensps_sfh.f90:401-455)
+            t_max = 10.0_wp**(tmax_log - 9.0_wp) - p(P_SF_START)
+            t_prime = p(P_TAGE) - p(P_SF_START)
+            if (p(P_SF_TRUNC) < SAFE_FLOOR .or. p(P_SF_TRUNC) < p(P_SF_START)) then
+                t_trunc = t_max
+            else
+                t_trunc = p(P_SF_TRUNC) - p(P_SF_START)
+            end if
+            tau_g = max(p(P_TAU), SAFE_FLOOR)
+            total_tau = tau_g * gammainc12(2, min(t_max, t_trunc) / tau_g)
+            if (t_prime < 0.0_wp) then
+                mass_tau = 0.0_wp
+                sfr_tau = 0.0_wp
+            else
+                tr = min(t_prime, t_trunc) / tau_g
+                mass_tau = tau_g * gammainc12(2, tr)
+                sfr_tau = tr * exp(-tr)
+            end if
+            slope_m = -p(P_SF_SLOPE)
+            if (slope_m > 0.0_wp) then
+                t_zero = t_trunc + 1.0_wp / slope_m
+            else
+                t_zero = t_max
+            end if
+            mass_lin = 0.0_wp
+            total_lin = 0.0_wp
+            if (t_trunc <= 0.0_wp .or. t_trunc > t_max) then
+                sfr = sfr_tau / total_tau
+            else
+                t_hi = min(t_zero, t_max)
+                sfr_trunc = (t_trunc / p(P_TAU)) * exp(-t_trunc / p(P_TAU))
+                total_lin = sfr_trunc * ((t_hi - t_trunc) - (slope_m / 2.0_wp) * (t_hi**2 + t_trunc**2) + &
+                                         (slope_m * t_hi * t_trunc))
+                if (t_prime <= t_trunc) then
+                    sfr = sfr_tau / (total_tau + total_lin)
+                else
+                    t_hi = min(t_zero, t_prime)
+                    mass_lin = sfr_trunc * ((t_hi - t_trunc) - (slope_m / 2.0_wp) * (t_hi**2 + t_trunc**2) + &
+                                            (slope_m * t_hi * t_trunc))
+                    sfr = max(sfr_trunc * (1.0_wp - slope_m * (t_prime - t_trunc)), 0.0_wp) / (total_tau + total_lin)
+                end if
+            end if
+            if (t_prime >= 0.0_wp) then
+                mfrac = (mass_tau + mass_lin) / (total_tau + total_lin)
+            else
+                mfrac = 0.0_wp
+                sfr = 0.0_wp
+            end if
+            sfr = sfr / 1.0e9_wp
+            if (mfrac > SAFE_FLOOR) sfr = sfr / mfrac
+            return
+        end if
         if (sfh /= 1 .and. sfh /= 4) return
         t_max = 10.0_wp**(tmax_log - 9.0_wp) - p(P_SF_START)
         t_prime = p(P_TAGE) - p(P_SF_START)

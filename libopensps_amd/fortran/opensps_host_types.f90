@@ -25,6 +25,7 @@ module opensps_host_types
     integer, parameter, public :: P_TAGE = 1, P_SFH = 9, P_LOGZSOL = 10, P_DUST1 = 11, P_DUST2 = 12, P_DUST3 = 13, &
                                   P_FRAC_OBRUN = 17, P_ZRED = 19, P_IGM_FACTOR = 20, P_SIGMA_SMOOTH = 21, &
                                   P_TAU = 2, P_CONST = 3, P_FBURST = 4, P_TBURST = 5, P_SF_START = 6, P_SF_TRUNC = 7, &
+                                  P_SF_SLOPE = 8, &
                                   P_IMF1 = 24
 
     !> Parameters the batch refuses to vary per model (SSP-sensitive: they change the resident basis;

@@ -121,7 +121,7 @@ static void convert_sfhparams(const double *p, double tage, sfhp_t *s) {
     s->type = 0;
 }
 
-/* eval_indefinite_moments_log (opensps_sfh.f90:852-910), types 0, 1, 4 */
+/* eval_indefinite_moments_log (opensps_sfh.f90:852-910), types 0, 1, 4, 5 */
 static void moments_log(double log_t, const sfhp_t *s, double *m0, double *m1) {
     double t_lin = pow(10.0, log_t);
     switch (s->type) {
@@ -147,6 +147,13 @@ static void moments_log(double log_t, const sfhp_t *s, double *m0, double *m1) {
         *m0 = -(t_lin - s->tage - s->tau) * et;
         double term_const = s->tau * LOG_E;
         *m1 = -((t_lin - s->tage - s->tau) * log_t - term_const) * et - (s->tage + s->tau) * LOG_E * ei;
+        return;
+    }
+    case 5: {   /* Simha's linear part: SFR ~ 1 - slope (tq - t) in lookback time */
+        const double tp = fmax(0.0, s->tage - s->sf_trunc), sf = 1.0 - s->sf_slope * tp;
+        const double q = s->sf_slope * (t_lin * t_lin) / 2.0;
+        *m0 = sf * t_lin + q;
+        *m1 = sf * t_lin * (log_t - LOG_E) + q * (log_t - 0.5 * LOG_E);
         return;
     }
     default:
@@ -217,7 +224,42 @@ static void compute_ssp_weights(const double *tf, int nt, double tiny_logt, cons
     }
 }
 
-/* sfh_weights_core_pc (opensps_sfh.f90:1077-1294) for sfh = 0, 1, 4 (with const and burst).
+/* expm1 as the reference forms it (opensps_special_functions.f90:219-233): a cubic below |x| = 1e-5 */
+static double expm1_ref(double x) {
+    if (fabs(x) < 1.0e-5) return x * (1.0 + x * (0.5 + x * (1.0 / 6.0)));
+    return exp(x) - 1.0;
+}
+
+/* gammainc(2, x), the regularised lower incomplete gamma (opensps_special_functions.f90:142-167) */
+static double gammainc2(double x) {
+    if (x < 0.0) return 0.0;
+    return -expm1_ref(-x) - x * exp(-x);
+}
+
+/* The Simha SFH's linear mass fraction at age tage (Gyr): get_sfh_properties_at_age_pc's frac_linear for sfh = 5
+ * (opensps_sfh.f90:327-455); log_tmax: the oldest grid age (log10 yr) */
+double orc_simha_frac(const double *p, double log_tmax) {
+    const double t_max = pow(10.0, log_tmax - 9.0) - p[OPS_P_SF_START];
+    const double t_prime = p[OPS_P_TAGE] - p[OPS_P_SF_START];
+    const double sf_trunc = p[OPS_P_SF_TRUNC], sf_start = p[OPS_P_SF_START];
+    const double t_trunc = (sf_trunc < SAFE_FLOOR || sf_trunc < sf_start) ? t_max : sf_trunc - sf_start;
+    const double tau = fmax(p[OPS_P_TAU], SAFE_FLOOR);
+    double mass_tau = 0.0;
+    if (t_prime >= 0.0) mass_tau = tau * gammainc2(fmin(t_prime, t_trunc) / tau);
+    const double slope_m = -p[OPS_P_SF_SLOPE];
+    const double t_zero = slope_m > 0.0 ? t_trunc + 1.0 / slope_m : t_max;
+    double mass_linear = 0.0;
+    if (!(t_trunc <= 0.0 || t_trunc > t_max) && t_prime > t_trunc) {
+        const double sfr_trunc = (t_trunc / p[OPS_P_TAU]) * exp(-t_trunc / p[OPS_P_TAU]);
+        const double t_hi = fmin(t_zero, t_prime);
+        mass_linear = sfr_trunc * ((t_hi - t_trunc) - (slope_m / 2.0) * (t_hi * t_hi + t_trunc * t_trunc) +
+                                   slope_m * t_hi * t_trunc);
+    }
+    if (t_prime >= 0.0 && mass_linear + mass_tau > SAFE_FLOOR) return mass_linear / (mass_linear + mass_tau);
+    return 0.0;
+}
+
+/* sfh_weights_core_pc (opensps_sfh.f90:1077-1294) for sfh = 0, 1, 4 (with const and burst) and 5 (Simha).
  * Returns 0, or 2 for an SFH type this build does not compute. */
 int orc_sfh_weights(const double *tf, int nt, double tiny_logt, const double *p, double *w, double *tmp1,
                     double *tmp2) {
@@ -234,6 +276,22 @@ int orc_sfh_weights(const double *tf, int nt, double tiny_logt, const double *p,
         s.tb = s.tage;
         int imin = imax - 2 > 1 ? imax - 2 : 1;
         compute_ssp_weights(tf, nt, tiny_logt, &s, imin, imax, w);
+        return 0;
+    }
+    if (sfh == 5) {   /* the delayed-tau part over [tq, tage] and the linear part over [t0, tq], mixed */
+        s.type = 4;
+        compute_ssp_weights(tf, nt, tiny_logt, &s, 0, imax, tmp1);
+        double m1 = 0.0, m2 = 0.0;
+        for (int i = 0; i < imax; i++) m1 += tmp1[i];
+        s.type = 5;
+        s.use_simha_limits = 1;
+        compute_ssp_weights(tf, nt, tiny_logt, &s, 0, imax, tmp2);
+        s.use_simha_limits = 0;
+        for (int i = 0; i < imax; i++) m2 += tmp2[i];
+        if (m1 < SAFE_FLOOR) m1 = 1.0;
+        if (m2 < SAFE_FLOOR) m2 = 1.0;
+        const double fl = orc_simha_frac(p, tf[nt - 1]);
+        for (int i = 0; i < nt; i++) w[i] = (tmp1[i] / m1) * (1.0 - fl) + (tmp2[i] / m2) * fl;
         return 0;
     }
     if (sfh != 1 && sfh != 4) return 2;
@@ -265,7 +323,7 @@ int orc_sfh_weights(const double *tf, int nt, double tiny_logt, const double *p,
 
 /* ---------------------------------------------------------------------------------------------
  * SFH tangents (src/physics/opensps_sfh_tangent.f90): the weights' derivative with respect to one
- * SFH parameter, forward mode, sfh = 0, 1, 4 with const, fburst and truncation, logarithmic
+ * SFH parameter, forward mode, sfh = 0, 1, 4 with const, fburst and truncation, and 5 (Simha), logarithmic
  * interpolation. Pinned by central differences of orc_sfh_weights (tests/test_oracle_sfh_tangent.py).
  * ------------------------------------------------------------------------------------------- */
 #define ORC_GYR 1.0e9
@@ -279,14 +337,26 @@ static void clamp_t(double node, double lts, double dlts, double lte, double dlt
     if (lte < *v) { *v = lte; *dv = dlte; }
 }
 
-/* moments_t (opensps_sfh_tangent.f90:426-467), logarithmic branch, types 0, 1, 4: the indefinite moments at
- * log time lt and their total derivatives, with the time's (dlt) and the parameters' (dtg, dtau) tangents */
-static void moments_t(double lt, double dlt, const sfhp_t *s, double dtg, double dtau, double *m0, double *dm0,
-                      double *m1, double *dm1) {
+/* moments_t (opensps_sfh_tangent.f90:426-467), logarithmic branch, types 0, 1, 4, 5: the indefinite moments at
+ * log time lt and their total derivatives, with the time's (dlt) and the parameters' (dtg, dtau, dtq, ds: of tage,
+ * tau, tq and the slope) tangents */
+static void moments_t(double lt, double dlt, const sfhp_t *s, double dtg, double dtau, double dtq, double ds,
+                      double *m0, double *dm0, double *m1, double *dm1) {
     const double ln10 = log(10.0);
     const double t = pow(10.0, lt), dt = t * ln10 * dlt;
     *m0 = 0.0; *dm0 = 0.0; *m1 = 0.0; *dm1 = 0.0;
     if ((s->type == 1 || s->type == 4) && s->tau <= SAFE_FLOOR) return;
+    if (s->type == 5) {   /* Simha's linear part: slope factor 1 - s t', t' = max(0, tage - sf_trunc) */
+        const double sl = s->sf_slope, tp = fmax(0.0, s->tage - s->sf_trunc);
+        const double dtp = s->tage - s->sf_trunc > 0.0 ? dtq : 0.0;
+        const double sf = 1.0 - sl * tp, dsf = -ds * tp - sl * dtp;
+        const double h = sl * (t * t) / 2.0, dh = ds * (t * t) / 2.0 + sl * t * dt;
+        *m0 = sf * t + h;
+        *dm0 = dsf * t + sf * dt + dh;
+        *m1 = sf * t * (lt - LOG_E) + h * (lt - 0.5 * LOG_E);
+        *dm1 = dsf * t * (lt - LOG_E) + sf * (dt * (lt - LOG_E) + t * dlt) + dh * (lt - 0.5 * LOG_E) + h * dlt;
+        return;
+    }
     if (s->type == 0) {
         *m0 = t;
         *dm0 = dt;
@@ -320,7 +390,8 @@ static void moments_t(double lt, double dlt, const sfhp_t *s, double dtg, double
  * tangents dtg, dtau, dtb, dtq of tage, tau, tb and tq (years). Segments of zero width stay (their tangent is the
  * one-sided one where a window end sits on a grid age). */
 static void ssp_weights_t(const double *tf, int nt, double tiny_logt, const sfhp_t *s, double dtg, double dtau,
-                          double dtb, double dtq, int idx_min, int idx_max, double *w, double *dw) {
+                          double dtb, double dtq, double dt0, double ds, int idx_min, int idx_max, double *w,
+                          double *dw) {
     const double ln10 = log(10.0), lmin = pow(10.0, tiny_logt);
     for (int i = 0; i < nt; i++) { w[i] = 0.0; dw[i] = 0.0; }
     if (s->type == -1) {
@@ -338,8 +409,11 @@ static void ssp_weights_t(const double *tf, int nt, double tiny_logt, const sfhp
         dw[j] = dltb / bdt;
         return;
     }
-    const double lts = log10(fmax(s->tq, lmin)), dlts = s->tq > lmin ? dtq / (s->tq * ln10) : 0.0;
-    const double lte = log10(fmax(s->tage, lmin)), dlte = s->tage > lmin ? dtg / (s->tage * ln10) : 0.0;
+    /* the window [tq, tage] (lookback years); Simha's linear part: [t0, tq] */
+    const double ts = s->use_simha_limits ? s->t0 : s->tq, dts = s->use_simha_limits ? dt0 : dtq;
+    const double te = s->use_simha_limits ? s->tq : s->tage, dte = s->use_simha_limits ? dtq : dtg;
+    const double lts = log10(fmax(ts, lmin)), dlts = ts > lmin ? dts / (ts * ln10) : 0.0;
+    const double lte = log10(fmax(te, lmin)), dlte = te > lmin ? dte / (te * ln10) : 0.0;
     int j_start = orc_find_interval(tf, nt, lts);
     int j_end = orc_find_interval(tf, nt, lte) + 1;
     if (j_start < (idx_min > 1 ? idx_min : 1)) j_start = idx_min > 1 ? idx_min : 1;
@@ -347,13 +421,13 @@ static void ssp_weights_t(const double *tf, int nt, double tiny_logt, const sfhp
     double l1, dl1;
     clamp_t(tf[j_start - 1], lts, dlts, lte, dlte, &l1, &dl1);
     double lp = l1, m0p, dm0p, m1p, dm1p;
-    moments_t(l1, dl1, s, dtg, dtau, &m0p, &dm0p, &m1p, &dm1p);
+    moments_t(l1, dl1, s, dtg, dtau, dtq, ds, &m0p, &dm0p, &m1p, &dm1p);
     for (int j = j_start; j <= j_end; j++) {
         double l2, dl2;
         clamp_t(tf[j], lts, dlts, lte, dlte, &l2, &dl2);
         if (l2 < lp) { lp = l2; continue; }
         double m0c, dm0c, m1c, dm1c;
-        moments_t(l2, dl2, s, dtg, dtau, &m0c, &dm0c, &m1c, &dm1c);
+        moments_t(l2, dl2, s, dtg, dtau, dtq, ds, &m0c, &dm0c, &m1c, &dm1c);
         const double m0 = m0c - m0p, dm0 = dm0c - dm0p, m1 = m1c - m1p, dm1 = dm1c - dm1p;
         m0p = m0c; dm0p = dm0c; m1p = m1c; dm1p = dm1c;
         const double nlo = tf[j - 1], nhi = tf[j], inv = 1.0 / (nhi - nlo);
@@ -369,15 +443,55 @@ static void ssp_weights_t(const double *tf, int nt, double tiny_logt, const sfhp
         clamp_t(tf[0], lts, dlts, lte, dlte, &lb, &dlb);
         if (la < lb) {
             double ma, dma, mb, dmb, x1, x2;
-            moments_t(lb, dlb, s, dtg, dtau, &mb, &dmb, &x1, &x2);
-            moments_t(la, dla, s, dtg, dtau, &ma, &dma, &x1, &x2);
+            moments_t(lb, dlb, s, dtg, dtau, dtq, ds, &mb, &dmb, &x1, &x2);
+            moments_t(la, dla, s, dtg, dtau, dtq, ds, &ma, &dma, &x1, &x2);
             w[0] += mb - ma;
             dw[0] += dmb - dma;
         }
     }
 }
 
-/* sfh_weights_tangent (opensps_sfh_tangent.f90:66-251) for sfh = 0, 1, 4: the weights w and their derivative dw
+/* simha_frac_t (opensps_sfh_tangent.f90:362-403): Simha's linear mass fraction f (orc_simha_frac) and its derivative
+ * df with respect to parameter q (per Gyr, per unit of sf_slope) */
+void orc_simha_frac_t(const double *p, double log_tmax, int q, double *f, double *df) {
+    const double sf_start = p[OPS_P_SF_START], sf_trunc = p[OPS_P_SF_TRUNC], tau_p = p[OPS_P_TAU];
+    const double tmax = pow(10.0, log_tmax - 9.0) - sf_start, tpr = p[OPS_P_TAGE] - sf_start;
+    const int trunc_on = !(sf_trunc < SAFE_FLOOR || sf_trunc < sf_start);
+    const double ttr = trunc_on ? sf_trunc - sf_start : tmax, tau = fmax(tau_p, SAFE_FLOOR), sm = -p[OPS_P_SF_SLOPE];
+    double dtmax = 0.0, dtpr = 0.0, dttr = 0.0, dtau = 0.0, dsm = 0.0;
+    switch (q) {
+    case SFHT_TAGE: dtpr = 1.0; break;
+    case SFHT_TAU: if (tau_p > SAFE_FLOOR) dtau = 1.0; break;
+    case SFHT_SF_START: dtmax = -1.0; dtpr = -1.0; dttr = -1.0; break;
+    case SFHT_SF_TRUNC: if (trunc_on) dttr = 1.0; break;
+    case SFHT_SF_SLOPE: dsm = -1.0; break;
+    default: break;
+    }
+    if (!trunc_on) dttr = dtmax;
+    *f = 0.0;
+    *df = 0.0;
+    if (tpr < 0.0 || ttr <= 0.0 || ttr > tmax) return;
+    const double mn = tpr <= ttr ? tpr : ttr, dmn = tpr <= ttr ? dtpr : dttr;
+    const double x = mn / tau, dx = (dmn - x * dtau) / tau;
+    const double mt = tau * gammainc2(x), dmt = dtau * gammainc2(x) + tau * x * exp(-x) * dx;
+    if (tpr <= ttr) return;
+    double tz, dtz;
+    if (sm > 0.0) { tz = ttr + 1.0 / sm; dtz = dttr - dsm / (sm * sm); }
+    else { tz = tmax; dtz = dtmax; }
+    const double th = tz <= tpr ? tz : tpr, dth = tz <= tpr ? dtz : dtpr;
+    const double r = ttr / tau_p, dr = (dttr - r * dtau) / tau_p;
+    const double sfr = r * exp(-r), dsfr = exp(-r) * (1.0 - r) * dr;
+    const double bb = (th - ttr) - (sm / 2.0) * (th * th + ttr * ttr) + sm * th * ttr;
+    const double dbb = (dth - dttr) - (dsm / 2.0) * (th * th + ttr * ttr) - sm * (th * dth + ttr * dttr) +
+                       dsm * th * ttr + sm * (dth * ttr + th * dttr);
+    const double ml = sfr * bb, dml = dsfr * bb + sfr * dbb;
+    if (ml + mt > SAFE_FLOOR) {
+        *f = ml / (ml + mt);
+        *df = (dml * mt - ml * dmt) / ((ml + mt) * (ml + mt));
+    }
+}
+
+/* sfh_weights_tangent (opensps_sfh_tangent.f90:66-251) for sfh = 0, 1, 4, 5: the weights w and their derivative dw
  * with respect to parameter q (SFHT_*, in the units of the parameters: Gyr). s: scratch (4 nt). Returns 0, or 2 for
  * an SFH type this build does not compute. */
 int orc_sfh_weights_tangent(const double *tf, int nt, double tiny_logt, const double *p, int q, double *w,
@@ -385,16 +499,17 @@ int orc_sfh_weights_tangent(const double *tf, int nt, double tiny_logt, const do
     sfhp_t h;
     const int sfh = (int)p[OPS_P_SFH];
     for (int i = 0; i < nt; i++) { w[i] = 0.0; dw[i] = 0.0; }
-    if (sfh != 0 && sfh != 1 && sfh != 4) return 2;
+    if (sfh != 0 && sfh != 1 && sfh != 4 && sfh != 5) return 2;
     convert_sfhparams(p, p[OPS_P_TAGE], &h);
-    double dtg = 0.0, dtau = 0.0, dtb = 0.0, dtq = 0.0;
+    double dtg = 0.0, dtau = 0.0, dtb = 0.0, dtq = 0.0, ds = 0.0;
     switch (q) {
+    case SFHT_SF_SLOPE: ds = -1.0 / ORC_GYR; break;   /* the internal slope is -sf_slope / 1e9 per year */
     case SFHT_TAGE: dtg = ORC_GYR; dtb = ORC_GYR; if (h.tq > 0.0) dtq = ORC_GYR; break;
     case SFHT_SF_TRUNC: if (h.tq > 0.0) dtq = -ORC_GYR; break;
     case SFHT_TAU: dtau = ORC_GYR; break;
     case SFHT_TBURST: dtb = -ORC_GYR; break;
     case SFHT_SF_START: if (sfh != 0) dtg = -ORC_GYR; break;
-    default: break;   /* const, fburst: the mixture below; sf_slope: only Simha's linear part has it */
+    default: break;   /* const, fburst: the mixture below */
     }
     int imax = orc_find_interval(tf, nt, log10(h.tage)) + 2;
     if (imax < 1) imax = 1;
@@ -402,18 +517,40 @@ int orc_sfh_weights_tangent(const double *tf, int nt, double tiny_logt, const do
     if (sfh == 0) {   /* a burst at tb = tage */
         h.type = -1;
         h.tb = h.tage;
-        ssp_weights_t(tf, nt, tiny_logt, &h, 0.0, 0.0, dtg, 0.0, imax - 2 > 1 ? imax - 2 : 1, imax, w, dw);
+        ssp_weights_t(tf, nt, tiny_logt, &h, 0.0, 0.0, dtg, 0.0, 0.0, 0.0, imax - 2 > 1 ? imax - 2 : 1, imax, w, dw);
         return 0;
     }
     double *c0 = s, *c1 = s + nt, *b0 = s + 2 * nt, *b1 = s + 3 * nt;
+    if (sfh == 5) {   /* Simha: both parts normalised, mixed by the linear part's mass fraction */
+        h.type = 4;
+        ssp_weights_t(tf, nt, tiny_logt, &h, dtg, dtau, 0.0, dtq, 0.0, 0.0, 0, imax, c0, c1);
+        h.type = 5;
+        h.use_simha_limits = 1;
+        /* t0 = tq - 1 / slope, where the linear SFR reaches zero (if inside (0, tq]) */
+        const double dt0 = h.t0 > 0.0 ? dtq + ds / (h.sf_slope * h.sf_slope) : 0.0;
+        ssp_weights_t(tf, nt, tiny_logt, &h, dtg, 0.0, 0.0, dtq, dt0, ds, 0, imax, b0, b1);
+        double m1 = 0.0, dm1 = 0.0, m2 = 0.0, dm2 = 0.0;
+        for (int i = 0; i < imax; i++) { m1 += c0[i]; dm1 += c1[i]; m2 += b0[i]; dm2 += b1[i]; }
+        if (m1 < SAFE_FLOOR) { m1 = 1.0; dm1 = 0.0; }
+        if (m2 < SAFE_FLOOR) { m2 = 1.0; dm2 = 0.0; }
+        double fl, dfl;
+        orc_simha_frac_t(p, tf[nt - 1], q, &fl, &dfl);
+        for (int i = 0; i < nt; i++) {
+            const double wt = c0[i] / m1, dwt = (c1[i] - wt * dm1) / m1;
+            const double cn = b0[i] / m2, dcn = (b1[i] - cn * dm2) / m2;
+            w[i] = wt * (1.0 - fl) + cn * fl;
+            dw[i] = dwt * (1.0 - fl) + dcn * fl + (cn - wt) * dfl;
+        }
+        return 0;
+    }
     h.type = sfh;
-    ssp_weights_t(tf, nt, tiny_logt, &h, dtg, dtau, 0.0, dtq, 0, imax, w, dw);
+    ssp_weights_t(tf, nt, tiny_logt, &h, dtg, dtau, 0.0, dtq, 0.0, 0.0, 0, imax, w, dw);
     double mass1 = 0.0, dmass1 = 0.0;
     for (int i = 0; i < imax; i++) { mass1 += w[i]; dmass1 += dw[i]; }
     if (mass1 < SAFE_FLOOR) { mass1 = 1.0; dmass1 = 0.0; }
     /* the constant and burst components, whatever const and fburst (one-sided derivatives at 0) */
     h.type = 0;
-    ssp_weights_t(tf, nt, tiny_logt, &h, dtg, 0.0, 0.0, dtq, 0, imax, c0, c1);
+    ssp_weights_t(tf, nt, tiny_logt, &h, dtg, 0.0, 0.0, dtq, 0.0, 0.0, 0, imax, c0, c1);
     double massc = 0.0, dmassc = 0.0;
     for (int i = 0; i < imax; i++) { massc += c0[i]; dmassc += c1[i]; }
     if (massc < SAFE_FLOOR) { massc = 1.0; dmassc = 0.0; }
@@ -421,7 +558,7 @@ int orc_sfh_weights_tangent(const double *tf, int nt, double tiny_logt, const do
     double fb = 0.0;
     if (h.tb >= 0.0) {
         h.type = -1;
-        ssp_weights_t(tf, nt, tiny_logt, &h, 0.0, 0.0, dtb, 0.0, 0, imax, b0, b1);
+        ssp_weights_t(tf, nt, tiny_logt, &h, 0.0, 0.0, dtb, 0.0, 0.0, 0.0, 0, imax, b0, b1);
         fb = p[OPS_P_FBURST];
     }
     double cst = p[OPS_P_CONST];

@@ -149,14 +149,16 @@ def test_grad_statuses():
     ctx.close()
 
 
-SFH_ROWS = ["tage", "tau", "const", "fburst", "tburst", "sf_start", "sf_trunc"]
+SFH_ROWS = ["tage", "tau", "const", "fburst", "tburst", "sf_start", "sf_trunc", "sf_slope"]
 
 
 def _sfh_prm(n, seed):
-    """SFH types 0, 1, 4 mixed, with const, bursts (before and after tage), start times and truncation."""
+    """SFH types 0, 1, 4, 5 mixed, with const, bursts (before and after tage), start times, truncation and (Simha)
+    slopes of both signs."""
     rng = np.random.default_rng(seed)
     prm = synth.adjoint_params(n, seed=seed, birth_cloud=True)
-    prm[SLOT["sfh"]] = rng.choice([0, 1, 4], n)
+    prm[SLOT["sfh"]] = rng.choice([0, 1, 4, 5], n)
+    prm[SLOT["sf_slope"]] = np.where(prm[SLOT["sfh"]] == 5, rng.uniform(-1.5, 1.0, n), 0.0)
     tage = rng.uniform(0.2, 13.0, n)
     prm[SLOT["tage"]] = tage
     prm[SLOT["tau"]] = rng.uniform(0.2, 9.0, n)
@@ -282,7 +284,7 @@ def test_autograd_sfh_parameters():
     (mags * torch.tensor(dmag, device="cuda")).sum().backward()
     ctx.close()
     ref = O.csp_batch_grad(g, b, prm, mags_bar=dmag, cosmo=cosmo())
-    for name in SFH_ROWS + ["sf_slope", "dust2", "logzsol"]:
+    for name in SFH_ROWS + ["dust2", "logzsol"]:
         a = p_t.grad[SLOT[name]].cpu().numpy()
         e = ref["params_bar"][:, O.GRAD_ROWS[name]]
         err = float(np.max(np.abs(a - e)) / (np.max(np.abs(e)) + 1e-300))

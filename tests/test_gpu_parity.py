@@ -177,11 +177,20 @@ def test_birth_cloud_and_parts(precision):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("sfh", [0, 1])
+@pytest.mark.parametrize("sfh", [0, 1, 5])
 def test_other_sfh_types(sfh):
-    """SSP (sfh 0) and tau model (sfh 1) with constant and burst components, FP64 path."""
+    """SSP (sfh 0), tau model (sfh 1) with constant and burst components, Simha (sfh 5: truncation, slopes of both
+    signs, a start time), FP64 path."""
     g, b, prm = _parametric_case(64, seed=17)
     prm[SLOT["sfh"]] = sfh
+    if sfh == 5:
+        rng = np.random.default_rng(6)
+        n = prm.shape[1]
+        start = np.where(rng.random(n) < 0.3, 0.1 * prm[SLOT["tage"]], 0.0)
+        prm[SLOT["sf_start"]] = start
+        prm[SLOT["sf_trunc"]] = np.where(rng.random(n) < 0.8, start + (prm[SLOT["tage"]] - start) *
+                                         rng.uniform(0.2, 0.9, n), 0.0)
+        prm[SLOT["sf_slope"]] = rng.uniform(-1.5, 1.0, n)
     if sfh == 1:
         rng = np.random.default_rng(4)
         prm[SLOT["const"]] = rng.uniform(0.0, 0.3, prm.shape[1])
@@ -195,7 +204,7 @@ def test_other_sfh_types(sfh):
 def test_unsupported_sfh_status():
     from libopensps_amd import hip as H
     g, b, prm = _parametric_case(8)
-    prm[SLOT["sfh"]] = 5
+    prm[SLOT["sfh"]] = 2   # tabular: no table in this build
     ctx = _ctx(g, b)
     with pytest.raises(H.OpenSPSHipError) as e:
         ctx.batch(_dev(prm), precision=H.FP64)

@@ -89,6 +89,36 @@ def test_sfh_weights_against_quadrature(sfh):
         assert np.max(np.abs(w - ref)) < 1e-8, (tage, tau, np.max(np.abs(w - ref)))
 
 
+def test_sfh_weights_simha_against_quadrature():
+    """sfh 5 (Simha): delayed tau up to the truncation, then linear from the truncation's SFR with slope sf_slope
+    (per Gyr) until it reaches zero; the weights are the hat-kernel integrals of that continuous SFR, normalised (the
+    reference mixes its two parts by the linear part's mass fraction, get_sfh_properties_at_age)."""
+    tf = synth.age_grid(107)
+    rng = np.random.default_rng(17)
+    checked = 0
+    for slope in (-0.8, -0.15, 0.0, 0.3):
+        for _ in range(4):
+            tage, tau = rng.uniform(2.0, 13.5), rng.uniform(0.3, 8.0)
+            start = rng.choice([0.0, rng.uniform(0.05, 0.5)])
+            trunc = start + (tage - start) * rng.uniform(0.3, 0.9)
+            p = synth.default_params(1)[:, 0]
+            p[SLOT["sfh"]], p[SLOT["tage"]], p[SLOT["tau"]] = 5, tage, tau
+            p[SLOT["sf_start"]], p[SLOT["sf_trunc"]], p[SLOT["sf_slope"]] = start, trunc, slope
+            w, st = O.sfh_weights(tf, p)
+            assert st == 0
+            T, tr, tu, sm = (tage - start) * 1e9, (trunc - start) * 1e9, tau * 1e9, -slope * 1e-9
+            s_tr = (tr / tu) * np.exp(-tr / tu)
+
+            def sfr(t):   # lookback time t (yr); forward time since the start x = T - t
+                x = T - t
+                return (x / tu) * np.exp(-x / tu) if x <= tr else s_tr * max(1.0 - sm * (x - tr), 0.0)
+            ref = _hat_weights_quad(tf, sfr, 0.0, T)
+            assert abs(w.sum() - 1.0) < 1e-12
+            assert np.max(np.abs(w - ref)) < 1e-8, (tage, tau, start, trunc, slope, np.max(np.abs(w - ref)))
+            checked += 1
+    assert checked == 16
+
+
 def test_sfh_weights_constant_and_burst():
     """sfh 1 with const and fburst: (1 - C - B) tau part + C constant part + B burst at tburst (two nodes)."""
     tf = synth.age_grid(107)

@@ -166,9 +166,12 @@ def _sfh_models(sfh, g):
             start = tage * rng.uniform(0.05, 0.3) if sfh != 0 and m % 2 else 0.0
             trunc = start + (tage - start) * rng.uniform(0.2, 0.8) if sfh != 0 and m % 3 else 0.0
             tburst = tage * rng.uniform(0.1, 0.9)
+            slope = rng.uniform(-1.5, 1.0) if sfh == 5 else 0.0
             ends = [(tage - start) * 1e9, tage * 1e9, (tage - tburst) * 1e9]
             if trunc > 0.0:
                 ends.append((tage - trunc) * 1e9)
+                if slope < 0.0 and (tage - trunc) * 1e9 + 1e9 / slope > 0.0:   # Simha's zero crossing t0
+                    ends.append((tage - trunc) * 1e9 + 1e9 / slope)
             if all(np.min(np.abs(g.log_age - np.log10(e))) > 2e-3 for e in ends):
                 break
         prm[SLOT["tage"], m] = tage
@@ -178,10 +181,11 @@ def _sfh_models(sfh, g):
         prm[SLOT["tburst"], m] = tburst
         prm[SLOT["sf_start"], m] = start
         prm[SLOT["sf_trunc"], m] = trunc
+        prm[SLOT["sf_slope"], m] = slope
     return prm
 
 
-@pytest.mark.parametrize("sfh", [0, 1, 4])
+@pytest.mark.parametrize("sfh", [0, 1, 4, 5])
 def test_sfh_rows(sfh):
     """params_bar's SFH rows (batch_sfh_gradients, opensps_csp_batch.f90:5198-5217) against central differences of
     the oracle's forward batch (h = 1e-4 relative: the SFH weights round at ~1e-10); const and fburst at zero against
@@ -191,7 +195,8 @@ def test_sfh_rows(sfh):
     n = prm.shape[1]
     cot = _cot(n, g, seed=19)
     an = _grad(g, b, cz, prm, cot)["params_bar"]
-    names = ["tage"] if sfh == 0 else ["tage", "tau", "const", "fburst", "tburst", "sf_start", "sf_trunc"]
+    names = ["tage"] if sfh == 0 else (["tage", "tau", "sf_start", "sf_trunc", "sf_slope"] if sfh == 5 else
+                                       ["tage", "tau", "const", "fburst", "tburst", "sf_start", "sf_trunc"])
     for name in names:
         s = SLOT[name]
         x = prm[s].copy()
@@ -206,7 +211,9 @@ def test_sfh_rows(sfh):
         if name == "sf_trunc":   # zero at zero: no truncation (a small positive sf_trunc truncates almost all)
             assert np.all(a[x == 0.0] == 0.0), name
         _check(fd[ok], a[ok], 2e-3 if np.any(x[ok] == 0.0) else 1e-4, name)
-    for name in ("sf_slope",) + (("tau", "const", "fburst", "tburst", "sf_start", "sf_trunc") if sfh == 0 else ()):
+    zero = {0: ("sf_slope", "tau", "const", "fburst", "tburst", "sf_start", "sf_trunc"),
+            5: ("const", "fburst", "tburst")}.get(sfh, ("sf_slope",))
+    for name in zero:
         assert np.all(an[:, O.GRAD_ROWS[name]] == 0.0), name
 
 
