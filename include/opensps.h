@@ -98,12 +98,17 @@ void opensps_filterset_count(int fset, int *n, int *status);
 void opensps_filterset_free(int fset, int *status);
 void opensps_context_set_filters(int handle, int fset, int resample, int grid_policy, int vega_ref_band, int *status);
 void opensps_context_set_mag_compute(int handle, int n_bands, const int *mask, int *status);
+/* The SFH table of sfh = 2, 3 (reference include/opensps.h:229-232): ntab forward ages [Gyr], SFRs [M_sun/yr] and
+ * metallicities (met may be NULL; the batch uses one metallicity, as the reference's). The reference loads sfh = 2's
+ * table from a file at set-up; here both types read this table. status 1 bad handle, 3 ntab out of 1 .. 20000. */
+void opensps_context_set_sfh_tab(int handle, int ntab, const double *age, const double *sfr, const double *met,
+                                 int *status);
 /* No reference counterpart: the context's bands on the model grid (nspec x nbands), normalised. */
 void opensps_context_get_bands(int handle, double *bands, int *status);
 
 /* ---- the batch (reference include/opensps.h:264-316) ---- */
 /* status 0, 1 bad handle / no basis, 2 unsupported (a parameter that changes the SSPs, spectral indices, dust
- * emission, nebular emission, Vega magnitudes without reference spectra, an SFH type other than 0, 1, 4, 5), 3 unknown
+ * emission, nebular emission, Vega magnitudes without reference spectra, an SFH type other than 0 to 5), 3 unknown
  * parameter or bad shape, 5 no HIP device, 10 device error. */
 void opensps_context_compute_csp_batch(int handle, int n, int n_params, const char *const *names,
                                        const double *const *values, const double *ssp_weights, double *mags,

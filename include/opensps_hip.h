@@ -96,6 +96,11 @@ int opensps_hip_set_grid(opensps_hip_ctx_t *ctx, int nspec, const double *lam, i
 int opensps_hip_set_bands(opensps_hip_ctx_t *ctx, int nbands, const double *bands);
 /* The computed bands (opensps_context_set_mag_compute, include/opensps.h:233): mask[b] != 0; NULL: all. */
 int opensps_hip_set_mag_mask(opensps_hip_ctx_t *ctx, int nbands, const int *mask);
+/* The SFH table of sfh = 2, 3 (ctx%state%sfh_tab: opensps_context_set_sfh_tab, src/bindings/c/opensps_c_api.f90:
+ * 2062-2089; load_tabular_sfh for sfh = 2): ntab forward ages in years, increasing, and SFRs (M_sun / yr), host;
+ * 0 <= ntab <= 20000 (NTABMAX), else status 3. The weights of a tabular model (sfh_weights_core_pc,
+ * src/physics/opensps_sfh.f90:1219-1292) are the table's bins as linear SFR segments up to its tage. */
+int opensps_hip_set_sfh_tab(opensps_hip_ctx_t *ctx, int ntab, const double *age_yr, const double *sfr);
 /* The Vega system (magnitudes_from_fluxes, src/spectra/opensps_batch_photometry.f90:185-232, with
  * compute_vega_mags = 1): magvega (host, nbands) Vega's AB magnitude per band (filters_zero_points,
  * src/spectra/opensps_filters.f90:536-556), vref the reference band (1-based; 0: Vega is magnitude 0 in every band;

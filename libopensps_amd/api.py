@@ -50,6 +50,7 @@ def lib():
         L.opensps_context_get_params.argtypes = [i, vp, ip]
         L.opensps_context_get_filter_data.argtypes = [i, vp, vp, vp, ip]
         L.opensps_context_set_reference_spectra.argtypes = [i, i, vp, vp, i, i, vp, vp, i, ip]
+        L.opensps_context_set_sfh_tab.argtypes = [i, i, vp, vp, vp, ip]
         L.opensps_filterset_create.argtypes = [ip, ip]
         L.opensps_filterset_add.argtypes = [i, cp, i, vp, vp, i, d, d, ip]
         L.opensps_filterset_count.argtypes = [i, ip, ip]
@@ -171,6 +172,15 @@ class Context:
         lib().opensps_context_set_reference_spectra(self.handle, lv.size, lv.ctypes.data, fv.ctypes.data, int(bool(flv)),
                                                     ls.size, ls.ctypes.data, fs.ctypes.data, int(bool(fls)),
                                                     C.byref(st))
+        _check(st)
+
+    def set_sfh_tab(self, age_gyr, sfr, met=None):
+        """The SFH table of sfh 2, 3 (opensps_context_set_sfh_tab): forward ages (Gyr), SFRs (M_sun / yr), Z."""
+        a, s = _d(age_gyr), _d(sfr)
+        m = None if met is None else _d(met)
+        st = C.c_int()
+        lib().opensps_context_set_sfh_tab(self.handle, a.size, a.ctypes.data, s.ctypes.data,
+                                          None if m is None else m.ctypes.data, C.byref(st))
         _check(st)
 
     def set_filters(self, fset, resample=0, grid_policy=2, vega_ref_band=0):

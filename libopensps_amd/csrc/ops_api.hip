@@ -147,7 +147,7 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
                   c->d_bc32, c->d_mask, c->d_cz, c->d_cdl, c->yo.p, c->wcol.p, c->w64.p, c->order.p, c->tiles.p,
                   c->misc.p, c->spec.p, c->flag.p, c->d_rdlam, c->d_ltab32, c->d_wl32, c->d_sm32, c->d_bkt, c->hdr.p, c->wsfh.p,
                   c->opu.p, c->opwc.p, c->opsum.p, c->opseg.p, c->opimf.p, c->opspart.p, c->d_bp_ptr, c->d_bp_band,
-                  c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p, c->gwb.p, c->d_magvega};
+                  c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p, c->gwb.p, c->d_magvega, c->d_tab};
     for (void* p : ps) free_ptr(p);
     free_operator(c);
     free_full(c);
@@ -412,6 +412,21 @@ int opensps_hip_set_mag_mask(opensps_hip_ctx_t* c, int nbands, const int* mask) 
     if (nbands != c->nbands) return ops_fail(c, 3, "set_mag_mask: %d bands, context has %d", nbands, c->nbands);
     for (int b = 0; b < nbands; b++) c->h_mask[b] = mask ? (mask[b] != 0) : 1;
     return upload_mask(c);
+}
+
+int opensps_hip_set_sfh_tab(opensps_hip_ctx_t* c, int ntab, const double* age_yr, const double* sfr) {
+    if (!c) return 1;
+    if (ntab < 0 || ntab > OPS_NTABMAX || (ntab > 0 && (!age_yr || !sfr)))
+        return ops_fail(c, 3, "set_sfh_tab: ntab %d out of range (0 to %d) or NULL arrays", ntab, OPS_NTABMAX);
+    std::vector<double> t(2 * (size_t)ntab);
+    for (int i = 0; i < ntab; i++) {
+        t[i] = age_yr[i];
+        t[ntab + i] = sfr[i];
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // (a batch in flight may still read the old table)
+    UPLOAD(c, c->d_tab, t.data(), t.size());
+    c->ntab = ntab;
+    return 0;
 }
 
 int opensps_hip_set_vega(opensps_hip_ctx_t* c, int nbands, const double* magvega, int vref) {
@@ -871,6 +886,8 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     pa.zlo = d_zlo;
     pa.gcnt = d_gcnt;
     pa.rank = d_rank;
+    pa.tab = c->d_tab;
+    pa.ntab = c->ntab;
     pa.it = d_it;
     pa.idx = a->idx;
     pa.status = d_status;

@@ -88,6 +88,9 @@ struct OpsCtx {
     // the Vega system (set_vega): per band Vega's AB magnitude, the 1-based reference band (0: none) and its value
     double* d_magvega = nullptr;
     int vega_on = 0, vega_ref = 0;
+    // the SFH table of sfh 2, 3 (set_sfh_tab): ntab forward ages (yr), then ntab SFRs
+    double* d_tab = nullptr;
+    int ntab = 0;
     double vega_anchor = 0.0;
 
     // ---- cosmology ----

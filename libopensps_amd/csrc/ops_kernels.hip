@@ -223,12 +223,53 @@ __device__ double d_simha_frac(const double* p, double log_tmax) {
     return (t_prime >= 0.0 && mass_linear + mass_tau > OPS_SAFE_FLOOR) ? mass_linear / (mass_linear + mass_tau) : 0.0;
 }
 
-// sfh_weights_core_pc (opensps_sfh.f90:1077-1294) for sfh 0, 1, 4, 5. Returns 0, or 2 (unsupported).
+// The tabular SFH (sfh_weights_core_pc, opensps_sfh.f90:1219-1292, one metallicity as the batch calls it) by one
+// wavefront: each bin of the context's table (forward ages tab[0..ntab), years; SFRs tab[ntab..2 ntab)) a linear SFR
+// segment over its lookback interval (the type-5 moments with the bin's slope), its hat weights normalised and
+// scaled by the segment's mass. The bins go one after another (the weights of neighbouring bins share nodes).
+__device__ void wave_tab_weights(const double* __restrict__ tf, int nt, double tiny_logt, const double* __restrict__ tab,
+                                 int ntab, double tage, double* w, double* t1, double* m0s, double* m1s, int lane) {
+    for (int i = lane; i < nt; i += 64) w[i] = 0.0;
+    WAVE_SYNC();
+    const double lmin = pow(10.0, tiny_logt), lmax = pow(10.0, tf[nt - 1]);
+    const double* age = tab;
+    const double* sfr = tab + ntab;
+    for (int j = 0; j < ntab - 1; j++) {
+        const double ta = tage * 1.0e9 - age[j + 1], tb = tage * 1.0e9 - age[j];
+        if (tb < 0.0) continue;
+        const double dt = fmax(tb - ta, OPS_SAFE_FLOOR);
+        SfhP s;
+        s.tau = 0.0; s.tburst = 0.0; s.tb = 0.0; s.t0 = 0.0; s.simha = 0;
+        s.type = 5;
+        s.sf_slope = -(sfr[j + 1] - sfr[j]) / dt / fmax(sfr[j + 1], OPS_SAFE_FLOOR);
+        s.tq = fmin(fmax(ta, lmin), lmax);
+        s.tage = fmin(fmax(tb, lmin), lmax);
+        s.sf_trunc = s.tage - s.tq;
+        const double mass2 = sfr[j + 1] * (1.0 + s.sf_slope / 2.0 * (s.tage + s.tq - 2.0 * ta)) * (s.tage - s.tq);
+        const int imin = min(max(wave_find_interval(tf, nt, log10(fmax(ta, OPS_SAFE_FLOOR)), lane) - 1, 0), nt);
+        const int imax = min(max(wave_find_interval(tf, nt, log10(fmax(tb, OPS_SAFE_FLOOR)), lane) + 2, 0), nt);
+        wave_ssp_weights(tf, nt, tiny_logt, s, imin, imax, t1, m0s, m1s, lane);
+        double mass1 = 0.0;
+        for (int i = lane; i < nt; i += 64) mass1 += t1[i];
+        mass1 = wave_sum(mass1);
+        if (mass1 < OPS_SAFE_FLOOR) mass1 = 1.0;
+        for (int i = lane; i < nt; i += 64) w[i] = w[i] + t1[i] * (mass2 / mass1);
+        WAVE_SYNC();
+    }
+}
+
+// sfh_weights_core_pc (opensps_sfh.f90:1077-1294) for sfh 0, 1, 4, 5 and 2, 3 (the context's table). Returns 0, or 2
+// (unsupported).
 __device__ int wave_sfh_weights(const double* __restrict__ tf, int nt, double tiny_logt, const double* p, double* w,
-                                double* t1, double* t2, double* m0s, double* m1s, int lane) {
+                                double* t1, double* t2, double* m0s, double* m1s, const double* tab, int ntab,
+                                int lane) {
     SfhP s;
-    d_convert_sfh(p, s);
     int sfh = (int)p[OPS_P_SFH];
+    if (sfh == 2 || sfh == 3) {
+        wave_tab_weights(tf, nt, tiny_logt, tab, ntab, p[OPS_P_TAGE], w, t1, m0s, m1s, lane);
+        return 0;
+    }
+    d_convert_sfh(p, s);
     int imax = wave_find_interval(tf, nt, log10(s.tage), lane) + 2;
     imax = min(max(imax, 1), nt);
     if (sfh == 0) {
@@ -702,7 +743,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
             for (int k = lane; k < nt; k += 64) w[k] = a.weights_in[(size_t)m * nt + k];
             WAVE_SYNC();
         } else {
-            st = wave_sfh_weights(a.time, nt, a.tiny_logt, p, w, t1, t2, m0s, m1s, lane);
+            st = wave_sfh_weights(a.time, nt, a.tiny_logt, p, w, t1, t2, m0s, m1s, a.tab, a.ntab, lane);
         }
     }
     if (st != 0) {

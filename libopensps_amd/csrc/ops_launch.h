@@ -38,6 +38,8 @@ struct PrepArgs {
     uint16_t *whi, *wlo;
     int *zlo, *it, *idx, *status, *flag;
     int *gcnt, *rank;           // per pair the models counted so far; per model its rank in its pair (k_group)
+    const double* tab;          // the SFH table of sfh 2, 3: ntab forward ages (yr), then ntab SFRs (M_sun / yr)
+    int ntab;
     // the chain's per-model constants
     int add_igm, jtop, ncosmo;
     LamLookup lk;
@@ -45,7 +47,8 @@ struct PrepArgs {
     double* hdr;
 };
 
-#define OPS_GCNT_SLOTS 32   // counters a metallicity pair (k_prep's ranks, k_group's prefix)
+#define OPS_GCNT_SLOTS 32
+#define OPS_NTABMAX 20000   // rows of the SFH table (NTABMAX, src/core/opensps_constants.f90:147)   // counters a metallicity pair (k_prep's ranks, k_group's prefix)
 
 struct GroupArgs {
     int n, npair, parts;

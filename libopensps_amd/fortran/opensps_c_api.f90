@@ -360,6 +360,37 @@ contains
         end function table_ok
     end subroutine
 
+    !> opensps_context_set_sfh_tab (include/opensps.h:229-232; src/bindings/c/opensps_c_api.f90:2062-2089): the SFH
+    !> table of sfh = 2, 3, ntab forward ages (Gyr, stored in years), SFRs (M_sun / yr) and metallicities (met may be
+    !> NULL; the batch uses one metallicity, as the reference's). status 1 bad handle, 3 ntab out of 1 .. 20000.
+    subroutine opensps_context_set_sfh_tab(handle, ntab, age, sfr, met, status) bind(C, name="opensps_context_set_sfh_tab")
+        integer(c_int), value :: handle, ntab
+        type(c_ptr), value :: age, sfr, met
+        integer(c_int), intent(out) :: status
+        real(c_double), pointer :: f(:)
+        if (bad_handle(handle, status)) return
+        if (ntab < 1 .or. ntab > 20000 .or. .not. c_associated(age) .or. .not. c_associated(sfr)) then
+            status = 3
+            call set_error(3, 'opensps_context_set_sfh_tab: ntab out of range')
+            return
+        end if
+        associate(c => ctxs(handle))
+            c%ntab = ntab
+            call c_f_pointer(age, f, [ntab])
+            c%tab_age = f * 1.0e9_wp
+            call c_f_pointer(sfr, f, [ntab])
+            c%tab_sfr = f
+            if (allocated(c%tab_met)) deallocate(c%tab_met)
+            allocate(c%tab_met(ntab))
+            c%tab_met = 0.0_wp
+            if (c_associated(met)) then
+                call c_f_pointer(met, f, [ntab])
+                c%tab_met = f
+            end if
+            c%dev_tab = .false.
+        end associate
+    end subroutine
+
     !> The filters' zero points from the reference spectra (opensps_context_set_filters's filters_zero_points after
     !> the calibration, src/spectra/opensps_context_filters.f90:113-124, with sun_to_vega_system when
     !> compute_vega_mags is 1); without reference spectra none (99)

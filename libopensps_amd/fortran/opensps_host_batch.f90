@@ -52,6 +52,44 @@ contains
         end if
     end function gammainc12
 
+    !> The SFR of a tabular model (sfh 2, 3) at its age (Gyr): the table interpolated linearly in time, at least 0, in
+    !> M_sun / yr (get_sfh_properties_at_age_pc, src/physics/opensps_sfh.f90:460-481; its mass fraction stays 1)
+    real(WP) function sfr_tab(c, age_gyr) result(sfr)
+        type(host_ctx_t), intent(in) :: c
+        real(WP), intent(in) :: age_gyr
+        real(WP) :: t, slope, a1, a2, s1, s2
+        integer :: lo, hi, mid, i, n
+        n = c%ntab
+        sfr = 0.0_wp
+        if (n < 1) return
+        t = age_gyr * 1.0e9_wp
+        ! find_interval_tab_time (:1041-1073) on the stored table, clamped to a bin; rows beyond the table read 0
+        i = 1
+        if (n >= 2) then
+            lo = 0
+            hi = n + 1
+            do while (hi - lo > 1)
+                mid = (hi + lo) / 2
+                if ((c%tab_age(n) >= c%tab_age(1)) .eqv. (t >= c%tab_age(mid))) then
+                    lo = mid
+                else
+                    hi = mid
+                end if
+            end do
+            i = lo
+            if (t == c%tab_age(1)) i = 1
+            if (t == c%tab_age(n)) i = n - 1
+        end if
+        i = max(min(i, n - 1), 1)
+        a1 = c%tab_age(i); s1 = c%tab_sfr(i)
+        a2 = 0.0_wp; s2 = 0.0_wp
+        if (i + 1 <= n) then
+            a2 = c%tab_age(i + 1); s2 = c%tab_sfr(i + 1)
+        end if
+        slope = (s2 - s1) / (a2 - a1)
+        sfr = max(s1 + slope * (t - a1), 0.0_wp)
+    end function sfr_tab
+
     !> The SFR of a model at its age per unit mass formed: get_sfh_properties_at_age (src/physics/opensps_sfh.f90:282-486)
     !> for sfh 0, 1, 4, 5, divided by the mass fraction formed as batch_model_indices does (opensps_csp_batch.f90:1556-1560)
     real(WP) function sfr_at_age(p, tmax_log) result(sfr)
@@ -375,7 +413,11 @@ contains
                 if (c_associated(c_weights)) then
                     outv(m) = ieee_nan()
                 else
-                    outv(m) = sfr_at_age(prm(m, :), c%log_age(nt))
+                    if (nint(prm(m, P_SFH)) == 2 .or. nint(prm(m, P_SFH)) == 3) then
+                        outv(m) = sfr_tab(c, prm(m, P_TAGE))
+                    else
+                        outv(m) = sfr_at_age(prm(m, :), c%log_age(nt))
+                    end if
                 end if
             end do
         end if

@@ -40,13 +40,16 @@ module opensps_host_context
         logical :: has_ref = .false., vega_flambda = .true., sun_flambda = .false.
         real(WP), allocatable :: vega_lam(:), vega_flux(:), sun_lam(:), sun_flux(:), magvega(:), magsun(:)
         integer :: vega_ref = 0
+        ! the SFH table of sfh 2, 3 (opensps_context_set_sfh_tab): forward ages (yr), SFRs (M_sun / yr), metallicities
+        integer :: ntab = 0
+        real(WP), allocatable :: tab_age(:), tab_sfr(:), tab_met(:)
         ! cosmology
         real(WP) :: cz(500) = 0.0_wp, cdl(500) = 0.0_wp
         logical :: cosmo_dirty = .true.
         ! device side
         type(c_ptr) :: dctx = c_null_ptr
         logical :: dev_grid = .false., dev_bands = .false., dev_cosmo = .false., dev_mask = .false.
-        logical :: dev_vega = .false.
+        logical :: dev_vega = .false., dev_tab = .false.
         type(dbuf_t) :: d_prm, d_w, d_mags, d_spec, d_mass, d_lbol, d_fband, d_young, d_old, d_idx
         integer :: last_n = 0, last_bytes = 8
         real(WP), allocatable :: last_maggies(:, :)
@@ -305,7 +308,7 @@ contains
                 return
             end if
             c%dev_grid = .false.; c%dev_bands = .false.; c%dev_cosmo = .false.; c%dev_mask = .false.
-            c%dev_prof = .false.; c%dev_vega = .false.
+            c%dev_prof = .false.; c%dev_vega = .false.; c%dev_tab = .false.
         end if
         if (.not. c%dev_grid) then
             rc = opensps_hip_set_grid(c%dctx, int(c%nspec, c_int), c%lam, int(c%nt, c_int), c%log_age, &
@@ -335,6 +338,15 @@ contains
             if (rc /= 0) goto 900
             c%dev_mask = .true.
             c%dev_vega = .false.
+        end if
+        if (.not. c%dev_tab) then
+            if (c%ntab > 0) then
+                rc = opensps_hip_set_sfh_tab(c%dctx, int(c%ntab, c_int), c%tab_age, c%tab_sfr)
+            else
+                rc = opensps_hip_set_sfh_tab(c%dctx, 0_c_int, [0.0_wp], [0.0_wp])
+            end if
+            if (rc /= 0) goto 900
+            c%dev_tab = .true.
         end if
         if (.not. c%dev_vega) then
             ! the Vega system (magnitudes_from_fluxes, src/spectra/opensps_batch_photometry.f90:185-232)

@@ -64,6 +64,7 @@ def lib():
         L.opensps_hip_set_bands.argtypes = [vp, i, vp]
         L.opensps_hip_set_mag_mask.argtypes = [vp, i, vp]
         L.opensps_hip_set_cosmology.argtypes = [vp, i, vp, vp]
+        L.opensps_hip_set_sfh_tab.argtypes = [vp, i, vp, vp]
         L.opensps_hip_set_options.argtypes = [vp, i, i]
         L.opensps_hip_csp_batch.argtypes = [vp, C.POINTER(BatchArgs)]
         L.opensps_hip_sync.argtypes = [vp]
@@ -142,6 +143,12 @@ class DeviceContext:
     def set_mag_mask(self, mask):
         m = None if mask is None else _host(mask, np.int32)
         self._check(lib().opensps_hip_set_mag_mask(self._ctx, self.nbands, None if m is None else m.ctypes.data))
+
+    def set_sfh_tab(self, age_gyr, sfr):
+        """The SFH table of sfh 2, 3: forward ages (Gyr) and SFRs (M_sun / yr)."""
+        a = np.ascontiguousarray(np.asarray(age_gyr, dtype=np.float64) * 1.0e9)
+        s = np.ascontiguousarray(sfr, dtype=np.float64)
+        self._check(lib().opensps_hip_set_sfh_tab(self._ctx, a.size, a.ctypes.data, s.ctypes.data))
 
     def set_cosmology(self, z, dl):
         z, dl = _host(z), _host(dl)

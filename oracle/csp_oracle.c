@@ -259,14 +259,61 @@ double orc_simha_frac(const double *p, double log_tmax) {
     return 0.0;
 }
 
-/* sfh_weights_core_pc (opensps_sfh.f90:1077-1294) for sfh = 0, 1, 4 (with const and burst) and 5 (Simha).
- * Returns 0, or 2 for an SFH type this build does not compute. */
+/* The context's SFH table (opensps_context_set_sfh_tab, src/bindings/c/opensps_c_api.f90:2062-2089): forward ages
+ * in years and SFRs (M_sun / yr), read by sfh = 2, 3 */
+#define ORC_NTABMAX 20000
+static int orc_ntab = 0;
+static double orc_tab_age[ORC_NTABMAX + 1], orc_tab_sfr[ORC_NTABMAX + 1];
+int orc_set_sfh_tab(int ntab, const double *age_yr, const double *sfr) {
+    if (ntab < 0 || ntab > ORC_NTABMAX) return 3;
+    for (int i = 0; i <= ORC_NTABMAX; i++) { orc_tab_age[i] = 0.0; orc_tab_sfr[i] = 0.0; }
+    for (int i = 0; i < ntab; i++) { orc_tab_age[i] = age_yr[i]; orc_tab_sfr[i] = sfr[i]; }
+    orc_ntab = ntab;
+    return 0;
+}
+
+/* The tabular SFH (sfh_weights_core_pc, opensps_sfh.f90:1219-1292, one metallicity as the batch calls it): each bin
+ * of the table a linear SFR segment (the type-5 moments with the bin's slope) over its lookback interval, its hat
+ * weights normalised and scaled by the segment's mass */
+static void tab_weights(const double *tf, int nt, double tiny_logt, double tage, double *w, double *tmp1) {
+    for (int i = 0; i < nt; i++) w[i] = 0.0;
+    const double lmin = pow(10.0, tiny_logt), lmax = pow(10.0, tf[nt - 1]);
+    for (int j = 0; j < orc_ntab - 1; j++) {
+        const double t1 = tage * 1.0e9 - orc_tab_age[j + 1], t2 = tage * 1.0e9 - orc_tab_age[j];
+        if (t2 < 0.0) continue;
+        const double dt = fmax(t2 - t1, SAFE_FLOOR);
+        sfhp_t s;
+        memset(&s, 0, sizeof s);
+        s.sf_slope = -(orc_tab_sfr[j + 1] - orc_tab_sfr[j]) / dt / fmax(orc_tab_sfr[j + 1], SAFE_FLOOR);
+        s.type = 5;
+        s.tq = fmin(fmax(t1, lmin), lmax);
+        s.tage = fmin(fmax(t2, lmin), lmax);
+        s.sf_trunc = s.tage - s.tq;
+        const double mass2 = orc_tab_sfr[j + 1] * (1.0 + s.sf_slope / 2.0 * (s.tage + s.tq - 2.0 * t1)) * (s.tage - s.tq);
+        int imin = orc_find_interval(tf, nt, log10(fmax(t1, SAFE_FLOOR))) - 1;
+        imin = imin < 0 ? 0 : (imin > nt ? nt : imin);
+        int imax = orc_find_interval(tf, nt, log10(fmax(t2, SAFE_FLOOR))) + 2;
+        imax = imax < 0 ? 0 : (imax > nt ? nt : imax);
+        compute_ssp_weights(tf, nt, tiny_logt, &s, imin, imax, tmp1);
+        double mass1 = 0.0;
+        for (int i = 0; i < nt; i++) mass1 += tmp1[i];
+        if (mass1 < SAFE_FLOOR) mass1 = 1.0;
+        for (int i = 0; i < nt; i++) w[i] = w[i] + tmp1[i] * (mass2 / mass1);
+    }
+}
+
+/* sfh_weights_core_pc (opensps_sfh.f90:1077-1294) for sfh = 0, 1, 4 (with const and burst), 5 (Simha) and 2, 3
+ * (the context's table). Returns 0, or 2 for an SFH type this build does not compute. */
 int orc_sfh_weights(const double *tf, int nt, double tiny_logt, const double *p, double *w, double *tmp1,
                     double *tmp2) {
     sfhp_t s;
     int sfh = (int)p[OPS_P_SFH];
     double tage = p[OPS_P_TAGE];
     for (int i = 0; i < nt; i++) { w[i] = 0.0; tmp1[i] = 0.0; tmp2[i] = 0.0; }
+    if (sfh == 2 || sfh == 3) {
+        tab_weights(tf, nt, tiny_logt, tage, w, tmp1);
+        return 0;
+    }
     convert_sfhparams(p, tage, &s);
     int imax = orc_find_interval(tf, nt, log10(s.tage)) + 2;
     if (imax < 1) imax = 1;

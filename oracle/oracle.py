@@ -35,6 +35,8 @@ def lib():
         L.orc_expi.restype = C.c_double
         L.orc_sfh_weights.argtypes = [_dp, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
         L.orc_sfh_weights.restype = C.c_int
+        L.orc_set_sfh_tab.argtypes = [C.c_int, _dp, _dp]
+        L.orc_set_sfh_tab.restype = C.c_int
         L.orc_sfh_weights_tangent.argtypes = [_dp, C.c_int, C.c_double, _dp, C.c_int, _dp, _dp, _dp]
         L.orc_sfh_weights_tangent.restype = C.c_int
         L.orc_sfh_gradients.argtypes = [_dp, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
@@ -95,6 +97,13 @@ def sfh_weights(time_full, p, tiny_logt=0.0):
     t2 = np.zeros(tf.size)
     st = lib().orc_sfh_weights(tf, tf.size, float(tiny_logt), _d(p), w, t1, t2)
     return w, st
+
+
+def set_sfh_tab(age_gyr, sfr):
+    """The oracle's SFH table for sfh = 2, 3 (forward ages in Gyr, SFR in M_sun / yr; stored in years)."""
+    a = _d(np.asarray(age_gyr, float) * 1.0e9)
+    st = lib().orc_set_sfh_tab(a.size, a, _d(sfr))
+    assert st == 0
 
 
 # the SFH tangents' parameters in the order of sfh_weights_tangent's q = 1 .. 8

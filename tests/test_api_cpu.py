@@ -146,6 +146,17 @@ def test_batch_statuses_before_the_device():
         assert st.value == 2
 
 
+def test_sfh_tab_statuses():
+    """opensps_context_set_sfh_tab: ntab out of 1 .. 20000 is status 3."""
+    with api.Context() as ctx:
+        ctx.set_sfh_tab([0.0, 1.0], [1.0, 2.0], [0.01, 0.02])
+        ctx.set_sfh_tab([0.0, 1.0], [1.0, 2.0])
+        for bad in (np.zeros(0), np.zeros(20001)):
+            with pytest.raises(api.OpenSPSError) as e:
+                ctx.set_sfh_tab(bad, bad)
+            assert e.value.status == 3
+
+
 def test_batch_without_device_is_status_5():
     import torch
     if torch.cuda.is_available():

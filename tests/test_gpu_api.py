@@ -185,3 +185,51 @@ def test_api_vega_system(precision):
                                        want=("mags",), grads=("params_bar",))
         np.testing.assert_allclose(r["mags"][ok], ref3[ok], rtol=0, atol=1e-12 if precision == "double" else 1e-6)
         assert np.all(np.isfinite(r["params_bar"]))
+
+
+@pytest.mark.gpu
+def test_api_tabular_sfr():
+    """sfr of a tabular model: the table interpolated linearly at its age, M_sun / yr, at least 0."""
+    g = grid(1)
+    fs = synth.gaussian_filters(4)
+    age = np.array([0.0, 1.0, 3.0, 6.0, 10.0])
+    sfr = np.array([0.0, 5.0, 2.0, 8.0, 1.0])
+    tage = np.array([0.5, 2.0, 3.0, 7.5, 9.9])
+    print("case: api tabular sfr", flush=True)
+    with _ctx(g, fs, "double", igm=0) as ctx:
+        ctx.set_sfh_tab(age, sfr)
+        r = ctx.compute_csp_batch({"sfh": np.full(5, 3.0), "tage": tage}, n=5, want=("mags", "sfr", "mass"))
+    np.testing.assert_allclose(r["sfr"], np.interp(tage, age, sfr), rtol=1e-13)
+    assert np.all(r["mass"] > 0)
+
+
+@pytest.mark.gpu
+def test_api_sfr_output():
+    """The sfr output (batch_model_indices, opensps_csp_batch.f90:1556-1560: get_sfh_properties_at_age's SFR over the
+    fraction formed) is the SFR at the model's age per unit mass formed by then, SFR(t') / integral_0^t' SFR, per
+    year: against quadrature of the SFR for tau (1), delayed tau (4) and Simha (5: truncation, slopes of both signs)."""
+    from scipy import integrate
+    g = grid(1)
+    fs = synth.gaussian_filters(4)
+    cases = [(1, 6.0, 2.0, 0.0, 0.0), (4, 9.0, 3.0, 0.0, 0.0), (4, 0.5, 7.0, 0.0, 0.0), (5, 10.0, 2.5, 6.0, -0.2),
+             (5, 12.0, 1.5, 4.0, 0.3), (5, 8.0, 4.0, 7.5, -2.0), (5, 5.0, 3.0, 0.0, -0.5)]
+    n = len(cases)
+    vals = {k: np.array([c[i] for c in cases], dtype=float)
+            for i, k in enumerate(("sfh", "tage", "tau", "sf_trunc", "sf_slope"))}
+    print(f"case: api sfr n={n}", flush=True)
+    with _ctx(g, fs, "double", igm=0) as ctx:
+        r = ctx.compute_csp_batch(vals, n=n, want=("mags", "sfr"))
+    for m, (sfh, tage, tau, trunc, slope) in enumerate(cases):
+        on = trunc > 0.0 and trunc < tage
+        s_tr = (trunc / tau) * np.exp(-trunc / tau)
+
+        def sfr(x):   # forward time since the start, Gyr
+            if sfh == 1:
+                return np.exp(-x / tau)
+            if sfh == 4 or not on or x <= trunc:
+                return (x / tau) * np.exp(-x / tau)
+            return s_tr * max(1.0 - (-slope) * (x - trunc), 0.0)
+        pts = [trunc] if on else None
+        mass = integrate.quad(sfr, 0.0, tage, epsabs=0, epsrel=1e-12, limit=200, points=pts)[0]
+        want = sfr(tage) / mass / 1e9
+        assert abs(r["sfr"][m] - want) <= 1e-9 * abs(want) + 1e-30, (cases[m], r["sfr"][m], want)

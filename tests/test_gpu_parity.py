@@ -204,12 +204,41 @@ def test_other_sfh_types(sfh):
 def test_unsupported_sfh_status():
     from libopensps_amd import hip as H
     g, b, prm = _parametric_case(8)
-    prm[SLOT["sfh"]] = 2   # tabular: no table in this build
+    prm[SLOT["sfh"]] = 6   # no such SFH type
     ctx = _ctx(g, b)
     with pytest.raises(H.OpenSPSHipError) as e:
         ctx.batch(_dev(prm), precision=H.FP64)
     assert e.value.status == 2
     ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_tabular_sfh(precision):
+    """sfh 3 (and 2): the context's SFH table as linear segments up to each model's age, one metallicity with the
+    per-model blend, against the oracle; ages before, inside and after the table's span."""
+    from libopensps_amd import hip as H
+    g, b, prm = _parametric_case(48, seed=31)
+    n = prm.shape[1]
+    rng = np.random.default_rng(8)
+    age = np.sort(np.concatenate([[0.0], rng.uniform(0.02, 11.0, 40)]))
+    sfr = rng.uniform(0.1, 30.0, age.size)
+    prm[SLOT["sfh"]] = np.where(np.arange(n) % 2, 3.0, 2.0)
+    prm[SLOT["tage"]] = np.linspace(0.05, 13.5, n)
+    O.set_sfh_tab(age, sfr)
+    try:
+        print(f"case: tabular sfh n={n} precision={precision}", flush=True)
+        want = ("mags", "spec", "mass", "lbol", "idx")
+        ref = O.csp_batch(g, b, prm, cosmo=cosmo(), want=want + ("w",))
+        assert ref["status"] == 0
+        ctx = _ctx(g, b)
+        ctx.set_sfh_tab(age, sfr)
+        out = ctx.batch(_dev(prm), precision=precision, want=want + ("weights",))
+        got = {k: v.detach().cpu().numpy() for k, v in out.items()}
+        ctx.close()
+    finally:
+        O.set_sfh_tab([], [])
+    _check(ref, got, precision)
 
 
 def _lcg(seed):

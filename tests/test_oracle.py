@@ -119,6 +119,73 @@ def test_sfh_weights_simha_against_quadrature():
     assert checked == 16
 
 
+@pytest.mark.parametrize("case", ["past", "on_a_node", "straddles_now"])
+def test_sfh_weights_tabular_against_quadrature(case):
+    """sfh 2, 3: the table's SFR, linear in forward time between its ages, weighted by the hat kernel of each node and
+    integrated (M_sun: the weights are not normalised). The bin holding the model's age is the reference's own
+    approximation (its shape is taken from the clamped end), so it is checked apart: that bin only moves weights
+    among the youngest nodes and keeps the bin's mass."""
+    tf = synth.age_grid(107)
+    rng = np.random.default_rng(23)
+    age = np.sort(np.concatenate([[0.0], rng.uniform(0.05, 12.0, 30)]))
+    sfr = rng.uniform(0.5, 20.0, age.size)
+    O.set_sfh_tab(age, sfr)
+    try:
+        tage = {"past": 12.5, "on_a_node": age[-6], "straddles_now": 0.5 * (age[-8] + age[-7])}[case]
+        p = synth.default_params(1)[:, 0]
+        p[SLOT["sfh"]], p[SLOT["tage"]] = 3, tage
+        w, st = O.sfh_weights(tf, p)
+        assert st == 0
+        p[SLOT["sfh"]] = 2
+        w2, _ = O.sfh_weights(tf, p)
+        np.testing.assert_array_equal(w, w2)
+        T = tage * 1e9
+        A, S = age * 1e9, sfr
+
+        def sfr_lb(t):   # lookback t (yr) -> forward time T - t
+            return float(np.interp(T - t, A, S)) if 0.0 <= T - t <= A[-1] else 0.0
+        past = A[A <= T]
+        ref = _hat_weights_quad_mass(tf, sfr_lb, T - past[-1], T, breaks=T - A)
+        if case == "straddles_now":
+            # the current bin: its mass is the reference's formula, on nodes below its older edge's cell
+            j = np.searchsorted(A, T) - 1
+            m_bin = S[j + 1] * (1.0 + (-(S[j + 1] - S[j]) / (A[j + 1] - A[j]) / S[j + 1]) / 2.0 *
+                                ((T - A[j]) + 1.0 - 2.0 * (T - A[j + 1]))) * ((T - A[j]) - 1.0)
+            assert abs(w.sum() - (ref.sum() + m_bin)) < 1e-8 * w.sum()
+            ref_old = _hat_weights_quad_mass(tf, sfr_lb, T - A[j], T, breaks=T - A)
+            k = np.searchsorted(tf, np.log10(T - A[j])) - 1   # nodes above the current bin's cell are the old bins'
+            np.testing.assert_allclose(w[k + 2:], ref_old[k + 2:], rtol=1e-8, atol=1e-8 * w.max())
+        else:
+            np.testing.assert_allclose(w, ref, rtol=1e-8, atol=1e-9 * w.max())
+    finally:
+        O.set_sfh_tab([], [])
+
+
+def _hat_weights_quad_mass(tf, sfr, t_lo, t_hi, breaks=()):
+    """_hat_weights_quad without the normalisation, with the SFR's kinks given to the integrator."""
+    nt = len(tf)
+    w = np.zeros(nt)
+    lt_lo, lt_hi = np.log10(max(t_lo, 1.0)), np.log10(t_hi)
+    bk = np.asarray(breaks, float)
+    a, b = max(t_lo, 1.0), min(t_hi, 10 ** tf[0])
+    if b > a:
+        pts = bk[(bk > a) & (bk < b)]
+        w[0] += integrate.quad(sfr, a, b, epsabs=0, epsrel=1e-12, limit=400, points=pts if pts.size else None)[0]
+    for j in range(nt - 1):
+        x0, x1 = max(tf[j], lt_lo), min(tf[j + 1], lt_hi)
+        if x1 <= x0:
+            continue
+        dx = tf[j + 1] - tf[j]
+        a, b = 10 ** x0, 10 ** x1
+        pts = bk[(bk > a) & (bk < b)]
+        pts = pts if pts.size else None
+        f_lo = lambda t: sfr(t) * (tf[j + 1] - np.log10(t)) / dx
+        f_hi = lambda t: sfr(t) * (np.log10(t) - tf[j]) / dx
+        w[j] += integrate.quad(f_lo, a, b, epsabs=0, epsrel=1e-12, limit=400, points=pts)[0]
+        w[j + 1] += integrate.quad(f_hi, a, b, epsabs=0, epsrel=1e-12, limit=400, points=pts)[0]
+    return w
+
+
 def test_sfh_weights_constant_and_burst():
     """sfh 1 with const and fburst: (1 - C - B) tau part + C constant part + B burst at tburst (two nodes)."""
     tf = synth.age_grid(107)
