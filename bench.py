@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-operator", action="store_true", help="skip the IMF-per-model workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the batch-adjoint workload")
     ap.add_argument("--no-emulator", action="store_true", help="skip the emulator workload")
+    ap.add_argument("--adjoint-sfh", action="store_true",
+                    help="also the batch adjoint with SFH parameters (params_bar's SFH rows: the SFH tangents)")
     ap.add_argument("--emu-train", type=int, default=4_000_000, help="emulator training spectra")
     ap.add_argument("--emu-epochs", type=int, default=30)
     ap.add_argument("--copy-probe", action="store_true",
@@ -144,12 +146,14 @@ def operator_workload(args, n=4096):
     return result
 
 
-def adjoint_workload(args, n=8192):
+def adjoint_workload(args, n=8192, sfh=False):
     """The batch-adjoint workload: 8192 models on the 107 ages x 22 Z x 5994-pixel grid with weights over ages and
     metallicities (a log-normal SFH of 1 dex times a 22-bin enrichment history), dust, redshift U(0, 4) with IGM and
     smoothing U(50, 300) km/s, and the chi^2 loss of 100-band photometry against a held-out seed with 5 % errors.
     A step is the forward batch and its reverse pass (params_bar and weights_z_bar), timed through libopensps_hip
-    in both precisions; the oracle's reverse pass on a bounded sample."""
+    in both precisions; the oracle's reverse pass on a bounded sample. sfh: the SFH parameters instead of the
+    weights (half delayed tau, half Simha with truncation and a slope; the per-model metallicity blend), params_bar
+    with the SFH rows."""
     import numpy as np
     import torch
     from libopensps_amd import hip as H, synth
@@ -159,6 +163,17 @@ def adjoint_workload(args, n=8192):
     fset = synth.gaussian_filters(100, seed=7)
     prm = synth.adjoint_params(n, seed=19)
     wz = synth.lognormal_weights_z(n, grid.log_age, grid.logz, seed=17)
+    if sfh:
+        from libopensps_amd.params import SLOT
+        pp = synth.parametric_params(n, seed=19)
+        for k in ("sfh", "tage", "tau", "logzsol"):
+            prm[SLOT[k]] = pp[SLOT[k]]
+        rng = np.random.default_rng(29)
+        simha = np.arange(n) % 2 == 1
+        prm[SLOT["sfh"]] = np.where(simha, 5.0, 4.0)
+        prm[SLOT["sf_trunc"]] = np.where(simha, prm[SLOT["tage"]] * rng.uniform(0.3, 0.9, n), 0.0)
+        prm[SLOT["sf_slope"]] = np.where(simha, rng.uniform(-1.0, 1.0, n), 0.0)
+        wz = None
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     bands = O.build_bands(fset, grid.lam)
@@ -169,7 +184,7 @@ def adjoint_workload(args, n=8192):
     ctx.set_cosmology(cz, cdl)
     ctx.set_options(1, 1)
     d_prm = torch.as_tensor(prm, device="cuda")
-    d_wz = torch.as_tensor(wz, device="cuda")
+    d_wz = torch.as_tensor(wz, device="cuda") if wz is not None else None
     f64 = ctx.batch(d_prm, precision=H.FP64, ssp_weights_z=d_wz, want=("mags",))
     loss, dmag = synth.chi2_cotangent(f64["mags"].cpu().numpy(), seed=23)
     d_mb = torch.as_tensor(dmag, device="cuda")
@@ -181,8 +196,9 @@ def adjoint_workload(args, n=8192):
     res, outs = {}, {}
     for prec, name in ((H.MIXED, "mixed"), (H.FP64, "fp64")):
         out = ctx.alloc_outputs(n, prec, want=("mags",))
-        grads = {"params_bar": torch.zeros((n, H.NBGRAD), dtype=torch.float64, device="cuda"),
-                 "weights_z_bar": torch.zeros((n, grid.nz, grid.nt), dtype=torch.float64, device="cuda")}
+        grads = {"params_bar": torch.zeros((n, H.NBGRAD), dtype=torch.float64, device="cuda")}
+        if wz is not None:
+            grads["weights_z_bar"] = torch.zeros((n, grid.nz, grid.nt), dtype=torch.float64, device="cuda")
         lib.opensps_hip_set_profiling(ctx._ctx, 1)
         ctx.launch_grad(d_prm, out, {"mags": d_mb}, grads, prec, ssp_weights_z=d_wz)
         ctx.sync()
@@ -206,13 +222,17 @@ def adjoint_workload(args, n=8192):
                                             "rev_product", "rev_out"], [round(x, 4) for x in fk + gk]))}
         outs[name] = {k: v.detach().cpu().numpy() for k, v in grads.items()}
     lib.opensps_hip_set_profiling(ctx._ctx, 0)
-    rows = {k: O.GRAD_ROWS[k] for k in ("dust2", "dust_index", "zred", "sigma_smooth", "igm_factor")}
+    names = ("dust2", "dust_index", "zred", "sigma_smooth", "igm_factor")
+    if sfh:
+        names = names + ("logzsol", "tage", "tau", "sf_trunc", "sf_slope")
+    rows = {k: O.GRAD_ROWS[k] for k in names}
     acc = {}
     for k, r in rows.items():
         a, b = outs["mixed"]["params_bar"][:, r], outs["fp64"]["params_bar"][:, r]
         acc[k] = float(np.max(np.abs(a - b)) / (np.max(np.abs(b)) + 1e-300))
-    wa, wb = outs["mixed"]["weights_z_bar"].reshape(n, -1), outs["fp64"]["weights_z_bar"].reshape(n, -1)
-    acc["weights_z_bar"] = float(np.max(np.abs(wa - wb) / (np.max(np.abs(wb), axis=1, keepdims=True) + 1e-300)))
+    if wz is not None:
+        wa, wb = outs["mixed"]["weights_z_bar"].reshape(n, -1), outs["fp64"]["weights_z_bar"].reshape(n, -1)
+        acc["weights_z_bar"] = float(np.max(np.abs(wa - wb) / (np.max(np.abs(wb), axis=1, keepdims=True) + 1e-300)))
     # the transposed product: 2 kfull nsp flops per (model, part) column, both parts
     kfull, nsp = grid.nt * grid.nz, ((grid.nspec + 63) // 64) * 64
     flops = 2.0 * kfull * grid.nspec * 2 * n
@@ -220,15 +240,18 @@ def adjoint_workload(args, n=8192):
     roof = {"bound": "mfma", "kernel": "k_mm_bf16x3 (w_bar = basis^T y_bar)", "achieved": flops / (tk * 1e-3) / 1e12,
             "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": flops / (tk * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
             "algorithmic_per_launch": flops}
-    result = {"workload": f"batch adjoint: {n} models, weights over 107 ages x 22 Z (log-normal SFH x enrichment "
-                          "history), 5994 px, dust, IGM, z U(0,4), LSF, chi^2 of 100 filters; forward + reverse",
+    what = ("SFH parameters (delayed tau; Simha with truncation and slope), logzsol blend over 22 Z" if sfh else
+            "weights over 107 ages x 22 Z (log-normal SFH x enrichment history)")
+    result = {"workload": f"batch adjoint: {n} models, {what}, 5994 px, dust, IGM, z U(0,4), LSF, chi^2 of 100 "
+                          "filters; forward + reverse",
               "value": res["mixed"]["models_per_s"], "unit": "models/s (forward + gradient)", "mixed": res["mixed"],
               "fp64": res["fp64"], "speedup_vs_fp64": res["mixed"]["models_per_s"] / res["fp64"]["models_per_s"],
               "accuracy_vs_fp64_gpu": acc, "loss": loss, "roofline": roof, "setup_s": setup}
     if not args.no_cpu:
         m1 = 8
         t1 = time.perf_counter()
-        r = O.csp_batch_grad(grid, bands, np.ascontiguousarray(prm[:, :m1]), ssp_weights_z=wz[:m1],
+        r = O.csp_batch_grad(grid, bands, np.ascontiguousarray(prm[:, :m1]),
+                             ssp_weights_z=wz[:m1] if wz is not None else None,
                              mags_bar=dmag[:m1], cosmo=(cz, cdl))
         dt = time.perf_counter() - t1
         ga, gb = outs["mixed"]["params_bar"][:m1], r["params_bar"]
@@ -476,6 +499,8 @@ def main():
         out["operator"] = operator_workload(args)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_workload(args)
+    if args.adjoint_sfh:
+        out["adjoint_sfh"] = adjoint_workload(args, sfh=True)
     if not args.no_emulator:
         out["emulator"] = emulator_workload(args, value)
     print(json.dumps(out))
