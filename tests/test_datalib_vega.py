@@ -154,3 +154,39 @@ def test_vega_batch_needs_reference_spectra():
         r = ctx.compute_csp_batch({"tage": np.full(2, 5.0), "sfh": np.full(2, 4.0)}, raw_status=True)
         assert r["status"] == 2
         assert "Vega" in api.last_error()
+
+
+def _clean(lam, resp):
+    """filter_clean, strict (src/spectra/opensps_filters.f90:186-230): increasing, repeated wavelengths dropped,
+    negative responses clipped."""
+    l, r = np.asarray(lam, float), np.asarray(resp, float)
+    if l.size > 1 and l[-1] < l[0]:
+        l, r = l[::-1], r[::-1]
+    keep = np.concatenate([[True], np.abs(np.diff(l)) >= np.finfo(float).tiny])
+    return l[keep], np.maximum(r[keep], 0.0)
+
+
+@pytest.mark.parametrize("resample", [0, 1])
+def test_host_bands_of_the_filter_library(resample):
+    """The test release's allfilters (159 curves as tabulated, the IR ones calibrated) through the Fortran host's
+    filters_build equal the oracle's resampling and normalisation with the calibration (division by
+    integral (lambda / lambda_ref)^expo T / lambda d lambda, opensps_filters.f90:507-519) restated here."""
+    g = grid(1)
+    fl = datalib.filter_library(ROOT, "allfilters")
+    with api.Context() as ctx:
+        ctx.set_ssp_basis(g)
+        ctx.set_filters(fl, resample=resample)
+        got = ctx.bands()
+    assert got.shape == (159, g.nspec)
+    ncal = 0
+    for b in range(159):
+        lc, rc = _clean(fl.lam[b], fl.resp[b])
+        ref, st = O.filter_band(lc, rc, g.lam, resample, fl.detector[b])
+        assert st == 0
+        lref, expo = fl.cal[b]
+        if lref > 0.0:
+            y = (g.lam / lref) ** expo * ref / (g.lam + O.TINY)
+            ref = ref / np.sum(0.5 * np.diff(g.lam) * (y[1:] + y[:-1]))
+            ncal += 1
+        np.testing.assert_allclose(got[b], ref, rtol=1e-12, atol=1e-14 * np.max(ref), err_msg=fl.names[b])
+    assert ncal >= 10

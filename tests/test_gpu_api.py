@@ -233,3 +233,26 @@ def test_api_sfr_output():
         mass = integrate.quad(sfr, 0.0, tage, epsabs=0, epsrel=1e-12, limit=200, points=pts)[0]
         want = sfr(tage) / mass / 1e9
         assert abs(r["sfr"][m] - want) <= 1e-9 * abs(want) + 1e-30, (cases[m], r["sfr"][m], want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_api_filter_library_batch(precision):
+    """The parametric batch through the 159 filters of the test release's allfilters (narrow and wide, calibrated IR
+    bands, supports from the UV to the far IR), against the oracle with the host's bands."""
+    import os
+    from libopensps_amd import datalib
+    root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "opensps-test-data")
+    g = grid(22)
+    fl = datalib.filter_library(root, "allfilters")
+    n = 64
+    prm = synth.parametric_params(n, seed=37)
+    print(f"case: api allfilters n={n} {precision}", flush=True)
+    with _ctx(g, synth.gaussian_filters(4), precision) as ctx:
+        ctx.set_filters(fl)
+        r = ctx.compute_csp_batch_params(prm, synth.varied_names(), want=("mags", "spec"))
+        bands = ctx.bands()
+    ref = O.csp_batch(g, bands, prm, add_igm=1, cosmo=O.cosmology_table())
+    ts, tm = TOL[precision]
+    assert spec_rel_err(r["spec"], ref["spec"]) < ts
+    assert mag_err(r["mags"], ref["mags"]) < tm
