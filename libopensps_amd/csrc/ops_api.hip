@@ -460,7 +460,9 @@ static void free_operator(OpsCtx* c) {
 
 // k_mm_bf16x3 over every row tile and at most maxct column tiles (grouped: the kernel reads the count on the device)
 static int launch_mm(OpsCtx* c, const MMArgs& mm, int maxct) {
-    const size_t lds = 8 * 512 * sizeof(bf16x8_t) + 2 * OPS_MM_COLS * sizeof(int);
+    // the operand buffers (64 KiB), overlaid in the epilogue by the C tile (128 columns of 132 floats), then the
+    // column tables
+    const size_t lds = (size_t)OPS_MM_COLS * OPS_MM_TS * sizeof(float) + 2 * OPS_MM_COLS * sizeof(int);
     const unsigned blocks = 8u * (unsigned)((mm.nrt + 7) / 8) * (unsigned)maxct;
     if (blocks == 0) return 0;
     HIPCHK(c, hipFuncSetAttribute((const void*)k_mm_bf16x3, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -73,6 +73,7 @@ struct GemmArgs {
 // columns, K contiguous; C by columns, rows contiguous (FP32).
 #define OPS_MM_ROWS 128
 #define OPS_MM_COLS 128
+#define OPS_MM_TS 132   // floats a column of k_mm_bf16x3's C tile takes in LDS in its epilogue (128 rows + 16 B skew)
 #define OPS_MM_GROUPED 0   // columns cc of a tile {slab, first, count}: model order[cc / parts], part cc % parts
 #define OPS_MM_DIRECT 1    // columns 0 .. ncols - 1 as they are
 

@@ -20,7 +20,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     extern __shared__ __align__(16) unsigned char mm_lds[];
     bf16x8_t* sA = (bf16x8_t*)mm_lds;        // [buffer][hi, lo][MM_PIECES]
     bf16x8_t* sB = sA + 4 * MM_PIECES;       // [buffer][hi, lo][MM_PIECES]
-    int* sColB = (int*)(sB + 4 * MM_PIECES);
+    float* sT = (float*)mm_lds;              // the epilogue's C tile [128 columns][OPS_MM_TS], over the operands
+    int* sColB = (int*)(mm_lds + OPS_MM_COLS * OPS_MM_TS * sizeof(float));
     int* sColC = sColB + OPS_MM_COLS;
 
     // blockIdx -> tiles: each XCD (blockIdx % 8) takes one eighth of the row tiles, row tiles fastest, so A's rows
@@ -132,16 +133,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         __syncthreads();
     }
 
-    // 16x16x32 result layout: register r of lane l holds row 4 (l / 16) + r, column l % 16
+    // The tile goes out through LDS (the loop's last barrier has freed the operand buffers): C is column-major, a
+    // column one model's part with its rows (pixels) contiguous, so each column's 128 rows leave as one 512-byte run
+    // by 32 lanes instead of 64-byte pieces of 16 columns an instruction. 16x16x32 result layout: register r of lane
+    // l holds row 4 (l / 16) + r, column l % 16.
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int cc = sColC[wc * 64 + j * 16 + (lane & 15)];
-        if (cc < 0) continue;
-        float* cp = a.C + (size_t)cc * a.ldc + 4 * (lane >> 4);
+    for (int j = 0; j < 4; j++)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int blk = rt * 8 + wr * 4 + i;
-            if (blk < a.mblk) *(f32x4_t*)(cp + blk * 16) = acc[i][j];
+            const int col = wc * 64 + j * 16 + (lane & 15), row = wr * 64 + i * 16 + 4 * (lane >> 4);
+            *(f32x4_t*)(sT + col * OPS_MM_TS + row) = acc[i][j];
         }
+    __syncthreads();
+    const int rows_valid = min(OPS_MM_ROWS, a.mblk * 16 - rt * OPS_MM_ROWS);   // whole 16-row blocks inside C
+#pragma unroll 4
+    for (int k = 0; k < 16; k++) {
+        const int q = tid + 256 * k, col = q >> 5, part = q & 31;
+        const int cc = sColC[col];
+        if (cc < 0 || part * 4 >= rows_valid) continue;
+        *(f32x4_t*)(a.C + (size_t)cc * a.ldc + rt * OPS_MM_ROWS + part * 4) =
+            *(const f32x4_t*)(sT + col * OPS_MM_TS + part * 4);
     }
 }
