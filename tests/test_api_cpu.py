@@ -197,3 +197,4 @@ def test_batch_grad_statuses_before_the_device():
             ctx.handle, n, 1, C.cast(names, C.c_void_p), C.cast(vp, C.c_void_p), w.ctypes.data, None, zm.ctypes.data,
             None, None, None, None, mags.ctypes.data, *([None] * 10), C.byref(st))
         assert st.value == 2
+
