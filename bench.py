@@ -328,8 +328,9 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build runs the batched CSP path on one GPU only; the multi-GPU layout (models "
-              "sharded across GPUs, no collective) is described in DESIGN.md. Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: the batched CSP path runs on one GPU only (on several, each GPU would take a contiguous "
+              "slice of the models with its own copy of the basis and filters; no collective). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
