@@ -96,6 +96,12 @@ int opensps_hip_set_grid(opensps_hip_ctx_t *ctx, int nspec, const double *lam, i
 int opensps_hip_set_bands(opensps_hip_ctx_t *ctx, int nbands, const double *bands);
 /* The computed bands (opensps_context_set_mag_compute, include/opensps.h:233): mask[b] != 0; NULL: all. */
 int opensps_hip_set_mag_mask(opensps_hip_ctx_t *ctx, int nbands, const int *mask);
+/* A fixed diffuse attenuation curve A_lambda / A_V on the grid (nspec, host; the grid's wavelengths): dust_type 2
+ * (Calzetti et al. 2000) or 6 (Reddy et al. 2015) as attenuation_table_row tabulates them
+ * (src/physics/opensps_csp_batch.f90:3525-3566, opensps_dust.f90:567-662); tau = dust2 curve, dust3 curve for the
+ * old stars' screen (pixel_transmissions :3765-3818). NULL: the power law (dust_type 0). Call after set_grid; the
+ * reverse pass with a curve is status 2. */
+int opensps_hip_set_dust_curve(opensps_hip_ctx_t *ctx, int nspec, const double *curve);
 /* The SFH table of sfh = 2, 3 (ctx%state%sfh_tab: opensps_context_set_sfh_tab, src/bindings/c/opensps_c_api.f90:
  * 2062-2089; load_tabular_sfh for sfh = 2): ntab forward ages in years, increasing, and SFRs (M_sun / yr), host;
  * 0 <= ntab <= 20000 (NTABMAX), else status 3. The weights of a tabular model (sfh_weights_core_pc,

@@ -147,7 +147,7 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
                   c->d_bc32, c->d_mask, c->d_cz, c->d_cdl, c->yo.p, c->wcol.p, c->w64.p, c->order.p, c->tiles.p,
                   c->misc.p, c->spec.p, c->flag.p, c->d_rdlam, c->d_ltab32, c->d_wl32, c->d_sm32, c->d_bkt, c->hdr.p, c->wsfh.p,
                   c->opu.p, c->opwc.p, c->opsum.p, c->opseg.p, c->opimf.p, c->opspart.p, c->d_bp_ptr, c->d_bp_band,
-                  c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p, c->gwb.p, c->d_magvega, c->d_tab};
+                  c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p, c->gwb.p, c->d_magvega, c->d_tab, c->d_dcurve, c->d_dcurve32};
     for (void* p : ps) free_ptr(p);
     free_operator(c);
     free_full(c);
@@ -191,6 +191,7 @@ int opensps_hip_set_grid(opensps_hip_ctx_t* c, int nspec, const double* lam, int
     c->nt = nt;
     c->nz = nz;
     c->nsp = ((nspec + OPS_TILE_PX - 1) / OPS_TILE_PX) * OPS_TILE_PX;
+    c->dcurve_on = 0;   // (a curve is per pixel of the grid)
     c->nbands = 0;   // the band tables index the old grid: set_bands again
     c->band_j0 = 0;
     c->band_j1 = -1;
@@ -412,6 +413,27 @@ int opensps_hip_set_mag_mask(opensps_hip_ctx_t* c, int nbands, const int* mask) 
     if (nbands != c->nbands) return ops_fail(c, 3, "set_mag_mask: %d bands, context has %d", nbands, c->nbands);
     for (int b = 0; b < nbands; b++) c->h_mask[b] = mask ? (mask[b] != 0) : 1;
     return upload_mask(c);
+}
+
+int opensps_hip_set_dust_curve(opensps_hip_ctx_t* c, int nspec, const double* curve) {
+    if (!c) return 1;
+    if (!curve) {
+        c->dcurve_on = 0;
+        return 0;
+    }
+    if (nspec != c->nspec || c->nsp < nspec)
+        return ops_fail(c, 3, "set_dust_curve: %d pixels, the grid has %d (set the grid first)", nspec, c->nspec);
+    std::vector<double> d(c->nsp, 0.0);
+    std::vector<float> f(c->nsp, 0.0f);
+    for (int i = 0; i < nspec; i++) {
+        d[i] = curve[i];
+        f[i] = (float)curve[i];
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    UPLOAD(c, c->d_dcurve, d.data(), d.size());
+    UPLOAD(c, c->d_dcurve32, f.data(), f.size());
+    c->dcurve_on = 1;
+    return 0;
 }
 
 int opensps_hip_set_sfh_tab(opensps_hip_ctx_t* c, int ntab, const double* age_yr, const double* sfr) {
@@ -1131,6 +1153,8 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ca.old = a->old;
     ca.mags = a->mags;
     ca.fband = a->fband;
+    ca.dcurve = c->dcurve_on ? c->d_dcurve : nullptr;
+    ca.dcurve32 = c->dcurve_on ? c->d_dcurve32 : nullptr;
     // One LDS spectrum buffer when the band quads fit the registers of the single-buffer chain, else two;
     // then the band sums (up to 2048 bands) for the magnitudes a thread per band
     const int chain_threads = 512;
@@ -1181,6 +1205,7 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     const opensps_hip_batch_t& f = g->fwd;
     const int n = f.n;
     if (f.use_ops) return ops_fail(c, 2, "batch_grad: the SSP operator's reverse pass (the IMF rows) is not built");
+    if (c->dcurve_on) return ops_fail(c, 2, "batch_grad: the reverse pass with a tabulated dust curve is not built");
     if (g->weights_z_bar && !f.ssp_weights_z) return ops_fail(c, 3, "batch_grad: weights_z_bar needs ssp_weights_z");
     if (n < 0) return ops_fail(c, 3, "batch_grad: n < 0");
     if (n == 0) return 0;

@@ -91,6 +91,10 @@ struct OpsCtx {
     // the SFH table of sfh 2, 3 (set_sfh_tab): ntab forward ages (yr), then ntab SFRs
     double* d_tab = nullptr;
     int ntab = 0;
+    // the diffuse attenuation curve of dust types 2, 6 per padded pixel (set_dust_curve), FP64 and FP32
+    double* d_dcurve = nullptr;
+    float* d_dcurve32 = nullptr;
+    int dcurve_on = 0;
     double vega_anchor = 0.0;
 
     // ---- cosmology ----

@@ -1446,7 +1446,8 @@ void k_chain(ChainArgs a) {
                 const float cf = (float)fob, cn = (float)fnd;
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    const float curve = (ed != 0.0) ? __expf((float)ed * L[u][e]) : 1.0f;
+                    const float curve = a.dcurve32 ? a.dcurve32[4 * q + e]
+                                                   : ((ed != 0.0) ? __expf((float)ed * L[u][e]) : 1.0f);
                     const float tr_d = __expf(-((float)dust2 * curve));
                     float tr_b = 1.0f, tr_o = 1.0f;
                     if (!no_birth) tr_b = __expf(-((float)dust1 * __expf((float)ed1 * L[u][e])));
@@ -1457,7 +1458,7 @@ void k_chain(ChainArgs a) {
             } else {
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    const double curve = (ed != 0.0) ? exp(ed * L[u][e]) : 1.0;
+                    const double curve = a.dcurve ? a.dcurve[4 * q + e] : ((ed != 0.0) ? exp(ed * L[u][e]) : 1.0);
                     const double tr_d = exp(-(dust2 * curve));
                     double tr_b = 1.0, tr_o = 1.0;
                     if (!no_birth) tr_b = exp(-(dust1 * exp(ed1 * L[u][e])));

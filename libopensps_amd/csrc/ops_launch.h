@@ -209,6 +209,9 @@ struct ChainArgs {
     void *spec, *young, *old;
     void* stage;   // n x nspec staging rows of wide smoothing windows without a spectrum out (single-buffer chain)
     double *mags, *fband;
+    // the diffuse attenuation curve per padded pixel (a fixed curve: dust types 2, 6) or NULL (the power law)
+    const double* dcurve;
+    const float* dcurve32;
 };
 
 // ---- the reverse pass (ops_grad.hip) ----

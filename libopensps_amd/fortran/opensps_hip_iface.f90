@@ -24,7 +24,7 @@ module opensps_hip_iface
     end type hip_grad_t
 
     public :: opensps_hip_create, opensps_hip_destroy, opensps_hip_set_grid, opensps_hip_set_bands, &
-              opensps_hip_set_mag_mask, opensps_hip_set_vega, opensps_hip_set_sfh_tab, opensps_hip_set_cosmology, opensps_hip_set_options, opensps_hip_csp_batch, &
+              opensps_hip_set_mag_mask, opensps_hip_set_vega, opensps_hip_set_sfh_tab, opensps_hip_set_dust_curve, opensps_hip_set_cosmology, opensps_hip_set_options, opensps_hip_csp_batch, &
               opensps_hip_sync, opensps_hip_malloc, opensps_hip_free, opensps_hip_memcpy_h2d, opensps_hip_memcpy_d2h, &
               opensps_hip_last_error, opensps_hip_device_count, opensps_hip_set_profiling, opensps_hip_kernel_times, &
               opensps_hip_csp_batch_grad
@@ -61,6 +61,12 @@ module opensps_hip_iface
             type(c_ptr), value :: ctx
             integer(c_int), value :: nbands
             integer(c_int), intent(in) :: mask(*)
+        end function
+
+        integer(c_int) function opensps_hip_set_dust_curve(ctx, nspec, curve) bind(C, name="opensps_hip_set_dust_curve")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, curve
+            integer(c_int), value :: nspec
         end function
 
         integer(c_int) function opensps_hip_set_sfh_tab(ctx, ntab, age_yr, sfr) bind(C, name="opensps_hip_set_sfh_tab")

@@ -210,9 +210,9 @@ contains
         else if (c%redshift_colors /= 0) then
             st = 2
             call set_error(2, 'batch: redshifted colours need zred_mags')
-        else if (c%dust_type /= 0) then
+        else if (c%dust_type /= 0 .and. c%dust_type /= 2 .and. c%dust_type /= 6) then
             st = 2
-            call set_error(2, 'batch: only the power-law attenuation curve (dust_type 0) is part of this build')
+            call set_error(2, 'batch: dust_type 0 (power law), 2 (Calzetti) and 6 (Reddy) are part of this build')
         end if
     end subroutine model_checks
 

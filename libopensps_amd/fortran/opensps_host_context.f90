@@ -49,7 +49,7 @@ module opensps_host_context
         ! device side
         type(c_ptr) :: dctx = c_null_ptr
         logical :: dev_grid = .false., dev_bands = .false., dev_cosmo = .false., dev_mask = .false.
-        logical :: dev_vega = .false., dev_tab = .false.
+        logical :: dev_vega = .false., dev_tab = .false., dev_dust = .false.
         type(dbuf_t) :: d_prm, d_w, d_mags, d_spec, d_mass, d_lbol, d_fband, d_young, d_old, d_idx
         integer :: last_n = 0, last_bytes = 8
         real(WP), allocatable :: last_maggies(:, :)
@@ -139,7 +139,7 @@ contains
         case ('add_agn_dust'); c%add_agn_dust = v
         case ('smooth_velocity'); c%smooth_velocity = v; c%dev_grid = .false.
         case ('smoothspec_fast'); c%smoothspec_fast = v; c%dev_grid = .false.
-        case ('dust_type'); c%dust_type = v
+        case ('dust_type'); c%dust_type = v; c%dev_dust = .false.
         case ('compute_vega_mags'); c%compute_vega_mags = v; c%dev_vega = .false.
         case ('redshift_colors'); c%redshift_colors = v
         case ('keep_maggies'); c%keep_maggies = v
@@ -261,6 +261,34 @@ contains
         end associate
     end subroutine ctx_set_basis
 
+    !> The fixed attenuation curves A_lambda / A_V at wavelength lam (Angstrom): dust_type 2, Calzetti et al. (2000)
+    !> eq. 4 extended beyond 0.12-2.2 um as the reference extends it (clipped at zero), and 6, Reddy et al. (2015)
+    !> (compute_attenuation_curve_point, src/physics/opensps_dust.f90:567-594, 630-662)
+    pure real(WP) function dust_curve_point(lam, dt) result(curve)
+        real(WP), intent(in) :: lam
+        integer, intent(in) :: dt
+        real(WP) :: x, k
+        x = 1.0e4_wp / lam
+        curve = 0.0_wp
+        if (dt == 2) then
+            if (lam > 6300.0_wp) then
+                k = 2.659_wp * (-1.857_wp + 1.040_wp * x) + 4.05_wp
+            else
+                k = 4.05_wp + 2.659_wp * (-2.156_wp + x * (1.509_wp + x * (-0.198_wp + x * 0.011_wp)))
+            end if
+            curve = max(k, 0.0_wp) / 4.05_wp
+        else if (dt == 6) then
+            k = 0.0_wp
+            if (lam < 6000.0_wp) then
+                x = min(x, 1.0e4_wp / 1500.0_wp)
+                k = -5.726_wp + x * (4.004_wp + x * (-0.525_wp + x * 0.029_wp)) + 2.505_wp
+            else if (lam < 28500.0_wp) then
+                k = -2.672_wp + x * (-0.010_wp + x * (1.532_wp + x * (-0.412_wp))) + 2.505_wp - 0.036221981_wp
+            end if
+            curve = k / 2.505_wp
+        end if
+    end function dust_curve_point
+
     subroutine ctx_refresh_cosmology(h)
         integer, intent(in) :: h
         associate(c => ctxs(h))
@@ -292,6 +320,8 @@ contains
         integer, intent(in) :: h
         integer(c_int) :: ndev, rc
         integer, allocatable, target :: mask(:)
+        real(WP), allocatable, target :: curve(:)
+        integer :: j
         st = 0
         associate(c => ctxs(h))
         if (.not. c_associated(c%dctx)) then
@@ -308,7 +338,7 @@ contains
                 return
             end if
             c%dev_grid = .false.; c%dev_bands = .false.; c%dev_cosmo = .false.; c%dev_mask = .false.
-            c%dev_prof = .false.; c%dev_vega = .false.; c%dev_tab = .false.
+            c%dev_prof = .false.; c%dev_vega = .false.; c%dev_tab = .false.; c%dev_dust = .false.
         end if
         if (.not. c%dev_grid) then
             rc = opensps_hip_set_grid(c%dctx, int(c%nspec, c_int), c%lam, int(c%nt, c_int), c%log_age, &
@@ -319,6 +349,7 @@ contains
             if (rc /= 0) goto 900
             c%dev_grid = .true.
             c%dev_bands = .false.
+            c%dev_dust = .false.
         end if
         if (.not. c%dev_bands) then
             if (c%nbands > 0) then
@@ -338,6 +369,19 @@ contains
             if (rc /= 0) goto 900
             c%dev_mask = .true.
             c%dev_vega = .false.
+        end if
+        if (.not. c%dev_dust) then
+            if (c%dust_type == 2 .or. c%dust_type == 6) then
+                allocate(curve(c%nspec))
+                do j = 1, c%nspec
+                    curve(j) = dust_curve_point(c%lam(j), c%dust_type)
+                end do
+                rc = opensps_hip_set_dust_curve(c%dctx, int(c%nspec, c_int), c_loc(curve))
+            else
+                rc = opensps_hip_set_dust_curve(c%dctx, 0_c_int, c_null_ptr)
+            end if
+            if (rc /= 0) goto 900
+            c%dev_dust = .true.
         end if
         if (.not. c%dev_tab) then
             if (c%ntab > 0) then

@@ -1223,6 +1223,9 @@ typedef struct {
     /* weights over ages and metallicities (ssp_weights_z: nt x nz x n) or NULL; they replace the SFH and the
      * metallicity blend (age_sums_w / model_sums_zw, opensps_csp_batch.f90:2086-2127, 2269-2324) */
     const double *ssp_weights_z;
+    /* a fixed diffuse attenuation curve A_lambda / A_V per pixel (dust_type 2, 6: attenuation_table_row's column 0,
+     * opensps_csp_batch.f90:3525-3566) or NULL: the power law */
+    const double *dcurve;
 } orc_ctx_t;
 
 /* Per-model birth-cloud age index: batch_model_indices (opensps_csp_batch.f90:1541-1576) */
@@ -1426,7 +1429,8 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
             int no_old = fabs(p[OPS_P_DUST3]) <= SAFE_FLOOR;
             for (int i = 0; i < nspec; i++) {
                 double curve = 1.0;
-                if (ed != 0.0) curve = curve * exp(ed * ltab[i]);
+                if (c->dcurve) curve = c->dcurve[i];
+                else if (ed != 0.0) curve = curve * exp(ed * ltab[i]);
                 double tau_d = p[OPS_P_DUST2] * curve, tr_d = exp(-tau_d);
                 double tr_b = 1.0, tr_o = 1.0;
                 if (!no_birth) tr_b = exp(-(p[OPS_P_DUST1] * exp(p[OPS_P_DUST1_INDEX] * ltab[i])));

@@ -217,6 +217,28 @@ def build_bands(fset, grid, mode=0):
     return out
 
 
+# ---- fixed attenuation curves (numpy restatement from the papers) ----
+def calzetti_curve(lam):
+    """A_lambda / A_V of Calzetti et al. (2000) eq. 4 (k' = 2.659, R_V = 4.05), the UV polynomial below 0.63 um
+    and the linear law above, both carried beyond 0.12-2.2 um, k clipped at 0 (opensps_dust.f90:567-594)."""
+    x = 1.0e4 / np.asarray(lam, float)
+    k = np.where(np.asarray(lam) > 6300.0, 2.659 * (-1.857 + 1.040 * x) + 4.05,
+                 4.05 + 2.659 * (-2.156 + 1.509 * x - 0.198 * x ** 2 + 0.011 * x ** 3))
+    return np.maximum(k, 0.0) / 4.05
+
+
+def reddy_curve(lam):
+    """A_lambda / A_V of Reddy et al. (2015) (R_V = 2.505): their UV law below 0.60 um (x clipped at 1/0.15 um), the
+    optical law to 2.85 um with the reference's continuity offset -0.036221981, zero beyond
+    (opensps_dust.f90:630-662)."""
+    lam = np.asarray(lam, float)
+    x = 1.0e4 / lam
+    xu = np.minimum(x, 1.0e4 / 1500.0)
+    uv = -5.726 + 4.004 * xu - 0.525 * xu ** 2 + 0.029 * xu ** 3 + 2.505
+    op = -2.672 - 0.010 * x + 1.532 * x ** 2 - 0.412 * x ** 3 + 2.505 - 0.036221981
+    return np.where(lam < 6000.0, uv, np.where(lam < 28500.0, op, 0.0)) / 2.505
+
+
 # ---- the reference spectra and the Vega system (numpy restatement) ----
 TINY = np.finfo(np.float64).tiny
 
@@ -337,7 +359,7 @@ class _Ctx(C.Structure):
                 ("add_igm", C.c_int), ("smooth_fast", C.c_int), ("tiny_logt", C.c_double),
                 ("ops", C.c_void_p), ("imf_type", C.c_int), ("remnants", C.c_int), ("salp_ind", C.c_double),
                 ("mdave", C.c_double), ("mlim_bh", C.c_double), ("mlim_ns", C.c_double), ("op_count", C.c_void_p),
-                ("ssp_weights_z", C.c_void_p)]
+                ("ssp_weights_z", C.c_void_p), ("dcurve", C.c_void_p)]
 
 
 def _context(grid, bands, add_igm, smooth_fast, mag_mask, cosmo, ptr):
@@ -402,7 +424,7 @@ def csp_batch_grad(grid, bands, prm, ssp_weights=None, ssp_weights_z=None, mags_
 
 def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_mask=None, cosmo=None,
               want=("mags", "spec", "mass", "lbol"), ops=None, imf_type=2, remnants=1, salp_ind=2.35, mdave=0.5,
-              mlim_bh=40.0, mlim_ns=8.5, ssp_weights_z=None):
+              mlim_bh=40.0, mlim_ns=8.5, ssp_weights_z=None, dust_curve=None):
     """Runs the oracle's batch. grid: synth.SSPGrid; bands (nbands, nspec); prm (NPAR, n); ops: a
     synth.SSPOperator (the IMF per model; the grid's bases are then unused); ssp_weights_z (n, nz, nt): weights
     over ages and metallicities.
@@ -420,6 +442,8 @@ def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_
     ctx = _Ctx()
     if ssp_weights_z is not None:
         ctx.ssp_weights_z = ptr(_d(ssp_weights_z))
+    if dust_curve is not None:
+        ctx.dcurve = ptr(_d(dust_curve))
     ctx.nspec, ctx.nt, ctx.nz, ctx.nbands = grid.nspec, grid.nt, grid.nz, nb
     ctx.lambda_ = ptr(_d(grid.lam))
     ctx.time_full = ptr(_d(grid.log_age))

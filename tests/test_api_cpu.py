@@ -198,3 +198,19 @@ def test_batch_grad_statuses_before_the_device():
             None, None, None, None, mags.ctypes.data, *([None] * 10), C.byref(st))
         assert st.value == 2
 
+
+def test_dust_type_statuses():
+    """dust_type 0, 2 (Calzetti) and 6 (Reddy) reach the device (status 5 here without one); the others are 2."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    g = grid(1)
+    with api.Context() as ctx:
+        ctx.set_ssp_basis(g)
+        ctx.set_filters(synth.gaussian_filters(4))
+        ctx.set_int("add_dust_emission", 0)
+        for dt, want in ((1, 2), (2, 5), (3, 2), (4, 2), (5, 2), (6, 5)):
+            ctx.set_int("dust_type", dt)
+            r = ctx.compute_csp_batch({"tage": np.full(2, 5.0), "sfh": np.full(2, 4.0)}, raw_status=True)
+            assert r["status"] == want, (dt, r["status"])
+

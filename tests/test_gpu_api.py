@@ -256,3 +256,38 @@ def test_api_filter_library_batch(precision):
     ts, tm = TOL[precision]
     assert spec_rel_err(r["spec"], ref["spec"]) < ts
     assert mag_err(r["mags"], ref["mags"]) < tm
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("dust_type", [2, 6])
+def test_api_fixed_dust_curves(precision, dust_type):
+    """dust_type 2 (Calzetti) and 6 (Reddy): the host's curve on the grid, the chain's diffuse and old-star screens,
+    against the oracle with the papers' curves; the reverse pass with such a curve is status 2."""
+    g = grid(22)
+    fs = synth.gaussian_filters(30)
+    n = 40
+    prm = synth.parametric_params(n, seed=41)
+    prm[SLOT["dust3"]] = np.linspace(0.0, 0.4, n)
+    names = synth.varied_names() + ["dust3"]
+    print(f"case: api dust_type {dust_type} {precision}", flush=True)
+    with _ctx(g, fs, precision) as ctx:
+        ctx.set_int("dust_type", dust_type)
+        r = ctx.compute_csp_batch_params(prm, names, want=("mags", "spec"))
+        bands = ctx.bands()
+        rg = ctx.compute_csp_batch_grad({k: prm[SLOT[k]] for k in names}, n=n, mags_bar=np.ones((n, 30)),
+                                        grads=("params_bar",), raw_status=True)
+        assert rg["status"] == 2
+    curve = O.calzetti_curve(g.lam) if dust_type == 2 else O.reddy_curve(g.lam)
+    ref = O.csp_batch(g, bands, prm, add_igm=1, cosmo=O.cosmology_table(), dust_curve=curve)
+    ts, tm = TOL[precision]
+    assert spec_rel_err(r["spec"], ref["spec"]) < ts
+    if precision == "double":
+        assert mag_err(r["mags"], ref["mags"]) < tm
+    else:   # FP32 spectra: bands within 40 mag of each model's brightest (fainter far-UV ones can underflow to 0)
+        q = ref["mags"]
+        bright = np.nanmin(np.where(np.isfinite(q), q, np.nan), axis=1, keepdims=True)
+        ok = np.isfinite(q) & (q - bright <= 40.0)
+        assert np.all(np.isfinite(r["mags"][ok]))
+        assert float(np.max(np.abs(r["mags"][ok] - q[ok]))) < tm
+

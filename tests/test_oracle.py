@@ -461,3 +461,19 @@ def test_filter_resampling_modes():
     hat, _ = O.filter_band(lc, rc, lam, mode=1)
     pt, _ = O.filter_band(lc, rc, lam, mode=0)
     np.testing.assert_allclose(hat[pt > 0.05 * pt.max()], pt[pt > 0.05 * pt.max()], rtol=2e-3)
+
+
+def test_fixed_attenuation_curves():
+    """The oracle's fixed curves against values worked from the papers: Calzetti et al. (2000) k(0.55 um) / 4.05 and
+    its two laws at 0.63 um (they differ by 0.7 %: the published law is not continuous there), zero beyond the linear
+    law's root (~3.1 um); Reddy et al. (2015) at 0.55 um, continuous at 0.60 um with the 0.036 offset, zero beyond
+    2.85 um (unclipped below it)."""
+    c = O.calzetti_curve(np.array([5500.0, 6299.999, 6300.001, 2.0e4, 5.0e4, 1500.0]))
+    k55 = 4.05 + 2.659 * (-2.156 + 1.509 / 0.55 - 0.198 / 0.55 ** 2 + 0.011 / 0.55 ** 3)
+    assert abs(c[0] - k55 / 4.05) < 1e-14 and abs(c[0] - 0.9995) < 1e-3
+    assert abs(c[1] * 4.05 - 3.4764) < 1e-3 and abs(c[2] * 4.05 - 3.5018) < 1e-3
+    assert c[3] > 0.0 and c[4] == 0.0 and c[5] > 2.0
+    r = O.reddy_curve(np.array([5500.0, 5999.9999, 6000.0, 28499.0, 28500.0]))
+    assert abs(r[0] * 2.505 - (-5.726 + 4.004 / 0.55 - 0.525 / 0.55 ** 2 + 0.029 / 0.55 ** 3 + 2.505)) < 1e-13
+    assert abs(r[1] - r[2]) < 1e-6 and r[4] == 0.0
+    assert -0.02 < r[3] < 0.0   # (the optical law dips just below zero before 2.85 um; unclipped, as the reference)
