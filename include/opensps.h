@@ -108,7 +108,8 @@ void opensps_context_get_bands(int handle, double *bands, int *status);
 
 /* ---- the batch (reference include/opensps.h:264-316) ---- */
 /* status 0, 1 bad handle / no basis, 2 unsupported (a parameter that changes the SSPs, spectral indices, dust
- * emission, nebular emission, Vega magnitudes without reference spectra, an SFH type other than 0 to 5), 3 unknown
+ * emission, nebular emission, Vega magnitudes without reference spectra, dust_type other than 0, 2, 6, an SFH type
+ * other than 0 to 5), 3 unknown
  * parameter or bad shape, 5 no HIP device, 10 device error. */
 void opensps_context_compute_csp_batch(int handle, int n, int n_params, const char *const *names,
                                        const double *const *values, const double *ssp_weights, double *mags,
