@@ -58,7 +58,8 @@ void opensps_context_set_ssp_basis(int handle, int nspec, const double *lambda, 
 /* Parameter setters (reference include/opensps.h:145-147). Besides the reference's keys, set_str
  * "kernel_precision" = "double" (default: FP64 age sums and chain) or "mixed" (split-BF16 age sums on the matrix
  * cores, FP32 chain), and set_int "mixed_precision" = 0 / 1 select the kernels at run time (the reference selects
- * them at build time, meson option kernel_precision). */
+ * them at build time, meson option kernel_precision). The SSP operator's IMF follows set_int "imf_type",
+ * "add_stellar_remnants" and set_float "mdave"; evaluate_ssp_op's metallicity is set_int "zmet". */
 void opensps_context_set_int(int handle, const char *key, int value, int *status);
 void opensps_context_set_float(int handle, const char *key, double value, int *status);
 void opensps_context_set_str(int handle, const char *key, const char *value, int *status);
@@ -107,10 +108,14 @@ void opensps_context_set_sfh_tab(int handle, int ntab, const double *age, const 
 void opensps_context_get_bands(int handle, double *bands, int *status);
 
 /* ---- the batch (reference include/opensps.h:264-316) ---- */
-/* status 0, 1 bad handle / no basis, 2 unsupported (a parameter that changes the SSPs, spectral indices, dust
- * emission, nebular emission, Vega magnitudes without reference spectra, dust_type other than 0, 2, 6, an SFH type
- * other than 0 to 5), 3 unknown
- * parameter or bad shape, 5 no HIP device, 10 device error. */
+/* status 0, 1 bad handle / no basis, 2 unsupported (a parameter that changes the SSPs; imf1, imf2, imf3,
+ * imf_lower_limit, imf_upper_limit without an SSP operator on the context, and vdmc, mdave per model; spectral
+ * indices, dust emission, nebular emission, AGN emission (fagn > 0), Vega magnitudes without reference spectra,
+ * dust_type other than 0, 2, 6, an SFH type other than 0 to 5), 3 unknown parameter or bad shape, 5 no HIP device,
+ * 10 device error. Members of the reference's parameter set that no stage here reads (pmetals, dust_clumps, uvb,
+ * mwr, gas_*, duste_*, agn_tau, fagn = 0) are accepted. With an SSP operator on the context
+ * (opensps_context_set_ssp_ops) every batch takes each model's IMF from it, as the reference's does
+ * (src/bindings/c/opensps_c_api.f90:913-917). */
 void opensps_context_compute_csp_batch(int handle, int n, int n_params, const char *const *names,
                                        const double *const *values, const double *ssp_weights, double *mags,
                                        double *spec, double *mass, double *lbol, double *mdust, double *sfr,
@@ -149,6 +154,73 @@ void opensps_context_compute_csp_batch_grad_rows(int handle, int n, int n_params
 void opensps_context_get_batch_maggies(int handle, double *maggies, int *n_bands, int *n, int *status);
 /* *bytes: 4 with kernel_precision "mixed", 8 with "double". */
 void opensps_context_batch_spec_device(int handle, void **ptr, int *nspec, int *n, int *bytes, int *status);
+
+/* The struct form of the batch with gradients (reference include/opensps.h:366-421, header-only there too): the
+ * options and the result buffers of opensps_context_compute_csp_batch_grad_rows in two structs; params_bar_rows 0
+ * means OPENSPS_NBGRAD. NULL opts, res or status: status 1 (when status is not NULL). */
+typedef struct {
+    int n;                          /* models */
+    int n_params;                   /* entries of names / values */
+    const char *const *names;       /* n_params parameter names */
+    const double *const *values;    /* n_params arrays of n values */
+    const double *ssp_weights;      /* ntfull x n, or NULL */
+    const double *ssp_weights_z;    /* ntfull x nz x n, or NULL */
+    const double *zred_mags;        /* n, or NULL */
+    const double *mags_bar;         /* nbands x n, or NULL: the cotangents */
+    const double *spec_bar;         /* nspec x n, or NULL */
+    const double *mass_bar;         /* n, or NULL */
+    const double *lbol_bar;         /* n, or NULL */
+    int params_bar_rows;            /* leading dimension of params_bar; 0: OPENSPS_NBGRAD */
+} opensps_batch_options_t;
+
+typedef struct {
+    double *mags, *spec, *mass, *lbol, *mdust, *sfr, *emlines, *indx;   /* forward outputs, any NULL but mags */
+    double *params_bar;             /* rows x n */
+    double *weights_bar;            /* ntfull x n */
+    double *weights_z_bar;          /* ntfull x nz x n */
+} opensps_batch_results_t;
+
+static inline void opensps_context_compute_csp_batch_ext(int handle, const opensps_batch_options_t *opts,
+                                                         opensps_batch_results_t *res, int *status) {
+    int rows;
+    if (opts == 0 || res == 0 || status == 0) {
+        if (status != 0) *status = 1;
+        return;
+    }
+    rows = opts->params_bar_rows > 0 ? opts->params_bar_rows : OPENSPS_NBGRAD;
+    opensps_context_compute_csp_batch_grad_rows(handle, opts->n, opts->n_params, opts->names, opts->values,
+                                                opts->ssp_weights, opts->ssp_weights_z, opts->zred_mags,
+                                                opts->mags_bar, opts->spec_bar, opts->mass_bar, opts->lbol_bar,
+                                                res->mags, res->spec, res->mass, res->lbol, res->mdust, res->sfr,
+                                                res->emlines, res->indx, res->params_bar, rows, res->weights_bar,
+                                                res->weights_z_bar, status);
+}
+
+/* ---- the SSP operator: the IMF per model (reference include/opensps.h:425-455) ---- */
+/* Building the operator from a data release's isochrones and libraries is not part of this build: status 301 (the
+ * reference's status for a context that is not set up). */
+void opensps_context_build_ssp_ops(int handle, int all_metallicities, int *status);
+void opensps_context_build_ssp_ops_free_mods(int handle, int all_metallicities, int *status);
+/* No reference counterpart (the reference builds its operator set from isochrones, ssp_operator_build /
+ * ssp_opset_build): an operator given by the caller, installed on the GPU. Every metallicity and age of the basis:
+ * points of (iz, t) are [age_ptr[iz ntfull + t], age_ptr[iz ntfull + t + 1]), sorted by initial mass mini within
+ * an age; m1, m2 their mass bins (first = 1: an age's first bin, whose lower edge is the IMF's lower bound); cm,
+ * cl, q surviving mass, L_bol and ionizing photons per star; age_mmax (ntfull x nz) the largest living initial
+ * mass of each isochrone; nent entries per point: ent_tpl (template, -1 none) and ent_val; etpl (nspec x ntpl) the
+ * template library on the basis' wavelengths. status 1 bad handle / no basis, 3 bad sizes or tables, 5 no HIP
+ * device, 10 device error. A new basis drops the operator. */
+void opensps_context_set_ssp_ops(int handle, int nz, int ntfull, int npts, int ntpl, int nent, const int *age_ptr,
+                                 const double *mini, const double *m1, const double *m2, const int *first,
+                                 const double *cm, const double *cl, const double *q, const double *age_mmax,
+                                 const int *ent_tpl, const double *ent_val, const double *etpl, int *status);
+void opensps_context_free_ssp_ops(int handle, int *status);                   /* include/opensps.h:443 */
+void opensps_context_has_ssp_ops(int handle, int *has_ops, int *status);      /* include/opensps.h:446 */
+/* include/opensps.h:451-455: the SSP of every age for one IMF (imf_type < 0: the context's; imf_params [imf1, imf2,
+ * imf3, vdmc, mdave] or NULL: the context's) at the context's metallicity zmet, in FP64: mass_grid (ntfull)
+ * surviving mass, lbol_grid (ntfull) log10 L_bol, spec_grid (nspec x ntfull). status 303 without an operator, 3 NULL
+ * outputs, 2 an IMF type other than 0, 2, 4 or (Dave's, type 4) an mdave other than the context's. */
+void opensps_context_evaluate_ssp_op(int handle, int imf_type, const double *imf_params, double *mass_grid,
+                                     double *lbol_grid, double *spec_grid, int *status);
 
 /* No reference counterpart: the batch on device-resident arrays for callers that keep their data on the GPU
  * (the reference's batch_spec_device serves them on the output side). d_prm: device OPS_NPAR x n doubles in the

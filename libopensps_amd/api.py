@@ -68,6 +68,12 @@ def lib():
         L.opensps_get_nbgrad.argtypes = [ip]
         L.opensps_get_grad_param_name.argtypes = [i, C.c_char_p, i, ip]
         L.opensps_get_version_string.restype = C.c_char_p
+        L.opensps_context_build_ssp_ops.argtypes = [i, i, ip]
+        L.opensps_context_build_ssp_ops_free_mods.argtypes = [i, i, ip]
+        L.opensps_context_set_ssp_ops.argtypes = [i, i, i, i, i, i] + [vp] * 12 + [ip]
+        L.opensps_context_free_ssp_ops.argtypes = [i, ip]
+        L.opensps_context_has_ssp_ops.argtypes = [i, ip, ip]
+        L.opensps_context_evaluate_ssp_op.argtypes = [i, i, vp, vp, vp, vp, ip]
         _lib = L
     return _lib
 
@@ -242,6 +248,47 @@ class Context:
                                               C.byref(st))
         _check(st)
         return out[0], out[1], out[2]
+
+    # ---- the SSP operator (the IMF per model) ----
+    def build_ssp_ops(self, all_metallicities=1):
+        """opensps_context_build_ssp_ops: building from a data release is not part of this build (status 301)."""
+        st = C.c_int()
+        lib().opensps_context_build_ssp_ops(self.handle, int(all_metallicities), C.byref(st))
+        _check(st)
+
+    def set_ssp_ops(self, op):
+        """Installs a caller-built operator (a synth.SSPOperator or anything with its fields) on the GPU."""
+        ints = {k: np.ascontiguousarray(getattr(op, k), dtype=np.int32) for k in ("age_ptr", "first", "ent_tpl")}
+        dbl = {k: _d(getattr(op, k)) for k in ("mini", "m1", "m2", "cm", "cl", "q", "age_mmax", "ent_val", "etpl")}
+        P = lambda a: a.ctypes.data
+        st = C.c_int()
+        lib().opensps_context_set_ssp_ops(self.handle, op.nz, op.nt, op.npts, op.ntpl, op.nent, P(ints["age_ptr"]),
+                                          P(dbl["mini"]), P(dbl["m1"]), P(dbl["m2"]), P(ints["first"]), P(dbl["cm"]),
+                                          P(dbl["cl"]), P(dbl["q"]), P(dbl["age_mmax"]), P(ints["ent_tpl"]),
+                                          P(dbl["ent_val"]), P(dbl["etpl"]), C.byref(st))
+        _check(st)
+
+    def free_ssp_ops(self):
+        st = C.c_int()
+        lib().opensps_context_free_ssp_ops(self.handle, C.byref(st))
+        _check(st)
+
+    def has_ssp_ops(self) -> bool:
+        h, st = C.c_int(), C.c_int()
+        lib().opensps_context_has_ssp_ops(self.handle, C.byref(h), C.byref(st))
+        _check(st)
+        return h.value == 1
+
+    def evaluate_ssp_op(self, imf_type=-1, imf_params=None):
+        """opensps_context_evaluate_ssp_op: (mass (nt,), log10 lbol (nt,), spec (nt, nspec)) of every age for one
+        IMF; imf_params = (imf1, imf2, imf3, vdmc, mdave) or None for the context's."""
+        mass, lbol, spec = np.zeros(self.nt), np.zeros(self.nt), np.zeros((self.nt, self.nspec))
+        ip = None if imf_params is None else _d(imf_params)
+        st = C.c_int()
+        lib().opensps_context_evaluate_ssp_op(self.handle, int(imf_type), None if ip is None else ip.ctypes.data,
+                                              mass.ctypes.data, lbol.ctypes.data, spec.ctypes.data, C.byref(st))
+        _check(st)
+        return mass, lbol, spec
 
     # ---- batches ----
     def compute_csp_batch(self, values: dict, n=None, ssp_weights=None, want=("mags", "spec", "mass", "lbol"),

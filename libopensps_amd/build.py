@@ -68,6 +68,21 @@ def build_fortran() -> str:
     return API_SO
 
 
+SHIM_SO = os.path.join(ROOT, "tests", "c", "libbatch_ext_shim.so")
+
+
+def build_test_shims() -> str:
+    """The tests' C shim over the header-only struct entry point (tests/c/batch_ext_shim.c): test infrastructure,
+    compiled against include/opensps.h and linked to libopensps.so."""
+    src = os.path.join(ROOT, "tests", "c", "batch_ext_shim.c")
+    if not os.path.exists(src):
+        return ""
+    rel = os.path.relpath(LIBDIR, os.path.dirname(SHIM_SO))
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"), src, "-o",
+          SHIM_SO, "-L" + LIBDIR, "-lopensps", "-Wl,-rpath,$ORIGIN/" + rel])
+    return SHIM_SO
+
+
 def build_all() -> None:
     build_hip()
     build_fortran()

@@ -9,6 +9,7 @@ module opensps_c_api
     use opensps_host_context
     use opensps_host_filters
     use opensps_host_batch
+    use opensps_hip_iface, only: hip_ssp_ops_t, opensps_hip_set_ssp_ops, opensps_hip_free_ssp_ops
     implicit none
     private
 
@@ -734,6 +735,110 @@ contains
                 o = c%last_maggies
             end if
         end associate
+    end subroutine
+
+    ! ---- the SSP operator (reference include/opensps.h:425-455) --------------------------------------------------
+    !> opensps_context_build_ssp_ops (src/bindings/c/opensps_c_api.f90:1172-1223) builds the operator from the
+    !> isochrones and spectral libraries of a data release set-up, which is not part of this build: status 301, as
+    !> the reference returns for a context that is not set up. The operator is given by opensps_context_set_ssp_ops.
+    subroutine opensps_context_build_ssp_ops(handle, all_metallicities, status) bind(C, name="opensps_context_build_ssp_ops")
+        integer(c_int), value :: handle, all_metallicities
+        integer(c_int), intent(out) :: status
+        if (bad_handle(handle, status)) return
+        status = 301
+        call set_error(301, 'opensps_context_build_ssp_ops: context not set up from a data release; give the ' // &
+                       'operator with opensps_context_set_ssp_ops')
+    end subroutine
+
+    !> opensps_context_build_ssp_ops_free_mods (src/bindings/c/opensps_c_api.f90:1229-1262): status 301 (as above)
+    subroutine opensps_context_build_ssp_ops_free_mods(handle, all_metallicities, status) &
+        bind(C, name="opensps_context_build_ssp_ops_free_mods")
+        integer(c_int), value :: handle, all_metallicities
+        integer(c_int), intent(out) :: status
+        if (bad_handle(handle, status)) return
+        status = 301
+        call set_error(301, 'opensps_context_build_ssp_ops_free_mods: context not set up from a data release')
+    end subroutine
+
+    !> No reference counterpart (the reference builds the operator set from isochrones, ssp_operator_build and
+    !> ssp_opset_build, src/physics/opensps_ssp_operator.f90, opensps_ssp_batch.f90:99-147): installs an operator
+    !> given by the caller on the GPU. With it every batch takes each model's IMF (imf1, imf2, imf3,
+    !> imf_lower_limit, imf_upper_limit may vary per model). Arrays as opensps_hip_ssp_ops_t (include/opensps_hip.h).
+    !> status 1 bad handle or no basis, 3 bad sizes or tables, 5 no HIP device, 10 device error.
+    subroutine opensps_context_set_ssp_ops(handle, nz, ntfull, npts, ntpl, nent, age_ptr, mini, m1, m2, first, cm, &
+                                           cl, q, age_mmax, ent_tpl, ent_val, etpl, status) &
+        bind(C, name="opensps_context_set_ssp_ops")
+        integer(c_int), value :: handle, nz, ntfull, npts, ntpl, nent
+        type(c_ptr), value :: age_ptr, mini, m1, m2, first, cm, cl, q, age_mmax, ent_tpl, ent_val, etpl
+        integer(c_int), intent(out) :: status
+        type(hip_ssp_ops_t) :: o
+        integer(c_int) :: rc
+        integer :: st
+        if (bad_handle(handle, status)) return
+        associate(c => ctxs(handle))
+            if (.not. c%has_basis) then
+                status = 1
+                call set_error(1, 'set_ssp_ops: no SSP basis installed')
+                return
+            end if
+            if (nz /= c%nz .or. ntfull /= c%nt) then
+                status = 3
+                call set_error(3, 'set_ssp_ops: the operator''s metallicities and ages must be the basis''s')
+                return
+            end if
+            st = ctx_device(int(handle))
+            if (st /= 0) then
+                status = st
+                return
+            end if
+            o%nz = nz; o%nt = ntfull; o%npts = npts; o%ntpl = ntpl; o%nent = nent
+            o%age_ptr = age_ptr; o%mini = mini; o%m1 = m1; o%m2 = m2; o%first = first
+            o%cm = cm; o%cl = cl; o%q = q; o%age_mmax = age_mmax
+            o%ent_tpl = ent_tpl; o%ent_val = ent_val; o%etpl = etpl
+            rc = opensps_hip_set_ssp_ops(c%dctx, o)
+            if (rc /= 0) then
+                status = merge(int(rc), 10, rc == 1 .or. rc == 2 .or. rc == 3)
+                call set_error(int(status), 'set_ssp_ops: ' // trim(hip_message(c%dctx)))
+                c%has_ops = .false.
+                return
+            end if
+            c%has_ops = .true.
+            c%dev_imf = .false.
+        end associate
+    end subroutine
+
+    !> opensps_context_free_ssp_ops (reference include/opensps.h:443; src/bindings/c/opensps_c_api.f90:1266-1278)
+    subroutine opensps_context_free_ssp_ops(handle, status) bind(C, name="opensps_context_free_ssp_ops")
+        integer(c_int), value :: handle
+        integer(c_int), intent(out) :: status
+        integer(c_int) :: rc
+        if (bad_handle(handle, status)) return
+        associate(c => ctxs(handle))
+            if (c%has_ops .and. c_associated(c%dctx)) rc = opensps_hip_free_ssp_ops(c%dctx)
+            c%has_ops = .false.
+        end associate
+    end subroutine
+
+    !> opensps_context_has_ssp_ops (reference include/opensps.h:446; src/bindings/c/opensps_c_api.f90:1281-1293)
+    subroutine opensps_context_has_ssp_ops(handle, has_ops, status) bind(C, name="opensps_context_has_ssp_ops")
+        integer(c_int), value :: handle
+        integer(c_int), intent(out) :: has_ops, status
+        has_ops = 0
+        if (bad_handle(handle, status)) return
+        if (ctxs(handle)%has_ops) has_ops = 1
+    end subroutine
+
+    !> opensps_context_evaluate_ssp_op (reference include/opensps.h:451-455): the SSPs of every age for one IMF
+    !> (opensps_host_batch's ssp_op_evaluate)
+    subroutine opensps_context_evaluate_ssp_op(handle, imf_type, imf_params, mass_grid, lbol_grid, spec_grid, status) &
+        bind(C, name="opensps_context_evaluate_ssp_op")
+        integer(c_int), value :: handle, imf_type
+        type(c_ptr), value :: imf_params, mass_grid, lbol_grid, spec_grid
+        integer(c_int), intent(out) :: status
+        integer :: st
+        if (bad_handle(handle, status)) return
+        call ssp_op_evaluate(int(handle), int(imf_type), imf_params, mass_grid, lbol_grid, spec_grid, st)
+        status = st
     end subroutine
 
 end module opensps_c_api

@@ -23,6 +23,15 @@ module opensps_hip_iface
         type(c_ptr) :: params_bar = c_null_ptr, weights_bar = c_null_ptr, weights_z_bar = c_null_ptr
     end type hip_grad_t
 
+    !> opensps_hip_ssp_ops_t
+    type, bind(C), public :: hip_ssp_ops_t
+        integer(c_int) :: nz = 0, nt = 0, npts = 0, ntpl = 0, nent = 0
+        type(c_ptr) :: age_ptr = c_null_ptr, mini = c_null_ptr, m1 = c_null_ptr, m2 = c_null_ptr, first = c_null_ptr
+        type(c_ptr) :: cm = c_null_ptr, cl = c_null_ptr, q = c_null_ptr, age_mmax = c_null_ptr
+        type(c_ptr) :: ent_tpl = c_null_ptr, ent_val = c_null_ptr, etpl = c_null_ptr
+    end type hip_ssp_ops_t
+
+    public :: opensps_hip_set_ssp_ops, opensps_hip_free_ssp_ops, opensps_hip_set_imf
     public :: opensps_hip_create, opensps_hip_destroy, opensps_hip_set_grid, opensps_hip_set_bands, &
               opensps_hip_set_mag_mask, opensps_hip_set_vega, opensps_hip_set_sfh_tab, opensps_hip_set_dust_curve, opensps_hip_set_cosmology, opensps_hip_set_options, opensps_hip_csp_batch, &
               opensps_hip_sync, opensps_hip_malloc, opensps_hip_free, opensps_hip_memcpy_h2d, opensps_hip_memcpy_d2h, &
@@ -155,6 +164,25 @@ module opensps_hip_iface
             real(c_double), intent(out) :: ms(4)
             integer(c_int), intent(out) :: launches(4)
             integer(c_int), value :: reset
+        end function
+
+        integer(c_int) function opensps_hip_set_ssp_ops(ctx, ops) bind(C, name="opensps_hip_set_ssp_ops")
+            import :: c_int, c_ptr, hip_ssp_ops_t
+            type(c_ptr), value :: ctx
+            type(hip_ssp_ops_t), intent(in) :: ops
+        end function
+
+        integer(c_int) function opensps_hip_free_ssp_ops(ctx) bind(C, name="opensps_hip_free_ssp_ops")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function opensps_hip_set_imf(ctx, imf_type, remnants, salp_ind, mdave, mlim_bh, mlim_ns) &
+            bind(C, name="opensps_hip_set_imf")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: imf_type, remnants
+            real(c_double), value :: salp_ind, mdave, mlim_bh, mlim_ns
         end function
 
         integer(c_int) function opensps_hip_device_count(n) bind(C, name="opensps_hip_device_count")

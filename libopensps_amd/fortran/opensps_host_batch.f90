@@ -9,7 +9,7 @@ module opensps_host_batch
     use opensps_hip_iface
     implicit none
     private
-    public :: batch_run, batch_run_device, cstring, resolve_params, model_checks
+    public :: batch_run, batch_run_device, cstring, resolve_params, model_checks, ssp_op_evaluate
 
 contains
 
@@ -192,13 +192,21 @@ contains
 
     !> Per-model checks of stage 0 that this build adds to the reference's: options whose stages it does
     !> not compute. status 2 with a message, else 0.
-    subroutine model_checks(c, prm, n, st)
+    subroutine model_checks(c, prm, n, st, fagn)
         type(host_ctx_t), intent(in) :: c
         real(WP), intent(in) :: prm(n, NPAR)
         integer, intent(in) :: n
         integer, intent(out) :: st
+        real(WP), intent(in) :: fagn(:)
         st = 0
-        if (c%add_dust_emission == 1 .and. any(prm(:, P_DUST1) > SAFE_FLOOR .or. prm(:, P_DUST2) > SAFE_FLOOR)) then
+        if (c%add_agn_dust == 1 .and. any(fagn > SAFE_FLOOR)) then
+            ! the AGN stage runs for fagn > 0 (opensps_csp_batch.f90:1582-1588)
+            st = 2
+            call set_error(2, 'batch: AGN emission (fagn > 0 with add_agn_dust = 1) is not part of this build')
+        else if (c%has_ops .and. c%imf_type /= 0 .and. c%imf_type /= 2 .and. c%imf_type /= 4) then
+            st = 2
+            call set_error(2, 'batch: the SSP operator integrates imf_type 0 (Salpeter), 2 (Kroupa) and 4 (Dave)')
+        else if (c%add_dust_emission == 1 .and. any(prm(:, P_DUST1) > SAFE_FLOOR .or. prm(:, P_DUST2) > SAFE_FLOOR)) then
             st = 2
             call set_error(2, 'batch: dust emission (add_dust_emission = 1 with dust) is not part of this build')
         else if (c%add_neb_emission == 1) then
@@ -217,23 +225,30 @@ contains
     end subroutine model_checks
 
     !> names[] / values[] -> prm(n, NPAR): the context's values, overwritten per model by the named parameters
-    !> (batch_handle's name resolution, src/bindings/c/opensps_c_api.f90:797-853); status 2 for a parameter that
-    !> changes the SSPs, 3 for an unknown name or NULL values.
-    subroutine resolve_params(c, n, n_params, c_names, c_values, prm, status)
+    !> (batch_handle's name resolution, src/bindings/c/opensps_c_api.f90:797-853). Status 2 for a parameter that
+    !> changes the SSPs, and for the IMF's without an SSP operator on the context (:710-714, 770-780); 3 for an
+    !> unknown name or NULL values. Members no stage here reads are accepted; fagn > 0 (the AGN stage) is refused in
+    !> model_checks.
+    subroutine resolve_params(c, n, n_params, c_names, c_values, prm, status, fagn)
         type(host_ctx_t), intent(in) :: c
         integer, intent(in) :: n, n_params
         type(c_ptr), intent(in) :: c_names, c_values
         real(WP), allocatable, intent(out) :: prm(:, :)
         integer, intent(out) :: status
+        real(WP), allocatable, intent(out), optional :: fagn(:)
         type(c_ptr), pointer :: np(:), vp(:)
         real(c_double), pointer :: v(:)
         character(len=:), allocatable :: nm
-        integer :: i, k
+        integer :: i, k, ki
         status = 0
         allocate(prm(n, NPAR))
         do k = 1, NPAR
             prm(:, k) = c%pset(k)
         end do
+        if (present(fagn)) then
+            allocate(fagn(n))
+            fagn = c%inert(I_FAGN)
+        end if
         if (n_params <= 0) return
         if (.not. c_associated(c_names) .or. .not. c_associated(c_values)) then
             status = 3
@@ -249,8 +264,28 @@ contains
                 call set_error(2, 'batch: ' // nm // ' changes the SSPs and cannot vary in a batch')
                 return
             end if
+            if (any(IMF_NAMES == nm)) then
+                if (.not. c%has_ops) then
+                    status = 2
+                    call set_error(2, 'batch: ' // nm // ' changes the SSPs; the IMF varies per model only with an ' // &
+                                   'SSP operator on the context (opensps_context_set_ssp_ops)')
+                    return
+                end if
+                if (nm == 'vdmc' .or. nm == 'mdave') then
+                    status = 2
+                    call set_error(2, 'batch: ' // nm // ' per model is not part of this build (imf1, imf2, imf3 ' // &
+                                   'and the IMF limits are)')
+                    return
+                end if
+            end if
+            if (nm == 'afe') then
+                status = 2
+                call set_error(2, 'batch: afe needs the [alpha/Fe] bases of opensps_context_build_afe_basis')
+                return
+            end if
             k = param_slot(nm)
-            if (k == 0) then
+            ki = inert_slot(nm)
+            if (k == 0 .and. ki == 0) then
                 status = 3
                 call set_error(3, 'batch: unknown parameter ' // nm)
                 return
@@ -261,7 +296,11 @@ contains
                 return
             end if
             call c_f_pointer(vp(i), v, [n])
-            prm(:, k) = v(1:n)
+            if (k > 0) then
+                prm(:, k) = v(1:n)
+            else if (ki == I_FAGN .and. present(fagn)) then
+                fagn = v(1:n)
+            end if
         end do
     end subroutine resolve_params
 
@@ -274,7 +313,7 @@ contains
         integer, intent(out) :: status
         type(c_ptr), pointer :: np(:), vp(:)
         real(c_double), pointer :: v(:), outv(:), outm(:, :), w_in(:, :)
-        real(WP), allocatable, target :: prm(:, :), wts(:, :), hbuf(:, :), mblend(:)
+        real(WP), allocatable, target :: prm(:, :), wts(:, :), hbuf(:, :), mblend(:), fagn(:)
         real(c_float), allocatable, target :: fbuf(:, :)
         integer(c_int), allocatable, target :: idx(:, :)
         integer :: i, k, m, st, nb, ns, nt, bytes, it
@@ -317,9 +356,9 @@ contains
         ns = c%nspec
         nt = c%nt
         ! names -> slots; the values of the named parameters per model over the context's
-        call resolve_params(c, n, n_params, c_names, c_values, prm, status)
+        call resolve_params(c, n, n_params, c_names, c_values, prm, status, fagn)
         if (status /= 0) return
-        call model_checks(c, prm, n, st)
+        call model_checks(c, prm, n, st, fagn)
         if (st /= 0) then
             status = st
             return
@@ -329,6 +368,11 @@ contains
                                abs(prm(:, P_DUST3)) > SAFE_FLOOR)
         want_w = c_associated(c_mass_young) .or. c_associated(c_mass_old) .or. c_associated(c_lbol_young) .or. &
                  c_associated(c_lbol_old)
+        if (want_w .and. c%has_ops) then
+            status = 2
+            call set_error(2, 'batch: the young and old masses and L_bol with the SSP operator are not part of this build')
+            return
+        end if
         st = ctx_device(h)
         if (st /= 0) then
             status = st
@@ -363,6 +407,7 @@ contains
         a%n = n
         a%precision = merge(HIP_MIXED, HIP_FP64, c%mixed == 1)
         a%split_parts = merge(1, 0, split)
+        a%use_ops = merge(1, 0, c%has_ops)
         a%prm = c%d_prm%p
         a%ssp_weights = c_null_ptr
         if (c_associated(c_weights)) a%ssp_weights = c%d_w%p
@@ -566,6 +611,7 @@ contains
         a%n = n
         a%precision = merge(HIP_MIXED, HIP_FP64, (precision < 0 .and. ctxs(h)%mixed == 1) .or. precision == 1)
         a%split_parts = merge(1, 0, split /= 0)
+        a%use_ops = merge(1, 0, ctxs(h)%has_ops)
         a%prm = d_prm
         a%ssp_weights = d_weights
         a%mags = d_mags
@@ -579,5 +625,90 @@ contains
             call set_error(status, 'batch_device: ' // trim(hip_message(ctxs(h)%dctx)))
         end if
     end subroutine batch_run_device
+
+    !> opensps_context_evaluate_ssp_op (src/bindings/c/opensps_c_api.f90:1299-1358; ssp_operator_evaluate,
+    !> src/physics/opensps_ssp_operator.f90:1346-1450): the SSP of every age for one IMF from the context's operator,
+    !> mass_grid (ntfull) surviving mass, lbol_grid (ntfull) log10 L_bol, spec_grid (nspec x ntfull). One batch of
+    !> ntfull models on the GPU, model t holding unit weight at age t (its CSP is the SSP of that age) with the given
+    !> IMF, no dust, no redshift, no smoothing, at the metallicity zmet of the context, in FP64 (the reference's
+    !> working precision). imf_type < 0: the context's; imf_params (imf1, imf2, imf3, vdmc, mdave) or NULL: the
+    !> context's values. Status 303 without an operator, 3 for NULL outputs, 2 for an IMF type the operator does not
+    !> integrate or, with Dave's IMF, an mdave other than the context's.
+    subroutine ssp_op_evaluate(h, imf_type, c_imf_params, c_mass, c_lbol, c_spec, status)
+        integer, intent(in) :: h, imf_type
+        type(c_ptr), intent(in) :: c_imf_params, c_mass, c_lbol, c_spec
+        integer, intent(out) :: status
+        integer, parameter :: NV = 9
+        character(kind=c_char, len=20), target :: nm(NV)
+        type(c_ptr), target :: np(NV), vp(NV)
+        real(WP), allocatable, target :: vals(:, :), w(:, :), mags(:, :)
+        real(c_double), pointer :: ip(:)
+        integer :: nt, k, zidx, it_save, mixed_save
+        logical :: imf_dirty
+        status = 0
+        if (.not. ctx_valid(h)) then
+            status = 1
+            call set_error(1, 'evaluate_ssp_op: bad handle')
+            return
+        end if
+        associate(c => ctxs(h))
+        if (.not. c%has_ops) then
+            status = 303
+            call set_error(303, 'opensps_context_evaluate_ssp_op: no SSP operator on the context')
+            return
+        end if
+        if (.not. c_associated(c_mass) .or. .not. c_associated(c_lbol) .or. .not. c_associated(c_spec)) then
+            status = 3
+            call set_error(3, 'opensps_context_evaluate_ssp_op: output pointers must not be NULL')
+            return
+        end if
+        nt = c%nt
+        allocate(vals(nt, NV), w(nt, nt), mags(max(c%nbands, 1), nt))
+        w = 0.0_wp
+        do k = 1, nt
+            w(k, k) = 1.0_wp
+        end do
+        zidx = max(1, min(c%nz, c%zmet))
+        nm(1) = 'dust1'; vals(:, 1) = 0.0_wp
+        nm(2) = 'dust2'; vals(:, 2) = 0.0_wp
+        nm(3) = 'dust3'; vals(:, 3) = 0.0_wp
+        nm(4) = 'zred'; vals(:, 4) = 0.0_wp
+        nm(5) = 'sigma_smooth'; vals(:, 5) = 0.0_wp
+        nm(6) = 'logzsol'; vals(:, 6) = c%logz(zidx)
+        nm(7) = 'imf1'; vals(:, 7) = c%pset(P_IMF1)
+        nm(8) = 'imf2'; vals(:, 8) = c%pset(P_IMF1 + 1)
+        nm(9) = 'imf3'; vals(:, 9) = c%pset(P_IMF1 + 2)
+        if (c_associated(c_imf_params)) then
+            call c_f_pointer(c_imf_params, ip, [5])
+            vals(:, 7) = ip(1); vals(:, 8) = ip(2); vals(:, 9) = ip(3)
+            if (merge(imf_type, c%imf_type, imf_type >= 0) == 4 .and. ip(5) /= c%mdave) then
+                status = 2
+                call set_error(2, 'opensps_context_evaluate_ssp_op: mdave other than the context''s (set_float "mdave")')
+                return
+            end if
+        end if
+        do k = 1, NV
+            nm(k) = trim(nm(k)) // c_null_char
+            np(k) = c_loc(nm(k))
+            vp(k) = c_loc(vals(1, k))
+        end do
+        it_save = c%imf_type
+        mixed_save = c%mixed
+        imf_dirty = imf_type >= 0 .and. imf_type /= c%imf_type
+        if (imf_dirty) then
+            c%imf_type = imf_type
+            c%dev_imf = .false.
+        end if
+        c%mixed = 0
+        end associate
+        call batch_run(h, nt, NV, c_loc(np), c_loc(vp), c_loc(w), c_loc(mags), c_spec, c_mass, c_lbol, c_null_ptr, &
+                       c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, &
+                       c_null_ptr, c_null_ptr, status)
+        ctxs(h)%mixed = mixed_save
+        if (imf_dirty) then
+            ctxs(h)%imf_type = it_save
+            ctxs(h)%dev_imf = .false.
+        end if
+    end subroutine ssp_op_evaluate
 
 end module opensps_host_batch

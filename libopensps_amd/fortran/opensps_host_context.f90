@@ -25,6 +25,12 @@ module opensps_host_context
         logical :: dev_prof = .false.
         real(WP) :: h0 = 72.0_wp, om0 = 0.27_wp, ol0 = 0.73_wp
         real(WP) :: pset(NPAR) = PARAM_DEFAULTS
+        ! the members no stage here reads (INERT_NAMES), kept as set; the context's IMF settings beyond the slots
+        real(WP) :: inert(NINERT) = INERT_DEFAULTS
+        real(WP) :: mdave = 0.5_wp, vdmc = 0.08_wp
+        integer :: add_stellar_remnants = 1, zmet = 1
+        ! an SSP operator installed on the device (opensps_context_set_ssp_ops): the batch then takes each model's IMF
+        logical :: has_ops = .false.
         ! the resident basis
         logical :: has_basis = .false.
         integer :: nspec = 0, nt = 0, nz = 0
@@ -49,7 +55,7 @@ module opensps_host_context
         ! device side
         type(c_ptr) :: dctx = c_null_ptr
         logical :: dev_grid = .false., dev_bands = .false., dev_cosmo = .false., dev_mask = .false.
-        logical :: dev_vega = .false., dev_tab = .false., dev_dust = .false.
+        logical :: dev_vega = .false., dev_tab = .false., dev_dust = .false., dev_imf = .false.
         type(dbuf_t) :: d_prm, d_w, d_mags, d_spec, d_mass, d_lbol, d_fband, d_young, d_old, d_idx
         integer :: last_n = 0, last_bytes = 8
         real(WP), allocatable :: last_maggies(:, :)
@@ -124,6 +130,15 @@ contains
         if (st == 0) b%bytes = max(bytes, 256_c_size_t)
     end function dbuf_reserve
 
+    !> An SSP parameter of the context changed: an installed SSP operator is dropped, as the reference's setters
+    !> invalidate it (invalidate_ssps, src/bindings/c/opensps_c_api.f90:2196-2203, 470-473, 497-502)
+    subroutine drop_ops(c)
+        type(host_ctx_t), intent(inout) :: c
+        integer(c_int) :: rc
+        if (c%has_ops .and. c_associated(c%dctx)) rc = opensps_hip_free_ssp_ops(c%dctx)
+        c%has_ops = .false.
+    end subroutine drop_ops
+
     !> Integer settings (opensps_context_set_int). status 0, or 101 for an unknown key
     subroutine ctx_set_int(h, key, v, status)
         integer, intent(in) :: h, v
@@ -143,7 +158,13 @@ contains
         case ('compute_vega_mags'); c%compute_vega_mags = v; c%dev_vega = .false.
         case ('redshift_colors'); c%redshift_colors = v
         case ('keep_maggies'); c%keep_maggies = v
-        case ('imf_type'); c%imf_type = v
+        case ('imf_type')
+            if (v /= c%imf_type) call drop_ops(c)
+            c%imf_type = v; c%dev_imf = .false.
+        case ('add_stellar_remnants')
+            if (v /= c%add_stellar_remnants) call drop_ops(c)
+            c%add_stellar_remnants = v; c%dev_imf = .false.
+        case ('zmet'); c%zmet = v
         case ('mixed_precision'); c%mixed = merge(1, 0, v /= 0)
         case ('profile_kernels'); c%profile = merge(1, 0, v /= 0); c%dev_prof = .false.
         case ('sfh'); c%pset(P_SFH) = real(v, WP)
@@ -183,10 +204,19 @@ contains
                 if (v /= c%ol0) c%cosmo_dirty = .true.
                 c%ol0 = v
             end if
+        case ('mdave')
+            if (v /= c%mdave) call drop_ops(c)
+            c%mdave = v; c%dev_imf = .false.
+        case ('vdmc')
+            if (v /= c%vdmc) call drop_ops(c)
+            c%vdmc = v
         case default
             k = param_slot(key)
             if (k > 0) then
+                if (any(IMF_NAMES == lower(trim(key))) .and. v /= c%pset(k)) call drop_ops(c)
                 c%pset(k) = v
+            else if (inert_slot(key) > 0) then
+                c%inert(inert_slot(key)) = v
             else
                 status = 102
             end if
@@ -251,6 +281,9 @@ contains
             c%spec = spec; c%mass = mass; c%lbol = lbol
             c%has_basis = .true.
             c%dev_grid = .false.
+            ! an SSP operator belongs to the grid it was given for
+            if (c%has_ops .and. c_associated(c%dctx)) i = opensps_hip_free_ssp_ops(c%dctx)
+            c%has_ops = .false.
             ! a new grid invalidates the filters (they are built on the grid)
             c%nbands = 0
             if (allocated(c%bands)) deallocate(c%bands)
@@ -339,6 +372,7 @@ contains
             end if
             c%dev_grid = .false.; c%dev_bands = .false.; c%dev_cosmo = .false.; c%dev_mask = .false.
             c%dev_prof = .false.; c%dev_vega = .false.; c%dev_tab = .false.; c%dev_dust = .false.
+            c%dev_imf = .false.
         end if
         if (.not. c%dev_grid) then
             rc = opensps_hip_set_grid(c%dctx, int(c%nspec, c_int), c%lam, int(c%nt, c_int), c%log_age, &
@@ -401,6 +435,15 @@ contains
             end if
             if (rc /= 0) goto 900
             c%dev_vega = .true.
+        end if
+        if (.not. c%dev_imf .and. c%has_ops) then
+            ! the IMF of the SSP operator: the context's type, remnants and Dave's mdave; Salpeter's slope 2.35 and the
+            ! remnant progenitor limits 40 and 8.5 M_sun are the context state's defaults
+            ! (src/core/opensps_context_types.f90:120-125)
+            rc = opensps_hip_set_imf(c%dctx, int(c%imf_type, c_int), int(c%add_stellar_remnants, c_int), 2.35_wp, &
+                                     c%mdave, 40.0_wp, 8.5_wp)
+            if (rc /= 0) goto 900
+            c%dev_imf = .true.
         end if
         if (.not. c%dev_prof) then
             rc = opensps_hip_set_profiling(c%dctx, int(c%profile, c_int))

@@ -24,7 +24,7 @@ contains
                                    c_mass_bar, c_lbol_bar, c_mags, c_spec, c_mass, c_lbol, c_mdust, c_sfr, c_emlines, &
                                    c_indx, c_params_bar, c_weights_bar, c_weights_z_bar
         integer, intent(out) :: status
-        real(WP), allocatable, target :: prm(:, :), pb(:, :), hspec(:, :)
+        real(WP), allocatable, target :: prm(:, :), pb(:, :), hspec(:, :), fagn(:)
         real(c_float), allocatable, target :: fspec(:, :)
         real(c_double), pointer :: o2(:, :), o1(:)
         type(hip_grad_t) :: g
@@ -76,9 +76,14 @@ contains
             call set_error(3, 'batch_grad: ssp_weights and ssp_weights_z both given')
             return
         end if
-        call resolve_params(c, n, n_params, c_names, c_values, prm, status)
+        if (c%has_ops) then
+            status = 2
+            call set_error(2, 'batch_grad: the reverse pass with the SSP operator (the IMF rows) is not part of this build')
+            return
+        end if
+        call resolve_params(c, n, n_params, c_names, c_values, prm, status, fagn)
         if (status /= 0) return
-        call model_checks(c, prm, n, st)
+        call model_checks(c, prm, n, st, fagn)
         if (st /= 0) then
             status = st
             return

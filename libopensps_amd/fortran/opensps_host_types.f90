@@ -28,11 +28,30 @@ module opensps_host_types
                                   P_SF_SLOPE = 8, &
                                   P_IMF1 = 24
 
-    !> Parameters the batch refuses to vary per model (SSP-sensitive: they change the resident basis;
-    !> the reference's batch_handle returns status 2 for them, src/bindings/c/opensps_c_api.f90:761-788)
-    character(len=16), parameter, public :: SSP_NAMES(14) = [character(len=16) :: &
-        'imf_type', 'zmet', 'pagb', 'dell', 'delt', 'fbhb', 'sbss', 'redgb', 'agb', 'masscut', 'vdmc', 'mdave', &
-        'evtype', 'fcstar']
+    !> Parameters the batch refuses to vary per model: they change the SSPs (is_ssp_sensitive_float_param,
+    !> src/core/opensps_context.f90:958-969, and the integer ones), so the reference's batch_handle returns status 2
+    !> for them (src/bindings/c/opensps_c_api.f90:761-788)
+    character(len=20), parameter, public :: SSP_NAMES(18) = [character(len=20) :: &
+        'imf_type', 'zmet', 'pagb', 'dell', 'delt', 'fbhb', 'sbss', 'redgb', 'agb', 'masscut', 'evtype', 'fcstar', &
+        'tiny_logt', 'logt_wmb_hot', 'nebular_smooth_init', 'frac_xrb', 'agb_dust', 'imf_filename']
+
+    !> The IMF's parameters: per model only with an SSP operator on the context (the reference's IMF_PARAMS,
+    !> src/bindings/c/opensps_c_api.f90:713-714, 772-776); status 2 without one
+    character(len=16), parameter, public :: IMF_NAMES(7) = [character(len=16) :: &
+        'imf1', 'imf2', 'imf3', 'vdmc', 'mdave', 'imf_lower_limit', 'imf_upper_limit']
+
+    !> Members of PARAMS_CORE (src/core/opensps_types.f90:33-45) that no stage this build runs reads: the dust
+    !> curves of dust_type 1, 3, 4, 5 (mwr, uvb), the clumpy screen and pmetals (not read by the batch), nebular
+    !> emission (gas_*), dust emission (duste_*) and the AGN (fagn, agn_tau). They are accepted per model and at the
+    !> context, as the reference accepts them; the host refuses the stages that would read them (status 2: dust
+    !> emission with dust, fagn > 0 with add_agn_dust = 1)
+    integer, parameter, public :: NINERT = 14, I_FAGN = 14
+    character(len=16), parameter, public :: INERT_NAMES(NINERT) = [character(len=16) :: &
+        'pmetals', 'dust_clumps', 'uvb', 'mwr', 'gas_logu', 'gas_logz', 'gas_logn', 'gas_logno', 'gas_logco', &
+        'agn_tau', 'duste_gamma', 'duste_umin', 'duste_qpah', 'fagn']
+    real(WP), parameter, public :: INERT_DEFAULTS(NINERT) = [ &
+        0.02_wp, -99.0_wp, 1.0_wp, 3.1_wp, -2.0_wp, 0.0_wp, 2.0_wp, 0.0_wp, 0.0_wp, 10.0_wp, 0.01_wp, 1.0_wp, &
+        3.5_wp, 0.0_wp]
 
     !> Rows of params_bar (include/opensps.h:12-59), as opensps_get_nbgrad reports them
     integer, parameter, public :: NBGRAD = 44
@@ -46,7 +65,7 @@ module opensps_host_types
     character(len=512), public :: last_error = ''
     integer, public :: last_status = 0
 
-    public :: param_slot, set_error, lower
+    public :: param_slot, inert_slot, set_error, lower
 
 contains
 
@@ -73,6 +92,19 @@ contains
             end if
         end do
     end function param_slot
+
+    !> The index of a parameter in INERT_NAMES, 0 if it is not one
+    pure integer function inert_slot(name) result(k)
+        character(len=*), intent(in) :: name
+        integer :: i
+        k = 0
+        do i = 1, NINERT
+            if (trim(INERT_NAMES(i)) == lower(trim(name))) then
+                k = i
+                return
+            end if
+        end do
+    end function inert_slot
 
     subroutine set_error(status, msg)
         integer, intent(in) :: status

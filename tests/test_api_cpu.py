@@ -22,7 +22,8 @@ def _declared(header):
     txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     txt = re.sub(r"//.*", "", txt)
-    return sorted(set(re.findall(r"\b(opensps_\w+)\s*\(", txt)))
+    inline = set(re.findall(r"static\s+inline\s+\w+\s+(opensps_\w+)\s*\(", txt))   # header-only, not exported
+    return sorted(set(re.findall(r"\b(opensps_\w+)\s*\(", txt)) - inline)
 
 
 def _exported(so):
@@ -214,3 +215,95 @@ def test_dust_type_statuses():
             r = ctx.compute_csp_batch({"tage": np.full(2, 5.0), "sfh": np.full(2, 4.0)}, raw_status=True)
             assert r["status"] == want, (dt, r["status"])
 
+
+
+def test_imf_and_ssp_parameters_without_operator():
+    """Without an SSP operator on the context the IMF's parameters change the SSPs: a batch naming them is status 2,
+    as the reference's C test checks for imf1 (tests/test_c_batch.c:136-146); so is every parameter of
+    is_ssp_sensitive_float_param (src/core/opensps_context.f90:958-969) that cannot act in the CSP."""
+    g = grid(1)
+    with api.Context() as ctx:
+        ctx.set_ssp_basis(g)
+        ctx.set_filters(synth.gaussian_filters(4))
+        ctx.set_int("add_dust_emission", 0)
+        assert not ctx.has_ssp_ops()
+        for nm in ("imf1", "imf2", "imf3", "imf_upper_limit", "imf_lower_limit", "vdmc", "mdave", "tiny_logt",
+                   "logt_wmb_hot", "frac_xrb", "agb_dust", "nebular_smooth_init", "pagb", "masscut", "afe"):
+            r = ctx.compute_csp_batch({"tage": np.full(3, 5.0), nm: np.ones(3)}, raw_status=True)
+            assert r["status"] == 2, (nm, r["status"])
+        r = ctx.compute_csp_batch({"not_a_parameter": np.ones(3)}, raw_status=True)
+        assert r["status"] == 3
+        # fagn > 0 asks for the AGN stage (add_agn_dust = 1 by default)
+        r = ctx.compute_csp_batch({"tage": np.full(3, 5.0), "fagn": np.full(3, 0.1)}, raw_status=True)
+        assert r["status"] == 2 and "AGN" in api.last_error()
+
+
+def test_parameters_no_stage_reads_are_accepted():
+    """Members of the parameter set that no stage of this build reads (dust curves of other types, nebular, dust
+    emission without dust, fagn = 0) pass the host's checks and reach the device (status 5 here without one)."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    g = grid(1)
+    with api.Context() as ctx:
+        ctx.set_ssp_basis(g)
+        ctx.set_filters(synth.gaussian_filters(4))
+        ctx.set_int("add_dust_emission", 0)
+        vals = {k: np.ones(2) for k in ("pmetals", "dust_clumps", "uvb", "mwr", "gas_logu", "gas_logz", "gas_logn",
+                                        "gas_logno", "gas_logco", "agn_tau", "duste_gamma", "duste_umin",
+                                        "duste_qpah")}
+        vals["fagn"] = np.zeros(2)
+        vals["tage"] = np.full(2, 5.0)
+        r = ctx.compute_csp_batch(vals, raw_status=True)
+        assert r["status"] == 5
+        ctx.set_float("mwr", 4.0)
+        ctx.set_float("gas_logu", -3.0)
+
+
+def test_ssp_operator_api_statuses():
+    """The SSP-operator calls' statuses (the reference's tests/test_c_ssp_operator.c:14-59, 149-160): building from a
+    data release is 301 here (this build takes the operator from the caller), bad handles 1, evaluate before an
+    operator 303, has_ssp_ops 0, free is a no-op."""
+    st = C.c_int()
+    h = C.c_int()
+    L = api.lib()
+    for fn in ("opensps_context_build_ssp_ops", "opensps_context_build_ssp_ops_free_mods"):
+        getattr(L, fn)(999, 0, C.byref(st))
+        assert st.value == 1
+    L.opensps_context_free_ssp_ops(999, C.byref(st))
+    assert st.value == 1
+    L.opensps_context_has_ssp_ops(999, C.byref(h), C.byref(st))
+    assert st.value == 1
+    L.opensps_context_evaluate_ssp_op(999, 2, None, None, None, None, C.byref(st))
+    assert st.value == 1
+    g = grid(3, 600, 20)
+    with api.Context() as ctx:
+        with pytest.raises(api.OpenSPSError) as e:
+            ctx.build_ssp_ops(0)
+        assert e.value.status == 301
+        L.opensps_context_build_ssp_ops_free_mods(ctx.handle, 0, C.byref(st))
+        assert st.value == 301
+        op = synth.ssp_operator(g.lam, g.log_age, 3, npts=8, ntpl=20)
+        with pytest.raises(api.OpenSPSError) as e:   # no basis yet
+            ctx.set_ssp_ops(op)
+        assert e.value.status == 1
+        ctx.set_ssp_basis(g)
+        assert not ctx.has_ssp_ops()
+        with pytest.raises(api.OpenSPSError) as e:
+            ctx.evaluate_ssp_op(2)
+        assert e.value.status == 303
+        ctx.free_ssp_ops()
+        bad = synth.ssp_operator(g.lam, g.log_age[:10], 3, npts=8, ntpl=20)
+        with pytest.raises(api.OpenSPSError) as e:   # ages other than the basis's
+            ctx.set_ssp_ops(bad)
+        assert e.value.status == 3
+
+
+def test_batch_ext_null_arguments():
+    """The header-only struct entry point (include/opensps.h, the reference's :366-421) returns status 1 for NULL
+    options, compiled from C through the tests' shim."""
+    shim = C.CDLL(build.SHIM_SO)
+    st = C.c_int(-7)
+    shim.shim_batch_ext(1, 0, 0, None, None, None, None, None, None, None, None, None, None, None, None, None, 0,
+                        None, None, 1, C.byref(st))
+    assert st.value == 1

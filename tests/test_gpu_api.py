@@ -291,3 +291,122 @@ def test_api_fixed_dust_curves(precision, dust_type):
         assert np.all(np.isfinite(r["mags"][ok]))
         assert float(np.max(np.abs(r["mags"][ok] - q[ok]))) < tm
 
+
+
+def _imf_models(n, nz, seed):
+    prm = synth.parametric_params(n, seed=seed, logzsol=(-2.1, 0.45))
+    prm = synth.imf_params(n, seed=seed + 1, p=prm)
+    prm[SLOT["imf_lower_limit"]] = np.linspace(0.08, 0.6, n)
+    return prm, synth.varied_names() + ["imf1", "imf2", "imf3", "imf_upper_limit", "imf_lower_limit"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_api_imf_per_model(precision):
+    """The IMF per model through the public C API: an operator given by opensps_context_set_ssp_ops, Kroupa slopes
+    and limits per model as names[] / values[] (the reference's IMF_PARAMS with ssp_ops), against the oracle's
+    operator path; without the operator the same call is status 2."""
+    from helpers import operator
+    nz = 3
+    g = grid(nz)
+    op = operator(nz, 40, 300)
+    fs = synth.gaussian_filters(20)
+    n = 64
+    prm, names = _imf_models(n, nz, 51)
+    print(f"case: api imf per model n={n} {precision}", flush=True)
+    with _ctx(g, fs, precision) as ctx:
+        r0 = ctx.compute_csp_batch_params(prm, names, raw_status=True)
+        assert r0["status"] == 2
+        ctx.set_ssp_ops(op)
+        assert ctx.has_ssp_ops()
+        r = ctx.compute_csp_batch_params(prm, names, want=("mags", "spec", "mass", "lbol"))
+        bands = ctx.bands()
+        # an SSP parameter of the context changed: the operator is dropped (the reference's invalidate_ssps)
+        ctx.set_int("imf_type", 0)
+        assert not ctx.has_ssp_ops()
+    ref = O.csp_batch(g, bands, prm, add_igm=1, cosmo=O.cosmology_table(), ops=op, imf_type=2, remnants=1)
+    ts, tm = TOL[precision]
+    es, em = spec_rel_err(r["spec"], ref["spec"]), mag_err(r["mags"], ref["mags"])
+    print(f"api imf per model: spectra {es:.2e}, mags {em:.2e}")
+    assert es < ts and em < tm
+    np.testing.assert_allclose(r["mass"], ref["mass"], rtol=1e-9)
+    np.testing.assert_allclose(r["lbol"], ref["lbol"], rtol=0, atol=1e-9)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("imf_type", [2, 0])
+def test_api_evaluate_ssp_op(imf_type):
+    """opensps_context_evaluate_ssp_op: the SSP of every age for one IMF (ssp_operator_evaluate), at the context's
+    metallicity zmet, against the oracle's operator path with a unit weight at each age; imf_params NULL takes the
+    context's slopes."""
+    from helpers import operator
+    nz = 3
+    g = grid(nz)
+    op = operator(nz, 40, 300)
+    fs = synth.gaussian_filters(8)
+    slopes = (0.9, 1.8, 2.7)
+    print(f"case: api evaluate_ssp_op imf_type={imf_type}", flush=True)
+    with _ctx(g, fs, "mixed") as ctx:     # evaluate runs in FP64 whatever the batch precision
+        ctx.set_int("imf_type", imf_type)
+        ctx.set_ssp_ops(op)
+        ctx.set_int("zmet", 2)
+        mass, lbol, spec = ctx.evaluate_ssp_op(imf_type, (*slopes, 0.08, 0.5))
+        for k, v in zip(("imf1", "imf2", "imf3"), slopes):
+            ctx.set_float(k, v)          # (drops the operator: given again)
+        ctx.set_ssp_ops(op)
+        mass2, lbol2, spec2 = ctx.evaluate_ssp_op(-1, None)
+        bands = ctx.bands()
+    nt = g.nt
+    prm = synth.default_params(nt)
+    prm[SLOT["logzsol"]] = g.logz[1]
+    for k, v in zip(("imf1", "imf2", "imf3"), slopes):
+        prm[SLOT[k]] = v
+    ref = O.csp_batch(g, bands, prm, ssp_weights=np.eye(nt), add_igm=1, cosmo=O.cosmology_table(), ops=op,
+                      imf_type=imf_type, remnants=1)
+    es = spec_rel_err(spec, ref["spec"])
+    print(f"evaluate_ssp_op: spectra {es:.2e}")
+    assert es < 1e-9
+    np.testing.assert_allclose(mass, ref["mass"], rtol=1e-11)
+    np.testing.assert_allclose(lbol, ref["lbol"], rtol=0, atol=1e-11)
+    np.testing.assert_array_equal(mass2, mass)
+    np.testing.assert_array_equal(spec2, spec)
+
+
+@pytest.mark.gpu
+def test_api_batch_ext_equals_grad_rows():
+    """The header-only struct entry point (opensps_context_compute_csp_batch_ext, compiled from C by the tests'
+    shim) gives what opensps_context_compute_csp_batch_grad_rows gives for the same call: forward outputs and
+    params_bar, weights_z_bar, bit for bit."""
+    import ctypes as C
+    from libopensps_amd import api, build
+    g = grid(4)
+    fs = synth.gaussian_filters(12)
+    n = 16
+    prm = synth.parametric_params(n, seed=61)
+    names = ["dust2", "dust_index", "zred", "sigma_smooth"]
+    wz = np.random.Generator(np.random.PCG64(62)).dirichlet(np.full(g.nt * g.nz, 0.5), n).reshape(n, g.nz, g.nt)
+    mb = np.random.Generator(np.random.PCG64(63)).normal(size=(n, 12))
+    print("case: api batch_ext vs grad_rows", flush=True)
+    with _ctx(g, fs, "double") as ctx:
+        vals = {k: prm[SLOT[k]] for k in names}
+        ref = ctx.compute_csp_batch_grad(vals, n=n, ssp_weights_z=wz, mags_bar=mb, want=("mags", "spec", "mass"),
+                                         grads=("params_bar", "weights_z_bar"), params_bar_rows=api.NBGRAD)
+        keep = [np.ascontiguousarray(vals[k]) for k in names]
+        cn = (C.c_char_p * len(names))(*[k.encode() for k in names])
+        cv = (C.c_void_p * len(names))(*[a.ctypes.data for a in keep])
+        wzc, mbc = np.ascontiguousarray(wz), np.ascontiguousarray(mb)
+        mags, spec, mass = np.zeros((n, 12)), np.zeros((n, g.nspec)), np.zeros(n)
+        pb, wzb = np.zeros((n, api.NBGRAD)), np.zeros((n, g.nz, g.nt))
+        st = C.c_int()
+        shim = C.CDLL(build.SHIM_SO)
+        shim.shim_batch_ext(ctx.handle, n, len(names), C.cast(cn, C.c_void_p), C.cast(cv, C.c_void_p), None,
+                            C.c_void_p(wzc.ctypes.data), C.c_void_p(mbc.ctypes.data), None, None, None,
+                            C.c_void_p(mags.ctypes.data), C.c_void_p(spec.ctypes.data), C.c_void_p(mass.ctypes.data),
+                            None, C.c_void_p(pb.ctypes.data), 0, None, C.c_void_p(wzb.ctypes.data), 0, C.byref(st))
+        assert st.value == 0, api.last_error()
+    np.testing.assert_array_equal(mags, ref["mags"])
+    np.testing.assert_array_equal(spec, ref["spec"])
+    np.testing.assert_array_equal(mass, ref["mass"])
+    np.testing.assert_array_equal(pb, ref["params_bar"])
+    np.testing.assert_array_equal(wzb, ref["weights_z_bar"])
+    assert np.any(pb != 0.0)
