@@ -122,6 +122,15 @@ int opensps_hip_set_options(opensps_hip_ctx_t *ctx, int add_igm, int smooth_fast
 /* The batch: compute_csp_batch's stages 1-8 (src/physics/opensps_csp_batch.f90:431-1214) on the device,
  * asynchronous on the context's stream; the status of the models is checked by opensps_hip_sync. */
 int opensps_hip_csp_batch(opensps_hip_ctx_t *ctx, const opensps_hip_batch_t *args);
+/* No counterpart in the reference: opensps_hip_csp_batch for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The same kernels and results; the batch's
+ * scratch comes from a fixed array of 64 MiB per device, held while the process lives, instead of the context's
+ * grow-only buffers, and nothing is allocated or freed. Status 3 (nothing launched) when the batch needs more than
+ * 64 MiB of scratch: about 25 KB per model in mixed precision with spectra requested (~2600 models), 100 KB per
+ * model in FP64 with young and old apart (~650 models); 2 with use_ops or ssp_weights_z. Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (several contexts or streams at once) race, and
+ * opensps_hip_sync must follow each call before the next one. */
+int opensps_hip_csp_batch_noalloc(opensps_hip_ctx_t *ctx, const opensps_hip_batch_t *args);
 /* The forward batch and its reverse pass, asynchronous on the context's stream (opensps_hip_sync waits). Replaces
  * batch_backward's stages for the outputs above (opensps_csp_batch.f90:4713-5136; backward_observed_frame :6958,
  * smooth_columns_adjoint / smooth_sigma_adjoint, opensps_batch_photometry.f90:604-700, atten_adjoint_chunk

@@ -16,6 +16,20 @@
 #include "ops_launch.h"
 
 static std::mutex g_err_mu;
+
+__global__ void k_fill_i32(int* __restrict__ p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---- the fixed scratch of the no-allocation entry point (opensps_hip_csp_batch_noalloc) ----
+// One array per device, held while the process lives; its device address is taken once per device at context
+// creation (hipGetSymbolAddress with that device current) and used wherever host code needs the memory.
+#define OPS_NOALLOC_BYTES ((size_t)64 << 20)
+static __device__ __align__(256) unsigned char scratch[OPS_NOALLOC_BYTES];
+#define OPS_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[OPS_MAX_DEVICES] = {};
 static std::string g_err;
 
 int ops_fail(OpsCtx* c, int code, const char* fmt, ...) {
@@ -41,6 +55,17 @@ int ops_fail(OpsCtx* c, int code, const char* fmt, ...) {
 
 int ops_reserve(OpsCtx* c, OpsBuffer& b, size_t bytes) {
     if (bytes == 0) bytes = 256;
+    if (c->arena) {   // the no-allocation entry point: a 256-byte aligned piece of the fixed scratch
+        OpsArena& A = *c->arena;
+        const size_t off = (A.used + 255) & ~(size_t)255;
+        if (off > A.cap || bytes > A.cap - off)
+            return ops_fail(c, 3, "batch_noalloc: the batch needs more than the %zu bytes of fixed scratch (%zu used, "
+                                  "%zu more asked): fewer models per call", A.cap, off, bytes);
+        b.p = A.base + off;
+        b.bytes = bytes;
+        A.used = off + bytes;
+        return 0;
+    }
     if (b.bytes >= bytes) return 0;
     if (b.p) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -114,6 +139,18 @@ int opensps_hip_create(int device, opensps_hip_ctx_t** out) {
         return ops_fail(nullptr, 10, "opensps_hip_create: no HIP device");
     if (device < 0 || device >= ndev) return ops_fail(nullptr, 3, "opensps_hip_create: device %d of %d", device, ndev);
     if (hipSetDevice(device) != hipSuccess) return ops_fail(nullptr, 10, "opensps_hip_create: hipSetDevice failed");
+    if (device >= OPS_MAX_DEVICES) return ops_fail(nullptr, 3, "opensps_hip_create: device %d beyond %d", device,
+                                                     OPS_MAX_DEVICES);
+    {   // the fixed scratch's address on this device, once
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        if (!g_scratch_ptr[device]) {
+            void* p = nullptr;
+            hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return ops_fail(nullptr, 10, "opensps_hip_create: the fixed scratch's address: %s", hipGetErrorString(e));
+            g_scratch_ptr[device] = p;
+        }
+    }
     OpsCtx* c = new OpsCtx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -879,7 +916,17 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         if ((st = ops_reserve(c, c->opspart, (3 * sizeof(double) + sizeof(int)) * (size_t)npass * maxtiles * c->nt * 64)))
             return st;
     }
-    HIPCHK(c, hipMemsetAsync(d_ntiles, 0, ncnt * sizeof(int), c->stream));   // the counters and the pair slot counts
+    // The chain's layout (one LDS spectrum buffer when the band quads fit the registers of the single-buffer chain,
+    // else two) and its staging rows for smoothing windows wider than the registers hold: reserved before the first
+    // launch, so that every scratch failure returns before any work is queued
+    const int chain_threads = 512;
+    const int qbands = c->band_j1 >= c->band_j0 ? (c->band_j1 + 1 - (c->band_j0 & ~3)) / 4 : 0;
+    const bool one = qbands <= OPS_CHAIN_KQ * chain_threads && !c->chain_two;
+    if (one && !a->spec)
+        if ((st = ops_reserve(c, c->spec, tsz * (size_t)n * c->nspec))) return st;
+    // the counters and the pair slot counts start at zero (a fill kernel: no memset node where a caller captures)
+    k_fill_i32<<<(unsigned)((ncnt + 255) / 256), 256, 0, c->stream>>>(d_ntiles, (int)ncnt, 0);
+    HIPCHK(c, hipGetLastError());
     c->d_last_flag = d_flag;
 
     PrepArgs pa;
@@ -1000,7 +1047,10 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         w.poly = c->op.d_poly;
         w.spart = (double*)c->opspart.p;
         w.scnt = (int*)(w.spart + 3 * (size_t)npass * maxtiles * c->nt * 64);
-        if (a->op_count) HIPCHK(c, hipMemsetAsync(a->op_count, 0, sizeof(int) * 2 * (size_t)n, c->stream));
+        if (a->op_count) {
+            k_fill_i32<<<(unsigned)((2 * (size_t)n + 255) / 256), 256, 0, c->stream>>>(a->op_count, 2 * n, 0);
+            HIPCHK(c, hipGetLastError());
+        }
         k_op_imf<<<(n + 255) / 256, 256, 0, c->stream>>>(w);
         HIPCHK(c, hipGetLastError());
         dim3 gw((c->nt + 3) / 4, maxtiles, npass);
@@ -1155,17 +1205,9 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ca.fband = a->fband;
     ca.dcurve = c->dcurve_on ? c->d_dcurve : nullptr;
     ca.dcurve32 = c->dcurve_on ? c->d_dcurve32 : nullptr;
-    // One LDS spectrum buffer when the band quads fit the registers of the single-buffer chain, else two;
-    // then the band sums (up to 2048 bands) for the magnitudes a thread per band
-    const int chain_threads = 512;
-    const int qbands = c->band_j1 >= c->band_j0 ? (c->band_j1 + 1 - (c->band_j0 & ~3)) / 4 : 0;
-    const bool one = qbands <= OPS_CHAIN_KQ * chain_threads && !c->chain_two;
+    // the band sums (up to 2048 bands) for the magnitudes a thread per band
     ca.sband_staged = (size_t)c->nbands * sizeof(double) <= 16384;
-    ca.stage = nullptr;
-    if (one && !a->spec) {   // staging rows for smoothing windows wider than the registers hold
-        if ((st = ops_reserve(c, c->spec, tsz * (size_t)n * c->nspec))) return st;
-        ca.stage = c->spec.p;
-    }
+    ca.stage = one && !a->spec ? c->spec.p : nullptr;
     const size_t lds_chain = (one ? 1 : 2) * tsz * (size_t)c->nsp + (ca.sband_staged ? sizeof(double) * c->nbands : 0);
     const void* kfn = mixed ? (one ? (const void*)k_chain<float, true> : (const void*)k_chain<float, false>)
                             : (one ? (const void*)k_chain<double, true> : (const void*)k_chain<double, false>);
@@ -1193,6 +1235,36 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         HIPCHK(c, hipGetLastError());
     }
     return 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// The batch without allocation: the same launches as opensps_hip_csp_batch, its scratch carved from the device's
+// fixed array (OPS_NOALLOC_BYTES) instead of the context's grow-only buffers. The context's own buffers are set
+// aside for the call and restored after it, so both entry points can be mixed on one context.
+// ------------------------------------------------------------------------------------------------
+int opensps_hip_csp_batch_noalloc(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
+    if (!c) return 1;
+    if (!a) return ops_fail(c, 3, "batch_noalloc: NULL arguments");
+    if (a->use_ops || a->ssp_weights_z)
+        return ops_fail(c, 2, "batch_noalloc: the SSP operator and ssp_weights_z need more scratch than the fixed "
+                              "%zu bytes (use opensps_hip_csp_batch)", OPS_NOALLOC_BYTES);
+    void* base = g_scratch_ptr[c->device];
+    if (!base) return ops_fail(c, 1, "batch_noalloc: no fixed scratch on device %d", c->device);
+    OpsBuffer* mine[] = {&c->yo, &c->wcol, &c->w64, &c->tiles, &c->misc, &c->spec, &c->hdr};
+    const int nb = (int)(sizeof(mine) / sizeof(mine[0]));
+    OpsBuffer saved[sizeof(mine) / sizeof(mine[0])];
+    for (int i = 0; i < nb; i++) {
+        saved[i] = *mine[i];
+        *mine[i] = OpsBuffer();
+    }
+    OpsArena arena;
+    arena.base = (char*)base;
+    arena.cap = OPS_NOALLOC_BYTES;
+    c->arena = &arena;
+    const int st = opensps_hip_csp_batch(c, a);
+    c->arena = nullptr;
+    for (int i = 0; i < nb; i++) *mine[i] = saved[i];
+    return st;
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -20,6 +20,12 @@ struct OpsBuffer {
 
 struct OpsWPoint;   // ops_launch.h
 
+// A fixed region the batch's scratch is carved from instead of allocated (the no-allocation entry point)
+struct OpsArena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
 struct OpsCtx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -109,6 +115,7 @@ struct OpsCtx {
     bool chain_two = false;    // the two-buffer chain always (OPENSPS_CHAIN=two)
 
     // ---- batch scratch (grown on demand) ----
+    OpsArena* arena = nullptr;   // when set, ops_reserve carves from it and never allocates
     OpsBuffer yo, wcol, w64, order, tiles, misc, spec, flag;
     OpsBuffer hdr;   // per-model constants of the chain (OPS_HDR_N doubles per model)
     OpsBuffer wsfh, opu, opwc, opsum, opseg;   // the operator path: SFH weights, point weights, age factors,

@@ -67,6 +67,7 @@ def lib():
         L.opensps_hip_set_sfh_tab.argtypes = [vp, i, vp, vp]
         L.opensps_hip_set_options.argtypes = [vp, i, i]
         L.opensps_hip_csp_batch.argtypes = [vp, C.POINTER(BatchArgs)]
+        L.opensps_hip_csp_batch_noalloc.argtypes = [vp, C.POINTER(BatchArgs)]
         L.opensps_hip_sync.argtypes = [vp]
         L.opensps_hip_band_fluxes.argtypes = [i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp]
         L.opensps_hip_observed_frame.argtypes = [i, i, i, vp, vp, vp, vp, vp]
@@ -245,21 +246,24 @@ class DeviceContext:
             setattr(a, k, grads[k].data_ptr() if grads.get(k) is not None else None)
         self._check(lib().opensps_hip_csp_batch_grad(self._ctx, C.byref(a)))
 
-    def launch(self, prm, out, precision=MIXED, split_parts=0, ssp_weights=None, use_ops=0, ssp_weights_z=None):
-        """Enqueues one batch on device-resident prm (NPAR, n) float64 and the buffers of out."""
+    def launch(self, prm, out, precision=MIXED, split_parts=0, ssp_weights=None, use_ops=0, ssp_weights_z=None,
+               noalloc=False):
+        """Enqueues one batch on device-resident prm (NPAR, n) float64 and the buffers of out; noalloc: through
+        opensps_hip_csp_batch_noalloc (the fixed scratch, no device allocation)."""
         a = self._batch_args(prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z)
-        self._check(lib().opensps_hip_csp_batch(self._ctx, C.byref(a)))
+        fn = lib().opensps_hip_csp_batch_noalloc if noalloc else lib().opensps_hip_csp_batch
+        self._check(fn(self._ctx, C.byref(a)))
 
     def sync(self):
         self._check(lib().opensps_hip_sync(self._ctx))
 
     def batch(self, prm, precision=MIXED, split_parts=0, ssp_weights=None, want=("mags", "spec", "mass", "lbol"),
-              use_ops=0, ssp_weights_z=None):
+              use_ops=0, ssp_weights_z=None, noalloc=False):
         import torch
         n = prm.shape[1]
         out = self.alloc_outputs(n, precision, want)
         torch.cuda.synchronize()
-        self.launch(prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z)
+        self.launch(prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z, noalloc)
         self.sync()
         return out
 
