@@ -31,6 +31,28 @@ CHAIN_KERNEL = "k_chain<float, true>"
 TRAFFIC_REPORT = "k_chain_f32_parametric16k_traffic.md"
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (the first 'model name' of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def product_tables(grid, fset):
+    """The bands and the cosmology table as the product's host forms them (libopensps.so: filters_build and the
+    cosmology of a context with the reference's default H0, Om0, OL0), for the legs that drive libopensps_hip."""
+    from libopensps_amd import api
+    with api.Context() as c:
+        c.set_ssp_basis(grid)
+        c.set_filters(fset)
+        return c.bands(), c.cosmology()
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -40,6 +62,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-operator", action="store_true", help="skip the IMF-per-model workload")
+    ap.add_argument("--no-weights", action="store_true", help="skip the 256-model weight-matrix workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the batch-adjoint workload")
     ap.add_argument("--no-emulator", action="store_true", help="skip the emulator workload")
     ap.add_argument("--adjoint-sfh", action="store_true",
@@ -61,45 +84,45 @@ def operator_workload(args, n=4096):
     from libopensps_amd import hip as H, synth
 
     t0 = time.perf_counter()
+    from libopensps_amd import api
     grid = synth.ssp_grid(seed=1, nspec=5994, nt=107, nz=22)
     op = synth.ssp_operator(grid.lam, grid.log_age, 22, npts=1500, ntpl=10000, seed=11)
     fset = synth.gaussian_filters(100, seed=7)
     prm = synth.imf_params(n, seed=13, p=synth.parametric_params(n, seed=5))
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    bands = O.build_bands(fset, grid.lam)
-    cz, cdl = O.cosmology_table()
-    ctx = H.DeviceContext(0)
-    ctx.set_grid(grid)
-    ctx.set_bands(bands)
-    ctx.set_cosmology(cz, cdl)
-    ctx.set_options(1, 1)
+    # through the public API: the operator installed with opensps_context_set_ssp_ops, the batch on device-resident
+    # arrays (opensps_context_compute_csp_batch_device), each model's IMF from its slots
+    ctx = api.Context()
+    ctx.set_ssp_basis(grid)
+    ctx.set_filters(fset)
+    ctx.set_int("add_dust_emission", 0)
+    ctx.set_int("add_igm_absorption", 1)
+    ctx.set_int("profile_kernels", 1)
+    ctx.set_int("imf_type", 2)
     ctx.set_ssp_ops(op)
-    ctx.set_imf(2, 1)
+    bands = ctx.bands()
+    cz, cdl = ctx.cosmology()
     d_prm = torch.as_tensor(prm, device="cuda")
     setup = time.perf_counter() - t0
-    H.lib().opensps_hip_set_profiling.argtypes = [H.C.c_void_p, H.C.c_int]
-    H.lib().opensps_hip_kernel_times.argtypes = [H.C.c_void_p, H.C.c_void_p, H.C.c_void_p, H.C.c_int]
     res = {}
     outs = {}
     for prec, name in ((1, "mixed"), (0, "fp64")):
-        out = ctx.alloc_outputs(n, prec, want=("mags", "spec", "mass", "lbol"))
-        torch.cuda.synchronize()
-        H.lib().opensps_hip_set_profiling(ctx._ctx, 1)
-        ctx.launch(d_prm, out, prec, 0, None, 1)
-        ctx.sync()
-        ms = (H.C.c_double * 4)()
-        cnt = (H.C.c_int * 4)()
-        H.lib().opensps_hip_kernel_times(ctx._ctx, ms, cnt, 1)
+        dt_ = torch.float32 if prec == 1 else torch.float64
+        out = {"mags": torch.empty((n, ctx.nbands), dtype=torch.float64, device="cuda"),
+               "spec": torch.empty((n, grid.nspec), dtype=dt_, device="cuda"),
+               "mass": torch.empty(n, dtype=torch.float64, device="cuda"),
+               "lbol": torch.empty(n, dtype=torch.float64, device="cuda")}
+        run = lambda: ctx.compute_csp_batch_device(d_prm, out["mags"], d_spec=out["spec"], d_mass=out["mass"],
+                                                   d_lbol=out["lbol"], precision=prec)
+        run()
+        kernel_times(ctx, reset=1)
         steps = max(2, args.steps // 4)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         for _ in range(steps):
-            ctx.launch(d_prm, out, prec, 0, None, 1)
-            ctx.sync()
+            run()
+        torch.cuda.synchronize()
         dt = time.perf_counter() - t1
-        H.lib().opensps_hip_kernel_times(ctx._ctx, ms, cnt, 1)
-        kms = [ms[i] / max(cnt[i], 1) for i in range(4)]
+        kms, _ = kernel_times(ctx, reset=1)
         res[name] = {"models_per_s": n * steps / dt, "ms_per_step": 1e3 * dt / steps,
                      "kernel_ms": dict(zip(["prep", "imf_weights_fold_sums", "template_product", "chain"],
                                            [round(x, 4) for x in kms]))}
@@ -128,6 +151,8 @@ def operator_workload(args, n=4096):
                                        "mass_rel_err": float(((outs["mixed"]["mass"] / outs["fp64"]["mass"]) - 1).abs().max())},
               "roofline": roof, "setup_s": setup}
     if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
         m1 = 16
         sub = np.ascontiguousarray(prm[:, :m1])
         t1 = time.perf_counter()
@@ -144,6 +169,92 @@ def operator_workload(args, n=4096):
                                   "mixed_vs_oracle_max_mag_err": float(np.max(np.abs(hm[okr] - r["mags"][okr])))}
     ctx.close()
     return result
+
+
+def weight_matrix_workload(args, n=256):
+    """The plain weight-matrix workload: 256 models, Dirichlet(0.5) weights over 107 ages (1 M_sun formed), one
+    metallicity, 5994 px, no dust, z = 0, spectra and 20 Gaussian-filter magnitudes. Timed through the public C API's
+    host-array entry (what a reference caller calls: opensps_context_compute_csp_batch with ssp_weights, spectra and
+    magnitudes copied to the host) and its device-resident entry, in FP64 (the plain GPU path) and mixed precision;
+    the FP64 oracle on the whole batch. At 256 models the batch is bound by launch latency and transfers."""
+    import numpy as np
+    import torch
+    from libopensps_amd import api, synth
+
+    grid = synth.ssp_grid(seed=1, nspec=5994, nt=107, nz=1)
+    fset = synth.gaussian_filters(20, seed=7)
+    w = synth.dirichlet_weights(n, grid.nt, seed=21)
+    ctx = api.Context()
+    ctx.set_ssp_basis(grid)
+    ctx.set_filters(fset)
+    ctx.set_int("add_dust_emission", 0)
+    ctx.set_int("add_igm_absorption", 0)
+    ctx.set_int("profile_kernels", 1)
+    d_prm = torch.as_tensor(synth.default_params(n), device="cuda")
+    d_w = torch.as_tensor(w, device="cuda")
+    d_mags = torch.empty((n, ctx.nbands), dtype=torch.float64, device="cuda")
+    res = {}
+    steps = max(10, args.steps)
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        ctx.set_str("kernel_precision", "double" if prec == 0 else "mixed")
+        d_spec = torch.empty((n, grid.nspec), dtype=torch.float64 if prec == 0 else torch.float32, device="cuda")
+        host = lambda: ctx.compute_csp_batch({}, n=n, ssp_weights=w, want=("mags", "spec"))
+        dev = lambda: ctx.compute_csp_batch_device(d_prm, d_mags, d_spec=d_spec, d_weights=d_w, precision=prec)
+        r = host()
+        dev()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            host()
+        dt_h = (time.perf_counter() - t1) / steps
+        kernel_times(ctx, reset=1)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            dev()
+        torch.cuda.synchronize()
+        dt_d = (time.perf_counter() - t1) / steps
+        kms, _ = kernel_times(ctx, reset=1)
+        res[name] = {"host_api_models_per_s": n / dt_h, "host_api_ms": 1e3 * dt_h,
+                     "device_models_per_s": n / dt_d, "device_ms": 1e3 * dt_d,
+                     "kernel_ms": dict(zip(KERNELS, [round(x, 4) for x in kms]))}
+        res[name + "_mags"] = r["mags"]
+    out = {"workload": f"weight matrix: {n} models, Dirichlet(0.5) weights over 107 ages, 1 Z, 5994 px, no dust, "
+                       "z = 0, 20 Gaussian filters",
+           "value": res["fp64"]["host_api_models_per_s"], "unit": "models/s (FP64, public host-array call)",
+           "fp64": res["fp64"], "mixed": res["mixed"],
+           "mixed_vs_fp64_max_mag_err": float(np.nanmax(np.abs(res["mixed_mags"] - res["fp64_mags"])))}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        bands, cosmo = ctx.bands(), ctx.cosmology()
+        reps = 5
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            ref = O.csp_batch(grid, bands, synth.default_params(n), ssp_weights=w, add_igm=0, cosmo=cosmo,
+                              want=("mags", "spec"))
+        dt = (time.perf_counter() - t1) / reps
+        out["cpu_baseline"] = {"value": n / dt, "unit": "models/s", "cores": O.num_threads(), "kind": "port",
+                               "cpu": cpu_model(),
+                               "sample": f"the whole batch ({n} models) x {reps}, FP64 oracle (OpenMP, "
+                                         f"{O.num_threads()} threads), {reps * dt:.2f} s",
+                               "fp64_vs_oracle_max_mag_err": float(np.nanmax(np.abs(res["fp64_mags"] - ref["mags"])))}
+    for k in ("fp64_mags", "mixed_mags"):
+        res.pop(k)
+    ctx.close()
+    return out
+
+
+def kernel_times(ctx, reset=1):
+    """Per-launch device milliseconds of the batch's four kernel groups (opensps_context_kernel_times)."""
+    import ctypes as C
+    from libopensps_amd import api
+    ms = (C.c_double * 4)()
+    cnt = (C.c_int * 4)()
+    st = C.c_int()
+    api.lib().opensps_context_kernel_times.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    api.lib().opensps_context_kernel_times(ctx.handle, ms, cnt, reset, C.byref(st))
+    return [ms[i] / max(cnt[i], 1) for i in range(4)], [cnt[i] for i in range(4)]
 
 
 def adjoint_workload(args, n=8192, sfh=False):
@@ -174,10 +285,7 @@ def adjoint_workload(args, n=8192, sfh=False):
         prm[SLOT["sf_trunc"]] = np.where(simha, prm[SLOT["tage"]] * rng.uniform(0.3, 0.9, n), 0.0)
         prm[SLOT["sf_slope"]] = np.where(simha, rng.uniform(-1.0, 1.0, n), 0.0)
         wz = None
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    bands = O.build_bands(fset, grid.lam)
-    cz, cdl = O.cosmology_table()
+    bands, (cz, cdl) = product_tables(grid, fset)
     ctx = H.DeviceContext(0)
     ctx.set_grid(grid)
     ctx.set_bands(bands)
@@ -225,7 +333,9 @@ def adjoint_workload(args, n=8192, sfh=False):
     names = ("dust2", "dust_index", "zred", "sigma_smooth", "igm_factor")
     if sfh:
         names = names + ("logzsol", "tage", "tau", "sf_trunc", "sf_slope")
-    rows = {k: O.GRAD_ROWS[k] for k in names}
+    from libopensps_amd import api
+    row_of = {api.grad_param_name(i): i for i in range(api.nbgrad())}
+    rows = {k: row_of[k] for k in names}
     acc = {}
     for k, r in rows.items():
         a, b = outs["mixed"]["params_bar"][:, r], outs["fp64"]["params_bar"][:, r]
@@ -248,6 +358,8 @@ def adjoint_workload(args, n=8192, sfh=False):
               "fp64": res["fp64"], "speedup_vs_fp64": res["mixed"]["models_per_s"] / res["fp64"]["models_per_s"],
               "accuracy_vs_fp64_gpu": acc, "loss": loss, "roofline": roof, "setup_s": setup}
     if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
         m1 = 8
         t1 = time.perf_counter()
         r = O.csp_batch_grad(grid, bands, np.ascontiguousarray(prm[:, :m1]),
@@ -276,14 +388,12 @@ def emulator_workload(args, csp_models_per_s):
     from libopensps_amd import emu_train as T, hip as H, synth
 
     t0 = time.perf_counter()
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
     grid = synth.ssp_grid(seed=1, nspec=5994, nt=107, nz=22)
     fset = synth.gaussian_filters(100, seed=7)
-    cz, cdl = O.cosmology_table()
+    bands, (cz, cdl) = product_tables(grid, fset)
     dctx = H.DeviceContext(0)
     dctx.set_grid(grid)
-    dctx.set_bands(O.build_bands(fset, grid.lam))
+    dctx.set_bands(bands)
     dctx.set_cosmology(cz, cdl)
     dctx.set_options(1, 1)
     emu = T.Emulator.train(dctx, (cz, cdl), n_train=args.emu_train, n_pca_fit=200_000, n_pca=128,
@@ -306,8 +416,13 @@ def emulator_workload(args, csp_models_per_s):
     flops = 2.0 * (T.N_FEAT * 512 + 3 * 512 * 512 + 512 * (128 + 100)) * n
     # the serving path: 16 batches of 65536 with a 1 % spot-check at 0.01 mag
     emu.stats = {k: 0 for k in emu.stats}
+    serve_x = [T.sample_inputs(65536, 2000 + b) for b in range(16)]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
     for b in range(16):
-        emu.predict(T.sample_inputs(65536, 2000 + b), spot=0.01, tol=0.01, seed=b)
+        emu.predict(serve_x[b], spot=0.01, tol=0.01, seed=b)
+    torch.cuda.synchronize()
+    serve_rate = 16 * 65536 / (time.perf_counter() - t1)
     info = emu.train_info
     out = {"workload": "emulator: 7 parameters -> 4 x 512 GELU -> 128 PCA coefficients of the log-spectrum + 100 "
                        f"magnitudes; trained on {info['n_train']} spectra of the batch (PCA from {info['n_pca_fit']})",
@@ -317,6 +432,9 @@ def emulator_workload(args, csp_models_per_s):
                         "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": flops / dt / 1e12 / BF16_PEAK_TFLOPS},
            "validation_vs_fp64_batch": val,
            "fallback": {"rate": emu.fallback_rate(), "spot": 0.01, "tol_mag": 0.01, **emu.stats},
+           "serving_models_per_s": serve_rate,
+           "serving": "16 batches of 65536 models: inputs from the host, the emulator, a 1 % spot-check against the "
+                      "mixed batch and, where it fails, the whole batch recomputed by the batch (fallback included)",
            "training": {k: v for k, v in info.items() if k != "loss_history"},
            "loss_first_last": [info["loss_history"][0], info["loss_history"][-1]], "total_s": t_total}
     emu.mlp.close()
@@ -387,6 +505,18 @@ def main():
     # mixed precision: the headline
     dt_mix, kms_mix = timed(1, d_spec32, args.steps, args.warmup)
     mags_mix = d_mags.clone()
+    # the same batch through the drop-in host-array call (names[] / values[] from host memory, magnitudes back to the
+    # host, no spectra requested: spec = NULL), as a reference caller makes it
+    ctx.set_str("kernel_precision", "mixed")
+    names = synth.varied_names()
+    ctx.compute_csp_batch_params(prm, names, want=("mags",))
+    hsteps = max(3, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(hsteps):
+        ctx.compute_csp_batch_params(prm, names, want=("mags",))
+    dt_host = (time.perf_counter() - t0) / hsteps
+    ctx.set_str("kernel_precision", "double")
+    kernel_times(reset=1)
     value = n * args.steps / dt_mix
     # FP64 GPU path
     steps64 = max(2, args.steps // 2)
@@ -461,7 +591,7 @@ def main():
         ok = np.isfinite(r["mags"]) & np.isfinite(hm)
         br = np.min(np.where(ok, r["mags"], np.inf), axis=1, keepdims=True)
         ok_r = ok & (r["mags"] - br <= MAG_RANGE)
-        cpu = {"value": m1 * reps / dt, "unit": "models/s", "cores": O.num_threads(), "kind": "port",
+        cpu = {"value": m1 * reps / dt, "unit": "models/s", "cores": O.num_threads(), "kind": "port", "cpu": cpu_model(),
                "sample": f"the first {m1} models of the same batch x {reps} passes, FP64 oracle (csp_oracle.c, "
                          f"OpenMP, {O.num_threads()} threads), {dt:.1f} s",
                "mixed_vs_oracle_max_mag_err": float(np.max(np.abs(hm[ok_r] - r["mags"][ok_r]))) if ok_r.any() else None,
@@ -483,6 +613,9 @@ def main():
         "config": {"workload": "parametric CSP, 16384 models, 107 ages x 22 Z x 5994 px, delayed-tau, logzsol blend, "
                                "power-law dust, IGM, z U(0,4), LSF U(50,300) km/s, 100 Gaussian filters",
                    "n_models": n, "nspec": ns, "nt": nt, "nz": grid.nz, "nbands": nb, "seeds": [1, 7, 5]},
+        "host_api": {"models_per_s": n / dt_host, "ms_per_batch": 1e3 * dt_host,
+                     "call": "opensps_context_compute_csp_batch, mixed, host names/values in, host magnitudes out, "
+                             "spec NULL (parameter upload and magnitude download included)"},
         "fp64_value": value64,
         "fp64_ms_per_step": 1e3 * dt_64 / steps64,
         "speedup_vs_fp64": value / value64,
@@ -495,8 +628,10 @@ def main():
         "cpu_baseline": cpu,
         "setup_s": t_setup,
     }
+    ctx.close()
+    if not args.no_weights:
+        out["weight_matrix"] = weight_matrix_workload(args)
     if not args.no_operator:
-        ctx.close()
         out["operator"] = operator_workload(args)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_workload(args)

@@ -106,6 +106,9 @@ void opensps_context_set_sfh_tab(int handle, int ntab, const double *age, const 
                                  int *status);
 /* No reference counterpart: the context's bands on the model grid (nspec x nbands), normalised. */
 void opensps_context_get_bands(int handle, double *bands, int *status);
+/* No reference counterpart: the context's cosmology table, *nrow rows of redshift and luminosity distance (pc), the
+ * table the batch's distance term interpolates (cosmospl); z, dl may be NULL. */
+void opensps_context_get_cosmology(int handle, int *nrow, double *z, double *dl, int *status);
 
 /* ---- the batch (reference include/opensps.h:264-316) ---- */
 /* status 0, 1 bad handle / no basis, 2 unsupported (a parameter that changes the SSPs; imf1, imf2, imf3,

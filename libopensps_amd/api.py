@@ -47,6 +47,7 @@ def lib():
         L.opensps_context_get_nz.argtypes = [i, ip, ip]
         L.opensps_context_get_lambda.argtypes = [i, vp, ip]
         L.opensps_context_get_bands.argtypes = [i, vp, ip]
+        L.opensps_context_get_cosmology.argtypes = [i, ip, vp, vp, ip]
         L.opensps_context_get_params.argtypes = [i, vp, ip]
         L.opensps_context_get_filter_data.argtypes = [i, vp, vp, vp, ip]
         L.opensps_context_set_reference_spectra.argtypes = [i, i, vp, vp, i, i, vp, vp, i, ip]
@@ -225,6 +226,16 @@ class Context:
         lib().opensps_context_get_bands(self.handle, out.ctypes.data, C.byref(st))
         _check(st)
         return out
+
+    def cosmology(self):
+        """(z, dl): the context's cosmology table (redshift, luminosity distance in pc)."""
+        nr, st = C.c_int(), C.c_int()
+        lib().opensps_context_get_cosmology(self.handle, C.byref(nr), None, None, C.byref(st))
+        _check(st)
+        z, dl = np.zeros(nr.value), np.zeros(nr.value)
+        lib().opensps_context_get_cosmology(self.handle, C.byref(nr), z.ctypes.data, dl.ctypes.data, C.byref(st))
+        _check(st)
+        return z, dl
 
     def params(self):
         out = np.zeros(NPAR)

@@ -125,3 +125,36 @@ class MLP:
 
     def sync(self):
         H.check(lib().opensps_emu_sync(self._e))
+
+
+def _infer_main(argv=None):
+    """The inference kernel alone on a randomly initialised network (the serving shape: n_in features -> layers x
+    hidden -> n_out outputs), timed with the device synchronised around `reps` calls; for profiling runs."""
+    import argparse
+    import time
+
+    import torch
+    ap = argparse.ArgumentParser(description="emulator inference micro-benchmark")
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--n-in", type=int, default=8)
+    ap.add_argument("--n-out", type=int, default=228)
+    ap.add_argument("--hidden", type=int, default=512)
+    ap.add_argument("--layers", type=int, default=4)
+    a = ap.parse_args(argv)
+    mlp = MLP(a.n_in, a.n_out, a.hidden, a.layers, seed=3)
+    x = torch.rand((a.n, a.n_in), dtype=torch.float32, device="cuda")
+    out = mlp.predict(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.reps):
+        mlp.predict(x, out=out, sync=False)
+    mlp.sync()
+    dt = (time.perf_counter() - t0) / a.reps
+    flops = 2.0 * (a.n_in * a.hidden + (a.layers - 1) * a.hidden * a.hidden + a.hidden * a.n_out) * a.n
+    print(f"emulator inference: {a.n} models, {1e3 * dt:.3f} ms, {a.n / dt:.4g} models/s, {flops / dt / 1e12:.1f} TFLOP/s")
+    mlp.close()
+
+
+if __name__ == "__main__":
+    _infer_main()

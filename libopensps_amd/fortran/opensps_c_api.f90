@@ -841,4 +841,28 @@ contains
         status = st
     end subroutine
 
+    !> No reference counterpart: the context's cosmology table (redshift, luminosity distance in pc; nrow = 500 rows,
+    !> the table the batch's distance term interpolates, cosmospl, src/physics/opensps_cosmology.f90:288-300), for callers
+    !> that drive the device layer (libopensps_hip) themselves. z, dl may be NULL (nrow only).
+    subroutine opensps_context_get_cosmology(handle, nrow, z, dl, status) bind(C, name="opensps_context_get_cosmology")
+        integer(c_int), value :: handle
+        integer(c_int), intent(out) :: nrow, status
+        type(c_ptr), value :: z, dl
+        real(c_double), pointer :: oz(:), odl(:)
+        nrow = 0
+        if (bad_handle(handle, status)) return
+        call ctx_refresh_cosmology(int(handle))
+        associate(c => ctxs(handle))
+            nrow = size(c%cz)
+            if (c_associated(z)) then
+                call c_f_pointer(z, oz, [nrow])
+                oz = c%cz
+            end if
+            if (c_associated(dl)) then
+                call c_f_pointer(dl, odl, [nrow])
+                odl = c%cdl
+            end if
+        end associate
+    end subroutine
+
 end module opensps_c_api

@@ -5,6 +5,9 @@ Two reports, written as short Markdown files:
 * ``stats``: the per-kernel table of a ``rocprofv3 --kernel-trace --stats`` run (its SQLite results file):
   calls, total and average duration; with ``--by-launch`` one row per kernel and launch shape (a kernel that
   serves several workloads of the run, such as the products, then shows each apart).
+* ``counters``: SQ counters per kernel from several ``--pmc`` passes over the same program (CSV output): each
+  counter summed over its rows per dispatch, averaged over the kernel's dispatches, and per wavefront; with the
+  ratios the text quotes (wait and issue shares, MFMA busy share).
 * ``traffic``: the HBM bytes per launch of one kernel from two counter runs, ``--pmc FETCH_SIZE`` and
   ``--pmc WRITE_SIZE`` (CSV output, KiB per dispatch). Each counter is scaled by what it reads for a plain
   device-to-device copy of known size in the same run (the last two ``__amd_rocclr_copyBuffer`` dispatches,
@@ -88,6 +91,49 @@ def traffic(fetch_csv, write_csv, kernel, copy_bytes):
     return out
 
 
+def counters(csv_paths, kernels):
+    """{kernel: {counter: mean over dispatches of the per-dispatch sum}} for kernels whose name contains one of the
+    given substrings, over the --pmc CSV files of several passes."""
+    acc = {}
+    for path in csv_paths:
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                name = r["Kernel_Name"]
+                if not any(k in name for k in kernels):
+                    continue
+                d = acc.setdefault(name, {}).setdefault(r["Counter_Name"], {})
+                key = (path, r["Dispatch_Id"])
+                d[key] = d.get(key, 0.0) + float(r["Counter_Value"])
+    return {k: {c: statistics.mean(v.values()) for c, v in cs.items()} for k, cs in acc.items()}
+
+
+RATIOS = [("SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES", "SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES", 1.0),
+          ("SQ_WAIT_ANY / SQ_WAVE_CYCLES", "SQ_WAIT_ANY", "SQ_WAVE_CYCLES", 1.0),
+          ("SQ_ACTIVE_INST_VALU / SQ_BUSY_CU_CYCLES", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CU_CYCLES", 1.0),
+          ("SQ_VALU_MFMA_BUSY_CYCLES / (4 SQ_BUSY_CU_CYCLES)", "SQ_VALU_MFMA_BUSY_CYCLES", "SQ_BUSY_CU_CYCLES", 0.25)]
+
+
+def _write_counters(args):
+    res = counters(args.csv, args.kernel)
+    lines = [f"# {args.title}", "", "Commands (one pass a counter group):", "", "```", args.command, "```", "",
+             "Each counter: summed over its rows of a dispatch, averaged over the kernel's dispatches; per wavefront: "
+             "divided by SQ_WAVES.", ""]
+    for name in sorted(res, key=lambda k: -res[k].get("SQ_WAVES", 0)):
+        c = res[name]
+        waves = c.get("SQ_WAVES", 0.0)
+        lines += [f"## `{name[:100]}`", "", "| counter | per dispatch | per wavefront |", "|---|---|---|"]
+        for cn in sorted(c):
+            pw = f"{c[cn] / waves:.1f}" if waves else ""
+            lines.append(f"| {cn} | {c[cn]:.4g} | {pw} |")
+        lines += ["", "| ratio | value |", "|---|---|"]
+        for label, a, b, f in RATIOS:
+            if a in c and b in c and c[b] > 0:
+                lines.append(f"| {label} | {f * c[a] / c[b]:.3f} |")
+        lines.append("")
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def _write_stats(args):
     lines = [f"# {args.title}", "", "Command:", "", "```", args.command, "```", ""]
     if args.by_launch:
@@ -143,6 +189,12 @@ def main(argv=None):
     s.add_argument("--command", required=True)
     s.add_argument("--top", type=int, default=12)
     s.add_argument("--by-launch", action="store_true", help="one row per kernel and launch shape")
+    q = sub.add_parser("counters")
+    q.add_argument("--csv", nargs="+", required=True, help="counter_collection.csv of each pass")
+    q.add_argument("--kernel", nargs="+", required=True, help="substrings of the kernel names to report")
+    q.add_argument("--out", required=True)
+    q.add_argument("--title", default="SQ counters per kernel")
+    q.add_argument("--command", required=True)
     t = sub.add_parser("traffic")
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)
@@ -154,6 +206,8 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         _write_stats(args)
+    elif args.cmd == "counters":
+        _write_counters(args)
     else:
         _write_traffic(args)
 

@@ -313,3 +313,16 @@ def test_batch_ext_null_arguments():
     shim.shim_batch_ext(1, 0, 0, None, None, None, None, None, None, None, None, None, None, None, None, None, 0,
                         None, None, 1, C.byref(st))
     assert st.value == 1
+
+
+def test_cosmology_getter_matches_oracle():
+    """opensps_context_get_cosmology: the host's cosmology table (the one the distance term interpolates) equals the
+    oracle's, which is pinned by quadrature of 1/E(z); it follows set_float H0."""
+    with api.Context() as ctx:
+        z, dl = ctx.cosmology()
+        oz, odl = O.cosmology_table(72.0, 0.27, 0.73)
+        np.testing.assert_array_equal(z, oz)
+        np.testing.assert_allclose(dl, odl, rtol=1e-13, atol=0)
+        ctx.set_float("H0", 70.0)
+        z2, dl2 = ctx.cosmology()
+        np.testing.assert_allclose(dl2[1:], dl[1:] * 72.0 / 70.0, rtol=1e-12)
