@@ -183,7 +183,7 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
                   c->d_igm_a, c->d_igm64s, c->d_igm32s, c->d_basis64, c->d_A64, c->d_Ahi, c->d_Alo, c->d_bmeta, c->d_bc64,
                   c->d_bc32, c->d_mask, c->d_cz, c->d_cdl, c->yo.p, c->wcol.p, c->w64.p, c->order.p, c->tiles.p,
                   c->misc.p, c->spec.p, c->flag.p, c->d_rdlam, c->d_ltab32, c->d_wl32, c->d_sm32, c->d_bkt, c->hdr.p, c->wsfh.p,
-                  c->opu.p, c->opwc.p, c->opsum.p, c->opseg.p, c->opimf.p, c->opspart.p, c->d_bp_ptr, c->d_bp_band,
+                  c->opu.p, c->opwc.p, c->opsum.p, c->opseg.p, c->opimf.p, c->opspart.p, c->opgp.p, c->d_bp_ptr, c->d_bp_band,
                   c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p, c->gwb.p, c->d_magvega, c->d_tab, c->d_dcurve, c->d_dcurve32};
     for (void* p : ps) free_ptr(p);
     free_operator(c);
@@ -695,6 +695,8 @@ int opensps_hip_set_ssp_ops(opensps_hip_ctx_t* c, const opensps_hip_ssp_ops_t* o
     int npz = 1;
     for (int iz = 0; iz <= nz; iz++) zoff[iz] = o->age_ptr[iz * nt];
     for (int iz = 0; iz < nz; iz++) npz = std::max(npz, zoff[iz + 1] - zoff[iz]);
+    int npa_max = 1;
+    for (int a = 0; a < nz * nt; a++) npa_max = std::max(npa_max, o->age_ptr[a + 1] - o->age_ptr[a]);
     // entries by template per metallicity (CSR), in point order within a template, so sorted by age
     std::vector<int> tz_ptr((size_t)nz * (ntpl + 1), 0);
     for (int iz = 0; iz < nz; iz++)
@@ -748,6 +750,7 @@ int opensps_hip_set_ssp_ops(opensps_hip_ctx_t* c, const opensps_hip_ssp_ops_t* o
     c->op.npts = np;
     c->op.ntpl = ntpl;
     c->op.npz = npz;
+    c->op.npa_max = npa_max;
     // K padded to whole blocks of the fold (8 templates; a multiple of the f64 instruction's 4 and of 32 for bf16)
     c->op.k64 = ((ntpl + OPS_TPL_BLOCK - 1) / OPS_TPL_BLOCK) * OPS_TPL_BLOCK;
     c->op.kbf = ((ntpl + OPS_KSTEP_BF16 - 1) / OPS_KSTEP_BF16) * OPS_KSTEP_BF16;
@@ -906,12 +909,18 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     int* d_rank = misc + 4 * n + ncnt;
     int4* mtiles = (int4*)c->tiles.p + maxtiles + (uni ? optiles : 0);   // then the uniform ones
     const int npass = (ops && c->nz > 1) ? 2 : 1;
+    // the operator's weights, segments and fold take every column tile in one round (rounds of fewer tiles, their
+    // scratch reused while still in the last-level cache, measured slower: too few wavefronts a round)
+    const int chunk = ops ? std::max(1, std::min(c->op_chunk, maxtiles)) : 1;
+    const int ngroup = ops ? (c->op.npa_max + OPS_OP_GPTS - 1) / OPS_OP_GPTS : 1;
+    const size_t ngp = (size_t)npass * chunk * c->nt * ngroup * 64;
     if (ops) {
         if ((st = ops_reserve(c, c->wsfh, sizeof(double) * (size_t)n * c->nt))) return st;
-        if ((st = ops_reserve(c, c->opu, tsz * (size_t)npass * maxtiles * c->op.npz * 64))) return st;
+        if ((st = ops_reserve(c, c->opu, tsz * (size_t)npass * chunk * c->op.npz * 64))) return st;
         if ((st = ops_reserve(c, c->opwc, tsz * (size_t)npass * maxtiles * c->nt * 64))) return st;
         if ((st = ops_reserve(c, c->opsum, sizeof(double) * 2 * (size_t)npass * n * c->nt))) return st;
-        if ((st = ops_reserve(c, c->opseg, 2 * tsz * (size_t)npass * maxtiles * c->op.nseg_max * 64))) return st;
+        if ((st = ops_reserve(c, c->opseg, 2 * tsz * (size_t)npass * chunk * c->op.nseg_max * 64))) return st;
+        if ((st = ops_reserve(c, c->opgp, (2 * sizeof(double) + sizeof(int)) * ngp))) return st;
         if ((st = ops_reserve(c, c->opimf, sizeof(double) * OPS_IMFC_N * (size_t)n))) return st;
         if ((st = ops_reserve(c, c->opspart, (3 * sizeof(double) + sizeof(int)) * (size_t)npass * maxtiles * c->nt * 64)))
             return st;
@@ -1047,20 +1056,17 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         w.poly = c->op.d_poly;
         w.spart = (double*)c->opspart.p;
         w.scnt = (int*)(w.spart + 3 * (size_t)npass * maxtiles * c->nt * 64);
+        w.tile0 = 0;
+        w.utiles = chunk;
+        w.ngroup = ngroup;
+        w.u_f32 = mixed ? 1 : 0;
+        w.gsum = (double*)c->opgp.p;
+        w.gcnt = (int*)(w.gsum + 2 * ngp);
         if (a->op_count) {
             k_fill_i32<<<(unsigned)((2 * (size_t)n + 255) / 256), 256, 0, c->stream>>>(a->op_count, 2 * n, 0);
             HIPCHK(c, hipGetLastError());
         }
         k_op_imf<<<(n + 255) / 256, 256, 0, c->stream>>>(w);
-        HIPCHK(c, hipGetLastError());
-        dim3 gw((c->nt + 3) / 4, maxtiles, npass);
-        if (mixed) {
-            k_op_slow<float><<<gw, 256, 0, c->stream>>>(w);
-            k_op_weights<float><<<gw, 256, 0, c->stream>>>(w);
-        } else {
-            k_op_slow<double><<<gw, 256, 0, c->stream>>>(w);
-            k_op_weights<double><<<gw, 256, 0, c->stream>>>(w);
-        }
         HIPCHK(c, hipGetLastError());
         OpGArgs g;
         g.nt = c->nt;
@@ -1068,6 +1074,8 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         g.npz = c->op.npz;
         g.maxtiles = maxtiles;
         g.nseg_max = c->op.nseg_max;
+        g.tile0 = 0;
+        g.utiles = chunk;
         g.ntiles = d_ntiles;
         g.tiles = (const int4*)c->tiles.p;
         g.order = d_order;
@@ -1080,12 +1088,6 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         g.u = c->opu.p;
         g.wc = c->opwc.p;
         g.part = c->opseg.p;
-        dim3 gg((c->op.nseg_max + 3) / 4, maxtiles, npass);
-        if (mixed)
-            k_op_segments<float><<<gg, 256, 0, c->stream>>>(g);
-        else
-            k_op_segments<double><<<gg, 256, 0, c->stream>>>(g);
-        HIPCHK(c, hipGetLastError());
         OpFArgs f;
         f.ntpl = c->op.ntpl;
         f.npass = npass;
@@ -1093,6 +1095,8 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         f.nseg_max = c->op.nseg_max;
         f.kpad = kpad;
         f.parts = parts;
+        f.tile0 = 0;
+        f.utiles = chunk;
         f.ntiles = d_ntiles;
         f.tiles = (const int4*)c->tiles.p;
         f.order = d_order;
@@ -1101,12 +1105,30 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         f.whi = mixed ? (uint16_t*)c->wcol.p : nullptr;
         f.wlo = mixed ? (uint16_t*)c->wcol.p + (size_t)n * parts * kpad : nullptr;
         f.w64 = mixed ? nullptr : (double*)c->w64.p;
-        dim3 gf((kpad + 4 * OPS_TPL_BLOCK - 1) / (4 * OPS_TPL_BLOCK), maxtiles);
-        if (mixed)
-            k_op_fold<float><<<gf, 256, 0, c->stream>>>(f);
-        else
-            k_op_fold<double><<<gf, 256, 0, c->stream>>>(f);
-        HIPCHK(c, hipGetLastError());
+        const dim3 gw((c->nt + 3) / 4, chunk, npass), gwg((c->nt * ngroup + 3) / 4, chunk, npass);
+        const dim3 gg((c->op.nseg_max + 3) / 4, chunk, npass);
+        const dim3 gf((kpad + 4 * OPS_TPL_BLOCK - 1) / (4 * OPS_TPL_BLOCK), chunk);
+        // one round a chunk of column tiles: the off-path bins and remnants, the common bins by point groups, the
+        // ages' sums, the segments, the fold (each round's tiles past the batch's exit at once)
+        for (int t0 = 0; t0 < maxtiles; t0 += chunk) {
+            w.tile0 = g.tile0 = f.tile0 = t0;
+            if (mixed) {
+                k_op_slow<float><<<gw, 256, 0, c->stream>>>(w);
+                k_op_weights<float><<<gwg, 256, 0, c->stream>>>(w);
+            } else {
+                k_op_slow<double><<<gw, 256, 0, c->stream>>>(w);
+                k_op_weights<double><<<gwg, 256, 0, c->stream>>>(w);
+            }
+            k_op_agesums<<<gw, 256, 0, c->stream>>>(w);
+            if (mixed) {
+                k_op_segments<float><<<gg, 256, 0, c->stream>>>(g);
+                k_op_fold<float><<<gf, 256, 0, c->stream>>>(f);
+            } else {
+                k_op_segments<double><<<gg, 256, 0, c->stream>>>(g);
+                k_op_fold<double><<<gf, 256, 0, c->stream>>>(f);
+            }
+            HIPCHK(c, hipGetLastError());
+        }
         OpSArgs sa;
         sa.n = n;
         sa.nt = c->nt;
@@ -1116,7 +1138,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         sa.slum_p = slum_p;
         sa.mass = a->mass;
         sa.lbol = a->lbol;
-        k_op_sums<<<(n + 255) / 256, 256, 0, c->stream>>>(sa);
+        k_op_sums<<<(n + 3) / 4, 256, 0, c->stream>>>(sa);
         HIPCHK(c, hipGetLastError());
         k_op_tiles<<<1, 256, 0, c->stream>>>(n * parts, OPS_TILE_COL, d_optiles, (int4*)c->tiles.p + maxtiles);
         HIPCHK(c, hipGetLastError());

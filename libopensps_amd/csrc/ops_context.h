@@ -58,6 +58,7 @@ struct OpsCtx {
     struct OpsOperator {
         bool on = false;
         int nz = 0, nt = 0, npts = 0, ntpl = 0, npz = 0;   // npz: the most points of one metallicity
+        int npa_max = 1;                                   // the most points of one (metallicity, age)
         int k64 = 0, kbf = 0;                              // ntpl padded to the f64 / bf16 instruction K
         int *d_age_ptr = nullptr, *d_zoff = nullptr, *d_first = nullptr, *d_tz_ptr = nullptr;
         double *d_pt = nullptr, *d_age_mmax = nullptr;     // points: OPS_PT_N doubles each
@@ -113,6 +114,7 @@ struct OpsCtx {
     double tiny_logt = 0.0;
     bool f64_vector = false;   // FP64 age sums with vector FMAs (OPENSPS_F64_GEMM=vector)
     bool chain_two = false;    // the two-buffer chain always (OPENSPS_CHAIN=two)
+    int op_chunk = 1 << 30;    // column tiles the operator's weights, segments and fold take per round of launches
 
     // ---- batch scratch (grown on demand) ----
     OpsArena* arena = nullptr;   // when set, ops_reserve carves from it and never allocates
@@ -121,6 +123,7 @@ struct OpsCtx {
     OpsBuffer wsfh, opu, opwc, opsum, opseg;   // the operator path: SFH weights, point weights, age factors,
                                                // per-age sums, segment sums
     OpsBuffer opimf, opspart;                  // per-model IMF constants; the off-path bins' per-age sums and counts
+    OpsBuffer opgp;                            // per (pass, tile slot, age, point group): the weights' partial sums
     // ---- weights over ages and metallicities, and the reverse pass ----
     struct Full {
         int kfull = 0, k64 = 0, kbf = 0, nrt = 0;  // K = nt nz; padded to the products' K; row tiles of 64

@@ -119,8 +119,13 @@ struct alignas(64) OpsWPoint {
     double pad1, pad2;
 };
 
-struct OpWArgs {           // k_op_weights
+#define OPS_OP_GPTS (4 * OPS_PIECE)   // points of one age a wavefront of k_op_weights takes (whole pieces)
+
+struct OpWArgs {           // k_op_weights, k_op_slow, k_op_agesums
     int n, nt, npass, npz, maxtiles, imf_type, remnants;
+    int tile0, utiles;     // the round's first column tile; the tile slots of u and gsum (tile - tile0)
+    int ngroup;            // point groups of OPS_OP_GPTS per age
+    int u_f32;             // u and wc hold floats (mixed precision) or doubles
     double salp_ind, mdave, mlim_bh, mlim_ns;
     const int* ntiles;
     const int4* tiles;
@@ -128,7 +133,8 @@ struct OpWArgs {           // k_op_weights
     const double *prm, *hdr, *wsfh;
     const int *age_ptr, *zoff, *first;
     const double *pt, *age_mmax;
-    void *u, *wc;          // per (pass, tile): the points' weights (npz x 64), the ages' SFH x blend factors (nt x 64)
+    void *u, *wc;          // per (pass, tile slot): the points' weights (npz x 64); per (pass, tile): the ages' SFH x
+                           // blend factors (nt x 64)
     double *smass_p, *slum_p;   // (pass, model, age)
     int* count;                 // (2, n) points with a nonzero weight per pass, or NULL
     const OpsWPoint* wp;        // per point: k_op_weights' record for the context's IMF type
@@ -137,6 +143,8 @@ struct OpWArgs {           // k_op_weights
     const int *slow_ptr, *slow_pt;   // per (metallicity, age) the bins that leave the common path
     double* spart;              // (pass, tile, age): their sums and the remnants, 3 x 64 per wavefront
     int* scnt;                  // (pass, tile, age): their nonzero weights, 64 per wavefront
+    double* gsum;               // (pass, tile slot, age, group): the common bins' mass and L_bol sums, 2 x 64
+    int* gcnt;                  // (pass, tile slot, age, group): their nonzero weights, 64
 };
 // The power-law segment of a bin [lo, hi] (imf_bin_weights_tab's classification, :483-498): 0 when the bin
 // crosses a break or lies below 0.08 M_sun (the general integral then)
@@ -161,6 +169,7 @@ __host__ __device__ inline int bin_class(int t, double lo, double hi, double mda
 
 struct OpGArgs {           // k_op_segments: one segment's entries per wavefront
     int nt, npass, npz, maxtiles, nseg_max;
+    int tile0, utiles;     // the round's first column tile; the tile slots of u and part
     const int* ntiles;
     const int4* tiles;
     const int *order, *it;
@@ -175,6 +184,7 @@ struct OpGArgs {           // k_op_segments: one segment's entries per wavefront
 
 struct OpFArgs {           // k_op_fold: per template the sums of its segments, as the product's B operand
     int ntpl, npass, maxtiles, nseg_max, kpad, parts;
+    int tile0, utiles;     // the round's first column tile; the tile slots of part
     const int* ntiles;
     const int4* tiles;
     const int* order;
@@ -297,6 +307,7 @@ __global__ void k_op_weights(OpWArgs a);
 template <typename T>
 __global__ void k_op_slow(OpWArgs a);
 __global__ void k_op_imf(OpWArgs a);
+__global__ void k_op_agesums(OpWArgs a);
 template <typename T>
 __global__ void k_op_segments(OpGArgs a);
 template <typename T>

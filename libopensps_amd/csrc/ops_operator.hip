@@ -253,7 +253,7 @@ __global__ void k_op_imf(OpWArgs a) {
 template <typename T>
 __global__ __launch_bounds__(256) void k_op_slow(OpWArgs a) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int t = blockIdx.x * 4 + wv, tile = blockIdx.y, pass = blockIdx.z;
+    const int t = blockIdx.x * 4 + wv, tile = a.tile0 + blockIdx.y, pass = blockIdx.z;
     if (t >= a.nt || tile >= *a.ntiles) return;
     const int4 td = a.tiles[tile];
     const int iz = a.npass == 1 ? 0 : td.x + pass;
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) void k_op_slow(OpWArgs a) {
     const ImfP p = model_imf(a, m);
     const double inv_total = a.impc[(size_t)m * OPS_IMFC_N + OPS_IMFC_INV_TOTAL];
     const int zo = a.zoff[iz];
-    T* u = (T*)a.u + ((size_t)pass * a.maxtiles + tile) * a.npz * 64 + lane;
+    T* u = (T*)a.u + ((size_t)pass * a.utiles + blockIdx.y) * a.npz * 64 + lane;
     double sm = 0.0, sl = 0.0, sq = 0.0;
     int cnt = 0;
     const int j0 = a.slow_ptr[iz * nt + t], j1 = a.slow_ptr[iz * nt + t + 1];
@@ -302,8 +302,22 @@ __device__ __forceinline__ double d_expm1_over_y(double y, const double* pc) {
 typedef __attribute__((address_space(4))) const double* OpsDoubleC;
 typedef __attribute__((address_space(4))) const OpsWPoint* OpsWPointC;
 
+// A point record through scalar loads, field by field
+__device__ __forceinline__ OpsWPoint wp_load(OpsWPointC p) {
+    OpsWPoint r;
+    r.mini = p->mini;
+    r.llo = p->llo;
+    r.dl = p->dl;
+    r.cm = p->cm;
+    r.cl = p->cl;
+    r.code = p->code;
+    return r;
+}
+
 // ------------------------------------------------------------------------------------------------
-// k_op_weights: grid (ceil(nt / 4), max tiles, passes), 256 threads = 4 ages. Pass 0 takes the operator of
+// k_op_weights: grid (ceil(nt ngroup / 4), the round's tiles, passes), 256 threads = 4 (age, point group) pairs:
+// a wavefront takes OPS_OP_GPTS consecutive points of one age (whole pieces of 32, where the chain of shared edges
+// restarts anyway), so the weights of one round of tiles still fill the GPU. Pass 0 takes the operator of
 // metallicity zlo with weight 1 - dz, pass 1 that of zlo + 1 with weight dz (one pass with one metallicity).
 // The common bins of an age (not its first, inside one power-law segment, not empty): a point's record is the
 // same for every lane (the lanes are models), so the records come through scalar loads and the segment and the
@@ -316,7 +330,8 @@ typedef __attribute__((address_space(4))) const OpsWPoint* OpsWPointC;
 template <typename T>
 __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int t = blockIdx.x * 4 + wv, tile = blockIdx.y, pass = blockIdx.z;
+    const int wi = blockIdx.x * 4 + wv, tile = a.tile0 + blockIdx.y, pass = blockIdx.z;
+    const int t = wi / a.ngroup, grp = wi - t * a.ngroup;
     if (t >= a.nt || tile >= *a.ntiles) return;
     const int4 td = a.tiles[tile];   // {metallicity pair (zlo - 1), first column, columns}
     const int iz = a.npass == 1 ? 0 : __builtin_amdgcn_readfirstlane(td.x) + pass;
@@ -327,12 +342,11 @@ __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
     const double* ic = a.impc + (size_t)m * OPS_IMFC_N;
     const double e1 = ic[OPS_IMFC_E], e2 = ic[OPS_IMFC_E + 1], e3 = ic[OPS_IMFC_E + 2];
     const double c1 = ic[OPS_IMFC_CT], c2 = ic[OPS_IMFC_CT + 1], c3 = ic[OPS_IMFC_CT + 2];
-    const double dz = a.hdr[(size_t)m * OPS_HDR_N + OPS_HDR_DZ];
-    const double cz = a.npass == 1 ? 1.0 : (pass == 0 ? 1.0 - dz : dz);
-    const int k0 = __builtin_amdgcn_readfirstlane(a.age_ptr[iz * nt + t]);
-    const int k1 = __builtin_amdgcn_readfirstlane(a.age_ptr[iz * nt + t + 1]);
+    const int ka = __builtin_amdgcn_readfirstlane(a.age_ptr[iz * nt + t]);
+    const int kb = __builtin_amdgcn_readfirstlane(a.age_ptr[iz * nt + t + 1]);
+    const int k0 = min(kb, ka + grp * OPS_OP_GPTS), k1 = min(kb, k0 + OPS_OP_GPTS);
     const int zo = __builtin_amdgcn_readfirstlane(a.zoff[iz]);
-    T* u = (T*)a.u + ((size_t)pass * a.maxtiles + tile) * a.npz * 64 + lane;
+    T* u = (T*)a.u + ((size_t)pass * a.utiles + blockIdx.y) * a.npz * 64 + lane;
     const OpsWPointC wp = (OpsWPointC)a.wp;
     const OpsDoubleC pcs = (OpsDoubleC)a.poly;
     double pc[13];
@@ -340,34 +354,59 @@ __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
     for (int j = 0; j < 13; j++) pc[j] = pcs[j];
     double sm = 0.0, sl = 0.0, phi = 1.0;
     int cnt = 0;
-    int k = k0;
-    while (k < k1) {   // runs of one power-law segment
-        const int seg = wp[k].code & 3;
-        if (seg == 0) {   // k_op_slow's bin
-            k++;
-            continue;
-        }
+    // one point a step; the next point's record is fetched (scalar loads) before this point's arithmetic, so its
+    // latency hides behind it
+    OpsWPoint nxt;
+    if (k0 < k1) nxt = wp_load(wp + k0);
+    for (int k = k0; k < k1; k++) {
+        const OpsWPoint r = nxt;
+        if (k + 1 < k1) nxt = wp_load(wp + k + 1);
+        const int seg = r.code & 3;
+        if (seg == 0) continue;   // k_op_slow's bin
         const double e = seg == 1 ? e1 : (seg == 2 ? e2 : e3);
         const double c = seg == 1 ? c1 : (seg == 2 ? c2 : c3);
-        for (; k < k1; k++) {
-            const int code = wp[k].code;
-            if ((code & 3) == 0) continue;
-            if ((code & 3) != seg) break;
-            const double dl = wp[k].dl;
-            const double plo = (code & 4) ? phi : exp(e * wp[k].llo);
-            const double y = e * dl;
-            double s = d_expm1_over_y(y, pc);
-            if (!(fabs(y) < 0.25)) s = (exp(y) - 1.0) / y;
-            phi = fma(plo, y * s, plo);
-            const double mini = wp[k].mini;
-            const double w = (mini < lower || mini > upper) ? 0.0 : c * dl * plo * s;
-            u[(size_t)(k - zo) * 64] = (T)w;
-            cnt += w > 0.0 ? 1 : 0;
-            sm = fma(wp[k].cm, w, sm);
-            sl = fma(wp[k].cl, w, sl);
-        }
+        const double dl = r.dl;
+        const double plo = (r.code & 4) ? phi : exp(e * r.llo);
+        const double y = e * dl;
+        double sx = d_expm1_over_y(y, pc);
+        if (!(fabs(y) < 0.25)) sx = (exp(y) - 1.0) / y;
+        phi = fma(plo, y * sx, plo);
+        const double w = (r.mini < lower || r.mini > upper) ? 0.0 : c * dl * plo * sx;
+        u[(size_t)(k - zo) * 64] = (T)w;
+        cnt += w > 0.0 ? 1 : 0;
+        sm = fma(r.cm, w, sm);
+        sl = fma(r.cl, w, sl);
     }
-    // the slow bins' and the remnants' sums (k_op_slow)
+    // the group's sums (k_op_agesums adds the groups in order, then k_op_slow's)
+    const size_t rg = (((size_t)pass * a.utiles + blockIdx.y) * nt + t) * a.ngroup + grp;
+    a.gsum[rg * 2 * 64 + lane] = sm;
+    a.gsum[rg * 2 * 64 + 64 + lane] = sl;
+    a.gcnt[rg * 64 + lane] = cnt;
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_op_agesums: grid (ceil(nt / 4), the round's tiles, passes), a wavefront per (age, tile, pass): the age's mass and
+// L_bol sums from its point groups (in order) and the slow bins with the remnants (k_op_slow), times the blend
+// factor, the count of points with a nonzero weight, and the age's SFH x blend factor for the segments.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_op_agesums(OpWArgs a) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int t = blockIdx.x * 4 + wv, tile = a.tile0 + blockIdx.y, pass = blockIdx.z;
+    if (t >= a.nt || tile >= *a.ntiles) return;
+    const int4 td = a.tiles[tile];
+    const bool valid = lane < td.z;
+    const int m = a.order[td.y + (valid ? lane : 0)];
+    const int n = a.n, nt = a.nt;
+    const double dz = a.hdr[(size_t)m * OPS_HDR_N + OPS_HDR_DZ];
+    const double cz = a.npass == 1 ? 1.0 : (pass == 0 ? 1.0 - dz : dz);
+    double sm = 0.0, sl = 0.0;
+    int cnt = 0;
+    const size_t rg0 = (((size_t)pass * a.utiles + blockIdx.y) * nt + t) * a.ngroup;
+    for (int g = 0; g < a.ngroup; g++) {
+        sm = sm + a.gsum[(rg0 + g) * 2 * 64 + lane];
+        sl = sl + a.gsum[(rg0 + g) * 2 * 64 + 64 + lane];
+        cnt += a.gcnt[(rg0 + g) * 64 + lane];
+    }
     const size_t ra = ((size_t)pass * a.maxtiles + tile) * nt + t;
     const double* sp = a.spart + ra * 3 * 64 + lane;
     sm = sm + sp[0];
@@ -380,18 +419,24 @@ __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
         if (a.count && cnt && cz != 0.0) atomicAdd(a.count + 2 * m + pass, cnt);
     }
     const double ws = a.wsfh[(size_t)m * nt + t];
-    T* wc = (T*)a.wc + (ra * 64) + lane;
-    *wc = (T)((ws > OPS_SAFE_FLOOR ? ws : 0.0) * cz);
+    double* wc64 = (double*)a.wc;
+    float* wc32 = (float*)a.wc;
+    if (a.u_f32)
+        wc32[ra * 64 + lane] = (float)((ws > OPS_SAFE_FLOOR ? ws : 0.0) * cz);
+    else
+        wc64[ra * 64 + lane] = (ws > OPS_SAFE_FLOOR ? ws : 0.0) * cz;
 }
 #pragma clang fp contract(off)
 
-// k_op_sums: one thread per model; the passes' per-age sums, then model_sums (MSUMS_OWN)
-__global__ void k_op_sums(OpSArgs a) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+// k_op_sums: a wavefront per model (4 a workgroup): each lane the passes' sums of its ages, then model_sums
+// (MSUMS_OWN) over the lanes
+__global__ __launch_bounds__(256) void k_op_sums(OpSArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (m >= a.n) return;
     const int nt = a.nt, n = a.n;
     double ms = 0.0, ls = 0.0;
-    for (int t = 0; t < nt; t++) {
+    for (int t = lane; t < nt; t += 64) {
         double smass = 0.0, slog = 0.0;
         for (int pass = 0; pass < a.npass; pass++) {
             const size_t r = ((size_t)pass * n + m) * nt + t;
@@ -404,8 +449,14 @@ __global__ void k_op_sums(OpSArgs a) {
             ls = ls + w * pow(10.0, slog);
         }
     }
-    if (a.mass) a.mass[m] = ms;
-    if (a.lbol) a.lbol[m] = ls > OPS_SAFE_FLOOR ? log10(ls) : 0.0;
+    for (int o = 32; o > 0; o >>= 1) {
+        ms += __shfl_xor(ms, o, 64);
+        ls += __shfl_xor(ls, o, 64);
+    }
+    if (lane == 0) {
+        if (a.mass) a.mass[m] = ms;
+        if (a.lbol) a.lbol[m] = ls > OPS_SAFE_FLOOR ? log10(ls) : 0.0;
+    }
 }
 
 #pragma clang fp contract(on)
@@ -425,7 +476,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_op_segments(OpGArgs a) {
     typedef typename Pair<T>::type T2;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int sg = blockIdx.x * 4 + wv, tile = blockIdx.y, pass = blockIdx.z;
+    const int sg = blockIdx.x * 4 + wv, tile = a.tile0 + blockIdx.y, pass = blockIdx.z;
     if (tile >= *a.ntiles) return;
     const int4 td = a.tiles[tile];
     const int iz = a.npass == 1 ? 0 : td.x + pass;
@@ -434,7 +485,7 @@ __global__ __launch_bounds__(256) void k_op_segments(OpGArgs a) {
     const int m = a.order[td.y + (valid ? lane : 0)];
     const int it = a.it[m];
     const int2 rng = a.seg[a.segoff[iz] + sg];
-    const T* u = (const T*)a.u + ((size_t)pass * a.maxtiles + tile) * a.npz * 64 + lane;
+    const T* u = (const T*)a.u + ((size_t)pass * a.utiles + blockIdx.y) * a.npz * 64 + lane;
     const T* wc = (const T*)a.wc + ((size_t)pass * a.maxtiles + tile) * a.nt * 64 + lane;
     // lane l holds entry rng.x + l (a segment has at most 64), then the entries go round by broadcast
     int2 me = make_int2(0, 0);
@@ -473,7 +524,7 @@ __global__ __launch_bounds__(256) void k_op_segments(OpGArgs a) {
     T2 r;
     r.x = y;
     r.y = o;
-    ((T2*)a.part)[(((size_t)pass * a.maxtiles + tile) * a.nseg_max + sg) * 64 + lane] = r;
+    ((T2*)a.part)[(((size_t)pass * a.utiles + blockIdx.y) * a.nseg_max + sg) * 64 + lane] = r;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -486,7 +537,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_op_fold(OpFArgs a) {
     typedef typename Pair<T>::type T2;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int e0 = (blockIdx.x * 4 + wv) * OPS_TPL_BLOCK, tile = blockIdx.y;
+    const int e0 = (blockIdx.x * 4 + wv) * OPS_TPL_BLOCK, tile = a.tile0 + blockIdx.y;
     if (e0 >= a.kpad || tile >= *a.ntiles) return;
     const int4 td = a.tiles[tile];
     const bool valid = lane < td.z;
@@ -497,7 +548,7 @@ __global__ __launch_bounds__(256) void k_op_fold(OpFArgs a) {
     for (int pass = 0; pass < a.npass; pass++) {
         const int iz = a.npass == 1 ? 0 : td.x + pass;
         const int* ts = a.tseg + (size_t)iz * (a.ntpl + 1);
-        const T2* P = (const T2*)a.part + ((size_t)pass * a.maxtiles + tile) * a.nseg_max * 64 + lane;
+        const T2* P = (const T2*)a.part + ((size_t)pass * a.utiles + blockIdx.y) * a.nseg_max * 64 + lane;
 #pragma unroll
         for (int q = 0; q < OPS_TPL_BLOCK; q++) {
             const int e = e0 + q;
