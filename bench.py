@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--adjoint-sfh", action="store_true",
                     help="also the batch adjoint with SFH parameters (params_bar's SFH rows: the SFH tangents)")
     ap.add_argument("--emu-train", type=int, default=4_000_000, help="emulator training spectra")
-    ap.add_argument("--emu-epochs", type=int, default=30)
+    ap.add_argument("--emu-epochs", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a 256 MiB device-to-device copy (the calibration of the HBM counter runs)")
     return ap.parse_args()
@@ -397,7 +397,7 @@ def emulator_workload(args, csp_models_per_s):
     dctx.set_cosmology(cz, cdl)
     dctx.set_options(1, 1)
     emu = T.Emulator.train(dctx, (cz, cdl), n_train=args.emu_train, n_pca_fit=200_000, n_pca=128,
-                           epochs=args.emu_epochs, batch=8192, lr=1e-3, hidden=512, layers=4)
+                           epochs=args.emu_epochs, batch=8192, lr=2e-3, hidden=512, layers=4)
     t_total = time.perf_counter() - t0
     val = T.validate(emu, dctx, n=16384)
     # inference on 1M models

@@ -60,14 +60,16 @@ typedef struct {
 
 /* The reverse pass of a batch (compute_csp_batch with params_bar / weights_bar / weights_z_bar, include/opensps.h:
  * 324-348; batch_backward, opensps_csp_batch.f90:4713-5136): the forward batch fwd (its outputs are written as
- * opensps_hip_csp_batch writes them; fwd.use_ops must be 0), then the derivatives of the loss whose cotangents are
+ * opensps_hip_csp_batch writes them), then the derivatives of the loss whose cotangents are
  * given (device arrays, any NULL): mags_bar (nbands, n), spec_bar (nspec, n), mass_bar, lbol_bar (n). Outputs
  * (device, any NULL): params_bar (OPENSPS_HIP_NBGRAD, n) with the rows of the reference's BGRAD_* numbering
  * (opensps_csp_batch.f90:166-213) that this path computes (dust1, dust2, dust3, dust_index, dust1_index,
  * frac_obrun, frac_nodust, logzsol, zred, sigma_smooth, igm_factor and, with SFH parameters (no given weights),
  * the SFH rows tage, tau, const, fburst, tburst, sf_start, sf_trunc, sf_slope of batch_sfh_gradients
- * (opensps_csp_batch.f90:5198-5217, SFHT_ROWS :215; sfh 0, 1, 4, 5); the others are 0), weights_bar (nt, n) (with
- * fwd.ssp_weights or SFH parameters), weights_z_bar (nt, nz, n) (with fwd.ssp_weights_z). */
+ * (opensps_csp_batch.f90:5198-5217, SFHT_ROWS :215; sfh 0, 1, 4, 5) and, with fwd.use_ops (the operator's reverse
+ * pass, ssp_batch_age_sums_adjoint, opensps_ssp_batch.f90:991-1248), the IMF rows imf1, imf2 and imf3 (Kroupa) or
+ * imf1, imf2 and mdave (Dave), none for Salpeter; the others are 0), weights_bar (nt, n) (with fwd.ssp_weights or
+ * SFH parameters), weights_z_bar (nt, nz, n) (with fwd.ssp_weights_z, which fwd.use_ops excludes: status 2). */
 #define OPENSPS_HIP_NBGRAD 44
 typedef struct {
     opensps_hip_batch_t fwd;

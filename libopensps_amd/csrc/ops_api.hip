@@ -184,7 +184,8 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
                   c->d_bc32, c->d_mask, c->d_cz, c->d_cdl, c->yo.p, c->wcol.p, c->w64.p, c->order.p, c->tiles.p,
                   c->misc.p, c->spec.p, c->flag.p, c->d_rdlam, c->d_ltab32, c->d_wl32, c->d_sm32, c->d_bkt, c->hdr.p, c->wsfh.p,
                   c->opu.p, c->opwc.p, c->opsum.p, c->opseg.p, c->opimf.p, c->opspart.p, c->opgp.p, c->d_bp_ptr, c->d_bp_band,
-                  c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p, c->gwb.p, c->d_magvega, c->d_tab, c->d_dcurve, c->d_dcurve32};
+                  c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p, c->gwb.p, c->d_magvega, c->d_tab, c->d_dcurve, c->d_dcurve32,
+                  c->opsraw.p, c->opct.p, c->oppart.p, c->opimfd.p};
     for (void* p : ps) free_ptr(p);
     free_operator(c);
     free_full(c);
@@ -512,7 +513,7 @@ static void free_operator(OpsCtx* c) {
     void* ps[] = {c->op.d_age_ptr, c->op.d_zoff, c->op.d_first, c->op.d_tz_ptr, c->op.d_pt, c->op.d_age_mmax,
                   c->op.d_te, c->op.d_te_val64, c->op.d_te_val32, c->op.d_A64, c->op.d_Ahi, c->op.d_Alo,
                   c->op.d_seg, c->op.d_segoff, c->op.d_tseg, c->op.d_slow_ptr, c->op.d_slow_pt, c->op.d_wp,
-                  c->op.d_poly};
+                  c->op.d_poly, c->op.d_pent, c->op.d_pval, c->op.d_hpoly, c->op.d_At64, c->op.d_Ath, c->op.d_Atl};
     for (void* p : ps) free_ptr(p);
     c->op = OpsCtx::OpsOperator();
 }
@@ -796,6 +797,26 @@ int opensps_hip_set_ssp_ops(opensps_hip_ctx_t* c, const opensps_hip_ssp_ops_t* o
         k_pack_bf16<<<(unsigned)((nbf + 255) / 256), 256, 0, c->stream>>>(pa);
         HIPCHK(c, hipGetLastError());
     }
+    // the reverse pass's operands: the library as rows (template, pixel) in FP64, each point's entries, h's series
+    c->op.nent = ne;
+    c->op.nrt = (ntpl + 63) / 64;
+    {
+        const size_t nel = (size_t)c->op.nrt * 64 * c->nsp;
+        HIPCHK(c, hipMalloc((void**)&c->op.d_At64, nel * sizeof(double)));
+        k_pack_t<<<(unsigned)((nel + 255) / 256), 256, 0, c->stream>>>(nspec, c->nsp, ntpl, c->op.nrt * 64, lib,
+                                                                      c->op.d_At64, nullptr, nullptr);
+        HIPCHK(c, hipGetLastError());
+    }
+    UPLOAD(c, c->op.d_pent, o->ent_tpl, (size_t)np * ne);
+    UPLOAD(c, c->op.d_pval, o->ent_val, (size_t)np * ne);
+    {
+        double hp[13], f = 1.0;
+        for (int j = 0; j < 13; j++) {
+            if (j > 0) f *= (double)j;
+            hp[j] = 1.0 / (f * (j + 2));
+        }
+        UPLOAD(c, c->op.d_hpoly, hp, 13);
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipFree(lib));
     c->op.h_age_ptr.assign(o->age_ptr, o->age_ptr + (size_t)nz * nt + 1);
@@ -1062,6 +1083,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         w.u_f32 = mixed ? 1 : 0;
         w.gsum = (double*)c->opgp.p;
         w.gcnt = (int*)(w.gsum + 2 * ngp);
+        w.sraw = c->op_sraw_out;
         if (a->op_count) {
             k_fill_i32<<<(unsigned)((2 * (size_t)n + 255) / 256), 256, 0, c->stream>>>(a->op_count, 2 * n, 0);
             HIPCHK(c, hipGetLastError());
@@ -1138,6 +1160,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         sa.slum_p = slum_p;
         sa.mass = a->mass;
         sa.lbol = a->lbol;
+        sa.lsum = c->lsum_out;
         k_op_sums<<<(n + 3) / 4, 256, 0, c->stream>>>(sa);
         HIPCHK(c, hipGetLastError());
         k_op_tiles<<<1, 256, 0, c->stream>>>(n * parts, OPS_TILE_COL, d_optiles, (int4*)c->tiles.p + maxtiles);
@@ -1298,7 +1321,9 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     if (!g) return ops_fail(c, 3, "batch_grad: NULL arguments");
     const opensps_hip_batch_t& f = g->fwd;
     const int n = f.n;
-    if (f.use_ops) return ops_fail(c, 2, "batch_grad: the SSP operator's reverse pass (the IMF rows) is not built");
+    const bool ops = f.use_ops != 0;
+    if (ops && !c->op.on) return ops_fail(c, 1, "batch_grad: no SSP operator installed (opensps_hip_set_ssp_ops)");
+    if (ops && f.ssp_weights_z) return ops_fail(c, 2, "batch_grad: ssp_weights_z with the SSP operator");
     if (c->dcurve_on) return ops_fail(c, 2, "batch_grad: the reverse pass with a tabulated dust curve is not built");
     if (g->weights_z_bar && !f.ssp_weights_z) return ops_fail(c, 3, "batch_grad: weights_z_bar needs ssp_weights_z");
     if (n < 0) return ops_fail(c, 3, "batch_grad: n < 0");
@@ -1310,8 +1335,27 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     const int parts = 2, nsp = c->nsp, nt = c->nt, nz = c->nz;
     const bool wz = f.ssp_weights_z != nullptr;
     int st;
-    if ((st = ensure_full(c, mixed, true))) return st;
-    const int kfull = c->full.kfull, nrt = c->full.nrt, ldc = nrt * 64, ncols = n * parts;
+    if (!ops && (st = ensure_full(c, mixed, true))) return st;
+    if (ops && mixed && !c->op.d_Ath) {
+        // the library's rows in split BF16 (from the FP64 rows packed at set_ssp_ops), once
+        const size_t nel = (size_t)c->op.nrt * 64 * nsp;
+        HIPCHK(c, hipMalloc((void**)&c->op.d_Ath, nel * sizeof(uint16_t)));
+        HIPCHK(c, hipMalloc((void**)&c->op.d_Atl, nel * sizeof(uint16_t)));
+        k_pack_t<<<(unsigned)((nel + 255) / 256), 256, 0, c->stream>>>(nsp, nsp, c->op.ntpl, c->op.nrt * 64,
+                                                                      c->op.d_At64, nullptr, c->op.d_Ath, c->op.d_Atl);
+        HIPCHK(c, hipGetLastError());
+    }
+    // the transposed product's rows: every (age, metallicity) of the basis, or the operator's templates
+    const int kfull = ops ? c->op.ntpl : c->full.kfull, nrt = ops ? c->op.nrt : c->full.nrt, ldc = nrt * 64;
+    const int ncols = n * parts;
+    const int npass = (ops && nz > 1) ? 2 : 1;
+    const size_t tsz = mixed ? sizeof(float) : sizeof(double);
+    if (ops) {
+        if ((st = ops_reserve(c, c->opsraw, sizeof(double) * 2 * (size_t)npass * n * nt))) return st;
+        if ((st = ops_reserve(c, c->opct, tsz * 2 * (size_t)c->op.ntpl * n))) return st;
+        if ((st = ops_reserve(c, c->oppart, sizeof(double) * 4 * (size_t)npass * nt * n))) return st;
+        if ((st = ops_reserve(c, c->opimfd, sizeof(double) * OPS_IMFD_N * (size_t)n))) return st;
+    }
     if ((st = ops_reserve(c, c->glsum, sizeof(double) * (size_t)n))) return st;
     if ((st = ops_reserve(c, c->gfband, sizeof(double) * (size_t)n * (c->nbands > 0 ? c->nbands : 1)))) return st;
     if ((st = ops_reserve(c, c->gw, sizeof(double) * (size_t)n * nt))) return st;
@@ -1328,8 +1372,10 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     if (!fw.fband) fw.fband = (double*)c->gfband.p;
     if (!fw.weights) fw.weights = (double*)c->gw.p;
     c->lsum_out = (double*)c->glsum.p;
+    c->op_sraw_out = ops ? (double*)c->opsraw.p : nullptr;
     st = opensps_hip_csp_batch(c, &fw);
     c->lsum_out = nullptr;
+    c->op_sraw_out = nullptr;
     if (st) return st;
     int* misc = (int*)c->misc.p;
     const int *d_zlo = misc, *d_it = misc + n;
@@ -1379,7 +1425,6 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     ga.ybh = c->gyb.p;
     ga.ybl = mixed ? (void*)((uint16_t*)c->gyb.p + (size_t)ncols * nsp) : nullptr;
     ga.gout = (double*)c->gout.p;
-    const size_t tsz = mixed ? sizeof(float) : sizeof(double);
     const size_t lds = 3 * tsz * (size_t)nsp + sizeof(double) * ((c->nbands > 0 ? c->nbands : 1) + 8 * 16);
     const void* kfn = mixed ? (const void*)k_grad_chain<float> : (const void*)k_grad_chain<double>;
     HIPCHK(c, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1398,7 +1443,7 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     gt.nct = (ncols + 63) / 64;
     gt.nsp = nsp;
     gt.ldc = ldc;
-    gt.A64 = c->full.At64;
+    gt.A64 = ops ? c->op.d_At64 : c->full.At64;
     gt.B64 = (const double*)c->gyb.p;
     gt.C = c->gC.p;
     if (mixed) {
@@ -1414,8 +1459,8 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
         mm.tiles = nullptr;
         mm.ncols = ncols;
         mm.nct = (ncols + OPS_MM_COLS - 1) / OPS_MM_COLS;
-        mm.Ah = c->full.Ath;
-        mm.Al = c->full.Atl;
+        mm.Ah = ops ? c->op.d_Ath : c->full.Ath;
+        mm.Al = ops ? c->op.d_Atl : c->full.Atl;
         mm.slab = 0;
         mm.Bh = (const uint16_t*)ga.ybh;
         mm.Bl = (const uint16_t*)ga.ybl;
@@ -1429,6 +1474,75 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
         HIPCHK(c, hipGetLastError());
     }
     if (c->profiling) HIPCHK(c, hipEventRecord(c->gev[2], c->stream));
+
+    double* wbar_out = g->weights_bar ? g->weights_bar : (sfh_rows ? (double*)c->gwb.p : nullptr);
+    if (ops) {
+        // the operator's reverse pass: the template cotangents by position, the IMF constants' derivatives, the
+        // points' sums per (pass, age, model), then w_bar, the metallicity's and the IMF's rows and params_bar
+        OpAdjArgs q;
+        q.n = n;
+        q.nt = nt;
+        q.nz = nz;
+        q.npass = npass;
+        q.ntpl = c->op.ntpl;
+        q.nent = c->op.nent;
+        q.imf_type = c->imf_type;
+        q.remnants = c->remnants;
+        q.salp_ind = c->salp_ind;
+        q.mdave = c->mdave;
+        q.mlim_bh = c->mlim_bh;
+        q.mlim_ns = c->mlim_ns;
+        q.ntiles = misc + 4 * n;
+        q.tiles = (const int4*)c->tiles.p;
+        q.order = misc + 3 * n;
+        q.it = d_it;
+        q.zlo = d_zlo;
+        q.prm = f.prm;
+        q.hdr = (const double*)c->hdr.p;
+        q.wsfh = (const double*)c->wsfh.p;
+        q.gout = (const double*)c->gout.p;
+        q.mass_bar = g->mass_bar;
+        q.logz = c->d_logz;
+        q.impc = (const double*)c->opimf.p;
+        q.impd = (double*)c->opimfd.p;
+        q.age_ptr = c->op.d_age_ptr;
+        q.first = c->op.d_first;
+        q.slow_ptr = c->op.d_slow_ptr;
+        q.slow_pt = c->op.d_slow_pt;
+        q.pt = c->op.d_pt;
+        q.age_mmax = c->op.d_age_mmax;
+        q.wp = c->op.d_wp;
+        q.poly = c->op.d_poly;
+        q.hpoly = c->op.d_hpoly;
+        q.pent = c->op.d_pent;
+        q.pval = c->op.d_pval;
+        q.smass_p = (const double*)c->opsum.p;
+        q.slum_p = q.smass_p + (size_t)npass * n * nt;
+        q.sraw = (const double*)c->opsraw.p;
+        q.CT = c->opct.p;
+        q.part = (double*)c->oppart.p;
+        q.params_bar = g->params_bar;
+        q.weights_bar = wbar_out;
+        const dim3 gt3((unsigned)((c->op.ntpl + 63) / 64), (unsigned)((n + 63) / 64), 2);
+        if (mixed)
+            k_op_adj_t<float><<<gt3, 256, 0, c->stream>>>(n, c->op.ntpl, ldc, q.order, (const float*)c->gC.p,
+                                                           (float*)c->opct.p);
+        else
+            k_op_adj_t<double><<<gt3, 256, 0, c->stream>>>(n, c->op.ntpl, ldc, q.order, (const double*)c->gC.p,
+                                                            (double*)c->opct.p);
+        HIPCHK(c, hipGetLastError());
+        k_op_imf_grad<<<(n + 255) / 256, 256, 0, c->stream>>>(q);
+        HIPCHK(c, hipGetLastError());
+        const int maxtiles = (n + OPS_TILE_COL - 1) / OPS_TILE_COL + c->npair;
+        const dim3 gp((unsigned)((nt + 3) / 4), (unsigned)maxtiles, (unsigned)npass);
+        if (mixed)
+            k_op_adj_points<float><<<gp, 256, 0, c->stream>>>(q);
+        else
+            k_op_adj_points<double><<<gp, 256, 0, c->stream>>>(q);
+        HIPCHK(c, hipGetLastError());
+        k_op_adj_out<<<(n + 3) / 4, 256, 0, c->stream>>>(q);
+        HIPCHK(c, hipGetLastError());
+    }
 
     GradOutArgs go;
     go.n = n;
@@ -1450,10 +1564,12 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     go.C = c->gC.p;
     go.mixed = mixed;
     go.params_bar = g->params_bar;
-    go.weights_bar = g->weights_bar ? g->weights_bar : (sfh_rows ? (double*)c->gwb.p : nullptr);
+    go.weights_bar = wbar_out;
     go.weights_z_bar = g->weights_z_bar;
-    k_grad_out<<<n, 256, 0, c->stream>>>(go);
-    HIPCHK(c, hipGetLastError());
+    if (!ops) {
+        k_grad_out<<<n, 256, 0, c->stream>>>(go);
+        HIPCHK(c, hipGetLastError());
+    }
     if (sfh_rows) {
         SfhGradArgs sa;
         sa.n = n;

@@ -78,6 +78,14 @@ struct OpsCtx {
         OpsWPoint* d_wp = nullptr;                         // k_op_weights' records
         double* d_poly = nullptr;                          // k_op_weights' expm1 coefficients
         int nslow = 0;
+        // the reverse pass: each point's template entries; the library as the transposed product's rows (template,
+        // pixel) in FP64 (nrt x 64 rows, zero past ntpl) and, once a mixed reverse pass asks, split BF16
+        int nent = 0, nrt = 0;
+        int* d_pent = nullptr;
+        double* d_pval = nullptr;
+        double* d_hpoly = nullptr;                         // 1 / (j! (j + 2)), j = 0 .. 12
+        double* d_At64 = nullptr;
+        uint16_t *d_Ath = nullptr, *d_Atl = nullptr;
     } op;
     int imf_type = 2, remnants = 1;
     double salp_ind = 2.35, mdave = 0.5, mlim_bh = 40.0, mlim_ns = 8.5;
@@ -136,6 +144,9 @@ struct OpsCtx {
                                                   // scalars, y_bar rows, the transposed product
     OpsBuffer gwb;                                // reverse pass: d L / d w for the SFH rows (no weights_bar given)
     double* lsum_out = nullptr;                   // set by the reverse pass around its forward batch
+    double* op_sraw_out = nullptr;                // the same: the operator's per-pass sums before the blend
+    OpsBuffer opsraw, opct, oppart, opimfd;       // the operator's reverse pass: raw per-age sums, template
+                                                  // cotangents by position, per-(pass, age) sums, IMF constants
     int* d_last_flag = nullptr;   // the per-batch status flag of the last batch (read by sync)
 
     // ---- per-kernel timing (HIP events on the context's stream) ----

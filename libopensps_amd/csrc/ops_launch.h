@@ -145,6 +145,7 @@ struct OpWArgs {           // k_op_weights, k_op_slow, k_op_agesums
     int* scnt;                  // (pass, tile, age): their nonzero weights, 64 per wavefront
     double* gsum;               // (pass, tile slot, age, group): the common bins' mass and L_bol sums, 2 x 64
     int* gcnt;                  // (pass, tile slot, age, group): their nonzero weights, 64
+    double* sraw;               // (pass, model, age) x 2: the mass and L_bol sums before the blend, or NULL
 };
 // The power-law segment of a bin [lo, hi] (imf_bin_weights_tab's classification, :483-498): 0 when the bin
 // crosses a break or lies below 0.08 M_sun (the general integral then)
@@ -198,6 +199,30 @@ struct OpSArgs {           // k_op_sums
     int n, nt, npass;
     const double *wsfh, *smass_p, *slum_p;
     double *mass, *lbol;
+    double* lsum;          // (n) the linear L_bol sum (the reverse pass's), or NULL
+};
+
+// The operator's reverse pass (k_op_imf_grad, k_op_adj_t, k_op_adj_points, k_op_adj_out)
+#define OPS_IMFD_N 9       // per model: d ln(c_seg / total) / d direction, segment-major (3 segments x 3 directions)
+struct OpAdjArgs {
+    int n, nt, nz, npass, ntpl, nent, imf_type, remnants;
+    double salp_ind, mdave, mlim_bh, mlim_ns;
+    const int* ntiles;
+    const int4* tiles;
+    const int *order, *it, *zlo;
+    const double *prm, *hdr, *wsfh, *gout, *mass_bar, *logz;
+    const double* impc;         // k_op_imf's per-model constants
+    double* impd;               // per model OPS_IMFD_N
+    const int *age_ptr, *first, *slow_ptr, *slow_pt;
+    const double *pt, *age_mmax;
+    const OpsWPoint* wp;
+    const double *poly, *hpoly; // 1 / (j + 1)! and 1 / (j! (j + 2)), j = 0 .. 12
+    const int* pent;            // per point nent template indices (-1: none)
+    const double* pval;         // per point nent values
+    const double *smass_p, *slum_p, *sraw;
+    const void* CT;             // (part, template, position): the template cotangents, kernel precision
+    double* part;               // (pass, age, position) x 4: the fold's sum and the three IMF sums (scaled)
+    double *params_bar, *weights_bar;
 };
 
 struct ChainArgs {
@@ -313,4 +338,10 @@ __global__ void k_op_segments(OpGArgs a);
 template <typename T>
 __global__ void k_op_fold(OpFArgs a);
 __global__ void k_op_sums(OpSArgs a);
+__global__ void k_op_imf_grad(OpAdjArgs a);
+template <typename T>
+__global__ void k_op_adj_t(int n, int ntpl, int ldc, const int* order, const T* C, T* CT);
+template <typename T>
+__global__ void k_op_adj_points(OpAdjArgs a);
+__global__ void k_op_adj_out(OpAdjArgs a);
 __global__ void k_op_tiles(int ncols, int width, int* ntiles, int4* tiles);

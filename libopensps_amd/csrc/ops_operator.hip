@@ -28,21 +28,85 @@
 // ------------------------------------------------------------------------------------------------
 #pragma clang fp contract(off)
 
-struct ImfP {
-    double salp, a1, a2, a3, mdave, lower, upper, lower_bound, mlim_bh, mlim_ns;
+// A value with its derivatives along three directions (forward mode): the reverse pass's IMF rows take the bins off
+// the common path, the total mass and the remnants through the same functions as the forward, with this in place of
+// double. Comparisons and branches follow the value.
+struct D3 {
+    double v, d[3];
+    __device__ D3() : v(0.0), d{0.0, 0.0, 0.0} {}
+    __device__ D3(double x) : v(x), d{0.0, 0.0, 0.0} {}
 };
+__device__ inline D3 operator+(const D3& a, const D3& b) {
+    D3 r(a.v + b.v);
+    for (int i = 0; i < 3; i++) r.d[i] = a.d[i] + b.d[i];
+    return r;
+}
+__device__ inline D3 operator-(const D3& a, const D3& b) {
+    D3 r(a.v - b.v);
+    for (int i = 0; i < 3; i++) r.d[i] = a.d[i] - b.d[i];
+    return r;
+}
+__device__ inline D3 operator-(const D3& a) {
+    D3 r(-a.v);
+    for (int i = 0; i < 3; i++) r.d[i] = -a.d[i];
+    return r;
+}
+__device__ inline D3 operator*(const D3& a, const D3& b) {
+    D3 r(a.v * b.v);
+    for (int i = 0; i < 3; i++) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+    return r;
+}
+__device__ inline D3 operator/(const D3& a, const D3& b) {
+    D3 r(a.v / b.v);
+    for (int i = 0; i < 3; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) / b.v;
+    return r;
+}
+__device__ inline bool operator<(const D3& a, const D3& b) { return a.v < b.v; }
+__device__ inline bool operator<=(const D3& a, const D3& b) { return a.v <= b.v; }
+__device__ inline bool operator>(const D3& a, const D3& b) { return a.v > b.v; }
+__device__ inline bool operator>=(const D3& a, const D3& b) { return a.v >= b.v; }
+__device__ inline bool operator==(const D3& a, const D3& b) { return a.v == b.v; }
+__device__ inline bool operator!=(const D3& a, const D3& b) { return a.v != b.v; }
+__device__ inline D3 exp(const D3& a) {
+    D3 r(::exp(a.v));
+    for (int i = 0; i < 3; i++) r.d[i] = r.v * a.d[i];
+    return r;
+}
+__device__ inline D3 log(const D3& a) {
+    D3 r(::log(a.v));
+    for (int i = 0; i < 3; i++) r.d[i] = a.d[i] / a.v;
+    return r;
+}
+__device__ inline D3 pow(const D3& a, const D3& b) {
+    D3 r(::pow(a.v, b.v));
+    const double la = a.v > 0.0 ? ::log(a.v) : 0.0;
+    for (int i = 0; i < 3; i++) r.d[i] = r.v * (b.d[i] * la + (a.d[i] != 0.0 ? b.v * a.d[i] / a.v : 0.0));
+    return r;
+}
+__device__ inline D3 fabs(const D3& a) { return a.v < 0.0 ? -a : a; }
+__device__ inline D3 fmin(const D3& a, const D3& b) { return b.v < a.v ? b : a; }
+__device__ inline D3 fmax(const D3& a, const D3& b) { return b.v > a.v ? b : a; }
+__device__ inline double val(double x) { return x; }
+__device__ inline double val(const D3& x) { return x.v; }
+
+template <typename S>
+struct ImfPT {
+    S salp, a1, a2, a3, mdave, lower, upper, lower_bound, mlim_bh, mlim_ns;
+};
+typedef ImfPT<double> ImfP;
 
 // powerlaw_bin (opensps_imf.f90:754-792): int_lo^hi m^(e-1) dm from plo = lo^e, dl = ln(hi/lo) by Kahan's log1p
-__device__ double d_powerlaw_bin(double plo, double lo, double hi, double e) {
+template <typename S>
+__device__ S d_powerlaw_bin(S plo, S lo, S hi, S e) {
     const double I2 = 1.0 / 2, I3 = 1.0 / 3, I4 = 1.0 / 4, I5 = 1.0 / 5, I6 = 1.0 / 6, I7 = 1.0 / 7, I8 = 1.0 / 8,
                  I9 = 1.0 / 9, I10 = 1.0 / 10, I11 = 1.0 / 11, I12 = 1.0 / 12, I13 = 1.0 / 13;
-    double x = (hi - lo) / lo, u = 1.0 + x, dl;
+    S x = (hi - lo) / lo, u = 1.0 + x, dl;
     if (u == 1.0) dl = x;
     else dl = log(u) * x / (u - 1.0);
     if (e == 0.0) return plo * dl;
-    double y = e * dl;
+    S y = e * dl;
     if (fabs(y) < 0.25) {
-        double r = 1.0 + y * I13;
+        S r = 1.0 + y * I13;
         r = 1.0 + y * I12 * r; r = 1.0 + y * I11 * r; r = 1.0 + y * I10 * r; r = 1.0 + y * I9 * r;
         r = 1.0 + y * I8 * r; r = 1.0 + y * I7 * r; r = 1.0 + y * I6 * r; r = 1.0 + y * I5 * r;
         r = 1.0 + y * I4 * r; r = 1.0 + y * I3 * r; r = 1.0 + y * I2 * r;
@@ -52,11 +116,12 @@ __device__ double d_powerlaw_bin(double plo, double lo, double hi, double e) {
 }
 
 // expm1_series (opensps_imf.f90:641-667)
-__device__ double d_expm1_series(double y) {
+template <typename S>
+__device__ S d_expm1_series(S y) {
     const double I2 = 1.0 / 2, I3 = 1.0 / 3, I4 = 1.0 / 4, I5 = 1.0 / 5, I6 = 1.0 / 6, I7 = 1.0 / 7, I8 = 1.0 / 8,
                  I9 = 1.0 / 9, I10 = 1.0 / 10, I11 = 1.0 / 11, I12 = 1.0 / 12, I13 = 1.0 / 13;
     if (fabs(y) < 0.25) {
-        double r = 1.0 + y * I13;
+        S r = 1.0 + y * I13;
         r = 1.0 + y * I12 * r; r = 1.0 + y * I11 * r; r = 1.0 + y * I10 * r; r = 1.0 + y * I9 * r;
         r = 1.0 + y * I8 * r; r = 1.0 + y * I7 * r; r = 1.0 + y * I6 * r; r = 1.0 + y * I5 * r;
         r = 1.0 + y * I4 * r; r = 1.0 + y * I3 * r; r = 1.0 + y * I2 * r;
@@ -66,79 +131,86 @@ __device__ double d_expm1_series(double y) {
 }
 
 // integrate_powerlaw (opensps_imf.f90:725-748)
-__device__ double d_integrate_powerlaw(double lo, double hi, double alpha, bool mw, double coeff) {
-    if (hi <= lo) return 0.0;
-    const double denom = mw ? 2.0 - alpha : 1.0 - alpha;
-    return coeff * d_powerlaw_bin(pow(lo, denom), lo, hi, denom);
+template <typename S>
+__device__ S d_integrate_powerlaw(S lo, S hi, S alpha, bool mw, S coeff) {
+    if (hi <= lo) return S(0.0);
+    const S denom = mw ? 2.0 - alpha : 1.0 - alpha;
+    return coeff * d_powerlaw_bin<S>(pow(lo, denom), lo, hi, denom);
 }
 
 // imf_integral (opensps_imf.f90:206-315) for the power-law types 0 (Salpeter), 2 (Kroupa), 4 (Dave). Not inlined:
 // the weights' hot loop calls it only for bins across a break, and keeps its registers for the common path.
-__device__ __attribute__((noinline)) double d_imf_integral(int type, const ImfP& p, double m1, double m2, bool mw) {
-    const double lo = fmin(m1, m2), hi = fmax(m1, m2);
-    double v = 0.0;
-    if (hi <= lo) return 0.0;
+template <typename S>
+__device__ __attribute__((noinline)) S d_imf_integral(int type, const ImfPT<S>& p, S m1, S m2, bool mw) {
+    const S lo = fmin(m1, m2), hi = fmax(m1, m2);
+    S v = 0.0;
+    if (hi <= lo) return S(0.0);
     const int t = type % 10;
     if (t == 0) {
-        v = d_integrate_powerlaw(lo, hi, p.salp, mw, 1.0);
+        v = d_integrate_powerlaw<S>(lo, hi, p.salp, mw, S(1.0));
     } else if (t == 2) {
-        const double c2 = pow(0.5, -p.a1 + p.a2);
-        double sl = fmax(lo, 0.08), sh = fmin(hi, 0.5);
-        if (sh > sl) v = v + d_integrate_powerlaw(sl, sh, p.a1, mw, 1.0);
-        sl = fmax(lo, 0.5); sh = fmin(hi, 1.0);
-        if (sh > sl) v = v + d_integrate_powerlaw(sl, sh, p.a2, mw, c2);
-        sl = fmax(lo, 1.0); sh = hi;
-        if (sh > sl) v = v + d_integrate_powerlaw(sl, sh, p.a3, mw, c2);
+        const S c2 = pow(S(0.5), -p.a1 + p.a2);
+        S sl = fmax(lo, S(0.08)), sh = fmin(hi, S(0.5));
+        if (sh > sl) v = v + d_integrate_powerlaw<S>(sl, sh, p.a1, mw, S(1.0));
+        sl = fmax(lo, S(0.5)); sh = fmin(hi, S(1.0));
+        if (sh > sl) v = v + d_integrate_powerlaw<S>(sl, sh, p.a2, mw, c2);
+        sl = fmax(lo, S(1.0)); sh = hi;
+        if (sh > sl) v = v + d_integrate_powerlaw<S>(sl, sh, p.a3, mw, c2);
     } else if (t == 4) {
-        double sl = fmax(lo, 0.08), sh = fmin(hi, p.mdave);
-        if (sh > sl) v = v + d_integrate_powerlaw(sl, sh, p.a1, mw, 1.0);
+        S sl = fmax(lo, S(0.08)), sh = fmin(hi, p.mdave);
+        if (sh > sl) v = v + d_integrate_powerlaw<S>(sl, sh, p.a1, mw, S(1.0));
         sl = fmax(lo, p.mdave); sh = hi;
-        if (sh > sl) v = v + d_integrate_powerlaw(sl, sh, p.a2, mw, pow(p.mdave, -p.a1 + p.a2));
+        if (sh > sl) v = v + d_integrate_powerlaw<S>(sl, sh, p.a2, mw, pow(p.mdave, -p.a1 + p.a2));
     }
     return v;
 }
 
 // add_remnant_mass_p (opensps_stellar_modifications.f90:729-816): black holes, neutron stars, white dwarfs
-__device__ __attribute__((noinline)) double d_remnant_mass(int type, const ImfP& p, double mmax) {
-    const double hi = p.upper, lbh = p.mlim_bh, lns = p.mlim_ns;
-    const double norm = d_imf_integral(type, p, p.lower, hi, true);
-    double add = 0.0;
-    if (norm <= 0.0) return 0.0;
-    double a = fmin(fmax(lbh, mmax), hi);
-    if (a < hi) add = add + 0.5 * d_imf_integral(type, p, a, hi, true) / norm;
-    if (mmax <= lbh) {
-        a = fmin(fmax(lns, mmax), hi);
-        const double b = fmin(lbh, hi);
-        if (a < b) add = add + 1.4 * d_imf_integral(type, p, a, b, false) / norm;
+template <typename S>
+__device__ __attribute__((noinline)) S d_remnant_mass(int type, const ImfPT<S>& p, double mmax) {
+    const S hi = p.upper, lbh = p.mlim_bh, lns = p.mlim_ns;
+    const S norm = d_imf_integral<S>(type, p, p.lower, hi, true);
+    S add = 0.0;
+    if (norm <= 0.0) return S(0.0);
+    S a = fmin(fmax(lbh, S(mmax)), hi);
+    if (a < hi) add = add + 0.5 * d_imf_integral<S>(type, p, a, hi, true) / norm;
+    if (mmax <= val(lbh)) {
+        a = fmin(fmax(lns, S(mmax)), hi);
+        const S b = fmin(lbh, hi);
+        if (a < b) add = add + 1.4 * d_imf_integral<S>(type, p, a, b, false) / norm;
     }
-    if (mmax <= lns) {
-        a = fmin(mmax, hi);
-        const double b = fmin(lns, hi);
+    if (mmax <= val(lns)) {
+        a = fmin(S(mmax), hi);
+        const S b = fmin(lns, hi);
         if (a < b)
-            add = add + (0.48 * d_imf_integral(type, p, a, b, false)) / norm +
-                  (0.077 * d_imf_integral(type, p, a, b, true)) / norm;
+            add = add + (0.48 * d_imf_integral<S>(type, p, a, b, false)) / norm +
+                  (0.077 * d_imf_integral<S>(type, p, a, b, true)) / norm;
     }
     return add;
 }
 
 // The state of imf_bin_weights_tab over one piece: the current power-law segment and the chain of shared edges
-struct BinState {
+template <typename S>
+struct BinStateT {
     int cur;
-    double e, c, d, ce, prev_hi, prev_phi;
+    S e, c, d, ce, prev_hi, prev_phi;
 };
+typedef BinStateT<double> BinState;
 
-__device__ __forceinline__ void bin_reset(BinState& s) {
+template <typename S>
+__device__ __forceinline__ void bin_reset(BinStateT<S>& s) {
     s.cur = 0; s.e = 0.0; s.c = 0.0; s.d = 1.0; s.ce = 0.0; s.prev_hi = -1.0; s.prev_phi = 0.0;
 }
 
 // A new power-law segment's constants (imf_bin_weights_tab, opensps_imf.f90:500-525), once per segment change
-__device__ __attribute__((noinline)) void bin_segment(int t, const ImfP& p, int seg, BinState& s) {
-    double alpha, coeff = 1.0;
+template <typename S>
+__device__ __attribute__((noinline)) void bin_segment(int t, const ImfPT<S>& p, int seg, BinStateT<S>& s) {
+    S alpha, coeff = 1.0;
     if (t == 0) {
         alpha = p.salp;
     } else if (t == 2) {
         alpha = seg == 1 ? p.a1 : (seg == 2 ? p.a2 : p.a3);
-        if (seg > 1) coeff = pow(0.5, -p.a1 + p.a2);
+        if (seg > 1) coeff = pow(S(0.5), -p.a1 + p.a2);
     } else {
         alpha = seg == 1 ? p.a1 : p.a2;
         if (seg > 1) coeff = pow(p.mdave, -p.a1 + p.a2);
@@ -151,51 +223,53 @@ __device__ __attribute__((noinline)) void bin_segment(int t, const ImfP& p, int 
 }
 
 // The first bin of an age (lower edge at the IMF's lower bound): the difference of powers (:531-535)
-__device__ __attribute__((noinline)) double bin_first(const BinState& s, double lo, double hi) {
+template <typename S>
+__device__ __attribute__((noinline)) S bin_first(const BinStateT<S>& s, S lo, S hi) {
     if (fabs(s.d) <= 1.0e-12) return s.c * log(hi / lo);
     return s.c * (pow(hi, s.e) - pow(lo, s.e)) / s.d;
 }
 
 // One bin of imf_bin_weights_tab (opensps_imf.f90:466-557), FP64 build: stars per unit initial mass. The
 // segment classification and the limits decide which bins weigh nothing, exactly as the reference.
-__device__ __attribute__((noinline)) double d_bin_weight(int type, const ImfP& p, double inv_total, double mini, double m1,
-                                                      double m2,
-                               double llo, double dl, int first, BinState& s) {
-    if (mini < p.lower || mini > p.upper) return 0.0;
-    const double lo = first == 1 ? p.lower_bound : m1, hi = m2;
-    if (hi <= lo) return 0.0;
+template <typename S>
+__device__ __attribute__((noinline)) S d_bin_weight(int type, const ImfPT<S>& p, S inv_total, double mini, double m1,
+                                                    double m2, double llo, double dl, int first, BinStateT<S>& s) {
+    if (mini < val(p.lower) || mini > val(p.upper)) return S(0.0);
+    const S lo = first == 1 ? p.lower_bound : S(m1), hi = m2;
+    if (hi <= lo) return S(0.0);
     const int t = type % 10;
+    const double lov = val(lo), hiv = val(hi);
     int seg = 0;
     if (t == 0) {
         seg = 1;
     } else if (t == 2) {
-        if (lo >= 0.08 && hi <= 0.5) seg = 1;
-        else if (lo >= 0.5 && hi <= 1.0) seg = 2;
-        else if (lo >= 1.0) seg = 3;
+        if (lov >= 0.08 && hiv <= 0.5) seg = 1;
+        else if (lov >= 0.5 && hiv <= 1.0) seg = 2;
+        else if (lov >= 1.0) seg = 3;
     } else if (t == 4) {
-        if (lo >= 0.08 && hi <= p.mdave) seg = 1;
-        else if (lo >= p.mdave) seg = 2;
+        if (lov >= 0.08 && hiv <= val(p.mdave)) seg = 1;
+        else if (lov >= val(p.mdave)) seg = 2;
     }
-    if (seg > 0 && seg != s.cur) bin_segment(t, p, seg, s);
-    double val;
+    if (seg > 0 && seg != s.cur) bin_segment<S>(t, p, seg, s);
+    S v;
     if (seg > 0 && s.cur == seg) {
         if (first == 1) {
-            val = bin_first(s, lo, hi);
+            v = bin_first<S>(s, lo, hi);
         } else if (s.e == 0.0) {
-            val = s.c * dl;
+            v = s.c * dl;
         } else {
-            const double plo = (lo == s.prev_hi) ? s.prev_phi : exp(s.e * llo);
-            const double em1 = d_expm1_series(s.e * dl);
-            val = s.ce * plo * em1;
+            const S plo = (lo == s.prev_hi) ? s.prev_phi : exp(s.e * llo);
+            const S em1 = d_expm1_series<S>(s.e * dl);
+            v = s.ce * plo * em1;
             s.prev_hi = hi;
             s.prev_phi = plo + plo * em1;
         }
     } else {
-        val = d_imf_integral(type, p, lo, hi, false);
+        v = d_imf_integral<S>(type, p, lo, hi, false);
         s.prev_hi = -1.0;
         if (seg == 0) s.cur = 0;
     }
-    return val * inv_total;
+    return v * inv_total;
 }
 
 // The model's IMF from its parameter slots and the context's settings
@@ -416,6 +490,10 @@ __global__ __launch_bounds__(256) void k_op_agesums(OpWArgs a) {
         const size_t r = ((size_t)pass * n + m) * nt + t;
         a.smass_p[r] = cz * sm;
         a.slum_p[r] = cz * log10(fmax(sl, OPS_SAFE_FLOOR));
+        if (a.sraw) {
+            a.sraw[2 * r] = sm;
+            a.sraw[2 * r + 1] = sl;
+        }
         if (a.count && cnt && cz != 0.0) atomicAdd(a.count + 2 * m + pass, cnt);
     }
     const double ws = a.wsfh[(size_t)m * nt + t];
@@ -456,6 +534,7 @@ __global__ __launch_bounds__(256) void k_op_sums(OpSArgs a) {
     if (lane == 0) {
         if (a.mass) a.mass[m] = ms;
         if (a.lbol) a.lbol[m] = ls > OPS_SAFE_FLOOR ? log10(ls) : 0.0;
+        if (a.lsum) a.lsum[m] = ls;
     }
 }
 
@@ -593,6 +672,285 @@ __global__ void k_op_tiles(int ncols, int width, int* ntiles, int4* tiles) {
     if (threadIdx.x == 0) *ntiles = nt;
 }
 
+// ================================================================================================
+// The reverse pass of the IMF per model (ssp_batch_age_sums_adjoint, src/physics/opensps_ssp_batch.f90:991-1248,
+// with adj_template_bar, adj_fold_gather, adj_lq, adj_mass_dot and adj_imf_analytic). The template cotangents
+// v_b = etpl y_b (the transposed product, ops_api.hip) arrive transposed to (part, template, position), so a
+// wavefront's lanes (the models of a column tile) read each template's cotangents as one contiguous run. Per point
+// k of an age: g_k = sum_j val(k, j) v_b(tpl(k, j)); per (pass, age, model) the fold's sum F = sum_k w_k g_k and the
+// IMF sums sum_k dw_k (x g_k + A cm_k + B cl_k) (+ A drem), x the SFH weight times the blend factor, A and B the
+// mass and L_bol cotangents per point; k_op_adj_out turns them into w_bar, the metallicity's and the IMF's rows.
+// Directions of the IMF rows: Kroupa (2) imf1, imf2, imf3; Dave (4) imf1, imf2, mdave; Salpeter none.
+// ================================================================================================
+#pragma clang fp contract(on)
+
+// The model's IMF with the directions seeded (value types as model_imf)
+__device__ ImfPT<D3> model_imf_d3(const OpAdjArgs& a, int m) {
+    const int n = a.n, t = a.imf_type % 10;
+    ImfPT<D3> p;
+    p.salp = a.salp_ind;
+    p.a1 = a.prm[(size_t)OPS_P_IMF1 * n + m];
+    p.a2 = a.prm[(size_t)OPS_P_IMF2 * n + m];
+    p.a3 = a.prm[(size_t)OPS_P_IMF3 * n + m];
+    p.mdave = a.mdave;
+    p.lower = a.prm[(size_t)OPS_P_IMF_LOWER * n + m];
+    p.upper = a.prm[(size_t)OPS_P_IMF_UPPER * n + m];
+    p.lower_bound = p.lower;
+    p.mlim_bh = a.mlim_bh;
+    p.mlim_ns = a.mlim_ns;
+    if (t == 2 || t == 4) {
+        p.a1.d[0] = 1.0;
+        p.a2.d[1] = 1.0;
+        if (t == 2) p.a3.d[2] = 1.0;
+        else p.mdave.d[2] = 1.0;
+    }
+    return p;
+}
+
+// k_op_imf_grad: one thread per model: d ln(c_seg / total) along the directions for each power-law segment (the
+// common bins' weights are c_seg / total x a function of e_seg = 1 - alpha_seg and the bin alone)
+__global__ void k_op_imf_grad(OpAdjArgs a) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.n) return;
+    const ImfPT<D3> p = model_imf_d3(a, m);
+    const int t = a.imf_type % 10;
+    const D3 tot = d_imf_integral<D3>(a.imf_type, p, p.lower, p.upper, true);
+    double* o = a.impd + (size_t)m * OPS_IMFD_N;
+    for (int seg = 1; seg <= 3; seg++) {
+        const bool on = !(t == 0 && seg > 1) && !(t == 4 && seg > 2);
+        BinStateT<D3> s;
+        bin_reset(s);
+        if (on) bin_segment<D3>(t, p, seg, s);
+        for (int i = 0; i < 3; i++) {
+            double v = 0.0;
+            if (on && s.c.v != 0.0) v = s.c.d[i] / s.c.v - (tot.v > 0.0 ? tot.d[i] / tot.v : 0.0);
+            o[3 * (seg - 1) + i] = v;
+        }
+    }
+}
+
+// k_op_adj_t: grid (ceil(ntpl / 64), ceil(n / 64), 2 parts), 256 threads: C (column m * 2 + part, template) with
+// row stride ldc -> CT (part, template, position) through a 64 x 64 LDS tile, the positions' models from order
+template <typename T>
+__global__ __launch_bounds__(256) void k_op_adj_t(int n, int ntpl, int ldc, const int* order, const T* C, T* CT) {
+    __shared__ T tile[64][65];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int e0 = blockIdx.x * 64, p0 = blockIdx.y * 64, part = blockIdx.z;
+    for (int r = wv; r < 64; r += 4) {
+        const int pos = p0 + r, e = e0 + lane;
+        T v = (T)0;
+        if (pos < n && e < ntpl) v = C[((size_t)order[pos] * 2 + part) * ldc + e];
+        tile[r][lane] = v;
+    }
+    __syncthreads();
+    for (int r = wv; r < 64; r += 4) {
+        const int e = e0 + r, pos = p0 + lane;
+        if (e < ntpl && pos < n) CT[((size_t)part * ntpl + e) * n + pos] = tile[lane][r];
+    }
+}
+
+// h(y) = (e^y (y - 1) + 1) / y^2 = int_0^1 s e^(y s) ds: d/dy of expm1(y) / y. Its series sum_j y^j / (j! (j + 2))
+// below |y| = 1/4 (Horner, 12 FMAs), the closed form above.
+__device__ __forceinline__ double d_hfun(double y, const double* hc) {
+    if (fabs(y) < 0.25) {
+        double s = hc[12];
+#pragma unroll
+        for (int j = 11; j >= 0; j--) s = fma(s, y, hc[j]);
+        return s;
+    }
+    return (exp(y) * (y - 1.0) + 1.0) / (y * y);
+}
+
+// k_op_adj_points: grid (ceil(nt / 4), tiles, passes), a wavefront per (age, column tile, pass) as k_op_slow. The
+// common bins as k_op_weights forms them (records through scalar loads, the chain of shared edges), each with its
+// derivative along e: dw/de = c / total lo^e dl (ln lo sx(e dl) + dl h(e dl)), sx = expm1(y) / y; along a direction
+// dw = w d ln(c / total) - [the segment's slope is the direction] dw/de. The bins off the common path and the
+// remnants by the forward's functions on D3.
+template <typename T>
+__global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int t = blockIdx.x * 4 + wv, tile = blockIdx.y, pass = blockIdx.z;
+    if (t >= a.nt || tile >= *a.ntiles) return;
+    const int4 td = a.tiles[tile];
+    const int iz = a.npass == 1 ? 0 : __builtin_amdgcn_readfirstlane(td.x) + pass;
+    const bool valid = lane < td.z;
+    const int pos = td.y + (valid ? lane : 0);
+    const int m = a.order[pos];
+    const int n = a.n, nt = a.nt, nent = a.nent, ty = a.imf_type % 10;
+    const double dz = a.hdr[(size_t)m * OPS_HDR_N + OPS_HDR_DZ];
+    const double cz = a.npass == 1 ? 1.0 : (pass == 0 ? 1.0 - dz : dz);
+    const int part = t + 1 <= a.it[m] ? 0 : 1;
+    const T* ct = (const T*)a.CT + (size_t)part * a.ntpl * n + pos;
+    const double* go = a.gout + (size_t)m * OPS_GOUT_N;
+    const double cs = go[OPS_GOUT_C];
+    const double mbs = a.mass_bar ? a.mass_bar[m] / cs : 0.0, lsbs = go[OPS_GOUT_LSB];
+    const double ws = a.wsfh[(size_t)m * nt + t];
+    const double x0 = (ws > OPS_SAFE_FLOOR ? ws : 0.0) * cz;
+    double slog = 0.0;
+    for (int q = 0; q < a.npass; q++) slog += a.slum_p[((size_t)q * n + m) * nt + t];
+    const double sl = a.sraw[(((size_t)pass * n + m) * nt + t) * 2 + 1];
+    const double A = mbs * x0, B = sl > OPS_SAFE_FLOOR ? lsbs * x0 * pow(10.0, slog) / sl : 0.0;
+    const double lower = a.prm[(size_t)OPS_P_IMF_LOWER * n + m], upper = a.prm[(size_t)OPS_P_IMF_UPPER * n + m];
+    const double* ic = a.impc + (size_t)m * OPS_IMFC_N;
+    const double e1 = ic[OPS_IMFC_E], e2 = ic[OPS_IMFC_E + 1], e3 = ic[OPS_IMFC_E + 2];
+    const double c1 = ic[OPS_IMFC_CT], c2 = ic[OPS_IMFC_CT + 1], c3 = ic[OPS_IMFC_CT + 2];
+    double dc[OPS_IMFD_N];
+#pragma unroll
+    for (int q = 0; q < OPS_IMFD_N; q++) dc[q] = a.impd[(size_t)m * OPS_IMFD_N + q];
+    // the direction of each segment's slope (Kroupa: segment s has alpha_s; Dave: alpha_1, alpha_2; Salpeter: none)
+    const int sd1 = (ty == 2 || ty == 4) ? 0 : -1, sd2 = (ty == 2 || ty == 4) ? 1 : -1, sd3 = ty == 2 ? 2 : -1;
+    const int ka = __builtin_amdgcn_readfirstlane(a.age_ptr[iz * nt + t]);
+    const int kb = __builtin_amdgcn_readfirstlane(a.age_ptr[iz * nt + t + 1]);
+    typedef __attribute__((address_space(4))) const OpsWPoint* WPC;
+    typedef __attribute__((address_space(4))) const double* DC;
+    typedef __attribute__((address_space(4))) const int* IC;
+    const WPC wp = (WPC)a.wp;
+    const DC pcs = (DC)a.poly, hcs = (DC)a.hpoly, pv = (DC)a.pval;
+    const IC pe = (IC)a.pent;
+    double pc[13], hc[13];
+#pragma unroll
+    for (int j = 0; j < 13; j++) {
+        pc[j] = pcs[j];
+        hc[j] = hcs[j];
+    }
+    double F = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, phi = 1.0;
+    for (int k = ka; k < kb; k++) {
+        const OpsWPoint r = wp_load(wp + k);
+        const int seg = r.code & 3;
+        if (seg == 0) continue;   // off the common path: below
+        // the template cotangents of the point's entries (a contiguous run of the tile's models each)
+        double gk = 0.0;
+        for (int j = 0; j < nent; j++) {
+            const int e = pe[(size_t)k * nent + j];
+            if (e >= 0) gk = fma(pv[(size_t)k * nent + j], (double)ct[(size_t)e * n], gk);
+        }
+        const double e = seg == 1 ? e1 : (seg == 2 ? e2 : e3);
+        const double c = seg == 1 ? c1 : (seg == 2 ? c2 : c3);
+        const int sd = seg == 1 ? sd1 : (seg == 2 ? sd2 : sd3);
+        const double dl = r.dl;
+        const double plo = (r.code & 4) ? phi : exp(e * r.llo);
+        const double y = e * dl;
+        double sx = d_expm1_over_y(y, pc);
+        if (!(fabs(y) < 0.25)) sx = (exp(y) - 1.0) / y;
+        phi = fma(plo, y * sx, plo);
+        if (r.mini < lower || r.mini > upper) continue;
+        const double w = c * dl * plo * sx;
+        const double dwde = c * plo * dl * fma(dl, d_hfun(y, hc), r.llo * sx);
+        const double ge = x0 * gk + A * r.cm + B * r.cl;
+        const double* d = dc + 3 * (seg - 1);
+        F = fma(w, gk, F);
+        g0 = fma(ge, w * d[0] - (sd == 0 ? dwde : 0.0), g0);
+        g1 = fma(ge, w * d[1] - (sd == 1 ? dwde : 0.0), g1);
+        g2 = fma(ge, w * d[2] - (sd == 2 ? dwde : 0.0), g2);
+    }
+    const int j0 = a.slow_ptr[iz * nt + t], j1 = a.slow_ptr[iz * nt + t + 1];
+    const bool rem = a.remnants && kb > ka;
+    if (j1 > j0 || rem) {
+        const ImfPT<D3> p = model_imf_d3(a, m);
+        const D3 tot = d_imf_integral<D3>(a.imf_type, p, p.lower, p.upper, true);
+        const D3 inv = tot.v > 0.0 ? D3(1.0) / tot : D3(1.0);
+        for (int j = j0; j < j1; j++) {
+            const int k = a.slow_pt[j];
+            const double* q = a.pt + (size_t)k * OPS_PT_N;
+            BinStateT<D3> s;
+            bin_reset(s);
+            const D3 w = d_bin_weight<D3>(a.imf_type, p, inv, q[0], q[1], q[2], q[3], q[4], a.first[k], s);
+            double gk = 0.0;
+            for (int jj = 0; jj < nent; jj++) {
+                const int e = a.pent[(size_t)k * nent + jj];
+                if (e >= 0) gk = fma(a.pval[(size_t)k * nent + jj], (double)ct[(size_t)e * n], gk);
+            }
+            const double ge = x0 * gk + A * q[5] + B * q[6];
+            F = fma(w.v, gk, F);
+            g0 = fma(ge, w.d[0], g0);
+            g1 = fma(ge, w.d[1], g1);
+            g2 = fma(ge, w.d[2], g2);
+        }
+        if (rem) {
+            const D3 r = d_remnant_mass<D3>(a.imf_type, p, a.age_mmax[iz * nt + t]);
+            g0 = fma(A, r.d[0], g0);
+            g1 = fma(A, r.d[1], g1);
+            g2 = fma(A, r.d[2], g2);
+        }
+    }
+    if (!valid) return;
+    if (cz == 0.0) {   // a pass of zero weight is left out, as the forward's grouping leaves it
+        F = 0.0; g0 = 0.0; g1 = 0.0; g2 = 0.0;
+    }
+    double* o = a.part + (((size_t)pass * nt + t) * n + pos) * 4;
+    o[0] = F;
+    o[1] = g0;
+    o[2] = g1;
+    o[3] = g2;
+}
+
+// k_op_adj_out: a wavefront per position (4 a workgroup), lanes over the ages: w_bar (the fold's part plus the
+// per-age L_bol and mass terms), the metallicity weight's cotangent (the fold, the mass and the log-blended L_bol of
+// both passes), the IMF rows, and params_bar as k_grad_out writes it.
+__global__ __launch_bounds__(256) void k_op_adj_out(OpAdjArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int pos = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pos >= a.n) return;
+    const int n = a.n, nt = a.nt, m = a.order[pos];
+    const double* g = a.gout + (size_t)m * OPS_GOUT_N;
+    const double c = g[OPS_GOUT_C], lsb = g[OPS_GOUT_LSB] * c;
+    const double mb = a.mass_bar ? a.mass_bar[m] : 0.0;
+    const double dz = a.hdr[(size_t)m * OPS_HDR_N + OPS_HDR_DZ];
+    double dzb = 0.0, i0 = 0.0, i1 = 0.0, i2 = 0.0;
+    for (int t = lane; t < nt; t += 64) {
+        const double ws = a.wsfh[(size_t)m * nt + t];
+        const double x = ws > OPS_SAFE_FLOOR ? ws : 0.0;
+        double smass = 0.0, slog = 0.0;
+        for (int q = 0; q < a.npass; q++) {
+            smass += a.smass_p[((size_t)q * n + m) * nt + t];
+            slog += a.slum_p[((size_t)q * n + m) * nt + t];
+        }
+        const double slum = pow(10.0, slog);
+        double wf = 0.0;
+        for (int q = 0; q < a.npass; q++) {
+            const double cz = a.npass == 1 ? 1.0 : (q == 0 ? 1.0 - dz : dz);
+            const double* P = a.part + (((size_t)q * nt + t) * n + pos) * 4;
+            const double F = P[0] * c;
+            wf += cz * F;
+            i0 += P[1];
+            i1 += P[2];
+            i2 += P[3];
+            if (a.npass == 2 && cz != 0.0) {
+                const double* sr = a.sraw + (((size_t)q * n + m) * nt + t) * 2;
+                dzb += (q == 0 ? -1.0 : 1.0) * x * (F + mb * sr[0] + lsb * slum * log(fmax(sr[1], OPS_SAFE_FLOOR)));
+            }
+        }
+        if (a.weights_bar) a.weights_bar[(size_t)m * nt + t] = wf + lsb * slum + mb * smass;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        dzb += __shfl_xor(dzb, o, 64);
+        i0 += __shfl_xor(i0, o, 64);
+        i1 += __shfl_xor(i1, o, 64);
+        i2 += __shfl_xor(i2, o, 64);
+    }
+    if (lane == 0 && a.params_bar) {
+        double* pb = a.params_bar + (size_t)m * OPENSPS_HIP_NBGRAD_ROWS;
+        for (int r = 0; r < OPENSPS_HIP_NBGRAD_ROWS; r++) pb[r] = 0.0;
+        for (int q = 0; q < 7; q++) pb[1 + q] = g[OPS_GOUT_D + q] * c;   // dust1 .. frac_nodust (BGRAD 2..8)
+        const int zl = a.zlo[m];
+        if (a.nz > 1) pb[15] = dzb / (a.logz[zl] - a.logz[zl - 1]);    // logzsol (BGRAD 16)
+        pb[22] = (g[OPS_GOUT_DZO] + g[OPS_GOUT_DZI]) * c + g[OPS_GOUT_DZD];  // zred (BGRAD 23)
+        pb[23] = g[OPS_GOUT_DSG] * c;                                        // sigma_smooth (BGRAD 24)
+        pb[32] = g[OPS_GOUT_DIGM] * c;                                       // igm_factor (BGRAD 33)
+        const int ty = a.imf_type % 10;
+        if (ty == 2 || ty == 4) {                                           // imf1, imf2 (BGRAD 27, 28)
+            pb[26] = i0 * c;
+            pb[27] = i1 * c;
+            pb[ty == 2 ? 28 : 30] = i2 * c;                                  // imf3 (29) or mdave (31)
+        }
+    }
+}
+#pragma clang fp contract(off)
+
+template __global__ void k_op_adj_t<float>(int, int, int, const int*, const float*, float*);
+template __global__ void k_op_adj_t<double>(int, int, int, const int*, const double*, double*);
+template __global__ void k_op_adj_points<float>(OpAdjArgs);
+template __global__ void k_op_adj_points<double>(OpAdjArgs);
 template __global__ void k_op_weights<float>(OpWArgs);
 template __global__ void k_op_weights<double>(OpWArgs);
 template __global__ void k_op_slow<float>(OpWArgs);

@@ -155,7 +155,7 @@ class Emulator:
         self.stats = {"models": 0, "fallback_models": 0, "spot_checked": 0, "batches_failed": 0}
 
     @staticmethod
-    def train(dctx, cosmo, n_train=4_000_000, n_pca_fit=200_000, n_pca=128, epochs=30, batch=8192, lr=1e-3,
+    def train(dctx, cosmo, n_train=4_000_000, n_pca_fit=200_000, n_pca=128, epochs=100, batch=8192, lr=2e-3,
               seed=101, hidden=512, layers=4, log=None, precision=H.MIXED, pca_weight=0.1):
         import torch
         t0 = time.perf_counter()

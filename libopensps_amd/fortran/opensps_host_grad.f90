@@ -1,7 +1,9 @@
 !> The batch with gradients through the C API (compute_csp_batch with its cotangents and params_bar /
 !> weights_bar / weights_z_bar; batch_handle's gradient path, src/bindings/c/opensps_c_api.f90:663-961, and
 !> batch_backward, src/physics/opensps_csp_batch.f90:4713-5136): the host resolves the parameters and checks the
-!> options, the device runs the forward batch and its reverse pass (opensps_hip_csp_batch_grad).
+!> options, the device runs the forward batch and its reverse pass (opensps_hip_csp_batch_grad); with SSP operators
+!> on the context the IMF per model and its rows of params_bar (ssp_batch_age_sums_adjoint,
+!> src/physics/opensps_ssp_batch.f90:991-1248).
 module opensps_host_grad
     use, intrinsic :: iso_c_binding
     use opensps_host_types
@@ -76,9 +78,10 @@ contains
             call set_error(3, 'batch_grad: ssp_weights and ssp_weights_z both given')
             return
         end if
-        if (c%has_ops) then
+        if (c%has_ops .and. c_associated(c_weights_z)) then
+            ! (the forward refuses weights over metallicities with the operator, opensps_csp_batch.f90:616-618)
             status = 2
-            call set_error(2, 'batch_grad: the reverse pass with the SSP operator (the IMF rows) is not part of this build')
+            call set_error(2, 'batch_grad: ssp_weights_z with the SSP operator')
             return
         end if
         call resolve_params(c, n, n_params, c_names, c_values, prm, status, fagn)
@@ -137,6 +140,7 @@ contains
         g%fwd%n = n
         g%fwd%precision = merge(HIP_MIXED, HIP_FP64, c%mixed == 1)
         g%fwd%split_parts = 1
+        g%fwd%use_ops = merge(1, 0, c%has_ops)
         g%fwd%prm = d(1)
         g%fwd%ssp_weights = d(2)
         g%fwd%ssp_weights_z = d(3)

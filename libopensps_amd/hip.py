@@ -222,10 +222,11 @@ class DeviceContext:
             setattr(a, k, out[k].data_ptr() if k in out else None)
         return a
 
-    def launch_grad(self, prm, out, cot, grads, precision=MIXED, split_parts=0, ssp_weights=None, ssp_weights_z=None):
+    def launch_grad(self, prm, out, cot, grads, precision=MIXED, split_parts=0, ssp_weights=None, ssp_weights_z=None,
+                    use_ops=0):
         """Enqueues a batch and its reverse pass. cot: device cotangents {"mags", "spec", "mass", "lbol"} (any
         absent); grads: device outputs {"params_bar" (n, NBGRAD), "weights_bar" (n, nt), "weights_z_bar"
-        (n, nz, nt)} (any absent), float64."""
+        (n, nz, nt)} (any absent), float64. use_ops: the IMF per model (the operator), with its rows in params_bar."""
         n = prm.shape[1]
         g = self.grid
         want = {"mags": n * self.nbands, "spec": n * g.nspec, "mass": n, "lbol": n}
@@ -239,7 +240,7 @@ class DeviceContext:
             if t is not None and (t.element_size() != 8 or not t.is_contiguous() or t.numel() < numel):
                 raise OpenSPSHipError(3, f"{k}: {numel} contiguous float64 required")
         a = GradArgs()
-        a.fwd = self._batch_args(prm, out, precision, split_parts, ssp_weights, 0, ssp_weights_z)
+        a.fwd = self._batch_args(prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z)
         for k in ("mags", "spec", "mass", "lbol"):
             setattr(a, k + "_bar", cot[k].data_ptr() if cot.get(k) is not None else None)
         for k in ("params_bar", "weights_bar", "weights_z_bar"):
@@ -268,7 +269,8 @@ class DeviceContext:
         return out
 
     def batch_grad(self, prm, cot, precision=MIXED, split_parts=0, ssp_weights=None, ssp_weights_z=None,
-                   want=("mags", "spec", "mass", "lbol"), grads=("params_bar", "weights_bar", "weights_z_bar")):
+                   want=("mags", "spec", "mass", "lbol"), grads=("params_bar", "weights_bar", "weights_z_bar"),
+                   use_ops=0):
         """The forward outputs and the reverse pass's outputs (device tensors)."""
         import torch
         n = prm.shape[1]
@@ -277,7 +279,7 @@ class DeviceContext:
         shapes = {"params_bar": (n, NBGRAD), "weights_bar": (n, g.nt), "weights_z_bar": (n, g.nz, g.nt)}
         gr = {k: torch.full(shapes[k], float("nan"), dtype=torch.float64, device="cuda") for k in grads}
         torch.cuda.synchronize()
-        self.launch_grad(prm, out, cot, gr, precision, split_parts, ssp_weights, ssp_weights_z)
+        self.launch_grad(prm, out, cot, gr, precision, split_parts, ssp_weights, ssp_weights_z, use_ops)
         self.sync()
         out.update(gr)
         return out

@@ -1195,6 +1195,232 @@ void orc_ops_model(const orc_ops_t *op, int nspec, int imf_type, int remnants, c
 }
 
 /* ---------------------------------------------------------------------------------------------
+ * The IMF's derivatives along its per-model parameters (the rows imf1, imf2, imf3 / mdave of the
+ * operator's reverse pass, ssp_batch_age_sums_adjoint, opensps_ssp_batch.f90:991-1248, whose
+ * adj_imf_analytic takes the power-law types 2 and 4 analytically). Written here from the integrals
+ * themselves: a segment xi(m) = c m^-alpha contributes int m^(s-1) dm over its part of [lo, hi]
+ * (s = 1 - alpha for numbers, 2 - alpha for mass); d/d alpha of it is -int m^(s-1) ln m dm, the
+ * coefficient's own derivative multiplies int m^(s-1) dm, and a break inside the interval moves two
+ * limits whose terms cancel where the IMF is continuous. The moments come from Gauss-Legendre
+ * quadrature in ln m (not from the reference's closed forms), so this restatement checks them.
+ * Directions: Kroupa (2) imf1, imf2, imf3; Dave (4) imf1, imf2, mdave; the others none.
+ * ------------------------------------------------------------------------------------------- */
+#define ORC_GL_N 20
+static double gl_x[ORC_GL_N], gl_w[ORC_GL_N];
+static int gl_ready = 0;
+
+/* Gauss-Legendre nodes and weights on [-1, 1]: Newton on P_n from the Chebyshev guesses */
+static void gl_init(void) {
+    if (gl_ready) return;
+    const int n = ORC_GL_N;
+    for (int i = 0; i < n; i++) {
+        double x = cos(acos(-1.0) * (i + 0.75) / (n + 0.5)), dp = 1.0;
+        for (int it = 0; it < 100; it++) {
+            double p0 = 1.0, p1 = x;
+            for (int k = 2; k <= n; k++) {
+                double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (x * p1 - p0) / (x * x - 1.0);
+            double dx = p1 / dp;
+            x -= dx;
+            if (fabs(dx) < 1e-16) break;
+        }
+        gl_x[i] = x;
+        gl_w[i] = 2.0 / ((1.0 - x * x) * dp * dp);
+    }
+    gl_ready = 1;
+}
+
+/* int_lo^hi m^(s-1) (ln m)^q dm = int e^(s u) u^q du over [ln lo, ln hi], q = 0 or 1, in pieces of at most
+ * 0.25 in u of ORC_GL_N nodes each */
+static double moment_ln(double lo, double hi, double s, int q) {
+    if (!(hi > lo) || lo <= 0.0) return 0.0;
+    const double u0 = log(lo), u1 = log(hi);
+    int np = (int)ceil((u1 - u0) / 0.25);
+    if (np < 1) np = 1;
+    const double h = (u1 - u0) / np;
+    double acc = 0.0;
+    for (int j = 0; j < np; j++) {
+        const double a = u0 + j * h, mid = a + 0.5 * h, half = 0.5 * h;
+        double part = 0.0;
+        for (int i = 0; i < ORC_GL_N; i++) {
+            const double u = mid + half * gl_x[i];
+            part += gl_w[i] * exp(s * u) * (q ? u : 1.0);
+        }
+        acc += half * part;
+    }
+    return acc;
+}
+
+/* d/d(the type's three directions) of orc_imf_integral(imf_type, p, m1, m2, mw) */
+static void imf_integral_grad(int imf_type, const double *p, double m1, double m2, int mw, double *d) {
+    d[0] = 0.0; d[1] = 0.0; d[2] = 0.0;
+    const double lo = fmin(m1, m2), hi = fmax(m1, m2), s0 = mw ? 2.0 : 1.0;
+    if (!(hi > lo)) return;
+    const int t = imf_type % 10;
+    if (t == 2) {
+        const double a1 = p[ORC_IMF_ALPHA], a2 = p[ORC_IMF_ALPHA + 1], a3 = p[ORC_IMF_ALPHA + 2];
+        const double l2 = log(0.5), c2 = pow(0.5, -a1 + a2);
+        double sl = fmax(lo, 0.08), sh = fmin(hi, 0.5);
+        if (sh > sl) d[0] += -moment_ln(sl, sh, s0 - a1, 1);
+        sl = fmax(lo, 0.5); sh = fmin(hi, 1.0);
+        if (sh > sl) {
+            const double i0 = moment_ln(sl, sh, s0 - a2, 0), i1 = moment_ln(sl, sh, s0 - a2, 1);
+            d[0] += -l2 * c2 * i0;
+            d[1] += l2 * c2 * i0 - c2 * i1;
+        }
+        sl = fmax(lo, 1.0); sh = hi;
+        if (sh > sl) {
+            const double i0 = moment_ln(sl, sh, s0 - a3, 0), i1 = moment_ln(sl, sh, s0 - a3, 1);
+            d[0] += -l2 * c2 * i0;
+            d[1] += l2 * c2 * i0;
+            d[2] += -c2 * i1;
+        }
+    } else if (t == 4) {
+        const double a1 = p[ORC_IMF_ALPHA], a2 = p[ORC_IMF_ALPHA + 1], md = p[ORC_IMF_MDAVE];
+        const double lm = log(md), c = pow(md, -a1 + a2);
+        double sl = fmax(lo, 0.08), sh = fmin(hi, md);
+        if (sh > sl) {
+            d[0] += -moment_ln(sl, sh, s0 - a1, 1);
+            if (hi > md) d[2] += pow(md, s0 - a1 - 1.0);          /* the upper limit at the break */
+        }
+        sl = fmax(lo, md); sh = hi;
+        if (sh > sl) {
+            const double i0 = moment_ln(sl, sh, s0 - a2, 0), i1 = moment_ln(sl, sh, s0 - a2, 1);
+            d[0] += -lm * c * i0;
+            d[1] += lm * c * i0 - c * i1;
+            d[2] += (a2 - a1) / md * c * i0;
+            if (lo < md) d[2] -= c * pow(md, s0 - a2 - 1.0);      /* the lower limit at the break */
+        }
+    }
+}
+
+/* d/d(directions) of orc_remnant_mass: each term's integral over the IMF's mass normalisation */
+static void remnant_mass_grad(int imf_type, const double *p, double mmax, double *d) {
+    d[0] = 0.0; d[1] = 0.0; d[2] = 0.0;
+    const double hi = p[ORC_IMF_UPPER_LIMIT], lbh = p[ORC_IMF_MLIM_BH], lns = p[ORC_IMF_MLIM_NS];
+    const double norm = orc_imf_integral(imf_type, p, p[ORC_IMF_LOWER_LIMIT], hi, 1);
+    if (norm <= 0.0) return;
+    double dn[3], g[3], num = 0.0, dnum[3] = {0.0, 0.0, 0.0};
+    imf_integral_grad(imf_type, p, p[ORC_IMF_LOWER_LIMIT], hi, 1, dn);
+    double a = fmin(fmax(lbh, mmax), hi);
+    if (a < hi) {
+        num += 0.5 * orc_imf_integral(imf_type, p, a, hi, 1);
+        imf_integral_grad(imf_type, p, a, hi, 1, g);
+        for (int i = 0; i < 3; i++) dnum[i] += 0.5 * g[i];
+    }
+    if (mmax <= lbh) {
+        a = fmin(fmax(lns, mmax), hi);
+        const double b = fmin(lbh, hi);
+        if (a < b) {
+            num += 1.4 * orc_imf_integral(imf_type, p, a, b, 0);
+            imf_integral_grad(imf_type, p, a, b, 0, g);
+            for (int i = 0; i < 3; i++) dnum[i] += 1.4 * g[i];
+        }
+    }
+    if (mmax <= lns) {
+        a = fmin(mmax, hi);
+        const double b = fmin(lns, hi);
+        if (a < b) {
+            num += 0.48 * orc_imf_integral(imf_type, p, a, b, 0) + 0.077 * orc_imf_integral(imf_type, p, a, b, 1);
+            imf_integral_grad(imf_type, p, a, b, 0, g);
+            for (int i = 0; i < 3; i++) dnum[i] += 0.48 * g[i];
+            imf_integral_grad(imf_type, p, a, b, 1, g);
+            for (int i = 0; i < 3; i++) dnum[i] += 0.077 * g[i];
+        }
+    }
+    for (int i = 0; i < 3; i++) d[i] = dnum[i] / norm - num * dn[i] / (norm * norm);
+}
+
+/* The reverse of orc_ops_model for one model (ssp_batch_age_sums_adjoint, opensps_ssp_batch.f90:991-1248, with
+ * adj_template_bar, adj_fold_gather, adj_lq, adj_mass_dot and adj_imf_analytic): given the cotangents yb, ob of the
+ * young and old age sums, lsb of the linear L_bol sum (sum_t wsfh_t slum_t) and mb of the mass (sum_t wsfh_t
+ * smass_t), and the forward's smass, slum (orc_ops_model's outputs):
+ *   vyb, vob = etpl yb, etpl ob                         (the transposed template products)
+ *   g_k = sum_j ent_val(k, j) v_b(ent_tpl(k, j))          (the fold's transpose; young or old by the age)
+ *   wfold_t = sum_pass cz sum_k w_k g_k                   (d L / d wsfh_t through the fold)
+ *   dzb = sum_pass sign sum_t wsfh_t (sum_k w_k g_k + mb sm_t + lsb slum_t ln sl_t)
+ *   imfb_i = sum_pass cz sum_t wsfh_t (sum_k dw_k/di (g_k + mb cm_k + lsb slum_t cl_k / sl_t) + mb drem_t/di)
+ * with w_k = I_k / total, dw_k = dI_k / total - w_k dtotal / total. Passes of zero weight are left out, as the
+ * forward leaves them (group_models). Scratch: vyb, vob (ntpl), w, dw (3 x the largest age's points), llo, dl. */
+void orc_ops_model_adj(const orc_ops_t *op, int nspec, int imf_type, int remnants, const double *pimf, int zlo,
+                       double dz, const double *wsfh, int it_tesc, const double *yb, const double *ob, double lsb,
+                       double mb, const double *slum, double *wfold, double *dzb, double *imfb, double *vyb,
+                       double *vob, double *w, double *dw, double *llo, double *dl) {
+    const int nt = op->nt, nent = op->nent;
+    const int npass = op->nz > 1 ? 2 : 1;
+    gl_init();
+    const double tot = orc_imf_integral(imf_type, pimf, pimf[ORC_IMF_LOWER_LIMIT], pimf[ORC_IMF_UPPER_LIMIT], 1);
+    double dtot[3];
+    imf_integral_grad(imf_type, pimf, pimf[ORC_IMF_LOWER_LIMIT], pimf[ORC_IMF_UPPER_LIMIT], 1, dtot);
+    const double inv_total = tot > 0.0 ? 1.0 / tot : 1.0;
+    for (int e = 0; e < op->ntpl; e++) {
+        const double *s = op->etpl + (size_t)e * nspec;
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < nspec; i++) { a += s[i] * yb[i]; b += s[i] * ob[i]; }
+        vyb[e] = a;
+        vob[e] = b;
+    }
+    for (int t = 0; t < nt; t++) wfold[t] = 0.0;
+    *dzb = 0.0;
+    imfb[0] = 0.0; imfb[1] = 0.0; imfb[2] = 0.0;
+    for (int pass = 0; pass < npass; pass++) {
+        const int iz = (npass == 1) ? 0 : zlo - 1 + pass;
+        const double cz = (npass == 1) ? 1.0 : (pass == 0 ? 1.0 - dz : dz), sgn = pass == 0 ? -1.0 : 1.0;
+        if (cz == 0.0) continue;
+        for (int t = 0; t < nt; t++) {
+            const int k0 = op->age_ptr[iz * nt + t], k1 = op->age_ptr[iz * nt + t + 1];
+            const double *vb = (t + 1 <= it_tesc) ? vyb : vob;
+            for (int k = k0; k < k1; k++) orc_imf_bin_logs(op->m1[k], op->m2[k], &llo[k - k0], &dl[k - k0]);
+            for (int q0 = k0; q0 < k1; q0 += ORC_PIECE) {
+                const int np = k1 - q0 < ORC_PIECE ? k1 - q0 : ORC_PIECE;
+                orc_imf_bin_weights_tab(imf_type, pimf, tot, np, op->mini + q0, op->m1 + q0, op->m2 + q0,
+                                        llo + (q0 - k0), dl + (q0 - k0), op->first + q0, w + (q0 - k0));
+            }
+            double sm = 0.0, sl = 0.0, fk = 0.0;
+            for (int k = k0; k < k1; k++) {
+                sm += op->cm[k] * w[k - k0];
+                sl += op->cl[k] * w[k - k0];
+            }
+            /* the bins' derivatives (zero where the limits or an empty bin give no weight) */
+            for (int k = k0; k < k1; k++) {
+                double *d = dw + 3 * (k - k0);
+                d[0] = 0.0; d[1] = 0.0; d[2] = 0.0;
+                const double lo = op->first[k] == 1 ? pimf[ORC_IMF_LOWER_BOUND] : op->m1[k], hi = op->m2[k];
+                if (op->mini[k] < pimf[ORC_IMF_LOWER_LIMIT] || op->mini[k] > pimf[ORC_IMF_UPPER_LIMIT] || hi <= lo)
+                    continue;
+                double di[3];
+                imf_integral_grad(imf_type, pimf, lo, hi, 0, di);
+                for (int i = 0; i < 3; i++) d[i] = di[i] * inv_total - w[k - k0] * dtot[i] * inv_total;
+            }
+            if (remnants && k1 > k0) sm += orc_remnant_mass(imf_type, pimf, op->age_mmax[iz * nt + t]);
+            const double sl_on = sl > SAFE_FLOOR ? 1.0 : 0.0;
+            const double lfac = sl_on * lsb * slum[t] / (sl > SAFE_FLOOR ? sl : 1.0);
+            for (int k = k0; k < k1; k++) {
+                double g = 0.0;
+                for (int j = 0; j < nent; j++) {
+                    const int e = op->ent_tpl[(size_t)k * nent + j];
+                    if (e >= 0) g += op->ent_val[(size_t)k * nent + j] * vb[e];
+                }
+                fk += w[k - k0] * g;
+                const double ge = g + mb * op->cm[k] + lfac * op->cl[k];
+                for (int i = 0; i < 3; i++) imfb[i] += cz * wsfh[t] * ge * dw[3 * (k - k0) + i];
+            }
+            if (remnants && k1 > k0) {
+                double dr[3];
+                remnant_mass_grad(imf_type, pimf, op->age_mmax[iz * nt + t], dr);
+                for (int i = 0; i < 3; i++) imfb[i] += cz * wsfh[t] * mb * dr[i];
+            }
+            wfold[t] += cz * fk;
+            *dzb += sgn * wsfh[t] * (fk + mb * sm + lsb * slum[t] * log(fmax(sl, SAFE_FLOOR)));
+        }
+    }
+    if (npass == 1) *dzb = 0.0;
+}
+
+/* ---------------------------------------------------------------------------------------------
  * The batch (src/physics/opensps_csp_batch.f90:431-1214, the stages that the configurations of
  * this build exercise: parametric or given SFH weights, the resident bases blended in logzsol,
  * power-law dust with birth cloud and old-star screens, per-model IGM, velocity smoothing,
@@ -1227,6 +1453,22 @@ typedef struct {
      * opensps_csp_batch.f90:3525-3566) or NULL: the power law */
     const double *dcurve;
 } orc_ctx_t;
+
+/* The model's IMF parameters (batch_imf_vectors, opensps_csp_batch.f90:1823-1853): the slopes and limits per model,
+ * the rest from the context */
+static void model_pimf(const orc_ctx_t *c, const double *p, double *pimf) {
+    for (int i = 0; i < ORC_IMF_NPAR; i++) pimf[i] = 0.0;
+    pimf[ORC_IMF_SALP] = c->salp_ind;
+    pimf[ORC_IMF_ALPHA] = p[OPS_P_IMF1];
+    pimf[ORC_IMF_ALPHA + 1] = p[OPS_P_IMF2];
+    pimf[ORC_IMF_ALPHA + 2] = p[OPS_P_IMF3];
+    pimf[ORC_IMF_MDAVE] = c->mdave;
+    pimf[ORC_IMF_LOWER_LIMIT] = p[OPS_P_IMF_LOWER];
+    pimf[ORC_IMF_UPPER_LIMIT] = p[OPS_P_IMF_UPPER];
+    pimf[ORC_IMF_LOWER_BOUND] = p[OPS_P_IMF_LOWER];
+    pimf[ORC_IMF_MLIM_BH] = c->mlim_bh;
+    pimf[ORC_IMF_MLIM_NS] = c->mlim_ns;
+}
 
 /* Per-model birth-cloud age index: batch_model_indices (opensps_csp_batch.f90:1541-1576) */
 static int model_it_tesc(const orc_ctx_t *c, double dust_tesc) {
@@ -1369,17 +1611,8 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
              * ssp_batch_age_sums_z, model_sums MSUMS_OWN) */
             double ms = 0.0, ls = 0.0;
             if (c->ops) {
-                double pimf[ORC_IMF_NPAR] = {0};
-                pimf[ORC_IMF_SALP] = c->salp_ind;
-                pimf[ORC_IMF_ALPHA] = p[OPS_P_IMF1];
-                pimf[ORC_IMF_ALPHA + 1] = p[OPS_P_IMF2];
-                pimf[ORC_IMF_ALPHA + 2] = p[OPS_P_IMF3];
-                pimf[ORC_IMF_MDAVE] = c->mdave;
-                pimf[ORC_IMF_LOWER_LIMIT] = p[OPS_P_IMF_LOWER];
-                pimf[ORC_IMF_UPPER_LIMIT] = p[OPS_P_IMF_UPPER];
-                pimf[ORC_IMF_LOWER_BOUND] = p[OPS_P_IMF_LOWER];
-                pimf[ORC_IMF_MLIM_BH] = c->mlim_bh;
-                pimf[ORC_IMF_MLIM_NS] = c->mlim_ns;
+                double pimf[ORC_IMF_NPAR];
+                model_pimf(c, p, pimf);
                 for (int k = 0; k < nt; k++) t1[k] = w[k] > SAFE_FLOOR ? w[k] : 0.0;
                 orc_ops_model(c->ops, nspec, c->imf_type, c->remnants, pimf, zlo, dz, t1, it, y, o, smass, slum, vy, vo,
                               wpt, llo, dl, c->op_count ? c->op_count + 2 * m : NULL);
@@ -1520,7 +1753,8 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
                        const double *lbol_bar, double *mags_out, double *params_bar, double *weights_bar,
                        double *weights_z_bar) {
     const int nspec = c->nspec, nt = c->nt, nz = c->nz, nb = c->nbands;
-    if (n < 0 || nspec < 2 || nt < 2 || nz < 1 || c->ops) return 3;
+    if (n < 0 || nspec < 2 || nt < 2 || nz < 1) return 3;
+    if (c->ops && c->ssp_weights_z) return 2;
     if (n == 0) return 0;
     const double amzl = abs_mag_zeropoint_log(), ln10 = log(10.0);
     int status = 0;
@@ -1560,6 +1794,20 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
         int *soff = (int *)malloc(sizeof(int) * nspec);
         double *scoef = (double *)malloc(sizeof(double) * cap);
         double *wbl = (double *)malloc(sizeof(double) * nt), *tsc = (double *)malloc(sizeof(double) * 6 * nt);
+        double *smass = NULL, *slum = NULL, *vy = NULL, *vo = NULL, *vyb = NULL, *vob = NULL, *wpt = NULL, *dwp = NULL;
+        double *llo = NULL, *dl = NULL, *wfold = NULL;
+        if (c->ops) {
+            int maxpts = 1;
+            for (int a = 0; a < c->ops->nz * nt; a++)
+                if (c->ops->age_ptr[a + 1] - c->ops->age_ptr[a] > maxpts) maxpts = c->ops->age_ptr[a + 1] - c->ops->age_ptr[a];
+            const size_t ntp = (size_t)c->ops->ntpl;
+            smass = (double *)malloc(sizeof(double) * nt); slum = (double *)malloc(sizeof(double) * nt);
+            wfold = (double *)malloc(sizeof(double) * nt);
+            vy = (double *)malloc(sizeof(double) * ntp); vo = (double *)malloc(sizeof(double) * ntp);
+            vyb = (double *)malloc(sizeof(double) * ntp); vob = (double *)malloc(sizeof(double) * ntp);
+            wpt = (double *)malloc(sizeof(double) * maxpts); dwp = (double *)malloc(sizeof(double) * 3 * maxpts);
+            llo = (double *)malloc(sizeof(double) * maxpts); dl = (double *)malloc(sizeof(double) * maxpts);
+        }
 
         #pragma omp for schedule(dynamic, 4)
         for (int m = 0; m < n; m++) {
@@ -1578,7 +1826,26 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
             double ms = 0.0, ls = 0.0;
             for (int i = 0; i < nspec; i++) { y[i] = 0.0; o[i] = 0.0; }
             const double *wz = wz_on ? c->ssp_weights_z + (size_t)m * nt * nz : NULL;
-            if (wz_on) {
+            double pimf[ORC_IMF_NPAR];
+            if (c->ops) {
+                /* the IMF per model: the operator's age sums and per-age sums, as orc_csp_batch */
+                if (ssp_weights) {
+                    for (int k = 0; k < nt; k++) w[k] = ssp_weights[(size_t)m * nt + k];
+                } else {
+                    int st = orc_sfh_weights(c->time_full, nt, c->tiny_logt, p, w, t1, t2);
+                    if (st != 0) {
+                        #pragma omp atomic write
+                        status = st;
+                        continue;
+                    }
+                }
+                model_pimf(c, p, pimf);
+                for (int k = 0; k < nt; k++) t1[k] = w[k] > SAFE_FLOOR ? w[k] : 0.0;
+                orc_ops_model(c->ops, nspec, c->imf_type, c->remnants, pimf, zlo, dz, t1, it, y, o, smass, slum, vy, vo,
+                              wpt, llo, dl, NULL);
+                for (int k = 0; k < nt; k++)
+                    if (w[k] > SAFE_FLOOR) { ms = ms + w[k] * smass[k]; ls = ls + w[k] * slum[k]; }
+            } else if (wz_on) {
                 for (int zz = 0; zz < nz; zz++)
                     for (int k = 1; k <= nt; k++) {
                         double wk = wz[(size_t)zz * nt + k - 1];
@@ -1803,8 +2070,16 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
             /* the age sums */
             const double lb = lbol_bar ? lbol_bar[m] : 0.0, mb = mass_bar ? mass_bar[m] : 0.0;
             const double lsb = ls > SAFE_FLOOR ? lb / (ls * ln10) : 0.0;
-            double dzb = 0.0;
-            if (wz_on) {
+            double dzb = 0.0, imfb[3] = {0.0, 0.0, 0.0};
+            if (c->ops) {
+                /* the operator's reverse pass: the fold's and the per-age sums' parts of w_bar, dz, the IMF rows */
+                orc_ops_model_adj(c->ops, nspec, c->imf_type, c->remnants, pimf, zlo, dz, t1, it, yb, ob, lsb, mb, slum,
+                                  wfold, &dzb, imfb, vyb, vob, wpt, dwp, llo, dl);
+                for (int k = 0; k < nt; k++) {
+                    wbl[k] = wfold[k] + lsb * slum[k] + mb * smass[k];
+                    if (weights_bar) weights_bar[(size_t)m * nt + k] = wbl[k];
+                }
+            } else if (wz_on) {
                 for (int zz = 0; zz < nz; zz++)
                     for (int k = 1; k <= nt; k++) {
                         const double *bs = c->basis + ((size_t)zz * nt + k - 1) * nspec;
@@ -1850,6 +2125,15 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
                 for (int r = 0; r < ORC_NBGRAD; r++) pb[r] = 0.0;
                 for (int r = 1; r <= 7; r++) pb[r] = d[r];
                 if (blend) pb[15] = dzb / (c->logz[zlo] - c->logz[zlo - 1]);
+                if (c->ops) {
+                    /* imf1, imf2 (BGRAD 27, 28), then imf3 (29) for Kroupa or mdave (31) for Dave */
+                    const int ty = c->imf_type % 10;
+                    if (ty == 2 || ty == 4) {
+                        pb[26] = imfb[0];
+                        pb[27] = imfb[1];
+                        pb[ty == 2 ? 28 : 30] = imfb[2];
+                    }
+                }
                 pb[22] = dzo + d13 + dzd;
                 pb[23] = dsg;
                 pb[32] = d14;
@@ -1866,6 +2150,8 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
         free(p); free(w); free(t1); free(t2); free(y); free(o); free(A); free(S0); free(S1); free(tig); free(tauv);
         free(work); free(fobs); free(fb); free(fbb); free(vb); free(g1); free(g0); free(yb); free(ob);
         free(slo); free(shi); free(soff); free(scoef); free(wbl); free(tsc);
+        free(smass); free(slum); free(wfold); free(vy); free(vo); free(vyb); free(vob); free(wpt); free(dwp);
+        free(llo); free(dl);
     }
     free(jlo); free(jhi); free(do_band); free(bt); free(wtl); free(ltab); free(pa);
     return status;

@@ -383,19 +383,35 @@ def _context(grid, bands, add_igm, smooth_fast, mag_mask, cosmo, ptr):
     return ctx
 
 
+def _set_ops(ctx, ops, ptr, keep, imf_type, remnants, salp_ind, mdave, mlim_bh, mlim_ns):
+    """The operator set (synth.SSPOperator) and the IMF settings on an oracle context."""
+    o = _Ops()
+    o.nz, o.nt, o.npts, o.ntpl, o.nent = ops.nz, ops.nt, ops.npts, ops.ntpl, ops.nent
+    for k in ("age_ptr", "first", "ent_tpl"):
+        setattr(o, k, ptr(_i(getattr(ops, k))))
+    for k in ("mini", "m1", "m2", "cm", "cl", "q", "age_mmax", "ent_val", "etpl"):
+        setattr(o, k, ptr(_d(getattr(ops, k))))
+    keep.append(o)
+    ctx.ops = C.cast(C.pointer(o), C.c_void_p)
+    ctx.imf_type, ctx.remnants = int(imf_type), int(remnants)
+    ctx.salp_ind, ctx.mdave, ctx.mlim_bh, ctx.mlim_ns = salp_ind, mdave, mlim_bh, mlim_ns
+
+
 NBGRAD = 44
 # params_bar rows (0-based) this build computes: the reference's BGRAD_* - 1 (opensps_csp_batch.f90:166-213)
 GRAD_ROWS = {"dust1": 1, "dust2": 2, "dust3": 3, "dust_index": 4, "dust1_index": 5, "frac_obrun": 6,
              "frac_nodust": 7, "logzsol": 15, "zred": 22, "sigma_smooth": 23, "igm_factor": 32,
              "tage": 16, "tau": 17, "const": 18, "fburst": 19, "tburst": 20, "sf_start": 21, "sf_trunc": 24,
-             "sf_slope": 25}
+             "sf_slope": 25, "imf1": 26, "imf2": 27, "imf3": 28, "mdave": 30}
 
 
 def csp_batch_grad(grid, bands, prm, ssp_weights=None, ssp_weights_z=None, mags_bar=None, spec_bar=None,
-                   mass_bar=None, lbol_bar=None, add_igm=1, smooth_fast=1, mag_mask=None, cosmo=None):
+                   mass_bar=None, lbol_bar=None, add_igm=1, smooth_fast=1, mag_mask=None, cosmo=None, ops=None,
+                   imf_type=2, remnants=1, salp_ind=2.35, mdave=0.5, mlim_bh=40.0, mlim_ns=8.5):
     """The oracle's reverse pass (FP64). prm (NPAR, n); ssp_weights (n, nt); ssp_weights_z (n, nz, nt);
-    cotangents mags_bar (n, nbands), spec_bar (n, nspec), mass_bar, lbol_bar (n). Returns mags (n, nbands),
-    params_bar (n, NBGRAD), weights_bar (n, nt), weights_z_bar (n, nz, nt) and the status."""
+    cotangents mags_bar (n, nbands), spec_bar (n, nspec), mass_bar, lbol_bar (n); ops: a synth.SSPOperator (the
+    IMF per model and its rows imf1, imf2 and imf3 or mdave). Returns mags (n, nbands), params_bar (n, NBGRAD),
+    weights_bar (n, nt), weights_z_bar (n, nz, nt) and the status."""
     keep = []
 
     def ptr(a):
@@ -407,6 +423,8 @@ def csp_batch_grad(grid, bands, prm, ssp_weights=None, ssp_weights_z=None, mags_
     ctx = _context(grid, bands, add_igm, smooth_fast, mag_mask, cosmo, ptr)
     if ssp_weights_z is not None:
         ctx.ssp_weights_z = ptr(_d(ssp_weights_z))
+    if ops is not None:
+        _set_ops(ctx, ops, ptr, keep, imf_type, remnants, salp_ind, mdave, mlim_bh, mlim_ns)
     out = {"mags": np.zeros((n, bands.shape[0])), "params_bar": np.zeros((n, NBGRAD)),
            "weights_bar": np.zeros((n, grid.nt)), "weights_z_bar": np.zeros((n, grid.nz, grid.nt))}
     vp = C.c_void_p
@@ -460,16 +478,7 @@ def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_
     ctx.smooth_fast = int(smooth_fast)
     ctx.tiny_logt = 0.0
     if ops is not None:
-        o = _Ops()
-        o.nz, o.nt, o.npts, o.ntpl, o.nent = ops.nz, ops.nt, ops.npts, ops.ntpl, ops.nent
-        for k in ("age_ptr", "first", "ent_tpl"):
-            setattr(o, k, ptr(_i(getattr(ops, k))))
-        for k in ("mini", "m1", "m2", "cm", "cl", "q", "age_mmax", "ent_val", "etpl"):
-            setattr(o, k, ptr(_d(getattr(ops, k))))
-        keep.append(o)
-        ctx.ops = C.cast(C.pointer(o), C.c_void_p)
-        ctx.imf_type, ctx.remnants = int(imf_type), int(remnants)
-        ctx.salp_ind, ctx.mdave, ctx.mlim_bh, ctx.mlim_ns = salp_ind, mdave, mlim_bh, mlim_ns
+        _set_ops(ctx, ops, ptr, keep, imf_type, remnants, salp_ind, mdave, mlim_bh, mlim_ns)
         if "op_count" in want:
             cnt = np.zeros((n, 2), np.int32)
             keep.append(cnt)

@@ -334,6 +334,39 @@ def test_api_imf_per_model(precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_api_imf_grad(precision):
+    """The batch with gradients and the IMF per model through the public C API (compute_csp_batch_grad with an
+    operator on the context): params_bar's IMF rows (imf1 .. imf3), logzsol and the dust and SFH rows against the
+    oracle's operator reverse pass; magnitude cotangents on the bands within 30 mag of each model's brightest."""
+    from helpers import operator
+    nz = 3
+    g = grid(nz)
+    op = operator(nz, 40, 300)
+    fs = synth.gaussian_filters(20)
+    n = 32
+    prm, names = _imf_models(n, nz, 57)
+    print(f"case: api imf grad n={n} {precision}", flush=True)
+    rng = np.random.default_rng(59)
+    with _ctx(g, fs, precision) as ctx:
+        ctx.set_ssp_ops(op)
+        bands = ctx.bands()
+        fwd = O.csp_batch(g, bands, prm, add_igm=1, cosmo=O.cosmology_table(), ops=op, imf_type=2, remnants=1)
+        mb = np.where(fwd["mags"] <= np.nanmin(fwd["mags"], axis=1, keepdims=True) + 30.0,
+                      rng.standard_normal((n, 20)), 0.0)
+        r = ctx.compute_csp_batch_grad({k: prm[SLOT[k]] for k in names}, n=n, mags_bar=mb, grads=("params_bar",))
+    ref = O.csp_batch_grad(g, bands, prm, mags_bar=mb, cosmo=O.cosmology_table(), ops=op, imf_type=2, remnants=1)
+    assert ref["status"] == 0
+    tol = 1e-8 if precision == "double" else 1e-4
+    for name in ("imf1", "imf2", "imf3", "logzsol", "dust2", "tage", "tau", "zred"):
+        row = O.GRAD_ROWS[name]
+        e = float(np.max(np.abs(r["params_bar"][:, row] - ref["params_bar"][:, row]))) / (
+            np.max(np.abs(ref["params_bar"][:, row])) + 1e-300)
+        print(f"  {name:8s} {e:.2e}")
+        assert e < tol, name
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("imf_type", [2, 0])
 def test_api_evaluate_ssp_op(imf_type):
     """opensps_context_evaluate_ssp_op: the SSP of every age for one IMF (ssp_operator_evaluate), at the context's

@@ -62,18 +62,21 @@ ROWS = ["dust1", "dust2", "dust3", "dust_index", "dust1_index", "frac_obrun", "f
         "sigma_smooth", "igm_factor"]
 
 
-def _compare(ref, got, precision, rows):
-    tol = TOL[precision]
+def _compare(ref, got, precision, rows, tols=TOL):
+    tol = tols[precision]
     em = mag_err(got["mags"], ref["mags"])
     print(f"  mags {em:.2e}")
     assert em < tol["mags"]
     pr, pg = ref["params_bar"], got["params_bar"]
+    errs = {}
     for name in rows:
         r = O.GRAD_ROWS[name]
         scale = np.max(np.abs(pr[:, r])) + 1e-300
-        e = float(np.max(np.abs(pg[:, r] - pr[:, r]))) / scale
-        print(f"  {name:12s} {e:.2e}")
-        assert e < tol["params"], name
+        err = np.abs(pg[:, r] - pr[:, r]) / scale
+        errs[name] = float(np.max(err))
+        print(f"  {name:12s} {errs[name]:.2e} (model {int(np.argmax(err))})")
+    for name in rows:
+        assert errs[name] < tol["params"], name
     other = [r for r in range(O.NBGRAD) if r not in [O.GRAD_ROWS[k] for k in rows]]
     assert np.all(pg[:, other] == 0.0)
 
@@ -133,7 +136,7 @@ def test_grad_blend_weights(precision):
 
 
 def test_grad_statuses():
-    """The operator's reverse pass (the IMF rows) is status 2; weights_bar alone is computed with SFH parameters."""
+    """weights_bar alone is computed with SFH parameters."""
     from libopensps_amd import hip as H
     import torch
     g, b = _setup()
@@ -290,3 +293,61 @@ def test_autograd_sfh_parameters():
         err = float(np.max(np.abs(a - e)) / (np.max(np.abs(e)) + 1e-300))
         print(f"  {name:12s} {err:.2e}")
         assert err < 1e-9 or np.all(a == e), name
+
+
+IMF_ROWS = {2: ["imf1", "imf2", "imf3"], 4: ["imf1", "imf2", "mdave"], 0: []}
+# the operator's forward in FP64 sums 10^4-template products and the fold in another order than the oracle (its
+# magnitudes agree to 1e-7 there, the faintest bands ~40 mag below the brightest setting the figure)
+TOL_OP = {0: {"params": 1.0e-8, "weights": 1.0e-8, "mags": 1.0e-7},
+          1: {"params": 1.0e-4, "weights": 5.0e-5, "mags": 1.0e-2}}
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("imf_type,nz,remnants", [(2, 3, 1), (4, 3, 1), (0, 1, 0)])
+def test_grad_operator(precision, imf_type, nz, remnants):
+    """The reverse pass with the IMF per model (the operator: k_op_adj_t, k_op_imf_grad, k_op_adj_points,
+    k_op_adj_out after the transposed template product) against the oracle's (orc_ops_model_adj, pinned by central
+    differences): the IMF rows (Kroupa's three slopes, Dave's two slopes and break, none for Salpeter), logzsol
+    through both metallicities' operators, the SFH rows from w_bar, the chain's rows, and weights_bar."""
+    from helpers import bands, grid, operator
+    op, g, b = operator(nz, 40, 300), grid(nz), bands(NB)
+    n = 40
+    prm = synth.parametric_params(n, seed=81, logzsol=(-2.1, 0.45) if nz > 1 else (0.0, 0.0))
+    prm = synth.imf_params(n, seed=82, p=prm)
+    prm[SLOT["dust1"]] = np.linspace(0.0, 1.0, n)
+    prm[SLOT["imf_lower_limit"]] = np.linspace(0.08, 0.6, n)
+    rng = np.random.default_rng(83)
+    cot = {"mags": rng.standard_normal((n, NB)), "spec": rng.standard_normal((n, g.nspec)) * 1e2,
+           "mass": rng.standard_normal(n), "lbol": rng.standard_normal(n)}
+    mdave = 0.7
+    # magnitude cotangents on the bands within 30 mag of each model's brightest: a band 40 mag fainter carries FP32
+    # subnormal fluxes in the mixed chain, and its d mag / d F = -2.5 / (F ln 10) would set the mixed figures
+    fwd = O.csp_batch(g, b, prm, add_igm=1, cosmo=cosmo(), want=("mags",), ops=op, imf_type=imf_type,
+                      remnants=remnants, mdave=mdave)["mags"]
+    cot["mags"] = np.where(fwd <= np.nanmin(fwd, axis=1, keepdims=True) + 30.0, cot["mags"], 0.0)
+    ref = O.csp_batch_grad(g, b, prm, mags_bar=cot["mags"], spec_bar=cot["spec"], mass_bar=cot["mass"],
+                           lbol_bar=cot["lbol"], cosmo=cosmo(), ops=op, imf_type=imf_type, remnants=remnants,
+                           mdave=mdave)
+    assert ref["status"] == 0
+    ctx = _ctx(g, b)
+    ctx.set_ssp_ops(op)
+    ctx.set_imf(imf_type, remnants, 2.35, mdave, 40.0, 8.5)
+    print(f"case: grad operator n={n} nz={nz} imf={imf_type} precision={precision}", flush=True)
+    out = ctx.batch_grad(_dev(prm), {k: _dev(v) for k, v in cot.items()}, precision=precision, split_parts=1,
+                         grads=("params_bar", "weights_bar"), use_ops=1)
+    got = {k: v.detach().cpu().numpy() for k, v in out.items()}
+    ctx.close()
+    rows = ["dust1", "dust2", "dust3", "dust_index", "dust1_index", "frac_obrun", "frac_nodust", "zred",
+            "sigma_smooth", "igm_factor"] + SFH_ROWS + (["logzsol"] if nz > 1 else []) + IMF_ROWS[imf_type]
+    # the magnitudes of the bands within 30 mag of each model's brightest (the fainter ones to TOL_OP's figure). The
+    # metallicities below and above the grid extrapolate (dz < 0 or > 1): one operator's sums are then subtracted
+    # from the other's, which magnifies the summation-order differences (FP64: ~1e-9 mag there, 1e-12 inside)
+    rm = ref["mags"]
+    near = rm <= np.nanmin(rm, axis=1, keepdims=True) + 30.0
+    en = float(np.nanmax(np.abs(got["mags"] - rm)[near]))
+    print(f"  mags within 30 mag {en:.2e}")
+    assert en < (1.0e-8 if precision == 0 else 1.0e-3)
+    _compare(ref, got, precision, rows, TOL_OP)
+    ew = _rel(got["weights_bar"], ref["weights_bar"])
+    print(f"  weights_bar {ew:.2e}")
+    assert ew < TOL_OP[precision]["weights"]

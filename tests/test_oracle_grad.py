@@ -226,3 +226,60 @@ def test_sfh_rows_not_with_given_weights():
     an = _grad(g, b, cz, prm, _cot(n, g, seed=19), ssp_weights=w)["params_bar"]
     for name in O.SFHT_PARAMS:
         assert np.all(an[:, O.GRAD_ROWS[name]] == 0.0), name
+
+
+@pytest.mark.parametrize("imf_type", [2, 4])
+def test_params_bar_operator(imf_type):
+    """The operator's reverse pass (the IMF per model): the IMF rows (imf1, imf2 and imf3 for Kroupa, mdave for Dave,
+    a context setting differenced there), logzsol through both metallicities' operators and a dust row, against
+    central differences of the oracle's operator batch with every cotangent (magnitudes, spectrum, mass, L_bol) in
+    the loss. (The SFH rows come from w_bar, pinned below, by the path the resident bases' tests pin.)"""
+    from helpers import bands, cosmo, grid, operator
+    nz = 3
+    op, g, b, cz = operator(nz, 40, 300), grid(nz), bands(NB), cosmo()
+    n = 4
+    prm = synth.parametric_params(n, seed=61, logzsol=(-2.1, 0.45))
+    prm = synth.imf_params(n, seed=62, p=prm)
+    prm[SLOT["dust1"]] = 0.5   # a birth-cloud screen: the young and old sums apart
+    kw = dict(ops=op, imf_type=imf_type, remnants=1, mdave=0.6)
+    rng = np.random.default_rng(63)
+    cot = {"mags": rng.standard_normal((n, NB)), "spec": rng.standard_normal((n, g.nspec)) * 1e3,
+           "mass": rng.standard_normal(n), "lbol": rng.standard_normal(n)}
+    pb = _grad(g, b, cz, prm, cot, **kw)["params_bar"]
+    names = ["imf1", "imf2"] + (["imf3"] if imf_type == 2 else []) + ["logzsol", "dust2"]
+    for name in names:
+        fd = _fd_param(g, b, cz, prm, cot, SLOT[name], **kw)
+        _check(fd, pb[:, O.GRAD_ROWS[name]], 2e-5, name)
+    if imf_type == 4:
+        h = 0.6e-6
+        lp = _loss(_fwd(g, b, cz, prm, **{**kw, "mdave": 0.6 + h}), cot)
+        lm = _loss(_fwd(g, b, cz, prm, **{**kw, "mdave": 0.6 - h}), cot)
+        _check((lp - lm) / (2 * h), pb[:, O.GRAD_ROWS["mdave"]], 2e-5, "mdave")
+    else:
+        assert np.all(pb[:, O.GRAD_ROWS["mdave"]] == 0.0)
+
+
+def test_weights_bar_operator():
+    """The operator's w_bar (the fold's transpose plus the per-age mass and L_bol terms) with given SSP weights,
+    against central differences along single weights (h = 1e-4 relative: the loss is smooth in a weight, and a
+    smaller step meets the loss's rounding)."""
+    from helpers import bands, cosmo, grid, operator
+    nz = 3
+    op, g, b, cz = operator(nz, 40, 300), grid(nz), bands(NB), cosmo()
+    n = 3
+    prm = synth.parametric_params(n, seed=71, logzsol=(-2.1, 0.45))
+    prm = synth.imf_params(n, seed=72, p=prm)
+    w = synth.dirichlet_weights(n, g.nt, seed=73)
+    kw = dict(ops=op, imf_type=2, remnants=1)
+    rng = np.random.default_rng(74)
+    cot = {"mags": rng.standard_normal((n, NB)), "spec": rng.standard_normal((n, g.nspec)) * 1e3,
+           "mass": rng.standard_normal(n), "lbol": rng.standard_normal(n)}
+    wb = _grad(g, b, cz, prm, cot, ssp_weights=w, **kw)["weights_bar"]
+    for t in (3, 40, 90):
+        h = 1e-4 * np.maximum(w[:, t], 1e-3)
+        wp, wm = w.copy(), w.copy()
+        wp[:, t] += h
+        wm[:, t] -= h
+        fd = (_loss(_fwd(g, b, cz, prm, ssp_weights=wp, **kw), cot) -
+              _loss(_fwd(g, b, cz, prm, ssp_weights=wm, **kw), cot)) / (2 * h)
+        _check(fd, wb[:, t], 2e-5, f"weights_bar[{t}]")
