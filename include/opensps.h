@@ -135,8 +135,11 @@ void opensps_context_compute_csp_batch_parts(int handle, int n, int n_params, co
  * ssp_weights_z (ntfull x nz x n) replaces ssp_weights and the metallicity blend. This build computes the rows of
  * dust1, dust2, dust3, dust_index, dust1_index, frac_obrun, frac_nodust, logzsol, zred, sigma_smooth, igm_factor
  * and, with SFH parameters (no weights given; sfh 0, 1, 4, 5), tage, tau, const, fburst, tburst, sf_start, sf_trunc
- * and sf_slope (the others are 0); zred_mags and the SSP operator's rows are status 2. _rows: params_bar's leading dimension (fewer than NBGRAD: status 3, nothing
- * written; rows beyond NBGRAD are left alone). */
+ * and sf_slope and, with SSP operators on the context, imf1, imf2 and imf3 (Kroupa) or imf1, imf2 and mdave (Dave)
+ * (the others are 0). zred_mags (n, or NULL): each model's magnitude redshift (the observed frame and the distance
+ * term at it, the IGM at zred; zred's row is then the IGM's part); redshift_colors = 1 needs it (else status 2).
+ * _rows: params_bar's leading dimension (fewer than NBGRAD: status 3, nothing written; rows beyond NBGRAD are left
+ * alone). */
 void opensps_context_compute_csp_batch_grad(int handle, int n, int n_params, const char *const *names,
                                             const double *const *values, const double *ssp_weights,
                                             const double *ssp_weights_z, const double *zred_mags,

@@ -56,6 +56,10 @@ typedef struct {
     const double *ssp_weights_z; /* device (nt, nz, n) or NULL: weights over ages and metallicities (compute_csp_batch's
                                     ssp_weights_z, opensps_csp_batch.f90:616-618: age_sums_w / model_sums_zw); they
                                     replace the SFH and the per-model metallicity blend */
+    const double *zred_mags;     /* device (n) or NULL: each model's magnitude redshift (compute_csp_batch's zred_mags,
+                                    opensps_csp_batch.f90:1054-1061): the observed frame and the distance term at it,
+                                    the IGM at zred; the reverse pass then gives zred's row the IGM's part alone
+                                    (batch_backward's lzm, :4830-4832, 4928) */
 } opensps_hip_batch_t;
 
 /* The reverse pass of a batch (compute_csp_batch with params_bar / weights_bar / weights_z_bar, include/opensps.h:

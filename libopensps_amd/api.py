@@ -344,11 +344,11 @@ class Context:
     def compute_csp_batch_grad(self, values: dict, n=None, ssp_weights=None, ssp_weights_z=None, mags_bar=None,
                                spec_bar=None, mass_bar=None, lbol_bar=None, want=("mags",),
                                grads=("params_bar", "weights_bar", "weights_z_bar"), params_bar_rows=None,
-                               raw_status=False):
+                               raw_status=False, zred_mags=None):
         """The batch with gradients (opensps_context_compute_csp_batch_grad / _rows). values: {name: array of n};
-        ssp_weights (n, ntfull); ssp_weights_z (n, nz, ntfull); cotangents mags_bar (n, nbands), spec_bar
-        (n, nspec), mass_bar, lbol_bar (n). Returns the forward outputs and params_bar (n, rows), weights_bar
-        (n, ntfull), weights_z_bar (n, nz, ntfull)."""
+        ssp_weights (n, ntfull); ssp_weights_z (n, nz, ntfull); zred_mags (n): each model's magnitude redshift;
+        cotangents mags_bar (n, nbands), spec_bar (n, nspec), mass_bar, lbol_bar (n). Returns the forward outputs
+        and params_bar (n, rows), weights_bar (n, ntfull), weights_z_bar (n, nz, ntfull)."""
         names = list(values.keys())
         if n is None:
             n = len(next(iter(values.values())))
@@ -365,11 +365,11 @@ class Context:
                 out[k] = np.zeros(shapes[k])
         opt = lambda a: None if a is None else _d(a)
         ins = {k: opt(v) for k, v in (("w", ssp_weights), ("wz", ssp_weights_z), ("mb", mags_bar), ("sb", spec_bar),
-                                      ("mab", mass_bar), ("lb", lbol_bar))}
+                                      ("mab", mass_bar), ("lb", lbol_bar), ("zm", zred_mags))}
         I = lambda k: None if ins[k] is None else ins[k].ctypes.data
         P = lambda k: out[k].ctypes.data if k in out else None
         st = C.c_int()
-        args = [self.handle, n, len(names), C.cast(cnames, C.c_void_p), C.cast(cvals, C.c_void_p), I("w"), I("wz"), None,
+        args = [self.handle, n, len(names), C.cast(cnames, C.c_void_p), C.cast(cvals, C.c_void_p), I("w"), I("wz"), I("zm"),
                 I("mb"), I("sb"), I("mab"), I("lb"), P("mags"), P("spec"), P("mass"), P("lbol"), P("mdust"), P("sfr"),
                 None, None, P("params_bar")]
         if params_bar_rows is None:

@@ -970,6 +970,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     pa.smooth_fast = c->smooth_fast;
     pa.ops = ops;
     pa.wz = a->ssp_weights_z;
+    pa.zmags = a->zred_mags;
     pa.lsum = c->lsum_out;
     pa.tiny_logt = c->tiny_logt;
     pa.prm = a->prm;
@@ -1231,6 +1232,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ca.wl32 = c->d_wl32;
     ca.sm32 = c->d_sm32;
     ca.prm = a->prm;
+    ca.zmags = a->zred_mags;
     ca.yo = c->yo.p;
     ca.lam = c->d_lam;
     ca.loglam = c->d_loglam;
@@ -1425,6 +1427,7 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     ga.ybh = c->gyb.p;
     ga.ybl = mixed ? (void*)((uint16_t*)c->gyb.p + (size_t)ncols * nsp) : nullptr;
     ga.gout = (double*)c->gout.p;
+    ga.zmags = f.zred_mags;
     const size_t lds = 3 * tsz * (size_t)nsp + sizeof(double) * ((c->nbands > 0 ? c->nbands : 1) + 8 * 16);
     const void* kfn = mixed ? (const void*)k_grad_chain<float> : (const void*)k_grad_chain<double>;
     HIPCHK(c, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -230,7 +230,8 @@ __global__ __launch_bounds__(512) void k_grad_chain(GradArgs a) {
     __syncthreads();
 
     // ---- 3. the observed frame: fobs(j) = max(v(j), 0) / lambda_j, v the interpolation of S1 at lambda_j / (1 + z) ----
-    const bool rest = fabs(z) <= OPS_SAFE_FLOOR;
+    const double zm = a.zmags ? a.zmags[m] : z;   // the magnitudes' redshift
+    const bool rest = fabs(zm) <= OPS_SAFE_FLOOR;
     const double rz = h[OPS_HDR_RZ], lnopz = h[OPS_HDR_LNOPZ];
     double dzo = 0.0;
     const int jb1 = min(a.band_j1, nspec - 1);
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(512) void k_grad_chain(GradArgs a) {
         if (rest) {
             atomicAdd(&G1[j], (T)gv);
         } else {
-            dzo += gv * ((double)S1[il + 1] - (double)S1[il]) * (-a.lam[j] * rz * rz) * a.rdlam[il];
+            if (!a.zmags) dzo += gv * ((double)S1[il + 1] - (double)S1[il]) * (-a.lam[j] * rz * rz) * a.rdlam[il];
             atomicAdd(&G1[il], (T)((1.0 - t) * gv));
             if (t != 0.0) atomicAdd(&G1[il + 1], (T)(t * gv));
         }
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(512) void k_grad_chain(GradArgs a) {
         g[OPS_GOUT_DSG] = ds;
         // the distance term (distance_term_dz, opensps_batch_photometry.f90:798-813): d_L linear in the table
         double dzd = 0.0;
-        if (v[11] != 0.0 && fabs(z) > OPS_SAFE_FLOOR) {
+        if (v[11] != 0.0 && fabs(z) > OPS_SAFE_FLOOR && !a.zmags) {   // (with zred_mags: not zred's)
             const bool asc = a.cz[a.ncosmo - 1] >= a.cz[0];
             int cnt = 0;
             for (int q = 0; q < a.ncosmo; q++) cnt += (asc ? a.cz[q] <= z : z < a.cz[q]) ? 1 : 0;

@@ -1097,7 +1097,9 @@ __global__ void k_pack_bf16(PackArgs a) {
 // ------------------------------------------------------------------------------------------------
 
 __device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, double dz, int lane) {
+    // the IGM at zred; the observed frame and the distance term at the magnitudes' redshift zm (zred_mags or zred)
     const double z = p[OPS_P_ZRED], opz = 1.0 + z;
+    const double zm = a.zmags ? a.zmags[m] : z;
     double pw = 0.0;
     if (lane < 6) {
         const double e = lane == 0 ? 3.46 : lane == 1 ? 1.68 : lane == 2 ? 3.0 + 0.46
@@ -1136,7 +1138,7 @@ __device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, dou
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int q = q0 + 64 * u;
-            if (q < a.ncosmo) cnt += (asc ? a.cz[q] <= z : z < a.cz[q]) ? 1 : 0;
+            if (q < a.ncosmo) cnt += (asc ? a.cz[q] <= zm : zm < a.cz[q]) ? 1 : 0;
         }
     }
     cnt = wave_sum(cnt);
@@ -1146,15 +1148,15 @@ __device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, dou
     const bool window = smooth && minl <= maxl;
     const double sb = sig / (OPS_C_LIGHT * 1.0e-13);
     double dm = 0.0;
-    const bool dist_on = fabs(z) > OPS_SAFE_FLOOR;
+    const bool dist_on = fabs(zm) > OPS_SAFE_FLOOR;
     if (dist_on) {
         const int i = max(1, min(cnt, a.ncosmo - 1));
         const double slope = (a.cdl[i] - a.cdl[i - 1]) / (a.cz[i] - a.cz[i - 1]);
-        dm = a.cdl[i - 1] + slope * (z - a.cz[i - 1]);
+        dm = a.cdl[i - 1] + slope * (zm - a.cz[i - 1]);
     }
     // the logarithms and exponentials of the header in one call each over the lanes (the lanes' cases would
     // otherwise run one after another): ln(1 + z), ln(dm / 10), ln(minl), ln(maxl) as log1p; exp(-+4 sigma)
-    const double larg = lane == OPS_HDR_LNOPZ ? z : lane == OPS_HDR_DIST ? dm / 10.0 - 1.0
+    const double larg = lane == OPS_HDR_LNOPZ ? zm : lane == OPS_HDR_DIST ? dm / 10.0 - 1.0
                       : lane == OPS_HDR_W0 ? minl - 1.0 : lane == OPS_HDR_W1 ? maxl - 1.0 : 0.0;
     const double lg = log1p(larg);
     const double ln_opz = __shfl(lg, OPS_HDR_LNOPZ, 64);
@@ -1162,7 +1164,7 @@ __device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, dou
     const LamLookup& L = a.lk;
     double v = 0.0;
     switch (lane) {
-    case OPS_HDR_RZ: v = 1.0 / opz; break;
+    case OPS_HDR_RZ: v = 1.0 / (1.0 + zm); break;
     case OPS_HDR_LNOPZ: v = lg; break;
     case OPS_HDR_INV2: v = smooth ? 0.5 * (1.0 / sb) * (1.0 / sb) : 0.0; break;
     case OPS_HDR_FBLUE: v = smooth ? ex : 1.0; break;
@@ -1366,7 +1368,7 @@ void k_chain(ChainArgs a) {
     const bool igm_on = a.add_igm && z > OPS_SAFE_FLOOR && a.jtop > 0;
     const double inv2 = h[OPS_HDR_INV2], fblue = h[OPS_HDR_FBLUE], fred = h[OPS_HDR_FRED];
     const int w0 = (int)h[OPS_HDR_W0], w1 = (int)h[OPS_HDR_W1];
-    const bool rest = fabs(z) <= OPS_SAFE_FLOOR;
+    const bool rest = fabs(a.zmags ? a.zmags[m] : z) <= OPS_SAFE_FLOOR;   // the magnitudes' redshift
     const double rz = h[OPS_HDR_RZ], lnopz = h[OPS_HDR_LNOPZ];
     const int qb0 = a.band_j0 >> 2, qb1 = (a.band_j1 + 1) >> 2;   // band quads [qb0, qb1)
 

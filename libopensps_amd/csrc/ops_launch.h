@@ -34,6 +34,7 @@ struct PrepArgs {
     const double *prm, *weights_in, *time, *logz, *bmass, *blbol;
     const double* wz;           // weights over ages and metallicities (nt nz per model) or NULL
     double* lsum;               // per model the L_bol sum before the log (the reverse pass), or NULL
+    const double* zmags;        // per model the magnitudes' redshift (zred_mags) or NULL: zred
     double *w_out, *mass, *lbol, *w64;
     uint16_t *whi, *wlo;
     int *zlo, *it, *idx, *status, *flag;
@@ -247,6 +248,7 @@ struct ChainArgs {
     // the diffuse attenuation curve per padded pixel (a fixed curve: dust types 2, 6) or NULL (the power law)
     const double* dcurve;
     const float* dcurve32;
+    const double* zmags;   // per model the magnitudes' redshift (zred_mags) or NULL: zred
 };
 
 // ---- the reverse pass (ops_grad.hip) ----
@@ -275,6 +277,7 @@ struct GradArgs {
     const double *mags_bar, *spec_bar, *lbol_bar;
     void *ybh, *ybl;               // y_bar rows (model part) x nsp: bf16 hi / lo (mixed) or FP64 (ybh)
     double* gout;                  // per model OPS_GOUT_N
+    const double* zmags;           // per model the magnitudes' redshift (zred_mags) or NULL: zred
 };
 
 struct GemmTArgs {                 // C(col, k) = sum_j At(k, j) Yb(col, j)

@@ -14,6 +14,7 @@ module opensps_hip_iface
         type(c_ptr) :: prm = c_null_ptr, ssp_weights = c_null_ptr, mags = c_null_ptr, spec = c_null_ptr
         type(c_ptr) :: mass = c_null_ptr, lbol = c_null_ptr, fband = c_null_ptr, young = c_null_ptr, old = c_null_ptr
         type(c_ptr) :: idx = c_null_ptr, weights = c_null_ptr, op_count = c_null_ptr, ssp_weights_z = c_null_ptr
+        type(c_ptr) :: zred_mags = c_null_ptr
     end type hip_batch_t
 
     !> opensps_hip_grad_t

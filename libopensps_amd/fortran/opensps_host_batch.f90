@@ -192,12 +192,17 @@ contains
 
     !> Per-model checks of stage 0 that this build adds to the reference's: options whose stages it does
     !> not compute. status 2 with a message, else 0.
-    subroutine model_checks(c, prm, n, st, fagn)
+    !> have_zm: the call gives each model's magnitude redshift (zred_mags), which redshifted colours need
+    subroutine model_checks(c, prm, n, st, fagn, have_zm)
         type(host_ctx_t), intent(in) :: c
         real(WP), intent(in) :: prm(n, NPAR)
         integer, intent(in) :: n
         integer, intent(out) :: st
         real(WP), intent(in) :: fagn(:)
+        logical, intent(in), optional :: have_zm
+        logical :: zm
+        zm = .false.
+        if (present(have_zm)) zm = have_zm
         st = 0
         if (c%add_agn_dust == 1 .and. any(fagn > SAFE_FLOOR)) then
             ! the AGN stage runs for fagn > 0 (opensps_csp_batch.f90:1582-1588)
@@ -215,7 +220,7 @@ contains
         else if (c%compute_vega_mags == 1 .and. .not. allocated(c%magvega)) then
             st = 2
             call set_error(2, 'batch: Vega magnitudes need the Vega spectrum (opensps_context_set_reference_spectra)')
-        else if (c%redshift_colors /= 0) then
+        else if (c%redshift_colors /= 0 .and. .not. zm) then
             st = 2
             call set_error(2, 'batch: redshifted colours need zred_mags')
         else if (c%dust_type /= 0 .and. c%dust_type /= 2 .and. c%dust_type /= 6) then

@@ -68,11 +68,6 @@ contains
             call set_error(2, 'batch_grad: spectral indices are not part of this build')
             return
         end if
-        if (c_associated(c_zred_mags)) then
-            status = 2
-            call set_error(2, 'batch_grad: magnitudes at a second redshift (zred_mags) are not part of this build')
-            return
-        end if
         if (c_associated(c_weights) .and. c_associated(c_weights_z)) then
             status = 3
             call set_error(3, 'batch_grad: ssp_weights and ssp_weights_z both given')
@@ -86,7 +81,7 @@ contains
         end if
         call resolve_params(c, n, n_params, c_names, c_values, prm, status, fagn)
         if (status /= 0) return
-        call model_checks(c, prm, n, st, fagn)
+        call model_checks(c, prm, n, st, fagn, c_associated(c_zred_mags))
         if (st /= 0) then
             status = st
             return
@@ -102,7 +97,7 @@ contains
         nz = c%nz
         bytes = merge(4, 8, c%mixed == 1)
         ! device buffers: 1 prm, 2 weights, 3 weights_z, 4-7 cotangents, 8 mags, 9 spec, 10 mass, 11 lbol,
-        ! 12 params_bar, 13 weights_bar, 14 weights_z_bar
+        ! 12 params_bar, 13 weights_bar, 14 weights_z_bar, 15 zred_mags
         d = c_null_ptr
         sz = 0
         sz(1) = 8_c_size_t * NPAR * n
@@ -119,7 +114,8 @@ contains
         if (c_associated(c_params_bar)) sz(12) = 8_c_size_t * NBGRAD * n
         if (c_associated(c_weights_bar)) sz(13) = 8_c_size_t * nt * n
         if (c_associated(c_weights_z_bar) .and. c_associated(c_weights_z)) sz(14) = 8_c_size_t * nt * nz * n
-        do k = 1, 14
+        if (c_associated(c_zred_mags)) sz(15) = 8_c_size_t * n
+        do k = 1, 15
             if (sz(k) > 0) then
                 rc = opensps_hip_malloc(d(k), sz(k))
                 if (rc /= 0) then
@@ -137,6 +133,7 @@ contains
         if (sz(5) > 0) rc = opensps_hip_memcpy_h2d(d(5), c_spec_bar, sz(5))
         if (sz(6) > 0) rc = opensps_hip_memcpy_h2d(d(6), c_mass_bar, sz(6))
         if (sz(7) > 0) rc = opensps_hip_memcpy_h2d(d(7), c_lbol_bar, sz(7))
+        if (sz(15) > 0) rc = opensps_hip_memcpy_h2d(d(15), c_zred_mags, sz(15))
         g%fwd%n = n
         g%fwd%precision = merge(HIP_MIXED, HIP_FP64, c%mixed == 1)
         g%fwd%split_parts = 1
@@ -144,6 +141,7 @@ contains
         g%fwd%prm = d(1)
         g%fwd%ssp_weights = d(2)
         g%fwd%ssp_weights_z = d(3)
+        g%fwd%zred_mags = d(15)
         g%fwd%mags = d(8)
         g%fwd%spec = d(9)
         g%fwd%mass = d(10)

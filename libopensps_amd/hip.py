@@ -25,7 +25,7 @@ class BatchArgs(C.Structure):
                 ("ssp_weights", C.c_void_p), ("mags", C.c_void_p), ("spec", C.c_void_p), ("mass", C.c_void_p),
                 ("lbol", C.c_void_p), ("fband", C.c_void_p), ("young", C.c_void_p), ("old", C.c_void_p),
                 ("idx", C.c_void_p), ("weights", C.c_void_p), ("op_count", C.c_void_p),
-                ("ssp_weights_z", C.c_void_p)]
+                ("ssp_weights_z", C.c_void_p), ("zred_mags", C.c_void_p)]
 
 
 class GradArgs(C.Structure):
@@ -195,7 +195,7 @@ class DeviceContext:
                 out[k] = torch.empty(shapes[k][0], dtype=shapes[k][1], device=dev)
         return out
 
-    def _batch_args(self, prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z=None):
+    def _batch_args(self, prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z=None, zred_mags=None):
         n = prm.shape[1]
         assert prm.shape[0] == NPAR and prm.is_contiguous() and prm.dtype.itemsize == 8
         g = self.grid
@@ -213,20 +213,25 @@ class DeviceContext:
         if ssp_weights_z is not None and (ssp_weights_z.element_size() != 8 or not ssp_weights_z.is_contiguous()
                                           or ssp_weights_z.numel() < n * g.nt * g.nz):
             raise OpenSPSHipError(3, "ssp_weights_z: nt x nz x n float64 required")
+        if zred_mags is not None and (zred_mags.element_size() != 8 or not zred_mags.is_contiguous()
+                                      or zred_mags.numel() < n):
+            raise OpenSPSHipError(3, "zred_mags: n float64 required")
         a = BatchArgs()
         a.n, a.precision, a.split_parts, a.use_ops = n, precision, int(split_parts), int(use_ops)
         a.prm = prm.data_ptr()
         a.ssp_weights = ssp_weights.data_ptr() if ssp_weights is not None else None
         a.ssp_weights_z = ssp_weights_z.data_ptr() if ssp_weights_z is not None else None
+        a.zred_mags = zred_mags.data_ptr() if zred_mags is not None else None
         for k in ("mags", "spec", "mass", "lbol", "fband", "young", "old", "idx", "weights", "op_count"):
             setattr(a, k, out[k].data_ptr() if k in out else None)
         return a
 
     def launch_grad(self, prm, out, cot, grads, precision=MIXED, split_parts=0, ssp_weights=None, ssp_weights_z=None,
-                    use_ops=0):
+                    use_ops=0, zred_mags=None):
         """Enqueues a batch and its reverse pass. cot: device cotangents {"mags", "spec", "mass", "lbol"} (any
         absent); grads: device outputs {"params_bar" (n, NBGRAD), "weights_bar" (n, nt), "weights_z_bar"
-        (n, nz, nt)} (any absent), float64. use_ops: the IMF per model (the operator), with its rows in params_bar."""
+        (n, nz, nt)} (any absent), float64. use_ops: the IMF per model (the operator), with its rows in params_bar;
+        zred_mags: device (n) float64, each model's magnitude redshift (zred's row then holds the IGM's part)."""
         n = prm.shape[1]
         g = self.grid
         want = {"mags": n * self.nbands, "spec": n * g.nspec, "mass": n, "lbol": n}
@@ -240,7 +245,7 @@ class DeviceContext:
             if t is not None and (t.element_size() != 8 or not t.is_contiguous() or t.numel() < numel):
                 raise OpenSPSHipError(3, f"{k}: {numel} contiguous float64 required")
         a = GradArgs()
-        a.fwd = self._batch_args(prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z)
+        a.fwd = self._batch_args(prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z, zred_mags)
         for k in ("mags", "spec", "mass", "lbol"):
             setattr(a, k + "_bar", cot[k].data_ptr() if cot.get(k) is not None else None)
         for k in ("params_bar", "weights_bar", "weights_z_bar"):
@@ -248,10 +253,10 @@ class DeviceContext:
         self._check(lib().opensps_hip_csp_batch_grad(self._ctx, C.byref(a)))
 
     def launch(self, prm, out, precision=MIXED, split_parts=0, ssp_weights=None, use_ops=0, ssp_weights_z=None,
-               noalloc=False):
+               noalloc=False, zred_mags=None):
         """Enqueues one batch on device-resident prm (NPAR, n) float64 and the buffers of out; noalloc: through
         opensps_hip_csp_batch_noalloc (the fixed scratch, no device allocation)."""
-        a = self._batch_args(prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z)
+        a = self._batch_args(prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z, zred_mags)
         fn = lib().opensps_hip_csp_batch_noalloc if noalloc else lib().opensps_hip_csp_batch
         self._check(fn(self._ctx, C.byref(a)))
 
@@ -259,18 +264,18 @@ class DeviceContext:
         self._check(lib().opensps_hip_sync(self._ctx))
 
     def batch(self, prm, precision=MIXED, split_parts=0, ssp_weights=None, want=("mags", "spec", "mass", "lbol"),
-              use_ops=0, ssp_weights_z=None, noalloc=False):
+              use_ops=0, ssp_weights_z=None, noalloc=False, zred_mags=None):
         import torch
         n = prm.shape[1]
         out = self.alloc_outputs(n, precision, want)
         torch.cuda.synchronize()
-        self.launch(prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z, noalloc)
+        self.launch(prm, out, precision, split_parts, ssp_weights, use_ops, ssp_weights_z, noalloc, zred_mags)
         self.sync()
         return out
 
     def batch_grad(self, prm, cot, precision=MIXED, split_parts=0, ssp_weights=None, ssp_weights_z=None,
                    want=("mags", "spec", "mass", "lbol"), grads=("params_bar", "weights_bar", "weights_z_bar"),
-                   use_ops=0):
+                   use_ops=0, zred_mags=None):
         """The forward outputs and the reverse pass's outputs (device tensors)."""
         import torch
         n = prm.shape[1]
@@ -279,7 +284,7 @@ class DeviceContext:
         shapes = {"params_bar": (n, NBGRAD), "weights_bar": (n, g.nt), "weights_z_bar": (n, g.nz, g.nt)}
         gr = {k: torch.full(shapes[k], float("nan"), dtype=torch.float64, device="cuda") for k in grads}
         torch.cuda.synchronize()
-        self.launch_grad(prm, out, cot, gr, precision, split_parts, ssp_weights, ssp_weights_z, use_ops)
+        self.launch_grad(prm, out, cot, gr, precision, split_parts, ssp_weights, ssp_weights_z, use_ops, zred_mags)
         self.sync()
         out.update(gr)
         return out

@@ -1452,6 +1452,10 @@ typedef struct {
     /* a fixed diffuse attenuation curve A_lambda / A_V per pixel (dust_type 2, 6: attenuation_table_row's column 0,
      * opensps_csp_batch.f90:3525-3566) or NULL: the power law */
     const double *dcurve;
+    /* each model's magnitude redshift (compute_csp_batch's zred_mags, opensps_csp_batch.f90:1054-1061) or NULL:
+     * the observed frame and the distance term at it, the IGM at zred; the reverse pass then leaves the observed
+     * frame's and the distance term's dz out of zred's row (batch_backward's lzm, :4830-4832, 4928) */
+    const double *zred_mags;
 } orc_ctx_t;
 
 /* The model's IMF parameters (batch_imf_vectors, opensps_csp_batch.f90:1823-1853): the slopes and limits per model,
@@ -1704,9 +1708,10 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
             }
             if (spec) memcpy(spec + (size_t)m * nspec, tot, sizeof(double) * nspec);
             /* stage 8: magnitudes (compute_magnitudes_columns, observed_frame_z, band_fluxes) */
-            orc_observed_frame(c->lambda, nspec, z, tot, fobs);
+            const double zm = c->zred_mags ? c->zred_mags[m] : z;
+            orc_observed_frame(c->lambda, nspec, zm, tot, fobs);
             orc_band_fluxes_col(nspec, nb, bt, jlo, jhi, do_band, wtl, fobs, fb);
-            double dist = orc_distance_term(c->cz, c->cdl, c->ncosmo, z);
+            double dist = orc_distance_term(c->cz, c->cdl, c->ncosmo, zm);
             for (int b = 0; b < nb; b++) {
                 double v = NAN;
                 if (do_band[b] != 0) v = mag_from_flux(fb[b]) - AB_ZEROPOINT - 2.5 * amzl + dist;
@@ -1940,9 +1945,10 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
                                            soff, scoef, cap);
                 if (sm_n > 0) apply_velocity_kernel(nspec, slo, shi, soff, scoef, S1, work);
             }
-            orc_observed_frame(c->lambda, nspec, z, S1, fobs);
+            const double zm = c->zred_mags ? c->zred_mags[m] : z;   /* the magnitudes' redshift */
+            orc_observed_frame(c->lambda, nspec, zm, S1, fobs);
             orc_band_fluxes_col(nspec, nb, bt, jlo, jhi, do_band, wtl, fobs, fb);
-            const double dist = orc_distance_term(c->cz, c->cdl, c->ncosmo, z);
+            const double dist = orc_distance_term(c->cz, c->cdl, c->ncosmo, zm);
             double *mg = mags_out ? mags_out + (size_t)m * nb : NULL;
             for (int b = 0; b < nb; b++) {
                 double v = NAN;
@@ -1954,7 +1960,7 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
             /* ---- reverse ---- */
             /* the distance term (distance_term_dz) */
             double dzd = 0.0;
-            if (mags_bar && fabs(z) > SAFE_FLOOR) {
+            if (mags_bar && fabs(z) > SAFE_FLOOR && !c->zred_mags) {
                 double msum = 0.0;
                 for (int b = 0; b < nb; b++) if (fbb[b] != 0.0) msum = msum + mags_bar[(size_t)m * nb + b];
                 const double dm = interp_linear(c->cz, c->cdl, c->ncosmo, z);
@@ -1980,9 +1986,9 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
             for (int j = 0; j < nspec; j++) {
                 double v, t = 0.0;
                 int il = j + 1;
-                const int moved = fabs(z) > SAFE_FLOOR;
+                const int moved = fabs(zm) > SAFE_FLOOR;
                 if (moved) {
-                    double tg = c->lambda[j] / (1.0 + z);
+                    double tg = c->lambda[j] / (1.0 + zm);
                     il = orc_find_interval(c->lambda, nspec, tg);
                     if (il < 1) il = 1;
                     if (il > nspec - 1) il = nspec - 1;
@@ -1995,8 +2001,9 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
                 double gv = (v == v && v > 0.0) ? vb[j] * wl : 0.0;
                 if (gv == 0.0) continue;
                 if (moved) {
-                    dzo += gv * (S1[il] - S1[il - 1]) * (-c->lambda[j] / ((1.0 + z) * (1.0 + z))) /
-                           (c->lambda[il] - c->lambda[il - 1]);
+                    if (!c->zred_mags)
+                        dzo += gv * (S1[il] - S1[il - 1]) * (-c->lambda[j] / ((1.0 + z) * (1.0 + z))) /
+                               (c->lambda[il] - c->lambda[il - 1]);
                     g1[il - 1] += (1.0 - t) * gv;
                     g1[il] += t * gv;
                 } else {

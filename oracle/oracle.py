@@ -359,7 +359,7 @@ class _Ctx(C.Structure):
                 ("add_igm", C.c_int), ("smooth_fast", C.c_int), ("tiny_logt", C.c_double),
                 ("ops", C.c_void_p), ("imf_type", C.c_int), ("remnants", C.c_int), ("salp_ind", C.c_double),
                 ("mdave", C.c_double), ("mlim_bh", C.c_double), ("mlim_ns", C.c_double), ("op_count", C.c_void_p),
-                ("ssp_weights_z", C.c_void_p), ("dcurve", C.c_void_p)]
+                ("ssp_weights_z", C.c_void_p), ("dcurve", C.c_void_p), ("zred_mags", C.c_void_p)]
 
 
 def _context(grid, bands, add_igm, smooth_fast, mag_mask, cosmo, ptr):
@@ -407,7 +407,7 @@ GRAD_ROWS = {"dust1": 1, "dust2": 2, "dust3": 3, "dust_index": 4, "dust1_index":
 
 def csp_batch_grad(grid, bands, prm, ssp_weights=None, ssp_weights_z=None, mags_bar=None, spec_bar=None,
                    mass_bar=None, lbol_bar=None, add_igm=1, smooth_fast=1, mag_mask=None, cosmo=None, ops=None,
-                   imf_type=2, remnants=1, salp_ind=2.35, mdave=0.5, mlim_bh=40.0, mlim_ns=8.5):
+                   imf_type=2, remnants=1, salp_ind=2.35, mdave=0.5, mlim_bh=40.0, mlim_ns=8.5, zred_mags=None):
     """The oracle's reverse pass (FP64). prm (NPAR, n); ssp_weights (n, nt); ssp_weights_z (n, nz, nt);
     cotangents mags_bar (n, nbands), spec_bar (n, nspec), mass_bar, lbol_bar (n); ops: a synth.SSPOperator (the
     IMF per model and its rows imf1, imf2 and imf3 or mdave). Returns mags (n, nbands), params_bar (n, NBGRAD),
@@ -425,6 +425,8 @@ def csp_batch_grad(grid, bands, prm, ssp_weights=None, ssp_weights_z=None, mags_
         ctx.ssp_weights_z = ptr(_d(ssp_weights_z))
     if ops is not None:
         _set_ops(ctx, ops, ptr, keep, imf_type, remnants, salp_ind, mdave, mlim_bh, mlim_ns)
+    if zred_mags is not None:
+        ctx.zred_mags = ptr(_d(zred_mags))
     out = {"mags": np.zeros((n, bands.shape[0])), "params_bar": np.zeros((n, NBGRAD)),
            "weights_bar": np.zeros((n, grid.nt)), "weights_z_bar": np.zeros((n, grid.nz, grid.nt))}
     vp = C.c_void_p
@@ -442,7 +444,7 @@ def csp_batch_grad(grid, bands, prm, ssp_weights=None, ssp_weights_z=None, mags_
 
 def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_mask=None, cosmo=None,
               want=("mags", "spec", "mass", "lbol"), ops=None, imf_type=2, remnants=1, salp_ind=2.35, mdave=0.5,
-              mlim_bh=40.0, mlim_ns=8.5, ssp_weights_z=None, dust_curve=None):
+              mlim_bh=40.0, mlim_ns=8.5, ssp_weights_z=None, dust_curve=None, zred_mags=None):
     """Runs the oracle's batch. grid: synth.SSPGrid; bands (nbands, nspec); prm (NPAR, n); ops: a
     synth.SSPOperator (the IMF per model; the grid's bases are then unused); ssp_weights_z (n, nz, nt): weights
     over ages and metallicities.
@@ -462,6 +464,8 @@ def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_
         ctx.ssp_weights_z = ptr(_d(ssp_weights_z))
     if dust_curve is not None:
         ctx.dcurve = ptr(_d(dust_curve))
+    if zred_mags is not None:
+        ctx.zred_mags = ptr(_d(zred_mags))
     ctx.nspec, ctx.nt, ctx.nz, ctx.nbands = grid.nspec, grid.nt, grid.nz, nb
     ctx.lambda_ = ptr(_d(grid.lam))
     ctx.time_full = ptr(_d(grid.log_age))

@@ -179,8 +179,8 @@ def test_batch_without_device_is_status_5():
 
 def test_batch_grad_statuses_before_the_device():
     """compute_csp_batch_grad's host checks (opensps_c_api.f90:729-735 for the row count): fewer params_bar rows
-    than NBGRAD is status 3 with nothing written; zred_mags and spectral indices are status 2; both weight forms
-    at once is status 3."""
+    than NBGRAD is status 3 with nothing written; spectral indices are status 2; both weight forms at once is
+    status 3; zred_mags passes the host's checks (redshifted colours need it) and reaches the device."""
     g = grid(1, 600, 20)
     fs = synth.gaussian_filters(4, seed=7)
     with api.Context() as ctx:
@@ -200,10 +200,20 @@ def test_batch_grad_statuses_before_the_device():
         names = (C.c_char_p * 1)(b"dust2")
         vp = (C.c_void_p * 1)(keep[0].ctypes.data)
         mags = np.zeros((n, 4))
+        indx = np.zeros((n, 4))
         api.lib().opensps_context_compute_csp_batch_grad(
-            ctx.handle, n, 1, C.cast(names, C.c_void_p), C.cast(vp, C.c_void_p), w.ctypes.data, None, zm.ctypes.data,
-            None, None, None, None, mags.ctypes.data, *([None] * 10), C.byref(st))
+            ctx.handle, n, 1, C.cast(names, C.c_void_p), C.cast(vp, C.c_void_p), w.ctypes.data, None, None,
+            None, None, None, None, mags.ctypes.data, None, None, None, None, None, None, indx.ctypes.data,
+            None, None, None, C.byref(st))
         assert st.value == 2
+        import torch
+        if not torch.cuda.is_available():
+            ctx.set_int("add_dust_emission", 0)
+            ctx.set_int("redshift_colors", 1)
+            r = ctx.compute_csp_batch_grad(vals, ssp_weights=w, grads=(), raw_status=True)
+            assert r["status"] == 2                       # redshifted colours without zred_mags
+            r = ctx.compute_csp_batch_grad(vals, ssp_weights=w, grads=(), zred_mags=zm, raw_status=True)
+            assert r["status"] == 5                       # past the host's checks: no device here
 
 
 def test_dust_type_statuses():

@@ -367,6 +367,33 @@ def test_api_imf_grad(precision):
 
 
 @pytest.mark.gpu
+def test_api_zred_mags():
+    """compute_csp_batch_grad with zred_mags through the public C API (redshifted colours): the magnitudes at the
+    second redshift with the IGM at zred, and zred's row (the IGM's part), against the oracle."""
+    g = grid(22)
+    fs = synth.gaussian_filters(12)
+    n = 24
+    prm = synth.parametric_params(n, seed=131)
+    zm = np.random.default_rng(133).uniform(0.0, 3.0, n)
+    names = synth.varied_names()
+    print("case: api zred_mags", flush=True)
+    with _ctx(g, fs, "double") as ctx:
+        ctx.set_int("redshift_colors", 1)
+        mb = np.ones((n, 12))
+        r = ctx.compute_csp_batch_grad({k: prm[SLOT[k]] for k in names}, n=n, mags_bar=mb, grads=("params_bar",),
+                                       zred_mags=zm)
+        bands = ctx.bands()
+    ref = O.csp_batch_grad(g, bands, prm, mags_bar=mb, cosmo=O.cosmology_table(), zred_mags=zm)
+    assert ref["status"] == 0
+    em = mag_err(r["mags"], ref["mags"])
+    row = O.GRAD_ROWS["zred"]
+    ez = float(np.max(np.abs(r["params_bar"][:, row] - ref["params_bar"][:, row]))) / (
+        np.max(np.abs(ref["params_bar"][:, row])) + 1e-300)
+    print(f"api zred_mags: mags {em:.2e}, zred row {ez:.2e}")
+    assert em < 1e-7 and ez < 1e-7
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("imf_type", [2, 0])
 def test_api_evaluate_ssp_op(imf_type):
     """opensps_context_evaluate_ssp_op: the SSP of every age for one IMF (ssp_operator_evaluate), at the context's

@@ -351,3 +351,32 @@ def test_grad_operator(precision, imf_type, nz, remnants):
     ew = _rel(got["weights_bar"], ref["weights_bar"])
     print(f"  weights_bar {ew:.2e}")
     assert ew < TOL_OP[precision]["weights"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_grad_zred_mags(precision):
+    """Magnitudes at a second redshift (zred_mags): the forward's observed frame and distance term at it, the IGM
+    at zred, and zred's row the IGM's part alone, against the oracle's; z = 0 and zred_mags = 0 included."""
+    g, b = _setup()
+    n = 24
+    prm = synth.adjoint_params(n, seed=121, birth_cloud=True)
+    rng = np.random.default_rng(123)
+    prm[SLOT["zred"]] = rng.uniform(0.0, 4.0, n)
+    zm = rng.uniform(0.0, 4.0, n)
+    prm[SLOT["zred"], 0] = 0.0
+    zm[1] = 0.0
+    w = synth.dirichlet_weights(n, NT, seed=125, alpha=0.7)
+    cot = _cot(n, seed=127)
+    ref = O.csp_batch_grad(g, b, prm, ssp_weights=w, mags_bar=cot["mags"], spec_bar=cot["spec"],
+                           mass_bar=cot["mass"], lbol_bar=cot["lbol"], cosmo=cosmo(), zred_mags=zm)
+    assert ref["status"] == 0
+    ctx = _ctx(g, b)
+    print(f"case: grad zred_mags n={n} precision={precision}", flush=True)
+    out = ctx.batch_grad(_dev(prm), {k: _dev(v) for k, v in cot.items()}, precision=precision, split_parts=1,
+                         ssp_weights=_dev(w), grads=("params_bar", "weights_bar"), zred_mags=_dev(zm))
+    got = {k: v.detach().cpu().numpy() for k, v in out.items()}
+    ctx.close()
+    _compare(ref, got, precision, ROWS)
+    ew = _rel(got["weights_bar"], ref["weights_bar"])
+    print(f"  weights_bar {ew:.2e}")
+    assert ew < TOL[precision]["weights"]

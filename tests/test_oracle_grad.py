@@ -283,3 +283,28 @@ def test_weights_bar_operator():
         fd = (_loss(_fwd(g, b, cz, prm, ssp_weights=wp, **kw), cot) -
               _loss(_fwd(g, b, cz, prm, ssp_weights=wm, **kw), cot)) / (2 * h)
         _check(fd, wb[:, t], 2e-5, f"weights_bar[{t}]")
+
+
+def test_zred_mags():
+    """Magnitudes at a second redshift (zred_mags): without the IGM they equal the batch run at that redshift; with
+    it, zred's row is the IGM's part alone (the observed frame and distance follow zred_mags), against central
+    differences in zred with zred_mags held; the other rows against differences as usual."""
+    g, b, cz = _setup()
+    n = 5
+    prm = _params(n, 91, True)
+    prm[SLOT["zred"]] = np.array([0.0, 0.4, 1.3, 2.6, 3.7])
+    zm = np.array([0.2, 0.0, 1.9, 2.1, 3.7])
+    w = synth.dirichlet_weights(n, g.nt, seed=95)
+    base = prm.copy()
+    base[SLOT["zred"]] = zm
+    a = _fwd(g, b, cz, prm, ssp_weights=w, zred_mags=zm, add_igm=0)
+    r = _fwd(g, b, cz, base, ssp_weights=w, add_igm=0)
+    np.testing.assert_array_equal(a["mags"], r["mags"])
+    np.testing.assert_array_equal(a["spec"], r["spec"])
+    cot = _cot(n, g, seed=93)
+    pb = _grad(g, b, cz, prm, cot, ssp_weights=w, zred_mags=zm)["params_bar"]
+    for name in ("zred", "dust2", "sigma_smooth", "igm_factor"):
+        fd = _fd_param(g, b, cz, prm, cot, SLOT[name], ssp_weights=w, zred_mags=zm)
+        k = slice(1, None) if name == "zred" else slice(None)   # (the IGM switches on above z = 0: a step there)
+        _check(fd[k], pb[k, O.GRAD_ROWS[name]], 2e-4, name)
+    assert pb[0, O.GRAD_ROWS["zred"]] == 0.0   # z = 0: no IGM, and the magnitudes do not follow zred
