@@ -27,7 +27,7 @@ BF16_PEAK_TFLOPS = 2500.0
 FP64_PEAK_TFLOPS = 78.6
 KERNELS = ["prep", "group", "age_sums", "chain"]
 MAG_RANGE = 40.0
-CHAIN_KERNEL = "k_chain<float, true>"
+CHAIN_KERNEL = "k_chain<float, true, false>"
 TRAFFIC_REPORT = "k_chain_f32_parametric16k_traffic.md"
 
 

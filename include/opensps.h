@@ -90,6 +90,25 @@ void opensps_context_get_filter_data(int handle, double *wave_eff, double *mag_v
 void opensps_context_set_reference_spectra(int handle, int n_vega, const double *lam_vega, const double *flux_vega,
                                            int vega_flambda, int n_sun, const double *lam_sun, const double *flux_sun,
                                            int sun_flambda, int *status);
+/* No reference counterpart (the reference reads its dust emission library at set-up from its data directory,
+ * load_dust_emission_table, src/io/opensps_io.f90:959-1069): the Draine & Li (2007) templates (or any library of that
+ * layout): qpah (nqpah), umin (numin) and lam (nlam, Angstrom) ascending, spectra nlam x 2 x numin x nqpah
+ * column-major (the data library's [qpah][umin][component][wavelength]: per (q_PAH, U_min) the spectrum of U_max =
+ * U_min, then of the power-law U distribution). The batch places them on the grid as the reference does (linear in
+ * wavelength from 1 micron on) and, with add_dust_emission = 1, adds each dusty model's emission scaled to the light
+ * its dust absorbs (energy balance; duste_qpah, duste_umin, duste_gamma per model; mdust out). Without them a dusty
+ * batch with add_dust_emission = 1 is status 2. status 1 bad handle, 3 bad sizes, NULL arrays or tables not
+ * ascending. */
+void opensps_context_set_dust_emission_templates(int handle, int nqpah, const double *qpah, int numin,
+                                                 const double *umin, int nlam, const double *lam,
+                                                 const double *spectra, int *status);
+/* No reference counterpart (the reference reads its AGN torus library at set-up, load_agn_dust_models,
+ * src/io/opensps_io.f90:1101-1155): the templates: tau (ntau) and lam (nlam, Angstrom) ascending, spectra nlam x ntau
+ * column-major (the data library's [tau][wavelength]). With add_agn_dust = 1 each model with fagn > 0 adds fagn L_bol
+ * times its agn_tau template, attenuated by the diffuse dust, after the IGM. Without them such a batch is status 2.
+ * status 1 bad handle, 3 bad sizes, NULL arrays or tables not ascending. */
+void opensps_context_set_agn_templates(int handle, int ntau, const double *tau, int nlam, const double *lam,
+                                       const double *spectra, int *status);
 
 /* ---- filters (reference include/opensps.h:192-233) ---- */
 void opensps_filterset_create(int *fset, int *status);
@@ -113,10 +132,13 @@ void opensps_context_get_cosmology(int handle, int *nrow, double *z, double *dl,
 /* ---- the batch (reference include/opensps.h:264-316) ---- */
 /* status 0, 1 bad handle / no basis, 2 unsupported (a parameter that changes the SSPs; imf1, imf2, imf3,
  * imf_lower_limit, imf_upper_limit without an SSP operator on the context, and vdmc, mdave per model; spectral
- * indices, dust emission, nebular emission, AGN emission (fagn > 0), Vega magnitudes without reference spectra,
+ * indices, nebular emission, dust emission or the AGN torus without their templates
+ * (opensps_context_set_dust_emission_templates, _set_agn_templates), Vega magnitudes without reference spectra,
  * dust_type other than 0, 2, 6, an SFH type other than 0 to 5), 3 unknown parameter or bad shape, 5 no HIP device,
- * 10 device error. Members of the reference's parameter set that no stage here reads (pmetals, dust_clumps, uvb,
- * mwr, gas_*, duste_*, agn_tau, fagn = 0) are accepted. With an SSP operator on the context
+ * 10 device error. Dust emission (add_dust_emission = 1, the default: models with dust1 or dust2 > 0; mdust out) and
+ * the AGN torus (add_agn_dust = 1: models with fagn > 0) run with their templates. Members of the reference's
+ * parameter set that no stage here reads (pmetals, dust_clumps, uvb, mwr, gas_*) are accepted. With an SSP operator
+ * on the context
  * (opensps_context_set_ssp_ops) every batch takes each model's IMF from it, as the reference's does
  * (src/bindings/c/opensps_c_api.f90:913-917). */
 void opensps_context_compute_csp_batch(int handle, int n, int n_params, const char *const *names,
@@ -136,7 +158,8 @@ void opensps_context_compute_csp_batch_parts(int handle, int n, int n_params, co
  * dust1, dust2, dust3, dust_index, dust1_index, frac_obrun, frac_nodust, logzsol, zred, sigma_smooth, igm_factor
  * and, with SFH parameters (no weights given; sfh 0, 1, 4, 5), tage, tau, const, fburst, tburst, sf_start, sf_trunc
  * and sf_slope and, with SSP operators on the context, imf1, imf2 and imf3 (Kroupa) or imf1, imf2 and mdave (Dave)
- * (the others are 0). zred_mags (n, or NULL): each model's magnitude redshift (the observed frame and the distance
+ * (the others are 0); a batch whose dust emission or AGN torus acts is status 2 here (their reverse pass is not
+ * built). zred_mags (n, or NULL): each model's magnitude redshift (the observed frame and the distance
  * term at it, the IGM at zred; zred's row is then the IGM's part); redshift_colors = 1 needs it (else status 2).
  * _rows: params_bar's leading dimension (fewer than NBGRAD: status 3, nothing written; rows beyond NBGRAD are left
  * alone). */

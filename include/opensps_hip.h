@@ -60,6 +60,9 @@ typedef struct {
                                     opensps_csp_batch.f90:1054-1061): the observed frame and the distance term at it,
                                     the IGM at zred; the reverse pass then gives zred's row the IGM's part alone
                                     (batch_backward's lzm, :4830-4832, 4928) */
+    double *mdust;               /* device (n) or NULL: the dust mass of the dust emission's templates
+                                    (compute_csp_batch's mdust, opensps_csp_batch.f90:1206-1211; SAFE_FLOOR for a
+                                    model without emission) */
 } opensps_hip_batch_t;
 
 /* The reverse pass of a batch (compute_csp_batch with params_bar / weights_bar / weights_z_bar, include/opensps.h:
@@ -124,6 +127,20 @@ int opensps_hip_set_vega(opensps_hip_ctx_t *ctx, int nbands, const double *magve
 int opensps_hip_set_cosmology(opensps_hip_ctx_t *ctx, int n, const double *z, const double *dl);
 /* add_igm_absorption; smooth_fast = (smooth_velocity == 1 and smoothspec_fast == 1). */
 int opensps_hip_set_options(opensps_hip_ctx_t *ctx, int add_igm, int smooth_fast);
+/* add_dust_emission and add_agn_dust (the reference's defaults are 1 and 1): a model emits dust when add_dust_emission
+ * is 1 and dust1 or dust2 > 0 (batch_model_indices, opensps_csp_batch.f90:1569-1570), the AGN torus when
+ * add_agn_dust is 1 and fagn > 0 (agn_indices :1591-1606); a batch that needs templates the context lacks is
+ * status 2. */
+int opensps_hip_set_emission(opensps_hip_ctx_t *ctx, int add_dust_emission, int add_agn_dust);
+/* The Draine & Li (2007) dust emission templates on the grid, host: table (nspec, nqpah, 2 numin) column-major, for
+ * each (q_PAH, U_min) the spectrum of U_max = U_min, then of the power-law U distribution to U_max = 1e6
+ * (dustem2_dustem, src/io/opensps_io.f90:959-1069, as interpolated there onto the grid); qpah (nqpah) and umin
+ * (numin) ascending. nqpah = 0 removes them. Call after opensps_hip_set_grid. */
+int opensps_hip_set_dust_emission(opensps_hip_ctx_t *ctx, int nqpah, const double *qpah, int numin, const double *umin,
+                                  const double *table);
+/* The AGN torus templates on the grid, host: table (nspec, ntau) column-major, tau (ntau) ascending (agndust_spec,
+ * src/io/opensps_io.f90:1101-1155). ntau = 0 removes them. Call after opensps_hip_set_grid. */
+int opensps_hip_set_agn(opensps_hip_ctx_t *ctx, int ntau, const double *tau, const double *table);
 
 /* The batch: compute_csp_batch's stages 1-8 (src/physics/opensps_csp_batch.f90:431-1214) on the device,
  * asynchronous on the context's stream; the status of the models is checked by opensps_hip_sync. */

@@ -40,7 +40,12 @@
 #define OPS_P_IMF3           25
 #define OPS_P_IMF_UPPER      26
 #define OPS_P_IMF_LOWER      27
-#define OPS_NPAR             28
+#define OPS_P_DUSTE_UMIN     28
+#define OPS_P_DUSTE_QPAH     29
+#define OPS_P_DUSTE_GAMMA    30
+#define OPS_P_FAGN           31
+#define OPS_P_AGN_TAU        32
+#define OPS_NPAR             33
 
 /* Default value of every slot: the PARAMS_CORE defaults (src/core/opensps_types.f90:33-45). */
 #define OPS_PARAM_DEFAULTS { \
@@ -51,12 +56,13 @@
     0.0,  /* frac_obrun */    0.0,  /* frac_nodust */  0.0, /* zred */         1.0,   /* igm_factor */ \
     0.0,  /* sigma_smooth */  1.0e3,/* min_wave_smooth */ 1.0e4, /* max_wave_smooth */                \
     1.3,  /* imf1 */          2.3,  /* imf2 */         2.3, /* imf3 */       120.0,   /* imf_upper_limit */ \
-    0.08  /* imf_lower_limit */ }
+    0.08, /* imf_lower_limit */ 1.0, /* duste_umin */  3.5, /* duste_qpah */ 0.01,   /* duste_gamma */ \
+    0.0,  /* fagn */          10.0  /* agn_tau */ }
 
 #define OPS_PARAM_NAMES { \
     "tage", "tau", "const", "fburst", "tburst", "sf_start", "sf_trunc", "sf_slope", "sfh", "logzsol", \
     "dust1", "dust2", "dust3", "dust_index", "dust1_index", "dust_tesc", "frac_obrun", "frac_nodust", \
     "zred", "igm_factor", "sigma_smooth", "min_wave_smooth", "max_wave_smooth", "imf1", "imf2", "imf3", \
-    "imf_upper_limit", "imf_lower_limit" }
+    "imf_upper_limit", "imf_lower_limit", "duste_umin", "duste_qpah", "duste_gamma", "fagn", "agn_tau" }
 
 #endif

@@ -72,6 +72,8 @@ def lib():
         L.opensps_context_build_ssp_ops.argtypes = [i, i, ip]
         L.opensps_context_build_ssp_ops_free_mods.argtypes = [i, i, ip]
         L.opensps_context_set_ssp_ops.argtypes = [i, i, i, i, i, i] + [vp] * 12 + [ip]
+        L.opensps_context_set_dust_emission_templates.argtypes = [i, i, vp, i, vp, i, vp, vp, ip]
+        L.opensps_context_set_agn_templates.argtypes = [i, i, vp, i, vp, vp, ip]
         L.opensps_context_free_ssp_ops.argtypes = [i, ip]
         L.opensps_context_has_ssp_ops.argtypes = [i, ip, ip]
         L.opensps_context_evaluate_ssp_op.argtypes = [i, i, vp, vp, vp, vp, ip]
@@ -265,6 +267,26 @@ class Context:
         """opensps_context_build_ssp_ops: building from a data release is not part of this build (status 301)."""
         st = C.c_int()
         lib().opensps_context_build_ssp_ops(self.handle, int(all_metallicities), C.byref(st))
+        _check(st)
+
+    def set_dust_emission_templates(self, qpah, umin, lam, spectra):
+        """opensps_context_set_dust_emission_templates: spectra (nqpah, numin, 2, nlam) as datalib.dust_emission
+        returns them (per (q_PAH, U_min) the U_max = U_min spectrum, then the power-law one), lam in Angstrom."""
+        q, u, lm = _d(qpah), _d(umin), _d(lam)
+        sp = _d(spectra)
+        assert sp.shape == (q.size, u.size, 2, lm.size)
+        st = C.c_int()
+        lib().opensps_context_set_dust_emission_templates(self.handle, q.size, q.ctypes.data, u.size, u.ctypes.data,
+                                                          lm.size, lm.ctypes.data, sp.ctypes.data, C.byref(st))
+        _check(st)
+
+    def set_agn_templates(self, tau, lam, spectra):
+        """opensps_context_set_agn_templates: spectra (ntau, nlam) as datalib.agn_torus returns them."""
+        t, lm, sp = _d(tau), _d(lam), _d(spectra)
+        assert sp.shape == (t.size, lm.size)
+        st = C.c_int()
+        lib().opensps_context_set_agn_templates(self.handle, t.size, t.ctypes.data, lm.size, lm.ctypes.data,
+                                                sp.ctypes.data, C.byref(st))
         _check(st)
 
     def set_ssp_ops(self, op):

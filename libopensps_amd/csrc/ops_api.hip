@@ -185,7 +185,8 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
                   c->misc.p, c->spec.p, c->flag.p, c->d_rdlam, c->d_ltab32, c->d_wl32, c->d_sm32, c->d_bkt, c->hdr.p, c->wsfh.p,
                   c->opu.p, c->opwc.p, c->opsum.p, c->opseg.p, c->opimf.p, c->opspart.p, c->opgp.p, c->d_bp_ptr, c->d_bp_band,
                   c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p, c->gwb.p, c->d_magvega, c->d_tab, c->d_dcurve, c->d_dcurve32,
-                  c->opsraw.p, c->opct.p, c->oppart.p, c->opimfd.p};
+                  c->opsraw.p, c->opct.p, c->oppart.p, c->opimfd.p,
+                  c->d_qpah, c->d_umin, c->d_dustem, c->d_agn_tau, c->d_agn, c->d_wnu};
     for (void* p : ps) free_ptr(p);
     free_operator(c);
     free_full(c);
@@ -230,6 +231,9 @@ int opensps_hip_set_grid(opensps_hip_ctx_t* c, int nspec, const double* lam, int
     c->nz = nz;
     c->nsp = ((nspec + OPS_TILE_PX - 1) / OPS_TILE_PX) * OPS_TILE_PX;
     c->dcurve_on = 0;   // (a curve is per pixel of the grid)
+    c->nqpah = 0;       // (so are the dust emission and AGN templates: set them again)
+    c->numin = 0;
+    c->ntau = 0;
     c->nbands = 0;   // the band tables index the old grid: set_bands again
     c->band_j0 = 0;
     c->band_j1 = -1;
@@ -277,6 +281,15 @@ int opensps_hip_set_grid(opensps_hip_ctx_t* c, int nspec, const double* lam, int
     c->nbk = nbk;
     c->lnl0 = loglam[0];
     c->bkinv = 1.0 / bw;
+    // the frequency trapezoid of the dust emission's sums (dust_emission_reductions with prepare_batch_grid's wt,
+    // src/spectra/opensps_batch_photometry.f90:262): sum_i wt(i) (f(i) + f(i+1)) = sum_i f(i) (wt(i-1) + wt(i))
+    std::vector<double> wnu(c->nsp, 0.0);
+    for (int j = 0; j + 1 < nspec; j++) {
+        const double wt = 0.5 * fabs(OPS_C_LIGHT / lam[j + 1] - OPS_C_LIGHT / lam[j]);
+        wnu[j] += wt;
+        wnu[j + 1] += wt;
+    }
+    UPLOAD(c, c->d_wnu, wnu.data(), wnu.size());
     UPLOAD(c, c->d_rdlam, rdlam.data(), (size_t)nspec);
     UPLOAD(c, c->d_ltab32, ltab32.data(), ltab32.size());
     UPLOAD(c, c->d_wl32, wl32.data(), (size_t)nspec);
@@ -870,6 +883,64 @@ int opensps_hip_set_options(opensps_hip_ctx_t* c, int add_igm, int smooth_fast) 
     return 0;
 }
 
+int opensps_hip_set_emission(opensps_hip_ctx_t* c, int add_dust_emission, int add_agn_dust) {
+    if (!c) return 1;
+    c->add_duste = add_dust_emission == 1 ? 1 : 0;
+    c->add_agn = add_agn_dust == 1 ? 1 : 0;
+    return 0;
+}
+
+int opensps_hip_set_dust_emission(opensps_hip_ctx_t* c, int nqpah, const double* qpah, int numin, const double* umin,
+                                  const double* table) {
+    if (!c) return 1;
+    if (c->nspec < 2) return ops_fail(c, 1, "set_dust_emission: no grid installed");
+    if (nqpah == 0) {
+        c->nqpah = 0;
+        c->numin = 0;
+        return 0;
+    }
+    if (nqpah < 2 || numin < 2 || !qpah || !umin || !table)
+        return ops_fail(c, 3, "set_dust_emission: at least 2 q_PAH and 2 U_min values and the table are required");
+    for (int k = 1; k < nqpah; k++)
+        if (!(qpah[k] > qpah[k - 1])) return ops_fail(c, 3, "set_dust_emission: q_PAH not strictly ascending");
+    for (int k = 1; k < numin; k++)
+        if (!(umin[k] > umin[k - 1])) return ops_fail(c, 3, "set_dust_emission: U_min not strictly ascending");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // padded to nsp pixels per column (zero past nspec)
+    const int ncol = nqpah * 2 * numin;
+    std::vector<double> t((size_t)c->nsp * ncol, 0.0);
+    for (int k = 0; k < ncol; k++)
+        for (int i = 0; i < c->nspec; i++) t[(size_t)k * c->nsp + i] = table[(size_t)k * c->nspec + i];
+    UPLOAD(c, c->d_dustem, t.data(), t.size());
+    UPLOAD(c, c->d_qpah, qpah, (size_t)nqpah);
+    UPLOAD(c, c->d_umin, umin, (size_t)numin);
+    c->nqpah = nqpah;
+    c->numin = numin;
+    return 0;
+}
+
+int opensps_hip_set_agn(opensps_hip_ctx_t* c, int ntau, const double* tau, const double* table) {
+    if (!c) return 1;
+    if (c->nspec < 2) return ops_fail(c, 1, "set_agn: no grid installed");
+    if (ntau == 0) {
+        c->ntau = 0;
+        return 0;
+    }
+    if (ntau < 2 || !tau || !table) return ops_fail(c, 3, "set_agn: at least 2 optical depths and the table are required");
+    for (int k = 1; k < ntau; k++)
+        if (!(tau[k] > tau[k - 1])) return ops_fail(c, 3, "set_agn: optical depths not strictly ascending");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> t((size_t)c->nsp * ntau, 0.0);
+    for (int k = 0; k < ntau; k++)
+        for (int i = 0; i < c->nspec; i++) t[(size_t)k * c->nsp + i] = table[(size_t)k * c->nspec + i];
+    UPLOAD(c, c->d_agn, t.data(), t.size());
+    UPLOAD(c, c->d_agn_tau, tau, (size_t)ntau);
+    c->ntau = ntau;
+    return 0;
+}
+
 int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     if (!c) return 1;
     if (!a) return ops_fail(c, 3, "batch: NULL arguments");
@@ -1162,6 +1233,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         sa.mass = a->mass;
         sa.lbol = a->lbol;
         sa.lsum = c->lsum_out;
+        sa.hdr = (double*)c->hdr.p;
         k_op_sums<<<(n + 3) / 4, 256, 0, c->stream>>>(sa);
         HIPCHK(c, hipGetLastError());
         k_op_tiles<<<1, 256, 0, c->stream>>>(n * parts, OPS_TILE_COL, d_optiles, (int4*)c->tiles.p + maxtiles);
@@ -1213,6 +1285,42 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     HIPCHK(c, hipGetLastError());
     if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
 
+    // dust emission and the AGN torus: their indices and the energy balance into the chain's header, the dust masses;
+    // the chain with its emission stages (EM) when either can act
+    const bool em = c->add_duste || c->add_agn;
+    if (em || a->mdust) {
+        DustArgs da;
+        da.n = n;
+        da.nspec = c->nspec;
+        da.nsp = c->nsp;
+        da.parts = parts;
+        da.nqpah = c->nqpah;
+        da.numin = c->numin;
+        da.ntau = c->ntau;
+        da.duste_on = c->add_duste;
+        da.agn_on = c->add_agn;
+        da.qpah = c->d_qpah;
+        da.umin = c->d_umin;
+        da.agn_tau = c->d_agn_tau;
+        da.status = d_status;
+        da.flag = d_flag;
+        da.prm = a->prm;
+        da.ltab = c->d_ltab;
+        da.ltab32 = c->d_ltab32;
+        da.dcurve = c->dcurve_on ? c->d_dcurve : nullptr;
+        da.dcurve32 = c->dcurve_on ? c->d_dcurve32 : nullptr;
+        da.dustem = c->d_dustem;
+        da.wnu = c->d_wnu;
+        da.yo = c->yo.p;
+        da.hdr = (double*)c->hdr.p;
+        da.mdust = a->mdust;
+        if (mixed)
+            k_emission<float><<<n, 256, 0, c->stream>>>(da);
+        else
+            k_emission<double><<<n, 256, 0, c->stream>>>(da);
+        HIPCHK(c, hipGetLastError());
+    }
+
     ChainArgs ca;
     ca.n = n;
     ca.nspec = c->nspec;
@@ -1233,6 +1341,11 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ca.sm32 = c->d_sm32;
     ca.prm = a->prm;
     ca.zmags = a->zred_mags;
+    ca.dustem = c->d_dustem;
+    ca.agn = c->d_agn;
+    ca.nqpah = c->nqpah;
+    ca.numin = c->numin;
+    ca.ntau = c->ntau;
     ca.yo = c->yo.p;
     ca.lam = c->d_lam;
     ca.loglam = c->d_loglam;
@@ -1256,21 +1369,19 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ca.sband_staged = (size_t)c->nbands * sizeof(double) <= 16384;
     ca.stage = one && !a->spec ? c->spec.p : nullptr;
     const size_t lds_chain = (one ? 1 : 2) * tsz * (size_t)c->nsp + (ca.sband_staged ? sizeof(double) * c->nbands : 0);
-    const void* kfn = mixed ? (one ? (const void*)k_chain<float, true> : (const void*)k_chain<float, false>)
-                            : (one ? (const void*)k_chain<double, true> : (const void*)k_chain<double, false>);
+    // the chain's instance: precision, one LDS buffer or two, the emission stages or none
+    const void* kfn;
+    if (em)
+        kfn = mixed ? (one ? (const void*)k_chain<float, true, true> : (const void*)k_chain<float, false, true>)
+                    : (one ? (const void*)k_chain<double, true, true> : (const void*)k_chain<double, false, true>);
+    else
+        kfn = mixed ? (one ? (const void*)k_chain<float, true, false> : (const void*)k_chain<float, false, false>)
+                    : (one ? (const void*)k_chain<double, true, false> : (const void*)k_chain<double, false, false>);
     HIPCHK(c, hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_chain));
-    if (mixed) {
-        if (one)
-            k_chain<float, true><<<n, chain_threads, lds_chain, c->stream>>>(ca);
-        else
-            k_chain<float, false><<<n, chain_threads, lds_chain, c->stream>>>(ca);
-    } else {
-        if (one)
-            k_chain<double, true><<<n, chain_threads, lds_chain, c->stream>>>(ca);
-        else
-            k_chain<double, false><<<n, chain_threads, lds_chain, c->stream>>>(ca);
+    {
+        void* kargs[] = {(void*)&ca};
+        HIPCHK(c, hipLaunchKernel(kfn, dim3(n), dim3(chain_threads), kargs, lds_chain, c->stream));
     }
-    HIPCHK(c, hipGetLastError());
     if (c->profiling) {
         HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
         c->ev_pending = true;

@@ -112,6 +112,13 @@ struct OpsCtx {
     int dcurve_on = 0;
     double vega_anchor = 0.0;
 
+    // ---- dust emission and the AGN torus (set_emission, set_dust_emission, set_agn) ----
+    int add_duste = 0, add_agn = 0;
+    int nqpah = 0, numin = 0, ntau = 0;
+    double *d_qpah = nullptr, *d_umin = nullptr, *d_dustem = nullptr;   // templates (nsp, nqpah, 2 numin)
+    double *d_agn_tau = nullptr, *d_agn = nullptr;                      // templates (nsp, ntau)
+    double* d_wnu = nullptr;   // per pixel the frequency trapezoid's weight wt(i - 1) + wt(i), wt(i) = |c/l(i+1) - c/l(i)| / 2
+
     // ---- cosmology ----
     int ncosmo = 0;
     double *d_cz = nullptr, *d_cdl = nullptr;

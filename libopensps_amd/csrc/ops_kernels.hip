@@ -7,6 +7,7 @@
 //               the split-BF16 ones are k_mm_bf16x3 (ops_mm.hip)
 //   k_chain     one workgroup per model: dust, IGM, velocity smoothing, the spectrum out, the
 //               observed frame, the band integrals and the magnitudes, the spectrum staged in LDS
+#include <limits>
 #include <type_traits>
 
 #include "ops_common.h"
@@ -711,7 +712,7 @@ __global__ void k_vega_mags(int n, int nb, double* mags, const double* magvega, 
 // ------------------------------------------------------------------------------------------------
 // k_prep: one wavefront per model (4 per workgroup)
 // ------------------------------------------------------------------------------------------------
-__device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, double dz, int lane);
+__device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, double dz, double ls, int lane);
 
 __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
     extern __shared__ double lds[];
@@ -845,7 +846,7 @@ __global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
             }
         }
     }
-    wave_chain_header(a, m, p, dz, lane);
+    wave_chain_header(a, m, p, dz, ls, lane);
     // the model's rank in its pair and slot (k_group): last, after every load of the wavefront (the vector memory
     // counter completes in order, so an atomic issued earlier would hold up every later wait); OPS_GCNT_SLOTS
     // counters a pair keep the atomics on one address few
@@ -1096,7 +1097,7 @@ __global__ void k_pack_bf16(PackArgs a) {
 // table, descending in z as the reference builds it, or ascending).
 // ------------------------------------------------------------------------------------------------
 
-__device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, double dz, int lane) {
+__device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, double dz, double ls, int lane) {
     // the IGM at zred; the observed frame and the distance term at the magnitudes' redshift zm (zred_mags or zred)
     const double z = p[OPS_P_ZRED], opz = 1.0 + z;
     const double zm = a.zmags ? a.zmags[m] : z;
@@ -1176,6 +1177,7 @@ __device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, dou
     case OPS_HDR_TMAX: v = tmax; break;
     case OPS_HDR_JMAX: v = (double)jmax; break;
     case OPS_HDR_DZ: v = dz; break;
+    case OPS_HDR_LSUM: v = ls; break;   // (the operator path's comes from k_op_sums)
     default: break;
     }
     // #{lam < minl} and #{lam <= maxl}: the bucket lookup from the logarithm, then the exact walk
@@ -1333,6 +1335,182 @@ __device__ __forceinline__ typename Quad<T>::type obs_quad(const ChainArgs& a, c
     return out;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Dust emission (the Draine & Li templates with the energy balance) and the AGN torus
+// ------------------------------------------------------------------------------------------------
+// The diffuse transmission of pixel i as the chain's attenuation forms it (pixel_transmissions,
+// opensps_csp_batch.f90:3765-3818): the curve (a fixed one or the power law), then exp(-dust2 curve), in T
+template <typename T>
+__device__ __forceinline__ T pixel_td(const double* ltab, const float* ltab32, const double* dcurve,
+                                      const float* dcurve32, double ed, double dust2, int i) {
+    if constexpr (std::is_same<T, float>::value) {
+        const float curve = dcurve32 ? dcurve32[i] : ((ed != 0.0) ? __expf((float)ed * ltab32[i]) : 1.0f);
+        return __expf(-((float)dust2 * curve));
+    } else {
+        const double curve = dcurve ? dcurve[i] : ((ed != 0.0) ? exp(ed * ltab[i]) : 1.0);
+        return exp(-(dust2 * curve));
+    }
+}
+
+// The Draine & Li shape at pixel i (dl07_shape, opensps_csp_batch.f90:4103-4122): bilinear in q_PAH and U_min in
+// FP64 on the table (nsp, nqpah, 2 numin) (column q + nqpah k; k = 2 u for U_max = U_min, 2 u + 1 for the power-law
+// U distribution), the two mixed by gamma, then rounded to T and floored at T's smallest normal
+template <typename T>
+__device__ __forceinline__ T dl07_shape(const double* tb, int nsp, int nqpah, int iq, double wq, int iu, double wu,
+                                        double gam, int i) {
+    const size_t c00 = (size_t)iq + (size_t)nqpah * (2 * iu), c10 = (size_t)(iq + 1) + (size_t)nqpah * (2 * iu);
+    const size_t c11 = (size_t)(iq + 1) + (size_t)nqpah * (2 * iu + 2), c01 = (size_t)iq + (size_t)nqpah * (2 * iu + 2);
+    const size_t nq = (size_t)nqpah;
+    const double lo = (1.0 - wq) * (1.0 - wu) * tb[c00 * nsp + i] + wq * (1.0 - wu) * tb[c10 * nsp + i] +
+                      wq * wu * tb[c11 * nsp + i] + (1.0 - wq) * wu * tb[c01 * nsp + i];
+    const double hi = (1.0 - wq) * (1.0 - wu) * tb[(c00 + nq) * nsp + i] + wq * (1.0 - wu) * tb[(c10 + nq) * nsp + i] +
+                      wq * wu * tb[(c11 + nq) * nsp + i] + (1.0 - wq) * wu * tb[(c01 + nq) * nsp + i];
+    const T sh = (T)((1.0 - gam) * lo + gam * hi);
+    return sh > std::numeric_limits<T>::min() ? sh : std::numeric_limits<T>::min();
+}
+
+// 1 - t for t = exp(-x) without cancellation (one_minus_trans_kp, opensps_csp_batch.f90:3925-3939)
+template <typename T>
+__device__ __forceinline__ T one_minus_trans(T x, T t) {
+    if (t == (T)1) return x;
+    if (fabs(x) < (T)1) {
+        if constexpr (std::is_same<T, float>::value)
+            return (1.0f - t) * x / (-logf(t));
+        else
+            return (1.0 - t) * x / (-log(t));
+    }
+    return (T)1 - t;
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_emission: one workgroup (256 threads) per model, after the age sums. Into the chain's header: the dust emission's
+// template indices (dl07_indices, opensps_csp_batch.f90:1611-1626: the bracketing intervals of q_PAH and U_min,
+// clamped to the grid, their weights not; gamma clamped) and its energy balance (dust_emission_reductions :3963-4042:
+// the absorbed light of the attenuation, young through the birth cloud and the diffuse dust, old through its screen
+// and the diffuse dust, attenuate_batch :3640-3656; the template; the template times the diffuse transmission; each
+// summed over frequency with prepare_batch_grid's trapezoid) as the normalisation s_abs / s_esc that
+// add_dust_emission (:4045-4099) applies, for a model with add_dust_emission and dust1 or dust2 > 0
+// (batch_model_indices :1569-1570); the AGN torus's interval of agn_tau with its weight clamped (agn_indices
+// :1591-1606) and its luminosity fagn L_bol through log10 (add_agn_batch :4342-4433), for add_agn_dust and fagn > 0.
+// mdust = 3.21e-3 / (4 pi) s_abs / s_norm (:1206-1211). A model that needs templates the context lacks: status 2.
+// Transmissions in T as the chain forms them, the sums in FP64.
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_emission(DustArgs a) {
+    __shared__ double red[3][4];
+    const int m = blockIdx.x, tid = threadIdx.x, n = a.n;
+    double* h = a.hdr + (size_t)m * OPS_HDR_N;
+    const double dust1 = a.prm[(size_t)OPS_P_DUST1 * n + m], dust2 = a.prm[(size_t)OPS_P_DUST2 * n + m];
+    const double fagn = a.prm[(size_t)OPS_P_FAGN * n + m];
+    const bool dusty = dust1 > OPS_SAFE_FLOOR || dust2 > OPS_SAFE_FLOOR;
+    const bool want_e = a.duste_on && dusty, want_a = a.agn_on && fagn > OPS_SAFE_FLOOR;
+    const bool emis = want_e && a.nqpah > 0, agn = want_a && a.ntau > 0;
+    if (tid == 0) {
+        if ((want_e && !emis) || (want_a && !agn)) {
+            atomicMax(a.flag, 2);
+            if (a.status) a.status[m] = 2;
+        }
+        double it = 0.0, wt = 0.0, lagn = 0.0;
+        if (agn) {
+            const double tau = a.prm[(size_t)OPS_P_AGN_TAU * n + m];
+            const int ia = max(1, min(find_interval_ref(a.agn_tau, a.ntau, tau), a.ntau - 1));
+            it = (double)(ia - 1);
+            wt = fmax(0.0, fmin((tau - a.agn_tau[ia - 1]) / (a.agn_tau[ia] - a.agn_tau[ia - 1]), 1.0));
+            const double ls = h[OPS_HDR_LSUM];
+            lagn = (ls > OPS_SAFE_FLOOR ? pow(10.0, log10(ls)) : 1.0) * fagn;
+        }
+        h[OPS_HDR_AGN_IT] = it;
+        h[OPS_HDR_AGN_WT] = wt;
+        h[OPS_HDR_AGN_L] = lagn;
+        h[OPS_HDR_DL_ON] = emis ? 1.0 : 0.0;
+    }
+    if (!emis) {
+        if (tid == 0) {
+            h[OPS_HDR_DL_RN] = 0.0;
+            if (a.mdust) a.mdust[m] = OPS_SAFE_FLOOR;
+        }
+        return;
+    }
+    const double qp = a.prm[(size_t)OPS_P_DUSTE_QPAH * n + m], um = a.prm[(size_t)OPS_P_DUSTE_UMIN * n + m];
+    const int iq = max(1, min(find_interval_ref(a.qpah, a.nqpah, qp), a.nqpah - 1)) - 1;
+    const int iu = max(1, min(find_interval_ref(a.umin, a.numin, um), a.numin - 1)) - 1;
+    const double wq = (qp - a.qpah[iq]) / (a.qpah[iq + 1] - a.qpah[iq]);
+    const double wu = (um - a.umin[iu]) / (a.umin[iu + 1] - a.umin[iu]);
+    const double gam = fmax(0.0, fmin(a.prm[(size_t)OPS_P_DUSTE_GAMMA * n + m], 1.0));
+    const double dust3 = a.prm[(size_t)OPS_P_DUST3 * n + m];
+    const double ed = a.prm[(size_t)OPS_P_DUST_INDEX * n + m], ed1 = a.prm[(size_t)OPS_P_DUST1_INDEX * n + m];
+    const double fob = a.prm[(size_t)OPS_P_FRAC_OBRUN * n + m], fnd = a.prm[(size_t)OPS_P_FRAC_NODUST * n + m];
+    const bool no_birth = dust1 <= OPS_SAFE_FLOOR || 1.0 - fob <= OPS_SAFE_FLOOR;
+    const bool no_old = fabs(dust3) <= OPS_SAFE_FLOOR;
+    const bool no_diffuse_mix = 1.0 - fnd <= OPS_SAFE_FLOOR;
+    const T* y = (const T*)a.yo + (size_t)m * a.nsp;
+    const T* o = a.parts == 2 ? (const T*)a.yo + ((size_t)n + m) * a.nsp : nullptr;
+    const T cf = (T)fob, cn = (T)fnd;
+    double sa = 0.0, sn = 0.0, se = 0.0;
+    for (int i = tid; i < a.nspec; i += blockDim.x) {
+        T curve, tr_d;
+        if constexpr (std::is_same<T, float>::value) {
+            curve = a.dcurve32 ? a.dcurve32[i] : ((ed != 0.0) ? __expf((float)ed * a.ltab32[i]) : 1.0f);
+            tr_d = __expf(-((float)dust2 * curve));
+        } else {
+            curve = a.dcurve ? a.dcurve[i] : ((ed != 0.0) ? exp(ed * a.ltab[i]) : 1.0);
+            tr_d = exp(-(dust2 * curve));
+        }
+        const T tau_d = (T)dust2 * curve;
+        T tau_b = (T)0, tr_b = (T)1, tau_o = (T)0, tr_o = (T)1;
+        if (!no_birth) {
+            if constexpr (std::is_same<T, float>::value) {
+                tau_b = (float)dust1 * __expf((float)ed1 * a.ltab32[i]);
+                tr_b = __expf(-tau_b);
+            } else {
+                tau_b = dust1 * exp(ed1 * a.ltab[i]);
+                tr_b = exp(-tau_b);
+            }
+        }
+        if (!no_old) {
+            tau_o = (T)dust3 * curve;
+            if constexpr (std::is_same<T, float>::value)
+                tr_o = __expf(-tau_o);
+            else
+                tr_o = exp(-tau_o);
+        }
+        const T omtb = one_minus_trans<T>(tau_b, tr_b), omto = one_minus_trans<T>(tau_o, tr_o);
+        const T omtd = no_diffuse_mix ? (T)0 : ((T)1 - cn) * one_minus_trans<T>(tau_d, tr_d);
+        const T ay = tr_b * ((T)1 - cf) + cf;
+        const T yv = y[i], ov = o ? o[i] : (T)0;
+        const T absorbed = yv * (((T)1 - cf) * omtb + ay * omtd) + ov * (omto + tr_o * omtd);
+        const T sh = dl07_shape<T>(a.dustem, a.nsp, a.nqpah, iq, wq, iu, wu, gam, i);
+        const double W = a.wnu[i];
+        sa += W * (double)absorbed;
+        sn += W * (double)sh;
+        se += W * (double)(sh * tr_d);
+    }
+    sa = wave_sum(sa);
+    sn = wave_sum(sn);
+    se = wave_sum(se);
+    if ((tid & 63) == 0) {
+        red[0][tid >> 6] = sa;
+        red[1][tid >> 6] = sn;
+        red[2][tid >> 6] = se;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        sa = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        sn = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        se = red[2][0] + red[2][1] + red[2][2] + red[2][3];
+        h[OPS_HDR_DL_IQ] = (double)iq;
+        h[OPS_HDR_DL_WQ] = wq;
+        h[OPS_HDR_DL_IU] = (double)iu;
+        h[OPS_HDR_DL_WU] = wu;
+        h[OPS_HDR_DL_GAM] = gam;
+        h[OPS_HDR_DL_RN] = se > OPS_SAFE_FLOOR ? (double)(T)(sa / se) : 0.0;
+        if (a.mdust)
+            a.mdust[m] = sn > OPS_SAFE_FLOOR ? 3.21e-3 / (4.0 * 3.14159265358979323846) * (sa / sn) : OPS_SAFE_FLOOR;
+    }
+}
+template __global__ void k_emission<float>(DustArgs);
+template __global__ void k_emission<double>(DustArgs);
+
 // ONE: one spectrum buffer in LDS (nsp values): the smoothed values and the observed-frame quads wait in
 // registers for a barrier and then overwrite it (windows wider than OPS_CHAIN_KS pixels a thread go through
 // the spectrum out, or a staging row, in global memory). Otherwise two buffers. The host takes ONE when the
@@ -1342,7 +1520,9 @@ __device__ __forceinline__ typename Quad<T>::type obs_quad(const ChainArgs& a, c
 // (pixel-parallel, applied after the attenuation's barrier) before the attenuation's loads, the source intervals of
 // its first observed-frame quad before the smoothing; the attenuation fetches KA quads a thread before it computes,
 // and the band sums fetch four bands a group at a time.
-template <typename T, bool ONE>
+// EM: the emission stages (dust emission before the IGM, the AGN torus after it), an instance of its own so that a
+// batch without them runs the chain without their code.
+template <typename T, bool ONE, bool EM>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value ? 8 : 6)))
 void k_chain(ChainArgs a) {
     constexpr bool F32 = std::is_same<T, float>::value;
@@ -1483,6 +1663,19 @@ void k_chain(ChainArgs a) {
         }
     }
     __syncthreads();
+    // dust emission (add_dust_emission, opensps_csp_batch.f90:4045-4099): the template times the diffuse
+    // transmission times the energy balance's normalisation (k_dust_sums), before the IGM as there
+    const double rn = EM ? h[OPS_HDR_DL_RN] : 0.0;
+    if (EM && rn != 0.0) {
+        const int iq = (int)h[OPS_HDR_DL_IQ], iu = (int)h[OPS_HDR_DL_IU];
+        const double wq = h[OPS_HDR_DL_WQ], wu = h[OPS_HDR_DL_WU], gam = h[OPS_HDR_DL_GAM];
+        for (int i = tid; i < nspec; i += nth) {
+            const T td = pixel_td<T>(a.ltab, a.ltab32, a.dcurve, a.dcurve32, ed, dust2, i);
+            const T sh = dl07_shape<T>(a.dustem, a.nsp, a.nqpah, iq, wq, iu, wu, gam, i);
+            s0[i] = s0[i] + sh * td * (T)rn;
+        }
+        __syncthreads();
+    }
     // the IGM (apply_igm_z): pixels below jt times their transmissions; pixels past KI a thread (grids with more
     // than KI x 512 pixels below Lyman-alpha), and every pixel in the FP64 chain, form theirs here
     if (jt > 0) {
@@ -1502,6 +1695,20 @@ void k_chain(ChainArgs a) {
                       g[3 * jn] * h[OPS_HDR_B6 + 3] + g[4 * jn] * h[OPS_HDR_B6 + 4] + g[5 * jn] * h[OPS_HDR_B6 + 5];
             }
             s0[i] *= (T)exp(-tau * igmf);
+        }
+        __syncthreads();
+    }
+    // the AGN torus (add_agn_batch, opensps_csp_batch.f90:4342-4433): fagn L_bol times the agn_tau template (linear
+    // between its two optical depths) times the diffuse transmission, after the IGM as there
+    const double lagn = EM ? h[OPS_HDR_AGN_L] : 0.0;
+    if (EM && lagn != 0.0) {
+        const int ia = (int)h[OPS_HDR_AGN_IT];
+        const double wta = h[OPS_HDR_AGN_WT];
+        for (int i = tid; i < nspec; i += nth) {
+            const T td = pixel_td<T>(a.ltab, a.ltab32, a.dcurve, a.dcurve32, ed, dust2, i);
+            double tmpl = (1.0 - wta) * a.agn[(size_t)ia * a.nsp + i] + wta * a.agn[(size_t)(ia + 1) * a.nsp + i];
+            tmpl = tmpl * (double)td;
+            s0[i] = (T)((double)s0[i] + lagn * tmpl);
         }
         __syncthreads();
     }
@@ -1682,10 +1889,14 @@ void k_chain(ChainArgs a) {
     }
 }
 
-template __global__ void k_chain<double, false>(ChainArgs);
-template __global__ void k_chain<float, false>(ChainArgs);
-template __global__ void k_chain<double, true>(ChainArgs);
-template __global__ void k_chain<float, true>(ChainArgs);
+template __global__ void k_chain<double, false, false>(ChainArgs);
+template __global__ void k_chain<float, false, false>(ChainArgs);
+template __global__ void k_chain<double, true, false>(ChainArgs);
+template __global__ void k_chain<float, true, false>(ChainArgs);
+template __global__ void k_chain<double, false, true>(ChainArgs);
+template __global__ void k_chain<float, false, true>(ChainArgs);
+template __global__ void k_chain<double, true, true>(ChainArgs);
+template __global__ void k_chain<float, true, true>(ChainArgs);
 
 // ------------------------------------------------------------------------------------------------
 // Kernel-level entry points: band_fluxes and observed_frame_z in the reference's own layouts

@@ -18,7 +18,20 @@
 #define OPS_HDR_W1 9
 #define OPS_HDR_B6 10      // the six (1 + z) powers of the IGM optical depth
 #define OPS_HDR_DZ 16      // the metallicity blend weight dz (not clamped)
-#define OPS_HDR_N 20
+// dust emission (dl07_indices, opensps_csp_batch.f90:1611-1626) and the AGN torus (agn_indices :1591-1606), formed
+// by k_emission and read by the chain with its emission stages (EM)
+#define OPS_HDR_DL_IQ 17   // the q_PAH interval (0-based lower column) and its weight (not clamped)
+#define OPS_HDR_DL_WQ 18
+#define OPS_HDR_DL_IU 19   // the U_min interval and its weight (not clamped)
+#define OPS_HDR_DL_WU 20
+#define OPS_HDR_DL_GAM 21  // gamma clamped to [0, 1]
+#define OPS_HDR_DL_ON 22   // 1: the model emits (add_dust_emission with dust1 or dust2 > 0)
+#define OPS_HDR_DL_RN 23   // the emission's normalisation s_abs / s_esc (k_dust_sums; 0: none)
+#define OPS_HDR_AGN_IT 24  // the agn_tau interval (0-based lower column) and its weight (clamped)
+#define OPS_HDR_AGN_WT 25
+#define OPS_HDR_AGN_L 26   // fagn L_bol (the linear L_bol sum through its logarithm; 0: no AGN)
+#define OPS_HDR_LSUM 27    // the linear L_bol sum (k_prep, or k_op_sums with the operator)
+#define OPS_HDR_N 28
 
 // The wavelength lookup: the ascending grid, and buckets over ln(lambda) no wider than its finest pixel
 struct LamLookup {
@@ -201,6 +214,7 @@ struct OpSArgs {           // k_op_sums
     const double *wsfh, *smass_p, *slum_p;
     double *mass, *lbol;
     double* lsum;          // (n) the linear L_bol sum (the reverse pass's), or NULL
+    double* hdr;           // the L_bol sum into the chain's header
 };
 
 // The operator's reverse pass (k_op_imf_grad, k_op_adj_t, k_op_adj_points, k_op_adj_out)
@@ -249,6 +263,22 @@ struct ChainArgs {
     const double* dcurve;
     const float* dcurve32;
     const double* zmags;   // per model the magnitudes' redshift (zred_mags) or NULL: zred
+    // dust emission (the Draine & Li templates (nsp, nqpah, 2 numin)) and the AGN torus templates (nsp, ntau)
+    const double *dustem, *agn;
+    int nqpah, numin, ntau;
+};
+
+// k_emission: per model the dust emission's template indices and energy balance (dl07_indices, dust_emission_reductions,
+// opensps_csp_batch.f90:1611-1626, 3963-4042) and the AGN torus's (agn_indices :1591-1606) into the chain's header;
+// mdust; status 2 for a model that needs templates the context lacks
+struct DustArgs {
+    int n, nspec, nsp, parts, nqpah, numin, ntau, duste_on, agn_on;
+    const double *prm, *ltab, *dcurve, *dustem, *wnu, *qpah, *umin, *agn_tau;
+    const float *ltab32, *dcurve32;
+    const void* yo;
+    double* hdr;
+    double* mdust;         // (n) or NULL
+    int *status, *flag;    // per-model status and the batch's flag (as k_prep's)
 };
 
 // ---- the reverse pass (ops_grad.hip) ----
@@ -322,7 +352,7 @@ __global__ void k_pack_bf16(PackArgs a);
 #define OPS_CHAIN_KQ 3   // observed-frame quads a thread holds in registers (single-buffer chain)
 #define OPS_CHAIN_KW 2   // smoothing windows a thread looks up before the spectrum is formed
 #define OPS_CHAIN_KI 3   // IGM transmissions a thread forms before the spectrum is (pixels tid + k 512)
-template <typename T, bool ONE>
+template <typename T, bool ONE, bool EM>
 __global__ void k_chain(ChainArgs a);
 template <typename T>
 __global__ void k_band_fluxes(int n, int nspec, int nbands, const T* bt, const int* jlo, const int* jhi, const int* do_band,
@@ -342,6 +372,8 @@ template <typename T>
 __global__ void k_op_fold(OpFArgs a);
 __global__ void k_op_sums(OpSArgs a);
 __global__ void k_op_imf_grad(OpAdjArgs a);
+template <typename T>
+__global__ void k_emission(DustArgs a);
 template <typename T>
 __global__ void k_op_adj_t(int n, int ntpl, int ldc, const int* order, const T* C, T* CT);
 template <typename T>

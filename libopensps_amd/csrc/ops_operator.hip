@@ -535,6 +535,7 @@ __global__ __launch_bounds__(256) void k_op_sums(OpSArgs a) {
         if (a.mass) a.mass[m] = ms;
         if (a.lbol) a.lbol[m] = ls > OPS_SAFE_FLOOR ? log10(ls) : 0.0;
         if (a.lsum) a.lsum[m] = ls;
+        a.hdr[(size_t)m * OPS_HDR_N + OPS_HDR_LSUM] = ls;   // (the AGN's luminosity: k_emission)
     }
 }
 

@@ -4,9 +4,12 @@ file, C order, no header), their dtype, shape and SHA-256 in the manifest. The r
 (src/io/opensps_datalib.f90, src/io/opensps_json.f90; the format: doc/DATA_FORMAT.md sections 2-3).
 
 The kinds the batched path uses: reference_spectra (Vega and the Sun, for the filters' zero points and the Vega
-system; the reference's load_standard_sed, src/io/opensps_io.f90:1516-1582) and filters (the curves of
-opensps_context_set_filters_library, src/spectra/opensps_context_filters.f90:148-170). Host-side I/O only: the
-tables go to the context through its C ABI (Context.set_reference_spectra, Context.set_filters)."""
+system; the reference's load_standard_sed, src/io/opensps_io.f90:1516-1582), filters (the curves of
+opensps_context_set_filters_library, src/spectra/opensps_context_filters.f90:148-170), dust_emission and agn_torus
+(the templates of the dust emission and the AGN torus, load_dust_emission_table and load_agn_dust_models,
+src/io/opensps_io.f90:959-1155). Host-side I/O only: the tables go to the context through its C ABI
+(Context.set_reference_spectra, Context.set_filters, Context.set_dust_emission_templates,
+Context.set_agn_templates)."""
 from __future__ import annotations
 
 import hashlib
@@ -131,3 +134,37 @@ def filter_library(root: str, name: str = "allfilters", verify: bool = True) -> 
         out.cal.append((float(cal["lambda_ref"]), float(cal["exponent"])) if cal else (0.0, 0.0))
         off += int(n)
     return out
+
+
+def dust_emission(root: str, name: str = "DL07", verify: bool = True):
+    """dust_emission/<name>: (qpah (nqpah), umin (numin), wavelength (nlam, A), spectra (nqpah, numin, 2, nlam) FP64),
+    the arrays load_dust_emission_table reads (src/io/opensps_io.f90:986-1011): per (q_PAH, U_min) the spectrum of
+    U_max = U_min, then of the power-law U distribution (components umin, umin_to_umax)."""
+    d = library_dir(root, "dust_emission", name)
+    m = read_manifest(d, "dust_emission")
+    q = _axis(d, m["axes"]["qpah"], verify)
+    u = _axis(d, m["axes"]["umin"], verify)
+    lam = np.asarray(_axis(d, m["axes"]["wavelength"], verify), dtype=np.float64)
+    sp = m["arrays"]["spectra"]
+    if sp.get("axes") != ["qpah", "umin", "component", "wavelength"]:
+        raise DataLibraryError(f"{d}: spectra axes {sp.get('axes')} are not [qpah, umin, component, wavelength]")
+    s = np.asarray(read_array(d, sp, verify), dtype=np.float64)
+    if s.shape != (q.size, u.size, 2, lam.size):
+        raise DataLibraryError(f"{d}: spectra {s.shape} against the axes {(q.size, u.size, 2, lam.size)}")
+    return q, u, lam, s
+
+
+def agn_torus(root: str, name: str = "clumpy", verify: bool = True):
+    """agn_torus/<name>: (tau (ntau), wavelength (nlam, A), spectra (ntau, nlam) FP64), the arrays
+    load_agn_dust_models reads (src/io/opensps_io.f90:1110-1120)."""
+    d = library_dir(root, "agn_torus", name)
+    m = read_manifest(d, "agn_torus")
+    tau = _axis(d, m["axes"]["tau"], verify)
+    lam = np.asarray(_axis(d, m["axes"]["wavelength"], verify), dtype=np.float64)
+    sp = m["arrays"]["spectra"]
+    if sp.get("axes") != ["tau", "wavelength"]:
+        raise DataLibraryError(f"{d}: spectra axes {sp.get('axes')} are not [tau, wavelength]")
+    s = np.asarray(read_array(d, sp, verify), dtype=np.float64)
+    if s.shape != (tau.size, lam.size):
+        raise DataLibraryError(f"{d}: spectra {s.shape} against the axes {(tau.size, lam.size)}")
+    return tau, lam, s

@@ -760,6 +760,61 @@ contains
         call set_error(301, 'opensps_context_build_ssp_ops_free_mods: context not set up from a data release')
     end subroutine
 
+    !> No reference counterpart (the reference reads its dust emission library at set-up from its data directory,
+    !> load_dust_emission_table, src/io/opensps_io.f90:959-1069): the Draine & Li (2007) templates (or any library of
+    !> that layout) as given: qpah (nqpah) and umin (numin) ascending, lam (nlam, A) ascending, spectra (nlam, 2, numin,
+    !> nqpah) column-major = the data library's [qpah][umin][component][wavelength] (per (q_PAH, U_min) the spectrum
+    !> of U_max = U_min, then of the power-law U distribution). The batch places them on the grid as the reference
+    !> does and, with add_dust_emission = 1, adds each dusty model's emission by energy balance. status 1 bad handle,
+    !> 3 bad sizes or order.
+    subroutine opensps_context_set_dust_emission_templates(handle, nqpah, qpah, numin, umin, nlam, lam, spectra, &
+                                                           status) bind(C, name="opensps_context_set_dust_emission_templates")
+        integer(c_int), value :: handle, nqpah, numin, nlam
+        type(c_ptr), value :: qpah, umin, lam, spectra
+        integer(c_int), intent(out) :: status
+        real(c_double), pointer :: q(:), u(:), l(:), sp(:, :, :)
+        integer :: st
+        if (bad_handle(handle, status)) return
+        if (nqpah < 2 .or. numin < 2 .or. nlam < 2 .or. .not. c_associated(qpah) .or. .not. c_associated(umin) .or. &
+            .not. c_associated(lam) .or. .not. c_associated(spectra)) then
+            status = 3
+            call set_error(3, 'set_dust_emission_templates: bad sizes or NULL arrays')
+            return
+        end if
+        call c_f_pointer(qpah, q, [nqpah])
+        call c_f_pointer(umin, u, [numin])
+        call c_f_pointer(lam, l, [nlam])
+        call c_f_pointer(spectra, sp, [int(nlam), 2 * int(numin), int(nqpah)])
+        call ctx_set_dust_emission(int(handle), int(nqpah), q, int(numin), u, int(nlam), l, sp, st)
+        status = st
+    end subroutine
+
+    !> No reference counterpart (the reference reads its AGN torus library at set-up, load_agn_dust_models,
+    !> src/io/opensps_io.f90:1101-1155): the templates as given: tau (ntau) and lam (nlam, A) ascending, spectra
+    !> (nlam, ntau) column-major = the data library's [tau][wavelength]. With add_agn_dust = 1 each model with
+    !> fagn > 0 adds fagn L_bol times its agn_tau template, attenuated by the diffuse dust. status 1 bad handle,
+    !> 3 bad sizes or order.
+    subroutine opensps_context_set_agn_templates(handle, ntau, tau, nlam, lam, spectra, status) &
+        bind(C, name="opensps_context_set_agn_templates")
+        integer(c_int), value :: handle, ntau, nlam
+        type(c_ptr), value :: tau, lam, spectra
+        integer(c_int), intent(out) :: status
+        real(c_double), pointer :: t(:), l(:), sp(:, :)
+        integer :: st
+        if (bad_handle(handle, status)) return
+        if (ntau < 2 .or. nlam < 2 .or. .not. c_associated(tau) .or. .not. c_associated(lam) .or. &
+            .not. c_associated(spectra)) then
+            status = 3
+            call set_error(3, 'set_agn_templates: bad sizes or NULL arrays')
+            return
+        end if
+        call c_f_pointer(tau, t, [ntau])
+        call c_f_pointer(lam, l, [nlam])
+        call c_f_pointer(spectra, sp, [int(nlam), int(ntau)])
+        call ctx_set_agn(int(handle), int(ntau), t, int(nlam), l, sp, st)
+        status = st
+    end subroutine
+
     !> No reference counterpart (the reference builds the operator set from isochrones, ssp_operator_build and
     !> ssp_opset_build, src/physics/opensps_ssp_operator.f90, opensps_ssp_batch.f90:99-147): installs an operator
     !> given by the caller on the GPU. With it every batch takes each model's IMF (imf1, imf2, imf3,

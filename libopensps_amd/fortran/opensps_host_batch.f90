@@ -192,28 +192,39 @@ contains
 
     !> Per-model checks of stage 0 that this build adds to the reference's: options whose stages it does
     !> not compute. status 2 with a message, else 0.
-    !> have_zm: the call gives each model's magnitude redshift (zred_mags), which redshifted colours need
-    subroutine model_checks(c, prm, n, st, fagn, have_zm)
+    !> have_zm: the call gives each model's magnitude redshift (zred_mags), which redshifted colours need; grad: the
+    !> batch with gradients (whose reverse pass of the dust emission and the AGN torus is not part of this build)
+    subroutine model_checks(c, prm, n, st, have_zm, grad)
         type(host_ctx_t), intent(in) :: c
         real(WP), intent(in) :: prm(n, NPAR)
         integer, intent(in) :: n
         integer, intent(out) :: st
-        real(WP), intent(in) :: fagn(:)
-        logical, intent(in), optional :: have_zm
+        logical, intent(in), optional :: have_zm, grad
+        logical :: dusty(n), agn(n), gr
         logical :: zm
         zm = .false.
         if (present(have_zm)) zm = have_zm
+        gr = .false.
+        if (present(grad)) gr = grad
         st = 0
-        if (c%add_agn_dust == 1 .and. any(fagn > SAFE_FLOOR)) then
-            ! the AGN stage runs for fagn > 0 (opensps_csp_batch.f90:1582-1588)
+        ! who emits dust (batch_model_indices, opensps_csp_batch.f90:1569-1570) and who has an AGN (agn_indices :1591-1606)
+        dusty = c%add_dust_emission == 1 .and. (prm(:, P_DUST1) > SAFE_FLOOR .or. prm(:, P_DUST2) > SAFE_FLOOR)
+        agn = c%add_agn_dust == 1 .and. prm(:, P_FAGN) > SAFE_FLOOR
+        if (any(agn) .and. .not. c%has_agn) then
             st = 2
-            call set_error(2, 'batch: AGN emission (fagn > 0 with add_agn_dust = 1) is not part of this build')
+            call set_error(2, 'batch: the AGN torus (fagn > 0 with add_agn_dust = 1) needs its templates ' // &
+                           '(opensps_context_set_agn_templates)')
+        else if (any(dusty) .and. .not. c%has_dustem) then
+            st = 2
+            call set_error(2, 'batch: dust emission (add_dust_emission = 1 with dust) needs its templates ' // &
+                           '(opensps_context_set_dust_emission_templates)')
+        else if (gr .and. (any(dusty) .or. any(agn))) then
+            st = 2
+            call set_error(2, 'batch_grad: the reverse pass of the dust emission and the AGN torus is not part of ' // &
+                           'this build')
         else if (c%has_ops .and. c%imf_type /= 0 .and. c%imf_type /= 2 .and. c%imf_type /= 4) then
             st = 2
             call set_error(2, 'batch: the SSP operator integrates imf_type 0 (Salpeter), 2 (Kroupa) and 4 (Dave)')
-        else if (c%add_dust_emission == 1 .and. any(prm(:, P_DUST1) > SAFE_FLOOR .or. prm(:, P_DUST2) > SAFE_FLOOR)) then
-            st = 2
-            call set_error(2, 'batch: dust emission (add_dust_emission = 1 with dust) is not part of this build')
         else if (c%add_neb_emission == 1) then
             st = 2
             call set_error(2, 'batch: nebular emission is not part of this build')
@@ -232,15 +243,13 @@ contains
     !> names[] / values[] -> prm(n, NPAR): the context's values, overwritten per model by the named parameters
     !> (batch_handle's name resolution, src/bindings/c/opensps_c_api.f90:797-853). Status 2 for a parameter that
     !> changes the SSPs, and for the IMF's without an SSP operator on the context (:710-714, 770-780); 3 for an
-    !> unknown name or NULL values. Members no stage here reads are accepted; fagn > 0 (the AGN stage) is refused in
-    !> model_checks.
-    subroutine resolve_params(c, n, n_params, c_names, c_values, prm, status, fagn)
+    !> unknown name or NULL values. Members no stage here reads are accepted.
+    subroutine resolve_params(c, n, n_params, c_names, c_values, prm, status)
         type(host_ctx_t), intent(in) :: c
         integer, intent(in) :: n, n_params
         type(c_ptr), intent(in) :: c_names, c_values
         real(WP), allocatable, intent(out) :: prm(:, :)
         integer, intent(out) :: status
-        real(WP), allocatable, intent(out), optional :: fagn(:)
         type(c_ptr), pointer :: np(:), vp(:)
         real(c_double), pointer :: v(:)
         character(len=:), allocatable :: nm
@@ -250,10 +259,6 @@ contains
         do k = 1, NPAR
             prm(:, k) = c%pset(k)
         end do
-        if (present(fagn)) then
-            allocate(fagn(n))
-            fagn = c%inert(I_FAGN)
-        end if
         if (n_params <= 0) return
         if (.not. c_associated(c_names) .or. .not. c_associated(c_values)) then
             status = 3
@@ -301,11 +306,7 @@ contains
                 return
             end if
             call c_f_pointer(vp(i), v, [n])
-            if (k > 0) then
-                prm(:, k) = v(1:n)
-            else if (ki == I_FAGN .and. present(fagn)) then
-                fagn = v(1:n)
-            end if
+            if (k > 0) prm(:, k) = v(1:n)
         end do
     end subroutine resolve_params
 
@@ -318,7 +319,7 @@ contains
         integer, intent(out) :: status
         type(c_ptr), pointer :: np(:), vp(:)
         real(c_double), pointer :: v(:), outv(:), outm(:, :), w_in(:, :)
-        real(WP), allocatable, target :: prm(:, :), wts(:, :), hbuf(:, :), mblend(:), fagn(:)
+        real(WP), allocatable, target :: prm(:, :), wts(:, :), hbuf(:, :), mblend(:)
         real(c_float), allocatable, target :: fbuf(:, :)
         integer(c_int), allocatable, target :: idx(:, :)
         integer :: i, k, m, st, nb, ns, nt, bytes, it
@@ -361,9 +362,9 @@ contains
         ns = c%nspec
         nt = c%nt
         ! names -> slots; the values of the named parameters per model over the context's
-        call resolve_params(c, n, n_params, c_names, c_values, prm, status, fagn)
+        call resolve_params(c, n, n_params, c_names, c_values, prm, status)
         if (status /= 0) return
-        call model_checks(c, prm, n, st, fagn)
+        call model_checks(c, prm, n, st)
         if (st /= 0) then
             status = st
             return
@@ -397,6 +398,7 @@ contains
             st = st + dbuf_reserve(c%d_young, int(bytes, c_size_t) * ns * n)
             st = st + dbuf_reserve(c%d_old, int(bytes, c_size_t) * ns * n)
         end if
+        if (c_associated(c_mdust)) st = st + dbuf_reserve(c%d_mdust, int(8, c_size_t) * n)
         if (st /= 0) then
             status = 10
             call set_error(10, 'batch: device allocation failed')
@@ -431,6 +433,8 @@ contains
         a%idx = c%d_idx%p
         a%weights = c_null_ptr
         if (want_w .and. .not. c_associated(c_weights)) a%weights = c%d_w%p
+        a%mdust = c_null_ptr
+        if (c_associated(c_mdust)) a%mdust = c%d_mdust%p
         rc = opensps_hip_csp_batch(c%dctx, a)
         if (rc == 0) rc = opensps_hip_sync(c%dctx)
         if (rc /= 0) then
@@ -449,10 +453,7 @@ contains
             call copy_spectra(c_young, c%d_young%p)
             call copy_spectra(c_old, c%d_old%p)
         end if
-        if (c_associated(c_mdust)) then
-            call c_f_pointer(c_mdust, outv, [n])
-            outv = SAFE_FLOOR
-        end if
+        if (c_associated(c_mdust)) rc = opensps_hip_memcpy_d2h(c_mdust, c%d_mdust%p, int(8, c_size_t) * n)
         if (c_associated(c_emlines)) then
             call c_f_pointer(c_emlines, outm, [166, n])
             outm = 0.0_wp

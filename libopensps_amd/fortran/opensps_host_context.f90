@@ -31,6 +31,11 @@ module opensps_host_context
         integer :: add_stellar_remnants = 1, zmet = 1
         ! an SSP operator installed on the device (opensps_context_set_ssp_ops): the batch then takes each model's IMF
         logical :: has_ops = .false.
+        ! the dust emission and AGN torus templates as given (opensps_context_set_dust_emission_templates,
+        ! opensps_context_set_agn_templates): the library's wavelengths and spectra, placed on the grid at upload
+        logical :: has_dustem = .false., has_agn = .false.
+        real(WP), allocatable :: dl_qpah(:), dl_umin(:), dl_lam(:), dl_spec(:, :, :)   ! (nlam, 2 numin, nqpah)
+        real(WP), allocatable :: agn_tau(:), agn_lam(:), agn_spec(:, :)                 ! (nlam, ntau)
         ! the resident basis
         logical :: has_basis = .false.
         integer :: nspec = 0, nt = 0, nz = 0
@@ -55,8 +60,8 @@ module opensps_host_context
         ! device side
         type(c_ptr) :: dctx = c_null_ptr
         logical :: dev_grid = .false., dev_bands = .false., dev_cosmo = .false., dev_mask = .false.
-        logical :: dev_vega = .false., dev_tab = .false., dev_dust = .false., dev_imf = .false.
-        type(dbuf_t) :: d_prm, d_w, d_mags, d_spec, d_mass, d_lbol, d_fband, d_young, d_old, d_idx
+        logical :: dev_vega = .false., dev_tab = .false., dev_dust = .false., dev_imf = .false., dev_emis = .false.
+        type(dbuf_t) :: d_prm, d_w, d_mags, d_spec, d_mass, d_lbol, d_fband, d_young, d_old, d_idx, d_mdust
         integer :: last_n = 0, last_bytes = 8
         real(WP), allocatable :: last_maggies(:, :)
     end type host_ctx_t
@@ -65,7 +70,7 @@ module opensps_host_context
     type(host_ctx_t), save :: blank_ctx   ! (default initialised: the state of a new context)
 
     public :: ctx_create, ctx_destroy, ctx_valid, ctx_set_int, ctx_set_float, ctx_set_str, ctx_set_basis, &
-              ctx_device, dbuf_reserve, ctx_refresh_cosmology, hip_message
+              ctx_device, dbuf_reserve, ctx_refresh_cosmology, hip_message, ctx_set_dust_emission, ctx_set_agn
 
 contains
 
@@ -113,7 +118,7 @@ contains
         associate(c => ctxs(h))
             call dbuf_free(c%d_prm); call dbuf_free(c%d_w); call dbuf_free(c%d_mags); call dbuf_free(c%d_spec)
             call dbuf_free(c%d_mass); call dbuf_free(c%d_lbol); call dbuf_free(c%d_fband); call dbuf_free(c%d_young)
-            call dbuf_free(c%d_old); call dbuf_free(c%d_idx)
+            call dbuf_free(c%d_old); call dbuf_free(c%d_idx); call dbuf_free(c%d_mdust)
             if (c_associated(c%dctx)) st = opensps_hip_destroy(c%dctx)
         end associate
         ctxs(h) = blank_ctx
@@ -149,9 +154,9 @@ contains
         associate(c => ctxs(h))
         select case (lower(trim(key)))
         case ('add_igm_absorption'); c%add_igm_absorption = v; c%dev_grid = .false.
-        case ('add_dust_emission'); c%add_dust_emission = v
+        case ('add_dust_emission'); c%add_dust_emission = v; c%dev_emis = .false.
         case ('add_neb_emission'); c%add_neb_emission = v
-        case ('add_agn_dust'); c%add_agn_dust = v
+        case ('add_agn_dust'); c%add_agn_dust = v; c%dev_emis = .false.
         case ('smooth_velocity'); c%smooth_velocity = v; c%dev_grid = .false.
         case ('smoothspec_fast'); c%smoothspec_fast = v; c%dev_grid = .false.
         case ('dust_type'); c%dust_type = v; c%dev_dust = .false.
@@ -372,7 +377,7 @@ contains
             end if
             c%dev_grid = .false.; c%dev_bands = .false.; c%dev_cosmo = .false.; c%dev_mask = .false.
             c%dev_prof = .false.; c%dev_vega = .false.; c%dev_tab = .false.; c%dev_dust = .false.
-            c%dev_imf = .false.
+            c%dev_imf = .false.; c%dev_emis = .false.
         end if
         if (.not. c%dev_grid) then
             rc = opensps_hip_set_grid(c%dctx, int(c%nspec, c_int), c%lam, int(c%nt, c_int), c%log_age, &
@@ -384,6 +389,7 @@ contains
             c%dev_grid = .true.
             c%dev_bands = .false.
             c%dev_dust = .false.
+            c%dev_emis = .false.
         end if
         if (.not. c%dev_bands) then
             if (c%nbands > 0) then
@@ -416,6 +422,11 @@ contains
             end if
             if (rc /= 0) goto 900
             c%dev_dust = .true.
+        end if
+        if (.not. c%dev_emis) then
+            st = emission_upload(c)
+            if (st /= 0) return
+            c%dev_emis = .true.
         end if
         if (.not. c%dev_tab) then
             if (c%ntab > 0) then
@@ -462,5 +473,158 @@ contains
         call set_error(st, 'device: ' // trim(hip_message(c%dctx)))
         end associate
     end function ctx_device
+
+    ! ---------------------------------------------------------------------------------------------
+    ! Dust emission and AGN torus templates
+    ! ---------------------------------------------------------------------------------------------
+
+    !> The reference's find_interval (src/math/opensps_interpolation.f90:149-198) on an ascending table: i with
+    !> a(i) <= v < a(i+1), 0 below, n above, 1 at a(1), n-1 at a(n)
+    pure integer function find_interval_asc(a, v) result(lower)
+        real(WP), intent(in) :: a(:), v
+        integer :: upper, mid, n
+        n = size(a)
+        lower = 0
+        upper = n + 1
+        do while (upper - lower > 1)
+            mid = (upper + lower) / 2
+            if (v >= a(mid)) then
+                lower = mid
+            else
+                upper = mid
+            end if
+        end do
+        if (v == a(1)) lower = 1
+        if (v == a(n)) lower = n - 1
+    end function find_interval_asc
+
+    !> interpolate_linear_scalar (src/math/opensps_interpolation.f90:45-72): the bracketing interval clamped to the
+    !> table, so that the ends extrapolate linearly
+    pure real(WP) function interp_lin(x, y, xo) result(v)
+        real(WP), intent(in) :: x(:), y(:), xo
+        integer :: i
+        i = max(1, min(find_interval_asc(x, xo), size(x) - 1))
+        v = y(i) + (y(i + 1) - y(i)) / (x(i + 1) - x(i)) * (xo - x(i))
+    end function interp_lin
+
+    logical function ascending(a)
+        real(WP), intent(in) :: a(:)
+        integer :: i
+        ascending = size(a) >= 2
+        do i = 2, size(a)
+            if (.not. (a(i) > a(i - 1))) ascending = .false.
+        end do
+    end function ascending
+
+    !> The Draine & Li (or any two-component) dust emission library as given: q_PAH (nqpah), U_min (numin) and
+    !> wavelengths (nlam, A) ascending; spectra (nlam, 2, numin, nqpah): per (q_PAH, U_min) the spectrum of U_max =
+    !> U_min, then of the power-law U distribution (the data library's layout, load_dust_emission_table,
+    !> src/io/opensps_io.f90:959-1069). Status 3 for bad sizes or tables not ascending.
+    subroutine ctx_set_dust_emission(h, nqpah, qpah, numin, umin, nlam, lam, spectra, status)
+        integer, intent(in) :: h, nqpah, numin, nlam
+        real(WP), intent(in) :: qpah(nqpah), umin(numin), lam(nlam), spectra(nlam, 2 * numin, nqpah)
+        integer, intent(out) :: status
+        status = 0
+        if (nqpah < 2 .or. numin < 2 .or. nlam < 2) then
+            status = 3
+            call set_error(3, 'set_dust_emission_templates: at least 2 q_PAH, 2 U_min and 2 wavelengths are required')
+            return
+        end if
+        if (.not. (ascending(qpah) .and. ascending(umin) .and. ascending(lam))) then
+            status = 3
+            call set_error(3, 'set_dust_emission_templates: q_PAH, U_min and wavelengths must ascend')
+            return
+        end if
+        associate(c => ctxs(h))
+            c%dl_qpah = qpah
+            c%dl_umin = umin
+            c%dl_lam = lam
+            c%dl_spec = spectra
+            c%has_dustem = .true.
+            c%dev_emis = .false.
+        end associate
+    end subroutine ctx_set_dust_emission
+
+    !> The AGN torus library as given: optical depths tau (ntau) and wavelengths (nlam, A) ascending, spectra
+    !> (nlam, ntau) (load_agn_dust_models, src/io/opensps_io.f90:1101-1155). Status 3 for bad sizes or order.
+    subroutine ctx_set_agn(h, ntau, tau, nlam, lam, spectra, status)
+        integer, intent(in) :: h, ntau, nlam
+        real(WP), intent(in) :: tau(ntau), lam(nlam), spectra(nlam, ntau)
+        integer, intent(out) :: status
+        status = 0
+        if (ntau < 2 .or. nlam < 2) then
+            status = 3
+            call set_error(3, 'set_agn_templates: at least 2 optical depths and 2 wavelengths are required')
+            return
+        end if
+        if (.not. (ascending(tau) .and. ascending(lam))) then
+            status = 3
+            call set_error(3, 'set_agn_templates: optical depths and wavelengths must ascend')
+            return
+        end if
+        associate(c => ctxs(h))
+            c%agn_tau = tau
+            c%agn_lam = lam
+            c%agn_spec = spectra
+            c%has_agn = .true.
+            c%dev_emis = .false.
+        end associate
+    end subroutine ctx_set_agn
+
+    !> The templates on the context's grid and to the device, with the two settings. Dust emission as
+    !> load_dust_emission_table places it: every column linear in wavelength from the pixel of 1 micron
+    !> (find_interval) on, extrapolating linearly past the library's ends, zero below; the AGN as load_agn_dust_models:
+    !> log-linear (log10 of flux + tiny) over the pixels from the one of the library's first wavelength to the one of
+    !> its last, zero elsewhere.
+    integer function emission_upload(c) result(st)
+        type(host_ctx_t), intent(inout), target :: c
+        real(WP), allocatable, target :: tb(:, :, :), ta(:, :)
+        real(WP), allocatable :: llam(:), lflux(:)
+        integer :: i, k, j, i0, i1, i2, nq, nu, nta
+        integer(c_int) :: rc
+        st = 0
+        rc = opensps_hip_set_emission(c%dctx, int(c%add_dust_emission, c_int), int(c%add_agn_dust, c_int))
+        if (rc /= 0) goto 900
+        if (c%has_dustem) then
+            nq = size(c%dl_qpah)
+            nu = size(c%dl_umin)
+            allocate(tb(c%nspec, nq, 2 * nu))
+            tb = 0.0_wp
+            i0 = max(find_interval_asc(c%lam, 1.0e4_wp), 1)
+            do k = 1, nq
+                do j = 1, 2 * nu
+                    do i = i0, c%nspec
+                        tb(i, k, j) = interp_lin(c%dl_lam, c%dl_spec(:, j, k), c%lam(i))
+                    end do
+                end do
+            end do
+            rc = opensps_hip_set_dust_emission(c%dctx, int(nq, c_int), c_loc(c%dl_qpah), int(nu, c_int), &
+                                               c_loc(c%dl_umin), c_loc(tb))
+        else
+            rc = opensps_hip_set_dust_emission(c%dctx, 0_c_int, c_null_ptr, 0_c_int, c_null_ptr, c_null_ptr)
+        end if
+        if (rc /= 0) goto 900
+        if (c%has_agn) then
+            nta = size(c%agn_tau)
+            allocate(ta(c%nspec, nta), llam(size(c%agn_lam)), lflux(size(c%agn_lam)))
+            ta = 0.0_wp
+            i1 = max(find_interval_asc(c%lam, c%agn_lam(1)), 1)
+            i2 = max(find_interval_asc(c%lam, c%agn_lam(size(c%agn_lam))), 1)
+            llam = log10(c%agn_lam)
+            do k = 1, nta
+                lflux = log10(c%agn_spec(:, k) + SAFE_FLOOR)
+                do i = i1, i2
+                    ta(i, k) = 10.0_wp**interp_lin(llam, lflux, log10(c%lam(i))) - SAFE_FLOOR
+                end do
+            end do
+            rc = opensps_hip_set_agn(c%dctx, int(nta, c_int), c_loc(c%agn_tau), c_loc(ta))
+        else
+            rc = opensps_hip_set_agn(c%dctx, 0_c_int, c_null_ptr, c_null_ptr)
+        end if
+        if (rc /= 0) goto 900
+        return
+900     st = 10
+        call set_error(10, 'batch: ' // trim(hip_message(c%dctx)))
+    end function emission_upload
 
 end module opensps_host_context

@@ -26,7 +26,7 @@ contains
                                    c_mass_bar, c_lbol_bar, c_mags, c_spec, c_mass, c_lbol, c_mdust, c_sfr, c_emlines, &
                                    c_indx, c_params_bar, c_weights_bar, c_weights_z_bar
         integer, intent(out) :: status
-        real(WP), allocatable, target :: prm(:, :), pb(:, :), hspec(:, :), fagn(:)
+        real(WP), allocatable, target :: prm(:, :), pb(:, :), hspec(:, :)
         real(c_float), allocatable, target :: fspec(:, :)
         real(c_double), pointer :: o2(:, :), o1(:)
         type(hip_grad_t) :: g
@@ -79,9 +79,9 @@ contains
             call set_error(2, 'batch_grad: ssp_weights_z with the SSP operator')
             return
         end if
-        call resolve_params(c, n, n_params, c_names, c_values, prm, status, fagn)
+        call resolve_params(c, n, n_params, c_names, c_values, prm, status)
         if (status /= 0) return
-        call model_checks(c, prm, n, st, fagn, c_associated(c_zred_mags))
+        call model_checks(c, prm, n, st, c_associated(c_zred_mags), .true.)
         if (st /= 0) then
             status = st
             return

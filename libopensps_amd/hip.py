@@ -25,7 +25,7 @@ class BatchArgs(C.Structure):
                 ("ssp_weights", C.c_void_p), ("mags", C.c_void_p), ("spec", C.c_void_p), ("mass", C.c_void_p),
                 ("lbol", C.c_void_p), ("fband", C.c_void_p), ("young", C.c_void_p), ("old", C.c_void_p),
                 ("idx", C.c_void_p), ("weights", C.c_void_p), ("op_count", C.c_void_p),
-                ("ssp_weights_z", C.c_void_p), ("zred_mags", C.c_void_p)]
+                ("ssp_weights_z", C.c_void_p), ("zred_mags", C.c_void_p), ("mdust", C.c_void_p)]
 
 
 class GradArgs(C.Structure):
@@ -78,6 +78,9 @@ def lib():
         L.opensps_hip_free_ssp_ops.argtypes = [vp]
         L.opensps_hip_set_imf.argtypes = [vp, i, i, d, d, d, d]
         L.opensps_hip_csp_batch_grad.argtypes = [vp, C.POINTER(GradArgs)]
+        L.opensps_hip_set_emission.argtypes = [vp, i, i]
+        L.opensps_hip_set_dust_emission.argtypes = [vp, i, vp, i, vp, vp]
+        L.opensps_hip_set_agn.argtypes = [vp, i, vp, vp]
         _lib = L
     return _lib
 
@@ -173,6 +176,30 @@ class DeviceContext:
             setattr(a, k, v.ctypes.data)
         self._check(lib().opensps_hip_set_ssp_ops(self._ctx, C.byref(a)))
 
+    def set_emission(self, add_dust_emission=1, add_agn_dust=1):
+        self._check(lib().opensps_hip_set_emission(self._ctx, int(add_dust_emission), int(add_agn_dust)))
+
+    def set_dust_emission(self, qpah, umin, dustem):
+        """The Draine & Li templates on the grid: dustem (nqpah, 2 numin, nspec) (C order; for each (q_PAH, U_min) the
+        U_max = U_min spectrum, then the power-law one), qpah, umin ascending; None removes them."""
+        if dustem is None:
+            self._check(lib().opensps_hip_set_dust_emission(self._ctx, 0, None, 0, None, None))
+            return
+        q, u = _host(qpah), _host(umin)
+        t = np.ascontiguousarray(np.transpose(_host(dustem), (1, 0, 2)))   # (2 numin, nqpah, nspec) C order
+        assert t.shape == (2 * u.size, q.size, self.grid.nspec)
+        self._check(lib().opensps_hip_set_dust_emission(self._ctx, q.size, q.ctypes.data, u.size, u.ctypes.data,
+                                                        t.ctypes.data))
+
+    def set_agn(self, tau, agn):
+        """The AGN torus templates on the grid: agn (ntau, nspec), tau ascending; None removes them."""
+        if agn is None:
+            self._check(lib().opensps_hip_set_agn(self._ctx, 0, None, None))
+            return
+        t, a = _host(tau), _host(agn)
+        assert a.shape == (t.size, self.grid.nspec)
+        self._check(lib().opensps_hip_set_agn(self._ctx, t.size, t.ctypes.data, a.ctypes.data))
+
     def free_ssp_ops(self):
         self._check(lib().opensps_hip_free_ssp_ops(self._ctx))
 
@@ -189,7 +216,7 @@ class DeviceContext:
         shapes = {"spec": ((n, g.nspec), sdt), "mass": ((n,), torch.float64), "lbol": ((n,), torch.float64),
                   "fband": ((n, self.nbands), torch.float64), "young": ((n, g.nspec), sdt),
                   "old": ((n, g.nspec), sdt), "idx": ((n, 3), torch.int32), "weights": ((n, g.nt), torch.float64),
-                  "op_count": ((n, 2), torch.int32)}
+                  "op_count": ((n, 2), torch.int32), "mdust": ((n,), torch.float64)}
         for k in want:
             if k in shapes:
                 out[k] = torch.empty(shapes[k][0], dtype=shapes[k][1], device=dev)
@@ -202,7 +229,7 @@ class DeviceContext:
         isz = 4 if precision == MIXED else 8
         sizes = {"mags": (n * self.nbands, 8), "spec": (n * g.nspec, isz), "young": (n * g.nspec, isz),
                  "old": (n * g.nspec, isz), "mass": (n, 8), "lbol": (n, 8), "fband": (n * self.nbands, 8),
-                 "idx": (3 * n, 4), "weights": (n * g.nt, 8), "op_count": (2 * n, 4)}
+                 "idx": (3 * n, 4), "weights": (n * g.nt, 8), "op_count": (2 * n, 4), "mdust": (n, 8)}
         for k, (numel, itemsize) in sizes.items():
             t = out.get(k)
             if t is not None and (not t.is_contiguous() or t.element_size() != itemsize or t.numel() < numel):
@@ -222,7 +249,7 @@ class DeviceContext:
         a.ssp_weights = ssp_weights.data_ptr() if ssp_weights is not None else None
         a.ssp_weights_z = ssp_weights_z.data_ptr() if ssp_weights_z is not None else None
         a.zred_mags = zred_mags.data_ptr() if zred_mags is not None else None
-        for k in ("mags", "spec", "mass", "lbol", "fband", "young", "old", "idx", "weights", "op_count"):
+        for k in ("mags", "spec", "mass", "lbol", "fband", "young", "old", "idx", "weights", "op_count", "mdust"):
             setattr(a, k, out[k].data_ptr() if k in out else None)
         return a
 

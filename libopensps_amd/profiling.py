@@ -17,7 +17,7 @@ The traffic report ends in a line ``traffic_bytes_per_launch: <int>`` that ``ben
 ``traffic`` field when the kernel matches.
 
     python -m libopensps_amd.profiling stats --db RUN_results.db --out profiles/X.md --command "..."
-    python -m libopensps_amd.profiling traffic --fetch F.csv --write W.csv --kernel 'k_chain<float, true>' \\
+    python -m libopensps_amd.profiling traffic --fetch F.csv --write W.csv --kernel 'k_chain<float, true, false>' \\
         --copy-bytes 268435456 --out profiles/Y.md --command "..."
 """
 from __future__ import annotations

@@ -339,3 +339,49 @@ def imf_params(n: int, seed: int = 13, slopes=(0.3, 3.3), upper=(60.0, 120.0), p
     out[SLOT["imf3"]] = rng.uniform(slopes[0], slopes[1], n)
     out[SLOT["imf_upper_limit"]] = rng.uniform(upper[0], upper[1], n)
     return out
+
+
+DL07_QPAH = np.array([0.47, 1.12, 1.77, 2.5, 3.19, 3.9, 4.58])
+DL07_UMIN = np.array([0.1, 0.15, 0.2, 0.3, 0.4, 0.5, 0.7, 0.8, 1.0, 1.2, 1.5, 2.0, 2.5, 3.0, 4.0, 5.0, 7.0, 8.0,
+                      12.0, 15.0, 20.0, 25.0])
+AGN_TAU = np.array([5.0, 10.0, 20.0, 30.0, 40.0, 60.0, 80.0, 100.0, 150.0])
+
+
+def dust_emission_templates(lam: np.ndarray, qpah=DL07_QPAH, umin=DL07_UMIN) -> np.ndarray:
+    """Synthetic dust emission templates on the grid, (nqpah, 2 numin, nspec), in the layout of the Draine & Li
+    tables as the reference loads them (zero below 1 micron): for each (q_PAH, U_min) a modified blackbody of
+    T = 18 K U^(1/6) (beta = 2) for U_max = U_min, then a warmer mixture with PAH bands (Gaussians at 3.3, 6.2, 7.7,
+    8.6, 11.3 micron, strength ~ q_PAH) for the power-law U distribution."""
+    nu = 2.99792458e18 / lam
+    out = np.zeros((len(qpah), 2 * len(umin), lam.size))
+    on = lam >= 1.0e4
+    um = lam * 1e-4
+    pah = sum(a * np.exp(-0.5 * ((um - c) / (0.03 * c)) ** 2)
+              for a, c in ((0.3, 3.3), (1.0, 6.2), (1.6, 7.7), (0.6, 8.6), (0.9, 11.3)))
+    for k, q in enumerate(qpah):
+        for j, u in enumerate(umin):
+            for comp in range(2):
+                t = 18.0 * u ** (1.0 / 6.0) * (1.0 if comp == 0 else 1.6)
+                x = np.minimum(HC_OVER_K / (lam * t), 700.0)
+                mbb = nu ** 5 / np.expm1(x)
+                mbb = mbb / mbb[on].max()
+                v = mbb + (0.02 * q * pah if comp == 1 else 0.005 * q * pah)
+                out[k, 2 * j + comp] = np.where(on, 1.0e-24 * v, 0.0)
+    return out
+
+
+def agn_templates(lam: np.ndarray, tau=AGN_TAU) -> np.ndarray:
+    """Synthetic AGN torus templates on the grid, (ntau, nspec), zero outside 0.1 - 1000 micron: a broken power law in
+    nu L_nu peaking between 8 and 30 micron with the optical depth, per unit bolometric luminosity (1/Hz)."""
+    nu = 2.99792458e18 / lam
+    um = lam * 1e-4
+    out = np.zeros((len(tau), lam.size))
+    on = (um >= 0.1) & (um <= 1000.0)
+    for k, t in enumerate(tau):
+        peak = 8.0 * (t / 5.0) ** 0.4
+        x = um / peak
+        nulnu = x ** 1.5 / (1.0 + x ** 4.0)
+        f = np.where(on, nulnu / nu, 0.0)
+        dnu = np.abs(np.gradient(nu))
+        out[k] = f / np.sum(f * dnu)
+    return out

@@ -1456,7 +1456,58 @@ typedef struct {
      * the observed frame and the distance term at it, the IGM at zred; the reverse pass then leaves the observed
      * frame's and the distance term's dz out of zred's row (batch_backward's lzm, :4830-4832, 4928) */
     const double *zred_mags;
+    /* dust emission (add_dust_emission) with the Draine & Li templates on the grid (dustem2_dustem: nspec x nqpah x
+     * 2 numin, column-major; qpah, umin ascending) and the AGN torus (add_agn_dust) with its templates (agndust_spec:
+     * nspec x ntau; agn_tau ascending); nqpah / ntau 0: none; mdust_out (n) or NULL */
+    int add_dust_emission, add_agn_dust;
+    int nqpah, numin, ntau;
+    const double *qpah, *umin, *dustem, *agn_tau, *agn;
+    double *mdust_out;
 } orc_ctx_t;
+
+/* one_minus_trans_kp (opensps_csp_batch.f90:3925-3939), FP64: 1 - exp(-x) without cancellation */
+static double one_minus_trans(double x, double t) {
+    if (t == 1.0) return x;
+    if (fabs(x) < 1.0) return (1.0 - t) * x / (-log(t));
+    return 1.0 - t;
+}
+
+/* kadd: Kahan's compensated sum (the reference's CPU build of dust_emission_reductions, :4026-4030) */
+static void kadd(double *s, double *c, double v) {
+    double y = v - *c, t = *s + y;
+    *c = (t - *s) - y;
+    *s = t;
+}
+
+/* dl07_indices (opensps_csp_batch.f90:1611-1626) and dl07_shape (:4103-4122): the Draine & Li template at pixel i,
+ * bilinear in q_PAH and U_min (intervals clamped to the grid, weights not), the two U_max spectra mixed by gamma
+ * (clamped to [0, 1]), floored at the smallest normal */
+static void dl07_indices(const orc_ctx_t *c, const double *p, int *iq, double *wq, int *iu, double *wu, double *gam) {
+    int q = orc_find_interval(c->qpah, c->nqpah, p[OPS_P_DUSTE_QPAH]);
+    if (q > c->nqpah - 1) q = c->nqpah - 1;
+    if (q < 1) q = 1;
+    *iq = q;
+    *wq = (p[OPS_P_DUSTE_QPAH] - c->qpah[q - 1]) / (c->qpah[q] - c->qpah[q - 1]);
+    int u = orc_find_interval(c->umin, c->numin, p[OPS_P_DUSTE_UMIN]);
+    if (u > c->numin - 1) u = c->numin - 1;
+    if (u < 1) u = 1;
+    *iu = u;
+    *wu = (p[OPS_P_DUSTE_UMIN] - c->umin[u - 1]) / (c->umin[u] - c->umin[u - 1]);
+    *gam = fmax(0.0, fmin(p[OPS_P_DUSTE_GAMMA], 1.0));
+}
+
+/* the table's value at pixel i (0-based), q_PAH column q and spectrum column k (1-based, dustem2_dustem(i, q, k)) */
+static double dustem_at(const orc_ctx_t *c, int i, int q, int k) {
+    return c->dustem[((size_t)(q - 1) + (size_t)c->nqpah * (k - 1)) * c->nspec + i];
+}
+
+static double dl07_shape(const orc_ctx_t *c, int i, int iq, double wq, int iu, double wu, double gam) {
+    double lo = (1 - wq) * (1 - wu) * dustem_at(c, i, iq, 2 * iu - 1) + wq * (1 - wu) * dustem_at(c, i, iq + 1, 2 * iu - 1) +
+                wq * wu * dustem_at(c, i, iq + 1, 2 * (iu + 1) - 1) + (1 - wq) * wu * dustem_at(c, i, iq, 2 * (iu + 1) - 1);
+    double hi = (1 - wq) * (1 - wu) * dustem_at(c, i, iq, 2 * iu) + wq * (1 - wu) * dustem_at(c, i, iq + 1, 2 * iu) +
+                wq * wu * dustem_at(c, i, iq + 1, 2 * (iu + 1)) + (1 - wq) * wu * dustem_at(c, i, iq, 2 * (iu + 1));
+    return fmax((1.0 - gam) * lo + gam * hi, DBL_MIN);
+}
 
 /* The model's IMF parameters (batch_imf_vectors, opensps_csp_batch.f90:1823-1853): the slopes and limits per model,
  * the rest from the context */
@@ -1543,6 +1594,7 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
         int *soff = (int *)malloc(sizeof(int) * nspec);
         double *scoef = (double *)malloc(sizeof(double) * cap);
         double *tau = (double *)malloc(sizeof(double) * (jtop + 1));
+        double *tdv = (double *)malloc(sizeof(double) * nspec);
         double sm_key[3] = {NAN, NAN, NAN};
         int sm_n = 0;
         double *smass = NULL, *slum = NULL, *vy = NULL, *vo = NULL, *wpt = NULL, *llo = NULL, *dl = NULL;
@@ -1566,6 +1618,7 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
             int blend = nz > 1;
             if (blend) model_zlo(c, p[OPS_P_LOGZSOL], &zlo, &dz);
             if (idx) { idx[3 * m] = it; idx[3 * m + 1] = zlo; idx[3 * m + 2] = 0; }
+            double lsum_m = 0.0;   /* the linear L_bol sum (the AGN's luminosity) */
             /* stage 0 checks (batch_check_models :1312-1320; plan_smoothing host modes are not built) */
             if ((!ssp_weights && !c->ssp_weights_z && p[OPS_P_TAGE] <= 0.0) || (!c->smooth_fast && p[OPS_P_SIGMA_SMOOTH] > SAFE_FLOOR)) {
                 #pragma omp atomic write
@@ -1594,6 +1647,7 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
                 if (w_out) for (int k = 0; k < nt; k++) w_out[(size_t)m * nt + k] = 0.0;
                 if (mass) mass[m] = ms;
                 if (lbol) lbol[m] = ls > SAFE_FLOOR ? log10(ls) : 0.0;
+                lsum_m = ls;
                 if (young) memcpy(young + (size_t)m * nspec, y, sizeof(double) * nspec);
                 if (old) memcpy(old + (size_t)m * nspec, o, sizeof(double) * nspec);
                 goto dust;
@@ -1656,6 +1710,7 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
         sums_done:
             if (mass) mass[m] = ms;
             if (lbol) lbol[m] = ls > SAFE_FLOOR ? log10(ls) : 0.0;
+            lsum_m = ls;
             if (young) memcpy(young + (size_t)m * nspec, y, sizeof(double) * nspec);
             if (old) memcpy(old + (size_t)m * nspec, o, sizeof(double) * nspec);
         dust:;
@@ -1664,17 +1719,57 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
             double ed = p[OPS_P_DUST_INDEX];
             int no_birth = p[OPS_P_DUST1] <= SAFE_FLOOR || 1.0 - fob <= SAFE_FLOOR;
             int no_old = fabs(p[OPS_P_DUST3]) <= SAFE_FLOOR;
+            /* dust emission and the AGN torus: who emits (batch_model_indices :1569-1570, agn_indices :1591-1606) */
+            const int dusty = p[OPS_P_DUST1] > SAFE_FLOOR || p[OPS_P_DUST2] > SAFE_FLOOR;
+            const int emis = c->add_dust_emission == 1 && dusty;
+            const int agn_on = c->add_agn_dust == 1 && p[OPS_P_FAGN] > SAFE_FLOOR;
+            if ((emis && c->nqpah == 0) || (agn_on && c->ntau == 0)) {
+                #pragma omp atomic write
+                status = 2;
+                continue;
+            }
+            int iq = 1, iu = 1;
+            double wq = 0.0, wu = 0.0, gam = 0.0;
+            if (emis) dl07_indices(c, p, &iq, &wq, &iu, &wu, &gam);
+            double *absb = work;   /* (free until the smoothing) */
             for (int i = 0; i < nspec; i++) {
                 double curve = 1.0;
                 if (c->dcurve) curve = c->dcurve[i];
                 else if (ed != 0.0) curve = curve * exp(ed * ltab[i]);
                 double tau_d = p[OPS_P_DUST2] * curve, tr_d = exp(-tau_d);
-                double tr_b = 1.0, tr_o = 1.0;
-                if (!no_birth) tr_b = exp(-(p[OPS_P_DUST1] * exp(p[OPS_P_DUST1_INDEX] * ltab[i])));
-                if (!no_old) tr_o = exp(-(p[OPS_P_DUST3] * curve));
+                double tau_b = 0.0, tau_o = 0.0, tr_b = 1.0, tr_o = 1.0;
+                if (!no_birth) { tau_b = p[OPS_P_DUST1] * exp(p[OPS_P_DUST1_INDEX] * ltab[i]); tr_b = exp(-tau_b); }
+                if (!no_old) { tau_o = p[OPS_P_DUST3] * curve; tr_o = exp(-tau_o); }
                 double ssum = (y[i] * tr_b * (1.0 - fob) + y[i] * fob) + o[i] * tr_o;
                 if (1.0 - fnd <= SAFE_FLOOR) tot[i] = ssum;
                 else tot[i] = ssum * (tr_d * (1.0 - fnd) + fnd);
+                if (emis) {
+                    /* the absorbed light (attenuate_batch :3640-3656) */
+                    double omtb = one_minus_trans(tau_b, tr_b), omto = one_minus_trans(tau_o, tr_o);
+                    double omtd = (1.0 - fnd <= SAFE_FLOOR) ? 0.0 : (1.0 - fnd) * one_minus_trans(tau_d, tr_d);
+                    double ay = tr_b * (1.0 - fob) + fob;
+                    absb[i] = y[i] * ((1.0 - fob) * omtb + ay * omtd) + o[i] * (omto + tr_o * omtd);
+                    fobs[i] = dl07_shape(c, i, iq, wq, iu, wu, gam);   /* the template (fobs: free until stage 8) */
+                    tdv[i] = tr_d;
+                }
+            }
+            if (emis) {
+                /* dust_emission_reductions (:3963-4042, the CPU build's compensated sums) and add_dust_emission
+                 * (:4045-4099): the template times the diffuse transmission, normalised to the absorbed light over
+                 * frequency (prepare_batch_grid's wt, src/spectra/opensps_batch_photometry.f90:262) */
+                double sa = 0.0, sn = 0.0, se = 0.0, ca = 0.0, cn = 0.0, ce = 0.0;
+                for (int i = 0; i < nspec - 1; i++) {
+                    double wt = 0.5 * fabs(2.99792458e18 / c->lambda[i + 1] - 2.99792458e18 / c->lambda[i]);
+                    kadd(&sa, &ca, wt * (absb[i] + absb[i + 1]));
+                    kadd(&sn, &cn, wt * (fobs[i] + fobs[i + 1]));
+                    kadd(&se, &ce, wt * (fobs[i] * tdv[i] + fobs[i + 1] * tdv[i + 1]));
+                }
+                double rn = se > SAFE_FLOOR ? sa / se : 0.0;
+                if (rn != 0.0)
+                    for (int i = 0; i < nspec; i++) tot[i] = tot[i] + fobs[i] * tdv[i] * rn;
+                if (c->mdust_out) c->mdust_out[m] = sn > SAFE_FLOOR ? 3.21e-3 / (4.0 * acos(-1.0)) * (sa / sn) : SAFE_FLOOR;
+            } else if (c->mdust_out) {
+                c->mdust_out[m] = SAFE_FLOOR;
             }
             /* stage 6: IGM (apply_igm_z :4205-4319; uniform z: compute_igm_transmission) */
             double z = p[OPS_P_ZRED];
@@ -1695,6 +1790,27 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
                 for (int j = 1; j <= jtop; j++) if (tau[j - 1] > tmax) { tmax = tau[j - 1]; jmax = j; }
                 if (jmax > 1) for (int j = 1; j <= jmax; j++) tau[j - 1] = tmax;
                 for (int j = 0; j < jtop; j++) tot[j] = tot[j] * exp(-tau[j] * p[OPS_P_IGM_FACTOR]);
+            }
+            /* the AGN torus (add_agn_batch :4342-4433): fagn L_bol (L_bol through its logarithm) times the agn_tau
+             * template, linear between its bracketing depths (agn_indices :1591-1606: weight clamped), times the
+             * diffuse transmission; after the IGM */
+            if (agn_on) {
+                int ia = orc_find_interval(c->agn_tau, c->ntau, p[OPS_P_AGN_TAU]);
+                if (ia > c->ntau - 1) ia = c->ntau - 1;
+                if (ia < 1) ia = 1;
+                double wta = (p[OPS_P_AGN_TAU] - c->agn_tau[ia - 1]) / (c->agn_tau[ia] - c->agn_tau[ia - 1]);
+                wta = fmax(0.0, fmin(wta, 1.0));
+                double lb = lsum_m > SAFE_FLOOR ? log10(lsum_m) : 0.0;
+                double lagn = pow(10.0, lb) * p[OPS_P_FAGN];
+                for (int i = 0; i < nspec; i++) {
+                    double curve = 1.0;
+                    if (c->dcurve) curve = c->dcurve[i];
+                    else if (ed != 0.0) curve = curve * exp(ed * ltab[i]);
+                    double tr_d = exp(-(p[OPS_P_DUST2] * curve));
+                    double tmpl = (1.0 - wta) * c->agn[(size_t)(ia - 1) * nspec + i] + wta * c->agn[(size_t)ia * nspec + i];
+                    tmpl = tmpl * tr_d;
+                    tot[i] = tot[i] + lagn * tmpl;
+                }
             }
             /* stage 7: velocity smoothing (smooth_columns / smooth_kernel FP64, build_velocity_kernel) */
             double sig = p[OPS_P_SIGMA_SMOOTH];
@@ -1720,7 +1836,7 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
             }
         }
         free(p); free(w); free(t1); free(t2); free(y); free(o); free(tot); free(work); free(fobs); free(fb);
-        free(slo); free(shi); free(soff); free(scoef); free(tau);
+        free(slo); free(shi); free(soff); free(scoef); free(tau); free(tdv);
         free(smass); free(slum); free(vy); free(vo); free(wpt); free(llo); free(dl);
     }
     free(jlo); free(jhi); free(do_band); free(bt); free(wtl); free(ltab);
@@ -1822,7 +1938,11 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
             double dz = 0.0;
             const int blend = nz > 1 && !wz_on;
             if (blend) model_zlo(c, p[OPS_P_LOGZSOL], &zlo, &dz);
-            if ((!ssp_weights && !wz_on && p[OPS_P_TAGE] <= 0.0) || (!c->smooth_fast && p[OPS_P_SIGMA_SMOOTH] > SAFE_FLOOR)) {
+            /* (the reverse pass of the dust emission and the AGN torus is not restated: status 2 where they act) */
+            const int emits = (c->add_dust_emission == 1 && (p[OPS_P_DUST1] > SAFE_FLOOR || p[OPS_P_DUST2] > SAFE_FLOOR)) ||
+                              (c->add_agn_dust == 1 && p[OPS_P_FAGN] > SAFE_FLOOR);
+            if (emits || (!ssp_weights && !wz_on && p[OPS_P_TAGE] <= 0.0) ||
+                (!c->smooth_fast && p[OPS_P_SIGMA_SMOOTH] > SAFE_FLOOR)) {
                 #pragma omp atomic write
                 status = 2;
                 continue;

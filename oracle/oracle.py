@@ -359,7 +359,10 @@ class _Ctx(C.Structure):
                 ("add_igm", C.c_int), ("smooth_fast", C.c_int), ("tiny_logt", C.c_double),
                 ("ops", C.c_void_p), ("imf_type", C.c_int), ("remnants", C.c_int), ("salp_ind", C.c_double),
                 ("mdave", C.c_double), ("mlim_bh", C.c_double), ("mlim_ns", C.c_double), ("op_count", C.c_void_p),
-                ("ssp_weights_z", C.c_void_p), ("dcurve", C.c_void_p), ("zred_mags", C.c_void_p)]
+                ("ssp_weights_z", C.c_void_p), ("dcurve", C.c_void_p), ("zred_mags", C.c_void_p),
+                ("add_dust_emission", C.c_int), ("add_agn_dust", C.c_int), ("nqpah", C.c_int), ("numin", C.c_int),
+                ("ntau", C.c_int), ("qpah", C.c_void_p), ("umin", C.c_void_p), ("dustem", C.c_void_p),
+                ("agn_tau", C.c_void_p), ("agn", C.c_void_p), ("mdust_out", C.c_void_p)]
 
 
 def _context(grid, bands, add_igm, smooth_fast, mag_mask, cosmo, ptr):
@@ -395,6 +398,81 @@ def _set_ops(ctx, ops, ptr, keep, imf_type, remnants, salp_ind, mdave, mlim_bh, 
     ctx.ops = C.cast(C.pointer(o), C.c_void_p)
     ctx.imf_type, ctx.remnants = int(imf_type), int(remnants)
     ctx.salp_ind, ctx.mdave, ctx.mlim_bh, ctx.mlim_ns = salp_ind, mdave, mlim_bh, mlim_ns
+
+
+class Emission:
+    """Dust emission and the AGN torus for the oracle: the Draine & Li templates on the grid (dustem (nqpah, 2 numin,
+    nspec) C order = the reference's dustem2_dustem (nspec, nqpah, 2 numin)), q_PAH and U_min; the AGN templates
+    (agn (ntau, nspec)) and their optical depths; add_dust_emission, add_agn_dust."""
+
+    def __init__(self, qpah=None, umin=None, dustem=None, agn_tau=None, agn=None, add_dust_emission=1,
+                 add_agn_dust=1):
+        self.qpah, self.umin, self.agn_tau = qpah, umin, agn_tau
+        self.dustem = None if dustem is None else np.ascontiguousarray(dustem, dtype=np.float64)
+        self.agn = None if agn is None else np.ascontiguousarray(agn, dtype=np.float64)
+        self.add_dust_emission, self.add_agn_dust = add_dust_emission, add_agn_dust
+
+
+def _find_interval_asc(a, v):
+    """find_interval (src/math/opensps_interpolation.f90:149-198) on an ascending table, vectorised: the 1-based i
+    with a(i) <= v < a(i+1), 0 below, n above, 1 at a(1), n-1 at a(n)."""
+    a = np.asarray(a, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    i = np.searchsorted(a, v, side="right")
+    i = np.where(v == a[0], 1, i)
+    return np.where(v == a[-1], a.size - 1, i)
+
+
+def _interp_extrap(x, y, xo):
+    """interpolate_linear_scalar (src/math/opensps_interpolation.f90:45-72): the bracketing interval clamped to the
+    table, so that the ends extrapolate linearly."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    i = np.clip(_find_interval_asc(x, xo), 1, x.size - 1)
+    return y[i - 1] + (y[i] - y[i - 1]) / (x[i] - x[i - 1]) * (xo - x[i - 1])
+
+
+def emission_on_grid(lam, dust=None, agn=None, add_dust_emission=1, add_agn_dust=1):
+    """The libraries of datalib.dust_emission (qpah, umin, wavelength, spectra (nqpah, numin, 2, nlam)) and
+    datalib.agn_torus (tau, wavelength, spectra (ntau, nlam)) placed on the grid lam as the reference loads them:
+    load_dust_emission_table (src/io/opensps_io.f90:1049-1059: each column linear in wavelength from the pixel of
+    1 micron on, extrapolated past the library's ends, zero below) and load_agn_dust_models (:1144-1154: log10 of
+    flux + tiny linear in log10 wavelength over the pixels from the library's first to its last wavelength, zero
+    elsewhere). Returns an Emission."""
+    lam = np.asarray(lam, dtype=np.float64)
+    tiny = np.finfo(np.float64).tiny
+    qp = um = dt = at = tb = None
+    if dust is not None:
+        qp, um, dl, ds = dust
+        i0 = max(int(_find_interval_asc(lam, 1.0e4)), 1)
+        nq, nu = len(qp), len(um)
+        dt = np.zeros((nq, 2 * nu, lam.size))
+        for k in range(nq):
+            for j in range(nu):
+                for comp in range(2):
+                    dt[k, 2 * j + comp, i0 - 1:] = _interp_extrap(dl, ds[k, j, comp], lam[i0 - 1:])
+    if agn is not None:
+        at, al, asp = agn
+        i1 = max(int(_find_interval_asc(lam, al[0])), 1)
+        i2 = max(int(_find_interval_asc(lam, al[-1])), 1)
+        tb = np.zeros((len(at), lam.size))
+        for k in range(len(at)):
+            tb[k, i1 - 1:i2] = 10.0 ** _interp_extrap(np.log10(al), np.log10(asp[k] + tiny),
+                                                      np.log10(lam[i1 - 1:i2])) - tiny
+    return Emission(qp, um, dt, at, tb, add_dust_emission=add_dust_emission, add_agn_dust=add_agn_dust)
+
+
+def _set_emission(ctx, e, ptr):
+    ctx.add_dust_emission, ctx.add_agn_dust = int(e.add_dust_emission), int(e.add_agn_dust)
+    if e.dustem is not None:
+        ctx.nqpah, ctx.numin = len(e.qpah), len(e.umin)
+        assert e.dustem.shape[:2] == (ctx.nqpah, 2 * ctx.numin)
+        # (nqpah, 2 numin, nspec) C order -> (2 numin, nqpah, nspec) C order = (nspec, nqpah, 2 numin) column-major
+        ctx.dustem = ptr(np.ascontiguousarray(np.transpose(e.dustem, (1, 0, 2))))
+        ctx.qpah, ctx.umin = ptr(_d(e.qpah)), ptr(_d(e.umin))
+    if e.agn is not None:
+        ctx.ntau = len(e.agn_tau)
+        ctx.agn, ctx.agn_tau = ptr(e.agn), ptr(_d(e.agn_tau))
 
 
 NBGRAD = 44
@@ -444,10 +522,11 @@ def csp_batch_grad(grid, bands, prm, ssp_weights=None, ssp_weights_z=None, mags_
 
 def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_mask=None, cosmo=None,
               want=("mags", "spec", "mass", "lbol"), ops=None, imf_type=2, remnants=1, salp_ind=2.35, mdave=0.5,
-              mlim_bh=40.0, mlim_ns=8.5, ssp_weights_z=None, dust_curve=None, zred_mags=None):
+              mlim_bh=40.0, mlim_ns=8.5, ssp_weights_z=None, dust_curve=None, zred_mags=None, emission=None):
     """Runs the oracle's batch. grid: synth.SSPGrid; bands (nbands, nspec); prm (NPAR, n); ops: a
     synth.SSPOperator (the IMF per model; the grid's bases are then unused); ssp_weights_z (n, nz, nt): weights
-    over ages and metallicities.
+    over ages and metallicities; emission: an Emission (dust emission and AGN templates on the grid, with the two
+    settings).
     Returns a dict of outputs (C-order arrays: mags (n, nbands), spec (n, nspec), ...) and the status."""
     keep = []
 
@@ -481,6 +560,8 @@ def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_
     ctx.add_igm = int(add_igm)
     ctx.smooth_fast = int(smooth_fast)
     ctx.tiny_logt = 0.0
+    if emission is not None:
+        _set_emission(ctx, emission, ptr)
     if ops is not None:
         _set_ops(ctx, ops, ptr, keep, imf_type, remnants, salp_ind, mdave, mlim_bh, mlim_ns)
         if "op_count" in want:
@@ -489,9 +570,11 @@ def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_
             ctx.op_count = cnt.ctypes.data
     out = {"mags": np.zeros((n, nb))}
     for k, shape in (("spec", (n, grid.nspec)), ("mass", (n,)), ("lbol", (n,)), ("young", (n, grid.nspec)),
-                     ("old", (n, grid.nspec)), ("w", (n, grid.nt)), ("fband", (n, nb))):
+                     ("old", (n, grid.nspec)), ("w", (n, grid.nt)), ("fband", (n, nb)), ("mdust", (n,))):
         if k in want:
             out[k] = np.zeros(shape)
+    if "mdust" in out:
+        ctx.mdust_out = out["mdust"].ctypes.data
     idx = np.zeros((n, 3), np.int32) if "idx" in want else None
     if idx is not None:
         out["idx"] = idx

@@ -47,7 +47,7 @@ def test_param_slots_match_header():
     src = open(os.path.join(ROOT, "libopensps_amd", "fortran", "opensps_host_types.f90")).read()
     names = re.search(r"PARAM_NAMES\(NPAR\) = \[character\(len=16\) :: &(.*?)\]", src, re.S).group(1)
     assert re.findall(r"'(\w+)'", names) == PARAM_NAMES
-    assert NPAR == 28
+    assert NPAR == 33
 
 
 def test_handles_and_setters():
@@ -141,7 +141,7 @@ def test_batch_statuses_before_the_device():
         assert r["status"] == 2
         r = ctx.compute_csp_batch({"tage": np.ones(3), "dust2": np.ones(3)}, raw_status=True)
         assert r["status"] == 2      # add_dust_emission = 1 by default (reference src/core/opensps_context.f90:78)
-        assert "dust emission" in api.last_error()
+        assert "dust emission" in api.last_error() and "templates" in api.last_error()
         ctx.set_int("add_dust_emission", 0)
         r = ctx.compute_csp_batch({}, n=0, raw_status=True)
         assert r["status"] == 0
@@ -249,9 +249,38 @@ def test_imf_and_ssp_parameters_without_operator():
             assert r["status"] == 2, (nm, r["status"])
         r = ctx.compute_csp_batch({"not_a_parameter": np.ones(3)}, raw_status=True)
         assert r["status"] == 3
-        # fagn > 0 asks for the AGN stage (add_agn_dust = 1 by default)
+        # fagn > 0 asks for the AGN stage (add_agn_dust = 1 by default): its templates are needed
         r = ctx.compute_csp_batch({"tage": np.full(3, 5.0), "fagn": np.full(3, 0.1)}, raw_status=True)
         assert r["status"] == 2 and "AGN" in api.last_error()
+
+
+def test_emission_templates_api():
+    """opensps_context_set_dust_emission_templates / _set_agn_templates: the test release's DL07 and clumpy libraries
+    are taken; sizes below two or wavelengths not ascending are status 3; with the templates, dust emission and the
+    AGN pass the host's checks (status 5 here: no device); the batch with gradients refuses them (status 2)."""
+    import torch
+    from libopensps_amd import datalib
+    root = os.path.join(ROOT, "tests", "golden", "opensps-test-data")
+    q, u, lam, sp = datalib.dust_emission(root)
+    tau, alam, asp = datalib.agn_torus(root)
+    g = grid(1)
+    with api.Context() as ctx:
+        ctx.set_ssp_basis(g)
+        ctx.set_filters(synth.gaussian_filters(4))
+        with pytest.raises(api.OpenSPSError) as e:
+            ctx.set_dust_emission_templates(q, u, lam[::-1].copy(), sp)
+        assert e.value.status == 3
+        with pytest.raises(api.OpenSPSError) as e:
+            ctx.set_agn_templates(tau[:1], alam, asp[:1])
+        assert e.value.status == 3
+        ctx.set_dust_emission_templates(q, u, lam, sp)
+        ctx.set_agn_templates(tau, alam, asp)
+        vals = {"tage": np.full(3, 5.0), "dust2": np.full(3, 0.5), "fagn": np.full(3, 0.1)}
+        r = ctx.compute_csp_batch_grad(vals, mags_bar=np.ones((3, 4)), grads=("params_bar",), raw_status=True)
+        assert r["status"] == 2 and "reverse pass" in api.last_error()
+        if not torch.cuda.is_available():
+            r = ctx.compute_csp_batch(vals, raw_status=True)
+            assert r["status"] == 5
 
 
 def test_parameters_no_stage_reads_are_accepted():
