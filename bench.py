@@ -25,6 +25,9 @@ METRIC = "CSP models/s, 16k batch, mixed vs FP64 GPU at ≤1e-4 rel err; emulato
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 FP64_PEAK_TFLOPS = 78.6
+# FP64 operations per (model, common isochrone point) of k_op_weights: the 12-FMA series of expm1(y) / y (24), y = e dl
+# (1), the chain's next power phi = plo + plo y sx (3), w = c dl plo sx (3), the mass and L_bol sums (4)
+OP_WEIGHT_FLOPS = 35
 KERNELS = ["prep", "group", "age_sums", "chain"]
 MAG_RANGE = 40.0
 CHAIN_KERNEL = "k_chain<float, true, false>"
@@ -123,9 +126,10 @@ def operator_workload(args, n=4096):
         torch.cuda.synchronize()
         dt = time.perf_counter() - t1
         kms, _ = kernel_times(ctx, reset=1)
+        oms = op_kernel_times(ctx, reset=1)
         res[name] = {"models_per_s": n * steps / dt, "ms_per_step": 1e3 * dt / steps,
-                     "kernel_ms": dict(zip(["prep", "imf_weights_fold_sums", "template_product", "chain"],
-                                           [round(x, 4) for x in kms]))}
+                     "kernel_ms": dict(zip(["prep", "imf_weights_fold_sums", "template_product", "chain",
+                                            "op_imf_weights", "op_fold_sums"], [round(x, 4) for x in kms + oms]))}
         outs[name] = {k: v.clone() for k, v in out.items()}
     ref = outs["fp64"]["spec"]
     peak = ref.abs().amax(dim=1, keepdim=True)
@@ -140,16 +144,37 @@ def operator_workload(args, n=4096):
     # template product: useful flops 2 nspec ntpl per model column
     kmix = res["mixed"]["kernel_ms"]["template_product"]
     flops = 2.0 * grid.nspec * op.ntpl * n
-    roof = {"bound": "mfma", "kernel": "k_mm_bf16x3 (template product)", "achieved": flops / (kmix * 1e-3) / 1e12,
-            "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": flops / (kmix * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
-            "algorithmic_per_launch": flops}
+    roofs = {"template_product": {
+        "bound": "mfma", "kernel": "k_mm_bf16x3 (template product)", "achieved": flops / (kmix * 1e-3) / 1e12,
+        "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": flops / (kmix * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
+        "algorithmic_per_launch": flops, "ms": kmix}}
+    # the IMF weights (k_op_imf, k_op_slow, k_op_weights): every model's points in its two metallicities (each
+    # metallicity of the operator holds npts / nz points), OP_WEIGHT_FLOPS FP64 operations a point, and the weights
+    # written in the path's precision (FP32 mixed, FP64) for the segments
+    pts = 2.0 * op.npts / op.nz * n
+    for name, wsize in (("mixed", 4), ("fp64", 8)):
+        kw = res[name]["kernel_ms"]["op_imf_weights"]
+        if kw <= 0:
+            continue
+        fl, by = OP_WEIGHT_FLOPS * pts, wsize * pts
+        t_fl, t_by = fl / (FP64_PEAK_TFLOPS * 1e12), by / (HBM_PEAK_GBS * 1e9)
+        bound = "fp64" if t_fl >= t_by else "hbm"
+        ach = fl / (kw * 1e-3) / 1e12 if bound == "fp64" else by / (kw * 1e-3) / 1e9
+        peak = FP64_PEAK_TFLOPS if bound == "fp64" else HBM_PEAK_GBS
+        roofs["imf_weights_" + name] = {
+            "bound": bound, "kernel": "k_op_imf + k_op_slow + k_op_weights", "achieved": ach, "peak": peak,
+            "unit": "TFLOP/s" if bound == "fp64" else "GB/s", "frac": ach / peak, "ms": kw,
+            "fp64_tflops": fl / (kw * 1e-3) / 1e12, "weights_write_gbs": by / (kw * 1e-3) / 1e9,
+            "algorithmic_per_launch": {"fp64_flops": fl, "weight_bytes": by, "model_points": pts}}
+    dom = max(("template_product", "imf_weights_mixed"), key=lambda k: roofs.get(k, {}).get("ms", 0.0))
+    roof = dict(roofs[dom], phase=dom)
     result = {"workload": f"IMF per model: {n} models, 107 ages x 22 Z x 1500 points, 10000 templates, 5994 px, "
                           "Kroupa slopes U(0.3,3.3), upper U(60,120), parametric SFH, dust, IGM, z, LSF, 100 filters",
               "value": res["mixed"]["models_per_s"], "unit": "models/s", "mixed": res["mixed"], "fp64": res["fp64"],
               "speedup_vs_fp64": res["mixed"]["models_per_s"] / res["fp64"]["models_per_s"],
               "accuracy_vs_fp64_gpu": {"max_spec_rel_err": float(rel.max()), "max_mag_err": float(dmag.max()),
                                        "mass_rel_err": float(((outs["mixed"]["mass"] / outs["fp64"]["mass"]) - 1).abs().max())},
-              "roofline": roof, "setup_s": setup}
+              "roofline": roof, "rooflines": roofs, "setup_s": setup}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -168,7 +193,75 @@ def operator_workload(args, n=4096):
                                             f"{dt:.1f} s",
                                   "mixed_vs_oracle_max_mag_err": float(np.max(np.abs(hm[okr] - r["mags"][okr])))}
     ctx.close()
+    result["adjoint"] = operator_adjoint(args, grid, op, fset, prm, (cz, cdl), n)
     return result
+
+
+def operator_adjoint(args, grid, op, fset, prm, cosmo, n):
+    """The IMF-per-model workload's reverse pass: the same 4096 models and operator, the chi^2 of the 100-band
+    magnitudes against a held-out seed; params_bar with the IMF rows (imf1..imf3) through the operator's reverse
+    (the transposed template product, then the points' weights' derivatives). Timed through libopensps_hip in both
+    precisions."""
+    import numpy as np
+    import torch
+    from libopensps_amd import hip as H, synth
+
+    bands, _ = product_tables(grid, fset)
+    ctx = H.DeviceContext(0)
+    ctx.set_grid(grid)
+    ctx.set_bands(bands)
+    ctx.set_cosmology(*cosmo)
+    ctx.set_options(1, 1)
+    ctx.set_ssp_ops(op)
+    ctx.set_imf(2, 1)
+    d_prm = torch.as_tensor(prm, device="cuda")
+    f64 = ctx.batch(d_prm, precision=H.FP64, use_ops=1, want=("mags",))
+    loss, dmag = synth.chi2_cotangent(f64["mags"].cpu().numpy(), seed=31, detect_range=30.0)
+    d_mb = torch.as_tensor(dmag, device="cuda")
+    lib = H.lib()
+    lib.opensps_hip_set_profiling.argtypes = [H.C.c_void_p, H.C.c_int]
+    lib.opensps_hip_grad_kernel_times.argtypes = [H.C.c_void_p, H.C.c_void_p, H.C.c_void_p, H.C.c_int]
+    res, outs = {}, {}
+    for prec, name in ((H.MIXED, "mixed"), (H.FP64, "fp64")):
+        out = ctx.alloc_outputs(n, prec, want=("mags",))
+        grads = {"params_bar": torch.zeros((n, H.NBGRAD), dtype=torch.float64, device="cuda")}
+        lib.opensps_hip_set_profiling(ctx._ctx, 1)
+        ctx.launch_grad(d_prm, out, {"mags": d_mb}, grads, prec, use_ops=1)
+        ctx.sync()
+        lib.opensps_hip_grad_kernel_times(ctx._ctx, None, None, 1)
+        steps = max(2, args.steps // 4)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            ctx.launch_grad(d_prm, out, {"mags": d_mb}, grads, prec, use_ops=1)
+            ctx.sync()
+        dt = time.perf_counter() - t1
+        gms, gcnt = (H.C.c_double * 3)(), (H.C.c_int * 3)()
+        lib.opensps_hip_grad_kernel_times(ctx._ctx, gms, gcnt, 1)
+        gk = [gms[i] / max(gcnt[i], 1) for i in range(3)]
+        res[name] = {"models_per_s": n * steps / dt, "ms_per_step": 1e3 * dt / steps,
+                     "kernel_ms": dict(zip(["rev_chain", "rev_template_product", "rev_operator_out"],
+                                           [round(x, 4) for x in gk]))}
+        outs[name] = grads["params_bar"].cpu().numpy()
+    lib.opensps_hip_set_profiling(ctx._ctx, 0)
+    from libopensps_amd import api
+    row_of = {api.grad_param_name(i): i for i in range(api.nbgrad())}
+    acc = {}
+    for k in ("imf1", "imf2", "imf3", "logzsol", "dust2", "zred"):
+        a, b = outs["mixed"][:, row_of[k]], outs["fp64"][:, row_of[k]]
+        acc[k] = float(np.max(np.abs(a - b)) / (np.max(np.abs(b)) + 1e-300))
+    # the transposed template product: 2 ntpl nspec flops per (model, part) column, both parts
+    flops = 2.0 * op.ntpl * grid.nspec * 2 * n
+    tk = res["mixed"]["kernel_ms"]["rev_template_product"]
+    roof = {"bound": "mfma", "kernel": "k_mm_bf16x3 (w_bar = templates^T y_bar)",
+            "achieved": flops / (tk * 1e-3) / 1e12, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": flops / (tk * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, "algorithmic_per_launch": flops}
+    ctx.close()
+    return {"workload": f"IMF per model, reverse: {n} models, the operator's 10000 templates, chi^2 of 100 filters; "
+                        "forward + reverse, params_bar with imf1..imf3",
+            "value": res["mixed"]["models_per_s"], "unit": "models/s (forward + gradient)", "mixed": res["mixed"],
+            "fp64": res["fp64"], "speedup_vs_fp64": res["mixed"]["models_per_s"] / res["fp64"]["models_per_s"],
+            "accuracy_vs_fp64_gpu": acc, "loss": loss, "roofline": roof}
 
 
 def weight_matrix_workload(args, n=256):
@@ -243,6 +336,20 @@ def weight_matrix_workload(args, n=256):
         res.pop(k)
     ctx.close()
     return out
+
+
+def op_kernel_times(ctx, reset=1):
+    """Per-batch device milliseconds of the SSP operator's two phases [IMF weights, ages' sums + segments + fold +
+    mass sums] (opensps_context_op_kernel_times)."""
+    import ctypes as C
+    from libopensps_amd import api
+    ms = (C.c_double * 2)()
+    cnt = (C.c_int * 2)()
+    st = C.c_int()
+    api.lib().opensps_context_op_kernel_times.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                                          C.POINTER(C.c_int)]
+    api.lib().opensps_context_op_kernel_times(ctx.handle, ms, cnt, reset, C.byref(st))
+    return [ms[i] / max(cnt[i], 1) for i in range(2)]
 
 
 def kernel_times(ctx, reset=1):
@@ -347,16 +454,28 @@ def adjoint_workload(args, n=8192, sfh=False):
     kfull, nsp = grid.nt * grid.nz, ((grid.nspec + 63) // 64) * 64
     flops = 2.0 * kfull * grid.nspec * 2 * n
     tk = res["mixed"]["kernel_ms"]["rev_product"]
-    roof = {"bound": "mfma", "kernel": "k_mm_bf16x3 (w_bar = basis^T y_bar)", "achieved": flops / (tk * 1e-3) / 1e12,
-            "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": flops / (tk * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
-            "algorithmic_per_launch": flops}
+    roofs = {"rev_product": {
+        "bound": "mfma", "kernel": "k_mm_bf16x3 (w_bar = basis^T y_bar)", "achieved": flops / (tk * 1e-3) / 1e12,
+        "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": flops / (tk * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
+        "algorithmic_per_launch": flops, "ms": tk}}
+    # the reverse chain (k_grad_chain<float>): per model it reads the forward's two parts' age sums (FP32, nspec
+    # each), the magnitudes, band fluxes and their cotangents (3 nbands doubles) and the parameters (33 doubles), and
+    # writes the two parts' cotangents y_bar as split BF16 (2 x 2 bytes a pixel) and 16 per-model partials
+    nb = bands.shape[0]
+    tc = res["mixed"]["kernel_ms"]["rev_chain"]
+    byts = n * (2 * grid.nspec * 4 + 2 * grid.nspec * 4 + 3 * nb * 8 + 33 * 8 + 16 * 8)
+    roofs["rev_chain"] = {"bound": "hbm", "kernel": "k_grad_chain<float>", "achieved": byts / (tc * 1e-3) / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": byts / (tc * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "algorithmic_per_launch": byts, "ms": tc}
+    dom = max(roofs, key=lambda k: roofs[k]["ms"])
+    roof = dict(roofs[dom], phase=dom)
     what = ("SFH parameters (delayed tau; Simha with truncation and slope), logzsol blend over 22 Z" if sfh else
             "weights over 107 ages x 22 Z (log-normal SFH x enrichment history)")
     result = {"workload": f"batch adjoint: {n} models, {what}, 5994 px, dust, IGM, z U(0,4), LSF, chi^2 of 100 "
                           "filters; forward + reverse",
               "value": res["mixed"]["models_per_s"], "unit": "models/s (forward + gradient)", "mixed": res["mixed"],
               "fp64": res["fp64"], "speedup_vs_fp64": res["mixed"]["models_per_s"] / res["fp64"]["models_per_s"],
-              "accuracy_vs_fp64_gpu": acc, "loss": loss, "roofline": roof, "setup_s": setup}
+              "accuracy_vs_fp64_gpu": acc, "loss": loss, "roofline": roof, "rooflines": roofs, "setup_s": setup}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O

@@ -263,6 +263,10 @@ void opensps_context_compute_csp_batch_device(int handle, int n, const double *d
  * group, age-sum product, chain], from HIP events on the context's stream once set_int("profile_kernels", 1);
  * reset != 0 clears them. status 5 before the first device batch. */
 void opensps_context_kernel_times(int handle, double *ms, int *launches, int reset, int *status);
+/* No reference counterpart: with profile_kernels on, the SSP operator's two phases of each batch whose operator
+ * launches take one round, [the IMF weights (k_op_imf, k_op_slow, k_op_weights), the ages' sums, segments, fold and
+ * mass sums], accumulated ms and batch counts; reset != 0 clears them. status 5 before the first device batch. */
+void opensps_context_op_kernel_times(int handle, double *ms, int *launches, int reset, int *status);
 /* No reference counterpart: the context's parameter values in the slot order of opensps_params.h. */
 void opensps_context_get_params(int handle, double *prm, int *status);
 

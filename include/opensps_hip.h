@@ -198,6 +198,10 @@ int opensps_hip_set_imf(opensps_hip_ctx_t *ctx, int imf_type, int remnants, doub
  * operator, "group" also holds the IMF weights, the fold and the mass sums, and the product is the template one. */
 int opensps_hip_set_profiling(opensps_hip_ctx_t *ctx, int on);
 int opensps_hip_kernel_times(opensps_hip_ctx_t *ctx, double *ms, int *launches, int reset);
+/* No reference counterpart (profiling): with profiling on, the SSP operator's phases of each batch whose operator
+ * launches take one round: [the IMF weights (k_op_imf, k_op_slow, k_op_weights), the fold (k_op_agesums,
+ * k_op_segments, k_op_fold, k_op_sums)], accumulated milliseconds and batch counts. */
+int opensps_hip_op_kernel_times(opensps_hip_ctx_t *ctx, double *ms, int *launches, int reset);
 /* The same for the reverse pass's kernels: [reverse chain, transposed product, weights and params_bar with the
  * SFH tangents] (ms[3],
  * launches[3]); the forward half of a batch with gradients counts in opensps_hip_kernel_times. */

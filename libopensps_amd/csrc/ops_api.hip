@@ -194,6 +194,8 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     for (int k = 0; k <= OpsCtx::NGPROF; k++)
         if (c->gev[k]) (void)hipEventDestroy(c->gev[k]);
+    for (int k = 0; k <= OpsCtx::NOPROF; k++)
+        if (c->opev[k]) (void)hipEventDestroy(c->opev[k]);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -1160,6 +1162,8 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
             k_fill_i32<<<(unsigned)((2 * (size_t)n + 255) / 256), 256, 0, c->stream>>>(a->op_count, 2 * n, 0);
             HIPCHK(c, hipGetLastError());
         }
+        const bool optime = c->profiling && chunk >= maxtiles;   // (one round: its phases are timed)
+        if (optime) HIPCHK(c, hipEventRecord(c->opev[0], c->stream));
         k_op_imf<<<(n + 255) / 256, 256, 0, c->stream>>>(w);
         HIPCHK(c, hipGetLastError());
         OpGArgs g;
@@ -1213,6 +1217,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
                 k_op_slow<double><<<gw, 256, 0, c->stream>>>(w);
                 k_op_weights<double><<<gwg, 256, 0, c->stream>>>(w);
             }
+            if (optime) HIPCHK(c, hipEventRecord(c->opev[1], c->stream));
             k_op_agesums<<<gw, 256, 0, c->stream>>>(w);
             if (mixed) {
                 k_op_segments<float><<<gg, 256, 0, c->stream>>>(g);
@@ -1236,6 +1241,10 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
         sa.hdr = (double*)c->hdr.p;
         k_op_sums<<<(n + 3) / 4, 256, 0, c->stream>>>(sa);
         HIPCHK(c, hipGetLastError());
+        if (optime) {
+            HIPCHK(c, hipEventRecord(c->opev[2], c->stream));
+            c->opev_pending = true;
+        }
         k_op_tiles<<<1, 256, 0, c->stream>>>(n * parts, OPS_TILE_COL, d_optiles, (int4*)c->tiles.p + maxtiles);
         HIPCHK(c, hipGetLastError());
     }
@@ -1466,7 +1475,9 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     if (ops) {
         if ((st = ops_reserve(c, c->opsraw, sizeof(double) * 2 * (size_t)npass * n * nt))) return st;
         if ((st = ops_reserve(c, c->opct, tsz * 2 * (size_t)c->op.ntpl * n))) return st;
-        if ((st = ops_reserve(c, c->oppart, sizeof(double) * 4 * (size_t)npass * nt * n))) return st;
+        // the (pass, age, position) sums, then k_op_adj_points's per block of 4 point groups
+        const size_t ngb = (size_t)((c->op.npa_max + OPS_OP_GPTS - 1) / OPS_OP_GPTS + 3) / 4;
+        if ((st = ops_reserve(c, c->oppart, sizeof(double) * 4 * (size_t)npass * nt * n * (1 + ngb)))) return st;
         if ((st = ops_reserve(c, c->opimfd, sizeof(double) * OPS_IMFD_N * (size_t)n))) return st;
     }
     if ((st = ops_reserve(c, c->glsum, sizeof(double) * (size_t)n))) return st;
@@ -1635,6 +1646,8 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
         q.sraw = (const double*)c->opsraw.p;
         q.CT = c->opct.p;
         q.part = (double*)c->oppart.p;
+        q.gpart = q.part + 4 * (size_t)npass * nt * n;
+        q.ngroup = (c->op.npa_max + OPS_OP_GPTS - 1) / OPS_OP_GPTS;
         q.params_bar = g->params_bar;
         q.weights_bar = wbar_out;
         const dim3 gt3((unsigned)((c->op.ntpl + 63) / 64), (unsigned)((n + 63) / 64), 2);
@@ -1648,11 +1661,15 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
         k_op_imf_grad<<<(n + 255) / 256, 256, 0, c->stream>>>(q);
         HIPCHK(c, hipGetLastError());
         const int maxtiles = (n + OPS_TILE_COL - 1) / OPS_TILE_COL + c->npair;
-        const dim3 gp((unsigned)((nt + 3) / 4), (unsigned)maxtiles, (unsigned)npass);
-        if (mixed)
+        const dim3 gp((unsigned)(nt * ((q.ngroup + 3) / 4)), (unsigned)maxtiles, (unsigned)npass);
+        const dim3 gs((unsigned)((nt + 3) / 4), (unsigned)maxtiles, (unsigned)npass);
+        if (mixed) {
             k_op_adj_points<float><<<gp, 256, 0, c->stream>>>(q);
-        else
+            k_op_adj_slow<float><<<gs, 256, 0, c->stream>>>(q);
+        } else {
             k_op_adj_points<double><<<gp, 256, 0, c->stream>>>(q);
+            k_op_adj_slow<double><<<gs, 256, 0, c->stream>>>(q);
+        }
         HIPCHK(c, hipGetLastError());
         k_op_adj_out<<<(n + 3) / 4, 256, 0, c->stream>>>(q);
         HIPCHK(c, hipGetLastError());
@@ -1731,6 +1748,15 @@ int opensps_hip_sync(opensps_hip_ctx_t* c) {
         }
         c->ev_pending = false;
     }
+    if (c->opev_pending) {
+        for (int k = 0; k < OpsCtx::NOPROF; k++) {
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->opev[k], c->opev[k + 1]));
+            c->opprof_ms[k] += ms;
+            c->opprof_n[k] += 1;
+        }
+        c->opev_pending = false;
+    }
     if (c->gev_pending) {
         for (int k = 0; k < OpsCtx::NGPROF; k++) {
             float ms = 0.f;
@@ -1745,7 +1771,8 @@ int opensps_hip_sync(opensps_hip_ctx_t* c) {
     HIPCHK(c, hipMemcpy(&flag, c->d_last_flag, sizeof(int), hipMemcpyDeviceToHost));
     c->d_last_flag = nullptr;
     if (flag == 2) return ops_fail(c, 2, "batch: a model needs an option this build does not compute (SFH type, "
-                                         "tage <= 0 without weights, or smoothing other than the fast velocity operator)");
+                                         "tage <= 0 without weights, smoothing other than the fast velocity operator, "
+                                         "or dust emission / the AGN torus without their templates)");
     if (flag == 4) return ops_fail(c, 3, "batch: split_parts = 0 with a birth-cloud or old-star screen");
     return 0;
 }
@@ -1757,9 +1784,25 @@ int opensps_hip_set_profiling(opensps_hip_ctx_t* c, int on) {
         for (int k = 0; k <= OpsCtx::NPROF; k++) HIPCHK(c, hipEventCreate(&c->ev[k]));
     if (on && !c->gev[0])
         for (int k = 0; k <= OpsCtx::NGPROF; k++) HIPCHK(c, hipEventCreate(&c->gev[k]));
+    if (on && !c->opev[0])
+        for (int k = 0; k <= OpsCtx::NOPROF; k++) HIPCHK(c, hipEventCreate(&c->opev[k]));
+    c->opev_pending = false;
     c->profiling = on != 0;
     c->ev_pending = false;
     c->gev_pending = false;
+    return 0;
+}
+
+int opensps_hip_op_kernel_times(opensps_hip_ctx_t* c, double* ms, int* launches, int reset) {
+    if (!c) return 1;
+    for (int k = 0; k < OpsCtx::NOPROF; k++) {
+        if (ms) ms[k] = c->opprof_ms[k];
+        if (launches) launches[k] = (int)c->opprof_n[k];
+        if (reset) {
+            c->opprof_ms[k] = 0.0;
+            c->opprof_n[k] = 0;
+        }
+    }
     return 0;
 }
 

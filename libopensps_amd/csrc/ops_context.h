@@ -169,6 +169,13 @@ struct OpsCtx {
     bool gev_pending = false;
     double gprof_ms[NGPROF] = {0, 0, 0};
     long gprof_n[NGPROF] = {0, 0, 0};
+    // the operator's phases: [the IMF weights (k_op_imf, k_op_slow, k_op_weights), the fold (k_op_agesums,
+    // k_op_segments, k_op_fold, k_op_sums)], timed when the operator's launches take one round
+    static const int NOPROF = 2;
+    hipEvent_t opev[NOPROF + 1] = {};
+    bool opev_pending = false;
+    double opprof_ms[NOPROF] = {0, 0};
+    long opprof_n[NOPROF] = {0, 0};
 
     std::string last_error;
 };

@@ -217,7 +217,7 @@ struct OpSArgs {           // k_op_sums
     double* hdr;           // the L_bol sum into the chain's header
 };
 
-// The operator's reverse pass (k_op_imf_grad, k_op_adj_t, k_op_adj_points, k_op_adj_out)
+// The operator's reverse pass (k_op_imf_grad, k_op_adj_t, k_op_adj_points, k_op_adj_slow, k_op_adj_out)
 #define OPS_IMFD_N 9       // per model: d ln(c_seg / total) / d direction, segment-major (3 segments x 3 directions)
 struct OpAdjArgs {
     int n, nt, nz, npass, ntpl, nent, imf_type, remnants;
@@ -237,6 +237,8 @@ struct OpAdjArgs {
     const double *smass_p, *slum_p, *sraw;
     const void* CT;             // (part, template, position): the template cotangents, kernel precision
     double* part;               // (pass, age, position) x 4: the fold's sum and the three IMF sums (scaled)
+    double* gpart;              // (pass, age, block of 4 point groups, quantity, position): k_op_adj_points's sums
+    int ngroup;                 // point groups of OPS_OP_GPTS per age
     double *params_bar, *weights_bar;
 };
 
@@ -378,5 +380,7 @@ template <typename T>
 __global__ void k_op_adj_t(int n, int ntpl, int ldc, const int* order, const T* C, T* CT);
 template <typename T>
 __global__ void k_op_adj_points(OpAdjArgs a);
+template <typename T>
+__global__ void k_op_adj_slow(OpAdjArgs a);
 __global__ void k_op_adj_out(OpAdjArgs a);
 __global__ void k_op_tiles(int ncols, int width, int* ntiles, int4* tiles);

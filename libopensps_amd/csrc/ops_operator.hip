@@ -762,50 +762,78 @@ __device__ __forceinline__ double d_hfun(double y, const double* hc) {
     return (exp(y) * (y - 1.0) + 1.0) / (y * y);
 }
 
-// k_op_adj_points: grid (ceil(nt / 4), tiles, passes), a wavefront per (age, column tile, pass) as k_op_slow. The
-// common bins as k_op_weights forms them (records through scalar loads, the chain of shared edges), each with its
-// derivative along e: dw/de = c / total lo^e dl (ln lo sx(e dl) + dl h(e dl)), sx = expm1(y) / y; along a direction
-// dw = w d ln(c / total) - [the segment's slope is the direction] dw/de. The bins off the common path and the
-// remnants by the forward's functions on D3.
+// A lane's model in the reverse pass's point kernels: its position in the tile, the blend factor of the pass, the
+// template cotangents' row of its part, and the factors of the points' cotangent ge = x0 gk + A cm + B cl (the
+// fold through the SFH weight x0, the mass and the log-blended L_bol sums)
 template <typename T>
-__global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
-    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int t = blockIdx.x * 4 + wv, tile = blockIdx.y, pass = blockIdx.z;
-    if (t >= a.nt || tile >= *a.ntiles) return;
-    const int4 td = a.tiles[tile];
-    const int iz = a.npass == 1 ? 0 : __builtin_amdgcn_readfirstlane(td.x) + pass;
-    const bool valid = lane < td.z;
-    const int pos = td.y + (valid ? lane : 0);
-    const int m = a.order[pos];
-    const int n = a.n, nt = a.nt, nent = a.nent, ty = a.imf_type % 10;
+struct AdjLane {
+    int m, pos;
+    bool valid;
+    double cz, x0, A, B, lower, upper;
+    const T* ct;
+};
+
+template <typename T>
+__device__ __forceinline__ AdjLane<T> adj_lane(const OpAdjArgs& a, int4 td, int lane, int t, int pass) {
+    AdjLane<T> L;
+    const int n = a.n, nt = a.nt;
+    L.valid = lane < td.z;
+    L.pos = td.y + (L.valid ? lane : 0);
+    L.m = a.order[L.pos];
+    const int m = L.m;
     const double dz = a.hdr[(size_t)m * OPS_HDR_N + OPS_HDR_DZ];
-    const double cz = a.npass == 1 ? 1.0 : (pass == 0 ? 1.0 - dz : dz);
+    L.cz = a.npass == 1 ? 1.0 : (pass == 0 ? 1.0 - dz : dz);
     const int part = t + 1 <= a.it[m] ? 0 : 1;
-    const T* ct = (const T*)a.CT + (size_t)part * a.ntpl * n + pos;
+    L.ct = (const T*)a.CT + (size_t)part * a.ntpl * n + L.pos;
     const double* go = a.gout + (size_t)m * OPS_GOUT_N;
     const double cs = go[OPS_GOUT_C];
     const double mbs = a.mass_bar ? a.mass_bar[m] / cs : 0.0, lsbs = go[OPS_GOUT_LSB];
     const double ws = a.wsfh[(size_t)m * nt + t];
-    const double x0 = (ws > OPS_SAFE_FLOOR ? ws : 0.0) * cz;
+    L.x0 = (ws > OPS_SAFE_FLOOR ? ws : 0.0) * L.cz;
     double slog = 0.0;
     for (int q = 0; q < a.npass; q++) slog += a.slum_p[((size_t)q * n + m) * nt + t];
     const double sl = a.sraw[(((size_t)pass * n + m) * nt + t) * 2 + 1];
-    const double A = mbs * x0, B = sl > OPS_SAFE_FLOOR ? lsbs * x0 * pow(10.0, slog) / sl : 0.0;
-    const double lower = a.prm[(size_t)OPS_P_IMF_LOWER * n + m], upper = a.prm[(size_t)OPS_P_IMF_UPPER * n + m];
-    const double* ic = a.impc + (size_t)m * OPS_IMFC_N;
+    L.A = mbs * L.x0;
+    L.B = sl > OPS_SAFE_FLOOR ? lsbs * L.x0 * pow(10.0, slog) / sl : 0.0;
+    L.lower = a.prm[(size_t)OPS_P_IMF_LOWER * n + m];
+    L.upper = a.prm[(size_t)OPS_P_IMF_UPPER * n + m];
+    return L;
+}
+
+// k_op_adj_points: grid (nt x ceil(ngroup / 4), tiles, passes), 256 threads = 4 point groups of one age (OPS_OP_GPTS
+// points each, whole pieces as k_op_weights takes them, so each group starts its chain of shared edges afresh); the
+// lanes are a column tile's models. The common bins as k_op_weights forms them (records through scalar loads, the
+// next point's record and entries fetched ahead of this point's arithmetic, the point's template cotangents
+// gathered before it), each with its derivative along e: dw/de = c / total lo^e dl (ln lo sx(e dl) + dl h(e dl)),
+// sx = expm1(y) / y; along a direction dw = w d ln(c / total) - [the segment's slope is the direction] dw/de. The
+// four groups' sums are added in order through LDS and written per (pass, age, block of groups, quantity, position).
+#define OPS_ADJ_NENT 4   // entries per point fetched ahead (points with more take the rest in place)
+template <typename T>
+__global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
+    __shared__ double red[4][4][64];
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ngb = (a.ngroup + 3) / 4;
+    const int t = blockIdx.x / ngb, gb = blockIdx.x - t * ngb, grp = gb * 4 + wv;
+    const int tile = blockIdx.y, pass = blockIdx.z;
+    if (t >= a.nt || tile >= *a.ntiles) return;   // (uniform over the workgroup)
+    const int4 td = a.tiles[tile];
+    const int iz = a.npass == 1 ? 0 : __builtin_amdgcn_readfirstlane(td.x) + pass;
+    const int nt = a.nt, n = a.n, nent = a.nent, ty = a.imf_type % 10;
+    const AdjLane<T> L = adj_lane<T>(a, td, lane, t, pass);
+    const double* ic = a.impc + (size_t)L.m * OPS_IMFC_N;
     const double e1 = ic[OPS_IMFC_E], e2 = ic[OPS_IMFC_E + 1], e3 = ic[OPS_IMFC_E + 2];
     const double c1 = ic[OPS_IMFC_CT], c2 = ic[OPS_IMFC_CT + 1], c3 = ic[OPS_IMFC_CT + 2];
     double dc[OPS_IMFD_N];
 #pragma unroll
-    for (int q = 0; q < OPS_IMFD_N; q++) dc[q] = a.impd[(size_t)m * OPS_IMFD_N + q];
+    for (int q = 0; q < OPS_IMFD_N; q++) dc[q] = a.impd[(size_t)L.m * OPS_IMFD_N + q];
     // the direction of each segment's slope (Kroupa: segment s has alpha_s; Dave: alpha_1, alpha_2; Salpeter: none)
     const int sd1 = (ty == 2 || ty == 4) ? 0 : -1, sd2 = (ty == 2 || ty == 4) ? 1 : -1, sd3 = ty == 2 ? 2 : -1;
     const int ka = __builtin_amdgcn_readfirstlane(a.age_ptr[iz * nt + t]);
     const int kb = __builtin_amdgcn_readfirstlane(a.age_ptr[iz * nt + t + 1]);
-    typedef __attribute__((address_space(4))) const OpsWPoint* WPC;
+    const int k0 = min(kb, ka + grp * OPS_OP_GPTS), k1 = min(kb, k0 + OPS_OP_GPTS);
     typedef __attribute__((address_space(4))) const double* DC;
     typedef __attribute__((address_space(4))) const int* IC;
-    const WPC wp = (WPC)a.wp;
+    const OpsWPointC wp = (OpsWPointC)a.wp;
     const DC pcs = (DC)a.poly, hcs = (DC)a.hpoly, pv = (DC)a.pval;
     const IC pe = (IC)a.pent;
     double pc[13], hc[13];
@@ -814,17 +842,39 @@ __global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
         pc[j] = pcs[j];
         hc[j] = hcs[j];
     }
+    const int nf = min(nent, OPS_ADJ_NENT);
     double F = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, phi = 1.0;
-    for (int k = ka; k < kb; k++) {
-        const OpsWPoint r = wp_load(wp + k);
-        const int seg = r.code & 3;
-        if (seg == 0) continue;   // off the common path: below
-        // the template cotangents of the point's entries (a contiguous run of the tile's models each)
-        double gk = 0.0;
-        for (int j = 0; j < nent; j++) {
-            const int e = pe[(size_t)k * nent + j];
-            if (e >= 0) gk = fma(pv[(size_t)k * nent + j], (double)ct[(size_t)e * n], gk);
+    OpsWPoint nxt;
+    int ne[OPS_ADJ_NENT];
+    double nv[OPS_ADJ_NENT];
+    if (k0 < k1) {
+        nxt = wp_load(wp + k0);
+#pragma unroll
+        for (int j = 0; j < OPS_ADJ_NENT; j++) {
+            ne[j] = j < nf ? pe[(size_t)k0 * nent + j] : -1;
+            nv[j] = j < nf ? pv[(size_t)k0 * nent + j] : 0.0;
         }
+    }
+    for (int k = k0; k < k1; k++) {
+        const OpsWPoint r = nxt;
+        // the point's template cotangents (a contiguous run of the tile's models each), gathered first
+        T cv[OPS_ADJ_NENT];
+        double cw[OPS_ADJ_NENT];
+#pragma unroll
+        for (int j = 0; j < OPS_ADJ_NENT; j++) {
+            cv[j] = ne[j] >= 0 ? L.ct[(size_t)ne[j] * n] : (T)0;
+            cw[j] = ne[j] >= 0 ? nv[j] : 0.0;
+        }
+        if (k + 1 < k1) {
+            nxt = wp_load(wp + k + 1);
+#pragma unroll
+            for (int j = 0; j < OPS_ADJ_NENT; j++) {
+                ne[j] = j < nf ? pe[(size_t)(k + 1) * nent + j] : -1;
+                nv[j] = j < nf ? pv[(size_t)(k + 1) * nent + j] : 0.0;
+            }
+        }
+        const int seg = r.code & 3;
+        if (seg == 0) continue;   // off the common path: k_op_adj_slow
         const double e = seg == 1 ? e1 : (seg == 2 ? e2 : e3);
         const double c = seg == 1 ? c1 : (seg == 2 ? c2 : c3);
         const int sd = seg == 1 ? sd1 : (seg == 2 ? sd2 : sd3);
@@ -834,20 +884,66 @@ __global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
         double sx = d_expm1_over_y(y, pc);
         if (!(fabs(y) < 0.25)) sx = (exp(y) - 1.0) / y;
         phi = fma(plo, y * sx, plo);
-        if (r.mini < lower || r.mini > upper) continue;
+        if (r.mini < L.lower || r.mini > L.upper) continue;
         const double w = c * dl * plo * sx;
         const double dwde = c * plo * dl * fma(dl, d_hfun(y, hc), r.llo * sx);
-        const double ge = x0 * gk + A * r.cm + B * r.cl;
-        const double* d = dc + 3 * (seg - 1);
+        double gk = 0.0;
+#pragma unroll
+        for (int j = 0; j < OPS_ADJ_NENT; j++) gk = fma(cw[j], (double)cv[j], gk);
+        for (int j = OPS_ADJ_NENT; j < nent; j++) {
+            const int ej = pe[(size_t)k * nent + j];
+            if (ej >= 0) gk = fma(pv[(size_t)k * nent + j], (double)L.ct[(size_t)ej * n], gk);
+        }
+        const double ge = L.x0 * gk + L.A * r.cm + L.B * r.cl;
+        // the segment's d ln(c / total) (selected by the uniform segment, the table stays in registers)
+        const double d0 = seg == 1 ? dc[0] : (seg == 2 ? dc[3] : dc[6]);
+        const double d1 = seg == 1 ? dc[1] : (seg == 2 ? dc[4] : dc[7]);
+        const double d2 = seg == 1 ? dc[2] : (seg == 2 ? dc[5] : dc[8]);
         F = fma(w, gk, F);
-        g0 = fma(ge, w * d[0] - (sd == 0 ? dwde : 0.0), g0);
-        g1 = fma(ge, w * d[1] - (sd == 1 ? dwde : 0.0), g1);
-        g2 = fma(ge, w * d[2] - (sd == 2 ? dwde : 0.0), g2);
+        g0 = fma(ge, w * d0 - (sd == 0 ? dwde : 0.0), g0);
+        g1 = fma(ge, w * d1 - (sd == 1 ? dwde : 0.0), g1);
+        g2 = fma(ge, w * d2 - (sd == 2 ? dwde : 0.0), g2);
     }
+    red[wv][0][lane] = F;
+    red[wv][1][lane] = g0;
+    red[wv][2][lane] = g1;
+    red[wv][3][lane] = g2;
+    __syncthreads();
+    if (wv == 0 && L.valid) {
+        double* o = a.gpart + (((size_t)pass * nt + t) * ngb + gb) * 4 * n + L.pos;
+#pragma unroll
+        for (int q = 0; q < 4; q++) o[(size_t)q * n] = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) +
+                                                       red[3][q][lane];
+    }
+}
+
+// k_op_adj_slow: grid (ceil(nt / 4), tiles, passes), a wavefront per (age, column tile, pass) as k_op_slow: the
+// bins off the common path and the remnants by the forward's functions on D3, added to k_op_adj_points's blocks
+// of groups in order; the (pass, age, position) sums for k_op_adj_out (zero for a pass of zero weight, as the
+// forward's grouping leaves it out).
+template <typename T>
+__global__ __launch_bounds__(256) void k_op_adj_slow(OpAdjArgs a) {
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int t = blockIdx.x * 4 + wv, tile = blockIdx.y, pass = blockIdx.z;
+    if (t >= a.nt || tile >= *a.ntiles) return;
+    const int4 td = a.tiles[tile];
+    const int iz = a.npass == 1 ? 0 : __builtin_amdgcn_readfirstlane(td.x) + pass;
+    const int n = a.n, nt = a.nt, nent = a.nent;
+    const int ngb = (a.ngroup + 3) / 4;
+    const AdjLane<T> L = adj_lane<T>(a, td, lane, t, pass);
+    double F = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    const double* gp = a.gpart + ((size_t)pass * nt + t) * ngb * 4 * n + L.pos;
+    for (int b = 0; b < ngb; b++) {
+        F = F + gp[((size_t)b * 4 + 0) * n];
+        g0 = g0 + gp[((size_t)b * 4 + 1) * n];
+        g1 = g1 + gp[((size_t)b * 4 + 2) * n];
+        g2 = g2 + gp[((size_t)b * 4 + 3) * n];
+    }
+    const int ka = a.age_ptr[iz * nt + t], kb = a.age_ptr[iz * nt + t + 1];
     const int j0 = a.slow_ptr[iz * nt + t], j1 = a.slow_ptr[iz * nt + t + 1];
     const bool rem = a.remnants && kb > ka;
     if (j1 > j0 || rem) {
-        const ImfPT<D3> p = model_imf_d3(a, m);
+        const ImfPT<D3> p = model_imf_d3(a, L.m);
         const D3 tot = d_imf_integral<D3>(a.imf_type, p, p.lower, p.upper, true);
         const D3 inv = tot.v > 0.0 ? D3(1.0) / tot : D3(1.0);
         for (int j = j0; j < j1; j++) {
@@ -859,9 +955,9 @@ __global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
             double gk = 0.0;
             for (int jj = 0; jj < nent; jj++) {
                 const int e = a.pent[(size_t)k * nent + jj];
-                if (e >= 0) gk = fma(a.pval[(size_t)k * nent + jj], (double)ct[(size_t)e * n], gk);
+                if (e >= 0) gk = fma(a.pval[(size_t)k * nent + jj], (double)L.ct[(size_t)e * n], gk);
             }
-            const double ge = x0 * gk + A * q[5] + B * q[6];
+            const double ge = L.x0 * gk + L.A * q[5] + L.B * q[6];
             F = fma(w.v, gk, F);
             g0 = fma(ge, w.d[0], g0);
             g1 = fma(ge, w.d[1], g1);
@@ -869,16 +965,16 @@ __global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
         }
         if (rem) {
             const D3 r = d_remnant_mass<D3>(a.imf_type, p, a.age_mmax[iz * nt + t]);
-            g0 = fma(A, r.d[0], g0);
-            g1 = fma(A, r.d[1], g1);
-            g2 = fma(A, r.d[2], g2);
+            g0 = fma(L.A, r.d[0], g0);
+            g1 = fma(L.A, r.d[1], g1);
+            g2 = fma(L.A, r.d[2], g2);
         }
     }
-    if (!valid) return;
-    if (cz == 0.0) {   // a pass of zero weight is left out, as the forward's grouping leaves it
+    if (!L.valid) return;
+    if (L.cz == 0.0) {
         F = 0.0; g0 = 0.0; g1 = 0.0; g2 = 0.0;
     }
-    double* o = a.part + (((size_t)pass * nt + t) * n + pos) * 4;
+    double* o = a.part + (((size_t)pass * nt + t) * n + L.pos) * 4;
     o[0] = F;
     o[1] = g0;
     o[2] = g1;
@@ -952,6 +1048,8 @@ template __global__ void k_op_adj_t<float>(int, int, int, const int*, const floa
 template __global__ void k_op_adj_t<double>(int, int, int, const int*, const double*, double*);
 template __global__ void k_op_adj_points<float>(OpAdjArgs);
 template __global__ void k_op_adj_points<double>(OpAdjArgs);
+template __global__ void k_op_adj_slow<float>(OpAdjArgs);
+template __global__ void k_op_adj_slow<double>(OpAdjArgs);
 template __global__ void k_op_weights<float>(OpWArgs);
 template __global__ void k_op_weights<double>(OpWArgs);
 template __global__ void k_op_slow<float>(OpWArgs);

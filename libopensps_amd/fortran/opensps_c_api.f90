@@ -698,6 +698,27 @@ contains
         status = rc
     end subroutine
 
+    !> No reference counterpart: with profile_kernels on, the SSP operator's phases of each batch whose operator
+    !> launches take one round, [the IMF weights, the fold and mass sums], in accumulated milliseconds and batch
+    !> counts; reset != 0 clears them. status 1 bad handle, 5 before any device batch.
+    subroutine opensps_context_op_kernel_times(handle, ms, launches, reset, status) &
+        bind(C, name="opensps_context_op_kernel_times")
+        use opensps_hip_iface, only: opensps_hip_op_kernel_times
+        integer(c_int), value :: handle, reset
+        real(c_double), intent(out) :: ms(2)
+        integer(c_int), intent(out) :: launches(2), status
+        integer(c_int) :: rc
+        ms = 0.0_wp
+        launches = 0
+        if (bad_handle(handle, status)) return
+        if (.not. c_associated(ctxs(handle)%dctx)) then
+            status = 5
+            return
+        end if
+        rc = opensps_hip_op_kernel_times(ctxs(handle)%dctx, ms, launches, reset)
+        status = rc
+    end subroutine
+
     subroutine opensps_context_batch_spec_device(handle, ptr, nspec, n, bytes, status) &
         bind(C, name="opensps_context_batch_spec_device")
         integer(c_int), value :: handle

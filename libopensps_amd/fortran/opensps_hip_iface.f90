@@ -37,6 +37,7 @@ module opensps_hip_iface
               opensps_hip_set_mag_mask, opensps_hip_set_vega, opensps_hip_set_sfh_tab, opensps_hip_set_dust_curve, opensps_hip_set_cosmology, opensps_hip_set_options, opensps_hip_csp_batch, &
               opensps_hip_sync, opensps_hip_malloc, opensps_hip_free, opensps_hip_memcpy_h2d, opensps_hip_memcpy_d2h, &
               opensps_hip_last_error, opensps_hip_device_count, opensps_hip_set_profiling, opensps_hip_kernel_times, &
+              opensps_hip_op_kernel_times, &
               opensps_hip_csp_batch_grad, opensps_hip_set_emission, opensps_hip_set_dust_emission, opensps_hip_set_agn
 
     interface
@@ -186,6 +187,15 @@ module opensps_hip_iface
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: ms(4)
             integer(c_int), intent(out) :: launches(4)
+            integer(c_int), value :: reset
+        end function
+
+        integer(c_int) function opensps_hip_op_kernel_times(ctx, ms, launches, reset) &
+            bind(C, name="opensps_hip_op_kernel_times")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: ms(2)
+            integer(c_int), intent(out) :: launches(2)
             integer(c_int), value :: reset
         end function
 
