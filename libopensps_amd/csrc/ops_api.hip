@@ -528,7 +528,7 @@ static void free_operator(OpsCtx* c) {
     void* ps[] = {c->op.d_age_ptr, c->op.d_zoff, c->op.d_first, c->op.d_tz_ptr, c->op.d_pt, c->op.d_age_mmax,
                   c->op.d_te, c->op.d_te_val64, c->op.d_te_val32, c->op.d_A64, c->op.d_Ahi, c->op.d_Alo,
                   c->op.d_seg, c->op.d_segoff, c->op.d_tseg, c->op.d_slow_ptr, c->op.d_slow_pt, c->op.d_wp,
-                  c->op.d_poly, c->op.d_pent, c->op.d_pval, c->op.d_hpoly, c->op.d_At64, c->op.d_Ath, c->op.d_Atl};
+                  c->op.d_poly, c->op.d_pent, c->op.d_pval, c->op.d_At64, c->op.d_Ath, c->op.d_Atl};
     for (void* p : ps) free_ptr(p);
     c->op = OpsCtx::OpsOperator();
 }
@@ -824,14 +824,6 @@ int opensps_hip_set_ssp_ops(opensps_hip_ctx_t* c, const opensps_hip_ssp_ops_t* o
     }
     UPLOAD(c, c->op.d_pent, o->ent_tpl, (size_t)np * ne);
     UPLOAD(c, c->op.d_pval, o->ent_val, (size_t)np * ne);
-    {
-        double hp[13], f = 1.0;
-        for (int j = 0; j < 13; j++) {
-            if (j > 0) f *= (double)j;
-            hp[j] = 1.0 / (f * (j + 2));
-        }
-        UPLOAD(c, c->op.d_hpoly, hp, 13);
-    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipFree(lib));
     c->op.h_age_ptr.assign(o->age_ptr, o->age_ptr + (size_t)nz * nt + 1);
@@ -1638,7 +1630,6 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
         q.age_mmax = c->op.d_age_mmax;
         q.wp = c->op.d_wp;
         q.poly = c->op.d_poly;
-        q.hpoly = c->op.d_hpoly;
         q.pent = c->op.d_pent;
         q.pval = c->op.d_pval;
         q.smass_p = (const double*)c->opsum.p;

@@ -83,7 +83,6 @@ struct OpsCtx {
         int nent = 0, nrt = 0;
         int* d_pent = nullptr;
         double* d_pval = nullptr;
-        double* d_hpoly = nullptr;                         // 1 / (j! (j + 2)), j = 0 .. 12
         double* d_At64 = nullptr;
         uint16_t *d_Ath = nullptr, *d_Atl = nullptr;
     } op;

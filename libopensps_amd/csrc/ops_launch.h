@@ -231,7 +231,7 @@ struct OpAdjArgs {
     const int *age_ptr, *first, *slow_ptr, *slow_pt;
     const double *pt, *age_mmax;
     const OpsWPoint* wp;
-    const double *poly, *hpoly; // 1 / (j + 1)! and 1 / (j! (j + 2)), j = 0 .. 12
+    const double* poly;         // 1 / (j + 1)!, j = 0 .. 12
     const int* pent;            // per point nent template indices (-1: none)
     const double* pval;         // per point nent values
     const double *smass_p, *slum_p, *sraw;

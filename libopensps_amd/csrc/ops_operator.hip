@@ -428,6 +428,10 @@ __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
     for (int j = 0; j < 13; j++) pc[j] = pcs[j];
     double sm = 0.0, sl = 0.0, phi = 1.0;
     int cnt = 0;
+    // the segment's e and c, selected when the (uniform) segment changes: the points are sorted by mass, so it
+    // changes at most at the IMF's breaks
+    int cur = 0;
+    double e = 0.0, c = 0.0;
     // one point a step; the next point's record is fetched (scalar loads) before this point's arithmetic, so its
     // latency hides behind it
     OpsWPoint nxt;
@@ -437,8 +441,11 @@ __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
         if (k + 1 < k1) nxt = wp_load(wp + k + 1);
         const int seg = r.code & 3;
         if (seg == 0) continue;   // k_op_slow's bin
-        const double e = seg == 1 ? e1 : (seg == 2 ? e2 : e3);
-        const double c = seg == 1 ? c1 : (seg == 2 ? c2 : c3);
+        if (seg != cur) {
+            cur = seg;
+            e = seg == 1 ? e1 : (seg == 2 ? e2 : e3);
+            c = seg == 1 ? c1 : (seg == 2 ? c2 : c3);
+        }
         const double dl = r.dl;
         const double plo = (r.code & 4) ? phi : exp(e * r.llo);
         const double y = e * dl;
@@ -750,16 +757,23 @@ __global__ __launch_bounds__(256) void k_op_adj_t(int n, int ntpl, int ldc, cons
     }
 }
 
-// h(y) = (e^y (y - 1) + 1) / y^2 = int_0^1 s e^(y s) ds: d/dy of expm1(y) / y. Its series sum_j y^j / (j! (j + 2))
-// below |y| = 1/4 (Horner, 12 FMAs), the closed form above.
-__device__ __forceinline__ double d_hfun(double y, const double* hc) {
-    if (fabs(y) < 0.25) {
-        double s = hc[12];
+// sx(y) = expm1(y) / y and h(y) = sx'(y) = (e^y (y - 1) + 1) / y^2 = int_0^1 s e^(y s) ds. Below |y| = 1/4 both from
+// the series of d_expm1_over_y (the same FMAs for sx) by Horner's scheme with its derivative alongside (h's terms
+// (j + 1) / (j + 2)! = 1 / (j! (j + 2))); above it the closed forms.
+__device__ __forceinline__ void d_sx_h(double y, const double* pc, double& sx, double& h) {
+    double p = pc[12], dp = 0.0;
 #pragma unroll
-        for (int j = 11; j >= 0; j--) s = fma(s, y, hc[j]);
-        return s;
+    for (int j = 11; j >= 0; j--) {
+        dp = fma(dp, y, p);
+        p = fma(p, y, pc[j]);
     }
-    return (exp(y) * (y - 1.0) + 1.0) / (y * y);
+    sx = p;
+    h = dp;
+    if (!(fabs(y) < 0.25)) {
+        const double ey = exp(y);
+        sx = (ey - 1.0) / y;
+        h = (ey * (y - 1.0) + 1.0) / (y * y);
+    }
 }
 
 // A lane's model in the reverse pass's point kernels: its position in the tile, the blend factor of the pass, the
@@ -807,7 +821,7 @@ __device__ __forceinline__ AdjLane<T> adj_lane(const OpAdjArgs& a, int4 td, int 
 // gathered before it), each with its derivative along e: dw/de = c / total lo^e dl (ln lo sx(e dl) + dl h(e dl)),
 // sx = expm1(y) / y; along a direction dw = w d ln(c / total) - [the segment's slope is the direction] dw/de. The
 // four groups' sums are added in order through LDS and written per (pass, age, block of groups, quantity, position).
-#define OPS_ADJ_NENT 4   // entries per point fetched ahead (points with more take the rest in place)
+#define OPS_ADJ_NENT 2   // entries per point fetched ahead (points with more take the rest in place)
 template <typename T>
 __global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
     __shared__ double red[4][4][64];
@@ -820,12 +834,10 @@ __global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
     const int iz = a.npass == 1 ? 0 : __builtin_amdgcn_readfirstlane(td.x) + pass;
     const int nt = a.nt, n = a.n, nent = a.nent, ty = a.imf_type % 10;
     const AdjLane<T> L = adj_lane<T>(a, td, lane, t, pass);
+    // the model's per-segment e, c and d ln(c / total), read when a run of one segment starts or ends (rarely: not
+    // held in registers through the loop)
     const double* ic = a.impc + (size_t)L.m * OPS_IMFC_N;
-    const double e1 = ic[OPS_IMFC_E], e2 = ic[OPS_IMFC_E + 1], e3 = ic[OPS_IMFC_E + 2];
-    const double c1 = ic[OPS_IMFC_CT], c2 = ic[OPS_IMFC_CT + 1], c3 = ic[OPS_IMFC_CT + 2];
-    double dc[OPS_IMFD_N];
-#pragma unroll
-    for (int q = 0; q < OPS_IMFD_N; q++) dc[q] = a.impd[(size_t)L.m * OPS_IMFD_N + q];
+    const double* dc = a.impd + (size_t)L.m * OPS_IMFD_N;
     // the direction of each segment's slope (Kroupa: segment s has alpha_s; Dave: alpha_1, alpha_2; Salpeter: none)
     const int sd1 = (ty == 2 || ty == 4) ? 0 : -1, sd2 = (ty == 2 || ty == 4) ? 1 : -1, sd3 = ty == 2 ? 2 : -1;
     const int ka = __builtin_amdgcn_readfirstlane(a.age_ptr[iz * nt + t]);
@@ -834,16 +846,28 @@ __global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
     typedef __attribute__((address_space(4))) const double* DC;
     typedef __attribute__((address_space(4))) const int* IC;
     const OpsWPointC wp = (OpsWPointC)a.wp;
-    const DC pcs = (DC)a.poly, hcs = (DC)a.hpoly, pv = (DC)a.pval;
+    const DC pcs = (DC)a.poly, pv = (DC)a.pval;
     const IC pe = (IC)a.pent;
-    double pc[13], hc[13];
+    double pc[13];
 #pragma unroll
-    for (int j = 0; j < 13; j++) {
-        pc[j] = pcs[j];
-        hc[j] = hcs[j];
-    }
+    for (int j = 0; j < 13; j++) pc[j] = pcs[j];
     const int nf = min(nent, OPS_ADJ_NENT);
     double F = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, phi = 1.0;
+    // per run of one segment (uniform; the points are sorted by mass, so a run ends at most at the IMF's breaks):
+    // its e and c, and S = sum ge w, D = sum ge dw/de; at the run's end g_i += d ln(c / total)_i S - [the segment's
+    // slope is direction i] D
+    int cur = 0;
+    double e = 0.0, c = 0.0, S = 0.0, D = 0.0;
+#define OPS_ADJ_FLUSH                                                                                                  \
+    if (cur != 0) {                                                                                                    \
+        const double d0 = dc[3 * (cur - 1)], d1 = dc[3 * (cur - 1) + 1], d2 = dc[3 * (cur - 1) + 2];                   \
+        const int sd = cur == 1 ? sd1 : (cur == 2 ? sd2 : sd3);                                                        \
+        g0 = fma(d0, S, g0) - (sd == 0 ? D : 0.0);                                                                     \
+        g1 = fma(d1, S, g1) - (sd == 1 ? D : 0.0);                                                                     \
+        g2 = fma(d2, S, g2) - (sd == 2 ? D : 0.0);                                                                     \
+        S = 0.0;                                                                                                       \
+        D = 0.0;                                                                                                       \
+    }
     OpsWPoint nxt;
     int ne[OPS_ADJ_NENT];
     double nv[OPS_ADJ_NENT];
@@ -875,18 +899,21 @@ __global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
         }
         const int seg = r.code & 3;
         if (seg == 0) continue;   // off the common path: k_op_adj_slow
-        const double e = seg == 1 ? e1 : (seg == 2 ? e2 : e3);
-        const double c = seg == 1 ? c1 : (seg == 2 ? c2 : c3);
-        const int sd = seg == 1 ? sd1 : (seg == 2 ? sd2 : sd3);
+        if (seg != cur) {
+            OPS_ADJ_FLUSH
+            cur = seg;
+            e = ic[OPS_IMFC_E + seg - 1];
+            c = ic[OPS_IMFC_CT + seg - 1];
+        }
         const double dl = r.dl;
         const double plo = (r.code & 4) ? phi : exp(e * r.llo);
         const double y = e * dl;
-        double sx = d_expm1_over_y(y, pc);
-        if (!(fabs(y) < 0.25)) sx = (exp(y) - 1.0) / y;
+        double sx, h;
+        d_sx_h(y, pc, sx, h);
         phi = fma(plo, y * sx, plo);
         if (r.mini < L.lower || r.mini > L.upper) continue;
         const double w = c * dl * plo * sx;
-        const double dwde = c * plo * dl * fma(dl, d_hfun(y, hc), r.llo * sx);
+        const double dwde = c * plo * dl * fma(dl, h, r.llo * sx);
         double gk = 0.0;
 #pragma unroll
         for (int j = 0; j < OPS_ADJ_NENT; j++) gk = fma(cw[j], (double)cv[j], gk);
@@ -895,15 +922,12 @@ __global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
             if (ej >= 0) gk = fma(pv[(size_t)k * nent + j], (double)L.ct[(size_t)ej * n], gk);
         }
         const double ge = L.x0 * gk + L.A * r.cm + L.B * r.cl;
-        // the segment's d ln(c / total) (selected by the uniform segment, the table stays in registers)
-        const double d0 = seg == 1 ? dc[0] : (seg == 2 ? dc[3] : dc[6]);
-        const double d1 = seg == 1 ? dc[1] : (seg == 2 ? dc[4] : dc[7]);
-        const double d2 = seg == 1 ? dc[2] : (seg == 2 ? dc[5] : dc[8]);
         F = fma(w, gk, F);
-        g0 = fma(ge, w * d0 - (sd == 0 ? dwde : 0.0), g0);
-        g1 = fma(ge, w * d1 - (sd == 1 ? dwde : 0.0), g1);
-        g2 = fma(ge, w * d2 - (sd == 2 ? dwde : 0.0), g2);
+        S = fma(ge, w, S);
+        D = fma(ge, dwde, D);
     }
+    OPS_ADJ_FLUSH
+#undef OPS_ADJ_FLUSH
     red[wv][0][lane] = F;
     red[wv][1][lane] = g0;
     red[wv][2][lane] = g1;
