@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
     double sm = 0.0, sl = 0.0, phi = 1.0;
     int cnt = 0;
     // the segment's e and c, selected when the (uniform) segment changes: the points are sorted by mass, so it
-    // changes at most at the IMF's breaks
+    // changes at most at the IMF's breaks (a branch the wavefront takes rarely, not selects on every point)
     int cur = 0;
     double e = 0.0, c = 0.0;
     // one point a step; the next point's record is fetched (scalar loads) before this point's arithmetic, so its
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
         if (k + 1 < k1) nxt = wp_load(wp + k + 1);
         const int seg = r.code & 3;
         if (seg == 0) continue;   // k_op_slow's bin
-        if (seg != cur) {
+        if (__builtin_expect(seg != cur, 0)) {
             cur = seg;
             e = seg == 1 ? e1 : (seg == 2 ? e2 : e3);
             c = seg == 1 ? c1 : (seg == 2 ? c2 : c3);
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(256) void k_op_weights(OpWArgs a) {
         if (!(fabs(y) < 0.25)) sx = (exp(y) - 1.0) / y;
         phi = fma(plo, y * sx, plo);
         const double w = (r.mini < lower || r.mini > upper) ? 0.0 : c * dl * plo * sx;
-        u[(size_t)(k - zo) * 64] = (T)w;
+        __builtin_nontemporal_store((T)w, u + (size_t)(k - zo) * 64);   // streamed: read back by the segments
         cnt += w > 0.0 ? 1 : 0;
         sm = fma(r.cm, w, sm);
         sl = fma(r.cl, w, sl);
@@ -834,10 +834,13 @@ __global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
     const int iz = a.npass == 1 ? 0 : __builtin_amdgcn_readfirstlane(td.x) + pass;
     const int nt = a.nt, n = a.n, nent = a.nent, ty = a.imf_type % 10;
     const AdjLane<T> L = adj_lane<T>(a, td, lane, t, pass);
-    // the model's per-segment e, c and d ln(c / total), read when a run of one segment starts or ends (rarely: not
-    // held in registers through the loop)
+    // the model's per-segment e, c and d ln(c / total)
     const double* ic = a.impc + (size_t)L.m * OPS_IMFC_N;
-    const double* dc = a.impd + (size_t)L.m * OPS_IMFD_N;
+    const double e1 = ic[OPS_IMFC_E], e2 = ic[OPS_IMFC_E + 1], e3 = ic[OPS_IMFC_E + 2];
+    const double c1 = ic[OPS_IMFC_CT], c2 = ic[OPS_IMFC_CT + 1], c3 = ic[OPS_IMFC_CT + 2];
+    double dc[OPS_IMFD_N];
+#pragma unroll
+    for (int q = 0; q < OPS_IMFD_N; q++) dc[q] = a.impd[(size_t)L.m * OPS_IMFD_N + q];
     // the direction of each segment's slope (Kroupa: segment s has alpha_s; Dave: alpha_1, alpha_2; Salpeter: none)
     const int sd1 = (ty == 2 || ty == 4) ? 0 : -1, sd2 = (ty == 2 || ty == 4) ? 1 : -1, sd3 = ty == 2 ? 2 : -1;
     const int ka = __builtin_amdgcn_readfirstlane(a.age_ptr[iz * nt + t]);
@@ -860,7 +863,9 @@ __global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
     double e = 0.0, c = 0.0, S = 0.0, D = 0.0;
 #define OPS_ADJ_FLUSH                                                                                                  \
     if (cur != 0) {                                                                                                    \
-        const double d0 = dc[3 * (cur - 1)], d1 = dc[3 * (cur - 1) + 1], d2 = dc[3 * (cur - 1) + 2];                   \
+        const double d0 = cur == 1 ? dc[0] : (cur == 2 ? dc[3] : dc[6]);                                               \
+        const double d1 = cur == 1 ? dc[1] : (cur == 2 ? dc[4] : dc[7]);                                               \
+        const double d2 = cur == 1 ? dc[2] : (cur == 2 ? dc[5] : dc[8]);                                               \
         const int sd = cur == 1 ? sd1 : (cur == 2 ? sd2 : sd3);                                                        \
         g0 = fma(d0, S, g0) - (sd == 0 ? D : 0.0);                                                                     \
         g1 = fma(d1, S, g1) - (sd == 1 ? D : 0.0);                                                                     \
@@ -881,12 +886,13 @@ __global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
     }
     for (int k = k0; k < k1; k++) {
         const OpsWPoint r = nxt;
-        // the point's template cotangents (a contiguous run of the tile's models each), gathered first
+        // the point's template cotangents (a contiguous run of the tile's models each), gathered first and without a
+        // branch (an absent entry reads template 0's row with the value 0), so their waits fall at their use
         T cv[OPS_ADJ_NENT];
         double cw[OPS_ADJ_NENT];
 #pragma unroll
         for (int j = 0; j < OPS_ADJ_NENT; j++) {
-            cv[j] = ne[j] >= 0 ? L.ct[(size_t)ne[j] * n] : (T)0;
+            cv[j] = L.ct[(size_t)max(ne[j], 0) * n];
             cw[j] = ne[j] >= 0 ? nv[j] : 0.0;
         }
         if (k + 1 < k1) {
@@ -899,11 +905,11 @@ __global__ __launch_bounds__(256) void k_op_adj_points(OpAdjArgs a) {
         }
         const int seg = r.code & 3;
         if (seg == 0) continue;   // off the common path: k_op_adj_slow
-        if (seg != cur) {
+        if (__builtin_expect(seg != cur, 0)) {
             OPS_ADJ_FLUSH
             cur = seg;
-            e = ic[OPS_IMFC_E + seg - 1];
-            c = ic[OPS_IMFC_CT + seg - 1];
+            e = seg == 1 ? e1 : (seg == 2 ? e2 : e3);
+            c = seg == 1 ? c1 : (seg == 2 ? c2 : c3);
         }
         const double dl = r.dl;
         const double plo = (r.code & 4) ? phi : exp(e * r.llo);
