@@ -5,8 +5,10 @@
     emu = Emulator.train(dctx, cosmo, ...)       # targets [PCA coefficients, magnitudes], Adam with cosine decay
     out = emu.predict(params)                    # the network, with the batch as fallback and judge
 
-Inputs: the per-model parameters the parametric workload varies (tage, tau, logzsol, dust2, dust_index, zred,
-sigma_smooth), uniform over its ranges. The log-spectrum is log10(max(spec, 1e-20 x the model's peak)) on the grid;
+Inputs (an InputSpace): the per-model parameters the parametric workload varies (tage, tau, logzsol, dust2,
+dust_index, zred, sigma_smooth), uniform over its ranges (BASE); or those and the three IMF slopes (the batch on the
+SSP operator) and the dust emission and AGN parameters (duste_umin, duste_qpah, duste_gamma, fagn; the context holds
+the templates) (FULL, 14 inputs). The log-spectrum is log10(max(spec, 1e-20 x the model's peak)) on the grid;
 the magnitudes are clipped at the model's brightest band + 30 mag (the faintest bands at high redshift, below the
 Lyman limit, carry no information and NaN for masked or empty bands becomes that clip). The PCA fit (a covariance
 and its eigenvectors) and the projections run through torch on the GPU: set-up of the training targets, not the
@@ -24,23 +26,82 @@ from . import synth
 from .emulator import MLP
 from .params import NPAR, SLOT
 
-INPUTS = ["tage", "tau", "logzsol", "dust2", "dust_index", "zred", "sigma_smooth"]
-BOX = {"tage": (0.1, 13.7), "tau": (0.1, 10.0), "logzsol": (-2.0, 0.3), "dust2": (0.0, 2.0),
-       "dust_index": (-1.2, -0.4), "zred": (0.0, 4.0), "sigma_smooth": (50.0, 300.0)}
 MAG_CLIP = 30.0
 SPEC_FLOOR = 1e-20
+# parameters that act logarithmically: their features are log10 of them
+LOG_INPUTS = ("tage", "tau", "duste_umin")
+
+
+class InputSpace:
+    """The emulator's inputs: parameter names (slots of opensps_params.h), their box (uniform sampling and the
+    serving path's range), whether the batch runs on the SSP operator (per-model IMF slopes), and the features the
+    network takes: the inputs (log10 of LOG_INPUTS) and log10(1 + z) after z (the distance acts logarithmically)."""
+
+    def __init__(self, names, box, use_ops=False):
+        self.names = list(names)
+        self.box = dict(box)
+        self.use_ops = bool(use_ops)
+        self.iz = self.names.index("zred")
+        self.n_feat = len(self.names) + 1
+
+    def sample(self, n, seed):
+        """(n, k) inputs uniform in the box."""
+        rng = np.random.Generator(np.random.PCG64(seed))
+        return np.stack([rng.uniform(*self.box[k], n) for k in self.names], axis=1)
+
+    def bounds(self):
+        return (np.array([self.box[k][0] for k in self.names]), np.array([self.box[k][1] for k in self.names]))
+
+    def features(self, x):
+        x = np.asarray(x, dtype=np.float64)
+        cols = []
+        for i, k in enumerate(self.names):
+            cols.append(np.log10(x[:, i]) if k in LOG_INPUTS else x[:, i])
+            if i == self.iz:
+                cols.append(np.log10(1.0 + x[:, i]))
+        return np.stack(cols, axis=1).astype(np.float32)
+
+    def to_prm(self, x, base=None):
+        """(n, k) inputs -> (NPAR, n) parameter slots (delayed-tau SFH; the others at the context's defaults)."""
+        n = x.shape[0]
+        p = synth.default_params(n) if base is None else np.repeat(base[:, None], n, axis=1).copy()
+        p[SLOT["sfh"]] = 4.0
+        for i, k in enumerate(self.names):
+            p[SLOT[k]] = x[:, i]
+        return p
+
+    def to_prm_device(self, x):
+        """The same on the device: x (n, k) tensor -> (NPAR, n) float64 tensor."""
+        import torch
+        n = x.shape[0]
+        base = torch.as_tensor(np.asarray(synth.default_params(1))[:, 0], device=x.device)
+        p = base[:, None].repeat(1, n).contiguous()
+        p[SLOT["sfh"]] = 4.0
+        for i, k in enumerate(self.names):
+            p[SLOT[k]] = x[:, i].double()
+        return p.contiguous()
+
+
+BASE = InputSpace(["tage", "tau", "logzsol", "dust2", "dust_index", "zred", "sigma_smooth"],
+                  {"tage": (0.1, 13.7), "tau": (0.1, 10.0), "logzsol": (-2.0, 0.3), "dust2": (0.0, 2.0),
+                   "dust_index": (-1.2, -0.4), "zred": (0.0, 4.0), "sigma_smooth": (50.0, 300.0)})
+# the full input set: BASE, the Kroupa slopes in the operator workload's range and the dust emission / AGN parameters
+# over the Draine & Li tables' q_PAH and U_min, a fraction gamma of dust in the power-law U distribution up to 0.5, and
+# an AGN of up to half the stars' bolometric luminosity
+FULL = InputSpace(BASE.names + ["imf1", "imf2", "imf3", "duste_umin", "duste_qpah", "duste_gamma", "fagn"],
+                  dict(BASE.box, imf1=(0.3, 3.3), imf2=(0.3, 3.3), imf3=(0.3, 3.3), duste_umin=(0.1, 25.0),
+                       duste_qpah=(0.47, 4.58), duste_gamma=(0.0, 0.5), fagn=(0.0, 0.5)), use_ops=True)
+
+# the base space's forms, as module-level names
+INPUTS = BASE.names
+BOX = BASE.box
+N_FEAT = BASE.n_feat
 
 
 def features(x):
-    """The network's inputs from the parameters (n, 7): log10 tage, log10 tau, logzsol, dust2, dust_index, z,
-    log10(1 + z), sigma_smooth (the ages and the distance act logarithmically)."""
-    x = np.asarray(x, dtype=np.float64)
-    z = x[:, 5]
-    return np.stack([np.log10(x[:, 0]), np.log10(x[:, 1]), x[:, 2], x[:, 3], x[:, 4], z, np.log10(1.0 + z), x[:, 6]],
-                    axis=1).astype(np.float32)
-
-
-N_FEAT = 8
+    """The network's inputs from BASE's parameters (n, 7): log10 tage, log10 tau, logzsol, dust2, dust_index, z,
+    log10(1 + z), sigma_smooth."""
+    return BASE.features(x)
 
 
 def distance_term(z, cosmo):
@@ -58,32 +119,17 @@ def distance_term(z, cosmo):
     return out
 
 
-def sample_inputs(n, seed):
-    """(n, 7) inputs uniform in the box (the parametric workload's ranges)."""
-    rng = np.random.Generator(np.random.PCG64(seed))
-    return np.stack([rng.uniform(*BOX[k], n) for k in INPUTS], axis=1)
+def sample_inputs(n, seed, space=BASE):
+    """(n, k) inputs uniform in the space's box."""
+    return space.sample(n, seed)
 
 
-def to_prm(x, base=None):
-    """(n, 7) inputs -> (NPAR, n) parameter slots (delayed-tau SFH; the others at the context's defaults)."""
-    n = x.shape[0]
-    p = synth.default_params(n) if base is None else np.repeat(base[:, None], n, axis=1).copy()
-    p[SLOT["sfh"]] = 4.0
-    for i, k in enumerate(INPUTS):
-        p[SLOT[k]] = x[:, i]
-    return p
+def to_prm(x, base=None, space=BASE):
+    return space.to_prm(x, base)
 
 
-def to_prm_device(x):
-    """The same on the device: x (n, 7) float tensor -> (NPAR, n) float64 tensor."""
-    import torch
-    n = x.shape[0]
-    base = torch.as_tensor(np.asarray(synth.default_params(1))[:, 0], device=x.device)
-    p = base[:, None].repeat(1, n).contiguous()
-    p[SLOT["sfh"]] = 4.0
-    for i, k in enumerate(INPUTS):
-        p[SLOT[k]] = x[:, i].double()
-    return p.contiguous()
+def to_prm_device(x, space=BASE):
+    return space.to_prm_device(x)
 
 
 def log_spectra(spec):
@@ -102,8 +148,8 @@ def clip_mags(mags):
     return torch.where(fin, torch.minimum(mags, lim), lim)
 
 
-def generate(dctx, x, precision=H.MIXED, batch=16384, pca=None, cosmo=None):
-    """Runs the batch on the inputs x (n, 7) (host) in chunks. Returns device tensors: the log-spectra (n, nspec)
+def generate(dctx, x, precision=H.MIXED, batch=16384, pca=None, cosmo=None, space=BASE):
+    """Runs the batch on the inputs x (n, k) (host) in chunks. Returns device tensors: the log-spectra (n, nspec)
     when pca is None, else their PCA coefficients (n, k), and the clipped magnitudes (n, nbands) less the distance
     term when cosmo is given, float32."""
     import torch
@@ -111,13 +157,13 @@ def generate(dctx, x, precision=H.MIXED, batch=16384, pca=None, cosmo=None):
     outs_s, outs_m = [], []
     for i0 in range(0, n, batch):
         xb = torch.as_tensor(x[i0:i0 + batch], device="cuda")
-        prm = to_prm_device(xb)
-        o = dctx.batch(prm, precision=precision, want=("mags", "spec"))
+        prm = space.to_prm_device(xb)
+        o = dctx.batch(prm, precision=precision, want=("mags", "spec"), use_ops=int(space.use_ops))
         ls = log_spectra(o["spec"].float())
         outs_s.append(ls if pca is None else pca.project(ls))
         m = clip_mags(o["mags"])
         if cosmo is not None:
-            m = m - torch.as_tensor(distance_term(x[i0:i0 + batch, 5], cosmo), device="cuda")[:, None]
+            m = m - torch.as_tensor(distance_term(x[i0:i0 + batch, space.iz], cosmo), device="cuda")[:, None]
         outs_m.append(m.float())
     return torch.cat(outs_s), torch.cat(outs_m)
 
@@ -148,31 +194,34 @@ class Emulator:
     """A trained emulator with the batch as fallback: inputs outside the training box, and batches whose
     spot-check (a random subset recomputed by the batch) has a median magnitude error above tol, go to the batch."""
 
-    def __init__(self, dctx, mlp, pca, n_pca, nbands, box, cosmo, precision=H.MIXED):
+    def __init__(self, dctx, mlp, pca, n_pca, nbands, box, cosmo, precision=H.MIXED, space=BASE):
         self.dctx, self.mlp, self.pca, self.n_pca, self.nbands, self.box = dctx, mlp, pca, n_pca, nbands, box
         self.cosmo = cosmo
         self.precision = precision
+        self.space = space
         self.stats = {"models": 0, "fallback_models": 0, "spot_checked": 0, "batches_failed": 0}
 
     @staticmethod
     def train(dctx, cosmo, n_train=4_000_000, n_pca_fit=200_000, n_pca=128, epochs=100, batch=8192, lr=2e-3,
-              seed=101, hidden=512, layers=4, log=None, precision=H.MIXED, pca_weight=0.1):
+              seed=101, hidden=512, layers=4, log=None, precision=H.MIXED, pca_weight=0.1, space=BASE):
+        """Trains on n_train samples of the space (the context set up for it: the operator and the IMF when the space
+        takes IMF slopes, the emission templates and flags for the dust emission inputs)."""
         import torch
         t0 = time.perf_counter()
         nb = dctx.nbands
-        x_fit = sample_inputs(n_pca_fit, seed)
-        ls_fit, _ = generate(dctx, x_fit, precision)
+        x_fit = space.sample(n_pca_fit, seed)
+        ls_fit, _ = generate(dctx, x_fit, precision, space=space)
         pca = PCA(ls_fit, n_pca)
         del ls_fit
         t_pca = time.perf_counter() - t0
-        x_tr = sample_inputs(n_train, seed + 1)
-        c_tr, m_tr = generate(dctx, x_tr, precision, pca=pca, cosmo=cosmo)
+        x_tr = space.sample(n_train, seed + 1)
+        c_tr, m_tr = generate(dctx, x_tr, precision, pca=pca, cosmo=cosmo, space=space)
         t_data = time.perf_counter() - t0 - t_pca
-        xd = torch.as_tensor(features(x_tr), device="cuda")
+        xd = torch.as_tensor(space.features(x_tr), device="cuda")
         td = torch.cat([c_tr, m_tr], dim=1).contiguous()
         del c_tr, m_tr
         n_out = n_pca + nb
-        mlp = MLP(N_FEAT, n_out, hidden, layers, seed=seed)
+        mlp = MLP(space.n_feat, n_out, hidden, layers, seed=seed)
         in_mu, in_sd = xd.mean(dim=0), xd.std(dim=0)
         out_mu, out_sd = td.mean(dim=0), td.std(dim=0).clamp_min(1e-6)
         mlp.set_scaling(in_mu.cpu().numpy(), in_sd.cpu().numpy(), out_mu.cpu().numpy(), out_sd.cpu().numpy())
@@ -202,9 +251,8 @@ class Emulator:
                         log(f"epoch {ep + 1}/{epochs}: loss {l:.5f}")
         mlp.sync()
         t_train = time.perf_counter() - t1
-        box = (np.array([BOX[k][0] for k in INPUTS]), np.array([BOX[k][1] for k in INPUTS]))   # the sampled box
-        emu = Emulator(dctx, mlp, pca, n_pca, nb, box, cosmo, precision)
-        emu.train_info = {"n_train": n_train, "n_pca_fit": n_pca_fit, "pca_explained": pca.explained, "epochs": epochs,
+        emu = Emulator(dctx, mlp, pca, n_pca, nb, space.bounds(), cosmo, precision, space)   # the sampled box
+        emu.train_info = {"inputs": space.names, "n_feat": space.n_feat, "n_train": n_train, "n_pca_fit": n_pca_fit, "pca_explained": pca.explained, "epochs": epochs,
                           "batch": batch, "lr": lr, "steps": total, "loss_history": hist, "pca_s": t_pca,
                           "data_s": t_data, "train_s": t_train,
                           "train_samples_per_s": total * batch / t_train}
@@ -220,17 +268,17 @@ class Emulator:
         return y[:, :self.n_pca], m
 
     def inputs(self, x):
-        """Host parameters (n, 7) -> device features and distance terms."""
+        """Host parameters (n, k) -> device features and distance terms."""
         import torch
-        return (torch.as_tensor(features(x), device="cuda"),
-                torch.as_tensor(distance_term(x[:, 5], self.cosmo).astype(np.float32), device="cuda"))
+        return (torch.as_tensor(self.space.features(x), device="cuda"),
+                torch.as_tensor(distance_term(x[:, self.space.iz], self.cosmo).astype(np.float32), device="cuda"))
 
     def in_box(self, x):
         lo, hi = self.box
         return np.all((x >= lo) & (x <= hi), axis=1)
 
     def predict(self, x, spot=0.01, tol=0.01, seed=0):
-        """Magnitudes (n, nbands) of the inputs x (n, 7) (host): the emulator inside the box, the batch outside it
+        """Magnitudes (n, nbands) of the inputs x (n, k) (host): the emulator inside the box, the batch outside it
         and for the whole batch when the spot-check's median |dmag| exceeds tol. Returns (mags, info)."""
         import torch
         n = x.shape[0]
@@ -259,8 +307,8 @@ class Emulator:
 
     def _batch(self, x):
         import torch
-        prm = to_prm_device(torch.as_tensor(x, device="cuda"))
-        o = self.dctx.batch(prm, precision=self.precision, want=("mags",))
+        prm = self.space.to_prm_device(torch.as_tensor(x, device="cuda"))
+        o = self.dctx.batch(prm, precision=self.precision, want=("mags",), use_ops=int(self.space.use_ops))
         return clip_mags(o["mags"])
 
     def fallback_rate(self):
@@ -272,9 +320,11 @@ def validate(emu, dctx, n=16384, seed=999, mag_range=20.0):
     within mag_range of each model's brightest. Returns the median, 95th percentile and max, and the spectra's
     reconstruction error (rms dex of the log-spectrum over the pixels within 10 dex of the peak)."""
     import torch
-    x = sample_inputs(n, seed)
+    sp = emu.space
+    x = sp.sample(n, seed)
     c, m = emu.predict_raw(*emu.inputs(x))
-    o = dctx.batch(to_prm_device(torch.as_tensor(x, device="cuda")), precision=H.FP64, want=("mags", "spec"))
+    o = dctx.batch(sp.to_prm_device(torch.as_tensor(x, device="cuda")), precision=H.FP64, want=("mags", "spec"),
+                   use_ops=int(sp.use_ops))
     ref = o["mags"]
     fin = torch.isfinite(ref)
     bright = torch.where(fin, ref, torch.full_like(ref, float("inf"))).amin(dim=1, keepdim=True)
