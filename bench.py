@@ -28,6 +28,9 @@ FP64_PEAK_TFLOPS = 78.6
 # FP64 operations per (model, common isochrone point) of k_op_weights: the 12-FMA series of expm1(y) / y (24), y = e dl
 # (1), the chain's next power phi = plo + plo y sx (3), w = c dl plo sx (3), the mass and L_bol sums (4)
 OP_WEIGHT_FLOPS = 35
+# FP64 operations per (model, common point) of the operator's reverse (k_op_adj_points): the forward weight's 35, h's
+# series alongside (24), dw/de (5), and F, S, D with the cotangent ge (6 FMAs: 12)
+OP_ADJ_FLOPS = 76
 KERNELS = ["prep", "group", "age_sums", "chain"]
 MAG_RANGE = 40.0
 CHAIN_KERNEL = "k_chain<float, true, false>"
@@ -98,6 +101,7 @@ def operator_workload(args, n=4096):
     ctx.set_ssp_basis(grid)
     ctx.set_filters(fset)
     ctx.set_int("add_dust_emission", 0)
+    ctx.set_int("add_agn_dust", 0)
     ctx.set_int("add_igm_absorption", 1)
     ctx.set_int("profile_kernels", 1)
     ctx.set_int("imf_type", 2)
@@ -253,15 +257,30 @@ def operator_adjoint(args, grid, op, fset, prm, cosmo, n):
     # the transposed template product: 2 ntpl nspec flops per (model, part) column, both parts
     flops = 2.0 * op.ntpl * grid.nspec * 2 * n
     tk = res["mixed"]["kernel_ms"]["rev_template_product"]
-    roof = {"bound": "mfma", "kernel": "k_mm_bf16x3 (w_bar = templates^T y_bar)",
-            "achieved": flops / (tk * 1e-3) / 1e12, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": flops / (tk * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, "algorithmic_per_launch": flops}
+    roofs = {"rev_template_product": {
+        "bound": "mfma", "kernel": "k_mm_bf16x3 (w_bar = templates^T y_bar)", "achieved": flops / (tk * 1e-3) / 1e12,
+        "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": flops / (tk * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
+        "algorithmic_per_launch": flops, "ms": tk}}
+    # the operator's reverse (k_op_adj_t, k_op_imf_grad, k_op_adj_points, k_op_adj_slow, k_op_adj_out), dominated by
+    # k_op_adj_points: per (model, point of its two metallicities) the forward weight's OP_WEIGHT_FLOPS, h's series
+    # (24), dw/de (5) and the sums F, S, D with the cotangent ge (12): OP_ADJ_FLOPS FP64 operations; and the
+    # template cotangents gathered, two per point (nent) in the path's precision
+    to = res["mixed"]["kernel_ms"]["rev_operator_out"]
+    pts = 2.0 * op.npts / op.nz * n
+    fl, by = OP_ADJ_FLOPS * pts, 4.0 * op.nent * pts
+    roofs["rev_operator_out"] = {
+        "bound": "fp64", "kernel": "k_op_adj_points (+ k_op_adj_t, k_op_imf_grad, k_op_adj_slow, k_op_adj_out)",
+        "achieved": fl / (to * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": fl / (to * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "gather_gbs": by / (to * 1e-3) / 1e9,
+        "algorithmic_per_launch": {"fp64_flops": fl, "gather_bytes": by, "model_points": pts}, "ms": to}
+    dom = max(roofs, key=lambda k: roofs[k]["ms"])
+    roof = dict(roofs[dom], phase=dom)
     ctx.close()
     return {"workload": f"IMF per model, reverse: {n} models, the operator's 10000 templates, chi^2 of 100 filters; "
                         "forward + reverse, params_bar with imf1..imf3",
             "value": res["mixed"]["models_per_s"], "unit": "models/s (forward + gradient)", "mixed": res["mixed"],
             "fp64": res["fp64"], "speedup_vs_fp64": res["mixed"]["models_per_s"] / res["fp64"]["models_per_s"],
-            "accuracy_vs_fp64_gpu": acc, "loss": loss, "roofline": roof}
+            "accuracy_vs_fp64_gpu": acc, "loss": loss, "roofline": roof, "rooflines": roofs}
 
 
 def weight_matrix_workload(args, n=256):
@@ -281,6 +300,7 @@ def weight_matrix_workload(args, n=256):
     ctx.set_ssp_basis(grid)
     ctx.set_filters(fset)
     ctx.set_int("add_dust_emission", 0)
+    ctx.set_int("add_agn_dust", 0)
     ctx.set_int("add_igm_absorption", 0)
     ctx.set_int("profile_kernels", 1)
     d_prm = torch.as_tensor(synth.default_params(n), device="cuda")
@@ -610,6 +630,7 @@ def main():
     ctx.set_ssp_basis(grid)
     ctx.set_filters(fset)
     ctx.set_int("add_dust_emission", 0)
+    ctx.set_int("add_agn_dust", 0)
     ctx.set_int("add_igm_absorption", 1)
     ctx.set_int("profile_kernels", 1)
     nb, ns = ctx.nbands, grid.nspec
