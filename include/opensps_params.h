@@ -45,7 +45,9 @@
 #define OPS_P_DUSTE_GAMMA    30
 #define OPS_P_FAGN           31
 #define OPS_P_AGN_TAU        32
-#define OPS_NPAR             33
+#define OPS_P_GAS_LOGZ       33
+#define OPS_P_GAS_LOGU       34
+#define OPS_NPAR             35
 
 /* Default value of every slot: the PARAMS_CORE defaults (src/core/opensps_types.f90:33-45). */
 #define OPS_PARAM_DEFAULTS { \
@@ -57,12 +59,13 @@
     0.0,  /* sigma_smooth */  1.0e3,/* min_wave_smooth */ 1.0e4, /* max_wave_smooth */                \
     1.3,  /* imf1 */          2.3,  /* imf2 */         2.3, /* imf3 */       120.0,   /* imf_upper_limit */ \
     0.08, /* imf_lower_limit */ 1.0, /* duste_umin */  3.5, /* duste_qpah */ 0.01,   /* duste_gamma */ \
-    0.0,  /* fagn */          10.0  /* agn_tau */ }
+    0.0,  /* fagn */          10.0, /* agn_tau */      0.0, /* gas_logz */    -2.0    /* gas_logu */ }
 
 #define OPS_PARAM_NAMES { \
     "tage", "tau", "const", "fburst", "tburst", "sf_start", "sf_trunc", "sf_slope", "sfh", "logzsol", \
     "dust1", "dust2", "dust3", "dust_index", "dust1_index", "dust_tesc", "frac_obrun", "frac_nodust", \
     "zred", "igm_factor", "sigma_smooth", "min_wave_smooth", "max_wave_smooth", "imf1", "imf2", "imf3", \
-    "imf_upper_limit", "imf_lower_limit", "duste_umin", "duste_qpah", "duste_gamma", "fagn", "agn_tau" }
+    "imf_upper_limit", "imf_lower_limit", "duste_umin", "duste_qpah", "duste_gamma", "fagn", "agn_tau", \
+    "gas_logz", "gas_logu" }
 
 #endif

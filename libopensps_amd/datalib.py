@@ -168,3 +168,29 @@ def agn_torus(root: str, name: str = "clumpy", verify: bool = True):
     if s.shape != (tau.size, lam.size):
         raise DataLibraryError(f"{d}: spectra {s.shape} against the axes {(tau.size, lam.size)}")
     return tau, lam, s
+
+
+def nebular(root: str, name: str = "mist_nodust", verify: bool = True):
+    """nebular/<name>: (logz (nnz), age (nna, yr), logu (nnu), wavelength (nlam, A), continuum (nnz, nna, nnu, nlam)
+    L_sun/Hz per unit ionizing photon rate, line wavelengths (nline, A), lines (nnz, nna, nnu, nline) L_sun per unit
+    ionizing photon rate), FP64: the arrays read_nebular_standard reads (src/io/opensps_io.f90:914-937)."""
+    d = library_dir(root, "nebular", name)
+    m = read_manifest(d, "nebular")
+    ax = m["axes"]
+    logz = np.asarray(ax["logZ"]["values"], dtype=np.float64)
+    age = np.asarray(ax["age"]["values"], dtype=np.float64)
+    logu = np.asarray(ax["logU"]["values"], dtype=np.float64)
+    lam = np.asarray(_axis(d, ax["wavelength"], verify), dtype=np.float64)
+    llam = np.asarray(_axis(d, ax["line_wavelength"], verify), dtype=np.float64)
+    c, l = m["arrays"]["continuum"], m["arrays"]["lines"]
+    if c.get("axes") != ["logZ", "age", "logU", "wavelength"]:
+        raise DataLibraryError(f"{d}: continuum axes {c.get('axes')} are not [logZ, age, logU, wavelength]")
+    if l.get("axes") != ["logZ", "age", "logU", "line_wavelength"]:
+        raise DataLibraryError(f"{d}: lines axes {l.get('axes')} are not [logZ, age, logU, line_wavelength]")
+    cont = np.asarray(read_array(d, c, verify), dtype=np.float64)
+    lines = np.asarray(read_array(d, l, verify), dtype=np.float64)
+    if cont.shape != (logz.size, age.size, logu.size, lam.size):
+        raise DataLibraryError(f"{d}: continuum {cont.shape} against the axes")
+    if lines.shape != (logz.size, age.size, logu.size, llam.size):
+        raise DataLibraryError(f"{d}: lines {lines.shape} against the axes")
+    return logz, age, logu, lam, cont, llam, lines

@@ -11,23 +11,24 @@ module opensps_host_types
     integer, parameter, public :: WP = c_double
     real(WP), parameter, public :: SAFE_FLOOR = tiny(1.0_wp)
 
-    integer, parameter, public :: NPAR = 33
+    integer, parameter, public :: NPAR = 35
     character(len=16), parameter, public :: PARAM_NAMES(NPAR) = [character(len=16) :: &
         'tage', 'tau', 'const', 'fburst', 'tburst', 'sf_start', 'sf_trunc', 'sf_slope', 'sfh', 'logzsol', &
         'dust1', 'dust2', 'dust3', 'dust_index', 'dust1_index', 'dust_tesc', 'frac_obrun', 'frac_nodust', &
         'zred', 'igm_factor', 'sigma_smooth', 'min_wave_smooth', 'max_wave_smooth', 'imf1', 'imf2', 'imf3', &
-        'imf_upper_limit', 'imf_lower_limit', 'duste_umin', 'duste_qpah', 'duste_gamma', 'fagn', 'agn_tau']
+        'imf_upper_limit', 'imf_lower_limit', 'duste_umin', 'duste_qpah', 'duste_gamma', 'fagn', 'agn_tau', &
+        'gas_logz', 'gas_logu']
     real(WP), parameter, public :: PARAM_DEFAULTS(NPAR) = [ &
         0.0_wp, 1.0_wp, 0.0_wp, 0.0_wp, 11.0_wp, 0.0_wp, 0.0_wp, 0.0_wp, 0.0_wp, 0.0_wp, &
         0.0_wp, 0.0_wp, 0.0_wp, -0.7_wp, -1.0_wp, 7.0_wp, 0.0_wp, 0.0_wp, &
         0.0_wp, 1.0_wp, 0.0_wp, 1.0e3_wp, 1.0e4_wp, 1.3_wp, 2.3_wp, 2.3_wp, &
-        120.0_wp, 0.08_wp, 1.0_wp, 3.5_wp, 0.01_wp, 0.0_wp, 10.0_wp]
+        120.0_wp, 0.08_wp, 1.0_wp, 3.5_wp, 0.01_wp, 0.0_wp, 10.0_wp, 0.0_wp, -2.0_wp]
     integer, parameter, public :: P_TAGE = 1, P_SFH = 9, P_LOGZSOL = 10, P_DUST1 = 11, P_DUST2 = 12, P_DUST3 = 13, &
                                   P_FRAC_OBRUN = 17, P_ZRED = 19, P_IGM_FACTOR = 20, P_SIGMA_SMOOTH = 21, &
                                   P_TAU = 2, P_CONST = 3, P_FBURST = 4, P_TBURST = 5, P_SF_START = 6, P_SF_TRUNC = 7, &
                                   P_SF_SLOPE = 8, &
                                   P_IMF1 = 24, P_DUSTE_UMIN = 29, P_DUSTE_QPAH = 30, P_DUSTE_GAMMA = 31, &
-                                  P_FAGN = 32, P_AGN_TAU = 33
+                                  P_FAGN = 32, P_AGN_TAU = 33, P_GAS_LOGZ = 34, P_GAS_LOGU = 35
 
     !> Parameters the batch refuses to vary per model: they change the SSPs (is_ssp_sensitive_float_param,
     !> src/core/opensps_context.f90:958-969, and the integer ones), so the reference's batch_handle returns status 2
@@ -42,14 +43,14 @@ module opensps_host_types
         'imf1', 'imf2', 'imf3', 'vdmc', 'mdave', 'imf_lower_limit', 'imf_upper_limit']
 
     !> Members of PARAMS_CORE (src/core/opensps_types.f90:33-45) that no stage this build runs reads: the dust
-    !> curves of dust_type 1, 3, 4, 5 (mwr, uvb), the clumpy screen and pmetals (not read by the batch) and nebular
-    !> emission (gas_*). They are accepted per model and at the context, as the reference accepts them; the host
-    !> refuses the stages that would read them (status 2: nebular emission)
-    integer, parameter, public :: NINERT = 9
+    !> curves of dust_type 1, 3, 4, 5 (mwr, uvb), the clumpy screen and pmetals (not read by the batch) and cue's gas
+    !> parameters (gas_logn, gas_logno, gas_logco: nebular_model = 1, not built). They are accepted per model and at
+    !> the context, as the reference accepts them
+    integer, parameter, public :: NINERT = 7
     character(len=16), parameter, public :: INERT_NAMES(NINERT) = [character(len=16) :: &
-        'pmetals', 'dust_clumps', 'uvb', 'mwr', 'gas_logu', 'gas_logz', 'gas_logn', 'gas_logno', 'gas_logco']
+        'pmetals', 'dust_clumps', 'uvb', 'mwr', 'gas_logn', 'gas_logno', 'gas_logco']
     real(WP), parameter, public :: INERT_DEFAULTS(NINERT) = [ &
-        0.02_wp, -99.0_wp, 1.0_wp, 3.1_wp, -2.0_wp, 0.0_wp, 2.0_wp, 0.0_wp, 0.0_wp]
+        0.02_wp, -99.0_wp, 1.0_wp, 3.1_wp, 2.0_wp, 0.0_wp, 0.0_wp]
 
     !> Rows of params_bar (include/opensps.h:12-59), as opensps_get_nbgrad reports them
     integer, parameter, public :: NBGRAD = 44

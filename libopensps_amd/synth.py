@@ -385,3 +385,45 @@ def agn_templates(lam: np.ndarray, tau=AGN_TAU) -> np.ndarray:
         dnu = np.abs(np.gradient(nu))
         out[k] = f / np.sum(f * dnu)
     return out
+
+
+C_LIGHT_A = 2.99792458e18   # speed of light, A/s
+NEB_LOGZ = np.linspace(-2.0, 0.2, 11)
+NEB_AGE = np.array([0.5e6, 1.0e6, 2.0e6, 3.0e6, 4.0e6, 5.0e6, 6.0e6, 8.0e6, 1.0e7, 2.0e7])
+NEB_LOGU = np.linspace(-4.0, -1.0, 7)
+
+
+def nebular_library(grid: SSPGrid, nline: int = 166, seed: int = 31, cont_frac: float = 0.3):
+    """A synthetic CLOUDY library in the layout datalib.nebular returns (logz, age (yr), logu, wavelength (nlam),
+    continuum (nnz, nna, nnu, nlam) L_sun/Hz per unit ionizing photon rate, line wavelengths (nline), lines (nnz,
+    nna, nnu, nline) L_sun per unit ionizing photon rate), scaled to the grid: per unit Q of the grid's youngest
+    solar-metallicity SSP, its continuum is about cont_frac of that SSP's optical L_nu, smooth in wavelength (a power
+    law whose slope follows log U, a Balmer-like step at 3646 A, 1e-6 of it below the Lyman limit), fading with age
+    and rising with Z; the lines (log-uniform wavelengths 923 A - 6 micron) have equivalent widths of 1 - 300 A
+    against that continuum, varying with Z, U and age."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lam = np.geomspace(91.0, 1.0e8, 200)
+    llam = np.sort(np.exp(rng.uniform(np.log(923.0), np.log(6.0e4), nline)))
+    iz = int(np.argmin(np.abs(grid.logz)))
+    b = grid.spec[iz, 0]
+    nu = C_LIGHT_A / grid.lam
+    wl = int(np.searchsorted(grid.lam, 912.0, side="right"))
+    q = np.sum(0.5 * np.abs(np.diff(nu[:wl])) * (b[:wl - 1] / nu[:wl - 1] + b[1:wl] / nu[1:wl])) / 6.62607015e-27 * 3.828e33
+    opt = (grid.lam > 3000.0) & (grid.lam < 7000.0)
+    lref = float(np.median(b[opt]))
+    base = cont_frac * lref / max(q, 1e-300)
+    nz, na, nu_ = NEB_LOGZ.size, NEB_AGE.size, NEB_LOGU.size
+    cont = np.empty((nz, na, nu_, lam.size))
+    lines = np.empty((nz, na, nu_, nline))
+    ew = 10.0 ** rng.uniform(0.0, 2.5, nline)
+    zs, us = rng.uniform(-0.3, 0.3, nline), rng.uniform(-0.4, 0.4, nline)
+    for i, lz in enumerate(NEB_LOGZ):
+        for j, age in enumerate(NEB_AGE):
+            fade = np.exp(-age / 6.0e6)
+            for k, lu in enumerate(NEB_LOGU):
+                shape = (lam / 5000.0) ** (-0.3 + 0.1 * lu) * np.where(lam < 3646.0, 0.5, 1.0)
+                shape = np.where(lam < 912.0, 1.0e-6 * shape, shape)
+                cont[i, j, k] = base * shape * fade * (1.0 + 0.15 * lz)
+                cl = base * (llam / 5000.0) ** (-0.3 + 0.1 * lu) * fade * (1.0 + 0.15 * lz)
+                lines[i, j, k] = ew * (10.0 ** (zs * lz + us * (lu + 2.5))) * cl * C_LIGHT_A / llam ** 2
+    return NEB_LOGZ.copy(), NEB_AGE.copy(), NEB_LOGU.copy(), lam, cont, llam, lines

@@ -1463,6 +1463,15 @@ typedef struct {
     int nqpah, numin, ntau;
     const double *qpah, *umin, *dustem, *agn_tau, *agn;
     double *mdust_out;
+    /* nebular emission (add_neb_emission = 1, the CLOUDY grids) with its tables on the grid as load_nebular_grid
+     * leaves them (src/io/opensps_io.f90:749-823): neb_cont nspec x nnz x nna x nnu and neb_line nline x nnz x nna x
+     * nnu, column-major, log10(L + 1e-95) per unit ionizing photon rate (the continuum linear in lambda onto the
+     * grid); the axes ascending: gas log Z, log10 age (yr), log U; neb_lpos the lines' wavelengths. neb_cont_on:
+     * add_neb_continuum; neb_lines_in_spec: nebemlineinspec. emlines_out (nline x n) or NULL */
+    int add_neb, neb_cont_on, neb_lines_in_spec;
+    int nnz, nna, nnu, nline;
+    const double *neb_logz, *neb_lage, *neb_logu, *neb_cont, *neb_line, *neb_lpos;
+    double *emlines_out;
 } orc_ctx_t;
 
 /* one_minus_trans_kp (opensps_csp_batch.f90:3925-3939), FP64: 1 - exp(-x) without cancellation */
@@ -1551,6 +1560,235 @@ static void model_zlo(const orc_ctx_t *c, double logzsol, int *zlo, double *dz) 
  * 1-based; zlo = 1 with one basis), w_out (nt x n, the FP64 SFH weights), fband (nbands x n).
  * Returns 0, 2 (unsupported SFH type), 3 (bad arguments).
  */
+/* ---------------------------------------------------------------------------------------------
+ * Nebular emission with the CLOUDY grids (nebular_model = 0): nebular_setup (opensps_csp_batch.f90:1630-1770),
+ * nebular_kernels (:2612-2998: the ionizing photons, the EUV rows with the escaping fraction, the continuum and the
+ * line luminosities summed over the nebular ages), line_profiles (:3133-3160) and line_attenuated (:3440-3473). The
+ * lines' profiles are compute_line_gaussians' (opensps_gas.f90:298-340, velocity smoothing) with the minimum width
+ * of compute_nebular_kernels (opensps_initialization.f90:1377-1385), built from the first model's sigma_smooth as
+ * the batch builds them (prepare_line_gaussians(ctx, psets(1))), each cut where it falls to 1e-14 of its peak.
+ * ------------------------------------------------------------------------------------------- */
+#define H_PLANCK 6.62607015e-27           /* erg s: opensps_constants.f90 */
+#define L_SOL 3.828e33                    /* erg / s: opensps_constants.f90 */
+#define NEB_PROFILE_CUT 1.0e-14           /* LINE_PROFILE_CUT (opensps_csp_batch.f90:1638) */
+
+typedef struct {
+    int nt_neb, whlylim;
+    int *ia;                  /* per age 1..nt_neb: the CLOUDY age interval (1-based) */
+    double *wa;               /* and its weight */
+    double *q0;               /* nt_neb x nz: ionizing photons of each basis spectrum (frac_obrun = 0) */
+    int *llo;                 /* per line: the pixel interval of its wavelength (1-based) */
+    int *jlo, *jhi;           /* per line: the window of its profile (1-based; jlo > jhi: none) */
+    double *gauss;            /* per line: the profile over its window (at goff[i]) */
+    int *goff;
+} orc_neb_t;
+
+/* get_grid_indices_weights (opensps_gas.f90:394-412): the clamped interval and weight */
+static void neb_index(const double *g, int ng, double v, int *idx, double *w) {
+    int i = orc_find_interval(g, ng, v);
+    i = i < 1 ? 1 : (i > ng - 1 ? ng - 1 : i);
+    double x = (v - g[i - 1]) / (g[i] - g[i - 1]);
+    *idx = i;
+    *w = x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x);
+}
+
+/* zu_point (:3942-3955): a table column (stride s0 between rows of the first axis, column-major) interpolated in
+ * log Z and log U at CLOUDY age a (1-based) */
+static double neb_zu(const orc_ctx_t *c, const double *tab, int nrow, int row, int a, int iz, int iu, double wz,
+                     double wu) {
+    const size_t sz = (size_t)nrow, sa = sz * c->nnz, su = sa * c->nna;
+#define NT_(z, u) tab[(size_t)(row - 1) + (size_t)((z) - 1) * sz + (size_t)(a - 1) * sa + (size_t)((u) - 1) * su]
+    return (1.0 - wz) * (1.0 - wu) * NT_(iz, iu) + (1.0 - wz) * wu * NT_(iz, iu + 1) + wz * (1.0 - wu) * NT_(iz + 1, iu) +
+           wz * wu * NT_(iz + 1, iu + 1);
+#undef NT_
+}
+
+static void neb_free(orc_neb_t *nb) {
+    free(nb->ia); free(nb->wa); free(nb->q0); free(nb->llo); free(nb->jlo); free(nb->jhi); free(nb->gauss);
+    free(nb->goff);
+    memset(nb, 0, sizeof(*nb));
+}
+
+/* nebular_setup's batch-wide part (the per-model intervals are formed in the model's loop), with step 1 of
+ * nebular_kernels (the ionizing photons of each basis spectrum of a nebular age, the photon-number integral of
+ * process_ionizing_radiation without the absorbed fraction) */
+static void neb_setup(const orc_ctx_t *c, double sigma0, orc_neb_t *nb) {
+    const int nspec = c->nspec, nt = c->nt, nl = c->nline;
+    memset(nb, 0, sizeof(*nb));
+    int ntn = orc_find_interval(c->time_full, nt, c->neb_lage[c->nna - 1]);
+    nb->nt_neb = ntn < 0 ? 0 : (ntn > nt ? nt : ntn);
+    nb->whlylim = orc_find_interval(c->lambda, nspec, 912.0);
+    const int n1 = nb->nt_neb > 0 ? nb->nt_neb : 1;
+    nb->ia = (int *)malloc(sizeof(int) * n1);
+    nb->wa = (double *)malloc(sizeof(double) * n1);
+    nb->q0 = (double *)malloc(sizeof(double) * n1 * c->nz);
+    for (int t = 1; t <= nb->nt_neb; t++) neb_index(c->neb_lage, c->nna, c->time_full[t - 1], &nb->ia[t - 1], &nb->wa[t - 1]);
+    for (int iz = 1; iz <= c->nz; iz++)
+        for (int t = 1; t <= nb->nt_neb; t++) {
+            const double *b = c->basis + ((size_t)(iz - 1) * nt + (t - 1)) * nspec;
+            double s = 0.0;
+            if (nb->whlylim >= 2) {
+                for (int i = 1; i <= nb->whlylim - 1; i++) {
+                    double nu0 = C_LIGHT / c->lambda[i - 1], nu1 = C_LIGHT / c->lambda[i];
+                    double wt = 0.5 * fabs(nu1 - nu0);
+                    s = s + wt * (b[i - 1] / nu0 + b[i] / nu1);
+                }
+                nb->q0[(size_t)(iz - 1) * n1 + (t - 1)] = s / H_PLANCK * L_SOL;
+            } else {
+                nb->q0[(size_t)(iz - 1) * n1 + (t - 1)] = 0.0;
+            }
+        }
+    /* the lines' pixel intervals and profiles */
+    nb->llo = (int *)malloc(sizeof(int) * (nl > 0 ? nl : 1));
+    nb->jlo = (int *)malloc(sizeof(int) * (nl > 0 ? nl : 1));
+    nb->jhi = (int *)malloc(sizeof(int) * (nl > 0 ? nl : 1));
+    nb->goff = (int *)malloc(sizeof(int) * (nl + 1));
+    double *g = (double *)malloc(sizeof(double) * nspec);
+    const double pi = acos(-1.0), sq2pi = sqrt(2.0 * pi), kms = 1.0e13 / C_LIGHT;
+    int tot = 0, cap = 1024;
+    nb->gauss = (double *)malloc(sizeof(double) * cap);
+    for (int i = 1; i <= nl; i++) {
+        const double pos = c->neb_lpos[i - 1];
+        int jc = 1, j = nspec + 1;
+        while (j - jc > 1) {
+            int p = (jc + j) / 2;
+            if (pos >= c->lambda[p - 1]) jc = p; else j = p;
+        }
+        nb->llo[i - 1] = jc < 1 ? 1 : (jc > nspec - 1 ? nspec - 1 : jc);
+        nb->goff[i - 1] = tot;
+        nb->jlo[i - 1] = 1;
+        nb->jhi[i - 1] = 0;
+        if (!c->neb_lines_in_spec) continue;
+        int jr = orc_find_interval(c->lambda, nspec, pos);
+        jr = jr < 1 ? 1 : (jr > nspec - 1 ? nspec - 1 : jr);
+        const double res = c->lambda[jr] - c->lambda[jr - 1];
+        double sig = pos * sigma0 * kms;
+        if (sig < res * 2.0) sig = res * 2.0;
+        const double norm = (1.0 / (sq2pi * sig)) * (pos * pos / C_LIGHT);
+        for (int k = 0; k < nspec; k++) {
+            double x = (c->lambda[k] - pos) / sig;
+            g[k] = norm * exp(-0.5 * x * x);
+        }
+        int jcc = orc_find_interval(c->lambda, nspec, pos);
+        jcc = jcc < 1 ? 1 : (jcc > nspec ? nspec : jcc);
+        int jh = jcc + 1 < nspec ? jcc + 1 : nspec;
+        double cut = NEB_PROFILE_CUT * (g[jcc - 1] > g[jh - 1] ? g[jcc - 1] : g[jh - 1]);
+        if (cut <= 0.0) continue;
+        int lo = jcc, hi = jh;
+        while (lo > 1 && g[lo - 2] > cut) lo--;
+        while (hi < nspec && g[hi] > cut) hi++;
+        nb->jlo[i - 1] = lo;
+        nb->jhi[i - 1] = hi;
+        if (tot + (hi - lo + 1) > cap) {
+            while (tot + (hi - lo + 1) > cap) cap *= 2;
+            nb->gauss = (double *)realloc(nb->gauss, sizeof(double) * cap);
+        }
+        for (int k = lo; k <= hi; k++) nb->gauss[tot++] = g[k - 1];
+    }
+    nb->goff[nl] = tot;
+    free(g);
+}
+
+/* The nebular emission of one model on the basis path (nebular_kernels steps 2-5 for column m): the EUV rows of the
+ * young and old age sums redone with the escaping fraction fesc = clamp(frac_obrun) on the nebular ages; the
+ * continuum sum_t w(t) Q(t) 10^lg(pixel, t) and the lines' luminosities likewise (lg interpolated bilinearly in the
+ * gas log Z and log U, linearly in log age between the CLOUDY ages; Q the blend of the bracketing metallicities'
+ * photons times 1 - frac_obrun); the lines spread over their profiles. y, o: the age sums (in, out); w: the SFH
+ * weights (those <= SAFE_FLOOR taken as 0); it: it_tesc; lum: 2 x nline (young, then old) out. */
+static void neb_model(const orc_ctx_t *c, const orc_neb_t *nb, const double *p, const double *w, int it, int zlo,
+                      double dz, int blend, double *y, double *o, double *lum) {
+    const int nspec = c->nspec, nt = c->nt, nl = c->nline, ntn = nb->nt_neb;
+    const int n1 = ntn > 0 ? ntn : 1;
+    for (int i = 0; i < 2 * nl; i++) lum[i] = 0.0;
+    if (ntn < 1) return;
+    int iz = 1, iu = 1;
+    double wz = 0.0, wu = 0.0;
+    neb_index(c->neb_logz, c->nnz, p[OPS_P_GAS_LOGZ], &iz, &wz);
+    neb_index(c->neb_logu, c->nnu, p[OPS_P_GAS_LOGU], &iu, &wu);
+    double fesc = p[OPS_P_FRAC_OBRUN];
+    fesc = fesc < 0.0 ? 0.0 : (fesc > 1.0 ? 1.0 : fesc);
+    const double fabs_ = 1.0 - p[OPS_P_FRAC_OBRUN];
+    /* 2. the EUV rows */
+    const double *blo = c->basis + (size_t)(zlo - 1) * nspec * nt;
+    const double *bhi = blend ? c->basis + (size_t)zlo * nspec * nt : blo;
+    for (int i = 1; i <= nb->whlylim; i++) {
+        double sy = 0.0, so = 0.0;
+        for (int k = 1; k <= nt; k++) {
+            double wk = w[k - 1] > SAFE_FLOOR ? w[k - 1] : 0.0;
+            double bk = blend ? (1.0 - dz) * blo[(size_t)(k - 1) * nspec + i - 1] + dz * bhi[(size_t)(k - 1) * nspec + i - 1]
+                              : (1.0 - dz) * blo[(size_t)(k - 1) * nspec + i - 1];
+            if (k <= ntn) bk = bk * fesc;
+            if (k <= it) sy = sy + wk * bk;
+            else so = so + wk * bk;
+        }
+        y[i - 1] = sy;
+        o[i - 1] = so;
+    }
+    /* the weights times Q per nebular age */
+    double *cq = (double *)malloc(sizeof(double) * n1);
+    for (int t = 1; t <= ntn; t++) {
+        double q = blend ? (1.0 - dz) * nb->q0[(size_t)(zlo - 1) * n1 + t - 1] + dz * nb->q0[(size_t)zlo * n1 + t - 1]
+                         : nb->q0[t - 1];
+        q = fabs(q * fabs_);
+        double wk = w[t - 1] > SAFE_FLOOR ? w[t - 1] : 0.0;
+        cq[t - 1] = (wk != 0.0 && q > SAFE_FLOOR) ? wk * q : 0.0;
+    }
+    /* 3. the continuum */
+    if (c->neb_cont_on)
+        for (int j = 1; j <= nspec; j++) {
+            double sy = 0.0, so = 0.0;
+            for (int t = 1; t <= ntn; t++) {
+                if (cq[t - 1] == 0.0) continue;
+                int a = nb->ia[t - 1];
+                double ra = neb_zu(c, c->neb_cont, nspec, j, a, iz, iu, wz, wu);
+                double rb = neb_zu(c, c->neb_cont, nspec, j, a + 1, iz, iu, wz, wu);
+                double v = cq[t - 1] * pow(10.0, (1.0 - nb->wa[t - 1]) * ra + nb->wa[t - 1] * rb);
+                if (t <= it) sy = sy + v;
+                else so = so + v;
+            }
+            y[j - 1] = y[j - 1] + sy;
+            o[j - 1] = o[j - 1] + so;
+        }
+    /* 4. the lines' luminosities */
+    for (int i = 1; i <= nl; i++) {
+        double sy = 0.0, so = 0.0;
+        for (int t = 1; t <= ntn; t++) {
+            double wk = w[t - 1] > SAFE_FLOOR ? w[t - 1] : 0.0;
+            if (wk == 0.0) continue;
+            double q = blend ? (1.0 - dz) * nb->q0[(size_t)(zlo - 1) * n1 + t - 1] + dz * nb->q0[(size_t)zlo * n1 + t - 1]
+                             : nb->q0[t - 1];
+            q = fabs(q * fabs_);
+            if (q <= SAFE_FLOOR) continue;
+            int a = nb->ia[t - 1];
+            double ra = neb_zu(c, c->neb_line, nl, i, a, iz, iu, wz, wu);
+            double rb = neb_zu(c, c->neb_line, nl, i, a + 1, iz, iu, wz, wu);
+            double lg = (1.0 - nb->wa[t - 1]) * ra + nb->wa[t - 1] * rb;
+            if (t <= it) sy = sy + wk * (pow(10.0, lg) * q);
+            else so = so + wk * (pow(10.0, lg) * q);
+        }
+        lum[i - 1] = sy;
+        lum[nl + i - 1] = so;
+    }
+    free(cq);
+    /* 5. the lines' profiles: per pixel, the lines whose window holds it, in line order (line_profiles) */
+    if (c->neb_lines_in_spec)
+        for (int j = 1; j <= nspec; j++) {
+            double sy = 0.0, so = 0.0;
+            int any = 0;
+            for (int i = 1; i <= nl; i++) {
+                if (j < nb->jlo[i - 1] || j > nb->jhi[i - 1]) continue;
+                double g = nb->gauss[nb->goff[i - 1] + (j - nb->jlo[i - 1])];
+                sy = sy + g * lum[i - 1];
+                so = so + g * lum[nl + i - 1];
+                any = 1;
+            }
+            if (any) {
+                y[j - 1] = y[j - 1] + sy;
+                o[j - 1] = o[j - 1] + so;
+            }
+        }
+}
+
 int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ssp_weights, double *mags,
                   double *spec, double *mass, double *lbol, double *young, double *old, int *idx, double *w_out,
                   double *fband_out) {
@@ -1581,6 +1819,23 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
         if (prm[OPS_P_ZRED * n + m] != prm[OPS_P_ZRED * n] ||
             prm[OPS_P_IGM_FACTOR * n + m] != prm[OPS_P_IGM_FACTOR * n]) z_varies = 1;
     int cap = nspec * 64;
+    /* nebular emission: the batch-wide set-up; not with weights over metallicities or the IMF per model (their
+     * nebular paths are not restated here), nor lines in the spectrum with models of different sigma_smooth
+     * (status 2, opensps_csp_batch.f90:749-752), nor lines outside the spectrum with dust emission */
+    orc_neb_t neb;
+    memset(&neb, 0, sizeof(neb));
+    const int nl = c->add_neb ? c->nline : 0;
+    if (c->add_neb) {
+        int bad = c->ssp_weights_z != NULL || c->ops != NULL || (!c->neb_lines_in_spec && c->add_dust_emission == 1);
+        if (c->neb_lines_in_spec)
+            for (int m = 1; m < n; m++)
+                if (prm[OPS_P_SIGMA_SMOOTH * n + m] != prm[OPS_P_SIGMA_SMOOTH * n]) bad = 1;
+        if (bad) {
+            free(jlo); free(jhi); free(do_band); free(bt); free(wtl); free(ltab);
+            return 2;
+        }
+        neb_setup(c, prm[OPS_P_SIGMA_SMOOTH * n], &neb);
+    }
 
     #pragma omp parallel
     {
@@ -1595,6 +1850,7 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
         double *scoef = (double *)malloc(sizeof(double) * cap);
         double *tau = (double *)malloc(sizeof(double) * (jtop + 1));
         double *tdv = (double *)malloc(sizeof(double) * nspec);
+        double *nlum = (double *)malloc(sizeof(double) * (2 * nl + 1));
         double sm_key[3] = {NAN, NAN, NAN};
         int sm_n = 0;
         double *smass = NULL, *slum = NULL, *vy = NULL, *vo = NULL, *wpt = NULL, *llo = NULL, *dl = NULL;
@@ -1708,6 +1964,8 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
                     for (int i = 0; i < nspec; i++) acc[i] = acc[i] + wk * (c1 * bl[i]);
             }
         sums_done:
+            /* stage 4: nebular emission (add_nebular_batch, :2565-2596) into the age sums */
+            if (c->add_neb) neb_model(c, &neb, p, w, it, zlo, dz, blend, y, o, nlum);
             if (mass) mass[m] = ms;
             if (lbol) lbol[m] = ls > SAFE_FLOOR ? log10(ls) : 0.0;
             lsum_m = ls;
@@ -1727,6 +1985,31 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
                 #pragma omp atomic write
                 status = 2;
                 continue;
+            }
+            /* the lines' luminosities through the birth cloud and the diffuse dust (attenuate_lines,
+             * line_attenuated :3440-3473: T_d linear between the line's two pixels) */
+            if (c->add_neb && c->emlines_out) {
+                for (int i = 1; i <= nl; i++) {
+                    const double pos = c->neb_lpos[i - 1];
+                    const int lo = neb.llo[i - 1];
+                    double td = 1.0;
+                    if (!(1.0 - fnd <= SAFE_FLOOR)) {
+                        double cv0 = c->dcurve ? c->dcurve[lo - 1] : (ed != 0.0 ? exp(ed * ltab[lo - 1]) : 1.0);
+                        double cv1 = c->dcurve ? c->dcurve[lo] : (ed != 0.0 ? exp(ed * ltab[lo]) : 1.0);
+                        double t_lo = exp(-p[OPS_P_DUST2] * cv0), t_hi = exp(-p[OPS_P_DUST2] * cv1);
+                        td = t_lo + (t_hi - t_lo) / (c->lambda[lo] - c->lambda[lo - 1]) * (pos - c->lambda[lo - 1]);
+                    }
+                    if (td != td) td = 1.0;
+                    double birth = 1.0;
+                    if (!(p[OPS_P_DUST1] <= SAFE_FLOOR || 1.0 - fob <= SAFE_FLOOR))
+                        birth = exp(-p[OPS_P_DUST1] * pow(pos / 5500.0, p[OPS_P_DUST1_INDEX]));
+                    double ay = nlum[i - 1] * (birth * (1.0 - fob) + fob), ao = nlum[nl + i - 1];
+                    if (1.0 - fnd > SAFE_FLOOR) {
+                        ay = ay * (td * (1.0 - fnd) + fnd);
+                        ao = ao * (td * (1.0 - fnd) + fnd);
+                    }
+                    c->emlines_out[(size_t)m * nl + i - 1] = ay + ao;
+                }
             }
             int iq = 1, iu = 1;
             double wq = 0.0, wu = 0.0, gam = 0.0;
@@ -1836,10 +2119,11 @@ int orc_csp_batch(const orc_ctx_t *c, int n, const double *prm, const double *ss
             }
         }
         free(p); free(w); free(t1); free(t2); free(y); free(o); free(tot); free(work); free(fobs); free(fb);
-        free(slo); free(shi); free(soff); free(scoef); free(tau); free(tdv);
+        free(slo); free(shi); free(soff); free(scoef); free(tau); free(tdv); free(nlum);
         free(smass); free(slum); free(vy); free(vo); free(wpt); free(llo); free(dl);
     }
     free(jlo); free(jhi); free(do_band); free(bt); free(wtl); free(ltab);
+    neb_free(&neb);
     return status;
 }
 
@@ -1876,6 +2160,7 @@ int orc_csp_batch_grad(const orc_ctx_t *c, int n, const double *prm, const doubl
     const int nspec = c->nspec, nt = c->nt, nz = c->nz, nb = c->nbands;
     if (n < 0 || nspec < 2 || nt < 2 || nz < 1) return 3;
     if (c->ops && c->ssp_weights_z) return 2;
+    if (c->add_neb) return 2;   /* the nebular stage's reverse pass is not restated */
     if (n == 0) return 0;
     const double amzl = abs_mag_zeropoint_log(), ln10 = log(10.0);
     int status = 0;

@@ -362,7 +362,11 @@ class _Ctx(C.Structure):
                 ("ssp_weights_z", C.c_void_p), ("dcurve", C.c_void_p), ("zred_mags", C.c_void_p),
                 ("add_dust_emission", C.c_int), ("add_agn_dust", C.c_int), ("nqpah", C.c_int), ("numin", C.c_int),
                 ("ntau", C.c_int), ("qpah", C.c_void_p), ("umin", C.c_void_p), ("dustem", C.c_void_p),
-                ("agn_tau", C.c_void_p), ("agn", C.c_void_p), ("mdust_out", C.c_void_p)]
+                ("agn_tau", C.c_void_p), ("agn", C.c_void_p), ("mdust_out", C.c_void_p),
+                ("add_neb", C.c_int), ("neb_cont_on", C.c_int), ("neb_lines_in_spec", C.c_int), ("nnz", C.c_int),
+                ("nna", C.c_int), ("nnu", C.c_int), ("nline", C.c_int), ("neb_logz", C.c_void_p),
+                ("neb_lage", C.c_void_p), ("neb_logu", C.c_void_p), ("neb_cont", C.c_void_p), ("neb_line", C.c_void_p),
+                ("neb_lpos", C.c_void_p), ("emlines_out", C.c_void_p)]
 
 
 def _context(grid, bands, add_igm, smooth_fast, mag_mask, cosmo, ptr):
@@ -462,6 +466,49 @@ def emission_on_grid(lam, dust=None, agn=None, add_dust_emission=1, add_agn_dust
     return Emission(qp, um, dt, at, tb, add_dust_emission=add_dust_emission, add_agn_dust=add_agn_dust)
 
 
+NEBULAR_FLOOR = 1.0e-95   # load_nebular_grid's floor before the logarithm (src/io/opensps_io.f90:763)
+
+
+class Nebular:
+    """The CLOUDY tables for the oracle and the device, as load_nebular_grid leaves them (src/io/opensps_io.f90:749-823):
+    logz (nnz), lage (nna, log10 yr), logu (nnu) ascending; cont (nnu, nna, nnz, nspec) C order = the reference's
+    nebem_cont (nspec, nnz, nna, nnu): log10(L_nu + 1e-95) per unit ionizing photon rate, linear in wavelength onto
+    the grid; lines (nnu, nna, nnz, nline) C order = nebem_line (nline, nnz, nna, nnu): log10(L + 1e-95); lpos
+    (nline) the lines' wavelengths; add_neb_continuum and nebemlineinspec."""
+
+    def __init__(self, logz, lage, logu, cont, lines, lpos, cont_on=1, lines_in_spec=1):
+        self.logz, self.lage, self.logu = (np.ascontiguousarray(a, dtype=np.float64) for a in (logz, lage, logu))
+        self.cont = np.ascontiguousarray(cont, dtype=np.float64)
+        self.lines = np.ascontiguousarray(lines, dtype=np.float64)
+        self.lpos = np.ascontiguousarray(lpos, dtype=np.float64)
+        self.cont_on, self.lines_in_spec = int(cont_on), int(lines_in_spec)
+
+
+def nebular_on_grid(lam, logz, age, logu, lib_lam, cont, line_lam, lines, cont_on=1, lines_in_spec=1):
+    """The library of datalib.nebular (axes logZ, age (yr), logU; wavelength (nlam); continuum (nnz, nna, nnu, nlam)
+    L_sun/Hz per unit Q; line wavelengths (nline); lines (nnz, nna, nnu, nline) L_sun per unit Q) on the grid lam, as
+    load_nebular_grid places it: log10(x + 1e-95), the continuum linear in wavelength (the ends extrapolated, as
+    interpolate_linear_scalar does), the age axis to log10."""
+    cont = np.asarray(cont, dtype=np.float64)
+    lines = np.asarray(lines, dtype=np.float64)
+    nnz, nna, nnu = cont.shape[:3]
+    lc = np.log10(cont + NEBULAR_FLOOR)
+    out = np.empty((nnu, nna, nnz, len(lam)))
+    for i in range(nnz):
+        for j in range(nna):
+            for k in range(nnu):
+                out[k, j, i] = _interp_extrap(lib_lam, lc[i, j, k], lam)
+    ll = np.log10(lines + NEBULAR_FLOOR).transpose(2, 1, 0, 3)   # (nnu, nna, nnz, nline)
+    return Nebular(logz, np.log10(np.asarray(age, dtype=np.float64)), logu, out, ll, line_lam, cont_on, lines_in_spec)
+
+
+def _set_nebular(ctx, nb, ptr):
+    ctx.add_neb, ctx.neb_cont_on, ctx.neb_lines_in_spec = 1, nb.cont_on, nb.lines_in_spec
+    ctx.nnz, ctx.nna, ctx.nnu, ctx.nline = len(nb.logz), len(nb.lage), len(nb.logu), len(nb.lpos)
+    ctx.neb_logz, ctx.neb_lage, ctx.neb_logu = ptr(nb.logz), ptr(nb.lage), ptr(nb.logu)
+    ctx.neb_cont, ctx.neb_line, ctx.neb_lpos = ptr(nb.cont), ptr(nb.lines), ptr(nb.lpos)
+
+
 def _set_emission(ctx, e, ptr):
     ctx.add_dust_emission, ctx.add_agn_dust = int(e.add_dust_emission), int(e.add_agn_dust)
     if e.dustem is not None:
@@ -485,7 +532,8 @@ GRAD_ROWS = {"dust1": 1, "dust2": 2, "dust3": 3, "dust_index": 4, "dust1_index":
 
 def csp_batch_grad(grid, bands, prm, ssp_weights=None, ssp_weights_z=None, mags_bar=None, spec_bar=None,
                    mass_bar=None, lbol_bar=None, add_igm=1, smooth_fast=1, mag_mask=None, cosmo=None, ops=None,
-                   imf_type=2, remnants=1, salp_ind=2.35, mdave=0.5, mlim_bh=40.0, mlim_ns=8.5, zred_mags=None):
+                   imf_type=2, remnants=1, salp_ind=2.35, mdave=0.5, mlim_bh=40.0, mlim_ns=8.5, zred_mags=None,
+                   nebular=None):
     """The oracle's reverse pass (FP64). prm (NPAR, n); ssp_weights (n, nt); ssp_weights_z (n, nz, nt);
     cotangents mags_bar (n, nbands), spec_bar (n, nspec), mass_bar, lbol_bar (n); ops: a synth.SSPOperator (the
     IMF per model and its rows imf1, imf2 and imf3 or mdave). Returns mags (n, nbands), params_bar (n, NBGRAD),
@@ -505,6 +553,8 @@ def csp_batch_grad(grid, bands, prm, ssp_weights=None, ssp_weights_z=None, mags_
         _set_ops(ctx, ops, ptr, keep, imf_type, remnants, salp_ind, mdave, mlim_bh, mlim_ns)
     if zred_mags is not None:
         ctx.zred_mags = ptr(_d(zred_mags))
+    if nebular is not None:
+        _set_nebular(ctx, nebular, ptr)
     out = {"mags": np.zeros((n, bands.shape[0])), "params_bar": np.zeros((n, NBGRAD)),
            "weights_bar": np.zeros((n, grid.nt)), "weights_z_bar": np.zeros((n, grid.nz, grid.nt))}
     vp = C.c_void_p
@@ -522,11 +572,12 @@ def csp_batch_grad(grid, bands, prm, ssp_weights=None, ssp_weights_z=None, mags_
 
 def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_mask=None, cosmo=None,
               want=("mags", "spec", "mass", "lbol"), ops=None, imf_type=2, remnants=1, salp_ind=2.35, mdave=0.5,
-              mlim_bh=40.0, mlim_ns=8.5, ssp_weights_z=None, dust_curve=None, zred_mags=None, emission=None):
+              mlim_bh=40.0, mlim_ns=8.5, ssp_weights_z=None, dust_curve=None, zred_mags=None, emission=None,
+              nebular=None):
     """Runs the oracle's batch. grid: synth.SSPGrid; bands (nbands, nspec); prm (NPAR, n); ops: a
     synth.SSPOperator (the IMF per model; the grid's bases are then unused); ssp_weights_z (n, nz, nt): weights
     over ages and metallicities; emission: an Emission (dust emission and AGN templates on the grid, with the two
-    settings).
+    settings); nebular: a Nebular (add_neb_emission = 1; "emlines" in want: the attenuated line luminosities).
     Returns a dict of outputs (C-order arrays: mags (n, nbands), spec (n, nspec), ...) and the status."""
     keep = []
 
@@ -562,6 +613,8 @@ def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_
     ctx.tiny_logt = 0.0
     if emission is not None:
         _set_emission(ctx, emission, ptr)
+    if nebular is not None:
+        _set_nebular(ctx, nebular, ptr)
     if ops is not None:
         _set_ops(ctx, ops, ptr, keep, imf_type, remnants, salp_ind, mdave, mlim_bh, mlim_ns)
         if "op_count" in want:
@@ -575,6 +628,9 @@ def csp_batch(grid, bands, prm, ssp_weights=None, add_igm=1, smooth_fast=1, mag_
             out[k] = np.zeros(shape)
     if "mdust" in out:
         ctx.mdust_out = out["mdust"].ctypes.data
+    if nebular is not None and "emlines" in want:
+        out["emlines"] = np.zeros((n, len(nebular.lpos)))
+        ctx.emlines_out = out["emlines"].ctypes.data
     idx = np.zeros((n, 3), np.int32) if "idx" in want else None
     if idx is not None:
         out["idx"] = idx

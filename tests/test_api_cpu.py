@@ -47,7 +47,7 @@ def test_param_slots_match_header():
     src = open(os.path.join(ROOT, "libopensps_amd", "fortran", "opensps_host_types.f90")).read()
     names = re.search(r"PARAM_NAMES\(NPAR\) = \[character\(len=16\) :: &(.*?)\]", src, re.S).group(1)
     assert re.findall(r"'(\w+)'", names) == PARAM_NAMES
-    assert NPAR == 33
+    assert NPAR == 35
 
 
 def test_handles_and_setters():
