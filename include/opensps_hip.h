@@ -63,6 +63,8 @@ typedef struct {
     double *mdust;               /* device (n) or NULL: the dust mass of the dust emission's templates
                                     (compute_csp_batch's mdust, opensps_csp_batch.f90:1206-1211; SAFE_FLOOR for a
                                     model without emission) */
+    double *emlines;             /* device (nline, n) or NULL: with nebular emission the lines' luminosities through
+                                    the dust (compute_csp_batch's emlines, attenuate_lines :3365-3436), else 0 */
 } opensps_hip_batch_t;
 
 /* The reverse pass of a batch (compute_csp_batch with params_bar / weights_bar / weights_z_bar, include/opensps.h:
@@ -141,6 +143,23 @@ int opensps_hip_set_dust_emission(opensps_hip_ctx_t *ctx, int nqpah, const doubl
 /* The AGN torus templates on the grid, host: table (nspec, ntau) column-major, tau (ntau) ascending (agndust_spec,
  * src/io/opensps_io.f90:1101-1155). ntau = 0 removes them. Call after opensps_hip_set_grid. */
 int opensps_hip_set_agn(opensps_hip_ctx_t *ctx, int ntau, const double *tau, const double *table);
+/* Nebular emission with the CLOUDY grids (add_nebular_batch and nebular_kernels, opensps_csp_batch.f90:2565-2998;
+ * the library read as read_nebular_standard reads it, src/io/opensps_io.f90:914-937): axes logz (nnz), age (nna,
+ * yr), logu (nnu), the continuum's wavelengths lam (nlam), all ascending; cont (nnz, nna, nnu, nlam) L_sun/Hz per unit
+ * ionizing photon rate and lines (nnz, nna, nnu, nline) L_sun per unit rate, C order, with the lines' wavelengths.
+ * The library places them on the grid as load_nebular_grid does (:749-823) and derives the nebular ages (those up to
+ * the library's last age) and their photons from the grid's basis: set the grid first; setting the grid drops them.
+ * nnz = 0 removes them. */
+int opensps_hip_set_nebular(opensps_hip_ctx_t *ctx, int nnz, const double *logz, int nna, const double *age, int nnu,
+                            const double *logu, int nlam, const double *lam, const double *cont, int nline,
+                            const double *line_lam, const double *lines);
+/* The context's add_neb_emission (1: on), add_neb_continuum and nebemlineinspec (defaults 0, 1, 1, as the reference's).
+ * A batch with nebular emission: status 2 without tables, with use_ops or ssp_weights_z, with nebemlineinspec = 0 while
+ * dust emission is on, from opensps_hip_csp_batch_noalloc, in the reverse pass, and (the per-model status) for a
+ * model whose sigma_smooth differs from the first model's while the lines are in the spectrum (one profile width a
+ * batch, opensps_csp_batch.f90:749-752). */
+int opensps_hip_set_neb_options(opensps_hip_ctx_t *ctx, int add_neb_emission, int add_neb_continuum,
+                                int nebemlineinspec);
 
 /* The batch: compute_csp_batch's stages 1-8 (src/physics/opensps_csp_batch.f90:431-1214) on the device,
  * asynchronous on the context's stream; the status of the models is checked by opensps_hip_sync. */

@@ -22,6 +22,11 @@ __global__ void k_fill_i32(int* __restrict__ p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
+__global__ void k_fill_f64(double* __restrict__ p, size_t n, double v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // ---- the fixed scratch of the no-allocation entry point (opensps_hip_csp_batch_noalloc) ----
 // One array per device, held while the process lives; its device address is taken once per device at context
 // creation (hipGetSymbolAddress with that device current) and used wherever host code needs the memory.
@@ -186,7 +191,9 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
                   c->opu.p, c->opwc.p, c->opsum.p, c->opseg.p, c->opimf.p, c->opspart.p, c->opgp.p, c->d_bp_ptr, c->d_bp_band,
                   c->d_bp_c64, c->glsum.p, c->gfband.p, c->gw.p, c->gout.p, c->gyb.p, c->gC.p, c->gwb.p, c->d_magvega, c->d_tab, c->d_dcurve, c->d_dcurve32,
                   c->opsraw.p, c->opct.p, c->oppart.p, c->opimfd.p,
-                  c->d_qpah, c->d_umin, c->d_dustem, c->d_agn_tau, c->d_agn, c->d_wnu};
+                  c->d_qpah, c->d_umin, c->d_dustem, c->d_agn_tau, c->d_agn, c->d_wnu, c->nebw.p,
+                  c->d_neb_lz, c->d_neb_lu, c->d_neb_cont, c->d_neb_line, c->d_neb_lpos, c->d_neb_ia, c->d_neb_newr,
+                  c->d_neb_llo, c->d_neb_wa, c->d_neb_dwa, c->d_neb_q0, c->d_neb_lptr, c->d_neb_lid, c->d_neb_gval};
     for (void* p : ps) free_ptr(p);
     free_operator(c);
     free_full(c);
@@ -236,6 +243,7 @@ int opensps_hip_set_grid(opensps_hip_ctx_t* c, int nspec, const double* lam, int
     c->nqpah = 0;       // (so are the dust emission and AGN templates: set them again)
     c->numin = 0;
     c->ntau = 0;
+    c->nnz = 0;         // and the nebular tables
     c->nbands = 0;   // the band tables index the old grid: set_bands again
     c->band_j0 = 0;
     c->band_j1 = -1;
@@ -935,6 +943,175 @@ int opensps_hip_set_agn(opensps_hip_ctx_t* c, int ntau, const double* tau, const
     return 0;
 }
 
+int opensps_hip_set_neb_options(opensps_hip_ctx_t* c, int add_neb_emission, int add_neb_continuum, int nebemlineinspec) {
+    if (!c) return 1;
+    c->add_neb = add_neb_emission == 1 ? 1 : 0;
+    c->neb_cont_on = add_neb_continuum == 1 ? 1 : 0;
+    c->neb_lines_in_spec = nebemlineinspec == 1 ? 1 : 0;
+    return 0;
+}
+
+// The CLOUDY library placed on the grid as load_nebular_grid places it (src/io/opensps_io.f90:749-823): log10(x +
+// 1e-95), the continuum linear in wavelength onto the grid (the library's end intervals extrapolated, as
+// interpolate_linear_scalar does), the age axis to log10; then what nebular_setup derives from the grid (:1630-1770):
+// the nebular ages (those up to the library's last age), each one's CLOUDY interval and weight, the weight's steps
+// within an interval, the pixels below the Lyman limit, each line's pixel interval; and step 1 of nebular_kernels,
+// the ionizing photons of every nebular basis spectrum (k_neb_q0)
+int opensps_hip_set_nebular(opensps_hip_ctx_t* c, int nnz, const double* logz, int nna, const double* age, int nnu,
+                            const double* logu, int nlam, const double* lam, const double* cont, int nline,
+                            const double* line_lam, const double* lines) {
+    if (!c) return 1;
+    if (c->nspec < 2) return ops_fail(c, 1, "set_nebular: no grid installed");
+    if (nnz == 0) {
+        c->nnz = 0;
+        return 0;
+    }
+    if (nnz < 2 || nna < 2 || nnu < 2 || nlam < 2 || nline < 1 || !logz || !age || !logu || !lam || !cont ||
+        !line_lam || !lines)
+        return ops_fail(c, 3, "set_nebular: at least 2 values on every axis, one line and every array are required");
+    const double* axes[4] = {logz, age, logu, lam};
+    const int na[4] = {nnz, nna, nnu, nlam};
+    for (int k = 0; k < 4; k++)
+        for (int i = 1; i < na[k]; i++)
+            if (!(axes[k][i] > axes[k][i - 1])) return ops_fail(c, 3, "set_nebular: axis %d not strictly ascending", k);
+    for (int i = 0; i < nna; i++)
+        if (!(age[i] > 0.0)) return ops_fail(c, 3, "set_nebular: ages must be positive (yr)");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int nspec = c->nspec, nt = c->nt;
+    const double floor_ = 1.0e-95;
+    // the tables: cont (nnu, nna, nnz, nspec) and lines (nnu, nna, nnz, nline) in C order, from the library's
+    // (nnz, nna, nnu, nlam) and (nnz, nna, nnu, nline)
+    std::vector<double> tc((size_t)nnu * nna * nnz * nspec), tl((size_t)nnu * nna * nnz * nline), lc(nlam);
+    for (int i = 0; i < nnz; i++)
+        for (int j = 0; j < nna; j++)
+            for (int k = 0; k < nnu; k++) {
+                const double* src = cont + (((size_t)i * nna + j) * nnu + k) * nlam;
+                for (int l = 0; l < nlam; l++) lc[l] = log10(src[l] + floor_);
+                double* dst = tc.data() + (((size_t)k * nna + j) * nnz + i) * nspec;
+                for (int p = 0; p < nspec; p++) {
+                    const double x = c->h_lam[p];
+                    int q = find_interval_ref(lam, nlam, x);
+                    q = std::max(1, std::min(q, nlam - 1));
+                    const double slope = (lc[q] - lc[q - 1]) / (lam[q] - lam[q - 1]);
+                    dst[p] = lc[q - 1] + slope * (x - lam[q - 1]);
+                }
+                const double* ls = lines + (((size_t)i * nna + j) * nnu + k) * nline;
+                double* ld = tl.data() + (((size_t)k * nna + j) * nnz + i) * nline;
+                for (int l = 0; l < nline; l++) ld[l] = log10(ls[l] + floor_);
+            }
+    std::vector<double> lage(nna);
+    for (int j = 0; j < nna; j++) lage[j] = log10(age[j]);
+    // the nebular ages and their CLOUDY intervals
+    const int ntn = std::max(0, std::min(find_interval_ref(c->h_time.data(), nt, lage[nna - 1]), nt));
+    const int n1 = std::max(1, ntn);
+    std::vector<int> ia(n1, 1), newr(n1, 0);
+    std::vector<double> wa(n1, 0.0), dwa(n1, 0.0);
+    for (int t = 0; t < ntn; t++) {
+        int i = std::max(1, std::min(find_interval_ref(lage.data(), nna, c->h_time[t]), nna - 1));
+        ia[t] = i;
+        wa[t] = std::max(0.0, std::min((c->h_time[t] - lage[i - 1]) / (lage[i] - lage[i - 1]), 1.0));
+    }
+    for (int t = 1; t < ntn; t++) {
+        if (ia[t] != ia[t - 1]) continue;
+        dwa[t] = wa[t] - wa[t - 1];
+        newr[t] = (ia[t - 1] != ia[std::max(0, t - 2)] || t == 1 || dwa[t] != dwa[t - 1]) ? 1 : 0;
+    }
+    // each line's pixel interval (1-based, 1 .. nspec - 1)
+    std::vector<int> llo(nline);
+    for (int i = 0; i < nline; i++) {
+        int jc = 1, j = nspec + 1;
+        while (j - jc > 1) {
+            const int p = (jc + j) / 2;
+            if (line_lam[i] >= c->h_lam[p - 1]) jc = p;
+            else j = p;
+        }
+        llo[i] = std::max(1, std::min(jc, nspec - 1));
+    }
+    UPLOAD(c, c->d_neb_cont, tc.data(), tc.size());
+    UPLOAD(c, c->d_neb_line, tl.data(), tl.size());
+    UPLOAD(c, c->d_neb_lz, logz, (size_t)nnz);
+    UPLOAD(c, c->d_neb_lu, logu, (size_t)nnu);
+    UPLOAD(c, c->d_neb_lpos, line_lam, (size_t)nline);
+    UPLOAD(c, c->d_neb_ia, ia.data(), ia.size());
+    UPLOAD(c, c->d_neb_newr, newr.data(), newr.size());
+    UPLOAD(c, c->d_neb_wa, wa.data(), wa.size());
+    UPLOAD(c, c->d_neb_dwa, dwa.data(), dwa.size());
+    UPLOAD(c, c->d_neb_llo, llo.data(), llo.size());
+    c->h_neb_lpos.assign(line_lam, line_lam + nline);
+    c->nnz = nnz;
+    c->nna = nna;
+    c->nnu = nnu;
+    c->nline = nline;
+    c->nt_neb = ntn;
+    c->whlylim = find_interval_ref(c->h_lam.data(), nspec, 912.0);
+    c->neb_sigma = -1.0;   // the profiles: built again at the next batch
+    std::vector<double> zero((size_t)c->nz * n1, 0.0);
+    UPLOAD(c, c->d_neb_q0, zero.data(), zero.size());
+    if (ntn > 0) {
+        k_neb_q0<<<dim3((unsigned)ntn, (unsigned)c->nz), 256, 0, c->stream>>>(nspec, nt, ntn, c->whlylim, c->d_lam,
+                                                                                c->d_basis64, c->d_neb_q0);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+// The lines' profiles for one width (compute_line_gaussians, opensps_gas.f90:298-340, velocity smoothing, at least
+// twice the pixel spacing at the line, compute_nebular_kernels, opensps_initialization.f90:1377-1385), each from the
+// pixel nearest its centre until it falls to 1e-14 of its peak (nebular_setup :1725-1751), as CSR by pixel in line
+// order
+static int neb_profiles(OpsCtx* c, double sigma) {
+    if (c->neb_sigma == sigma && c->d_neb_lptr) return 0;
+    const int nspec = c->nspec, nl = c->nline;
+    const double* lam = c->h_lam.data();
+    const double pi = acos(-1.0), sq2pi = sqrt(2.0 * pi), kms = 1.0e13 / OPS_C_LIGHT;
+    std::vector<int> jlo(nl, 1), jhi(nl, 0);
+    std::vector<std::vector<double>> prof(nl);
+    std::vector<double> g(nspec);
+    for (int i = 0; i < nl; i++) {
+        const double pos = c->h_neb_lpos[i];
+        int jr = std::max(1, std::min(find_interval_ref(lam, nspec, pos), nspec - 1));
+        const double res = lam[jr] - lam[jr - 1];
+        double sig = pos * sigma * kms;
+        sig = std::max(sig, res * 2.0);
+        const double norm = (1.0 / (sq2pi * sig)) * (pos * pos / OPS_C_LIGHT);
+        for (int k = 0; k < nspec; k++) {
+            const double x = (lam[k] - pos) / sig;
+            g[k] = norm * exp(-0.5 * x * x);
+        }
+        const int jc = std::max(1, std::min(find_interval_ref(lam, nspec, pos), nspec));
+        const int jh = std::min(jc + 1, nspec);
+        const double cut = 1.0e-14 * std::max(g[jc - 1], g[jh - 1]);
+        if (cut <= 0.0) continue;
+        int lo = jc, hi = jh;
+        while (lo > 1 && g[lo - 2] > cut) lo--;
+        while (hi < nspec && g[hi] > cut) hi++;
+        jlo[i] = lo;
+        jhi[i] = hi;
+        prof[i].assign(g.begin() + (lo - 1), g.begin() + hi);
+    }
+    std::vector<int> lptr(nspec + 1, 0);
+    for (int i = 0; i < nl; i++)
+        for (int j = jlo[i]; j <= jhi[i]; j++) lptr[j]++;   // (pixel j - 1 counted at lptr[j])
+    for (int j = 0; j < nspec; j++) lptr[j + 1] += lptr[j];
+    const int tot = lptr[nspec];
+    std::vector<int> lid(std::max(1, tot)), fill(lptr.begin(), lptr.end() - 1);
+    std::vector<double> gval(std::max(1, tot));
+    for (int i = 0; i < nl; i++)
+        for (int j = jlo[i]; j <= jhi[i]; j++) {
+            const int p = fill[j - 1]++;
+            lid[p] = i;
+            gval[p] = prof[i][j - jlo[i]];
+        }
+    UPLOAD(c, c->d_neb_lptr, lptr.data(), lptr.size());
+    UPLOAD(c, c->d_neb_lid, lid.data(), lid.size());
+    UPLOAD(c, c->d_neb_gval, gval.data(), gval.size());
+    c->neb_nprof = tot;
+    c->neb_sigma = sigma;
+    return 0;
+}
+
 int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     if (!c) return 1;
     if (!a) return ops_fail(c, 3, "batch: NULL arguments");
@@ -954,6 +1131,18 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     const bool wz = a->ssp_weights_z != nullptr;
     if (wz && ops) return ops_fail(c, 2, "batch: ssp_weights_z with the SSP operator (the reference refuses it too)");
     if (wz && a->ssp_weights) return ops_fail(c, 3, "batch: ssp_weights and ssp_weights_z both given");
+    // nebular emission (the CLOUDY grids): with the resident bases and their SFH or given weights; not with the IMF
+    // per model or weights over metallicities (their nebular paths are not built), nor with lines outside the spectrum
+    // while dust emission is on (the lines' absorbed light in its energy balance is not built), nor from the fixed
+    // scratch (the profiles' tables are allocated)
+    const bool neb = c->add_neb != 0;
+    if (neb) {
+        if (c->nnz == 0) return ops_fail(c, 2, "batch: nebular emission without its tables (opensps_hip_set_nebular)");
+        if (ops || wz) return ops_fail(c, 2, "batch: nebular emission with the SSP operator or ssp_weights_z is not built");
+        if (!c->neb_lines_in_spec && c->add_duste)
+            return ops_fail(c, 2, "batch: nebular lines outside the spectrum with dust emission are not built");
+        if (c->arena) return ops_fail(c, 2, "batch_noalloc: nebular emission allocates its profiles (use opensps_hip_csp_batch)");
+    }
     int st;
     if (wz && (st = ensure_full(c, mixed, false))) return st;
     const bool uni = ops || wz;   // one A operand for every column (uniform column tiles)
@@ -1044,7 +1233,12 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     pa.logz = c->d_logz;
     pa.bmass = c->d_bmass;
     pa.blbol = c->d_blbol;
-    pa.w_out = ops ? (double*)c->wsfh.p : a->weights;
+    double* nebw = nullptr;
+    if (neb && !a->weights) {
+        if ((st = ops_reserve(c, c->nebw, sizeof(double) * (size_t)n * c->nt))) return st;
+        nebw = (double*)c->nebw.p;
+    }
+    pa.w_out = ops ? (double*)c->wsfh.p : (a->weights ? a->weights : nebw);
     pa.mass = a->mass;
     pa.lbol = a->lbol;
     pa.w64 = mixed ? nullptr : (double*)c->w64.p;
@@ -1286,6 +1480,70 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     HIPCHK(c, hipGetLastError());
     if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
 
+    // nebular emission into the age sums (before the emission stages and the chain, which read them)
+    if (neb) {
+        double sigma0 = 0.0;
+        if (c->neb_lines_in_spec) {
+            HIPCHK(c, hipMemcpyAsync(&sigma0, a->prm + (size_t)OPS_P_SIGMA_SMOOTH * n, sizeof(double),
+                                     hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            if ((st = neb_profiles(c, sigma0))) return st;
+        }
+        NebArgs na;
+        na.n = n;
+        na.nspec = c->nspec;
+        na.nsp = c->nsp;
+        na.nt = c->nt;
+        na.nz = c->nz;
+        na.parts = parts;
+        na.nt_neb = c->nt_neb;
+        na.whlylim = c->whlylim;
+        na.nline = c->nline;
+        na.nnz = c->nnz;
+        na.nna = c->nna;
+        na.nnu = c->nnu;
+        na.cont_on = c->neb_cont_on;
+        na.lines_in_spec = c->neb_lines_in_spec;
+        na.sigma0 = sigma0;
+        na.prm = a->prm;
+        na.hdr = (const double*)c->hdr.p;
+        na.w = a->weights ? a->weights : nebw;
+        na.zlo = d_zlo;
+        na.it = d_it;
+        na.basis = c->d_basis64;
+        na.q0 = c->d_neb_q0;
+        na.lz = c->d_neb_lz;
+        na.lu = c->d_neb_lu;
+        na.cont = c->d_neb_cont;
+        na.line = c->d_neb_line;
+        na.lpos = c->d_neb_lpos;
+        na.ia = c->d_neb_ia;
+        na.newr = c->d_neb_newr;
+        na.llo = c->d_neb_llo;
+        na.wa = c->d_neb_wa;
+        na.dwa = c->d_neb_dwa;
+        na.lptr = c->d_neb_lptr;
+        na.lid = c->d_neb_lid;
+        na.gval = c->d_neb_gval;
+        na.lam = c->d_lam;
+        na.ltab = c->d_ltab;
+        na.dcurve = c->dcurve_on ? c->d_dcurve : nullptr;
+        na.yo = c->yo.p;
+        na.emlines = a->emlines;
+        na.status = d_status;
+        na.flag = d_flag;
+        const size_t lds = sizeof(double) * (3 * (size_t)std::max(1, c->nt_neb) + 2 * (size_t)c->nline);
+        if (mixed)
+            k_neb<float><<<n, 256, lds, c->stream>>>(na);
+        else
+            k_neb<double><<<n, 256, lds, c->stream>>>(na);
+        HIPCHK(c, hipGetLastError());
+    } else if (a->emlines) {
+        k_fill_f64<<<(unsigned)(((size_t)n * std::max(1, c->nline) + 255) / 256), 256, 0, c->stream>>>(
+            a->emlines, (size_t)n * std::max(1, c->nline), 0.0);
+        HIPCHK(c, hipGetLastError());
+    }
+
     // dust emission and the AGN torus: their indices and the energy balance into the chain's header, the dust masses;
     // the chain with its emission stages (EM) when either can act
     const bool em = c->add_duste || c->add_agn;
@@ -1439,6 +1697,7 @@ int opensps_hip_csp_batch_grad(opensps_hip_ctx_t* c, const opensps_hip_grad_t* g
     if (ops && !c->op.on) return ops_fail(c, 1, "batch_grad: no SSP operator installed (opensps_hip_set_ssp_ops)");
     if (ops && f.ssp_weights_z) return ops_fail(c, 2, "batch_grad: ssp_weights_z with the SSP operator");
     if (c->dcurve_on) return ops_fail(c, 2, "batch_grad: the reverse pass with a tabulated dust curve is not built");
+    if (c->add_neb) return ops_fail(c, 2, "batch_grad: the reverse pass with nebular emission is not built");
     if (g->weights_z_bar && !f.ssp_weights_z) return ops_fail(c, 3, "batch_grad: weights_z_bar needs ssp_weights_z");
     if (n < 0) return ops_fail(c, 3, "batch_grad: n < 0");
     if (n == 0) return 0;

@@ -34,7 +34,7 @@ __device__ __forceinline__ int count_le(const double* __restrict__ a, int n, dou
 }
 
 // The reference's find_interval for either order with its end rules, 1-based (0 below, n above).
-__device__ __forceinline__ int find_interval_ref(const double* __restrict__ a, int n, double v) {
+__host__ __device__ __forceinline__ int find_interval_ref(const double* __restrict__ a, int n, double v) {
     if (n < 2) return 1;
     bool asc = a[n - 1] >= a[0];
     int lower = 0, upper = n + 1;

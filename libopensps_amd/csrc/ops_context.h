@@ -118,6 +118,23 @@ struct OpsCtx {
     double *d_agn_tau = nullptr, *d_agn = nullptr;                      // templates (nsp, ntau)
     double* d_wnu = nullptr;   // per pixel the frequency trapezoid's weight wt(i - 1) + wt(i), wt(i) = |c/l(i+1) - c/l(i)| / 2
 
+    // ---- nebular emission with the CLOUDY grids (set_nebular, set_neb_options) ----
+    int add_neb = 0, neb_cont_on = 1, neb_lines_in_spec = 1;
+    int nnz = 0, nna = 0, nnu = 0, nline = 0;      // the tables' axes (0: no tables)
+    int nt_neb = 0, whlylim = 0;                    // the nebular ages (1..nt_neb) and the pixels below 912 A
+    double *d_neb_lz = nullptr, *d_neb_lu = nullptr;             // gas log Z, log U axes
+    double *d_neb_cont = nullptr, *d_neb_line = nullptr;         // (nnu, nna, nnz, nspec) / (..., nline), log10
+    double *d_neb_lpos = nullptr;                                // the lines' wavelengths
+    int *d_neb_ia = nullptr, *d_neb_newr = nullptr, *d_neb_llo = nullptr;   // per age / per line
+    double *d_neb_wa = nullptr, *d_neb_dwa = nullptr;
+    double* d_neb_q0 = nullptr;                     // nz x nt_neb: ionizing photons of the basis spectra
+    std::vector<double> h_neb_lpos;
+    // the lines' profiles for one width (the first model's sigma_smooth): CSR by pixel
+    double neb_sigma = -1.0;
+    int neb_nprof = 0;
+    int *d_neb_lptr = nullptr, *d_neb_lid = nullptr;
+    double* d_neb_gval = nullptr;
+
     // ---- cosmology ----
     int ncosmo = 0;
     double *d_cz = nullptr, *d_cdl = nullptr;
@@ -134,6 +151,7 @@ struct OpsCtx {
     OpsArena* arena = nullptr;   // when set, ops_reserve carves from it and never allocates
     OpsBuffer yo, wcol, w64, order, tiles, misc, spec, flag;
     OpsBuffer hdr;   // per-model constants of the chain (OPS_HDR_N doubles per model)
+    OpsBuffer nebw;  // the SFH weights (n x nt) for the nebular stage when the caller does not ask for them
     OpsBuffer wsfh, opu, opwc, opsum, opseg;   // the operator path: SFH weights, point weights, age factors,
                                                // per-age sums, segment sums
     OpsBuffer opimf, opspart;                  // per-model IMF constants; the off-path bins' per-age sums and counts

@@ -1508,8 +1508,241 @@ __global__ __launch_bounds__(256) void k_emission(DustArgs a) {
             a.mdust[m] = sn > OPS_SAFE_FLOOR ? 3.21e-3 / (4.0 * 3.14159265358979323846) * (sa / sn) : OPS_SAFE_FLOOR;
     }
 }
+// ------------------------------------------------------------------------------------------------
+// Nebular emission with the CLOUDY grids
+// ------------------------------------------------------------------------------------------------
+#define OPS_H_PLANCK 6.62607015e-27   // erg s (opensps_constants.f90)
+#define OPS_L_SOL 3.828e33            // erg / s
+
+// k_neb_q0: grid (nt_neb, nz), a workgroup per (nebular age, metallicity): the ionizing photons of the basis spectrum,
+// the photon-number integral of L_nu / (h nu) below the Lyman limit by the frequency trapezoid over pixels
+// 1 .. whlylim (nebular_kernels step 1, opensps_csp_batch.f90:2680-2723), in FP64
+__global__ __launch_bounds__(256) void k_neb_q0(int nspec, int nt, int nt_neb, int whlylim, const double* lam,
+                                                const double* basis, double* q0) {
+    __shared__ double red[4];
+    const int t = blockIdx.x, iz = blockIdx.y, tid = threadIdx.x;
+    const double* b = basis + ((size_t)iz * nt + t) * nspec;
+    double s = 0.0;
+    for (int i = tid; i < whlylim - 1; i += blockDim.x) {
+        const double nu0 = OPS_C_LIGHT / lam[i], nu1 = OPS_C_LIGHT / lam[i + 1];
+        s += 0.5 * fabs(nu1 - nu0) * (b[i] / nu0 + b[i + 1] / nu1);
+    }
+    s = wave_sum(s);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        const double tot = (red[0] + red[1]) + (red[2] + red[3]);
+        q0[(size_t)iz * nt_neb + t] = whlylim >= 2 ? tot / OPS_H_PLANCK * OPS_L_SOL : 0.0;
+    }
+}
+
+// get_grid_indices_weights (opensps_gas.f90:394-412): the clamped interval (1-based) and weight on an ascending axis
+__device__ __forceinline__ void neb_index(const double* g, int ng, double v, int& idx, double& w) {
+    int i = find_interval_ref(g, ng, v);
+    i = max(1, min(i, ng - 1));
+    w = fmax(0.0, fmin((v - g[i - 1]) / (g[i] - g[i - 1]), 1.0));
+    idx = i;
+}
+
+// A table (rows x nnz x nna x nnu, column-major) at row r (0-based), CLOUDY age a (1-based), interpolated bilinearly in
+// log Z and log U (zu_point, :3942-3955)
+struct NebZU {
+    size_t sz, sa, su;
+    double c00, c01, c10, c11;
+    int iz, iu;
+    __device__ double at(const double* tab, int r, int a) const {
+        const double* p = tab + (size_t)r + (size_t)(iz - 1) * sz + (size_t)(a - 1) * sa + (size_t)(iu - 1) * su;
+        return c00 * p[0] + c01 * p[su] + c10 * p[sz] + c11 * p[sz + su];
+    }
+};
+
+// k_neb: a workgroup (256 threads) per model, after the age sums and before the emission stages and the chain.
+// (1) the model's gas intervals and, per nebular age, w Q (Q the blend of the bracketing metallicities' photons
+// times 1 - frac_obrun); (2) the EUV rows of the age sums redone with the escaping fraction clamp(frac_obrun) on the
+// nebular ages (they replace the product's rows); (3) the continuum: per pixel the sum over the nebular ages of
+// w Q 10^lg, lg interpolated in the gas parameters and linearly in log age, and within one CLOUDY age interval a
+// geometric progression (the reference's per-interval exponential and step ratio); (4) the lines' luminosities, the
+// same sums on the line tables; (5) the lines over their profiles (CSR by pixel, in line order); (6) emlines: the
+// luminosities through the birth cloud and the diffuse dust at the line's wavelength. FP64 sums; the age sums in
+// the path's precision. A thread takes pixels j = tid + 256 k in (2), (3) and (5), so each pixel's updates stay in
+// one thread's order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_neb(NebArgs a) {
+    extern __shared__ double nsh[];
+    __shared__ int s_iz, s_iu;
+    __shared__ double s_wz, s_wu;
+    const int m = blockIdx.x, tid = threadIdx.x, n = a.n, ntn = a.nt_neb, nl = a.nline, nt = a.nt;
+    double* cq = nsh;
+    double* wkv = nsh + ntn;
+    double* qv = nsh + 2 * ntn;
+    double* lum = nsh + 3 * ntn;
+    const double fob = a.prm[(size_t)OPS_P_FRAC_OBRUN * n + m];
+    if (tid == 0) {
+        int i;
+        double w;
+        neb_index(a.lz, a.nnz, a.prm[(size_t)OPS_P_GAS_LOGZ * n + m], i, w);
+        s_iz = i;
+        s_wz = w;
+        neb_index(a.lu, a.nnu, a.prm[(size_t)OPS_P_GAS_LOGU * n + m], i, w);
+        s_iu = i;
+        s_wu = w;
+        if (a.lines_in_spec && a.prm[(size_t)OPS_P_SIGMA_SMOOTH * n + m] != a.sigma0) {
+            atomicMax(a.flag, 2);   // one profile width a batch (opensps_csp_batch.f90:749-752)
+            if (a.status) a.status[m] = 2;
+        }
+    }
+    const int zlo = a.zlo[m], it = a.it[m];
+    const bool blend = a.nz > 1;
+    const double dz = blend ? a.hdr[(size_t)m * OPS_HDR_N + OPS_HDR_DZ] : 0.0;
+    const double fesc = fmin(fmax(fob, 0.0), 1.0), fabs_ = 1.0 - fob;
+    const double* wm = a.w + (size_t)m * nt;
+    for (int t = tid; t < ntn; t += blockDim.x) {
+        double q = blend ? (1.0 - dz) * a.q0[(size_t)(zlo - 1) * ntn + t] + dz * a.q0[(size_t)zlo * ntn + t] : a.q0[t];
+        q = fabs(q * fabs_);
+        const double wk = wm[t] > OPS_SAFE_FLOOR ? wm[t] : 0.0;
+        cq[t] = (wk != 0.0 && q > OPS_SAFE_FLOOR) ? wk * q : 0.0;
+        wkv[t] = wk;
+        qv[t] = q;
+    }
+    __syncthreads();
+    NebZU zu;
+    zu.iz = s_iz;
+    zu.iu = s_iu;
+    const double wz = s_wz, wu = s_wu;
+    zu.c00 = (1.0 - wz) * (1.0 - wu);
+    zu.c01 = (1.0 - wz) * wu;
+    zu.c10 = wz * (1.0 - wu);
+    zu.c11 = wz * wu;
+    T* y = (T*)a.yo + (size_t)m * a.nsp;
+    T* o = a.parts == 2 ? (T*)a.yo + ((size_t)n + m) * a.nsp : nullptr;
+    // (2) the EUV rows
+    const double* blo = a.basis + (size_t)(zlo - 1) * a.nspec * nt;
+    const double* bhi = blend ? a.basis + (size_t)zlo * a.nspec * nt : blo;
+    for (int i = tid; i < a.whlylim; i += blockDim.x) {
+        double sy = 0.0, so = 0.0;
+        for (int k = 0; k < nt; k++) {
+            const double wk = wm[k] > OPS_SAFE_FLOOR ? wm[k] : 0.0;
+            double bk = blend ? (1.0 - dz) * blo[(size_t)k * a.nspec + i] + dz * bhi[(size_t)k * a.nspec + i]
+                              : (1.0 - dz) * blo[(size_t)k * a.nspec + i];
+            if (k < ntn) bk = bk * fesc;
+            if (k + 1 <= it) sy = sy + wk * bk;
+            else so = so + wk * bk;
+        }
+        if (o) {
+            y[i] = (T)sy;
+            o[i] = (T)so;
+        } else {
+            y[i] = (T)(sy + so);
+        }
+    }
+    // (3) the continuum
+    if (a.cont_on) {
+        zu.sz = (size_t)a.nspec;
+        zu.sa = zu.sz * a.nnz;
+        zu.su = zu.sa * a.nna;
+        const double ln10 = 2.302585092994046;
+        for (int j = tid; j < a.nspec; j += blockDim.x) {
+            double ys = 0.0, os = 0.0, sy = 0.0, so = 0.0, ra = 0.0, rb = 0.0, dl = 0.0, f = 0.0, r = 1.0;
+            int aprev = -1;
+            for (int t = 0; t < ntn; t++) {
+                const int aa = a.ia[t];
+                if (aa != aprev) {
+                    ys = ys + sy;
+                    os = os + so;
+                    sy = 0.0;
+                    so = 0.0;
+                    ra = aa == aprev + 1 ? rb : zu.at(a.cont, j, aa);
+                    rb = zu.at(a.cont, j, aa + 1);
+                    aprev = aa;
+                    dl = (rb - ra) * ln10;
+                    f = exp(ra * ln10 + a.wa[t] * dl);
+                } else {
+                    if (a.newr[t]) r = exp(a.dwa[t] * dl);
+                    f = f * r;
+                }
+                if (t + 1 <= it) sy = sy + cq[t] * f;
+                else so = so + cq[t] * f;
+            }
+            ys = ys + sy;
+            os = os + so;
+            if (o) {
+                y[j] = (T)((double)y[j] + ys);
+                o[j] = (T)((double)o[j] + os);
+            } else {
+                y[j] = (T)((double)y[j] + (ys + os));
+            }
+        }
+    }
+    // (4) the lines' luminosities
+    zu.sz = (size_t)nl;
+    zu.sa = zu.sz * a.nnz;
+    zu.su = zu.sa * a.nna;
+    for (int i = tid; i < nl; i += blockDim.x) {
+        double sy = 0.0, so = 0.0;
+        for (int t = 0; t < ntn; t++) {
+            const double wk = wkv[t], q = qv[t];
+            if (wk == 0.0 || q <= OPS_SAFE_FLOOR) continue;
+            const int aa = a.ia[t];
+            const double lg = (1.0 - a.wa[t]) * zu.at(a.line, i, aa) + a.wa[t] * zu.at(a.line, i, aa + 1);
+            const double v = wk * (pow(10.0, lg) * q);
+            if (t + 1 <= it) sy = sy + v;
+            else so = so + v;
+        }
+        lum[i] = sy;
+        lum[nl + i] = so;
+    }
+    __syncthreads();
+    // (5) the profiles
+    if (a.lines_in_spec)
+        for (int j = tid; j < a.nspec; j += blockDim.x) {
+            const int p0 = a.lptr[j], p1 = a.lptr[j + 1];
+            if (p0 == p1) continue;
+            double sy = 0.0, so = 0.0;
+            for (int p = p0; p < p1; p++) {
+                const double g = a.gval[p];
+                const int i = a.lid[p];
+                sy = sy + g * lum[i];
+                so = so + g * lum[nl + i];
+            }
+            if (o) {
+                y[j] = (T)((double)y[j] + sy);
+                o[j] = (T)((double)o[j] + so);
+            } else {
+                y[j] = (T)((double)y[j] + (sy + so));
+            }
+        }
+    // (6) the lines through the dust
+    if (a.emlines) {
+        const double fnd = a.prm[(size_t)OPS_P_FRAC_NODUST * n + m], dust2 = a.prm[(size_t)OPS_P_DUST2 * n + m];
+        const double ed = a.prm[(size_t)OPS_P_DUST_INDEX * n + m], dust1 = a.prm[(size_t)OPS_P_DUST1 * n + m];
+        const double ed1 = a.prm[(size_t)OPS_P_DUST1_INDEX * n + m];
+        for (int i = tid; i < nl; i += blockDim.x) {
+            const double pos = a.lpos[i];
+            const int lo = a.llo[i];   // 1-based, 1 .. nspec - 1
+            double td = 1.0;
+            if (!(1.0 - fnd <= OPS_SAFE_FLOOR)) {
+                const double cv0 = a.dcurve ? a.dcurve[lo - 1] : (ed != 0.0 ? exp(ed * a.ltab[lo - 1]) : 1.0);
+                const double cv1 = a.dcurve ? a.dcurve[lo] : (ed != 0.0 ? exp(ed * a.ltab[lo]) : 1.0);
+                const double t0 = (double)(T)exp(-dust2 * cv0), t1 = (double)(T)exp(-dust2 * cv1);
+                td = t0 + (t1 - t0) / (a.lam[lo] - a.lam[lo - 1]) * (pos - a.lam[lo - 1]);
+            }
+            if (td != td) td = 1.0;
+            double birth = 1.0;
+            if (!(dust1 <= OPS_SAFE_FLOOR || 1.0 - fob <= OPS_SAFE_FLOOR)) birth = exp(-dust1 * pow(pos / 5500.0, ed1));
+            double ay = lum[i] * (birth * (1.0 - fob) + fob), ao = lum[nl + i];
+            if (1.0 - fnd > OPS_SAFE_FLOOR) {
+                ay = ay * (td * (1.0 - fnd) + fnd);
+                ao = ao * (td * (1.0 - fnd) + fnd);
+            }
+            a.emlines[(size_t)m * nl + i] = ay + ao;
+        }
+    }
+}
+
 template __global__ void k_emission<float>(DustArgs);
 template __global__ void k_emission<double>(DustArgs);
+template __global__ void k_neb<float>(NebArgs);
+template __global__ void k_neb<double>(NebArgs);
 
 // ONE: one spectrum buffer in LDS (nsp values): the smoothed values and the observed-frame quads wait in
 // registers for a barrier and then overwrite it (windows wider than OPS_CHAIN_KS pixels a thread go through

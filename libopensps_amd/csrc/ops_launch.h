@@ -283,6 +283,27 @@ struct DustArgs {
     int *status, *flag;    // per-model status and the batch's flag (as k_prep's)
 };
 
+// k_neb: per model the nebular emission with the CLOUDY grids (nebular_kernels steps 2-5, opensps_csp_batch.f90:
+// 2612-2998, line_profiles :3133-3160) into the age sums, and the lines' luminosities through the dust (line_attenuated
+// :3440-3473); k_neb_q0: step 1 once per grid and tables
+struct NebArgs {
+    int n, nspec, nsp, nt, nz, parts, nt_neb, whlylim, nline, nnz, nna, nnu, cont_on, lines_in_spec;
+    double sigma0;                 // the profiles' width (the first model's sigma_smooth)
+    const double *prm, *hdr, *w;   // parameters, the chain's header (dz), the SFH weights (n x nt)
+    const int *zlo, *it;
+    const double *basis;           // nspec x nt x nz FP64
+    const double *q0;              // nz x nt_neb
+    const double *lz, *lu, *cont, *line, *lpos;
+    const int *ia, *newr, *llo;
+    const double *wa, *dwa;
+    const int *lptr, *lid;         // the profiles (CSR by pixel)
+    const double* gval;
+    const double *lam, *ltab, *dcurve;
+    void* yo;
+    double* emlines;               // (n, nline) or NULL
+    int *status, *flag;
+};
+
 // ---- the reverse pass (ops_grad.hip) ----
 #define OPENSPS_HIP_NBGRAD_ROWS 44   // params_bar rows (the reference's NBGRAD, opensps_csp_batch.f90:213)
 #define OPS_GOUT_N 16   // per model: the reverse chain's scalars (k_grad_chain -> k_grad_out)
@@ -376,6 +397,9 @@ __global__ void k_op_sums(OpSArgs a);
 __global__ void k_op_imf_grad(OpAdjArgs a);
 template <typename T>
 __global__ void k_emission(DustArgs a);
+template <typename T>
+__global__ void k_neb(NebArgs a);
+__global__ void k_neb_q0(int nspec, int nt, int nt_neb, int whlylim, const double* lam, const double* basis, double* q0);
 template <typename T>
 __global__ void k_op_adj_t(int n, int ntpl, int ldc, const int* order, const T* C, T* CT);
 template <typename T>

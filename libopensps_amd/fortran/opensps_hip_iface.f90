@@ -14,7 +14,7 @@ module opensps_hip_iface
         type(c_ptr) :: prm = c_null_ptr, ssp_weights = c_null_ptr, mags = c_null_ptr, spec = c_null_ptr
         type(c_ptr) :: mass = c_null_ptr, lbol = c_null_ptr, fband = c_null_ptr, young = c_null_ptr, old = c_null_ptr
         type(c_ptr) :: idx = c_null_ptr, weights = c_null_ptr, op_count = c_null_ptr, ssp_weights_z = c_null_ptr
-        type(c_ptr) :: zred_mags = c_null_ptr, mdust = c_null_ptr
+        type(c_ptr) :: zred_mags = c_null_ptr, mdust = c_null_ptr, emlines = c_null_ptr
     end type hip_batch_t
 
     !> opensps_hip_grad_t
@@ -37,7 +37,7 @@ module opensps_hip_iface
               opensps_hip_set_mag_mask, opensps_hip_set_vega, opensps_hip_set_sfh_tab, opensps_hip_set_dust_curve, opensps_hip_set_cosmology, opensps_hip_set_options, opensps_hip_csp_batch, &
               opensps_hip_sync, opensps_hip_malloc, opensps_hip_free, opensps_hip_memcpy_h2d, opensps_hip_memcpy_d2h, &
               opensps_hip_last_error, opensps_hip_device_count, opensps_hip_set_profiling, opensps_hip_kernel_times, &
-              opensps_hip_op_kernel_times, &
+              opensps_hip_op_kernel_times, opensps_hip_set_nebular, opensps_hip_set_neb_options, &
               opensps_hip_csp_batch_grad, opensps_hip_set_emission, opensps_hip_set_dust_emission, opensps_hip_set_agn
 
     interface
@@ -197,6 +197,21 @@ module opensps_hip_iface
             real(c_double), intent(out) :: ms(2)
             integer(c_int), intent(out) :: launches(2)
             integer(c_int), value :: reset
+        end function
+
+        integer(c_int) function opensps_hip_set_nebular(ctx, nnz, logz, nna, age, nnu, logu, nlam, lam, cont, nline, &
+                                                        line_lam, lines) bind(C, name="opensps_hip_set_nebular")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nnz, nna, nnu, nlam, nline
+            real(c_double), intent(in) :: logz(*), age(*), logu(*), lam(*), cont(*), line_lam(*), lines(*)
+        end function
+
+        integer(c_int) function opensps_hip_set_neb_options(ctx, add_neb_emission, add_neb_continuum, nebemlineinspec) &
+            bind(C, name="opensps_hip_set_neb_options")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: add_neb_emission, add_neb_continuum, nebemlineinspec
         end function
 
         integer(c_int) function opensps_hip_set_ssp_ops(ctx, ops) bind(C, name="opensps_hip_set_ssp_ops")

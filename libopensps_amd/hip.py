@@ -25,7 +25,8 @@ class BatchArgs(C.Structure):
                 ("ssp_weights", C.c_void_p), ("mags", C.c_void_p), ("spec", C.c_void_p), ("mass", C.c_void_p),
                 ("lbol", C.c_void_p), ("fband", C.c_void_p), ("young", C.c_void_p), ("old", C.c_void_p),
                 ("idx", C.c_void_p), ("weights", C.c_void_p), ("op_count", C.c_void_p),
-                ("ssp_weights_z", C.c_void_p), ("zred_mags", C.c_void_p), ("mdust", C.c_void_p)]
+                ("ssp_weights_z", C.c_void_p), ("zred_mags", C.c_void_p), ("mdust", C.c_void_p),
+                ("emlines", C.c_void_p)]
 
 
 class GradArgs(C.Structure):
@@ -81,6 +82,8 @@ def lib():
         L.opensps_hip_set_emission.argtypes = [vp, i, i]
         L.opensps_hip_set_dust_emission.argtypes = [vp, i, vp, i, vp, vp]
         L.opensps_hip_set_agn.argtypes = [vp, i, vp, vp]
+        L.opensps_hip_set_nebular.argtypes = [vp, i, vp, i, vp, i, vp, i, vp, vp, i, vp, vp]
+        L.opensps_hip_set_neb_options.argtypes = [vp, i, i, i]
         _lib = L
     return _lib
 
@@ -200,6 +203,27 @@ class DeviceContext:
         assert a.shape == (t.size, self.grid.nspec)
         self._check(lib().opensps_hip_set_agn(self._ctx, t.size, t.ctypes.data, a.ctypes.data))
 
+    def set_nebular(self, library):
+        """The CLOUDY library (logz, age (yr), logu, wavelength (nlam), continuum (nnz, nna, nnu, nlam) L_sun/Hz per unit
+        Q, line wavelengths (nline), lines (nnz, nna, nnu, nline) L_sun per unit Q: datalib.nebular's tuple, or
+        synth.nebular_library's), placed on the grid by the library (opensps_hip_set_nebular); None removes it."""
+        if library is None:
+            self._check(lib().opensps_hip_set_nebular(self._ctx, 0, None, 0, None, 0, None, 0, None, None, 0, None,
+                                                      None))
+            self.nline = 0
+            return
+        lz, age, lu, lam, cont, llam, lines = (_host(x) for x in library)
+        assert cont.shape == (lz.size, age.size, lu.size, lam.size)
+        assert lines.shape == (lz.size, age.size, lu.size, llam.size)
+        self._check(lib().opensps_hip_set_nebular(self._ctx, lz.size, lz.ctypes.data, age.size, age.ctypes.data,
+                                                  lu.size, lu.ctypes.data, lam.size, lam.ctypes.data,
+                                                  cont.ctypes.data, llam.size, llam.ctypes.data, lines.ctypes.data))
+        self.nline = llam.size
+
+    def set_neb_options(self, add_neb_emission=1, add_neb_continuum=1, nebemlineinspec=1):
+        self._check(lib().opensps_hip_set_neb_options(self._ctx, int(add_neb_emission), int(add_neb_continuum),
+                                                      int(nebemlineinspec)))
+
     def free_ssp_ops(self):
         self._check(lib().opensps_hip_free_ssp_ops(self._ctx))
 
@@ -216,7 +240,8 @@ class DeviceContext:
         shapes = {"spec": ((n, g.nspec), sdt), "mass": ((n,), torch.float64), "lbol": ((n,), torch.float64),
                   "fband": ((n, self.nbands), torch.float64), "young": ((n, g.nspec), sdt),
                   "old": ((n, g.nspec), sdt), "idx": ((n, 3), torch.int32), "weights": ((n, g.nt), torch.float64),
-                  "op_count": ((n, 2), torch.int32), "mdust": ((n,), torch.float64)}
+                  "op_count": ((n, 2), torch.int32), "mdust": ((n,), torch.float64),
+                  "emlines": ((n, max(1, getattr(self, "nline", 0))), torch.float64)}
         for k in want:
             if k in shapes:
                 out[k] = torch.empty(shapes[k][0], dtype=shapes[k][1], device=dev)
@@ -229,7 +254,8 @@ class DeviceContext:
         isz = 4 if precision == MIXED else 8
         sizes = {"mags": (n * self.nbands, 8), "spec": (n * g.nspec, isz), "young": (n * g.nspec, isz),
                  "old": (n * g.nspec, isz), "mass": (n, 8), "lbol": (n, 8), "fband": (n * self.nbands, 8),
-                 "idx": (3 * n, 4), "weights": (n * g.nt, 8), "op_count": (2 * n, 4), "mdust": (n, 8)}
+                 "idx": (3 * n, 4), "weights": (n * g.nt, 8), "op_count": (2 * n, 4), "mdust": (n, 8),
+                 "emlines": (n * max(1, getattr(self, "nline", 0)), 8)}
         for k, (numel, itemsize) in sizes.items():
             t = out.get(k)
             if t is not None and (not t.is_contiguous() or t.element_size() != itemsize or t.numel() < numel):
@@ -249,7 +275,8 @@ class DeviceContext:
         a.ssp_weights = ssp_weights.data_ptr() if ssp_weights is not None else None
         a.ssp_weights_z = ssp_weights_z.data_ptr() if ssp_weights_z is not None else None
         a.zred_mags = zred_mags.data_ptr() if zred_mags is not None else None
-        for k in ("mags", "spec", "mass", "lbol", "fband", "young", "old", "idx", "weights", "op_count", "mdust"):
+        for k in ("mags", "spec", "mass", "lbol", "fband", "young", "old", "idx", "weights", "op_count", "mdust",
+                  "emlines"):
             setattr(a, k, out[k].data_ptr() if k in out else None)
         return a
 
