@@ -102,6 +102,18 @@ void opensps_context_set_reference_spectra(int handle, int n_vega, const double 
 void opensps_context_set_dust_emission_templates(int handle, int nqpah, const double *qpah, int numin,
                                                  const double *umin, int nlam, const double *lam,
                                                  const double *spectra, int *status);
+/* No reference counterpart (the reference reads its CLOUDY library at set-up, read_nebular_standard,
+ * src/io/opensps_io.f90:914-937, and places it on the grid, load_nebular_grid, :749-823): the tables: logz (nnz),
+ * age (nna, yr), logu (nnu), lam (nlam, Angstrom) ascending; cont nnz x nna x nnu x nlam (L_sun/Hz) and lines
+ * nnz x nna x nnu x nline (L_sun), per unit ionizing photon rate, C order; line_lam (nline, Angstrom). With
+ * add_neb_emission = 1 each model adds the nebular continuum (add_neb_continuum = 1) and lines (into the spectrum with
+ * nebemlineinspec = 1) of its young stars' ionizing photons at its gas_logz, gas_logu, escaping frac_obrun of them,
+ * and emlines (get_nemline = nline rows) holds the lines' attenuated luminosities. Without the tables such a batch is
+ * status 2, and so is add_neb_emission = 2 (nebular emission in the SSPs). status 1 bad handle, 3 bad sizes, NULL
+ * arrays or axes not ascending. */
+void opensps_context_set_nebular_tables(int handle, int nnz, const double *logz, int nna, const double *age, int nnu,
+                                        const double *logu, int nlam, const double *lam, const double *cont, int nline,
+                                        const double *line_lam, const double *lines, int *status);
 /* No reference counterpart (the reference reads its AGN torus library at set-up, load_agn_dust_models,
  * src/io/opensps_io.f90:1101-1155): the templates: tau (ntau) and lam (nlam, Angstrom) ascending, spectra nlam x ntau
  * column-major (the data library's [tau][wavelength]). With add_agn_dust = 1 each model with fagn > 0 adds fagn L_bol
@@ -133,13 +145,14 @@ void opensps_context_get_cosmology(int handle, int *nrow, double *z, double *dl,
 /* status 0, 1 bad handle / no basis, 2 unsupported (a parameter that changes the SSPs; imf1, imf2, imf3,
  * imf_lower_limit, imf_upper_limit without an SSP operator on the context, and vdmc, mdave per model; spectral
  * indices, nebular emission, dust emission or the AGN torus without their templates
- * (opensps_context_set_dust_emission_templates, _set_agn_templates), Vega magnitudes without reference spectra,
+ * (opensps_context_set_nebular_tables, _set_dust_emission_templates, _set_agn_templates), nebular emission in the
+ * SSPs (add_neb_emission = 2) or with an SSP operator, Vega magnitudes without reference spectra,
  * dust_type other than 0, 2, 6, an SFH type other than 0 to 5), 3 unknown parameter or bad shape, 5 no HIP device,
  * 10 device error. Dust emission (add_dust_emission = 1, the default: models with dust1 or dust2 > 0; mdust out) and
- * the AGN torus (add_agn_dust = 1: models with fagn > 0) run with their templates. Members of the reference's
- * parameter set that no stage here reads (pmetals, dust_clumps, uvb, mwr, gas_*) are accepted. With an SSP operator
- * on the context
- * (opensps_context_set_ssp_ops) every batch takes each model's IMF from it, as the reference's does
+ * the AGN torus (add_agn_dust = 1: models with fagn > 0) and nebular emission (add_neb_emission = 1: gas_logz,
+ * gas_logu, frac_obrun per model; emlines out) run with their tables. Members of the reference's parameter set that
+ * no stage here reads (pmetals, dust_clumps, uvb, mwr, gas_logn, gas_logno, gas_logco) are accepted. With an SSP
+ * operator on the context (opensps_context_set_ssp_ops) every batch takes each model's IMF from it, as the reference's does
  * (src/bindings/c/opensps_c_api.f90:913-917). */
 void opensps_context_compute_csp_batch(int handle, int n, int n_params, const char *const *names,
                                        const double *const *values, const double *ssp_weights, double *mags,

@@ -45,6 +45,7 @@ def lib():
         L.opensps_context_get_dims.argtypes = [i, ip, ip, ip]
         L.opensps_context_get_nbands.argtypes = [i, ip, ip]
         L.opensps_context_get_nz.argtypes = [i, ip, ip]
+        L.opensps_context_get_nemline.argtypes = [i, ip, ip]
         L.opensps_context_get_lambda.argtypes = [i, vp, ip]
         L.opensps_context_get_bands.argtypes = [i, vp, ip]
         L.opensps_context_get_cosmology.argtypes = [i, ip, vp, vp, ip]
@@ -74,6 +75,7 @@ def lib():
         L.opensps_context_set_ssp_ops.argtypes = [i, i, i, i, i, i] + [vp] * 12 + [ip]
         L.opensps_context_set_dust_emission_templates.argtypes = [i, i, vp, i, vp, i, vp, vp, ip]
         L.opensps_context_set_agn_templates.argtypes = [i, i, vp, i, vp, vp, ip]
+        L.opensps_context_set_nebular_tables.argtypes = [i, i, vp, i, vp, i, vp, i, vp, vp, i, vp, vp, ip]
         L.opensps_context_free_ssp_ops.argtypes = [i, ip]
         L.opensps_context_has_ssp_ops.argtypes = [i, ip, ip]
         L.opensps_context_evaluate_ssp_op.argtypes = [i, i, vp, vp, vp, vp, ip]
@@ -289,6 +291,26 @@ class Context:
                                                 sp.ctypes.data, C.byref(st))
         _check(st)
 
+    def set_nebular_tables(self, logz, age, logu, lam, cont, line_lam, lines):
+        """opensps_context_set_nebular_tables: the CLOUDY library as datalib.nebular returns it: axes logz, age (yr),
+        logu, the continuum's wavelengths lam, cont (nnz, nna, nnu, nlam), the lines' wavelengths line_lam and lines
+        (nnz, nna, nnu, nline)."""
+        z, a, u, lm, co, ll, li = (_d(x) for x in (logz, age, logu, lam, cont, line_lam, lines))
+        assert co.shape == (z.size, a.size, u.size, lm.size) and li.shape == (z.size, a.size, u.size, ll.size)
+        st = C.c_int()
+        lib().opensps_context_set_nebular_tables(self.handle, z.size, z.ctypes.data, a.size, a.ctypes.data, u.size,
+                                                 u.ctypes.data, lm.size, lm.ctypes.data, co.ctypes.data, ll.size,
+                                                 ll.ctypes.data, li.ctypes.data, C.byref(st))
+        _check(st)
+
+    @property
+    def nemline(self):
+        """opensps_context_get_nemline: the rows of emlines (the nebular library's lines, else 166)."""
+        n, st = C.c_int(), C.c_int()
+        lib().opensps_context_get_nemline(self.handle, C.byref(n), C.byref(st))
+        _check(st)
+        return n.value
+
     def set_ssp_ops(self, op):
         """Installs a caller-built operator (a synth.SSPOperator or anything with its fields) on the GPU."""
         ints = {k: np.ascontiguousarray(getattr(op, k), dtype=np.int32) for k in ("age_ptr", "first", "ent_tpl")}
@@ -335,7 +357,7 @@ class Context:
         cvals = (C.c_void_p * max(len(names), 1))(*[a.ctypes.data for a in keep])
         out = {"mags": np.zeros((n, self.nbands))}
         shapes = {"spec": (n, self.nspec), "mass": (n,), "lbol": (n,), "mdust": (n,), "sfr": (n,),
-                  "emlines": (n, 166), "young": (n, self.nspec), "old": (n, self.nspec), "mass_young": (n,),
+                  "emlines": (n, self.nemline), "young": (n, self.nspec), "old": (n, self.nspec), "mass_young": (n,),
                   "mass_old": (n,), "lbol_young": (n,), "lbol_old": (n,)}
         for k in want:
             if k in shapes:

@@ -248,6 +248,7 @@ contains
         integer(c_int), intent(out) :: n, status
         n = 166
         if (bad_handle(handle, status)) return
+        n = nemline(ctxs(handle))
     end subroutine
 
     subroutine opensps_context_get_zsol(handle, z, status) bind(C, name="opensps_context_get_zsol")
@@ -807,6 +808,40 @@ contains
         call c_f_pointer(lam, l, [nlam])
         call c_f_pointer(spectra, sp, [int(nlam), 2 * int(numin), int(nqpah)])
         call ctx_set_dust_emission(int(handle), int(nqpah), q, int(numin), u, int(nlam), l, sp, st)
+        status = st
+    end subroutine
+
+    !> No reference counterpart (the reference reads its CLOUDY library at set-up, read_nebular_standard,
+    !> src/io/opensps_io.f90:914-937, and places it on the grid, load_nebular_grid, :749-823): the tables as given:
+    !> logz (nnz), age (nna, yr), logu (nnu), lam (nlam, A) ascending; cont (nnz x nna x nnu x nlam, L_sun/Hz) and lines
+    !> (nnz x nna x nnu x nline, L_sun), per unit ionizing photon rate, C order; line_lam (nline, A). With
+    !> add_neb_emission = 1 every model adds the nebular continuum (add_neb_continuum = 1) and lines (into the
+    !> spectrum with nebemlineinspec = 1) of its young stars' ionizing photons at its gas_logz, gas_logu, and emlines
+    !> holds the lines' luminosities. status 1 bad handle, 3 bad sizes, NULL arrays or axes not ascending.
+    subroutine opensps_context_set_nebular_tables(handle, nnz, logz, nna, age, nnu, logu, nlam, lam, cont, nline, &
+                                                  line_lam, lines, status) bind(C, name="opensps_context_set_nebular_tables")
+        integer(c_int), value :: handle, nnz, nna, nnu, nlam, nline
+        type(c_ptr), value :: logz, age, logu, lam, cont, line_lam, lines
+        integer(c_int), intent(out) :: status
+        real(c_double), pointer :: z(:), a(:), u(:), l(:), co(:), ll(:), li(:)
+        integer :: st
+        if (bad_handle(handle, status)) return
+        if (nnz < 2 .or. nna < 2 .or. nnu < 2 .or. nlam < 2 .or. nline < 1 .or. .not. c_associated(logz) .or. &
+            .not. c_associated(age) .or. .not. c_associated(logu) .or. .not. c_associated(lam) .or. &
+            .not. c_associated(cont) .or. .not. c_associated(line_lam) .or. .not. c_associated(lines)) then
+            status = 3
+            call set_error(3, 'set_nebular_tables: bad sizes or NULL arrays')
+            return
+        end if
+        call c_f_pointer(logz, z, [nnz])
+        call c_f_pointer(age, a, [nna])
+        call c_f_pointer(logu, u, [nnu])
+        call c_f_pointer(lam, l, [nlam])
+        call c_f_pointer(cont, co, [int(nnz) * int(nna) * int(nnu) * int(nlam)])
+        call c_f_pointer(line_lam, ll, [nline])
+        call c_f_pointer(lines, li, [int(nnz) * int(nna) * int(nnu) * int(nline)])
+        call ctx_set_nebular(int(handle), int(nnz), z, int(nna), a, int(nnu), u, int(nlam), l, co, int(nline), ll, &
+                             li, st)
         status = st
     end subroutine
 

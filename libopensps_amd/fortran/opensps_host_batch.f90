@@ -225,9 +225,16 @@ contains
         else if (c%has_ops .and. c%imf_type /= 0 .and. c%imf_type /= 2 .and. c%imf_type /= 4) then
             st = 2
             call set_error(2, 'batch: the SSP operator integrates imf_type 0 (Salpeter), 2 (Kroupa) and 4 (Dave)')
-        else if (c%add_neb_emission == 1) then
+        else if (c%add_neb_emission == 2) then
             st = 2
-            call set_error(2, 'batch: nebular emission is not part of this build')
+            call set_error(2, 'batch: nebular emission in the SSPs (add_neb_emission = 2) is not part of this build')
+        else if (c%add_neb_emission == 1 .and. .not. c%has_neb) then
+            st = 2
+            call set_error(2, 'batch: nebular emission (add_neb_emission = 1) needs its tables ' // &
+                           '(opensps_context_set_nebular_tables)')
+        else if (gr .and. c%add_neb_emission == 1) then
+            st = 2
+            call set_error(2, 'batch_grad: the reverse pass of the nebular emission is not part of this build')
         else if (c%compute_vega_mags == 1 .and. .not. allocated(c%magvega)) then
             st = 2
             call set_error(2, 'batch: Vega magnitudes need the Vega spectrum (opensps_context_set_reference_spectra)')
@@ -323,7 +330,7 @@ contains
         real(c_float), allocatable, target :: fbuf(:, :)
         integer(c_int), allocatable, target :: idx(:, :)
         integer :: i, k, m, st, nb, ns, nt, bytes, it
-        logical :: parts, split, want_w
+        logical :: parts, split, want_w, neb
         character(len=:), allocatable :: nm
         type(hip_batch_t) :: a
         integer(c_int) :: rc
@@ -399,6 +406,8 @@ contains
             st = st + dbuf_reserve(c%d_old, int(bytes, c_size_t) * ns * n)
         end if
         if (c_associated(c_mdust)) st = st + dbuf_reserve(c%d_mdust, int(8, c_size_t) * n)
+        neb = c%add_neb_emission == 1 .and. c_associated(c_emlines)
+        if (neb) st = st + dbuf_reserve(c%d_eml, int(8, c_size_t) * c%neb_nline * n)
         if (st /= 0) then
             status = 10
             call set_error(10, 'batch: device allocation failed')
@@ -435,6 +444,8 @@ contains
         if (want_w .and. .not. c_associated(c_weights)) a%weights = c%d_w%p
         a%mdust = c_null_ptr
         if (c_associated(c_mdust)) a%mdust = c%d_mdust%p
+        a%emlines = c_null_ptr
+        if (neb) a%emlines = c%d_eml%p
         rc = opensps_hip_csp_batch(c%dctx, a)
         if (rc == 0) rc = opensps_hip_sync(c%dctx)
         if (rc /= 0) then
@@ -454,8 +465,10 @@ contains
             call copy_spectra(c_old, c%d_old%p)
         end if
         if (c_associated(c_mdust)) rc = opensps_hip_memcpy_d2h(c_mdust, c%d_mdust%p, int(8, c_size_t) * n)
-        if (c_associated(c_emlines)) then
-            call c_f_pointer(c_emlines, outm, [166, n])
+        if (neb) then
+            rc = opensps_hip_memcpy_d2h(c_emlines, c%d_eml%p, int(8, c_size_t) * c%neb_nline * n)
+        else if (c_associated(c_emlines)) then
+            call c_f_pointer(c_emlines, outm, [nemline(c), n])
             outm = 0.0_wp
         end if
         if (c_associated(c_sfr)) then

@@ -36,6 +36,13 @@ module opensps_host_context
         logical :: has_dustem = .false., has_agn = .false.
         real(WP), allocatable :: dl_qpah(:), dl_umin(:), dl_lam(:), dl_spec(:, :, :)   ! (nlam, 2 numin, nqpah)
         real(WP), allocatable :: agn_tau(:), agn_lam(:), agn_spec(:, :)                 ! (nlam, ntau)
+        ! the CLOUDY nebular library as given (opensps_context_set_nebular_tables): its axes, the continuum's
+        ! wavelengths, the continuum (nnz, nna, nnu, nlam) and the lines (nnz, nna, nnu, nline) flat in C order, the
+        ! lines' wavelengths; placed on the grid by the device at upload. add_neb_continuum, nebemlineinspec: the
+        ! reference's settings of the same names (defaults 1, 1)
+        logical :: has_neb = .false.
+        integer :: add_neb_continuum = 1, nebemlineinspec = 1, neb_nline = 0
+        real(WP), allocatable :: nb_logz(:), nb_age(:), nb_logu(:), nb_lam(:), nb_cont(:), nb_llam(:), nb_lines(:)
         ! the resident basis
         logical :: has_basis = .false.
         integer :: nspec = 0, nt = 0, nz = 0
@@ -61,7 +68,8 @@ module opensps_host_context
         type(c_ptr) :: dctx = c_null_ptr
         logical :: dev_grid = .false., dev_bands = .false., dev_cosmo = .false., dev_mask = .false.
         logical :: dev_vega = .false., dev_tab = .false., dev_dust = .false., dev_imf = .false., dev_emis = .false.
-        type(dbuf_t) :: d_prm, d_w, d_mags, d_spec, d_mass, d_lbol, d_fband, d_young, d_old, d_idx, d_mdust
+        logical :: dev_neb = .false.
+        type(dbuf_t) :: d_prm, d_w, d_mags, d_spec, d_mass, d_lbol, d_fband, d_young, d_old, d_idx, d_mdust, d_eml
         integer :: last_n = 0, last_bytes = 8
         real(WP), allocatable :: last_maggies(:, :)
     end type host_ctx_t
@@ -70,7 +78,8 @@ module opensps_host_context
     type(host_ctx_t), save :: blank_ctx   ! (default initialised: the state of a new context)
 
     public :: ctx_create, ctx_destroy, ctx_valid, ctx_set_int, ctx_set_float, ctx_set_str, ctx_set_basis, &
-              ctx_device, dbuf_reserve, ctx_refresh_cosmology, hip_message, ctx_set_dust_emission, ctx_set_agn
+              ctx_device, dbuf_reserve, ctx_refresh_cosmology, hip_message, ctx_set_dust_emission, ctx_set_agn, &
+              ctx_set_nebular, nemline
 
 contains
 
@@ -118,7 +127,7 @@ contains
         associate(c => ctxs(h))
             call dbuf_free(c%d_prm); call dbuf_free(c%d_w); call dbuf_free(c%d_mags); call dbuf_free(c%d_spec)
             call dbuf_free(c%d_mass); call dbuf_free(c%d_lbol); call dbuf_free(c%d_fband); call dbuf_free(c%d_young)
-            call dbuf_free(c%d_old); call dbuf_free(c%d_idx); call dbuf_free(c%d_mdust)
+            call dbuf_free(c%d_old); call dbuf_free(c%d_idx); call dbuf_free(c%d_mdust); call dbuf_free(c%d_eml)
             if (c_associated(c%dctx)) st = opensps_hip_destroy(c%dctx)
         end associate
         ctxs(h) = blank_ctx
@@ -155,7 +164,9 @@ contains
         select case (lower(trim(key)))
         case ('add_igm_absorption'); c%add_igm_absorption = v; c%dev_grid = .false.
         case ('add_dust_emission'); c%add_dust_emission = v; c%dev_emis = .false.
-        case ('add_neb_emission'); c%add_neb_emission = v
+        case ('add_neb_emission'); c%add_neb_emission = v; c%dev_neb = .false.
+        case ('add_neb_continuum'); c%add_neb_continuum = v; c%dev_neb = .false.
+        case ('nebemlineinspec'); c%nebemlineinspec = v; c%dev_neb = .false.
         case ('add_agn_dust'); c%add_agn_dust = v; c%dev_emis = .false.
         case ('smooth_velocity'); c%smooth_velocity = v; c%dev_grid = .false.
         case ('smoothspec_fast'); c%smoothspec_fast = v; c%dev_grid = .false.
@@ -377,7 +388,7 @@ contains
             end if
             c%dev_grid = .false.; c%dev_bands = .false.; c%dev_cosmo = .false.; c%dev_mask = .false.
             c%dev_prof = .false.; c%dev_vega = .false.; c%dev_tab = .false.; c%dev_dust = .false.
-            c%dev_imf = .false.; c%dev_emis = .false.
+            c%dev_imf = .false.; c%dev_emis = .false.; c%dev_neb = .false.
         end if
         if (.not. c%dev_grid) then
             rc = opensps_hip_set_grid(c%dctx, int(c%nspec, c_int), c%lam, int(c%nt, c_int), c%log_age, &
@@ -390,6 +401,7 @@ contains
             c%dev_bands = .false.
             c%dev_dust = .false.
             c%dev_emis = .false.
+            c%dev_neb = .false.
         end if
         if (.not. c%dev_bands) then
             if (c%nbands > 0) then
@@ -427,6 +439,22 @@ contains
             st = emission_upload(c)
             if (st /= 0) return
             c%dev_emis = .true.
+        end if
+        if (.not. c%dev_neb) then
+            if (c%has_neb) then
+                rc = opensps_hip_set_nebular(c%dctx, int(size(c%nb_logz), c_int), c%nb_logz, &
+                                             int(size(c%nb_age), c_int), c%nb_age, int(size(c%nb_logu), c_int), &
+                                             c%nb_logu, int(size(c%nb_lam), c_int), c%nb_lam, c%nb_cont, &
+                                             int(c%neb_nline, c_int), c%nb_llam, c%nb_lines)
+            else
+                rc = opensps_hip_set_nebular(c%dctx, 0_c_int, [0.0_wp], 0_c_int, [0.0_wp], 0_c_int, [0.0_wp], &
+                                             0_c_int, [0.0_wp], [0.0_wp], 0_c_int, [0.0_wp], [0.0_wp])
+            end if
+            if (rc == 0) rc = opensps_hip_set_neb_options(c%dctx, int(merge(1, 0, c%add_neb_emission == 1), c_int), &
+                                                          int(c%add_neb_continuum, c_int), &
+                                                          int(c%nebemlineinspec, c_int))
+            if (rc /= 0) goto 900
+            c%dev_neb = .true.
         end if
         if (.not. c%dev_tab) then
             if (c%ntab > 0) then
@@ -570,6 +598,53 @@ contains
             c%dev_emis = .false.
         end associate
     end subroutine ctx_set_agn
+
+    !> The CLOUDY nebular library as read_nebular_standard reads it (src/io/opensps_io.f90:914-937): axes logz (nnz),
+    !> age (nna, yr), logu (nnu) and the continuum's wavelengths lam (nlam, A), all ascending; cont (nnz, nna, nnu,
+    !> nlam) in L_sun/Hz and lines (nnz, nna, nnu, nline) in L_sun, per unit ionizing photon rate, flat in C order;
+    !> the lines' wavelengths line_lam (nline, A). Status 3 for bad sizes or axes not ascending.
+    subroutine ctx_set_nebular(h, nnz, logz, nna, age, nnu, logu, nlam, lam, cont, nline, line_lam, lines, status)
+        integer, intent(in) :: h, nnz, nna, nnu, nlam, nline
+        real(WP), intent(in) :: logz(nnz), age(nna), logu(nnu), lam(nlam), cont(nnz * nna * nnu * nlam)
+        real(WP), intent(in) :: line_lam(nline), lines(nnz * nna * nnu * nline)
+        integer, intent(out) :: status
+        status = 0
+        if (nnz < 2 .or. nna < 2 .or. nnu < 2 .or. nlam < 2 .or. nline < 1) then
+            status = 3
+            call set_error(3, 'set_nebular_tables: at least 2 gas metallicities, ages, ionization parameters and ' // &
+                           'wavelengths and 1 line are required')
+            return
+        end if
+        if (.not. (ascending(logz) .and. ascending(age) .and. ascending(logu) .and. ascending(lam))) then
+            status = 3
+            call set_error(3, 'set_nebular_tables: the axes and wavelengths must ascend')
+            return
+        end if
+        if (any(age <= 0.0_wp)) then
+            status = 3
+            call set_error(3, 'set_nebular_tables: ages must be positive (yr)')
+            return
+        end if
+        associate(c => ctxs(h))
+            c%nb_logz = logz
+            c%nb_age = age
+            c%nb_logu = logu
+            c%nb_lam = lam
+            c%nb_cont = cont
+            c%nb_llam = line_lam
+            c%nb_lines = lines
+            c%neb_nline = nline
+            c%has_neb = .true.
+            c%dev_neb = .false.
+        end associate
+    end subroutine ctx_set_nebular
+
+    !> The rows of emlines: the nebular library's lines once it is set, else the reference's NEMLINE (166,
+    !> src/core/opensps_constants.f90:173)
+    integer function nemline(c)
+        type(host_ctx_t), intent(in) :: c
+        nemline = merge(c%neb_nline, 166, c%has_neb)
+    end function nemline
 
     !> The templates on the context's grid and to the device, with the two settings. Dust emission as
     !> load_dust_emission_table places it: every column linear in wavelength from the pixel of 1 micron

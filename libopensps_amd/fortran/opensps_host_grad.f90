@@ -192,7 +192,7 @@ contains
             o1 = SAFE_FLOOR
         end if
         if (c_associated(c_emlines)) then
-            call c_f_pointer(c_emlines, o2, [166, n])
+            call c_f_pointer(c_emlines, o2, [nemline(c), n])
             o2 = 0.0_wp
         end if
         if (c_associated(c_sfr)) then

@@ -283,6 +283,46 @@ def test_emission_templates_api():
             assert r["status"] == 5
 
 
+def test_nebular_tables_api():
+    """opensps_context_set_nebular_tables: the test release's CLOUDY library is taken and sets get_nemline to its
+    lines; axes not ascending or below two points are status 3; nebular emission without the tables, in the SSPs
+    (add_neb_emission = 2) or with gradients is status 2; with them the batch passes the host's checks (status 5
+    here: no device)."""
+    import torch
+    from libopensps_amd import datalib
+    logz, age, logu, lam, cont, llam, lines = datalib.nebular(os.path.join(ROOT, "tests", "golden",
+                                                                         "opensps-test-data"))
+    g = grid(1)
+    with api.Context() as ctx:
+        ctx.set_ssp_basis(g)
+        ctx.set_filters(synth.gaussian_filters(4))
+        ctx.set_int("add_dust_emission", 0)
+        assert ctx.nemline == 166
+        vals = {"tage": np.full(3, 5.0), "gas_logu": np.full(3, -2.5), "gas_logz": np.zeros(3)}
+        ctx.set_int("add_neb_emission", 1)
+        r = ctx.compute_csp_batch(vals, raw_status=True)
+        assert r["status"] == 2 and "set_nebular_tables" in api.last_error()
+        with pytest.raises(api.OpenSPSError) as e:
+            ctx.set_nebular_tables(logz[::-1].copy(), age, logu, lam, cont, llam, lines)
+        assert e.value.status == 3
+        with pytest.raises(api.OpenSPSError) as e:
+            ctx.set_nebular_tables(logz[:1], age, logu, lam, cont[:1], llam, lines[:1])
+        assert e.value.status == 3
+        ctx.set_nebular_tables(logz, age, logu, lam, cont, llam, lines)
+        assert ctx.nemline == llam.size
+        r = ctx.compute_csp_batch_grad(vals, mags_bar=np.ones((3, 4)), grads=("params_bar",), raw_status=True)
+        assert r["status"] == 2 and "nebular" in api.last_error()
+        ctx.set_int("add_neb_emission", 2)
+        r = ctx.compute_csp_batch(vals, raw_status=True)
+        assert r["status"] == 2 and "add_neb_emission = 2" in api.last_error()
+        ctx.set_int("add_neb_emission", 1)
+        ctx.set_int("add_neb_continuum", 0)
+        ctx.set_int("nebemlineinspec", 0)
+        if not torch.cuda.is_available():
+            r = ctx.compute_csp_batch(vals, want=("mags", "emlines"), raw_status=True)
+            assert r["status"] == 5 and r["emlines"].shape == (3, llam.size)
+
+
 def test_parameters_no_stage_reads_are_accepted():
     """Members of the parameter set that no stage of this build reads (dust curves of other types, nebular, dust
     emission without dust, fagn = 0) pass the host's checks and reach the device (status 5 here without one)."""

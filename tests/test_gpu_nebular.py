@@ -109,3 +109,51 @@ def test_nebular_statuses():
         assert e.value.status == 2
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("library", ["release", "synthetic"])
+def test_api_nebular_library(precision, library):
+    """Through the public C API with a CLOUDY library given by opensps_context_set_nebular_tables: the host hands it
+    to the device, which places it on the grid; the batch adds the continuum and the lines and returns emlines
+    (get_nemline rows); against the oracle with the same library placed by the oracle's restatement of the
+    reference's loader. "release": the test release's libraries/nebular as datalib reads it (its line table holds
+    only FP32 round-off, so it exercises the reader, the placement and the continuum); "synthetic": a library with
+    lines of the CLOUDY grid's strengths on the same path."""
+    import os
+    from libopensps_amd import api, datalib
+    g = grid(3)
+    if library == "release":
+        lib = datalib.nebular(os.path.join(os.path.dirname(__file__), "golden", "opensps-test-data"))
+    else:
+        lib = synth.nebular_library(g)
+    fs = synth.gaussian_filters(20)
+    n = 40
+    prm = _params(n, seed=83)
+    names = synth.varied_names() + ["dust1", "frac_obrun", "gas_logz", "gas_logu"]
+    print(f"case: api nebular library {library} {precision}", flush=True)
+    with api.Context() as ctx:
+        ctx.set_ssp_basis(g)
+        ctx.set_filters(fs)
+        ctx.set_int("add_igm_absorption", 1)
+        ctx.set_int("add_dust_emission", 0)
+        ctx.set_int("add_neb_emission", 1)
+        ctx.set_str("kernel_precision", precision)
+        ctx.set_nebular_tables(*lib)
+        assert ctx.nemline == lib[5].size
+        r = ctx.compute_csp_batch_params(prm, names, want=("mags", "spec", "emlines"))
+        b = ctx.bands()
+    ref = O.csp_batch(g, b, prm, add_igm=1, cosmo=O.cosmology_table(), want=("mags", "spec", "emlines"),
+                      nebular=O.nebular_on_grid(g.lam, *lib))
+    plain = O.csp_batch(g, b, prm, add_igm=1, cosmo=O.cosmology_table(), want=("mags",))
+    assert ref["status"] == 0
+    t = TOL[0 if precision == "double" else 1]
+    es, em = spec_rel_err(r["spec"], ref["spec"]), mag_err(r["mags"], ref["mags"])
+    big = ref["emlines"] > 1e-6 * ref["emlines"].max(axis=1, keepdims=True)
+    el = float(np.max(np.abs(r["emlines"] / np.where(big, ref["emlines"], 1.0) - 1.0) * big))
+    shift = float(np.nanmax(np.abs(ref["mags"] - plain["mags"])))
+    print(f"  spectra {es:.3e}, mags {em:.3e}, emlines {el:.3e} (the nebular light moves mags by up to {shift:.3f})")
+    assert r["emlines"].shape == (n, lib[5].size) and shift > 0.01
+    if library == "synthetic":
+        assert big.sum() > n * 50
+    assert es < t["spec"] and em < t["mag"] and el < t["lines"]
