@@ -1213,9 +1213,20 @@ template <typename T> struct Quad;
 template <> struct Quad<float> { typedef f32x4_t type; };
 template <> struct Quad<double> { typedef f64x4_t type; };
 
+// The chain reads its arguments through the kernarg segment's address, made opaque again at each phase (chain_args),
+// so that each phase loads the pointers and sizes it uses (scalar loads) instead of holding all of them from the
+// kernel's start (with ~40 pointers live at once the compiler spilled them through VGPR lanes). The per-model
+// parameters and header, which the chain only reads, are read through the constant address space (scalar loads).
+typedef const __attribute__((address_space(4))) ChainArgs CArgs;
+typedef const __attribute__((address_space(4))) double cdouble;
+__device__ __forceinline__ CArgs* chain_args(CArgs* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 // The velocity kernel's window [il, ih] of pixel i (build_velocity_kernel's edges): from i down to the last pixel
 // at or below lambda_i exp(-4 sigma), up to the first at or above lambda_i exp(4 sigma)
-__device__ __forceinline__ int2 chain_window(const ChainArgs& a, int i, double fblue, double fred) {
+__device__ __forceinline__ int2 chain_window(CArgs& a, int i, double fblue, double fred) {
     const double li = a.lam[i];
     const double vl = li * fblue, vh = li * fred;
     int il = i, ih = i;
@@ -1226,7 +1237,7 @@ __device__ __forceinline__ int2 chain_window(const ChainArgs& a, int i, double f
 
 // The velocity-smoothed value of pixel i over its window w (the weights of build_velocity_kernel)
 template <typename T>
-__device__ __forceinline__ T smooth_px(const ChainArgs& a, const T* __restrict__ s0, int i, int2 w, double inv2, T out) {
+__device__ __forceinline__ T smooth_px(CArgs& a, const T* __restrict__ s0, int i, int2 w, double inv2, T out) {
     const int il = w.x, ih = w.y;
     if (ih - il + 1 < 2) return out;
     const double lli = a.loglam[i];
@@ -1282,9 +1293,9 @@ struct ObsIdx {
 };
 
 template <typename T>
-__device__ __forceinline__ ObsIdx<T> obs_index(const ChainArgs& a, int q, double rz, double lnopz) {
+__device__ __forceinline__ ObsIdx<T> obs_index(CArgs& a, int q, double rz, double lnopz) {
     const int nspec = a.nspec;
-    const LamLookup& L = a.lk;
+    const __attribute__((address_space(4))) LamLookup& L = a.lk;
     ObsIdx<T> r;
     int c = -1;
 #pragma unroll
@@ -1309,7 +1320,7 @@ __device__ __forceinline__ ObsIdx<T> obs_index(const ChainArgs& a, int q, double
 
 // The observed-frame quad q (times 1 / lambda; 0 past the grid) from the rest-frame spectrum src and its intervals
 template <typename T>
-__device__ __forceinline__ typename Quad<T>::type obs_quad(const ChainArgs& a, const T* __restrict__ src, int q,
+__device__ __forceinline__ typename Quad<T>::type obs_quad(CArgs& a, const T* __restrict__ src, int q,
                                                            bool rest, const ObsIdx<T>& x) {
     typename Quad<T>::type out;
 #pragma unroll
@@ -1757,33 +1768,35 @@ template __global__ void k_neb<double>(NebArgs);
 // batch without them runs the chain without their code.
 template <typename T, bool ONE, bool EM>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value ? 8 : 6)))
-void k_chain(ChainArgs a) {
+void k_chain(ChainArgs) {
+    CArgs* pa = chain_args((CArgs*)__builtin_amdgcn_kernarg_segment_ptr());
     constexpr bool F32 = std::is_same<T, float>::value;
     constexpr int KA = F32 ? 2 : 1;
     typedef typename Quad<T>::type Q;
     extern __shared__ __align__(32) unsigned char lds_raw[];
     T* s0 = (T*)lds_raw;
-    T* s1 = ONE ? s0 : s0 + a.nsp;
+    T* s1 = ONE ? s0 : s0 + pa->nsp;
     Q* s0q = (Q*)s0;
     Q* s1q = (Q*)s1;
-    const int m = blockIdx.x, n = a.n, nspec = a.nspec, nq = a.nsp >> 2;
+    const int m = blockIdx.x, n = pa->n, nspec = pa->nspec, nq = pa->nsp >> 2;
     const int tid = threadIdx.x, nth = blockDim.x;
-    const double* h = a.hdr + (size_t)m * OPS_HDR_N;
-    const double z = a.prm[(size_t)OPS_P_ZRED * n + m];
-    const double dust1 = a.prm[(size_t)OPS_P_DUST1 * n + m], dust2 = a.prm[(size_t)OPS_P_DUST2 * n + m];
-    const double dust3 = a.prm[(size_t)OPS_P_DUST3 * n + m];
-    const double ed = a.prm[(size_t)OPS_P_DUST_INDEX * n + m], ed1 = a.prm[(size_t)OPS_P_DUST1_INDEX * n + m];
-    const double fob = a.prm[(size_t)OPS_P_FRAC_OBRUN * n + m], fnd = a.prm[(size_t)OPS_P_FRAC_NODUST * n + m];
-    const double igmf = a.prm[(size_t)OPS_P_IGM_FACTOR * n + m];
+    cdouble* h = (cdouble*)(pa->hdr + (size_t)m * OPS_HDR_N);
+    cdouble* prm = (cdouble*)pa->prm;
+    const double z = prm[(size_t)OPS_P_ZRED * n + m];
+    const double dust1 = prm[(size_t)OPS_P_DUST1 * n + m], dust2 = prm[(size_t)OPS_P_DUST2 * n + m];
+    const double dust3 = prm[(size_t)OPS_P_DUST3 * n + m];
+    const double ed = prm[(size_t)OPS_P_DUST_INDEX * n + m], ed1 = prm[(size_t)OPS_P_DUST1_INDEX * n + m];
+    const double fob = prm[(size_t)OPS_P_FRAC_OBRUN * n + m], fnd = prm[(size_t)OPS_P_FRAC_NODUST * n + m];
+    const double igmf = prm[(size_t)OPS_P_IGM_FACTOR * n + m];
     const bool no_birth = dust1 <= OPS_SAFE_FLOOR || 1.0 - fob <= OPS_SAFE_FLOOR;
     const bool no_old = fabs(dust3) <= OPS_SAFE_FLOOR;
     const bool no_diffuse_mix = 1.0 - fnd <= OPS_SAFE_FLOOR;
-    const bool igm_on = a.add_igm && z > OPS_SAFE_FLOOR && a.jtop > 0;
+    const bool igm_on = pa->add_igm && z > OPS_SAFE_FLOOR && pa->jtop > 0;
     const double inv2 = h[OPS_HDR_INV2], fblue = h[OPS_HDR_FBLUE], fred = h[OPS_HDR_FRED];
     const int w0 = (int)h[OPS_HDR_W0], w1 = (int)h[OPS_HDR_W1];
-    const bool rest = fabs(a.zmags ? a.zmags[m] : z) <= OPS_SAFE_FLOOR;   // the magnitudes' redshift
+    const bool rest = fabs(pa->zmags ? ((cdouble*)pa->zmags)[m] : z) <= OPS_SAFE_FLOOR;   // the magnitudes' redshift
     const double rz = h[OPS_HDR_RZ], lnopz = h[OPS_HDR_LNOPZ];
-    const int qb0 = a.band_j0 >> 2, qb1 = (a.band_j1 + 1) >> 2;   // band quads [qb0, qb1)
+    const int qb0 = pa->band_j0 >> 2, qb1 = (pa->band_j1 + 1) >> 2;   // band quads [qb0, qb1)
 
     // ---- the lookups that need no spectrum (the FP64 chain, short of registers, does its lookups where it uses
     // their results) ----
@@ -1792,10 +1805,10 @@ void k_chain(ChainArgs a) {
 #pragma unroll
     for (int k = 0; k < OPS_CHAIN_KW; k++) {
         const int i = w0 + tid + k * nth;
-        win[k] = EARLY && i < w1 ? chain_window(a, i, fblue, fred) : make_int2(0, 0);
+        win[k] = EARLY && i < w1 ? chain_window(*pa, i, fblue, fred) : make_int2(0, 0);
     }
     // IGM transmissions of pixels tid + k nth below jt (pixels i < jcap take the capped depth)
-    const int jt = igm_on ? a.jtop : 0;
+    const int jt = igm_on ? pa->jtop : 0;
     const double tmax = h[OPS_HDR_TMAX];
     const int jmax = (int)h[OPS_HDR_JMAX];
     const int jcap = jmax > 1 ? jmax : 0;
@@ -1806,7 +1819,7 @@ void k_chain(ChainArgs a) {
         T b6[6];
 #pragma unroll
         for (int k = 0; k < 6; k++) b6[k] = (T)h[OPS_HDR_B6 + k];
-        const size_t jn = (size_t)a.jtop;
+        const size_t jn = (size_t)pa->jtop;
 #pragma unroll
         for (int k = 0; k < OPS_CHAIN_KI; k++) {
             const int i = tid + k * nth;
@@ -1814,7 +1827,7 @@ void k_chain(ChainArgs a) {
                 if constexpr (F32) {
                     float tau = (float)tmax;
                     if (i >= jcap) {
-                        const float* g = a.igm32s + i;
+                        const float* g = pa->igm32s + i;
                         tau = g[0] * b6[0] + g[jn] * b6[1] + g[2 * jn] * b6[2] + g[3 * jn] * b6[3] + g[4 * jn] * b6[4] +
                               g[5 * jn] * b6[5];
                     }
@@ -1822,7 +1835,7 @@ void k_chain(ChainArgs a) {
                 } else {
                     double tau = tmax;
                     if (i >= jcap) {
-                        const double* g = a.igm64s + i;
+                        const double* g = pa->igm64s + i;
                         tau = g[0] * b6[0] + g[jn] * b6[1] + g[2 * jn] * b6[2] + g[3 * jn] * b6[3] + g[4 * jn] * b6[4] +
                               g[5 * jn] * b6[5];
                     }
@@ -1832,9 +1845,10 @@ void k_chain(ChainArgs a) {
         }
     }
 
+    pa = chain_args(pa);
     // ---- phase 1: attenuation -> s0 (padded pixels: the zero age sums), KA quads a thread fetched first ----
-    const Q* y4 = (const Q*)((const T*)a.yo + (size_t)m * a.nsp);
-    const Q* o4 = a.parts == 2 ? (const Q*)((const T*)a.yo + ((size_t)n + m) * a.nsp) : nullptr;
+    const Q* y4 = (const Q*)((const T*)pa->yo + (size_t)m * pa->nsp);
+    const Q* o4 = pa->parts == 2 ? (const Q*)((const T*)pa->yo + ((size_t)n + m) * pa->nsp) : nullptr;
     for (int q0 = tid; q0 < nq; q0 += KA * nth) {
         Q y[KA], o[KA], L[KA];
 #pragma unroll
@@ -1847,9 +1861,9 @@ void k_chain(ChainArgs a) {
                 y[u] = y4[q];
                 if (o4) o[u] = o4[q];
                 if constexpr (F32)
-                    L[u] = ((const f32x4_t*)a.ltab32)[q];
+                    L[u] = ((const f32x4_t*)pa->ltab32)[q];
                 else
-                    L[u] = ((const f64x4_t*)a.ltab)[q];
+                    L[u] = ((const f64x4_t*)pa->ltab)[q];
             }
         }
 #pragma unroll
@@ -1861,7 +1875,7 @@ void k_chain(ChainArgs a) {
                 const float cf = (float)fob, cn = (float)fnd;
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    const float curve = a.dcurve32 ? a.dcurve32[4 * q + e]
+                    const float curve = pa->dcurve32 ? pa->dcurve32[4 * q + e]
                                                    : ((ed != 0.0) ? __expf((float)ed * L[u][e]) : 1.0f);
                     const float tr_d = __expf(-((float)dust2 * curve));
                     float tr_b = 1.0f, tr_o = 1.0f;
@@ -1873,7 +1887,7 @@ void k_chain(ChainArgs a) {
             } else {
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    const double curve = a.dcurve ? a.dcurve[4 * q + e] : ((ed != 0.0) ? exp(ed * L[u][e]) : 1.0);
+                    const double curve = pa->dcurve ? pa->dcurve[4 * q + e] : ((ed != 0.0) ? exp(ed * L[u][e]) : 1.0);
                     const double tr_d = exp(-(dust2 * curve));
                     double tr_b = 1.0, tr_o = 1.0;
                     if (!no_birth) tr_b = exp(-(dust1 * exp(ed1 * L[u][e])));
@@ -1883,19 +1897,20 @@ void k_chain(ChainArgs a) {
                 }
             }
             s0q[q] = tot;
-            if (a.young || a.old) {
+            if (pa->young || pa->old) {
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
                     const int i = 4 * q + e;
                     if (i < nspec) {
-                        if (a.young) ((T*)a.young)[(size_t)m * nspec + i] = y[u][e];
-                        if (a.old) ((T*)a.old)[(size_t)m * nspec + i] = o[u][e];
+                        if (pa->young) ((T*)pa->young)[(size_t)m * nspec + i] = y[u][e];
+                        if (pa->old) ((T*)pa->old)[(size_t)m * nspec + i] = o[u][e];
                     }
                 }
             }
         }
     }
     __syncthreads();
+    pa = chain_args(pa);
     // dust emission (add_dust_emission, opensps_csp_batch.f90:4045-4099): the template times the diffuse
     // transmission times the energy balance's normalisation (k_dust_sums), before the IGM as there
     const double rn = EM ? h[OPS_HDR_DL_RN] : 0.0;
@@ -1903,12 +1918,13 @@ void k_chain(ChainArgs a) {
         const int iq = (int)h[OPS_HDR_DL_IQ], iu = (int)h[OPS_HDR_DL_IU];
         const double wq = h[OPS_HDR_DL_WQ], wu = h[OPS_HDR_DL_WU], gam = h[OPS_HDR_DL_GAM];
         for (int i = tid; i < nspec; i += nth) {
-            const T td = pixel_td<T>(a.ltab, a.ltab32, a.dcurve, a.dcurve32, ed, dust2, i);
-            const T sh = dl07_shape<T>(a.dustem, a.nsp, a.nqpah, iq, wq, iu, wu, gam, i);
+            const T td = pixel_td<T>(pa->ltab, pa->ltab32, pa->dcurve, pa->dcurve32, ed, dust2, i);
+            const T sh = dl07_shape<T>(pa->dustem, pa->nsp, pa->nqpah, iq, wq, iu, wu, gam, i);
             s0[i] = s0[i] + sh * td * (T)rn;
         }
         __syncthreads();
     }
+    pa = chain_args(pa);
     // the IGM (apply_igm_z): pixels below jt times their transmissions; pixels past KI a thread (grids with more
     // than KI x 512 pixels below Lyman-alpha), and every pixel in the FP64 chain, form theirs here
     if (jt > 0) {
@@ -1922,8 +1938,8 @@ void k_chain(ChainArgs a) {
         for (int i = tid + (EARLY ? OPS_CHAIN_KI * nth : 0); i < jt; i += nth) {
             double tau = tmax;
             if (i >= jcap) {
-                const double* g = a.igm64s + i;
-                const size_t jn = (size_t)a.jtop;
+                const double* g = pa->igm64s + i;
+                const size_t jn = (size_t)pa->jtop;
                 tau = g[0] * h[OPS_HDR_B6] + g[jn] * h[OPS_HDR_B6 + 1] + g[2 * jn] * h[OPS_HDR_B6 + 2] +
                       g[3 * jn] * h[OPS_HDR_B6 + 3] + g[4 * jn] * h[OPS_HDR_B6 + 4] + g[5 * jn] * h[OPS_HDR_B6 + 5];
             }
@@ -1931,6 +1947,7 @@ void k_chain(ChainArgs a) {
         }
         __syncthreads();
     }
+    pa = chain_args(pa);
     // the AGN torus (add_agn_batch, opensps_csp_batch.f90:4342-4433): fagn L_bol times the agn_tau template (linear
     // between its two optical depths) times the diffuse transmission, after the IGM as there
     const double lagn = EM ? h[OPS_HDR_AGN_L] : 0.0;
@@ -1938,21 +1955,23 @@ void k_chain(ChainArgs a) {
         const int ia = (int)h[OPS_HDR_AGN_IT];
         const double wta = h[OPS_HDR_AGN_WT];
         for (int i = tid; i < nspec; i += nth) {
-            const T td = pixel_td<T>(a.ltab, a.ltab32, a.dcurve, a.dcurve32, ed, dust2, i);
-            double tmpl = (1.0 - wta) * a.agn[(size_t)ia * a.nsp + i] + wta * a.agn[(size_t)(ia + 1) * a.nsp + i];
+            const T td = pixel_td<T>(pa->ltab, pa->ltab32, pa->dcurve, pa->dcurve32, ed, dust2, i);
+            double tmpl = (1.0 - wta) * pa->agn[(size_t)ia * pa->nsp + i] + wta * pa->agn[(size_t)(ia + 1) * pa->nsp + i];
             tmpl = tmpl * (double)td;
             s0[i] = (T)((double)s0[i] + lagn * tmpl);
         }
         __syncthreads();
     }
 
+    pa = chain_args(pa);
     // the source intervals of the first observed-frame quad (their lookups overlap the smoothing's)
     ObsIdx<T> ox0;
-    if (EARLY && !rest && qb0 + tid < qb1) ox0 = obs_index<T>(a, qb0 + tid, rz, lnopz);
+    if (EARLY && !rest && qb0 + tid < qb1) ox0 = obs_index<T>(*pa, qb0 + tid, rz, lnopz);
 
+    pa = chain_args(pa);
     // ---- phase 2: velocity smoothing (-> s1, or in place with ONE), and the spectrum out. Pixels outside the
     // smoothed range [w0, w1) keep their values; the range's pixels are spread over the threads one by one ----
-    T* spec_out = a.spec ? (T*)a.spec + (size_t)m * nspec : nullptr;
+    T* spec_out = pa->spec ? (T*)pa->spec + (size_t)m * nspec : nullptr;
     for (int q = tid; q < nq; q += nth) {
         const int i0 = 4 * q;
         const bool outside = i0 + 3 < w0 || i0 >= w1;
@@ -1978,7 +1997,7 @@ void k_chain(ChainArgs a) {
     auto window = [&](int k, int i) -> int2 {
         if (EARLY && k == 0) return win[0];
         if (EARLY && k == 1) return win[1];
-        return chain_window(a, i, fblue, fred);
+        return chain_window(*pa, i, fblue, fred);
     };
     if constexpr (ONE) {
         if (w1 - w0 <= OPS_CHAIN_KS * nth) {
@@ -1986,7 +2005,7 @@ void k_chain(ChainArgs a) {
 #pragma unroll
             for (int k = 0; k < OPS_CHAIN_KS; k++) {
                 const int i = w0 + tid + k * nth;
-                sv[k] = i < w1 ? smooth_px<T>(a, s0, i, window(k, i), inv2, s0[i]) : (T)0;
+                sv[k] = i < w1 ? smooth_px<T>(*pa, s0, i, window(k, i), inv2, s0[i]) : (T)0;
                 if (i < w1 && spec_out) spec_out[i] = sv[k];
             }
             __syncthreads();
@@ -1996,33 +2015,34 @@ void k_chain(ChainArgs a) {
                 if (i < w1) s0[i] = sv[k];
             }
         } else {
-            T* G = spec_out ? spec_out : (T*)a.stage + (size_t)m * nspec;
+            T* G = spec_out ? spec_out : (T*)pa->stage + (size_t)m * nspec;
             int k = 0;
-            for (int i = w0 + tid; i < w1; i += nth, k++) G[i] = smooth_px<T>(a, s0, i, window(k, i), inv2, s0[i]);
+            for (int i = w0 + tid; i < w1; i += nth, k++) G[i] = smooth_px<T>(*pa, s0, i, window(k, i), inv2, s0[i]);
             __syncthreads();
             for (int i = w0 + tid; i < w1; i += nth) s0[i] = G[i];
         }
     } else {
         int k = 0;
         for (int i = w0 + tid; i < w1; i += nth, k++) {
-            const T out = smooth_px<T>(a, s0, i, window(k, i), inv2, s0[i]);
+            const T out = smooth_px<T>(*pa, s0, i, window(k, i), inv2, s0[i]);
             s1[i] = out;
             if (spec_out) spec_out[i] = out;
         }
     }
     __syncthreads();
 
+    pa = chain_args(pa);
     // ---- phase 3: the observed frame over the band quads (from s1) -> s0 ----
     auto oidx = [&](int k, int q) -> ObsIdx<T> {
         if (rest) return ObsIdx<T>{};
-        return EARLY && k == 0 ? ox0 : obs_index<T>(a, q, rz, lnopz);
+        return EARLY && k == 0 ? ox0 : obs_index<T>(*pa, q, rz, lnopz);
     };
     if constexpr (ONE) {
         Q fo[OPS_CHAIN_KQ];
 #pragma unroll
         for (int k = 0; k < OPS_CHAIN_KQ; k++) {
             const int q = qb0 + tid + k * nth;
-            if (q < qb1) fo[k] = obs_quad<T>(a, s1, q, rest, oidx(k, q));
+            if (q < qb1) fo[k] = obs_quad<T>(*pa, s1, q, rest, oidx(k, q));
         }
         __syncthreads();
 #pragma unroll
@@ -2032,19 +2052,20 @@ void k_chain(ChainArgs a) {
         }
     } else {
         int k = 0;
-        for (int q = qb0 + tid; q < qb1; q += nth, k++) s0q[q] = obs_quad<T>(a, s1, q, rest, oidx(k, q));
+        for (int q = qb0 + tid; q < qb1; q += nth, k++) s0q[q] = obs_quad<T>(*pa, s1, q, rest, oidx(k, q));
     }
     __syncthreads();
 
+    pa = chain_args(pa);
     // ---- phase 4: band sums (16 lanes per band, 4 pixels a lane, FP64 across the lanes, four bands a group at a
     // time) -> LDS; magnitudes, a thread per band ----
     // the sums wait in LDS after the spectrum (sband_staged: the host sized the LDS for them)
-    double* sband = (double*)(lds_raw + (ONE ? 1 : 2) * (size_t)a.nsp * sizeof(T));
-    const bool staged = a.sband_staged != 0;
+    double* sband = (double*)(lds_raw + (ONE ? 1 : 2) * (size_t)pa->nsp * sizeof(T));
+    const bool staged = pa->sband_staged != 0;
     const double dist = h[OPS_HDR_DIST];
     const int g = tid >> 4, gl = tid & 15, ng = nth >> 4;
     constexpr int KB = F32 ? 4 : 2;
-    for (int b0 = g; b0 < a.nbands; b0 += KB * ng) {
+    for (int b0 = g; b0 < pa->nbands; b0 += KB * ng) {
         int4 bm[KB];
         bool on[KB];
         int nit = 0;
@@ -2053,9 +2074,9 @@ void k_chain(ChainArgs a) {
             const int b = b0 + u * ng;
             bm[u] = make_int4(0, 0, 0, 0);
             on[u] = false;
-            if (b < a.nbands) {
-                bm[u] = a.bmeta[b];   // {coefficient offset, first pixel, pixels}: multiples of 4
-                on[u] = a.mask[b] != 0;
+            if (b < pa->nbands) {
+                bm[u] = pa->bmeta[b];   // {coefficient offset, first pixel, pixels}: multiples of 4
+                on[u] = pa->mask[b] != 0;
             }
             if (on[u]) nit = max(nit, (bm[u].z / 4 + 15) / 16);
         }
@@ -2069,7 +2090,7 @@ void k_chain(ChainArgs a) {
 #pragma unroll
                 for (int u = 0; u < KB; u++) {
                     if (on[u] && 4 * q < bm[u].z) {
-                        const f32x4_t cc = *(const f32x4_t*)(a.bc32 + (unsigned)(bm[u].x + 4 * q));
+                        const f32x4_t cc = *(const f32x4_t*)(pa->bc32 + (unsigned)(bm[u].x + 4 * q));
                         const f32x4_t ff = *(const f32x4_t*)(s0 + (unsigned)(bm[u].y + 4 * q));
                         acc[u] = fmaf(cc[0], ff[0], acc[u]);
                         acc[u] = fmaf(cc[1], ff[1], acc[u]);
@@ -2088,7 +2109,7 @@ void k_chain(ChainArgs a) {
 #pragma unroll
                 for (int u = 0; u < KB; u++) {
                     if (on[u] && 4 * q < bm[u].z) {
-                        const f64x4_t cc = *(const f64x4_t*)(a.bc64 + (unsigned)(bm[u].x + 4 * q));
+                        const f64x4_t cc = *(const f64x4_t*)(pa->bc64 + (unsigned)(bm[u].x + 4 * q));
                         const f64x4_t ff = *(const f64x4_t*)(s0 + (unsigned)(bm[u].y + 4 * q));
                         s[u] += cc[0] * ff[0] + cc[1] * ff[1] + cc[2] * ff[2] + cc[3] * ff[3];
                     }
@@ -2099,26 +2120,27 @@ void k_chain(ChainArgs a) {
         for (int u = 0; u < KB; u++) {
             for (int o = 8; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o, 16);
             const int b = b0 + u * ng;
-            if (gl == 0 && b < a.nbands) {
+            if (gl == 0 && b < pa->nbands) {
                 if (staged) {
                     sband[b] = s[u];
                 } else {
                     double mag = NAN;
-                    if (on[u]) mag = (s[u] > 0.0 ? -2.5 * log10(s[u]) : NAN) - 48.60 - 2.5 * a.amzl + dist;
-                    a.mags[(size_t)m * a.nbands + b] = mag;
-                    if (a.fband) a.fband[(size_t)m * a.nbands + b] = s[u];
+                    if (on[u]) mag = (s[u] > 0.0 ? -2.5 * log10(s[u]) : NAN) - 48.60 - 2.5 * pa->amzl + dist;
+                    pa->mags[(size_t)m * pa->nbands + b] = mag;
+                    if (pa->fband) pa->fband[(size_t)m * pa->nbands + b] = s[u];
                 }
             }
         }
     }
     if (!staged) return;
     __syncthreads();
-    for (int b = tid; b < a.nbands; b += nth) {
+    pa = chain_args(pa);
+    for (int b = tid; b < pa->nbands; b += nth) {
         const double s = sband[b];
         double mag = NAN;
-        if (a.mask[b]) mag = (s > 0.0 ? -2.5 * log10(s) : NAN) - 48.60 - 2.5 * a.amzl + dist;
-        a.mags[(size_t)m * a.nbands + b] = mag;
-        if (a.fband) a.fband[(size_t)m * a.nbands + b] = s;
+        if (pa->mask[b]) mag = (s > 0.0 ? -2.5 * log10(s) : NAN) - 48.60 - 2.5 * pa->amzl + dist;
+        pa->mags[(size_t)m * pa->nbands + b] = mag;
+        if (pa->fband) pa->fband[(size_t)m * pa->nbands + b] = s;
     }
 }
 
