@@ -322,7 +322,9 @@ int opensps_hip_set_grid(opensps_hip_ctx_t* c, int nspec, const double* lam, int
     UPLOAD(c, c->d_blbol, lbol, (size_t)nt * nz);
     UPLOAD(c, c->d_igm_a, igm.data(), igm.size());
     {
-        const size_t jn = jtop ? jtop : 1;
+        // term-major with the stride padded to whole quads (the chain reads a quad's terms as one 16- or 32-byte load)
+        const size_t jn = jtop ? (size_t)((jtop + 3) & ~3) : 4;
+        c->jq = (int)jn;
         std::vector<double> i64(6 * jn, 0.0);
         std::vector<float> i32(6 * jn, 0.0f);
         for (int j = 0; j < jtop; j++)
@@ -1613,6 +1615,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ca.dlam = c->d_dlam;
     ca.igm64s = c->d_igm64s;
     ca.igm32s = c->d_igm32s;
+    ca.jq = c->jq;
     ca.bmeta = c->d_bmeta;
     ca.mask = c->d_mask;
     ca.bc64 = c->d_bc64;

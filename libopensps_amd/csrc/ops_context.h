@@ -47,8 +47,9 @@ struct OpsCtx {
     double lnl0 = 0.0, bkinv = 0.0;   // bucket b starts at ln(lam) = lnl0 + b / bkinv
     int* d_bkt = nullptr;         // per bucket: the number of pixels with ln(lam) below the bucket's start
     int jtop = 0;                 // pixels below Lyman-alpha
+    int jq = 4;                   // jtop rounded up to a multiple of 4: the stride of d_igm64s, d_igm32s
     double* d_igm_a = nullptr;    // (jtop, 6) IGM wavelength factors
-    double* d_igm64s = nullptr;   // the same term-major (6, jtop): the chain's pixel-parallel reads
+    double* d_igm64s = nullptr;   // the same term-major (6, jq): the chain's pixel-parallel reads
     float* d_igm32s = nullptr;    // and in FP32 (the FP32 chain)
     double* d_basis64 = nullptr;  // nspec x nt x nz FP64 (for the packing)
     double* d_A64 = nullptr;      // packed f64 operands [pair][pxblk][kstep][lane]

@@ -1235,6 +1235,77 @@ __device__ __forceinline__ int2 chain_window(CArgs& a, int i, double fblue, doub
     return make_int2(il, ih);
 }
 
+// The same window from one batch of loads, lambda[i - R .. i + R]: the walks' tests on those pixels are made at once
+// (lambda increases, so the tests that pass are the first ones); a window wider than R a side is walked on from there
+template <int R>
+__device__ __forceinline__ int2 chain_window_batch(CArgs& a, int i, double fblue, double fred) {
+    const double* __restrict__ lam = a.lam;
+    const int nspec = a.nspec;
+    double v[2 * R + 1];
+#pragma unroll
+    for (int o = -R; o <= R; o++) v[o + R] = lam[min(max(i + o, 0), nspec - 1)];
+    const double vl = v[R] * fblue, vh = v[R] * fred;
+    int nl = 0, nh = 0;
+    bool rl = true, rh = true;
+#pragma unroll
+    for (int o = 0; o <= R; o++) {
+        rl = rl && i - o > 0 && v[R - o] > vl;
+        rh = rh && i + o < nspec - 1 && v[R + o] < vh;
+        nl += (int)rl;
+        nh += (int)rh;
+    }
+    int il = i - nl, ih = i + nh;
+    if (nl > R)
+        while (il > 0 && lam[il] > vl) il--;
+    if (nh > R)
+        while (ih < nspec - 1 && lam[ih] < vh) ih++;
+    return make_int2(il, ih);
+}
+
+// FP32 velocity smoothing of pixel i over a window within [i - 2, i + 2]: the five taps at once, those outside the
+// window weighted 0 (the sums in the window's order, as smooth_px forms them)
+__device__ __forceinline__ float smooth_px5(CArgs& a, const float* __restrict__ s0, int i, int2 w, float c2, float out) {
+    const int il = w.x, ih = w.y, nspec = a.nspec;
+    if (ih - il + 1 < 2) return out;
+    const double lli = a.loglam[i];
+    float4 f[5];
+    double ll[5];
+#pragma unroll
+    for (int o = 0; o < 5; o++) {
+        const int k = min(max(i + o - 2, 0), nspec - 1);
+        f[o] = a.sm32[k];
+        ll[o] = a.loglam[k];
+    }
+    float tw = 0.0f, acc = 0.0f;
+#pragma unroll
+    for (int o = 0; o < 5; o++) {
+        const int k = i + o - 2;
+        if (k >= il && k <= ih) {
+            const float d = (float)(lli - ll[o]);
+            float wk = __expf(-(c2 * (d * d))) * f[o].x;
+            wk *= (k > il ? f[o].y : 0.0f) + (k < ih ? f[o].z : 0.0f);
+            tw += wk;
+            acc += wk * s0[k];
+        }
+    }
+    if (tw > 0.0f) out = acc * __builtin_amdgcn_rcpf(tw);
+    return out;
+}
+
+// A wave-uniform FP32 value held in a scalar register (its FP64 source converts in a vector register)
+__device__ __forceinline__ float sfloat(double x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)x)));
+}
+
+// The sum over each row of 16 lanes, in every lane of the row (DPP: the quad's pairs, the quad, the half-row, the row)
+__device__ __forceinline__ float row16_sum(float v) {
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));    // quad_perm [1,0,3,2]
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));    // quad_perm [2,3,0,1]
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));   // row_half_mirror
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, false));   // row_mirror
+    return v;
+}
+
 // The velocity-smoothed value of pixel i over its window w (the weights of build_velocity_kernel)
 template <typename T>
 __device__ __forceinline__ T smooth_px(CArgs& a, const T* __restrict__ s0, int i, int2 w, double inv2, T out) {
@@ -1802,11 +1873,6 @@ void k_chain(ChainArgs) {
     // their results) ----
     constexpr bool EARLY = F32;
     int2 win[OPS_CHAIN_KW];
-#pragma unroll
-    for (int k = 0; k < OPS_CHAIN_KW; k++) {
-        const int i = w0 + tid + k * nth;
-        win[k] = EARLY && i < w1 ? chain_window(*pa, i, fblue, fred) : make_int2(0, 0);
-    }
     // IGM transmissions of pixels tid + k nth below jt (pixels i < jcap take the capped depth)
     const int jt = igm_on ? pa->jtop : 0;
     const double tmax = h[OPS_HDR_TMAX];
@@ -1819,7 +1885,7 @@ void k_chain(ChainArgs) {
         T b6[6];
 #pragma unroll
         for (int k = 0; k < 6; k++) b6[k] = (T)h[OPS_HDR_B6 + k];
-        const size_t jn = (size_t)pa->jtop;
+        const size_t jn = (size_t)pa->jq;
 #pragma unroll
         for (int k = 0; k < OPS_CHAIN_KI; k++) {
             const int i = tid + k * nth;
@@ -1909,6 +1975,12 @@ void k_chain(ChainArgs) {
             }
         }
     }
+    // the smoothing windows of the thread's first OPS_CHAIN_KW smoothed pixels (their loads wait with the barrier)
+#pragma unroll
+    for (int k = 0; k < OPS_CHAIN_KW; k++) {
+        const int i = w0 + tid + k * nth;
+        win[k] = EARLY && i < w1 ? chain_window_batch<3>(*pa, i, fblue, fred) : make_int2(0, 0);
+    }
     __syncthreads();
     pa = chain_args(pa);
     // dust emission (add_dust_emission, opensps_csp_batch.f90:4045-4099): the template times the diffuse
@@ -1939,7 +2011,7 @@ void k_chain(ChainArgs) {
             double tau = tmax;
             if (i >= jcap) {
                 const double* g = pa->igm64s + i;
-                const size_t jn = (size_t)pa->jtop;
+                const size_t jn = (size_t)pa->jq;
                 tau = g[0] * h[OPS_HDR_B6] + g[jn] * h[OPS_HDR_B6 + 1] + g[2 * jn] * h[OPS_HDR_B6 + 2] +
                       g[3 * jn] * h[OPS_HDR_B6 + 3] + g[4 * jn] * h[OPS_HDR_B6 + 4] + g[5 * jn] * h[OPS_HDR_B6 + 5];
             }
@@ -1997,7 +2069,16 @@ void k_chain(ChainArgs) {
     auto window = [&](int k, int i) -> int2 {
         if (EARLY && k == 0) return win[0];
         if (EARLY && k == 1) return win[1];
+        if constexpr (F32) return chain_window_batch<3>(*pa, i, fblue, fred);
         return chain_window(*pa, i, fblue, fred);
+    };
+    // the FP32 chain takes windows within two pixels a side (the usual case) five taps at once
+    const float c2f = sfloat(inv2);
+    auto smooth = [&](int i, int2 w) -> T {
+        if constexpr (F32) {
+            if (w.x >= i - 2 && w.y <= i + 2) return smooth_px5(*pa, s0, i, w, c2f, s0[i]);
+        }
+        return smooth_px<T>(*pa, s0, i, w, inv2, s0[i]);
     };
     if constexpr (ONE) {
         if (w1 - w0 <= OPS_CHAIN_KS * nth) {
@@ -2005,7 +2086,7 @@ void k_chain(ChainArgs) {
 #pragma unroll
             for (int k = 0; k < OPS_CHAIN_KS; k++) {
                 const int i = w0 + tid + k * nth;
-                sv[k] = i < w1 ? smooth_px<T>(*pa, s0, i, window(k, i), inv2, s0[i]) : (T)0;
+                sv[k] = i < w1 ? smooth(i, window(k, i)) : (T)0;
                 if (i < w1 && spec_out) spec_out[i] = sv[k];
             }
             __syncthreads();
@@ -2017,14 +2098,14 @@ void k_chain(ChainArgs) {
         } else {
             T* G = spec_out ? spec_out : (T*)pa->stage + (size_t)m * nspec;
             int k = 0;
-            for (int i = w0 + tid; i < w1; i += nth, k++) G[i] = smooth_px<T>(*pa, s0, i, window(k, i), inv2, s0[i]);
+            for (int i = w0 + tid; i < w1; i += nth, k++) G[i] = smooth(i, window(k, i));
             __syncthreads();
             for (int i = w0 + tid; i < w1; i += nth) s0[i] = G[i];
         }
     } else {
         int k = 0;
         for (int i = w0 + tid; i < w1; i += nth, k++) {
-            const T out = smooth_px<T>(*pa, s0, i, window(k, i), inv2, s0[i]);
+            const T out = smooth(i, window(k, i));
             s1[i] = out;
             if (spec_out) spec_out[i] = out;
         }
@@ -2099,8 +2180,9 @@ void k_chain(ChainArgs) {
                     }
                 }
             }
+            // the 16 lanes' partial sums added in FP32 (a tree across the row)
 #pragma unroll
-            for (int u = 0; u < KB; u++) s[u] = (double)acc[u];
+            for (int u = 0; u < KB; u++) s[u] = (double)row16_sum(acc[u]);
         } else {
 #pragma unroll
             for (int u = 0; u < KB; u++) s[u] = 0.0;
@@ -2118,7 +2200,8 @@ void k_chain(ChainArgs) {
         }
 #pragma unroll
         for (int u = 0; u < KB; u++) {
-            for (int o = 8; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o, 16);
+            if constexpr (!F32)
+                for (int o = 8; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o, 16);
             const int b = b0 + u * ng;
             if (gl == 0 && b < pa->nbands) {
                 if (staged) {

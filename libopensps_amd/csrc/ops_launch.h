@@ -244,13 +244,14 @@ struct OpAdjArgs {
 
 struct ChainArgs {
     int n, nspec, nsp, parts, nbands, jtop, add_igm, smooth_fast, band_j0, band_j1, sband_staged;
+    int jq;   // the stride of igm64s / igm32s: jtop rounded up to a multiple of 4
     double amzl;
     LamLookup lk;
     const double* prm;
     const double* hdr;
     const void* yo;
     const double *lam, *loglam, *ltab, *wl, *dlam, *rdlam;
-    const double* igm64s;   // the IGM wavelength factors term-major (6, jtop), FP64 and FP32
+    const double* igm64s;   // the IGM wavelength factors term-major (6, jq), FP64 and FP32
     const float* igm32s;
     const float *ltab32, *wl32;
     const float4* sm32;
