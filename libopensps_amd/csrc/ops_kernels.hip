@@ -1389,6 +1389,61 @@ __device__ __forceinline__ ObsIdx<T> obs_index(CArgs& a, int q, double rz, doubl
     return r;
 }
 
+// The same intervals with the quad's four pixels looked up side by side (the FP32 chain): each pixel's bucket start,
+// then the three entries around it (the count from them), then 1 / dlam of its interval.
+// Three rounds of independent loads instead of the walks from pixel to pixel; the counts and fractions are
+// obs_index's. A pixel whose count the four entries do not settle walks as lam_count does.
+__device__ __forceinline__ ObsIdx<float> obs_index_quad(CArgs& a, int q, double rz, double lnopz) {
+    const int nspec = a.nspec, nbk = a.lk.nbk;
+    const double lnl0 = a.lk.lnl0, bkinv = a.lk.bkinv;
+    const double* __restrict__ lam = a.lam;
+    ObsIdx<float> r;
+    double tg[4];
+    int c[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int j = min(4 * q + e, nspec - 1);
+        tg[e] = lam[j] * rz;
+        const double u = (a.loglam[j] - lnopz - lnl0) * bkinv;
+        int b = 0;
+        if (u > 0.0) b = u >= (double)(nbk - 1) ? nbk - 1 : (int)u;
+        c[e] = a.lk.bkt[b];
+    }
+    // the entries c - 1, c, c + 1 settle the count as c or c + 1 (the bucket holds at most one pixel boundary);
+    // otherwise (a rounding of the logarithm across a bucket edge) the walk
+    double v[4][3];
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int o = 0; o < 3; o++) v[e][o] = lam[min(max(c[e] - 1 + o, 0), nspec - 1)];
+    int il[4];
+    double lo[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const double x = tg[e];
+        const bool i1 = c[e] - 1 < 0 || v[e][0] <= x, i2 = c[e] < nspec && v[e][1] <= x;
+        const bool i3 = c[e] + 1 < nspec && v[e][2] <= x;
+        int cnt;
+        if (i1 && !i3) {
+            cnt = c[e] + (int)i2;
+        } else {
+            cnt = c[e];
+            while (cnt < nspec && lam[cnt] <= x) cnt++;
+            while (cnt > 0 && lam[cnt - 1] > x) cnt--;
+        }
+        il[e] = max(1, min(nspec - 1, cnt)) - 1;
+        const int d = il[e] - (c[e] - 1);
+        lo[e] = d == 0 ? v[e][0] : d == 1 ? v[e][1] : lam[il[e]];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const bool in = 4 * q + e < nspec;
+        r.il[e] = in ? il[e] : -1;
+        r.t[e] = in ? (float)((tg[e] - lo[e]) * a.rdlam[il[e]]) : 0.0f;
+    }
+    return r;
+}
+
 // The observed-frame quad q (times 1 / lambda; 0 past the grid) from the rest-frame spectrum src and its intervals
 template <typename T>
 __device__ __forceinline__ typename Quad<T>::type obs_quad(CArgs& a, const T* __restrict__ src, int q,
@@ -2038,7 +2093,8 @@ void k_chain(ChainArgs) {
     pa = chain_args(pa);
     // the source intervals of the first observed-frame quad (their lookups overlap the smoothing's)
     ObsIdx<T> ox0;
-    if (EARLY && !rest && qb0 + tid < qb1) ox0 = obs_index<T>(*pa, qb0 + tid, rz, lnopz);
+    constexpr bool OXE = false;   // (the FP32 chain looks its intervals up in phase 3, three rounds of loads)
+    if (OXE && !rest && qb0 + tid < qb1) ox0 = obs_index<T>(*pa, qb0 + tid, rz, lnopz);
 
     pa = chain_args(pa);
     // ---- phase 2: velocity smoothing (-> s1, or in place with ONE), and the spectrum out. Pixels outside the
@@ -2116,7 +2172,9 @@ void k_chain(ChainArgs) {
     // ---- phase 3: the observed frame over the band quads (from s1) -> s0 ----
     auto oidx = [&](int k, int q) -> ObsIdx<T> {
         if (rest) return ObsIdx<T>{};
-        return EARLY && k == 0 ? ox0 : obs_index<T>(*pa, q, rz, lnopz);
+        if (OXE && k == 0) return ox0;
+        if constexpr (F32) return obs_index_quad(*pa, q, rz, lnopz);
+        return obs_index<T>(*pa, q, rz, lnopz);
     };
     if constexpr (ONE) {
         Q fo[OPS_CHAIN_KQ];
