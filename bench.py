@@ -74,7 +74,7 @@ def parse():
     ap.add_argument("--adjoint-sfh", action="store_true",
                     help="also the batch adjoint with SFH parameters (params_bar's SFH rows: the SFH tangents)")
     ap.add_argument("--emu-train", type=int, default=4_000_000, help="emulator training spectra")
-    ap.add_argument("--emu-epochs", type=int, default=100)
+    ap.add_argument("--emu-epochs", type=int, default=200)
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with a 256 MiB device-to-device copy (the calibration of the HBM counter runs)")
     return ap.parse_args()
