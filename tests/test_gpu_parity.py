@@ -337,6 +337,18 @@ def test_wide_smoothing_window(precision, with_spec):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", [0, 1])
+def test_broad_velocity_kernels(precision):
+    """Velocity dispersions of 50 to 3000 km/s on the 5994-pixel grid (~700 km/s a pixel): smoothing windows from one
+    to about eighteen pixels a side, so the FP32 chain takes both its five-tap form (within two pixels a side) and the
+    general sum, and its batched window lookup (seven pixels) both settles windows and walks on past them."""
+    g, b, prm = _parametric_case(64, seed=41)
+    prm[SLOT["sigma_smooth"]] = np.random.default_rng(41).uniform(50.0, 3000.0, prm.shape[1])
+    ref, got = _run_both(g, b, prm, precision)
+    _check(ref, got, precision)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
 def test_two_buffer_chain(precision, monkeypatch):
     """The two-buffer chain (taken where the band pixels outgrow the single-buffer chain's registers), forced on the
     parametric batch."""
