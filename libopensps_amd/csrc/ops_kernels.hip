@@ -714,7 +714,9 @@ __global__ void k_vega_mags(int n, int nb, double* mags, const double* magvega, 
 // ------------------------------------------------------------------------------------------------
 __device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, double dz, double ls, int lane);
 
-__global__ __launch_bounds__(256) void k_prep(PrepArgs a) {
+// four wavefronts a SIMD (at most 128 VGPRs: a few spill, and the occupancy more than repays them; 0.199 -> 0.189 ms
+// on the 16 384-model batch)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_prep(PrepArgs a) {
     extern __shared__ double lds[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int m = blockIdx.x * 4 + wv;
