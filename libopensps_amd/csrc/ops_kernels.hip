@@ -1214,6 +1214,9 @@ __device__ void wave_chain_header(const PrepArgs& a, int m, const double* p, dou
 template <typename T> struct Quad;
 template <> struct Quad<float> { typedef f32x4_t type; };
 template <> struct Quad<double> { typedef f64x4_t type; };
+template <typename T> struct Pair;
+template <> struct Pair<float> { typedef float __attribute__((ext_vector_type(2))) type; };
+template <> struct Pair<double> { typedef double __attribute__((ext_vector_type(2))) type; };
 
 // The chain reads its arguments through the kernarg segment's address, made opaque again at each phase (chain_args),
 // so that each phase loads the pointers and sizes it uses (scalar loads) instead of holding all of them from the
@@ -2117,9 +2120,25 @@ void k_chain(ChainArgs) {
                 }
             }
             if (spec_out) {
+                // a whole quad outside the smoothed range leaves as one 16-byte store where the row allows it (FP32,
+                // row start a multiple of 4 values), else as two 2-value stores (row start even), else value by value
+                const size_t r0 = (size_t)m * nspec;
+                if (outside && i0 + 3 < nspec && (r0 & 1) == 0) {
+                    typedef typename Pair<T>::type P2;
+                    bool whole = false;
+                    if constexpr (F32) {
+                        whole = (r0 & 3) == 0;
+                        if (whole) *(Q*)(spec_out + i0) = v;
+                    }
+                    if (!whole) {
+                        *(P2*)(spec_out + i0) = (P2){v[0], v[1]};
+                        *(P2*)(spec_out + i0 + 2) = (P2){v[2], v[3]};
+                    }
+                } else {
 #pragma unroll
-                for (int e = 0; e < 4; e++)
-                    if (i0 + e < nspec && (i0 + e < w0 || i0 + e >= w1)) spec_out[i0 + e] = v[e];
+                    for (int e = 0; e < 4; e++)
+                        if (i0 + e < nspec && (i0 + e < w0 || i0 + e >= w1)) spec_out[i0 + e] = v[e];
+                }
             }
         }
     }
