@@ -163,6 +163,8 @@ int opensps_hip_create(int device, opensps_hip_ctx_t** out) {
         return ops_fail(nullptr, 10, "opensps_hip_create: stream");
     }
     c->own_stream = true;
+    // the batch's status flag comes back through pinned host memory, copied in stream order (NULL: a plain copy at sync)
+    if (hipHostMalloc((void**)&c->h_flag, sizeof(int), hipHostMallocDefault) != hipSuccess) c->h_flag = nullptr;
     // OPENSPS_F64_GEMM=vector: the FP64 age sums with vector FMAs instead of the f64 matrix instruction
     const char* e = getenv("OPENSPS_F64_GEMM");
     c->f64_vector = e && strcmp(e, "vector") == 0;
@@ -195,6 +197,7 @@ int opensps_hip_destroy(opensps_hip_ctx_t* c) {
                   c->d_neb_lz, c->d_neb_lu, c->d_neb_cont, c->d_neb_line, c->d_neb_lpos, c->d_neb_ia, c->d_neb_newr,
                   c->d_neb_llo, c->d_neb_wa, c->d_neb_dwa, c->d_neb_q0, c->d_neb_lptr, c->d_neb_lid, c->d_neb_gval};
     for (void* p : ps) free_ptr(p);
+    if (c->h_flag) (void)hipHostFree(c->h_flag);
     free_operator(c);
     free_full(c);
     for (int k = 0; k <= OpsCtx::NPROF; k++)
@@ -1214,6 +1217,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     k_fill_i32<<<(unsigned)((ncnt + 255) / 256), 256, 0, c->stream>>>(d_ntiles, (int)ncnt, 0);
     HIPCHK(c, hipGetLastError());
     c->d_last_flag = d_flag;
+    c->flag_async = false;
 
     PrepArgs pa;
     pa.n = n;
@@ -1654,6 +1658,13 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
                                                                           c->vega_ref, c->vega_anchor);
         HIPCHK(c, hipGetLastError());
     }
+    // the flag (final: only k_prep, k_neb and k_emission write it) to pinned host memory behind the batch, so that the
+    // sync after it reads the status without a second round trip
+    c->flag_async = false;
+    if (c->h_flag) {
+        HIPCHK(c, hipMemcpyAsync(c->h_flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        c->flag_async = true;
+    }
     return 0;
 }
 
@@ -2021,7 +2032,11 @@ int opensps_hip_sync(opensps_hip_ctx_t* c) {
     }
     if (!c->d_last_flag) return 0;
     int flag = 0;
-    HIPCHK(c, hipMemcpy(&flag, c->d_last_flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (c->flag_async)
+        flag = *c->h_flag;   // (the stream has drained: the batch's copy has landed)
+    else
+        HIPCHK(c, hipMemcpy(&flag, c->d_last_flag, sizeof(int), hipMemcpyDeviceToHost));
+    c->flag_async = false;
     c->d_last_flag = nullptr;
     if (flag == 2) return ops_fail(c, 2, "batch: a model needs an option this build does not compute (SFH type, "
                                          "tage <= 0 without weights, smoothing other than the fast velocity operator, "

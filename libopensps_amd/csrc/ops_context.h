@@ -173,6 +173,8 @@ struct OpsCtx {
     OpsBuffer opsraw, opct, oppart, opimfd;       // the operator's reverse pass: raw per-age sums, template
                                                   // cotangents by position, per-(pass, age) sums, IMF constants
     int* d_last_flag = nullptr;   // the per-batch status flag of the last batch (read by sync)
+    int* h_flag = nullptr;        // pinned host copy of that flag, written in stream order at the batch's end
+    bool flag_async = false;      // h_flag holds the last batch's flag once the stream has drained
 
     // ---- per-kernel timing (HIP events on the context's stream) ----
     static const int NPROF = 4;     // prep, group, age-sum product, chain
