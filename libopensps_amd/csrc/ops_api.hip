@@ -1292,7 +1292,7 @@ int opensps_hip_csp_batch(opensps_hip_ctx_t* c, const opensps_hip_batch_t* a) {
     ga.tiles = (int4*)c->tiles.p;
     ga.nmtiles = mixed && !uni ? d_nmtiles : nullptr;
     ga.mtiles = mixed && !uni ? mtiles : nullptr;
-    k_group<<<1, 1024, 0, c->stream>>>(ga);
+    k_group<<<(unsigned)max(1, (n + 1023) / 1024), 1024, 0, c->stream>>>(ga);
     HIPCHK(c, hipGetLastError());
     if (uni && mixed) {
         k_op_tiles<<<1, 256, 0, c->stream>>>(n * parts, OPS_MM_COLS, d_opmtiles, mtiles + maxm);

@@ -862,7 +862,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // in slot m % OPS_GCNT_SLOTS, and ranked each model in its slot with an atomic, so the order within a pair follows
 // the slots and the order of arrival: every column of the products is formed alone, so the results do not depend
 // on it), and the column tiles of each pair (OPS_TILE_COL columns, and with mtiles the OPS_MM_COLS columns of the
-// split-BF16 products). One workgroup: the slots' and pairs' prefixes, then the scatter.
+// split-BF16 products). A workgroup per 1024 models: each forms the slots' and pairs' prefixes (a few hundred
+// counters) and scatters its models; the first writes the tile lists.
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_group(GroupArgs a) {
     __shared__ int gstart[256], gtot[256], tbase[257], mbase[257];
@@ -889,14 +890,18 @@ __global__ __launch_bounds__(1024) void k_group(GroupArgs a) {
         }
         tbase[G] = tb;
         mbase[G] = mb;
-        *a.ntiles = tb;
-        if (a.mtiles) *a.nmtiles = mb;
+        if (blockIdx.x == 0) {
+            *a.ntiles = tb;
+            if (a.mtiles) *a.nmtiles = mb;
+        }
     }
     __syncthreads();
-    for (int m = t; m < n; m += blockDim.x) {
+    // each workgroup (all of them form the same prefix) places its slice of the models; the first writes the tiles
+    for (int m = blockIdx.x * blockDim.x + t; m < n; m += gridDim.x * blockDim.x) {
         const int g = a.zlo[m] - 1;
         a.order[gstart[g] + sstart[g * OPS_GCNT_SLOTS + m % OPS_GCNT_SLOTS] + a.rank[m]] = m;
     }
+    if (blockIdx.x != 0) return;
     for (int g = t; g < G; g += blockDim.x) {
         const int ncol = gtot[g] * a.parts;
         for (int i = 0; tbase[g] + i < tbase[g + 1]; i++) {
