@@ -151,7 +151,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const int q = tid + 256 * k, col = q >> 5, part = q & 31;
         const int cc = sColC[col];
         if (cc < 0 || part * 4 >= rows_valid) continue;
-        *(f32x4_t*)(a.C + (size_t)cc * a.ldc + rt * OPS_MM_ROWS + part * 4) =
-            *(const f32x4_t*)(sT + col * OPS_MM_TS + part * 4);
+        // non-temporal: C is written once and read by a later kernel; keeping it out of the caches leaves them to
+        // the operands (age sums 0.223 -> 0.214 ms, and the chain that reads C after it 0.500 -> 0.491 ms)
+        __builtin_nontemporal_store(*(const f32x4_t*)(sT + col * OPS_MM_TS + part * 4),
+                                    (f32x4_t*)(a.C + (size_t)cc * a.ldc + rt * OPS_MM_ROWS + part * 4));
     }
 }
