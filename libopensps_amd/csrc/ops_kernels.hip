@@ -2126,18 +2126,19 @@ void k_chain(ChainArgs) {
             }
             if (spec_out) {
                 // a whole quad outside the smoothed range leaves as one 16-byte store where the row allows it (FP32,
-                // row start a multiple of 4 values), else as two 2-value stores (row start even), else value by value
+                // row start a multiple of 4 values), else as two 2-value stores (row start even), else value by value;
+                // the vector stores non-temporal (the spectrum is written once, for the caller)
                 const size_t r0 = (size_t)m * nspec;
                 if (outside && i0 + 3 < nspec && (r0 & 1) == 0) {
                     typedef typename Pair<T>::type P2;
                     bool whole = false;
                     if constexpr (F32) {
                         whole = (r0 & 3) == 0;
-                        if (whole) *(Q*)(spec_out + i0) = v;
+                        if (whole) __builtin_nontemporal_store(v, (Q*)(spec_out + i0));
                     }
                     if (!whole) {
-                        *(P2*)(spec_out + i0) = (P2){v[0], v[1]};
-                        *(P2*)(spec_out + i0 + 2) = (P2){v[2], v[3]};
+                        __builtin_nontemporal_store((P2){v[0], v[1]}, (P2*)(spec_out + i0));
+                        __builtin_nontemporal_store((P2){v[2], v[3]}, (P2*)(spec_out + i0 + 2));
                     }
                 } else {
 #pragma unroll
